@@ -1,0 +1,135 @@
+"""Throughput of the BROMOC BD step path on one MI355X.
+
+Workload (default): ions in a cylindrical pore through a slab plus a 50-bp CG-DNA duplex
+-- 2000 K+/Cl- ions and 300 DNA sites in a 100x100x200 A box, 201x201x401 repulsion and
+static-field grids at 0.5 A, 15 EFPOT tables (r = 1.5..20 A at 0.01 A), FP64, Philox noise.
+One step = ENERGY + DYNAMICS0 + DYNAMICS0NUC over every moving particle (one kernel launch).
+
+Prints one JSON line: particle-steps/s over the timed steps, the roofline of the step
+kernel (bytes from HIP events on the kernel's own stream), and a CPU baseline: the
+reference Fortran routines (oracle/_ref, one thread) -- or the C oracle where that build
+is absent -- timed on this host over a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s + % HBM roofline, batched vs plain, FP64 rel err<=1e-10"
+HBM_PEAK_GBS = 8000.0
+
+
+def step_bytes(s, injected):
+    """Algorithmic HBM bytes per particle-step of the fused step kernel: read x,y,z (24)
+    and the type word (4), write x,y,z (24), 8 trilinear corners of each grid present
+    (phix float32: 32, phiv int8: 8), injected Gaussians (24).  Partner positions, pair
+    tables, DNA lists and type tables are re-read from L2 and are not HBM traffic."""
+    from bromoc_amd import abi
+    b = 24 + 4 + 24
+    if s.flags & abi.QPHIX:
+        b += 32
+    if s.flags & abi.QPHIV:
+        b += 8
+    if injected:
+        b += 24
+    return b
+
+
+def cpu_baseline(s, target_s=12.0):
+    """Time the reference routines (or the C oracle) on a bounded number of steps."""
+    import numpy as np
+    from bromoc_amd import abi
+    try:
+        from oracle.oracle import RefFortran, ref_available, Oracle
+    except ImportError:
+        return None
+    kind = "reference" if ref_available() else "port"
+    eng = RefFortran(s) if kind == "reference" else Oracle(s)
+    nmov = s.nsites + (s.ntot - s.nsites - s.nfix)
+    rng = np.random.default_rng(99)
+    # one probe step sizes the sample to about target_s seconds
+    eng.upload_noise(rng.standard_normal((1, 3 * s.ntot)))
+    t0 = time.perf_counter()
+    eng.step(1, abi.NOISE_INJECTED, 0)
+    probe = time.perf_counter() - t0
+    nsteps = max(1, min(2000, int(target_s / max(probe, 1e-6))))
+    eng.upload_noise(rng.standard_normal((nsteps, 3 * s.ntot)))
+    t0 = time.perf_counter()
+    eng.step(nsteps, abi.NOISE_INJECTED, 0)
+    dt = time.perf_counter() - t0
+    return {"value": nmov * nsteps / dt, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+            "sample": f"{nsteps} BD steps (ENERGY+DYNAMICS0+DYNAMICS0NUC) of the same "
+                      f"{s.ntot}-particle system, 1 thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="pore_dna", choices=["pore_dna", "plain_ions"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-parity", action="store_true", help="skip the start-state parity check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from bromoc_amd import abi, configs
+    from bromoc_amd.engine import Engine
+
+    t0 = time.perf_counter()
+    if args.workload == "pore_dna":
+        s = configs.pore_dna(seed=1)
+        wl = {"workload": "pore_dna", "ions": s.ntot - s.nsites, "dna_sites": s.nsites,
+              "dna_bp": s.meta["n_bp"], "box_A": [100, 100, 200], "grid": list(s.meta["grid"]),
+              "grid_spacing_A": s.meta["dcel"], "pair_tables": s.npair,
+              "table_knots": int(len(s.ep) // 3), "dt_ps": s.meta["dt"], "noise": "philox"}
+    else:
+        s = configs.plain_ions(seed=0)
+        wl = {"workload": "plain_ions", "ions": s.ntot, "box_A": 60, "pair_tables": s.npair,
+              "dt_ps": s.meta["dt"], "noise": "philox"}
+    build_s = time.perf_counter() - t0
+    nmov = s.nsites + (s.ntot - s.nsites - s.nfix)
+    eng = Engine(s)
+
+    parity = None
+    if not args.no_parity:
+        from oracle.oracle import Oracle
+        o = Oracle(s)
+        eg, eo = eng.energy(), o.energy()
+        fg, fo = eng.forces(), o.forces()
+        parity = {"force_normwise_relerr": float(np.max(np.abs(fg - fo)) / np.max(np.abs(fo))),
+                  "energy_relerr": abs(eg["ener"] - eo["ener"]) / abs(eo["ener"])}
+        del o
+
+    eng.step(args.warmup, abi.NOISE_PHILOX, seed=2024, counter0=0, energies=False)
+    total_ms, kernel_ms = eng.time_steps(args.steps, abi.NOISE_PHILOX, seed=2024,
+                                         counter0=args.warmup)
+    value = nmov * args.steps / (total_ms * 1e-3)
+    bpp = step_bytes(s, injected=False)
+    achieved = bpp * nmov / (kernel_ms * 1e-3) / 1e9
+    cpu = None if args.no_cpu else cpu_baseline(s)
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded positions, grids and EFPOT tables)", "config": wl,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step", "kernel_ms": kernel_ms, "bytes_per_particle_step": bpp},
+        "cpu_baseline": cpu, "parity_vs_oracle": parity, "setup_s": build_s,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+"""ctypes mirror of include/bromoc_gpu.h and the loader of the HIP library.
+
+The structs below must match the C header field for field; tests/test_abi.py checks
+sizes and offsets against the compiled library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+ABI_VERSION = 1
+
+OK, E_ARG, E_UNSUPPORTED, E_CAPACITY, E_HIP, E_NODEVICE, E_STATE = range(7)
+STATUS_NAMES = {0: "OK", 1: "E_ARG", 2: "E_UNSUPPORTED", 3: "E_CAPACITY", 4: "E_HIP",
+                5: "E_NODEVICE", 6: "E_STATE"}
+
+QPAR, QNUCL, QNONBOND, QNOBOND = 1 << 0, 1 << 1, 1 << 2, 1 << 3
+QPHIX, QPHIV, QSVDW, QSOLV = 1 << 4, 1 << 5, 1 << 6, 1 << 7
+QDEBY, QCOUNTION, QDNAFREE = 1 << 8, 1 << 9, 1 << 10
+QMEMB, QMMIJ, QBSPLINE, QNMCDEN = 1 << 16, 1 << 17, 1 << 18, 1 << 19
+QDEBYHYB, QPROXDIFF, QNOTRANS, QAPFOR = 1 << 20, 1 << 21, 1 << 22, 1 << 23
+QCONTRANS, QRFPAR, QSPHERE, QECYL = 1 << 24, 1 << 25, 1 << 26, 1 << 27
+QDIFFUSE, QPROFILE = 1 << 28, 1 << 29
+
+SITE_S, SITE_P, SITE_AB, SITE_TB, SITE_CB, SITE_GB = range(6)
+NOISE_INJECTED, NOISE_PHILOX = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_bp = C.POINTER(C.c_int8)
+
+
+class Pair(C.Structure):
+    _fields_ = [("efpot", C.c_int32), ("chr", C.c_int32), ("lj", C.c_int32),
+                ("srpmf", C.c_int32), ("nxi", C.c_int32), ("nknots", C.c_int32),
+                ("fct", C.c_double), ("epp4", C.c_double), ("sgp2", C.c_double),
+                ("c0", C.c_double), ("c1", C.c_double), ("c2", C.c_double),
+                ("c3", C.c_double), ("c4", C.c_double),
+                ("dmi", C.c_double), ("dm2lo", C.c_double), ("dm2hi", C.c_double),
+                ("sc1", C.c_double), ("sc2", C.c_double), ("sc3", C.c_double)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nclx", C.c_int32), ("ncly", C.c_int32), ("nclz", C.c_int32),
+                ("reserved", C.c_int32), ("dcel", C.c_double), ("xbcen", C.c_double),
+                ("ybcen", C.c_double), ("zbcen", C.c_double), ("data", C.c_void_p)]
+
+
+class SystemStruct(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("ntot", C.c_int32), ("nsites", C.c_int32),
+        ("nfix", C.c_int32), ("capacity", C.c_int32), ("ntype", C.c_int32),
+        ("nold", C.c_int32), ("nttyp", C.c_int32), ("npair", C.c_int32),
+        ("flags", C.c_uint32),
+        ("x", _dp), ("y", _dp), ("z", _dp), ("typei", _ip), ("typtyp", _ip),
+        ("namsite", _ip), ("stfree", _bp),
+        ("nwtype", _ip), ("cg", _dp), ("fact1a", _dp), ("fact2a", _dp), ("pairidx", _ip),
+        ("pairs", C.POINTER(Pair)),
+        ("nknots", C.c_int32), ("reserved0", C.c_int32), ("ep", _dp),
+        ("res", C.c_double), ("ires", C.c_double),
+        ("rth", C.c_double), ("srpx", C.c_double), ("srpk", C.c_double), ("srpy", C.c_double),
+        ("cgnuc", C.c_double), ("epsnuc", C.c_double), ("scalepairing", C.c_double),
+        ("fctn", C.c_double), ("kappa", C.c_double), ("ikappa", C.c_double),
+        ("epsolv", C.c_double),
+        ("nbond", C.c_int32), ("nangle", C.c_int32), ("ndihe", C.c_int32),
+        ("nstack", C.c_int32), ("nbp", C.c_int32), ("nex", C.c_int32), ("nqq", C.c_int32),
+        ("nsolv", C.c_int32),
+        ("sitebond", _ip), ("distbond", _dp), ("siteangle", _ip), ("valangle", _dp),
+        ("sitedihe", _ip), ("valdihe", _dp), ("sitestack", _ip), ("sgstack", _dp),
+        ("sitebp", _ip), ("sgbp", _dp), ("siteex", _ip), ("sgex", _dp),
+        ("siteqq", _ip), ("siteslv", _ip),
+        ("phix", Grid), ("phiv", Grid),
+        ("svdw", C.c_double), ("thold8", C.c_double), ("vzmin", C.c_double),
+        ("vzmax", C.c_double), ("scal", _dp),
+        ("bdmax", C.c_double), ("fact0n1", C.c_double), ("fact0n2", C.c_double),
+        ("lx2m", C.c_double), ("lx2p", C.c_double), ("ly2m", C.c_double),
+        ("ly2p", C.c_double), ("lz2m", C.c_double), ("lz2p", C.c_double),
+        ("cntpts", C.c_int32), ("reserved1", C.c_int32), ("zcont", _dp),
+    ]
+
+
+ENERGY_FIELDS = ["ener", "eelec", "evdw", "esrpmf", "esrpmfmx", "eefpot", "eefpotmx", "eqqmx",
+                 "evdwmx", "ebond", "eang", "edihe", "estack", "ebp", "eex", "eqq", "esolv",
+                 "egsbpa", "egsbpb", "evdwgd", "erfpar", "econ", "ememb"]
+
+
+class Energies(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ENERGY_FIELDS]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n in ENERGY_FIELDS}
+
+
+class GcmcStruct(C.Structure):
+    _fields_ = [("nbuffer", C.c_int32), ("reserved", C.c_int32),
+                ("ibfftyp", _ip), ("avnum", _dp), ("mu", _dp), ("kb", _dp),
+                ("lzmin", _dp), ("lzmax", _dp), ("bufferbias", _bp), ("ntotat", _ip),
+                ("lx", C.c_double), ("ly", C.c_double), ("cx", C.c_double),
+                ("cy", C.c_double), ("cz", C.c_double), ("ikbt", C.c_double)]
+
+
+class BromocError(RuntimeError):
+    def __init__(self, where, code, detail=""):
+        self.code = code
+        super().__init__(f"{where}: {STATUS_NAMES.get(code, code)} {detail}".strip())
+
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libbromoc_gpu.so")
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load libbromoc_gpu.so (the HIP path).  Raises if it is not built: there is no
+    fallback implementation behind this package."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"{p} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+    lib = C.CDLL(p)
+    _declare(lib)
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _declare(lib):
+    vp = C.c_void_p
+    P = C.POINTER
+    sig = {
+        "bromoc_create": ([P(SystemStruct), C.c_int, P(vp)], C.c_int),
+        "bromoc_destroy": ([vp], C.c_int),
+        "bromoc_last_error": ([vp], C.c_char_p),
+        "bromoc_set_positions": ([vp, C.c_int32, _dp, _dp, _dp, _ip], C.c_int),
+        "bromoc_get_positions": ([vp, _dp, _dp, _dp, _ip, _ip], C.c_int),
+        "bromoc_get_forces": ([vp, _dp, _dp, _dp], C.c_int),
+        "bromoc_energy": ([vp, C.c_int32, P(Energies)], C.c_int),
+        "bromoc_upload_noise": ([vp, C.c_int32, _dp], C.c_int),
+        "bromoc_step": ([vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_uint64,
+                         P(Energies)], C.c_int),
+        "bromoc_dynamics": ([vp, C.c_int32, C.c_int32, C.c_uint64, C.c_uint64], C.c_int),
+        "bromoc_get_flux": ([vp, _ip, _ip], C.c_int),
+        "bromoc_get_warn": ([vp, _ip], C.c_int),
+        "bromoc_interact": ([vp, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                             C.c_int32, _dp], C.c_int),
+        "bromoc_set_gcmc": ([vp, P(GcmcStruct)], C.c_int),
+        "bromoc_grand": ([vp, C.c_int32, _dp, C.c_int64, P(C.c_int64), _ip, _ip, C.c_int64,
+                          _dp], C.c_int),
+        "bromoc_build_table": ([C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp, P(Pair)],
+                               C.c_int),
+        "bromoc_set_ibuffer": ([vp, _ip], C.c_int),
+        "bromoc_time_steps": ([vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_uint64,
+                               _dp, _dp], C.c_int),
+        "bromoc_get_ibuffer": ([vp, _ip], C.c_int),
+        "bromoc_philox4x64": ([C.c_int32, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+
+
+def exported_symbols():
+    """Names every build of libbromoc_gpu.so must export (the declarations of
+    include/bromoc_gpu.h)."""
+    hdr = os.path.join(os.path.dirname(PKG_DIR), "include", "bromoc_gpu.h")
+    import re
+    names = []
+    with open(hdr) as f:
+        for line in f:
+            m = re.match(r"^\s*(?:int|const char \*|void)\s*\*?\s*(bromoc_\w+)\s*\(", line)
+            if m:
+                names.append(m.group(1))
+    return names

@@ -1,0 +1,472 @@
+"""Synthetic, seeded BROMOC workloads (no reference input decks are shipped).
+
+plain_ions     400-ion BD in a 60 A box, 3 EFPOT pair tables on r = 1.5..20 A at 0.01 A
+pore_dna       ions in a cylindrical pore through a slab (phiv/phix grids) with a 50-bp
+               CG-DNA duplex on the pore axis (3 sites per nucleotide)
+pore_replica   one replica of the batched ion+pore workload (no DNA, GCMC buffers)
+
+Every builder is size-parameterised so tests can build small instances of the same shape.
+The EFPOT (r,U) samples are turned into the reference's piecewise-quadratic table by the
+host-side SPLINEPOT of the HIP library (bromoc_build_table), or by a caller-given builder.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import abi
+from .system import (CELEC, COULOMB, KCALMOL, AVOGADRO, LITER, ANGSTROM, PI, Gcmc, GridData,
+                     System, kbt_of, nindex_table)
+
+RMIN, RMAX, RES = 1.5, 20.0, 0.01
+
+
+def lib_table_builder(n, r, u, chr_, fct):
+    lib = abi.load_library()
+    ep = np.zeros(3 * n)
+    p = abi.Pair()
+    rc = lib.bromoc_build_table(n, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                u.ctypes.data_as(C.POINTER(C.c_double)), chr_, fct,
+                                ep.ctypes.data_as(C.POINTER(C.c_double)), C.byref(p))
+    if rc:
+        raise abi.BromocError("bromoc_build_table", rc)
+    return ep, p
+
+
+def _pair_dict(p: abi.Pair):
+    return {name: getattr(p, name) for name, _ in abi.Pair._fields_}
+
+
+def knot_grid(rmin=RMIN, rmax=RMAX, res=RES):
+    """r samples as an EFPOT file lists them (decimal values, nearest binary64)."""
+    n = int(round((rmax - rmin) / res)) + 1
+    return np.array([float(f"{rmin + res * k:.6f}") for k in range(n)])
+
+
+def efpot_curve(r, sig, eps, qq, cdie, osc=0.3):
+    """12-6 LJ + screened Coulomb + damped oscillatory tail (kcal/mol)."""
+    sr6 = (sig / r) ** 6
+    return (4.0 * eps * (sr6 * sr6 - sr6) + qq * CELEC / (cdie * r)
+            + osc * np.exp(-r / 3.0) * np.cos(2.0 * PI * r / 2.8))
+
+
+def build_tables(pairs_spec, table_builder=None, rmin=RMIN, rmax=RMAX, res=RES):
+    """pairs_spec: list indexed by pair type (entry 0 None) of dicts with sig, eps, qq,
+    cdie.  Returns (pairs list of dicts, ep)."""
+    tb = table_builder or lib_table_builder
+    r = knot_grid(rmin, rmax, res)
+    pairs, eps_all = [None], []
+    off = 0
+    for spec in pairs_spec[1:]:
+        u = efpot_curve(r, spec["sig"], spec["eps"], spec["qq"], spec["cdie"],
+                        spec.get("osc", 0.3))
+        chr_ = 1 if spec["qq"] != 0.0 else 0
+        fct = CELEC * spec["qq"] / spec["cdie"] if chr_ else 0.0
+        ep, p = tb(len(r), r, u, chr_, fct)
+        d = _pair_dict(p)
+        d["nxi"] = off
+        d["efpot"] = 1
+        off += len(r)
+        pairs.append(d)
+        eps_all.append(ep)
+    return pairs, np.concatenate(eps_all) if eps_all else np.zeros(0)
+
+
+def _place(rng, n, lo, hi, dmin, accept=None, existing=None, max_tries=2_000_000):
+    """Uniform positions in the box [lo,hi], rejection-sampled to min separation dmin."""
+    pts = np.zeros((0, 3)) if existing is None else np.asarray(existing, float).reshape(-1, 3)
+    out = []
+    cell = max(dmin, 1e-9)
+    grid = {}
+
+    def key(p):
+        return tuple(np.floor(p / cell).astype(int))
+
+    for p in pts:
+        grid.setdefault(key(p), []).append(p)
+    tries = 0
+    while len(out) < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("could not place particles")
+        p = lo + (hi - lo) * rng.random(3)
+        if accept is not None and not accept(p):
+            continue
+        k = key(p)
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for q in grid.get((k[0] + dx, k[1] + dy, k[2] + dz), ()):
+                        if np.sum((p - q) ** 2) < dmin * dmin:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if ok:
+            grid.setdefault(k, []).append(p)
+            out.append(p)
+    return np.array(out).reshape(-1, 3)
+
+
+# ------------------------------------------------------------------ plain ions
+def plain_ions(seed=0, n_per_type=200, box=60.0, temp=300.0, cdie=80.0, dt=0.02,
+               diff=(0.196, 0.203), charges=(1.0, -1.0), table_builder=None, capacity=None,
+               extra_pair_types=0, rmax=RMAX):
+    """Plain ion-only BD: equal numbers of K+ and Cl- in a cubic box centred at 0."""
+    rng = np.random.default_rng(seed)
+    ntypes = len(charges)
+    nion = ntypes
+    ndna = 0
+    nttyp = nion
+    sig = rng.uniform(3.0, 4.5, ntypes)
+    eps = rng.uniform(0.1, 0.5, ntypes)
+    pidx = nindex_table(nttyp, ndna, nion)
+    npair = nion * (nion + 1) // 2
+    spec = [None] * (npair + 1)
+    for a in range(1, nttyp + 1):
+        for b in range(a, nttyp + 1):
+            ia, ib = a - 1, b - 1
+            spec[pidx[a * (nttyp + 1) + b]] = dict(
+                sig=0.5 * (sig[ia] + sig[ib]), eps=math.sqrt(eps[ia] * eps[ib]),
+                qq=charges[ia] * charges[ib], cdie=cdie)
+    pairs, ep = build_tables(spec, table_builder, rmax=rmax)
+    half = 0.5 * box
+    pos = _place(rng, n_per_type * ntypes, -half, half, 2.0)
+    typei = np.repeat(np.arange(1, ntypes + 1, dtype=np.int32), n_per_type)
+    order = rng.permutation(len(typei))
+    pos, typei = pos[order], typei[order]
+    kbt = kbt_of(temp)
+    kbtdt = dt * (1.0 / kbt)
+    d = np.array((0.0,) + tuple(diff))
+    s = System(
+        ntot=len(typei), nsites=0, nfix=0, ntype=ntypes, nold=0, nttyp=nttyp,
+        flags=abi.QPAR | abi.QNONBOND | abi.QNOBOND,
+        x=pos[:, 0].copy(), y=pos[:, 1].copy(), z=pos[:, 2].copy(), typei=typei,
+        nwtype=np.arange(0, ntypes + 1, dtype=np.int32),
+        cg=np.array((0.0,) + tuple(charges)), fact1a=d * kbtdt, fact2a=np.sqrt(2.0 * dt * d),
+        pairidx=pidx, pairs=pairs, ep=ep, res=RES,
+        capacity=capacity or len(typei), bdmax=1.0,
+        box=(-half, half, -half, half, -half, half), ndna=ndna, nion=nion,
+        meta=dict(workload="plain_ions", seed=seed, temp=temp, dt=dt, kbt=kbt, cdie=cdie),
+    )
+    s.fact2a[0] = 0.0
+    return s
+
+
+# ------------------------------------------------------------------- CG-DNA
+# B-form cylindrical coordinates (r [A], phi [deg], z [A]) of the 3-site model,
+# rise 3.38 A and twist 36 deg per base step.
+_CYL = {"P": (8.91, 94.9, 2.186), "S": (6.19, 70.5, 1.89), "A": (0.773, 41.905, 0.981),
+        "T": (2.349, 86.452, 1.281), "C": (1.837, 85.539, 1.347), "G": (0.817, 40.808, 0.930)}
+_COMP = {"A": "T", "T": "A", "C": "G", "G": "C"}
+_BASE_CODE = {"A": abi.SITE_AB, "T": abi.SITE_TB, "C": abi.SITE_CB, "G": abi.SITE_GB}
+
+
+def _dihedral(p1, p2, p3, p4):
+    """energy.mono.f90:283-300 convention."""
+    v1, v2, v3 = p1 - p2, p3 - p2, p3 - p4
+    n1, n2 = np.cross(v1, v2), np.cross(v2, v3)
+    return math.atan2(math.sqrt(v2 @ v2) * (v1 @ n2), n1 @ n2)
+
+
+def _angle(p1, p2, p3):
+    v1, v2 = p1 - p2, p3 - p2
+    return math.acos((v1 @ v2) / math.sqrt((v1 @ v1) * (v2 @ v2)))
+
+
+def dna_duplex(seq, z0=0.0, solv=True):
+    """Native B-form duplex of `seq` (strand 1, 5'->3') plus its complement: site arrays
+    and the bonded / nonbonded lists built by the rules of bonds.f90, angles.f90,
+    dihedral.f90 and go_qq.f90:45-116 on the native geometry."""
+    nb = len(seq)
+    rise, twist = 3.38, 36.0
+    names, code, strand, pos, nuc = [], [], [], [], []
+
+    def site(kind, step, st, letter, k):
+        r, ph, zz = _CYL[kind]
+        if st == 2:
+            ph, zz = -ph, -zz
+        a = math.radians(ph + step * twist)
+        pos.append((r * math.cos(a), r * math.sin(a), zz + step * rise))
+        names.append(kind)
+        code.append(abi.SITE_P if kind == "P" else abi.SITE_S if kind == "S" else _BASE_CODE[kind])
+        strand.append(st)
+        nuc.append(k)
+
+    k = 0
+    for j, b in enumerate(seq):
+        for kind in ("P", "S", b):
+            site(kind, j, 1, b, k)
+        k += 1
+    comp = [_COMP[b] for b in reversed(seq)]
+    for j, b in enumerate(comp):
+        step = nb - 1 - j
+        for kind in ("P", "S", b):
+            site(kind, step, 2, b, k)
+        k += 1
+    pos = np.array(pos)
+    pos[:, 2] += z0 - 0.5 * (nb - 1) * rise
+    n = len(names)
+    bonds = []
+    for st in (1, 2):
+        idx = [i for i in range(n) if strand[i] == st]
+        for t in range(0, len(idx), 3):
+            p, s_, b = idx[t], idx[t + 1], idx[t + 2]
+            bonds += [(p, s_), (s_, b)]
+            if t + 3 < len(idx):
+                bonds.append((s_, idx[t + 3]))
+    bset = {tuple(sorted(b)) for b in bonds}
+    nbr = {i: set() for i in range(n)}
+    for a, b in bonds:
+        nbr[a].add(b)
+        nbr[b].add(a)
+    angles = []
+    for c_ in range(n):
+        ns_ = sorted(nbr[c_])
+        for ii in range(len(ns_)):
+            for jj in range(ii + 1, len(ns_)):
+                angles.append((ns_[ii], c_, ns_[jj]))
+    aset = {tuple(sorted((a, d))) for a, _, d in angles}
+    dihes = []
+    for b_, c_ in bonds:
+        for a in sorted(nbr[b_] - {c_}):
+            for d in sorted(nbr[c_] - {b_}):
+                if a != d:
+                    dihes.append((a, b_, c_, d))
+    comp_pair = {("Gb", "Cb"), ("Cb", "Gb"), ("Ab", "Tb"), ("Tb", "Ab")}
+    nm = ["P " if x == "P" else "S " if x == "S" else x + "b" for x in names]
+    stack, bp, ex, qq, slv = [], [], [], [], []
+    for i in range(1, n):
+        for j in range(i):
+            if (j, i) not in bset:
+                dij = float(np.sqrt(np.sum((pos[j] - pos[i]) ** 2)))
+                if strand[i] == strand[j] and (dij - 9.0) < 0.0 and abs(dij - 9.0) >= 1e-6:
+                    stack.append((i, j, dij))
+                elif (nm[i], nm[j]) in comp_pair:
+                    bp.append((i, j, 2.8694 if nm[i] in ("Gb", "Cb") else 2.9002))
+                else:
+                    both_base = nm[i] not in ("P ", "S ") and nm[j] not in ("P ", "S ")
+                    ex.append((i, j, 1.0 if both_base else 6.86))
+                if (j, i) not in aset and nm[i] == "P " and nm[j] == "P ":
+                    qq.append((i, j))
+            if solv and strand[i] != strand[j] and nm[i] == "S " and nm[j] == "S ":
+                slv.append((i, j))
+
+    def cm(lst, k):
+        if not lst:
+            return np.zeros(0, np.int32)
+        a = np.array([t[:k] for t in lst], np.int32)
+        return a.T.reshape(-1).copy()
+
+    lists = {
+        "bond": (cm(bonds, 2), np.array([np.linalg.norm(pos[a] - pos[b]) for a, b in bonds])),
+        "angle": (cm(angles, 3), np.array([_angle(pos[a], pos[b], pos[c]) for a, b, c in angles])),
+        "dihe": (cm(dihes, 4), np.array([_dihedral(pos[a], pos[b], pos[c], pos[d])
+                                         for a, b, c, d in dihes])),
+        "stack": (cm(stack, 2), np.array([t[2] for t in stack])),
+        "bp": (cm(bp, 2), np.array([t[2] for t in bp])),
+        "ex": (cm(ex, 2), np.array([t[2] for t in ex])),
+        "qq": (cm(qq, 2), None),
+        "slv": (cm(slv, 2), None),
+    }
+    return dict(pos=pos, code=np.array(code, np.int32), strand=np.array(strand), nuc=nuc,
+                lists=lists, nnuc=k)
+
+
+# ----------------------------------------------------------------- pore grids
+def pore_repulsion(shape, dcel, rpore, half_slab):
+    """int8 map: 1 inside the slab wall (|z| <= half_slab and R > rpore), else 0
+    (the binarised phiv of readphi.f90:319-327)."""
+    nx, ny, nz = shape
+    x = (np.arange(nx) - 0.5 * (nx - 1)) * dcel
+    y = (np.arange(ny) - 0.5 * (ny - 1)) * dcel
+    z = (np.arange(nz) - 0.5 * (nz - 1)) * dcel
+    r2 = x[:, None] ** 2 + y[None, :] ** 2
+    wall_xy = r2 > rpore * rpore
+    wall_z = np.abs(z) <= half_slab
+    return (wall_xy[:, :, None] & wall_z[None, None, :]).astype(np.int8)
+
+
+def pore_field(shape, dcel, volt, half_slab, rng, noise_volt=0.01, smooth=2.0):
+    """float32 static field phix [e/A]: a linear drop of `volt` volts across the slab
+    plus Gaussian-smoothed noise, scaled so that E = q*phix*celec (staticf1.f90:110)."""
+    from scipy.ndimage import gaussian_filter
+    nx, ny, nz = shape
+    z = (np.arange(nz) - 0.5 * (nz - 1)) * dcel
+    v = volt * np.clip((half_slab - z) / (2.0 * half_slab), 0.0, 1.0)
+    noise = gaussian_filter(rng.standard_normal(shape).astype(np.float32), smooth)
+    noise *= noise_volt / max(float(noise.std()), 1e-30)
+    vol = v[None, None, :] + noise
+    return (vol * (COULOMB / KCALMOL) / CELEC).astype(np.float32)
+
+
+def pore_dna(seed=1, n_bp=50, n_ions=2000, box=(100.0, 100.0, 200.0), dcel=0.5, rpore=12.0,
+             half_slab=20.0, volt=0.5, svdw=1000.0, temp=300.0, cdie=80.0, dt=0.02,
+             table_builder=None, dna=True, capacity=None, zcont=(-20.0, 0.0, 20.0),
+             rmax=RMAX):
+    """Ions in a cylindrical pore through a slab, with a CG-DNA duplex on the axis."""
+    rng = np.random.default_rng(seed)
+    lx, ly, lz = box
+    shape = tuple(int(round(L / dcel)) + 1 for L in box)
+    phiv = pore_repulsion(shape, dcel, rpore, half_slab)
+    phix = pore_field(shape, dcel, volt, half_slab, rng)
+    if dna:
+        seq = "".join(rng.choice(list("ACGT"), n_bp))
+        for b in "ACGT":                      # all four base types present
+            if b not in seq:
+                seq = b + seq[1:]
+        d = dna_duplex(seq, z0=0.0)
+        spos = d["pos"]
+        ns = len(spos)
+        nold = d["nnuc"]
+        present = [c for c in (abi.SITE_AB, abi.SITE_TB, abi.SITE_CB, abi.SITE_GB)
+                   if c in set(d["code"].tolist())]
+        ndna = 2 + len(present)
+        t2 = {abi.SITE_S: 1, abi.SITE_P: 2}
+        for k, c in enumerate(present):
+            t2[c] = 3 + k
+        typtyp = np.array([t2[c] for c in d["code"]], np.int32)
+    else:
+        spos = np.zeros((0, 3))
+        ns = 0
+        nold = 0
+        ndna = 0
+        typtyp = np.zeros(0, np.int32)
+        d = None
+    nion = 2
+    ntype = nold + nion
+    nttyp = ndna + nion
+    charges = (1.0, -1.0)
+    diff = (0.196, 0.203)
+    isig = rng.uniform(3.0, 4.5, nion)
+    ieps = rng.uniform(0.1, 0.5, nion)
+    ssig = rng.uniform(3.0, 4.5, max(ndna, 1))
+    seps = rng.uniform(0.1, 0.5, max(ndna, 1))
+    pidx = nindex_table(nttyp, ndna, nion)
+    npair = ndna * nion + nion * (nion + 1) // 2
+    spec = [None] * (npair + 1)
+    cgnuc = -1.0
+    for a in range(1, nttyp + 1):
+        for b in range(a, nttyp + 1):
+            if b <= ndna:
+                continue
+            if a > ndna:
+                ia, ib = a - ndna - 1, b - ndna - 1
+                spec[pidx[a * (nttyp + 1) + b]] = dict(
+                    sig=0.5 * (isig[ia] + isig[ib]), eps=math.sqrt(ieps[ia] * ieps[ib]),
+                    qq=charges[ia] * charges[ib], cdie=cdie)
+            else:
+                ib = b - ndna - 1
+                qs = cgnuc if a == 2 else 0.0
+                spec[pidx[a * (nttyp + 1) + b]] = dict(
+                    sig=0.5 * (ssig[a - 1] + isig[ib]), eps=math.sqrt(seps[a - 1] * ieps[ib]),
+                    qq=qs * charges[ib], cdie=cdie)
+    pairs, ep = build_tables(spec, table_builder, rmax=rmax)
+
+    hx, hy, hz = 0.5 * lx, 0.5 * ly, 0.5 * lz
+    sgrid = {}
+    for p in spos:
+        sgrid.setdefault(tuple(np.floor(p / 4.0).astype(int)), []).append(p)
+
+    def accessible(p):
+        if abs(p[2]) <= half_slab + 1.0 and p[0] ** 2 + p[1] ** 2 > (rpore - 1.5) ** 2:
+            return False
+        k = np.floor(p / 4.0).astype(int)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for q in sgrid.get((k[0] + dx, k[1] + dy, k[2] + dz), ()):
+                        if np.sum((p - q) ** 2) < 3.5 * 3.5:
+                            return False
+        return True
+
+    lo = np.array([-hx + 0.5, -hy + 0.5, -hz + 0.5])
+    hi = -lo
+    ipos = _place(rng, n_ions, lo, hi, 2.0, accept=accessible)
+    ityp = np.array([nold + 1 + (k % 2) for k in range(n_ions)], np.int32)
+    pos = np.concatenate([spos, ipos]) if ns else ipos
+    typei = np.concatenate([np.zeros(ns, np.int32), ityp])
+    ntot = ns + n_ions
+    kbt = kbt_of(temp)
+    kbtdt = dt * (1.0 / kbt)
+    dvec = np.zeros(ntype + 1)
+    dvec[nold + 1:] = diff
+    cg = np.zeros(ntype + 1)
+    cg[1:nold + 1] = cgnuc
+    cg[nold + 1:] = charges
+    nwtype = np.zeros(ntype + 1, np.int32)
+    for k in range(nion):
+        nwtype[nold + 1 + k] = ndna + 1 + k
+    flags = abi.QPAR | abi.QNONBOND | abi.QNOBOND | abi.QPHIX | abi.QPHIV | abi.QCOUNTION
+    diffnuc = 0.1
+    kw = {}
+    if dna:
+        flags |= abi.QNUCL | abi.QSOLV
+        kw = dict(typtyp=typtyp, namsite=d["code"], stfree=np.ones(ns, np.int8),
+                  lists=d["lists"], epsnuc=rng.uniform(0.15, 0.25), scalepairing=1.0,
+                  fctn=CELEC * cgnuc ** 2 / cdie, epsolv=0.1)
+    tz = 0.5 * (shape[2] - 1) * dcel
+    s = System(
+        ntot=ntot, nsites=ns, nfix=0, ntype=ntype, nold=nold, nttyp=nttyp, flags=flags,
+        x=pos[:, 0].copy(), y=pos[:, 1].copy(), z=pos[:, 2].copy(), typei=typei,
+        nwtype=nwtype, cg=cg, fact1a=dvec * kbtdt, fact2a=np.sqrt(2.0 * dt * dvec),
+        pairidx=pidx, pairs=pairs, ep=ep, res=RES, capacity=capacity or ntot,
+        phix=GridData(phix, dcel), phiv=GridData(phiv, dcel), svdw=svdw, thold8=8.0,
+        vzmin=-tz, vzmax=tz, bdmax=1.0, cgnuc=cgnuc,
+        fact0n1=diffnuc * kbt * dt, fact0n2=math.sqrt(2.0 * dt * diffnuc),
+        box=(-hx, hx, -hy, hy, -hz, hz), zcont=np.array(zcont, float), ndna=ndna, nion=nion,
+        meta=dict(workload="pore_dna" if dna else "pore_ions", seed=seed, temp=temp, dt=dt,
+                  kbt=kbt, cdie=cdie, n_bp=n_bp if dna else 0, grid=shape, dcel=dcel),
+        **kw,
+    )
+    return s
+
+
+def gcmc_buffers(s: System, conc, buffer_len=10.0, mu=(0.0, 0.0), volt=0.0):
+    """One buffer per ion type at each box end (2 * nion buffers), as the BUFFER command
+    sets them up (shell_simul.f90:1015-1037): avnum = conc * volume, mu shifted by the
+    applied voltage."""
+    lx2m, lx2p, ly2m, ly2p, lz2m, lz2p = s.box
+    lx, ly = lx2p - lx2m, ly2p - ly2m
+    ib_types, lzmin, lzmax, avnum, mus = [0], [0.0], [0.0], [0.0], [0.0]
+    for end in (0, 1):
+        for k in range(s.nion):
+            t = s.nold + 1 + k
+            lo, hi = (lz2m, lz2m + buffer_len) if end == 0 else (lz2p - buffer_len, lz2p)
+            vol = lx * ly * (hi - lo)
+            dens = conc * (AVOGADRO / LITER) * ANGSTROM ** 3
+            ib_types.append(t)
+            lzmin.append(lo)
+            lzmax.append(hi)
+            avnum.append(dens * vol)
+            battery = volt if end == 1 else 0.0
+            mus.append(mu[k] + s.cg[t] * battery * COULOMB / KCALMOL)
+    nb = len(ib_types)
+    return Gcmc(ibfftyp=np.array(ib_types, np.int32), avnum=np.array(avnum), mu=np.array(mus),
+                kb=np.zeros(nb), lzmin=np.array(lzmin), lzmax=np.array(lzmax),
+                bufferbias=np.zeros(nb, np.int8),
+                ntotat=np.array([0] + [int(round(a)) for a in avnum[1:]], np.int32),
+                lx=lx, ly=ly, cx=0.0, cy=0.0, cz=0.0, ikbt=1.0 / s.meta["kbt"])
+
+
+def assign_buffers(s: System, g: Gcmc):
+    """ibuffer of each particle: the buffer whose slab holds it and whose type matches
+    (0 if none), as the reference's initial count does (count.f90:39-92)."""
+    ib = np.zeros(s.capacity, np.int32)
+    for i in range(s.nsites + s.nfix, s.ntot):
+        t = abs(int(s.typei[i]))
+        for b in range(1, g.nbuffer + 1):
+            if g.ibfftyp[b] == t and g.lzmin[b] <= s.z[i] <= g.lzmax[b]:
+                ib[i] = b
+                break
+    nat = np.zeros(g.nbuffer + 1, np.int32)
+    for b in range(1, g.nbuffer + 1):
+        nat[b] = int(np.sum(ib[:s.ntot] == b))
+    return ib, nat

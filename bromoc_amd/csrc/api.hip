@@ -1,0 +1,994 @@
+// api.hip -- C ABI of libbromoc_gpu.so (declared in include/bromoc_gpu.h).
+//
+// Host side of the gfx950 path: validates the reference state handed over by the caller
+// (every index a kernel will form is bounds-checked here, before any launch), lays it out
+// in HBM, and drives the kernels of kernels.hip.  GRAND's accept/reject loop runs here
+// around the INTERACT kernel, in the reference's draw order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bromoc_gpu.h"
+#include "engine.h"
+
+using namespace bromoc;
+
+namespace {
+
+constexpr uint32_t SUPPORTED = BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BROMOC_QNOBOND |
+                               BROMOC_QPHIX | BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV |
+                               BROMOC_QDEBY | BROMOC_QCOUNTION | BROMOC_QDNAFREE;
+
+// constamod.f90:21-32 (binary64)
+double celec_value() {
+    const double pi = 3.14159265358979323846264338327950288419716939937510;
+    const double kcalmol = 4184.0 / 6.0221412927E23;
+    const double coulomb = 1.60217656535E-19, epermit = 8.854187817620E-12;
+    return coulomb * coulomb / (4.0 * pi * epermit * kcalmol * 1E-10);
+}
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct bromoc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bromoc_system sys{};       // scalars (pointer members are not kept)
+    int cap = 0, ntot = 0, ns = 0, nf = 0;
+    // device state
+    double *x[2] = {nullptr, nullptr}, *y[2] = {nullptr, nullptr}, *z[2] = {nullptr, nullptr};
+    int cur = 0;
+    int32_t *typei = nullptr, *t2 = nullptr, *namsite = nullptr, *nwtype = nullptr;
+    int8_t *stfree = nullptr;
+    double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    double *epart = nullptr, *ered = nullptr;
+    int32_t *pairidx = nullptr;
+    DevPair *pairs = nullptr;
+    double4 *ep = nullptr;
+    double *cg = nullptr, *fact1a = nullptr, *fact2a = nullptr, *scal = nullptr, *zcont = nullptr;
+    int32_t *inc_ptr = nullptr;
+    uint32_t *inc = nullptr;
+    int32_t *lst[8] = {};
+    double *lval[6] = {};
+    float *phix = nullptr;
+    int8_t *phiv = nullptr;
+    int32_t *warn = nullptr, *nfwd = nullptr, *nbwd = nullptr;
+    double *noise = nullptr;
+    int noise_rows = 0;
+    int64_t noise_stride = 0;
+    int64_t *ctl = nullptr;     // [step, row0, counter0, seed]
+    double *iout = nullptr;
+    std::vector<void *> allocs;
+    // host mirrors
+    std::vector<int32_t> h_typei, h_nwtype, h_typtyp;
+    std::vector<double> h_cg;
+    // GCMC
+    bool have_gcmc = false;
+    bromoc_gcmc g{};
+    std::vector<int32_t> ibfftyp, ntotat, ibuffer;
+    std::vector<double> avnum, mu, kb, lzmin, lzmax;
+    std::vector<int8_t> bufferbias;
+    StepParams base{};
+};
+
+namespace {
+
+int fail(bromoc_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail((c), BROMOC_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(bromoc_ctx *c, T **p, size_t n) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    c->allocs.push_back(q);
+    *p = (T *)q;
+    return hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+template <typename T>
+hipError_t dupload(bromoc_ctx *c, T **p, const T *src, size_t n) {
+    hipError_t e = dalloc(c, p, n);
+    if (e != hipSuccess || n == 0 || !src) return e;
+    return hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice);
+}
+
+void energies_from(const double *r, bromoc_energies *out) {
+    bromoc_energies E;
+    double *f = &E.eelec;
+    for (int k = 0; k < NE; k++) f[k] = r[k];
+    // energy.mono.f90:83-115,689 (ememb and egsbpb stay zero on this path)
+    E.ener = E.ememb;
+    E.ener = E.ener + E.egsbpa;
+    E.ener = E.ener + E.evdwgd;
+    E.ener = E.ener + E.eelec + E.evdw + E.esrpmf + E.esrpmfmx + E.ebond + E.eang + E.edihe +
+             E.estack + E.ebp + E.eex + E.eqq + E.esolv + E.eqqmx + E.evdwmx + E.eefpot +
+             E.eefpotmx + E.econ;
+    *out = E;
+}
+
+int validate(const bromoc_system *s, std::string &msg) {
+    auto bad = [&](const char *m) { msg = m; return BROMOC_E_ARG; };
+    if (s->abi_version != BROMOC_ABI_VERSION) return bad("abi_version mismatch");
+    if (s->flags & ~SUPPORTED) {
+        msg = "unsupported reference option in flags";
+        return BROMOC_E_UNSUPPORTED;
+    }
+    if (s->ntot < 0 || s->nsites < 0 || s->nfix < 0 || s->nsites + s->nfix > s->ntot)
+        return bad("particle counts");
+    if (s->capacity < s->ntot || s->capacity > (1 << 26)) return bad("capacity");
+    if (!s->x || !s->y || !s->z || !s->typei) return bad("positions/typei missing");
+    if (s->ntype < 1 || s->nttyp < 1 || s->npair < 0) return bad("type counts");
+    if (s->nttyp + 1 > MAX_LDS_TYPES) return bad("too many atnam2 types for the LDS pair table");
+    if (s->npair + 1 > MAX_LDS_PAIRS) return bad("too many pair types for LDS");
+    if (!s->nwtype || !s->cg || !s->fact1a || !s->fact2a || !s->pairidx || !s->pairs)
+        return bad("type tables missing");
+    if (s->nsites > 0 && (!s->typtyp || !s->namsite || !s->stfree)) return bad("site arrays");
+    if ((s->flags & BROMOC_QNUCL) == 0 && s->nsites > 0) return bad("sites without QNUCL");
+    const int nt1 = s->nttyp + 1;
+    for (int i = 0; i < nt1 * nt1; i++)
+        if (s->pairidx[i] < 0 || s->pairidx[i] > s->npair) return bad("pairidx out of range");
+    for (int t = 1; t <= s->npair; t++) {
+        const bromoc_pair &p = s->pairs[t];
+        if (!p.efpot) continue;
+        if (!s->ep || p.nxi < 0 || p.nknots < 2 || (int64_t)p.nxi + p.nknots > s->nknots)
+            return bad("pair table outside ep");
+        if (!(s->ires > 0.0) || !(p.dm2lo > 0.0) || !(p.dm2hi >= p.dm2lo)) return bad("table range");
+        // largest index the lookup can form (polynom.f90:226): x <= sqrt(dm2hi)
+        double xmax = std::sqrt(p.dm2hi);
+        long long imax = (long long)((xmax - p.dmi) * s->ires);
+        if (imax < 0 || imax >= p.nknots) return bad("table range exceeds its knots");
+    }
+    std::vector<char> site_t2(nt1, 0), ion_t2(nt1, 0);
+    for (int i = 0; i < s->nsites; i++) {
+        if (s->typtyp[i] < 1 || s->typtyp[i] > s->nttyp) return bad("typtyp out of range");
+        if (s->namsite[i] < 0 || s->namsite[i] > 5) return bad("namsite out of range");
+        site_t2[s->typtyp[i]] = 1;
+    }
+    for (int i = s->nsites; i < s->ntot; i++) {
+        int t = std::abs(s->typei[i]);
+        if (t < 1 || t > s->ntype) return bad("typei out of range");
+        int t2 = s->nwtype[t];
+        if (t2 < 1 || t2 > s->nttyp) return bad("nwtype out of range");
+        ion_t2[t2] = 1;
+    }
+    // every ion type may meet every site type and every ion type
+    for (int a = 1; a < nt1; a++) {
+        if (!ion_t2[a]) continue;
+        for (int b = 1; b < nt1; b++)
+            if ((ion_t2[b] || site_t2[b]) && s->pairidx[a * nt1 + b] == 0)
+                return bad("pairidx has no pair type for a present type pair");
+    }
+    // ion types reachable later (GCMC inserts any ion type): every ion atnam type must have
+    // a pair type with every site type present and with every ion type
+    std::vector<char> any_ion_t2(nt1, 0);
+    for (int t = s->nold + 1; t <= s->ntype; t++) {
+        int t2 = s->nwtype[t];
+        if (t2 < 1 || t2 > s->nttyp) return bad("nwtype of an ion type out of range");
+        any_ion_t2[t2] = 1;
+    }
+    for (int a = 1; a < nt1; a++) {
+        if (!any_ion_t2[a]) continue;
+        for (int b = 1; b < nt1; b++)
+            if ((any_ion_t2[b] || site_t2[b]) && s->pairidx[a * nt1 + b] == 0)
+                return bad("pairidx incomplete for ion types");
+    }
+    for (int i = s->nsites; i < s->ntot; i++)
+        if (std::abs(s->typei[i]) <= s->nold) return bad("an ion carries a nucleotide type");
+    const int32_t *lp[8] = {s->sitebond, s->siteangle, s->sitedihe, s->sitestack,
+                            s->sitebp, s->siteex, s->siteqq, s->siteslv};
+    const int cnt[8] = {s->nbond, s->nangle, s->ndihe, s->nstack, s->nbp, s->nex, s->nqq, s->nsolv};
+    const int ar[8] = {2, 3, 4, 2, 2, 2, 2, 2};
+    if (s->flags & BROMOC_QNUCL) {
+        for (int k = 0; k < 8; k++) {
+            if (cnt[k] < 0 || cnt[k] >= (1 << 27)) return bad("list size");
+            if (cnt[k] && !lp[k]) return bad("list pointer missing");
+            for (int64_t q = 0; q < (int64_t)cnt[k] * ar[k]; q++)
+                if (lp[k][q] < 0 || lp[k][q] >= s->nsites) return bad("site list index out of range");
+        }
+        const double *vp[6] = {s->distbond, s->valangle, s->valdihe, s->sgstack, s->sgbp, s->sgex};
+        for (int k = 0; k < 6; k++)
+            if (cnt[k] && !vp[k]) return bad("list values missing");
+    }
+    const bromoc_grid *gs[2] = {&s->phix, &s->phiv};
+    const uint32_t gf[2] = {BROMOC_QPHIX, BROMOC_QPHIV};
+    for (int k = 0; k < 2; k++) {
+        if (!(s->flags & gf[k])) continue;
+        const bromoc_grid &g = *gs[k];
+        if (!g.data || g.nclx < 2 || g.ncly < 2 || g.nclz < 2 || !(g.dcel > 0.0))
+            return bad("grid geometry");
+    }
+    if (s->flags & BROMOC_QPHIV) {
+        double tz = 0.5 * (s->phiv.nclz - 1) * s->phiv.dcel;
+        if (s->vzmax > s->phiv.zbcen + tz || s->vzmin < s->phiv.zbcen - tz)
+            return bad("vzmin/vzmax outside the repulsion map");
+        if ((s->flags & BROMOC_QSVDW) && !s->scal) return bad("scal missing");
+    }
+    if ((s->flags & BROMOC_QCOUNTION) && (s->cntpts < 0 || (s->cntpts > 0 && !s->zcont)))
+        return bad("flux planes");
+    return BROMOC_OK;
+}
+
+GridGeom geom_of(const bromoc_grid &g, bool on) {
+    GridGeom r{};
+    r.nx = g.nclx; r.ny = g.ncly; r.nz = g.nclz; r.on = on;
+    r.dcel = g.dcel;
+    r.idcel = 1.0 / g.dcel;
+    r.xbcen = g.xbcen; r.ybcen = g.ybcen; r.zbcen = g.zbcen;
+    // readphi.f90:49-51
+    r.tranx = 0.5 * (g.nclx - 1) * g.dcel;
+    r.trany = 0.5 * (g.ncly - 1) * g.dcel;
+    r.tranz = 0.5 * (g.nclz - 1) * g.dcel;
+    return r;
+}
+
+StepParams params_for(bromoc_ctx *c) {
+    StepParams p = c->base;
+    p.ntot = c->ntot;
+    p.ntargets = c->ns + (c->ntot - c->ns - c->nf);
+    p.x = c->x[c->cur]; p.y = c->y[c->cur]; p.z = c->z[c->cur];
+    p.xo = c->x[c->cur ^ 1]; p.yo = c->y[c->cur ^ 1]; p.zo = c->z[c->cur ^ 1];
+    return p;
+}
+
+int sync_err(bromoc_ctx *c) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, BROMOC_E_HIP, std::string("kernel: ") + hipGetErrorString(e));
+    return BROMOC_OK;
+}
+
+int reduce_energies(bromoc_ctx *c, bromoc_energies *out) {
+    HIPCHK(c, launch_reduce_energy(c->epart, c->cap, c->ntot, c->ered, c->stream));
+    double r[NE];
+    HIPCHK(c, hipMemcpyAsync(r, c->ered, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    int st = sync_err(c);
+    if (st) return st;
+    energies_from(r, out);
+    return BROMOC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *bromoc_last_error(const bromoc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int bromoc_destroy(bromoc_ctx *c) {
+    if (!c) return BROMOC_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BROMOC_OK;
+}
+
+int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
+    if (!s || !out) return BROMOC_E_ARG;
+    *out = nullptr;
+    std::string msg;
+    int st = validate(s, msg);
+    if (st) {
+        fprintf(stderr, "bromoc_create: %s\n", msg.c_str());
+        return st;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev)
+        return BROMOC_E_NODEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return BROMOC_E_NODEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        fprintf(stderr, "bromoc_create: device %d is %s, this library is built for gfx950\n",
+                device, prop.gcnArchName);
+        return BROMOC_E_NODEVICE;
+    }
+    bromoc_ctx *c = new bromoc_ctx();
+    c->device = device;
+    c->sys = *s;
+    c->cap = s->capacity;
+    c->ntot = s->ntot;
+    c->ns = s->nsites;
+    c->nf = s->nfix;
+    auto cleanup = [&](int code) { c->err = c->err.empty() ? "create failed" : c->err; bromoc_destroy(c); return code; };
+    if (hipSetDevice(device) != hipSuccess) return cleanup(BROMOC_E_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(BROMOC_E_HIP);
+    const int cap = c->cap, ns = c->ns, nt1 = s->nttyp + 1;
+#define ACHK(expr) do { if ((expr) != hipSuccess) return cleanup(BROMOC_E_HIP); } while (0)
+    for (int b = 0; b < 2; b++) {
+        ACHK(dalloc(c, &c->x[b], cap));
+        ACHK(dalloc(c, &c->y[b], cap));
+        ACHK(dalloc(c, &c->z[b], cap));
+        ACHK(hipMemcpy(c->x[b], s->x, sizeof(double) * s->ntot, hipMemcpyHostToDevice));
+        ACHK(hipMemcpy(c->y[b], s->y, sizeof(double) * s->ntot, hipMemcpyHostToDevice));
+        ACHK(hipMemcpy(c->z[b], s->z, sizeof(double) * s->ntot, hipMemcpyHostToDevice));
+    }
+    c->h_typei.assign(s->typei, s->typei + s->ntot);
+    c->h_typei.resize(cap, 0);
+    c->h_nwtype.assign(s->nwtype, s->nwtype + s->ntype + 1);
+    c->h_cg.assign(s->cg, s->cg + s->ntype + 1);
+    if (ns) c->h_typtyp.assign(s->typtyp, s->typtyp + ns);
+    std::vector<int32_t> t2(cap, 0);
+    for (int i = 0; i < s->ntot; i++)
+        t2[i] = i < ns ? s->typtyp[i] : s->nwtype[std::abs(s->typei[i])];
+    ACHK(dupload(c, &c->typei, c->h_typei.data(), cap));
+    ACHK(dupload(c, &c->t2, t2.data(), cap));
+    ACHK(dupload(c, &c->stfree, ns ? s->stfree : (const int8_t *)nullptr, ns));
+    ACHK(dupload(c, &c->namsite, ns ? s->namsite : (const int32_t *)nullptr, ns));
+    ACHK(dupload(c, &c->nwtype, s->nwtype, s->ntype + 1));
+    ACHK(dupload(c, &c->cg, s->cg, s->ntype + 1));
+    ACHK(dupload(c, &c->fact1a, s->fact1a, s->ntype + 1));
+    ACHK(dupload(c, &c->fact2a, s->fact2a, s->ntype + 1));
+    ACHK(dalloc(c, &c->fx, cap));
+    ACHK(dalloc(c, &c->fy, cap));
+    ACHK(dalloc(c, &c->fz, cap));
+    ACHK(dalloc(c, &c->epart, (size_t)NE * cap));
+    ACHK(dalloc(c, &c->ered, NE));
+    ACHK(dupload(c, &c->pairidx, s->pairidx, (size_t)nt1 * nt1));
+    std::vector<DevPair> dp(s->npair + 1);
+    for (int t = 0; t <= s->npair; t++) {
+        const bromoc_pair &p = s->pairs[t];
+        DevPair &d = dp[t];
+        std::memset(&d, 0, sizeof(d));
+        if (t == 0) continue;
+        d.dmi = p.dmi; d.dm2lo = p.dm2lo; d.dm2hi = p.dm2hi;
+        d.sc1 = p.sc1; d.sc2 = p.sc2; d.sc3 = p.sc3; d.fct = p.fct;
+        d.epp4 = p.epp4; d.sgp2 = p.sgp2;
+        d.c0 = p.c0; d.c1 = p.c1; d.c2 = p.c2; d.c3 = p.c3; d.c4 = p.c4;
+        d.nxi = p.nxi;
+        d.flags = (p.efpot ? PF_EFPOT : 0) | (p.chr ? PF_CHR : 0) | (p.lj ? PF_LJ : 0) |
+                  (p.srpmf ? PF_SRPMF : 0);
+    }
+    ACHK(dupload(c, &c->pairs, dp.data(), dp.size()));
+    // two zero knots of padding: a lookup that rounds one knot past a table's end (a 1-ulp
+    // difference in sqrt at x = r_n) stays inside the allocation
+    std::vector<double4> ep4(std::max(s->nknots, 0) + 2, make_double4(0.0, 0.0, 0.0, 0.0));
+    for (int k = 0; k < s->nknots; k++)
+        ep4[k] = make_double4(s->ep[3 * k], s->ep[3 * k + 1], s->ep[3 * k + 2], 0.0);
+    ACHK(dupload(c, &c->ep, ep4.data(), ep4.size()));
+    // CG-DNA: component-major lists and the per-site incidence (CSR) of every term
+    const int32_t *lp[8] = {s->sitebond, s->siteangle, s->sitedihe, s->sitestack,
+                            s->sitebp, s->siteex, s->siteqq, s->siteslv};
+    const int cnt[8] = {s->nbond, s->nangle, s->ndihe, s->nstack, s->nbp, s->nex, s->nqq, s->nsolv};
+    const int ar[8] = {2, 3, 4, 2, 2, 2, 2, 2};
+    const double *vp[6] = {s->distbond, s->valangle, s->valdihe, s->sgstack, s->sgbp, s->sgex};
+    std::vector<int32_t> deg(ns + 1, 0);
+    bool dna = (s->flags & BROMOC_QNUCL) && ns > 0;
+    for (int k = 0; k < 8; k++) {
+        int n = dna ? cnt[k] : 0;
+        ACHK(dupload(c, &c->lst[k], n ? lp[k] : (const int32_t *)nullptr, (size_t)n * ar[k]));
+        if (k < 6) ACHK(dupload(c, &c->lval[k], n ? vp[k] : (const double *)nullptr, n));
+        for (int64_t q = 0; q < (int64_t)n * ar[k]; q++) deg[lp[k][q] + 1]++;
+    }
+    for (int i = 0; i < ns; i++) deg[i + 1] += deg[i];
+    std::vector<uint32_t> inc(std::max<int64_t>(deg[ns], 1));
+    {
+        std::vector<int32_t> fill(deg.begin(), deg.end());
+        for (int k = 0; k < 8 && dna; k++)
+            for (int r = 0; r < ar[k]; r++)
+                for (int t = 0; t < cnt[k]; t++) {
+                    int site = lp[k][(int64_t)r * cnt[k] + t];
+                    inc[fill[site]++] = inc_pack(k, r, t);
+                }
+    }
+    ACHK(dupload(c, &c->inc_ptr, deg.data(), deg.size()));
+    ACHK(dupload(c, &c->inc, inc.data(), inc.size()));
+    if (s->flags & BROMOC_QPHIX)
+        ACHK(dupload(c, &c->phix, (const float *)s->phix.data,
+                     (size_t)s->phix.nclx * s->phix.ncly * s->phix.nclz));
+    if (s->flags & BROMOC_QPHIV)
+        ACHK(dupload(c, &c->phiv, (const int8_t *)s->phiv.data,
+                     (size_t)s->phiv.nclx * s->phiv.ncly * s->phiv.nclz));
+    std::vector<double> scal(nt1, 1.0);
+    if ((s->flags & BROMOC_QSVDW) && s->scal) scal.assign(s->scal, s->scal + nt1);
+    ACHK(dupload(c, &c->scal, scal.data(), nt1));
+    ACHK(dalloc(c, &c->warn, nt1));
+    const int nion_t = std::max(1, s->ntype - s->nold), np = std::max(1, s->cntpts);
+    ACHK(dalloc(c, &c->nfwd, (size_t)nion_t * np));
+    ACHK(dalloc(c, &c->nbwd, (size_t)nion_t * np));
+    ACHK(dupload(c, &c->zcont, s->cntpts > 0 ? s->zcont : (const double *)nullptr,
+                 std::max(0, s->cntpts)));
+    ACHK(dalloc(c, &c->ctl, 4));
+    ACHK(dalloc(c, &c->iout, 1));
+#undef ACHK
+
+    StepParams &p = c->base;
+    p.ntot = s->ntot; p.nsites = ns; p.nfix = s->nfix;
+    p.nttyp = s->nttyp; p.ntype = s->ntype; p.nold = s->nold; p.npair = s->npair;
+    p.flags = s->flags;
+    p.typei = c->typei; p.t2 = c->t2; p.stfree = c->stfree; p.namsite = c->namsite;
+    p.fx = c->fx; p.fy = c->fy; p.fz = c->fz;
+    p.epart = c->epart; p.ecap = cap;
+    p.pairidx = c->pairidx; p.pairs = c->pairs; p.ep = c->ep;
+    p.ires = s->ires; p.celec = celec_value();
+    p.cg = c->cg; p.fact1a = c->fact1a; p.fact2a = c->fact2a; p.nwtype = c->nwtype;
+    p.rth = s->rth; p.srpx = s->srpx; p.srpk = s->srpk; p.srpy = s->srpy;
+    p.inc_ptr = c->inc_ptr; p.inc = c->inc;
+    for (int k = 0; k < 8; k++) { p.lst[k] = c->lst[k]; p.nlist[k] = dna ? cnt[k] : 0; }
+    for (int k = 0; k < 6; k++) p.lval[k] = c->lval[k];
+    p.epsnuc = s->epsnuc; p.scalepairing = s->scalepairing; p.fctn = s->fctn;
+    p.kappa = s->kappa; p.ikappa = s->ikappa; p.epsolv = s->epsolv; p.cgnuc = s->cgnuc;
+    p.gx = geom_of(s->phix, (s->flags & BROMOC_QPHIX) != 0);
+    p.gv = geom_of(s->phiv, (s->flags & BROMOC_QPHIV) != 0);
+    p.phix = c->phix; p.phiv = c->phiv;
+    p.svdw = s->svdw; p.thold8 = s->thold8; p.vzmin = s->vzmin; p.vzmax = s->vzmax;
+    p.scal = c->scal; p.warn = c->warn;
+    p.bdmax = s->bdmax; p.fact0n1 = s->fact0n1; p.fact0n2 = s->fact0n2;
+    p.lx2m = s->lx2m; p.lx2p = s->lx2p; p.ly2m = s->ly2m; p.ly2p = s->ly2p;
+    p.lz2m = s->lz2m; p.lz2p = s->lz2p;
+    p.cntpts = (s->flags & BROMOC_QCOUNTION) ? s->cntpts : 0;
+    p.zcont = c->zcont; p.nfwd = c->nfwd; p.nbwd = c->nbwd;
+    p.step_ctr = c->ctl;
+    p.stream = 0;
+    *out = c;
+    return BROMOC_OK;
+}
+
+int bromoc_set_positions(bromoc_ctx *c, int32_t ntot, const double *x, const double *y,
+                         const double *z, const int32_t *typei) {
+    if (!c || !x || !y || !z) return BROMOC_E_ARG;
+    if (ntot < c->ns + c->nf || ntot > c->cap) return fail(c, BROMOC_E_ARG, "ntot out of range");
+    (void)hipSetDevice(c->device);
+    std::vector<int32_t> t2(ntot);
+    if (typei) {
+        for (int i = c->ns; i < ntot; i++) {
+            int t = std::abs(typei[i]);
+            if (t <= c->sys.nold || t > c->sys.ntype) return fail(c, BROMOC_E_ARG, "typei out of range");
+            int tt = c->h_nwtype[t];
+            if (tt < 1 || tt > c->sys.nttyp) return fail(c, BROMOC_E_ARG, "nwtype out of range");
+        }
+        for (int i = c->ns; i < ntot; i++) c->h_typei[i] = typei[i];
+    } else if (ntot > c->ntot) {
+        return fail(c, BROMOC_E_ARG, "typei required when ntot grows");
+    }
+    for (int i = 0; i < ntot; i++)
+        t2[i] = i < c->ns ? c->h_typtyp[i] : c->h_nwtype[std::abs(c->h_typei[i])];
+    for (int b = 0; b < 2; b++) {
+        HIPCHK(c, hipMemcpyAsync(c->x[b], x, sizeof(double) * ntot, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->y[b], y, sizeof(double) * ntot, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->z[b], z, sizeof(double) * ntot, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->typei, c->h_typei.data(), sizeof(int32_t) * ntot, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->t2, t2.data(), sizeof(int32_t) * ntot, hipMemcpyHostToDevice, c->stream));
+    c->ntot = ntot;
+    return sync_err(c);
+}
+
+int bromoc_get_positions(bromoc_ctx *c, double *x, double *y, double *z, int32_t *typei,
+                         int32_t *ntot) {
+    if (!c) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    const size_t n = c->ntot;
+    if (x) HIPCHK(c, hipMemcpyAsync(x, c->x[c->cur], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (y) HIPCHK(c, hipMemcpyAsync(y, c->y[c->cur], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (z) HIPCHK(c, hipMemcpyAsync(z, c->z[c->cur], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    int st = sync_err(c);
+    if (typei) std::memcpy(typei, c->h_typei.data(), sizeof(int32_t) * n);
+    if (ntot) *ntot = c->ntot;
+    return st;
+}
+
+int bromoc_get_forces(bromoc_ctx *c, double *fx, double *fy, double *fz) {
+    if (!c || !fx || !fy || !fz) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    const size_t n = c->ntot;
+    HIPCHK(c, hipMemcpyAsync(fx, c->fx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(fy, c->fy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(fz, c->fz, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    return sync_err(c);
+}
+
+int bromoc_energy(bromoc_ctx *c, int32_t want_forces, bromoc_energies *out) {
+    if (!c) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    StepParams p = params_for(c);
+    p.mode = 0;
+    p.write_energy = 1;
+    p.write_forces = want_forces ? 1 : 0;
+    HIPCHK(c, launch_step(p, c->stream));
+    bromoc_energies E;
+    int st = reduce_energies(c, &E);
+    if (st) return st;
+    if (out) *out = E;
+    return BROMOC_OK;
+}
+
+int bromoc_upload_noise(bromoc_ctx *c, int32_t nsteps, const double *noise) {
+    if (!c || nsteps <= 0 || !noise) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    const int64_t stride = 3 * (int64_t)c->ntot;
+    const size_t n = (size_t)nsteps * stride;
+    if (c->noise && (size_t)c->noise_rows * c->noise_stride < n) {
+        (void)hipFree(c->noise);
+        c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void *)c->noise));
+        c->noise = nullptr;
+    }
+    if (!c->noise) HIPCHK(c, dalloc(c, &c->noise, n));
+    HIPCHK(c, hipMemcpy(c->noise, noise, n * sizeof(double), hipMemcpyHostToDevice));
+    c->noise_rows = nsteps;
+    c->noise_stride = stride;
+    return BROMOC_OK;
+}
+
+static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t first_row,
+                     uint64_t seed, uint64_t counter0, bool want_energy) {
+    if (noise_mode != BROMOC_NOISE_INJECTED && noise_mode != BROMOC_NOISE_PHILOX)
+        return fail(c, BROMOC_E_ARG, "noise_mode");
+    if (noise_mode == BROMOC_NOISE_INJECTED) {
+        if (!c->noise) return fail(c, BROMOC_E_STATE, "no injected noise uploaded");
+        if (first_row < 0 || (int64_t)first_row + nsteps > c->noise_rows)
+            return fail(c, BROMOC_E_STATE, "noise rows out of range");
+        if (3 * (int64_t)c->ntot > c->noise_stride)
+            return fail(c, BROMOC_E_STATE, "ntot grew beyond the uploaded noise rows");
+    }
+    int64_t ctl[4] = {0, first_row, (int64_t)counter0, (int64_t)seed};
+    HIPCHK(c, hipMemcpyAsync(c->ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice, c->stream));
+    StepParams p = params_for(c);
+    p.mode = 1;
+    p.noise_mode = noise_mode;
+    p.noise = c->noise;
+    p.noise_stride = c->noise_stride;
+    for (int s = 0; s < nsteps; s++) {
+        const bool last = s == nsteps - 1;
+        p.x = c->x[c->cur]; p.y = c->y[c->cur]; p.z = c->z[c->cur];
+        p.xo = c->x[c->cur ^ 1]; p.yo = c->y[c->cur ^ 1]; p.zo = c->z[c->cur ^ 1];
+        p.step_off = s;
+        p.write_energy = (last && want_energy) ? 1 : 0;
+        p.write_forces = last ? 1 : 0;
+        HIPCHK(c, launch_step(p, c->stream));
+        c->cur ^= 1;
+    }
+    return BROMOC_OK;
+}
+
+int bromoc_step(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t first_row,
+                uint64_t seed, uint64_t counter0, bromoc_energies *energies) {
+    if (!c || nsteps < 0) return BROMOC_E_ARG;
+    if (nsteps == 0) return BROMOC_OK;
+    (void)hipSetDevice(c->device);
+    int st = run_steps(c, nsteps, noise_mode, first_row, seed, counter0, energies != nullptr);
+    if (st) return st;
+    if (energies) return reduce_energies(c, energies);
+    return sync_err(c);
+}
+
+// nsteps BD steps with a HIP event pair around every launch, on the stream the kernels run
+// on: *total_ms is first-to-last, *kernel_ms the mean launch duration.
+int bromoc_time_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t first_row,
+                      uint64_t seed, uint64_t counter0, double *total_ms, double *kernel_ms) {
+    if (!c || nsteps <= 0 || !total_ms || !kernel_ms) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
+    for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+    if (noise_mode == BROMOC_NOISE_INJECTED) {
+        if (!c->noise || first_row < 0 || (int64_t)first_row + nsteps > c->noise_rows ||
+            3 * (int64_t)c->ntot > c->noise_stride)
+            return fail(c, BROMOC_E_STATE, "noise rows out of range");
+    }
+    int64_t ctl[4] = {0, first_row, (int64_t)counter0, (int64_t)seed};
+    HIPCHK(c, hipMemcpyAsync(c->ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice, c->stream));
+    StepParams p = params_for(c);
+    p.mode = 1;
+    p.noise_mode = noise_mode;
+    p.noise = c->noise;
+    p.noise_stride = c->noise_stride;
+    for (int s = 0; s < nsteps; s++) {
+        p.x = c->x[c->cur]; p.y = c->y[c->cur]; p.z = c->z[c->cur];
+        p.xo = c->x[c->cur ^ 1]; p.yo = c->y[c->cur ^ 1]; p.zo = c->z[c->cur ^ 1];
+        p.step_off = s;
+        p.write_energy = 0;
+        p.write_forces = s == nsteps - 1;
+        HIPCHK(c, hipEventRecord(ev[2 * s], c->stream));
+        HIPCHK(c, launch_step(p, c->stream));
+        HIPCHK(c, hipEventRecord(ev[2 * s + 1], c->stream));
+        c->cur ^= 1;
+    }
+    int st = sync_err(c);
+    if (st) return st;
+    double sum = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
+        sum += ms;
+    }
+    float tot = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&tot, ev[0], ev[2 * nsteps - 1]));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    *total_ms = tot;
+    *kernel_ms = sum / nsteps;
+    return BROMOC_OK;
+}
+
+int bromoc_dynamics(bromoc_ctx *c, int32_t noise_mode, int32_t row, uint64_t seed,
+                    uint64_t counter) {
+    if (!c) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    // The forces of the current positions are recomputed in the same fixed order as the
+    // preceding ENERGY, so the update sees bit-identical forces.
+    int st = run_steps(c, 1, noise_mode, row, seed, counter, false);
+    if (st) return st;
+    return sync_err(c);
+}
+
+int bromoc_get_flux(bromoc_ctx *c, int32_t *nforward, int32_t *nbackward) {
+    if (!c || !nforward || !nbackward) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    const size_t n = (size_t)std::max(0, c->sys.ntype - c->sys.nold) * std::max(0, c->sys.cntpts);
+    if (n == 0) return BROMOC_OK;
+    HIPCHK(c, hipMemcpyAsync(nforward, c->nfwd, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(nbackward, c->nbwd, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    return sync_err(c);
+}
+
+int bromoc_get_warn(bromoc_ctx *c, int32_t *warn) {
+    if (!c || !warn) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpyAsync(warn, c->warn, (c->sys.nttyp + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    return sync_err(c);
+}
+
+static int interact_async(bromoc_ctx *c, double xj, double yj, double zj, int32_t jtype, int32_t j,
+                          int32_t alert) {
+    const bool beion = (c->sys.flags & BROMOC_QPAR) && j >= c->ns;
+    if (j < 0 || j > c->ntot) return fail(c, BROMOC_E_ARG, "j out of range");
+    if (beion) {
+        if (jtype < 1 || jtype > c->sys.ntype) return fail(c, BROMOC_E_ARG, "jtype out of range");
+        int t2 = c->h_nwtype[jtype];
+        if (t2 < 1 || t2 > c->sys.nttyp) return fail(c, BROMOC_E_ARG, "nwtype(jtype) out of range");
+    } else if (j >= c->ns) {
+        return fail(c, BROMOC_E_ARG, "j is not a site and QPAR is off");
+    }
+    InteractParams q{};
+    q.sp = params_for(c);
+    q.xj = xj; q.yj = yj; q.zj = zj;
+    q.jtype = jtype; q.j = j; q.alert = alert;
+    q.out = c->iout;
+    HIPCHK(c, launch_interact(q, c->stream));
+    return BROMOC_OK;
+}
+
+int bromoc_interact(bromoc_ctx *c, double xj, double yj, double zj, int32_t jtype, int32_t j,
+                    int32_t alert, double *dener) {
+    if (!c || !dener) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    int st = interact_async(c, xj, yj, zj, jtype, j, alert);
+    if (st) return st;
+    HIPCHK(c, hipMemcpyAsync(dener, c->iout, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return sync_err(c);
+}
+
+int bromoc_set_gcmc(bromoc_ctx *c, const bromoc_gcmc *g) {
+    if (!c || !g || g->nbuffer < 0) return BROMOC_E_ARG;
+    const int nb = g->nbuffer;
+    if (nb > 0 && (!g->ibfftyp || !g->avnum || !g->mu || !g->kb || !g->lzmin || !g->lzmax ||
+                   !g->bufferbias || !g->ntotat))
+        return fail(c, BROMOC_E_ARG, "gcmc arrays missing");
+    for (int b = 1; b <= nb; b++)
+        if (g->ibfftyp[b] <= c->sys.nold || g->ibfftyp[b] > c->sys.ntype)
+            return fail(c, BROMOC_E_ARG, "buffer ion type out of range");
+    c->g = *g;
+    c->ibfftyp.assign(g->ibfftyp, g->ibfftyp + nb + 1);
+    c->ntotat.assign(g->ntotat, g->ntotat + nb + 1);
+    c->avnum.assign(g->avnum, g->avnum + nb + 1);
+    c->mu.assign(g->mu, g->mu + nb + 1);
+    c->kb.assign(g->kb, g->kb + nb + 1);
+    c->lzmin.assign(g->lzmin, g->lzmin + nb + 1);
+    c->lzmax.assign(g->lzmax, g->lzmax + nb + 1);
+    c->bufferbias.assign(g->bufferbias, g->bufferbias + nb + 1);
+    c->ibuffer.assign(c->cap + 1, 0);
+    c->have_gcmc = true;
+    return BROMOC_OK;
+}
+
+int bromoc_set_ibuffer(bromoc_ctx *c, const int32_t *ib) {
+    if (!c || !ib || !c->have_gcmc) return BROMOC_E_ARG;
+    for (int i = 0; i < c->ntot; i++)
+        if (ib[i] < 0 || ib[i] > c->g.nbuffer) return fail(c, BROMOC_E_ARG, "ibuffer out of range");
+    std::copy(ib, ib + c->ntot, c->ibuffer.begin());
+    return BROMOC_OK;
+}
+
+int bromoc_get_ibuffer(bromoc_ctx *c, int32_t *ib) {
+    if (!c || !ib || !c->have_gcmc) return BROMOC_E_ARG;
+    std::copy(c->ibuffer.begin(), c->ibuffer.begin() + c->ntot, ib);
+    return BROMOC_OK;
+}
+
+// GRAND, grand.f90:40-118 (INSERT insert.f90:29-41 rectangular; FIND find.f90:26-32)
+int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int64_t *used,
+                 int32_t *nat, int32_t *prob, int64_t icycle, double *ener) {
+    if (!c || !u || !nat || !prob || !ener || ngcmc < 0 || icycle <= 0) return BROMOC_E_ARG;
+    if (!c->have_gcmc) return fail(c, BROMOC_E_STATE, "bromoc_set_gcmc first");
+    (void)hipSetDevice(c->device);
+    const bromoc_gcmc &g = c->g;
+    const int nbuffer = g.nbuffer;
+    const double nbufferr = (double)nbuffer;
+    const int pstride = c->cap + 1;
+    int64_t k = 0;
+    auto rndm = [&](double &v) {
+        if (k >= nu) return false;
+        v = u[k++];
+        return true;
+    };
+#define RNDM(v) do { if (!rndm(v)) return fail(c, BROMOC_E_ARG, "uniform stream exhausted"); } while (0)
+    double dener = 0.0;
+    for (int ig = 0; ig < ngcmc; ig++) {
+        double r0;
+        RNDM(r0);
+        if (r0 < 0.5) {
+            double r1;
+            RNDM(r1);
+            const int ib = (int)(nbufferr * r1) + 1;
+            if (ib <= nbuffer && c->avnum[ib] > 0.0) {
+                const int itype = c->ibfftyp[ib];
+                double a, b, d;
+                RNDM(a); RNDM(b); RNDM(d);
+                const double xn = g.lx * (a - 0.5) + g.cx;
+                const double yn = g.ly * (b - 0.5) + g.cy;
+                const double zn = c->lzmin[ib] + (c->lzmax[ib] - c->lzmin[ib]) * d;
+                int st = bromoc_interact(c, xn, yn, zn, itype, c->ntot, 0, &dener);
+                if (st) return st;
+                double rate;
+                if (c->bufferbias[ib]) {
+                    rate = (c->avnum[ib] + c->kb[ib] * (c->avnum[ib] - (double)(c->ntotat[ib] / icycle)));
+                    rate = rate * std::exp(-(dener - c->mu[ib]) * g.ikbt) / (double)(nat[ib] + 1);
+                } else {
+                    rate = (c->avnum[ib] / (nat[ib] + 1)) * std::exp(-(dener - c->mu[ib]) * g.ikbt);
+                }
+                rate = rate / (1.0 + rate);
+                double r2;
+                RNDM(r2);
+                if (r2 <= rate) {
+                    if (c->ntot + 1 > c->cap) return fail(c, BROMOC_E_CAPACITY, "ntot exceeds capacity");
+                    *ener = *ener + dener;
+                    nat[ib] = nat[ib] + 1;
+                    const int n = c->ntot;
+                    const int32_t ti = zn < g.cz ? -itype : itype;
+                    const int32_t t2 = c->h_nwtype[itype];
+                    for (int bb = 0; bb < 2; bb++) {
+                        HIPCHK(c, hipMemcpyAsync(c->x[bb] + n, &xn, sizeof(double), hipMemcpyHostToDevice, c->stream));
+                        HIPCHK(c, hipMemcpyAsync(c->y[bb] + n, &yn, sizeof(double), hipMemcpyHostToDevice, c->stream));
+                        HIPCHK(c, hipMemcpyAsync(c->z[bb] + n, &zn, sizeof(double), hipMemcpyHostToDevice, c->stream));
+                    }
+                    HIPCHK(c, hipMemcpyAsync(c->typei + n, &ti, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+                    HIPCHK(c, hipMemcpyAsync(c->t2 + n, &t2, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+                    int st2 = sync_err(c);
+                    if (st2) return st2;
+                    c->h_typei[n] = ti;
+                    c->ibuffer[n] = ib;
+                    c->ntot = n + 1;
+                }
+            }
+        } else {
+            double r1, r2;
+            RNDM(r1);
+            const int ib = (int)(nbufferr * r1) + 1;
+            RNDM(r2);
+            const double natib = ib <= nbuffer ? (double)nat[ib] : 0.0;
+            const int ip = (int)(natib * r2) + 1;
+            if (ib <= nbuffer && ip <= nat[ib]) {
+                const int itype = c->ibfftyp[ib];
+                int iat = -1, ipx = 0;
+                for (int a = c->nf + c->ns; a < c->ntot; a++)
+                    if (c->ibuffer[a] == ib && ++ipx == ip) { iat = a; break; }
+                if (iat < 0) return fail(c, BROMOC_E_STATE, "buffer bookkeeping inconsistent");
+                double xa, ya, za;
+                HIPCHK(c, hipMemcpyAsync(&xa, c->x[c->cur] + iat, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(c, hipMemcpyAsync(&ya, c->y[c->cur] + iat, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(c, hipMemcpyAsync(&za, c->z[c->cur] + iat, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                int st = sync_err(c);
+                if (st) return st;
+                st = bromoc_interact(c, xa, ya, za, itype, iat, 1, &dener);
+                if (st) return st;
+                double rate;
+                if (c->bufferbias[ib]) {
+                    rate = (c->avnum[ib] + c->kb[ib] * (c->avnum[ib] - (double)(c->ntotat[ib] / icycle)));
+                    rate = rate * std::exp(-(dener - c->mu[ib]) * g.ikbt) / (double)nat[ib];
+                } else {
+                    rate = (c->avnum[ib] / nat[ib]) * std::exp(-(dener - c->mu[ib]) * g.ikbt);
+                }
+                rate = 1.0 / (1.0 + rate);
+                double r3;
+                RNDM(r3);
+                if (r3 <= rate) {
+                    *ener = *ener - dener;
+                    const int last = c->ntot - 1;
+                    for (int bb = 0; bb < 2; bb++) {
+                        HIPCHK(c, hipMemcpyAsync(c->x[bb] + iat, c->x[c->cur] + last, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+                        HIPCHK(c, hipMemcpyAsync(c->y[bb] + iat, c->y[c->cur] + last, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+                        HIPCHK(c, hipMemcpyAsync(c->z[bb] + iat, c->z[c->cur] + last, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+                    }
+                    HIPCHK(c, hipMemcpyAsync(c->typei + iat, c->typei + last, sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+                    HIPCHK(c, hipMemcpyAsync(c->t2 + iat, c->t2 + last, sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+                    int st2 = sync_err(c);
+                    if (st2) return st2;
+                    c->h_typei[iat] = c->h_typei[last];
+                    c->ibuffer[iat] = c->ibuffer[last];
+                    nat[ib] = nat[ib] - 1;
+                    c->ntot = last;
+                }
+            }
+        }
+        for (int b = 1; b <= nbuffer; b++) {
+            if (nat[b] < 0 || nat[b] > c->cap) return fail(c, BROMOC_E_STATE, "nat out of range");
+            prob[(size_t)(b - 1) * pstride + nat[b]] += 1;
+        }
+    }
+#undef RNDM
+    if (used) *used = k;
+    return BROMOC_OK;
+}
+
+// -------------------------------------------------------------- EFPOT table (host)
+// SPLINEPOT (splinepot.f90:19-96) with SQUARESPLINE (polynom.f90:20-48): the (r,U)
+// samples become the piecewise-quadratic table ep(1:3,k) plus the wall (sc1,sc2) and tail
+// (sc3) coefficients.  Integer powers are formed by binary powering, as the reference's
+// compiler expands x**n, so the coefficients match the reference bit for bit.
+static double ipow(double x, unsigned n) {
+    double r = 0.0, sq = x;
+    bool have = false;
+    for (; n; n >>= 1) {
+        if (n & 1u) { r = have ? r * sq : sq; have = true; }
+        if (n > 1) sq = sq * sq;
+    }
+    return have ? r : 1.0;
+}
+
+struct Quad {
+    double a, b, c;  // U = a + b r + c r^2 (ep1, ep2, ep3)
+};
+
+static void square_spline(const std::vector<double> &x, std::vector<double> y, std::vector<Quad> &q,
+                          double d1, double d2, double dd2) {
+    const int n = (int)x.size();
+    q.assign(n, Quad{0, 0, 0});
+    if (n == 1) { q[0] = {y[0], 0.0, 0.0}; return; }
+    if (n == 2) {
+        double b = (y[1] - y[0]) / (x[1] - x[0]);
+        q[0] = q[1] = {y[0] - b * x[0], b, 0.0};
+        return;
+    }
+    const double y0 = y[0], yn = y[n - 1];
+    // sqfit (polynom.f90:135-169): local quadratics through consecutive triples, averaged
+    auto local = [&](int i, Quad &o) {
+        double tt = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+        o.c = (tt - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) / (x[i + 1] - x[i - 1]);
+        o.b = tt - o.c * (x[i + 1] + x[i]);
+        o.a = y[i] - x[i] * (o.b + o.c * x[i]);
+    };
+    {
+        double tt = 1.0 / (x[1] - x[0]);
+        q[0].c = ((y[1] - y[0]) * tt - d1) * tt;
+        q[0].b = d1 - 2.0 * q[0].c * x[0];
+        q[0].a = y[0] - x[0] * (q[0].b + q[0].c * x[0]);
+    }
+    local(1, q[1]);
+    for (int i = 2; i <= n - 2; i++) {
+        local(i, q[i]);
+        q[i - 1].c = 0.5 * (q[i - 1].c + q[i].c);
+        q[i - 1].b = 0.5 * (q[i - 1].b + q[i].b);
+        q[i - 1].a = 0.5 * (q[i - 1].a + q[i].a);
+    }
+    {
+        const int m = n - 1;
+        double tt = 1.0 / (x[m] - x[m - 1]);
+        q[m].c = (d2 - (y[m] - y[m - 1]) * tt) * tt;
+        q[m].b = d2 - 2.0 * q[m].c * x[m];
+        q[m].a = y[m] - x[m] * (q[m].b + q[m].c * x[m]);
+        q[m - 1].c = 0.5 * (q[m - 1].c + q[m].c);
+        q[m - 1].b = 0.5 * (q[m - 1].b + q[m].b);
+        q[m - 1].a = 0.5 * (q[m - 1].a + q[m].a);
+    }
+    // fixder (polynom.f90:121-132): knot derivatives averaged across segments
+    std::vector<double> yp(n);
+    yp[0] = q[0].b + 2.0 * q[0].c * x[0];
+    for (int i = 1; i <= n - 2; i++)
+        yp[i] = 0.5 * (q[i].b + 2.0 * q[i].c * x[i] + q[i - 1].b + 2.0 * q[i - 1].c * x[i]);
+    yp[n - 1] = q[n - 1].b + 2.0 * q[n - 1].c * x[n - 1];
+    // linipol (polynom.f90:67-88): derivative as a piecewise-linear function
+    std::vector<double> s0(n), s1(n);
+    for (int i = 0; i <= n - 2; i++) {
+        double tt = 1.0 / (x[i + 1] - x[i]);
+        s1[i] = (yp[i + 1] - yp[i]) * tt;
+        s0[i] = (yp[i] * x[i + 1] - yp[i + 1] * x[i]) * tt;
+    }
+    s1[n - 1] = dd2;
+    s0[n - 1] = yp[n - 1] - s1[n - 1] * x[n - 1];
+    // intlin (polynom.f90:51-64): integrate it, continuous at the knots
+    q[0].c = s1[0] * 0.5;
+    q[0].b = s0[0];
+    q[0].a = y0 - x[0] * (q[0].b + q[0].c * x[0]);
+    for (int i = 1; i < n; i++) {
+        q[i].c = s1[i] * 0.5;
+        q[i].b = s0[i];
+        q[i].a = q[i - 1].a + x[i] * (q[i - 1].b - q[i].b + x[i] * (q[i - 1].c - q[i].c));
+    }
+    // fixdy (polynom.f90:91-118): restore U(x_n) with a correction preserving the ends
+    {
+        const int m = n - 1;
+        double ynn = q[m].a + x[m] * (q[m].b + x[m] * q[m].c);
+        double b = 2.0 * (yn - ynn) / (x[m] - x[0]);
+        double cc = -b * x[0] * 0.5;
+        double h = (x[1] - x[0]) / (double)(n - 1);
+        for (int i = 0; i < m; i++) {
+            q[i].a = q[i].a + cc;
+            q[i].b = q[i].b + b;
+            q[i].c = q[i].c + (-b * 0.5 / (x[i] + h * (double)i));
+        }
+        q[m].a = q[m].a + cc;
+        q[m].b = q[m].b + b;
+        q[m].c = q[m].c + (-b * 0.5 / x[m]);
+    }
+}
+
+int bromoc_build_table(int32_t n, const double *r, const double *u, int32_t chr, double fct,
+                       double *ep, bromoc_pair *pair) {
+    if (n < 2 || !r || !u || !ep || !pair) return BROMOC_E_ARG;
+    if (r[0] == 0.0 || r[n - 1] == 0.0) return BROMOC_E_ARG;
+    for (int k = 1; k < n; k++)
+        if (!(r[k] > r[k - 1])) return BROMOC_E_ARG;
+    const double r1 = r[0], r2 = r[1], rn = r[n - 1];
+    pair->dmi = r1;
+    pair->dm2lo = r1 * r1;
+    pair->dm2hi = rn * rn;
+    const double kf = chr ? fct : 0.0;
+    // tail U = kf/r + sc3/r^6 matched at r_n (splinepot.f90:66-69)
+    pair->sc3 = (rn * u[n - 1] - kf) * ipow(rn, 5);
+    const double d2 = -kf / ipow(rn, 2) - 6.0 * pair->sc3 / ipow(rn, 7);
+    const double dd2 = 2.0 * kf / ipow(rn, 3) + 42.0 * pair->sc3 / ipow(rn, 8);
+    // wall U = sc1/r^12 + sc2 through the first two samples (splinepot.f90:79-82)
+    const double p1 = ipow(r1, 12), p2 = ipow(r2, 12);
+    const double inv = 1.0 / (p1 - p2);
+    pair->sc1 = (u[1] - u[0]) * p2 * p1 * inv;
+    pair->sc2 = (u[0] * p1 - u[1] * p2) * inv;
+    const double d1 = -12.0 * pair->sc1 / ipow(r1, 13);
+    std::vector<double> x(r, r + n), y(u, u + n);
+    std::vector<Quad> q;
+    square_spline(x, y, q, d1, d2, dd2);
+    for (int k = 0; k < n; k++) {
+        ep[3 * k] = q[k].a;
+        ep[3 * k + 1] = q[k].b;
+        ep[3 * k + 2] = q[k].c;
+    }
+    pair->nknots = n;
+    pair->efpot = 1;
+    pair->chr = chr;
+    pair->fct = fct;
+    return BROMOC_OK;
+}
+
+// raw Philox4x64-10 blocks on the device (the generator behind BROMOC_NOISE_PHILOX)
+int bromoc_philox4x64(int32_t n, const uint64_t *ctr, const uint64_t *key, uint64_t *out) {
+    if (n < 0 || (n > 0 && (!ctr || !key || !out))) return BROMOC_E_ARG;
+    if (n == 0) return BROMOC_OK;
+    uint64_t *d = nullptr;
+    if (hipMalloc(&d, sizeof(uint64_t) * 10 * n) != hipSuccess) return BROMOC_E_HIP;
+    (void)hipMemcpy(d, ctr, sizeof(uint64_t) * 4 * n, hipMemcpyHostToDevice);
+    (void)hipMemcpy(d + 4 * n, key, sizeof(uint64_t) * 2 * n, hipMemcpyHostToDevice);
+    hipError_t e = launch_philox_raw(d, d + 4 * n, n, d + 6 * n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, d + 6 * n, sizeof(uint64_t) * 4 * n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? BROMOC_OK : BROMOC_E_HIP;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// engine.h -- device-side state shared by the kernels (kernels.hip) and the C ABI
+// (api.hip) of the gfx950 BROMOC force / BD path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bromoc {
+
+// Number of per-particle energy slots: the bromoc_energies fields after `ener`.
+constexpr int NE = 22;
+enum ECat {
+    E_ELEC = 0, E_VDW, E_SRPMF, E_SRPMFMX, E_EFPOT, E_EFPOTMX, E_QQMX, E_VDWMX,
+    E_BOND, E_ANG, E_DIHE, E_STACK, E_BP, E_EX, E_QQ, E_SOLV,
+    E_GSBPA, E_GSBPB, E_VDWGD, E_RFPAR, E_CON, E_MEMB
+};
+
+// pair-type flags
+constexpr int PF_EFPOT = 1, PF_CHR = 2, PF_LJ = 4, PF_SRPMF = 8;
+
+// One pair type (efpmod/extramod/grandmod fields), laid out for LDS.
+struct DevPair {
+    double dmi, dm2lo, dm2hi, sc1, sc2, sc3, fct, epp4;
+    double sgp2, c0, c1, c2, c3, c4;
+    int32_t nxi, flags;
+};
+
+struct GridGeom {
+    int32_t nx, ny, nz, on;
+    double dcel, idcel, xbcen, ybcen, zbcen, tranx, trany, tranz;
+};
+
+// DNA term kinds in the per-site incidence list
+enum TermKind { T_BOND = 0, T_ANGLE, T_DIHE, T_STACK, T_BP, T_EX, T_QQ, T_SLV };
+// incidence word: kind (3 bits) | role (2 bits) | term index (27 bits)
+__host__ __device__ inline uint32_t inc_pack(int kind, int role, int term) {
+    return (uint32_t(kind) << 29) | (uint32_t(role) << 27) | uint32_t(term);
+}
+
+constexpr int MAX_LDS_PAIRS = 64;     // pair types held in LDS
+constexpr int MAX_LDS_TYPES = 16;     // atnam2 types (pair index table in LDS)
+
+struct StepParams {
+    // sizes
+    int32_t ntot, nsites, nfix, ntargets;
+    int32_t nttyp, ntype, nold, npair;
+    uint32_t flags;
+    int32_t mode;            // 0: ENERGY (forces out, no update), 1: ENERGY + DYNAMICS
+    int32_t write_energy, write_forces;
+    // positions (in) and updated positions (out)
+    const double *x, *y, *z;
+    double *xo, *yo, *zo;
+    const int32_t *typei;    // signed atnam type
+    const int32_t *t2;       // atnam2 type per particle
+    const int8_t *stfree;    // [nsites]
+    const int32_t *namsite;  // [nsites]
+    double *fx, *fy, *fz;
+    double *epart;           // [NE][ecap]
+    int32_t ecap, pad0;
+    // tables
+    const int32_t *pairidx;  // [(nttyp+1)^2]
+    const DevPair *pairs;    // [npair+1]
+    const double4 *ep;       // knots (ep1, ep2, ep3, 0)
+    double ires, celec;
+    const double *cg, *fact1a, *fact2a;
+    const int32_t *nwtype;
+    double rth, srpx, srpk, srpy;
+    // CG-DNA
+    const int32_t *inc_ptr;  // [nsites+1]
+    const uint32_t *inc;
+    const int32_t *lst[8];   // component-major site lists
+    const double *lval[6];
+    int32_t nlist[8];
+    double epsnuc, scalepairing, fctn, kappa, ikappa, epsolv, cgnuc;
+    // grids
+    GridGeom gx, gv;
+    const float *phix;
+    const int8_t *phiv;
+    double svdw, thold8, vzmin, vzmax;
+    const double *scal;
+    int32_t *warn;
+    // Brownian dynamics
+    double bdmax, fact0n1, fact0n2;
+    double lx2m, lx2p, ly2m, ly2p, lz2m, lz2p;
+    int32_t cntpts, noise_mode;
+    const double *zcont;
+    int32_t *nfwd, *nbwd;
+    const double *noise;           // injected: [row][ntot*3]
+    int64_t noise_stride;          // doubles per row
+    const int64_t *step_ctr;       // device step counter (graph-friendly)
+    int64_t step_off;              // this launch's step = *step_ctr + step_off
+    int64_t row0;                  // injected row of step counter 0
+    uint64_t seed;
+    uint64_t stream;
+};
+
+// launchers (kernels.hip)
+hipError_t launch_step(const StepParams &p, hipStream_t s);
+hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
+                                hipStream_t s);
+hipError_t launch_advance(int64_t *ctr, int64_t by, hipStream_t s);
+hipError_t launch_philox_raw(const uint64_t *ctr, const uint64_t *key, int n, uint64_t *out,
+                             hipStream_t s);
+
+struct InteractParams {
+    StepParams sp;              // system state (positions x,y,z; tables; grids; lists)
+    double xj, yj, zj;
+    int32_t jtype, j, alert, pad;
+    double *out;                // [1]
+};
+hipError_t launch_interact(const InteractParams &p, hipStream_t s);
+
+}  // namespace bromoc

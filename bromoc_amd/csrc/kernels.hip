@@ -1,0 +1,834 @@
+// kernels.hip -- gfx950 kernels of the BROMOC force / Brownian-dynamics step.
+//
+// k_step: one wavefront per target particle (every DNA site and every moving ion).
+//   The block's four wavefronts stream the whole particle set through LDS in tiles of 256;
+//   each lane evaluates its share of the tile against the wavefront's target (all-pairs, as
+//   the reference's ENERGY has no cutoff: energy.mono.f90:117-194,584-687), then walks the
+//   target's DNA-term incidence list (energy.mono.f90:196-508) and, on lanes 0..7, the
+//   8 trilinear corners of the static-field and repulsion grids (staticf1.f90:45-118,
+//   vdwgd1trln.f90:50-161).  A butterfly reduction gives the total force on lane 0, which
+//   applies the Ermak-McCammon update of DYNAMICS0 / DYNAMICS0NUC (dynamics0.mono.f90:31-46,
+//   dynamics0nuc.f90:31-44) into the other position buffer, so one launch is one BD step.
+//   Every force is a gather over partners in a fixed order: results are deterministic and
+//   need no atomics; energies are per-particle partials reduced in a fixed tree.
+#include <math.h>
+
+#include "engine.h"
+
+namespace bromoc {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+constexpr int WPB = BLOCK / WAVE;
+constexpr int TILE = BLOCK;
+
+// feature bits of the k_step specialisations
+constexpr int F_DNA = 1;     // CG-DNA sites and lists present
+constexpr int F_PHIX = 2;    // static field grid
+constexpr int F_PHIV = 4;    // repulsion grid
+constexpr int F_UPDATE = 8;  // apply the BD update (else ENERGY only)
+
+// host-side flag values (include/bromoc_gpu.h)
+constexpr uint32_t QPAR = 1u << 0, QNUCL = 1u << 1, QNONBOND = 1u << 2, QNOBOND = 1u << 3;
+constexpr uint32_t QPHIX = 1u << 4, QPHIV = 1u << 5, QSVDW = 1u << 6, QSOLV = 1u << 7;
+constexpr uint32_t QDEBY = 1u << 8, QCOUNTION = 1u << 9, QDNAFREE = 1u << 10;
+
+constexpr double PI = 3.14159265358979323846264338327950288419716939937510;
+constexpr double RSMALL = 1E-3;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// integer powers in the order the Fortran compiler expands them (binary powering)
+__device__ __forceinline__ double pow3(double a) { double a2 = a * a; return a * a2; }
+__device__ __forceinline__ double pow5(double a) { double a2 = a * a; double a4 = a2 * a2; return a * a4; }
+__device__ __forceinline__ double pow6(double a) { double a2 = a * a; double a4 = a2 * a2; return a2 * a4; }
+__device__ __forceinline__ double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+// getyd, polynom.f90:197-230: energy y and force factor yp (= -dU/dr / r)
+__device__ __forceinline__ void getyd(const DevPair &p, const double4 *__restrict__ ep,
+                                      double ires, double x2, double &y, double &yp) {
+    if (x2 < p.dm2lo) {
+        double ix2 = 1.0 / x2;
+        double c = p.sc1 * pow6(ix2);
+        y = c + p.sc2;
+        yp = 12.0 * c * ix2;
+    } else if (x2 > p.dm2hi) {
+        double ix2 = 1.0 / x2;
+        double c = p.sc3 * pow3(ix2);
+        if (p.flags & PF_CHR) {
+            double f = p.fct * sqrt(ix2);
+            y = c + f;
+            yp = (6.0 * c + f) * ix2;
+        } else {
+            y = c;
+            yp = 6.0 * c * ix2;
+        }
+    } else {
+        double x = sqrt(x2);
+        int i = (int)((x - p.dmi) * ires) + p.nxi;
+        double4 e = ep[i];
+        y = e.x + x * (e.y + x * e.z);
+        yp = -e.y / x - 2.0 * e.z;
+    }
+}
+
+// analytic pair branch, energy.mono.f90:135-178
+__device__ __forceinline__ double pair_analytic(const DevPair &p, const StepParams &P, double dist2,
+                                                double &eel, double &evd, double &esr) {
+    double idist2 = 1.0 / dist2, idist = 0.0, cofo = 0.0, dist6 = 0.0, dist12 = 0.0, dist = 0.0;
+    double e1 = 0.0, e2 = 0.0, e3 = 0.0, fdf = 0.0, fdv = 0.0;
+    const bool srp = (p.flags & PF_SRPMF) && dist2 <= P.rth;
+    if (p.flags & (PF_CHR | PF_SRPMF)) idist = sqrt(idist2);
+    if (p.flags & PF_CHR) {
+        cofo = p.fct * idist;
+        eel += cofo;
+    }
+    if (p.flags & PF_LJ) {
+        dist6 = pow3(p.sgp2 * idist2);
+        dist12 = dist6 * dist6;
+        evd += p.epp4 * (dist12 - dist6);
+    }
+    if (srp) {
+        dist = 1.0 / idist;
+        e1 = exp((p.c1 - dist) * p.c2);
+        e2 = cos(p.c3 * PI * (p.c1 - dist));
+        e3 = pow6(p.c1 * idist);
+        double e0 = p.c0 * e1 * e2 + p.c4 * e3;
+        if (dist >= P.srpx) {
+            fdf = exp(-P.srpk * (dist - P.srpx)) - P.srpy;
+            fdv = e0;
+            e0 = fdv * fdf;
+        }
+        esr += e0;
+    }
+    double de = 0.0;
+    if (p.flags & PF_CHR) de = cofo * idist2;
+    if (p.flags & PF_LJ) de = de + p.epp4 * (2.0 * dist12 - dist6) * 6.0 * idist2;
+    if (srp) {
+        double dc = (-p.c2 * e2 + p.c3 * PI * sin(p.c3 * PI * (p.c1 - dist))) * p.c0 * e1 -
+                    6.0 * p.c4 * e3 * idist;
+        if (dist >= P.srpx) dc = dc * fdf - fdv * P.srpk * (fdf + P.srpy);
+        de = de - dc * idist;
+    }
+    return de;
+}
+
+// Philox4x64-10 (Salmon et al. 2011) and Box-Muller on 53-bit uniforms
+__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint64_t M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
+        uint64_t hi0 = __umul64hi(M0, c[0]), lo0 = M0 * c[0];
+        uint64_t hi1 = __umul64hi(M1, c[2]), lo1 = M1 * c[2];
+        uint64_t n0 = hi1 ^ c[1] ^ k0, n1 = lo1, n2 = hi0 ^ c[3] ^ k1, n3 = lo0;
+        c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+        k0 += 0x9E3779B97F4A7C15ull;
+        k1 += 0xBB67AE8584CAA73Bull;
+    }
+}
+
+__device__ __forceinline__ void philox_gauss3(uint64_t seed, uint64_t step, uint64_t stream,
+                                              int64_t i, double g[3]) {
+    uint64_t c[4] = {step, (uint64_t)i, stream, 0};
+    philox4x64(c, seed, 0);
+    const double inv53 = 1.0 / 9007199254740992.0;
+    double u1 = (double)((c[0] >> 11) + 1) * inv53;
+    double u2 = (double)(c[1] >> 11) * inv53;
+    double u3 = (double)((c[2] >> 11) + 1) * inv53;
+    double u4 = (double)(c[3] >> 11) * inv53;
+    double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    const double twopi = 2.0 * PI;
+    g[0] = r1 * cos(twopi * u2);
+    g[1] = r1 * sin(twopi * u2);
+    g[2] = r2 * cos(twopi * u4);
+}
+
+__device__ __forceinline__ void fixcoor(const StepParams &P, double &x, double &y, double &z) {
+    if (x < P.lx2m) x = P.lx2m;
+    else if (x > P.lx2p) x = P.lx2p;
+    if (y < P.ly2m) y = P.ly2m;
+    else if (y > P.ly2p) y = P.ly2p;
+    if (z < P.lz2m) z = P.lz2m;
+    else if (z > P.lz2p) z = P.lz2p;
+}
+
+// One trilinear corner (lane c = 0..7) of staticf1/vdwgd1trln.  Adds the corner's force
+// and energy to the lane's accumulators; returns the corner's grid value.
+template <typename T>
+__device__ __forceinline__ double tri_corner(const GridGeom &g, const T *__restrict__ data,
+                                             double xi, double yi, double zi, int ix, int iy,
+                                             int iz, int c, double scale, double &fx, double &fy,
+                                             double &fz, double &e) {
+    const double one = 1.0;
+    int n1 = ix + (c >> 2), n2 = iy + ((c >> 1) & 1), n3 = iz + (c & 1);
+    double ai = xi - n1 * g.dcel;
+    double aisign = fsign(one, ai);
+    ai = one - fabs(ai) * g.idcel;
+    double bi = yi - n2 * g.dcel;
+    double bisign = fsign(one, bi);
+    bi = one - fabs(bi) * g.idcel;
+    double ci = zi - n3 * g.dcel;
+    double cisign = fsign(one, ci);
+    ci = one - fabs(ci) * g.idcel;
+    double fi = ai * bi * ci;
+    size_t in3 = ((size_t)n1 * g.ny + n2) * g.nz + n3;
+    double ph = (double)data[in3];
+    double prefac = ph * scale * g.idcel;
+    if (ai < (1.0 - RSMALL) && ai > RSMALL) fx += aisign * bi * ci * prefac;
+    if (bi < (1.0 - RSMALL) && bi > RSMALL) fy += bisign * ai * ci * prefac;
+    if (ci < (1.0 - RSMALL) && ci > RSMALL) fz += cisign * ai * bi * prefac;
+    e += fi * scale * ph;
+    return ph;
+}
+
+__device__ __forceinline__ void grid_cell(const GridGeom &g, double x, double y, double z,
+                                          double &xi, double &yi, double &zi, int &ix, int &iy,
+                                          int &iz) {
+    xi = x + g.tranx - g.xbcen;
+    yi = y + g.trany - g.ybcen;
+    zi = z + g.tranz - g.zbcen;
+    ix = (int)(xi * g.idcel);
+    iy = (int)(yi * g.idcel);
+    iz = (int)(zi * g.idcel);
+    if (ix == g.nx - 1) ix = g.nx - 2;
+    if (iy == g.ny - 1) iy = g.ny - 2;
+    if (iz == g.nz - 1) iz = g.nz - 2;
+}
+
+struct V3 { double x, y, z; };
+__device__ __forceinline__ V3 ld3(const StepParams &P, int i) { return {P.x[i], P.y[i], P.z[i]}; }
+__device__ __forceinline__ V3 sub3(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ double dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross3(V3 u, V3 v) {
+    return {u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
+}
+__device__ __forceinline__ V3 scl3(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+
+// One DNA term incident on site `me` in role `role`: force on `me` (added to f) and the
+// term's energy (returned in e when role == 0).  energy.mono.f90:205-507.
+__device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, double *e) {
+    const int kind = w >> 29, role = (w >> 27) & 3, t = w & 0x7FFFFFF;
+    const double eps = P.epsnuc;
+    const int n = P.nlist[kind];
+    const int32_t *L = P.lst[kind];
+    if (kind == T_BOND) {
+        int a = L[t], b = L[n + t];
+        V3 v1 = sub3(ld3(P, b), ld3(P, a));
+        double dist = sqrt(dot3(v1, v1));
+        double vard = dist - P.lval[0][t];
+        double vard2 = vard * vard;
+        if (role == 0) e[E_BOND] += eps * vard2 * (1.0 + 100.0 * vard2);
+        if (free_me) {
+            double de = eps * vard * (2.0 + 400.0 * vard2) / dist;
+            V3 f1 = scl3(de, v1);
+            double s = role == 0 ? 1.0 : -1.0;
+            f.x += s * f1.x; f.y += s * f1.y; f.z += s * f1.z;
+        }
+    } else if (kind == T_ANGLE) {
+        int a = L[t], b = L[n + t], d = L[2 * n + t];
+        V3 rb = ld3(P, b);
+        V3 v1 = sub3(ld3(P, a), rb), v2 = sub3(ld3(P, d), rb);
+        double m1 = dot3(v1, v1), m2 = dot3(v2, v2), m3 = dot3(v1, v2);
+        double modval = 1.0 / sqrt(m1 * m2);
+        double ang = acos(m3 * modval);
+        double varang = ang - P.lval[1][t];
+        if (role == 0) e[E_ANG] += 700.0 * eps * (varang * varang);
+        if (free_me) {
+            double de = 1400.0 * eps * varang * modval / sin(ang);
+            V3 f1 = scl3(de, v1), f2 = scl3(de, v2);
+            if (role == 0) { f.x += f2.x; f.y += f2.y; f.z += f2.z; }
+            else if (role == 1) { f.x -= f1.x + f2.x; f.y -= f1.y + f2.y; f.z -= f1.z + f2.z; }
+            else { f.x += f1.x; f.y += f1.y; f.z += f1.z; }
+        }
+    } else if (kind == T_DIHE) {
+        int i1 = L[t], i2 = L[n + t], i3 = L[2 * n + t], i4 = L[3 * n + t];
+        V3 r2 = ld3(P, i2), r3 = ld3(P, i3);
+        V3 v1 = sub3(ld3(P, i1), r2), v2 = sub3(r3, r2), v3 = sub3(r3, ld3(P, i4));
+        V3 n1 = cross3(v1, v2), n2 = cross3(v2, v3);
+        double v22 = dot3(v2, v2);
+        double v2v = sqrt(v22);
+        double av = dot3(v1, n2), bv = dot3(n1, n2);
+        double dihe = atan2(v2v * av, bv);
+        double vardihe = dihe - P.lval[2][t];
+        if (role == 0) e[E_DIHE] += 28.0 * eps * (1.0 - cos(vardihe));
+        if (free_me) {
+            double n1v = dot3(n1, n1), n2v = dot3(n2, n2);
+            double v12 = dot3(v1, v2), v23 = dot3(v2, v3);
+            double de = 28.0 * eps * sin(vardihe) * v2v;
+            V3 f1 = scl3(-de / n1v, n1), f4 = scl3(de / n2v, n2);
+            double iv22 = 1.0 / v22;
+            v12 = v12 * iv22;
+            v23 = v23 * iv22;
+            V3 g;
+            if (role == 0) g = f1;
+            else if (role == 3) g = f4;
+            else if (role == 1) g = {-f1.x + v12 * f1.x - v23 * f4.x, -f1.y + v12 * f1.y - v23 * f4.y,
+                                     -f1.z + v12 * f1.z - v23 * f4.z};
+            else g = {-f4.x - v12 * f1.x + v23 * f4.x, -f4.y - v12 * f1.y + v23 * f4.y,
+                      -f4.z - v12 * f1.z + v23 * f4.z};
+            f.x += g.x; f.y += g.y; f.z += g.z;
+        }
+    } else {
+        int a = L[t], b = L[n + t];
+        V3 v1 = sub3(ld3(P, b), ld3(P, a));
+        double de = 0.0;
+        bool force = free_me;
+        if (kind == T_STACK) {
+            double dist2 = dot3(v1, v1);
+            double idist2 = 1.0 / dist2;
+            double sg = P.lval[3][t];
+            double cte1 = pow3(sg * sg * idist2);
+            if (role == 0) e[E_STACK] += 4.0 * eps * cte1 * (cte1 - 1.0);
+            de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
+        } else if (kind == T_BP) {
+            int na = P.namsite[a];
+            double epsln = (na == 5 || na == 4) ? 2.532 * eps * P.scalepairing
+                                                : 2.0 * eps * P.scalepairing;
+            double dist2 = dot3(v1, v1);
+            double idist2 = 1.0 / dist2;
+            double sg = P.lval[4][t];
+            double cte1 = sg * sg * idist2;
+            if (role == 0) e[E_BP] += epsln * pow5(cte1) * (20.0 * cte1 - 24.0);
+            de = 240.0 * epsln * pow5(cte1) * (cte1 - 1.0) * idist2;
+        } else if (kind == T_EX) {
+            double dist2 = dot3(v1, v1);
+            double idist2 = 1.0 / dist2;
+            double sgex2 = P.lval[5][t] * P.lval[5][t];
+            if (dist2 < sgex2) {
+                double cte1 = pow3(sgex2 * idist2);
+                if (role == 0) e[E_EX] += 4.0 * eps * cte1 * (cte1 - 1.0) + eps;
+                de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
+            } else {
+                force = false;
+            }
+        } else if (kind == T_QQ) {
+            double dist = sqrt(dot3(v1, v1));
+            double idist = 1.0 / dist;
+            if (P.flags & QDEBY) {
+                double idistkd = exp(-dist * P.ikappa);
+                if (role == 0) e[E_QQ] += P.fctn * idist * idistkd;
+                de = P.fctn * (dist + P.kappa) * pow3(idist) * P.ikappa * idistkd;
+            } else {
+                if (role == 0) e[E_QQ] += P.fctn * idist;
+                de = P.fctn * pow3(idist);
+            }
+        } else {  // T_SLV
+            double dist = sqrt(dot3(v1, v1));
+            double cte1 = exp((13.38 - dist) * 0.1875);
+            if (role == 0) e[E_SOLV] += P.epsolv * ((1.0 - cte1) * (1.0 - cte1)) - P.epsolv;
+            de = 0.375 * P.epsolv * cte1 * (cte1 - 1.0) / dist;
+        }
+        if (force) {
+            double s = role == 0 ? -de : de;
+            f.x += s * v1.x; f.y += s * v1.y; f.z += s * v1.z;
+        }
+    }
+}
+
+template <int FEAT>
+__global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
+    __shared__ double4 s_pos[TILE];
+    __shared__ DevPair s_pairs[MAX_LDS_PAIRS];
+    __shared__ int32_t s_pidx[MAX_LDS_TYPES * MAX_LDS_TYPES];
+
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const int ns = P.nsites, nf = P.nfix, ntot = P.ntot;
+    const int nt1 = P.nttyp + 1;
+    for (int k = tid; k <= P.npair; k += BLOCK) s_pairs[k] = P.pairs[k];
+    for (int k = tid; k < nt1 * nt1; k += BLOCK) s_pidx[k] = P.pairidx[k];
+
+    const int w = blockIdx.x * WPB + (tid >> 6);
+    const bool valid = w < P.ntargets;
+    const int i = !valid ? 0 : (w < ns ? w : w + nf);
+    const bool is_site = i < ns;
+    const double xi = valid ? P.x[i] : 0.0, yi = valid ? P.y[i] : 0.0, zi = valid ? P.z[i] : 0.0;
+    const int t2i = valid ? P.t2[i] : 0;
+    const bool free_me = !is_site || ((FEAT & F_DNA) && P.stfree[i] != 0);
+    const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
+    const int prow = t2i * nt1;
+
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double e_tab = 0.0, e_el = 0.0, e_vd = 0.0, e_sr = 0.0;
+
+    // ---- pair loop over LDS tiles of every particle (energy.mono.f90:117-194, 584-687)
+    for (int base = 0; base < ntot; base += TILE) {
+        __syncthreads();
+        {
+            int j = base + tid;
+            if (j < ntot) s_pos[tid] = make_double4(P.x[j], P.y[j], P.z[j], __hiloint2double(0, P.t2[j]));
+        }
+        __syncthreads();
+        if (valid && nonbond) {
+            const int kend = min(TILE, ntot - base);
+            const int kbeg = is_site ? max(0, ns - base) : 0;  // site targets: ion partners only
+            for (int k = kbeg + lane; k < kend; k += WAVE) {
+                const int j = base + k;
+                if (j == i) continue;
+                const double4 pj = s_pos[k];
+                const int t2j = __double2loint(pj.w);
+                const double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const DevPair &pp = s_pairs[s_pidx[prow + t2j]];
+                // energy belongs to this target unless the partner is a site (counted by the
+                // site) or a higher-index ion (counted by that ion)
+                const bool count_e = is_site || (j >= ns && j < i);
+                double de;
+                if (pp.flags & PF_EFPOT) {
+                    double y;
+                    getyd(pp, P.ep, P.ires, d2, y, de);
+                    if (count_e) e_tab += y;
+                } else {
+                    double a = 0.0, b = 0.0, c = 0.0;
+                    de = pair_analytic(pp, P, d2, a, b, c);
+                    if (count_e) { e_el += a; e_vd += b; e_sr += c; }
+                }
+                if (free_me) {
+                    fx += de * dx;
+                    fy += de * dy;
+                    fz += de * dz;
+                }
+            }
+        }
+    }
+
+    double e[NE];
+#pragma unroll
+    for (int k = 0; k < NE; k++) e[k] = 0.0;
+
+    // ---- CG-DNA terms incident on this site (energy.mono.f90:196-508)
+    if constexpr ((FEAT & F_DNA) != 0) {
+        if (valid && is_site) {
+            V3 f = {0.0, 0.0, 0.0};
+            const int k0 = P.inc_ptr[i], k1 = P.inc_ptr[i + 1];
+            for (int k = k0 + lane; k < k1; k += WAVE) {
+                uint32_t wd = P.inc[k];
+                int kind = wd >> 29;
+                bool on = kind <= T_DIHE ? (P.flags & QNOBOND) != 0
+                                         : ((P.flags & QNONBOND) != 0 &&
+                                            (kind != T_SLV || (P.flags & QSOLV) != 0));
+                if (on) dna_term(P, wd, free_me, f, e);
+            }
+            fx += f.x; fy += f.y; fz += f.z;
+        }
+    }
+
+    // ---- external grids on lanes 0..7 (staticf1.f90, vdwgd1trln.f90)
+    bool warned = false;
+    if constexpr ((FEAT & F_PHIX) != 0) {
+        if (valid && lane < 8) {
+            const GridGeom &g = P.gx;
+            double chi = 0.0;
+            bool ok;
+            if (is_site) { ok = P.namsite[i] == 1; chi = P.cgnuc; }
+            else { ok = true; chi = P.cg[abs(P.typei[i])]; }
+            ok = ok && xi <= g.xbcen + g.tranx && xi >= g.xbcen - g.tranx &&
+                 yi <= g.ybcen + g.trany && yi >= g.ybcen - g.trany &&
+                 zi <= g.zbcen + g.tranz && zi >= g.zbcen - g.tranz;
+            if (ok) {
+                double gx_, gy_, gz_;
+                int ix, iy, iz;
+                grid_cell(g, xi, yi, zi, gx_, gy_, gz_, ix, iy, iz);
+                // prefac = phix*celec*chi*idcel and E += fi*chi*phix*celec
+                double a = 0.0, b = 0.0, c = 0.0, en = 0.0;
+                const double one = 1.0;
+                int n1 = ix + (lane >> 2), n2 = iy + ((lane >> 1) & 1), n3 = iz + (lane & 1);
+                double ai = gx_ - n1 * g.dcel;
+                double aisign = fsign(one, ai);
+                ai = one - fabs(ai) * g.idcel;
+                double bi = gy_ - n2 * g.dcel;
+                double bisign = fsign(one, bi);
+                bi = one - fabs(bi) * g.idcel;
+                double ci = gz_ - n3 * g.dcel;
+                double cisign = fsign(one, ci);
+                ci = one - fabs(ci) * g.idcel;
+                double fi = ai * bi * ci;
+                double ph = (double)P.phix[((size_t)n1 * g.ny + n2) * g.nz + n3];
+                double prefac = ph * P.celec * chi * g.idcel;
+                if (ai < (1.0 - RSMALL) && ai > RSMALL) a = aisign * bi * ci * prefac;
+                if (bi < (1.0 - RSMALL) && bi > RSMALL) b = bisign * ai * ci * prefac;
+                if (ci < (1.0 - RSMALL) && ci > RSMALL) c = cisign * ai * bi * prefac;
+                en = fi * chi * ph * P.celec;
+                fx += a; fy += b; fz += c;
+                e[E_GSBPA] += en;
+            }
+        }
+    }
+    if constexpr ((FEAT & F_PHIV) != 0) {
+        const GridGeom &g = P.gv;
+        bool ok = valid && xi <= g.xbcen + g.tranx && xi >= g.xbcen - g.tranx &&
+                  yi <= g.ybcen + g.trany && yi >= g.ybcen - g.trany && zi <= P.vzmax &&
+                  zi >= P.vzmin;
+        double ph = 0.0;
+        if (ok && lane < 8) {
+            double gx_, gy_, gz_;
+            int ix, iy, iz;
+            grid_cell(g, xi, yi, zi, gx_, gy_, gz_, ix, iy, iz);
+            double esvdw = P.svdw;
+            if (P.flags & QSVDW) esvdw = P.svdw * P.scal[t2i];
+            double a = 0.0, b = 0.0, c = 0.0, en = 0.0;
+            ph = tri_corner(g, P.phiv, gx_, gy_, gz_, ix, iy, iz, lane, esvdw, a, b, c, en);
+            fx += a; fy += b; fz += c;
+            e[E_VDWGD] += en;
+        }
+        // phisum over the 8 corners (lanes 0..7) in corner order
+        double s = ph;
+        s += __shfl_xor(s, 1, WAVE);
+        s += __shfl_xor(s, 2, WAVE);
+        s += __shfl_xor(s, 4, WAVE);
+        warned = ok && s >= P.thold8;
+    }
+
+    // ---- reductions
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    if (P.write_energy) {
+        if (is_site) { e[E_EFPOTMX] += e_tab; e[E_QQMX] += e_el; e[E_VDWMX] += e_vd; e[E_SRPMFMX] += e_sr; }
+        else { e[E_EFPOT] += e_tab; e[E_ELEC] += e_el; e[E_VDW] += e_vd; e[E_SRPMF] += e_sr; }
+#pragma unroll
+        for (int k = 0; k < NE; k++) e[k] = wave_sum(e[k]);
+    }
+    if (!valid || lane != 0) return;
+
+    if (warned) atomicAdd(&P.warn[t2i], 1);
+    if (P.write_energy) {
+#pragma unroll
+        for (int k = 0; k < NE; k++) P.epart[(size_t)k * P.ecap + i] = e[k];
+    }
+    if (P.write_forces) {
+        P.fx[i] = fx;
+        P.fy[i] = fy;
+        P.fz[i] = fz;
+    }
+    if constexpr ((FEAT & F_UPDATE) != 0) {
+        // DYNAMICS0 / DYNAMICS0NUC
+        double fac1, fac2;
+        int itype = 0;
+        if (is_site) {
+            if (!free_me) {
+                P.xo[i] = xi; P.yo[i] = yi; P.zo[i] = zi;
+                return;
+            }
+            fac1 = P.fact0n1;
+            fac2 = P.fact0n2;
+        } else {
+            itype = abs(P.typei[i]);
+            fac1 = P.fact1a[itype];
+            fac2 = P.fact2a[itype];
+        }
+        const int64_t step = P.step_ctr[0] + P.step_off;
+        double g[3];
+        if (P.noise_mode == 0) {
+            const double *nr = P.noise + (P.step_ctr[1] + step) * P.noise_stride + 3 * (int64_t)i;
+            g[0] = nr[0]; g[1] = nr[1]; g[2] = nr[2];
+        } else {
+            philox_gauss3((uint64_t)P.step_ctr[3], (uint64_t)(P.step_ctr[2] + step), P.stream, i, g);
+        }
+        const double bdmax = P.bdmax;
+        double delx = fx * fac1, dely = fy * fac1, delz = fz * fac1;
+        if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
+        if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
+        if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
+        double xn = xi + delx + fac2 * g[0];
+        double yn = yi + dely + fac2 * g[1];
+        double zn = zi + delz + fac2 * g[2];
+        if (!is_site && (P.flags & QCOUNTION)) {
+            const int row = itype - P.nold - 1;
+            for (int c = 0; c < P.cntpts; c++) {
+                const double zc = P.zcont[c];
+                if (zi < zc && zn >= zc) atomicAdd(&P.nfwd[row * P.cntpts + c], 1);
+                else if (zi >= zc && zn < zc) atomicAdd(&P.nbwd[row * P.cntpts + c], 1);
+            }
+        }
+        if (!is_site || !(P.flags & QDNAFREE)) fixcoor(P, xn, yn, zn);
+        P.xo[i] = xn;
+        P.yo[i] = yn;
+        P.zo[i] = zn;
+    }
+}
+
+hipError_t launch_step(const StepParams &p, hipStream_t s) {
+    if (p.ntargets <= 0) return hipSuccess;
+    dim3 grid((p.ntargets + WPB - 1) / WPB), block(BLOCK);
+    int feat = 0;
+    if (p.flags & QNUCL) feat |= F_DNA;
+    if (p.flags & QPHIX) feat |= F_PHIX;
+    if (p.flags & QPHIV) feat |= F_PHIV;
+    if (p.mode == 1) feat |= F_UPDATE;
+#define CASE(F) case F: hipLaunchKernelGGL(k_step<F>, grid, block, 0, s, p); break;
+    switch (feat) {
+        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+        CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
+    }
+#undef CASE
+    return hipGetLastError();
+}
+
+// ---- energy reduction: one block per category, fixed-order tree
+__global__ __launch_bounds__(BLOCK) void k_reduce_energy(const double *epart, int ecap, int n,
+                                                         double *out) {
+    __shared__ double s[BLOCK];
+    const int cat = blockIdx.x, tid = threadIdx.x;
+    const double *src = epart + (size_t)cat * ecap;
+    double acc = 0.0;
+    for (int k = tid; k < n; k += BLOCK) acc += src[k];
+    s[tid] = acc;
+    __syncthreads();
+    for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        if (tid < h) s[tid] += s[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) out[cat] = s[0];
+}
+
+hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_energy, dim3(NE), dim3(BLOCK), 0, s, epart, ecap, n, out);
+    return hipGetLastError();
+}
+
+__global__ void k_advance(int64_t *ctr, int64_t by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctr[0] += by;
+}
+
+hipError_t launch_advance(int64_t *ctr, int64_t by, hipStream_t s) {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, ctr, by);
+    return hipGetLastError();
+}
+
+// ---- INTERACT (interact.f90:19-419): energy of one particle against the system
+// energy-only DNA term as INTERACT writes it (interact.f90:212-348): angle through
+// angbond.f90, dihedral through angdih.f90, base pairing without scalepairing
+__device__ double dna_term_interact(const StepParams &P, uint32_t w) {
+    const int kind = w >> 29, t = w & 0x7FFFFFF;
+    const double eps = P.epsnuc;
+    const int n = P.nlist[kind];
+    const int32_t *L = P.lst[kind];
+    if (kind == T_BOND) {
+        V3 d = sub3(ld3(P, L[t]), ld3(P, L[n + t]));
+        double dist = sqrt(dot3(d, d));
+        double vard = dist - P.lval[0][t];
+        return eps * (vard * vard) * (1.0 + 100.0 * (vard * vard));
+    } else if (kind == T_ANGLE) {
+        V3 rb = ld3(P, L[n + t]);
+        V3 v1 = sub3(ld3(P, L[t]), rb), v2 = sub3(ld3(P, L[2 * n + t]), rb);
+        double r1 = sqrt(dot3(v1, v1)), r2 = sqrt(dot3(v2, v2));
+        double ang = acos(dot3(v1, v2) / (r1 * r2));
+        double varang = ang - P.lval[1][t];
+        return 700.0 * eps * (varang * varang);
+    } else if (kind == T_DIHE) {
+        V3 r1 = ld3(P, L[t]), r2 = ld3(P, L[n + t]), r3 = ld3(P, L[2 * n + t]), r4 = ld3(P, L[3 * n + t]);
+        V3 b1 = sub3(r2, r1), b2 = sub3(r3, r2), b3 = sub3(r4, r3);
+        V3 n1 = cross3(b1, b2), n2 = cross3(b2, b3);
+        double rr = sqrt(dot3(b2, b2));
+        double sinphi = rr * dot3(b1, n2);
+        double dihe = atan2(sinphi, dot3(n1, n2));
+        double vardihe = dihe - P.lval[2][t];
+        return 28.0 * eps * (1.0 - cos(vardihe));
+    }
+    int a = L[t], b = L[n + t];
+    V3 v = sub3(ld3(P, b), ld3(P, a));
+    double dist2 = dot3(v, v);
+    if (kind == T_STACK) {
+        double sg = P.lval[3][t];
+        double cte1 = pow3(sg * sg / dist2);
+        return 4.0 * eps * cte1 * (cte1 - 1.0);
+    } else if (kind == T_BP) {
+        int na = P.namsite[a];
+        double epsln = (na == 5 || na == 4) ? 2.532 * eps : 2.0 * eps;
+        double sg = P.lval[4][t];
+        double cte1 = sg * sg / dist2;
+        return epsln * pow5(cte1) * (20.0 * cte1 - 24.0);
+    } else if (kind == T_EX) {
+        double sgex2 = P.lval[5][t] * P.lval[5][t];
+        if (dist2 < sgex2) {
+            double cte1 = pow3(sgex2 / dist2);
+            return 4.0 * eps * cte1 * (cte1 - 1.0) + eps;
+        }
+        return 0.0;
+    } else if (kind == T_QQ) {
+        double dist = sqrt(dist2);
+        double idist = 1.0 / dist;
+        if (!(P.flags & QDEBY)) return P.fctn * idist;
+        return P.fctn * idist * exp(-dist * P.ikappa);
+    }
+    double dist = sqrt(dist2);
+    double cte1 = exp((13.38 - dist) * 0.1875);
+    return P.epsolv * ((1.0 - cte1) * (1.0 - cte1)) - P.epsolv;
+}
+
+// analytic pair energy as INTERACT writes it (interact.f90:182-204,365-383): the SRPMF
+// smoothing reads e*exp(...)-srpy (ENERGY has e*(exp(...)-srpy))
+__device__ __forceinline__ double interact_analytic(const DevPair &p, const StepParams &P,
+                                                    double dist2) {
+    double e = 0.0, idist = 0.0, idist2 = 1.0 / dist2;
+    if (p.flags & (PF_CHR | PF_SRPMF)) idist = sqrt(idist2);
+    if (p.flags & PF_CHR) e = e + p.fct * idist;
+    if (p.flags & PF_LJ) {
+        double dist6 = pow3(p.sgp2 * idist2);
+        e = e + p.epp4 * dist6 * (dist6 - 1.0);
+    }
+    if ((p.flags & PF_SRPMF) && dist2 <= P.rth) {
+        double dist = 1.0 / idist;
+        double es = p.c0 * exp((p.c1 - dist) * p.c2) * cos(p.c3 * PI * (p.c1 - dist)) +
+                    p.c4 * pow6(p.c1 * idist);
+        if (dist >= P.srpx) es = es * exp(-P.srpk * (dist - P.srpx)) - P.srpy;
+        e = e + es;
+    }
+    return e;
+}
+
+// gety, polynom.f90:172-194
+__device__ __forceinline__ double gety(const DevPair &p, const double4 *__restrict__ ep,
+                                       double ires, double x2) {
+    if (x2 < p.dm2lo) {
+        double ix2 = 1.0 / x2;
+        return p.sc1 * pow6(ix2) + p.sc2;
+    } else if (x2 > p.dm2hi) {
+        double ix2 = 1.0 / x2;
+        if (p.flags & PF_CHR) return p.sc3 * pow3(ix2) + p.fct * sqrt(ix2);
+        return p.sc3 * pow3(ix2);
+    }
+    double x = sqrt(x2);
+    int i = (int)((x - p.dmi) * ires) + p.nxi;
+    double4 e = ep[i];
+    return e.x + x * (e.y + x * e.z);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_interact(InteractParams Q) {
+    const StepParams &P = Q.sp;
+    __shared__ DevPair s_pairs[MAX_LDS_PAIRS];
+    __shared__ int32_t s_pidx[MAX_LDS_TYPES * MAX_LDS_TYPES];
+    __shared__ double s_red[WPB];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const int ns = P.nsites, ntot = P.ntot, nt1 = P.nttyp + 1;
+    for (int k = tid; k <= P.npair; k += BLOCK) s_pairs[k] = P.pairs[k];
+    for (int k = tid; k < nt1 * nt1; k += BLOCK) s_pidx[k] = P.pairidx[k];
+    __syncthreads();
+    const int j = Q.j;
+    const double xj = Q.xj, yj = Q.yj, zj = Q.zj;
+    const bool beion = (P.flags & QPAR) && j >= ns;
+    const int jt2 = beion ? P.nwtype[Q.jtype] : P.t2[j];
+    const int prow = jt2 * nt1;
+    double e = 0.0;
+    // staticf0.f90:19-82 and vdwgd0trln.f90:19-117 on lanes 0..7 of wave 0
+    if (tid < WAVE) {
+        double es = 0.0, ev = 0.0, ph = 0.0;
+        bool okv = false;
+        if ((P.flags & QPHIX) && tid < 8) {
+            const GridGeom &g = P.gx;
+            bool besite = (P.flags & QNUCL) && j < ns && P.namsite[j] == 1;
+            bool ok = beion || besite;
+            double chi = beion ? P.cg[Q.jtype] : P.cgnuc;
+            ok = ok && xj <= g.xbcen + g.tranx && xj >= g.xbcen - g.tranx &&
+                 yj <= g.ybcen + g.trany && yj >= g.ybcen - g.trany &&
+                 zj <= g.zbcen + g.tranz && zj >= g.zbcen - g.tranz;
+            if (ok) {
+                double gx_, gy_, gz_;
+                int ix, iy, iz;
+                grid_cell(g, xj, yj, zj, gx_, gy_, gz_, ix, iy, iz);
+                int n1 = ix + (tid >> 2), n2 = iy + ((tid >> 1) & 1), n3 = iz + (tid & 1);
+                double ai = 1.0 - fabs(gx_ - n1 * g.dcel) * g.idcel;
+                double bi = 1.0 - fabs(gy_ - n2 * g.dcel) * g.idcel;
+                double ci = 1.0 - fabs(gz_ - n3 * g.dcel) * g.idcel;
+                double fi = ai * bi * ci;
+                es = fi * chi * (double)P.phix[((size_t)n1 * g.ny + n2) * g.nz + n3] * P.celec;
+            }
+        }
+        if (P.flags & QPHIV) {
+            const GridGeom &g = P.gv;
+            okv = xj <= g.xbcen + g.tranx && xj >= g.xbcen - g.tranx && yj <= g.ybcen + g.trany &&
+                  yj >= g.ybcen - g.trany && zj <= P.vzmax && zj >= P.vzmin;
+            if (okv && tid < 8) {
+                double gx_, gy_, gz_;
+                int ix, iy, iz;
+                grid_cell(g, xj, yj, zj, gx_, gy_, gz_, ix, iy, iz);
+                double esvdw = P.svdw;
+                if (P.flags & QSVDW) esvdw = esvdw * P.scal[jt2];
+                int n1 = ix + (tid >> 2), n2 = iy + ((tid >> 1) & 1), n3 = iz + (tid & 1);
+                double ai = 1.0 - fabs(gx_ - n1 * g.dcel) * g.idcel;
+                double bi = 1.0 - fabs(gy_ - n2 * g.dcel) * g.idcel;
+                double ci = 1.0 - fabs(gz_ - n3 * g.dcel) * g.idcel;
+                double fi = ai * bi * ci;
+                ph = (double)P.phiv[((size_t)n1 * g.ny + n2) * g.nz + n3];
+                ev = fi * esvdw * ph;
+            }
+        }
+        es = wave_sum(es);
+        ev = wave_sum(ev);
+        ph = wave_sum(ph);
+        if (tid == 0) {
+            e += es;
+            if (okv) {
+                if (ph >= P.thold8) {
+                    ev = 1.0e10;
+                    if (Q.alert) atomicAdd(&P.warn[jt2], 1);
+                }
+                e += ev;
+            }
+        }
+    }
+    const bool nonbond = (P.flags & QNONBOND) != 0;
+    if (beion && nonbond) {
+        for (int i = ns + tid; i < ntot; i += BLOCK) {
+            if (i == j) continue;
+            double dx = P.x[i] - xj, dy = P.y[i] - yj, dz = P.z[i] - zj;
+            double d2 = dx * dx + dy * dy + dz * dz;
+            const DevPair &pp = s_pairs[s_pidx[prow + P.t2[i]]];
+            e += (pp.flags & PF_EFPOT) ? gety(pp, P.ep, P.ires, d2) : interact_analytic(pp, P, d2);
+        }
+    }
+    if ((P.flags & QNUCL) && !beion) {
+        const int k0 = P.inc_ptr[j], k1 = P.inc_ptr[j + 1];
+        for (int k = k0 + tid; k < k1; k += BLOCK) {
+            uint32_t wd = P.inc[k];
+            int kind = wd >> 29;
+            bool on = kind <= T_DIHE ? (P.flags & QNOBOND) != 0
+                                     : (nonbond && (kind != T_SLV || (P.flags & QSOLV) != 0));
+            if (on) e += dna_term_interact(P, wd);
+        }
+    }
+    if ((P.flags & QPAR) && (P.flags & QNUCL) && nonbond) {
+        const int lo = beion ? 0 : ns, hi = beion ? ns : ntot;
+        for (int i = lo + tid; i < hi; i += BLOCK) {
+            double dx = P.x[i] - xj, dy = P.y[i] - yj, dz = P.z[i] - zj;
+            double d2 = dx * dx + dy * dy + dz * dz;
+            const DevPair &pp = s_pairs[s_pidx[prow + P.t2[i]]];
+            e += (pp.flags & PF_EFPOT) ? gety(pp, P.ep, P.ires, d2) : interact_analytic(pp, P, d2);
+        }
+    }
+    e = wave_sum(e);
+    if (lane == 0) s_red[tid >> 6] = e;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int k = 0; k < WPB; k++) t += s_red[k];
+        Q.out[0] = t;
+    }
+}
+
+hipError_t launch_interact(const InteractParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_interact, dim3(1), dim3(BLOCK), 0, s, p);
+    return hipGetLastError();
+}
+
+__global__ void k_philox_raw(const uint64_t *ctr, const uint64_t *key, int n, uint64_t *out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint64_t c[4] = {ctr[4 * k], ctr[4 * k + 1], ctr[4 * k + 2], ctr[4 * k + 3]};
+    philox4x64(c, key[2 * k], key[2 * k + 1]);
+    for (int q = 0; q < 4; q++) out[4 * k + q] = c[q];
+}
+
+hipError_t launch_philox_raw(const uint64_t *ctr, const uint64_t *key, int n, uint64_t *out,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_philox_raw, dim3((n + 255) / 256), dim3(256), 0, s, ctr, key, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace bromoc

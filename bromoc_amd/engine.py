@@ -1,0 +1,168 @@
+"""Python front-end of libbromoc_gpu.so, shaped like the reference's routines.
+
+    eng = Engine(system)            # the set-up shell_simul.f90 does before SIMUL
+    e   = eng.energy()              # SUBROUTINE ENERGY          (energy.mono.f90:19)
+    eng.step(nbd, ...)              # simul1.f90:363-390 BD loop (ENERGY + DYNAMICS0/0NUC)
+    d   = eng.interact(xyz, t, j)   # SUBROUTINE INTERACT        (interact.f90:19)
+    eng.grand(ngcmc, ...)           # SUBROUTINE GRAND           (grand.f90:19)
+
+Every call goes to the HIP library; a missing library raises ImportError and there is no
+CPU fallback.  Errors come back as BromocError (the reference instead stops through
+error(...,faterr), error.f90:30-32).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .system import Gcmc, System, ptr
+
+
+class Engine:
+    def __init__(self, system: System, device: int = 0):
+        self.lib = abi.load_library()
+        self.system = system
+        st, keep = system.to_struct()
+        ctx = C.c_void_p()
+        rc = self.lib.bromoc_create(C.byref(st), device, C.byref(ctx))
+        if rc:
+            raise abi.BromocError("bromoc_create", rc)
+        self.ctx = ctx
+        self.ntot = system.ntot
+        self._noise_rows = 0
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.bromoc_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, name, rc):
+        if rc:
+            msg = self.lib.bromoc_last_error(self.ctx)
+            raise abi.BromocError(name, rc, msg.decode() if msg else "")
+
+    # --------------------------------------------------------------- state
+    def positions(self):
+        n = self.ntot
+        x, y, z = np.zeros(n), np.zeros(n), np.zeros(n)
+        t = np.zeros(n, np.int32)
+        nn = C.c_int32()
+        self._chk("bromoc_get_positions",
+                  self.lib.bromoc_get_positions(self.ctx, ptr(x), ptr(y), ptr(z),
+                                                ptr(t, C.c_int32), C.byref(nn)))
+        return np.stack([x, y, z], 1), t
+
+    def set_positions(self, pos, typei=None):
+        pos = np.ascontiguousarray(pos, float)
+        n = len(pos)
+        x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+        tp = None
+        if typei is not None:
+            ta = np.ascontiguousarray(typei, np.int32)
+            tp = ptr(ta, C.c_int32)
+        self._chk("bromoc_set_positions",
+                  self.lib.bromoc_set_positions(self.ctx, n, ptr(x), ptr(y), ptr(z), tp))
+        self.ntot = n
+
+    def forces(self):
+        n = self.ntot
+        fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._chk("bromoc_get_forces", self.lib.bromoc_get_forces(self.ctx, ptr(fx), ptr(fy), ptr(fz)))
+        return np.stack([fx, fy, fz], 1)
+
+    # ------------------------------------------------------------ routines
+    def energy(self, forces=True):
+        e = abi.Energies()
+        self._chk("bromoc_energy", self.lib.bromoc_energy(self.ctx, 1 if forces else 0, C.byref(e)))
+        return e.as_dict()
+
+    def upload_noise(self, noise):
+        noise = np.ascontiguousarray(noise, float)
+        self._chk("bromoc_upload_noise",
+                  self.lib.bromoc_upload_noise(self.ctx, noise.shape[0], ptr(noise)))
+        self._noise_rows = noise.shape[0]
+
+    def step(self, nsteps, mode=abi.NOISE_INJECTED, first_row=0, seed=0, counter0=0,
+             energies=True):
+        e = abi.Energies()
+        self._chk("bromoc_step",
+                  self.lib.bromoc_step(self.ctx, nsteps, mode, first_row, seed, counter0,
+                                       C.byref(e) if energies else None))
+        return e.as_dict() if energies else None
+
+    def time_steps(self, nsteps, mode=abi.NOISE_PHILOX, first_row=0, seed=0, counter0=0):
+        """Run nsteps BD steps with HIP events around every launch: (total_ms, kernel_ms)."""
+        tot, ker = C.c_double(), C.c_double()
+        self._chk("bromoc_time_steps",
+                  self.lib.bromoc_time_steps(self.ctx, nsteps, mode, first_row, seed, counter0,
+                                             C.byref(tot), C.byref(ker)))
+        return tot.value, ker.value
+
+    def dynamics(self, mode=abi.NOISE_INJECTED, row=0, seed=0, counter=0):
+        self._chk("bromoc_dynamics", self.lib.bromoc_dynamics(self.ctx, mode, row, seed, counter))
+
+    def flux(self):
+        s = self.system
+        nt, np_ = s.ntype - s.nold, len(s.zcont)
+        nf = np.zeros(max(nt * np_, 1), np.int32)
+        nb = np.zeros(max(nt * np_, 1), np.int32)
+        self._chk("bromoc_get_flux",
+                  self.lib.bromoc_get_flux(self.ctx, ptr(nf, C.c_int32), ptr(nb, C.c_int32)))
+        return nf[:nt * np_].reshape(nt, np_), nb[:nt * np_].reshape(nt, np_)
+
+    def warn(self):
+        w = np.zeros(self.system.nttyp + 1, np.int32)
+        self._chk("bromoc_get_warn", self.lib.bromoc_get_warn(self.ctx, ptr(w, C.c_int32)))
+        return w
+
+    def interact(self, xyz, jtype, j, alert=False):
+        d = C.c_double()
+        self._chk("bromoc_interact",
+                  self.lib.bromoc_interact(self.ctx, float(xyz[0]), float(xyz[1]), float(xyz[2]),
+                                           int(jtype), int(j), 1 if alert else 0, C.byref(d)))
+        return d.value
+
+    def set_gcmc(self, g: Gcmc, ibuffer):
+        st, self._gkeep = g.to_struct()
+        self._chk("bromoc_set_gcmc", self.lib.bromoc_set_gcmc(self.ctx, C.byref(st)))
+        ib = np.ascontiguousarray(ibuffer[:self.ntot], np.int32)
+        self._chk("bromoc_set_ibuffer", self.lib.bromoc_set_ibuffer(self.ctx, ptr(ib, C.c_int32)))
+
+    def ibuffer(self):
+        ib = np.zeros(self.ntot, np.int32)
+        self._chk("bromoc_get_ibuffer", self.lib.bromoc_get_ibuffer(self.ctx, ptr(ib, C.c_int32)))
+        return ib
+
+    def grand(self, ngcmc, uniforms, nat, prob, icycle, ener):
+        u = np.ascontiguousarray(uniforms, float)
+        used = C.c_int64()
+        e = C.c_double(ener)
+        self._chk("bromoc_grand",
+                  self.lib.bromoc_grand(self.ctx, ngcmc, ptr(u), len(u), C.byref(used),
+                                        ptr(nat, C.c_int32), ptr(prob, C.c_int32), icycle,
+                                        C.byref(e)))
+        nn = C.c_int32()
+        self.lib.bromoc_get_positions(self.ctx, None, None, None, None, C.byref(nn))
+        self.ntot = nn.value
+        return used.value, e.value
+
+
+def philox4x64(ctr, key):
+    """Raw Philox4x64-10 blocks computed on the device."""
+    lib = abi.load_library()
+    ctr = np.ascontiguousarray(ctr, np.uint64).reshape(-1, 4)
+    key = np.ascontiguousarray(key, np.uint64).reshape(-1, 2)
+    out = np.zeros_like(ctr)
+    rc = lib.bromoc_philox4x64(len(ctr), ptr(ctr, C.c_uint64), ptr(key, C.c_uint64),
+                               ptr(out, C.c_uint64))
+    if rc:
+        raise abi.BromocError("bromoc_philox4x64", rc)
+    return out

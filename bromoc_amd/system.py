@@ -1,0 +1,278 @@
+"""Host-side description of one BROMOC system (the state shell_simul.f90 builds before
+SIMUL) and its conversion to the C ABI struct.
+
+Arrays use the ABI conventions of include/bromoc_gpu.h: 0-based particle indices, 1-based
+type tables (entry 0 unused), component-major site lists.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+# constamod.f90:21-36
+PI = 3.14159265358979323846264338327950288419716939937510
+KBOLTZ = 1.380648813E-23
+AVOGADRO = 6.0221412927E23
+KCAL = 4184.0
+KCALMOL = KCAL / AVOGADRO
+COULOMB = 1.60217656535E-19
+EPERMIT = 8.854187817620E-12
+ANGSTROM = 1E-10
+LITER = 1E-3
+CELEC = COULOMB ** 2 / (4.0 * PI * EPERMIT * KCALMOL * ANGSTROM)
+
+
+def kbt_of(temp):
+    """kBT in kcal/mol (shell_simul.f90:865)."""
+    return KBOLTZ * temp / KCALMOL
+
+
+@dataclass
+class GridData:
+    data: np.ndarray          # (nclx, ncly, nclz), float32 (phix) or int8 (phiv)
+    dcel: float
+    xbcen: float = 0.0
+    ybcen: float = 0.0
+    zbcen: float = 0.0
+
+    @property
+    def shape(self):
+        return self.data.shape
+
+    def tran(self):
+        n = self.data.shape
+        return tuple(0.5 * (k - 1) * self.dcel for k in n)
+
+
+@dataclass
+class System:
+    ntot: int
+    nsites: int
+    nfix: int
+    ntype: int
+    nold: int
+    nttyp: int
+    flags: int
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    typei: np.ndarray
+    nwtype: np.ndarray
+    cg: np.ndarray
+    fact1a: np.ndarray
+    fact2a: np.ndarray
+    pairidx: np.ndarray
+    pairs: list                     # list of dicts, index 0 unused (None)
+    ep: np.ndarray
+    res: float = 0.01
+    capacity: int = 0
+    typtyp: np.ndarray = None
+    namsite: np.ndarray = None
+    stfree: np.ndarray = None
+    rth: float = 0.0
+    srpx: float = 0.0
+    srpk: float = 0.0
+    srpy: float = 0.0
+    cgnuc: float = -1.0
+    epsnuc: float = 0.184
+    scalepairing: float = 1.0
+    fctn: float = 0.0
+    kappa: float = 0.0
+    ikappa: float = 0.0
+    epsolv: float = 0.0
+    lists: dict = field(default_factory=dict)   # name -> (sites int32 [k*n], vals or None)
+    phix: GridData | None = None
+    phiv: GridData | None = None
+    svdw: float = 50.0
+    thold8: float = 8.0
+    vzmin: float = 0.0
+    vzmax: float = 0.0
+    scal: np.ndarray = None
+    bdmax: float = 1.0
+    fact0n1: float = 0.0
+    fact0n2: float = 0.0
+    box: tuple = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0)   # lx2m, lx2p, ly2m, ly2p, lz2m, lz2p
+    zcont: np.ndarray = None
+    ndna: int = 0
+    nion: int = 0
+    meta: dict = field(default_factory=dict)
+
+    LIST_NAMES = ("bond", "angle", "dihe", "stack", "bp", "ex", "qq", "slv")
+    LIST_ARITY = {"bond": 2, "angle": 3, "dihe": 4, "stack": 2, "bp": 2, "ex": 2, "qq": 2,
+                  "slv": 2}
+
+    def __post_init__(self):
+        if not self.capacity:
+            self.capacity = self.ntot
+        ns = self.nsites
+        if self.typtyp is None:
+            self.typtyp = np.zeros(ns, np.int32)
+        if self.namsite is None:
+            self.namsite = np.zeros(ns, np.int32)
+        if self.stfree is None:
+            self.stfree = np.ones(ns, np.int8)
+        if self.scal is None:
+            self.scal = np.ones(self.nttyp + 1)
+        if self.zcont is None:
+            self.zcont = np.zeros(0)
+        for n in self.LIST_NAMES:
+            self.lists.setdefault(n, (np.zeros(0, np.int32), None))
+
+    @property
+    def npair(self):
+        return len(self.pairs) - 1
+
+    def count(self, name):
+        return len(self.lists[name][0]) // self.LIST_ARITY[name]
+
+    def copy(self):
+        import copy
+        return copy.deepcopy(self)
+
+    # ------------------------------------------------------------ C struct
+    def to_struct(self):
+        """Returns (SystemStruct, keepalive) -- keep both alive while C reads them."""
+        keep = []
+
+        def arr(a, dtype, ctype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ctype))
+
+        s = abi.SystemStruct()
+        s.abi_version = abi.ABI_VERSION
+        s.ntot, s.nsites, s.nfix = self.ntot, self.nsites, self.nfix
+        s.capacity = self.capacity
+        s.ntype, s.nold, s.nttyp = self.ntype, self.nold, self.nttyp
+        s.npair = self.npair
+        s.flags = self.flags
+        s.x = arr(self.x[:self.ntot], np.float64, C.c_double)
+        s.y = arr(self.y[:self.ntot], np.float64, C.c_double)
+        s.z = arr(self.z[:self.ntot], np.float64, C.c_double)
+        s.typei = arr(self.typei[:self.ntot], np.int32, C.c_int32)
+        s.typtyp = arr(self.typtyp, np.int32, C.c_int32)
+        s.namsite = arr(self.namsite, np.int32, C.c_int32)
+        s.stfree = arr(self.stfree, np.int8, C.c_int8)
+        s.nwtype = arr(self.nwtype, np.int32, C.c_int32)
+        s.cg = arr(self.cg, np.float64, C.c_double)
+        s.fact1a = arr(self.fact1a, np.float64, C.c_double)
+        s.fact2a = arr(self.fact2a, np.float64, C.c_double)
+        s.pairidx = arr(self.pairidx, np.int32, C.c_int32)
+        parr = (abi.Pair * (self.npair + 1))()
+        for k, p in enumerate(self.pairs):
+            if p is None:
+                continue
+            for name, _ in abi.Pair._fields_:
+                if name in p:
+                    setattr(parr[k], name, p[name])
+        keep.append(parr)
+        s.pairs = C.cast(parr, C.POINTER(abi.Pair))
+        s.nknots = len(self.ep) // 3
+        s.ep = arr(self.ep, np.float64, C.c_double)
+        s.res, s.ires = self.res, 1.0 / self.res
+        s.rth, s.srpx, s.srpk, s.srpy = self.rth, self.srpx, self.srpk, self.srpy
+        s.cgnuc, s.epsnuc, s.scalepairing = self.cgnuc, self.epsnuc, self.scalepairing
+        s.fctn, s.kappa, s.ikappa, s.epsolv = self.fctn, self.kappa, self.ikappa, self.epsolv
+        cnt = {n: self.count(n) for n in self.LIST_NAMES}
+        s.nbond, s.nangle, s.ndihe = cnt["bond"], cnt["angle"], cnt["dihe"]
+        s.nstack, s.nbp, s.nex, s.nqq, s.nsolv = (cnt["stack"], cnt["bp"], cnt["ex"],
+                                                  cnt["qq"], cnt["slv"])
+        for name, sfield, vfield in (("bond", "sitebond", "distbond"),
+                                     ("angle", "siteangle", "valangle"),
+                                     ("dihe", "sitedihe", "valdihe"),
+                                     ("stack", "sitestack", "sgstack"),
+                                     ("bp", "sitebp", "sgbp"), ("ex", "siteex", "sgex"),
+                                     ("qq", "siteqq", None), ("slv", "siteslv", None)):
+            sites, vals = self.lists[name]
+            setattr(s, sfield, arr(sites if len(sites) else np.zeros(1), np.int32, C.c_int32))
+            if vfield:
+                v = vals if vals is not None and len(vals) else np.zeros(1)
+                setattr(s, vfield, arr(v, np.float64, C.c_double))
+        for gname, dtype in (("phix", np.float32), ("phiv", np.int8)):
+            g = getattr(self, gname)
+            cg = getattr(s, gname)
+            if g is None:
+                continue
+            d = np.ascontiguousarray(g.data, dtype=dtype)
+            keep.append(d)
+            cg.nclx, cg.ncly, cg.nclz = d.shape
+            cg.dcel, cg.xbcen, cg.ybcen, cg.zbcen = g.dcel, g.xbcen, g.ybcen, g.zbcen
+            cg.data = d.ctypes.data
+        s.svdw, s.thold8, s.vzmin, s.vzmax = self.svdw, self.thold8, self.vzmin, self.vzmax
+        s.scal = arr(self.scal, np.float64, C.c_double)
+        s.bdmax, s.fact0n1, s.fact0n2 = self.bdmax, self.fact0n1, self.fact0n2
+        (s.lx2m, s.lx2p, s.ly2m, s.ly2p, s.lz2m, s.lz2p) = self.box
+        s.cntpts = len(self.zcont)
+        s.zcont = arr(self.zcont if len(self.zcont) else np.zeros(1), np.float64, C.c_double)
+        return s, keep
+
+
+def nindex_table(nttyp, ndna, nion):
+    """nindex.f90:19-35 over atnam2 types 1..nttyp, as a dense (nttyp+1)^2 table."""
+    t = np.zeros((nttyp + 1, nttyp + 1), np.int32)
+    for a in range(1, nttyp + 1):
+        for b in range(1, nttyp + 1):
+            mx, mn = (a, b) if a > b else (b, a)
+            maxi = mx - ndna
+            if mn > ndna:
+                t[a, b] = maxi * (maxi - 1) // 2 + mn - ndna + ndna * nion
+            elif mx > ndna:
+                t[a, b] = (mn - 1) * nion + maxi
+    return t.reshape(-1)
+
+
+@dataclass
+class Gcmc:
+    """GCMC buffers (grandmod) in ABI form; arrays are 1-based (entry 0 unused)."""
+    ibfftyp: np.ndarray
+    avnum: np.ndarray
+    mu: np.ndarray
+    kb: np.ndarray
+    lzmin: np.ndarray
+    lzmax: np.ndarray
+    bufferbias: np.ndarray
+    ntotat: np.ndarray
+    lx: float
+    ly: float
+    cx: float
+    cy: float
+    cz: float
+    ikbt: float
+
+    @property
+    def nbuffer(self):
+        return len(self.ibfftyp) - 1
+
+    def to_struct(self):
+        keep = []
+
+        def arr(a, dtype, ctype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ctype))
+
+        g = abi.GcmcStruct()
+        g.nbuffer = self.nbuffer
+        g.ibfftyp = arr(self.ibfftyp, np.int32, C.c_int32)
+        g.avnum = arr(self.avnum, np.float64, C.c_double)
+        g.mu = arr(self.mu, np.float64, C.c_double)
+        g.kb = arr(self.kb, np.float64, C.c_double)
+        g.lzmin = arr(self.lzmin, np.float64, C.c_double)
+        g.lzmax = arr(self.lzmax, np.float64, C.c_double)
+        g.bufferbias = arr(self.bufferbias, np.int8, C.c_int8)
+        g.ntotat = arr(self.ntotat, np.int32, C.c_int32)
+        g.lx, g.ly, g.cx, g.cy, g.cz, g.ikbt = (self.lx, self.ly, self.cx, self.cy, self.cz,
+                                                self.ikbt)
+        return g, keep
+
+
+def ptr(a, ctype=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+__all__ = ["System", "GridData", "Gcmc", "nindex_table", "CELEC", "kbt_of", "ptr", "math"]

@@ -1,0 +1,257 @@
+/*
+ * bromoc_gpu.h -- C ABI of the MI355X (gfx950) BROMOC force / Brownian-dynamics path.
+ *
+ * The reference driver (cg-gcmc-bd, Fortran) keeps all state in module globals and calls
+ * argument-less or by-reference routines.  This ABI takes that state explicitly, through
+ * plain pointers and sizes, so a thin iso_c_binding module can forward the module arrays
+ * and the Fortran driver keeps its routine names and call sites.
+ *
+ * Entry point                  replaces (reference interface)
+ * ---------------------------  -----------------------------------------------------------
+ * bromoc_create                one-time state set-up done by shell_simul.f90 (PARTICLE,
+ *                              NUCLEOTIDE, EFPOT, GSBP, SYSTEM commands) before SIMUL
+ * bromoc_energy                SUBROUTINE ENERGY            energy.mono.f90:19-692
+ * bromoc_step                  simul1.f90:363-390: ENERGY + DYNAMICS0 (dynamics0.mono.f90:19-48)
+ *                              + DYNAMICS0NUC (dynamics0nuc.f90:19-46), nbd times
+ * bromoc_dynamics              DYNAMICS0(ninit,nfinal) + DYNAMICS0NUC(1,nsites) alone
+ * bromoc_interact              SUBROUTINE INTERACT          interact.f90:19-419
+ * bromoc_grand                 SUBROUTINE GRAND             grand.f90:19-121
+ * bromoc_build_table           EFPOT table construction     splinepot.f90:19-96 +
+ *                              squarespline polynom.f90:20-48
+ *
+ * Conventions
+ *  - Particle indices are 0-based (C index i is Fortran index i+1).  Sites occupy
+ *    0..nsites-1, fixed ions nsites..nsites+nfix-1, moving ions nsites+nfix..ntot-1,
+ *    the same order as grandmod's x,y,z arrays.
+ *  - Type tables keep the reference's 1-based type numbering: arrays indexed by type
+ *    have length ntype+1 (or nttyp+1) and entry 0 unused.
+ *  - Two-index site lists (sitebond, siteangle, ...) are component-major:
+ *    list[k*n + t] is site k of term t (0-based site ids).
+ *  - Grids are x-major, z fastest: value(ix,iy,iz) = data[(ix*ncly + iy)*nclz + iz],
+ *    as in staticf1.f90:91.
+ *  - All arithmetic is IEEE binary64.  Grid payloads keep the reference storage
+ *    precision (phix float32, phiv int8; gsbpmod.f90:73-74) and are promoted exactly.
+ *  - Every entry point returns a status code; nothing aborts the process.
+ */
+#ifndef BROMOC_GPU_H
+#define BROMOC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BROMOC_ABI_VERSION 1
+
+/* status codes */
+#define BROMOC_OK              0
+#define BROMOC_E_ARG           1  /* null pointer, bad size or index */
+#define BROMOC_E_UNSUPPORTED   2  /* a reference option this path does not implement */
+#define BROMOC_E_CAPACITY      3  /* ntot would exceed capacity (reference: ntot>datom, grand.f90:66) */
+#define BROMOC_E_HIP           4  /* a HIP runtime call failed */
+#define BROMOC_E_NODEVICE      5  /* no gfx950 device visible */
+#define BROMOC_E_STATE         6  /* call order (e.g. injected noise not uploaded) */
+
+/* flags: the reference's logical switches that select ENERGY/DYNAMICS branches */
+#define BROMOC_QPAR        (1u << 0)   /* ions present (nucleotmod Qpar) */
+#define BROMOC_QNUCL       (1u << 1)   /* CG-DNA present (Qnucl) */
+#define BROMOC_QNONBOND    (1u << 2)   /* nonbonded terms (Qnonbond) */
+#define BROMOC_QNOBOND     (1u << 3)   /* bonded DNA terms (Qnobond) */
+#define BROMOC_QPHIX       (1u << 4)   /* static field grid (Qphix) */
+#define BROMOC_QPHIV       (1u << 5)   /* repulsion grid, trilinear (Qphiv .and. Qtrln) */
+#define BROMOC_QSVDW       (1u << 6)   /* per-type repulsion scaling (Qsvdw) */
+#define BROMOC_QSOLV       (1u << 7)   /* solvent-induced S-S term (Qsolv) */
+#define BROMOC_QDEBY       (1u << 8)   /* Debye screening of P-P Coulomb (Qdeby) */
+#define BROMOC_QCOUNTION   (1u << 9)   /* flux counting across z planes (Qcountion) */
+#define BROMOC_QDNAFREE    (1u << 10)  /* DNA sites not confined by fixcoor (Qdnafree) */
+/* options of the reference that this path rejects with BROMOC_E_UNSUPPORTED */
+#define BROMOC_QMEMB       (1u << 16)  /* analytic membrane (membrane.f90) */
+#define BROMOC_QMMIJ       (1u << 17)  /* MMIJ reaction field (rect_rf1/sphe_rf1) */
+#define BROMOC_QBSPLINE    (1u << 18)  /* 27-point B-spline phiv (vdwgd1spln.f90) */
+#define BROMOC_QNMCDEN     (1u << 19)  /* per-species phiv maps (Qnmcden) */
+#define BROMOC_QDEBYHYB    (1u << 20)  /* hybrid Debye (Qdebyhyb) */
+#define BROMOC_QPROXDIFF   (1u << 21)  /* proximity diffusion, DYNAMICS3 */
+#define BROMOC_QNOTRANS    (1u << 22)  /* NOTRANS centroid reset */
+#define BROMOC_QAPFOR      (1u << 23)  /* constant applied forces */
+#define BROMOC_QCONTRANS   (1u << 24)  /* centroid restraints */
+#define BROMOC_QRFPAR      (1u << 25)  /* reaction-field parameter grids (rfpar.f90) */
+#define BROMOC_QSPHERE     (1u << 26)  /* spherical system (fixcoor.f90:24-31) */
+#define BROMOC_QECYL       (1u << 27)  /* elliptic-cylinder system (fixcoor.f90:33-46) */
+#define BROMOC_QDIFFUSE    (1u << 28)  /* DYNAMICS1 switch diffusion */
+#define BROMOC_QPROFILE    (1u << 29)  /* DYNAMICS2 spline diffusion profile */
+
+/* site names (nucleotmod namsite) */
+#define BROMOC_SITE_S   0
+#define BROMOC_SITE_P   1
+#define BROMOC_SITE_AB  2
+#define BROMOC_SITE_TB  3
+#define BROMOC_SITE_CB  4
+#define BROMOC_SITE_GB  5
+
+/* Parameters of one pair type `is` (nindex.f90).  Fields follow efpmod/extramod/grandmod:
+ * Qefpot, Qchr, Qlj, Qsrpmfi, fct, epp4, sgp2, c0..c4, dmi, dm2(1:2), sc(1:3). */
+typedef struct bromoc_pair {
+    int32_t efpot, chr, lj, srpmf;
+    int32_t nxi;         /* 0-based index of this pair's first knot in ep */
+    int32_t nknots;      /* knots of this pair's table (0 when efpot == 0) */
+    double  fct, epp4, sgp2;
+    double  c0, c1, c2, c3, c4;
+    double  dmi, dm2lo, dm2hi;
+    double  sc1, sc2, sc3;
+} bromoc_pair;
+
+/* A regular grid as read by readphi.f90:43-49,160-170. */
+typedef struct bromoc_grid {
+    int32_t     nclx, ncly, nclz, reserved;
+    double      dcel;
+    double      xbcen, ybcen, zbcen;
+    const void *data;    /* phix: float[ncl^3]; phiv: int8_t[ncl^3] */
+} bromoc_grid;
+
+/* Complete ENERGY/DYNAMICS state.  Pointers are read during bromoc_create only. */
+typedef struct bromoc_system {
+    int32_t abi_version;
+    int32_t ntot, nsites, nfix;
+    int32_t capacity;            /* device slots for particles (>= ntot; GCMC growth) */
+    int32_t ntype, nold, nttyp;  /* atnam types, nucleotide types, atnam2 types */
+    int32_t npair;               /* pair types 1..npair */
+    uint32_t flags;              /* BROMOC_Q* */
+    /* per particle */
+    const double  *x, *y, *z;    /* [ntot] */
+    const int32_t *typei;        /* [ntot] signed atnam type; entries < nsites unused */
+    const int32_t *typtyp;       /* [nsites] atnam2 type of each site */
+    const int32_t *namsite;      /* [nsites] BROMOC_SITE_* */
+    const int8_t  *stfree;       /* [nsites] site moves / receives pair forces */
+    /* per type */
+    const int32_t *nwtype;       /* [ntype+1] atnam -> atnam2 */
+    const double  *cg;           /* [ntype+1] charge */
+    const double  *fact1a;       /* [ntype+1] D*dt/kT (simul1.f90:347) */
+    const double  *fact2a;       /* [ntype+1] sqrt(2*D*dt) (simul1.f90:348) */
+    const int32_t *pairidx;      /* [(nttyp+1)^2] nindex(a,b), 0 where undefined */
+    const bromoc_pair *pairs;    /* [npair+1], entry 0 unused */
+    int32_t nknots, reserved0;
+    const double  *ep;           /* [3*nknots] (ep1,ep2,ep3) per knot, efpmod ep(3,:) */
+    double res, ires;            /* table spacing and its inverse (efpmod) */
+    double rth, srpx, srpk, srpy;/* SRPMF globals (grandmod) */
+    /* CG-DNA (nucleotmod) */
+    double cgnuc, epsnuc, scalepairing, fctn, kappa, ikappa, epsolv;
+    int32_t nbond, nangle, ndihe, nstack, nbp, nex, nqq, nsolv;
+    const int32_t *sitebond;  const double *distbond;   /* [2*nbond], [nbond] */
+    const int32_t *siteangle; const double *valangle;   /* [3*nangle] */
+    const int32_t *sitedihe;  const double *valdihe;    /* [4*ndihe] */
+    const int32_t *sitestack; const double *sgstack;    /* [2*nstack] */
+    const int32_t *sitebp;    const double *sgbp;       /* [2*nbp] */
+    const int32_t *siteex;    const double *sgex;       /* [2*nex] */
+    const int32_t *siteqq;                              /* [2*nqq] */
+    const int32_t *siteslv;                             /* [2*nsolv] */
+    /* external grids (gsbpmod) */
+    bromoc_grid phix;            /* static field, float32 */
+    bromoc_grid phiv;            /* repulsion map, int8 */
+    double svdw, thold8, vzmin, vzmax;
+    const double *scal;          /* [nttyp+1] per-type svdw scale (QSVDW) */
+    /* Brownian dynamics (grandmod, simul1.f90:342-349) */
+    double bdmax, fact0n1, fact0n2;
+    double lx2m, lx2p, ly2m, ly2p, lz2m, lz2p;
+    int32_t cntpts, reserved1;
+    const double *zcont;         /* [cntpts] flux-counting planes */
+} bromoc_system;
+
+/* Energy terms as in grandmod/gsbpmod/nucleotmod (energy.mono.f90:46-68,689). */
+typedef struct bromoc_energies {
+    double ener;
+    double eelec, evdw, esrpmf, esrpmfmx, eefpot, eefpotmx, eqqmx, evdwmx;
+    double ebond, eang, edihe, estack, ebp, eex, eqq, esolv;
+    double egsbpa, egsbpb, evdwgd, erfpar, econ, ememb;
+} bromoc_energies;
+
+/* GCMC buffers (grandmod: nbuffer, ibfftyp, avnum, mu, kb, LZmin, LZmax, Qbufferbias,
+ * ntotat; box lx, ly, cx, cy, cz; ikbt). */
+typedef struct bromoc_gcmc {
+    int32_t nbuffer, reserved;
+    const int32_t *ibfftyp;      /* [nbuffer+1] ion type of each buffer */
+    const double  *avnum, *mu, *kb, *lzmin, *lzmax;   /* [nbuffer+1] */
+    const int8_t  *bufferbias;   /* [nbuffer+1] */
+    const int32_t *ntotat;       /* [nbuffer+1] */
+    double lx, ly, cx, cy, cz, ikbt;
+} bromoc_gcmc;
+
+typedef struct bromoc_ctx bromoc_ctx;
+
+/* noise modes of bromoc_step / bromoc_dynamics */
+#define BROMOC_NOISE_INJECTED 0   /* Gaussians uploaded with bromoc_upload_noise */
+#define BROMOC_NOISE_PHILOX   1   /* counter-based Philox4x64-10 + Box-Muller */
+
+/* Create a device context on HIP device `device` from `sys` (copied to HBM). */
+int bromoc_create(const bromoc_system *sys, int device, bromoc_ctx **out);
+int bromoc_destroy(bromoc_ctx *ctx);
+/* Human-readable text of the last error on this context (static storage). */
+const char *bromoc_last_error(const bromoc_ctx *ctx);
+
+/* Positions and types: ntot may change (GCMC); typei entries for sites are ignored. */
+int bromoc_set_positions(bromoc_ctx *ctx, int32_t ntot, const double *x, const double *y,
+                         const double *z, const int32_t *typei);
+int bromoc_get_positions(bromoc_ctx *ctx, double *x, double *y, double *z, int32_t *typei,
+                         int32_t *ntot);
+int bromoc_get_forces(bromoc_ctx *ctx, double *fx, double *fy, double *fz);
+
+/* ENERGY: forces (if want_forces) and energy terms for the current positions. */
+int bromoc_energy(bromoc_ctx *ctx, int32_t want_forces, bromoc_energies *out);
+
+/* Injected Gaussian noise for nsteps steps, layout [step][particle i][xyz] over ntot
+ * particles (entries of non-moving particles are ignored).  Kept in HBM. */
+int bromoc_upload_noise(bromoc_ctx *ctx, int32_t nsteps, const double *noise);
+
+/* nsteps x { ENERGY ; DYNAMICS0(ions) ; DYNAMICS0NUC(sites) }.  With injected noise the
+ * step s uses noise row (first_row + s).  With Philox, the Gaussian for (step, i, c) is
+ * a function of (seed, counter0 + s, i, c) only.  energies (optional) receives the terms
+ * of the last step's ENERGY. */
+int bromoc_step(bromoc_ctx *ctx, int32_t nsteps, int32_t noise_mode, int32_t first_row,
+                uint64_t seed, uint64_t counter0, bromoc_energies *energies);
+
+/* Measurement: nsteps steps as bromoc_step, with a HIP event pair around every kernel
+ * launch on the library's stream; *total_ms is first-to-last, *kernel_ms the mean launch
+ * duration.  No counterpart in the reference (it times runs with date_and_time,
+ * cputime.f90:19-39). */
+int bromoc_time_steps(bromoc_ctx *ctx, int32_t nsteps, int32_t noise_mode, int32_t first_row,
+                      uint64_t seed, uint64_t counter0, double *total_ms, double *kernel_ms);
+
+/* One DYNAMICS pass with the current forces (as left by bromoc_energy). */
+int bromoc_dynamics(bromoc_ctx *ctx, int32_t noise_mode, int32_t row, uint64_t seed,
+                    uint64_t counter);
+
+/* Flux counters nforward/nbackward, [ntype-nold][cntpts] row-major (countions.f90). */
+int bromoc_get_flux(bromoc_ctx *ctx, int32_t *nforward, int32_t *nbackward);
+/* Per-atnam2-type wall warnings warn(1:nttyp) (vdwgd1trln.f90:143-153), [nttyp+1]. */
+int bromoc_get_warn(bromoc_ctx *ctx, int32_t *warn);
+
+/* INTERACT: energy of particle j (0-based; j == ntot means an insertion trial) at
+ * (xj,yj,zj) with atnam type jtype against the rest of the system. */
+int bromoc_interact(bromoc_ctx *ctx, double xj, double yj, double zj, int32_t jtype,
+                    int32_t j, int32_t alert, double *dener);
+
+/* GCMC buffers for bromoc_grand. */
+int bromoc_set_gcmc(bromoc_ctx *ctx, const bromoc_gcmc *g);
+/* Buffer membership ibuffer(1:ntot) of grandmod (0 = none), 0-based particle order. */
+int bromoc_set_ibuffer(bromoc_ctx *ctx, const int32_t *ibuffer);
+int bromoc_get_ibuffer(bromoc_ctx *ctx, int32_t *ibuffer);
+/* GRAND(ngcmc): uniforms[] supplies every rndm()/rndm1() draw in call order; *used
+ * returns how many were consumed.  nat[nbuffer+1] is read and updated, prob is
+ * [nbuffer][capacity+1] (prob(ib,nat) of grand.f90:114-116), ener is updated. */
+int bromoc_grand(bromoc_ctx *ctx, int32_t ngcmc, const double *uniforms, int64_t nuniforms,
+                 int64_t *used, int32_t *nat, int32_t *prob, int64_t icycle, double *ener);
+
+/* EFPOT table from (r,U) samples at uniform spacing: writes ep[3*n] and pair fields
+ * dmi, dm2lo, dm2hi, sc1..sc3 (host-side, splinepot.f90:19-96). */
+int bromoc_build_table(int32_t n, const double *r, const double *u, int32_t chr, double fct,
+                       double *ep, bromoc_pair *pair);
+
+/* n raw Philox4x64-10 blocks computed on the device (the generator behind
+ * BROMOC_NOISE_PHILOX): out[4k..4k+3] = philox(ctr[4k..4k+3], key[2k..2k+1]).  No
+ * counterpart in the reference (its noise is libgfortran rand()). */
+int bromoc_philox4x64(int32_t n, const uint64_t *ctr, const uint64_t *key, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BROMOC_GPU_H */

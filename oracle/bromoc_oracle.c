@@ -1,0 +1,1382 @@
+/*
+ * bromoc_oracle.c -- CPU restatement of the BROMOC cg-gcmc-bd force / BD step path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the product path
+ * (bromoc_amd/), which fails loudly when its HIP library is missing.
+ *
+ * Every routine follows one reference routine loop-for-loop and in the same expression
+ * order (file:line cited at each function).  Built with -O2 -ffp-contract=off so that each
+ * Fortran expression rounds as written.  Indices are 0-based (Fortran index - 1).
+ *
+ * Parity pins (what checks each function; see tests/test_oracle.py):
+ *   getyd/gety, squarespline/splinepot, ENERGY, STATICF1, VDWGD1TRLN, DYNAMICS0/0NUC,
+ *   FIXCOOR, COUNTIONS, INTERACT, STATICF0, VDWGD0TRLN, GRAND: against the reference
+ *   Fortran compiled by oracle/Makefile (oracle/_ref), plus closed forms and invariants
+ *   (Newton's third law, F = -grad E by central differences, ENERGY = 1/2 sum INTERACT,
+ *   exact trilinear interpolation of linear fields, closed-form BD step, acceptance rule).
+ *   philox4x64: bit-exact against numpy.random.Philox (a published library routine).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/bromoc_gpu.h"
+
+#define EXPORT __attribute__((visibility("default")))
+
+/* constamod.f90:21-36 (binary64 under -fdefault-real-8) */
+static const double PI = 3.14159265358979323846264338327950288419716939937510;
+static double celec_value(void) {
+    const double kcal = 4184.0, avogadro = 6.0221412927E23;
+    const double kcalmol = kcal / avogadro;
+    const double coulomb = 1.60217656535E-19, epermit = 8.854187817620E-12, angstrom = 1E-10;
+    return coulomb * coulomb / (4.0 * PI * epermit * kcalmol * angstrom);
+}
+static const double RSMALL = 1E-3;
+
+typedef struct orc_ctx {
+    bromoc_system s;          /* scalars; pointer members re-pointed to owned copies */
+    int cap;
+    double *x, *y, *z, *fx, *fy, *fz;
+    int32_t *typei;
+    int32_t *typtyp, *namsite, *nwtype, *pairidx;
+    int8_t *stfree;
+    double *cg, *fact1a, *fact2a, *ep, *scal, *zcont;
+    bromoc_pair *pairs;
+    int32_t *lists[8];
+    double *lvals[6];
+    float *phix;
+    int8_t *phiv;
+    int32_t *warn, *nforward, *nbackward;
+    double *noise;
+    int noise_rows;
+    double celec;
+    /* GCMC */
+    bromoc_gcmc g;
+    int32_t *ibfftyp, *ntotat, *ibuffer;
+    double *avnum, *mu, *kb, *lzmin, *lzmax;
+    int8_t *bufferbias;
+} orc_ctx;
+
+static void *dupmem(const void *p, size_t n) {
+    void *q = calloc(1, n ? n : 1);
+    if (p && n) memcpy(q, p, n);
+    return q;
+}
+
+static double pow3(double a) { double a2 = a * a; return a2 * a; }
+static double pow6(double a) { double a2 = a * a; double a4 = a2 * a2; return a4 * a2; }
+static double pow5(double a) { double a2 = a * a; double a4 = a2 * a2; return a4 * a; }
+/* x**n for a constant integer n, evaluated by binary powering (square-and-multiply from
+ * the low bit), the order in which the Fortran compiler expands integer powers. */
+static double powi(double x, unsigned n) {
+    double r = 0.0, sq = x;
+    int have = 0;
+    while (n) {
+        if (n & 1u) { r = have ? r * sq : sq; have = 1; }
+        n >>= 1;
+        if (n) sq = sq * sq;
+    }
+    return have ? r : 1.0;
+}
+static double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); } /* SIGN(a,b) */
+
+/* nindex.f90:19-35 through the host-built pair table */
+static int pair_of(const orc_ctx *c, int a2, int b2) { return c->pairidx[a2 * (c->s.nttyp + 1) + b2]; }
+
+/* ---------------------------------------------------------------- table lookup */
+/* gety, polynom.f90:172-194 */
+static void gety(const orc_ctx *c, int is, double x2, double *y, double *x) {
+    const bromoc_pair *p = &c->pairs[is];
+    *x = 0.0;
+    if (x2 < p->dm2lo) {
+        double ix2 = 1.0 / x2;
+        *y = p->sc1 * pow6(ix2) + p->sc2;
+    } else if (x2 > p->dm2hi) {
+        double ix2 = 1.0 / x2;
+        if (p->chr) *y = p->sc3 * pow3(ix2) + p->fct * sqrt(ix2);
+        else *y = p->sc3 * pow3(ix2);
+    } else {
+        *x = sqrt(x2);
+        int i = (int)((*x - p->dmi) * c->s.ires) + p->nxi;
+        const double *e = c->ep + 3 * (size_t)i;
+        *y = e[0] + *x * (e[1] + *x * e[2]);
+    }
+}
+
+/* getyd, polynom.f90:197-230 */
+static void getyd(const orc_ctx *c, int is, double x2, double *y, double *yp, double *x) {
+    const bromoc_pair *p = &c->pairs[is];
+    *x = 0.0;
+    if (x2 < p->dm2lo) {
+        double ix2 = 1.0 / x2;
+        double cc = p->sc1 * pow6(ix2);
+        *y = cc + p->sc2;
+        *yp = 12.0 * cc * ix2;
+    } else if (x2 > p->dm2hi) {
+        double ix2 = 1.0 / x2;
+        double cc = p->sc3 * pow3(ix2);
+        if (p->chr) {
+            double f = p->fct * sqrt(ix2);
+            *y = cc + f;
+            *yp = (6.0 * cc + f) * ix2;
+        } else {
+            *y = cc;
+            *yp = 6.0 * cc * ix2;
+        }
+    } else {
+        *x = sqrt(x2);
+        int i = (int)((*x - p->dmi) * c->s.ires) + p->nxi;
+        const double *e = c->ep + 3 * (size_t)i;
+        *y = e[0] + *x * (e[1] + *x * e[2]);
+        *yp = -e[1] / *x - 2.0 * e[2];
+    }
+}
+
+/* -------------------------------------------------------- table construction */
+/* sqfit, polynom.f90:135-169 (p is ep(3,n), column-major: p[3*i+k]) */
+static void sqfit(int n, const double *x, const double *y, double *p, double d1, double d2) {
+#define P(k, i) p[3 * (i) + (k)]
+    int i = 0;
+    double tt = 1.0 / (x[i + 1] - x[i]);
+    P(2, i) = ((y[i + 1] - y[i]) * tt - d1) * tt;
+    P(1, i) = d1 - 2.0 * P(2, i) * x[i];
+    P(0, i) = y[i] - x[i] * (P(1, i) + P(2, i) * x[i]);
+    i = 1;
+    tt = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+    P(2, i) = (tt - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) / (x[i + 1] - x[i - 1]);
+    P(1, i) = tt - P(2, i) * (x[i + 1] + x[i]);
+    P(0, i) = y[i] - x[i] * (P(1, i) + P(2, i) * x[i]);
+    for (i = 2; i <= n - 2; i++) {
+        tt = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+        P(2, i) = (tt - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) / (x[i + 1] - x[i - 1]);
+        P(1, i) = tt - P(2, i) * (x[i + 1] + x[i]);
+        P(0, i) = y[i] - x[i] * (P(1, i) + P(2, i) * x[i]);
+        P(2, i - 1) = 0.5 * (P(2, i - 1) + P(2, i));
+        P(1, i - 1) = 0.5 * (P(1, i - 1) + P(1, i));
+        P(0, i - 1) = 0.5 * (P(0, i - 1) + P(0, i));
+    }
+    int m = n - 1;
+    tt = 1.0 / (x[m] - x[m - 1]);
+    P(2, m) = (d2 - (y[m] - y[m - 1]) * tt) * tt;
+    P(1, m) = d2 - 2.0 * P(2, m) * x[m];
+    P(0, m) = y[m] - x[m] * (P(1, m) + P(2, m) * x[m]);
+    P(2, m - 1) = 0.5 * (P(2, m - 1) + P(2, m));
+    P(1, m - 1) = 0.5 * (P(1, m - 1) + P(1, m));
+    P(0, m - 1) = 0.5 * (P(0, m - 1) + P(0, m));
+}
+
+/* fixder, polynom.f90:121-132 */
+static void fixder(int n, const double *x, const double *p, double *yp) {
+    yp[0] = P(1, 0) + 2.0 * P(2, 0) * x[0];
+    for (int i = 1; i <= n - 2; i++)
+        yp[i] = 0.5 * (P(1, i) + 2.0 * P(2, i) * x[i] + P(1, i - 1) + 2.0 * P(2, i - 1) * x[i]);
+    int i = n - 1;
+    yp[i] = P(1, i) + 2.0 * P(2, i) * x[i];
+}
+
+/* linipol, polynom.f90:67-88 (pp is pp(2,n): pp[2*i+k]) */
+static void linipol(int n, const double *x, const double *y, double *pp, double dd) {
+    for (int i = 0; i <= n - 2; i++) {
+        double tt = 1.0 / (x[i + 1] - x[i]);
+        pp[2 * i + 1] = (y[i + 1] - y[i]) * tt;
+        pp[2 * i + 0] = (y[i] * x[i + 1] - y[i + 1] * x[i]) * tt;
+    }
+    pp[2 * (n - 1) + 1] = dd;
+    pp[2 * (n - 1) + 0] = y[n - 1] - pp[2 * (n - 1) + 1] * x[n - 1];
+}
+
+/* intlin, polynom.f90:51-64 */
+static void intlin(int n, const double *x, const double *pp, double *p, double y0) {
+    P(2, 0) = pp[1] * 0.5;
+    P(1, 0) = pp[0];
+    P(0, 0) = y0 - x[0] * (P(1, 0) + P(2, 0) * x[0]);
+    for (int i = 1; i < n; i++) {
+        P(2, i) = pp[2 * i + 1] * 0.5;
+        P(1, i) = pp[2 * i + 0];
+        P(0, i) = P(0, i - 1) + x[i] * (P(1, i - 1) - P(1, i) + x[i] * (P(2, i - 1) - P(2, i)));
+    }
+}
+
+/* fixdy, polynom.f90:91-118 */
+static void fixdy(int n, const double *x, double *p, double yn) {
+    int i = n - 1;
+    double ynn = P(0, i) + x[i] * (P(1, i) + x[i] * P(2, i));
+    double b = 2.0 * (yn - ynn) / (x[n - 1] - x[0]);
+    double c = -b * x[0] * 0.5;
+    double h = (x[1] - x[0]) / (double)(n - 1);
+    for (i = 0; i <= n - 2; i++) {
+        P(0, i) = P(0, i) + c;
+        P(1, i) = P(1, i) + b;
+        double a = -b * 0.5 / (x[i] + h * (double)i);
+        P(2, i) = P(2, i) + a;
+    }
+    i = n - 1;
+    P(0, i) = P(0, i) + c;
+    P(1, i) = P(1, i) + b;
+    double a = -b * 0.5 / x[i];
+    P(2, i) = P(2, i) + a;
+}
+
+/* squarespline, polynom.f90:20-48 */
+static void squarespline(int n, const double *x, const double *y, double *p, double d1, double d2,
+                         double dd2) {
+    if (n == 1) {
+        P(2, 0) = 0.0; P(1, 0) = 0.0; P(0, 0) = y[0];
+    } else if (n == 2) {
+        P(2, 0) = 0.0;
+        P(1, 0) = (y[1] - y[0]) / (x[1] - x[0]);
+        P(0, 0) = y[0] - P(1, 0) * x[0];
+        P(2, 1) = P(2, 0); P(1, 1) = P(1, 0); P(0, 1) = P(0, 0);
+    } else {
+        double y0 = y[0], yn = y[n - 1];
+        double *yp = (double *)malloc(sizeof(double) * n);
+        double *pp = (double *)malloc(sizeof(double) * 2 * n);
+        memcpy(yp, y, sizeof(double) * n);
+        sqfit(n, x, yp, p, d1, d2);
+        fixder(n, x, p, yp);
+        linipol(n, x, yp, pp, dd2);
+        intlin(n, x, pp, p, y0);
+        fixdy(n, x, p, yn);
+        free(yp);
+        free(pp);
+    }
+}
+#undef P
+
+/* splinepot, splinepot.f90:19-96 */
+EXPORT int orc_build_table(int32_t n, const double *xx, const double *yy, int32_t chr, double fct,
+                           double *ep, bromoc_pair *pair) {
+    if (n < 2 || !xx || !yy || !ep || !pair) return BROMOC_E_ARG;
+    if (xx[n - 1] == 0.0 || xx[0] == 0.0) return BROMOC_E_ARG;
+    pair->dmi = xx[0];
+    pair->dm2lo = xx[0] * xx[0];
+    pair->dm2hi = xx[n - 1] * xx[n - 1];
+    double kf = chr ? fct : 0.0;
+    double xn = xx[n - 1];
+    pair->sc3 = (xn * yy[n - 1] - kf) * powi(xn, 5);
+    double d2 = -kf / powi(xn, 2) - 6.0 * pair->sc3 / powi(xn, 7);
+    double dd2 = 2.0 * kf / powi(xn, 3) + 42.0 * pair->sc3 / powi(xn, 8);
+    double x1_12 = powi(xx[0], 12), x2_12 = powi(xx[1], 12);
+    double dd1 = 1.0 / (x1_12 - x2_12);
+    pair->sc1 = (yy[1] - yy[0]) * x2_12 * x1_12 * dd1;
+    pair->sc2 = (yy[0] * x1_12 - yy[1] * x2_12) * dd1;
+    double d1 = -12.0 * pair->sc1 / powi(xx[0], 13);
+    squarespline(n, xx, yy, ep, d1, d2, dd2);
+    pair->nknots = n;
+    pair->efpot = 1;
+    pair->chr = chr;
+    pair->fct = fct;
+    return BROMOC_OK;
+}
+
+/* ------------------------------------------------------------ context */
+EXPORT void orc_destroy(orc_ctx *c) {
+    if (!c) return;
+    void *ps[] = {c->x, c->y, c->z, c->fx, c->fy, c->fz, c->typei, c->typtyp, c->namsite,
+                  c->nwtype, c->pairidx, c->stfree, c->cg, c->fact1a, c->fact2a, c->ep,
+                  c->scal, c->zcont, c->pairs, c->phix, c->phiv, c->warn, c->nforward,
+                  c->nbackward, c->noise, c->ibfftyp, c->ntotat, c->ibuffer, c->avnum, c->mu,
+                  c->kb, c->lzmin, c->lzmax, c->bufferbias};
+    for (size_t k = 0; k < sizeof(ps) / sizeof(ps[0]); k++) free(ps[k]);
+    for (int k = 0; k < 8; k++) free(c->lists[k]);
+    for (int k = 0; k < 6; k++) free(c->lvals[k]);
+    free(c);
+}
+
+EXPORT orc_ctx *orc_create(const bromoc_system *s) {
+    if (!s || s->ntot < 0 || s->capacity < s->ntot) return NULL;
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    c->s = *s;
+    int cap = s->capacity;
+    c->cap = cap;
+    size_t nd = sizeof(double) * (size_t)cap;
+    c->x = (double *)calloc(cap + 1, sizeof(double));
+    c->y = (double *)calloc(cap + 1, sizeof(double));
+    c->z = (double *)calloc(cap + 1, sizeof(double));
+    memcpy(c->x, s->x, sizeof(double) * s->ntot);
+    memcpy(c->y, s->y, sizeof(double) * s->ntot);
+    memcpy(c->z, s->z, sizeof(double) * s->ntot);
+    c->fx = (double *)calloc(cap + 1, sizeof(double));
+    c->fy = (double *)calloc(cap + 1, sizeof(double));
+    c->fz = (double *)calloc(cap + 1, sizeof(double));
+    (void)nd;
+    c->typei = (int32_t *)calloc(cap + 1, sizeof(int32_t));
+    if (s->typei) memcpy(c->typei, s->typei, sizeof(int32_t) * s->ntot);
+    int ns = s->nsites;
+    c->typtyp = (int32_t *)dupmem(s->typtyp, sizeof(int32_t) * ns);
+    c->namsite = (int32_t *)dupmem(s->namsite, sizeof(int32_t) * ns);
+    c->stfree = (int8_t *)dupmem(s->stfree, ns);
+    c->nwtype = (int32_t *)dupmem(s->nwtype, sizeof(int32_t) * (s->ntype + 1));
+    c->cg = (double *)dupmem(s->cg, sizeof(double) * (s->ntype + 1));
+    c->fact1a = (double *)dupmem(s->fact1a, sizeof(double) * (s->ntype + 1));
+    c->fact2a = (double *)dupmem(s->fact2a, sizeof(double) * (s->ntype + 1));
+    c->pairidx = (int32_t *)dupmem(s->pairidx, sizeof(int32_t) * (s->nttyp + 1) * (s->nttyp + 1));
+    c->pairs = (bromoc_pair *)dupmem(s->pairs, sizeof(bromoc_pair) * (s->npair + 1));
+    c->ep = (double *)dupmem(s->ep, sizeof(double) * 3 * (size_t)s->nknots);
+    c->scal = (double *)dupmem(s->scal, sizeof(double) * (s->nttyp + 1));
+    c->zcont = (double *)dupmem(s->zcont, sizeof(double) * (s->cntpts > 0 ? s->cntpts : 0));
+    int cnt[8] = {2 * s->nbond, 3 * s->nangle, 4 * s->ndihe, 2 * s->nstack,
+                  2 * s->nbp, 2 * s->nex, 2 * s->nqq, 2 * s->nsolv};
+    const int32_t *lp[8] = {s->sitebond, s->siteangle, s->sitedihe, s->sitestack,
+                            s->sitebp, s->siteex, s->siteqq, s->siteslv};
+    for (int k = 0; k < 8; k++) c->lists[k] = (int32_t *)dupmem(lp[k], sizeof(int32_t) * cnt[k]);
+    int vcnt[6] = {s->nbond, s->nangle, s->ndihe, s->nstack, s->nbp, s->nex};
+    const double *vp[6] = {s->distbond, s->valangle, s->valdihe, s->sgstack, s->sgbp, s->sgex};
+    for (int k = 0; k < 6; k++) c->lvals[k] = (double *)dupmem(vp[k], sizeof(double) * vcnt[k]);
+    if (s->flags & BROMOC_QPHIX) {
+        size_t n = (size_t)s->phix.nclx * s->phix.ncly * s->phix.nclz;
+        c->phix = (float *)dupmem(s->phix.data, sizeof(float) * n);
+    }
+    if (s->flags & BROMOC_QPHIV) {
+        size_t n = (size_t)s->phiv.nclx * s->phiv.ncly * s->phiv.nclz;
+        c->phiv = (int8_t *)dupmem(s->phiv.data, n);
+    }
+    c->warn = (int32_t *)calloc(s->nttyp + 2, sizeof(int32_t));
+    int nt = s->ntype - s->nold;
+    int np = s->cntpts > 0 ? s->cntpts : 1;
+    c->nforward = (int32_t *)calloc((size_t)(nt > 0 ? nt : 1) * np, sizeof(int32_t));
+    c->nbackward = (int32_t *)calloc((size_t)(nt > 0 ? nt : 1) * np, sizeof(int32_t));
+    c->celec = celec_value();
+    return c;
+}
+
+EXPORT int orc_set_positions(orc_ctx *c, int32_t ntot, const double *x, const double *y,
+                             const double *z, const int32_t *typei) {
+    if (ntot > c->cap || ntot < c->s.nsites + c->s.nfix) return BROMOC_E_ARG;
+    c->s.ntot = ntot;
+    memcpy(c->x, x, sizeof(double) * ntot);
+    memcpy(c->y, y, sizeof(double) * ntot);
+    memcpy(c->z, z, sizeof(double) * ntot);
+    if (typei) memcpy(c->typei, typei, sizeof(int32_t) * ntot);
+    return BROMOC_OK;
+}
+
+EXPORT int orc_get_positions(orc_ctx *c, double *x, double *y, double *z, int32_t *typei,
+                             int32_t *ntot) {
+    int n = c->s.ntot;
+    if (x) memcpy(x, c->x, sizeof(double) * n);
+    if (y) memcpy(y, c->y, sizeof(double) * n);
+    if (z) memcpy(z, c->z, sizeof(double) * n);
+    if (typei) memcpy(typei, c->typei, sizeof(int32_t) * n);
+    if (ntot) *ntot = n;
+    return BROMOC_OK;
+}
+
+EXPORT int orc_get_forces(orc_ctx *c, double *fx, double *fy, double *fz) {
+    int n = c->s.ntot;
+    memcpy(fx, c->fx, sizeof(double) * n);
+    memcpy(fy, c->fy, sizeof(double) * n);
+    memcpy(fz, c->fz, sizeof(double) * n);
+    return BROMOC_OK;
+}
+
+EXPORT int orc_get_flux(orc_ctx *c, int32_t *nf, int32_t *nb) {
+    int nt = c->s.ntype - c->s.nold, np = c->s.cntpts;
+    if (nt <= 0 || np <= 0) return BROMOC_OK;
+    memcpy(nf, c->nforward, sizeof(int32_t) * nt * np);
+    memcpy(nb, c->nbackward, sizeof(int32_t) * nt * np);
+    return BROMOC_OK;
+}
+
+EXPORT int orc_get_warn(orc_ctx *c, int32_t *warn) {
+    memcpy(warn, c->warn, sizeof(int32_t) * (c->s.nttyp + 1));
+    return BROMOC_OK;
+}
+
+static uint32_t supported_mask(void) {
+    return BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BROMOC_QNOBOND | BROMOC_QPHIX |
+           BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV | BROMOC_QDEBY | BROMOC_QCOUNTION |
+           BROMOC_QDNAFREE;
+}
+
+EXPORT int orc_check_flags(uint32_t flags) {
+    return (flags & ~supported_mask()) ? BROMOC_E_UNSUPPORTED : BROMOC_OK;
+}
+
+/* ------------------------------------------------------------- grids */
+/* STATICF1, staticf1.f90:19-118 */
+static void staticf1(orc_ctx *c, int qforces, double *egsbpa) {
+    const bromoc_system *s = &c->s;
+    const bromoc_grid *g = &s->phix;
+    const double one = 1.0, celec = c->celec;
+    int ncyz = g->ncly * g->nclz;
+    double dcel = g->dcel, idcel = 1.0 / dcel;
+    double tranx = 0.5 * (g->nclx - 1) * dcel, trany = 0.5 * (g->ncly - 1) * dcel,
+           tranz = 0.5 * (g->nclz - 1) * dcel;
+    *egsbpa = 0.0;
+    for (int i = 0; i < s->ntot; i++) {
+        if (!(i < s->nsites || i >= s->nsites + s->nfix)) continue;
+        int ok = 0;
+        double chi = 0.0;
+        if (i < s->nsites) {
+            if (c->namsite[i] == BROMOC_SITE_P) { chi = s->cgnuc; ok = 1; }
+        } else {
+            chi = c->cg[abs(c->typei[i])];
+            ok = 1;
+        }
+        if (ok)
+            ok = c->x[i] <= g->xbcen + tranx && c->x[i] >= g->xbcen - tranx &&
+                 c->y[i] <= g->ybcen + trany && c->y[i] >= g->ybcen - trany &&
+                 c->z[i] <= g->zbcen + tranz && c->z[i] >= g->zbcen - tranz;
+        if (!ok) continue;
+        double rxnfx = 0.0, rxnfy = 0.0, rxnfz = 0.0;
+        double xi = c->x[i] + tranx - g->xbcen;
+        double yi = c->y[i] + trany - g->ybcen;
+        double zi = c->z[i] + tranz - g->zbcen;
+        int ix = (int)(xi * idcel), iy = (int)(yi * idcel), iz = (int)(zi * idcel);
+        if (ix == g->nclx - 1) ix = g->nclx - 2;
+        if (iy == g->ncly - 1) iy = g->ncly - 2;
+        if (iz == g->nclz - 1) iz = g->nclz - 2;
+        for (int n1 = ix; n1 <= ix + 1; n1++) {
+            double ai = xi - n1 * dcel;
+            double aisign = fsign(one, ai);
+            ai = one - fabs(ai) * idcel;
+            for (int n2 = iy; n2 <= iy + 1; n2++) {
+                double bi = yi - n2 * dcel;
+                double bisign = fsign(one, bi);
+                bi = one - fabs(bi) * idcel;
+                for (int n3 = iz; n3 <= iz + 1; n3++) {
+                    double ci = zi - n3 * dcel;
+                    double cisign = fsign(one, ci);
+                    ci = one - fabs(ci) * idcel;
+                    double fi = ai * bi * ci;
+                    size_t in3 = (size_t)n1 * ncyz + (size_t)n2 * g->nclz + n3;
+                    double ph = (double)c->phix[in3];
+                    if (qforces) {
+                        double prefac = ph * celec * chi * idcel;
+                        if (ai < (1.0 - RSMALL) && ai > RSMALL) rxnfx = rxnfx + aisign * bi * ci * prefac;
+                        if (bi < (1.0 - RSMALL) && bi > RSMALL) rxnfy = rxnfy + bisign * ai * ci * prefac;
+                        if (ci < (1.0 - RSMALL) && ci > RSMALL) rxnfz = rxnfz + cisign * ai * bi * prefac;
+                    }
+                    *egsbpa = *egsbpa + (fi * chi * ph * celec);
+                }
+            }
+        }
+        if (qforces) {
+            c->fx[i] = c->fx[i] + rxnfx;
+            c->fy[i] = c->fy[i] + rxnfy;
+            c->fz[i] = c->fz[i] + rxnfz;
+        }
+    }
+}
+
+/* VDWGD1TRLN, vdwgd1trln.f90:19-161 (single map, Qnmcden off) */
+static void vdwgd1trln(orc_ctx *c, int qforces, double *evdwgd) {
+    const bromoc_system *s = &c->s;
+    const bromoc_grid *g = &s->phiv;
+    const double one = 1.0;
+    int ncyz = g->ncly * g->nclz;
+    double dcel = g->dcel, idcel = 1.0 / dcel;
+    double tranx = 0.5 * (g->nclx - 1) * dcel, trany = 0.5 * (g->ncly - 1) * dcel,
+           tranz = 0.5 * (g->nclz - 1) * dcel;
+    double esvdw = s->svdw;
+    int qsvdw = (s->flags & BROMOC_QSVDW) != 0;
+    *evdwgd = 0.0;
+    for (int i = 0; i < s->ntot; i++) {
+        if (!(i < s->nsites || i >= s->nsites + s->nfix)) continue;
+        int ok = c->x[i] <= g->xbcen + tranx && c->x[i] >= g->xbcen - tranx &&
+                 c->y[i] <= g->ybcen + trany && c->y[i] >= g->ybcen - trany &&
+                 c->z[i] <= s->vzmax && c->z[i] >= s->vzmin;
+        if (!ok) continue;
+        double xi = c->x[i] + tranx - g->xbcen;
+        double yi = c->y[i] + trany - g->ybcen;
+        double zi = c->z[i] + tranz - g->zbcen;
+        int itype2 = (i < s->nsites) ? c->typtyp[i] : c->nwtype[abs(c->typei[i])];
+        if (qsvdw) esvdw = s->svdw * c->scal[itype2];
+        double vdwfx = 0.0, vdwfy = 0.0, vdwfz = 0.0;
+        int ix = (int)(xi * idcel), iy = (int)(yi * idcel), iz = (int)(zi * idcel);
+        if (ix == g->nclx - 1) ix = g->nclx - 2;
+        if (iy == g->ncly - 1) iy = g->ncly - 2;
+        if (iz == g->nclz - 1) iz = g->nclz - 2;
+        double phisum = 0.0;
+        for (int n1 = ix; n1 <= ix + 1; n1++) {
+            double ai = xi - n1 * dcel;
+            double aisign = fsign(one, ai);
+            ai = 1.0 - fabs(ai) * idcel;
+            for (int n2 = iy; n2 <= iy + 1; n2++) {
+                double bi = yi - n2 * dcel;
+                double bisign = fsign(one, bi);
+                bi = 1.0 - fabs(bi) * idcel;
+                for (int n3 = iz; n3 <= iz + 1; n3++) {
+                    double ci = zi - n3 * dcel;
+                    double cisign = fsign(one, ci);
+                    ci = 1.0 - fabs(ci) * idcel;
+                    double fi = ai * bi * ci;
+                    size_t in3 = (size_t)n1 * ncyz + (size_t)n2 * g->nclz + n3;
+                    double phis = (double)c->phiv[in3];
+                    phisum = phisum + phis;
+                    if (qforces) {
+                        double prefac = phis * esvdw * idcel;
+                        if (ai < (1.0 - RSMALL) && ai > RSMALL) vdwfx = vdwfx + aisign * bi * ci * prefac;
+                        if (bi < (1.0 - RSMALL) && bi > RSMALL) vdwfy = vdwfy + bisign * ai * ci * prefac;
+                        if (ci < (1.0 - RSMALL) && ci > RSMALL) vdwfz = vdwfz + cisign * ai * bi * prefac;
+                    }
+                    *evdwgd = *evdwgd + fi * esvdw * phis;
+                }
+            }
+        }
+        if (phisum >= s->thold8) c->warn[itype2] = c->warn[itype2] + 1;
+        if (qforces) {
+            c->fx[i] = c->fx[i] + vdwfx;
+            c->fy[i] = c->fy[i] + vdwfy;
+            c->fz[i] = c->fz[i] + vdwfz;
+        }
+    }
+}
+
+/* ------------------------------------------------------------ ENERGY */
+/* analytic pair branch, energy.mono.f90:135-178 (energies into e[0..2], returns de) */
+static double pair_analytic(const orc_ctx *c, int is, double dist2, int qforces, double *eelec,
+                            double *evdw, double *esrpmf) {
+    const bromoc_pair *p = &c->pairs[is];
+    const bromoc_system *s = &c->s;
+    double idist2 = 1.0 / dist2, idist = 0.0, cofo = 0.0, dist6 = 0.0, dist12 = 0.0, dist = 0.0;
+    double esrpmf1 = 0.0, esrpmf2 = 0.0, esrpmf3 = 0.0, fdf = 0.0, fdv = 0.0;
+    if (p->chr || p->srpmf) idist = sqrt(idist2);
+    if (p->chr) {
+        cofo = p->fct * idist;
+        *eelec = *eelec + cofo;
+    }
+    if (p->lj) {
+        dist6 = pow3(p->sgp2 * idist2);
+        dist12 = dist6 * dist6;
+        *evdw = *evdw + p->epp4 * (dist12 - dist6);
+    }
+    if (p->srpmf && dist2 <= s->rth) {
+        dist = 1.0 / idist;
+        esrpmf1 = exp((p->c1 - dist) * p->c2);
+        esrpmf2 = cos(p->c3 * PI * (p->c1 - dist));
+        esrpmf3 = pow6(p->c1 * idist);
+        double esrpmf0 = p->c0 * esrpmf1 * esrpmf2 + p->c4 * esrpmf3;
+        if (dist >= s->srpx) {
+            fdf = exp(-s->srpk * (dist - s->srpx)) - s->srpy;
+            fdv = esrpmf0;
+            esrpmf0 = fdv * fdf;
+        }
+        *esrpmf = *esrpmf + esrpmf0;
+    }
+    double de = 0.0;
+    if (qforces) {
+        if (p->chr) de = cofo * idist2;
+        if (p->lj) de = de + p->epp4 * (2.0 * dist12 - dist6) * 6.0 * idist2;
+        if (p->srpmf && dist2 <= s->rth) {
+            double dc = (-p->c2 * esrpmf2 + p->c3 * PI * sin(p->c3 * PI * (p->c1 - dist))) * p->c0 * esrpmf1 -
+                        6.0 * p->c4 * esrpmf3 * idist;
+            if (dist >= s->srpx) dc = dc * fdf - fdv * s->srpk * (fdf + s->srpy);
+            de = de - dc * idist;
+        }
+    }
+    return de;
+}
+
+static void cross(const double *a, const double *b, double *r) {
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* The `ok` guard of every DNA term (energy.mono.f90:216 etc.) parses as
+ * (Qforces .and. stfree(a)) .or. stfree(b): with Qforces off the reference still adds
+ * pair forces into fx, which it never zeroed in that call.  Forces are only defined after
+ * a Qforces call (simul1.f90:318-322,363), so here a no-forces call leaves them alone. */
+
+static void add_f(orc_ctx *c, int i, double sgn, const double *f) {
+    c->fx[i] = c->fx[i] + sgn * f[0];
+    c->fy[i] = c->fy[i] + sgn * f[1];
+    c->fz[i] = c->fz[i] + sgn * f[2];
+}
+
+EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
+    const bromoc_system *s = &c->s;
+    if (orc_check_flags(s->flags)) return BROMOC_E_UNSUPPORTED;
+    bromoc_energies E;
+    memset(&E, 0, sizeof(E));
+    int ns = s->nsites, nf = s->nfix, nt = s->ntot;
+    uint32_t fl = s->flags;
+    if (qforces) {
+        for (int i = 0; i < nt; i++) c->fx[i] = c->fy[i] = c->fz[i] = 0.0;
+    }
+    E.ener = E.ememb;
+    if (fl & BROMOC_QPHIX) {
+        staticf1(c, qforces, &E.egsbpa);
+        E.ener = E.ener + E.egsbpa;
+    }
+    if (fl & BROMOC_QPHIV) {
+        vdwgd1trln(c, qforces, &E.evdwgd);
+        E.ener = E.ener + E.evdwgd;
+    }
+    /* ion-ion, energy.mono.f90:117-194 */
+    if ((fl & BROMOC_QPAR) && (fl & BROMOC_QNONBOND)) {
+        for (int j = ns + nf; j < nt; j++) {
+            int jtype2 = c->nwtype[abs(c->typei[j])];
+            for (int i = ns; i < j; i++) {
+                int itype2 = c->nwtype[abs(c->typei[i])];
+                int is = pair_of(c, itype2, jtype2);
+                double dx = c->x[i] - c->x[j], dy = c->y[i] - c->y[j], dz = c->z[i] - c->z[j];
+                double dist2 = (dx * dx + dy * dy + dz * dz);
+                double de = 0.0, eefp, dist;
+                if (c->pairs[is].efpot) {
+                    if (qforces) getyd(c, is, dist2, &eefp, &de, &dist);
+                    else gety(c, is, dist2, &eefp, &dist);
+                    E.eefpot = E.eefpot + eefp;
+                } else {
+                    de = pair_analytic(c, is, dist2, qforces, &E.eelec, &E.evdw, &E.esrpmf);
+                }
+                if (qforces && de != 0.0) {
+                    if (i >= ns + nf) {
+                        c->fx[i] = c->fx[i] + de * (c->x[i] - c->x[j]);
+                        c->fy[i] = c->fy[i] + de * (c->y[i] - c->y[j]);
+                        c->fz[i] = c->fz[i] + de * (c->z[i] - c->z[j]);
+                    }
+                    c->fx[j] = c->fx[j] - de * (c->x[i] - c->x[j]);
+                    c->fy[j] = c->fy[j] - de * (c->y[i] - c->y[j]);
+                    c->fz[j] = c->fz[j] - de * (c->z[i] - c->z[j]);
+                }
+            }
+        }
+    }
+    /* CG-DNA, energy.mono.f90:196-508 */
+    if (fl & BROMOC_QNUCL) {
+        const int8_t *sf = c->stfree;
+        double eps = s->epsnuc;
+        if (fl & BROMOC_QNOBOND) {
+            const int32_t *L = c->lists[0];
+            for (int t = 0; t < s->nbond; t++) {
+                int a = L[t], b = L[s->nbond + t];
+                double v1[3] = {c->x[b] - c->x[a], c->y[b] - c->y[a], c->z[b] - c->z[a]};
+                double dist = sqrt(dot3(v1, v1));
+                double vard = dist - c->lvals[0][t];
+                double vard2 = vard * vard;
+                E.ebond = E.ebond + eps * vard2 * (1.0 + 100.0 * vard2);
+                if (qforces && (sf[a] || sf[b])) {
+                    double de = eps * vard * (2.0 + 400.0 * vard2) / dist;
+                    double f1[3] = {de * v1[0], de * v1[1], de * v1[2]};
+                    if (sf[a]) add_f(c, a, 1.0, f1);
+                    if (sf[b]) add_f(c, b, -1.0, f1);
+                }
+            }
+            L = c->lists[1];
+            for (int t = 0; t < s->nangle; t++) {
+                int a = L[t], b = L[s->nangle + t], d = L[2 * s->nangle + t];
+                double v1[3] = {c->x[a] - c->x[b], c->y[a] - c->y[b], c->z[a] - c->z[b]};
+                double v2[3] = {c->x[d] - c->x[b], c->y[d] - c->y[b], c->z[d] - c->z[b]};
+                double m1 = dot3(v1, v1), m2 = dot3(v2, v2), m3 = dot3(v1, v2);
+                double modval = 1.0 / sqrt(m1 * m2);
+                double ang = acos(m3 * modval);
+                double varang = ang - c->lvals[1][t];
+                E.eang = E.eang + 700.0 * eps * (varang * varang);
+                if (qforces) {
+                    double de = 1400.0 * eps * varang * modval / sin(ang);
+                    double f1[3] = {de * v1[0], de * v1[1], de * v1[2]};
+                    double f2[3] = {de * v2[0], de * v2[1], de * v2[2]};
+                    if (sf[a]) add_f(c, a, 1.0, f2);
+                    if (sf[b]) {
+                        double f12[3] = {f1[0] + f2[0], f1[1] + f2[1], f1[2] + f2[2]};
+                        add_f(c, b, -1.0, f12);
+                    }
+                    if (sf[d]) add_f(c, d, 1.0, f1);
+                }
+            }
+            L = c->lists[2];
+            for (int t = 0; t < s->ndihe; t++) {
+                int i1 = L[t], i2 = L[s->ndihe + t], i3 = L[2 * s->ndihe + t], i4 = L[3 * s->ndihe + t];
+                double v1[3] = {c->x[i1] - c->x[i2], c->y[i1] - c->y[i2], c->z[i1] - c->z[i2]};
+                double v2[3] = {c->x[i3] - c->x[i2], c->y[i3] - c->y[i2], c->z[i3] - c->z[i2]};
+                double v3[3] = {c->x[i3] - c->x[i4], c->y[i3] - c->y[i4], c->z[i3] - c->z[i4]};
+                double n1[3], n2[3];
+                cross(v1, v2, n1);
+                cross(v2, v3, n2);
+                double v22 = dot3(v2, v2);
+                double v2v = sqrt(v22);
+                double av = dot3(v1, n2);
+                double bv = dot3(n1, n2);
+                double dihe = atan2(v2v * av, bv);
+                double vardihe = dihe - c->lvals[2][t];
+                E.edihe = E.edihe + 28.0 * eps * (1.0 - cos(vardihe));
+                if (qforces) {
+                    double n1v = dot3(n1, n1), n2v = dot3(n2, n2);
+                    double v12 = dot3(v1, v2), v23 = dot3(v2, v3);
+                    double de = 28.0 * eps * sin(vardihe) * v2v;
+                    double f1[3], f4[3], f2[3], f3[3];
+                    for (int k = 0; k < 3; k++) f1[k] = -de / n1v * n1[k];
+                    for (int k = 0; k < 3; k++) f4[k] = de / n2v * n2[k];
+                    double iv22 = 1.0 / v22;
+                    v12 = v12 * iv22;
+                    v23 = v23 * iv22;
+                    for (int k = 0; k < 3; k++) f2[k] = -f1[k] + v12 * f1[k] - v23 * f4[k];
+                    for (int k = 0; k < 3; k++) f3[k] = -f4[k] - v12 * f1[k] + v23 * f4[k];
+                    if (sf[i1]) add_f(c, i1, 1.0, f1);
+                    if (sf[i2]) add_f(c, i2, 1.0, f2);
+                    if (sf[i3]) add_f(c, i3, 1.0, f3);
+                    if (sf[i4]) add_f(c, i4, 1.0, f4);
+                }
+            }
+        }
+        if (fl & BROMOC_QNONBOND) {
+            const int32_t *L = c->lists[3];
+            for (int t = 0; t < s->nstack; t++) {
+                int a = L[t], b = L[s->nstack + t];
+                double v1[3] = {c->x[b] - c->x[a], c->y[b] - c->y[a], c->z[b] - c->z[a]};
+                double dist2 = dot3(v1, v1);
+                double idist2 = 1.0 / dist2;
+                double sg = c->lvals[3][t];
+                double cte1 = pow3(sg * sg * idist2);
+                E.estack = E.estack + 4.0 * eps * cte1 * (cte1 - 1.0);
+                if (qforces) {
+                    double de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
+                    double f1[3] = {de * v1[0], de * v1[1], de * v1[2]};
+                    if (sf[a]) add_f(c, a, -1.0, f1);
+                    if (sf[b]) add_f(c, b, 1.0, f1);
+                }
+            }
+            L = c->lists[4];
+            for (int t = 0; t < s->nbp; t++) {
+                int a = L[t], b = L[s->nbp + t];
+                double epsln;
+                if (c->namsite[a] == BROMOC_SITE_GB || c->namsite[a] == BROMOC_SITE_CB)
+                    epsln = 2.532 * eps * s->scalepairing;
+                else
+                    epsln = 2.0 * eps * s->scalepairing;
+                double v1[3] = {c->x[b] - c->x[a], c->y[b] - c->y[a], c->z[b] - c->z[a]};
+                double dist2 = dot3(v1, v1);
+                double idist2 = 1.0 / dist2;
+                double sg = c->lvals[4][t];
+                double cte1 = sg * sg * idist2;
+                E.ebp = E.ebp + epsln * pow5(cte1) * (20.0 * cte1 - 24.0);
+                if (qforces) {
+                    double de = 240.0 * epsln * pow5(cte1) * (cte1 - 1.0) * idist2;
+                    double f1[3] = {de * v1[0], de * v1[1], de * v1[2]};
+                    if (sf[a]) add_f(c, a, -1.0, f1);
+                    if (sf[b]) add_f(c, b, 1.0, f1);
+                }
+            }
+            L = c->lists[5];
+            for (int t = 0; t < s->nex; t++) {
+                int a = L[t], b = L[s->nex + t];
+                double v1[3] = {c->x[b] - c->x[a], c->y[b] - c->y[a], c->z[b] - c->z[a]};
+                double dist2 = dot3(v1, v1);
+                double idist2 = 1.0 / dist2;
+                double sgex2 = c->lvals[5][t] * c->lvals[5][t];
+                if (dist2 < sgex2) {
+                    double cte1 = pow3(sgex2 * idist2);
+                    E.eex = E.eex + 4.0 * eps * cte1 * (cte1 - 1.0) + eps;
+                    if (qforces) {
+                        double de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
+                        double f1[3] = {de * v1[0], de * v1[1], de * v1[2]};
+                        if (sf[a]) add_f(c, a, -1.0, f1);
+                        if (sf[b]) add_f(c, b, 1.0, f1);
+                    }
+                }
+            }
+            L = c->lists[6];
+            for (int t = 0; t < s->nqq; t++) {
+                int a = L[t], b = L[s->nqq + t];
+                double v1[3] = {c->x[b] - c->x[a], c->y[b] - c->y[a], c->z[b] - c->z[a]};
+                double dist = sqrt(dot3(v1, v1));
+                double idist = 1.0 / dist;
+                double de;
+                if (fl & BROMOC_QDEBY) {
+                    double idistkd = exp(-dist * s->ikappa);
+                    E.eqq = E.eqq + s->fctn * idist * idistkd;
+                    de = s->fctn * (dist + s->kappa) * pow3(idist) * s->ikappa * idistkd;
+                } else {
+                    E.eqq = E.eqq + s->fctn * idist;
+                    de = s->fctn * pow3(idist);
+                }
+                if (qforces) {
+                    double f1[3] = {de * v1[0], de * v1[1], de * v1[2]};
+                    if (sf[a]) add_f(c, a, -1.0, f1);
+                    if (sf[b]) add_f(c, b, 1.0, f1);
+                }
+            }
+            if (fl & BROMOC_QSOLV) {
+                L = c->lists[7];
+                for (int t = 0; t < s->nsolv; t++) {
+                    int a = L[t], b = L[s->nsolv + t];
+                    double v1[3] = {c->x[b] - c->x[a], c->y[b] - c->y[a], c->z[b] - c->z[a]};
+                    double dist = sqrt(dot3(v1, v1));
+                    double cte1 = exp((13.38 - dist) * 0.1875);
+                    E.esolv = E.esolv + s->epsolv * ((1.0 - cte1) * (1.0 - cte1)) - s->epsolv;
+                    if (qforces) {
+                        double de = 0.375 * s->epsolv * cte1 * (cte1 - 1.0) / dist;
+                        double f1[3] = {de * v1[0], de * v1[1], de * v1[2]};
+                        if (sf[a]) add_f(c, a, -1.0, f1);
+                        if (sf[b]) add_f(c, b, 1.0, f1);
+                    }
+                }
+            }
+        }
+    }
+    /* ion-DNA, energy.mono.f90:583-687 */
+    if ((fl & BROMOC_QPAR) && (fl & BROMOC_QNUCL) && (fl & BROMOC_QNONBOND)) {
+        for (int j = 0; j < ns; j++) {
+            int jtype2 = c->typtyp[j];
+            for (int i = ns; i < nt; i++) {
+                int itype2 = c->nwtype[abs(c->typei[i])];
+                int is = pair_of(c, itype2, jtype2);
+                double dx = c->x[i] - c->x[j], dy = c->y[i] - c->y[j], dz = c->z[i] - c->z[j];
+                double dist2 = dx * dx + dy * dy + dz * dz;
+                int ok2 = qforces && (i >= ns + nf || c->stfree[j]);
+                double de = 0.0, eefp, dist;
+                if (c->pairs[is].efpot) {
+                    if (ok2) getyd(c, is, dist2, &eefp, &de, &dist);
+                    else gety(c, is, dist2, &eefp, &dist);
+                    E.eefpotmx = E.eefpotmx + eefp;
+                } else {
+                    de = pair_analytic(c, is, dist2, ok2, &E.eqqmx, &E.evdwmx, &E.esrpmfmx);
+                }
+                if (ok2 && de != 0.0) {
+                    if (i >= ns + nf) {
+                        c->fx[i] = c->fx[i] + de * (c->x[i] - c->x[j]);
+                        c->fy[i] = c->fy[i] + de * (c->y[i] - c->y[j]);
+                        c->fz[i] = c->fz[i] + de * (c->z[i] - c->z[j]);
+                    }
+                    if (c->stfree[j]) {
+                        c->fx[j] = c->fx[j] - de * (c->x[i] - c->x[j]);
+                        c->fy[j] = c->fy[j] - de * (c->y[i] - c->y[j]);
+                        c->fz[j] = c->fz[j] - de * (c->z[i] - c->z[j]);
+                    }
+                }
+            }
+        }
+    }
+    /* energy.mono.f90:689 */
+    E.ener = E.ener + E.eelec + E.evdw + E.esrpmf + E.esrpmfmx + E.ebond + E.eang + E.edihe +
+             E.estack + E.ebp + E.eex + E.eqq + E.esolv + E.eqqmx + E.evdwmx + E.eefpot +
+             E.eefpotmx + E.econ;
+    if (out) *out = E;
+    return BROMOC_OK;
+}
+
+/* ---------------------------------------------------------- dynamics */
+/* FIXCOOR rectangular branch, fixcoor.f90:48-64 */
+static void fixcoor(const bromoc_system *s, double *xi, double *yi, double *zi) {
+    if (*xi < s->lx2m) *xi = s->lx2m;
+    else if (*xi > s->lx2p) *xi = s->lx2p;
+    if (*yi < s->ly2m) *yi = s->ly2m;
+    else if (*yi > s->ly2p) *yi = s->ly2p;
+    if (*zi < s->lz2m) *zi = s->lz2m;
+    else if (*zi > s->lz2p) *zi = s->lz2p;
+}
+
+/* COUNTIONS, countions.f90:19-32 (it is 0-based ion-type row) */
+static void countions(orc_ctx *c, double zold, double zi, int it) {
+    int np = c->s.cntpts;
+    for (int j = 0; j < np; j++) {
+        if (zold < c->zcont[j] && zi >= c->zcont[j]) c->nforward[it * np + j] += 1;
+        else if (zold >= c->zcont[j] && zi < c->zcont[j]) c->nbackward[it * np + j] += 1;
+    }
+}
+
+/* DYNAMICS0 (dynamics0.mono.f90:31-46) over moving ions, then DYNAMICS0NUC
+ * (dynamics0nuc.f90:31-44) over sites; g = Gaussian for (particle, component) */
+static void dynamics_pass(orc_ctx *c, const double *g) {
+    const bromoc_system *s = &c->s;
+    double bdmax = s->bdmax;
+    if (s->flags & BROMOC_QPAR) {
+        for (int i = s->nsites + s->nfix; i < s->ntot; i++) {
+            int itype = abs(c->typei[i]);
+            double delx = c->fx[i] * c->fact1a[itype];
+            double dely = c->fy[i] * c->fact1a[itype];
+            double delz = c->fz[i] * c->fact1a[itype];
+            if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
+            if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
+            if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
+            c->x[i] = c->x[i] + delx + c->fact2a[itype] * g[3 * i + 0];
+            c->y[i] = c->y[i] + dely + c->fact2a[itype] * g[3 * i + 1];
+            double zold = c->z[i];
+            c->z[i] = c->z[i] + delz + c->fact2a[itype] * g[3 * i + 2];
+            if (s->flags & BROMOC_QCOUNTION) countions(c, zold, c->z[i], itype - s->nold - 1);
+            fixcoor(s, &c->x[i], &c->y[i], &c->z[i]);
+        }
+    }
+    if (s->flags & BROMOC_QNUCL) {
+        for (int i = 0; i < s->nsites; i++) {
+            if (!c->stfree[i]) continue;
+            double delx = c->fx[i] * s->fact0n1;
+            double dely = c->fy[i] * s->fact0n1;
+            double delz = c->fz[i] * s->fact0n1;
+            if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
+            if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
+            if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
+            c->x[i] = c->x[i] + delx + s->fact0n2 * g[3 * i + 0];
+            c->y[i] = c->y[i] + dely + s->fact0n2 * g[3 * i + 1];
+            c->z[i] = c->z[i] + delz + s->fact0n2 * g[3 * i + 2];
+            if (!(s->flags & BROMOC_QDNAFREE)) fixcoor(s, &c->x[i], &c->y[i], &c->z[i]);
+        }
+    }
+}
+
+/* ---------------------------------------------------------- Philox */
+/* Philox4x64-10 (Salmon et al., SC'11), the generator of numpy.random.Philox. */
+static inline void mulhilo64(uint64_t a, uint64_t b, uint64_t *hi, uint64_t *lo) {
+    __uint128_t p = (__uint128_t)a * b;
+    *hi = (uint64_t)(p >> 64);
+    *lo = (uint64_t)p;
+}
+EXPORT void orc_philox4x64(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t out[4]) {
+    uint64_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint64_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t hi0, lo0, hi1, lo1;
+        mulhilo64(0xD2E7470EE14C6C93ull, c0, &hi0, &lo0);
+        mulhilo64(0xCA5A826395121157ull, c2, &hi1, &lo1);
+        uint64_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B97F4A7C15ull;
+        k1 += 0xBB67AE8584CAA73Bull;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Gaussians of particle i at step counter `step`: Box-Muller on two 53-bit uniforms
+ * per pair of outputs, u1 in (0,1], u2 in [0,1).  Counter (step, i, stream, 0),
+ * key (seed, 0). */
+EXPORT void orc_philox_gauss3(uint64_t seed, uint64_t step, uint64_t stream, int64_t i, double g[3]) {
+    uint64_t ctr[4] = {step, (uint64_t)i, stream, 0}, key[2] = {seed, 0}, r[4];
+    orc_philox4x64(ctr, key, r);
+    const double inv53 = 1.0 / 9007199254740992.0;
+    double u1 = (double)((r[0] >> 11) + 1) * inv53;
+    double u2 = (double)(r[1] >> 11) * inv53;
+    double u3 = (double)((r[2] >> 11) + 1) * inv53;
+    double u4 = (double)(r[3] >> 11) * inv53;
+    double rad1 = sqrt(-2.0 * log(u1)), rad2 = sqrt(-2.0 * log(u3));
+    const double twopi = 2.0 * PI;
+    g[0] = rad1 * cos(twopi * u2);
+    g[1] = rad1 * sin(twopi * u2);
+    g[2] = rad2 * cos(twopi * u4);
+}
+
+EXPORT int orc_upload_noise(orc_ctx *c, int32_t nsteps, const double *noise) {
+    free(c->noise);
+    size_t n = (size_t)nsteps * 3 * (size_t)c->cap;
+    c->noise = (double *)calloc(n ? n : 1, sizeof(double));
+    /* stored with row stride 3*ntot at upload time, as given */
+    memcpy(c->noise, noise, sizeof(double) * (size_t)nsteps * 3 * (size_t)c->s.ntot);
+    c->noise_rows = nsteps;
+    return BROMOC_OK;
+}
+
+static int fill_noise(orc_ctx *c, int mode, int row, uint64_t seed, uint64_t counter, double *g) {
+    int nt = c->s.ntot;
+    if (mode == BROMOC_NOISE_INJECTED) {
+        if (!c->noise || row < 0 || row >= c->noise_rows) return BROMOC_E_STATE;
+        memcpy(g, c->noise + (size_t)row * 3 * nt, sizeof(double) * 3 * nt);
+    } else {
+        for (int i = 0; i < nt; i++) orc_philox_gauss3(seed, counter, 0, i, g + 3 * i);
+    }
+    return BROMOC_OK;
+}
+
+EXPORT int orc_dynamics(orc_ctx *c, int32_t mode, int32_t row, uint64_t seed, uint64_t counter) {
+    double *g = (double *)malloc(sizeof(double) * 3 * (c->s.ntot + 1));
+    int st = fill_noise(c, mode, row, seed, counter, g);
+    if (st == BROMOC_OK) dynamics_pass(c, g);
+    free(g);
+    return st;
+}
+
+/* simul1.f90:363-390 with Qforces on: ENERGY then DYNAMICS0 then DYNAMICS0NUC */
+EXPORT int orc_step(orc_ctx *c, int32_t nsteps, int32_t mode, int32_t first_row, uint64_t seed,
+                    uint64_t counter0, bromoc_energies *energies) {
+    double *g = (double *)malloc(sizeof(double) * 3 * (c->s.ntot + 1));
+    int st = BROMOC_OK;
+    for (int s = 0; s < nsteps && st == BROMOC_OK; s++) {
+        st = orc_energy(c, 1, energies);
+        if (st) break;
+        st = fill_noise(c, mode, first_row + s, seed, counter0 + (uint64_t)s, g);
+        if (st) break;
+        dynamics_pass(c, g);
+    }
+    free(g);
+    return st;
+}
+
+/* -------------------------------------------------------- INTERACT */
+/* STATICF0, staticf0.f90:19-82 */
+static double staticf0(const orc_ctx *c, double xj, double yj, double zj, int j, int jtype) {
+    const bromoc_system *s = &c->s;
+    const bromoc_grid *g = &s->phix;
+    double egsbpa = 0.0;
+    int beion = (s->flags & BROMOC_QPAR) && j >= s->nsites;
+    int besite = 0;
+    if ((s->flags & BROMOC_QNUCL) && j < s->nsites) besite = c->namsite[j] == BROMOC_SITE_P;
+    int ok = beion || besite;
+    double chi = 0.0;
+    if (beion) chi = c->cg[jtype];
+    if (besite) chi = s->cgnuc;
+    double dcel = g->dcel, idcel = 1.0 / dcel;
+    double tranx = 0.5 * (g->nclx - 1) * dcel, trany = 0.5 * (g->ncly - 1) * dcel,
+           tranz = 0.5 * (g->nclz - 1) * dcel;
+    if (ok)
+        ok = xj <= g->xbcen + tranx && xj >= g->xbcen - tranx && yj <= g->ybcen + trany &&
+             yj >= g->ybcen - trany && zj <= g->zbcen + tranz && zj >= g->zbcen - tranz;
+    if (!ok) return egsbpa;
+    int ncyz = g->ncly * g->nclz;
+    double xi = xj + tranx - g->xbcen, yi = yj + trany - g->ybcen, zi = zj + tranz - g->zbcen;
+    int ix = (int)(xi * idcel), iy = (int)(yi * idcel), iz = (int)(zi * idcel);
+    if (ix == g->nclx - 1) ix = g->nclx - 2;
+    if (iy == g->ncly - 1) iy = g->ncly - 2;
+    if (iz == g->nclz - 1) iz = g->nclz - 2;
+    for (int n1 = ix; n1 <= ix + 1; n1++) {
+        double ai = xi - n1 * dcel;
+        ai = 1.0 - fabs(ai) * idcel;
+        for (int n2 = iy; n2 <= iy + 1; n2++) {
+            double bi = yi - n2 * dcel;
+            bi = 1.0 - fabs(bi) * idcel;
+            for (int n3 = iz; n3 <= iz + 1; n3++) {
+                double ci = zi - n3 * dcel;
+                ci = 1.0 - fabs(ci) * idcel;
+                double fi = ai * bi * ci;
+                size_t in3 = (size_t)n1 * ncyz + (size_t)n2 * g->nclz + n3;
+                egsbpa = egsbpa + (fi * chi * (double)c->phix[in3] * c->celec);
+            }
+        }
+    }
+    return egsbpa;
+}
+
+/* VDWGD0TRLN, vdwgd0trln.f90:19-117 */
+static double vdwgd0trln(orc_ctx *c, double xj, double yj, double zj, int jtype2, int alert) {
+    const bromoc_system *s = &c->s;
+    const bromoc_grid *g = &s->phiv;
+    double evdwgd = 0.0;
+    double dcel = g->dcel, idcel = 1.0 / dcel;
+    double tranx = 0.5 * (g->nclx - 1) * dcel, trany = 0.5 * (g->ncly - 1) * dcel,
+           tranz = 0.5 * (g->nclz - 1) * dcel;
+    int ok = xj <= g->xbcen + tranx && xj >= g->xbcen - tranx && yj <= g->ybcen + trany &&
+             yj >= g->ybcen - trany && zj <= s->vzmax && zj >= s->vzmin;
+    if (!ok) return evdwgd;
+    int ncyz = g->ncly * g->nclz;
+    double xi = xj + tranx - g->xbcen, yi = yj + trany - g->ybcen, zi = zj + tranz - g->zbcen;
+    double esvdw = s->svdw;
+    if (s->flags & BROMOC_QSVDW) esvdw = esvdw * c->scal[jtype2];
+    int ix = (int)(xi * idcel), iy = (int)(yi * idcel), iz = (int)(zi * idcel);
+    if (ix == g->nclx - 1) ix = g->nclx - 2;
+    if (iy == g->ncly - 1) iy = g->ncly - 2;
+    if (iz == g->nclz - 1) iz = g->nclz - 2;
+    double phisum = 0.0;
+    for (int n1 = ix; n1 <= ix + 1; n1++) {
+        double ai = xi - n1 * dcel;
+        ai = 1.0 - fabs(ai) * idcel;
+        for (int n2 = iy; n2 <= iy + 1; n2++) {
+            double bi = yi - n2 * dcel;
+            bi = 1.0 - fabs(bi) * idcel;
+            for (int n3 = iz; n3 <= iz + 1; n3++) {
+                double ci = zi - n3 * dcel;
+                ci = 1.0 - fabs(ci) * idcel;
+                double fi = ai * bi * ci;
+                size_t in3 = (size_t)n1 * ncyz + (size_t)n2 * g->nclz + n3;
+                double phis = (double)c->phiv[in3];
+                phisum = phisum + phis;
+                evdwgd = evdwgd + fi * esvdw * phis;
+            }
+        }
+    }
+    if (phisum >= s->thold8) {
+        evdwgd = 1.0e10;
+        if (alert) c->warn[jtype2] = c->warn[jtype2] + 1;
+    }
+    return evdwgd;
+}
+
+/* energy of one analytic pair as INTERACT writes it (interact.f90:184-204, 365-383):
+ * SRPMF smoothing is e*exp(...)-srpy there, unlike ENERGY's e*(exp(...)-srpy). */
+static double interact_analytic(const orc_ctx *c, int is, double dist2) {
+    const bromoc_pair *p = &c->pairs[is];
+    const bromoc_system *s = &c->s;
+    double e = 0.0, idist = 0.0;
+    double idist2 = 1.0 / dist2;
+    if (p->chr || p->srpmf) idist = sqrt(idist2);
+    if (p->chr) e = e + p->fct * idist;
+    if (p->lj) {
+        double dist6 = pow3(p->sgp2 * idist2);
+        e = e + p->epp4 * dist6 * (dist6 - 1.0);
+    }
+    if (p->srpmf && dist2 <= s->rth) {
+        double dist = 1.0 / idist;
+        double es = p->c0 * exp((p->c1 - dist) * p->c2) * cos(p->c3 * PI * (p->c1 - dist)) +
+                    p->c4 * pow6(p->c1 * idist);
+        if (dist >= s->srpx) es = es * exp(-s->srpk * (dist - s->srpx)) - s->srpy;
+        e = e + es;
+    }
+    return e;
+}
+
+EXPORT int orc_interact(orc_ctx *c, double xj, double yj, double zj, int32_t jtype, int32_t j,
+                        int32_t alert, double *dener_out) {
+    const bromoc_system *s = &c->s;
+    if (orc_check_flags(s->flags)) return BROMOC_E_UNSUPPORTED;
+    uint32_t fl = s->flags;
+    double dener = 0.0;
+    int ns = s->nsites, nt = s->ntot;
+    int beion = (fl & BROMOC_QPAR) && j >= ns;
+    int jtype2 = beion ? c->nwtype[jtype] : c->typtyp[j];
+    if (fl & BROMOC_QPHIX) dener = dener + staticf0(c, xj, yj, zj, j, jtype);
+    if (fl & BROMOC_QPHIV) dener = dener + vdwgd0trln(c, xj, yj, zj, jtype2, alert);
+    if (beion && (fl & BROMOC_QNONBOND)) {
+        for (int i = ns; i < nt; i++) {
+            if (i == j) continue;
+            int itype2 = c->nwtype[abs(c->typei[i])];
+            int is = pair_of(c, itype2, jtype2);
+            double dx = c->x[i] - xj, dy = c->y[i] - yj, dz = c->z[i] - zj;
+            double dist2 = dx * dx + dy * dy + dz * dz;
+            if (c->pairs[is].efpot) {
+                double eefp, dist;
+                gety(c, is, dist2, &eefp, &dist);
+                dener = dener + eefp;
+            } else {
+                dener = dener + interact_analytic(c, is, dist2);
+            }
+        }
+    }
+    if ((fl & BROMOC_QNUCL) && !beion) {
+        double eps = s->epsnuc;
+        if (fl & BROMOC_QNOBOND) {
+            const int32_t *L = c->lists[0];
+            for (int t = 0; t < s->nbond; t++) {
+                int a = L[t], b = L[s->nbond + t];
+                if (a != j && b != j) continue;
+                double dx = c->x[a] - c->x[b], dy = c->y[a] - c->y[b], dz = c->z[a] - c->z[b];
+                double dist = sqrt(dx * dx + dy * dy + dz * dz);
+                double vard = dist - c->lvals[0][t];
+                dener = dener + eps * (vard * vard) * (1.0 + 100.0 * (vard * vard));
+            }
+            L = c->lists[1];
+            for (int t = 0; t < s->nangle; t++) {
+                int a = L[t], b = L[s->nangle + t], d = L[2 * s->nangle + t];
+                if (a != j && b != j && d != j) continue;
+                /* angbond.f90:19-39 */
+                double v1[3] = {c->x[a] - c->x[b], c->y[a] - c->y[b], c->z[a] - c->z[b]};
+                double v2[3] = {c->x[d] - c->x[b], c->y[d] - c->y[b], c->z[d] - c->z[b]};
+                double r1 = sqrt(dot3(v1, v1)), r2 = sqrt(dot3(v2, v2));
+                double ang = acos(dot3(v1, v2) / (r1 * r2));
+                double varang = ang - c->lvals[1][t];
+                dener = dener + 700.0 * eps * (varang * varang);
+            }
+            L = c->lists[2];
+            for (int t = 0; t < s->ndihe; t++) {
+                int i1 = L[t], i2 = L[s->ndihe + t], i3 = L[2 * s->ndihe + t], i4 = L[3 * s->ndihe + t];
+                if (i1 != j && i2 != j && i3 != j && i4 != j) continue;
+                /* angdih.f90:19-48 on bond vectors b1=r2-r1, b2=r3-r2, b3=r4-r3 */
+                double b1[3] = {c->x[i2] - c->x[i1], c->y[i2] - c->y[i1], c->z[i2] - c->z[i1]};
+                double b2[3] = {c->x[i3] - c->x[i2], c->y[i3] - c->y[i2], c->z[i3] - c->z[i2]};
+                double b3[3] = {c->x[i4] - c->x[i3], c->y[i4] - c->y[i3], c->z[i4] - c->z[i3]};
+                double n1[3], n2[3];
+                cross(b1, b2, n1);
+                cross(b2, b3, n2);
+                double r2 = sqrt(dot3(b2, b2));
+                double sinphi = r2 * dot3(b1, n2);
+                double dihe = atan2(sinphi, dot3(n1, n2));
+                double vardihe = dihe - c->lvals[2][t];
+                dener = dener + 28.0 * eps * (1.0 - cos(vardihe));
+            }
+        }
+        if (fl & BROMOC_QNONBOND) {
+            const int32_t *L = c->lists[3];
+            for (int t = 0; t < s->nstack; t++) {
+                int a = L[t], b = L[s->nstack + t];
+                if (a != j && b != j) continue;
+                double dx = c->x[b] - c->x[a], dy = c->y[b] - c->y[a], dz = c->z[b] - c->z[a];
+                double dist2 = dx * dx + dy * dy + dz * dz;
+                double sg = c->lvals[3][t];
+                double cte1 = pow3(sg * sg / dist2);
+                dener = dener + 4.0 * eps * cte1 * (cte1 - 1.0);
+            }
+            L = c->lists[4];
+            for (int t = 0; t < s->nbp; t++) {
+                int a = L[t], b = L[s->nbp + t];
+                if (a != j && b != j) continue;
+                double epsln = (c->namsite[a] == BROMOC_SITE_GB || c->namsite[a] == BROMOC_SITE_CB)
+                                   ? 2.532 * eps : 2.0 * eps;
+                double dx = c->x[b] - c->x[a], dy = c->y[b] - c->y[a], dz = c->z[b] - c->z[a];
+                double dist2 = dx * dx + dy * dy + dz * dz;
+                double sg = c->lvals[4][t];
+                double cte1 = sg * sg / dist2;
+                dener = dener + epsln * pow5(cte1) * (20.0 * cte1 - 24.0);
+            }
+            L = c->lists[5];
+            for (int t = 0; t < s->nex; t++) {
+                int a = L[t], b = L[s->nex + t];
+                if (a != j && b != j) continue;
+                double dx = c->x[b] - c->x[a], dy = c->y[b] - c->y[a], dz = c->z[b] - c->z[a];
+                double dist2 = dx * dx + dy * dy + dz * dz;
+                double sgex2 = c->lvals[5][t] * c->lvals[5][t];
+                if (dist2 < sgex2) {
+                    double cte1 = pow3(sgex2 / dist2);
+                    dener = dener + 4.0 * eps * cte1 * (cte1 - 1.0) + eps;
+                }
+            }
+            L = c->lists[6];
+            for (int t = 0; t < s->nqq; t++) {
+                int a = L[t], b = L[s->nqq + t];
+                if (a != j && b != j) continue;
+                double dx = c->x[b] - c->x[a], dy = c->y[b] - c->y[a], dz = c->z[b] - c->z[a];
+                double dist = sqrt(dx * dx + dy * dy + dz * dz);
+                double idist = 1.0 / dist;
+                if (!(fl & BROMOC_QDEBY)) dener = dener + s->fctn * idist;
+                else dener = dener + s->fctn * idist * exp(-dist * s->ikappa);
+            }
+            if (fl & BROMOC_QSOLV) {
+                L = c->lists[7];
+                for (int t = 0; t < s->nsolv; t++) {
+                    int a = L[t], b = L[s->nsolv + t];
+                    if (a != j && b != j) continue;
+                    double dx = c->x[b] - c->x[a], dy = c->y[b] - c->y[a], dz = c->z[b] - c->z[a];
+                    double dist = sqrt(dx * dx + dy * dy + dz * dz);
+                    double cte1 = exp((13.38 - dist) * 0.1875);
+                    dener = dener + s->epsolv * ((1.0 - cte1) * (1.0 - cte1)) - s->epsolv;
+                }
+            }
+        }
+    }
+    if ((fl & BROMOC_QPAR) && (fl & BROMOC_QNUCL) && (fl & BROMOC_QNONBOND)) {
+        if (beion) {
+            for (int i = 0; i < ns; i++) {
+                int is = pair_of(c, c->typtyp[i], jtype2);
+                double dx = c->x[i] - xj, dy = c->y[i] - yj, dz = c->z[i] - zj;
+                double dist2 = dx * dx + dy * dy + dz * dz;
+                if (c->pairs[is].efpot) {
+                    double eefp, dist;
+                    gety(c, is, dist2, &eefp, &dist);
+                    dener = dener + eefp;
+                } else {
+                    dener = dener + interact_analytic(c, is, dist2);
+                }
+            }
+        } else {
+            for (int i = ns; i < nt; i++) {
+                int is = pair_of(c, c->nwtype[abs(c->typei[i])], jtype2);
+                double dx = c->x[i] - xj, dy = c->y[i] - yj, dz = c->z[i] - zj;
+                double dist2 = dx * dx + dy * dy + dz * dz;
+                if (c->pairs[is].efpot) {
+                    double eefp, dist;
+                    gety(c, is, dist2, &eefp, &dist);
+                    dener = dener + eefp;
+                } else {
+                    dener = dener + interact_analytic(c, is, dist2);
+                }
+            }
+        }
+    }
+    *dener_out = dener;
+    return BROMOC_OK;
+}
+
+/* ----------------------------------------------------------- GRAND */
+EXPORT int orc_set_gcmc(orc_ctx *c, const bromoc_gcmc *g, const int32_t *ibuffer) {
+    int nb = g->nbuffer;
+    c->g = *g;
+    free(c->ibfftyp); free(c->ntotat); free(c->avnum); free(c->mu); free(c->kb);
+    free(c->lzmin); free(c->lzmax); free(c->bufferbias); free(c->ibuffer);
+    c->ibfftyp = (int32_t *)dupmem(g->ibfftyp, sizeof(int32_t) * (nb + 1));
+    c->ntotat = (int32_t *)dupmem(g->ntotat, sizeof(int32_t) * (nb + 1));
+    c->avnum = (double *)dupmem(g->avnum, sizeof(double) * (nb + 1));
+    c->mu = (double *)dupmem(g->mu, sizeof(double) * (nb + 1));
+    c->kb = (double *)dupmem(g->kb, sizeof(double) * (nb + 1));
+    c->lzmin = (double *)dupmem(g->lzmin, sizeof(double) * (nb + 1));
+    c->lzmax = (double *)dupmem(g->lzmax, sizeof(double) * (nb + 1));
+    c->bufferbias = (int8_t *)dupmem(g->bufferbias, nb + 1);
+    c->ibuffer = (int32_t *)calloc(c->cap + 1, sizeof(int32_t));
+    if (ibuffer) memcpy(c->ibuffer, ibuffer, sizeof(int32_t) * c->s.ntot);
+    return BROMOC_OK;
+}
+
+EXPORT int orc_get_ibuffer(orc_ctx *c, int32_t *ib) {
+    memcpy(ib, c->ibuffer, sizeof(int32_t) * c->s.ntot);
+    return BROMOC_OK;
+}
+
+typedef struct { const double *u; int64_t n, k; int err; } ustream;
+static double rndm(ustream *st) {
+    if (st->k >= st->n) { st->err = 1; return 0.5; }
+    return st->u[st->k++];
+}
+
+/* GRAND, grand.f90:40-118 with INSERT (insert.f90:29-41, rectangular) and FIND
+ * (find.f90:26-32); every rndm() is taken from `uniforms` in call order. */
+EXPORT int orc_grand(orc_ctx *c, int32_t ngcmc, const double *uniforms, int64_t nuniforms,
+                     int64_t *used, int32_t *nat, int32_t *prob, int64_t icycle, double *ener) {
+    bromoc_system *s = &c->s;
+    const bromoc_gcmc *g = &c->g;
+    ustream st = {uniforms, nuniforms, 0, 0};
+    int nbuffer = g->nbuffer;
+    double nbufferr = (double)nbuffer;
+    int pstride = c->cap + 1;
+    for (int igrand = 0; igrand < ngcmc; igrand++) {
+        if (rndm(&st) < 0.5) {
+            int ib = (int)(nbufferr * rndm(&st)) + 1;
+            if (ib <= nbuffer && c->avnum[ib] > 0.0) {
+                int itype = c->ibfftyp[ib];
+                double xnew = g->lx * (rndm(&st) - 0.5) + g->cx;
+                double ynew = g->ly * (rndm(&st) - 0.5) + g->cy;
+                double znew = c->lzmin[ib] + (c->lzmax[ib] - c->lzmin[ib]) * rndm(&st);
+                double dener;
+                int rc = orc_interact(c, xnew, ynew, znew, itype, s->ntot, 0, &dener);
+                if (rc) return rc;
+                double rate;
+                if (c->bufferbias[ib]) {
+                    rate = (c->avnum[ib] + c->kb[ib] * (c->avnum[ib] - (double)(c->ntotat[ib] / icycle)));
+                    rate = rate * exp(-(dener - c->mu[ib]) * g->ikbt) / (double)(nat[ib] + 1);
+                } else {
+                    rate = (c->avnum[ib] / (nat[ib] + 1)) * exp(-(dener - c->mu[ib]) * g->ikbt);
+                }
+                rate = rate / (1.0 + rate);
+                if (rndm(&st) <= rate) {
+                    if (s->ntot + 1 > c->cap) return BROMOC_E_CAPACITY;
+                    *ener = *ener + dener;
+                    nat[ib] = nat[ib] + 1;
+                    int n = s->ntot;
+                    s->ntot = s->ntot + 1;
+                    c->x[n] = xnew; c->y[n] = ynew; c->z[n] = znew;
+                    c->typei[n] = itype;
+                    if (znew < g->cz) c->typei[n] = -itype;
+                    c->ibuffer[n] = ib;
+                }
+            }
+        } else {
+            int ib = (int)(nbufferr * rndm(&st)) + 1;
+            double natib = ib <= nbuffer ? (double)nat[ib] : 0.0; /* nat(nbuffer+1) is unused */
+            int ip = (int)(natib * rndm(&st)) + 1;
+            if (ib <= nbuffer && ip <= nat[ib]) {
+                int itype = c->ibfftyp[ib];
+                int iat = -1, ipx = 0;
+                for (int a = s->nfix + s->nsites; a < s->ntot; a++) {
+                    if (c->ibuffer[a] == ib) {
+                        ipx++;
+                        if (ip == ipx) { iat = a; break; }
+                    }
+                }
+                if (iat < 0) return BROMOC_E_STATE;
+                double dener;
+                int rc = orc_interact(c, c->x[iat], c->y[iat], c->z[iat], itype, iat, 1, &dener);
+                if (rc) return rc;
+                double rate;
+                if (c->bufferbias[ib]) {
+                    rate = (c->avnum[ib] + c->kb[ib] * (c->avnum[ib] - (double)(c->ntotat[ib] / icycle)));
+                    rate = rate * exp(-(dener - c->mu[ib]) * g->ikbt) / (double)nat[ib];
+                } else {
+                    rate = (c->avnum[ib] / nat[ib]) * exp(-(dener - c->mu[ib]) * g->ikbt);
+                }
+                rate = 1.0 / (1.0 + rate);
+                if (rndm(&st) <= rate) {
+                    *ener = *ener - dener;
+                    int n = s->ntot - 1;
+                    c->x[iat] = c->x[n]; c->y[iat] = c->y[n]; c->z[iat] = c->z[n];
+                    c->typei[iat] = c->typei[n];
+                    c->ibuffer[iat] = c->ibuffer[n];
+                    nat[ib] = nat[ib] - 1;
+                    s->ntot = s->ntot - 1;
+                }
+            }
+        }
+        for (int ib = 1; ib <= nbuffer; ib++) prob[(ib - 1) * pstride + nat[ib]] += 1;
+        if (st.err) return BROMOC_E_ARG;
+    }
+    if (used) *used = st.k;
+    return BROMOC_OK;
+}

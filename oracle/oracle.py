@@ -1,0 +1,338 @@
+"""ctypes front-ends of the two CPU checkers (TEST INFRASTRUCTURE ONLY).
+
+Oracle     -- oracle/liboracle.so, the C restatement (bromoc_oracle.c)
+RefFortran -- oracle/_ref/libbromoc_ref.so, the reference Fortran routines themselves
+              (built only where the reference sources exist)
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from bromoc_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libbromoc_ref.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+_olib = None
+
+
+def oracle_lib():
+    global _olib
+    if _olib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise ImportError(f"{ORACLE_SO} missing: make -C oracle")
+        L = C.CDLL(ORACLE_SO)
+        vp = C.c_void_p
+        P = C.POINTER
+        for name, args, res in [
+            ("orc_create", [P(abi.SystemStruct)], vp),
+            ("orc_destroy", [vp], None),
+            ("orc_set_positions", [vp, C.c_int32, _dp, _dp, _dp, _ip], C.c_int),
+            ("orc_get_positions", [vp, _dp, _dp, _dp, _ip, _ip], C.c_int),
+            ("orc_get_forces", [vp, _dp, _dp, _dp], C.c_int),
+            ("orc_get_flux", [vp, _ip, _ip], C.c_int),
+            ("orc_get_warn", [vp, _ip], C.c_int),
+            ("orc_energy", [vp, C.c_int32, P(abi.Energies)], C.c_int),
+            ("orc_upload_noise", [vp, C.c_int32, _dp], C.c_int),
+            ("orc_dynamics", [vp, C.c_int32, C.c_int32, C.c_uint64, C.c_uint64], C.c_int),
+            ("orc_step", [vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_uint64,
+                          P(abi.Energies)], C.c_int),
+            ("orc_interact", [vp, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                              C.c_int32, _dp], C.c_int),
+            ("orc_set_gcmc", [vp, P(abi.GcmcStruct), _ip], C.c_int),
+            ("orc_get_ibuffer", [vp, _ip], C.c_int),
+            ("orc_grand", [vp, C.c_int32, _dp, C.c_int64, P(C.c_int64), _ip, _ip, C.c_int64,
+                           _dp], C.c_int),
+            ("orc_build_table", [C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp,
+                                 P(abi.Pair)], C.c_int),
+            ("orc_philox4x64", [P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)], None),
+            ("orc_philox_gauss3", [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64, _dp], None),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _olib = L
+    return _olib
+
+
+def oracle_table_builder(n, r, u, chr_, fct):
+    ep = np.zeros(3 * n)
+    p = abi.Pair()
+    rc = oracle_lib().orc_build_table(n, _p(r), _p(u), chr_, fct, _p(ep), C.byref(p))
+    if rc:
+        raise RuntimeError(f"orc_build_table {rc}")
+    return ep, p
+
+
+class _Base:
+    def forces(self):
+        n = self.ntot
+        fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._get_forces(_p(fx), _p(fy), _p(fz))
+        return np.stack([fx, fy, fz], 1)
+
+    def positions(self):
+        n = self.ntot
+        x, y, z = np.zeros(n), np.zeros(n), np.zeros(n)
+        t = np.zeros(n, np.int32)
+        nn = C.c_int32()
+        self._get_positions(_p(x), _p(y), _p(z), _p(t, C.c_int32), C.byref(nn))
+        return np.stack([x, y, z], 1), t
+
+
+class Oracle(_Base):
+    """The C restatement."""
+
+    def __init__(self, system):
+        self.L = oracle_lib()
+        st, self._keep = system.to_struct()
+        self.ctx = self.L.orc_create(C.byref(st))
+        if not self.ctx:
+            raise RuntimeError("orc_create failed")
+        self.system = system
+        self.ntot = system.ntot
+
+    def __del__(self):
+        if getattr(self, "ctx", None):
+            self.L.orc_destroy(self.ctx)
+            self.ctx = None
+
+    def _get_forces(self, *a):
+        return self.L.orc_get_forces(self.ctx, *a)
+
+    def _get_positions(self, *a):
+        return self.L.orc_get_positions(self.ctx, *a)
+
+    def set_positions(self, pos, typei=None):
+        pos = np.ascontiguousarray(pos, float)
+        n = len(pos)
+        t = np.ascontiguousarray(typei if typei is not None else self.positions()[1][:n], np.int32)
+        x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+        rc = self.L.orc_set_positions(self.ctx, n, _p(x), _p(y), _p(z), _p(t, C.c_int32))
+        assert rc == 0
+        self.ntot = n
+
+    def energy(self, forces=True):
+        e = abi.Energies()
+        rc = self.L.orc_energy(self.ctx, 1 if forces else 0, C.byref(e))
+        if rc:
+            raise abi.BromocError("orc_energy", rc)
+        return e.as_dict()
+
+    def upload_noise(self, noise):
+        noise = np.ascontiguousarray(noise, float)
+        rc = self.L.orc_upload_noise(self.ctx, noise.shape[0], _p(noise))
+        assert rc == 0
+
+    def dynamics(self, mode=abi.NOISE_INJECTED, row=0, seed=0, counter=0):
+        rc = self.L.orc_dynamics(self.ctx, mode, row, seed, counter)
+        if rc:
+            raise abi.BromocError("orc_dynamics", rc)
+
+    def step(self, nsteps, mode=abi.NOISE_INJECTED, first_row=0, seed=0, counter0=0):
+        e = abi.Energies()
+        rc = self.L.orc_step(self.ctx, nsteps, mode, first_row, seed, counter0, C.byref(e))
+        if rc:
+            raise abi.BromocError("orc_step", rc)
+        return e.as_dict()
+
+    def flux(self):
+        s = self.system
+        nt, np_ = s.ntype - s.nold, len(s.zcont)
+        nf = np.zeros(max(nt * np_, 1), np.int32)
+        nb = np.zeros(max(nt * np_, 1), np.int32)
+        self.L.orc_get_flux(self.ctx, _p(nf, C.c_int32), _p(nb, C.c_int32))
+        return nf[:nt * np_].reshape(nt, np_), nb[:nt * np_].reshape(nt, np_)
+
+    def warn(self):
+        w = np.zeros(self.system.nttyp + 1, np.int32)
+        self.L.orc_get_warn(self.ctx, _p(w, C.c_int32))
+        return w
+
+    def interact(self, xyz, jtype, j, alert=False):
+        d = C.c_double()
+        rc = self.L.orc_interact(self.ctx, float(xyz[0]), float(xyz[1]), float(xyz[2]),
+                                 int(jtype), int(j), 1 if alert else 0, C.byref(d))
+        if rc:
+            raise abi.BromocError("orc_interact", rc)
+        return d.value
+
+    def set_gcmc(self, g, ibuffer):
+        st, self._gkeep = g.to_struct()
+        ib = np.ascontiguousarray(ibuffer, np.int32)
+        self.L.orc_set_gcmc(self.ctx, C.byref(st), _p(ib, C.c_int32))
+
+    def ibuffer(self):
+        ib = np.zeros(self.ntot, np.int32)
+        self.L.orc_get_ibuffer(self.ctx, _p(ib, C.c_int32))
+        return ib
+
+    def grand(self, ngcmc, uniforms, nat, prob, icycle, ener):
+        u = np.ascontiguousarray(uniforms, float)
+        used = C.c_int64()
+        e = C.c_double(ener)
+        rc = self.L.orc_grand(self.ctx, ngcmc, _p(u), len(u), C.byref(used),
+                              _p(nat, C.c_int32), _p(prob, C.c_int32), icycle, C.byref(e))
+        if rc:
+            raise abi.BromocError("orc_grand", rc)
+        nn = C.c_int32()
+        self.L.orc_get_positions(self.ctx, None, None, None, None, C.byref(nn))
+        self.ntot = nn.value
+        return used.value, e.value
+
+
+_rlib = None
+
+
+def ref_lib():
+    global _rlib
+    if _rlib is None:
+        if not ref_available():
+            raise ImportError(f"{REF_SO} missing (needs the reference sources; make -C oracle)")
+        L = C.CDLL(REF_SO)
+        P = C.POINTER
+        for name, args, res in [
+            ("ref_set_nindex", [C.c_int32, C.c_int32], None),
+            ("ref_init", [P(abi.SystemStruct)], None),
+            ("ref_energy", [C.c_int32, _dp], None),
+            ("ref_get_forces", [_dp, _dp, _dp], None),
+            ("ref_get_positions", [_dp, _dp, _dp, _ip, _ip], None),
+            ("ref_set_positions", [C.c_int32, _dp, _dp, _dp, _ip], None),
+            ("ref_get_flux", [_ip, _ip], None),
+            ("ref_get_warn", [_ip], None),
+            ("ref_dynamics", [_dp], None),
+            ("ref_interact", [C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                              C.c_int32, _dp], None),
+            ("ref_set_gcmc", [P(abi.GcmcStruct), _ip], None),
+            ("ref_get_ibuffer", [_ip], None),
+            ("ref_grand", [C.c_int32, _dp, C.c_int64, P(C.c_int64), _ip, _ip, C.c_int32,
+                           C.c_int64, _dp], None),
+            ("ref_build_table", [C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp,
+                                 P(abi.Pair)], None),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _rlib = L
+    return _rlib
+
+
+def ref_table_builder(n, r, u, chr_, fct):
+    ep = np.zeros(3 * n)
+    p = abi.Pair()
+    ref_lib().ref_build_table(n, _p(r), _p(u), chr_, fct, _p(ep), C.byref(p))
+    return ep, p
+
+
+class RefFortran(_Base):
+    """The reference routines (module-global state: one live instance at a time)."""
+
+    def __init__(self, system):
+        self.L = ref_lib()
+        self.system = system
+        self.L.ref_set_nindex(system.ndna, system.nion)
+        st, keep = system.to_struct()
+        self.L.ref_init(C.byref(st))
+        self.ntot = system.ntot
+        self._noise = None
+
+    def _get_forces(self, *a):
+        return self.L.ref_get_forces(*a)
+
+    def _get_positions(self, *a):
+        return self.L.ref_get_positions(*a)
+
+    def set_positions(self, pos, typei=None):
+        pos = np.ascontiguousarray(pos, float)
+        n = len(pos)
+        t = np.ascontiguousarray(typei if typei is not None else self.positions()[1][:n], np.int32)
+        x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+        self.L.ref_set_positions(n, _p(x), _p(y), _p(z), _p(t, C.c_int32))
+        self.ntot = n
+
+    def energy(self, forces=True):
+        e = np.zeros(23)
+        self.L.ref_energy(1 if forces else 0, _p(e))
+        return dict(zip(abi.ENERGY_FIELDS, e.tolist()))
+
+    def upload_noise(self, noise):
+        self._noise = np.ascontiguousarray(noise, float)
+
+    def dynamics(self, mode=abi.NOISE_INJECTED, row=0, seed=0, counter=0):
+        if mode == abi.NOISE_INJECTED:
+            g = np.ascontiguousarray(self._noise[row][: 3 * self.ntot])
+        else:
+            g = np.zeros(3 * self.ntot)
+            L = oracle_lib()
+            for i in range(self.ntot):
+                L.orc_philox_gauss3(seed, counter, 0, i, _p(g[3 * i:3 * i + 3]))
+        self.L.ref_dynamics(_p(g))
+
+    def step(self, nsteps, mode=abi.NOISE_INJECTED, first_row=0, seed=0, counter0=0):
+        e = None
+        for s in range(nsteps):
+            e = self.energy(True)
+            self.dynamics(mode, first_row + s, seed, counter0 + s)
+        return e
+
+    def flux(self):
+        s = self.system
+        nt, np_ = s.ntype - s.nold, len(s.zcont)
+        nf = np.zeros(max(nt * np_, 1), np.int32)
+        nb = np.zeros(max(nt * np_, 1), np.int32)
+        self.L.ref_get_flux(_p(nf, C.c_int32), _p(nb, C.c_int32))
+        return nf[:nt * np_].reshape(nt, np_), nb[:nt * np_].reshape(nt, np_)
+
+    def warn(self):
+        w = np.zeros(self.system.nttyp + 1, np.int32)
+        self.L.ref_get_warn(_p(w, C.c_int32))
+        return w
+
+    def interact(self, xyz, jtype, j, alert=False):
+        d = C.c_double()
+        self.L.ref_interact(float(xyz[0]), float(xyz[1]), float(xyz[2]), int(jtype), int(j),
+                            1 if alert else 0, C.byref(d))
+        return d.value
+
+    def set_gcmc(self, g, ibuffer):
+        st, self._gkeep = g.to_struct()
+        ib = np.ascontiguousarray(ibuffer, np.int32)
+        self.L.ref_set_gcmc(C.byref(st), _p(ib, C.c_int32))
+
+    def ibuffer(self):
+        ib = np.zeros(self.ntot, np.int32)
+        self.L.ref_get_ibuffer(_p(ib, C.c_int32))
+        return ib
+
+    def grand(self, ngcmc, uniforms, nat, prob, icycle, ener):
+        u = np.ascontiguousarray(uniforms, float)
+        used = C.c_int64()
+        e = C.c_double(ener)
+        pstride = prob.shape[1]
+        self.L.ref_grand(ngcmc, _p(u), len(u), C.byref(used), _p(nat, C.c_int32),
+                         _p(prob, C.c_int32), pstride, icycle, C.byref(e))
+        nn = C.c_int32()
+        x = np.zeros(self.system.capacity)
+        t = np.zeros(self.system.capacity, np.int32)
+        self.L.ref_get_positions(_p(x), _p(x.copy()), _p(x.copy()), _p(t, C.c_int32),
+                                 C.byref(nn))
+        self.ntot = nn.value
+        return used.value, e.value

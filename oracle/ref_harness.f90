@@ -1,0 +1,642 @@
+! ref_harness.f90 -- drives the reference cg-gcmc-bd routines (compiled unchanged from
+! their own sources by oracle/Makefile) from a bromoc_system description.
+!
+! TEST INFRASTRUCTURE ONLY: loaded by tests/ and bench.py's cpu_baseline leg as the
+! reference checker; never part of the product path.
+!
+! What this file supplies, and why:
+!  * ref_* entry points (bind(C)) that copy a bromoc_system into the reference's module
+!    globals (grandmod, efpmod, extramod, nucleotmod, gsbpmod, apfmod) and call ENERGY,
+!    DYNAMICS0, DYNAMICS0NUC, INTERACT, GRAND and SPLINEPOT themselves.
+!  * rgauss(), rndm(), rndm1(): the reference draws noise from gfortran's rand()
+!    (random.f90:19-31, rgauss.f90:19-26).  Here they return injected values in call
+!    order, the seam through which GPU and reference consume identical noise.
+module ref_harness
+use iso_c_binding
+implicit none
+
+type, bind(C) :: c_pair
+  integer(c_int32_t) :: efpot, chr, lj, srpmf, nxi, nknots
+  real(c_double) :: fct, epp4, sgp2, c0, c1, c2, c3, c4, dmi, dm2lo, dm2hi, sc1, sc2, sc3
+end type
+
+type, bind(C) :: c_grid
+  integer(c_int32_t) :: nclx, ncly, nclz, reserved
+  real(c_double) :: dcel, xbcen, ybcen, zbcen
+  type(c_ptr) :: data
+end type
+
+type, bind(C) :: c_system
+  integer(c_int32_t) :: abi_version, ntot, nsites, nfix, capacity, ntype, nold, nttyp, npair
+  integer(c_int32_t) :: flags
+  type(c_ptr) :: x, y, z, typei, typtyp, namsite, stfree
+  type(c_ptr) :: nwtype, cg, fact1a, fact2a, pairidx, pairs
+  integer(c_int32_t) :: nknots, reserved0
+  type(c_ptr) :: ep
+  real(c_double) :: res, ires, rth, srpx, srpk, srpy
+  real(c_double) :: cgnuc, epsnuc, scalepairing, fctn, kappa, ikappa, epsolv
+  integer(c_int32_t) :: nbond, nangle, ndihe, nstack, nbp, nex, nqq, nsolv
+  type(c_ptr) :: sitebond, distbond, siteangle, valangle, sitedihe, valdihe
+  type(c_ptr) :: sitestack, sgstack, sitebp, sgbp, siteex, sgex, siteqq, siteslv
+  type(c_grid) :: phix, phiv
+  real(c_double) :: svdw, thold8, vzmin, vzmax
+  type(c_ptr) :: scal
+  real(c_double) :: bdmax, fact0n1, fact0n2, lx2m, lx2p, ly2m, ly2p, lz2m, lz2p
+  integer(c_int32_t) :: cntpts, reserved1
+  type(c_ptr) :: zcont
+end type
+
+type, bind(C) :: c_gcmc
+  integer(c_int32_t) :: nbuffer, reserved
+  type(c_ptr) :: ibfftyp, avnum, mu, kb, lzmin, lzmax, bufferbias, ntotat
+  real(c_double) :: lx, ly, cx, cy, cz, ikbt
+end type
+
+! injected streams
+real(c_double), allocatable :: gstream(:), ustream(:)
+integer :: gpos = 0, upos = 0, ulen = 0
+integer(c_int32_t), parameter :: F_PAR=1, F_NUCL=2, F_NONBOND=4, F_NOBOND=8, F_PHIX=16, F_PHIV=32, &
+     F_SVDW=64, F_SOLV=128, F_DEBY=256, F_COUNTION=512, F_DNAFREE=1024
+integer :: site_names_code(0:5)
+
+contains
+
+logical(1) function flag(f, b)
+  integer(c_int32_t), intent(in) :: f, b
+  flag = iand(f, b) /= 0
+end function
+
+end module ref_harness
+
+! ---------------------------------------------------------------- noise seams
+function rgauss()
+use ref_harness
+implicit none
+real rgauss
+gpos = gpos + 1
+rgauss = gstream(gpos)
+end function
+
+function rndm()
+use ref_harness
+implicit none
+real rndm
+upos = upos + 1
+if (upos > ulen) then
+  rndm = 0.5
+else
+  rndm = ustream(upos)
+endif
+end function
+
+function rndm1()
+use ref_harness
+implicit none
+real rndm1
+upos = upos + 1
+if (upos > ulen) then
+  rndm1 = 0.5
+else
+  rndm1 = 1.0 - ustream(upos)
+endif
+end function
+
+! ------------------------------------------------------------------ set-up
+subroutine ref_init(s) bind(C, name='ref_init')
+use ref_harness
+use grandmod
+use efpmod
+use extramod
+use nucleotmod
+use gsbpmod
+use apfmod
+implicit none
+type(c_system), intent(in) :: s
+real(c_double), pointer :: xp(:), yp(:), zp(:), dp(:)
+integer(c_int32_t), pointer :: ip(:)
+integer(c_int8_t), pointer :: bp(:)
+real(c_float), pointer :: fp(:)
+type(c_pair), pointer :: pp(:)
+integer :: i, n, ns, ntc, ncel3, nt
+
+character(len=2), parameter :: names(0:5) = (/'S ','P ','Ab','Tb','Cb','Gb'/)
+
+ntot = s%ntot
+nsites = s%nsites
+nfix = s%nfix
+ntype = s%ntype
+nold = s%nold
+nttyp = s%nttyp
+natom = ntot
+Qpar = flag(s%flags, F_PAR)
+Qnucl = flag(s%flags, F_NUCL)
+Qnonbond = flag(s%flags, F_NONBOND)
+Qnobond = flag(s%flags, F_NOBOND)
+Qphix = flag(s%flags, F_PHIX)
+Qphiv = flag(s%flags, F_PHIV)
+Qsvdw = flag(s%flags, F_SVDW)
+Qsolv = flag(s%flags, F_SOLV)
+Qdeby = flag(s%flags, F_DEBY)
+Qcountion = flag(s%flags, F_COUNTION)
+Qdnafree = flag(s%flags, F_DNAFREE)
+Qenergy = .true.
+Qforces = .false.
+Qmemb = .false.
+Qmmij = .false.
+Qrfpar = .false.
+Qtrln = .true.
+Qnmcden = .false.
+Qproxdiff = .false.
+Qdebyhyb = .false.
+Qapfor = .false.
+Qcontrans = .false.
+Qwarn = .false.
+Qsphere = .false.
+Qecyl = .false.
+Qnotrans = .false.
+
+call c_f_pointer(s%x, xp, [ntot]); x(1:ntot) = xp
+call c_f_pointer(s%y, yp, [ntot]); y(1:ntot) = yp
+call c_f_pointer(s%z, zp, [ntot]); z(1:ntot) = zp
+call c_f_pointer(s%typei, ip, [ntot]); typei(1:ntot) = ip
+fx = 0.0; fy = 0.0; fz = 0.0
+ns = nsites
+if (ns > 0) then
+  call c_f_pointer(s%typtyp, ip, [ns]); typtyp(1:ns) = ip
+  if (allocated(namsite)) deallocate(namsite)
+  if (allocated(stfree)) deallocate(stfree)
+  allocate(namsite(ns), stfree(ns))
+  call c_f_pointer(s%namsite, ip, [ns])
+  do i = 1, ns
+    namsite(i) = names(ip(i))
+  enddo
+  call c_f_pointer(s%stfree, bp, [ns])
+  do i = 1, ns
+    stfree(i) = bp(i) /= 0
+  enddo
+endif
+call c_f_pointer(s%nwtype, ip, [ntype+1]); nwtype(1:ntype) = ip(2:ntype+1)
+call c_f_pointer(s%cg, dp, [ntype+1]); cg(1:ntype) = dp(2:ntype+1)
+call c_f_pointer(s%fact1a, dp, [ntype+1]); fact1a(1:ntype) = dp(2:ntype+1)
+call c_f_pointer(s%fact2a, dp, [ntype+1]); fact2a(1:ntype) = dp(2:ntype+1)
+
+! pair tables: the harness's nindex is the host table pairidx (nindex.f90 semantics)
+ntc = s%npair
+if (allocated(Qefpot)) deallocate(Qefpot, Qchr, Qlj, Qsrpmfi, fct, epp4, sgp2, c0, c1, c2, c3, c4, &
+                                  nxi, dmi, dm2, sc)
+allocate(Qefpot(ntc), Qchr(ntc), Qlj(ntc), Qsrpmfi(ntc), fct(ntc), epp4(ntc), sgp2(ntc), &
+         c0(ntc), c1(ntc), c2(ntc), c3(ntc), c4(ntc), nxi(ntc), dmi(ntc), dm2(2,ntc), sc(3,ntc))
+call c_f_pointer(s%pairs, pp, [ntc+1])
+do i = 1, ntc
+  Qefpot(i) = pp(i+1)%efpot /= 0
+  Qchr(i) = pp(i+1)%chr /= 0
+  Qlj(i) = pp(i+1)%lj /= 0
+  Qsrpmfi(i) = pp(i+1)%srpmf /= 0
+  fct(i) = pp(i+1)%fct
+  epp4(i) = pp(i+1)%epp4
+  sgp2(i) = pp(i+1)%sgp2
+  c0(i) = pp(i+1)%c0; c1(i) = pp(i+1)%c1; c2(i) = pp(i+1)%c2; c3(i) = pp(i+1)%c3; c4(i) = pp(i+1)%c4
+  nxi(i) = pp(i+1)%nxi + 1
+  dmi(i) = pp(i+1)%dmi
+  dm2(1,i) = pp(i+1)%dm2lo
+  dm2(2,i) = pp(i+1)%dm2hi
+  sc(1,i) = pp(i+1)%sc1; sc(2,i) = pp(i+1)%sc2; sc(3,i) = pp(i+1)%sc3
+enddo
+call set_pairidx(s)
+if (allocated(ep)) deallocate(ep)
+allocate(ep(3, max(1, s%nknots)))
+if (s%nknots > 0) then
+  call c_f_pointer(s%ep, dp, [3*s%nknots])
+  ep = reshape(dp, [3, s%nknots])
+endif
+res = s%res
+ires = s%ires
+rth = s%rth; srpx = s%srpx; srpk = s%srpk; srpy = s%srpy
+
+! CG-DNA
+cgnuc = s%cgnuc; epsnuc = s%epsnuc; scalepairing = s%scalepairing; fctn = s%fctn
+kappa = s%kappa; ikappa = s%ikappa; epsolv = s%epsolv
+call set_list2(s%sitebond, s%distbond, s%nbond, 1)
+call set_list3(s)
+call set_list4(s)
+call set_list2(s%sitestack, s%sgstack, s%nstack, 2)
+call set_list2(s%sitebp, s%sgbp, s%nbp, 3)
+call set_list2(s%siteex, s%sgex, s%nex, 4)
+call set_list2(s%siteqq, c_null_ptr, s%nqq, 5)
+call set_list2(s%siteslv, c_null_ptr, s%nsolv, 6)
+
+! grids (readphi.f90:49-51,160-170,330-340)
+if (Qphix) then
+  nclx1 = s%phix%nclx; ncly1 = s%phix%ncly; nclz1 = s%phix%nclz
+  dcel1 = s%phix%dcel; idcel1 = 1.0/dcel1
+  tranx1 = 0.5*(nclx1-1)*dcel1; trany1 = 0.5*(ncly1-1)*dcel1; tranz1 = 0.5*(nclz1-1)*dcel1
+  xbcen1 = s%phix%xbcen; ybcen1 = s%phix%ybcen; zbcen1 = s%phix%zbcen
+  ncel3 = nclx1*ncly1*nclz1
+  if (allocated(phix)) deallocate(phix)
+  allocate(phix(ncel3))
+  call c_f_pointer(s%phix%data, fp, [ncel3]); phix = fp
+endif
+if (Qphiv) then
+  nclx2 = s%phiv%nclx; ncly2 = s%phiv%ncly; nclz2 = s%phiv%nclz
+  dcel2 = s%phiv%dcel; idcel2 = 1.0/dcel2
+  tranx2 = 0.5*(nclx2-1)*dcel2; trany2 = 0.5*(ncly2-1)*dcel2; tranz2 = 0.5*(nclz2-1)*dcel2
+  xbcen2 = s%phiv%xbcen; ybcen2 = s%phiv%ybcen; zbcen2 = s%phiv%zbcen
+  ncel3 = nclx2*ncly2*nclz2
+  if (allocated(phiv)) deallocate(phiv)
+  allocate(phiv(ncel3))
+  call c_f_pointer(s%phiv%data, bp, [ncel3]); phiv = bp
+  svdw = s%svdw; thold8 = s%thold8; vzmin = s%vzmin; vzmax = s%vzmax
+  if (allocated(scal)) deallocate(scal)
+  allocate(scal(nttyp))
+  if (Qsvdw) then
+    call c_f_pointer(s%scal, dp, [nttyp+1]); scal(1:nttyp) = dp(2:nttyp+1)
+  else
+    scal = 1.0
+  endif
+endif
+if (allocated(warn)) deallocate(warn)
+allocate(warn(nttyp))
+warn = 0
+
+! dynamics (simul1.f90:342-349 precomputed by the caller)
+bdmax = s%bdmax; fact0n1 = s%fact0n1; fact0n2 = s%fact0n2
+lx2m = s%lx2m; lx2p = s%lx2p; ly2m = s%ly2m; ly2p = s%ly2p; lz2m = s%lz2m; lz2p = s%lz2p
+cntpts = s%cntpts
+nt = max(1, ntype - nold)
+if (allocated(nforward)) deallocate(nforward, nbackward)
+allocate(nforward(nt, max(1,cntpts)), nbackward(nt, max(1,cntpts)))
+nforward = 0; nbackward = 0
+if (allocated(zcont)) deallocate(zcont)
+allocate(zcont(max(1,cntpts)))
+if (cntpts > 0) then
+  call c_f_pointer(s%zcont, dp, [cntpts]); zcont(1:cntpts) = dp
+endif
+n = 0
+end subroutine
+
+subroutine set_list2(lp, vp, n, which)
+use ref_harness
+use nucleotmod
+implicit none
+type(c_ptr), intent(in) :: lp, vp
+integer(c_int32_t), intent(in) :: n
+integer, intent(in) :: which
+integer(c_int32_t), pointer :: ip(:)
+real(c_double), pointer :: dp(:)
+integer :: m
+m = max(1, n)
+if (n > 0) call c_f_pointer(lp, ip, [2*n])
+if (n > 0 .and. c_associated(vp)) call c_f_pointer(vp, dp, [n])
+select case (which)
+case (1)
+  nbond = n
+  if (allocated(sitebond)) deallocate(sitebond, distbond)
+  allocate(sitebond(m,2), distbond(m))
+  if (n > 0) then
+    sitebond(1:n,1) = ip(1:n) + 1; sitebond(1:n,2) = ip(n+1:2*n) + 1; distbond(1:n) = dp
+  endif
+case (2)
+  nstack = n
+  if (allocated(sitestack)) deallocate(sitestack, sgstack)
+  allocate(sitestack(m,2), sgstack(m))
+  if (n > 0) then
+    sitestack(1:n,1) = ip(1:n) + 1; sitestack(1:n,2) = ip(n+1:2*n) + 1; sgstack(1:n) = dp
+  endif
+case (3)
+  nbp = n
+  if (allocated(sitebp)) deallocate(sitebp, sgbp)
+  allocate(sitebp(m,2), sgbp(m))
+  if (n > 0) then
+    sitebp(1:n,1) = ip(1:n) + 1; sitebp(1:n,2) = ip(n+1:2*n) + 1; sgbp(1:n) = dp
+  endif
+case (4)
+  nex = n
+  if (allocated(siteex)) deallocate(siteex, sgex)
+  allocate(siteex(m,2), sgex(m))
+  if (n > 0) then
+    siteex(1:n,1) = ip(1:n) + 1; siteex(1:n,2) = ip(n+1:2*n) + 1; sgex(1:n) = dp
+  endif
+case (5)
+  nqq = n
+  if (allocated(siteqq)) deallocate(siteqq)
+  allocate(siteqq(m,2))
+  if (n > 0) then
+    siteqq(1:n,1) = ip(1:n) + 1; siteqq(1:n,2) = ip(n+1:2*n) + 1
+  endif
+case (6)
+  nsolv = n
+  if (allocated(siteslv)) deallocate(siteslv)
+  allocate(siteslv(m,2))
+  if (n > 0) then
+    siteslv(1:n,1) = ip(1:n) + 1; siteslv(1:n,2) = ip(n+1:2*n) + 1
+  endif
+end select
+end subroutine
+
+subroutine set_list3(s)
+use ref_harness
+use nucleotmod
+implicit none
+type(c_system), intent(in) :: s
+integer(c_int32_t), pointer :: ip(:)
+real(c_double), pointer :: dp(:)
+integer :: n, k
+n = s%nangle
+nangle = n
+if (allocated(siteangle)) deallocate(siteangle, valangle)
+allocate(siteangle(max(1,n),3), valangle(max(1,n)))
+if (n > 0) then
+  call c_f_pointer(s%siteangle, ip, [3*n]); call c_f_pointer(s%valangle, dp, [n])
+  do k = 1, 3
+    siteangle(1:n,k) = ip((k-1)*n+1:k*n) + 1
+  enddo
+  valangle(1:n) = dp
+endif
+end subroutine
+
+subroutine set_list4(s)
+use ref_harness
+use nucleotmod
+implicit none
+type(c_system), intent(in) :: s
+integer(c_int32_t), pointer :: ip(:)
+real(c_double), pointer :: dp(:)
+integer :: n, k
+n = s%ndihe
+ndihe = n
+if (allocated(sitedihe)) deallocate(sitedihe, valdihe)
+allocate(sitedihe(max(1,n),4), valdihe(max(1,n)))
+if (n > 0) then
+  call c_f_pointer(s%sitedihe, ip, [4*n]); call c_f_pointer(s%valdihe, dp, [n])
+  do k = 1, 4
+    sitedihe(1:n,k) = ip((k-1)*n+1:k*n) + 1
+  enddo
+  valdihe(1:n) = dp
+endif
+end subroutine
+
+! nindex() of the reference maps atnam2 pairs through ndna/nion bookkeeping set up by
+! reasig.f90; this harness sets that bookkeeping so nindex reproduces the host's pairidx,
+! and aborts if it cannot.
+subroutine set_pairidx(s)
+use ref_harness
+use nucleotmod
+implicit none
+type(c_system), intent(in) :: s
+integer(c_int32_t), pointer :: ip(:)
+integer :: a, b, n, nindex
+n = s%nttyp
+call c_f_pointer(s%pairidx, ip, [(n+1)*(n+1)])
+do a = 1, n
+  do b = 1, n
+    if (ip(a*(n+1)+b+1) /= 0) then
+      if (nindex(a, b) /= ip(a*(n+1)+b+1)) then
+        write(*,*) 'ref_harness: pairidx does not follow nindex.f90 for ', a, b, &
+                   nindex(a, b), ip(a*(n+1)+b+1)
+        stop 3
+      endif
+    endif
+  enddo
+enddo
+end subroutine
+
+subroutine ref_set_nindex(ndna_in, nion_in) bind(C, name='ref_set_nindex')
+use iso_c_binding
+use nucleotmod
+implicit none
+integer(c_int32_t), value :: ndna_in, nion_in
+ndna = ndna_in
+nion = nion_in
+ndnaxnion = ndna*nion
+end subroutine
+
+! --------------------------------------------------------------- ENERGY
+subroutine ref_energy(qf, e) bind(C, name='ref_energy')
+use ref_harness
+use grandmod
+use gsbpmod
+use nucleotmod
+implicit none
+integer(c_int32_t), value :: qf
+real(c_double), intent(out) :: e(23)
+Qforces = qf /= 0
+call energy
+e = 0.0
+e(1) = ener
+e(2) = eelec; e(3) = evdw; e(4) = esrpmf; e(5) = esrpmfmx; e(6) = eefpot; e(7) = eefpotmx
+e(8) = eqqmx; e(9) = evdwmx; e(10) = ebond; e(11) = eang; e(12) = edihe; e(13) = estack
+e(14) = ebp; e(15) = eex; e(16) = eqq; e(17) = esolv; e(18) = egsbpa; e(19) = egsbpb
+e(20) = evdwgd; e(21) = erfpar; e(22) = econ; e(23) = ememb
+end subroutine
+
+subroutine ref_get_forces(ofx, ofy, ofz) bind(C, name='ref_get_forces')
+use ref_harness
+use grandmod
+implicit none
+real(c_double), intent(out) :: ofx(*), ofy(*), ofz(*)
+ofx(1:ntot) = fx(1:ntot); ofy(1:ntot) = fy(1:ntot); ofz(1:ntot) = fz(1:ntot)
+end subroutine
+
+subroutine ref_get_positions(ox, oy, oz, ot, n) bind(C, name='ref_get_positions')
+use ref_harness
+use grandmod
+implicit none
+real(c_double), intent(out) :: ox(*), oy(*), oz(*)
+integer(c_int32_t), intent(out) :: ot(*), n
+n = ntot
+ox(1:ntot) = x(1:ntot); oy(1:ntot) = y(1:ntot); oz(1:ntot) = z(1:ntot)
+ot(1:ntot) = typei(1:ntot)
+end subroutine
+
+subroutine ref_set_positions(n, ix, iy, iz, it) bind(C, name='ref_set_positions')
+use ref_harness
+use grandmod
+implicit none
+integer(c_int32_t), value :: n
+real(c_double), intent(in) :: ix(*), iy(*), iz(*)
+integer(c_int32_t), intent(in) :: it(*)
+ntot = n
+natom = n
+x(1:n) = ix(1:n); y(1:n) = iy(1:n); z(1:n) = iz(1:n); typei(1:n) = it(1:n)
+end subroutine
+
+subroutine ref_get_flux(nf, nb) bind(C, name='ref_get_flux')
+use ref_harness
+use grandmod
+implicit none
+integer(c_int32_t), intent(out) :: nf(*), nb(*)
+integer :: i, j, nt
+nt = ntype - nold
+do i = 1, nt
+  do j = 1, cntpts
+    nf((i-1)*cntpts + j) = nforward(i, j)
+    nb((i-1)*cntpts + j) = nbackward(i, j)
+  enddo
+enddo
+end subroutine
+
+subroutine ref_get_warn(w) bind(C, name='ref_get_warn')
+use ref_harness
+use grandmod
+use nucleotmod
+implicit none
+integer(c_int32_t), intent(out) :: w(*)
+w(1) = 0
+w(2:nttyp+1) = warn(1:nttyp)
+end subroutine
+
+! ------------------------------------------------------------- DYNAMICS
+! One BD update with the forces left by ENERGY: DYNAMICS0 over moving ions then
+! DYNAMICS0NUC over sites (simul1.f90:369-381).  g is [ntot][3] per particle; the
+! stream handed to rgauss() follows the reference's draw order.
+subroutine ref_dynamics(g) bind(C, name='ref_dynamics')
+use ref_harness
+use grandmod
+use nucleotmod
+implicit none
+real(c_double), intent(in) :: g(3, *)
+integer :: i, k
+if (allocated(gstream)) deallocate(gstream)
+allocate(gstream(3*ntot + 3))
+k = 0
+if (Qpar) then
+  do i = nsites+nfix+1, ntot
+    gstream(k+1:k+3) = g(1:3, i); k = k + 3
+  enddo
+endif
+if (Qnucl) then
+  do i = 1, nsites
+    if (stfree(i)) then
+      gstream(k+1:k+3) = g(1:3, i); k = k + 3
+    endif
+  enddo
+endif
+gpos = 0
+if (Qpar) call dynamics0(nsites+nfix+1, ntot)
+if (Qnucl) call dynamics0nuc(1, nsites)
+end subroutine
+
+! ------------------------------------------------------------- INTERACT
+subroutine ref_interact(xj, yj, zj, jtype, j, alert, dener) bind(C, name='ref_interact')
+use ref_harness
+implicit none
+real(c_double), value :: xj, yj, zj
+integer(c_int32_t), value :: jtype, j, alert
+real(c_double), intent(out) :: dener
+real :: d, xx, yy, zz
+integer :: jt, jj
+logical(1) :: qa
+xx = xj; yy = yj; zz = zj; jt = jtype; jj = j + 1
+qa = alert /= 0
+call interact(d, xx, yy, zz, jt, jj, qa)
+dener = d
+end subroutine
+
+! ---------------------------------------------------------------- GRAND
+subroutine ref_set_gcmc(g, ib_in) bind(C, name='ref_set_gcmc')
+use ref_harness
+use grandmod
+implicit none
+type(c_gcmc), intent(in) :: g
+integer(c_int32_t), intent(in) :: ib_in(*)
+integer(c_int32_t), pointer :: ip(:)
+real(c_double), pointer :: dp(:)
+integer(c_int8_t), pointer :: bp(:)
+integer :: nb, i
+nb = g%nbuffer
+nbuffer = nb
+call c_f_pointer(g%ibfftyp, ip, [nb+1]); ibfftyp(1:nb) = ip(2:nb+1)
+call c_f_pointer(g%ntotat, ip, [nb+1]); ntotat(1:nb) = ip(2:nb+1)
+call c_f_pointer(g%avnum, dp, [nb+1]); avnum(1:nb) = dp(2:nb+1)
+call c_f_pointer(g%mu, dp, [nb+1]); mu(1:nb) = dp(2:nb+1)
+call c_f_pointer(g%kb, dp, [nb+1]); kb(1:nb) = dp(2:nb+1)
+call c_f_pointer(g%lzmin, dp, [nb+1]); LZmin(1:nb) = dp(2:nb+1)
+call c_f_pointer(g%lzmax, dp, [nb+1]); LZmax(1:nb) = dp(2:nb+1)
+call c_f_pointer(g%bufferbias, bp, [nb+1])
+do i = 1, nb
+  Qbufferbias(i) = bp(i+1) /= 0
+enddo
+lx = g%lx; ly = g%ly; cx = g%cx; cy = g%cy; cz = g%cz; ikbt = g%ikbt
+ibuffer(1:ntot) = ib_in(1:ntot)
+ninsert = 0; nremove = 0
+end subroutine
+
+subroutine ref_get_ibuffer(ib_out) bind(C, name='ref_get_ibuffer')
+use ref_harness
+use grandmod
+implicit none
+integer(c_int32_t), intent(out) :: ib_out(*)
+ib_out(1:ntot) = ibuffer(1:ntot)
+end subroutine
+
+subroutine ref_grand(ngcmc, u, nu, used, nat_io, prob_io, pstride, icycle, ener_io) &
+    bind(C, name='ref_grand')
+use ref_harness
+use grandmod
+implicit none
+integer(c_int32_t), value :: ngcmc, pstride
+integer(c_int64_t), value :: nu, icycle
+real(c_double), intent(in) :: u(*)
+integer(c_int64_t), intent(out) :: used
+integer(c_int32_t), intent(inout) :: nat_io(*), prob_io(*)
+real(c_double), intent(inout) :: ener_io
+integer, allocatable :: prob(:,:)
+integer :: ib, n, ng
+integer(8) :: ic
+if (allocated(ustream)) deallocate(ustream)
+allocate(ustream(max(1_8, nu)))
+ustream(1:nu) = u(1:nu)
+ulen = int(nu)
+upos = 0
+nat(1:nbuffer) = nat_io(2:nbuffer+1)
+allocate(prob(nbuffer, 0:datom))
+prob = 0
+do ib = 1, nbuffer
+  do n = 0, pstride-1
+    prob(ib, n) = prob_io((ib-1)*pstride + n + 1)
+  enddo
+enddo
+ener = ener_io
+ng = ngcmc
+ic = icycle
+call grand(ng, prob, ic)
+ener_io = ener
+nat_io(2:nbuffer+1) = nat(1:nbuffer)
+do ib = 1, nbuffer
+  do n = 0, pstride-1
+    prob_io((ib-1)*pstride + n + 1) = prob(ib, n)
+  enddo
+enddo
+used = upos
+deallocate(prob)
+end subroutine
+
+! ---------------------------------------------------------- table build
+! SPLINEPOT (splinepot.f90:19-96) for one pair type into a scratch slot.
+subroutine ref_build_table(n, xx, yy, chr, fctv, ep_out, pout) bind(C, name='ref_build_table')
+use ref_harness
+use efpmod
+use extramod
+implicit none
+integer(c_int32_t), value :: n, chr
+real(c_double), value :: fctv
+real(c_double), intent(in) :: xx(*), yy(*)
+real(c_double), intent(out) :: ep_out(3, *)
+type(c_pair), intent(inout) :: pout
+real, allocatable :: xs(:), ys(:)
+if (allocated(ep)) deallocate(ep)
+if (allocated(nxi)) deallocate(nxi, dmi, dm2, sc)
+if (allocated(fct)) deallocate(fct, Qchr)
+allocate(ep(3, n), nxi(1), dmi(1), dm2(2,1), sc(3,1), fct(1), Qchr(1))
+allocate(xs(n), ys(n))
+xs = xx(1:n); ys = yy(1:n)
+nxi(1) = 1
+fct(1) = fctv
+Qchr(1) = chr /= 0
+call splinepot(1, n, xs, ys, n)
+ep_out(1:3, 1:n) = ep(1:3, 1:n)
+pout%dmi = dmi(1); pout%dm2lo = dm2(1,1); pout%dm2hi = dm2(2,1)
+pout%sc1 = sc(1,1); pout%sc2 = sc(2,1); pout%sc3 = sc(3,1)
+pout%nknots = n; pout%efpot = 1; pout%chr = chr; pout%fct = fctv
+deallocate(xs, ys)
+end subroutine

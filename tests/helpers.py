@@ -1,0 +1,48 @@
+"""Shared builders and comparisons for the parity tests."""
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+
+from bromoc_amd import abi, configs
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def force_err(f, g):
+    """Norm-wise relative force error max|dF| / max|F|."""
+    return float(np.max(np.abs(f - g)) / max(np.max(np.abs(g)), 1e-300))
+
+
+def energy_errs(e, r, floor=1e-12):
+    """Per-term relative error; terms smaller than `floor` times the largest term are
+    compared absolutely against that scale."""
+    scale = max(abs(v) for v in r.values())
+    out = {}
+    for k in r:
+        d = abs(e[k] - r[k])
+        out[k] = d / max(abs(r[k]), floor * scale, 1e-300)
+    return out
+
+
+@functools.lru_cache(maxsize=None)
+def plain(seed=0, n_per_type=200, **kw):
+    return configs.plain_ions(seed=seed, n_per_type=n_per_type, **kw)
+
+
+@functools.lru_cache(maxsize=None)
+def pore_small(seed=5, n_bp=8, n_ions=200, dna=True):
+    return configs.pore_dna(seed=seed, n_bp=n_bp, n_ions=n_ions, box=(40.0, 40.0, 60.0),
+                            dcel=0.5, rpore=12.0, half_slab=10.0, dna=dna)
+
+
+def with_fixed(s, nfix, nonfree=()):
+    """Copy of s whose first nfix ions are fixed and whose listed sites are not free."""
+    t = s.copy()
+    t.nfix = nfix
+    for k in nonfree:
+        t.stfree[k] = 0
+    return t
