@@ -152,6 +152,7 @@ def _declare(lib):
         "bromoc_build_table": ([C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp, P(Pair)],
                                C.c_int),
         "bromoc_set_ibuffer": ([vp, _ip], C.c_int),
+        "bromoc_get_tuning": ([vp, _ip], C.c_int),
         "bromoc_time_steps": ([vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_uint64,
                                _dp, _dp], C.c_int),
         "bromoc_get_ibuffer": ([vp, _ip], C.c_int),

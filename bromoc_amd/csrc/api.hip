@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -19,6 +20,8 @@
 using namespace bromoc;
 
 namespace {
+
+constexpr int GRAPH_K = 32;   // BD steps per captured graph (even)
 
 constexpr uint32_t SUPPORTED = BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BROMOC_QNOBOND |
                                BROMOC_QPHIX | BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV |
@@ -43,18 +46,28 @@ struct DBuf {
 struct bromoc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;          // ion targets run here, beside the site work
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     bromoc_system sys{};       // scalars (pointer members are not kept)
     int cap = 0, ntot = 0, ns = 0, nf = 0;
     // device state
-    double *x[2] = {nullptr, nullptr}, *y[2] = {nullptr, nullptr}, *z[2] = {nullptr, nullptr};
+    double4 *pos[2] = {nullptr, nullptr};   // (x, y, z, atnam2 type | index) ping-pong buffers
+    double4 *spos[2] = {nullptr, nullptr};  // the same, in spatial order
+    SortBufs sb{};
+    bool sort_dirty = true;
+    int since_sort = 0;
+    double r2near = 0.0;
     int cur = 0;
+    int tpb = 0;                             // targets per block chosen by the autotuner
+    int32_t *warn_scratch = nullptr;
     int32_t *typei = nullptr, *t2 = nullptr, *namsite = nullptr, *nwtype = nullptr;
     int8_t *stfree = nullptr;
     double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    double4 *fdna = nullptr;
     double *epart = nullptr, *ered = nullptr;
     int32_t *pairidx = nullptr;
-    DevPair *pairs = nullptr;
+    DevPair *pairs = nullptr, *rows = nullptr;
     double4 *ep = nullptr;
     double *cg = nullptr, *fact1a = nullptr, *fact2a = nullptr, *scal = nullptr, *zcont = nullptr;
     int32_t *inc_ptr = nullptr;
@@ -68,6 +81,8 @@ struct bromoc_ctx {
     int noise_rows = 0;
     int64_t noise_stride = 0;
     int64_t *ctl = nullptr;     // [step, row0, counter0, seed]
+    int64_t h_ctl[4] = {0, 0, 0, 0};   // host source of the ctl upload (outlives the copy)
+    double4 h_move{};                  // staging of one particle moved by GRAND
     double *iout = nullptr;
     std::vector<void *> allocs;
     // host mirrors
@@ -80,6 +95,13 @@ struct bromoc_ctx {
     std::vector<double> avnum, mu, kb, lzmin, lzmax;
     std::vector<int8_t> bufferbias;
     StepParams base{};
+    // captured K-step graphs, one per starting buffer parity
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        int ntot = -1, tpb = -1, noise_mode = -1;
+        const double *noise = nullptr;
+        int64_t stride = -1;
+    } graph[2];
 };
 
 namespace {
@@ -229,6 +251,15 @@ int validate(const bromoc_system *s, std::string &msg) {
     return BROMOC_OK;
 }
 
+// the atnam2 type rides in the low word of the fourth component of each position, the
+// particle's own index in the high word
+double pack_type(int32_t t2, int32_t i) {
+    uint64_t bits = (uint64_t)(uint32_t)t2 | ((uint64_t)(uint32_t)i << 32);
+    double d;
+    std::memcpy(&d, &bits, sizeof(d));
+    return d;
+}
+
 GridGeom geom_of(const bromoc_grid &g, bool on) {
     GridGeom r{};
     r.nx = g.nclx; r.ny = g.ncly; r.nz = g.nclz; r.on = on;
@@ -246,9 +277,24 @@ StepParams params_for(bromoc_ctx *c) {
     StepParams p = c->base;
     p.ntot = c->ntot;
     p.ntargets = c->ns + (c->ntot - c->ns - c->nf);
-    p.x = c->x[c->cur]; p.y = c->y[c->cur]; p.z = c->z[c->cur];
-    p.xo = c->x[c->cur ^ 1]; p.yo = c->y[c->cur ^ 1]; p.zo = c->z[c->cur ^ 1];
+    p.tbeg = 0;
+    p.tend = p.ntargets;
+    p.pos = c->pos[c->cur];
+    p.poso = c->pos[c->cur ^ 1];
+    p.spos = c->spos[c->cur];
+    p.sposo = c->spos[c->cur ^ 1];
+    p.tpb = c->tpb > 0 ? c->tpb : 4;
     return p;
+}
+
+// Re-sort the partner array when positions were replaced or after GRAPH_K steps.
+int ensure_sorted(bromoc_ctx *c, bool force = false) {
+    if (!force && !c->sort_dirty && c->since_sort < GRAPH_K) return BROMOC_OK;
+    HIPCHK(c, sort_partners(c->sb, c->pos[c->cur], c->ntot, c->spos[c->cur], c->spos[c->cur ^ 1],
+                            c->stream));
+    c->sort_dirty = false;
+    c->since_sort = 0;
+    return BROMOC_OK;
 }
 
 int sync_err(bromoc_ctx *c) {
@@ -277,7 +323,13 @@ int bromoc_destroy(bromoc_ctx *c) {
     if (!c) return BROMOC_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    for (auto &g : c->graph)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (void *p : c->allocs) (void)hipFree(p);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return BROMOC_OK;
@@ -312,16 +364,11 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     auto cleanup = [&](int code) { c->err = c->err.empty() ? "create failed" : c->err; bromoc_destroy(c); return code; };
     if (hipSetDevice(device) != hipSuccess) return cleanup(BROMOC_E_HIP);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(BROMOC_E_HIP);
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) return cleanup(BROMOC_E_HIP);
+    if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) return cleanup(BROMOC_E_HIP);
+    if (hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) return cleanup(BROMOC_E_HIP);
     const int cap = c->cap, ns = c->ns, nt1 = s->nttyp + 1;
 #define ACHK(expr) do { if ((expr) != hipSuccess) return cleanup(BROMOC_E_HIP); } while (0)
-    for (int b = 0; b < 2; b++) {
-        ACHK(dalloc(c, &c->x[b], cap));
-        ACHK(dalloc(c, &c->y[b], cap));
-        ACHK(dalloc(c, &c->z[b], cap));
-        ACHK(hipMemcpy(c->x[b], s->x, sizeof(double) * s->ntot, hipMemcpyHostToDevice));
-        ACHK(hipMemcpy(c->y[b], s->y, sizeof(double) * s->ntot, hipMemcpyHostToDevice));
-        ACHK(hipMemcpy(c->z[b], s->z, sizeof(double) * s->ntot, hipMemcpyHostToDevice));
-    }
     c->h_typei.assign(s->typei, s->typei + s->ntot);
     c->h_typei.resize(cap, 0);
     c->h_nwtype.assign(s->nwtype, s->nwtype + s->ntype + 1);
@@ -330,6 +377,32 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     std::vector<int32_t> t2(cap, 0);
     for (int i = 0; i < s->ntot; i++)
         t2[i] = i < ns ? s->typtyp[i] : s->nwtype[std::abs(s->typei[i])];
+    {
+        std::vector<double4> p4(cap, make_double4(0.0, 0.0, 0.0, 0.0));
+        for (int i = 0; i < s->ntot; i++) p4[i] = make_double4(s->x[i], s->y[i], s->z[i], pack_type(t2[i], i));
+        for (int b = 0; b < 2; b++) ACHK(dupload(c, &c->pos[b], p4.data(), cap));
+    }
+    for (int b = 0; b < 2; b++) ACHK(dalloc(c, &c->spos[b], cap));
+    ACHK(dalloc(c, &c->sb.keys, cap));
+    ACHK(dalloc(c, &c->sb.keys_out, cap));
+    ACHK(dalloc(c, &c->sb.vals, cap));
+    ACHK(dalloc(c, &c->sb.order, cap));
+    ACHK(dalloc(c, &c->sb.rank, cap));
+    c->sb.temp_bytes = sort_temp_bytes(cap);
+    ACHK(dalloc(c, (char **)&c->sb.temp, c->sb.temp_bytes));
+    {
+        // Morton cells of 8 A over the box (10 bits per axis)
+        SortGeom &g = c->sb.geom;
+        const double lo[3] = {s->lx2m, s->ly2m, s->lz2m}, hi[3] = {s->lx2p, s->ly2p, s->lz2p};
+        const double cell = 8.0;
+        g.inv = 1.0 / cell;
+        for (int k = 0; k < 3; k++) {
+            g.lo[k] = lo[k];
+            const double ext = hi[k] - lo[k];
+            int n = ext > 0.0 ? (int)std::ceil(ext / cell) : 1;
+            g.n[k] = std::min(std::max(n, 1), 1024);
+        }
+    }
     ACHK(dupload(c, &c->typei, c->h_typei.data(), cap));
     ACHK(dupload(c, &c->t2, t2.data(), cap));
     ACHK(dupload(c, &c->stfree, ns ? s->stfree : (const int8_t *)nullptr, ns));
@@ -341,6 +414,7 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     ACHK(dalloc(c, &c->fx, cap));
     ACHK(dalloc(c, &c->fy, cap));
     ACHK(dalloc(c, &c->fz, cap));
+    ACHK(dalloc(c, &c->fdna, std::max(ns, 1)));
     ACHK(dalloc(c, &c->epart, (size_t)NE * cap));
     ACHK(dalloc(c, &c->ered, NE));
     ACHK(dupload(c, &c->pairidx, s->pairidx, (size_t)nt1 * nt1));
@@ -355,10 +429,17 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
         d.epp4 = p.epp4; d.sgp2 = p.sgp2;
         d.c0 = p.c0; d.c1 = p.c1; d.c2 = p.c2; d.c3 = p.c3; d.c4 = p.c4;
         d.nxi = p.nxi;
+        d.nlast = p.efpot ? p.nxi + p.nknots - 1 : p.nxi;
+        if (!p.chr) d.fct = 0.0;   // the table tail adds fct/r for charged pairs only
         d.flags = (p.efpot ? PF_EFPOT : 0) | (p.chr ? PF_CHR : 0) | (p.lj ? PF_LJ : 0) |
                   (p.srpmf ? PF_SRPMF : 0);
     }
     ACHK(dupload(c, &c->pairs, dp.data(), dp.size()));
+    {
+        std::vector<DevPair> rows((size_t)nt1 * nt1);
+        for (int k = 0; k < nt1 * nt1; k++) rows[k] = dp[s->pairidx[k]];
+        ACHK(dupload(c, &c->rows, rows.data(), rows.size()));
+    }
     // two zero knots of padding: a lookup that rounds one knot past a table's end (a 1-ulp
     // difference in sqrt at x = r_n) stays inside the allocation
     std::vector<double4> ep4(std::max(s->nknots, 0) + 2, make_double4(0.0, 0.0, 0.0, 0.0));
@@ -402,6 +483,7 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     if ((s->flags & BROMOC_QSVDW) && s->scal) scal.assign(s->scal, s->scal + nt1);
     ACHK(dupload(c, &c->scal, scal.data(), nt1));
     ACHK(dalloc(c, &c->warn, nt1));
+    ACHK(dalloc(c, &c->warn_scratch, nt1));
     const int nion_t = std::max(1, s->ntype - s->nold), np = std::max(1, s->cntpts);
     ACHK(dalloc(c, &c->nfwd, (size_t)nion_t * np));
     ACHK(dalloc(c, &c->nbwd, (size_t)nion_t * np));
@@ -417,8 +499,9 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.flags = s->flags;
     p.typei = c->typei; p.t2 = c->t2; p.stfree = c->stfree; p.namsite = c->namsite;
     p.fx = c->fx; p.fy = c->fy; p.fz = c->fz;
+    p.fdna = c->fdna;
     p.epart = c->epart; p.ecap = cap;
-    p.pairidx = c->pairidx; p.pairs = c->pairs; p.ep = c->ep;
+    p.pairidx = c->pairidx; p.pairs = c->pairs; p.rows = c->rows; p.ep = c->ep;
     p.ires = s->ires; p.celec = celec_value();
     p.cg = c->cg; p.fact1a = c->fact1a; p.fact2a = c->fact2a; p.nwtype = c->nwtype;
     p.rth = s->rth; p.srpx = s->srpx; p.srpk = s->srpk; p.srpy = s->srpy;
@@ -439,6 +522,17 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.zcont = c->zcont; p.nfwd = c->nfwd; p.nbwd = c->nbwd;
     p.step_ctr = c->ctl;
     p.stream = 0;
+    p.all_efpot = 1;
+    c->r2near = 0.0;
+    for (int k = 0; k < nt1 * nt1; k++) {
+        const int is = s->pairidx[k];
+        if (is > 0 && !s->pairs[is].efpot) p.all_efpot = 0;
+        if (is > 0 && s->pairs[is].efpot) c->r2near = std::max(c->r2near, s->pairs[is].dm2hi);
+    }
+    // the analytic-tail fast path needs every pair tabulated
+    if (!p.all_efpot) c->r2near = HUGE_VAL;
+    p.r2near = c->r2near;
+    p.rank = c->sb.rank;
     *out = c;
     return BROMOC_OK;
 }
@@ -462,15 +556,16 @@ int bromoc_set_positions(bromoc_ctx *c, int32_t ntot, const double *x, const dou
     }
     for (int i = 0; i < ntot; i++)
         t2[i] = i < c->ns ? c->h_typtyp[i] : c->h_nwtype[std::abs(c->h_typei[i])];
-    for (int b = 0; b < 2; b++) {
-        HIPCHK(c, hipMemcpyAsync(c->x[b], x, sizeof(double) * ntot, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->y[b], y, sizeof(double) * ntot, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->z[b], z, sizeof(double) * ntot, hipMemcpyHostToDevice, c->stream));
-    }
+    std::vector<double4> p4(ntot);
+    for (int i = 0; i < ntot; i++) p4[i] = make_double4(x[i], y[i], z[i], pack_type(t2[i], i));
+    for (int b = 0; b < 2; b++)
+        HIPCHK(c, hipMemcpyAsync(c->pos[b], p4.data(), sizeof(double4) * ntot, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->typei, c->h_typei.data(), sizeof(int32_t) * ntot, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->t2, t2.data(), sizeof(int32_t) * ntot, hipMemcpyHostToDevice, c->stream));
+    int st = sync_err(c);
     c->ntot = ntot;
-    return sync_err(c);
+    c->sort_dirty = true;
+    return st;
 }
 
 int bromoc_get_positions(bromoc_ctx *c, double *x, double *y, double *z, int32_t *typei,
@@ -478,10 +573,17 @@ int bromoc_get_positions(bromoc_ctx *c, double *x, double *y, double *z, int32_t
     if (!c) return BROMOC_E_ARG;
     (void)hipSetDevice(c->device);
     const size_t n = c->ntot;
-    if (x) HIPCHK(c, hipMemcpyAsync(x, c->x[c->cur], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    if (y) HIPCHK(c, hipMemcpyAsync(y, c->y[c->cur], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    if (z) HIPCHK(c, hipMemcpyAsync(z, c->z[c->cur], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    int st = sync_err(c);
+    int st = BROMOC_OK;
+    if (x || y || z) {
+        std::vector<double4> p4(n);
+        HIPCHK(c, hipMemcpyAsync(p4.data(), c->pos[c->cur], sizeof(double4) * n, hipMemcpyDeviceToHost, c->stream));
+        st = sync_err(c);
+        for (size_t i = 0; i < n; i++) {
+            if (x) x[i] = p4[i].x;
+            if (y) y[i] = p4[i].y;
+            if (z) z[i] = p4[i].z;
+        }
+    }
     if (typei) std::memcpy(typei, c->h_typei.data(), sizeof(int32_t) * n);
     if (ntot) *ntot = c->ntot;
     return st;
@@ -500,6 +602,8 @@ int bromoc_get_forces(bromoc_ctx *c, double *fx, double *fy, double *fz) {
 int bromoc_energy(bromoc_ctx *c, int32_t want_forces, bromoc_energies *out) {
     if (!c) return BROMOC_E_ARG;
     (void)hipSetDevice(c->device);
+    int sst = ensure_sorted(c);
+    if (sst) return sst;
     StepParams p = params_for(c);
     p.mode = 0;
     p.write_energy = 1;
@@ -529,6 +633,90 @@ int bromoc_upload_noise(bromoc_ctx *c, int32_t nsteps, const double *noise) {
     return BROMOC_OK;
 }
 
+// One BD step.  Site targets (with the CG-DNA list kernel before them) and ion targets
+// are independent within a step -- each reads the old positions and writes its own new
+// ones -- so the ion launch runs on a second stream, forked from and joined back into the
+// main one (also inside captured graphs).  timing, when given, brackets the ion launch.
+static hipError_t enqueue_step(bromoc_ctx *c, const StepParams &p, hipEvent_t t0 = nullptr,
+                               hipEvent_t t1 = nullptr) {
+    const int ns = p.nsites, nt = p.ntargets;
+    hipError_t e;
+    static const bool fork = [] {
+        const char *v = std::getenv("BROMOC_FORK");
+        return v && v[0] == '1';
+    }();
+    if (!fork || ns == 0 || nt == ns) {
+        if ((e = launch_dna(p, c->stream)) != hipSuccess) return e;
+        if (t0 && (e = hipEventRecord(t0, c->stream)) != hipSuccess) return e;
+        if ((e = launch_pairs(p, c->stream)) != hipSuccess) return e;
+        if (t1 && (e = hipEventRecord(t1, c->stream)) != hipSuccess) return e;
+        return hipSuccess;
+    }
+    if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork, 0)) != hipSuccess) return e;
+    StepParams q = p;
+    q.tbeg = ns;
+    q.tend = nt;
+    if (t0 && (e = hipEventRecord(t0, c->stream2)) != hipSuccess) return e;
+    if ((e = launch_pairs(q, c->stream2)) != hipSuccess) return e;
+    if (t1 && (e = hipEventRecord(t1, c->stream2)) != hipSuccess) return e;
+    q.tbeg = 0;
+    q.tend = ns;
+    if ((e = launch_dna(q, c->stream)) != hipSuccess) return e;
+    if ((e = launch_pairs(q, c->stream)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->ev_join, c->stream2)) != hipSuccess) return e;
+    return hipStreamWaitEvent(c->stream, c->ev_join, 0);
+}
+
+// Autotuner: targets per block of k_step (1, 2 or 4) for this system, chosen by timing
+// ENERGY-only launches (warnings go to a scratch counter) on the ctx's stream.  The
+// environment variable BROMOC_TPB forces a value.
+static int autotune(bromoc_ctx *c) {
+    if (c->tpb > 0) return BROMOC_OK;
+    if (const char *env = std::getenv("BROMOC_TPB")) {
+        int v = std::atoi(env);
+        if (v == 1 || v == 2 || v == 4) { c->tpb = v; return BROMOC_OK; }
+    }
+    int sst = ensure_sorted(c);
+    if (sst) return sst;
+    const int cand[3] = {1, 2, 4};
+    double best = 1e300;
+    int pick = 4;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    for (int t : cand) {
+        StepParams p = params_for(c);
+        p.tpb = t;
+        p.mode = 0;
+        p.write_energy = 0;
+        p.write_forces = 0;
+        p.warn = c->warn_scratch;
+        for (int r = 0; r < 2; r++) HIPCHK(c, launch_step(p, c->stream));
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        for (int r = 0; r < 5; r++) HIPCHK(c, launch_step(p, c->stream));
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        int st = sync_err(c);
+        if (st) return st;
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; pick = t; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tpb = pick;
+    return BROMOC_OK;
+}
+
+int bromoc_get_tuning(bromoc_ctx *c, int32_t *targets_per_block) {
+    if (!c || !targets_per_block) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    int st = autotune(c);
+    if (st) return st;
+    *targets_per_block = c->tpb;
+    return BROMOC_OK;
+}
+
 static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t first_row,
                      uint64_t seed, uint64_t counter0, bool want_energy) {
     if (noise_mode != BROMOC_NOISE_INJECTED && noise_mode != BROMOC_NOISE_PHILOX)
@@ -540,22 +728,77 @@ static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t 
         if (3 * (int64_t)c->ntot > c->noise_stride)
             return fail(c, BROMOC_E_STATE, "ntot grew beyond the uploaded noise rows");
     }
-    int64_t ctl[4] = {0, first_row, (int64_t)counter0, (int64_t)seed};
-    HIPCHK(c, hipMemcpyAsync(c->ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice, c->stream));
+    int st0 = autotune(c);
+    if (st0) return st0;
+    c->h_ctl[0] = 0; c->h_ctl[1] = first_row; c->h_ctl[2] = (int64_t)counter0; c->h_ctl[3] = (int64_t)seed;
+    HIPCHK(c, hipMemcpyAsync(c->ctl, c->h_ctl, sizeof(c->h_ctl), hipMemcpyHostToDevice, c->stream));
     StepParams p = params_for(c);
     p.mode = 1;
     p.noise_mode = noise_mode;
     p.noise = c->noise;
     p.noise_stride = c->noise_stride;
-    for (int s = 0; s < nsteps; s++) {
+    int done = 0;
+    static const bool use_graph = [] {
+        const char *e = std::getenv("BROMOC_GRAPH");
+        return !(e && e[0] == '0');
+    }();
+    // whole K-step chunks replay a captured graph; the last step (which writes forces and
+    // energies) and any remainder are launched directly
+    while (use_graph && nsteps - done > GRAPH_K) {
+        int sst = ensure_sorted(c, true);
+        if (sst) return sst;
+        bromoc_ctx::Graph &g = c->graph[c->cur];
+        if (!g.exec || g.ntot != c->ntot || g.tpb != c->tpb || g.noise_mode != noise_mode ||
+            g.noise != c->noise || g.stride != c->noise_stride) {
+            if (g.exec) {
+                (void)hipGraphExecDestroy(g.exec);
+                g.exec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
+            StepParams q = p;
+            q.write_energy = 0;
+            q.write_forces = 0;
+            hipError_t le = hipSuccess;
+            for (int k = 0; k < GRAPH_K && le == hipSuccess; k++) {
+                q.pos = c->pos[c->cur ^ (k & 1)];
+                q.poso = c->pos[c->cur ^ (k & 1) ^ 1];
+                q.spos = c->spos[c->cur ^ (k & 1)];
+                q.sposo = c->spos[c->cur ^ (k & 1) ^ 1];
+                q.step_off = k;
+                le = enqueue_step(c, q);
+            }
+            if (le == hipSuccess) le = launch_advance(c->ctl, GRAPH_K, c->stream);
+            hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(c, BROMOC_E_HIP, std::string("graph capture: ") +
+                                                 hipGetErrorString(le != hipSuccess ? le : ce));
+            }
+            hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return fail(c, BROMOC_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+            g.ntot = c->ntot; g.tpb = c->tpb; g.noise_mode = noise_mode;
+            g.noise = c->noise; g.stride = c->noise_stride;
+        }
+        HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+        done += GRAPH_K;   // K is even: the buffer parity is unchanged
+        c->since_sort += GRAPH_K;
+    }
+    for (int s = done; s < nsteps; s++) {
         const bool last = s == nsteps - 1;
-        p.x = c->x[c->cur]; p.y = c->y[c->cur]; p.z = c->z[c->cur];
-        p.xo = c->x[c->cur ^ 1]; p.yo = c->y[c->cur ^ 1]; p.zo = c->z[c->cur ^ 1];
-        p.step_off = s;
+        int sst = ensure_sorted(c);
+        if (sst) return sst;
+        p.pos = c->pos[c->cur];
+        p.poso = c->pos[c->cur ^ 1];
+        p.spos = c->spos[c->cur];
+        p.sposo = c->spos[c->cur ^ 1];
+        p.step_off = s - done;
         p.write_energy = (last && want_energy) ? 1 : 0;
         p.write_forces = last ? 1 : 0;
-        HIPCHK(c, launch_step(p, c->stream));
+        HIPCHK(c, enqueue_step(c, p));
         c->cur ^= 1;
+        c->since_sort++;
     }
     return BROMOC_OK;
 }
@@ -571,8 +814,9 @@ int bromoc_step(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t first
     return sync_err(c);
 }
 
-// nsteps BD steps with a HIP event pair around every launch, on the stream the kernels run
-// on: *total_ms is first-to-last, *kernel_ms the mean launch duration.
+// nsteps BD steps with a HIP event pair around every launch of the step kernel k_step, on
+// the stream the kernels run on: *total_ms is first-to-last, *kernel_ms the mean k_step
+// duration (k_dna, launched before each k_step for CG-DNA systems, is outside the pair).
 int bromoc_time_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t first_row,
                       uint64_t seed, uint64_t counter0, double *total_ms, double *kernel_ms) {
     if (!c || nsteps <= 0 || !total_ms || !kernel_ms) return BROMOC_E_ARG;
@@ -584,23 +828,28 @@ int bromoc_time_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t
             3 * (int64_t)c->ntot > c->noise_stride)
             return fail(c, BROMOC_E_STATE, "noise rows out of range");
     }
-    int64_t ctl[4] = {0, first_row, (int64_t)counter0, (int64_t)seed};
-    HIPCHK(c, hipMemcpyAsync(c->ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice, c->stream));
+    int st0 = autotune(c);
+    if (st0) return st0;
+    c->h_ctl[0] = 0; c->h_ctl[1] = first_row; c->h_ctl[2] = (int64_t)counter0; c->h_ctl[3] = (int64_t)seed;
+    HIPCHK(c, hipMemcpyAsync(c->ctl, c->h_ctl, sizeof(c->h_ctl), hipMemcpyHostToDevice, c->stream));
     StepParams p = params_for(c);
     p.mode = 1;
     p.noise_mode = noise_mode;
     p.noise = c->noise;
     p.noise_stride = c->noise_stride;
     for (int s = 0; s < nsteps; s++) {
-        p.x = c->x[c->cur]; p.y = c->y[c->cur]; p.z = c->z[c->cur];
-        p.xo = c->x[c->cur ^ 1]; p.yo = c->y[c->cur ^ 1]; p.zo = c->z[c->cur ^ 1];
+        int sst = ensure_sorted(c);
+        if (sst) return sst;
+        p.pos = c->pos[c->cur];
+        p.poso = c->pos[c->cur ^ 1];
+        p.spos = c->spos[c->cur];
+        p.sposo = c->spos[c->cur ^ 1];
         p.step_off = s;
         p.write_energy = 0;
         p.write_forces = s == nsteps - 1;
-        HIPCHK(c, hipEventRecord(ev[2 * s], c->stream));
-        HIPCHK(c, launch_step(p, c->stream));
-        HIPCHK(c, hipEventRecord(ev[2 * s + 1], c->stream));
+        HIPCHK(c, enqueue_step(c, p, ev[2 * s], ev[2 * s + 1]));
         c->cur ^= 1;
+        c->since_sort++;
     }
     int st = sync_err(c);
     if (st) return st;
@@ -611,7 +860,7 @@ int bromoc_time_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t
         sum += ms;
     }
     float tot = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&tot, ev[0], ev[2 * nsteps - 1]));
+    HIPCHK(c, hipEventElapsedTime(&tot, ev[0], ev[2 * nsteps - 1]));   // ion launches
     for (auto &e : ev) (void)hipEventDestroy(e);
     *total_ms = tot;
     *kernel_ms = sum / nsteps;
@@ -764,11 +1013,9 @@ int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int6
                     const int n = c->ntot;
                     const int32_t ti = zn < g.cz ? -itype : itype;
                     const int32_t t2 = c->h_nwtype[itype];
-                    for (int bb = 0; bb < 2; bb++) {
-                        HIPCHK(c, hipMemcpyAsync(c->x[bb] + n, &xn, sizeof(double), hipMemcpyHostToDevice, c->stream));
-                        HIPCHK(c, hipMemcpyAsync(c->y[bb] + n, &yn, sizeof(double), hipMemcpyHostToDevice, c->stream));
-                        HIPCHK(c, hipMemcpyAsync(c->z[bb] + n, &zn, sizeof(double), hipMemcpyHostToDevice, c->stream));
-                    }
+                    c->h_move = make_double4(xn, yn, zn, pack_type(t2, n));
+                    for (int bb = 0; bb < 2; bb++)
+                        HIPCHK(c, hipMemcpyAsync(c->pos[bb] + n, &c->h_move, sizeof(double4), hipMemcpyHostToDevice, c->stream));
                     HIPCHK(c, hipMemcpyAsync(c->typei + n, &ti, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
                     HIPCHK(c, hipMemcpyAsync(c->t2 + n, &t2, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
                     int st2 = sync_err(c);
@@ -776,6 +1023,7 @@ int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int6
                     c->h_typei[n] = ti;
                     c->ibuffer[n] = ib;
                     c->ntot = n + 1;
+                    c->sort_dirty = true;
                 }
             }
         } else {
@@ -791,13 +1039,11 @@ int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int6
                 for (int a = c->nf + c->ns; a < c->ntot; a++)
                     if (c->ibuffer[a] == ib && ++ipx == ip) { iat = a; break; }
                 if (iat < 0) return fail(c, BROMOC_E_STATE, "buffer bookkeeping inconsistent");
-                double xa, ya, za;
-                HIPCHK(c, hipMemcpyAsync(&xa, c->x[c->cur] + iat, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(c, hipMemcpyAsync(&ya, c->y[c->cur] + iat, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(c, hipMemcpyAsync(&za, c->z[c->cur] + iat, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                double4 pa;
+                HIPCHK(c, hipMemcpyAsync(&pa, c->pos[c->cur] + iat, sizeof(double4), hipMemcpyDeviceToHost, c->stream));
                 int st = sync_err(c);
                 if (st) return st;
-                st = bromoc_interact(c, xa, ya, za, itype, iat, 1, &dener);
+                st = bromoc_interact(c, pa.x, pa.y, pa.z, itype, iat, 1, &dener);
                 if (st) return st;
                 double rate;
                 if (c->bufferbias[ib]) {
@@ -812,11 +1058,13 @@ int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int6
                 if (r3 <= rate) {
                     *ener = *ener - dener;
                     const int last = c->ntot - 1;
-                    for (int bb = 0; bb < 2; bb++) {
-                        HIPCHK(c, hipMemcpyAsync(c->x[bb] + iat, c->x[c->cur] + last, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-                        HIPCHK(c, hipMemcpyAsync(c->y[bb] + iat, c->y[c->cur] + last, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-                        HIPCHK(c, hipMemcpyAsync(c->z[bb] + iat, c->z[c->cur] + last, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-                    }
+                    // grand.f90:102-109: the last particle takes the deleted one's slot
+                    HIPCHK(c, hipMemcpyAsync(&c->h_move, c->pos[c->cur] + last, sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+                    int st1 = sync_err(c);
+                    if (st1) return st1;
+                    c->h_move.w = pack_type(c->h_nwtype[std::abs(c->h_typei[last])], iat);
+                    for (int bb = 0; bb < 2; bb++)
+                        HIPCHK(c, hipMemcpyAsync(c->pos[bb] + iat, &c->h_move, sizeof(double4), hipMemcpyHostToDevice, c->stream));
                     HIPCHK(c, hipMemcpyAsync(c->typei + iat, c->typei + last, sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
                     HIPCHK(c, hipMemcpyAsync(c->t2 + iat, c->t2 + last, sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
                     int st2 = sync_err(c);
@@ -825,6 +1073,7 @@ int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int6
                     c->ibuffer[iat] = c->ibuffer[last];
                     nat[ib] = nat[ib] - 1;
                     c->ntot = last;
+                    c->sort_dirty = true;
                 }
             }
         }

@@ -22,7 +22,7 @@ constexpr int PF_EFPOT = 1, PF_CHR = 2, PF_LJ = 4, PF_SRPMF = 8;
 struct DevPair {
     double dmi, dm2lo, dm2hi, sc1, sc2, sc3, fct, epp4;
     double sgp2, c0, c1, c2, c3, c4;
-    int32_t nxi, flags;
+    int32_t nxi, nlast, flags, pad;   // first and last knot of the table
 };
 
 struct GridGeom {
@@ -47,19 +47,29 @@ struct StepParams {
     uint32_t flags;
     int32_t mode;            // 0: ENERGY (forces out, no update), 1: ENERGY + DYNAMICS
     int32_t write_energy, write_forces;
-    // positions (in) and updated positions (out)
-    const double *x, *y, *z;
-    double *xo, *yo, *zo;
+    int32_t tpb, all_efpot;  // targets per block of k_step (1, 2 or 4); every pair tabulated
+    int32_t tbeg, tend;      // target range of this launch (sites 0..nsites-1 come first)
+    // positions (x, y, z, atnam2 type in the low word of w) in, updated positions out
+    const double4 *pos;
+    double4 *poso;
+    // the same particles in spatial (partner) order, w = type | index << 32, and the slot
+    // of each particle in that order
+    const double4 *spos;
+    double4 *sposo;
+    const int32_t *rank;
+    double r2near;           // squared range beyond which every table pair is in its tail
     const int32_t *typei;    // signed atnam type
     const int32_t *t2;       // atnam2 type per particle
     const int8_t *stfree;    // [nsites]
     const int32_t *namsite;  // [nsites]
     double *fx, *fy, *fz;
+    double4 *fdna;           // [nsites] CG-DNA list-term force of each site (k_dna)
     double *epart;           // [NE][ecap]
     int32_t ecap, pad0;
     // tables
     const int32_t *pairidx;  // [(nttyp+1)^2]
     const DevPair *pairs;    // [npair+1]
+    const DevPair *rows;     // [(nttyp+1)^2] pairs[pairidx[a*(nttyp+1)+b]]
     const double4 *ep;       // knots (ep1, ep2, ep3, 0)
     double ires, celec;
     const double *cg, *fact1a, *fact2a;
@@ -94,8 +104,27 @@ struct StepParams {
     uint64_t stream;
 };
 
+// spatial partner order (sort.hip)
+struct SortGeom {
+    double lo[3];
+    double inv;        // 1 / cell size
+    int32_t n[3], pad;
+};
+struct SortBufs {
+    SortGeom geom;
+    uint32_t *keys, *keys_out;
+    int32_t *vals, *order, *rank;
+    void *temp;
+    size_t temp_bytes;
+};
+size_t sort_temp_bytes(int n);
+hipError_t sort_partners(const SortBufs &b, const double4 *pos, int n, double4 *spos0,
+                         double4 *spos1, hipStream_t s);
+
 // launchers (kernels.hip)
-hipError_t launch_step(const StepParams &p, hipStream_t s);
+hipError_t launch_step(const StepParams &p, hipStream_t s);   // launch_dna + launch_pairs
+hipError_t launch_dna(const StepParams &p, hipStream_t s);    // k_dna (CG-DNA list terms)
+hipError_t launch_pairs(const StepParams &p, hipStream_t s);  // k_step (pairs, grids, update)
 hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
                                 hipStream_t s);
 hipError_t launch_advance(int64_t *ctr, int64_t by, hipStream_t s);

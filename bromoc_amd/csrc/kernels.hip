@@ -20,7 +20,6 @@ namespace bromoc {
 constexpr int WAVE = 64;
 constexpr int BLOCK = 256;
 constexpr int WPB = BLOCK / WAVE;
-constexpr int TILE = BLOCK;
 
 // feature bits of the k_step specialisations
 constexpr int F_DNA = 1;     // CG-DNA sites and lists present
@@ -200,7 +199,10 @@ __device__ __forceinline__ void grid_cell(const GridGeom &g, double x, double y,
 }
 
 struct V3 { double x, y, z; };
-__device__ __forceinline__ V3 ld3(const StepParams &P, int i) { return {P.x[i], P.y[i], P.z[i]}; }
+__device__ __forceinline__ V3 ld3(const StepParams &P, int i) {
+    const double4 q = P.pos[i];
+    return {q.x, q.y, q.z};
+}
 __device__ __forceinline__ V3 sub3(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ double dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ V3 cross3(V3 u, V3 v) {
@@ -208,8 +210,14 @@ __device__ __forceinline__ V3 cross3(V3 u, V3 v) {
 }
 __device__ __forceinline__ V3 scl3(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
 
+// slots of the DNA/grid energy accumulator and the energy categories they feed
+constexpr int ND = 10;
+__device__ constexpr int ND_CAT[ND] = {E_BOND, E_ANG, E_DIHE, E_STACK, E_BP, E_EX, E_QQ, E_SOLV,
+                                       E_GSBPA, E_VDWGD};
+enum { D_BOND = 0, D_ANG, D_DIHE, D_STACK, D_BP, D_EX, D_QQ, D_SOLV, D_GSBPA, D_VDWGD };
+
 // One DNA term incident on site `me` in role `role`: force on `me` (added to f) and the
-// term's energy (returned in e when role == 0).  energy.mono.f90:205-507.
+// term's energy (added to e when role == 0).  energy.mono.f90:205-507.
 __device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, double *e) {
     const int kind = w >> 29, role = (w >> 27) & 3, t = w & 0x7FFFFFF;
     const double eps = P.epsnuc;
@@ -221,7 +229,7 @@ __device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, d
         double dist = sqrt(dot3(v1, v1));
         double vard = dist - P.lval[0][t];
         double vard2 = vard * vard;
-        if (role == 0) e[E_BOND] += eps * vard2 * (1.0 + 100.0 * vard2);
+        if (role == 0) e[D_BOND] += eps * vard2 * (1.0 + 100.0 * vard2);
         if (free_me) {
             double de = eps * vard * (2.0 + 400.0 * vard2) / dist;
             V3 f1 = scl3(de, v1);
@@ -236,7 +244,7 @@ __device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, d
         double modval = 1.0 / sqrt(m1 * m2);
         double ang = acos(m3 * modval);
         double varang = ang - P.lval[1][t];
-        if (role == 0) e[E_ANG] += 700.0 * eps * (varang * varang);
+        if (role == 0) e[D_ANG] += 700.0 * eps * (varang * varang);
         if (free_me) {
             double de = 1400.0 * eps * varang * modval / sin(ang);
             V3 f1 = scl3(de, v1), f2 = scl3(de, v2);
@@ -254,7 +262,7 @@ __device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, d
         double av = dot3(v1, n2), bv = dot3(n1, n2);
         double dihe = atan2(v2v * av, bv);
         double vardihe = dihe - P.lval[2][t];
-        if (role == 0) e[E_DIHE] += 28.0 * eps * (1.0 - cos(vardihe));
+        if (role == 0) e[D_DIHE] += 28.0 * eps * (1.0 - cos(vardihe));
         if (free_me) {
             double n1v = dot3(n1, n1), n2v = dot3(n2, n2);
             double v12 = dot3(v1, v2), v23 = dot3(v2, v3);
@@ -282,7 +290,7 @@ __device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, d
             double idist2 = 1.0 / dist2;
             double sg = P.lval[3][t];
             double cte1 = pow3(sg * sg * idist2);
-            if (role == 0) e[E_STACK] += 4.0 * eps * cte1 * (cte1 - 1.0);
+            if (role == 0) e[D_STACK] += 4.0 * eps * cte1 * (cte1 - 1.0);
             de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
         } else if (kind == T_BP) {
             int na = P.namsite[a];
@@ -292,7 +300,7 @@ __device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, d
             double idist2 = 1.0 / dist2;
             double sg = P.lval[4][t];
             double cte1 = sg * sg * idist2;
-            if (role == 0) e[E_BP] += epsln * pow5(cte1) * (20.0 * cte1 - 24.0);
+            if (role == 0) e[D_BP] += epsln * pow5(cte1) * (20.0 * cte1 - 24.0);
             de = 240.0 * epsln * pow5(cte1) * (cte1 - 1.0) * idist2;
         } else if (kind == T_EX) {
             double dist2 = dot3(v1, v1);
@@ -300,7 +308,7 @@ __device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, d
             double sgex2 = P.lval[5][t] * P.lval[5][t];
             if (dist2 < sgex2) {
                 double cte1 = pow3(sgex2 * idist2);
-                if (role == 0) e[E_EX] += 4.0 * eps * cte1 * (cte1 - 1.0) + eps;
+                if (role == 0) e[D_EX] += 4.0 * eps * cte1 * (cte1 - 1.0) + eps;
                 de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
             } else {
                 force = false;
@@ -310,16 +318,16 @@ __device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, d
             double idist = 1.0 / dist;
             if (P.flags & QDEBY) {
                 double idistkd = exp(-dist * P.ikappa);
-                if (role == 0) e[E_QQ] += P.fctn * idist * idistkd;
+                if (role == 0) e[D_QQ] += P.fctn * idist * idistkd;
                 de = P.fctn * (dist + P.kappa) * pow3(idist) * P.ikappa * idistkd;
             } else {
-                if (role == 0) e[E_QQ] += P.fctn * idist;
+                if (role == 0) e[D_QQ] += P.fctn * idist;
                 de = P.fctn * pow3(idist);
             }
         } else {  // T_SLV
             double dist = sqrt(dot3(v1, v1));
             double cte1 = exp((13.38 - dist) * 0.1875);
-            if (role == 0) e[E_SOLV] += P.epsolv * ((1.0 - cte1) * (1.0 - cte1)) - P.epsolv;
+            if (role == 0) e[D_SOLV] += P.epsolv * ((1.0 - cte1) * (1.0 - cte1)) - P.epsolv;
             de = 0.375 * P.epsolv * cte1 * (cte1 - 1.0) / dist;
         }
         if (force) {
@@ -329,102 +337,222 @@ __device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, d
     }
 }
 
-template <int FEAT>
-__global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
-    __shared__ double4 s_pos[TILE];
-    __shared__ DevPair s_pairs[MAX_LDS_PAIRS];
-    __shared__ int32_t s_pidx[MAX_LDS_TYPES * MAX_LDS_TYPES];
+// 1/sqrt(x2) to within an ulp: hardware estimate plus two Newton steps.  Every branch
+// of the table lookup derives r, 1/r and 1/r^2 from it.
+__device__ __forceinline__ double rsqrt_nr(double x2) {
+    double r = __builtin_amdgcn_rsq(x2);
+    double e = fma(-x2 * r, r, 1.0);
+    r = fma(r * e, fma(0.375, e, 0.5), r);
+    e = fma(-x2 * r, r, 1.0);
+    r = fma(r * e, 0.5, r);
+    return r;
+}
 
-    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+// Table pair (getyd, polynom.f90:197-230) from x2 and rinv = 1/sqrt(x2): wall below dmi,
+// analytic tail above the last knot, piecewise quadratic in between.
+__device__ __forceinline__ void efpot_eval(const DevPair &p, const double4 *__restrict__ ep,
+                                           double ires, double x2, double rinv, double &y,
+                                           double &yp) {
+    const double ix2 = rinv * rinv;
+    if (x2 < p.dm2lo) {
+        const double c = p.sc1 * pow6(ix2);
+        y = c + p.sc2;
+        yp = 12.0 * c * ix2;
+    } else if (x2 > p.dm2hi) {
+        const double c = p.sc3 * pow3(ix2);
+        const double f = p.fct * rinv;     // fct is zero for uncharged pairs
+        y = c + f;
+        yp = (6.0 * c + f) * ix2;
+    } else {
+        const double x = x2 * rinv;
+        int i = (int)((x - p.dmi) * ires) + p.nxi;
+        i = min(max(i, p.nxi), p.nlast);
+        const double4 e = ep[i];
+        y = e.x + x * (e.y + x * e.z);
+        yp = -e.y * rinv - 2.0 * e.z;
+    }
+}
+
+// One partner j of a target: the table pair of getyd (polynom.f90:197-230) evaluated
+// without branches.  Wall, tail and table forms are all formed from rinv = 1/sqrt(d2);
+// the knot index is clamped into the pair's table, so the gather is always in bounds and
+// is simply not selected outside [dmi, r_n].  Returns the force factor; adds the energy.
+__device__ __forceinline__ double pair_table(const DevPair &pp, const double4 *__restrict__ ep,
+                                            double ires, double d2, double &y) {
+    const double rinv = rsqrt_nr(d2);
+    const double ix2 = rinv * rinv;
+    const double x = d2 * rinv;
+    int k = (int)((x - pp.dmi) * ires) + pp.nxi;
+    k = min(max(k, pp.nxi), pp.nlast);
+    const double4 e = ep[k];
+    const double c3 = pp.sc3 * pow3(ix2);
+    const double f = pp.fct * rinv;              // zero for uncharged pairs
+    const double c12 = pp.sc1 * pow6(ix2);
+    const bool wall = d2 < pp.dm2lo;
+    const bool tail = d2 > pp.dm2hi;
+    const double yt = e.x + x * (e.y + x * e.z);
+    const double ypt = -e.y * rinv - 2.0 * e.z;
+    y = wall ? c12 + pp.sc2 : (tail ? c3 + f : yt);
+    return wall ? 12.0 * c12 * ix2 : (tail ? (6.0 * c3 + f) * ix2 : ypt);
+}
+
+// k_step: T target particles per block of 256 threads (targets tbeg.. of the launch).
+// Every thread walks the partner set with stride 256, two partners per iteration, and
+// evaluates each partner against all T targets.  Per-target sums: butterfly within each
+// wavefront, then a fixed-order sum over the 4 wavefronts in LDS.  Wavefront w < T also
+// handles target w's 8 grid corners; thread a < T finishes target a (force, energies,
+// BD update).  CG-DNA list forces of site targets come from k_dna (fdna).
+template <int FEAT, int T>
+__global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
+    __shared__ DevPair s_row[T][MAX_LDS_TYPES];   // pair parameters by partner type
+    __shared__ double2 s_tail[T][MAX_LDS_TYPES];  // (sc3, tail fct) by partner type
+    __shared__ double s_red[WPB][T][7];
+    __shared__ double s_dg[T][5];
+    __shared__ double s_g[T][3];
+    __shared__ int s_warn[T];
+
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
     const int ns = P.nsites, nf = P.nfix, ntot = P.ntot;
     const int nt1 = P.nttyp + 1;
-    for (int k = tid; k <= P.npair; k += BLOCK) s_pairs[k] = P.pairs[k];
-    for (int k = tid; k < nt1 * nt1; k += BLOCK) s_pidx[k] = P.pairidx[k];
+    const int wbase = P.tbeg + blockIdx.x * T;
 
-    const int w = blockIdx.x * WPB + (tid >> 6);
-    const bool valid = w < P.ntargets;
-    const int i = !valid ? 0 : (w < ns ? w : w + nf);
-    const bool is_site = i < ns;
-    const double xi = valid ? P.x[i] : 0.0, yi = valid ? P.y[i] : 0.0, zi = valid ? P.z[i] : 0.0;
-    const int t2i = valid ? P.t2[i] : 0;
-    const bool free_me = !is_site || ((FEAT & F_DNA) && P.stfree[i] != 0);
-    const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
-    const int prow = t2i * nt1;
-
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    double e_tab = 0.0, e_el = 0.0, e_vd = 0.0, e_sr = 0.0;
-
-    // ---- pair loop over LDS tiles of every particle (energy.mono.f90:117-194, 584-687)
-    for (int base = 0; base < ntot; base += TILE) {
-        __syncthreads();
-        {
-            int j = base + tid;
-            if (j < ntot) s_pos[tid] = make_double4(P.x[j], P.y[j], P.z[j], __hiloint2double(0, P.t2[j]));
-        }
-        __syncthreads();
-        if (valid && nonbond) {
-            const int kend = min(TILE, ntot - base);
-            const int kbeg = is_site ? max(0, ns - base) : 0;  // site targets: ion partners only
-            for (int k = kbeg + lane; k < kend; k += WAVE) {
-                const int j = base + k;
-                if (j == i) continue;
-                const double4 pj = s_pos[k];
-                const int t2j = __double2loint(pj.w);
-                const double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
-                const double d2 = dx * dx + dy * dy + dz * dz;
-                const DevPair &pp = s_pairs[s_pidx[prow + t2j]];
-                // energy belongs to this target unless the partner is a site (counted by the
-                // site) or a higher-index ion (counted by that ion)
-                const bool count_e = is_site || (j >= ns && j < i);
-                double de;
-                if (pp.flags & PF_EFPOT) {
-                    double y;
-                    getyd(pp, P.ep, P.ires, d2, y, de);
-                    if (count_e) e_tab += y;
-                } else {
-                    double a = 0.0, b = 0.0, c = 0.0;
-                    de = pair_analytic(pp, P, d2, a, b, c);
-                    if (count_e) { e_el += a; e_vd += b; e_sr += c; }
-                }
-                if (free_me) {
-                    fx += de * dx;
-                    fy += de * dy;
-                    fz += de * dz;
-                }
-            }
-        }
-    }
-
-    double e[NE];
+    int ti[T];
+    bool tval[T], tsite[T], tfree[T];
+    double tx[T], ty[T], tz[T];
 #pragma unroll
-    for (int k = 0; k < NE; k++) e[k] = 0.0;
+    for (int a = 0; a < T; a++) {
+        const int w = wbase + a;
+        tval[a] = w < P.tend;
+        const int i = !tval[a] ? 0 : (w < ns ? w : w + nf);
+        ti[a] = i;
+        tsite[a] = i < ns;
+        const double4 pa = P.pos[i];
+        tx[a] = pa.x; ty[a] = pa.y; tz[a] = pa.z;
+        tfree[a] = !tsite[a] || ((FEAT & F_DNA) && P.stfree[i] != 0);
+        // pair rows of this block's targets
+        const int row = __double2loint(pa.w) * nt1;
+        for (int k = tid; k < nt1; k += BLOCK) {
+            const DevPair d = P.rows[row + k];
+            s_row[a][k] = d;
+            s_tail[a][k] = make_double2(d.sc3, d.fct);
+        }
+    }
+    __syncthreads();
 
-    // ---- CG-DNA terms incident on this site (energy.mono.f90:196-508)
-    if constexpr ((FEAT & F_DNA) != 0) {
-        if (valid && is_site) {
-            V3 f = {0.0, 0.0, 0.0};
-            const int k0 = P.inc_ptr[i], k1 = P.inc_ptr[i + 1];
-            for (int k = k0 + lane; k < k1; k += WAVE) {
-                uint32_t wd = P.inc[k];
-                int kind = wd >> 29;
-                bool on = kind <= T_DIHE ? (P.flags & QNOBOND) != 0
-                                         : ((P.flags & QNONBOND) != 0 &&
-                                            (kind != T_SLV || (P.flags & QSOLV) != 0));
-                if (on) dna_term(P, wd, free_me, f, e);
+    // Gaussians of the BD update, drawn by the last wavefront (lanes 0..T-1) before its
+    // share of the pair loop, so their latency overlaps the other wavefronts' pair work
+    if constexpr ((FEAT & F_UPDATE) != 0) {
+        const int a = tid - (BLOCK - WAVE);
+        if (a >= 0 && a < T && tval[a]) {
+            const int i = ti[a];
+            const int64_t step = P.step_ctr[0] + P.step_off;
+            double g[3];
+            if (P.noise_mode == 0) {
+                const double *nr = P.noise + (P.step_ctr[1] + step) * P.noise_stride + 3 * (int64_t)i;
+                g[0] = nr[0]; g[1] = nr[1]; g[2] = nr[2];
+            } else {
+                philox_gauss3((uint64_t)P.step_ctr[3], (uint64_t)(P.step_ctr[2] + step), P.stream, i, g);
             }
-            fx += f.x; fy += f.y; fz += f.z;
+            s_g[a][0] = g[0]; s_g[a][1] = g[1]; s_g[a][2] = g[2];
         }
     }
 
-    // ---- external grids on lanes 0..7 (staticf1.f90, vdwgd1trln.f90)
+    double fx[T], fy[T], fz[T], et[T], eel[T], evd[T], esr[T];
+#pragma unroll
+    for (int a = 0; a < T; a++) fx[a] = fy[a] = fz[a] = et[a] = eel[a] = evd[a] = esr[a] = 0.0;
+
+    // ---- pair loop (energy.mono.f90:117-194, 584-687)
+    const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
+    if (nonbond) {
+        const double r2near = P.r2near;
+        for (int k0 = tid; k0 < ntot; k0 += 2 * BLOCK) {
+            // partners in spatial order, two per iteration (two load chains in flight)
+            const int k1 = k0 + BLOCK;
+            const bool has1 = k1 < ntot;
+            const double4 p0 = P.spos[k0];
+            const double4 p1 = P.spos[has1 ? k1 : k0];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const double4 pj = u == 0 ? p0 : p1;
+                const bool jok = u == 0 || has1;
+                const int t2j = __double2loint(pj.w);
+                const int j = __double2hiint(pj.w);
+#pragma unroll
+                for (int a = 0; a < T; a++) {
+                    const bool skip = !jok || !tval[a] || j == ti[a] || (tsite[a] && j < ns);
+                    const double dx = tx[a] - pj.x, dy = ty[a] - pj.y, dz = tz[a] - pj.z;
+                    double d2 = dx * dx + dy * dy + dz * dz;
+                    d2 = skip ? 1e30 : d2;
+                    // a pair's energy is counted once: by the site of a site-ion pair, by
+                    // the higher-index ion of an ion-ion pair
+                    const bool ce = !skip && (tsite[a] || (j >= ns && j < ti[a]));
+                    double y, de;
+                    if (__all(d2 > r2near)) {
+                        // every lane beyond every table: the analytic tail sc3/r^6 + fct/r
+                        // (getyd, polynom.f90:208-223), same arithmetic as the table path
+                        const double2 tp = s_tail[a][t2j];
+                        const double rinv = rsqrt_nr(d2);
+                        const double ix2 = rinv * rinv;
+                        const double c = tp.x * pow3(ix2);
+                        const double f = tp.y * rinv;
+                        y = c + f;
+                        de = (6.0 * c + f) * ix2;
+                        et[a] += ce ? y : 0.0;
+                    } else if (P.all_efpot || (s_row[a][t2j].flags & PF_EFPOT)) {
+                        de = pair_table(s_row[a][t2j], P.ep, P.ires, d2, y);
+                        et[a] += ce ? y : 0.0;
+                    } else {
+                        double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+                        de = pair_analytic(s_row[a][t2j], P, d2, q0, q1, q2);
+                        eel[a] += ce ? q0 : 0.0;
+                        evd[a] += ce ? q1 : 0.0;
+                        esr[a] += ce ? q2 : 0.0;
+                    }
+                    de = (skip || !tfree[a]) ? 0.0 : de;
+                    fx[a] += de * dx;
+                    fy[a] += de * dy;
+                    fz[a] += de * dz;
+                }
+            }
+        }
+    }
+
+    // pair sums to LDS (the pair accumulators are dead after this)
+#pragma unroll
+    for (int a = 0; a < T; a++) {
+        double v[7] = {fx[a], fy[a], fz[a], et[a], eel[a], evd[a], esr[a]};
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            if (k >= 3 && !P.write_energy) break;
+            v[k] = wave_sum(v[k]);
+            if (lane == 0) s_red[wv][a][k] = v[k];
+        }
+    }
+
+    // ---- grids of target `wv` (if any), lanes 0..7 of wavefront wv
+    double gfx = 0.0, gfy = 0.0, gfz = 0.0, egx = 0.0, egv = 0.0;
     bool warned = false;
+    const bool mine = wv < T;
+    int i_me = 0;
+    bool val_me = false, site_me = false;
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    int t2i = 0;
+    if (mine) {
+        const int w = wbase + wv;
+        val_me = w < P.tend;
+        i_me = !val_me ? 0 : (w < ns ? w : w + nf);
+        site_me = i_me < ns;
+        const double4 pa = P.pos[i_me];
+        xi = pa.x; yi = pa.y; zi = pa.z;
+        t2i = __double2loint(pa.w);
+    }
     if constexpr ((FEAT & F_PHIX) != 0) {
-        if (valid && lane < 8) {
+        if (mine && val_me && lane < 8) {
             const GridGeom &g = P.gx;
-            double chi = 0.0;
+            double chi;
             bool ok;
-            if (is_site) { ok = P.namsite[i] == 1; chi = P.cgnuc; }
-            else { ok = true; chi = P.cg[abs(P.typei[i])]; }
+            if (site_me) { ok = P.namsite[i_me] == 1; chi = P.cgnuc; }
+            else { ok = true; chi = P.cg[abs(P.typei[i_me])]; }
             ok = ok && xi <= g.xbcen + g.tranx && xi >= g.xbcen - g.tranx &&
                  yi <= g.ybcen + g.trany && yi >= g.ybcen - g.trany &&
                  zi <= g.zbcen + g.tranz && zi >= g.zbcen - g.tranz;
@@ -432,36 +560,32 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
                 double gx_, gy_, gz_;
                 int ix, iy, iz;
                 grid_cell(g, xi, yi, zi, gx_, gy_, gz_, ix, iy, iz);
-                // prefac = phix*celec*chi*idcel and E += fi*chi*phix*celec
-                double a = 0.0, b = 0.0, c = 0.0, en = 0.0;
-                const double one = 1.0;
-                int n1 = ix + (lane >> 2), n2 = iy + ((lane >> 1) & 1), n3 = iz + (lane & 1);
+                const double one1 = 1.0;
+                const int n1 = ix + (lane >> 2), n2 = iy + ((lane >> 1) & 1), n3 = iz + (lane & 1);
                 double ai = gx_ - n1 * g.dcel;
-                double aisign = fsign(one, ai);
-                ai = one - fabs(ai) * g.idcel;
+                const double aisign = fsign(one1, ai);
+                ai = one1 - fabs(ai) * g.idcel;
                 double bi = gy_ - n2 * g.dcel;
-                double bisign = fsign(one, bi);
-                bi = one - fabs(bi) * g.idcel;
+                const double bisign = fsign(one1, bi);
+                bi = one1 - fabs(bi) * g.idcel;
                 double ci = gz_ - n3 * g.dcel;
-                double cisign = fsign(one, ci);
-                ci = one - fabs(ci) * g.idcel;
-                double fi = ai * bi * ci;
-                double ph = (double)P.phix[((size_t)n1 * g.ny + n2) * g.nz + n3];
-                double prefac = ph * P.celec * chi * g.idcel;
-                if (ai < (1.0 - RSMALL) && ai > RSMALL) a = aisign * bi * ci * prefac;
-                if (bi < (1.0 - RSMALL) && bi > RSMALL) b = bisign * ai * ci * prefac;
-                if (ci < (1.0 - RSMALL) && ci > RSMALL) c = cisign * ai * bi * prefac;
-                en = fi * chi * ph * P.celec;
-                fx += a; fy += b; fz += c;
-                e[E_GSBPA] += en;
+                const double cisign = fsign(one1, ci);
+                ci = one1 - fabs(ci) * g.idcel;
+                const double fi = ai * bi * ci;
+                const double ph = (double)P.phix[((size_t)n1 * g.ny + n2) * g.nz + n3];
+                const double prefac = ph * P.celec * chi * g.idcel;
+                if (ai < (1.0 - RSMALL) && ai > RSMALL) gfx += aisign * bi * ci * prefac;
+                if (bi < (1.0 - RSMALL) && bi > RSMALL) gfy += bisign * ai * ci * prefac;
+                if (ci < (1.0 - RSMALL) && ci > RSMALL) gfz += cisign * ai * bi * prefac;
+                egx += fi * chi * ph * P.celec;
             }
         }
     }
     if constexpr ((FEAT & F_PHIV) != 0) {
         const GridGeom &g = P.gv;
-        bool ok = valid && xi <= g.xbcen + g.tranx && xi >= g.xbcen - g.tranx &&
-                  yi <= g.ybcen + g.trany && yi >= g.ybcen - g.trany && zi <= P.vzmax &&
-                  zi >= P.vzmin;
+        const bool ok = mine && val_me && xi <= g.xbcen + g.tranx && xi >= g.xbcen - g.tranx &&
+                        yi <= g.ybcen + g.trany && yi >= g.ybcen - g.trany && zi <= P.vzmax &&
+                        zi >= P.vzmin;
         double ph = 0.0;
         if (ok && lane < 8) {
             double gx_, gy_, gz_;
@@ -469,50 +593,92 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
             grid_cell(g, xi, yi, zi, gx_, gy_, gz_, ix, iy, iz);
             double esvdw = P.svdw;
             if (P.flags & QSVDW) esvdw = P.svdw * P.scal[t2i];
-            double a = 0.0, b = 0.0, c = 0.0, en = 0.0;
-            ph = tri_corner(g, P.phiv, gx_, gy_, gz_, ix, iy, iz, lane, esvdw, a, b, c, en);
-            fx += a; fy += b; fz += c;
-            e[E_VDWGD] += en;
+            ph = tri_corner(g, P.phiv, gx_, gy_, gz_, ix, iy, iz, lane, esvdw, gfx, gfy, gfz, egv);
         }
-        // phisum over the 8 corners (lanes 0..7) in corner order
-        double s = ph;
-        s += __shfl_xor(s, 1, WAVE);
-        s += __shfl_xor(s, 2, WAVE);
-        s += __shfl_xor(s, 4, WAVE);
-        warned = ok && s >= P.thold8;
+        double sum = ph;
+        sum += __shfl_xor(sum, 1, WAVE);
+        sum += __shfl_xor(sum, 2, WAVE);
+        sum += __shfl_xor(sum, 4, WAVE);
+        warned = ok && sum >= P.thold8;
     }
-
-    // ---- reductions
-    fx = wave_sum(fx);
-    fy = wave_sum(fy);
-    fz = wave_sum(fz);
-    if (P.write_energy) {
-        if (is_site) { e[E_EFPOTMX] += e_tab; e[E_QQMX] += e_el; e[E_VDWMX] += e_vd; e[E_SRPMFMX] += e_sr; }
-        else { e[E_EFPOT] += e_tab; e[E_ELEC] += e_el; e[E_VDW] += e_vd; e[E_SRPMF] += e_sr; }
-#pragma unroll
-        for (int k = 0; k < NE; k++) e[k] = wave_sum(e[k]);
+    if (mine) {
+        gfx = wave_sum(gfx);
+        gfy = wave_sum(gfy);
+        gfz = wave_sum(gfz);
+        if (P.write_energy) {
+            egx = wave_sum(egx);
+            egv = wave_sum(egv);
+        }
+        if (lane == 0) {
+            s_dg[wv][0] = gfx; s_dg[wv][1] = gfy; s_dg[wv][2] = gfz;
+            s_dg[wv][3] = egx; s_dg[wv][4] = egv;
+            s_warn[wv] = warned ? 1 : 0;
+        }
     }
-    if (!valid || lane != 0) return;
+    __syncthreads();
+    if (tid >= T) return;
 
-    if (warned) atomicAdd(&P.warn[t2i], 1);
-    if (P.write_energy) {
+    // ---- thread a finishes target a
+    const int a = tid;
+    const int w = wbase + a;
+    if (w >= P.tend) return;
+    const int i = w < ns ? w : w + nf;
+    const bool is_site = i < ns;
+    double F[3];
 #pragma unroll
-        for (int k = 0; k < NE; k++) P.epart[(size_t)k * P.ecap + i] = e[k];
+    for (int k = 0; k < 3; k++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < WPB; q++) acc += s_red[q][a][k];
+        F[k] = acc + s_dg[a][k];
+    }
+    if constexpr ((FEAT & F_DNA) != 0) {
+        if (is_site) {
+            const double4 fd = P.fdna[i];   // CG-DNA list terms from k_dna
+            F[0] += fd.x; F[1] += fd.y; F[2] += fd.z;
+        }
+    }
+    const double4 pa = P.pos[i];
+    const int t2a = __double2loint(pa.w);
+    if (s_warn[a]) atomicAdd(&P.warn[t2a], 1);
+    if (P.write_energy) {
+        double pe[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < WPB; q++) acc += s_red[q][a][3 + k];
+            pe[k] = acc;
+        }
+        // this kernel owns the pair and grid categories; k_dna writes the DNA ones
+        double *ep_ = P.epart + i;
+        const size_t cs = P.ecap;
+        ep_[E_GSBPA * cs] = s_dg[a][3];
+        ep_[E_VDWGD * cs] = s_dg[a][4];
+        if (is_site) {
+            ep_[E_EFPOTMX * cs] = pe[0]; ep_[E_QQMX * cs] = pe[1];
+            ep_[E_VDWMX * cs] = pe[2]; ep_[E_SRPMFMX * cs] = pe[3];
+        } else {
+            ep_[E_EFPOT * cs] = pe[0]; ep_[E_ELEC * cs] = pe[1];
+            ep_[E_VDW * cs] = pe[2]; ep_[E_SRPMF * cs] = pe[3];
+        }
     }
     if (P.write_forces) {
-        P.fx[i] = fx;
-        P.fy[i] = fy;
-        P.fz[i] = fz;
+        P.fx[i] = F[0];
+        P.fy[i] = F[1];
+        P.fz[i] = F[2];
     }
     if constexpr ((FEAT & F_UPDATE) != 0) {
-        // DYNAMICS0 / DYNAMICS0NUC
+        // DYNAMICS0 (dynamics0.mono.f90:31-46) / DYNAMICS0NUC (dynamics0nuc.f90:31-44)
+        const bool fr = !is_site || ((FEAT & F_DNA) && P.stfree[i] != 0);
+        if (!fr) {
+            P.poso[i] = pa;
+            P.sposo[P.rank[i]] = pa;
+            return;
+        }
         double fac1, fac2;
         int itype = 0;
         if (is_site) {
-            if (!free_me) {
-                P.xo[i] = xi; P.yo[i] = yi; P.zo[i] = zi;
-                return;
-            }
             fac1 = P.fact0n1;
             fac2 = P.fact0n2;
         } else {
@@ -520,52 +686,112 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
             fac1 = P.fact1a[itype];
             fac2 = P.fact2a[itype];
         }
-        const int64_t step = P.step_ctr[0] + P.step_off;
-        double g[3];
-        if (P.noise_mode == 0) {
-            const double *nr = P.noise + (P.step_ctr[1] + step) * P.noise_stride + 3 * (int64_t)i;
-            g[0] = nr[0]; g[1] = nr[1]; g[2] = nr[2];
-        } else {
-            philox_gauss3((uint64_t)P.step_ctr[3], (uint64_t)(P.step_ctr[2] + step), P.stream, i, g);
-        }
+        const double g[3] = {s_g[a][0], s_g[a][1], s_g[a][2]};
         const double bdmax = P.bdmax;
-        double delx = fx * fac1, dely = fy * fac1, delz = fz * fac1;
+        double delx = F[0] * fac1, dely = F[1] * fac1, delz = F[2] * fac1;
         if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
         if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
         if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
-        double xn = xi + delx + fac2 * g[0];
-        double yn = yi + dely + fac2 * g[1];
-        double zn = zi + delz + fac2 * g[2];
+        double xn = pa.x + delx + fac2 * g[0];
+        double yn = pa.y + dely + fac2 * g[1];
+        double zn = pa.z + delz + fac2 * g[2];
         if (!is_site && (P.flags & QCOUNTION)) {
             const int row = itype - P.nold - 1;
             for (int c = 0; c < P.cntpts; c++) {
                 const double zc = P.zcont[c];
-                if (zi < zc && zn >= zc) atomicAdd(&P.nfwd[row * P.cntpts + c], 1);
-                else if (zi >= zc && zn < zc) atomicAdd(&P.nbwd[row * P.cntpts + c], 1);
+                if (pa.z < zc && zn >= zc) atomicAdd(&P.nfwd[row * P.cntpts + c], 1);
+                else if (pa.z >= zc && zn < zc) atomicAdd(&P.nbwd[row * P.cntpts + c], 1);
             }
         }
         if (!is_site || !(P.flags & QDNAFREE)) fixcoor(P, xn, yn, zn);
-        P.xo[i] = xn;
-        P.yo[i] = yn;
-        P.zo[i] = zn;
+        const double4 pn = make_double4(xn, yn, zn, pa.w);
+        P.poso[i] = pn;
+        P.sposo[P.rank[i]] = pn;
     }
 }
 
-hipError_t launch_step(const StepParams &p, hipStream_t s) {
-    if (p.ntargets <= 0) return hipSuccess;
-    dim3 grid((p.ntargets + WPB - 1) / WPB), block(BLOCK);
-    int feat = 0;
-    if (p.flags & QNUCL) feat |= F_DNA;
-    if (p.flags & QPHIX) feat |= F_PHIX;
-    if (p.flags & QPHIV) feat |= F_PHIV;
-    if (p.mode == 1) feat |= F_UPDATE;
-#define CASE(F) case F: hipLaunchKernelGGL(k_step<F>, grid, block, 0, s, p); break;
+// k_dna: CG-DNA bonded and list terms (energy.mono.f90:196-508), one block per site.
+// The block's threads take the site's term incidences in order (they are grouped by term
+// kind, so wavefronts rarely diverge); sums go butterfly-then-LDS in a fixed order.  The
+// force on the site goes to fdna[i] (zero for a fixed site); the energies of the terms the
+// site heads go to epart.
+__global__ __launch_bounds__(BLOCK) void k_dna(StepParams P) {
+    __shared__ double s_r[WPB][3 + 8];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
+    const int i = blockIdx.x;
+    const bool free_me = P.stfree[i] != 0;
+    V3 f = {0.0, 0.0, 0.0};
+    double e[ND];
+#pragma unroll
+    for (int k = 0; k < ND; k++) e[k] = 0.0;
+    const int k0 = P.inc_ptr[i], k1 = P.inc_ptr[i + 1];
+    for (int k = k0 + tid; k < k1; k += BLOCK) {
+        const uint32_t wd = P.inc[k];
+        const int kind = wd >> 29;
+        const bool on = kind <= T_DIHE ? (P.flags & QNOBOND) != 0
+                                       : ((P.flags & QNONBOND) != 0 &&
+                                          (kind != T_SLV || (P.flags & QSOLV) != 0));
+        if (on) dna_term(P, wd, free_me, f, e);
+    }
+    f.x = wave_sum(f.x);
+    f.y = wave_sum(f.y);
+    f.z = wave_sum(f.z);
+    if (P.write_energy) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) e[k] = wave_sum(e[k]);
+    }
+    if (lane == 0) {
+        s_r[wv][0] = f.x; s_r[wv][1] = f.y; s_r[wv][2] = f.z;
+        for (int k = 0; k < 8; k++) s_r[wv][3 + k] = e[k];
+    }
+    __syncthreads();
+    if (tid < 3 + 8) {
+        double acc = 0.0;
+        for (int q = 0; q < WPB; q++) acc += s_r[q][tid];
+        if (tid < 3) {
+            reinterpret_cast<double *>(&P.fdna[i])[tid] = acc;
+        } else if (P.write_energy) {
+            P.epart[(size_t)ND_CAT[tid - 3] * P.ecap + i] = acc;
+        }
+    }
+}
+
+template <int T>
+static void dispatch_step(const StepParams &p, int feat, hipStream_t s) {
+    dim3 grid((p.tend - p.tbeg + T - 1) / T), block(BLOCK);
+#define CASE(F) case F: hipLaunchKernelGGL((k_step<F, T>), grid, block, 0, s, p); break;
     switch (feat) {
         CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
         CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
     }
 #undef CASE
+}
+
+hipError_t launch_dna(const StepParams &p, hipStream_t s) {
+    if ((p.flags & QNUCL) && p.nsites > 0)
+        hipLaunchKernelGGL(k_dna, dim3(p.nsites), dim3(BLOCK), 0, s, p);
     return hipGetLastError();
+}
+
+hipError_t launch_pairs(const StepParams &p, hipStream_t s) {
+    if (p.tend <= p.tbeg) return hipSuccess;
+    int feat = 0;
+    if (p.flags & QNUCL) feat |= F_DNA;
+    if (p.flags & QPHIX) feat |= F_PHIX;
+    if (p.flags & QPHIV) feat |= F_PHIV;
+    if (p.mode == 1) feat |= F_UPDATE;
+    switch (p.tpb) {
+        case 1: dispatch_step<1>(p, feat, s); break;
+        case 2: dispatch_step<2>(p, feat, s); break;
+        default: dispatch_step<4>(p, feat, s); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_step(const StepParams &p, hipStream_t s) {
+    hipError_t e = launch_dna(p, s);
+    if (e != hipSuccess) return e;
+    return launch_pairs(p, s);
 }
 
 // ---- energy reduction: one block per category, fixed-order tree
@@ -712,7 +938,7 @@ __global__ __launch_bounds__(BLOCK) void k_interact(InteractParams Q) {
     const int j = Q.j;
     const double xj = Q.xj, yj = Q.yj, zj = Q.zj;
     const bool beion = (P.flags & QPAR) && j >= ns;
-    const int jt2 = beion ? P.nwtype[Q.jtype] : P.t2[j];
+    const int jt2 = beion ? P.nwtype[Q.jtype] : __double2loint(P.pos[j].w);
     const int prow = jt2 * nt1;
     double e = 0.0;
     // staticf0.f90:19-82 and vdwgd0trln.f90:19-117 on lanes 0..7 of wave 0
@@ -776,9 +1002,10 @@ __global__ __launch_bounds__(BLOCK) void k_interact(InteractParams Q) {
     if (beion && nonbond) {
         for (int i = ns + tid; i < ntot; i += BLOCK) {
             if (i == j) continue;
-            double dx = P.x[i] - xj, dy = P.y[i] - yj, dz = P.z[i] - zj;
+            const double4 q = P.pos[i];
+            double dx = q.x - xj, dy = q.y - yj, dz = q.z - zj;
             double d2 = dx * dx + dy * dy + dz * dz;
-            const DevPair &pp = s_pairs[s_pidx[prow + P.t2[i]]];
+            const DevPair &pp = s_pairs[s_pidx[prow + __double2loint(q.w)]];
             e += (pp.flags & PF_EFPOT) ? gety(pp, P.ep, P.ires, d2) : interact_analytic(pp, P, d2);
         }
     }
@@ -795,9 +1022,10 @@ __global__ __launch_bounds__(BLOCK) void k_interact(InteractParams Q) {
     if ((P.flags & QPAR) && (P.flags & QNUCL) && nonbond) {
         const int lo = beion ? 0 : ns, hi = beion ? ns : ntot;
         for (int i = lo + tid; i < hi; i += BLOCK) {
-            double dx = P.x[i] - xj, dy = P.y[i] - yj, dz = P.z[i] - zj;
+            const double4 q = P.pos[i];
+            double dx = q.x - xj, dy = q.y - yj, dz = q.z - zj;
             double d2 = dx * dx + dy * dy + dz * dz;
-            const DevPair &pp = s_pairs[s_pidx[prow + P.t2[i]]];
+            const DevPair &pp = s_pairs[s_pidx[prow + __double2loint(q.w)]];
             e += (pp.flags & PF_EFPOT) ? gety(pp, P.ep, P.ires, d2) : interact_analytic(pp, P, d2);
         }
     }

@@ -106,6 +106,12 @@ class Engine:
                                              C.byref(tot), C.byref(ker)))
         return tot.value, ker.value
 
+    def tuning(self):
+        """Targets per block the autotuner picked for this system."""
+        t = C.c_int32()
+        self._chk("bromoc_get_tuning", self.lib.bromoc_get_tuning(self.ctx, C.byref(t)))
+        return t.value
+
     def dynamics(self, mode=abi.NOISE_INJECTED, row=0, seed=0, counter=0):
         self._chk("bromoc_dynamics", self.lib.bromoc_dynamics(self.ctx, mode, row, seed, counter))
 

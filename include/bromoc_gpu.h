@@ -209,12 +209,17 @@ int bromoc_upload_noise(bromoc_ctx *ctx, int32_t nsteps, const double *noise);
 int bromoc_step(bromoc_ctx *ctx, int32_t nsteps, int32_t noise_mode, int32_t first_row,
                 uint64_t seed, uint64_t counter0, bromoc_energies *energies);
 
-/* Measurement: nsteps steps as bromoc_step, with a HIP event pair around every kernel
- * launch on the library's stream; *total_ms is first-to-last, *kernel_ms the mean launch
- * duration.  No counterpart in the reference (it times runs with date_and_time,
+/* Measurement: nsteps steps as bromoc_step (launched one by one, no graph), with a HIP
+ * event pair around every launch of the step kernel on the library's stream; *total_ms is
+ * first-to-last, *kernel_ms the mean step-kernel duration.  No counterpart in the reference (it times runs with date_and_time,
  * cputime.f90:19-39). */
 int bromoc_time_steps(bromoc_ctx *ctx, int32_t nsteps, int32_t noise_mode, int32_t first_row,
                       uint64_t seed, uint64_t counter0, double *total_ms, double *kernel_ms);
+
+/* Launch shape picked by the library's autotuner for this system (targets per block of
+ * the step kernel: 1, 2 or 4); runs the tuner if it has not run.  No counterpart in the
+ * reference. */
+int bromoc_get_tuning(bromoc_ctx *ctx, int32_t *targets_per_block);
 
 /* One DYNAMICS pass with the current forces (as left by bromoc_energy). */
 int bromoc_dynamics(bromoc_ctx *ctx, int32_t noise_mode, int32_t row, uint64_t seed,

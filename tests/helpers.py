@@ -17,9 +17,10 @@ def force_err(f, g):
     return float(np.max(np.abs(f - g)) / max(np.max(np.abs(g)), 1e-300))
 
 
-def energy_errs(e, r, floor=1e-12):
-    """Per-term relative error; terms smaller than `floor` times the largest term are
-    compared absolutely against that scale."""
+def energy_errs(e, r, floor=1e-6):
+    """Per-term relative error, norm-wise for tiny terms: a term smaller than `floor` times
+    the largest term (e.g. native-geometry stacking energies, which are pure rounding noise
+    around zero) is measured against floor * largest instead of against itself."""
     scale = max(abs(v) for v in r.values())
     out = {}
     for k in r:

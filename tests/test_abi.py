@@ -1,0 +1,111 @@
+"""The C ABI without a GPU: the library loads, exports every entry point the header
+declares, its structs have the header's layout, and argument validation answers with
+status codes before any device is touched."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from bromoc_amd import abi, configs
+from oracle.oracle import oracle_table_builder
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "include", "bromoc_gpu.h")
+
+
+def test_exports_every_declared_symbol():
+    lib = abi.load_library()
+    names = abi.exported_symbols()
+    assert len(names) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", abi.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(\w+)$", out, re.M))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert hasattr(lib, n)
+
+
+def test_struct_layout_matches_header(tmp_path):
+    structs = {"bromoc_pair": abi.Pair, "bromoc_grid": abi.Grid, "bromoc_system": abi.SystemStruct,
+               "bromoc_energies": abi.Energies, "bromoc_gcmc": abi.GcmcStruct}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HDR}"', "int main(void){"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0;}")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    for line in out.splitlines():
+        cname, field, val = line.split()
+        py = structs[cname]
+        got = C.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert got == int(val), (cname, field, got, val)
+
+
+def _create(s):
+    lib = abi.load_library()
+    st, keep = s.to_struct()
+    ctx = C.c_void_p()
+    return lib.bromoc_create(C.byref(st), 0, C.byref(ctx)), ctx
+
+
+def _plain():
+    return configs.plain_ions(seed=1, n_per_type=5, table_builder=oracle_table_builder)
+
+
+def test_create_rejects_unsupported_options():
+    for flag in (abi.QMEMB, abi.QMMIJ, abi.QBSPLINE, abi.QRFPAR, abi.QSPHERE, abi.QPROFILE):
+        s = _plain()
+        s.flags |= flag
+        rc, _ = _create(s)
+        assert rc == abi.E_UNSUPPORTED
+
+
+def test_create_rejects_bad_indices():
+    s = _plain()
+    s.typei = s.typei.copy()
+    s.typei[0] = 7                       # no such atnam type
+    assert _create(s)[0] == abi.E_ARG
+    s = _plain()
+    s.pairidx = np.zeros_like(s.pairidx)  # no pair type for present ions
+    assert _create(s)[0] == abi.E_ARG
+    s = _plain()
+    s.pairs[1] = dict(s.pairs[1], nknots=100)   # table range exceeds its knots
+    assert _create(s)[0] == abi.E_ARG
+    s = _plain()
+    s.capacity = s.ntot - 1
+    assert _create(s)[0] == abi.E_ARG
+
+
+def test_create_without_device_reports_nodevice():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    rc, _ = _create(_plain())
+    assert rc == abi.E_NODEVICE
+
+
+def test_build_table_argument_errors():
+    lib = abi.load_library()
+    r = np.array([1.0])
+    ep = np.zeros(3)
+    p = abi.Pair()
+    dp = C.POINTER(C.c_double)
+    assert lib.bromoc_build_table(1, r.ctypes.data_as(dp), r.ctypes.data_as(dp), 0, 0.0,
+                                  ep.ctypes.data_as(dp), C.byref(p)) == abi.E_ARG
+    r2 = np.array([2.0, 1.0])          # not increasing
+    assert lib.bromoc_build_table(2, r2.ctypes.data_as(dp), r2.ctypes.data_as(dp), 0, 0.0,
+                                  np.zeros(6).ctypes.data_as(dp), C.byref(p)) == abi.E_ARG
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(ImportError):
+        abi.load_library(str(tmp_path / "nope.so"))

@@ -91,6 +91,24 @@ def test_steps_injected_noise(which):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("nsteps", [33, 70])
+def test_steps_graph_replays(nsteps):
+    """Long step calls replay captured 32-step graphs (one per buffer parity); the result
+    must equal step-by-step execution in the oracle."""
+    s = plain(seed=31, n_per_type=40)
+    noise = np.random.default_rng(7).standard_normal((nsteps + 1, 3 * s.ntot))
+    g, o = Engine(s), Oracle(s)
+    g.upload_noise(noise)
+    o.upload_noise(noise)
+    g.step(1, abi.NOISE_INJECTED, 0)          # odd start: the other buffer parity
+    o.step(1, abi.NOISE_INJECTED, 0)
+    g.step(nsteps, abi.NOISE_INJECTED, 1)
+    o.step(nsteps, abi.NOISE_INJECTED, 1)
+    err = float(np.max(np.abs(g.positions()[0] - o.positions()[0])))
+    print(f"{nsteps} steps through graphs: position err {err:.3e}")
+    assert err <= 1e-8
+
+
 def test_steps_philox():
     s = pore_small(seed=7, n_bp=6, n_ions=150)
     g, o = Engine(s), Oracle(s)
@@ -167,6 +185,32 @@ def test_grand_matches():
     po, to = o.positions()
     assert np.array_equal(pg, po) and np.array_equal(tg, to)
     assert np.array_equal(g.ibuffer(), o.ibuffer())
+
+
+def test_steps_after_grand():
+    """GCMC inserts/deletes change ntot and the particle order; the following BD steps
+    (new partner order, new slots) must still match the oracle."""
+    s = pore_small(seed=12, n_bp=0, n_ions=100, dna=False).copy()
+    s.capacity = s.ntot + 200
+    gc = configs.gcmc_buffers(s, conc=1.5, buffer_len=8.0, mu=(-0.3, -0.3))
+    ib, nat = configs.assign_buffers(s, gc)
+    g, o = Engine(s), Oracle(s)
+    g.set_gcmc(gc, ib)
+    o.set_gcmc(gc, ib)
+    u = np.random.default_rng(6).random(20000)
+    prob = np.zeros((gc.nbuffer, s.capacity + 1), np.int32)
+    natg, nato = nat.copy(), nat.copy()
+    g.grand(150, u, natg, prob.copy(), 1, 0.0)
+    o.grand(150, u, nato, prob.copy(), 1, 0.0)
+    assert g.ntot == o.ntot and g.ntot != s.ntot
+    noise = np.random.default_rng(8).standard_normal((40, 3 * g.ntot))
+    g.upload_noise(noise)
+    o.upload_noise(noise)
+    eg = g.step(40, abi.NOISE_INJECTED, 0)
+    eo = o.step(40, abi.NOISE_INJECTED, 0)
+    _check_energy(eg, eo)
+    err = float(np.max(np.abs(g.positions()[0] - o.positions()[0])))
+    assert err <= 1e-8, err
 
 
 def test_capacity_error():

@@ -1,0 +1,120 @@
+"""The C oracle against the reference's own Fortran routines (oracle/_ref, built from the
+reference sources by oracle/Makefile).  Skipped where that build is absent (the GPU box).
+
+Bar: bit-identical where both evaluate the same expression sequence; <= 1e-14 relative
+where the Fortran compiler's evaluation order of a long expression can differ by an ulp.
+"""
+import numpy as np
+import pytest
+
+from bromoc_amd import abi, configs
+from oracle.oracle import (Oracle, RefFortran, oracle_table_builder, ref_available,
+                           ref_table_builder)
+
+from helpers import energy_errs, force_err, with_fixed
+
+pytestmark = pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built here")
+
+
+def small_plain(seed, n, **kw):
+    return configs.plain_ions(seed=seed, n_per_type=n, table_builder=oracle_table_builder, **kw)
+
+
+def small_pore(seed=5, n_bp=8, n_ions=120, dna=True):
+    return configs.pore_dna(seed=seed, n_bp=n_bp, n_ions=n_ions, box=(40.0, 40.0, 60.0),
+                            dcel=0.5, rpore=12.0, half_slab=10.0, dna=dna,
+                            table_builder=oracle_table_builder)
+
+
+@pytest.mark.parametrize("q,lo,hi,res", [(-1.0, 1.5, 20.0, 0.01), (1.0, 2.0, 12.0, 0.05),
+                                         (0.0, 1.5, 15.0, 0.1)])
+def test_table_build_bitexact(q, lo, hi, res):
+    r = configs.knot_grid(lo, hi, res)
+    u = configs.efpot_curve(r, 3.7, 0.25, q, 80.0)
+    fct = configs.CELEC * q / 80.0
+    chr_ = 1 if q else 0
+    e1, p1 = oracle_table_builder(len(r), r, u, chr_, fct)
+    e2, p2 = ref_table_builder(len(r), r, u, chr_, fct)
+    e3, p3 = configs.lib_table_builder(len(r), r, u, chr_, fct)
+    assert np.array_equal(e1, e2) and np.array_equal(e3, e2)
+    for f in ("sc1", "sc2", "sc3", "dmi", "dm2lo", "dm2hi"):
+        assert getattr(p1, f) == getattr(p2, f) == getattr(p3, f), f
+
+
+@pytest.mark.parametrize("seed,n,nfix", [(0, 1, 0), (1, 30, 0), (2, 100, 0), (3, 60, 13)])
+def test_energy_plain(seed, n, nfix):
+    s = with_fixed(small_plain(seed, n), nfix)
+    o, r = Oracle(s), RefFortran(s)
+    eo, er = o.energy(), r.energy()
+    assert max(energy_errs(eo, er, floor=1e-12).values()) <= 1e-14
+    assert np.array_equal(o.forces(), r.forces())
+
+
+def test_energy_pore_dna():
+    s = with_fixed(small_pore(), 7, nonfree=(0, 5, 11))
+    o, r = Oracle(s), RefFortran(s)
+    eo, er = o.energy(), r.energy()
+    assert max(energy_errs(eo, er, floor=1e-12).values()) <= 1e-14
+    assert force_err(o.forces(), r.forces()) <= 1e-14
+    assert np.array_equal(o.warn(), r.warn())
+
+
+def test_energy_debye_and_svdw():
+    s = small_pore(seed=6, n_bp=6, n_ions=80).copy()
+    s.flags |= abi.QDEBY | abi.QSVDW
+    s.kappa, s.ikappa = 9.6, 1.0 / 9.6
+    s.scal = np.linspace(0.5, 1.5, s.nttyp + 1)
+    o, r = Oracle(s), RefFortran(s)
+    eo, er = o.energy(), r.energy()
+    assert max(energy_errs(eo, er, floor=1e-12).values()) <= 1e-14
+    assert force_err(o.forces(), r.forces()) <= 1e-14
+
+
+@pytest.mark.parametrize("which", ["plain", "pore"])
+def test_dynamics_bitexact(which):
+    s = small_plain(4, 50) if which == "plain" else small_pore(seed=7)
+    noise = np.random.default_rng(3).standard_normal((5, 3 * s.ntot))
+    o, r = Oracle(s), RefFortran(s)
+    o.upload_noise(noise)
+    r.upload_noise(noise)
+    for k in range(5):
+        o.step(1, abi.NOISE_INJECTED, k)
+        r.step(1, abi.NOISE_INJECTED, k)
+    po, pr = o.positions()[0], r.positions()[0]
+    assert float(np.max(np.abs(po - pr))) <= 1e-12
+    for a, b in zip(o.flux(), r.flux()):
+        assert np.array_equal(a, b)
+
+
+def test_interact():
+    s = small_pore(seed=8, n_bp=6, n_ions=100)
+    o, r = Oracle(s), RefFortran(s)
+    pos, typ = o.positions()
+    cases = [(pos[j], abs(typ[j]) if j >= s.nsites else 0, j)
+             for j in (0, 1, 2, 7, s.nsites - 1, s.nsites, s.ntot - 1)]
+    cases += [((1.0, 2.0, -25.0), s.nold + 1, s.ntot), ((15.0, 0.0, 0.0), s.nold + 2, s.ntot)]
+    for xyz, jt, j in cases:
+        a, b = o.interact(xyz, jt, j, True), r.interact(xyz, jt, j, True)
+        assert abs(a - b) <= 1e-14 * max(abs(b), 1.0), (j, a, b)
+    assert np.array_equal(o.warn(), r.warn())
+
+
+def test_grand():
+    s = small_pore(seed=10, n_bp=0, n_ions=80, dna=False).copy()
+    s.capacity = s.ntot + 150
+    gc = configs.gcmc_buffers(s, conc=1.0, buffer_len=8.0, mu=(-0.5, -0.4))
+    ib, nat = configs.assign_buffers(s, gc)
+    o, r = Oracle(s), RefFortran(s)
+    o.set_gcmc(gc, ib)
+    r.set_gcmc(gc, ib)
+    u = np.random.default_rng(4).random(10000)
+    pa = np.zeros((gc.nbuffer, s.capacity + 1), np.int32)
+    pb = pa.copy()
+    na, nb = nat.copy(), nat.copy()
+    ua, ea = o.grand(100, u, na, pa, 1, 0.0)
+    ub, eb = r.grand(100, u, nb, pb, 1, 0.0)
+    assert ua == ub and np.array_equal(na, nb) and np.array_equal(pa, pb)
+    assert abs(ea - eb) <= 1e-12 * max(abs(eb), 1.0)
+    assert o.ntot == r.ntot
+    assert np.array_equal(o.positions()[0], r.positions()[0])
+    assert np.array_equal(o.ibuffer(), r.ibuffer())
