@@ -79,6 +79,7 @@ struct bromoc_ctx {
     int32_t *warn = nullptr, *nfwd = nullptr, *nbwd = nullptr;
     double *noise = nullptr;
     int noise_rows = 0;
+    double *pbuf = nullptr;     // Philox Gaussians of the current chunk [GRAPH_K][3*cap]
     int64_t noise_stride = 0;
     int64_t *ctl = nullptr;     // [step, row0, counter0, seed]
     int64_t h_ctl[4] = {0, 0, 0, 0};   // host source of the ctl upload (outlives the copy)
@@ -490,6 +491,7 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     ACHK(dupload(c, &c->zcont, s->cntpts > 0 ? s->zcont : (const double *)nullptr,
                  std::max(0, s->cntpts)));
     ACHK(dalloc(c, &c->ctl, 4));
+    ACHK(dalloc(c, &c->pbuf, (size_t)GRAPH_K * 3 * cap));
     ACHK(dalloc(c, &c->iout, 1));
 #undef ACHK
 
@@ -675,11 +677,11 @@ static int autotune(bromoc_ctx *c) {
     if (c->tpb > 0) return BROMOC_OK;
     if (const char *env = std::getenv("BROMOC_TPB")) {
         int v = std::atoi(env);
-        if (v == 1 || v == 2 || v == 4) { c->tpb = v; return BROMOC_OK; }
+        if (v == 1 || v == 2 || v == 4 || v == 8) { c->tpb = v; return BROMOC_OK; }
     }
     int sst = ensure_sorted(c);
     if (sst) return sst;
-    const int cand[3] = {1, 2, 4};
+    const int cand[4] = {1, 2, 4, 8};   // k_step with 1, 2, 4 targets per block; k_wave
     double best = 1e300;
     int pick = 4;
     hipEvent_t e0, e1;
@@ -717,8 +719,15 @@ int bromoc_get_tuning(bromoc_ctx *c, int32_t *targets_per_block) {
     return BROMOC_OK;
 }
 
+// The BD loop.  Steps run in chunks of at most GRAPH_K: with Philox noise a chunk starts
+// with one launch that draws the Gaussians of all its steps (k_philox_fill), then its
+// steps follow; the device counter ctl[0] holds the chunk's first step.  Whole chunks
+// replay a captured graph (when graphs are on and no events are requested); the rest,
+// including the last step (which writes forces and energies), is launched directly.
+// ev, when given, receives an event pair around every step's pair kernel.
 static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t first_row,
-                     uint64_t seed, uint64_t counter0, bool want_energy) {
+                     uint64_t seed, uint64_t counter0, bool want_energy,
+                     std::vector<hipEvent_t> *ev = nullptr) {
     if (noise_mode != BROMOC_NOISE_INJECTED && noise_mode != BROMOC_NOISE_PHILOX)
         return fail(c, BROMOC_E_ARG, "noise_mode");
     if (noise_mode == BROMOC_NOISE_INJECTED) {
@@ -732,24 +741,23 @@ static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t 
     if (st0) return st0;
     c->h_ctl[0] = 0; c->h_ctl[1] = first_row; c->h_ctl[2] = (int64_t)counter0; c->h_ctl[3] = (int64_t)seed;
     HIPCHK(c, hipMemcpyAsync(c->ctl, c->h_ctl, sizeof(c->h_ctl), hipMemcpyHostToDevice, c->stream));
+    const bool philox = noise_mode == BROMOC_NOISE_PHILOX;
     StepParams p = params_for(c);
     p.mode = 1;
     p.noise_mode = noise_mode;
-    p.noise = c->noise;
-    p.noise_stride = c->noise_stride;
-    int done = 0;
-    static const bool use_graph = [] {
+    p.noise = philox ? c->pbuf : c->noise;
+    p.noise_stride = philox ? 3 * (int64_t)c->cap : c->noise_stride;
+    static const bool graphs_on = [] {
         const char *e = std::getenv("BROMOC_GRAPH");
         return !(e && e[0] == '0');
     }();
-    // whole K-step chunks replay a captured graph; the last step (which writes forces and
-    // energies) and any remainder are launched directly
-    while (use_graph && nsteps - done > GRAPH_K) {
+    int done = 0;
+    while (graphs_on && !ev && nsteps - done > GRAPH_K) {
         int sst = ensure_sorted(c, true);
         if (sst) return sst;
         bromoc_ctx::Graph &g = c->graph[c->cur];
         if (!g.exec || g.ntot != c->ntot || g.tpb != c->tpb || g.noise_mode != noise_mode ||
-            g.noise != c->noise || g.stride != c->noise_stride) {
+            g.noise != p.noise || g.stride != p.noise_stride) {
             if (g.exec) {
                 (void)hipGraphExecDestroy(g.exec);
                 g.exec = nullptr;
@@ -760,6 +768,9 @@ static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t 
             q.write_energy = 0;
             q.write_forces = 0;
             hipError_t le = hipSuccess;
+            if (philox)
+                le = launch_philox_fill(c->ctl, q.stream, GRAPH_K, c->ntot, q.noise_stride,
+                                        c->pbuf, c->stream);
             for (int k = 0; k < GRAPH_K && le == hipSuccess; k++) {
                 q.pos = c->pos[c->cur ^ (k & 1)];
                 q.poso = c->pos[c->cur ^ (k & 1) ^ 1];
@@ -779,26 +790,35 @@ static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t 
             (void)hipGraphDestroy(graph);
             if (ie != hipSuccess) return fail(c, BROMOC_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
             g.ntot = c->ntot; g.tpb = c->tpb; g.noise_mode = noise_mode;
-            g.noise = c->noise; g.stride = c->noise_stride;
+            g.noise = p.noise; g.stride = p.noise_stride;
         }
         HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
         done += GRAPH_K;   // K is even: the buffer parity is unchanged
         c->since_sort += GRAPH_K;
     }
-    for (int s = done; s < nsteps; s++) {
-        const bool last = s == nsteps - 1;
-        int sst = ensure_sorted(c);
-        if (sst) return sst;
-        p.pos = c->pos[c->cur];
-        p.poso = c->pos[c->cur ^ 1];
-        p.spos = c->spos[c->cur];
-        p.sposo = c->spos[c->cur ^ 1];
-        p.step_off = s - done;
-        p.write_energy = (last && want_energy) ? 1 : 0;
-        p.write_forces = last ? 1 : 0;
-        HIPCHK(c, enqueue_step(c, p));
-        c->cur ^= 1;
-        c->since_sort++;
+    while (done < nsteps) {
+        const int n = std::min(GRAPH_K, nsteps - done);
+        if (philox)
+            HIPCHK(c, launch_philox_fill(c->ctl, p.stream, n, c->ntot, p.noise_stride, c->pbuf, c->stream));
+        for (int r = 0; r < n; r++) {
+            const int s = done + r;
+            const bool last = s == nsteps - 1;
+            int sst = ensure_sorted(c);
+            if (sst) return sst;
+            p.pos = c->pos[c->cur];
+            p.poso = c->pos[c->cur ^ 1];
+            p.spos = c->spos[c->cur];
+            p.sposo = c->spos[c->cur ^ 1];
+            p.step_off = r;
+            p.write_energy = (last && want_energy) ? 1 : 0;
+            p.write_forces = last ? 1 : 0;
+            if (ev) HIPCHK(c, enqueue_step(c, p, (*ev)[2 * s], (*ev)[2 * s + 1]));
+            else HIPCHK(c, enqueue_step(c, p));
+            c->cur ^= 1;
+            c->since_sort++;
+        }
+        done += n;
+        if (done < nsteps) HIPCHK(c, launch_advance(c->ctl, n, c->stream));
     }
     return BROMOC_OK;
 }
@@ -821,50 +841,31 @@ int bromoc_time_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t
                       uint64_t seed, uint64_t counter0, double *total_ms, double *kernel_ms) {
     if (!c || nsteps <= 0 || !total_ms || !kernel_ms) return BROMOC_E_ARG;
     (void)hipSetDevice(c->device);
-    std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
-    for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
-    if (noise_mode == BROMOC_NOISE_INJECTED) {
-        if (!c->noise || first_row < 0 || (int64_t)first_row + nsteps > c->noise_rows ||
-            3 * (int64_t)c->ntot > c->noise_stride)
-            return fail(c, BROMOC_E_STATE, "noise rows out of range");
+    std::vector<hipEvent_t> ev(2 * (size_t)nsteps, nullptr);
+    int st = BROMOC_OK;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) st = fail(c, BROMOC_E_HIP, "hipEventCreate");
+    if (!st) st = run_steps(c, nsteps, noise_mode, first_row, seed, counter0, false, &ev);
+    if (!st) st = sync_err(c);
+    if (!st) {
+        double sum = 0.0;
+        for (int s = 0; s < nsteps; s++) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]) != hipSuccess) {
+                st = fail(c, BROMOC_E_HIP, "hipEventElapsedTime");
+                break;
+            }
+            sum += ms;
+        }
+        float tot = 0.f;
+        if (!st && hipEventElapsedTime(&tot, ev[0], ev[2 * nsteps - 1]) != hipSuccess)
+            st = fail(c, BROMOC_E_HIP, "hipEventElapsedTime");
+        *total_ms = tot;
+        *kernel_ms = sum / nsteps;
     }
-    int st0 = autotune(c);
-    if (st0) return st0;
-    c->h_ctl[0] = 0; c->h_ctl[1] = first_row; c->h_ctl[2] = (int64_t)counter0; c->h_ctl[3] = (int64_t)seed;
-    HIPCHK(c, hipMemcpyAsync(c->ctl, c->h_ctl, sizeof(c->h_ctl), hipMemcpyHostToDevice, c->stream));
-    StepParams p = params_for(c);
-    p.mode = 1;
-    p.noise_mode = noise_mode;
-    p.noise = c->noise;
-    p.noise_stride = c->noise_stride;
-    for (int s = 0; s < nsteps; s++) {
-        int sst = ensure_sorted(c);
-        if (sst) return sst;
-        p.pos = c->pos[c->cur];
-        p.poso = c->pos[c->cur ^ 1];
-        p.spos = c->spos[c->cur];
-        p.sposo = c->spos[c->cur ^ 1];
-        p.step_off = s;
-        p.write_energy = 0;
-        p.write_forces = s == nsteps - 1;
-        HIPCHK(c, enqueue_step(c, p, ev[2 * s], ev[2 * s + 1]));
-        c->cur ^= 1;
-        c->since_sort++;
-    }
-    int st = sync_err(c);
-    if (st) return st;
-    double sum = 0.0;
-    for (int s = 0; s < nsteps; s++) {
-        float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
-        sum += ms;
-    }
-    float tot = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&tot, ev[0], ev[2 * nsteps - 1]));   // ion launches
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    *total_ms = tot;
-    *kernel_ms = sum / nsteps;
-    return BROMOC_OK;
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return st;
 }
 
 int bromoc_dynamics(bromoc_ctx *c, int32_t noise_mode, int32_t row, uint64_t seed,

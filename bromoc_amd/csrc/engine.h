@@ -47,7 +47,7 @@ struct StepParams {
     uint32_t flags;
     int32_t mode;            // 0: ENERGY (forces out, no update), 1: ENERGY + DYNAMICS
     int32_t write_energy, write_forces;
-    int32_t tpb, all_efpot;  // targets per block of k_step (1, 2 or 4); every pair tabulated
+    int32_t tpb, all_efpot;  // k_step targets per block (1, 2, 4) or 8: k_wave; all tabulated
     int32_t tbeg, tend;      // target range of this launch (sites 0..nsites-1 come first)
     // positions (x, y, z, atnam2 type in the low word of w) in, updated positions out
     const double4 *pos;
@@ -95,7 +95,8 @@ struct StepParams {
     int32_t cntpts, noise_mode;
     const double *zcont;
     int32_t *nfwd, *nbwd;
-    const double *noise;           // injected: [row][ntot*3]
+    const double *noise;           // Gaussian rows [row][3*i]: injected rows, or the Philox
+                                   // rows of the current chunk (noise_row)
     int64_t noise_stride;          // doubles per row
     const int64_t *step_ctr;       // device step counter (graph-friendly)
     int64_t step_off;              // this launch's step = *step_ctr + step_off
@@ -128,6 +129,8 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s);  // k_step (pairs, 
 hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
                                 hipStream_t s);
 hipError_t launch_advance(int64_t *ctr, int64_t by, hipStream_t s);
+hipError_t launch_philox_fill(const int64_t *ctl, uint64_t stream, int nrows, int ntot,
+                              int64_t stride, double *out, hipStream_t s);
 hipError_t launch_philox_raw(const uint64_t *ctr, const uint64_t *key, int n, uint64_t *out,
                              hipStream_t s);
 

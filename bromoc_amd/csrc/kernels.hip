@@ -373,6 +373,15 @@ __device__ __forceinline__ void efpot_eval(const DevPair &p, const double4 *__re
     }
 }
 
+// Gaussians of particle i for this launch's step: injected rows are addressed by the
+// step counter, Philox rows were generated for the current chunk of steps by
+// k_philox_fill (row = step_off).
+__device__ __forceinline__ const double *noise_row(const StepParams &P, int i) {
+    const int64_t row = P.noise_mode == 0 ? P.step_ctr[1] + P.step_ctr[0] + P.step_off
+                                          : P.step_off;
+    return P.noise + row * P.noise_stride + 3 * (int64_t)i;
+}
+
 // One partner j of a target: the table pair of getyd (polynom.f90:197-230) evaluated
 // without branches.  Wall, tail and table forms are all formed from rinv = 1/sqrt(d2);
 // the knot index is clamped into the pair's table, so the gather is always in bounds and
@@ -396,6 +405,37 @@ __device__ __forceinline__ double pair_table(const DevPair &pp, const double4 *_
     return wall ? 12.0 * c12 * ix2 : (tail ? (6.0 * c3 + f) * ix2 : ypt);
 }
 
+// Table pair (getyd, polynom.f90:197-230) in tail-first form: every lane forms the
+// analytic tail sc3/r^6 + fct/r from rinv = 1/sqrt(d2); only lanes inside the table range
+// (d2 <= dm2hi) enter the branch that replaces it with the wall or the tabulated quadratic,
+// so far partners cost ~25 operations and near ones pay for the knot gather alone.
+// tl = (sc3, tail fct, dm2hi, unused) of the pair type.
+__device__ __forceinline__ double pair_near_far(const DevPair &pp, const double4 &tl,
+                                               const double4 *__restrict__ ep, double ires,
+                                               double d2, double &y) {
+    const double rinv = rsqrt_nr(d2);
+    const double ix2 = rinv * rinv;
+    const double c = tl.x * pow3(ix2);
+    const double f = tl.y * rinv;
+    y = c + f;
+    double de = (6.0 * c + f) * ix2;
+    if (d2 <= tl.z) {
+        if (d2 < pp.dm2lo) {
+            const double cw = pp.sc1 * pow6(ix2);
+            y = cw + pp.sc2;
+            de = 12.0 * cw * ix2;
+        } else {
+            const double x = d2 * rinv;
+            int k = (int)((x - pp.dmi) * ires) + pp.nxi;
+            k = min(max(k, pp.nxi), pp.nlast);
+            const double4 e = ep[k];
+            y = e.x + x * (e.y + x * e.z);
+            de = -e.y * rinv - 2.0 * e.z;
+        }
+    }
+    return de;
+}
+
 // k_step: T target particles per block of 256 threads (targets tbeg.. of the launch).
 // Every thread walks the partner set with stride 256, two partners per iteration, and
 // evaluates each partner against all T targets.  Per-target sums: butterfly within each
@@ -405,7 +445,7 @@ __device__ __forceinline__ double pair_table(const DevPair &pp, const double4 *_
 template <int FEAT, int T>
 __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
     __shared__ DevPair s_row[T][MAX_LDS_TYPES];   // pair parameters by partner type
-    __shared__ double2 s_tail[T][MAX_LDS_TYPES];  // (sc3, tail fct) by partner type
+    __shared__ double4 s_tail[T][MAX_LDS_TYPES];  // (sc3, tail fct, dm2hi, 0) by partner type
     __shared__ double s_red[WPB][T][7];
     __shared__ double s_dg[T][5];
     __shared__ double s_g[T][3];
@@ -434,26 +474,19 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
         for (int k = tid; k < nt1; k += BLOCK) {
             const DevPair d = P.rows[row + k];
             s_row[a][k] = d;
-            s_tail[a][k] = make_double2(d.sc3, d.fct);
+            s_tail[a][k] = make_double4(d.sc3, d.fct, d.dm2hi, 0.0);
         }
     }
     __syncthreads();
 
-    // Gaussians of the BD update, drawn by the last wavefront (lanes 0..T-1) before its
-    // share of the pair loop, so their latency overlaps the other wavefronts' pair work
+    // Gaussians of the BD update, fetched by the last wavefront (lanes 0..T-1) before its
+    // share of the pair loop
     if constexpr ((FEAT & F_UPDATE) != 0) {
         const int a = tid - (BLOCK - WAVE);
         if (a >= 0 && a < T && tval[a]) {
             const int i = ti[a];
-            const int64_t step = P.step_ctr[0] + P.step_off;
-            double g[3];
-            if (P.noise_mode == 0) {
-                const double *nr = P.noise + (P.step_ctr[1] + step) * P.noise_stride + 3 * (int64_t)i;
-                g[0] = nr[0]; g[1] = nr[1]; g[2] = nr[2];
-            } else {
-                philox_gauss3((uint64_t)P.step_ctr[3], (uint64_t)(P.step_ctr[2] + step), P.stream, i, g);
-            }
-            s_g[a][0] = g[0]; s_g[a][1] = g[1]; s_g[a][2] = g[2];
+            const double *nr = noise_row(P, i);
+            s_g[a][0] = nr[0]; s_g[a][1] = nr[1]; s_g[a][2] = nr[2];
         }
     }
 
@@ -464,7 +497,6 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
     // ---- pair loop (energy.mono.f90:117-194, 584-687)
     const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
     if (nonbond) {
-        const double r2near = P.r2near;
         for (int k0 = tid; k0 < ntot; k0 += 2 * BLOCK) {
             // partners in spatial order, two per iteration (two load chains in flight)
             const int k1 = k0 + BLOCK;
@@ -487,19 +519,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
                     // the higher-index ion of an ion-ion pair
                     const bool ce = !skip && (tsite[a] || (j >= ns && j < ti[a]));
                     double y, de;
-                    if (__all(d2 > r2near)) {
-                        // every lane beyond every table: the analytic tail sc3/r^6 + fct/r
-                        // (getyd, polynom.f90:208-223), same arithmetic as the table path
-                        const double2 tp = s_tail[a][t2j];
-                        const double rinv = rsqrt_nr(d2);
-                        const double ix2 = rinv * rinv;
-                        const double c = tp.x * pow3(ix2);
-                        const double f = tp.y * rinv;
-                        y = c + f;
-                        de = (6.0 * c + f) * ix2;
-                        et[a] += ce ? y : 0.0;
-                    } else if (P.all_efpot || (s_row[a][t2j].flags & PF_EFPOT)) {
-                        de = pair_table(s_row[a][t2j], P.ep, P.ires, d2, y);
+                    if (P.all_efpot || (s_row[a][t2j].flags & PF_EFPOT)) {
+                        de = pair_near_far(s_row[a][t2j], s_tail[a][t2j], P.ep, P.ires, d2, y);
                         et[a] += ce ? y : 0.0;
                     } else {
                         double q0 = 0.0, q1 = 0.0, q2 = 0.0;
@@ -756,6 +777,249 @@ __global__ __launch_bounds__(BLOCK) void k_dna(StepParams P) {
     }
 }
 
+// k_wave: one wavefront per target, four targets per block.  The block streams the
+// partner array through LDS in tiles of 256, double-buffered (tile t+1 is fetched from
+// global memory while tile t is evaluated); each lane evaluates four partners of a tile
+// against its wavefront's target, groups of 64 taking the analytic-tail path when every
+// lane is beyond the table range.  The wavefront's butterfly sum is the target's total;
+// lane 0 finishes it as in k_step.
+template <int FEAT>
+__global__ __launch_bounds__(BLOCK) void k_wave(StepParams P) {
+    __shared__ double4 s_part[2][BLOCK];
+    __shared__ DevPair s_row[WPB][MAX_LDS_TYPES];
+    __shared__ double4 s_tail[WPB][MAX_LDS_TYPES];
+
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
+    const int ns = P.nsites, nf = P.nfix, ntot = P.ntot;
+    const int nt1 = P.nttyp + 1;
+    const int w = P.tbeg + blockIdx.x * WPB + wv;
+    const bool valid = w < P.tend;
+    const int i = !valid ? 0 : (w < ns ? w : w + nf);
+    const bool is_site = i < ns;
+    const double4 pa = P.pos[i];
+    const int t2i = __double2loint(pa.w);
+    const bool free_me = !is_site || ((FEAT & F_DNA) && P.stfree[i] != 0);
+    if (lane < nt1) {
+        const DevPair d = P.rows[t2i * nt1 + lane];
+        s_row[wv][lane] = d;
+        s_tail[wv][lane] = make_double4(d.sc3, d.fct, d.dm2hi, 0.0);
+    }
+    double4 nxt = tid < ntot ? P.spos[tid] : make_double4(0.0, 0.0, 0.0, 0.0);
+    s_part[0][tid] = nxt;
+    __syncthreads();
+
+    double fx = 0.0, fy = 0.0, fz = 0.0, et = 0.0, eel = 0.0, evd = 0.0, esr = 0.0;
+    const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
+    const int ntiles = (ntot + BLOCK - 1) / BLOCK;
+    for (int t = 0; t < ntiles; t++) {
+        const int cur = t & 1;
+        const int jn = (t + 1) * BLOCK + tid;
+        if (t + 1 < ntiles) nxt = jn < ntot ? P.spos[jn] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const int kend = min(BLOCK, ntot - t * BLOCK);
+        if (valid && nonbond) {
+#pragma unroll
+            for (int q = 0; q < WPB; q++) {
+                const int k = q * WAVE + lane;
+                const bool jok = k < kend;
+                const double4 pj = s_part[cur][k];
+                const int t2j = __double2loint(pj.w);
+                const int j = __double2hiint(pj.w);
+                const bool skip = !jok || j == i || (is_site && j < ns);
+                const double dx = pa.x - pj.x, dy = pa.y - pj.y, dz = pa.z - pj.z;
+                double d2 = dx * dx + dy * dy + dz * dz;
+                d2 = skip ? 1e30 : d2;
+                const bool ce = !skip && (is_site || (j >= ns && j < i));
+                const int tj = skip ? 0 : t2j;
+                double y, de;
+                if (P.all_efpot || (s_row[wv][tj].flags & PF_EFPOT)) {
+                    de = pair_near_far(s_row[wv][tj], s_tail[wv][tj], P.ep, P.ires, d2, y);
+                    et += ce ? y : 0.0;
+                } else {
+                    double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+                    de = pair_analytic(s_row[wv][tj], P, d2, q0, q1, q2);
+                    eel += ce ? q0 : 0.0;
+                    evd += ce ? q1 : 0.0;
+                    esr += ce ? q2 : 0.0;
+                }
+                de = (skip || !free_me) ? 0.0 : de;
+                fx += de * dx;
+                fy += de * dy;
+                fz += de * dz;
+            }
+        }
+        if (t + 1 < ntiles) s_part[cur ^ 1][tid] = nxt;
+        __syncthreads();
+    }
+    if (!valid) return;
+
+    // ---- grids (lanes 0..7)
+    double egx = 0.0, egv = 0.0;
+    bool warned = false;
+    if constexpr ((FEAT & F_PHIX) != 0) {
+        if (lane < 8) {
+            const GridGeom &g = P.gx;
+            double chi;
+            bool ok;
+            if (is_site) { ok = P.namsite[i] == 1; chi = P.cgnuc; }
+            else { ok = true; chi = P.cg[abs(P.typei[i])]; }
+            ok = ok && pa.x <= g.xbcen + g.tranx && pa.x >= g.xbcen - g.tranx &&
+                 pa.y <= g.ybcen + g.trany && pa.y >= g.ybcen - g.trany &&
+                 pa.z <= g.zbcen + g.tranz && pa.z >= g.zbcen - g.tranz;
+            if (ok) {
+                double gx_, gy_, gz_;
+                int ix, iy, iz;
+                grid_cell(g, pa.x, pa.y, pa.z, gx_, gy_, gz_, ix, iy, iz);
+                const double one1 = 1.0;
+                const int n1 = ix + (lane >> 2), n2 = iy + ((lane >> 1) & 1), n3 = iz + (lane & 1);
+                double ai = gx_ - n1 * g.dcel;
+                const double aisign = fsign(one1, ai);
+                ai = one1 - fabs(ai) * g.idcel;
+                double bi = gy_ - n2 * g.dcel;
+                const double bisign = fsign(one1, bi);
+                bi = one1 - fabs(bi) * g.idcel;
+                double ci = gz_ - n3 * g.dcel;
+                const double cisign = fsign(one1, ci);
+                ci = one1 - fabs(ci) * g.idcel;
+                const double fi = ai * bi * ci;
+                const double ph = (double)P.phix[((size_t)n1 * g.ny + n2) * g.nz + n3];
+                const double prefac = ph * P.celec * chi * g.idcel;
+                if (ai < (1.0 - RSMALL) && ai > RSMALL) fx += aisign * bi * ci * prefac;
+                if (bi < (1.0 - RSMALL) && bi > RSMALL) fy += bisign * ai * ci * prefac;
+                if (ci < (1.0 - RSMALL) && ci > RSMALL) fz += cisign * ai * bi * prefac;
+                egx += fi * chi * ph * P.celec;
+            }
+        }
+    }
+    if constexpr ((FEAT & F_PHIV) != 0) {
+        const GridGeom &g = P.gv;
+        const bool ok = pa.x <= g.xbcen + g.tranx && pa.x >= g.xbcen - g.tranx &&
+                        pa.y <= g.ybcen + g.trany && pa.y >= g.ybcen - g.trany && pa.z <= P.vzmax &&
+                        pa.z >= P.vzmin;
+        double ph = 0.0;
+        if (ok && lane < 8) {
+            double gx_, gy_, gz_;
+            int ix, iy, iz;
+            grid_cell(g, pa.x, pa.y, pa.z, gx_, gy_, gz_, ix, iy, iz);
+            double esvdw = P.svdw;
+            if (P.flags & QSVDW) esvdw = P.svdw * P.scal[t2i];
+            ph = tri_corner(g, P.phiv, gx_, gy_, gz_, ix, iy, iz, lane, esvdw, fx, fy, fz, egv);
+        }
+        double sum = ph;
+        sum += __shfl_xor(sum, 1, WAVE);
+        sum += __shfl_xor(sum, 2, WAVE);
+        sum += __shfl_xor(sum, 4, WAVE);
+        warned = ok && sum >= P.thold8;
+    }
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    if (P.write_energy) {
+        et = wave_sum(et);
+        eel = wave_sum(eel);
+        evd = wave_sum(evd);
+        esr = wave_sum(esr);
+        egx = wave_sum(egx);
+        egv = wave_sum(egv);
+    }
+    if (lane != 0) return;
+    double F[3] = {fx, fy, fz};
+    if constexpr ((FEAT & F_DNA) != 0) {
+        if (is_site) {
+            const double4 fd = P.fdna[i];
+            F[0] += fd.x; F[1] += fd.y; F[2] += fd.z;
+        }
+    }
+    if (warned) atomicAdd(&P.warn[t2i], 1);
+    if (P.write_energy) {
+        double *ep_ = P.epart + i;
+        const size_t cs = P.ecap;
+        ep_[E_GSBPA * cs] = egx;
+        ep_[E_VDWGD * cs] = egv;
+        if (is_site) {
+            ep_[E_EFPOTMX * cs] = et; ep_[E_QQMX * cs] = eel;
+            ep_[E_VDWMX * cs] = evd; ep_[E_SRPMFMX * cs] = esr;
+        } else {
+            ep_[E_EFPOT * cs] = et; ep_[E_ELEC * cs] = eel;
+            ep_[E_VDW * cs] = evd; ep_[E_SRPMF * cs] = esr;
+        }
+    }
+    if (P.write_forces) {
+        P.fx[i] = F[0];
+        P.fy[i] = F[1];
+        P.fz[i] = F[2];
+    }
+    if constexpr ((FEAT & F_UPDATE) != 0) {
+        if (!free_me) {
+            P.poso[i] = pa;
+            P.sposo[P.rank[i]] = pa;
+            return;
+        }
+        double fac1, fac2;
+        int itype = 0;
+        if (is_site) {
+            fac1 = P.fact0n1;
+            fac2 = P.fact0n2;
+        } else {
+            itype = abs(P.typei[i]);
+            fac1 = P.fact1a[itype];
+            fac2 = P.fact2a[itype];
+        }
+        const double *g = noise_row(P, i);
+        const double bdmax = P.bdmax;
+        double delx = F[0] * fac1, dely = F[1] * fac1, delz = F[2] * fac1;
+        if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
+        if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
+        if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
+        double xn = pa.x + delx + fac2 * g[0];
+        double yn = pa.y + dely + fac2 * g[1];
+        double zn = pa.z + delz + fac2 * g[2];
+        if (!is_site && (P.flags & QCOUNTION)) {
+            const int row = itype - P.nold - 1;
+            for (int c = 0; c < P.cntpts; c++) {
+                const double zc = P.zcont[c];
+                if (pa.z < zc && zn >= zc) atomicAdd(&P.nfwd[row * P.cntpts + c], 1);
+                else if (pa.z >= zc && zn < zc) atomicAdd(&P.nbwd[row * P.cntpts + c], 1);
+            }
+        }
+        if (!is_site || !(P.flags & QDNAFREE)) fixcoor(P, xn, yn, zn);
+        const double4 pn = make_double4(xn, yn, zn, pa.w);
+        P.poso[i] = pn;
+        P.sposo[P.rank[i]] = pn;
+    }
+}
+
+// Philox Gaussians of rows 0..nrows-1 of the current chunk: row r holds step
+// ctl[0] + r, counter ctl[2] + ctl[0] + r, key (ctl[3], 0), particle i (BROMOC_NOISE_PHILOX)
+__global__ void k_philox_fill(const int64_t *ctl, uint64_t stream, int nrows, int ntot,
+                              int64_t stride, double *out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)nrows * ntot) return;
+    const int r = (int)(k / ntot), i = (int)(k % ntot);
+    double g[3];
+    philox_gauss3((uint64_t)ctl[3], (uint64_t)(ctl[2] + ctl[0] + r), stream, i, g);
+    double *o = out + r * stride + 3 * (int64_t)i;
+    o[0] = g[0]; o[1] = g[1]; o[2] = g[2];
+}
+
+hipError_t launch_philox_fill(const int64_t *ctl, uint64_t stream, int nrows, int ntot,
+                              int64_t stride, double *out, hipStream_t s) {
+    const int64_t n = (int64_t)nrows * ntot;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_philox_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ctl,
+                       stream, nrows, ntot, stride, out);
+    return hipGetLastError();
+}
+
+static void dispatch_wave(const StepParams &p, int feat, hipStream_t s) {
+    dim3 grid((p.tend - p.tbeg + WPB - 1) / WPB), block(BLOCK);
+#define CASE(F) case F: hipLaunchKernelGGL((k_wave<F>), grid, block, 0, s, p); break;
+    switch (feat) {
+        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+        CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
+    }
+#undef CASE
+}
+
 template <int T>
 static void dispatch_step(const StepParams &p, int feat, hipStream_t s) {
     dim3 grid((p.tend - p.tbeg + T - 1) / T), block(BLOCK);
@@ -783,7 +1047,8 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s) {
     switch (p.tpb) {
         case 1: dispatch_step<1>(p, feat, s); break;
         case 2: dispatch_step<2>(p, feat, s); break;
-        default: dispatch_step<4>(p, feat, s); break;
+        case 4: dispatch_step<4>(p, feat, s); break;
+        default: dispatch_wave(p, feat, s); break;   // one wavefront per target
     }
     return hipGetLastError();
 }

@@ -216,9 +216,9 @@ int bromoc_step(bromoc_ctx *ctx, int32_t nsteps, int32_t noise_mode, int32_t fir
 int bromoc_time_steps(bromoc_ctx *ctx, int32_t nsteps, int32_t noise_mode, int32_t first_row,
                       uint64_t seed, uint64_t counter0, double *total_ms, double *kernel_ms);
 
-/* Launch shape picked by the library's autotuner for this system (targets per block of
- * the step kernel: 1, 2 or 4); runs the tuner if it has not run.  No counterpart in the
- * reference. */
+/* Launch shape picked by the library's autotuner for this system: 1, 2 or 4 targets per
+ * block of the block-per-target step kernel, or 8 for the wavefront-per-target kernel;
+ * runs the tuner if it has not run.  No counterpart in the reference. */
 int bromoc_get_tuning(bromoc_ctx *ctx, int32_t *targets_per_block);
 
 /* One DYNAMICS pass with the current forces (as left by bromoc_energy). */
