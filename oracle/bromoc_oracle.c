@@ -1286,6 +1286,12 @@ EXPORT int orc_set_gcmc(orc_ctx *c, const bromoc_gcmc *g, const int32_t *ibuffer
     return BROMOC_OK;
 }
 
+EXPORT int orc_set_ibuffer(orc_ctx *c, const int32_t *ib) {
+    if (!c->ibuffer) return BROMOC_E_STATE;
+    memcpy(c->ibuffer, ib, sizeof(int32_t) * c->s.ntot);
+    return BROMOC_OK;
+}
+
 EXPORT int orc_get_ibuffer(orc_ctx *c, int32_t *ib) {
     memcpy(ib, c->ibuffer, sizeof(int32_t) * c->s.ntot);
     return BROMOC_OK;

@@ -2,7 +2,11 @@
 
 Oracle     -- oracle/liboracle.so, the C restatement (bromoc_oracle.c)
 RefFortran -- oracle/_ref/libbromoc_ref.so, the reference Fortran routines themselves
-              (built only where the reference sources exist)
+              (built only where the reference sources exist); with lib=SHIM_SO the same
+              harness drives oracle/_ref/libbromoc_shimcheck.so, where the reference's
+              ENERGY/DYNAMICS0/DYNAMICS0NUC/INTERACT/GRAND are replaced by the Fortran
+              front-end bromoc_amd/fortran/bromoc_gpu.f90 over the C ABI, served by the
+              C restatement (shim_adapter.c)
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
 """
@@ -18,6 +22,7 @@ from bromoc_amd import abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libbromoc_ref.so")
+SHIM_SO = os.path.join(HERE, "_ref", "libbromoc_shimcheck.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -199,15 +204,14 @@ class Oracle(_Base):
         return used.value, e.value
 
 
-_rlib = None
+_rlibs = {}
 
 
-def ref_lib():
-    global _rlib
-    if _rlib is None:
-        if not ref_available():
-            raise ImportError(f"{REF_SO} missing (needs the reference sources; make -C oracle)")
-        L = C.CDLL(REF_SO)
+def ref_lib(path=REF_SO):
+    if path not in _rlibs:
+        if not os.path.exists(path):
+            raise ImportError(f"{path} missing (needs the reference sources; make -C oracle)")
+        L = C.CDLL(path)
         P = C.POINTER
         for name, args, res in [
             ("ref_set_nindex", [C.c_int32, C.c_int32], None),
@@ -231,8 +235,13 @@ def ref_lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = res
-        _rlib = L
-    return _rlib
+        if path == SHIM_SO:
+            L.bromoc_gpu_reset.argtypes = []
+            L.bromoc_gpu_reset.restype = None
+            L.bromoc_gpu_set_seed.argtypes = [C.c_int64, C.c_int64]
+            L.bromoc_gpu_set_seed.restype = None
+        _rlibs[path] = L
+    return _rlibs[path]
 
 
 def ref_table_builder(n, r, u, chr_, fct):
@@ -245,8 +254,10 @@ def ref_table_builder(n, r, u, chr_, fct):
 class RefFortran(_Base):
     """The reference routines (module-global state: one live instance at a time)."""
 
-    def __init__(self, system):
-        self.L = ref_lib()
+    def __init__(self, system, lib=REF_SO):
+        self.L = ref_lib(lib)
+        if lib == SHIM_SO:
+            self.L.bromoc_gpu_reset()
         self.system = system
         self.L.ref_set_nindex(system.ndna, system.nion)
         st, keep = system.to_struct()

@@ -1,0 +1,85 @@
+/* TEST INFRASTRUCTURE ONLY.  The C ABI of include/bromoc_gpu.h served by the C oracle
+ * (liboracle.so), so that the Fortran front-end bromoc_amd/fortran/bromoc_gpu.f90 can be
+ * linked into the reference driver's routines and checked on a machine without a GPU
+ * (oracle/_ref/libbromoc_shimcheck.so).  Every entry point forwards to its orc_ twin. */
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../include/bromoc_gpu.h"
+
+typedef struct orc_ctx orc_ctx;
+orc_ctx *orc_create(const bromoc_system *s);
+void orc_destroy(orc_ctx *c);
+int orc_check_flags(uint32_t flags);
+int orc_set_positions(orc_ctx *c, int32_t n, const double *x, const double *y, const double *z,
+                      const int32_t *t);
+int orc_get_positions(orc_ctx *c, double *x, double *y, double *z, int32_t *t, int32_t *n);
+int orc_get_forces(orc_ctx *c, double *fx, double *fy, double *fz);
+int orc_energy(orc_ctx *c, int32_t qf, bromoc_energies *e);
+int orc_step(orc_ctx *c, int32_t n, int32_t mode, int32_t row, uint64_t seed, uint64_t ctr,
+             bromoc_energies *e);
+int orc_dynamics(orc_ctx *c, int32_t mode, int32_t row, uint64_t seed, uint64_t ctr);
+int orc_get_flux(orc_ctx *c, int32_t *nf, int32_t *nb);
+int orc_get_warn(orc_ctx *c, int32_t *w);
+int orc_interact(orc_ctx *c, double x, double y, double z, int32_t jt, int32_t j, int32_t al,
+                 double *d);
+int orc_set_gcmc(orc_ctx *c, const bromoc_gcmc *g, const int32_t *ib);
+int orc_set_ibuffer(orc_ctx *c, const int32_t *ib);
+int orc_get_ibuffer(orc_ctx *c, int32_t *ib);
+int orc_grand(orc_ctx *c, int32_t ng, const double *u, int64_t nu, int64_t *used, int32_t *nat,
+              int32_t *prob, int64_t icyc, double *ener);
+
+#define O(ctx) ((orc_ctx *)(ctx))
+
+int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
+    (void)device;
+    if (!s || !out) return BROMOC_E_ARG;
+    int rc = orc_check_flags((uint32_t)s->flags);
+    if (rc) return rc;
+    orc_ctx *c = orc_create(s);
+    if (!c) return BROMOC_E_ARG;
+    *out = (bromoc_ctx *)c;
+    return BROMOC_OK;
+}
+int bromoc_destroy(bromoc_ctx *ctx) { orc_destroy(O(ctx)); return BROMOC_OK; }
+const char *bromoc_last_error(const bromoc_ctx *ctx) { (void)ctx; return "oracle"; }
+int bromoc_set_positions(bromoc_ctx *ctx, int32_t n, const double *x, const double *y,
+                         const double *z, const int32_t *t) {
+    return orc_set_positions(O(ctx), n, x, y, z, t);
+}
+int bromoc_get_positions(bromoc_ctx *ctx, double *x, double *y, double *z, int32_t *t,
+                         int32_t *n) {
+    return orc_get_positions(O(ctx), x, y, z, t, n);
+}
+int bromoc_get_forces(bromoc_ctx *ctx, double *fx, double *fy, double *fz) {
+    return orc_get_forces(O(ctx), fx, fy, fz);
+}
+int bromoc_energy(bromoc_ctx *ctx, int32_t qf, bromoc_energies *e) {
+    return orc_energy(O(ctx), qf, e);
+}
+int bromoc_step(bromoc_ctx *ctx, int32_t n, int32_t mode, int32_t row, uint64_t seed,
+                uint64_t ctr, bromoc_energies *e) {
+    return orc_step(O(ctx), n, mode, row, seed, ctr, e);
+}
+int bromoc_dynamics(bromoc_ctx *ctx, int32_t mode, int32_t row, uint64_t seed, uint64_t ctr) {
+    return orc_dynamics(O(ctx), mode, row, seed, ctr);
+}
+int bromoc_get_flux(bromoc_ctx *ctx, int32_t *nf, int32_t *nb) {
+    return orc_get_flux(O(ctx), nf, nb);
+}
+int bromoc_get_warn(bromoc_ctx *ctx, int32_t *w) { return orc_get_warn(O(ctx), w); }
+int bromoc_interact(bromoc_ctx *ctx, double x, double y, double z, int32_t jt, int32_t j,
+                    int32_t al, double *d) {
+    return orc_interact(O(ctx), x, y, z, jt, j, al, d);
+}
+int bromoc_set_gcmc(bromoc_ctx *ctx, const bromoc_gcmc *g) {
+    return orc_set_gcmc(O(ctx), g, NULL);
+}
+int bromoc_set_ibuffer(bromoc_ctx *ctx, const int32_t *ib) {
+    return orc_set_ibuffer(O(ctx), ib);
+}
+int bromoc_get_ibuffer(bromoc_ctx *ctx, int32_t *ib) { return orc_get_ibuffer(O(ctx), ib); }
+int bromoc_grand(bromoc_ctx *ctx, int32_t ng, const double *u, int64_t nu, int64_t *used,
+                 int32_t *nat, int32_t *prob, int64_t icyc, double *ener) {
+    return orc_grand(O(ctx), ng, u, nu, used, nat, prob, icyc, ener);
+}

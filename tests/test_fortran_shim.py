@@ -1,0 +1,103 @@
+"""The Fortran front-end (bromoc_amd/fortran/bromoc_gpu.f90) linked into the reference's own
+routines in place of ENERGY, DYNAMICS0, DYNAMICS0NUC, INTERACT and GRAND, with its C-ABI
+calls served by the C oracle (oracle/shim_adapter.c), against the unmodified reference
+routines: shows that the front-end hands the driver's module state (flags, pair tables,
+knot ranges, DNA lists, grids, GCMC buffers) to the C ABI and writes the results back
+where the driver reads them.  CPU only; skipped where oracle/_ref is not built.
+"""
+import numpy as np
+import pytest
+
+from bromoc_amd import abi, configs
+from oracle.oracle import SHIM_SO, RefFortran, oracle_table_builder, ref_available
+
+from helpers import energy_errs, force_err, with_fixed
+
+pytestmark = pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built here")
+
+
+def small_plain(seed, n):
+    return configs.plain_ions(seed=seed, n_per_type=n, table_builder=oracle_table_builder)
+
+
+def small_pore(seed=5, n_bp=8, n_ions=120, dna=True):
+    return configs.pore_dna(seed=seed, n_bp=n_bp, n_ions=n_ions, box=(40.0, 40.0, 60.0),
+                            dcel=0.5, rpore=12.0, half_slab=10.0, dna=dna,
+                            table_builder=oracle_table_builder)
+
+
+def _pair(s):
+    # one live instance per library: module state is global in each
+    return RefFortran(s, lib=SHIM_SO), RefFortran(s)
+
+
+@pytest.mark.parametrize("which", ["plain", "pore", "pore_deby_svdw"])
+def test_energy_through_front_end(which):
+    if which == "plain":
+        s = with_fixed(small_plain(1, 40), 9)
+    elif which == "pore":
+        s = with_fixed(small_pore(), 7, nonfree=(0, 5, 11))
+    else:
+        s = small_pore(seed=6, n_bp=6, n_ions=80).copy()
+        s.flags |= abi.QDEBY | abi.QSVDW
+        s.kappa, s.ikappa = 9.6, 1.0 / 9.6
+        s.scal = np.linspace(0.5, 1.5, s.nttyp + 1)
+    f, r = _pair(s)
+    ef, er = f.energy(), r.energy()
+    assert max(energy_errs(ef, er, floor=1e-12).values()) <= 1e-14
+    assert force_err(f.forces(), r.forces()) <= 1e-14
+    assert np.array_equal(f.warn(), r.warn())
+    # the energy-only call leaves the energies the same
+    ef2 = f.energy(forces=False)
+    assert abs(ef2["ener"] - ef["ener"]) <= 1e-14 * abs(ef["ener"])
+
+
+@pytest.mark.parametrize("which", ["plain", "pore"])
+def test_bd_steps_through_front_end(which):
+    s = small_plain(4, 50) if which == "plain" else small_pore(seed=7)
+    seed, c0 = 1234, 17
+    f, r = _pair(s)
+    f.L.bromoc_gpu_set_seed(seed, c0)
+    for k in range(4):
+        f.energy()
+        f.dynamics(abi.NOISE_PHILOX)        # the front-end draws from its own seed/counter
+        r.energy()
+        r.dynamics(abi.NOISE_PHILOX, 0, seed, c0 + k)
+    pf, pr = f.positions()[0], r.positions()[0]
+    assert float(np.max(np.abs(pf - pr))) <= 1e-12
+    for a, b in zip(f.flux(), r.flux()):
+        assert np.array_equal(a, b)
+
+
+def test_interact_through_front_end():
+    s = small_pore(seed=8, n_bp=6, n_ions=100)
+    f, r = _pair(s)
+    pos, typ = r.positions()
+    cases = [(pos[j], abs(typ[j]) if j >= s.nsites else 0, j)
+             for j in (0, 3, s.nsites - 1, s.nsites, s.ntot - 1)]
+    cases += [((1.0, 2.0, -25.0), s.nold + 1, s.ntot)]
+    for xyz, jt, j in cases:
+        a, b = f.interact(xyz, jt, j, True), r.interact(xyz, jt, j, True)
+        assert abs(a - b) <= 1e-14 * max(abs(b), 1.0), (j, a, b)
+
+
+def test_grand_through_front_end():
+    s = small_pore(seed=10, n_bp=0, n_ions=80, dna=False).copy()
+    s.capacity = s.ntot + 150
+    gc = configs.gcmc_buffers(s, conc=1.0, buffer_len=8.0, mu=(-0.5, -0.4))
+    ib, nat = configs.assign_buffers(s, gc)
+    f, r = _pair(s)
+    f.set_gcmc(gc, ib)
+    r.set_gcmc(gc, ib)
+    u = np.random.default_rng(4).random(10000)
+    # the driver's prob(nbuffer, 0:datom) is handed over whole: compare its first columns
+    pa = np.zeros((gc.nbuffer, s.capacity + 1), np.int32)
+    pb = pa.copy()
+    na, nb = nat.copy(), nat.copy()
+    _, ea = f.grand(100, u, na, pa, 1, 0.0)
+    _, eb = r.grand(100, u, nb, pb, 1, 0.0)
+    assert np.array_equal(na, nb) and np.array_equal(pa, pb)
+    assert abs(ea - eb) <= 1e-12 * max(abs(eb), 1.0)
+    assert f.ntot == r.ntot
+    assert np.array_equal(f.positions()[0], r.positions()[0])
+    assert np.array_equal(f.ibuffer(), r.ibuffer())
