@@ -68,6 +68,35 @@ def cpu_baseline(s, target_s=12.0):
                       f"{s.ntot}-particle system, 1 thread, {dt:.1f} s"}
 
 
+def host_roundtrip(eng, nmov, nsteps=200):
+    """Rate of the per-routine call pattern a Fortran driver uses without the device-resident
+    loop: upload positions, ENERGY, DYNAMICS, download positions and forces, every step."""
+    from bromoc_amd import abi
+    pos, typ = eng.positions()
+    t0 = time.perf_counter()
+    for k in range(nsteps):
+        eng.set_positions(pos, typ)
+        eng.energy(forces=True)
+        eng.forces()
+        eng.dynamics(abi.NOISE_PHILOX, 0, seed=7, counter=k)
+        pos, typ = eng.positions()
+    dt = time.perf_counter() - t0
+    return {"value": nmov * nsteps / dt, "unit": "particle-steps/s", "steps": nsteps,
+            "pattern": "set_positions + energy + get_forces + dynamics + get_positions per step"}
+
+
+def copy_probe(mib):
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,7 +105,13 @@ def main():
     ap.add_argument("--workload", default="pore_dna", choices=["pore_dna", "plain_ions"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true", help="skip the start-state parity check")
+    ap.add_argument("--no-host", action="store_true", help="skip the host round-trip rate")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
@@ -130,6 +165,7 @@ def main():
     nions = s.ntot - s.nsites
     pairs = (s.ntot - s.nsites - s.nfix) * (s.ntot - 1) + s.nsites * nions
     gflops = 34.0 * pairs / (kernel_ms * 1e-3) / 1e9
+    host = None if args.no_host else host_roundtrip(eng, nmov)
     cpu = None if args.no_cpu else cpu_baseline(s)
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
@@ -142,7 +178,8 @@ def main():
                      "fp64_valu": {"achieved_tflops": gflops / 1e3, "peak_tflops": 78.6,
                                    "frac": gflops / 1e3 / 78.6, "pair_evals_per_step": pairs},
                      "targets_per_block": tpb},
-        "cpu_baseline": cpu, "parity_vs_oracle": parity, "setup_s": build_s,
+        "cpu_baseline": cpu, "host_roundtrip": host, "parity_vs_oracle": parity,
+        "setup_s": build_s,
     }
     print(json.dumps(out))
 
