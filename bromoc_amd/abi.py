@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK, E_ARG, E_UNSUPPORTED, E_CAPACITY, E_HIP, E_NODEVICE, E_STATE = range(7)
 STATUS_NAMES = {0: "OK", 1: "E_ARG", 2: "E_UNSUPPORTED", 3: "E_CAPACITY", 4: "E_HIP",
@@ -20,7 +20,7 @@ QDEBY, QCOUNTION, QDNAFREE = 1 << 8, 1 << 9, 1 << 10
 QMEMB, QMMIJ, QBSPLINE, QNMCDEN = 1 << 16, 1 << 17, 1 << 18, 1 << 19
 QDEBYHYB, QPROXDIFF, QNOTRANS, QAPFOR = 1 << 20, 1 << 21, 1 << 22, 1 << 23
 QCONTRANS, QRFPAR, QSPHERE, QECYL = 1 << 24, 1 << 25, 1 << 26, 1 << 27
-QDIFFUSE, QPROFILE = 1 << 28, 1 << 29
+QDIFFUSE, QPROFILE, QRFPSIN = 1 << 28, 1 << 29, 1 << 30
 
 SITE_S, SITE_P, SITE_AB, SITE_TB, SITE_CB, SITE_GB = range(6)
 NOISE_INJECTED, NOISE_PHILOX = 0, 1
@@ -76,6 +76,12 @@ class SystemStruct(C.Structure):
         ("lx2m", C.c_double), ("lx2p", C.c_double), ("ly2m", C.c_double),
         ("ly2p", C.c_double), ("lz2m", C.c_double), ("lz2p", C.c_double),
         ("cntpts", C.c_int32), ("reserved1", C.c_int32), ("zcont", _dp),
+        ("dt", C.c_double), ("kbtdt", C.c_double), ("diffusion", _dp),
+        ("plength2", C.c_double), ("pcenter", C.c_double), ("ampl3", _dp), ("p3", _dp),
+        ("nspline", C.c_int32), ("reserved2", C.c_int32),
+        ("xs", _dp), ("ys", _dp), ("bs", _dp), ("cs", _dp), ("ds", _dp),
+        ("rf", Grid), ("gsrfen", C.POINTER(C.c_float)), ("greff", C.POINTER(C.c_float)),
+        ("sqrfac", C.c_double),
     ]
 
 
@@ -157,6 +163,7 @@ def _declare(lib):
                                _dp, _dp], C.c_int),
         "bromoc_get_ibuffer": ([vp, _ip], C.c_int),
         "bromoc_philox4x64": ([C.c_int32, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)], C.c_int),
+        "bromoc_spline": ([C.c_int32, _dp, _dp, _dp, _dp, _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         if hasattr(lib, name):

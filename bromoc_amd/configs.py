@@ -154,6 +154,7 @@ def plain_ions(seed=0, n_per_type=200, box=60.0, temp=300.0, cdie=80.0, dt=0.02,
         pairidx=pidx, pairs=pairs, ep=ep, res=RES,
         capacity=capacity or len(typei), bdmax=1.0,
         box=(-half, half, -half, half, -half, half), ndna=ndna, nion=nion,
+        dt=dt, kbtdt=kbtdt, diffusion=d.copy(),
         meta=dict(workload="plain_ions", seed=seed, temp=temp, dt=dt, kbt=kbt, cdie=cdie),
     )
     s.fact2a[0] = 0.0
@@ -422,6 +423,7 @@ def pore_dna(seed=1, n_bp=50, n_ions=2000, box=(100.0, 100.0, 200.0), dcel=0.5, 
         vzmin=-tz, vzmax=tz, bdmax=1.0, cgnuc=cgnuc,
         fact0n1=diffnuc * kbt * dt, fact0n2=math.sqrt(2.0 * dt * diffnuc),
         box=(-hx, hx, -hy, hy, -hz, hz), zcont=np.array(zcont, float), ndna=ndna, nion=nion,
+        dt=dt, kbtdt=kbtdt, diffusion=dvec.copy(),
         meta=dict(workload="pore_dna" if dna else "pore_ions", seed=seed, temp=temp, dt=dt,
                   kbt=kbt, cdie=cdie, n_bp=n_bp if dna else 0, grid=shape, dcel=dcel),
         **kw,
@@ -470,3 +472,103 @@ def assign_buffers(s: System, g: Gcmc):
     for b in range(1, g.nbuffer + 1):
         nat[b] = int(np.sum(ib[:s.ntot] == b))
     return ib, nat
+
+
+# ------------------------------------------------------ diffusion and reaction field
+def lib_spline(x, y):
+    """Spline coefficients (b, c, d) from the HIP library's host routine (spline.f90)."""
+    lib = abi.load_library()
+    x = np.ascontiguousarray(x, float)
+    y = np.ascontiguousarray(y, float)
+    n = len(x)
+    b, c, d = np.zeros(n), np.zeros(n), np.zeros(n)
+    dp = C.POINTER(C.c_double)
+    rc = lib.bromoc_spline(n, *(a.ctypes.data_as(dp) for a in (x, y, b, c, d)))
+    if rc:
+        raise abi.BromocError("bromoc_spline", rc)
+    return b, c, d
+
+
+def with_switch_diffusion(s: System, porelength, pcenter=0.0, lowfraction=0.5,
+                          switchlength=4.0):
+    """DIFFUSION command (shell_simul.f90:1975-2020): every ion type's D drops to
+    lowfraction * D inside a pore of the given length, with a cubic switch of
+    switchlength at each end (DYNAMICS1)."""
+    s = s.copy()
+    s.flags = (s.flags | abi.QDIFFUSE) & ~abi.QPROFILE
+    s.plength2 = 0.5 * porelength
+    s.pcenter = pcenter
+    nt1 = s.ntype + 1
+    s.ampl3 = np.zeros(nt1)
+    s.p3 = np.ones(nt1)
+    lf = np.broadcast_to(np.asarray(lowfraction, float), (s.ntype - s.nold,))
+    sl = np.broadcast_to(np.asarray(switchlength, float), (s.ntype - s.nold,))
+    s.ampl3[s.nold + 1:] = lf
+    s.p3[s.nold + 1:] = sl
+    return s
+
+
+def tanh_profile(z0, z1, n, half_pore, width=3.0, low=0.5):
+    """Knots (z, D(z)/D0) of a tanh-shaped reduction to `low` inside |z| < half_pore."""
+    z = np.linspace(z0, z1, n)
+    f = 1.0 - (1.0 - low) * 0.5 * (np.tanh((z + half_pore) / width) -
+                                   np.tanh((z - half_pore) / width))
+    return z, f
+
+
+def with_profile_diffusion(s: System, zk, fk, spline_builder=None):
+    """PROFILE command (shell_simul.f90:1955-1972): D(z) = D0 * spline(z) (DYNAMICS2).
+    spline_builder(x, y) -> (b, c, d); default: the library's bromoc_spline."""
+    s = s.copy()
+    s.flags = (s.flags | abi.QPROFILE) & ~abi.QDIFFUSE
+    zk = np.asarray(zk, float)
+    fk = np.asarray(fk, float)
+    b, c, d = (spline_builder or lib_spline)(zk, fk)
+    s.spline = (zk.copy(), fk.copy(), b, c, d)
+    return s
+
+
+def with_rfpar(s: System, seed=0, shape=None, dcel=None, single=False, reffac=1.5,
+               wall_radius=None, zero_frac=0.0, const_reff=None):
+    """Synthetic reaction-field parameter grids (RFPAR command, rfpar.f90:373-582): per
+    ion type (or one grid with single) a smooth gsrfen field and a positive greff field
+    (constant if const_reff is given); gsrfen is zero beyond wall_radius from the z axis
+    and at a random zero_frac of the nodes (the reference stops a corner row at a zero
+    gsrfen node, rfpar.f90:121).  sqrfac = sqrt(reffac) (shell_simul.f90:2860)."""
+    s = s.copy()
+    rng = np.random.default_rng(seed)
+    lx2m, lx2p, ly2m, ly2p, lz2m, lz2p = s.box
+    if dcel is None:
+        dcel = 1.0
+    if shape is None:
+        shape = tuple(int(np.ceil((hi - lo) / dcel)) + 3 for lo, hi in
+                      ((lx2m, lx2p), (ly2m, ly2p), (lz2m, lz2p)))
+    ng = 1 if single else s.ntype - s.nold
+    nx, ny, nz = shape
+    x = (np.arange(nx) - 0.5 * (nx - 1)) * dcel
+    y = (np.arange(ny) - 0.5 * (ny - 1)) * dcel
+    z = (np.arange(nz) - 0.5 * (nz - 1)) * dcel
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    gs = np.zeros((ng,) + tuple(shape), np.float32)
+    gr = np.zeros_like(gs)
+    for k in range(ng):
+        a = rng.uniform(0.2, 0.5, 3)
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        f = (-0.6 + 0.15 * np.sin(a[0] * X + ph[0]) * np.cos(a[1] * Y + ph[1]) +
+             0.1 * np.sin(a[2] * Z + ph[2]))
+        if wall_radius is not None:
+            f = np.where(X ** 2 + Y ** 2 > wall_radius ** 2, 0.0, f)
+        if zero_frac > 0.0:
+            f = np.where(rng.random(f.shape) < zero_frac, 0.0, f)
+        gs[k] = f
+        if const_reff is not None:
+            gr[k] = const_reff
+        else:
+            gr[k] = 2.0 + 0.5 * np.cos(a[1] * X + ph[2]) * np.sin(a[0] * Z + ph[0]) + 0.3 * k
+    s.rf = GridData(gs, dcel)
+    s.rf_greff = gr
+    s.sqrfac = math.sqrt(reffac)
+    s.flags |= abi.QRFPAR
+    if single:
+        s.flags |= abi.QRFPSIN
+    return s

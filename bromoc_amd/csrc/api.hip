@@ -25,7 +25,9 @@ constexpr int GRAPH_K = 32;   // BD steps per captured graph (even)
 
 constexpr uint32_t SUPPORTED = BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BROMOC_QNOBOND |
                                BROMOC_QPHIX | BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV |
-                               BROMOC_QDEBY | BROMOC_QCOUNTION | BROMOC_QDNAFREE;
+                               BROMOC_QDEBY | BROMOC_QCOUNTION | BROMOC_QDNAFREE |
+                               BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR |
+                               BROMOC_QRFPSIN;
 
 // constamod.f90:21-32 (binary64)
 double celec_value() {
@@ -136,14 +138,18 @@ hipError_t dupload(bromoc_ctx *c, T **p, const T *src, size_t n) {
     return hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice);
 }
 
-void energies_from(const double *r, bromoc_energies *out) {
+void energies_from(const double *r, bool rfpar, bromoc_energies *out) {
     bromoc_energies E;
     double *f = &E.eelec;
     for (int k = 0; k < NE; k++) f[k] = r[k];
-    // energy.mono.f90:83-115,689 (ememb and egsbpb stay zero on this path)
+    // energy.mono.f90:83-115,689 (ememb stays zero on this path; egsbpb is erfpar)
     E.ener = E.ememb;
     E.ener = E.ener + E.egsbpa;
     E.ener = E.ener + E.evdwgd;
+    if (rfpar) {
+        E.ener = E.ener + E.erfpar;
+        E.egsbpb = E.erfpar;
+    }
     E.ener = E.ener + E.eelec + E.evdw + E.esrpmf + E.esrpmfmx + E.ebond + E.eang + E.edihe +
              E.estack + E.ebp + E.eex + E.eqq + E.esolv + E.eqqmx + E.evdwmx + E.eefpot +
              E.eefpotmx + E.econ;
@@ -249,6 +255,25 @@ int validate(const bromoc_system *s, std::string &msg) {
     }
     if ((s->flags & BROMOC_QCOUNTION) && (s->cntpts < 0 || (s->cntpts > 0 && !s->zcont)))
         return bad("flux planes");
+    // simul1.f90:338-340
+    if ((s->flags & BROMOC_QDIFFUSE) && (s->flags & BROMOC_QPROFILE))
+        return bad("DIFFUSION and PROFILE together");
+    if (s->flags & (BROMOC_QDIFFUSE | BROMOC_QPROFILE)) {
+        if (!s->diffusion || !std::isfinite(s->dt) || !std::isfinite(s->kbtdt))
+            return bad("diffusion constants missing");
+    }
+    if ((s->flags & BROMOC_QDIFFUSE) && (!s->ampl3 || !s->p3)) return bad("DIFFUSION arrays missing");
+    if (s->flags & BROMOC_QPROFILE) {
+        if (s->nspline < 2 || s->nspline > (1 << 20) || !s->xs || !s->ys || !s->bs || !s->cs || !s->ds)
+            return bad("PROFILE spline missing");
+        if (!(s->xs[0] != s->xs[s->nspline - 1])) return bad("PROFILE knots have equal ends");
+    }
+    if (s->flags & BROMOC_QRFPAR) {
+        const bromoc_grid &g = s->rf;
+        if (g.nclx < 2 || g.ncly < 2 || g.nclz < 2 || !(g.dcel > 0.0)) return bad("RFPAR grid geometry");
+        if (!s->gsrfen || !s->greff) return bad("RFPAR grids missing");
+        if (!(s->flags & BROMOC_QRFPSIN) && s->ntype - s->nold < 1) return bad("RFPAR ion types");
+    }
     return BROMOC_OK;
 }
 
@@ -310,7 +335,7 @@ int reduce_energies(bromoc_ctx *c, bromoc_energies *out) {
     HIPCHK(c, hipMemcpyAsync(r, c->ered, sizeof(r), hipMemcpyDeviceToHost, c->stream));
     int st = sync_err(c);
     if (st) return st;
-    energies_from(r, out);
+    energies_from(r, (c->sys.flags & BROMOC_QRFPAR) != 0, out);
     return BROMOC_OK;
 }
 
@@ -493,6 +518,30 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     ACHK(dalloc(c, &c->ctl, 4));
     ACHK(dalloc(c, &c->pbuf, (size_t)GRAPH_K * 3 * cap));
     ACHK(dalloc(c, &c->iout, 1));
+    // position-dependent diffusion (DYNAMICS1 / DYNAMICS2)
+    const bool dyn12 = (s->flags & (BROMOC_QDIFFUSE | BROMOC_QPROFILE)) != 0;
+    const int nsp = (s->flags & BROMOC_QPROFILE) ? s->nspline : 0;
+    double *d_diff = nullptr, *d_ampl3 = nullptr, *d_p3 = nullptr;
+    double *d_sp[5] = {};
+    ACHK(dupload(c, &d_diff, dyn12 ? s->diffusion : (const double *)nullptr, dyn12 ? s->ntype + 1 : 0));
+    const bool sw = (s->flags & BROMOC_QDIFFUSE) != 0;
+    ACHK(dupload(c, &d_ampl3, sw ? s->ampl3 : (const double *)nullptr, sw ? s->ntype + 1 : 0));
+    ACHK(dupload(c, &d_p3, sw ? s->p3 : (const double *)nullptr, sw ? s->ntype + 1 : 0));
+    const double *sp_src[5] = {s->xs, s->ys, s->bs, s->cs, s->ds};
+    for (int k = 0; k < 5; k++) ACHK(dupload(c, &d_sp[k], nsp ? sp_src[k] : (const double *)nullptr, nsp));
+    // reaction-field parameter grids and per-ion parameters
+    const bool rfon = (s->flags & BROMOC_QRFPAR) != 0;
+    float *d_gs = nullptr, *d_gr = nullptr;
+    {
+        const size_t ncel3 = rfon ? (size_t)s->rf.nclx * s->rf.ncly * s->rf.nclz : 0;
+        const size_t ng = (s->flags & BROMOC_QRFPSIN) ? 1 : (size_t)std::max(1, s->ntype - s->nold);
+        ACHK(dupload(c, &d_gs, rfon ? s->gsrfen : (const float *)nullptr, ncel3 * ng));
+        ACHK(dupload(c, &d_gr, rfon ? s->greff : (const float *)nullptr, ncel3 * ng));
+    }
+    double4 *d_rfa = nullptr, *d_rfd = nullptr, *d_frf = nullptr;
+    ACHK(dalloc(c, &d_rfa, rfon ? cap : 0));
+    ACHK(dalloc(c, &d_rfd, rfon ? 2 * (size_t)cap : 0));
+    ACHK(dalloc(c, &d_frf, rfon ? cap : 0));
 #undef ACHK
 
     StepParams &p = c->base;
@@ -524,6 +573,16 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.zcont = c->zcont; p.nfwd = c->nfwd; p.nbwd = c->nbwd;
     p.step_ctr = c->ctl;
     p.stream = 0;
+    p.dmode = (s->flags & BROMOC_QDIFFUSE) ? 1 : (s->flags & BROMOC_QPROFILE) ? 2 : 0;
+    p.nspline = nsp;
+    p.dt = s->dt; p.kbtdt = s->kbtdt; p.plength2 = s->plength2; p.pcenter = s->pcenter;
+    p.diffusion = d_diff; p.ampl3 = d_ampl3; p.p3 = d_p3;
+    p.xs = d_sp[0]; p.ys = d_sp[1]; p.bs = d_sp[2]; p.cs = d_sp[3]; p.ds = d_sp[4];
+    p.gr = geom_of(s->rf, rfon);
+    p.gsrfen = d_gs; p.greff = d_gr; p.sqrfac = s->sqrfac;
+    p.rfon = rfon ? 1 : 0;
+    p.rfsingle = (s->flags & BROMOC_QRFPSIN) ? 1 : 0;
+    p.rfa = d_rfa; p.rfd = d_rfd; p.frf = d_frf;
     p.all_efpot = 1;
     c->r2near = 0.0;
     for (int k = 0; k < nt1 * nt1; k++) {
@@ -643,6 +702,7 @@ static hipError_t enqueue_step(bromoc_ctx *c, const StepParams &p, hipEvent_t t0
                                hipEvent_t t1 = nullptr) {
     const int ns = p.nsites, nt = p.ntargets;
     hipError_t e;
+    if ((e = launch_rfpar(p, true, c->stream)) != hipSuccess) return e;
     static const bool fork = [] {
         const char *v = std::getenv("BROMOC_FORK");
         return v && v[0] == '1';
@@ -901,7 +961,10 @@ static int interact_async(bromoc_ctx *c, double xj, double yj, double zj, int32_
     const bool beion = (c->sys.flags & BROMOC_QPAR) && j >= c->ns;
     if (j < 0 || j > c->ntot) return fail(c, BROMOC_E_ARG, "j out of range");
     if (beion) {
-        if (jtype < 1 || jtype > c->sys.ntype) return fail(c, BROMOC_E_ARG, "jtype out of range");
+        // an ion carries an ion type (nold < jtype <= ntype, shell_simul.f90:530); the
+        // reaction-field grids are indexed by it
+        if (jtype <= c->sys.nold || jtype > c->sys.ntype)
+            return fail(c, BROMOC_E_ARG, "jtype is not an ion type");
         int t2 = c->h_nwtype[jtype];
         if (t2 < 1 || t2 > c->sys.nttyp) return fail(c, BROMOC_E_ARG, "nwtype(jtype) out of range");
     } else if (j >= c->ns) {
@@ -912,6 +975,7 @@ static int interact_async(bromoc_ctx *c, double xj, double yj, double zj, int32_
     q.xj = xj; q.yj = yj; q.zj = zj;
     q.jtype = jtype; q.j = j; q.alert = alert;
     q.out = c->iout;
+    HIPCHK(c, launch_rfpar(q.sp, false, c->stream));
     HIPCHK(c, launch_interact(q, c->stream));
     return BROMOC_OK;
 }
@@ -1223,6 +1287,63 @@ int bromoc_build_table(int32_t n, const double *r, const double *u, int32_t chr,
     pair->efpot = 1;
     pair->chr = chr;
     pair->fct = fct;
+    return BROMOC_OK;
+}
+
+// SPLINE (spline.f90:19-98): the Forsythe-Malcolm-Moler cubic interpolating spline with
+// end third derivatives from divided differences; a tridiagonal solve by forward
+// elimination and back substitution, in the reference's operation order.
+int bromoc_spline(int32_t n, const double *x, const double *y, double *b, double *c, double *d) {
+    if (n < 2 || !x || !y || !b || !c || !d) return BROMOC_E_ARG;
+    const int nm1 = n - 1;
+    // 1-based views
+    const double *X = x - 1, *Y = y - 1;
+    double *B = b - 1, *Cc = c - 1, *D = d - 1;
+    if (n > 2) {
+        D[1] = X[2] - X[1];
+        Cc[2] = (Y[2] - Y[1]) / D[1];
+        for (int i = 2; i <= nm1; i++) {
+            D[i] = X[i + 1] - X[i];
+            B[i] = 2.0 * (D[i - 1] + D[i]);
+            Cc[i + 1] = (Y[i + 1] - Y[i]) / D[i];
+            Cc[i] = Cc[i + 1] - Cc[i];
+        }
+        B[1] = -D[1];
+        B[n] = -D[n - 1];
+        Cc[1] = 0.0;
+        Cc[n] = 0.0;
+        if (n > 3) {
+            Cc[1] = Cc[3] / (X[4] - X[2]) - Cc[2] / (X[3] - X[1]);
+            Cc[n] = Cc[n - 1] / (X[n] - X[n - 2]) - Cc[n - 2] / (X[n - 1] - X[n - 3]);
+            Cc[1] = Cc[1] * (D[1] * D[1]) / (X[4] - X[1]);
+            Cc[n] = -Cc[n] * (D[n - 1] * D[n - 1]) / (X[n] - X[n - 3]);
+        }
+        for (int i = 2; i <= n; i++) {
+            const double t = D[i - 1] / B[i - 1];
+            B[i] = B[i] - t * D[i - 1];
+            Cc[i] = Cc[i] - t * Cc[i - 1];
+        }
+        Cc[n] = Cc[n] / B[n];
+        for (int ib = 1; ib <= nm1; ib++) {
+            const int i = n - ib;
+            Cc[i] = (Cc[i] - D[i] * Cc[i + 1]) / B[i];
+        }
+        B[n] = (Y[n] - Y[nm1]) / D[nm1] + D[nm1] * (Cc[nm1] + 2.0 * Cc[n]);
+        for (int i = 1; i <= nm1; i++) {
+            B[i] = (Y[i + 1] - Y[i]) / D[i] - D[i] * (Cc[i + 1] + 2.0 * Cc[i]);
+            D[i] = (Cc[i + 1] - Cc[i]) / D[i];
+            Cc[i] = 3.0 * Cc[i];
+        }
+        Cc[n] = 3.0 * Cc[n];
+        D[n] = D[n - 1];
+    } else {
+        B[1] = (Y[2] - Y[1]) / (X[2] - X[1]);
+        Cc[1] = 0.0;
+        D[1] = 0.0;
+        B[2] = B[1];
+        Cc[2] = 0.0;
+        D[2] = 0.0;
+    }
     return BROMOC_OK;
 }
 

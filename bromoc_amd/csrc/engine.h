@@ -103,6 +103,20 @@ struct StepParams {
     int64_t row0;                  // injected row of step counter 0
     uint64_t seed;
     uint64_t stream;
+    // ion update: 0 DYNAMICS0 (fact1a/fact2a), 1 DYNAMICS1 (pore switch), 2 DYNAMICS2
+    // (spline profile)
+    int32_t dmode, nspline;
+    double dt, kbtdt, plength2, pcenter;
+    const double *diffusion, *ampl3, *p3;
+    const double *xs, *ys, *bs, *cs, *ds;
+    // reaction-field parameter grids (RFPARION)
+    GridGeom gr;
+    const float *gsrfen, *greff;
+    double sqrfac;
+    int32_t rfon, rfsingle;
+    double4 *rfa;            // [cap] (srfe, reff, 1 if inside the grid box else 0, 0)
+    double4 *rfd;            // [2*cap] (d srfe / d r, 0), (d reff / d r, 0)
+    double4 *frf;            // [cap] reaction-field force on each moving ion (k_rfpair)
 };
 
 // spatial partner order (sort.hip)
@@ -126,6 +140,7 @@ hipError_t sort_partners(const SortBufs &b, const double4 *pos, int n, double4 *
 hipError_t launch_step(const StepParams &p, hipStream_t s);   // launch_dna + launch_pairs
 hipError_t launch_dna(const StepParams &p, hipStream_t s);    // k_dna (CG-DNA list terms)
 hipError_t launch_pairs(const StepParams &p, hipStream_t s);  // k_step (pairs, grids, update)
+hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s);  // RFPARION
 hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
                                 hipStream_t s);
 hipError_t launch_advance(int64_t *ctr, int64_t by, hipStream_t s);

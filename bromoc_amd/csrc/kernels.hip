@@ -198,6 +198,180 @@ __device__ __forceinline__ void grid_cell(const GridGeom &g, double x, double y,
     if (iz == g.nz - 1) iz = g.nz - 2;
 }
 
+// Diffusion profile of DYNAMICS2 (dynamics2.mono.f90:52-74): the end values outside the
+// knots, else the cubic spline and its derivative (seval/sevald, sevalmod.f90:23-67).  The
+// interval is found by the same bisection; seval's SAVEd interval is not kept (it only
+// changes the chosen interval when z equals a knot exactly).
+__device__ __forceinline__ void diff_profile(const StepParams &P, double zz, double &sw,
+                                             double &dsw) {
+    const int n = P.nspline;
+    const double *xs = P.xs;
+    const bool up = xs[0] < xs[n - 1];
+    if (up ? zz < xs[0] : zz > xs[0]) { sw = P.ys[0]; dsw = 0.0; return; }
+    if (up ? zz > xs[n - 1] : zz < xs[n - 1]) { sw = P.ys[n - 1]; dsw = 0.0; return; }
+    int i = 1, j = n + 1;   // 1-based, as the reference's bisection
+    for (int it = 0; it < 64 && j > i + 1; it++) {
+        const int k = (i + j) / 2;
+        if (zz < xs[k - 1]) j = k;
+        if (zz >= xs[k - 1]) i = k;
+    }
+    const double dx = zz - xs[i - 1];
+    sw = P.ys[i - 1] + dx * (P.bs[i - 1] + dx * (P.cs[i - 1] + dx * P.ds[i - 1]));
+    dsw = P.bs[i - 1] + dx * (P.cs[i - 1] + dx * P.ds[i - 1]);
+}
+
+// Drift and noise factors of an ion of type itype at height z: fact1a/fact2a for DYNAMICS0;
+// for DYNAMICS1 the pore switch (dynamics1.mono.f90:38-70) and for DYNAMICS2 the profile
+// scale D, with the drift dD/dz * dt on z.
+__device__ __forceinline__ void ion_factors(const StepParams &P, int itype, double z,
+                                            double &fac1, double &fac2, double &ddz) {
+    if (P.dmode == 0) {
+        fac1 = P.fact1a[itype];
+        fac2 = P.fact2a[itype];
+        ddz = 0.0;
+        return;
+    }
+    const double D = P.diffusion[itype];
+    double idf, didf;
+    if (P.dmode == 1) {
+        const double pp3 = P.p3[itype];
+        const double pore1 = -P.plength2 + P.pcenter, pore2 = P.plength2 + P.pcenter;
+        double sw = 1.0, dsw = 0.0;
+        if (z > pore1 - pp3 && z < pore2 + pp3) {
+            sw = 0.0;
+            if (!(z >= pore1 && z <= pore2)) {
+                const double ipp3 = 1.0 / pp3;
+                if (z > pore2) {
+                    const double t = (z - pore2) * ipp3;
+                    sw = 2.0 * (t * t * t) - 3.0 * (t * t);
+                    dsw = 6.0 * (t * t - t) * ipp3;
+                } else {
+                    const double t = (z - pore1) * ipp3;
+                    sw = -2.0 * (t * t * t) - 3.0 * (t * t);
+                    dsw = -6.0 * (t * t + t) * ipp3;
+                }
+                sw = -sw;
+                dsw = -dsw;
+            }
+        }
+        const double a = P.ampl3[itype];
+        idf = D * (a + (1.0 - a) * sw);
+        didf = D * (1.0 - a) * dsw;
+    } else {
+        double sw, dsw;
+        diff_profile(P, z, sw, dsw);
+        idf = D * sw;
+        didf = D * dsw;
+    }
+    fac1 = idf * P.kbtdt;
+    fac2 = sqrt(2.0 * P.dt * idf);
+    ddz = didf * P.dt;
+}
+
+// Ermak-McCammon update of moving particle i (position pa, force F, Gaussians g):
+// DYNAMICS0/1/2 for ions (dynamics0.mono.f90:31-46, dynamics1.mono.f90:71-86,
+// dynamics2.mono.f90:75-85) with countions, DYNAMICS0NUC (dynamics0nuc.f90:31-44) for
+// sites; fixcoor unless the site is free of the box (dnafree).
+__device__ __forceinline__ double4 bd_update(const StepParams &P, int i, bool is_site,
+                                             const double4 &pa, const double F[3],
+                                             const double *g) {
+    double fac1, fac2, ddz = 0.0;
+    int itype = 0;
+    if (is_site) {
+        fac1 = P.fact0n1;
+        fac2 = P.fact0n2;
+    } else {
+        itype = abs(P.typei[i]);
+        ion_factors(P, itype, pa.z, fac1, fac2, ddz);
+    }
+    const double bdmax = P.bdmax;
+    double delx = F[0] * fac1, dely = F[1] * fac1, delz = F[2] * fac1;
+    if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
+    if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
+    if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
+    double xn = pa.x + delx + fac2 * g[0];
+    double yn = pa.y + dely + fac2 * g[1];
+    double zn = pa.z + delz + fac2 * g[2];
+    if (!is_site && P.dmode != 0) zn = zn + ddz;
+    if (!is_site && (P.flags & QCOUNTION)) {
+        const int row = itype - P.nold - 1;
+        for (int c = 0; c < P.cntpts; c++) {
+            const double zc = P.zcont[c];
+            if (pa.z < zc && zn >= zc) atomicAdd(&P.nfwd[row * P.cntpts + c], 1);
+            else if (pa.z >= zc && zn < zc) atomicAdd(&P.nbwd[row * P.cntpts + c], 1);
+        }
+    }
+    if (!is_site || !(P.flags & QDNAFREE)) fixcoor(P, xn, yn, zn);
+    return make_double4(xn, yn, zn, pa.w);
+}
+
+// Reaction-field parameters of an ion at (x, y, z) of atnam type itype: srfe and reff
+// from the 8 corners of gsrfen / sqrfac*greff (rfpar.f90:75-142), with their gradients
+// when grad is set.  The corner loop leaves the innermost (z) loop at a zero gsrfen corner,
+// as the reference's `exit` does (rfpar.f90:121).  Returns false outside the grid box.
+__device__ bool rf_params(const StepParams &P, double x, double y, double z, int itype,
+                          bool grad, double &srfe, double &reff, double ds[3], double dr[3]) {
+    const GridGeom &g = P.gr;
+    srfe = 0.0; reff = 0.0;
+    ds[0] = ds[1] = ds[2] = dr[0] = dr[1] = dr[2] = 0.0;
+    if (!(x <= g.xbcen + g.tranx && x >= g.xbcen - g.tranx && y <= g.ybcen + g.trany &&
+          y >= g.ybcen - g.trany && z <= g.zbcen + g.tranz && z >= g.zbcen - g.tranz))
+        return false;
+    const size_t ncyz = (size_t)g.ny * g.nz, ncel3 = ncyz * g.nx;
+    const size_t ifir = P.rfsingle ? 0 : (size_t)(itype - P.nold - 1) * ncel3;
+    double xi, yi, zi;
+    int ix, iy, iz;
+    grid_cell(g, x, y, z, xi, yi, zi, ix, iy, iz);
+    ix = min(max(ix, 0), g.nx - 2);
+    iy = min(max(iy, 0), g.ny - 2);
+    iz = min(max(iz, 0), g.nz - 2);
+    const double one = 1.0;
+    double aux1 = 0.0, aux2 = 0.0;
+    for (int n1 = ix; n1 <= ix + 1; n1++) {
+        double ai = xi - n1 * g.dcel;
+        const double aisign = fsign(one, ai);
+        ai = one - fabs(ai) * g.idcel;
+        for (int n2 = iy; n2 <= iy + 1; n2++) {
+            double bi = yi - n2 * g.dcel;
+            const double bisign = fsign(one, bi);
+            bi = one - fabs(bi) * g.idcel;
+            for (int n3 = iz; n3 <= iz + 1; n3++) {
+                double ci = zi - n3 * g.dcel;
+                const double cisign = fsign(one, ci);
+                ci = one - fabs(ci) * g.idcel;
+                const double fi = ai * bi * ci;
+                const size_t in3 = n1 * ncyz + (size_t)n2 * g.nz + n3 + ifir;
+                const double gaux1 = (double)P.gsrfen[in3];
+                const double gaux2 = P.sqrfac * (double)P.greff[in3];
+                if (gaux1 == 0.0) break;
+                const double prefa1 = gaux1 * g.idcel, prefa2 = gaux2 * g.idcel;
+                aux1 = aux1 + fi * gaux1;
+                aux2 = aux2 + fi * gaux2;
+                if (grad) {
+                    if (ai < (one - RSMALL) && ai > RSMALL) {
+                        const double aux = aisign * bi * ci;
+                        ds[0] = ds[0] - aux * prefa1;
+                        dr[0] = dr[0] - aux * prefa2;
+                    }
+                    if (bi < (one - RSMALL) && bi > RSMALL) {
+                        const double aux = bisign * ai * ci;
+                        ds[1] = ds[1] - aux * prefa1;
+                        dr[1] = dr[1] - aux * prefa2;
+                    }
+                    if (ci < (one - RSMALL) && ci > RSMALL) {
+                        const double aux = cisign * ai * bi;
+                        ds[2] = ds[2] - aux * prefa1;
+                        dr[2] = dr[2] - aux * prefa2;
+                    }
+                }
+            }
+        }
+    }
+    srfe = aux1;
+    reff = aux2;
+    return true;
+}
+
 struct V3 { double x, y, z; };
 __device__ __forceinline__ V3 ld3(const StepParams &P, int i) {
     const double4 q = P.pos[i];
@@ -659,6 +833,10 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
             F[0] += fd.x; F[1] += fd.y; F[2] += fd.z;
         }
     }
+    if (P.rfon && !is_site) {   // reaction-field forces from k_rfpair
+        const double4 fr = P.frf[i];
+        F[0] += fr.x; F[1] += fr.y; F[2] += fr.z;
+    }
     const double4 pa = P.pos[i];
     const int t2a = __double2loint(pa.w);
     if (s_warn[a]) atomicAdd(&P.warn[t2a], 1);
@@ -690,42 +868,14 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
         P.fz[i] = F[2];
     }
     if constexpr ((FEAT & F_UPDATE) != 0) {
-        // DYNAMICS0 (dynamics0.mono.f90:31-46) / DYNAMICS0NUC (dynamics0nuc.f90:31-44)
         const bool fr = !is_site || ((FEAT & F_DNA) && P.stfree[i] != 0);
         if (!fr) {
             P.poso[i] = pa;
             P.sposo[P.rank[i]] = pa;
             return;
         }
-        double fac1, fac2;
-        int itype = 0;
-        if (is_site) {
-            fac1 = P.fact0n1;
-            fac2 = P.fact0n2;
-        } else {
-            itype = abs(P.typei[i]);
-            fac1 = P.fact1a[itype];
-            fac2 = P.fact2a[itype];
-        }
         const double g[3] = {s_g[a][0], s_g[a][1], s_g[a][2]};
-        const double bdmax = P.bdmax;
-        double delx = F[0] * fac1, dely = F[1] * fac1, delz = F[2] * fac1;
-        if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
-        if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
-        if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
-        double xn = pa.x + delx + fac2 * g[0];
-        double yn = pa.y + dely + fac2 * g[1];
-        double zn = pa.z + delz + fac2 * g[2];
-        if (!is_site && (P.flags & QCOUNTION)) {
-            const int row = itype - P.nold - 1;
-            for (int c = 0; c < P.cntpts; c++) {
-                const double zc = P.zcont[c];
-                if (pa.z < zc && zn >= zc) atomicAdd(&P.nfwd[row * P.cntpts + c], 1);
-                else if (pa.z >= zc && zn < zc) atomicAdd(&P.nbwd[row * P.cntpts + c], 1);
-            }
-        }
-        if (!is_site || !(P.flags & QDNAFREE)) fixcoor(P, xn, yn, zn);
-        const double4 pn = make_double4(xn, yn, zn, pa.w);
+        const double4 pn = bd_update(P, i, is_site, pa, F, g);
         P.poso[i] = pn;
         P.sposo[P.rank[i]] = pn;
     }
@@ -929,6 +1079,10 @@ __global__ __launch_bounds__(BLOCK) void k_wave(StepParams P) {
             F[0] += fd.x; F[1] += fd.y; F[2] += fd.z;
         }
     }
+    if (P.rfon && !is_site) {
+        const double4 fr = P.frf[i];
+        F[0] += fr.x; F[1] += fr.y; F[2] += fr.z;
+    }
     if (warned) atomicAdd(&P.warn[t2i], 1);
     if (P.write_energy) {
         double *ep_ = P.epart + i;
@@ -954,35 +1108,8 @@ __global__ __launch_bounds__(BLOCK) void k_wave(StepParams P) {
             P.sposo[P.rank[i]] = pa;
             return;
         }
-        double fac1, fac2;
-        int itype = 0;
-        if (is_site) {
-            fac1 = P.fact0n1;
-            fac2 = P.fact0n2;
-        } else {
-            itype = abs(P.typei[i]);
-            fac1 = P.fact1a[itype];
-            fac2 = P.fact2a[itype];
-        }
         const double *g = noise_row(P, i);
-        const double bdmax = P.bdmax;
-        double delx = F[0] * fac1, dely = F[1] * fac1, delz = F[2] * fac1;
-        if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
-        if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
-        if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
-        double xn = pa.x + delx + fac2 * g[0];
-        double yn = pa.y + dely + fac2 * g[1];
-        double zn = pa.z + delz + fac2 * g[2];
-        if (!is_site && (P.flags & QCOUNTION)) {
-            const int row = itype - P.nold - 1;
-            for (int c = 0; c < P.cntpts; c++) {
-                const double zc = P.zcont[c];
-                if (pa.z < zc && zn >= zc) atomicAdd(&P.nfwd[row * P.cntpts + c], 1);
-                else if (pa.z >= zc && zn < zc) atomicAdd(&P.nbwd[row * P.cntpts + c], 1);
-            }
-        }
-        if (!is_site || !(P.flags & QDNAFREE)) fixcoor(P, xn, yn, zn);
-        const double4 pn = make_double4(xn, yn, zn, pa.w);
+        const double4 pn = bd_update(P, i, is_site, pa, F, g);
         P.poso[i] = pn;
         P.sposo[P.rank[i]] = pn;
     }
@@ -1054,9 +1181,111 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s) {
 }
 
 hipError_t launch_step(const StepParams &p, hipStream_t s) {
-    hipError_t e = launch_dna(p, s);
+    hipError_t e = launch_rfpar(p, true, s);
+    if (e != hipSuccess) return e;
+    e = launch_dna(p, s);
     if (e != hipSuccess) return e;
     return launch_pairs(p, s);
+}
+
+// ---- RFPARION (rfpar.f90:20-228)
+// k_rfpar: srfe, reff and their gradients for every ion slot from the current positions.
+__global__ void k_rfpar(StepParams P, int grad) {
+    const int i = P.nsites + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.ntot) return;
+    const double4 pa = P.pos[i];
+    double srfe, reff, ds[3], dr[3];
+    const bool in = rf_params(P, pa.x, pa.y, pa.z, abs(P.typei[i]), grad != 0, srfe, reff, ds, dr);
+    P.rfa[i] = make_double4(srfe, reff, in ? 1.0 : 0.0, 0.0);
+    if (grad) {
+        P.rfd[2 * (size_t)i] = make_double4(ds[0], ds[1], ds[2], 0.0);
+        P.rfd[2 * (size_t)i + 1] = make_double4(dr[0], dr[1], dr[2], 0.0);
+    }
+}
+
+// k_rfpair: reaction-field energy and force of each ion, one wavefront per target.
+// The reference visits each ion pair once, from its higher index (rfpar.f90:157-223): a
+// pair counts when the lower ion's srfe is nonzero and the higher ion is moving and inside
+// the grid box.  Gathered here per target t over every other ion p, in the reference's
+// operand order (outer = higher index): aux0 = celec*q_hi*q_lo, energy charged to the
+// higher index, and the force on t from the variation of srfe_t, reff_t and |r_t - r_p|.
+// A moving ion's force goes to frf (added by the step kernel); a fixed ion, which the
+// step kernel does not visit, gets its (unused) reaction-field force in fx directly.
+__global__ __launch_bounds__(BLOCK) void k_rfpair(StepParams P) {
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x >> 6;
+    const int t = P.nsites + blockIdx.x * WPB + wv;
+    if (t >= P.ntot) return;
+    const int nmov0 = P.nsites + P.nfix;
+    const bool tmov = t >= nmov0;
+    const double4 rt = P.rfa[t];
+    if (rt.z == 0.0) {   // outside the grid box: no reaction-field term at all
+        if (lane == 0) {
+            P.frf[t] = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (P.write_energy) P.epart[(size_t)E_RFPAR * P.ecap + t] = 0.0;
+            if (!tmov && P.write_forces) { P.fx[t] = 0.0; P.fy[t] = 0.0; P.fz[t] = 0.0; }
+        }
+        return;
+    }
+    const double4 pt = P.pos[t];
+    const double cgt = P.cg[abs(P.typei[t])];
+    const double srt = rt.x, rft = rt.y;
+    double A = 0.0, B = 0.0, cx = 0.0, cy = 0.0, cz = 0.0, e = 0.0;
+    for (int p = P.nsites + lane; p < P.ntot; p += WAVE) {
+        if (p == t) continue;
+        const double4 rp = P.rfa[p];
+        // lower ion's srfe nonzero, higher ion moving and inside the box (t is inside)
+        const bool hi_t = p < t;
+        if (hi_t ? (rp.x == 0.0 || !tmov) : (srt == 0.0 || rp.z == 0.0 || p < nmov0)) continue;
+        const double4 pp = P.pos[p];
+        const double cgp = P.cg[abs(P.typei[p])];
+        const double s_lo = hi_t ? rp.x : srt, s_hi = hi_t ? srt : rp.x;
+        const double r_lo = hi_t ? rp.y : rft, r_hi = hi_t ? rft : rp.y;
+        const double x_lo = hi_t ? pp.x : pt.x, x_hi = hi_t ? pt.x : pp.x;
+        const double y_lo = hi_t ? pp.y : pt.y, y_hi = hi_t ? pt.y : pp.y;
+        const double z_lo = hi_t ? pp.z : pt.z, z_hi = hi_t ? pt.z : pp.z;
+        const double aux0 = P.celec * (hi_t ? cgt : cgp) * (hi_t ? cgp : cgt);
+        const double dx = x_lo - x_hi, dy = y_lo - y_hi, dz = z_lo - z_hi;
+        const double dist2 = dx * dx + dy * dy + dz * dz;
+        const double srfeij = s_lo * s_hi, reffij = r_lo * r_hi;
+        const double rfdn = 1.0 / sqrt(reffij * reffij + dist2);
+        const double rfcf = reffij * rfdn;
+        const double aux1 = aux0 * rfcf;
+        if (hi_t) e += aux1 * srfeij;
+        const double aux2 = aux0 * srfeij * (rfdn * (rfdn * rfdn));
+        const double aux3 = -aux2 * (dist2 + r_lo * r_hi * 0.5);
+        A += -aux1 * rp.x;             // times d srfe_t
+        B += aux3 * rp.y;              // times d reff_t
+        const double de = aux2 * reffij;
+        cx += de * (pt.x - pp.x);
+        cy += de * (pt.y - pp.y);
+        cz += de * (pt.z - pp.z);
+    }
+    A = wave_sum(A); B = wave_sum(B);
+    cx = wave_sum(cx); cy = wave_sum(cy); cz = wave_sum(cz);
+    if (P.write_energy) e = wave_sum(e);
+    if (lane != 0) return;
+    const double4 dst = P.rfd[2 * (size_t)t], drt = P.rfd[2 * (size_t)t + 1];
+    // self term of a moving ion (rfpar.f90:146-156): aux = celec*q*q*srfe, energy
+    // aux*srfe/2, force -aux*dsrfe
+    const double aux = tmov ? P.celec * cgt * cgt * srt : 0.0;
+    const double a_s = A - aux;
+    const double4 f = make_double4(a_s * dst.x + B * drt.x + cx, a_s * dst.y + B * drt.y + cy,
+                                   a_s * dst.z + B * drt.z + cz, 0.0);
+    if (tmov) {
+        P.frf[t] = f;
+        if (P.write_energy) P.epart[(size_t)E_RFPAR * P.ecap + t] = 0.5 * aux * srt + e;
+    } else if (P.write_forces) {
+        P.fx[t] = f.x; P.fy[t] = f.y; P.fz[t] = f.z;
+    }
+}
+
+hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s) {
+    const int nion = p.ntot - p.nsites;
+    if (!p.rfon || nion <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rfpar, dim3((nion + 255) / 256), dim3(256), 0, s, p, pairs ? 1 : 0);
+    if (pairs)
+        hipLaunchKernelGGL(k_rfpair, dim3((nion + WPB - 1) / WPB), dim3(BLOCK), 0, s, p);
+    return hipGetLastError();
 }
 
 // ---- energy reduction: one block per category, fixed-order tree
@@ -1195,6 +1424,7 @@ __global__ __launch_bounds__(BLOCK) void k_interact(InteractParams Q) {
     __shared__ DevPair s_pairs[MAX_LDS_PAIRS];
     __shared__ int32_t s_pidx[MAX_LDS_TYPES * MAX_LDS_TYPES];
     __shared__ double s_red[WPB];
+    __shared__ double s_rf[3];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
     const int ns = P.nsites, ntot = P.ntot, nt1 = P.nttyp + 1;
     for (int k = tid; k <= P.npair; k += BLOCK) s_pairs[k] = P.pairs[k];
@@ -1292,6 +1522,34 @@ __global__ __launch_bounds__(BLOCK) void k_interact(InteractParams Q) {
             double d2 = dx * dx + dy * dy + dz * dz;
             const DevPair &pp = s_pairs[s_pidx[prow + __double2loint(q.w)]];
             e += (pp.flags & PF_EFPOT) ? gety(pp, P.ep, P.ires, d2) : interact_analytic(pp, P, d2);
+        }
+    }
+    // RFPARIONJ (rfpar.f90:232-370) for an ion: its own srfe/reff at the trial position,
+    // the self term and every other ion with nonzero srfe (ions' parameters from k_rfpar).
+    // Nothing when the trial position is outside the grid box.  (For a site the reference
+    // reads srfe outside its array; this path gives sites no reaction-field term.)
+    if (P.rfon && beion) {
+        if (tid == 0) {
+            double sj, rj, ds[3], dr[3];
+            const bool in = rf_params(P, xj, yj, zj, Q.jtype, false, sj, rj, ds, dr);
+            s_rf[0] = sj; s_rf[1] = rj; s_rf[2] = in ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        if (s_rf[2] != 0.0) {
+            const double sj = s_rf[0], rj = s_rf[1];
+            const double tau = P.celec * P.cg[Q.jtype];
+            if (tid == 0) e += 0.5 * tau * P.cg[Q.jtype] * (sj * sj);
+            for (int i = ns + tid; i < ntot; i += BLOCK) {
+                if (i == j) continue;
+                const double4 ra = P.rfa[i];
+                if (ra.x == 0.0) continue;
+                const double4 q = P.pos[i];
+                const double dx = q.x - xj, dy = q.y - yj, dz = q.z - zj;
+                const double dist2 = dx * dx + dy * dy + dz * dz;
+                const double reffij = rj * ra.y;
+                e += tau * P.cg[abs(P.typei[i])] * reffij * sj * ra.x /
+                     sqrt(reffij * reffij + dist2);
+            }
         }
     }
     e = wave_sum(e);

@@ -98,6 +98,19 @@ class System:
     fact0n2: float = 0.0
     box: tuple = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0)   # lx2m, lx2p, ly2m, ly2p, lz2m, lz2p
     zcont: np.ndarray = None
+    # position-dependent diffusion (DYNAMICS1 / DYNAMICS2)
+    dt: float = 0.0
+    kbtdt: float = 0.0
+    diffusion: np.ndarray = None     # [ntype+1]
+    plength2: float = 0.0
+    pcenter: float = 0.0
+    ampl3: np.ndarray = None         # [ntype+1]
+    p3: np.ndarray = None            # [ntype+1]
+    spline: tuple = None             # (xs, ys, b, c, d) of the PROFILE command
+    # reaction-field parameter grids: rf.data is gsrfen (ngrid, nx, ny, nz) float32
+    rf: GridData | None = None
+    rf_greff: np.ndarray = None      # (ngrid, nx, ny, nz) float32
+    sqrfac: float = 1.0
     ndna: int = 0
     nion: int = 0
     meta: dict = field(default_factory=dict)
@@ -209,6 +222,32 @@ class System:
         (s.lx2m, s.lx2p, s.ly2m, s.ly2p, s.lz2m, s.lz2p) = self.box
         s.cntpts = len(self.zcont)
         s.zcont = arr(self.zcont if len(self.zcont) else np.zeros(1), np.float64, C.c_double)
+        s.dt, s.kbtdt = self.dt, self.kbtdt
+        nt1 = self.ntype + 1
+        one = np.zeros(1)
+        s.diffusion = arr(self.diffusion if self.diffusion is not None else np.zeros(nt1),
+                          np.float64, C.c_double)
+        s.plength2, s.pcenter = self.plength2, self.pcenter
+        s.ampl3 = arr(self.ampl3 if self.ampl3 is not None else np.zeros(nt1), np.float64, C.c_double)
+        s.p3 = arr(self.p3 if self.p3 is not None else np.ones(nt1), np.float64, C.c_double)
+        sp = self.spline if self.spline is not None else (one,) * 5
+        s.nspline = len(sp[0]) if self.spline is not None else 0
+        s.xs, s.ys, s.bs, s.cs, s.ds = (arr(a, np.float64, C.c_double) for a in sp)
+        if self.rf is not None:
+            gs = np.ascontiguousarray(self.rf.data, dtype=np.float32)
+            gr = np.ascontiguousarray(self.rf_greff, dtype=np.float32)
+            assert gs.shape == gr.shape and gs.ndim == 4
+            keep.extend([gs, gr])
+            s.rf.nclx, s.rf.ncly, s.rf.nclz = gs.shape[1:]
+            s.rf.dcel, s.rf.xbcen, s.rf.ybcen, s.rf.zbcen = (self.rf.dcel, self.rf.xbcen,
+                                                             self.rf.ybcen, self.rf.zbcen)
+            s.gsrfen = gs.ctypes.data_as(C.POINTER(C.c_float))
+            s.greff = gr.ctypes.data_as(C.POINTER(C.c_float))
+        else:
+            z = np.zeros(1, np.float32)
+            keep.append(z)
+            s.gsrfen = s.greff = z.ctypes.data_as(C.POINTER(C.c_float))
+        s.sqrfac = self.sqrfac
         return s, keep
 
 

@@ -18,6 +18,12 @@
  * bromoc_grand                 SUBROUTINE GRAND             grand.f90:19-121
  * bromoc_build_table           EFPOT table construction     splinepot.f90:19-96 +
  *                              squarespline polynom.f90:20-48
+ * bromoc_spline                SUBROUTINE SPLINE            spline.f90:19-98
+ *
+ * bromoc_step / bromoc_dynamics move the ions with DYNAMICS0, DYNAMICS1
+ * (dynamics1.mono.f90:19-90, flag BROMOC_QDIFFUSE) or DYNAMICS2 (dynamics2.mono.f90:19-89,
+ * BROMOC_QPROFILE), as simul1.f90:369-378 selects; ENERGY and INTERACT include RFPARION /
+ * RFPARIONJ (rfpar.f90:20-370) under BROMOC_QRFPAR.
  *
  * Conventions
  *  - Particle indices are 0-based (C index i is Fortran index i+1).  Sites occupy
@@ -42,7 +48,7 @@
 extern "C" {
 #endif
 
-#define BROMOC_ABI_VERSION 1
+#define BROMOC_ABI_VERSION 2
 
 /* status codes */
 #define BROMOC_OK              0
@@ -65,6 +71,10 @@ extern "C" {
 #define BROMOC_QDEBY       (1u << 8)   /* Debye screening of P-P Coulomb (Qdeby) */
 #define BROMOC_QCOUNTION   (1u << 9)   /* flux counting across z planes (Qcountion) */
 #define BROMOC_QDNAFREE    (1u << 10)  /* DNA sites not confined by fixcoor (Qdnafree) */
+#define BROMOC_QDIFFUSE    (1u << 28)  /* DYNAMICS1: pore-switched diffusion (Qdiffuse) */
+#define BROMOC_QPROFILE    (1u << 29)  /* DYNAMICS2: spline diffusion profile (Qprofile) */
+#define BROMOC_QRFPAR      (1u << 25)  /* reaction-field parameter grids, rfpar.f90 (Qrfpar) */
+#define BROMOC_QRFPSIN     (1u << 30)  /* one reaction-field grid for every ion type (Qrfpsin) */
 /* options of the reference that this path rejects with BROMOC_E_UNSUPPORTED */
 #define BROMOC_QMEMB       (1u << 16)  /* analytic membrane (membrane.f90) */
 #define BROMOC_QMMIJ       (1u << 17)  /* MMIJ reaction field (rect_rf1/sphe_rf1) */
@@ -75,11 +85,8 @@ extern "C" {
 #define BROMOC_QNOTRANS    (1u << 22)  /* NOTRANS centroid reset */
 #define BROMOC_QAPFOR      (1u << 23)  /* constant applied forces */
 #define BROMOC_QCONTRANS   (1u << 24)  /* centroid restraints */
-#define BROMOC_QRFPAR      (1u << 25)  /* reaction-field parameter grids (rfpar.f90) */
 #define BROMOC_QSPHERE     (1u << 26)  /* spherical system (fixcoor.f90:24-31) */
 #define BROMOC_QECYL       (1u << 27)  /* elliptic-cylinder system (fixcoor.f90:33-46) */
-#define BROMOC_QDIFFUSE    (1u << 28)  /* DYNAMICS1 switch diffusion */
-#define BROMOC_QPROFILE    (1u << 29)  /* DYNAMICS2 spline diffusion profile */
 
 /* site names (nucleotmod namsite) */
 #define BROMOC_SITE_S   0
@@ -155,6 +162,20 @@ typedef struct bromoc_system {
     double lx2m, lx2p, ly2m, ly2p, lz2m, lz2p;
     int32_t cntpts, reserved1;
     const double *zcont;         /* [cntpts] flux-counting planes */
+    /* position-dependent diffusion of the ions (QDIFFUSE: DYNAMICS1, dynamics1.mono.f90;
+     * QPROFILE: DYNAMICS2, dynamics2.mono.f90) */
+    double dt, kbtdt;            /* time step and dt/kT (simul1.f90:342) */
+    const double *diffusion;     /* [ntype+1] grandmod diffusion(itype) */
+    double plength2, pcenter;    /* DIFFUSION command: half pore length, pore centre */
+    const double *ampl3, *p3;    /* [ntype+1] lowfraction, switchlength per ion type */
+    int32_t nspline, reserved2;  /* PROFILE knots (splinemod) */
+    const double *xs, *ys;       /* [nspline] knots and values */
+    const double *bs, *cs, *ds;  /* [nspline] spline coefficients b, c, d (spline.f90) */
+    /* reaction-field parameter grids (QRFPAR, gsbpmod: nclx3.., gsrfen, greff) */
+    bromoc_grid rf;              /* geometry; data unused */
+    const float *gsrfen, *greff; /* [ngrid * nclx*ncly*nclz], ngrid = 1 with QRFPSIN, else
+                                  * ntype - nold (one grid per ion type, in type order) */
+    double sqrfac;               /* scale of greff (gsbpmod sqrfac) */
 } bromoc_system;
 
 /* Energy terms as in grandmod/gsbpmod/nucleotmod (energy.mono.f90:46-68,689). */
@@ -250,6 +271,11 @@ int bromoc_grand(bromoc_ctx *ctx, int32_t ngcmc, const double *uniforms, int64_t
  * dmi, dm2lo, dm2hi, sc1..sc3 (host-side, splinepot.f90:19-96). */
 int bromoc_build_table(int32_t n, const double *r, const double *u, int32_t chr, double fct,
                        double *ep, bromoc_pair *pair);
+
+/* Cubic-spline coefficients of the diffusion profile (spline.f90:19-98, the
+ * Forsythe-Malcolm-Moler SPLINE): b, c, d of n >= 2 knots x (strictly monotone), y.
+ * Host-side, as shell_simul.f90:1968 calls it for the PROFILE command. */
+int bromoc_spline(int32_t n, const double *x, const double *y, double *b, double *c, double *d);
 
 /* n raw Philox4x64-10 blocks computed on the device (the generator behind
  * BROMOC_NOISE_PHILOX): out[4k..4k+3] = philox(ctr[4k..4k+3], key[2k..2k+1]).  No

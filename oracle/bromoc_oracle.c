@@ -58,6 +58,12 @@ typedef struct orc_ctx {
     int32_t *ibfftyp, *ntotat, *ibuffer;
     double *avnum, *mu, *kb, *lzmin, *lzmax;
     int8_t *bufferbias;
+    /* DYNAMICS1/2 */
+    double *diffusion, *ampl3, *p3, *xs, *ys, *bs, *cs, *ds;
+    int seval_i, sevald_i;        /* the SAVEd intervals of seval and sevald (1-based) */
+    /* RFPAR */
+    float *gsrfen, *greff;
+    double *srfe, *reff, *srfed, *reffd;   /* per ion slot; derivatives [3*cap] */
 } orc_ctx;
 
 static void *dupmem(const void *p, size_t n) {
@@ -272,6 +278,61 @@ EXPORT int orc_build_table(int32_t n, const double *xx, const double *yy, int32_
     return BROMOC_OK;
 }
 
+/* SPLINE, spline.f90:19-98 (Forsythe, Malcolm and Moler's cubic spline) */
+EXPORT int orc_spline(int32_t n, const double *x, const double *y, double *b, double *c,
+                      double *d) {
+    if (n < 2) return BROMOC_E_ARG;
+    const double *X = x - 1, *Y = y - 1;
+    double *B = b - 1, *Cc = c - 1, *D = d - 1;
+    int nm1 = n - 1;
+    if (n > 2) {
+        D[1] = X[2] - X[1];
+        Cc[2] = (Y[2] - Y[1]) / D[1];
+        for (int i = 2; i <= nm1; i++) {
+            D[i] = X[i + 1] - X[i];
+            B[i] = 2.0 * (D[i - 1] + D[i]);
+            Cc[i + 1] = (Y[i + 1] - Y[i]) / D[i];
+            Cc[i] = Cc[i + 1] - Cc[i];
+        }
+        B[1] = -D[1];
+        B[n] = -D[n - 1];
+        Cc[1] = 0.0;
+        Cc[n] = 0.0;
+        if (n > 3) {
+            Cc[1] = Cc[3] / (X[4] - X[2]) - Cc[2] / (X[3] - X[1]);
+            Cc[n] = Cc[n - 1] / (X[n] - X[n - 2]) - Cc[n - 2] / (X[n - 1] - X[n - 3]);
+            Cc[1] = Cc[1] * powi(D[1], 2) / (X[4] - X[1]);
+            Cc[n] = -Cc[n] * powi(D[n - 1], 2) / (X[n] - X[n - 3]);
+        }
+        for (int i = 2; i <= n; i++) {
+            double t = D[i - 1] / B[i - 1];
+            B[i] = B[i] - t * D[i - 1];
+            Cc[i] = Cc[i] - t * Cc[i - 1];
+        }
+        Cc[n] = Cc[n] / B[n];
+        for (int ib = 1; ib <= nm1; ib++) {
+            int i = n - ib;
+            Cc[i] = (Cc[i] - D[i] * Cc[i + 1]) / B[i];
+        }
+        B[n] = (Y[n] - Y[nm1]) / D[nm1] + D[nm1] * (Cc[nm1] + 2.0 * Cc[n]);
+        for (int i = 1; i <= nm1; i++) {
+            B[i] = (Y[i + 1] - Y[i]) / D[i] - D[i] * (Cc[i + 1] + 2.0 * Cc[i]);
+            D[i] = (Cc[i + 1] - Cc[i]) / D[i];
+            Cc[i] = 3.0 * Cc[i];
+        }
+        Cc[n] = 3.0 * Cc[n];
+        D[n] = D[n - 1];
+    } else {
+        B[1] = (Y[2] - Y[1]) / (X[2] - X[1]);
+        Cc[1] = 0.0;
+        D[1] = 0.0;
+        B[2] = B[1];
+        Cc[2] = 0.0;
+        D[2] = 0.0;
+    }
+    return BROMOC_OK;
+}
+
 /* ------------------------------------------------------------ context */
 EXPORT void orc_destroy(orc_ctx *c) {
     if (!c) return;
@@ -279,7 +340,9 @@ EXPORT void orc_destroy(orc_ctx *c) {
                   c->nwtype, c->pairidx, c->stfree, c->cg, c->fact1a, c->fact2a, c->ep,
                   c->scal, c->zcont, c->pairs, c->phix, c->phiv, c->warn, c->nforward,
                   c->nbackward, c->noise, c->ibfftyp, c->ntotat, c->ibuffer, c->avnum, c->mu,
-                  c->kb, c->lzmin, c->lzmax, c->bufferbias};
+                  c->kb, c->lzmin, c->lzmax, c->bufferbias, c->diffusion, c->ampl3, c->p3,
+                  c->xs, c->ys, c->bs, c->cs, c->ds, c->gsrfen, c->greff, c->srfe, c->reff,
+                  c->srfed, c->reffd};
     for (size_t k = 0; k < sizeof(ps) / sizeof(ps[0]); k++) free(ps[k]);
     for (int k = 0; k < 8; k++) free(c->lists[k]);
     for (int k = 0; k < 6; k++) free(c->lvals[k]);
@@ -340,6 +403,31 @@ EXPORT orc_ctx *orc_create(const bromoc_system *s) {
     c->nforward = (int32_t *)calloc((size_t)(nt > 0 ? nt : 1) * np, sizeof(int32_t));
     c->nbackward = (int32_t *)calloc((size_t)(nt > 0 ? nt : 1) * np, sizeof(int32_t));
     c->celec = celec_value();
+    if (s->flags & (BROMOC_QDIFFUSE | BROMOC_QPROFILE))
+        c->diffusion = (double *)dupmem(s->diffusion, sizeof(double) * (s->ntype + 1));
+    if (s->flags & BROMOC_QDIFFUSE) {
+        c->ampl3 = (double *)dupmem(s->ampl3, sizeof(double) * (s->ntype + 1));
+        c->p3 = (double *)dupmem(s->p3, sizeof(double) * (s->ntype + 1));
+    }
+    if (s->flags & BROMOC_QPROFILE) {
+        size_t n = sizeof(double) * (size_t)s->nspline;
+        c->xs = (double *)dupmem(s->xs, n);
+        c->ys = (double *)dupmem(s->ys, n);
+        c->bs = (double *)dupmem(s->bs, n);
+        c->cs = (double *)dupmem(s->cs, n);
+        c->ds = (double *)dupmem(s->ds, n);
+    }
+    c->seval_i = c->sevald_i = 1;
+    if (s->flags & BROMOC_QRFPAR) {
+        size_t n = (size_t)s->rf.nclx * s->rf.ncly * s->rf.nclz;
+        size_t ng = (s->flags & BROMOC_QRFPSIN) ? 1 : (size_t)(s->ntype - s->nold);
+        c->gsrfen = (float *)dupmem(s->gsrfen, sizeof(float) * n * ng);
+        c->greff = (float *)dupmem(s->greff, sizeof(float) * n * ng);
+    }
+    c->srfe = (double *)calloc(cap + 1, sizeof(double));
+    c->reff = (double *)calloc(cap + 1, sizeof(double));
+    c->srfed = (double *)calloc(3 * (size_t)cap + 3, sizeof(double));
+    c->reffd = (double *)calloc(3 * (size_t)cap + 3, sizeof(double));
     return c;
 }
 
@@ -389,7 +477,7 @@ EXPORT int orc_get_warn(orc_ctx *c, int32_t *warn) {
 static uint32_t supported_mask(void) {
     return BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BROMOC_QNOBOND | BROMOC_QPHIX |
            BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV | BROMOC_QDEBY | BROMOC_QCOUNTION |
-           BROMOC_QDNAFREE;
+           BROMOC_QDNAFREE | BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR | BROMOC_QRFPSIN;
 }
 
 EXPORT int orc_check_flags(uint32_t flags) {
@@ -590,6 +678,183 @@ static void add_f(orc_ctx *c, int i, double sgn, const double *f) {
     c->fz[i] = c->fz[i] + sgn * f[2];
 }
 
+/* ------------------------------------------------------------ RFPAR */
+static int rf_inbox(const bromoc_system *s, double x, double y, double z) {
+    const bromoc_grid *g = &s->rf;
+    double tx = 0.5 * (g->nclx - 1) * g->dcel, ty = 0.5 * (g->ncly - 1) * g->dcel,
+           tz = 0.5 * (g->nclz - 1) * g->dcel;
+    return x <= g->xbcen + tx && x >= g->xbcen - tx && y <= g->ybcen + ty &&
+           y >= g->ybcen - ty && z <= g->zbcen + tz && z >= g->zbcen - tz;
+}
+
+/* srfe, reff (and gradients when qf) of one ion from the 8 corners of gsrfen and
+ * sqrfac*greff, rfpar.f90:86-142 (the same loop as rfpar.f90:306-345); the innermost loop
+ * is left at a zero gsrfen corner (`exit`, rfpar.f90:121). */
+static void rf_point(const orc_ctx *c, double x, double y, double z, int itype, int qf,
+                     double *srfe, double *reff, double *d1, double *d2) {
+    const bromoc_system *s = &c->s;
+    const bromoc_grid *g = &s->rf;
+    double dcel = g->dcel, idcel = 1.0 / dcel;
+    double tx = 0.5 * (g->nclx - 1) * dcel, ty = 0.5 * (g->ncly - 1) * dcel,
+           tz = 0.5 * (g->nclz - 1) * dcel;
+    long ncyz = (long)g->ncly * g->nclz, ncel3 = (long)g->nclx * ncyz;
+    long ifir = (s->flags & BROMOC_QRFPSIN) ? 0 : (long)(itype - s->nold - 1) * ncel3;
+    double one = 1.0, aux1 = 0.0, aux2 = 0.0;
+    double a1x = 0.0, a1y = 0.0, a1z = 0.0, a2x = 0.0, a2y = 0.0, a2z = 0.0;
+    double xi = x + tx - g->xbcen, yi = y + ty - g->ybcen, zi = z + tz - g->zbcen;
+    int ix = (int)(xi * idcel), iy = (int)(yi * idcel), iz = (int)(zi * idcel);
+    if (ix == g->nclx - 1) ix = g->nclx - 2;
+    if (iy == g->ncly - 1) iy = g->ncly - 2;
+    if (iz == g->nclz - 1) iz = g->nclz - 2;
+    for (int n1 = ix; n1 <= ix + 1; n1++) {
+        double ai = xi - n1 * dcel;
+        double aisign = fsign(one, ai);
+        ai = one - fabs(ai) * idcel;
+        for (int n2 = iy; n2 <= iy + 1; n2++) {
+            double bi = yi - n2 * dcel;
+            double bisign = fsign(one, bi);
+            bi = one - fabs(bi) * idcel;
+            for (int n3 = iz; n3 <= iz + 1; n3++) {
+                double ci = zi - n3 * dcel;
+                double cisign = fsign(one, ci);
+                ci = one - fabs(ci) * idcel;
+                double fi = ai * bi * ci;
+                long in3 = n1 * ncyz + (long)n2 * g->nclz + n3;
+                double gaux1 = c->gsrfen[in3 + ifir];
+                double gaux2 = s->sqrfac * c->greff[in3 + ifir];
+                if (gaux1 == 0.0) break;
+                double prefa1 = gaux1 * idcel, prefa2 = gaux2 * idcel;
+                aux1 = aux1 + fi * gaux1;
+                aux2 = aux2 + fi * gaux2;
+                if (qf) {
+                    if ((ai < (one - RSMALL)) && (ai > RSMALL)) {
+                        double aux = aisign * bi * ci;
+                        a1x = a1x - aux * prefa1;
+                        a2x = a2x - aux * prefa2;
+                    }
+                    if ((bi < (one - RSMALL)) && (bi > RSMALL)) {
+                        double aux = bisign * ai * ci;
+                        a1y = a1y - aux * prefa1;
+                        a2y = a2y - aux * prefa2;
+                    }
+                    if ((ci < (one - RSMALL)) && (ci > RSMALL)) {
+                        double aux = cisign * ai * bi;
+                        a1z = a1z - aux * prefa1;
+                        a2z = a2z - aux * prefa2;
+                    }
+                }
+            }
+        }
+    }
+    *srfe = aux1;
+    *reff = aux2;
+    if (qf) {
+        d1[0] = a1x; d1[1] = a1y; d1[2] = a1z;
+        d2[0] = a2x; d2[1] = a2y; d2[2] = a2z;
+    }
+}
+
+/* RFPARION, rfpar.f90:20-228 */
+static void rfparion(orc_ctx *c, int qforces, double *erfpar) {
+    const bromoc_system *s = &c->s;
+    int ns = s->nsites, nf = s->nfix, nt = s->ntot;
+    double E = 0.0;
+    for (int i = ns; i < nt; i++) c->srfe[i] = c->reff[i] = 0.0;
+    for (int i = ns; i < nt; i++) {
+        if (!rf_inbox(s, c->x[i], c->y[i], c->z[i])) continue;
+        int itype = abs(c->typei[i]);
+        double *dsi = c->srfed + 3 * (size_t)i, *dri = c->reffd + 3 * (size_t)i;
+        rf_point(c, c->x[i], c->y[i], c->z[i], itype, qforces, &c->srfe[i], &c->reff[i], dsi, dri);
+        if (i < ns + nf) continue;
+        double tau = c->celec * c->cg[itype];
+        double aux = tau * c->cg[itype] * c->srfe[i];
+        E = E + 0.5 * aux * c->srfe[i];
+        if (qforces) {
+            double de = -aux;
+            c->fx[i] = c->fx[i] + de * dsi[0];
+            c->fy[i] = c->fy[i] + de * dsi[1];
+            c->fz[i] = c->fz[i] + de * dsi[2];
+        }
+        for (int j = ns; j < i; j++) {
+            if (c->srfe[j] == 0.0) continue;
+            int jtype = abs(c->typei[j]);
+            double ddx = c->x[j] - c->x[i], ddy = c->y[j] - c->y[i], ddz = c->z[j] - c->z[i];
+            double dist2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            double srfeij = c->srfe[j] * c->srfe[i];
+            double reffij = c->reff[j] * c->reff[i];
+            double rfdn = 1.0 / sqrt(reffij * reffij + dist2);
+            double rfcf = reffij * rfdn;
+            double aux0 = tau * c->cg[jtype];
+            double aux1 = aux0 * rfcf;
+            E = E + aux1 * srfeij;
+            if (qforces) {
+                double *dsj = c->srfed + 3 * (size_t)j, *drj = c->reffd + 3 * (size_t)j;
+                double aux2 = aux0 * srfeij * (rfdn * (rfdn * rfdn));
+                double de = -aux1 * c->srfe[i];
+                c->fx[j] = c->fx[j] + de * dsj[0];
+                c->fy[j] = c->fy[j] + de * dsj[1];
+                c->fz[j] = c->fz[j] + de * dsj[2];
+                de = -aux1 * c->srfe[j];
+                c->fx[i] = c->fx[i] + de * dsi[0];
+                c->fy[i] = c->fy[i] + de * dsi[1];
+                c->fz[i] = c->fz[i] + de * dsi[2];
+                double aux3 = -aux2 * (dist2 + c->reff[j] * c->reff[i] * 0.5);
+                de = aux3 * c->reff[i];
+                c->fx[j] = c->fx[j] + de * drj[0];
+                c->fy[j] = c->fy[j] + de * drj[1];
+                c->fz[j] = c->fz[j] + de * drj[2];
+                de = aux3 * c->reff[j];
+                c->fx[i] = c->fx[i] + de * dri[0];
+                c->fy[i] = c->fy[i] + de * dri[1];
+                c->fz[i] = c->fz[i] + de * dri[2];
+                de = aux2 * reffij;
+                c->fx[j] = c->fx[j] + de * (c->x[j] - c->x[i]);
+                c->fy[j] = c->fy[j] + de * (c->y[j] - c->y[i]);
+                c->fz[j] = c->fz[j] + de * (c->z[j] - c->z[i]);
+                c->fx[i] = c->fx[i] - de * (c->x[j] - c->x[i]);
+                c->fy[i] = c->fy[i] - de * (c->y[j] - c->y[i]);
+                c->fz[i] = c->fz[i] - de * (c->z[j] - c->z[i]);
+            }
+        }
+    }
+    *erfpar = E;
+}
+
+/* RFPARIONJ, rfpar.f90:232-370, for an ion j (j == ntot: an insertion trial, for which the
+ * reference raises ntot by one around the call, interact.f90:145-148) */
+static double rfparionj(orc_ctx *c, double xj, double yj, double zj, int j, int jtype) {
+    const bromoc_system *s = &c->s;
+    int ns = s->nsites, n = (j == s->ntot) ? s->ntot + 1 : s->ntot;
+    if (!rf_inbox(s, xj, yj, zj)) return 0.0;
+    double *sr = (double *)calloc((size_t)n + 1, sizeof(double));
+    double *rf = (double *)calloc((size_t)n + 1, sizeof(double));
+    for (int i = ns; i < n; i++) {
+        int itype, ok;
+        double xi, yi, zi;
+        if (i == j) {
+            itype = jtype; xi = xj; yi = yj; zi = zj; ok = 1;
+        } else {
+            itype = abs(c->typei[i]); xi = c->x[i]; yi = c->y[i]; zi = c->z[i];
+            ok = rf_inbox(s, xi, yi, zi);
+        }
+        if (ok) rf_point(c, xi, yi, zi, itype, 0, &sr[i], &rf[i], NULL, NULL);
+    }
+    double tau = c->celec * c->cg[jtype];
+    double E = 0.0;
+    E = E + 0.5 * tau * c->cg[jtype] * (sr[j] * sr[j]);
+    for (int i = ns; i < n; i++) {
+        if (i == j || sr[i] == 0.0) continue;
+        int itype = abs(c->typei[i]);
+        double ddx = c->x[i] - xj, ddy = c->y[i] - yj, ddz = c->z[i] - zj;
+        double dist2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        double reffij = rf[j] * rf[i];
+        E = E + tau * c->cg[itype] * reffij * sr[j] * sr[i] / sqrt(reffij * reffij + dist2);
+    }
+    free(sr);
+    free(rf);
+    return E;
+}
+
 EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
     const bromoc_system *s = &c->s;
     if (orc_check_flags(s->flags)) return BROMOC_E_UNSUPPORTED;
@@ -608,6 +873,11 @@ EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
     if (fl & BROMOC_QPHIV) {
         vdwgd1trln(c, qforces, &E.evdwgd);
         E.ener = E.ener + E.evdwgd;
+    }
+    if (fl & BROMOC_QRFPAR) {   /* energy.mono.f90:111-115 */
+        rfparion(c, qforces, &E.erfpar);
+        E.ener = E.ener + E.erfpar;
+        E.egsbpb = E.erfpar;
     }
     /* ion-ion, energy.mono.f90:117-194 */
     if ((fl & BROMOC_QPAR) && (fl & BROMOC_QNONBOND)) {
@@ -872,24 +1142,109 @@ static void countions(orc_ctx *c, double zold, double zi, int it) {
     }
 }
 
-/* DYNAMICS0 (dynamics0.mono.f90:31-46) over moving ions, then DYNAMICS0NUC
- * (dynamics0nuc.f90:31-44) over sites; g = Gaussian for (particle, component) */
+/* SEVAL / SEVALD (sevalmod.f90:23-67) with their SAVEd intervals *ic (1-based) */
+static double seval_orc(const orc_ctx *c, int *ic, double u, int deriv) {
+    int n = c->s.nspline;
+    const double *X = c->xs - 1;
+    int i = *ic, j, k;
+    if (i >= n) i = 1;
+    if (!(u < X[i]) && u <= X[i + 1]) goto found;
+    i = 1;
+    j = n + 1;
+    for (int it = 0; it < 64; it++) {
+        k = (i + j) / 2;
+        if (u < X[k]) j = k;
+        if (u >= X[k]) i = k;
+        if (!(j > i + 1)) break;
+    }
+found:
+    *ic = i;
+    double dx = u - X[i];
+    if (deriv) return c->bs[i - 1] + dx * (c->cs[i - 1] + dx * c->ds[i - 1]);
+    return c->ys[i - 1] + dx * (c->bs[i - 1] + dx * (c->cs[i - 1] + dx * c->ds[i - 1]));
+}
+
+/* drift and noise factors of an ion: DYNAMICS0 fact1a/fact2a; DYNAMICS1 pore switch
+ * (dynamics1.mono.f90:36-70); DYNAMICS2 profile (dynamics2.mono.f90:38-74) */
+static void ion_factors(orc_ctx *c, int itype, double zz, double *fact1, double *fact2,
+                        double *delDz) {
+    const bromoc_system *s = &c->s;
+    double sw = 0.0, dsw = 0.0, idiffusion, didiffusion;
+    if (s->flags & BROMOC_QDIFFUSE) {
+        double pp3 = c->p3[itype];
+        double pore1 = -s->plength2 + s->pcenter, pore2 = s->plength2 + s->pcenter;
+        if (zz > (pore1 - pp3) && zz < (pore2 + pp3)) {
+            if (zz >= pore1 && zz <= pore2) {
+                sw = 0.0;
+                dsw = 0.0;
+            } else {
+                double ipp3 = 1.0 / pp3, delz;
+                if (zz > pore2) {
+                    delz = (zz - pore2);
+                    sw = 2.0 * powi(delz * ipp3, 3) - 3.0 * powi(delz * ipp3, 2);
+                    dsw = 6.0 * (powi(delz * ipp3, 2) - delz * ipp3) * ipp3;
+                } else if (zz < pore1) {
+                    delz = (zz - pore1);
+                    sw = -2.0 * powi(delz * ipp3, 3) - 3.0 * powi(delz * ipp3, 2);
+                    dsw = -6.0 * (powi(delz * ipp3, 2) + delz * ipp3) * ipp3;
+                }
+                sw = -sw;
+                dsw = -dsw;
+            }
+        } else {
+            sw = 1.0;
+            dsw = 0.0;
+        }
+        idiffusion = c->diffusion[itype] * (c->ampl3[itype] + (1.0 - c->ampl3[itype]) * sw);
+        didiffusion = c->diffusion[itype] * (1.0 - c->ampl3[itype]) * dsw;
+    } else if (s->flags & BROMOC_QPROFILE) {
+        int n = s->nspline;
+        const double *xs = c->xs, *ys = c->ys;
+        if (xs[0] < xs[n - 1]) {
+            if (zz < xs[0]) { sw = ys[0]; dsw = 0.0; }
+            else if (zz > xs[n - 1]) { sw = ys[n - 1]; dsw = 0.0; }
+            else { sw = seval_orc(c, &c->seval_i, zz, 0); dsw = seval_orc(c, &c->sevald_i, zz, 1); }
+        } else if (xs[0] > xs[n - 1]) {
+            if (zz > xs[0]) { sw = ys[0]; dsw = 0.0; }
+            else if (zz < xs[n - 1]) { sw = ys[n - 1]; dsw = 0.0; }
+            else { sw = seval_orc(c, &c->seval_i, zz, 0); dsw = seval_orc(c, &c->sevald_i, zz, 1); }
+        }
+        idiffusion = c->diffusion[itype] * sw;
+        didiffusion = c->diffusion[itype] * dsw;
+    } else {
+        *fact1 = c->fact1a[itype];
+        *fact2 = c->fact2a[itype];
+        *delDz = 0.0;
+        return;
+    }
+    *fact1 = idiffusion * s->kbtdt;
+    *fact2 = sqrt(2.0 * s->dt * idiffusion);
+    *delDz = didiffusion * s->dt;
+}
+
+/* DYNAMICS0/1/2 (dynamics0.mono.f90:31-46, dynamics1.mono.f90:71-86,
+ * dynamics2.mono.f90:75-85) over moving ions, then DYNAMICS0NUC (dynamics0nuc.f90:31-44)
+ * over sites; g = Gaussian for (particle, component) */
 static void dynamics_pass(orc_ctx *c, const double *g) {
     const bromoc_system *s = &c->s;
     double bdmax = s->bdmax;
+    int dyn12 = (s->flags & (BROMOC_QDIFFUSE | BROMOC_QPROFILE)) != 0;
     if (s->flags & BROMOC_QPAR) {
         for (int i = s->nsites + s->nfix; i < s->ntot; i++) {
             int itype = abs(c->typei[i]);
-            double delx = c->fx[i] * c->fact1a[itype];
-            double dely = c->fy[i] * c->fact1a[itype];
-            double delz = c->fz[i] * c->fact1a[itype];
+            double fact1, fact2, delDz;
+            ion_factors(c, itype, c->z[i], &fact1, &fact2, &delDz);
+            double delx = c->fx[i] * fact1;
+            double dely = c->fy[i] * fact1;
+            double delz = c->fz[i] * fact1;
             if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
             if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
             if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
-            c->x[i] = c->x[i] + delx + c->fact2a[itype] * g[3 * i + 0];
-            c->y[i] = c->y[i] + dely + c->fact2a[itype] * g[3 * i + 1];
+            c->x[i] = c->x[i] + delx + fact2 * g[3 * i + 0];
+            c->y[i] = c->y[i] + dely + fact2 * g[3 * i + 1];
             double zold = c->z[i];
-            c->z[i] = c->z[i] + delz + c->fact2a[itype] * g[3 * i + 2];
+            if (dyn12) c->z[i] = c->z[i] + delz + fact2 * g[3 * i + 2] + delDz;
+            else c->z[i] = c->z[i] + delz + fact2 * g[3 * i + 2];
             if (s->flags & BROMOC_QCOUNTION) countions(c, zold, c->z[i], itype - s->nold - 1);
             fixcoor(s, &c->x[i], &c->y[i], &c->z[i]);
         }
@@ -1116,6 +1471,9 @@ EXPORT int orc_interact(orc_ctx *c, double xj, double yj, double zj, int32_t jty
     int ns = s->nsites, nt = s->ntot;
     int beion = (fl & BROMOC_QPAR) && j >= ns;
     int jtype2 = beion ? c->nwtype[jtype] : c->typtyp[j];
+    /* interact.f90:144-154; the reference also calls it for a site j, reading srfe
+     * outside its array (rfpar.f90:356); sites get no reaction-field term here */
+    if ((fl & BROMOC_QRFPAR) && beion) dener = dener + rfparionj(c, xj, yj, zj, j, jtype);
     if (fl & BROMOC_QPHIX) dener = dener + staticf0(c, xj, yj, zj, j, jtype);
     if (fl & BROMOC_QPHIV) dener = dener + vdwgd0trln(c, xj, yj, zj, jtype2, alert);
     if (beion && (fl & BROMOC_QNONBOND)) {

@@ -50,6 +50,7 @@ def oracle_lib():
         for name, args, res in [
             ("orc_create", [P(abi.SystemStruct)], vp),
             ("orc_destroy", [vp], None),
+            ("orc_spline", [C.c_int32, _dp, _dp, _dp, _dp, _dp], C.c_int),
             ("orc_set_positions", [vp, C.c_int32, _dp, _dp, _dp, _ip], C.c_int),
             ("orc_get_positions", [vp, _dp, _dp, _dp, _ip, _ip], C.c_int),
             ("orc_get_forces", [vp, _dp, _dp, _dp], C.c_int),
@@ -231,6 +232,7 @@ def ref_lib(path=REF_SO):
                            C.c_int64, _dp], None),
             ("ref_build_table", [C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp,
                                  P(abi.Pair)], None),
+            ("ref_spline", [C.c_int32, _dp, _dp, _dp, _dp, _dp], None),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -242,6 +244,25 @@ def ref_lib(path=REF_SO):
             L.bromoc_gpu_set_seed.restype = None
         _rlibs[path] = L
     return _rlibs[path]
+
+
+def _spline_with(fn, x, y):
+    x = np.ascontiguousarray(x, float)
+    y = np.ascontiguousarray(y, float)
+    n = len(x)
+    b, c, d = np.zeros(n), np.zeros(n), np.zeros(n)
+    fn(n, _p(x), _p(y), _p(b), _p(c), _p(d))
+    return b, c, d
+
+
+def oracle_spline(x, y):
+    """(b, c, d) of spline.f90 from the C restatement."""
+    return _spline_with(oracle_lib().orc_spline, x, y)
+
+
+def ref_spline(x, y):
+    """(b, c, d) from the reference's own SPLINE."""
+    return _spline_with(ref_lib().ref_spline, x, y)
 
 
 def ref_table_builder(n, r, u, chr_, fct):

@@ -44,6 +44,15 @@ type, bind(C) :: c_system
   real(c_double) :: bdmax, fact0n1, fact0n2, lx2m, lx2p, ly2m, ly2p, lz2m, lz2p
   integer(c_int32_t) :: cntpts, reserved1
   type(c_ptr) :: zcont
+  real(c_double) :: dt, kbtdt
+  type(c_ptr) :: diffusion
+  real(c_double) :: plength2, pcenter
+  type(c_ptr) :: ampl3, p3
+  integer(c_int32_t) :: nspline, reserved2
+  type(c_ptr) :: xs, ys, bs, cs, ds
+  type(c_grid) :: rf
+  type(c_ptr) :: gsrfen, greff
+  real(c_double) :: sqrfac
 end type
 
 type, bind(C) :: c_gcmc
@@ -56,7 +65,8 @@ end type
 real(c_double), allocatable :: gstream(:), ustream(:)
 integer :: gpos = 0, upos = 0, ulen = 0
 integer(c_int32_t), parameter :: F_PAR=1, F_NUCL=2, F_NONBOND=4, F_NOBOND=8, F_PHIX=16, F_PHIV=32, &
-     F_SVDW=64, F_SOLV=128, F_DEBY=256, F_COUNTION=512, F_DNAFREE=1024
+     F_SVDW=64, F_SOLV=128, F_DEBY=256, F_COUNTION=512, F_DNAFREE=1024, &
+     F_RFPAR=33554432, F_DIFFUSE=268435456, F_PROFILE=536870912, F_RFPSIN=1073741824
 integer :: site_names_code(0:5)
 
 contains
@@ -154,6 +164,10 @@ Qwarn = .false.
 Qsphere = .false.
 Qecyl = .false.
 Qnotrans = .false.
+Qdiffuse = flag(s%flags, F_DIFFUSE)
+Qprofile = flag(s%flags, F_PROFILE)
+Qrfpar = flag(s%flags, F_RFPAR)
+Qrfpsin = flag(s%flags, F_RFPSIN)
 
 call c_f_pointer(s%x, xp, [ntot]); x(1:ntot) = xp
 call c_f_pointer(s%y, yp, [ntot]); y(1:ntot) = yp
@@ -258,6 +272,37 @@ if (allocated(warn)) deallocate(warn)
 allocate(warn(nttyp))
 warn = 0
 
+! position-dependent diffusion (DIFFUSION / PROFILE commands, simul1.f90:342)
+dt = s%dt
+kbtdt = s%kbtdt
+if (Qdiffuse .or. Qprofile) then
+  call c_f_pointer(s%diffusion, dp, [ntype+1]); diffusion(1:ntype) = dp(2:ntype+1)
+endif
+if (Qdiffuse) then
+  plength2 = s%plength2; pcenter = s%pcenter
+  call c_f_pointer(s%ampl3, dp, [ntype+1]); ampl3(1:ntype) = dp(2:ntype+1)
+  call c_f_pointer(s%p3, dp, [ntype+1]); p3(1:ntype) = dp(2:ntype+1)
+endif
+if (Qprofile) call set_profile(s)
+! reaction-field parameter grids (rfpar.f90:430-445)
+if (Qrfpar) then
+  nclx3 = s%rf%nclx; ncly3 = s%rf%ncly; nclz3 = s%rf%nclz
+  dcel3 = s%rf%dcel; idcel3 = 1.0/dcel3
+  tranx3 = 0.5*(nclx3-1)*dcel3; trany3 = 0.5*(ncly3-1)*dcel3; tranz3 = 0.5*(nclz3-1)*dcel3
+  xbcen3 = s%rf%xbcen; ybcen3 = s%rf%ybcen; zbcen3 = s%rf%zbcen
+  ncel3 = nclx3*ncly3*nclz3
+  if (Qrfpsin) then
+    n = ncel3
+  else
+    n = ncel3*(ntype - nold)
+  endif
+  if (allocated(gsrfen)) deallocate(gsrfen, greff)
+  allocate(gsrfen(n), greff(n))
+  call c_f_pointer(s%gsrfen, fp, [n]); gsrfen = fp
+  call c_f_pointer(s%greff, fp, [n]); greff = fp
+  sqrfac = s%sqrfac
+endif
+
 ! dynamics (simul1.f90:342-349 precomputed by the caller)
 bdmax = s%bdmax; fact0n1 = s%fact0n1; fact0n2 = s%fact0n2
 lx2m = s%lx2m; lx2p = s%lx2p; ly2m = s%ly2m; ly2p = s%ly2p; lz2m = s%lz2m; lz2p = s%lz2p
@@ -272,6 +317,38 @@ if (cntpts > 0) then
   call c_f_pointer(s%zcont, dp, [cntpts]); zcont(1:cntpts) = dp
 endif
 n = 0
+end subroutine
+
+subroutine set_profile(s)
+use ref_harness
+use splinemod, only: sp_b => b, sp_c => c, sp_d => d, xs, ys, nspline
+implicit none
+type(c_system), intent(in) :: s
+real(c_double), pointer :: dp(:)
+integer :: n
+n = s%nspline
+nspline = n
+call c_f_pointer(s%xs, dp, [n]); xs(1:n) = dp
+call c_f_pointer(s%ys, dp, [n]); ys(1:n) = dp
+call c_f_pointer(s%bs, dp, [n]); sp_b(1:n) = dp
+call c_f_pointer(s%cs, dp, [n]); sp_c(1:n) = dp
+call c_f_pointer(s%ds, dp, [n]); sp_d(1:n) = dp
+end subroutine
+
+! SPLINE (spline.f90:19-98) on caller arrays
+subroutine ref_spline(n, x, y, b, c, d) bind(C, name='ref_spline')
+use iso_c_binding
+implicit none
+integer(c_int32_t), value :: n
+real(c_double), intent(in) :: x(n), y(n)
+real(c_double), intent(out) :: b(n), c(n), d(n)
+real, allocatable :: xx(:), yy(:), bb(:), cc(:), dd(:)
+integer :: m
+m = n
+allocate(xx(m), yy(m), bb(m), cc(m), dd(m))
+xx = x; yy = y
+call spline(m, xx, yy, bb, cc, dd)
+b = bb; c = cc; d = dd
 end subroutine
 
 subroutine set_list2(lp, vp, n, which)
@@ -512,7 +589,16 @@ if (Qnucl) then
   enddo
 endif
 gpos = 0
-if (Qpar) call dynamics0(nsites+nfix+1, ntot)
+! simul1.f90:369-382
+if (Qpar) then
+  if (Qdiffuse) then
+    call dynamics1(nsites+nfix+1, ntot)
+  elseif (Qprofile) then
+    call dynamics2(nsites+nfix+1, ntot)
+  else
+    call dynamics0(nsites+nfix+1, ntot)
+  endif
+endif
 if (Qnucl) call dynamics0nuc(1, nsites)
 end subroutine
 

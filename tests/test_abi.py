@@ -62,7 +62,7 @@ def _plain():
 
 
 def test_create_rejects_unsupported_options():
-    for flag in (abi.QMEMB, abi.QMMIJ, abi.QBSPLINE, abi.QRFPAR, abi.QSPHERE, abi.QPROFILE):
+    for flag in (abi.QMEMB, abi.QMMIJ, abi.QBSPLINE, abi.QSPHERE, abi.QPROXDIFF, abi.QNOTRANS):
         s = _plain()
         s.flags |= flag
         rc, _ = _create(s)
@@ -82,6 +82,21 @@ def test_create_rejects_bad_indices():
     assert _create(s)[0] == abi.E_ARG
     s = _plain()
     s.capacity = s.ntot - 1
+    assert _create(s)[0] == abi.E_ARG
+
+
+def test_create_rejects_inconsistent_options():
+    s = _plain()
+    s.flags |= abi.QDIFFUSE | abi.QPROFILE        # simul1.f90:338-340
+    assert _create(s)[0] == abi.E_ARG
+    s = _plain()
+    s.flags |= abi.QPROFILE                       # no spline given
+    assert _create(s)[0] == abi.E_ARG
+    s = configs.with_profile_diffusion(_plain(), [0.0, 1.0, 2.0], [1.0, 0.5, 1.0])
+    s.spline = (np.array([1.0, 0.0, 1.0]),) + s.spline[1:]   # equal end knots
+    assert _create(s)[0] == abi.E_ARG
+    s = _plain()
+    s.flags |= abi.QRFPAR                         # no grids given
     assert _create(s)[0] == abi.E_ARG
 
 

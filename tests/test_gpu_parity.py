@@ -213,6 +213,101 @@ def test_steps_after_grand():
     assert err <= 1e-8, err
 
 
+# --------------------------------- position-dependent diffusion and reaction field
+def _steps_parity(s, nsteps=6, seed=1, tol_pos=1e-9):
+    noise = np.random.default_rng(seed).standard_normal((nsteps, 3 * s.ntot))
+    g, o = Engine(s), Oracle(s)
+    g.upload_noise(noise)
+    o.upload_noise(noise)
+    eg = g.step(nsteps, abi.NOISE_INJECTED)
+    eo = o.step(nsteps, abi.NOISE_INJECTED)
+    _check_energy(eg, eo)
+    err = float(np.max(np.abs(g.positions()[0] - o.positions()[0])))
+    print(f"{nsteps}-step position err {err:.3e}")
+    assert err <= tol_pos
+    assert force_err(g.forces(), o.forces()) <= TOL
+    for a, b in zip(g.flux(), o.flux()):
+        assert np.array_equal(a, b)
+    return g, o
+
+
+@pytest.mark.parametrize("which", ["plain", "pore"])
+def test_switch_diffusion_steps(which):
+    if which == "plain":
+        s = configs.with_switch_diffusion(plain(seed=41, n_per_type=80), porelength=20.0,
+                                          pcenter=3.0, lowfraction=(0.3, 0.6),
+                                          switchlength=(6.0, 9.0))
+    else:
+        s = configs.with_switch_diffusion(pore_small(seed=42), porelength=20.0,
+                                          lowfraction=0.4, switchlength=5.0)
+    _steps_parity(s)
+
+
+@pytest.mark.parametrize("desc", [False, True])
+def test_profile_diffusion_steps(desc):
+    zk, fk = configs.tanh_profile(-24.0, 24.0, 41, 8.0)
+    if desc:
+        zk, fk = zk[::-1].copy(), fk[::-1].copy()
+    s = configs.with_profile_diffusion(plain(seed=43, n_per_type=80), zk, fk)
+    _steps_parity(s)
+
+
+def test_profile_diffusion_philox_graphs():
+    zk, fk = configs.tanh_profile(-15.0, 15.0, 25, 6.0, low=0.3)
+    s = configs.with_profile_diffusion(pore_small(seed=44, n_bp=6, n_ions=150), zk, fk)
+    g, o = Engine(s), Oracle(s)
+    g.step(70, abi.NOISE_PHILOX, seed=99, counter0=5)
+    o.step(70, abi.NOISE_PHILOX, seed=99, counter0=5)
+    err = float(np.max(np.abs(g.positions()[0] - o.positions()[0])))
+    print(f"70 Philox steps with a D(z) profile: position err {err:.3e}")
+    assert err <= 1e-8
+
+
+def _rf_plain(seed, n, nfix=0, **kw):
+    s = with_fixed(plain(seed=seed, n_per_type=n), nfix)
+    return configs.with_rfpar(s, seed=seed, shape=(45, 45, 45), dcel=1.2, **kw)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(single=True), dict(wall_radius=14.0, zero_frac=0.1)])
+def test_rfpar_energy_forces(kw):
+    s = _rf_plain(51, 60, nfix=7, **kw)
+    g, o, eg, eo, fg, fo = _energy_forces(s)
+    print(f"erfpar {eg['erfpar']!r} vs {eo['erfpar']!r}")
+    assert eo["erfpar"] != 0.0
+    _check_energy(eg, eo)
+    assert force_err(fg, fo) <= TOL
+
+
+def test_rfpar_pore_dna_steps():
+    s = configs.with_rfpar(pore_small(seed=52), seed=3, shape=(37, 37, 55), dcel=1.0,
+                           wall_radius=12.0, zero_frac=0.05)
+    _steps_parity(s, nsteps=5)
+
+
+def test_rfpar_graph_replays():
+    s = _rf_plain(53, 40, nfix=3, zero_frac=0.05)
+    noise = np.random.default_rng(9).standard_normal((40, 3 * s.ntot))
+    g, o = Engine(s), Oracle(s)
+    g.upload_noise(noise)
+    o.upload_noise(noise)
+    g.step(40, abi.NOISE_INJECTED, 0)
+    o.step(40, abi.NOISE_INJECTED, 0)
+    err = float(np.max(np.abs(g.positions()[0] - o.positions()[0])))
+    assert err <= 1e-8, err
+
+
+def test_rfpar_interact():
+    s = _rf_plain(54, 40, nfix=4, wall_radius=16.0, zero_frac=0.1)
+    g, o = Engine(s), Oracle(s)
+    pos, typ = o.positions()
+    cases = [(pos[j], abs(typ[j]), j) for j in (0, 3, 5, 17, s.ntot - 1)]
+    cases += [((1.0, 2.0, -5.0), 1, s.ntot), ((4.0, -3.0, 9.0), 2, s.ntot),
+              ((29.0, 0.0, 0.0), 1, s.ntot)]
+    for xyz, jt, j in cases:
+        a, b = g.interact(xyz, jt, j, True), o.interact(xyz, jt, j, True)
+        assert abs(a - b) <= TOL * max(abs(b), 1.0), (j, a, b)
+
+
 def test_capacity_error():
     s = plain(seed=2, n_per_type=10)
     g = Engine(s)

@@ -298,3 +298,151 @@ def test_grand_acceptance_rule():
         if accepted:
             assert abs(ener - de) <= 1e-12 * max(abs(de), 1.0)
             assert np.array_equal(o.positions()[0][-1], [xn, yn, zn])
+
+
+# ------------------------------------------ diffusion profiles and reaction field
+def test_spline_reproduces_a_cubic():
+    """SPLINE's end conditions take the third derivative from the cubic through the four
+    end knots (spline.f90:58-66), so a cubic is reproduced exactly: b, c, d are p', p''/2
+    and p'''/6 at the knots."""
+    from oracle.oracle import oracle_spline
+    x = np.array([-20.0, -17.5, -13.0, -12.0, -6.5, -1.0, 0.5, 4.0, 9.0, 13.5, 17.0, 20.0])
+    p = np.polynomial.Polynomial([0.7, -0.02, 0.003, -0.00011])
+    b, c, d = oracle_spline(x, p(x))
+    dp, d2p, d3p = p.deriv(1), p.deriv(2), p.deriv(3)
+    assert np.allclose(b, dp(x), rtol=0, atol=1e-12)
+    assert np.allclose(c, 0.5 * d2p(x), rtol=0, atol=1e-12)
+    assert np.allclose(d, d3p(x) / 6.0, rtol=0, atol=1e-12)
+
+
+def _free_ions(seed, n):
+    """Ions without pair terms or grids: F = 0, so one BD step is the noise and drift."""
+    s = plain(seed, n).copy()
+    s.flags &= ~abi.QNONBOND
+    s.bdmax = 100.0
+    return s
+
+
+def _one_step(s, seed=4):
+    o = Oracle(s)
+    noise = np.random.default_rng(seed).standard_normal((1, 3 * s.ntot))
+    o.upload_noise(noise)
+    pos0, typ = o.positions()
+    o.step(1, abi.NOISE_INJECTED, 0)
+    return pos0, typ, noise[0].reshape(-1, 3), o.positions()[0]
+
+
+def test_switch_diffusion_closed_form():
+    """DYNAMICS1 with F = 0: D(z) = D0 (a + (1-a) S) with S the smoothstep 3t^2 - 2t^3 of
+    the depth t into either switch region (0 inside the pore, 1 outside both switches), and
+    z gains dD/dz dt."""
+    L, zc, a, w = 16.0, 2.0, 0.35, 7.0
+    s = configs.with_switch_diffusion(_free_ions(7, 150), porelength=L, pcenter=zc,
+                                      lowfraction=a, switchlength=w)
+    pos0, typ, g, got = _one_step(s)
+    lo, hi = zc - L / 2, zc + L / 2
+    seen = set()
+    for i in range(s.ntot):
+        z = pos0[i, 2]
+        if z > hi + w or z < lo - w:
+            S, dS = 1.0, 0.0
+            seen.add("out")
+        elif lo <= z <= hi:
+            S, dS = 0.0, 0.0
+            seen.add("pore")
+        else:
+            t = (z - hi) / w if z > hi else (lo - z) / w
+            S = t * t * (3.0 - 2.0 * t)
+            dS = 6.0 * t * (1.0 - t) / w * (1.0 if z > hi else -1.0)
+            seen.add("switch")
+        D0 = s.diffusion[abs(typ[i])]
+        D, dD = D0 * (a + (1.0 - a) * S), D0 * (1.0 - a) * dS
+        f2 = np.sqrt(2.0 * s.dt * D)
+        want = pos0[i] + f2 * g[i] + np.array([0.0, 0.0, dD * s.dt])
+        want = np.clip(want, np.array(s.box[0::2]), np.array(s.box[1::2]))
+        assert np.allclose(got[i], want, rtol=0, atol=1e-12), i
+    assert seen == {"out", "pore", "switch"}
+
+
+def test_profile_diffusion_closed_form():
+    """DYNAMICS2 with F = 0 and a cubic profile p (reproduced exactly by the spline):
+    D = D0 p(z) between the end knots, the end values outside.  The drift is D0 q(z) dt
+    with the reference's SEVALD, q = b + dx (c + dx d) (sevalmod.f90:61), which is not the
+    spline's slope b + dx (2c + 3 dx d): for a cubic, q = p1(x_k) + p2(x_k) dx/2 +
+    p3 dx^2/6 with p1, p2, p3 the derivatives of p, x_k the knot below z, dx = z - x_k."""
+    x = np.linspace(-22.0, 22.0, 15)
+    p = np.polynomial.Polynomial([0.6, 0.004, 0.0006, -0.00001])
+    from oracle.oracle import oracle_spline
+    s = configs.with_profile_diffusion(_free_ions(8, 150), x, p(x), oracle_spline)
+    pos0, typ, g, got = _one_step(s)
+    inside = 0
+    for i in range(s.ntot):
+        z = pos0[i, 2]
+        if z < x[0]:
+            f, df = p(x[0]), 0.0
+        elif z > x[-1]:
+            f, df = p(x[-1]), 0.0
+        else:
+            k = np.searchsorted(x, z, side="right") - 1
+            dx = z - x[k]
+            f = p(z)
+            df = (p.deriv(1)(x[k]) + 0.5 * p.deriv(2)(x[k]) * dx +
+                  p.deriv(3)(x[k]) / 6.0 * dx * dx)
+            inside += 1
+        D0 = s.diffusion[abs(typ[i])]
+        want = pos0[i] + np.sqrt(2.0 * s.dt * D0 * f) * g[i] + np.array([0, 0, D0 * df * s.dt])
+        want = np.clip(want, np.array(s.box[0::2]), np.array(s.box[1::2]))
+        assert np.allclose(got[i], want, rtol=0, atol=1e-11), i
+    assert 0 < inside < s.ntot
+
+
+def _rf_system(const_reff):
+    s = plain(9, 40).copy()
+    s.flags &= ~abi.QNONBOND
+    return configs.with_rfpar(s, seed=5, shape=(57, 57, 57), dcel=1.2, const_reff=const_reff)
+
+
+def _rf_cd(s, pick, h=1e-5):
+    """Largest |F + dE/dr| / max(|F|, 1) over the picked ions, E = INTERACT(j) (which holds
+    every reaction-field term of ion j), skipping components near a cell face where the
+    reference gates the trilinear gradient (rsmall)."""
+    o = Oracle(s)
+    o.energy()
+    f = o.forces()
+    pos, typ = o.positions()
+    g = s.rf
+    tran = np.array(g.tran()[1:]) if g.data.ndim == 4 else np.array(g.tran())
+    worst, n = 0.0, 0
+    for j in pick:
+        for k in range(3):
+            frac = ((pos[j, k] + tran[k]) / g.dcel) % 1.0
+            if min(frac, 1.0 - frac) < 1e-2:
+                continue
+            e = []
+            for sgn in (1.0, -1.0):
+                q = pos[j].copy()
+                q[k] += sgn * h
+                e.append(o.interact(q, abs(typ[j]), j))
+            cd = -(e[0] - e[1]) / (2 * h)
+            worst = max(worst, abs(cd - f[j, k]) / max(abs(f[j, k]), 1.0))
+            n += 1
+    return worst, n
+
+
+def test_rfpar_force_is_minus_gradient():
+    """With a constant greff (so reff has no gradient) the RFPARION force on each ion is
+    minus the gradient of its reaction-field energy: self term and pair terms through srfe
+    and the distance (rfpar.f90:146-223)."""
+    s = _rf_system(const_reff=3.0)
+    worst, n = _rf_cd(s, range(s.ntot))
+    assert n >= 100
+    assert worst <= 1e-6, worst
+
+
+def test_rfpar_reff_force_is_not_a_gradient():
+    """With a varying greff the reference's force uses -aux2*(r^2 + reff_i*reff_j/2)
+    (rfpar.f90:198) where the derivative of its energy has -aux2*r^2: the restatement keeps
+    that term, so the force differs from minus the gradient."""
+    s = _rf_system(const_reff=None)
+    worst, _ = _rf_cd(s, range(12))
+    assert worst > 1e-4, worst

@@ -118,3 +118,95 @@ def test_grand():
     assert o.ntot == r.ntot
     assert np.array_equal(o.positions()[0], r.positions()[0])
     assert np.array_equal(o.ibuffer(), r.ibuffer())
+
+
+# ------------------------------------------------ DYNAMICS1 / DYNAMICS2 / SPLINE / RFPAR
+from oracle.oracle import oracle_spline, ref_spline  # noqa: E402
+
+
+@pytest.mark.parametrize("n,desc", [(2, False), (3, False), (4, False), (11, False),
+                                    (60, False), (25, True)])
+def test_spline_bitexact(n, desc):
+    rng = np.random.default_rng(n)
+    x = np.cumsum(rng.uniform(0.3, 2.0, n)) - 10.0
+    if desc:
+        x = x[::-1].copy()
+    y = rng.uniform(0.2, 1.0, n)
+    b1, c1, d1 = oracle_spline(x, y)
+    b2, c2, d2 = ref_spline(x, y)
+    b3, c3, d3 = configs.lib_spline(x, y)
+    for a, b, c in ((b1, b2, b3), (c1, c2, c3), (d1, d2, d3)):
+        assert np.array_equal(a, b) and np.array_equal(c, b)
+
+
+def _steps_match(s, nsteps=5, seed=3):
+    noise = np.random.default_rng(seed).standard_normal((nsteps, 3 * s.ntot))
+    o, r = Oracle(s), RefFortran(s)
+    o.upload_noise(noise)
+    r.upload_noise(noise)
+    for k in range(nsteps):
+        o.step(1, abi.NOISE_INJECTED, k)
+        r.step(1, abi.NOISE_INJECTED, k)
+    po, pr = o.positions()[0], r.positions()[0]
+    assert float(np.max(np.abs(po - pr))) <= 1e-12
+    for a, b in zip(o.flux(), r.flux()):
+        assert np.array_equal(a, b)
+    return po
+
+
+@pytest.mark.parametrize("which", ["plain", "pore"])
+def test_dynamics1_bitexact(which):
+    if which == "plain":
+        s = configs.with_switch_diffusion(small_plain(11, 60), porelength=20.0, pcenter=3.0,
+                                          lowfraction=(0.3, 0.6), switchlength=(6.0, 9.0))
+    else:
+        s = configs.with_switch_diffusion(small_pore(seed=12), porelength=20.0,
+                                          lowfraction=0.4, switchlength=5.0)
+    _steps_match(s)
+
+
+@pytest.mark.parametrize("desc", [False, True])
+def test_dynamics2_bitexact(desc):
+    zk, fk = configs.tanh_profile(-24.0, 24.0, 33, 8.0)
+    if desc:
+        zk, fk = zk[::-1].copy(), fk[::-1].copy()
+    s = configs.with_profile_diffusion(small_plain(13, 60), zk, fk, oracle_spline)
+    _steps_match(s)
+
+
+def _rf_plain(seed, n, nfix=0, **kw):
+    s = with_fixed(small_plain(seed, n), nfix)
+    return configs.with_rfpar(s, seed=seed, shape=(45, 45, 45), dcel=1.2, **kw)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(single=True), dict(wall_radius=14.0),
+                                dict(zero_frac=0.15)])
+def test_rfpar_energy(kw):
+    s = _rf_plain(21, 50, nfix=6, **kw)
+    o, r = Oracle(s), RefFortran(s)
+    eo, er = o.energy(), r.energy()
+    assert er["erfpar"] != 0.0 and er["egsbpb"] == er["erfpar"]
+    assert max(energy_errs(eo, er, floor=1e-12).values()) <= 1e-14
+    assert force_err(o.forces(), r.forces()) <= 1e-14
+
+
+def test_rfpar_pore_dna_steps():
+    s = configs.with_rfpar(small_pore(seed=22), seed=3, shape=(37, 37, 55), dcel=1.0,
+                           wall_radius=12.0)
+    o, r = Oracle(s), RefFortran(s)
+    eo, er = o.energy(), r.energy()
+    assert max(energy_errs(eo, er, floor=1e-12).values()) <= 1e-14
+    assert force_err(o.forces(), r.forces()) <= 1e-14
+    _steps_match(s, nsteps=3)
+
+
+def test_rfpar_interact():
+    s = _rf_plain(23, 40, nfix=4, wall_radius=16.0, zero_frac=0.15)
+    o, r = Oracle(s), RefFortran(s)
+    pos, typ = o.positions()
+    cases = [(pos[j], abs(typ[j]), j) for j in (0, 3, 5, 17, s.ntot - 1)]
+    cases += [((1.0, 2.0, -5.0), 1, s.ntot), ((4.0, -3.0, 9.0), 2, s.ntot),
+              ((29.0, 0.0, 0.0), 1, s.ntot)]       # outside the grid box
+    for xyz, jt, j in cases:
+        a, b = o.interact(xyz, jt, j, True), r.interact(xyz, jt, j, True)
+        assert abs(a - b) <= 1e-14 * max(abs(b), 1.0), (j, a, b)
