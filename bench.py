@@ -114,7 +114,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
