@@ -1,8 +1,9 @@
 ! bromoc_gpu.f90 -- iso_c_binding front-end of libbromoc_gpu.so for the cg-gcmc-bd driver.
 !
-! Link this file's objects instead of energy.mono.o, dynamics0.mono.o, dynamics0nuc.o,
-! interact.o and grand.o: the externals at the end carry the reference routines' names
-! and argument lists (energy.mono.f90:19, dynamics0.mono.f90:19, dynamics0nuc.f90:19,
+! Link this file's objects instead of energy.mono.o, dynamics0.mono.o, dynamics1.mono.o,
+! dynamics2.mono.o, dynamics0nuc.o, interact.o and grand.o: the externals at the end carry
+! the reference routines' names and argument lists (energy.mono.f90:19,
+! dynamics0.mono.f90:19, dynamics1.mono.f90:19, dynamics2.mono.f90:19, dynamics0nuc.f90:19,
 ! interact.f90:19, grand.f90:19), so simul1.f90, metropolis.f90 and shell_simul.f90 call
 ! the GPU path unchanged.  The first call hands the module state (grandmod, efpmod,
 ! extramod, nucleotmod, gsbpmod) to bromoc_create; positions travel to the device before
@@ -43,6 +44,15 @@ type, bind(C) :: bgpu_system
   real(c_double) :: bdmax, fact0n1, fact0n2, lx2m, lx2p, ly2m, ly2p, lz2m, lz2p
   integer(c_int32_t) :: cntpts, reserved1
   type(c_ptr) :: zcont
+  real(c_double) :: dt, kbtdt
+  type(c_ptr) :: diffusion
+  real(c_double) :: plength2, pcenter
+  type(c_ptr) :: ampl3, p3
+  integer(c_int32_t) :: nspline, reserved2
+  type(c_ptr) :: xs, ys, bs, cs, ds
+  type(bgpu_grid) :: rf
+  type(c_ptr) :: gsrfen, greff
+  real(c_double) :: sqrfac
 end type
 
 type, bind(C) :: bgpu_energies
@@ -62,7 +72,7 @@ integer(c_int32_t), parameter :: F_PAR=1, F_NUCL=2, F_NONBOND=4, F_NOBOND=8, F_P
      F_MEMB=65536, F_MMIJ=131072, F_BSPLINE=262144, F_NMCDEN=524288, F_DEBYHYB=1048576, &
      F_PROXDIFF=2097152, F_NOTRANS=4194304, F_APFOR=8388608, F_CONTRANS=16777216, &
      F_RFPAR=33554432, F_SPHERE=67108864, F_ECYL=134217728, F_DIFFUSE=268435456, &
-     F_PROFILE=536870912
+     F_PROFILE=536870912, F_RFPSIN=1073741824
 integer(c_int32_t), parameter :: NOISE_PHILOX = 1
 
 interface
@@ -206,6 +216,7 @@ subroutine bromoc_gpu_init(seed)
   use nucleotmod
   use gsbpmod
   use apfmod
+  use splinemod, only: sp_b => b, sp_c => c, sp_d => d, sp_x => xs, sp_y => ys, nspline
   integer(c_int64_t), intent(in), optional :: seed
   type(bgpu_system) :: s
   type(bgpu_pair), allocatable, target :: pr(:)
@@ -215,7 +226,9 @@ subroutine bromoc_gpu_init(seed)
   real(c_double), allocatable, target :: xs(:), ys(:), zs(:), cgs(:), f1(:), f2(:), scs(:), zc(:)
   real(c_double), allocatable, target :: db(:), va(:), vd(:), ss(:), sb(:), se(:)
   real(c_double), allocatable, target :: epc(:,:)
-  real(c_float), allocatable, target :: gx(:)
+  real(c_float), allocatable, target :: gx(:), rfs(:), rfr(:)
+  real(c_double), allocatable, target :: dif(:), am3(:), pp3(:), spx(:), spy(:), spb(:), spc(:), spd(:)
+  integer :: nsp, ngrid
   integer(c_int8_t), allocatable, target :: gv(:)
   integer :: i, a, b, nk, nt1, ntc, nindex, cap
   integer(c_uint32_t) :: fl
@@ -243,6 +256,7 @@ subroutine bromoc_gpu_init(seed)
   if (Qapfor) fl = ior(fl, F_APFOR)
   if (Qnucl .and. Qcontrans) fl = ior(fl, F_CONTRANS)
   if (Qrfpar) fl = ior(fl, F_RFPAR)
+  if (Qrfpar .and. Qrfpsin) fl = ior(fl, F_RFPSIN)
   if (Qsphere) fl = ior(fl, F_SPHERE)
   if (Qecyl) fl = ior(fl, F_ECYL)
   if (Qdiffuse) fl = ior(fl, F_DIFFUSE)
@@ -326,7 +340,7 @@ subroutine bromoc_gpu_init(seed)
   if (Qphiv .and. Qsvdw) scs(2:nt1) = scal(1:nttyp)
   if (cntpts > 0) zc(1:cntpts) = zcont(1:cntpts)
 
-  s%abi_version = 1
+  s%abi_version = 2
   s%ntot = ntot; s%nsites = nsites; s%nfix = nfix; s%capacity = cap
   s%ntype = ntype; s%nold = nold; s%nttyp = nttyp; s%npair = ntc; s%flags = int(fl, c_int32_t)
   s%x = c_loc(xs); s%y = c_loc(ys); s%z = c_loc(zs); s%typei = c_loc(ityp)
@@ -359,6 +373,31 @@ subroutine bromoc_gpu_init(seed)
   s%bdmax = bdmax; s%fact0n1 = fact0n1; s%fact0n2 = fact0n2
   s%lx2m = lx2m; s%lx2p = lx2p; s%ly2m = ly2m; s%ly2p = ly2p; s%lz2m = lz2m; s%lz2p = lz2p
   s%cntpts = merge(cntpts, 0, Qcountion); s%zcont = c_loc(zc)
+  ! position-dependent diffusion (simul1.f90:342; DIFFUSION and PROFILE commands)
+  allocate(dif(ntype+1), am3(ntype+1), pp3(ntype+1))
+  dif(1) = 0.0; am3(1) = 0.0; pp3(1) = 1.0
+  dif(2:) = diffusion(1:ntype); am3(2:) = ampl3(1:ntype); pp3(2:) = p3(1:ntype)
+  s%dt = dt; s%kbtdt = kbtdt; s%diffusion = c_loc(dif)
+  s%plength2 = plength2; s%pcenter = pcenter; s%ampl3 = c_loc(am3); s%p3 = c_loc(pp3)
+  nsp = merge(nspline, 0, Qprofile)
+  allocate(spx(max(1,nsp)), spy(max(1,nsp)), spb(max(1,nsp)), spc(max(1,nsp)), spd(max(1,nsp)))
+  if (nsp > 0) then
+    spx = sp_x(1:nsp); spy = sp_y(1:nsp); spb = sp_b(1:nsp); spc = sp_c(1:nsp); spd = sp_d(1:nsp)
+  endif
+  s%nspline = nsp
+  s%xs = c_loc(spx); s%ys = c_loc(spy); s%bs = c_loc(spb); s%cs = c_loc(spc); s%ds = c_loc(spd)
+  ! reaction-field parameter grids (gsbpmod, rfpar.f90:430-445)
+  s%rf = bgpu_grid(0, 0, 0, 0, 0d0, 0d0, 0d0, 0d0, c_null_ptr)
+  if (Qrfpar) then
+    s%rf = bgpu_grid(nclx3, ncly3, nclz3, 0, dcel3, xbcen3, ybcen3, zbcen3, c_null_ptr)
+    ngrid = size(gsrfen)
+    allocate(rfs(ngrid), rfr(ngrid))
+    rfs = gsrfen; rfr = greff
+  else
+    allocate(rfs(1), rfr(1))
+    rfs = 0.0; rfr = 0.0
+  endif
+  s%gsrfen = c_loc(rfs); s%greff = c_loc(rfr); s%sqrfac = sqrfac
   call bgpu_check(bromoc_create(s, 0, gctx), 'bromoc_create')
 end subroutine
 
@@ -470,6 +509,20 @@ bd_counter = bd_counter + 1
 call bromoc_gpu_download()
 call bromoc_gpu_counters()
 sites_moved = .true.
+end subroutine
+
+! DYNAMICS1 (switch diffusion) and DYNAMICS2 (spline profile): the device pass picks the
+! ion update from the flags handed over at set-up, as simul1.f90:370-378 does.
+SUBROUTINE DYNAMICS1(ninit, nfinal)
+implicit none
+integer ninit, nfinal
+call DYNAMICS0(ninit, nfinal)
+end subroutine
+
+SUBROUTINE DYNAMICS2(ninit, nfinal)
+implicit none
+integer ninit, nfinal
+call DYNAMICS0(ninit, nfinal)
 end subroutine
 
 SUBROUTINE DYNAMICS0NUC(ninit, nfinal)

@@ -1,5 +1,5 @@
 """The Fortran front-end (bromoc_amd/fortran/bromoc_gpu.f90) linked into the reference's own
-routines in place of ENERGY, DYNAMICS0, DYNAMICS0NUC, INTERACT and GRAND, with its C-ABI
+routines in place of ENERGY, DYNAMICS0/1/2, DYNAMICS0NUC, INTERACT and GRAND, with its C-ABI
 calls served by the C oracle (oracle/shim_adapter.c), against the unmodified reference
 routines: shows that the front-end hands the driver's module state (flags, pair tables,
 knot ranges, DNA lists, grids, GCMC buffers) to the C ABI and writes the results back
@@ -101,3 +101,34 @@ def test_grand_through_front_end():
     assert f.ntot == r.ntot
     assert np.array_equal(f.positions()[0], r.positions()[0])
     assert np.array_equal(f.ibuffer(), r.ibuffer())
+
+
+@pytest.mark.parametrize("which", ["switch", "profile", "rfpar"])
+def test_diffusion_and_rfpar_through_front_end(which):
+    from oracle.oracle import oracle_spline
+    base = with_fixed(small_plain(31, 40), 5)
+    if which == "switch":
+        s = configs.with_switch_diffusion(base, porelength=20.0, pcenter=2.0,
+                                          lowfraction=(0.3, 0.6), switchlength=(6.0, 9.0))
+    elif which == "profile":
+        zk, fk = configs.tanh_profile(-24.0, 24.0, 33, 8.0)
+        s = configs.with_profile_diffusion(base, zk, fk, oracle_spline)
+    else:
+        s = configs.with_rfpar(base, seed=4, shape=(45, 45, 45), dcel=1.2, zero_frac=0.1)
+    seed, c0 = 77, 3
+    f, r = _pair(s)
+    f.L.bromoc_gpu_set_seed(seed, c0)
+    for k in range(3):
+        ef = f.energy()
+        f.dynamics(abi.NOISE_PHILOX)
+        er = r.energy()
+        r.dynamics(abi.NOISE_PHILOX, 0, seed, c0 + k)
+        assert max(energy_errs(ef, er, floor=1e-12).values()) <= 1e-14
+    pf, pr = f.positions()[0], r.positions()[0]
+    assert float(np.max(np.abs(pf - pr))) <= 1e-12
+    if which == "rfpar":
+        pos, typ = r.positions()
+        for j in (5, 9, s.ntot - 1):
+            a = f.interact(pos[j], abs(typ[j]), j, True)
+            b = r.interact(pos[j], abs(typ[j]), j, True)
+            assert abs(a - b) <= 1e-14 * max(abs(b), 1.0)
