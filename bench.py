@@ -85,6 +85,52 @@ def host_roundtrip(eng, nmov, nsteps=200):
             "pattern": "set_positions + energy + get_forces + dynamics + get_positions per step"}
 
 
+def batch_legs(steps=300, gcmc_every=10, ngcmc=20, small_n=200, small_steps=400):
+    """Replica batching: (1) 256 independent ion + pore systems (log-uniform 100..4000 ions,
+    own 129x129x257 static field at 0.75 A, shared repulsion map, GCMC bath buffers at both
+    ends) advanced together with GRAND every `gcmc_every` steps; (2) at small N, 256
+    replicas of `small_n` ions against the single-system path on one of them."""
+    import numpy as np
+    from bromoc_amd import abi, configs
+    from bromoc_amd.engine import Batch, Engine
+    from bromoc_amd.system import replica_system
+    t0 = time.perf_counter()
+    sh, reps = configs.replica_batch(seed=3)
+    gen_s = time.perf_counter() - t0
+    b = Batch(sh, reps)
+    b.run(20, seed=7, counter0=0, gcmc_every=gcmc_every, ngcmc=ngcmc)
+    nmov = sum(b.get(r)[2] - rep.nfix for r, rep in enumerate(reps))
+    t0 = time.perf_counter()
+    b.run(steps, seed=7, counter0=20, gcmc_every=gcmc_every, ngcmc=ngcmc, cycle0=2)
+    el = time.perf_counter() - t0
+    nmov_end = sum(b.get(r)[2] - rep.nfix for r, rep in enumerate(reps))
+    _, kms = b.time_steps(10, seed=7, counter0=10000)
+    full = {"value": 0.5 * (nmov + nmov_end) * steps / el, "unit": "particle-steps/s",
+            "replicas": len(reps), "ions": [int(nmov), int(nmov_end)],
+            "ion_range": [min(r.ntot for r in reps), max(r.ntot for r in reps)],
+            "steps": steps, "gcmc_every": gcmc_every, "ngcmc": ngcmc,
+            "ms_per_step": 1e3 * el / steps, "step_kernel_ms": kms,
+            "launch_shape_blocks": b.tuning(), "grid": list(sh.phix.data.shape),
+            "setup_s": gen_s}
+    b.close()
+    del b
+    sh2, reps2 = configs.replica_batch(seed=4, n_range=(small_n, small_n), gcmc=False)
+    e = Engine(replica_system(sh2, reps2[0]))
+    e.step(50, abi.NOISE_PHILOX, seed=1, counter0=0, energies=False)
+    t0 = time.perf_counter()
+    e.step(small_steps, abi.NOISE_PHILOX, seed=1, counter0=50, energies=False)
+    plain = reps2[0].ntot * small_steps / (time.perf_counter() - t0)
+    e.close()
+    b2 = Batch(sh2, reps2)
+    b2.run(20, seed=1)
+    t0 = time.perf_counter()
+    b2.run(small_steps, seed=1, counter0=20)
+    batched = sum(r.ntot for r in reps2) * small_steps / (time.perf_counter() - t0)
+    small = {"n": small_n, "replicas": len(reps2), "plain": plain, "batched": batched,
+             "ratio": batched / plain, "unit": "particle-steps/s", "steps": small_steps}
+    return full, small
+
+
 def copy_probe(mib):
     """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
     import torch
@@ -106,6 +152,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true", help="skip the start-state parity check")
     ap.add_argument("--no-host", action="store_true", help="skip the host round-trip rate")
+    ap.add_argument("--no-batch", action="store_true", help="skip the replica-batch legs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -166,6 +213,8 @@ def main():
     pairs = (s.ntot - s.nsites - s.nfix) * (s.ntot - 1) + s.nsites * nions
     gflops = 34.0 * pairs / (kernel_ms * 1e-3) / 1e9
     host = None if args.no_host else host_roundtrip(eng, nmov)
+    eng.close()
+    batched, small_n = (None, None) if args.no_batch else batch_legs()
     cpu = None if args.no_cpu else cpu_baseline(s)
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
@@ -178,7 +227,8 @@ def main():
                      "fp64_valu": {"achieved_tflops": gflops / 1e3, "peak_tflops": 78.6,
                                    "frac": gflops / 1e3 / 78.6, "pair_evals_per_step": pairs},
                      "targets_per_block": tpb},
-        "cpu_baseline": cpu, "host_roundtrip": host, "parity_vs_oracle": parity,
+        "cpu_baseline": cpu, "host_roundtrip": host, "batched": batched,
+        "batched_vs_plain_small_n": small_n, "parity_vs_oracle": parity,
         "setup_s": build_s,
     }
     print(json.dumps(out))

@@ -105,6 +105,13 @@ class GcmcStruct(C.Structure):
                 ("cy", C.c_double), ("cz", C.c_double), ("ikbt", C.c_double)]
 
 
+class ReplicaStruct(C.Structure):
+    _fields_ = [("ntot", C.c_int32), ("nfix", C.c_int32), ("capacity", C.c_int32),
+                ("reserved", C.c_int32), ("x", _dp), ("y", _dp), ("z", _dp), ("typei", _ip),
+                ("phix", C.POINTER(C.c_float)), ("gcmc", C.POINTER(GcmcStruct)),
+                ("ibuffer", _ip), ("nat", _ip)]
+
+
 class BromocError(RuntimeError):
     def __init__(self, where, code, detail=""):
         self.code = code
@@ -164,6 +171,19 @@ def _declare(lib):
         "bromoc_get_ibuffer": ([vp, _ip], C.c_int),
         "bromoc_philox4x64": ([C.c_int32, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)], C.c_int),
         "bromoc_spline": ([C.c_int32, _dp, _dp, _dp, _dp, _dp], C.c_int),
+        "bromoc_batch_create": ([P(SystemStruct), C.c_int32, P(ReplicaStruct), C.c_int, P(vp)],
+                                C.c_int),
+        "bromoc_batch_destroy": ([vp], C.c_int),
+        "bromoc_batch_last_error": ([vp], C.c_char_p),
+        "bromoc_batch_run": ([vp, C.c_int32, C.c_uint64, C.c_uint64, C.c_int32, C.c_int32,
+                              C.c_uint64, P(Energies)], C.c_int),
+        "bromoc_batch_energy": ([vp, P(Energies)], C.c_int),
+        "bromoc_batch_get": ([vp, C.c_int32, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _dp], C.c_int),
+        "bromoc_batch_get_forces": ([vp, C.c_int32, _dp, _dp, _dp], C.c_int),
+        "bromoc_batch_get_flux": ([vp, C.c_int32, _ip, _ip], C.c_int),
+        "bromoc_batch_get_prob": ([vp, C.c_int32, _ip], C.c_int),
+        "bromoc_batch_time_steps": ([vp, C.c_int32, C.c_uint64, C.c_uint64, _dp, _dp], C.c_int),
+        "bromoc_batch_get_tuning": ([vp, _ip], C.c_int),
     }
     for name, (args, res) in sig.items():
         if hasattr(lib, name):

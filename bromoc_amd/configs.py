@@ -572,3 +572,58 @@ def with_rfpar(s: System, seed=0, shape=None, dcel=None, single=False, reffac=1.
     if single:
         s.flags |= abi.QRFPSIN
     return s
+
+
+# ------------------------------------------------------------------- replica batches
+def replica_batch(seed=0, nrep=256, n_range=(100, 4000), box=(96.0, 96.0, 192.0), dcel=0.75,
+                  rpore=12.0, half_slab=20.0, conc_range=(0.1, 2.0), volt_range=(-0.5, 0.5),
+                  gcmc=True, buffer_len=10.0, mu=(-0.3, -0.3), cap_factor=1.5,
+                  table_builder=None, noise_volt=0.01):
+    """Independent ion + pore systems for a replica batch: a shared System (pair tables of
+    K+/Cl-, the pore repulsion map, box, flux planes, BD constants) and `nrep` Replicas with
+    a log-uniform ion count in n_range, their own static field (a linear drop of a uniform
+    voltage in volt_range across the slab plus a smooth perturbation) and, with gcmc, bath
+    buffers of buffer_len at both box ends for a uniform concentration in conc_range."""
+    from .system import Replica, replica_system
+    rng = np.random.default_rng(seed)
+    shared = pore_dna(seed=seed, n_ions=0, box=box, dcel=dcel, rpore=rpore,
+                      half_slab=half_slab, dna=False, table_builder=table_builder, volt=0.0)
+    shared.capacity = 1
+    shape = shared.phix.data.shape
+    nx, ny, nz = shape
+    x = (np.arange(nx) - 0.5 * (nx - 1)) * dcel
+    y = (np.arange(ny) - 0.5 * (ny - 1)) * dcel
+    z = (np.arange(nz) - 0.5 * (nz - 1)) * dcel
+    drop = np.clip((half_slab - z) / (2.0 * half_slab), 0.0, 1.0)      # 1 V profile
+    to_field = (COULOMB / KCALMOL) / CELEC
+    hx, hy, hz = 0.5 * box[0], 0.5 * box[1], 0.5 * box[2]
+
+    def accessible(p):
+        return not (abs(p[2]) <= half_slab + 1.0 and p[0] ** 2 + p[1] ** 2 > (rpore - 1.5) ** 2)
+
+    lo = np.array([-hx + 0.5, -hy + 0.5, -hz + 0.5])
+    reps = []
+    for r in range(nrep):
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        n += n % 2
+        volt = rng.uniform(*volt_range)
+        conc = rng.uniform(*conc_range)
+        a = rng.uniform(0.1, 0.3, 3)
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        pert = (np.sin(a[0] * x + ph[0])[:, None, None] * np.sin(a[1] * y + ph[1])[None, :, None] *
+                np.sin(a[2] * z + ph[2])[None, None, :])
+        field = ((volt * drop)[None, None, :] + noise_volt * pert) * to_field
+        pos = _place(rng, n, lo, -lo, 2.0, accept=accessible)
+        typei = np.array([shared.nold + 1 + (k % 2) for k in range(n)], np.int32)
+        order = rng.permutation(n)
+        pos, typei = pos[order], typei[order]
+        rep = Replica(x=pos[:, 0].copy(), y=pos[:, 1].copy(), z=pos[:, 2].copy(), typei=typei,
+                      nfix=0, capacity=int(cap_factor * n) + 64,
+                      phix=field.astype(np.float32), meta=dict(volt=volt, conc=conc))
+        if gcmc:
+            rs = replica_system(shared, rep)
+            g = gcmc_buffers(rs, conc=conc, buffer_len=buffer_len, mu=mu, volt=volt)
+            ib, nat = assign_buffers(rs, g)
+            rep.gcmc, rep.ibuffer, rep.nat = g, ib[:n].copy(), nat
+        reps.append(rep)
+    return shared, reps

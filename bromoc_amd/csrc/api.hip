@@ -341,6 +341,14 @@ int reduce_energies(bromoc_ctx *c, bromoc_energies *out) {
 
 }  // namespace
 
+int bromoc::shared_context(const bromoc_system *s, int device, bromoc_ctx **out,
+                           StepParams *base) {
+    int st = bromoc_create(s, device, out);
+    if (st) return st;
+    *base = (*out)->base;
+    return BROMOC_OK;
+}
+
 extern "C" {
 
 const char *bromoc_last_error(const bromoc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }

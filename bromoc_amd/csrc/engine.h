@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/bromoc_gpu.h"
+
 namespace bromoc {
 
 // Number of per-particle energy slots: the bromoc_energies fields after `ener`.
@@ -156,5 +158,48 @@ struct InteractParams {
     double *out;                // [1]
 };
 hipError_t launch_interact(const InteractParams &p, hipStream_t s);
+
+// ---- replica batches (batch.hip)
+struct BatchParams {
+    StepParams sp;              // shared tables, grid geometry, BD constants
+    int32_t nrep, persistent;   // replicas; 1: waves take items from queue[0]
+    int32_t mode, write_energy; // mode 1: BD step, 0: ENERGY only
+    int32_t write_forces, ngcmc;
+    const int32_t *order;       // [nrep] replicas by decreasing capacity
+    const int32_t *off, *cap, *nfix;   // [nrep] first slot, slots, leading fixed ions
+    int32_t *ntot;              // [nrep] particles now (GRAND changes it)
+    int32_t *wstart;            // [nrep+1] work-item prefix over `order` (k_batch_prep)
+    int32_t *queue;             // [2] work counter, error flag
+    const float *const *phix;   // [nrep] static field of each replica
+    const double4 *pos;         // [slots] (x, y, z, atnam2 type in the low word)
+    double4 *poso;              // the other buffer (BD step output)
+    double4 *posw;              // the current buffer, written by GRAND
+    int32_t *typei, *ibuffer;   // [slots]
+    double *fx, *fy, *fz;       // [slots]
+    double *epart;              // [NE][ecap]
+    int64_t ecap;
+    int32_t *nfwd, *nbwd, *warn;   // per replica, strides below
+    int32_t flux_stride, warn_stride;
+    int32_t gstride, pad;       // per-replica stride of the buffer arrays (nbuffer max + 1)
+    const int32_t *nbuf;        // [nrep] buffers of each replica (0: no GCMC)
+    const int32_t *ibfftyp, *ntotat;              // [nrep][gstride]
+    const double *avnum, *mu, *kb, *lzmin, *lzmax;   // [nrep][gstride]
+    const int8_t *bias;                            // [nrep][gstride]
+    const double *gbox;         // [nrep][6] lx, ly, cx, cy, cz, ikbt
+    int32_t *nat;               // [nrep][gstride]
+    int32_t *prob;              // per replica [nbuffer][cap+1] at probofs[r]
+    const int64_t *probofs;
+    double *gener;              // [nrep] GRAND's energy accumulator
+    int64_t *ctl;               // [8]: steps, counter0, cycles, seed, step counter, cycle,
+                                //      cycle0, spare
+};
+hipError_t launch_batch_prep(const BatchParams &b, int gcmc, hipStream_t s);
+hipError_t launch_batch_step(const BatchParams &b, int blocks, hipStream_t s);
+hipError_t launch_batch_gcmc(const BatchParams &b, hipStream_t s);
+hipError_t launch_batch_reduce(const BatchParams &b, double *out, hipStream_t s);
+
+// the shared part of a batch: a single-system context over the shared bromoc_system
+// (no particles), whose device tables the batch kernels read (api.hip)
+int shared_context(const bromoc_system *s, int device, bromoc_ctx **out, StepParams *base);
 
 }  // namespace bromoc

@@ -1,614 +1,17 @@
-// kernels.hip -- gfx950 kernels of the BROMOC force / Brownian-dynamics step.
+// kernels.hip -- gfx950 kernels of the BROMOC force / Brownian-dynamics step of one system.
 //
-// k_step: one wavefront per target particle (every DNA site and every moving ion).
-//   The block's four wavefronts stream the whole particle set through LDS in tiles of 256;
-//   each lane evaluates its share of the tile against the wavefront's target (all-pairs, as
-//   the reference's ENERGY has no cutoff: energy.mono.f90:117-194,584-687), then walks the
-//   target's DNA-term incidence list (energy.mono.f90:196-508) and, on lanes 0..7, the
-//   8 trilinear corners of the static-field and repulsion grids (staticf1.f90:45-118,
-//   vdwgd1trln.f90:50-161).  A butterfly reduction gives the total force on lane 0, which
-//   applies the Ermak-McCammon update of DYNAMICS0 / DYNAMICS0NUC (dynamics0.mono.f90:31-46,
-//   dynamics0nuc.f90:31-44) into the other position buffer, so one launch is one BD step.
-//   Every force is a gather over partners in a fixed order: results are deterministic and
-//   need no atomics; energies are per-particle partials reduced in a fixed tree.
-#include <math.h>
-
-#include "engine.h"
+// One BD step is k_rfpar/k_rfpair (reaction-field parameters, when present), k_dna (CG-DNA
+// list terms of each site, energy.mono.f90:196-508) and k_step (or k_wave): every target
+// particle -- each free DNA site and each moving ion -- gathers its pair forces over all
+// other particles (the reference's ENERGY has no cutoff: energy.mono.f90:117-194,584-687),
+// its 8 trilinear corners of the static-field and repulsion grids (staticf1.f90:45-118,
+// vdwgd1trln.f90:50-161), then applies the Ermak-McCammon update (DYNAMICS0/1/2,
+// DYNAMICS0NUC) into the other position buffer.  Every force is a gather over partners in
+// a fixed order: results are deterministic and need no atomics; energies are per-particle
+// partials reduced in a fixed tree.
+#include "device.h"
 
 namespace bromoc {
-
-constexpr int WAVE = 64;
-constexpr int BLOCK = 256;
-constexpr int WPB = BLOCK / WAVE;
-
-// feature bits of the k_step specialisations
-constexpr int F_DNA = 1;     // CG-DNA sites and lists present
-constexpr int F_PHIX = 2;    // static field grid
-constexpr int F_PHIV = 4;    // repulsion grid
-constexpr int F_UPDATE = 8;  // apply the BD update (else ENERGY only)
-
-// host-side flag values (include/bromoc_gpu.h)
-constexpr uint32_t QPAR = 1u << 0, QNUCL = 1u << 1, QNONBOND = 1u << 2, QNOBOND = 1u << 3;
-constexpr uint32_t QPHIX = 1u << 4, QPHIV = 1u << 5, QSVDW = 1u << 6, QSOLV = 1u << 7;
-constexpr uint32_t QDEBY = 1u << 8, QCOUNTION = 1u << 9, QDNAFREE = 1u << 10;
-
-constexpr double PI = 3.14159265358979323846264338327950288419716939937510;
-constexpr double RSMALL = 1E-3;
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-    return v;
-}
-
-// integer powers in the order the Fortran compiler expands them (binary powering)
-__device__ __forceinline__ double pow3(double a) { double a2 = a * a; return a * a2; }
-__device__ __forceinline__ double pow5(double a) { double a2 = a * a; double a4 = a2 * a2; return a * a4; }
-__device__ __forceinline__ double pow6(double a) { double a2 = a * a; double a4 = a2 * a2; return a2 * a4; }
-__device__ __forceinline__ double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
-
-// getyd, polynom.f90:197-230: energy y and force factor yp (= -dU/dr / r)
-__device__ __forceinline__ void getyd(const DevPair &p, const double4 *__restrict__ ep,
-                                      double ires, double x2, double &y, double &yp) {
-    if (x2 < p.dm2lo) {
-        double ix2 = 1.0 / x2;
-        double c = p.sc1 * pow6(ix2);
-        y = c + p.sc2;
-        yp = 12.0 * c * ix2;
-    } else if (x2 > p.dm2hi) {
-        double ix2 = 1.0 / x2;
-        double c = p.sc3 * pow3(ix2);
-        if (p.flags & PF_CHR) {
-            double f = p.fct * sqrt(ix2);
-            y = c + f;
-            yp = (6.0 * c + f) * ix2;
-        } else {
-            y = c;
-            yp = 6.0 * c * ix2;
-        }
-    } else {
-        double x = sqrt(x2);
-        int i = (int)((x - p.dmi) * ires) + p.nxi;
-        double4 e = ep[i];
-        y = e.x + x * (e.y + x * e.z);
-        yp = -e.y / x - 2.0 * e.z;
-    }
-}
-
-// analytic pair branch, energy.mono.f90:135-178
-__device__ __forceinline__ double pair_analytic(const DevPair &p, const StepParams &P, double dist2,
-                                                double &eel, double &evd, double &esr) {
-    double idist2 = 1.0 / dist2, idist = 0.0, cofo = 0.0, dist6 = 0.0, dist12 = 0.0, dist = 0.0;
-    double e1 = 0.0, e2 = 0.0, e3 = 0.0, fdf = 0.0, fdv = 0.0;
-    const bool srp = (p.flags & PF_SRPMF) && dist2 <= P.rth;
-    if (p.flags & (PF_CHR | PF_SRPMF)) idist = sqrt(idist2);
-    if (p.flags & PF_CHR) {
-        cofo = p.fct * idist;
-        eel += cofo;
-    }
-    if (p.flags & PF_LJ) {
-        dist6 = pow3(p.sgp2 * idist2);
-        dist12 = dist6 * dist6;
-        evd += p.epp4 * (dist12 - dist6);
-    }
-    if (srp) {
-        dist = 1.0 / idist;
-        e1 = exp((p.c1 - dist) * p.c2);
-        e2 = cos(p.c3 * PI * (p.c1 - dist));
-        e3 = pow6(p.c1 * idist);
-        double e0 = p.c0 * e1 * e2 + p.c4 * e3;
-        if (dist >= P.srpx) {
-            fdf = exp(-P.srpk * (dist - P.srpx)) - P.srpy;
-            fdv = e0;
-            e0 = fdv * fdf;
-        }
-        esr += e0;
-    }
-    double de = 0.0;
-    if (p.flags & PF_CHR) de = cofo * idist2;
-    if (p.flags & PF_LJ) de = de + p.epp4 * (2.0 * dist12 - dist6) * 6.0 * idist2;
-    if (srp) {
-        double dc = (-p.c2 * e2 + p.c3 * PI * sin(p.c3 * PI * (p.c1 - dist))) * p.c0 * e1 -
-                    6.0 * p.c4 * e3 * idist;
-        if (dist >= P.srpx) dc = dc * fdf - fdv * P.srpk * (fdf + P.srpy);
-        de = de - dc * idist;
-    }
-    return de;
-}
-
-// Philox4x64-10 (Salmon et al. 2011) and Box-Muller on 53-bit uniforms
-__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        const uint64_t M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
-        uint64_t hi0 = __umul64hi(M0, c[0]), lo0 = M0 * c[0];
-        uint64_t hi1 = __umul64hi(M1, c[2]), lo1 = M1 * c[2];
-        uint64_t n0 = hi1 ^ c[1] ^ k0, n1 = lo1, n2 = hi0 ^ c[3] ^ k1, n3 = lo0;
-        c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
-        k0 += 0x9E3779B97F4A7C15ull;
-        k1 += 0xBB67AE8584CAA73Bull;
-    }
-}
-
-__device__ __forceinline__ void philox_gauss3(uint64_t seed, uint64_t step, uint64_t stream,
-                                              int64_t i, double g[3]) {
-    uint64_t c[4] = {step, (uint64_t)i, stream, 0};
-    philox4x64(c, seed, 0);
-    const double inv53 = 1.0 / 9007199254740992.0;
-    double u1 = (double)((c[0] >> 11) + 1) * inv53;
-    double u2 = (double)(c[1] >> 11) * inv53;
-    double u3 = (double)((c[2] >> 11) + 1) * inv53;
-    double u4 = (double)(c[3] >> 11) * inv53;
-    double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
-    const double twopi = 2.0 * PI;
-    g[0] = r1 * cos(twopi * u2);
-    g[1] = r1 * sin(twopi * u2);
-    g[2] = r2 * cos(twopi * u4);
-}
-
-__device__ __forceinline__ void fixcoor(const StepParams &P, double &x, double &y, double &z) {
-    if (x < P.lx2m) x = P.lx2m;
-    else if (x > P.lx2p) x = P.lx2p;
-    if (y < P.ly2m) y = P.ly2m;
-    else if (y > P.ly2p) y = P.ly2p;
-    if (z < P.lz2m) z = P.lz2m;
-    else if (z > P.lz2p) z = P.lz2p;
-}
-
-// One trilinear corner (lane c = 0..7) of staticf1/vdwgd1trln.  Adds the corner's force
-// and energy to the lane's accumulators; returns the corner's grid value.
-template <typename T>
-__device__ __forceinline__ double tri_corner(const GridGeom &g, const T *__restrict__ data,
-                                             double xi, double yi, double zi, int ix, int iy,
-                                             int iz, int c, double scale, double &fx, double &fy,
-                                             double &fz, double &e) {
-    const double one = 1.0;
-    int n1 = ix + (c >> 2), n2 = iy + ((c >> 1) & 1), n3 = iz + (c & 1);
-    double ai = xi - n1 * g.dcel;
-    double aisign = fsign(one, ai);
-    ai = one - fabs(ai) * g.idcel;
-    double bi = yi - n2 * g.dcel;
-    double bisign = fsign(one, bi);
-    bi = one - fabs(bi) * g.idcel;
-    double ci = zi - n3 * g.dcel;
-    double cisign = fsign(one, ci);
-    ci = one - fabs(ci) * g.idcel;
-    double fi = ai * bi * ci;
-    size_t in3 = ((size_t)n1 * g.ny + n2) * g.nz + n3;
-    double ph = (double)data[in3];
-    double prefac = ph * scale * g.idcel;
-    if (ai < (1.0 - RSMALL) && ai > RSMALL) fx += aisign * bi * ci * prefac;
-    if (bi < (1.0 - RSMALL) && bi > RSMALL) fy += bisign * ai * ci * prefac;
-    if (ci < (1.0 - RSMALL) && ci > RSMALL) fz += cisign * ai * bi * prefac;
-    e += fi * scale * ph;
-    return ph;
-}
-
-__device__ __forceinline__ void grid_cell(const GridGeom &g, double x, double y, double z,
-                                          double &xi, double &yi, double &zi, int &ix, int &iy,
-                                          int &iz) {
-    xi = x + g.tranx - g.xbcen;
-    yi = y + g.trany - g.ybcen;
-    zi = z + g.tranz - g.zbcen;
-    ix = (int)(xi * g.idcel);
-    iy = (int)(yi * g.idcel);
-    iz = (int)(zi * g.idcel);
-    if (ix == g.nx - 1) ix = g.nx - 2;
-    if (iy == g.ny - 1) iy = g.ny - 2;
-    if (iz == g.nz - 1) iz = g.nz - 2;
-}
-
-// Diffusion profile of DYNAMICS2 (dynamics2.mono.f90:52-74): the end values outside the
-// knots, else the cubic spline and its derivative (seval/sevald, sevalmod.f90:23-67).  The
-// interval is found by the same bisection; seval's SAVEd interval is not kept (it only
-// changes the chosen interval when z equals a knot exactly).
-__device__ __forceinline__ void diff_profile(const StepParams &P, double zz, double &sw,
-                                             double &dsw) {
-    const int n = P.nspline;
-    const double *xs = P.xs;
-    const bool up = xs[0] < xs[n - 1];
-    if (up ? zz < xs[0] : zz > xs[0]) { sw = P.ys[0]; dsw = 0.0; return; }
-    if (up ? zz > xs[n - 1] : zz < xs[n - 1]) { sw = P.ys[n - 1]; dsw = 0.0; return; }
-    int i = 1, j = n + 1;   // 1-based, as the reference's bisection
-    for (int it = 0; it < 64 && j > i + 1; it++) {
-        const int k = (i + j) / 2;
-        if (zz < xs[k - 1]) j = k;
-        if (zz >= xs[k - 1]) i = k;
-    }
-    const double dx = zz - xs[i - 1];
-    sw = P.ys[i - 1] + dx * (P.bs[i - 1] + dx * (P.cs[i - 1] + dx * P.ds[i - 1]));
-    dsw = P.bs[i - 1] + dx * (P.cs[i - 1] + dx * P.ds[i - 1]);
-}
-
-// Drift and noise factors of an ion of type itype at height z: fact1a/fact2a for DYNAMICS0;
-// for DYNAMICS1 the pore switch (dynamics1.mono.f90:38-70) and for DYNAMICS2 the profile
-// scale D, with the drift dD/dz * dt on z.
-__device__ __forceinline__ void ion_factors(const StepParams &P, int itype, double z,
-                                            double &fac1, double &fac2, double &ddz) {
-    if (P.dmode == 0) {
-        fac1 = P.fact1a[itype];
-        fac2 = P.fact2a[itype];
-        ddz = 0.0;
-        return;
-    }
-    const double D = P.diffusion[itype];
-    double idf, didf;
-    if (P.dmode == 1) {
-        const double pp3 = P.p3[itype];
-        const double pore1 = -P.plength2 + P.pcenter, pore2 = P.plength2 + P.pcenter;
-        double sw = 1.0, dsw = 0.0;
-        if (z > pore1 - pp3 && z < pore2 + pp3) {
-            sw = 0.0;
-            if (!(z >= pore1 && z <= pore2)) {
-                const double ipp3 = 1.0 / pp3;
-                if (z > pore2) {
-                    const double t = (z - pore2) * ipp3;
-                    sw = 2.0 * (t * t * t) - 3.0 * (t * t);
-                    dsw = 6.0 * (t * t - t) * ipp3;
-                } else {
-                    const double t = (z - pore1) * ipp3;
-                    sw = -2.0 * (t * t * t) - 3.0 * (t * t);
-                    dsw = -6.0 * (t * t + t) * ipp3;
-                }
-                sw = -sw;
-                dsw = -dsw;
-            }
-        }
-        const double a = P.ampl3[itype];
-        idf = D * (a + (1.0 - a) * sw);
-        didf = D * (1.0 - a) * dsw;
-    } else {
-        double sw, dsw;
-        diff_profile(P, z, sw, dsw);
-        idf = D * sw;
-        didf = D * dsw;
-    }
-    fac1 = idf * P.kbtdt;
-    fac2 = sqrt(2.0 * P.dt * idf);
-    ddz = didf * P.dt;
-}
-
-// Ermak-McCammon update of moving particle i (position pa, force F, Gaussians g):
-// DYNAMICS0/1/2 for ions (dynamics0.mono.f90:31-46, dynamics1.mono.f90:71-86,
-// dynamics2.mono.f90:75-85) with countions, DYNAMICS0NUC (dynamics0nuc.f90:31-44) for
-// sites; fixcoor unless the site is free of the box (dnafree).
-__device__ __forceinline__ double4 bd_update(const StepParams &P, int i, bool is_site,
-                                             const double4 &pa, const double F[3],
-                                             const double *g) {
-    double fac1, fac2, ddz = 0.0;
-    int itype = 0;
-    if (is_site) {
-        fac1 = P.fact0n1;
-        fac2 = P.fact0n2;
-    } else {
-        itype = abs(P.typei[i]);
-        ion_factors(P, itype, pa.z, fac1, fac2, ddz);
-    }
-    const double bdmax = P.bdmax;
-    double delx = F[0] * fac1, dely = F[1] * fac1, delz = F[2] * fac1;
-    if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
-    if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
-    if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
-    double xn = pa.x + delx + fac2 * g[0];
-    double yn = pa.y + dely + fac2 * g[1];
-    double zn = pa.z + delz + fac2 * g[2];
-    if (!is_site && P.dmode != 0) zn = zn + ddz;
-    if (!is_site && (P.flags & QCOUNTION)) {
-        const int row = itype - P.nold - 1;
-        for (int c = 0; c < P.cntpts; c++) {
-            const double zc = P.zcont[c];
-            if (pa.z < zc && zn >= zc) atomicAdd(&P.nfwd[row * P.cntpts + c], 1);
-            else if (pa.z >= zc && zn < zc) atomicAdd(&P.nbwd[row * P.cntpts + c], 1);
-        }
-    }
-    if (!is_site || !(P.flags & QDNAFREE)) fixcoor(P, xn, yn, zn);
-    return make_double4(xn, yn, zn, pa.w);
-}
-
-// Reaction-field parameters of an ion at (x, y, z) of atnam type itype: srfe and reff
-// from the 8 corners of gsrfen / sqrfac*greff (rfpar.f90:75-142), with their gradients
-// when grad is set.  The corner loop leaves the innermost (z) loop at a zero gsrfen corner,
-// as the reference's `exit` does (rfpar.f90:121).  Returns false outside the grid box.
-__device__ bool rf_params(const StepParams &P, double x, double y, double z, int itype,
-                          bool grad, double &srfe, double &reff, double ds[3], double dr[3]) {
-    const GridGeom &g = P.gr;
-    srfe = 0.0; reff = 0.0;
-    ds[0] = ds[1] = ds[2] = dr[0] = dr[1] = dr[2] = 0.0;
-    if (!(x <= g.xbcen + g.tranx && x >= g.xbcen - g.tranx && y <= g.ybcen + g.trany &&
-          y >= g.ybcen - g.trany && z <= g.zbcen + g.tranz && z >= g.zbcen - g.tranz))
-        return false;
-    const size_t ncyz = (size_t)g.ny * g.nz, ncel3 = ncyz * g.nx;
-    const size_t ifir = P.rfsingle ? 0 : (size_t)(itype - P.nold - 1) * ncel3;
-    double xi, yi, zi;
-    int ix, iy, iz;
-    grid_cell(g, x, y, z, xi, yi, zi, ix, iy, iz);
-    ix = min(max(ix, 0), g.nx - 2);
-    iy = min(max(iy, 0), g.ny - 2);
-    iz = min(max(iz, 0), g.nz - 2);
-    const double one = 1.0;
-    double aux1 = 0.0, aux2 = 0.0;
-    for (int n1 = ix; n1 <= ix + 1; n1++) {
-        double ai = xi - n1 * g.dcel;
-        const double aisign = fsign(one, ai);
-        ai = one - fabs(ai) * g.idcel;
-        for (int n2 = iy; n2 <= iy + 1; n2++) {
-            double bi = yi - n2 * g.dcel;
-            const double bisign = fsign(one, bi);
-            bi = one - fabs(bi) * g.idcel;
-            for (int n3 = iz; n3 <= iz + 1; n3++) {
-                double ci = zi - n3 * g.dcel;
-                const double cisign = fsign(one, ci);
-                ci = one - fabs(ci) * g.idcel;
-                const double fi = ai * bi * ci;
-                const size_t in3 = n1 * ncyz + (size_t)n2 * g.nz + n3 + ifir;
-                const double gaux1 = (double)P.gsrfen[in3];
-                const double gaux2 = P.sqrfac * (double)P.greff[in3];
-                if (gaux1 == 0.0) break;
-                const double prefa1 = gaux1 * g.idcel, prefa2 = gaux2 * g.idcel;
-                aux1 = aux1 + fi * gaux1;
-                aux2 = aux2 + fi * gaux2;
-                if (grad) {
-                    if (ai < (one - RSMALL) && ai > RSMALL) {
-                        const double aux = aisign * bi * ci;
-                        ds[0] = ds[0] - aux * prefa1;
-                        dr[0] = dr[0] - aux * prefa2;
-                    }
-                    if (bi < (one - RSMALL) && bi > RSMALL) {
-                        const double aux = bisign * ai * ci;
-                        ds[1] = ds[1] - aux * prefa1;
-                        dr[1] = dr[1] - aux * prefa2;
-                    }
-                    if (ci < (one - RSMALL) && ci > RSMALL) {
-                        const double aux = cisign * ai * bi;
-                        ds[2] = ds[2] - aux * prefa1;
-                        dr[2] = dr[2] - aux * prefa2;
-                    }
-                }
-            }
-        }
-    }
-    srfe = aux1;
-    reff = aux2;
-    return true;
-}
-
-struct V3 { double x, y, z; };
-__device__ __forceinline__ V3 ld3(const StepParams &P, int i) {
-    const double4 q = P.pos[i];
-    return {q.x, q.y, q.z};
-}
-__device__ __forceinline__ V3 sub3(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ double dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V3 cross3(V3 u, V3 v) {
-    return {u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
-}
-__device__ __forceinline__ V3 scl3(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
-
-// slots of the DNA/grid energy accumulator and the energy categories they feed
-constexpr int ND = 10;
-__device__ constexpr int ND_CAT[ND] = {E_BOND, E_ANG, E_DIHE, E_STACK, E_BP, E_EX, E_QQ, E_SOLV,
-                                       E_GSBPA, E_VDWGD};
-enum { D_BOND = 0, D_ANG, D_DIHE, D_STACK, D_BP, D_EX, D_QQ, D_SOLV, D_GSBPA, D_VDWGD };
-
-// One DNA term incident on site `me` in role `role`: force on `me` (added to f) and the
-// term's energy (added to e when role == 0).  energy.mono.f90:205-507.
-__device__ void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, double *e) {
-    const int kind = w >> 29, role = (w >> 27) & 3, t = w & 0x7FFFFFF;
-    const double eps = P.epsnuc;
-    const int n = P.nlist[kind];
-    const int32_t *L = P.lst[kind];
-    if (kind == T_BOND) {
-        int a = L[t], b = L[n + t];
-        V3 v1 = sub3(ld3(P, b), ld3(P, a));
-        double dist = sqrt(dot3(v1, v1));
-        double vard = dist - P.lval[0][t];
-        double vard2 = vard * vard;
-        if (role == 0) e[D_BOND] += eps * vard2 * (1.0 + 100.0 * vard2);
-        if (free_me) {
-            double de = eps * vard * (2.0 + 400.0 * vard2) / dist;
-            V3 f1 = scl3(de, v1);
-            double s = role == 0 ? 1.0 : -1.0;
-            f.x += s * f1.x; f.y += s * f1.y; f.z += s * f1.z;
-        }
-    } else if (kind == T_ANGLE) {
-        int a = L[t], b = L[n + t], d = L[2 * n + t];
-        V3 rb = ld3(P, b);
-        V3 v1 = sub3(ld3(P, a), rb), v2 = sub3(ld3(P, d), rb);
-        double m1 = dot3(v1, v1), m2 = dot3(v2, v2), m3 = dot3(v1, v2);
-        double modval = 1.0 / sqrt(m1 * m2);
-        double ang = acos(m3 * modval);
-        double varang = ang - P.lval[1][t];
-        if (role == 0) e[D_ANG] += 700.0 * eps * (varang * varang);
-        if (free_me) {
-            double de = 1400.0 * eps * varang * modval / sin(ang);
-            V3 f1 = scl3(de, v1), f2 = scl3(de, v2);
-            if (role == 0) { f.x += f2.x; f.y += f2.y; f.z += f2.z; }
-            else if (role == 1) { f.x -= f1.x + f2.x; f.y -= f1.y + f2.y; f.z -= f1.z + f2.z; }
-            else { f.x += f1.x; f.y += f1.y; f.z += f1.z; }
-        }
-    } else if (kind == T_DIHE) {
-        int i1 = L[t], i2 = L[n + t], i3 = L[2 * n + t], i4 = L[3 * n + t];
-        V3 r2 = ld3(P, i2), r3 = ld3(P, i3);
-        V3 v1 = sub3(ld3(P, i1), r2), v2 = sub3(r3, r2), v3 = sub3(r3, ld3(P, i4));
-        V3 n1 = cross3(v1, v2), n2 = cross3(v2, v3);
-        double v22 = dot3(v2, v2);
-        double v2v = sqrt(v22);
-        double av = dot3(v1, n2), bv = dot3(n1, n2);
-        double dihe = atan2(v2v * av, bv);
-        double vardihe = dihe - P.lval[2][t];
-        if (role == 0) e[D_DIHE] += 28.0 * eps * (1.0 - cos(vardihe));
-        if (free_me) {
-            double n1v = dot3(n1, n1), n2v = dot3(n2, n2);
-            double v12 = dot3(v1, v2), v23 = dot3(v2, v3);
-            double de = 28.0 * eps * sin(vardihe) * v2v;
-            V3 f1 = scl3(-de / n1v, n1), f4 = scl3(de / n2v, n2);
-            double iv22 = 1.0 / v22;
-            v12 = v12 * iv22;
-            v23 = v23 * iv22;
-            V3 g;
-            if (role == 0) g = f1;
-            else if (role == 3) g = f4;
-            else if (role == 1) g = {-f1.x + v12 * f1.x - v23 * f4.x, -f1.y + v12 * f1.y - v23 * f4.y,
-                                     -f1.z + v12 * f1.z - v23 * f4.z};
-            else g = {-f4.x - v12 * f1.x + v23 * f4.x, -f4.y - v12 * f1.y + v23 * f4.y,
-                      -f4.z - v12 * f1.z + v23 * f4.z};
-            f.x += g.x; f.y += g.y; f.z += g.z;
-        }
-    } else {
-        int a = L[t], b = L[n + t];
-        V3 v1 = sub3(ld3(P, b), ld3(P, a));
-        double de = 0.0;
-        bool force = free_me;
-        if (kind == T_STACK) {
-            double dist2 = dot3(v1, v1);
-            double idist2 = 1.0 / dist2;
-            double sg = P.lval[3][t];
-            double cte1 = pow3(sg * sg * idist2);
-            if (role == 0) e[D_STACK] += 4.0 * eps * cte1 * (cte1 - 1.0);
-            de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
-        } else if (kind == T_BP) {
-            int na = P.namsite[a];
-            double epsln = (na == 5 || na == 4) ? 2.532 * eps * P.scalepairing
-                                                : 2.0 * eps * P.scalepairing;
-            double dist2 = dot3(v1, v1);
-            double idist2 = 1.0 / dist2;
-            double sg = P.lval[4][t];
-            double cte1 = sg * sg * idist2;
-            if (role == 0) e[D_BP] += epsln * pow5(cte1) * (20.0 * cte1 - 24.0);
-            de = 240.0 * epsln * pow5(cte1) * (cte1 - 1.0) * idist2;
-        } else if (kind == T_EX) {
-            double dist2 = dot3(v1, v1);
-            double idist2 = 1.0 / dist2;
-            double sgex2 = P.lval[5][t] * P.lval[5][t];
-            if (dist2 < sgex2) {
-                double cte1 = pow3(sgex2 * idist2);
-                if (role == 0) e[D_EX] += 4.0 * eps * cte1 * (cte1 - 1.0) + eps;
-                de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
-            } else {
-                force = false;
-            }
-        } else if (kind == T_QQ) {
-            double dist = sqrt(dot3(v1, v1));
-            double idist = 1.0 / dist;
-            if (P.flags & QDEBY) {
-                double idistkd = exp(-dist * P.ikappa);
-                if (role == 0) e[D_QQ] += P.fctn * idist * idistkd;
-                de = P.fctn * (dist + P.kappa) * pow3(idist) * P.ikappa * idistkd;
-            } else {
-                if (role == 0) e[D_QQ] += P.fctn * idist;
-                de = P.fctn * pow3(idist);
-            }
-        } else {  // T_SLV
-            double dist = sqrt(dot3(v1, v1));
-            double cte1 = exp((13.38 - dist) * 0.1875);
-            if (role == 0) e[D_SOLV] += P.epsolv * ((1.0 - cte1) * (1.0 - cte1)) - P.epsolv;
-            de = 0.375 * P.epsolv * cte1 * (cte1 - 1.0) / dist;
-        }
-        if (force) {
-            double s = role == 0 ? -de : de;
-            f.x += s * v1.x; f.y += s * v1.y; f.z += s * v1.z;
-        }
-    }
-}
-
-// 1/sqrt(x2) to within an ulp: hardware estimate plus two Newton steps.  Every branch
-// of the table lookup derives r, 1/r and 1/r^2 from it.
-__device__ __forceinline__ double rsqrt_nr(double x2) {
-    double r = __builtin_amdgcn_rsq(x2);
-    double e = fma(-x2 * r, r, 1.0);
-    r = fma(r * e, fma(0.375, e, 0.5), r);
-    e = fma(-x2 * r, r, 1.0);
-    r = fma(r * e, 0.5, r);
-    return r;
-}
-
-// Table pair (getyd, polynom.f90:197-230) from x2 and rinv = 1/sqrt(x2): wall below dmi,
-// analytic tail above the last knot, piecewise quadratic in between.
-__device__ __forceinline__ void efpot_eval(const DevPair &p, const double4 *__restrict__ ep,
-                                           double ires, double x2, double rinv, double &y,
-                                           double &yp) {
-    const double ix2 = rinv * rinv;
-    if (x2 < p.dm2lo) {
-        const double c = p.sc1 * pow6(ix2);
-        y = c + p.sc2;
-        yp = 12.0 * c * ix2;
-    } else if (x2 > p.dm2hi) {
-        const double c = p.sc3 * pow3(ix2);
-        const double f = p.fct * rinv;     // fct is zero for uncharged pairs
-        y = c + f;
-        yp = (6.0 * c + f) * ix2;
-    } else {
-        const double x = x2 * rinv;
-        int i = (int)((x - p.dmi) * ires) + p.nxi;
-        i = min(max(i, p.nxi), p.nlast);
-        const double4 e = ep[i];
-        y = e.x + x * (e.y + x * e.z);
-        yp = -e.y * rinv - 2.0 * e.z;
-    }
-}
-
-// Gaussians of particle i for this launch's step: injected rows are addressed by the
-// step counter, Philox rows were generated for the current chunk of steps by
-// k_philox_fill (row = step_off).
-__device__ __forceinline__ const double *noise_row(const StepParams &P, int i) {
-    const int64_t row = P.noise_mode == 0 ? P.step_ctr[1] + P.step_ctr[0] + P.step_off
-                                          : P.step_off;
-    return P.noise + row * P.noise_stride + 3 * (int64_t)i;
-}
-
-// One partner j of a target: the table pair of getyd (polynom.f90:197-230) evaluated
-// without branches.  Wall, tail and table forms are all formed from rinv = 1/sqrt(d2);
-// the knot index is clamped into the pair's table, so the gather is always in bounds and
-// is simply not selected outside [dmi, r_n].  Returns the force factor; adds the energy.
-__device__ __forceinline__ double pair_table(const DevPair &pp, const double4 *__restrict__ ep,
-                                            double ires, double d2, double &y) {
-    const double rinv = rsqrt_nr(d2);
-    const double ix2 = rinv * rinv;
-    const double x = d2 * rinv;
-    int k = (int)((x - pp.dmi) * ires) + pp.nxi;
-    k = min(max(k, pp.nxi), pp.nlast);
-    const double4 e = ep[k];
-    const double c3 = pp.sc3 * pow3(ix2);
-    const double f = pp.fct * rinv;              // zero for uncharged pairs
-    const double c12 = pp.sc1 * pow6(ix2);
-    const bool wall = d2 < pp.dm2lo;
-    const bool tail = d2 > pp.dm2hi;
-    const double yt = e.x + x * (e.y + x * e.z);
-    const double ypt = -e.y * rinv - 2.0 * e.z;
-    y = wall ? c12 + pp.sc2 : (tail ? c3 + f : yt);
-    return wall ? 12.0 * c12 * ix2 : (tail ? (6.0 * c3 + f) * ix2 : ypt);
-}
-
-// Table pair (getyd, polynom.f90:197-230) in tail-first form: every lane forms the
-// analytic tail sc3/r^6 + fct/r from rinv = 1/sqrt(d2); only lanes inside the table range
-// (d2 <= dm2hi) enter the branch that replaces it with the wall or the tabulated quadratic,
-// so far partners cost ~25 operations and near ones pay for the knot gather alone.
-// tl = (sc3, tail fct, dm2hi, unused) of the pair type.
-__device__ __forceinline__ double pair_near_far(const DevPair &pp, const double4 &tl,
-                                               const double4 *__restrict__ ep, double ires,
-                                               double d2, double &y) {
-    const double rinv = rsqrt_nr(d2);
-    const double ix2 = rinv * rinv;
-    const double c = tl.x * pow3(ix2);
-    const double f = tl.y * rinv;
-    y = c + f;
-    double de = (6.0 * c + f) * ix2;
-    if (d2 <= tl.z) {
-        if (d2 < pp.dm2lo) {
-            const double cw = pp.sc1 * pow6(ix2);
-            y = cw + pp.sc2;
-            de = 12.0 * cw * ix2;
-        } else {
-            const double x = d2 * rinv;
-            int k = (int)((x - pp.dmi) * ires) + pp.nxi;
-            k = min(max(k, pp.nxi), pp.nlast);
-            const double4 e = ep[k];
-            y = e.x + x * (e.y + x * e.z);
-            de = -e.y * rinv - 2.0 * e.z;
-        }
-    }
-    return de;
-}
 
 // k_step: T target particles per block of 256 threads (targets tbeg.. of the launch).
 // Every thread walks the partner set with stride 256, two partners per iteration, and
@@ -1381,44 +784,6 @@ __device__ double dna_term_interact(const StepParams &P, uint32_t w) {
     return P.epsolv * ((1.0 - cte1) * (1.0 - cte1)) - P.epsolv;
 }
 
-// analytic pair energy as INTERACT writes it (interact.f90:182-204,365-383): the SRPMF
-// smoothing reads e*exp(...)-srpy (ENERGY has e*(exp(...)-srpy))
-__device__ __forceinline__ double interact_analytic(const DevPair &p, const StepParams &P,
-                                                    double dist2) {
-    double e = 0.0, idist = 0.0, idist2 = 1.0 / dist2;
-    if (p.flags & (PF_CHR | PF_SRPMF)) idist = sqrt(idist2);
-    if (p.flags & PF_CHR) e = e + p.fct * idist;
-    if (p.flags & PF_LJ) {
-        double dist6 = pow3(p.sgp2 * idist2);
-        e = e + p.epp4 * dist6 * (dist6 - 1.0);
-    }
-    if ((p.flags & PF_SRPMF) && dist2 <= P.rth) {
-        double dist = 1.0 / idist;
-        double es = p.c0 * exp((p.c1 - dist) * p.c2) * cos(p.c3 * PI * (p.c1 - dist)) +
-                    p.c4 * pow6(p.c1 * idist);
-        if (dist >= P.srpx) es = es * exp(-P.srpk * (dist - P.srpx)) - P.srpy;
-        e = e + es;
-    }
-    return e;
-}
-
-// gety, polynom.f90:172-194
-__device__ __forceinline__ double gety(const DevPair &p, const double4 *__restrict__ ep,
-                                       double ires, double x2) {
-    if (x2 < p.dm2lo) {
-        double ix2 = 1.0 / x2;
-        return p.sc1 * pow6(ix2) + p.sc2;
-    } else if (x2 > p.dm2hi) {
-        double ix2 = 1.0 / x2;
-        if (p.flags & PF_CHR) return p.sc3 * pow3(ix2) + p.fct * sqrt(ix2);
-        return p.sc3 * pow3(ix2);
-    }
-    double x = sqrt(x2);
-    int i = (int)((x - p.dmi) * ires) + p.nxi;
-    double4 e = ep[i];
-    return e.x + x * (e.y + x * e.z);
-}
-
 __global__ __launch_bounds__(BLOCK) void k_interact(InteractParams Q) {
     const StepParams &P = Q.sp;
     __shared__ DevPair s_pairs[MAX_LDS_PAIRS];
@@ -1438,59 +803,13 @@ __global__ __launch_bounds__(BLOCK) void k_interact(InteractParams Q) {
     double e = 0.0;
     // staticf0.f90:19-82 and vdwgd0trln.f90:19-117 on lanes 0..7 of wave 0
     if (tid < WAVE) {
-        double es = 0.0, ev = 0.0, ph = 0.0;
-        bool okv = false;
-        if ((P.flags & QPHIX) && tid < 8) {
-            const GridGeom &g = P.gx;
-            bool besite = (P.flags & QNUCL) && j < ns && P.namsite[j] == 1;
-            bool ok = beion || besite;
-            double chi = beion ? P.cg[Q.jtype] : P.cgnuc;
-            ok = ok && xj <= g.xbcen + g.tranx && xj >= g.xbcen - g.tranx &&
-                 yj <= g.ybcen + g.trany && yj >= g.ybcen - g.trany &&
-                 zj <= g.zbcen + g.tranz && zj >= g.zbcen - g.tranz;
-            if (ok) {
-                double gx_, gy_, gz_;
-                int ix, iy, iz;
-                grid_cell(g, xj, yj, zj, gx_, gy_, gz_, ix, iy, iz);
-                int n1 = ix + (tid >> 2), n2 = iy + ((tid >> 1) & 1), n3 = iz + (tid & 1);
-                double ai = 1.0 - fabs(gx_ - n1 * g.dcel) * g.idcel;
-                double bi = 1.0 - fabs(gy_ - n2 * g.dcel) * g.idcel;
-                double ci = 1.0 - fabs(gz_ - n3 * g.dcel) * g.idcel;
-                double fi = ai * bi * ci;
-                es = fi * chi * (double)P.phix[((size_t)n1 * g.ny + n2) * g.nz + n3] * P.celec;
-            }
-        }
-        if (P.flags & QPHIV) {
-            const GridGeom &g = P.gv;
-            okv = xj <= g.xbcen + g.tranx && xj >= g.xbcen - g.tranx && yj <= g.ybcen + g.trany &&
-                  yj >= g.ybcen - g.trany && zj <= P.vzmax && zj >= P.vzmin;
-            if (okv && tid < 8) {
-                double gx_, gy_, gz_;
-                int ix, iy, iz;
-                grid_cell(g, xj, yj, zj, gx_, gy_, gz_, ix, iy, iz);
-                double esvdw = P.svdw;
-                if (P.flags & QSVDW) esvdw = esvdw * P.scal[jt2];
-                int n1 = ix + (tid >> 2), n2 = iy + ((tid >> 1) & 1), n3 = iz + (tid & 1);
-                double ai = 1.0 - fabs(gx_ - n1 * g.dcel) * g.idcel;
-                double bi = 1.0 - fabs(gy_ - n2 * g.dcel) * g.idcel;
-                double ci = 1.0 - fabs(gz_ - n3 * g.dcel) * g.idcel;
-                double fi = ai * bi * ci;
-                ph = (double)P.phiv[((size_t)n1 * g.ny + n2) * g.nz + n3];
-                ev = fi * esvdw * ph;
-            }
-        }
-        es = wave_sum(es);
-        ev = wave_sum(ev);
-        ph = wave_sum(ph);
+        const bool besite = (P.flags & QNUCL) && j < ns && P.namsite[j] == 1;
+        const double chi = beion ? P.cg[Q.jtype] : P.cgnuc;
+        bool wall = false;
+        const double eg = interact_grids(P, P.phix, beion || besite, chi, xj, yj, zj, jt2, wall);
         if (tid == 0) {
-            e += es;
-            if (okv) {
-                if (ph >= P.thold8) {
-                    ev = 1.0e10;
-                    if (Q.alert) atomicAdd(&P.warn[jt2], 1);
-                }
-                e += ev;
-            }
+            e += eg;
+            if (wall && Q.alert) atomicAdd(&P.warn[jt2], 1);
         }
     }
     const bool nonbond = (P.flags & QNONBOND) != 0;

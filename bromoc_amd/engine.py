@@ -172,3 +172,113 @@ def philox4x64(ctr, key):
     if rc:
         raise abi.BromocError("bromoc_philox4x64", rc)
     return out
+
+
+class Batch:
+    """Many ion systems sharing one System's tables (replica batches of the HIP library).
+
+        b = Batch(shared, [Replica(...), ...])
+        b.run(nsteps, seed, counter0, gcmc_every, ngcmc)   # GRAND + BD cycles, all replicas
+        x, typei, ntot, ibuffer, nat, ener = b.get(r)
+    """
+
+    def __init__(self, shared: System, replicas, device: int = 0):
+        from .system import Replica  # noqa: F401
+        self.lib = abi.load_library()
+        self.shared = shared
+        self.replicas = list(replicas)
+        st, keep = shared.to_struct()
+        arr = (abi.ReplicaStruct * len(self.replicas))()
+        for k, rep in enumerate(self.replicas):
+            arr[k] = rep.to_struct(keep)
+        self._keep = keep
+        ctx = C.c_void_p()
+        rc = self.lib.bromoc_batch_create(C.byref(st), len(self.replicas), arr, device,
+                                          C.byref(ctx))
+        if rc:
+            raise abi.BromocError("bromoc_batch_create", rc)
+        self.ctx = ctx
+        self.nrep = len(self.replicas)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.bromoc_batch_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, name, rc):
+        if rc:
+            msg = self.lib.bromoc_batch_last_error(self.ctx)
+            raise abi.BromocError(name, rc, msg.decode() if msg else "")
+
+    def run(self, nsteps, seed=0, counter0=0, gcmc_every=0, ngcmc=0, cycle0=0, energies=False):
+        e = (abi.Energies * self.nrep)() if energies else None
+        self._chk("bromoc_batch_run",
+                  self.lib.bromoc_batch_run(self.ctx, nsteps, seed, counter0, gcmc_every, ngcmc,
+                                            cycle0, e))
+        return [x.as_dict() for x in e] if energies else None
+
+    def energy(self):
+        e = (abi.Energies * self.nrep)()
+        self._chk("bromoc_batch_energy", self.lib.bromoc_batch_energy(self.ctx, e))
+        return [x.as_dict() for x in e]
+
+    def get(self, r):
+        rep = self.replicas[r]
+        cap = max(rep.capacity, rep.ntot, 1)
+        x, y, z = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        t = np.zeros(cap, np.int32)
+        ib = np.zeros(cap, np.int32)
+        nb = rep.gcmc.nbuffer if rep.gcmc is not None else 0
+        nat = np.zeros(nb + 1, np.int32)
+        n = C.c_int32()
+        en = C.c_double()
+        self._chk("bromoc_batch_get",
+                  self.lib.bromoc_batch_get(self.ctx, r, ptr(x), ptr(y), ptr(z), ptr(t, C.c_int32),
+                                            C.byref(n), ptr(ib, C.c_int32), ptr(nat, C.c_int32),
+                                            C.byref(en)))
+        k = n.value
+        return (np.stack([x[:k], y[:k], z[:k]], axis=1), t[:k].copy(), k, ib[:k].copy(), nat,
+                en.value)
+
+    def forces(self, r):
+        k = self.get(r)[2]
+        fx, fy, fz = np.zeros(max(k, 1)), np.zeros(max(k, 1)), np.zeros(max(k, 1))
+        self._chk("bromoc_batch_get_forces",
+                  self.lib.bromoc_batch_get_forces(self.ctx, r, ptr(fx), ptr(fy), ptr(fz)))
+        return np.stack([fx[:k], fy[:k], fz[:k]], axis=1)
+
+    def flux(self, r):
+        s = self.shared
+        nt, np_ = s.ntype - s.nold, len(s.zcont)
+        nf = np.zeros(max(nt * np_, 1), np.int32)
+        nb = np.zeros(max(nt * np_, 1), np.int32)
+        self._chk("bromoc_batch_get_flux",
+                  self.lib.bromoc_batch_get_flux(self.ctx, r, ptr(nf, C.c_int32), ptr(nb, C.c_int32)))
+        return nf[:nt * np_].reshape(nt, np_), nb[:nt * np_].reshape(nt, np_)
+
+    def prob(self, r):
+        rep = self.replicas[r]
+        nb = rep.gcmc.nbuffer if rep.gcmc is not None else 0
+        cap = max(rep.capacity, rep.ntot, 1)
+        p = np.zeros((max(nb, 1), cap + 1), np.int32)
+        self._chk("bromoc_batch_get_prob",
+                  self.lib.bromoc_batch_get_prob(self.ctx, r, ptr(p, C.c_int32)))
+        return p[:nb]
+
+    def time_steps(self, nsteps, seed=0, counter0=0):
+        tot, ker = C.c_double(), C.c_double()
+        self._chk("bromoc_batch_time_steps",
+                  self.lib.bromoc_batch_time_steps(self.ctx, nsteps, seed, counter0,
+                                                   C.byref(tot), C.byref(ker)))
+        return tot.value, ker.value
+
+    def tuning(self):
+        v = C.c_int32()
+        self._chk("bromoc_batch_get_tuning", self.lib.bromoc_batch_get_tuning(self.ctx, C.byref(v)))
+        return v.value

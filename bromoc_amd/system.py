@@ -315,3 +315,61 @@ def ptr(a, ctype=C.c_double):
 
 
 __all__ = ["System", "GridData", "Gcmc", "nindex_table", "CELEC", "kbt_of", "ptr", "math"]
+
+
+@dataclass
+class Replica:
+    """One system of a replica batch: its particles, its static field (on the shared phix
+    geometry) and its GCMC buffers."""
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    typei: np.ndarray
+    nfix: int = 0
+    capacity: int = 0
+    phix: np.ndarray = None          # float32 (nclx, ncly, nclz)
+    gcmc: Gcmc | None = None
+    ibuffer: np.ndarray = None       # [ntot]
+    nat: np.ndarray = None           # [nbuffer+1]
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def ntot(self):
+        return len(self.typei)
+
+    def to_struct(self, keep):
+        def arr(a, dtype, ctype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ctype))
+
+        r = abi.ReplicaStruct()
+        r.ntot, r.nfix = self.ntot, self.nfix
+        r.capacity = max(self.capacity, self.ntot, 1)
+        r.x = arr(self.x, np.float64, C.c_double)
+        r.y = arr(self.y, np.float64, C.c_double)
+        r.z = arr(self.z, np.float64, C.c_double)
+        r.typei = arr(self.typei, np.int32, C.c_int32)
+        if self.phix is not None:
+            r.phix = arr(self.phix, np.float32, C.c_float)
+        if self.gcmc is not None:
+            g, gk = self.gcmc.to_struct()
+            keep.extend([g, gk])
+            r.gcmc = C.pointer(g)
+            r.ibuffer = arr(self.ibuffer[:self.ntot], np.int32, C.c_int32)
+            r.nat = arr(self.nat, np.int32, C.c_int32)
+        return r
+
+
+def replica_system(shared: System, rep: Replica) -> System:
+    """The single system that replica `rep` of a batch over `shared` is."""
+    s = shared.copy()
+    n = rep.ntot
+    s.ntot, s.nsites, s.nfix = n, 0, rep.nfix
+    s.capacity = max(rep.capacity, n, 1)
+    s.x, s.y, s.z = (np.asarray(a, float).copy() for a in (rep.x, rep.y, rep.z))
+    s.typei = np.asarray(rep.typei, np.int32).copy()
+    if rep.phix is not None and s.phix is not None:
+        s.phix = GridData(np.asarray(rep.phix, np.float32), s.phix.dcel, s.phix.xbcen,
+                          s.phix.ybcen, s.phix.zbcen)
+    return s

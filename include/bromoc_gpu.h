@@ -282,6 +282,62 @@ int bromoc_spline(int32_t n, const double *x, const double *y, double *b, double
  * counterpart in the reference (its noise is libgfortran rand()). */
 int bromoc_philox4x64(int32_t n, const uint64_t *ctr, const uint64_t *key, uint64_t *out);
 
+/* ------------------------------------------------------------------ replica batches
+ * Many independent ion systems (no CG-DNA, no reaction-field grids) that share one
+ * bromoc_system's types, pair tables, repulsion map, box, flux planes and BD constants, and
+ * differ in their particles, their static field and their GCMC buffers.  Every replica
+ * runs the same loop as simul1.f90:352-390 -- GRAND (grand.f90:19-121) every gcmc_every
+ * steps, then ENERGY + DYNAMICS per step -- and all replicas advance together: one launch
+ * per BD step covers every replica, its waves taking (replica, ion) work items from a
+ * device queue ordered by replica size; one workgroup per replica runs its GCMC attempts.
+ * No counterpart in the reference (one system per process); results per replica are those
+ * of the single-system entry points with Philox stream r (replica r draws its Gaussians
+ * with counter (step, ion, r, 0), its GCMC uniforms with counter (cycle, k/4, r, 1): word
+ * k%4 of the block, (w >> 11) * 2^-53). */
+typedef struct bromoc_replica {
+    int32_t ntot, nfix, capacity, reserved;  /* particles now, leading fixed ions, slots */
+    const double  *x, *y, *z;                /* [ntot] */
+    const int32_t *typei;                    /* [ntot] signed atnam type (ions only) */
+    const float   *phix;                     /* own static field on the shared phix
+                                              * geometry (QPHIX), [nclx*ncly*nclz] */
+    const bromoc_gcmc *gcmc;                 /* own buffers, or NULL for no GCMC */
+    const int32_t *ibuffer;                  /* [ntot] buffer of each particle (gcmc) */
+    const int32_t *nat;                      /* [nbuffer+1] particles per buffer (gcmc) */
+} bromoc_replica;
+
+typedef struct bromoc_batch bromoc_batch;
+
+int bromoc_batch_create(const bromoc_system *shared, int32_t nrep, const bromoc_replica *reps,
+                        int device, bromoc_batch **out);
+int bromoc_batch_destroy(bromoc_batch *b);
+const char *bromoc_batch_last_error(const bromoc_batch *b);
+/* nsteps BD steps of every replica with Philox noise (seed, step counters counter0..);
+ * with gcmc_every > 0, ngcmc GRAND attempts per replica before every gcmc_every-th step
+ * (simul1.f90:353-356 order: GRAND, then the cycle's nbd steps), GCMC cycles numbered
+ * from cycle0.  energies (optional, [nrep]) receive each replica's ENERGY terms of the
+ * last step. */
+int bromoc_batch_run(bromoc_batch *b, int32_t nsteps, uint64_t seed, uint64_t counter0,
+                     int32_t gcmc_every, int32_t ngcmc, uint64_t cycle0,
+                     bromoc_energies *energies);
+/* ENERGY of every replica at its current positions; forces stay on the device. */
+int bromoc_batch_energy(bromoc_batch *b, bromoc_energies *energies);
+/* Replica r's particles (x, y, z, typei up to capacity entries), ntot, buffer membership,
+ * nat[nbuffer+1], GCMC energy accumulator (GRAND's ener), any may be NULL. */
+int bromoc_batch_get(bromoc_batch *b, int32_t r, double *x, double *y, double *z,
+                     int32_t *typei, int32_t *ntot, int32_t *ibuffer, int32_t *nat,
+                     double *gcmc_ener);
+int bromoc_batch_get_forces(bromoc_batch *b, int32_t r, double *fx, double *fy, double *fz);
+/* flux counters [ntype-nold][cntpts] and prob[nbuffer][capacity+1] of replica r */
+int bromoc_batch_get_flux(bromoc_batch *b, int32_t r, int32_t *nforward, int32_t *nbackward);
+int bromoc_batch_get_prob(bromoc_batch *b, int32_t r, int32_t *prob);
+/* Measurement: nsteps steps launched one by one (no graph, no GCMC) with a HIP event pair
+ * around each step kernel; mean step-kernel duration in *kernel_ms. */
+int bromoc_batch_time_steps(bromoc_batch *b, int32_t nsteps, uint64_t seed, uint64_t counter0,
+                            double *total_ms, double *kernel_ms);
+/* Launch shape the batch autotuner chose: waves per workgroup-slot count of the persistent
+ * step kernel (blocks), 0 for one workgroup per four work items. */
+int bromoc_batch_get_tuning(bromoc_batch *b, int32_t *blocks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,7 @@ typedef struct orc_ctx {
     /* DYNAMICS1/2 */
     double *diffusion, *ampl3, *p3, *xs, *ys, *bs, *cs, *ds;
     int seval_i, sevald_i;        /* the SAVEd intervals of seval and sevald (1-based) */
+    uint64_t stream;              /* Philox stream of this system's Gaussians */
     /* RFPAR */
     float *gsrfen, *greff;
     double *srfe, *reff, *srfed, *reffd;   /* per ion slot; derivatives [3*cap] */
@@ -1322,10 +1323,13 @@ static int fill_noise(orc_ctx *c, int mode, int row, uint64_t seed, uint64_t cou
         if (!c->noise || row < 0 || row >= c->noise_rows) return BROMOC_E_STATE;
         memcpy(g, c->noise + (size_t)row * 3 * nt, sizeof(double) * 3 * nt);
     } else {
-        for (int i = 0; i < nt; i++) orc_philox_gauss3(seed, counter, 0, i, g + 3 * i);
+        for (int i = 0; i < nt; i++) orc_philox_gauss3(seed, counter, c->stream, i, g + 3 * i);
     }
     return BROMOC_OK;
 }
+
+/* Philox stream of the Gaussians (replica r of a batch draws from stream r) */
+EXPORT void orc_set_stream(orc_ctx *c, uint64_t stream) { c->stream = stream; }
 
 EXPORT int orc_dynamics(orc_ctx *c, int32_t mode, int32_t row, uint64_t seed, uint64_t counter) {
     double *g = (double *)malloc(sizeof(double) * 3 * (c->s.ntot + 1));

@@ -51,6 +51,7 @@ def oracle_lib():
             ("orc_create", [P(abi.SystemStruct)], vp),
             ("orc_destroy", [vp], None),
             ("orc_spline", [C.c_int32, _dp, _dp, _dp, _dp, _dp], C.c_int),
+            ("orc_set_stream", [vp, C.c_uint64], None),
             ("orc_set_positions", [vp, C.c_int32, _dp, _dp, _dp, _ip], C.c_int),
             ("orc_get_positions", [vp, _dp, _dp, _dp, _ip, _ip], C.c_int),
             ("orc_get_forces", [vp, _dp, _dp, _dp], C.c_int),
@@ -107,12 +108,13 @@ class _Base:
 class Oracle(_Base):
     """The C restatement."""
 
-    def __init__(self, system):
+    def __init__(self, system, stream=0):
         self.L = oracle_lib()
         st, self._keep = system.to_struct()
         self.ctx = self.L.orc_create(C.byref(st))
         if not self.ctx:
             raise RuntimeError("orc_create failed")
+        self.L.orc_set_stream(self.ctx, stream)
         self.system = system
         self.ntot = system.ntot
 
@@ -244,6 +246,22 @@ def ref_lib(path=REF_SO):
             L.bromoc_gpu_set_seed.restype = None
         _rlibs[path] = L
     return _rlibs[path]
+
+
+def batch_uniforms(seed, cycle, r, n):
+    """The first n GCMC uniforms of replica r in GCMC cycle `cycle` of a replica batch:
+    uniform k is word k%4 of Philox4x64-10(counter (cycle, k/4, r, 1), key (seed, 0)),
+    (w >> 11) * 2^-53."""
+    L = oracle_lib()
+    nb = (n + 3) // 4
+    out = np.zeros(4 * nb)
+    key = np.array([seed, 0], np.uint64)
+    w = np.zeros(4, np.uint64)
+    for b in range(nb):
+        ctr = np.array([cycle, b, r, 1], np.uint64)
+        L.orc_philox4x64(_p(ctr, C.c_uint64), _p(key, C.c_uint64), _p(w, C.c_uint64))
+        out[4 * b:4 * b + 4] = (w >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return out[:n]
 
 
 def _spline_with(fn, x, y):

@@ -124,3 +124,27 @@ def test_build_table_argument_errors():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(ImportError):
         abi.load_library(str(tmp_path / "nope.so"))
+
+
+def test_batch_create_validation():
+    from bromoc_amd.engine import Batch
+    sh, reps = configs.replica_batch(seed=1, nrep=2, n_range=(10, 20), box=(30.0, 30.0, 60.0),
+                                     dcel=1.5, rpore=8.0, half_slab=8.0,
+                                     table_builder=oracle_table_builder)
+    bad = sh.copy()
+    bad.flags |= abi.QNUCL
+    with pytest.raises(abi.BromocError) as ei:
+        Batch(bad, reps)
+    assert ei.value.code == abi.E_UNSUPPORTED
+    r0 = reps[0]
+    r0.typei = r0.typei.copy()
+    r0.typei[0] = 9                     # not an ion type
+    with pytest.raises(abi.BromocError) as ei:
+        Batch(sh, reps)
+    assert ei.value.code == abi.E_ARG
+    r0.typei[0] = 1
+    r0.nat = r0.nat.copy()
+    r0.nat[1] += 1                      # nat out of step with ibuffer
+    with pytest.raises(abi.BromocError) as ei:
+        Batch(sh, reps)
+    assert ei.value.code == abi.E_ARG
