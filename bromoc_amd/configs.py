@@ -617,8 +617,12 @@ def replica_batch(seed=0, nrep=256, n_range=(100, 4000), box=(96.0, 96.0, 192.0)
         typei = np.array([shared.nold + 1 + (k % 2) for k in range(n)], np.int32)
         order = rng.permutation(n)
         pos, typei = pos[order], typei[order]
+        # GRAND drives the ion count towards the bath concentration: leave room for twice
+        # the bulk count of the box at that concentration
+        bulk = 2 * conc * (AVOGADRO / LITER) * ANGSTROM ** 3 * box[0] * box[1] * box[2]
+        cap = max(int(cap_factor * n) + 64, int(2.0 * bulk) + 64) if gcmc else int(cap_factor * n) + 64
         rep = Replica(x=pos[:, 0].copy(), y=pos[:, 1].copy(), z=pos[:, 2].copy(), typei=typei,
-                      nfix=0, capacity=int(cap_factor * n) + 64,
+                      nfix=0, capacity=cap,
                       phix=field.astype(np.float32), meta=dict(volt=volt, conc=conc))
         if gcmc:
             rs = replica_system(shared, rep)
