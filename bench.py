@@ -110,7 +110,8 @@ def batch_legs(steps=300, gcmc_every=10, ngcmc=20, small_n=200, small_steps=400)
             "ion_range": [min(r.ntot for r in reps), max(r.ntot for r in reps)],
             "steps": steps, "gcmc_every": gcmc_every, "ngcmc": ngcmc,
             "ms_per_step": 1e3 * el / steps, "step_kernel_ms": kms,
-            "launch_shape_blocks": b.tuning(), "grid": list(sh.phix.data.shape),
+            "launch_shape": dict(zip(("blocks", "lanes_per_target"), b.tuning())),
+            "grid": list(sh.phix.data.shape),
             "setup_s": gen_s}
     b.close()
     del b

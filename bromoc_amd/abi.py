@@ -183,7 +183,7 @@ def _declare(lib):
         "bromoc_batch_get_flux": ([vp, C.c_int32, _ip, _ip], C.c_int),
         "bromoc_batch_get_prob": ([vp, C.c_int32, _ip], C.c_int),
         "bromoc_batch_time_steps": ([vp, C.c_int32, C.c_uint64, C.c_uint64, _dp, _dp], C.c_int),
-        "bromoc_batch_get_tuning": ([vp, _ip], C.c_int),
+        "bromoc_batch_get_tuning": ([vp, _ip, _ip], C.c_int),
     }
     for name, (args, res) in sig.items():
         if hasattr(lib, name):

@@ -26,6 +26,7 @@ namespace bromoc {
 
 // ---- work-item prefix (one block of 1024 threads) and step / cycle counters
 constexpr int PREP_THREADS = 1024;
+constexpr int BATCH_LDS_WSTART = 4096;   // work-item prefix held in LDS up to this size
 
 __global__ __launch_bounds__(PREP_THREADS) void k_batch_prep(BatchParams B, int gcmc) {
     const int tid = threadIdx.x;
@@ -85,31 +86,75 @@ hipError_t launch_batch_prep(const BatchParams &b, int gcmc, hipStream_t s) {
     return hipGetLastError();
 }
 
-// the wavefront's own LDS rows are rewritten per work item: order the lanes' LDS accesses
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// ---- one BD step (or ENERGY) of every replica
+// G lanes per target: each wavefront holds 64/G targets, every lane group of G lanes takes
+// its own work item (a moving ion of some replica), strides over that replica's particles
+// and reduces its sums within the group.  Small replicas want small groups (more targets
+// in flight, the per-item latencies overlap); the autotuner picks G.
+// feature bits of the k_batch_step specialisations
+constexpr int BF_PHIX = 1;       // static field grids
+constexpr int BF_PHIV = 2;       // repulsion map
+constexpr int BF_ANALYTIC = 4;   // some pair type is analytic (else every pair is tabulated)
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
 }
 
-// ---- one BD step (or ENERGY) of every replica
-template <int FEAT>
+// component c of philox_gauss3 (the same operations, so the same bits)
+__device__ __forceinline__ double philox_gauss1(uint64_t seed, uint64_t step, uint64_t stream,
+                                                int64_t i, int c) {
+    uint64_t w[4] = {step, (uint64_t)i, stream, 0};
+    philox4x64(w, seed, 0);
+    const double inv53 = 1.0 / 9007199254740992.0;
+    const double twopi = 2.0 * PI;
+    if (c < 2) {
+        const double u1 = (double)((w[0] >> 11) + 1) * inv53;
+        const double u2 = (double)(w[1] >> 11) * inv53;
+        const double r1 = sqrt(-2.0 * log(u1));
+        return c == 0 ? r1 * cos(twopi * u2) : r1 * sin(twopi * u2);
+    }
+    const double u3 = (double)((w[2] >> 11) + 1) * inv53;
+    const double u4 = (double)(w[3] >> 11) * inv53;
+    const double r2 = sqrt(-2.0 * log(u3));
+    return r2 * cos(twopi * u4);
+}
+
+template <int FEAT, int G>
 __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchParams B) {
+    static_assert(G >= 8 && G <= WAVE && (WAVE % G) == 0, "lane group");
     const StepParams &P = B.sp;
-    __shared__ DevPair s_row[WPB][MAX_LDS_TYPES];
-    __shared__ double4 s_tail[WPB][MAX_LDS_TYPES];
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x >> 6;
-    const int nt1 = P.nttyp + 1;
-    const int W = B.wstart[B.nrep];
+    // every (target type, partner type) row of the shared tables, and the work-item
+    // prefix when it fits
+    extern __shared__ double4 s_dyn[];
+    const int nt1 = P.nttyp + 1, nrow = nt1 * nt1;
+    DevPair *s_rows = reinterpret_cast<DevPair *>(s_dyn);
+    double4 *s_tail = s_dyn + (size_t)nrow * (sizeof(DevPair) / sizeof(double4));
+    int32_t *s_ws = reinterpret_cast<int32_t *>(s_tail + nrow);
+    const bool ws_lds = B.nrep + 1 <= BATCH_LDS_WSTART;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
+    const int sub = lane / G, sl = lane % G;
+    for (int k = tid; k < nrow; k += BLOCK) {
+        const DevPair d = P.rows[k];
+        s_rows[k] = d;
+        s_tail[k] = make_double4(d.sc3, d.fct, d.dm2hi, 0.0);
+    }
+    if (ws_lds)
+        for (int k = tid; k <= B.nrep; k += BLOCK) s_ws[k] = B.wstart[k];
+    __syncthreads();
+    const int32_t *ws = ws_lds ? s_ws : B.wstart;
+    const int W = ws[B.nrep];
     const uint64_t step = (uint64_t)B.ctl[4], seed = (uint64_t)B.ctl[3];
     const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
-    int next = blockIdx.x * WPB + wv;
+    int next = (blockIdx.x * WPB + wv) * (WAVE / G) + sub;
     for (;;) {
         int w;
         if (B.persistent) {
             int v = 0;
-            if (lane == 0) v = atomicAdd(&B.queue[0], 1);
-            w = __shfl(v, 0, WAVE);
+            if (sl == 0) v = atomicAdd(&B.queue[0], 1);
+            w = __shfl(v, sub * G, WAVE);
         } else {
             w = next;
             next = INT_MAX;
@@ -119,25 +164,20 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchParams B) {
         int lo = 0, hi = B.nrep;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (B.wstart[mid] <= w) lo = mid;
+            if (ws[mid] <= w) lo = mid;
             else hi = mid;
         }
         const int r = B.order[lo];
         const int off = B.off[r], n = B.ntot[r];
-        const int t = B.nfix[r] + (w - B.wstart[lo]);
+        const int t = B.nfix[r] + (w - ws[lo]);
         const double4 pa = B.pos[off + t];
         const int t2 = __double2loint(pa.w);
         const int itype = abs(B.typei[off + t]);
-        wave_sync();
-        if (lane < nt1) {
-            const DevPair d = P.rows[t2 * nt1 + lane];
-            s_row[wv][lane] = d;
-            s_tail[wv][lane] = make_double4(d.sc3, d.fct, d.dm2hi, 0.0);
-        }
-        wave_sync();
+        const DevPair *rowp = s_rows + t2 * nt1;
+        const double4 *tailp = s_tail + t2 * nt1;
         double fx = 0.0, fy = 0.0, fz = 0.0, et = 0.0, eel = 0.0, evd = 0.0, esr = 0.0;
         if (nonbond) {
-            for (int p = lane; p < n; p += WAVE) {
+            for (int p = sl; p < n; p += G) {
                 if (p == t) continue;
                 const double4 q = B.pos[off + p];
                 const int tj = __double2loint(q.w);
@@ -146,12 +186,12 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchParams B) {
                 // an ion pair's energy is counted by its higher-index ion (energy.mono.f90:117)
                 const bool ce = p < t;
                 double y, de;
-                if (P.all_efpot || (s_row[wv][tj].flags & PF_EFPOT)) {
-                    de = pair_near_far(s_row[wv][tj], s_tail[wv][tj], P.ep, P.ires, d2, y);
+                if ((FEAT & BF_ANALYTIC) == 0 || (rowp[tj].flags & PF_EFPOT)) {
+                    de = pair_near_far(rowp[tj], tailp[tj], P.ep, P.ires, d2, y);
                     et += ce ? y : 0.0;
                 } else {
                     double q0 = 0.0, q1 = 0.0, q2 = 0.0;
-                    de = pair_analytic(s_row[wv][tj], P, d2, q0, q1, q2);
+                    de = pair_analytic(rowp[tj], P, d2, q0, q1, q2);
                     eel += ce ? q0 : 0.0;
                     evd += ce ? q1 : 0.0;
                     esr += ce ? q2 : 0.0;
@@ -161,22 +201,23 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchParams B) {
                 fz += de * dz;
             }
         }
-        // grids: lanes 0..7 take the corners (staticf1.f90:45-118, vdwgd1trln.f90:50-161)
+        // grids: lanes 0..7 of the group take the corners (staticf1.f90:45-118,
+        // vdwgd1trln.f90:50-161)
         double egx = 0.0, egv = 0.0;
         bool warned = false;
-        if constexpr ((FEAT & F_PHIX) != 0) {
+        if constexpr ((FEAT & BF_PHIX) != 0) {
             const GridGeom &g = P.gx;
-            const float *phix = B.phix[r];
             const bool ok = pa.x <= g.xbcen + g.tranx && pa.x >= g.xbcen - g.tranx &&
                             pa.y <= g.ybcen + g.trany && pa.y >= g.ybcen - g.trany &&
                             pa.z <= g.zbcen + g.tranz && pa.z >= g.zbcen - g.tranz;
-            if (ok && lane < 8) {
+            if (ok && sl < 8) {
+                const float *phix = B.phix[r];
                 double gx_, gy_, gz_;
                 int ix, iy, iz;
                 grid_cell(g, pa.x, pa.y, pa.z, gx_, gy_, gz_, ix, iy, iz);
                 const double chi = P.cg[itype];
                 const double one1 = 1.0;
-                const int n1 = ix + (lane >> 2), n2 = iy + ((lane >> 1) & 1), n3 = iz + (lane & 1);
+                const int n1 = ix + (sl >> 2), n2 = iy + ((sl >> 1) & 1), n3 = iz + (sl & 1);
                 double ai = gx_ - n1 * g.dcel;
                 const double aisign = fsign(one1, ai);
                 ai = one1 - fabs(ai) * g.idcel;
@@ -195,19 +236,19 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchParams B) {
                 egx += fi * chi * ph * P.celec;
             }
         }
-        if constexpr ((FEAT & F_PHIV) != 0) {
+        if constexpr ((FEAT & BF_PHIV) != 0) {
             const GridGeom &g = P.gv;
             const bool ok = pa.x <= g.xbcen + g.tranx && pa.x >= g.xbcen - g.tranx &&
                             pa.y <= g.ybcen + g.trany && pa.y >= g.ybcen - g.trany &&
                             pa.z <= P.vzmax && pa.z >= P.vzmin;
             double ph = 0.0;
-            if (ok && lane < 8) {
+            if (ok && sl < 8) {
                 double gx_, gy_, gz_;
                 int ix, iy, iz;
                 grid_cell(g, pa.x, pa.y, pa.z, gx_, gy_, gz_, ix, iy, iz);
                 double esvdw = P.svdw;
                 if (P.flags & QSVDW) esvdw = P.svdw * P.scal[t2];
-                ph = tri_corner(g, P.phiv, gx_, gy_, gz_, ix, iy, iz, lane, esvdw, fx, fy, fz, egv);
+                ph = tri_corner(g, P.phiv, gx_, gy_, gz_, ix, iy, iz, sl, esvdw, fx, fy, fz, egv);
             }
             double sum = ph;
             sum += __shfl_xor(sum, 1, WAVE);
@@ -215,18 +256,22 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchParams B) {
             sum += __shfl_xor(sum, 4, WAVE);
             warned = ok && sum >= P.thold8;
         }
-        fx = wave_sum(fx);
-        fy = wave_sum(fy);
-        fz = wave_sum(fz);
+        // Gaussians of the update: component sl on lanes 0..2 of the group
+        double gk = 0.0;
+        if (B.mode == 1 && sl < 3) gk = philox_gauss1(seed, step, (uint64_t)r, t, sl);
+        fx = group_sum<G>(fx);
+        fy = group_sum<G>(fy);
+        fz = group_sum<G>(fz);
         if (B.write_energy) {
-            et = wave_sum(et);
-            eel = wave_sum(eel);
-            evd = wave_sum(evd);
-            esr = wave_sum(esr);
-            egx = wave_sum(egx);
-            egv = wave_sum(egv);
+            et = group_sum<G>(et);
+            eel = group_sum<G>(eel);
+            evd = group_sum<G>(evd);
+            esr = group_sum<G>(esr);
+            egx = group_sum<G>(egx);
+            egv = group_sum<G>(egv);
         }
-        if (lane == 0) {
+        const double g1 = __shfl(gk, sub * G + 1, WAVE), g2 = __shfl(gk, sub * G + 2, WAVE);
+        if (sl == 0) {
             const int64_t i = off + t;
             if (warned) atomicAdd(&B.warn[(size_t)r * B.warn_stride + t2], 1);
             if (B.write_energy) {
@@ -245,8 +290,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchParams B) {
                 B.fz[i] = fz;
             }
             if (B.mode == 1) {
-                double g[3];
-                philox_gauss3(seed, step, (uint64_t)r, t, g);
+                const double g[3] = {gk, g1, g2};
                 const double F[3] = {fx, fy, fz};
                 B.poso[i] = bd_move(P, false, itype, pa, F, g,
                                     B.nfwd + (size_t)r * B.flux_stride,
@@ -256,20 +300,39 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchParams B) {
     }
 }
 
-hipError_t launch_batch_step(const BatchParams &b, int blocks, hipStream_t s) {
+template <int FEAT>
+static void launch_batch_feat(const BatchParams &q, int lanes, dim3 grid, size_t lds,
+                              hipStream_t s) {
+    if (lanes == 16)
+        hipLaunchKernelGGL((k_batch_step<FEAT, 16>), grid, dim3(BLOCK), lds, s, q);
+    else if (lanes == 32)
+        hipLaunchKernelGGL((k_batch_step<FEAT, 32>), grid, dim3(BLOCK), lds, s, q);
+    else
+        hipLaunchKernelGGL((k_batch_step<FEAT, 64>), grid, dim3(BLOCK), lds, s, q);
+}
+
+hipError_t launch_batch_step(const BatchParams &b, int blocks, int lanes, hipStream_t s) {
     int feat = 0;
-    if (b.sp.flags & QPHIX) feat |= F_PHIX;
-    if (b.sp.flags & QPHIV) feat |= F_PHIV;
+    if (b.sp.flags & QPHIX) feat |= BF_PHIX;
+    if (b.sp.flags & QPHIV) feat |= BF_PHIV;
+    if (!b.sp.all_efpot) feat |= BF_ANALYTIC;
+    if (lanes != 16 && lanes != 32) lanes = 64;
     BatchParams q = b;
     q.persistent = blocks > 0 ? 1 : 0;
-    dim3 grid(blocks > 0 ? blocks : std::max(1, (b.sp.ntargets + WPB - 1) / WPB));
+    const int per_block = WPB * (WAVE / lanes);
+    dim3 grid(blocks > 0 ? blocks : std::max(1, (b.sp.ntargets + per_block - 1) / per_block));
+    const int nt1 = b.sp.nttyp + 1;
+    size_t lds = (size_t)nt1 * nt1 * (sizeof(DevPair) + sizeof(double4));
+    if (b.nrep + 1 <= BATCH_LDS_WSTART) lds += sizeof(int32_t) * (b.nrep + 1);
     switch (feat) {
-        case 0: hipLaunchKernelGGL((k_batch_step<0>), grid, dim3(BLOCK), 0, s, q); break;
-        case F_PHIX: hipLaunchKernelGGL((k_batch_step<F_PHIX>), grid, dim3(BLOCK), 0, s, q); break;
-        case F_PHIV: hipLaunchKernelGGL((k_batch_step<F_PHIV>), grid, dim3(BLOCK), 0, s, q); break;
-        default:
-            hipLaunchKernelGGL((k_batch_step<F_PHIX | F_PHIV>), grid, dim3(BLOCK), 0, s, q);
-            break;
+        case 0: launch_batch_feat<0>(q, lanes, grid, lds, s); break;
+        case 1: launch_batch_feat<1>(q, lanes, grid, lds, s); break;
+        case 2: launch_batch_feat<2>(q, lanes, grid, lds, s); break;
+        case 3: launch_batch_feat<3>(q, lanes, grid, lds, s); break;
+        case 4: launch_batch_feat<4>(q, lanes, grid, lds, s); break;
+        case 5: launch_batch_feat<5>(q, lanes, grid, lds, s); break;
+        case 6: launch_batch_feat<6>(q, lanes, grid, lds, s); break;
+        default: launch_batch_feat<7>(q, lanes, grid, lds, s); break;
     }
     return hipGetLastError();
 }
@@ -543,11 +606,12 @@ struct bromoc_batch {
     int cur = 0;
     int32_t *warn_scratch = nullptr;
     double *ered = nullptr;
-    int blocks = -1;   // autotuned launch shape (0: one item per wave, no queue)
+    int blocks = -1;   // autotuned launch shape (0: one item per lane group, no queue)
+    int lanes = 64;    // lanes per target
     int64_t h_ctl[8] = {};
     struct Graph {
         hipGraphExec_t exec = nullptr;
-        int len = -1, gcmc = -1, ngcmc = -1, blocks = -1;
+        int len = -1, gcmc = -1, ngcmc = -1, blocks = -1, lanes = -1;
     } graph[2];
 };
 
@@ -591,7 +655,7 @@ void batch_energies(const double *r, bromoc_energies *out) {
 hipError_t enqueue_batch_step(bromoc_batch *b, BatchParams p, hipStream_t s) {
     hipError_t e = launch_batch_prep(p, 0, s);
     if (e != hipSuccess) return e;
-    return launch_batch_step(p, b->blocks, s);
+    return launch_batch_step(p, b->blocks, b->lanes, s);
 }
 
 hipError_t enqueue_batch_gcmc(const BatchParams &p, hipStream_t s) {
@@ -606,44 +670,52 @@ int batch_sync(bromoc_batch *b) {
     return BROMOC_OK;
 }
 
-// Autotuner: persistent queue with 2, 4 or 8 workgroups per CU, or one wavefront per work
-// item; timed on ENERGY-only launches whose warnings go to a scratch counter.  The
-// environment variable BROMOC_BATCH_BLOCKS forces a shape (0: one item per wavefront).
+// Autotuner: 16, 32 or 64 lanes per target, and a persistent queue with 2 or 8
+// workgroups per CU or one work item per lane group; timed on ENERGY-only launches whose
+// warnings go to a scratch counter.  BROMOC_BATCH_BLOCKS and BROMOC_BATCH_LANES force a
+// shape (blocks 0: one item per lane group).
 int batch_autotune(bromoc_batch *b) {
     if (b->blocks >= 0) return BROMOC_OK;
-    if (const char *env = std::getenv("BROMOC_BATCH_BLOCKS")) {   // forces a launch shape
-        b->blocks = std::max(0, std::atoi(env));
+    const char *eb = std::getenv("BROMOC_BATCH_BLOCKS"), *el = std::getenv("BROMOC_BATCH_LANES");
+    if (eb || el) {
+        b->blocks = eb ? std::max(0, std::atoi(eb)) : 0;
+        b->lanes = el ? std::atoi(el) : 64;
+        if (b->lanes != 16 && b->lanes != 32) b->lanes = 64;
         return BROMOC_OK;
     }
     hipDeviceProp_t prop;
     BCHK(b, hipGetDeviceProperties(&prop, b->device));
     const int ncu = prop.multiProcessorCount;
-    const int cand[4] = {0, 2 * ncu, 4 * ncu, 8 * ncu};
+    const int cb[3] = {0, 2 * ncu, 8 * ncu}, cl[3] = {16, 32, 64};
     hipEvent_t e0, e1;
     BCHK(b, hipEventCreate(&e0));
     BCHK(b, hipEventCreate(&e1));
     double best = 1e300;
-    int pick = 4 * ncu;
-    for (int c : cand) {
-        b->blocks = c;
-        BatchParams p = params_now(b);
-        p.mode = 0;
-        p.write_energy = 0;
-        p.write_forces = 0;
-        p.warn = b->warn_scratch;
-        for (int r = 0; r < 2; r++) BCHK(b, enqueue_batch_step(b, p, b->stream));
-        BCHK(b, hipEventRecord(e0, b->stream));
-        for (int r = 0; r < 4; r++) BCHK(b, enqueue_batch_step(b, p, b->stream));
-        BCHK(b, hipEventRecord(e1, b->stream));
-        int st = batch_sync(b);
-        if (st) return st;
-        float ms = 0.f;
-        BCHK(b, hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; pick = c; }
+    int pick_b = 0, pick_l = 64;
+    for (int l : cl) {
+        for (int c : cb) {
+            b->blocks = c;
+            b->lanes = l;
+            BatchParams p = params_now(b);
+            p.mode = 0;
+            p.write_energy = 0;
+            p.write_forces = 0;
+            p.warn = b->warn_scratch;
+            for (int r = 0; r < 2; r++) BCHK(b, enqueue_batch_step(b, p, b->stream));
+            BCHK(b, hipEventRecord(e0, b->stream));
+            for (int r = 0; r < 4; r++) BCHK(b, enqueue_batch_step(b, p, b->stream));
+            BCHK(b, hipEventRecord(e1, b->stream));
+            int st = batch_sync(b);
+            if (st) return st;
+            float ms = 0.f;
+            BCHK(b, hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; pick_b = c; pick_l = l; }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    b->blocks = pick;
+    b->blocks = pick_b;
+    b->lanes = pick_l;
     return BROMOC_OK;
 }
 
@@ -922,7 +994,8 @@ int bromoc_batch_run(bromoc_batch *b, int32_t nsteps, uint64_t seed, uint64_t co
         }
         bromoc_batch::Graph &g = b->graph[b->cur];
         const int gflag = with_gcmc ? 1 : 0;
-        if (!g.exec || g.len != len || g.gcmc != gflag || g.ngcmc != ngcmc || g.blocks != b->blocks) {
+        if (!g.exec || g.len != len || g.gcmc != gflag || g.ngcmc != ngcmc ||
+            g.blocks != b->blocks || g.lanes != b->lanes) {
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
             g.exec = nullptr;
             const int cur0 = b->cur;
@@ -937,6 +1010,7 @@ int bromoc_batch_run(bromoc_batch *b, int32_t nsteps, uint64_t seed, uint64_t co
             (void)hipGraphDestroy(graph);
             BCHK(b, ie);
             g.len = len; g.gcmc = gflag; g.ngcmc = ngcmc; g.blocks = b->blocks;
+            g.lanes = b->lanes;
         }
         BCHK(b, hipGraphLaunch(g.exec, b->stream));
         if (len & 1) b->cur ^= 1;
@@ -1053,7 +1127,7 @@ int bromoc_batch_time_steps(bromoc_batch *b, int32_t nsteps, uint64_t seed, uint
         p.mode = 1;
         BCHK(b, launch_batch_prep(p, 0, b->stream));
         BCHK(b, hipEventRecord(ev[2 * k], b->stream));
-        BCHK(b, launch_batch_step(p, b->blocks, b->stream));
+        BCHK(b, launch_batch_step(p, b->blocks, b->lanes, b->stream));
         BCHK(b, hipEventRecord(ev[2 * k + 1], b->stream));
         b->cur ^= 1;
     }
@@ -1073,12 +1147,13 @@ int bromoc_batch_time_steps(bromoc_batch *b, int32_t nsteps, uint64_t seed, uint
     return BROMOC_OK;
 }
 
-int bromoc_batch_get_tuning(bromoc_batch *b, int32_t *blocks) {
-    if (!b || !blocks) return BROMOC_E_ARG;
+int bromoc_batch_get_tuning(bromoc_batch *b, int32_t *blocks, int32_t *lanes) {
+    if (!b || !blocks || !lanes) return BROMOC_E_ARG;
     (void)hipSetDevice(b->device);
     int st = batch_autotune(b);
     if (st) return st;
     *blocks = b->blocks;
+    *lanes = b->lanes;
     return BROMOC_OK;
 }
 

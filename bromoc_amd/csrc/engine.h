@@ -194,7 +194,7 @@ struct BatchParams {
                                 //      cycle0, spare
 };
 hipError_t launch_batch_prep(const BatchParams &b, int gcmc, hipStream_t s);
-hipError_t launch_batch_step(const BatchParams &b, int blocks, hipStream_t s);
+hipError_t launch_batch_step(const BatchParams &b, int blocks, int lanes, hipStream_t s);
 hipError_t launch_batch_gcmc(const BatchParams &b, hipStream_t s);
 hipError_t launch_batch_reduce(const BatchParams &b, double *out, hipStream_t s);
 

@@ -279,6 +279,8 @@ class Batch:
         return tot.value, ker.value
 
     def tuning(self):
-        v = C.c_int32()
-        self._chk("bromoc_batch_get_tuning", self.lib.bromoc_batch_get_tuning(self.ctx, C.byref(v)))
-        return v.value
+        """(workgroups of the persistent step kernel or 0, lanes per target)"""
+        v, l = C.c_int32(), C.c_int32()
+        self._chk("bromoc_batch_get_tuning",
+                  self.lib.bromoc_batch_get_tuning(self.ctx, C.byref(v), C.byref(l)))
+        return v.value, l.value

@@ -334,9 +334,9 @@ int bromoc_batch_get_prob(bromoc_batch *b, int32_t r, int32_t *prob);
  * around each step kernel; mean step-kernel duration in *kernel_ms. */
 int bromoc_batch_time_steps(bromoc_batch *b, int32_t nsteps, uint64_t seed, uint64_t counter0,
                             double *total_ms, double *kernel_ms);
-/* Launch shape the batch autotuner chose: waves per workgroup-slot count of the persistent
- * step kernel (blocks), 0 for one workgroup per four work items. */
-int bromoc_batch_get_tuning(bromoc_batch *b, int32_t *blocks);
+/* Launch shape the batch autotuner chose: workgroups of the persistent step kernel (0: one
+ * work item per lane group, no queue) and lanes per target (16, 32 or 64). */
+int bromoc_batch_get_tuning(bromoc_batch *b, int32_t *blocks, int32_t *lanes);
 
 #ifdef __cplusplus
 }

@@ -42,14 +42,16 @@ def test_batch_energy_matches_single_systems():
         assert force_err(b.forces(r)[rep.nfix:], o.forces()[rep.nfix:]) <= TOL, r
 
 
-@pytest.mark.parametrize("blocks", ["auto", "0", "512"])
-def test_batch_steps_match_single_systems(blocks, monkeypatch):
-    if blocks != "auto":
+@pytest.mark.parametrize("shape", ["auto", "0/64", "512/64", "0/16", "256/32"])
+def test_batch_steps_match_single_systems(shape, monkeypatch):
+    if shape != "auto":
+        blocks, lanes = shape.split("/")
         monkeypatch.setenv("BROMOC_BATCH_BLOCKS", blocks)
+        monkeypatch.setenv("BROMOC_BATCH_LANES", lanes)
     sh, reps = small_batch(seed=3)
     reps[2].nfix = 4
     b = Batch(sh, reps)
-    print("batch launch shape:", b.tuning())
+    print("batch launch shape (blocks, lanes):", b.tuning())
     nsteps, seed, c0 = 40, 17, 100
     eb = b.run(nsteps, seed=seed, counter0=c0, energies=True)
     for r, rep in enumerate(reps):
