@@ -23,6 +23,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s + % HBM roofline, batched vs plain, FP64 rel err<=1e-10"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the step kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (KiB per dispatch, one counter per run) on this workload, scaled by what the same counters
+# read for a 256 MiB device copy (bench.py --copy-probe 256): FETCH_SIZE x 2.0, WRITE_SIZE
+# x 1.0.  The counters cannot be read from inside this process.
+TRAFFIC_KIB = {"pore_dna": (3069.0, 144.5)}
+TRAFFIC_SCALE = (2.0, 1.0)
 
 
 def step_bytes(s, injected):
@@ -207,6 +213,10 @@ def main():
     _, kernel_ms = eng.time_steps(nk, abi.NOISE_PHILOX, seed=2024,
                                   counter0=args.warmup + args.steps)
     bpp = step_bytes(s, injected=False)
+    traffic = None
+    if args.workload in TRAFFIC_KIB:
+        fk, wk = TRAFFIC_KIB[args.workload]
+        traffic = 1024.0 * (fk * TRAFFIC_SCALE[0] + wk * TRAFFIC_SCALE[1])
     achieved = bpp * nmov / (kernel_ms * 1e-3) / 1e9
     # FP64 work of the kernel: ~34 flops per table pair (r^2 8, sqrt 1, index 2, U 4,
     # F/r 3, energy 1, force update ~15), every ordered pair of a target with its partners
@@ -223,7 +233,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded positions, grids and EFPOT tables)", "config": wl,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_step", "kernel_ms": kernel_ms, "bytes_per_particle_step": bpp,
                      "fp64_valu": {"achieved_tflops": gflops / 1e3, "peak_tflops": 78.6,
                                    "frac": gflops / 1e3 / 78.6, "pair_evals_per_step": pairs},
