@@ -631,3 +631,87 @@ def replica_batch(seed=0, nrep=256, n_range=(100, 4000), box=(96.0, 96.0, 192.0)
             rep.gcmc, rep.ibuffer, rep.nat = g, ib[:n].copy(), nat
         reps.append(rep)
     return shared, reps
+
+
+# ------------------------------------------------------- adjoint (differentiation) system
+PARAM_PAIRS = ((1, 1), (1, 2), (2, 2), (1, 3))   # K-K, K-Cl, Cl-Cl, K-W
+
+
+def adjoint_system(seed=0, n_per_type=200, box=60.0, temp=300.0, cdie=80.0, dt=0.02,
+                   curve_seed=None, table_builder=None, w_pos=(0.0, 0.0, 0.0)):
+    """Plain K+/Cl- ions in a cubic box plus one fixed uncharged wall site W (nfix = 1, the
+    first particle).  Six pair types, all tabulated on r = 1.5..20 A at 0.01 A from 12-6 LJ
+    (sigma, eps drawn from curve_seed) + Coulomb + a damped oscillatory tail.  The knot
+    values of K-K, K-Cl, Cl-Cl and K-W are the fitting parameters:
+    meta['params'][is] = (u, chr, fct) of pair type is, meta['rgrid'] the knots' r."""
+    rng = np.random.default_rng(seed)
+    crng = np.random.default_rng(seed if curve_seed is None else curve_seed)
+    charges = (1.0, -1.0, 0.0)
+    diff = (0.196, 0.203, 0.0)
+    ntypes = 3
+    sig = crng.uniform(3.0, 4.5, ntypes)
+    eps = crng.uniform(0.1, 0.5, ntypes)
+    pidx = nindex_table(ntypes, 0, ntypes)
+    npair = ntypes * (ntypes + 1) // 2
+    spec = [None] * (npair + 1)
+    for a in range(1, ntypes + 1):
+        for b in range(a, ntypes + 1):
+            spec[pidx[a * (ntypes + 1) + b]] = dict(
+                sig=0.5 * (sig[a - 1] + sig[b - 1]), eps=math.sqrt(eps[a - 1] * eps[b - 1]),
+                qq=charges[a - 1] * charges[b - 1], cdie=cdie)
+    pairs, ep = build_tables(spec, table_builder)
+    r = knot_grid()
+    params = {}
+    for a, b in PARAM_PAIRS:
+        is_ = int(pidx[a * (ntypes + 1) + b])
+        sp = spec[is_]
+        u = efpot_curve(r, sp["sig"], sp["eps"], sp["qq"], sp["cdie"], sp.get("osc", 0.3))
+        chr_ = 1 if sp["qq"] != 0.0 else 0
+        params[is_] = (u, chr_, CELEC * sp["qq"] / sp["cdie"] if chr_ else 0.0)
+    half = 0.5 * box
+    wp = np.asarray(w_pos, float)
+    pos = _place(rng, 2 * n_per_type, np.full(3, -half), np.full(3, half), 2.0,
+                 accept=lambda p: np.sum((p - wp) ** 2) > 9.0)
+    typei = np.repeat(np.arange(1, 3, dtype=np.int32), n_per_type)
+    order = rng.permutation(len(typei))
+    pos, typei = pos[order], typei[order]
+    pos = np.concatenate([wp[None, :], pos])
+    typei = np.concatenate([np.array([3], np.int32), typei])
+    kbt = kbt_of(temp)
+    kbtdt = dt * (1.0 / kbt)
+    d = np.array((0.0,) + diff)
+    s = System(
+        ntot=len(typei), nsites=0, nfix=1, ntype=ntypes, nold=0, nttyp=ntypes,
+        flags=abi.QPAR | abi.QNONBOND | abi.QNOBOND,
+        x=pos[:, 0].copy(), y=pos[:, 1].copy(), z=pos[:, 2].copy(), typei=typei,
+        nwtype=np.arange(0, ntypes + 1, dtype=np.int32),
+        cg=np.array((0.0,) + charges), fact1a=d * kbtdt, fact2a=np.sqrt(2.0 * dt * d),
+        pairidx=pidx, pairs=pairs, ep=ep, res=RES, capacity=len(typei), bdmax=1.0,
+        box=(-half, half, -half, half, -half, half), ndna=0, nion=ntypes,
+        dt=dt, kbtdt=kbtdt, diffusion=d.copy(),
+        meta=dict(workload="adjoint", seed=seed, temp=temp, dt=dt, kbt=kbt, cdie=cdie,
+                  params=params, rgrid=r),
+    )
+    return s
+
+
+def with_knots(s: System, new_u, table_builder=None):
+    """Copy of s whose parameter pair types `is` (keys of new_u) take the knot values
+    new_u[is]; their tables are rebuilt in place in ep."""
+    tb = table_builder or lib_table_builder
+    t = s.copy()
+    r = t.meta["rgrid"]
+    n = len(r)
+    ep = t.ep.copy()
+    for is_, u in new_u.items():
+        _, chr_, fct = t.meta["params"][is_]
+        e, p = tb(n, r, np.ascontiguousarray(u, float), chr_, fct)
+        off = t.pairs[is_]["nxi"]
+        ep[3 * off:3 * (off + n)] = e
+        d = _pair_dict(p)
+        d["nxi"] = off
+        d["efpot"] = 1
+        t.pairs[is_] = d
+        t.meta["params"][is_] = (np.asarray(u, float).copy(), chr_, fct)
+    t.ep = ep
+    return t

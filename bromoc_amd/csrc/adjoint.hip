@@ -1,0 +1,239 @@
+// adjoint.hip -- reverse-mode derivative of a Brownian-dynamics trajectory with respect to
+// the tabulated pair potentials, for a loss on the trajectory's smoothed radial
+// distribution functions.  No counterpart in the reference (its potentials are fitted by
+// the separate inverse-Monte-Carlo program); the forward model is the reference's BD step
+// (ENERGY pair forces through getyd, polynom.f90:197-230; DYNAMICS0, dynamics0.mono.f90:
+// 31-46 with its bdmax clamp; FIXCOOR's box clamp, fixcoor.f90:49-64) with injected noise.
+//
+// Step map x' = fixcoor(x + clamp(F(x) f1) + f2 xi).  With D_box, D_bd the 0/1 masks of
+// the components the two clamps leave alone, the adjoint of one step is
+//   mu      = f1 * D_bd * D_box * lambda'
+//   lambda  = D_box * lambda' + H(x) mu + dl/dx           (H = dF/dx, symmetric)
+//   dL/dth += (dF/dth)^T mu
+// For a pair with separation d = x_i - x_j, r = |d|, F_i = phi(r) d, so
+//   (H mu)_i   = sum_j phi (mu_i - mu_j) + (phi'/r) ((mu_i - mu_j).d) d
+//   dL/dth    += sum_pairs (dphi/dth) (mu_i - mu_j).d
+// with phi = -ep2/r - 2 ep3 inside the table (knot k), 12 sc1/r^14 below it and
+// 6 sc3/r^8 + fct/r^3 above it.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace bromoc {
+
+// region of a table pair at squared distance d2 (getyd, polynom.f90:197-230): phi and
+// dphi/dr, and which parameters it depends on (kind 0 wall: sc1, 1 table: knot k, 2 tail)
+__device__ __forceinline__ void pair_phi(const DevPair &p, const double4 *__restrict__ ep,
+                                         double ires, double d2, double r, double &phi,
+                                         double &dphi, int &kind, int &k) {
+    const double ir = 1.0 / r, ix2 = ir * ir;
+    if (d2 < p.dm2lo) {
+        const double c = p.sc1 * pow6(ix2);
+        phi = 12.0 * c * ix2;
+        dphi = -14.0 * phi * ir;
+        kind = 0;
+        k = -1;
+    } else if (d2 > p.dm2hi) {
+        const double c = p.sc3 * pow3(ix2);
+        const double f = p.fct * ir;          // zero for uncharged pairs
+        phi = (6.0 * c + f) * ix2;
+        dphi = (-48.0 * c - 3.0 * f) * ix2 * ir;
+        kind = 2;
+        k = -1;
+    } else {
+        k = (int)((r - p.dmi) * ires) + p.nxi;
+        k = min(max(k, p.nxi), p.nlast);
+        const double4 e = ep[k];
+        phi = -e.y * ir - 2.0 * e.z;
+        dphi = e.y * ix2;
+        kind = 1;
+    }
+}
+
+// smoothed RDF kernel exp(-u^2 / (2 sigma^2)) within |u| <= cut
+__device__ __forceinline__ double rdf_k(double u, double sigma) {
+    return exp(-0.5 * u * u / (sigma * sigma));
+}
+
+// k_rdf: raw smoothed histograms of the pairs each moving ion heads (a moving partner of
+// higher index, or any fixed partner), added to the ion's own row.
+__global__ __launch_bounds__(256) void k_rdf(AdjParams A) {
+    extern __shared__ double s_h[];
+    const StepParams &P = A.sp;
+    const int nf0 = P.nsites + P.nfix;
+    const int i = nf0 + blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nh = A.ngroup * A.nbins, nt1 = P.nttyp + 1;
+    for (int k = tid; k < nh; k += blockDim.x) s_h[k] = 0.0;
+    __syncthreads();
+    const double4 xi = A.x[i];
+    const int ti = __double2loint(xi.w);
+    for (int j = P.nsites + tid; j < P.ntot; j += blockDim.x) {
+        if (j == i || (j >= nf0 && j < i)) continue;
+        const double4 xj = A.x[j];
+        const int g = A.group[ti * nt1 + __double2loint(xj.w)];
+        if (g < 0) continue;
+        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const int b0 = max(0, (int)ceil((r - A.cut - A.r0) / A.dr - 0.5));
+        const int b1 = min(A.nbins - 1, (int)floor((r + A.cut - A.r0) / A.dr - 0.5));
+        for (int b = b0; b <= b1; b++) {
+            const double u = r - (A.r0 + (b + 0.5) * A.dr);
+            if (fabs(u) <= A.cut) atomicAdd(&s_h[g * A.nbins + b], rdf_k(u, A.sigma));
+        }
+    }
+    __syncthreads();
+    double *row = A.hist + (size_t)i * nh;
+    for (int k = tid; k < nh; k += blockDim.x) row[k] += s_h[k];
+}
+
+// k_adj_force: F(x_s) of each moving ion, pairs in index order
+__global__ __launch_bounds__(256) void k_adj_force(AdjParams A) {
+    __shared__ double s_r[3][4];
+    const StepParams &P = A.sp;
+    const int i = P.nsites + P.nfix + blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nt1 = P.nttyp + 1;
+    const double4 xi = A.x[i];
+    const int ti = __double2loint(xi.w);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int j = tid; j < P.ntot; j += blockDim.x) {
+        if (j == i) continue;
+        const double4 xj = A.x[j];
+        const DevPair &p = P.rows[ti * nt1 + __double2loint(xj.w)];
+        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        double y, yp;
+        getyd(p, P.ep, P.ires, d2, y, yp);
+        fx += yp * dx;
+        fy += yp * dy;
+        fz += yp * dz;
+    }
+    fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+    if (lane == 0) { s_r[0][wv] = fx; s_r[1][wv] = fy; s_r[2][wv] = fz; }
+    __syncthreads();
+    if (tid == 0)
+        A.f[i] = make_double4(s_r[0][0] + s_r[0][1] + s_r[0][2] + s_r[0][3],
+                              s_r[1][0] + s_r[1][1] + s_r[1][2] + s_r[1][3],
+                              s_r[2][0] + s_r[2][1] + s_r[2][2] + s_r[2][3], 0.0);
+}
+
+// k_adj_mu: the clamp masks of DYNAMICS0 + FIXCOOR at x_s and mu = f1 D_bd D_box lambda'
+__global__ void k_adj_mu(AdjParams A) {
+    const StepParams &P = A.sp;
+    const int i = P.nsites + P.nfix + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.ntot) return;
+    const int itype = abs(P.typei[i]);
+    const double f1 = P.fact1a[itype], f2 = P.fact2a[itype];
+    const double4 x = A.x[i], F = A.f[i], l = A.lam1[i];
+    const double xv[3] = {x.x, x.y, x.z}, fv[3] = {F.x, F.y, F.z}, lv[3] = {l.x, l.y, l.z};
+    const double lo[3] = {P.lx2m, P.ly2m, P.lz2m}, hi[3] = {P.lx2p, P.ly2p, P.lz2p};
+    double m[3], b[3];
+    for (int c = 0; c < 3; c++) {
+        double del = fv[c] * f1;
+        const bool free_bd = !(fabs(del) > P.bdmax);
+        if (!free_bd) del = fsign(P.bdmax, del);
+        const double xt = xv[c] + del + f2 * A.xi[3 * (size_t)i + c];
+        const bool free_box = xt >= lo[c] && xt <= hi[c];
+        b[c] = free_box ? 1.0 : 0.0;
+        m[c] = (free_bd && free_box) ? f1 * lv[c] : 0.0;
+    }
+    A.mu[i] = make_double4(m[0], m[1], m[2], 0.0);
+    A.dbox[i] = make_double4(b[0], b[1], b[2], 0.0);
+}
+
+// k_adj_pairs: lambda_s of each moving ion (Hessian-vector product, box mask, RDF-loss
+// term) and the parameter gradient of the pairs it heads (a moving partner of lower index,
+// or a fixed partner), one block per moving ion
+__global__ __launch_bounds__(256) void k_adj_pairs(AdjParams A) {
+    __shared__ double s_r[6][4];
+    const StepParams &P = A.sp;
+    const int nf0 = P.nsites + P.nfix;
+    const int i = nf0 + blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nt1 = P.nttyp + 1;
+    const double4 xi = A.x[i], mi = A.mu[i];
+    const int ti = __double2loint(xi.w);
+    double hx = 0.0, hy = 0.0, hz = 0.0, ox = 0.0, oy = 0.0, oz = 0.0;
+    const double is2 = 1.0 / (A.sigma * A.sigma);
+    for (int j = tid; j < P.ntot; j += blockDim.x) {
+        if (j == i) continue;
+        const double4 xj = A.x[j];
+        const int tj = __double2loint(xj.w);
+        const bool jmov = j >= nf0;
+        const double4 mj = jmov ? A.mu[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const double r = sqrt(d2);
+        const DevPair &p = P.rows[ti * nt1 + tj];
+        double phi, dphi;
+        int kind, k;
+        pair_phi(p, P.ep, P.ires, d2, r, phi, dphi, kind, k);
+        const double ux = mi.x - mj.x, uy = mi.y - mj.y, uz = mi.z - mj.z;
+        const double s = ux * dx + uy * dy + uz * dz;
+        const double q = dphi / r * s;
+        hx += phi * ux + q * dx;
+        hy += phi * uy + q * dy;
+        hz += phi * uz + q * dz;
+        // parameter gradient, once per pair
+        if (!jmov || j < i) {
+            if (kind == 1) {
+                atomicAdd(&A.gep[2 * (size_t)k], -s / r);
+                atomicAdd(&A.gep[2 * (size_t)k + 1], -2.0 * s);
+            } else {
+                const int is = P.pairidx[ti * nt1 + tj];
+                const double ix2 = 1.0 / d2;
+                if (kind == 0) atomicAdd(&A.gsc[2 * is], 12.0 * pow6(ix2) * ix2 * s);
+                else atomicAdd(&A.gsc[2 * is + 1], 6.0 * pow3(ix2) * ix2 * s);
+            }
+        }
+        // RDF-loss term: d/dx_i of the group's smoothed histogram, weighted
+        if (A.obs) {
+            const int g = A.group[ti * nt1 + tj];
+            if (g >= 0) {
+                const int b0 = max(0, (int)ceil((r - A.cut - A.r0) / A.dr - 0.5));
+                const int b1 = min(A.nbins - 1, (int)floor((r + A.cut - A.r0) / A.dr - 0.5));
+                double om = 0.0;
+                for (int b = b0; b <= b1; b++) {
+                    const double u = r - (A.r0 + (b + 0.5) * A.dr);
+                    if (fabs(u) <= A.cut) om += A.w[g * A.nbins + b] * (-u * is2) * rdf_k(u, A.sigma);
+                }
+                const double c = om / r;
+                ox += c * dx;
+                oy += c * dy;
+                oz += c * dz;
+            }
+        }
+    }
+    double v[6] = {hx, hy, hz, ox, oy, oz};
+    for (int c = 0; c < 6; c++) {
+        v[c] = wave_sum(v[c]);
+        if (lane == 0) s_r[c][wv] = v[c];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double t[6];
+        for (int c = 0; c < 6; c++) t[c] = s_r[c][0] + s_r[c][1] + s_r[c][2] + s_r[c][3];
+        const double4 l1 = A.lam1[i], db = A.dbox[i];
+        A.lam0[i] = make_double4(db.x * l1.x + t[0] + t[3], db.y * l1.y + t[1] + t[4],
+                                 db.z * l1.z + t[2] + t[5], 0.0);
+    }
+}
+
+hipError_t launch_adj(int which, const AdjParams &a, hipStream_t s) {
+    const int nmov = a.sp.ntot - a.sp.nsites - a.sp.nfix;
+    if (nmov <= 0) return hipSuccess;
+    switch (which) {
+        case 0: {
+            const size_t lds = sizeof(double) * (size_t)a.ngroup * a.nbins;
+            hipLaunchKernelGGL(k_rdf, dim3(nmov), dim3(256), lds, s, a);
+            break;
+        }
+        case 1: hipLaunchKernelGGL(k_adj_force, dim3(nmov), dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_adj_mu, dim3((nmov + 255) / 256), dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_adj_pairs, dim3(nmov), dim3(256), 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace bromoc

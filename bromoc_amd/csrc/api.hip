@@ -1355,6 +1355,202 @@ int bromoc_spline(int32_t n, const double *x, const double *y, double *b, double
     return BROMOC_OK;
 }
 
+int bromoc_table_vjp(int32_t n, const double *r, const double *grad_ep, const double *grad_sc,
+                     double *grad_u) {
+    if (n < 2 || !r || !grad_ep || !grad_sc || !grad_u) return BROMOC_E_ARG;
+    std::vector<double> u(n, 0.0), ep(3 * (size_t)n);
+    bromoc_pair p{};
+    for (int k = 0; k < n; k++) {
+        u[k] = 1.0;
+        const int st = bromoc_build_table(n, r, u.data(), 0, 0.0, ep.data(), &p);
+        u[k] = 0.0;
+        if (st) return st;
+        double acc = p.sc1 * grad_sc[0] + p.sc2 * grad_sc[1] + p.sc3 * grad_sc[2];
+        for (int m = 0; m < 3 * n; m++) acc += ep[m] * grad_ep[m];
+        grad_u[k] = acc;
+    }
+    return BROMOC_OK;
+}
+
+// ---- trajectory adjoint of an RDF loss (kernels in adjoint.hip)
+int bromoc_adjoint_rdf(bromoc_ctx *c, int32_t nsteps, int32_t K, const bromoc_rdf *rdf,
+                       double *loss, double *rdf_out, double *grad_ep, double *grad_sc) {
+    if (!c || nsteps < 1 || K < 1 || !rdf || !loss || rdf->ngroup < 1 || rdf->nbins < 1 ||
+        rdf->ngroup * (int64_t)rdf->nbins > 4096 || !rdf->groups || !(rdf->dr > 0.0) ||
+        !(rdf->sigma > 0.0) || !(rdf->cut >= 0.0))
+        return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    const bromoc_system &S = c->sys;
+    const uint32_t allowed = BROMOC_QPAR | BROMOC_QNONBOND | BROMOC_QNOBOND | BROMOC_QCOUNTION;
+    if ((S.flags & ~allowed) || c->ns != 0 || !c->base.all_efpot)
+        return fail(c, BROMOC_E_UNSUPPORTED,
+                    "adjoint: moving ions with tabulated pairs only (no grids, CG-DNA, RFPAR, "
+                    "diffusion profiles)");
+    const int nt = c->ntot, nt1 = S.nttyp + 1, nf0 = c->nf;
+    if (!c->noise || c->noise_rows < nsteps || c->noise_stride < 3 * (int64_t)nt)
+        return fail(c, BROMOC_E_STATE, "adjoint: injected noise rows 0..nsteps-1 required");
+    const int ng = rdf->ngroup, nb = rdf->nbins, nh = ng * nb;
+    // RDF groups by atnam2 type, and each group's pair count
+    std::vector<int32_t> group((size_t)nt1 * nt1, -1);
+    std::vector<int64_t> nmov(S.ntype + 1, 0), nfix(S.ntype + 1, 0);
+    for (int i = 0; i < nt; i++) (i < nf0 ? nfix : nmov)[std::abs(c->h_typei[i])]++;
+    std::vector<double> npairs(ng, 0.0);
+    for (int g = 0; g < ng; g++) {
+        const int a = rdf->groups[2 * g], b = rdf->groups[2 * g + 1];
+        if (a <= S.nold || a > S.ntype || b <= S.nold || b > S.ntype)
+            return fail(c, BROMOC_E_ARG, "adjoint: RDF group type is not an ion type");
+        const int ta = c->h_nwtype[a], tb = c->h_nwtype[b];
+        if (group[ta * nt1 + tb] >= 0) return fail(c, BROMOC_E_ARG, "adjoint: RDF groups overlap");
+        group[ta * nt1 + tb] = group[tb * nt1 + ta] = g;
+        double pc = a == b ? 0.5 * (double)nmov[a] * (double)(nmov[a] - 1)
+                           : (double)nmov[a] * (double)nmov[b];
+        pc += (double)nmov[a] * (double)nfix[b] + (a != b ? (double)nmov[b] * (double)nfix[a] : 0.0);
+        npairs[g] = pc;
+    }
+    int st = autotune(c);
+    if (st) return st;
+    const int nseg = (nsteps + K - 1) / K;
+    // scratch of this call
+    std::vector<void *> tmp;
+    auto tfree = [&]() { for (void *q : tmp) (void)hipFree(q); tmp.clear(); };
+    auto talloc = [&](void **q, size_t bytes) -> hipError_t {
+        hipError_t e = hipMalloc(q, std::max<size_t>(bytes, 8));
+        if (e == hipSuccess) { tmp.push_back(*q); e = hipMemset(*q, 0, std::max<size_t>(bytes, 8)); }
+        return e;
+    };
+#define TCHK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) { tfree(); return fail(c, BROMOC_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); } } while (0)
+    const size_t v4 = sizeof(double4) * (size_t)nt;
+    double4 *ckpt, *traj, *xS, *F, *mu, *dbox, *lam[2];
+    double *hist, *d_w, *gep, *gsc;
+    int32_t *d_group;
+    TCHK(talloc((void **)&ckpt, v4 * nseg));
+    TCHK(talloc((void **)&traj, v4 * K));
+    TCHK(talloc((void **)&xS, v4));
+    TCHK(talloc((void **)&F, v4));
+    TCHK(talloc((void **)&mu, v4));
+    TCHK(talloc((void **)&dbox, v4));
+    TCHK(talloc((void **)&lam[0], v4));
+    TCHK(talloc((void **)&lam[1], v4));
+    TCHK(talloc((void **)&hist, sizeof(double) * (size_t)nt * nh));
+    TCHK(talloc((void **)&d_w, sizeof(double) * nh));
+    const int nk = S.nknots + 2;
+    TCHK(talloc((void **)&gep, sizeof(double) * 2 * (size_t)nk));
+    TCHK(talloc((void **)&gsc, sizeof(double) * 2 * (size_t)(S.npair + 1)));
+    TCHK(talloc((void **)&d_group, sizeof(int32_t) * group.size()));
+    TCHK(hipMemcpy(d_group, group.data(), sizeof(int32_t) * group.size(), hipMemcpyHostToDevice));
+    // injected noise row s of step s: ctl = {0, 0, ...} and step_off = s
+    c->h_ctl[0] = 0; c->h_ctl[1] = 0; c->h_ctl[2] = 0; c->h_ctl[3] = 0;
+    TCHK(hipMemcpyAsync(c->ctl, c->h_ctl, sizeof(c->h_ctl), hipMemcpyHostToDevice, c->stream));
+    auto step_params = [&](int s) {
+        StepParams p = params_for(c);
+        p.mode = 1;
+        p.write_energy = 0;
+        p.write_forces = 0;
+        p.noise_mode = BROMOC_NOISE_INJECTED;
+        p.noise = c->noise;
+        p.noise_stride = c->noise_stride;
+        p.step_off = s;
+        p.cntpts = 0;          // the trajectory is replayed: no flux counting
+        return p;
+    };
+    AdjParams A{};
+    A.sp = params_for(c);
+    A.group = d_group;
+    A.ngroup = ng; A.nbins = nb;
+    A.r0 = rdf->r0; A.dr = rdf->dr; A.sigma = rdf->sigma; A.cut = rdf->cut;
+    A.w = d_w; A.hist = hist; A.gep = gep; A.gsc = gsc;
+    A.f = F; A.mu = mu; A.dbox = dbox;
+    // ---- forward: checkpoints every K steps, smoothed RDF of every state x_1..x_nsteps
+    for (int s = 0; s < nsteps; s++) {
+        if (s % K == 0) {
+            TCHK(hipMemcpyAsync(ckpt + (size_t)(s / K) * nt, c->pos[c->cur], v4, hipMemcpyDeviceToDevice, c->stream));
+            int sst = ensure_sorted(c, true);
+            if (sst) { tfree(); return sst; }
+        }
+        TCHK(launch_step(step_params(s), c->stream));
+        c->cur ^= 1;
+        A.x = c->pos[c->cur];
+        TCHK(launch_adj(0, A, c->stream));
+    }
+    TCHK(hipMemcpyAsync(xS, c->pos[c->cur], v4, hipMemcpyDeviceToDevice, c->stream));
+    std::vector<double> H((size_t)nt * nh);
+    TCHK(hipMemcpyAsync(H.data(), hist, sizeof(double) * H.size(), hipMemcpyDeviceToHost, c->stream));
+    st = sync_err(c);
+    if (st) { tfree(); return st; }
+    const double PI_ = 3.14159265358979323846264338327950288419716939937510;
+    const double V = (S.lx2p - S.lx2m) * (S.ly2p - S.ly2m) * (S.lz2p - S.lz2m);
+    std::vector<double> g(nh, 0.0), w(nh, 0.0);
+    double L = 0.0;
+    for (int k = 0; k < nh; k++) {
+        double h = 0.0;
+        for (int i = nf0; i < nt; i++) h += H[(size_t)i * nh + k];
+        const int gi = k / nb, b = k % nb;
+        const double rb = rdf->r0 + (b + 0.5) * rdf->dr;
+        const double cn = npairs[gi] > 0.0
+            ? V / ((double)nsteps * npairs[gi] * 4.0 * PI_ * rb * rb * rdf->sigma * std::sqrt(2.0 * PI_))
+            : 0.0;
+        g[k] = cn * h;
+        const double res = g[k] - (rdf->target ? rdf->target[k] : 0.0);
+        L += res * res;
+        w[k] = 2.0 * res * cn;
+    }
+    *loss = L;
+    if (rdf_out) std::memcpy(rdf_out, g.data(), sizeof(double) * nh);
+    if (grad_ep || grad_sc) {
+        TCHK(hipMemcpyAsync(d_w, w.data(), sizeof(double) * nh, hipMemcpyHostToDevice, c->stream));
+        // lambda_nsteps = dl/dx at the last state (mu = 0: no step after it)
+        A.x = xS; A.lam1 = lam[0]; A.lam0 = lam[1]; A.obs = 1;
+        TCHK(launch_adj(3, A, c->stream));
+        int lc = 1;
+        for (int seg = nseg - 1; seg >= 0; seg--) {
+            const int s0 = seg * K, s1 = std::min(s0 + K, nsteps);
+            TCHK(hipMemcpyAsync(c->pos[c->cur], ckpt + (size_t)seg * nt, v4, hipMemcpyDeviceToDevice, c->stream));
+            int sst = ensure_sorted(c, true);
+            if (sst) { tfree(); return sst; }
+            for (int s = s0; s < s1; s++) {
+                TCHK(hipMemcpyAsync(traj + (size_t)(s - s0) * nt, c->pos[c->cur], v4, hipMemcpyDeviceToDevice, c->stream));
+                TCHK(launch_step(step_params(s), c->stream));
+                c->cur ^= 1;
+            }
+            for (int s = s1 - 1; s >= s0; s--) {
+                A.x = traj + (size_t)(s - s0) * nt;
+                A.xi = c->noise + (size_t)s * c->noise_stride;
+                A.lam1 = lam[lc];
+                A.lam0 = lam[lc ^ 1];
+                A.obs = s >= 1 ? 1 : 0;
+                TCHK(launch_adj(1, A, c->stream));
+                TCHK(launch_adj(2, A, c->stream));
+                TCHK(launch_adj(3, A, c->stream));
+                lc ^= 1;
+            }
+        }
+        std::vector<double> hg(2 * (size_t)nk), hs(2 * (size_t)(S.npair + 1));
+        TCHK(hipMemcpyAsync(hg.data(), gep, sizeof(double) * hg.size(), hipMemcpyDeviceToHost, c->stream));
+        TCHK(hipMemcpyAsync(hs.data(), gsc, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, c->stream));
+        // the last state back in place
+        TCHK(hipMemcpyAsync(c->pos[c->cur], xS, v4, hipMemcpyDeviceToDevice, c->stream));
+        st = sync_err(c);
+        if (st) { tfree(); return st; }
+        if (grad_ep)
+            for (int k = 0; k < S.nknots; k++) {
+                grad_ep[3 * k] = 0.0;
+                grad_ep[3 * k + 1] = hg[2 * k];
+                grad_ep[3 * k + 2] = hg[2 * k + 1];
+            }
+        if (grad_sc)
+            for (int t = 0; t <= S.npair; t++) {
+                grad_sc[3 * t] = hs[2 * t];
+                grad_sc[3 * t + 1] = 0.0;
+                grad_sc[3 * t + 2] = hs[2 * t + 1];
+            }
+    }
+    c->sort_dirty = true;
+    st = sync_err(c);
+    tfree();
+#undef TCHK
+    return st;
+}
+
 // raw Philox4x64-10 blocks on the device (the generator behind BROMOC_NOISE_PHILOX)
 int bromoc_philox4x64(int32_t n, const uint64_t *ctr, const uint64_t *key, uint64_t *out) {
     if (n < 0 || (n > 0 && (!ctr || !key || !out))) return BROMOC_E_ARG;

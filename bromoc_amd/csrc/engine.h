@@ -159,6 +159,29 @@ struct InteractParams {
 };
 hipError_t launch_interact(const InteractParams &p, hipStream_t s);
 
+// ---- trajectory adjoint (adjoint.hip)
+struct AdjParams {
+    StepParams sp;
+    const double4 *x;       // state x_s [ntot]
+    const double4 *lam1;    // lambda_{s+1} [ntot]
+    double4 *lam0;          // lambda_s [ntot]
+    double4 *f;             // forces F(x_s) [ntot]
+    double4 *mu;            // [ntot]
+    double4 *dbox;          // [ntot] 1/0 per component: fixcoor left it alone
+    const double *xi;       // noise row of step s, [3*ntot]
+    const int32_t *group;   // [nt1*nt1] RDF group of an atnam2 type pair (-1: none)
+    int32_t ngroup, nbins;
+    double r0, dr, sigma, cut;
+    const double *w;        // [ngroup*nbins] residual weights dL/dH
+    double *hist;           // [ntot][ngroup*nbins] per-target raw histograms
+    double *gep;            // [nknots][2]: dL/d ep2, dL/d ep3 per knot
+    double *gsc;            // [npair+1][2]: dL/d sc1, dL/d sc3 per pair type
+    int32_t obs, pad;
+};
+
+// which: 0 k_rdf, 1 k_adj_force, 2 k_adj_mu, 3 k_adj_pairs
+hipError_t launch_adj(int which, const AdjParams &a, hipStream_t s);
+
 // ---- replica batches (batch.hip)
 struct BatchParams {
     StepParams sp;              // shared tables, grid geometry, BD constants

@@ -282,6 +282,40 @@ int bromoc_spline(int32_t n, const double *x, const double *y, double *b, double
  * counterpart in the reference (its noise is libgfortran rand()). */
 int bromoc_philox4x64(int32_t n, const uint64_t *ctr, const uint64_t *key, uint64_t *out);
 
+/* ------------------------------------------------------------- trajectory adjoint
+ * Gradient of a loss on the smoothed radial distribution functions of a BD trajectory
+ * with respect to the pair tables, by a reverse pass through the steps (no counterpart in
+ * the reference, whose potentials are fitted by the separate inverse-Monte-Carlo
+ * program).  The trajectory: nsteps steps of ENERGY + DYNAMICS0 (moving ions, every pair
+ * tabulated, no grids, no CG-DNA) from the current positions with the injected noise rows
+ * 0..nsteps-1.  The RDF of group g (a pair of atnam ion types) in bin b is
+ *   g[g][b] = V / (nsteps * P_g * 4 pi r_b^2 sigma sqrt(2 pi)) * sum over the steps' states
+ *             x_1..x_nsteps and the group's pairs of exp(-(r - r_b)^2 / (2 sigma^2)),
+ * terms with |r - r_b| > cut left out, r_b = r0 + (b + 1/2) dr, V the box volume, P_g the
+ * number of the group's pairs (moving-moving, or moving-fixed); the loss is
+ * L = sum_{g,b} (g[g][b] - target[g][b])^2. */
+typedef struct bromoc_rdf {
+    int32_t ngroup, nbins;
+    const int32_t *groups;       /* [2*ngroup] atnam types (a, b) of each group */
+    double r0, dr, sigma, cut;
+    const double *target;        /* [ngroup*nbins], NULL: zero */
+} bromoc_rdf;
+
+/* Forward pass (storing the positions every checkpoint_every steps) and, when grad_ep is
+ * given, the reverse pass recomputing each segment from its checkpoint.  Outputs: *loss,
+ * rdf[ngroup*nbins] (optional), grad_ep[3*nknots] = dL/d(ep1, ep2, ep3) of every knot,
+ * grad_sc[3*(npair+1)] = dL/d(sc1, sc2, sc3) of every pair type.  Positions end at the
+ * trajectory's last state; flux counters are not updated. */
+int bromoc_adjoint_rdf(bromoc_ctx *ctx, int32_t nsteps, int32_t checkpoint_every,
+                       const bromoc_rdf *rdf, double *loss, double *rdf_out, double *grad_ep,
+                       double *grad_sc);
+
+/* Chain rule through bromoc_build_table for one pair type's table of n samples at r: the
+ * table (ep[3n], sc1..sc3) is affine in the sample values u (the charge term is the
+ * offset), so grad_u[k] = sum of grad_ep . d ep / du_k + grad_sc . d(sc1, sc2, sc3) / du_k. */
+int bromoc_table_vjp(int32_t n, const double *r, const double *grad_ep, const double *grad_sc,
+                     double *grad_u);
+
 /* ------------------------------------------------------------------ replica batches
  * Many independent ion systems (no CG-DNA, no reaction-field grids) that share one
  * bromoc_system's types, pair tables, repulsion map, box, flux planes and BD constants, and

@@ -1,0 +1,57 @@
+"""CPU restatement of the smoothed-RDF loss of a BD trajectory (TEST INFRASTRUCTURE ONLY).
+
+The trajectory comes from the C oracle or the reference routines stepping with injected
+noise; the loss follows the definition in include/bromoc_gpu.h (bromoc_rdf): for group
+(a, b) and bin b, g = V / (S P 4 pi r_b^2 sigma sqrt(2 pi)) * sum over states x_1..x_S and
+the group's pairs (moving-moving once, moving-fixed) of exp(-(r - r_b)^2 / (2 sigma^2)),
+|r - r_b| <= cut; L = sum (g - target)^2.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from bromoc_amd import abi
+
+
+def trajectory(engine, noise, nsteps):
+    """States x_1..x_nsteps of `engine` (Oracle or RefFortran) stepping with noise rows."""
+    engine.upload_noise(noise)
+    out = []
+    for s in range(nsteps):
+        engine.step(1, abi.NOISE_INJECTED, s)
+        out.append(engine.positions()[0].copy())
+    return out
+
+
+def rdf_loss(states, system, spec, target=None):
+    """(loss, g [ngroup, nbins]) of the states of a trajectory of `system`."""
+    t = np.abs(np.asarray(system.typei[:system.ntot]))
+    nf = system.nfix
+    n = system.ntot
+    mov = np.arange(n) >= nf
+    rb = spec.r0 + (np.arange(spec.nbins) + 0.5) * spec.dr
+    ng = len(spec.groups)
+    H = np.zeros((ng, spec.nbins))
+    iu, ju = np.triu_indices(n, 1)          # i < j
+    pair_ok = mov[iu] | mov[ju]             # at least one moving; fixed-fixed never
+    for x in states:
+        d = np.sqrt(np.sum((x[iu] - x[ju]) ** 2, axis=1))
+        for g, (a, b) in enumerate(spec.groups):
+            m = pair_ok & (((t[iu] == a) & (t[ju] == b)) | ((t[iu] == b) & (t[ju] == a)))
+            r = d[m]
+            u = r[:, None] - rb[None, :]
+            k = np.where(np.abs(u) <= spec.cut, np.exp(-0.5 * u * u / spec.sigma ** 2), 0.0)
+            H[g] += k.sum(0)
+    lx2m, lx2p, ly2m, ly2p, lz2m, lz2p = system.box
+    V = (lx2p - lx2m) * (ly2p - ly2m) * (lz2p - lz2m)
+    S = len(states)
+    g_out = np.zeros_like(H)
+    for g, (a, b) in enumerate(spec.groups):
+        na, nb_ = np.sum(mov & (t == a)), np.sum(mov & (t == b))
+        fa, fb = np.sum(~mov & (t == a)), np.sum(~mov & (t == b))
+        P = (0.5 * na * (na - 1) if a == b else na * nb_) + na * fb + (nb_ * fa if a != b else 0)
+        if P > 0:
+            c = V / (S * P * 4.0 * np.pi * rb ** 2 * spec.sigma * np.sqrt(2.0 * np.pi))
+            g_out[g] = c * H[g]
+    tgt = np.zeros_like(g_out) if target is None else np.asarray(target).reshape(g_out.shape)
+    return float(np.sum((g_out - tgt) ** 2)), g_out
