@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK, E_ARG, E_UNSUPPORTED, E_CAPACITY, E_HIP, E_NODEVICE, E_STATE = range(7)
 STATUS_NAMES = {0: "OK", 1: "E_ARG", 2: "E_UNSUPPORTED", 3: "E_CAPACITY", 4: "E_HIP",
@@ -82,6 +82,14 @@ class SystemStruct(C.Structure):
         ("xs", _dp), ("ys", _dp), ("bs", _dp), ("cs", _dp), ("ds", _dp),
         ("rf", Grid), ("gsrfen", C.POINTER(C.c_float)), ("greff", C.POINTER(C.c_float)),
         ("sqrfac", C.c_double),
+        ("afn", C.c_int32), ("ctn", C.c_int32), ("sn", _ip), ("af", _dp),
+        ("csn", _ip), ("kx", _dp), ("ky", _dp), ("kz", _dp),
+        ("contrx", _dp), ("contry", _dp), ("contrz", _dp),
+        ("unsplit", C.c_int32), ("nsites1st", C.c_int32),
+        ("notrans_axes", C.c_int32), ("reserved3", C.c_int32),
+        ("notrx", C.c_double), ("notry", C.c_double), ("notrz", C.c_double),
+        ("ibeta", C.c_double), ("diff0", C.c_double), ("diffcutoff", C.c_double),
+        ("thold27", C.c_double),
     ]
 
 
@@ -162,6 +170,8 @@ def _declare(lib):
         "bromoc_set_gcmc": ([vp, P(GcmcStruct)], C.c_int),
         "bromoc_grand": ([vp, C.c_int32, _dp, C.c_int64, P(C.c_int64), _ip, _ip, C.c_int64,
                           _dp], C.c_int),
+        "bromoc_metropolis": ([vp, C.c_int32, C.c_double, C.c_double, _dp, C.c_int64,
+                               P(C.c_int64), _ip, _dp], C.c_int),
         "bromoc_build_table": ([C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp, P(Pair)],
                                C.c_int),
         "bromoc_set_ibuffer": ([vp, _ip], C.c_int),

@@ -528,6 +528,89 @@ def with_profile_diffusion(s: System, zk, fk, spline_builder=None):
     return s
 
 
+def with_bspline(s: System, thold27=27.0):
+    """Repulsion map by the 27-point quadratic B-spline (Qtrln off: vdwgd1spln.f90,
+    vdwgd0spln.f90); thold27 as the SIMUL keyword (default 27, shell_simul.f90:2706)."""
+    s = s.copy()
+    s.flags |= abi.QBSPLINE
+    s.thold27 = float(thold27)
+    return s
+
+
+def with_proxdiff(s: System, beta=2.93, diff0=0.101, diffcutoff=None):
+    """PROXDIFF command (shell_simul.f90:1901-1910): ion D relaxes towards diff0 near the
+    DNA, D(d) = D - (D - diff0) exp(-d / beta) within diffcutoff of the nearest site's table
+    start (DYNAMICS3).  diffcutoff defaults to the box diagonal plus 1 A (maxl)."""
+    s = s.copy()
+    s.flags |= abi.QPROXDIFF
+    s.flags &= ~(abi.QDIFFUSE | abi.QPROFILE)
+    s.ibeta = 1.0 / beta
+    s.diff0 = float(diff0)
+    if diffcutoff is None:
+        lx, ly, lz = s.box[1] - s.box[0], s.box[3] - s.box[2], s.box[5] - s.box[4]
+        diffcutoff = math.sqrt(max(lx, ly) ** 2 + lz ** 2) + 1.0
+    s.diffcutoff = float(diffcutoff)
+    if s.diffusion is None:
+        raise ValueError("PROXDIFF needs the per-type diffusion constants")
+    return s
+
+
+def with_apfor(s: System, seed=0, n=6, fmax=2.0, repeat=True):
+    """APFOR: constant forces on n random sites (one site twice when repeat is set)."""
+    rng = np.random.default_rng(seed)
+    s = s.copy()
+    s.flags |= abi.QAPFOR
+    sn = rng.choice(s.nsites, n, replace=False).astype(np.int32)
+    if repeat and n > 1:
+        sn[-1] = sn[0]
+    s.apfor = (sn, rng.uniform(-fmax, fmax, (n, 3)))
+    return s
+
+
+def _centroid(v, lo, hi, insites, scale=1.0):
+    t = 0.0
+    for a in v[lo:hi]:          # Fortran sum, in order
+        t += a
+    return t * insites * scale
+
+
+def with_contrans(s: System, entries, unsplit=False, nsites1st=None):
+    """CONTRANS command (shell_simul.f90:724-770): entries of (site, (kx, ky, kz)) with site
+    1-based or 0 for the centroid; centres as the command sets them from the current
+    positions (the site's position, or the centroid; with unsplit each strand's)."""
+    s = s.copy()
+    s.flags |= abi.QCONTRANS
+    ns = s.nsites
+    insites = 1.0 / ns
+    n1 = ns // 2 if nsites1st is None else nsites1st
+    pos = (s.x, s.y, s.z)
+    csn = np.array([e[0] for e in entries], np.int32)
+    k = np.array([e[1] for e in entries], float).reshape(len(entries), 3)
+    centre = np.zeros((len(entries) + 1, 3))
+    for q, (site, _) in enumerate(entries):
+        for a in range(3):
+            if site == 0:
+                if unsplit:
+                    centre[q, a] = _centroid(pos[a], 0, n1, insites, 2.0)
+                    centre[-1, a] = _centroid(pos[a], n1, ns, insites, 2.0)
+                else:
+                    centre[q, a] = _centroid(pos[a], 0, ns, insites)
+            else:
+                centre[q, a] = pos[a][site - 1]
+    s.contrans = dict(csn=csn, k=k, centre=centre, unsplit=bool(unsplit), nsites1st=n1)
+    return s
+
+
+def with_notrans(s: System, axes=7):
+    """NOTRANS command (shell_simul.f90:692-712): after every step the sites' centroid is
+    put back, on the axes of the bitmask (1 x, 2 y, 4 z), where it was at the command."""
+    s = s.copy()
+    s.flags |= abi.QNOTRANS
+    insites = 1.0 / s.nsites
+    s.notrans = (axes, tuple(_centroid(v, 0, s.nsites, insites) for v in (s.x, s.y, s.z)))
+    return s
+
+
 def with_rfpar(s: System, seed=0, shape=None, dcel=None, single=False, reffac=1.5,
                wall_radius=None, zero_frac=0.0, const_reff=None):
     """Synthetic reaction-field parameter grids (RFPAR command, rfpar.f90:373-582): per

@@ -27,7 +27,8 @@ constexpr uint32_t SUPPORTED = BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BR
                                BROMOC_QPHIX | BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV |
                                BROMOC_QDEBY | BROMOC_QCOUNTION | BROMOC_QDNAFREE |
                                BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR |
-                               BROMOC_QRFPSIN;
+                               BROMOC_QRFPSIN | BROMOC_QBSPLINE | BROMOC_QPROXDIFF |
+                               BROMOC_QNOTRANS | BROMOC_QAPFOR | BROMOC_QCONTRANS;
 
 // constamod.f90:21-32 (binary64)
 double celec_value() {
@@ -274,6 +275,35 @@ int validate(const bromoc_system *s, std::string &msg) {
         if (!s->gsrfen || !s->greff) return bad("RFPAR grids missing");
         if (!(s->flags & BROMOC_QRFPSIN) && s->ntype - s->nold < 1) return bad("RFPAR ion types");
     }
+    if ((s->flags & BROMOC_QBSPLINE) && !(s->flags & BROMOC_QPHIV)) return bad("QBSPLINE needs QPHIV");
+    if ((s->flags & BROMOC_QBSPLINE) && (s->phiv.nclx < 3 || s->phiv.ncly < 3 || s->phiv.nclz < 3))
+        return bad("B-spline repulsion map needs 3 nodes per axis");
+    // shell_simul.f90:1903 (PROXDIFF needs ions and DNA)
+    if (s->flags & BROMOC_QPROXDIFF) {
+        if (!(s->flags & BROMOC_QPAR) || !(s->flags & BROMOC_QNUCL) || s->nsites < 1)
+            return bad("PROXDIFF needs ions and DNA");
+        if (!s->diffusion || !std::isfinite(s->dt) || !std::isfinite(s->kbtdt) ||
+            !std::isfinite(s->ibeta) || !std::isfinite(s->diff0))
+            return bad("PROXDIFF constants missing");
+    }
+    const uint32_t dnaopts = BROMOC_QNOTRANS | BROMOC_QAPFOR | BROMOC_QCONTRANS;
+    if ((s->flags & dnaopts) && (!(s->flags & BROMOC_QNUCL) || s->nsites < 1))
+        return bad("NOTRANS / APFOR / CONTRANS need DNA sites");
+    if (s->flags & BROMOC_QAPFOR) {
+        if (s->afn < 0 || (s->afn > 0 && (!s->sn || !s->af))) return bad("APFOR arrays");
+        for (int k = 0; k < s->afn; k++)
+            if (s->sn[k] < 0 || s->sn[k] >= s->nsites) return bad("APFOR site out of range");
+    }
+    if (s->flags & BROMOC_QCONTRANS) {
+        if (s->ctn < 0 || (s->ctn > 0 && (!s->csn || !s->kx || !s->ky || !s->kz)) ||
+            !s->contrx || !s->contry || !s->contrz)
+            return bad("CONTRANS arrays");
+        for (int k = 0; k < s->ctn; k++)
+            if (s->csn[k] < 0 || s->csn[k] > s->nsites) return bad("CONTRANS site out of range");
+        if (s->unsplit && (s->nsites1st < 0 || s->nsites1st > s->nsites))
+            return bad("CONTRANS strand split");
+    }
+    if ((s->flags & BROMOC_QNOTRANS) && (s->notrans_axes & ~7)) return bad("NOTRANS axes");
     return BROMOC_OK;
 }
 
@@ -525,7 +555,7 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
                  std::max(0, s->cntpts)));
     ACHK(dalloc(c, &c->ctl, 4));
     ACHK(dalloc(c, &c->pbuf, (size_t)GRAPH_K * 3 * cap));
-    ACHK(dalloc(c, &c->iout, 1));
+    ACHK(dalloc(c, &c->iout, 2));
     // position-dependent diffusion (DYNAMICS1 / DYNAMICS2)
     const bool dyn12 = (s->flags & (BROMOC_QDIFFUSE | BROMOC_QPROFILE)) != 0;
     const int nsp = (s->flags & BROMOC_QPROFILE) ? s->nspline : 0;
@@ -550,6 +580,31 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     ACHK(dalloc(c, &d_rfa, rfon ? cap : 0));
     ACHK(dalloc(c, &d_rfd, rfon ? 2 * (size_t)cap : 0));
     ACHK(dalloc(c, &d_frf, rfon ? cap : 0));
+    // DYNAMICS3 needs the diffusion constants even without DYNAMICS1/2
+    if (!dyn12 && (s->flags & BROMOC_QPROXDIFF)) ACHK(dupload(c, &d_diff, s->diffusion, s->ntype + 1));
+    // APFOR: the applied force of each site summed over its entries in entry order
+    double4 *d_apf = nullptr;
+    {
+        std::vector<double4> apf(std::max(ns, 1), make_double4(0.0, 0.0, 0.0, 0.0));
+        if (s->flags & BROMOC_QAPFOR)
+            for (int k = 0; k < s->afn; k++) {
+                double4 &a = apf[s->sn[k]];
+                a.x = a.x + s->af[3 * k]; a.y = a.y + s->af[3 * k + 1]; a.z = a.z + s->af[3 * k + 2];
+            }
+        ACHK(dupload(c, &d_apf, apf.data(), apf.size()));
+    }
+    // CONTRANS
+    const bool con = (s->flags & BROMOC_QCONTRANS) != 0;
+    const int ctn = con ? s->ctn : 0;
+    int32_t *d_csn = nullptr;
+    double *d_k[3] = {}, *d_c[3] = {}, *d_cen = nullptr;
+    ACHK(dupload(c, &d_csn, ctn ? s->csn : (const int32_t *)nullptr, ctn));
+    const double *ksrc[3] = {s->kx, s->ky, s->kz}, *csrc[3] = {s->contrx, s->contry, s->contrz};
+    for (int a = 0; a < 3; a++) {
+        ACHK(dupload(c, &d_k[a], ctn ? ksrc[a] : (const double *)nullptr, ctn));
+        ACHK(dupload(c, &d_c[a], con ? csrc[a] : (const double *)nullptr, con ? ctn + 1 : 0));
+    }
+    ACHK(dalloc(c, &d_cen, 9));
 #undef ACHK
 
     StepParams &p = c->base;
@@ -581,7 +636,20 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.zcont = c->zcont; p.nfwd = c->nfwd; p.nbwd = c->nbwd;
     p.step_ctr = c->ctl;
     p.stream = 0;
-    p.dmode = (s->flags & BROMOC_QDIFFUSE) ? 1 : (s->flags & BROMOC_QPROFILE) ? 2 : 0;
+    // simul1.f90:370-379: DIFFUSION, else PROFILE, else PROXDIFF
+    p.dmode = (s->flags & BROMOC_QDIFFUSE) ? 1 : (s->flags & BROMOC_QPROFILE) ? 2
+            : (s->flags & BROMOC_QPROXDIFF) ? 3 : 0;
+    p.ibeta = s->ibeta; p.diff0 = s->diff0; p.diffcutoff = s->diffcutoff;
+    p.thold27 = s->thold27;
+    p.apf = d_apf;
+    p.ctn = ctn; p.csn = d_csn;
+    p.kx = d_k[0]; p.ky = d_k[1]; p.kz = d_k[2];
+    p.contrx = d_c[0]; p.contry = d_c[1]; p.contrz = d_c[2];
+    p.unsplit = s->unsplit ? 1 : 0; p.nsites1st = s->nsites1st;
+    p.insites = ns > 0 ? 1.0 / ns : 0.0;   // insites = 1.0/nsites (shell_simul.f90:701)
+    p.cen = d_cen;
+    p.notrans_axes = (s->flags & BROMOC_QNOTRANS) ? s->notrans_axes : 0;
+    p.notrx = s->notrx; p.notry = s->notry; p.notrz = s->notrz;
     p.nspline = nsp;
     p.dt = s->dt; p.kbtdt = s->kbtdt; p.plength2 = s->plength2; p.pcenter = s->pcenter;
     p.diffusion = d_diff; p.ampl3 = d_ampl3; p.p3 = d_p3;
@@ -720,7 +788,7 @@ static hipError_t enqueue_step(bromoc_ctx *c, const StepParams &p, hipEvent_t t0
         if (t0 && (e = hipEventRecord(t0, c->stream)) != hipSuccess) return e;
         if ((e = launch_pairs(p, c->stream)) != hipSuccess) return e;
         if (t1 && (e = hipEventRecord(t1, c->stream)) != hipSuccess) return e;
-        return hipSuccess;
+        return launch_notrans(p, c->stream);
     }
     if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork, 0)) != hipSuccess) return e;
@@ -735,7 +803,8 @@ static hipError_t enqueue_step(bromoc_ctx *c, const StepParams &p, hipEvent_t t0
     if ((e = launch_dna(q, c->stream)) != hipSuccess) return e;
     if ((e = launch_pairs(q, c->stream)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_join, c->stream2)) != hipSuccess) return e;
-    return hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
+    return launch_notrans(p, c->stream);
 }
 
 // Autotuner: targets per block of k_step (1, 2 or 4) for this system, chosen by timing
@@ -795,7 +864,7 @@ int bromoc_get_tuning(bromoc_ctx *c, int32_t *targets_per_block) {
 // ev, when given, receives an event pair around every step's pair kernel.
 static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t first_row,
                      uint64_t seed, uint64_t counter0, bool want_energy,
-                     std::vector<hipEvent_t> *ev = nullptr) {
+                     std::vector<hipEvent_t> *ev = nullptr, bool notrans = true) {
     if (noise_mode != BROMOC_NOISE_INJECTED && noise_mode != BROMOC_NOISE_PHILOX)
         return fail(c, BROMOC_E_ARG, "noise_mode");
     if (noise_mode == BROMOC_NOISE_INJECTED) {
@@ -812,6 +881,7 @@ static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t 
     const bool philox = noise_mode == BROMOC_NOISE_PHILOX;
     StepParams p = params_for(c);
     p.mode = 1;
+    if (!notrans) p.flags &= ~BROMOC_QNOTRANS;   // NOTRANS belongs to the step loop
     p.noise_mode = noise_mode;
     p.noise = philox ? c->pbuf : c->noise;
     p.noise_stride = philox ? 3 * (int64_t)c->cap : c->noise_stride;
@@ -942,7 +1012,7 @@ int bromoc_dynamics(bromoc_ctx *c, int32_t noise_mode, int32_t row, uint64_t see
     (void)hipSetDevice(c->device);
     // The forces of the current positions are recomputed in the same fixed order as the
     // preceding ENERGY, so the update sees bit-identical forces.
-    int st = run_steps(c, 1, noise_mode, row, seed, counter, false);
+    int st = run_steps(c, 1, noise_mode, row, seed, counter, false, nullptr, false);
     if (st) return st;
     return sync_err(c);
 }
@@ -965,7 +1035,7 @@ int bromoc_get_warn(bromoc_ctx *c, int32_t *warn) {
 }
 
 static int interact_async(bromoc_ctx *c, double xj, double yj, double zj, int32_t jtype, int32_t j,
-                          int32_t alert) {
+                          int32_t alert, int slot = 0) {
     const bool beion = (c->sys.flags & BROMOC_QPAR) && j >= c->ns;
     if (j < 0 || j > c->ntot) return fail(c, BROMOC_E_ARG, "j out of range");
     if (beion) {
@@ -982,7 +1052,7 @@ static int interact_async(bromoc_ctx *c, double xj, double yj, double zj, int32_
     q.sp = params_for(c);
     q.xj = xj; q.yj = yj; q.zj = zj;
     q.jtype = jtype; q.j = j; q.alert = alert;
-    q.out = c->iout;
+    q.out = c->iout + slot;
     HIPCHK(c, launch_rfpar(q.sp, false, c->stream));
     HIPCHK(c, launch_interact(q, c->stream));
     return BROMOC_OK;
@@ -1157,6 +1227,76 @@ int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int6
     }
 #undef RNDM
     if (used) *used = k;
+    return BROMOC_OK;
+}
+
+// METROPOLIS, metropolis.f90:31-55, with MOVE (move.f90:32-47, rectangular box).  The
+// positions live on the device; a host copy taken at entry serves MOVE's draws and is kept
+// in step with every accepted move.  Per trial the two INTERACT calls (old position with
+// Qalert, trial position without) run back to back and are read in one copy.
+int bromoc_metropolis(bromoc_ctx *c, int32_t nmcm, double mcmax, double ikbt,
+                      const double *u, int64_t nu, int64_t *used, int32_t *done, double *ener) {
+    if (!c || (!u && nu > 0) || nu < 0 || !ener || nmcm < 0) return BROMOC_E_ARG;
+    if (used) *used = 0;
+    if (done) *done = 0;
+    if (nmcm == 0) return BROMOC_OK;
+    if (!(c->sys.flags & BROMOC_QPAR)) return fail(c, BROMOC_E_STATE, "METROPOLIS needs ions");
+    const int first = c->ns + c->nf, nmove = c->ntot - first;
+    if (nmove <= 0) return fail(c, BROMOC_E_STATE, "METROPOLIS: no moving ions");
+    (void)hipSetDevice(c->device);
+    std::vector<double4> hp(c->ntot);
+    HIPCHK(c, hipMemcpyAsync(hp.data(), c->pos[c->cur], sizeof(double4) * c->ntot,
+                             hipMemcpyDeviceToHost, c->stream));
+    int st = sync_err(c);
+    if (st) return st;
+    const bromoc_system &s = c->sys;
+    const double natom2 = (double)nmove;   // natom - nsites (move.f90:33)
+    int64_t k = 0;
+    int32_t ndone = 0;
+    for (int im = 0; im < nmcm; im++) {
+        int64_t kk = k;
+        int iat = -1;
+        double xn = 0.0, yn = 0.0, zn = 0.0;
+        bool out = true;
+        while (out) {
+            if (kk + 4 > nu) break;
+            if (!(u[kk] >= 0.0 && u[kk] < 1.0)) return fail(c, BROMOC_E_ARG, "uniform outside [0,1)");
+            iat = first + (int)(natom2 * u[kk]);
+            const double4 pa = hp[iat];
+            xn = pa.x + mcmax * (u[kk + 1] - 0.5);
+            yn = pa.y + mcmax * (u[kk + 2] - 0.5);
+            zn = pa.z + mcmax * (u[kk + 3] - 0.5);
+            kk += 4;
+            out = xn < s.lx2m || xn > s.lx2p || yn < s.ly2m || yn > s.ly2p || zn < s.lz2m ||
+                  zn > s.lz2p;
+        }
+        if (out || kk + 1 > nu) break;   // the trial could not finish
+        const int jt = std::abs(c->h_typei[iat]);
+        const double4 pa = hp[iat];
+        if ((st = interact_async(c, pa.x, pa.y, pa.z, jt, iat, 1, 0))) return st;
+        if ((st = interact_async(c, xn, yn, zn, jt, iat, 0, 1))) return st;
+        double eo[2];
+        HIPCHK(c, hipMemcpyAsync(eo, c->iout, sizeof(eo), hipMemcpyDeviceToHost, c->stream));
+        if ((st = sync_err(c))) return st;
+        const double eold = eo[0], enew = eo[1];
+        bool acc = enew <= eold;
+        if (!acc) {
+            const double bltz = std::exp(-(enew - eold) * ikbt);
+            acc = u[kk++] < bltz;
+        }
+        if (acc) {
+            hp[iat] = make_double4(xn, yn, zn, pa.w);
+            for (int b = 0; b < 2; b++)
+                HIPCHK(c, hipMemcpyAsync(c->pos[b] + iat, &hp[iat], sizeof(double4), hipMemcpyHostToDevice, c->stream));
+            *ener = *ener + (enew - eold);
+            c->sort_dirty = true;
+        }
+        k = kk;
+        ndone = im + 1;
+    }
+    if ((st = sync_err(c))) return st;
+    if (used) *used = k;
+    if (done) *done = ndone;
     return BROMOC_OK;
 }
 

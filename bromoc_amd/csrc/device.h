@@ -17,11 +17,14 @@ constexpr int F_DNA = 1;     // CG-DNA sites and lists present
 constexpr int F_PHIX = 2;    // static field grid
 constexpr int F_PHIV = 4;    // repulsion grid
 constexpr int F_UPDATE = 8;  // apply the BD update (else ENERGY only)
+constexpr int F_PROX = 16;   // track each ion's nearest DNA site (PROXDIFF, DYNAMICS3)
 
 // host-side flag values (include/bromoc_gpu.h)
 constexpr uint32_t QPAR = 1u << 0, QNUCL = 1u << 1, QNONBOND = 1u << 2, QNOBOND = 1u << 3;
 constexpr uint32_t QPHIX = 1u << 4, QPHIV = 1u << 5, QSVDW = 1u << 6, QSOLV = 1u << 7;
 constexpr uint32_t QDEBY = 1u << 8, QCOUNTION = 1u << 9, QDNAFREE = 1u << 10;
+constexpr uint32_t QBSPLINE = 1u << 18, QPROXDIFF = 1u << 21, QNOTRANS = 1u << 22;
+constexpr uint32_t QAPFOR = 1u << 23, QCONTRANS = 1u << 24;
 
 constexpr double PI = 3.14159265358979323846264338327950288419716939937510;
 constexpr double RSMALL = 1E-3;
@@ -189,6 +192,85 @@ __device__ __forceinline__ void grid_cell(const GridGeom &g, double x, double y,
     if (iz == g.nz - 1) iz = g.nz - 2;
 }
 
+// Quadratic B-spline M3 and its derivative DM3 (m3.f90:19-42, dm3.f90:19-38).
+__device__ __forceinline__ void bspline_m3(double x, double &m, double &dm) {
+    const double m2m = (x >= 0.0 && x <= 2.0) ? 1.0 - fabs(x - 1.0) : 0.0;
+    const double m2 = (x >= 1.0 && x <= 3.0) ? 1.0 - fabs(x - 2.0) : 0.0;
+    m = (x >= 0.0 && x <= 3.0) ? x * 0.5 * m2m + (3.0 - x) * 0.5 * m2 : 0.0;
+    dm = m2m - m2;
+}
+
+// Centre node of the 27-point stencil (vdwgd1spln.f90:84-96): nearest node, kept one node
+// inside every face.
+__device__ __forceinline__ void spline_cell(const GridGeom &g, double x, double y, double z,
+                                            double &xi, double &yi, double &zi, int &ix,
+                                            int &iy, int &iz) {
+    xi = x + g.tranx - g.xbcen;
+    yi = y + g.trany - g.ybcen;
+    zi = z + g.tranz - g.zbcen;
+    ix = (int)round(xi * g.idcel);
+    iy = (int)round(yi * g.idcel);
+    iz = (int)round(zi * g.idcel);
+    if (ix == 0) ix = 1;
+    if (iy == 0) iy = 1;
+    if (iz == 0) iz = 1;
+    if (ix == g.nx - 1) ix = g.nx - 2;
+    if (iy == g.ny - 1) iy = g.ny - 2;
+    if (iz == g.nz - 1) iz = g.nz - 2;
+}
+
+// Repulsion map of one ENERGY target at (x, y, z), atnam2 type t2, by a whole wavefront:
+// the trilinear stencil of vdwgd1trln.f90:50-161 on lanes 0..7, or with QBSPLINE the
+// 27-point B-spline of vdwgd1spln.f90:50-170 on lanes 0..26 (a node counts only where
+// none of its three factors vanishes with its derivative, vdwgd1spln.f90:106-124).  Adds
+// the lane's force and energy shares; returns, on every lane, whether the map values
+// summed over the stencil reach the wall threshold (warn).
+__device__ inline bool vdw_grid(const StepParams &P, double x, double y, double z, int t2,
+                                double &fx, double &fy, double &fz, double &e) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const GridGeom &g = P.gv;
+    double esvdw = P.svdw;
+    if (P.flags & QSVDW) esvdw = P.svdw * P.scal[t2];
+    double ph = 0.0;
+    bool ok;
+    if (P.flags & QBSPLINE) {
+        ok = x <= g.xbcen + g.tranx - g.dcel && x >= g.xbcen - g.tranx + g.dcel &&
+             y <= g.ybcen + g.trany - g.dcel && y >= g.ybcen - g.trany + g.dcel &&
+             z <= P.vzmax && z >= P.vzmin;
+        if (ok && lane < 27) {
+            double xi, yi, zi;
+            int ix, iy, iz;
+            spline_cell(g, x, y, z, xi, yi, zi, ix, iy, iz);
+            const int k = ix - 1 + lane / 9, l = iy - 1 + (lane / 3) % 3, m = iz - 1 + lane % 3;
+            double ai, dai, bi, dbi, ci, dci;
+            bspline_m3(1.5 - (xi - k * g.dcel) * g.idcel, ai, dai);
+            bspline_m3(1.5 - (yi - l * g.dcel) * g.idcel, bi, dbi);
+            bspline_m3(1.5 - (zi - m * g.dcel) * g.idcel, ci, dci);
+            if ((ai != 0.0 || dai != 0.0) && (bi != 0.0 || dbi != 0.0) && (ci != 0.0 || dci != 0.0)) {
+                ph = (double)P.phiv[((size_t)k * g.ny + l) * g.nz + m];
+                const double phivsv = ph * esvdw;
+                const double prefac = phivsv * g.idcel;
+                fx += dai * bi * ci * prefac;
+                fy += ai * dbi * ci * prefac;
+                fz += ai * bi * dci * prefac;
+                e += ai * bi * ci * phivsv;
+            }
+        }
+    } else {
+        ok = x <= g.xbcen + g.tranx && x >= g.xbcen - g.tranx && y <= g.ybcen + g.trany &&
+             y >= g.ybcen - g.trany && z <= P.vzmax && z >= P.vzmin;
+        if (ok && lane < 8) {
+            double xi, yi, zi;
+            int ix, iy, iz;
+            grid_cell(g, x, y, z, xi, yi, zi, ix, iy, iz);
+            ph = tri_corner(g, P.phiv, xi, yi, zi, ix, iy, iz, lane, esvdw, fx, fy, fz, e);
+        }
+    }
+    // map values are integers: the sum is exact in any order
+    const double sum = wave_sum(ph);
+    return ok && sum >= ((P.flags & QBSPLINE) ? P.thold27 : P.thold8);
+}
+
 // Diffusion profile of DYNAMICS2 (dynamics2.mono.f90:52-74): the end values outside the
 // knots, else the cubic spline and its derivative (seval/sevald, sevalmod.f90:23-67).  The
 // interval is found by the same bisection; seval's SAVEd interval is not kept (it only
@@ -260,16 +342,37 @@ __device__ __forceinline__ void ion_factors(const StepParams &P, int itype, doub
 }
 
 // Ermak-McCammon update of moving particle i (position pa, force F, Gaussians g):
-// DYNAMICS0/1/2 for ions (dynamics0.mono.f90:31-46, dynamics1.mono.f90:71-86,
-// dynamics2.mono.f90:75-85) with countions, DYNAMICS0NUC (dynamics0nuc.f90:31-44) for
-// sites; fixcoor unless the site is free of the box (dnafree).
+// DYNAMICS0/1/2/3 for ions (dynamics0.mono.f90:31-46, dynamics1.mono.f90:71-86,
+// dynamics2.mono.f90:75-85, dynamics3.mono.f90:37-72) with countions, DYNAMICS0NUC
+// (dynamics0nuc.f90:31-44) for sites; fixcoor unless the site is free of the box
+// (dnafree).  dids (DYNAMICS3 only): the ion's nearest-site record of ENERGY
+// (energy.mono.f90:603-620): dist - dmi, dist, and the ion-minus-site vector.
 __device__ __forceinline__ double4 bd_move(const StepParams &P, bool is_site, int itype,
                                            const double4 &pa, const double F[3],
-                                           const double *g, int32_t *nfwd, int32_t *nbwd) {
+                                           const double *g, int32_t *nfwd, int32_t *nbwd,
+                                           const double *dids = nullptr) {
     double fac1, fac2, ddz = 0.0;
+    double ddx3 = 0.0, ddy3 = 0.0, ddz3 = 0.0;
+    const bool prox = !is_site && P.dmode == 3;
     if (is_site) {
         fac1 = P.fact0n1;
         fac2 = P.fact0n2;
+    } else if (prox) {
+        if (dids[0] > 0.0 && dids[0] <= P.diffcutoff) {
+            const double D = P.diffusion[itype];
+            const double ebet = exp(-dids[0] * P.ibeta);
+            const double idf = D - (D - P.diff0) * ebet;
+            const double didf = (D - P.diff0) * P.ibeta * ebet;
+            const double idist = 1.0 / dids[1];
+            fac1 = idf * P.kbtdt;
+            fac2 = sqrt(2.0 * P.dt * idf);
+            ddx3 = didf * P.dt * dids[2] * idist;
+            ddy3 = didf * P.dt * dids[3] * idist;
+            ddz3 = didf * P.dt * dids[4] * idist;
+        } else {
+            fac1 = P.fact1a[itype];
+            fac2 = P.fact2a[itype];
+        }
     } else {
         ion_factors(P, itype, pa.z, fac1, fac2, ddz);
     }
@@ -281,7 +384,13 @@ __device__ __forceinline__ double4 bd_move(const StepParams &P, bool is_site, in
     double xn = pa.x + delx + fac2 * g[0];
     double yn = pa.y + dely + fac2 * g[1];
     double zn = pa.z + delz + fac2 * g[2];
-    if (!is_site && P.dmode != 0) zn = zn + ddz;
+    if (prox) {
+        xn = xn + ddx3;
+        yn = yn + ddy3;
+        zn = zn + ddz3;
+    } else if (!is_site && P.dmode != 0) {
+        zn = zn + ddz;
+    }
     if (!is_site && (P.flags & QCOUNTION)) {
         const int row = itype - P.nold - 1;
         for (int c = 0; c < P.cntpts; c++) {
@@ -296,8 +405,8 @@ __device__ __forceinline__ double4 bd_move(const StepParams &P, bool is_site, in
 
 __device__ __forceinline__ double4 bd_update(const StepParams &P, int i, bool is_site,
                                              const double4 &pa, const double F[3],
-                                             const double *g) {
-    return bd_move(P, is_site, is_site ? 0 : abs(P.typei[i]), pa, F, g, P.nfwd, P.nbwd);
+                                             const double *g, const double *dids = nullptr) {
+    return bd_move(P, is_site, is_site ? 0 : abs(P.typei[i]), pa, F, g, P.nfwd, P.nbwd, dids);
 }
 
 // Reaction-field parameters of an ion at (x, y, z) of atnam type itype: srfe and reff
@@ -672,7 +781,29 @@ __device__ inline double interact_grids(const StepParams &P, const float *phix, 
             es = fi * chi * (double)phix[((size_t)n1 * g.ny + n2) * g.nz + n3] * P.celec;
         }
     }
-    if (P.flags & QPHIV) {
+    if ((P.flags & QPHIV) && (P.flags & QBSPLINE)) {
+        // VDWGD0SPLN (vdwgd0spln.f90:19-130): a node counts where its weight is nonzero
+        const GridGeom &g = P.gv;
+        okv = xj <= g.xbcen + g.tranx && xj >= g.xbcen - g.tranx && yj <= g.ybcen + g.trany &&
+              yj >= g.ybcen - g.trany && zj <= P.vzmax && zj >= P.vzmin;
+        if (okv && lane < 27) {
+            double xi, yi, zi;
+            int ix, iy, iz;
+            spline_cell(g, xj, yj, zj, xi, yi, zi, ix, iy, iz);
+            double esvdw = P.svdw;
+            if (P.flags & QSVDW) esvdw = esvdw * P.scal[jt2];
+            const int k = ix - 1 + lane / 9, l = iy - 1 + (lane / 3) % 3, m = iz - 1 + lane % 3;
+            double ai, bi, ci, dd;
+            bspline_m3(1.5 - (xi - k * g.dcel) * g.idcel, ai, dd);
+            bspline_m3(1.5 - (yi - l * g.dcel) * g.idcel, bi, dd);
+            bspline_m3(1.5 - (zi - m * g.dcel) * g.idcel, ci, dd);
+            const double fi = ai * bi * ci;
+            if (ai != 0.0 && bi != 0.0 && fi != 0.0) {
+                ph = (double)P.phiv[((size_t)k * g.ny + l) * g.nz + m];
+                ev = fi * esvdw * ph;
+            }
+        }
+    } else if (P.flags & QPHIV) {
         const GridGeom &g = P.gv;
         okv = xj <= g.xbcen + g.tranx && xj >= g.xbcen - g.tranx && yj <= g.ybcen + g.trany &&
               yj >= g.ybcen - g.trany && zj <= P.vzmax && zj >= P.vzmin;
@@ -697,7 +828,7 @@ __device__ inline double interact_grids(const StepParams &P, const float *phix, 
     double e = es;
     wall = false;
     if (okv) {
-        if (ph >= P.thold8) {
+        if (ph >= ((P.flags & QBSPLINE) ? P.thold27 : P.thold8)) {
             ev = 1.0e10;
             wall = true;
         }

@@ -88,7 +88,7 @@ struct StepParams {
     GridGeom gx, gv;
     const float *phix;
     const int8_t *phiv;
-    double svdw, thold8, vzmin, vzmax;
+    double svdw, thold8, vzmin, vzmax, thold27;
     const double *scal;
     int32_t *warn;
     // Brownian dynamics
@@ -106,8 +106,9 @@ struct StepParams {
     uint64_t seed;
     uint64_t stream;
     // ion update: 0 DYNAMICS0 (fact1a/fact2a), 1 DYNAMICS1 (pore switch), 2 DYNAMICS2
-    // (spline profile)
+    // (spline profile), 3 DYNAMICS3 (DNA proximity: ibeta, diff0, diffcutoff)
     int32_t dmode, nspline;
+    double ibeta, diff0, diffcutoff;
     double dt, kbtdt, plength2, pcenter;
     const double *diffusion, *ampl3, *p3;
     const double *xs, *ys, *bs, *cs, *ds;
@@ -119,6 +120,18 @@ struct StepParams {
     double4 *rfa;            // [cap] (srfe, reff, 1 if inside the grid box else 0, 0)
     double4 *rfd;            // [2*cap] (d srfe / d r, 0), (d reff / d r, 0)
     double4 *frf;            // [cap] reaction-field force on each moving ion (k_rfpair)
+    // APFOR: summed applied force of each site (zero where none) [nsites]
+    const double4 *apf;
+    // CONTRANS: ctn entries (csn 1-based, 0 = centroid), restraint centres [ctn+1]; cen:
+    // site position sums of the current positions (all sites, first strand, second
+    // strand), written by k_centroid
+    int32_t ctn, unsplit, nsites1st, notrans_axes;
+    const int32_t *csn;
+    const double *kx, *ky, *kz, *contrx, *contry, *contrz;
+    double insites;
+    double *cen;             // [9]
+    // NOTRANS centre
+    double notrx, notry, notrz;
 };
 
 // spatial partner order (sort.hip)
@@ -143,6 +156,7 @@ hipError_t launch_step(const StepParams &p, hipStream_t s);   // launch_dna + la
 hipError_t launch_dna(const StepParams &p, hipStream_t s);    // k_dna (CG-DNA list terms)
 hipError_t launch_pairs(const StepParams &p, hipStream_t s);  // k_step (pairs, grids, update)
 hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s);  // RFPARION
+hipError_t launch_notrans(const StepParams &p, hipStream_t s);  // NOTRANS after a step
 hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
                                 hipStream_t s);
 hipError_t launch_advance(int64_t *ctr, int64_t by, hipStream_t s);

@@ -13,6 +13,39 @@
 
 namespace bromoc {
 
+// The nearest-site record dids(1:5,i) of ion i (energy.mono.f90:583,603-620): the block's
+// per-wavefront minima of (dist - dmi, site) combined; the reference starts the record
+// only at site 1 (j.eq.1) and later sites replace it only when strictly smaller, so when
+// site 1 is outside its table range the record stays zero unless some site gives a
+// negative dist - dmi.
+template <int T>
+__device__ inline void prox_record(const StepParams &P, int i, const double4 &pa, int a,
+                                   const double (*s_pxd)[T], const int (*s_pxj)[T],
+                                   const DevPair *row, double dids[5]) {
+    double d = INFINITY;
+    int jj = 0x7fffffff;
+    for (int q = 0; q < WPB; q++) {
+        const double od = s_pxd[q][a];
+        const int oj = s_pxj[q][a];
+        if (od < d || (od == d && oj < jj)) { d = od; jj = oj; }
+    }
+    if (jj == 0x7fffffff) return;
+    const double4 p0 = P.pos[0];
+    const double e0x = pa.x - p0.x, e0y = pa.y - p0.y, e0z = pa.z - p0.z;
+    const double d20 = e0x * e0x + e0y * e0y + e0z * e0z;
+    const DevPair &r0 = row[__double2loint(p0.w)];
+    const bool first_in = (P.all_efpot || (r0.flags & PF_EFPOT)) && !(d20 < r0.dm2lo) &&
+                          !(d20 > r0.dm2hi);
+    if (!first_in && !(d < 0.0)) return;
+    const double4 pj = P.pos[jj];
+    const double ex = pa.x - pj.x, ey = pa.y - pj.y, ez = pa.z - pj.z;
+    dids[0] = d;
+    dids[1] = sqrt(ex * ex + ey * ey + ez * ez);
+    dids[2] = ex;
+    dids[3] = ey;
+    dids[4] = ez;
+}
+
 // k_step: T target particles per block of 256 threads (targets tbeg.. of the launch).
 // Every thread walks the partner set with stride 256, two partners per iteration, and
 // evaluates each partner against all T targets.  Per-target sums: butterfly within each
@@ -70,6 +103,12 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
     double fx[T], fy[T], fz[T], et[T], eel[T], evd[T], esr[T];
 #pragma unroll
     for (int a = 0; a < T; a++) fx[a] = fy[a] = fz[a] = et[a] = eel[a] = evd[a] = esr[a] = 0.0;
+    // PROXDIFF: per target ion, the tabulated site partner within its table range of least
+    // dist - dmi, ties to the lower site index (energy.mono.f90:603-620 in site order)
+    double pxd[T];
+    int pxj[T];
+#pragma unroll
+    for (int a = 0; a < T; a++) { pxd[a] = INFINITY; pxj[a] = 0x7fffffff; }
 
     // ---- pair loop (energy.mono.f90:117-194, 584-687)
     const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
@@ -106,12 +145,36 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
                         evd[a] += ce ? q1 : 0.0;
                         esr[a] += ce ? q2 : 0.0;
                     }
+                    if constexpr ((FEAT & F_PROX) != 0) {
+                        const DevPair &pr = s_row[a][t2j];
+                        if (!skip && !tsite[a] && j < ns && (P.all_efpot || (pr.flags & PF_EFPOT)) &&
+                            !(d2 < pr.dm2lo) && !(d2 > pr.dm2hi)) {
+                            const double dd = sqrt(d2) - pr.dmi;
+                            if (dd < pxd[a] || (dd == pxd[a] && j < pxj[a])) { pxd[a] = dd; pxj[a] = j; }
+                        }
+                    }
                     de = (skip || !tfree[a]) ? 0.0 : de;
                     fx[a] += de * dx;
                     fy[a] += de * dy;
                     fz[a] += de * dz;
                 }
             }
+        }
+    }
+    __shared__ double s_pxd[(FEAT & F_PROX) ? WPB : 1][T];
+    __shared__ int s_pxj[(FEAT & F_PROX) ? WPB : 1][T];
+    if constexpr ((FEAT & F_PROX) != 0) {
+#pragma unroll
+        for (int a = 0; a < T; a++) {
+            double d = pxd[a];
+            int jj = pxj[a];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double od = __shfl_xor(d, off, WAVE);
+                const int oj = __shfl_xor(jj, off, WAVE);
+                if (od < d || (od == d && oj < jj)) { d = od; jj = oj; }
+            }
+            if (lane == 0) { s_pxd[wv][a] = d; s_pxj[wv][a] = jj; }
         }
     }
 
@@ -180,24 +243,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
         }
     }
     if constexpr ((FEAT & F_PHIV) != 0) {
-        const GridGeom &g = P.gv;
-        const bool ok = mine && val_me && xi <= g.xbcen + g.tranx && xi >= g.xbcen - g.tranx &&
-                        yi <= g.ybcen + g.trany && yi >= g.ybcen - g.trany && zi <= P.vzmax &&
-                        zi >= P.vzmin;
-        double ph = 0.0;
-        if (ok && lane < 8) {
-            double gx_, gy_, gz_;
-            int ix, iy, iz;
-            grid_cell(g, xi, yi, zi, gx_, gy_, gz_, ix, iy, iz);
-            double esvdw = P.svdw;
-            if (P.flags & QSVDW) esvdw = P.svdw * P.scal[t2i];
-            ph = tri_corner(g, P.phiv, gx_, gy_, gz_, ix, iy, iz, lane, esvdw, gfx, gfy, gfz, egv);
-        }
-        double sum = ph;
-        sum += __shfl_xor(sum, 1, WAVE);
-        sum += __shfl_xor(sum, 2, WAVE);
-        sum += __shfl_xor(sum, 4, WAVE);
-        warned = ok && sum >= P.thold8;
+        if (mine && val_me) warned = vdw_grid(P, xi, yi, zi, t2i, gfx, gfy, gfz, egv);
     }
     if (mine) {
         gfx = wave_sum(gfx);
@@ -278,10 +324,130 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
             return;
         }
         const double g[3] = {s_g[a][0], s_g[a][1], s_g[a][2]};
-        const double4 pn = bd_update(P, i, is_site, pa, F, g);
+        double dids[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if constexpr ((FEAT & F_PROX) != 0) {
+            if (!is_site) prox_record(P, i, pa, a, s_pxd, s_pxj, s_row[a], dids);
+        }
+        const double4 pn = bd_update(P, i, is_site, pa, F, g, dids);
         P.poso[i] = pn;
         P.sposo[P.rank[i]] = pn;
     }
+}
+
+// CONTRANS force on site i along axis ax (energy.mono.f90:520-580), added to f entry by
+// entry: a restrained site is pulled to its centre, every site by a restrained centroid
+// (of all sites, or of its strand with unsplit); no stfree gate, as in the reference.
+__device__ inline double contrans_force(const StepParams &P, int i, int ax, double f) {
+    const double *kk = ax == 0 ? P.kx : (ax == 1 ? P.ky : P.kz);
+    const double *cc = ax == 0 ? P.contrx : (ax == 1 ? P.contry : P.contrz);
+    const double xi = ax == 0 ? P.pos[i].x : (ax == 1 ? P.pos[i].y : P.pos[i].z);
+    for (int q = 0; q < P.ctn; q++) {
+        const double k = kk[q];
+        if (k == 0.0) continue;
+        const int j = P.csn[q];
+        if (j == 0) {
+            if (P.unsplit) {
+                if (i < P.nsites1st) {
+                    const double con = P.cen[3 + ax] * P.insites * 2.0 - cc[q];
+                    f = f - k * con * P.insites * 2.0;
+                } else {
+                    const double con = P.cen[6 + ax] * P.insites * 2.0 - cc[P.ctn];
+                    f = f - k * con * P.insites * 2.0;
+                }
+            } else {
+                const double con = P.cen[ax] * P.insites - cc[q];
+                f = f - k * con * P.insites;
+            }
+        } else if (j - 1 == i) {
+            f = f - k * (xi - cc[q]);
+        }
+    }
+    return f;
+}
+
+// k_centroid: site position sums of the current positions -- all sites, the first strand
+// (sites 0..nsites1st-1) and the second -- in a fixed-order tree, then the CONTRANS energy
+// econ in the reference's entry and axis order (energy.mono.f90:520-580), written to the
+// first site's slot of its energy category.
+__global__ __launch_bounds__(BLOCK) void k_centroid(StepParams P) {
+    __shared__ double s[9][BLOCK];
+    const int tid = threadIdx.x;
+    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = tid; i < P.nsites; i += BLOCK) {
+        const double4 q = P.pos[i];
+        const int h = i < P.nsites1st ? 3 : 6;
+        a[0] += q.x; a[1] += q.y; a[2] += q.z;
+        a[h] += q.x; a[h + 1] += q.y; a[h + 2] += q.z;
+    }
+    for (int k = 0; k < 9; k++) s[k][tid] = a[k];
+    __syncthreads();
+    for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        if (tid < h)
+            for (int k = 0; k < 9; k++) s[k][tid] += s[k][tid + h];
+        __syncthreads();
+    }
+    if (tid < 9) P.cen[tid] = s[tid][0];
+    if (tid != 0 || !P.write_energy) return;
+    double econ = 0.0;
+    for (int q = 0; q < P.ctn; q++) {
+        const int j = P.csn[q];
+        for (int ax = 0; ax < 3; ax++) {
+            const double k = ax == 0 ? P.kx[q] : (ax == 1 ? P.ky[q] : P.kz[q]);
+            if (k == 0.0) continue;
+            const double *cc = ax == 0 ? P.contrx : (ax == 1 ? P.contry : P.contrz);
+            if (j == 0) {
+                if (P.unsplit) {
+                    double con = s[3 + ax][0] * P.insites * 2.0 - cc[q];
+                    econ = econ + 0.5 * k * (con * con);
+                    con = s[6 + ax][0] * P.insites * 2.0 - cc[P.ctn];
+                    econ = econ + 0.5 * k * (con * con);
+                } else {
+                    const double con = s[ax][0] * P.insites - cc[q];
+                    econ = econ + 0.5 * k * (con * con);
+                }
+            } else {
+                const double4 pj = P.pos[j - 1];
+                const double d = (ax == 0 ? pj.x : (ax == 1 ? pj.y : pj.z)) - cc[q];
+                econ = econ + 0.5 * k * (d * d);
+            }
+        }
+    }
+    P.epart[(size_t)E_CON * P.ecap] = econ;
+}
+
+// k_notrans: NOTRANS after the step (simul1.f90:382-386): every site moves so that the
+// sites' centroid sits at notr* on the selected axes, x = x - sum(x)/nsites + notrx, on the
+// new positions and their spatial-order copies.
+__global__ __launch_bounds__(BLOCK) void k_notrans(StepParams P) {
+    __shared__ double s[3][BLOCK];
+    const int tid = threadIdx.x;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int i = tid; i < P.nsites; i += BLOCK) {
+        const double4 q = P.poso[i];
+        a[0] += q.x; a[1] += q.y; a[2] += q.z;
+    }
+    for (int k = 0; k < 3; k++) s[k][tid] = a[k];
+    __syncthreads();
+    for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        if (tid < h)
+            for (int k = 0; k < 3; k++) s[k][tid] += s[k][tid + h];
+        __syncthreads();
+    }
+    const double sx = s[0][0], sy = s[1][0], sz = s[2][0];
+    for (int i = tid; i < P.nsites; i += BLOCK) {
+        double4 q = P.poso[i];
+        if (P.notrans_axes & 1) q.x = q.x - sx * P.insites + P.notrx;
+        if (P.notrans_axes & 2) q.y = q.y - sy * P.insites + P.notry;
+        if (P.notrans_axes & 4) q.z = q.z - sz * P.insites + P.notrz;
+        P.poso[i] = q;
+        P.sposo[P.rank[i]] = q;
+    }
+}
+
+hipError_t launch_notrans(const StepParams &p, hipStream_t s) {
+    if ((p.flags & QNOTRANS) && p.nsites > 0)
+        hipLaunchKernelGGL(k_notrans, dim3(1), dim3(BLOCK), 0, s, p);
+    return hipGetLastError();
 }
 
 // k_dna: CG-DNA bonded and list terms (energy.mono.f90:196-508), one block per site.
@@ -323,6 +489,8 @@ __global__ __launch_bounds__(BLOCK) void k_dna(StepParams P) {
         double acc = 0.0;
         for (int q = 0; q < WPB; q++) acc += s_r[q][tid];
         if (tid < 3) {
+            if ((P.flags & QAPFOR) && free_me) acc += reinterpret_cast<const double *>(&P.apf[i])[tid];
+            if (P.flags & QCONTRANS) acc = contrans_force(P, i, tid, acc);
             reinterpret_cast<double *>(&P.fdna[i])[tid] = acc;
         } else if (P.write_energy) {
             P.epart[(size_t)ND_CAT[tid - 3] * P.ecap + i] = acc;
@@ -443,26 +611,7 @@ __global__ __launch_bounds__(BLOCK) void k_wave(StepParams P) {
             }
         }
     }
-    if constexpr ((FEAT & F_PHIV) != 0) {
-        const GridGeom &g = P.gv;
-        const bool ok = pa.x <= g.xbcen + g.tranx && pa.x >= g.xbcen - g.tranx &&
-                        pa.y <= g.ybcen + g.trany && pa.y >= g.ybcen - g.trany && pa.z <= P.vzmax &&
-                        pa.z >= P.vzmin;
-        double ph = 0.0;
-        if (ok && lane < 8) {
-            double gx_, gy_, gz_;
-            int ix, iy, iz;
-            grid_cell(g, pa.x, pa.y, pa.z, gx_, gy_, gz_, ix, iy, iz);
-            double esvdw = P.svdw;
-            if (P.flags & QSVDW) esvdw = P.svdw * P.scal[t2i];
-            ph = tri_corner(g, P.phiv, gx_, gy_, gz_, ix, iy, iz, lane, esvdw, fx, fy, fz, egv);
-        }
-        double sum = ph;
-        sum += __shfl_xor(sum, 1, WAVE);
-        sum += __shfl_xor(sum, 2, WAVE);
-        sum += __shfl_xor(sum, 4, WAVE);
-        warned = ok && sum >= P.thold8;
-    }
+    if constexpr ((FEAT & F_PHIV) != 0) warned = vdw_grid(P, pa.x, pa.y, pa.z, t2i, fx, fy, fz, egv);
     fx = wave_sum(fx);
     fy = wave_sum(fy);
     fz = wave_sum(fz);
@@ -561,9 +710,20 @@ static void dispatch_step(const StepParams &p, int feat, hipStream_t s) {
 #undef CASE
 }
 
+// BD steps of DYNAMICS3 (DNA and ions present, update on): one target per block
+static void dispatch_prox(const StepParams &p, int feat, hipStream_t s) {
+    dim3 grid(p.tend - p.tbeg), block(BLOCK);
+#define CASE(F) case F: hipLaunchKernelGGL((k_step<F | F_PROX, 1>), grid, block, 0, s, p); break;
+    switch (feat) {
+        CASE(9) CASE(11) CASE(13) CASE(15)
+    }
+#undef CASE
+}
+
 hipError_t launch_dna(const StepParams &p, hipStream_t s) {
-    if ((p.flags & QNUCL) && p.nsites > 0)
-        hipLaunchKernelGGL(k_dna, dim3(p.nsites), dim3(BLOCK), 0, s, p);
+    if (!(p.flags & QNUCL) || p.nsites <= 0) return hipSuccess;
+    if (p.flags & QCONTRANS) hipLaunchKernelGGL(k_centroid, dim3(1), dim3(BLOCK), 0, s, p);
+    hipLaunchKernelGGL(k_dna, dim3(p.nsites), dim3(BLOCK), 0, s, p);
     return hipGetLastError();
 }
 
@@ -574,6 +734,10 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s) {
     if (p.flags & QPHIX) feat |= F_PHIX;
     if (p.flags & QPHIV) feat |= F_PHIV;
     if (p.mode == 1) feat |= F_UPDATE;
+    if (p.dmode == 3 && (feat & F_UPDATE) && (feat & F_DNA)) {
+        dispatch_prox(p, feat, s);
+        return hipGetLastError();
+    }
     switch (p.tpb) {
         case 1: dispatch_step<1>(p, feat, s); break;
         case 2: dispatch_step<2>(p, feat, s); break;

@@ -160,6 +160,16 @@ class Engine:
         self.ntot = nn.value
         return used.value, e.value
 
+    def metropolis(self, nmcm, mcmax, ikbt, uniforms, ener=0.0):
+        """METROPOLIS(nmcm) (metropolis.f90:19-58): returns (uniforms used, trials done,
+        ener)."""
+        u = np.ascontiguousarray(uniforms, float)
+        used, done, e = C.c_int64(), C.c_int32(), C.c_double(ener)
+        self._chk("bromoc_metropolis",
+                  self.lib.bromoc_metropolis(self.ctx, nmcm, mcmax, ikbt, ptr(u), len(u),
+                                             C.byref(used), C.byref(done), C.byref(e)))
+        return used.value, done.value, e.value
+
 
 def philox4x64(ctr, key):
     """Raw Philox4x64-10 blocks computed on the device."""

@@ -1,11 +1,11 @@
 ! bromoc_gpu.f90 -- iso_c_binding front-end of libbromoc_gpu.so for the cg-gcmc-bd driver.
 !
 ! Link this file's objects instead of energy.mono.o, dynamics0.mono.o, dynamics1.mono.o,
-! dynamics2.mono.o, dynamics0nuc.o, interact.o and grand.o: the externals at the end carry
-! the reference routines' names and argument lists (energy.mono.f90:19,
-! dynamics0.mono.f90:19, dynamics1.mono.f90:19, dynamics2.mono.f90:19, dynamics0nuc.f90:19,
-! interact.f90:19, grand.f90:19), so simul1.f90, metropolis.f90 and shell_simul.f90 call
-! the GPU path unchanged.  The first call hands the module state (grandmod, efpmod,
+! dynamics2.mono.o, dynamics3.mono.o, dynamics0nuc.o, interact.o, grand.o and metropolis.o:
+! the externals at the end carry the reference routines' names and argument lists
+! (energy.mono.f90:19, dynamics0.mono.f90:19, dynamics1.mono.f90:19, dynamics2.mono.f90:19,
+! dynamics3.mono.f90:19, dynamics0nuc.f90:19, interact.f90:19, grand.f90:19,
+! metropolis.f90:19), so simul1.f90 and shell_simul.f90 call the GPU path unchanged.  The first call hands the module state (grandmod, efpmod,
 ! extramod, nucleotmod, gsbpmod) to bromoc_create; positions travel to the device before
 ! every ENERGY/INTERACT and back after every DYNAMICS/GRAND.  A faster loop that keeps
 ! positions on the device for nbd steps is bromoc_gpu_bd(nbd) (see the caller notes at
@@ -53,6 +53,10 @@ type, bind(C) :: bgpu_system
   type(bgpu_grid) :: rf
   type(c_ptr) :: gsrfen, greff
   real(c_double) :: sqrfac
+  integer(c_int32_t) :: afn, ctn
+  type(c_ptr) :: sn, af, csn, kx, ky, kz, contrx, contry, contrz
+  integer(c_int32_t) :: unsplit, nsites1st, notrans_axes, reserved3
+  real(c_double) :: notrx, notry, notrz, ibeta, diff0, diffcutoff, thold27
 end type
 
 type, bind(C) :: bgpu_energies
@@ -162,6 +166,18 @@ interface
     integer(c_int64_t), value :: icyc
     real(c_double), intent(inout) :: ener
   end function
+  integer(c_int) function bromoc_metropolis(ctx, nm, mcm, ikt, u, nu, used, done, ener) &
+      bind(C, name='bromoc_metropolis')
+    import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: nm
+    real(c_double), value :: mcm, ikt
+    real(c_double), intent(in) :: u(*)
+    integer(c_int64_t), value :: nu
+    integer(c_int64_t), intent(out) :: used
+    integer(c_int32_t), intent(out) :: done
+    real(c_double), intent(inout) :: ener
+  end function
   integer(c_int) function bromoc_destroy(ctx) bind(C, name='bromoc_destroy')
     import :: c_int, c_ptr
     type(c_ptr), value :: ctx
@@ -228,6 +244,9 @@ subroutine bromoc_gpu_init(seed)
   real(c_double), allocatable, target :: epc(:,:)
   real(c_float), allocatable, target :: gx(:), rfs(:), rfr(:)
   real(c_double), allocatable, target :: dif(:), am3(:), pp3(:), spx(:), spy(:), spb(:), spc(:), spd(:)
+  integer(c_int32_t), allocatable, target :: asn(:), ccsn(:)
+  real(c_double), allocatable, target :: aaf(:), ckx(:), cky(:), ckz(:), ccx(:), ccy(:), ccz(:)
+  integer :: nc
   integer :: nsp, ngrid
   integer(c_int8_t), allocatable, target :: gv(:)
   integer :: i, a, b, nk, nt1, ntc, nindex, cap
@@ -340,7 +359,7 @@ subroutine bromoc_gpu_init(seed)
   if (Qphiv .and. Qsvdw) scs(2:nt1) = scal(1:nttyp)
   if (cntpts > 0) zc(1:cntpts) = zcont(1:cntpts)
 
-  s%abi_version = 2
+  s%abi_version = 3
   s%ntot = ntot; s%nsites = nsites; s%nfix = nfix; s%capacity = cap
   s%ntype = ntype; s%nold = nold; s%nttyp = nttyp; s%npair = ntc; s%flags = int(fl, c_int32_t)
   s%x = c_loc(xs); s%y = c_loc(ys); s%z = c_loc(zs); s%typei = c_loc(ityp)
@@ -398,6 +417,40 @@ subroutine bromoc_gpu_init(seed)
     rfs = 0.0; rfr = 0.0
   endif
   s%gsrfen = c_loc(rfs); s%greff = c_loc(rfr); s%sqrfac = sqrfac
+  ! APFOR (apfmod) and CONTRANS / NOTRANS (nucleotmod), shell_simul.f90:650-770
+  s%afn = 0; s%ctn = 0
+  allocate(asn(1), aaf(3))
+  asn = 0; aaf = 0.0
+  if (Qapfor .and. afn > 0) then
+    deallocate(asn, aaf)
+    allocate(asn(afn), aaf(3*afn))
+    asn = sn(1:afn) - 1
+    aaf = reshape(af(1:3, 1:afn), [3*afn])
+    s%afn = afn
+  endif
+  s%sn = c_loc(asn); s%af = c_loc(aaf)
+  nc = 0
+  if (Qnucl .and. Qcontrans) nc = ctn
+  allocate(ccsn(max(1,nc)), ckx(max(1,nc)), cky(max(1,nc)), ckz(max(1,nc)), ccx(nc+1), &
+           ccy(nc+1), ccz(nc+1))
+  ccsn = 0; ckx = 0.0; cky = 0.0; ckz = 0.0; ccx = 0.0; ccy = 0.0; ccz = 0.0
+  if (nc > 0) then
+    ccsn(1:nc) = csn(1:nc); ckx(1:nc) = kx(1:nc); cky(1:nc) = ky(1:nc); ckz(1:nc) = kz(1:nc)
+    ccx = contrx(1:nc+1); ccy = contry(1:nc+1); ccz = contrz(1:nc+1)
+  endif
+  s%ctn = nc; s%csn = c_loc(ccsn); s%kx = c_loc(ckx); s%ky = c_loc(cky); s%kz = c_loc(ckz)
+  s%contrx = c_loc(ccx); s%contry = c_loc(ccy); s%contrz = c_loc(ccz)
+  s%unsplit = merge(1, 0, Qnucl .and. Qcontrans .and. Qunsplit); s%nsites1st = nsites1st
+  s%notrans_axes = 0
+  if (Qnucl .and. Qnotrans) then
+    if (Qnotrx) s%notrans_axes = ior(s%notrans_axes, 1)
+    if (Qnotry) s%notrans_axes = ior(s%notrans_axes, 2)
+    if (Qnotrz) s%notrans_axes = ior(s%notrans_axes, 4)
+  endif
+  s%notrx = notrx; s%notry = notry; s%notrz = notrz
+  ! PROXDIFF (grandmod beta, diff0, diffcutoff) and the B-spline wall threshold
+  s%ibeta = ibeta; s%diff0 = diff0; s%diffcutoff = diffcutoff
+  s%thold27 = thold27
   call bgpu_check(bromoc_create(s, 0, gctx), 'bromoc_create')
 end subroutine
 
@@ -451,8 +504,8 @@ subroutine bromoc_gpu_counters()
   warn(1:nttyp) = w(2:nttyp+1)
 end subroutine
 
-! nbd BD steps (ENERGY + DYNAMICS0 + DYNAMICS0NUC) with the positions kept on the device:
-! the replacement for simul1.f90:363-390 when neither NOTRANS nor the pressure virial is on.
+! nbd BD steps (ENERGY + DYNAMICS0/1/2/3 + DYNAMICS0NUC + NOTRANS) with the positions
+! kept on the device: the replacement for simul1.f90:363-390 when the pressure virial is off.
 subroutine bromoc_gpu_bd(nbd)
   use grandmod
   integer, intent(in) :: nbd
@@ -520,6 +573,14 @@ call DYNAMICS0(ninit, nfinal)
 end subroutine
 
 SUBROUTINE DYNAMICS2(ninit, nfinal)
+implicit none
+integer ninit, nfinal
+call DYNAMICS0(ninit, nfinal)
+end subroutine
+
+! DYNAMICS3 (DNA-proximity diffusion): the device pass recomputes each ion's nearest-site
+! record from the same positions as the preceding ENERGY.
+SUBROUTINE DYNAMICS3(ninit, nfinal)
 implicit none
 integer ninit, nfinal
 call DYNAMICS0(ninit, nfinal)
@@ -616,4 +677,44 @@ deallocate(ib)
 allocate(ib(ntot))
 call bgpu_check(bromoc_get_ibuffer(gctx, ib), 'bromoc_get_ibuffer')
 ibuffer(1:ntot) = ib
+end subroutine
+
+! METROPOLIS draws its uniforms from the reference's rndm() ahead of the device loop: 5 per
+! trial and a margin; trials that the batch cannot finish (the box-rejection loop of MOVE
+! draws 4 more per retry) run in the next call with the unused uniforms first.
+SUBROUTINE METROPOLIS(nmcm)
+use bromoc_gpu
+use grandmod
+implicit none
+integer nmcm
+real(c_double), allocatable :: u(:), w(:)
+integer(c_int64_t) :: used
+integer(c_int32_t) :: done, left
+real(c_double) :: e
+integer :: k, nu, have
+call bromoc_gpu_init()
+call bromoc_gpu_upload()
+left = nmcm
+have = 0
+nu = 5*nmcm + 64
+allocate(u(nu))
+e = ener
+do while (left > 0)
+  do k = have + 1, nu
+    u(k) = rndm()
+  enddo
+  call bgpu_check(bromoc_metropolis(gctx, left, mcmax, ikbt, u, int(nu, c_int64_t), used, done, e), &
+                  'bromoc_metropolis')
+  left = left - done
+  have = nu - int(used)
+  u(1:have) = u(int(used)+1:nu)
+  if (done == 0) then          ! one trial needs more than the batch holds: widen it
+    allocate(w(2*nu))
+    w(1:have) = u(1:have)
+    call move_alloc(w, u)
+    nu = 2*nu
+  endif
+enddo
+ener = e
+call bromoc_gpu_download()
 end subroutine

@@ -111,6 +111,18 @@ class System:
     rf: GridData | None = None
     rf_greff: np.ndarray = None      # (ngrid, nx, ny, nz) float32
     sqrfac: float = 1.0
+    # APFOR: (sites [afn] 0-based, forces [afn, 3])
+    apfor: tuple = None
+    # CONTRANS: dict(csn [ctn] 1-based with 0 = centroid, k [ctn, 3], centre [ctn+1, 3],
+    # unsplit, nsites1st)
+    contrans: dict = None
+    # NOTRANS: (axes bitmask, (notrx, notry, notrz))
+    notrans: tuple = None
+    # PROXDIFF: 1/beta, diff0, diffcutoff
+    ibeta: float = 0.0
+    diff0: float = 0.0
+    diffcutoff: float = 0.0
+    thold27: float = 1.0                 # B-spline wall threshold
     ndna: int = 0
     nion: int = 0
     meta: dict = field(default_factory=dict)
@@ -248,6 +260,28 @@ class System:
             keep.append(z)
             s.gsrfen = s.greff = z.ctypes.data_as(C.POINTER(C.c_float))
         s.sqrfac = self.sqrfac
+        if self.apfor is not None:
+            sn, af = self.apfor
+            s.afn = len(sn)
+            s.sn = arr(sn if len(sn) else np.zeros(1), np.int32, C.c_int32)
+            s.af = arr(np.asarray(af, float).reshape(-1) if len(sn) else np.zeros(3), np.float64,
+                       C.c_double)
+        ct = self.contrans
+        if ct is not None:
+            n = len(ct["csn"])
+            s.ctn = n
+            s.csn = arr(ct["csn"] if n else np.zeros(1), np.int32, C.c_int32)
+            k = np.asarray(ct["k"], float).reshape(n, 3) if n else np.zeros((1, 3))
+            c0 = np.asarray(ct["centre"], float).reshape(n + 1, 3)
+            s.kx, s.ky, s.kz = (arr(k[:, a], np.float64, C.c_double) for a in range(3))
+            s.contrx, s.contry, s.contrz = (arr(c0[:, a], np.float64, C.c_double) for a in range(3))
+            s.unsplit = 1 if ct.get("unsplit") else 0
+            s.nsites1st = int(ct.get("nsites1st", 0))
+        if self.notrans is not None:
+            s.notrans_axes = int(self.notrans[0])
+            s.notrx, s.notry, s.notrz = (float(v) for v in self.notrans[1])
+        s.ibeta, s.diff0, s.diffcutoff = self.ibeta, self.diff0, self.diffcutoff
+        s.thold27 = self.thold27
         return s, keep
 
 

@@ -19,11 +19,17 @@
  * bromoc_build_table           EFPOT table construction     splinepot.f90:19-96 +
  *                              squarespline polynom.f90:20-48
  * bromoc_spline                SUBROUTINE SPLINE            spline.f90:19-98
+ * bromoc_metropolis            SUBROUTINE METROPOLIS        metropolis.f90:19-58 + MOVE
+ *                              (move.f90:19-50)
  *
  * bromoc_step / bromoc_dynamics move the ions with DYNAMICS0, DYNAMICS1
- * (dynamics1.mono.f90:19-90, flag BROMOC_QDIFFUSE) or DYNAMICS2 (dynamics2.mono.f90:19-89,
- * BROMOC_QPROFILE), as simul1.f90:369-378 selects; ENERGY and INTERACT include RFPARION /
- * RFPARIONJ (rfpar.f90:20-370) under BROMOC_QRFPAR.
+ * (dynamics1.mono.f90:19-90, flag BROMOC_QDIFFUSE), DYNAMICS2 (dynamics2.mono.f90:19-89,
+ * BROMOC_QPROFILE) or DYNAMICS3 (dynamics3.mono.f90:19-75, BROMOC_QPROXDIFF), as
+ * simul1.f90:369-380 selects; bromoc_step also applies NOTRANS (simul1.f90:382-386,
+ * BROMOC_QNOTRANS) after each step.  ENERGY and INTERACT include RFPARION / RFPARIONJ
+ * (rfpar.f90:20-370) under BROMOC_QRFPAR, the B-spline repulsion map (vdwgd1spln.f90:19-
+ * 170, vdwgd0spln.f90) under BROMOC_QBSPLINE; ENERGY includes APFOR and CONTRANS
+ * (energy.mono.f90:509-580) under BROMOC_QAPFOR / BROMOC_QCONTRANS.
  *
  * Conventions
  *  - Particle indices are 0-based (C index i is Fortran index i+1).  Sites occupy
@@ -48,7 +54,7 @@
 extern "C" {
 #endif
 
-#define BROMOC_ABI_VERSION 2
+#define BROMOC_ABI_VERSION 3
 
 /* status codes */
 #define BROMOC_OK              0
@@ -75,16 +81,19 @@ extern "C" {
 #define BROMOC_QPROFILE    (1u << 29)  /* DYNAMICS2: spline diffusion profile (Qprofile) */
 #define BROMOC_QRFPAR      (1u << 25)  /* reaction-field parameter grids, rfpar.f90 (Qrfpar) */
 #define BROMOC_QRFPSIN     (1u << 30)  /* one reaction-field grid for every ion type (Qrfpsin) */
+#define BROMOC_QBSPLINE    (1u << 18)  /* repulsion map by 27-point B-spline (Qbspline:
+                                        * vdwgd1spln.f90, vdwgd0spln.f90) instead of the
+                                        * trilinear stencil; needs QPHIV */
+#define BROMOC_QPROXDIFF   (1u << 21)  /* DYNAMICS3: DNA-proximity diffusion (Qproxdiff) */
+#define BROMOC_QNOTRANS    (1u << 22)  /* NOTRANS: DNA centroid reset after each step of
+                                        * bromoc_step (simul1.f90:382-386) */
+#define BROMOC_QAPFOR      (1u << 23)  /* APFOR: constant forces on sites (Qapfor) */
+#define BROMOC_QCONTRANS   (1u << 24)  /* CONTRANS: harmonic site/centroid restraints */
 /* options of the reference that this path rejects with BROMOC_E_UNSUPPORTED */
 #define BROMOC_QMEMB       (1u << 16)  /* analytic membrane (membrane.f90) */
 #define BROMOC_QMMIJ       (1u << 17)  /* MMIJ reaction field (rect_rf1/sphe_rf1) */
-#define BROMOC_QBSPLINE    (1u << 18)  /* 27-point B-spline phiv (vdwgd1spln.f90) */
 #define BROMOC_QNMCDEN     (1u << 19)  /* per-species phiv maps (Qnmcden) */
 #define BROMOC_QDEBYHYB    (1u << 20)  /* hybrid Debye (Qdebyhyb) */
-#define BROMOC_QPROXDIFF   (1u << 21)  /* proximity diffusion, DYNAMICS3 */
-#define BROMOC_QNOTRANS    (1u << 22)  /* NOTRANS centroid reset */
-#define BROMOC_QAPFOR      (1u << 23)  /* constant applied forces */
-#define BROMOC_QCONTRANS   (1u << 24)  /* centroid restraints */
 #define BROMOC_QSPHERE     (1u << 26)  /* spherical system (fixcoor.f90:24-31) */
 #define BROMOC_QECYL       (1u << 27)  /* elliptic-cylinder system (fixcoor.f90:33-46) */
 
@@ -176,6 +185,23 @@ typedef struct bromoc_system {
     const float *gsrfen, *greff; /* [ngrid * nclx*ncly*nclz], ngrid = 1 with QRFPSIN, else
                                   * ntype - nold (one grid per ion type, in type order) */
     double sqrfac;               /* scale of greff (gsbpmod sqrfac) */
+    /* APFOR (apfmod, energy.mono.f90:510-518): force af(1:3,k) on site sn(k) */
+    int32_t afn, ctn;            /* APFOR entries; CONTRANS entries */
+    const int32_t *sn;           /* [afn] 0-based site index */
+    const double  *af;           /* [3*afn] */
+    /* CONTRANS (nucleotmod, energy.mono.f90:520-580): entry k restrains site csn(k)
+     * (1-based as the reference stores it; 0 = the centroid of all sites, or with unsplit
+     * of each strand) along the axes whose k* is nonzero */
+    const int32_t *csn;          /* [ctn] */
+    const double  *kx, *ky, *kz; /* [ctn] */
+    const double  *contrx, *contry, *contrz;   /* [ctn+1]; entry ctn: the second strand */
+    int32_t unsplit, nsites1st;  /* Qunsplit; sites of the first strand */
+    /* NOTRANS (nucleotmod notr*): axes bit 0 x, 1 y, 2 z (Qnotrx, Qnotry, Qnotrz) */
+    int32_t notrans_axes, reserved3;
+    double notrx, notry, notrz;
+    /* PROXDIFF (grandmod): 1/beta, diff0, diffcutoff of DYNAMICS3 */
+    double ibeta, diff0, diffcutoff;
+    double thold27;              /* B-spline wall threshold (vdwgd1spln.f90:151) */
 } bromoc_system;
 
 /* Energy terms as in grandmod/gsbpmod/nucleotmod (energy.mono.f90:46-68,689). */
@@ -266,6 +292,17 @@ int bromoc_get_ibuffer(bromoc_ctx *ctx, int32_t *ibuffer);
  * [nbuffer][capacity+1] (prob(ib,nat) of grand.f90:114-116), ener is updated. */
 int bromoc_grand(bromoc_ctx *ctx, int32_t ngcmc, const double *uniforms, int64_t nuniforms,
                  int64_t *used, int32_t *nat, int32_t *prob, int64_t icycle, double *ener);
+
+/* METROPOLIS(nmcm) (metropolis.f90:31-55 with MOVE, move.f90:32-47, rectangular box):
+ * nmcm trials, each moving one random moving ion by up to mcmax/2 per axis (mcmax as
+ * grandmod holds it: twice the MCMAX keyword), accepted when the INTERACT energy falls or
+ * with probability exp(-dE ikbt).  uniforms[] supplies every rndm() in call order.  A trial
+ * runs only when its MOVE draws and one more uniform are available: otherwise the call
+ * stops there with *done < nmcm (no state changed by the unfinished trial); *used counts
+ * the uniforms of the finished trials.  *ener is updated as grandmod ener. */
+int bromoc_metropolis(bromoc_ctx *ctx, int32_t nmcm, double mcmax, double ikbt,
+                      const double *uniforms, int64_t nuniforms, int64_t *used, int32_t *done,
+                      double *ener);
 
 /* EFPOT table from (r,U) samples at uniform spacing: writes ep[3*n] and pair fields
  * dmi, dm2lo, dm2hi, sc1..sc3 (host-side, splinepot.f90:19-96). */

@@ -9,12 +9,16 @@
  * order (file:line cited at each function).  Built with -O2 -ffp-contract=off so that each
  * Fortran expression rounds as written.  Indices are 0-based (Fortran index - 1).
  *
- * Parity pins (what checks each function; see tests/test_oracle.py):
- *   getyd/gety, squarespline/splinepot, ENERGY, STATICF1, VDWGD1TRLN, DYNAMICS0/0NUC,
- *   FIXCOOR, COUNTIONS, INTERACT, STATICF0, VDWGD0TRLN, GRAND: against the reference
- *   Fortran compiled by oracle/Makefile (oracle/_ref), plus closed forms and invariants
- *   (Newton's third law, F = -grad E by central differences, ENERGY = 1/2 sum INTERACT,
- *   exact trilinear interpolation of linear fields, closed-form BD step, acceptance rule).
+ * Parity pins (what checks each function; tests/test_oracle_vs_reference.py and
+ * tests/test_oracle_math.py):
+ *   getyd/gety, squarespline/splinepot, ENERGY, STATICF1, VDWGD1TRLN, VDWGD1SPLN,
+ *   APFOR/CONTRANS, the PROXDIFF records, DYNAMICS0/0NUC/1/2/3, NOTRANS, FIXCOOR, COUNTIONS,
+ *   SPLINE/SEVAL, RFPARION/RFPARIONJ, INTERACT, STATICF0, VDWGD0TRLN, VDWGD0SPLN, GRAND,
+ *   METROPOLIS: against the reference Fortran compiled by oracle/Makefile (oracle/_ref),
+ *   plus closed forms and invariants (Newton's third law, F = -grad E by central
+ *   differences, ENERGY = 1/2 sum INTERACT, exact trilinear interpolation of linear fields,
+ *   the B-spline's reproduction of a linear map, closed-form BD step, acceptance rule,
+ *   METROPOLIS energy bookkeeping).
  *   philox4x64: bit-exact against numpy.random.Philox (a published library routine).
  */
 #include <math.h>
@@ -65,6 +69,11 @@ typedef struct orc_ctx {
     /* RFPAR */
     float *gsrfen, *greff;
     double *srfe, *reff, *srfed, *reffd;   /* per ion slot; derivatives [3*cap] */
+    /* PROXDIFF: nearest-site record dids(1:5,i) of ENERGY, [5*cap] */
+    double *dids;
+    /* APFOR / CONTRANS */
+    int32_t *sn, *csn;
+    double *af, *kx, *ky, *kz, *contrx, *contry, *contrz;
 } orc_ctx;
 
 static void *dupmem(const void *p, size_t n) {
@@ -343,7 +352,8 @@ EXPORT void orc_destroy(orc_ctx *c) {
                   c->nbackward, c->noise, c->ibfftyp, c->ntotat, c->ibuffer, c->avnum, c->mu,
                   c->kb, c->lzmin, c->lzmax, c->bufferbias, c->diffusion, c->ampl3, c->p3,
                   c->xs, c->ys, c->bs, c->cs, c->ds, c->gsrfen, c->greff, c->srfe, c->reff,
-                  c->srfed, c->reffd};
+                  c->srfed, c->reffd, c->dids, c->sn, c->csn, c->af, c->kx, c->ky, c->kz,
+                  c->contrx, c->contry, c->contrz};
     for (size_t k = 0; k < sizeof(ps) / sizeof(ps[0]); k++) free(ps[k]);
     for (int k = 0; k < 8; k++) free(c->lists[k]);
     for (int k = 0; k < 6; k++) free(c->lvals[k]);
@@ -429,6 +439,22 @@ EXPORT orc_ctx *orc_create(const bromoc_system *s) {
     c->reff = (double *)calloc(cap + 1, sizeof(double));
     c->srfed = (double *)calloc(3 * (size_t)cap + 3, sizeof(double));
     c->reffd = (double *)calloc(3 * (size_t)cap + 3, sizeof(double));
+    c->dids = (double *)calloc(5 * (size_t)cap + 5, sizeof(double));
+    if ((s->flags & BROMOC_QPROXDIFF) && !c->diffusion)
+        c->diffusion = (double *)dupmem(s->diffusion, sizeof(double) * (s->ntype + 1));
+    if (s->flags & BROMOC_QAPFOR) {
+        c->sn = (int32_t *)dupmem(s->sn, sizeof(int32_t) * s->afn);
+        c->af = (double *)dupmem(s->af, sizeof(double) * 3 * s->afn);
+    }
+    if (s->flags & BROMOC_QCONTRANS) {
+        c->csn = (int32_t *)dupmem(s->csn, sizeof(int32_t) * s->ctn);
+        c->kx = (double *)dupmem(s->kx, sizeof(double) * s->ctn);
+        c->ky = (double *)dupmem(s->ky, sizeof(double) * s->ctn);
+        c->kz = (double *)dupmem(s->kz, sizeof(double) * s->ctn);
+        c->contrx = (double *)dupmem(s->contrx, sizeof(double) * (s->ctn + 1));
+        c->contry = (double *)dupmem(s->contry, sizeof(double) * (s->ctn + 1));
+        c->contrz = (double *)dupmem(s->contrz, sizeof(double) * (s->ctn + 1));
+    }
     return c;
 }
 
@@ -478,7 +504,9 @@ EXPORT int orc_get_warn(orc_ctx *c, int32_t *warn) {
 static uint32_t supported_mask(void) {
     return BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BROMOC_QNOBOND | BROMOC_QPHIX |
            BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV | BROMOC_QDEBY | BROMOC_QCOUNTION |
-           BROMOC_QDNAFREE | BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR | BROMOC_QRFPSIN;
+           BROMOC_QDNAFREE | BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR | BROMOC_QRFPSIN |
+           BROMOC_QBSPLINE | BROMOC_QPROXDIFF | BROMOC_QNOTRANS | BROMOC_QAPFOR |
+           BROMOC_QCONTRANS;
 }
 
 EXPORT int orc_check_flags(uint32_t flags) {
@@ -608,6 +636,96 @@ static void vdwgd1trln(orc_ctx *c, int qforces, double *evdwgd) {
             }
         }
         if (phisum >= s->thold8) c->warn[itype2] = c->warn[itype2] + 1;
+        if (qforces) {
+            c->fx[i] = c->fx[i] + vdwfx;
+            c->fy[i] = c->fy[i] + vdwfy;
+            c->fz[i] = c->fz[i] + vdwfz;
+        }
+    }
+}
+
+/* M3 / DM3, m3.f90:19-42, dm3.f90:19-38 */
+static double m3(double x) {
+    double m2m = (x >= 0.0 && x <= 2.0) ? 1.0 - fabs(x - 1.0) : 0.0;
+    double m2 = (x >= 1.0 && x <= 3.0) ? 1.0 - fabs(x - 2.0) : 0.0;
+    return (x >= 0.0 && x <= 3.0) ? x * 0.5 * m2m + (3.0 - x) * 0.5 * m2 : 0.0;
+}
+static double dm3(double x) {
+    double m2m = (x >= 0.0 && x <= 2.0) ? 1.0 - fabs(x - 1.0) : 0.0;
+    double m2 = (x >= 1.0 && x <= 3.0) ? 1.0 - fabs(x - 2.0) : 0.0;
+    return m2m - m2;
+}
+
+/* NINT: nearest integer, halves away from zero */
+static int nint(double a) { return (int)round(a); }
+
+/* VDWGD1SPLN, vdwgd1spln.f90:19-170 (without the Qnmcden per-species maps) */
+static void vdwgd1spln(orc_ctx *c, int qforces, double *evdwgd) {
+    const bromoc_system *s = &c->s;
+    const bromoc_grid *g = &s->phiv;
+    int nclx2 = g->nclx, ncly2 = g->ncly, nclz2 = g->nclz;
+    int ncyz = ncly2 * nclz2;
+    double dcel2 = g->dcel, idcel2 = 1.0 / dcel2;
+    double tranx2 = 0.5 * (nclx2 - 1) * dcel2, trany2 = 0.5 * (ncly2 - 1) * dcel2,
+           tranz2 = 0.5 * (nclz2 - 1) * dcel2;
+    double esvdw = s->svdw;
+    *evdwgd = 0.0;
+    for (int i = 0; i < s->ntot; i++) {
+        if (!(i < s->nsites || i >= s->nsites + s->nfix)) continue;
+        int ok = c->x[i] <= g->xbcen + tranx2 - dcel2 && c->x[i] >= g->xbcen - tranx2 + dcel2 &&
+                 c->y[i] <= g->ybcen + trany2 - dcel2 && c->y[i] >= g->ybcen - trany2 + dcel2 &&
+                 c->z[i] <= s->vzmax && c->z[i] >= s->vzmin;
+        if (!ok) continue;
+        double xi = c->x[i] + tranx2 - g->xbcen;
+        double yi = c->y[i] + trany2 - g->ybcen;
+        double zi = c->z[i] + tranz2 - g->zbcen;
+        int itype2 = (i < s->nsites) ? c->typtyp[i] : c->nwtype[abs(c->typei[i])];
+        if (s->flags & BROMOC_QSVDW) esvdw = s->svdw * c->scal[itype2];
+        double vdwfx = 0.0, vdwfy = 0.0, vdwfz = 0.0;
+        int ix = nint(xi * idcel2), iy = nint(yi * idcel2), iz = nint(zi * idcel2);
+        if (ix == 0) ix = 1;
+        if (iy == 0) iy = 1;
+        if (iz == 0) iz = 1;
+        if (ix == nclx2 - 1) ix = nclx2 - 2;
+        if (iy == ncly2 - 1) iy = ncly2 - 2;
+        if (iz == nclz2 - 1) iz = nclz2 - 2;
+        double phisum = 0.0;
+        for (int k = ix - 1; k <= ix + 1; k++) {
+            size_t ipx = (size_t)k * ncyz;
+            double xc = k * dcel2;
+            double ai = 1.5 - (xi - xc) * idcel2;
+            double dai = dm3(ai);
+            ai = m3(ai);
+            if (!(ai != 0.0 || dai != 0.0)) continue;
+            for (int l = iy - 1; l <= iy + 1; l++) {
+                size_t ipy = (size_t)l * nclz2 + ipx;
+                double yc = l * dcel2;
+                double bi = 1.5 - (yi - yc) * idcel2;
+                double dbi = dm3(bi);
+                bi = m3(bi);
+                if (!(bi != 0.0 || dbi != 0.0)) continue;
+                for (int m = iz - 1; m <= iz + 1; m++) {
+                    size_t ipz = m + ipy;
+                    double zc = m * dcel2;
+                    double ci = 1.5 - (zi - zc) * idcel2;
+                    double dci = dm3(ci);
+                    ci = m3(ci);
+                    if (!(ci != 0.0 || dci != 0.0)) continue;
+                    double fi = ai * bi * ci;
+                    double phivi = (double)c->phiv[ipz];
+                    double phivsv = phivi * esvdw;
+                    phisum = phisum + phivi;
+                    if (qforces) {
+                        double prefac = phivsv * idcel2;
+                        vdwfx = vdwfx + dai * bi * ci * prefac;
+                        vdwfy = vdwfy + ai * dbi * ci * prefac;
+                        vdwfz = vdwfz + ai * bi * dci * prefac;
+                    }
+                    *evdwgd = *evdwgd + fi * phivsv;
+                }
+            }
+        }
+        if (phisum >= s->thold27) c->warn[itype2] = c->warn[itype2] + 1;
         if (qforces) {
             c->fx[i] = c->fx[i] + vdwfx;
             c->fy[i] = c->fy[i] + vdwfy;
@@ -872,7 +990,8 @@ EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
         E.ener = E.ener + E.egsbpa;
     }
     if (fl & BROMOC_QPHIV) {
-        vdwgd1trln(c, qforces, &E.evdwgd);
+        if (fl & BROMOC_QBSPLINE) vdwgd1spln(c, qforces, &E.evdwgd);
+        else vdwgd1trln(c, qforces, &E.evdwgd);
         E.ener = E.ener + E.evdwgd;
     }
     if (fl & BROMOC_QRFPAR) {   /* energy.mono.f90:111-115 */
@@ -1082,7 +1201,58 @@ EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
             }
         }
     }
-    /* ion-DNA, energy.mono.f90:583-687 */
+    if ((fl & BROMOC_QNUCL) && (fl & BROMOC_QAPFOR)) {   /* APFOR, energy.mono.f90:510-518 */
+        for (int k = 0; k < s->afn; k++) {
+            int j = c->sn[k];
+            if (qforces && c->stfree[j]) {
+                c->fx[j] = c->fx[j] + c->af[3 * k];
+                c->fy[j] = c->fy[j] + c->af[3 * k + 1];
+                c->fz[j] = c->fz[j] + c->af[3 * k + 2];
+            }
+        }
+    }
+    if ((fl & BROMOC_QNUCL) && (fl & BROMOC_QCONTRANS)) {   /* energy.mono.f90:520-580 */
+        double insites = 1.0 / ns;
+        int n1 = s->nsites1st;
+        for (int k = 0; k < s->ctn; k++) {
+            int j = c->csn[k];
+            double *pos[3] = {c->x, c->y, c->z}, *frc[3] = {c->fx, c->fy, c->fz};
+            const double *kk[3] = {c->kx, c->ky, c->kz};
+            const double *cc[3] = {c->contrx, c->contry, c->contrz};
+            for (int ax = 0; ax < 3; ax++) {
+                double kv = kk[ax][k];
+                if (kv == 0.0) continue;
+                double *X = pos[ax], *F = frc[ax];
+                if (j == 0) {
+                    if (s->unsplit) {
+                        double sum = 0.0;
+                        for (int q = 0; q < n1; q++) sum += X[q];
+                        double con = sum * insites * 2.0 - cc[ax][k];
+                        for (int q = 0; q < n1; q++) F[q] = F[q] - kv * con * insites * 2.0;
+                        E.econ = E.econ + 0.5 * kv * (con * con);
+                        sum = 0.0;
+                        for (int q = n1; q < ns; q++) sum += X[q];
+                        con = sum * insites * 2.0 - cc[ax][s->ctn];
+                        for (int q = n1; q < ns; q++) F[q] = F[q] - kv * con * insites * 2.0;
+                        E.econ = E.econ + 0.5 * kv * (con * con);
+                    } else {
+                        double sum = 0.0;
+                        for (int q = 0; q < ns; q++) sum += X[q];
+                        double con = sum * insites - cc[ax][k];
+                        for (int q = 0; q < ns; q++) F[q] = F[q] - kv * con * insites;
+                        E.econ = E.econ + 0.5 * kv * (con * con);
+                    }
+                } else {
+                    double d = X[j - 1] - cc[ax][k];
+                    F[j - 1] = F[j - 1] - kv * d;
+                    E.econ = E.econ + 0.5 * kv * (d * d);
+                }
+            }
+        }
+    }
+    /* ion-DNA, energy.mono.f90:583-687; the nearest-site record dids (:583,603-620) */
+    for (int i = ns; i < nt; i++)
+        for (int q = 0; q < 5; q++) c->dids[5 * i + q] = 0.0;
     if ((fl & BROMOC_QPAR) && (fl & BROMOC_QNUCL) && (fl & BROMOC_QNONBOND)) {
         for (int j = 0; j < ns; j++) {
             int jtype2 = c->typtyp[j];
@@ -1097,6 +1267,17 @@ EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
                     if (ok2) getyd(c, is, dist2, &eefp, &de, &dist);
                     else gety(c, is, dist2, &eefp, &dist);
                     E.eefpotmx = E.eefpotmx + eefp;
+                    if ((fl & BROMOC_QPROXDIFF) && dist > 0.0) {
+                        double *dd = c->dids + 5 * i;
+                        double didstmp = dist - c->pairs[is].dmi;
+                        if (j == 0 || didstmp < dd[0]) {
+                            dd[0] = didstmp;
+                            dd[1] = dist;
+                            dd[2] = c->x[i] - c->x[j];
+                            dd[3] = c->y[i] - c->y[j];
+                            dd[4] = c->z[i] - c->z[j];
+                        }
+                    }
                 } else {
                     de = pair_analytic(c, is, dist2, ok2, &E.eqqmx, &E.evdwmx, &E.esrpmfmx);
                 }
@@ -1230,7 +1411,42 @@ static void dynamics_pass(orc_ctx *c, const double *g) {
     const bromoc_system *s = &c->s;
     double bdmax = s->bdmax;
     int dyn12 = (s->flags & (BROMOC_QDIFFUSE | BROMOC_QPROFILE)) != 0;
-    if (s->flags & BROMOC_QPAR) {
+    if ((s->flags & BROMOC_QPAR) && !dyn12 && (s->flags & BROMOC_QPROXDIFF)) {
+        /* DYNAMICS3, dynamics3.mono.f90:37-72 */
+        for (int i = s->nsites + s->nfix; i < s->ntot; i++) {
+            int itype = abs(c->typei[i]);
+            double zold = c->z[i];
+            const double *dd = c->dids + 5 * i;
+            double fact1, fact2, delDx, delDy, delDz;
+            if (dd[0] > 0.0 && dd[0] <= s->diffcutoff) {
+                double ebet = exp(-dd[0] * s->ibeta);
+                double D = c->diffusion[itype];
+                double idiffusion = D - (D - s->diff0) * ebet;
+                double didiffusion = (D - s->diff0) * s->ibeta * ebet;
+                double idist = 1.0 / dd[1];
+                fact1 = idiffusion * s->kbtdt;
+                fact2 = sqrt(2.0 * s->dt * idiffusion);
+                delDx = didiffusion * s->dt * dd[2] * idist;
+                delDy = didiffusion * s->dt * dd[3] * idist;
+                delDz = didiffusion * s->dt * dd[4] * idist;
+            } else {
+                fact1 = c->fact1a[itype];
+                fact2 = c->fact2a[itype];
+                delDx = delDy = delDz = 0.0;
+            }
+            double delx = c->fx[i] * fact1;
+            double dely = c->fy[i] * fact1;
+            double delz = c->fz[i] * fact1;
+            if (fabs(delx) > bdmax) delx = fsign(bdmax, delx);
+            if (fabs(dely) > bdmax) dely = fsign(bdmax, dely);
+            if (fabs(delz) > bdmax) delz = fsign(bdmax, delz);
+            c->x[i] = c->x[i] + delx + fact2 * g[3 * i + 0] + delDx;
+            c->y[i] = c->y[i] + dely + fact2 * g[3 * i + 1] + delDy;
+            c->z[i] = c->z[i] + delz + fact2 * g[3 * i + 2] + delDz;
+            if (s->flags & BROMOC_QCOUNTION) countions(c, zold, c->z[i], itype - s->nold - 1);
+            fixcoor(s, &c->x[i], &c->y[i], &c->z[i]);
+        }
+    } else if (s->flags & BROMOC_QPAR) {
         for (int i = s->nsites + s->nfix; i < s->ntot; i++) {
             int itype = abs(c->typei[i]);
             double fact1, fact2, delDz;
@@ -1339,7 +1555,24 @@ EXPORT int orc_dynamics(orc_ctx *c, int32_t mode, int32_t row, uint64_t seed, ui
     return st;
 }
 
-/* simul1.f90:363-390 with Qforces on: ENERGY then DYNAMICS0 then DYNAMICS0NUC */
+/* NOTRANS after DYNAMICS0NUC, simul1.f90:382-386 */
+static void notrans(orc_ctx *c) {
+    const bromoc_system *s = &c->s;
+    int ns = s->nsites;
+    if (!(s->flags & BROMOC_QNOTRANS) || !(s->flags & BROMOC_QNUCL) || ns < 1) return;
+    double insites = 1.0 / ns;
+    double *pos[3] = {c->x, c->y, c->z};
+    const double notr[3] = {s->notrx, s->notry, s->notrz};
+    for (int ax = 0; ax < 3; ax++) {
+        if (!(s->notrans_axes & (1 << ax))) continue;
+        double *X = pos[ax], sum = 0.0;
+        for (int q = 0; q < ns; q++) sum += X[q];
+        for (int q = 0; q < ns; q++) X[q] = X[q] - sum * insites + notr[ax];
+    }
+}
+
+/* simul1.f90:363-390 with Qforces on: ENERGY then DYNAMICS0/1/2/3 then DYNAMICS0NUC and
+ * NOTRANS */
 EXPORT int orc_step(orc_ctx *c, int32_t nsteps, int32_t mode, int32_t first_row, uint64_t seed,
                     uint64_t counter0, bromoc_energies *energies) {
     double *g = (double *)malloc(sizeof(double) * 3 * (c->s.ntot + 1));
@@ -1350,6 +1583,7 @@ EXPORT int orc_step(orc_ctx *c, int32_t nsteps, int32_t mode, int32_t first_row,
         st = fill_noise(c, mode, first_row + s, seed, counter0 + (uint64_t)s, g);
         if (st) break;
         dynamics_pass(c, g);
+        notrans(c);
     }
     free(g);
     return st;
@@ -1443,6 +1677,60 @@ static double vdwgd0trln(orc_ctx *c, double xj, double yj, double zj, int jtype2
     return evdwgd;
 }
 
+/* VDWGD0SPLN, vdwgd0spln.f90:19-130 (without Qnmcden) */
+static double vdwgd0spln(orc_ctx *c, double xj, double yj, double zj, int jtype2, int alert) {
+    const bromoc_system *s = &c->s;
+    const bromoc_grid *g = &s->phiv;
+    int nclx2 = g->nclx, ncly2 = g->ncly, nclz2 = g->nclz;
+    int ncyz = ncly2 * nclz2;
+    double dcel2 = g->dcel, idcel2 = 1.0 / dcel2;
+    double tranx2 = 0.5 * (nclx2 - 1) * dcel2, trany2 = 0.5 * (ncly2 - 1) * dcel2,
+           tranz2 = 0.5 * (nclz2 - 1) * dcel2;
+    double evdwgd = 0.0;
+    int ok = xj <= g->xbcen + tranx2 && xj >= g->xbcen - tranx2 && yj <= g->ybcen + trany2 &&
+             yj >= g->ybcen - trany2 && zj <= s->vzmax && zj >= s->vzmin;
+    if (!ok) return evdwgd;
+    double xi = xj + tranx2 - g->xbcen, yi = yj + trany2 - g->ybcen, zi = zj + tranz2 - g->zbcen;
+    double esvdw = s->svdw;
+    if (s->flags & BROMOC_QSVDW) esvdw = esvdw * c->scal[jtype2];
+    int ix = nint(xi * idcel2), iy = nint(yi * idcel2), iz = nint(zi * idcel2);
+    if (ix == 0) ix = 1;
+    if (iy == 0) iy = 1;
+    if (iz == 0) iz = 1;
+    if (ix == nclx2 - 1) ix = nclx2 - 2;
+    if (iy == ncly2 - 1) iy = ncly2 - 2;
+    if (iz == nclz2 - 1) iz = nclz2 - 2;
+    double phisum = 0.0;
+    for (int k = ix - 1; k <= ix + 1; k++) {
+        size_t ipx = (size_t)k * ncyz;
+        double xc = k * dcel2;
+        double ai = m3(1.5 - (xi - xc) * idcel2);
+        if (ai == 0.0) continue;
+        for (int l = iy - 1; l <= iy + 1; l++) {
+            size_t ipy = (size_t)l * nclz2 + ipx;
+            double yc = l * dcel2;
+            double bi = m3(1.5 - (yi - yc) * idcel2);
+            if (bi == 0.0) continue;
+            for (int m = iz - 1; m <= iz + 1; m++) {
+                size_t ipz = m + ipy;
+                double zc = m * dcel2;
+                double ci = 1.5 - (zi - zc) * idcel2;
+                double fi = ai * bi * m3(ci);
+                if (fi != 0.0) {
+                    double phis = (double)c->phiv[ipz];
+                    phisum = phisum + phis;
+                    evdwgd = evdwgd + fi * esvdw * phis;
+                }
+            }
+        }
+    }
+    if (phisum >= s->thold27) {
+        evdwgd = 1.0e10;
+        if (alert) c->warn[jtype2] = c->warn[jtype2] + 1;
+    }
+    return evdwgd;
+}
+
 /* energy of one analytic pair as INTERACT writes it (interact.f90:184-204, 365-383):
  * SRPMF smoothing is e*exp(...)-srpy there, unlike ENERGY's e*(exp(...)-srpy). */
 static double interact_analytic(const orc_ctx *c, int is, double dist2) {
@@ -1479,7 +1767,10 @@ EXPORT int orc_interact(orc_ctx *c, double xj, double yj, double zj, int32_t jty
      * outside its array (rfpar.f90:356); sites get no reaction-field term here */
     if ((fl & BROMOC_QRFPAR) && beion) dener = dener + rfparionj(c, xj, yj, zj, j, jtype);
     if (fl & BROMOC_QPHIX) dener = dener + staticf0(c, xj, yj, zj, j, jtype);
-    if (fl & BROMOC_QPHIV) dener = dener + vdwgd0trln(c, xj, yj, zj, jtype2, alert);
+    if (fl & BROMOC_QPHIV) {
+        if (fl & BROMOC_QBSPLINE) dener = dener + vdwgd0spln(c, xj, yj, zj, jtype2, alert);
+        else dener = dener + vdwgd0trln(c, xj, yj, zj, jtype2, alert);
+    }
     if (beion && (fl & BROMOC_QNONBOND)) {
         for (int i = ns; i < nt; i++) {
             if (i == j) continue;
@@ -1746,5 +2037,62 @@ EXPORT int orc_grand(orc_ctx *c, int32_t ngcmc, const double *uniforms, int64_t 
         if (st.err) return BROMOC_E_ARG;
     }
     if (used) *used = st.k;
+    return BROMOC_OK;
+}
+
+/* METROPOLIS, metropolis.f90:31-55, with MOVE (move.f90:32-47, rectangular box); every
+ * rndm() from `uniforms` in call order.  A trial runs only when its MOVE draws and one more
+ * uniform are available, else the call stops with *done < nmcm. */
+EXPORT int orc_metropolis(orc_ctx *c, int32_t nmcm, double mcmax, double ikbt,
+                          const double *u, int64_t nu, int64_t *used, int32_t *done,
+                          double *ener) {
+    bromoc_system *s = &c->s;
+    int first = s->nsites + s->nfix;
+    double natom2 = (double)(s->ntot - first);
+    int64_t k = 0;
+    int32_t nd = 0;
+    if (natom2 <= 0.0 && nmcm > 0) return BROMOC_E_STATE;
+    for (int im = 0; im < nmcm; im++) {
+        int64_t kk = k;
+        int iat = 0, out = 1;
+        double xnew = 0.0, ynew = 0.0, znew = 0.0;
+        while (out) {
+            if (kk + 4 > nu) break;
+            iat = first + (int)(natom2 * u[kk]);
+            xnew = c->x[iat] + mcmax * (u[kk + 1] - 0.5);
+            ynew = c->y[iat] + mcmax * (u[kk + 2] - 0.5);
+            znew = c->z[iat] + mcmax * (u[kk + 3] - 0.5);
+            kk += 4;
+            out = xnew < s->lx2m || xnew > s->lx2p || ynew < s->ly2m || ynew > s->ly2p ||
+                  znew < s->lz2m || znew > s->lz2p;
+        }
+        if (out || kk + 1 > nu) break;
+        double eold, enew;
+        int rc = orc_interact(c, c->x[iat], c->y[iat], c->z[iat], abs(c->typei[iat]), iat, 1, &eold);
+        if (rc) return rc;
+        rc = orc_interact(c, xnew, ynew, znew, abs(c->typei[iat]), iat, 0, &enew);
+        if (rc) return rc;
+        int acc = enew <= eold;
+        if (!acc) {
+            double bltz = exp(-(enew - eold) * ikbt);
+            acc = u[kk++] < bltz;
+        }
+        if (acc) {
+            c->x[iat] = xnew;
+            c->y[iat] = ynew;
+            c->z[iat] = znew;
+            *ener = *ener + (enew - eold);
+        }
+        k = kk;
+        nd = im + 1;
+    }
+    if (used) *used = k;
+    if (done) *done = nd;
+    return BROMOC_OK;
+}
+
+/* the nearest-site records dids(1:5, i) left by the last ENERGY, [5*ntot] */
+EXPORT int orc_get_dids(orc_ctx *c, double *out) {
+    memcpy(out, c->dids, sizeof(double) * 5 * (size_t)c->s.ntot);
     return BROMOC_OK;
 }

@@ -71,6 +71,9 @@ def oracle_lib():
             ("orc_build_table", [C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp,
                                  P(abi.Pair)], C.c_int),
             ("orc_philox4x64", [P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)], None),
+            ("orc_get_dids", [vp, _dp], C.c_int),
+            ("orc_metropolis", [vp, C.c_int32, C.c_double, C.c_double, _dp, C.c_int64,
+                                P(C.c_int64), _ip, _dp], C.c_int),
             ("orc_philox_gauss3", [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64, _dp], None),
         ]:
             f = getattr(L, name)
@@ -206,6 +209,21 @@ class Oracle(_Base):
         self.ntot = nn.value
         return used.value, e.value
 
+    def dids(self):
+        d = np.zeros(5 * self.system.capacity)
+        self.L.orc_get_dids(self.ctx, _p(d))
+        return d[:5 * self.ntot].reshape(-1, 5)
+
+    def metropolis(self, nmcm, mcmax, ikbt, uniforms, ener=0.0):
+        """METROPOLIS(nmcm); returns (uniforms used, trials done, ener)."""
+        u = np.ascontiguousarray(uniforms, float)
+        used, done, e = C.c_int64(), C.c_int32(), C.c_double(ener)
+        rc = self.L.orc_metropolis(self.ctx, nmcm, mcmax, ikbt, _p(u), len(u), C.byref(used),
+                                   C.byref(done), C.byref(e))
+        if rc:
+            raise abi.BromocError("orc_metropolis", rc)
+        return used.value, done.value, e.value
+
 
 _rlibs = {}
 
@@ -235,6 +253,10 @@ def ref_lib(path=REF_SO):
             ("ref_build_table", [C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp,
                                  P(abi.Pair)], None),
             ("ref_spline", [C.c_int32, _dp, _dp, _dp, _dp, _dp], None),
+            ("ref_notrans", [], None),
+            ("ref_get_dids", [_dp], None),
+            ("ref_metropolis", [C.c_int32, C.c_double, C.c_double, _dp, C.c_int64,
+                                P(C.c_int64), _dp], None),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -341,7 +363,19 @@ class RefFortran(_Base):
         for s in range(nsteps):
             e = self.energy(True)
             self.dynamics(mode, first_row + s, seed, counter0 + s)
+            self.L.ref_notrans()
         return e
+
+    def dids(self):
+        d = np.zeros(5 * self.system.capacity)
+        self.L.ref_get_dids(_p(d))
+        return d[:5 * self.ntot].reshape(-1, 5)
+
+    def metropolis(self, nmcm, mcmax, ikbt, uniforms, ener=0.0):
+        u = np.ascontiguousarray(uniforms, float)
+        used, e = C.c_int64(), C.c_double(ener)
+        self.L.ref_metropolis(nmcm, mcmax, ikbt, _p(u), len(u), C.byref(used), C.byref(e))
+        return used.value, nmcm, e.value
 
     def flux(self):
         s = self.system

@@ -53,6 +53,10 @@ type, bind(C) :: c_system
   type(c_grid) :: rf
   type(c_ptr) :: gsrfen, greff
   real(c_double) :: sqrfac
+  integer(c_int32_t) :: afn, ctn
+  type(c_ptr) :: sn, af, csn, kx, ky, kz, contrx, contry, contrz
+  integer(c_int32_t) :: unsplit, nsites1st, notrans_axes, reserved3
+  real(c_double) :: notrx, notry, notrz, ibeta, diff0, diffcutoff, thold27
 end type
 
 type, bind(C) :: c_gcmc
@@ -66,7 +70,9 @@ real(c_double), allocatable :: gstream(:), ustream(:)
 integer :: gpos = 0, upos = 0, ulen = 0
 integer(c_int32_t), parameter :: F_PAR=1, F_NUCL=2, F_NONBOND=4, F_NOBOND=8, F_PHIX=16, F_PHIV=32, &
      F_SVDW=64, F_SOLV=128, F_DEBY=256, F_COUNTION=512, F_DNAFREE=1024, &
-     F_RFPAR=33554432, F_DIFFUSE=268435456, F_PROFILE=536870912, F_RFPSIN=1073741824
+     F_RFPAR=33554432, F_DIFFUSE=268435456, F_PROFILE=536870912, F_RFPSIN=1073741824, &
+     F_BSPLINE=262144, F_PROXDIFF=2097152, F_NOTRANS=4194304, F_APFOR=8388608, &
+     F_CONTRANS=16777216
 integer :: site_names_code(0:5)
 
 contains
@@ -137,7 +143,7 @@ nfix = s%nfix
 ntype = s%ntype
 nold = s%nold
 nttyp = s%nttyp
-natom = ntot
+natom = ntot - nfix              ! shell_simul.f90:2435
 Qpar = flag(s%flags, F_PAR)
 Qnucl = flag(s%flags, F_NUCL)
 Qnonbond = flag(s%flags, F_NONBOND)
@@ -154,16 +160,16 @@ Qforces = .false.
 Qmemb = .false.
 Qmmij = .false.
 Qrfpar = .false.
-Qtrln = .true.
+Qtrln = .not. flag(s%flags, F_BSPLINE)
 Qnmcden = .false.
-Qproxdiff = .false.
+Qproxdiff = flag(s%flags, F_PROXDIFF)
 Qdebyhyb = .false.
-Qapfor = .false.
-Qcontrans = .false.
+Qapfor = flag(s%flags, F_APFOR)
+Qcontrans = flag(s%flags, F_CONTRANS)
 Qwarn = .false.
 Qsphere = .false.
 Qecyl = .false.
-Qnotrans = .false.
+Qnotrans = flag(s%flags, F_NOTRANS)
 Qdiffuse = flag(s%flags, F_DIFFUSE)
 Qprofile = flag(s%flags, F_PROFILE)
 Qrfpar = flag(s%flags, F_RFPAR)
@@ -259,7 +265,7 @@ if (Qphiv) then
   if (allocated(phiv)) deallocate(phiv)
   allocate(phiv(ncel3))
   call c_f_pointer(s%phiv%data, bp, [ncel3]); phiv = bp
-  svdw = s%svdw; thold8 = s%thold8; vzmin = s%vzmin; vzmax = s%vzmax
+  svdw = s%svdw; thold8 = s%thold8; vzmin = s%vzmin; vzmax = s%vzmax; thold27 = s%thold27
   if (allocated(scal)) deallocate(scal)
   allocate(scal(nttyp))
   if (Qsvdw) then
@@ -275,9 +281,15 @@ warn = 0
 ! position-dependent diffusion (DIFFUSION / PROFILE commands, simul1.f90:342)
 dt = s%dt
 kbtdt = s%kbtdt
-if (Qdiffuse .or. Qprofile) then
+if (Qdiffuse .or. Qprofile .or. Qproxdiff) then
   call c_f_pointer(s%diffusion, dp, [ntype+1]); diffusion(1:ntype) = dp(2:ntype+1)
 endif
+! PROXDIFF command (shell_simul.f90:1903-1910)
+if (Qproxdiff) then
+  ibeta = s%ibeta; diff0 = s%diff0; diffcutoff = s%diffcutoff
+endif
+dids = 0.0
+call set_dna_options(s)
 if (Qdiffuse) then
   plength2 = s%plength2; pcenter = s%pcenter
   call c_f_pointer(s%ampl3, dp, [ntype+1]); ampl3(1:ntype) = dp(2:ntype+1)
@@ -333,6 +345,53 @@ call c_f_pointer(s%ys, dp, [n]); ys(1:n) = dp
 call c_f_pointer(s%bs, dp, [n]); sp_b(1:n) = dp
 call c_f_pointer(s%cs, dp, [n]); sp_c(1:n) = dp
 call c_f_pointer(s%ds, dp, [n]); sp_d(1:n) = dp
+end subroutine
+
+! APFOR (apfmod), CONTRANS and NOTRANS (nucleotmod) state as shell_simul.f90:650-770 leaves it
+subroutine set_dna_options(s)
+use ref_harness
+use apfmod
+use nucleotmod
+implicit none
+type(c_system), intent(in) :: s
+integer(c_int32_t), pointer :: ip(:)
+real(c_double), pointer :: dp(:)
+integer :: n
+if (nsites > 0) insites = 1.0/nsites
+if (Qapfor) then
+  n = s%afn
+  afn = n
+  if (allocated(sn)) deallocate(sn, af)
+  allocate(sn(max(1,n)), af(3, max(1,n)))
+  if (n > 0) then
+    call c_f_pointer(s%sn, ip, [n]); sn(1:n) = ip + 1
+    call c_f_pointer(s%af, dp, [3*n]); af(:, 1:n) = reshape(dp, [3, n])
+  endif
+endif
+if (Qcontrans) then
+  n = s%ctn
+  ctn = n
+  if (allocated(kx)) deallocate(kx, ky, kz, csn, contrx, contry, contrz)
+  allocate(kx(max(1,n)), ky(max(1,n)), kz(max(1,n)), csn(max(1,n)), contrx(n+1), contry(n+1), &
+           contrz(n+1))
+  if (n > 0) then
+    call c_f_pointer(s%csn, ip, [n]); csn(1:n) = ip
+    call c_f_pointer(s%kx, dp, [n]); kx(1:n) = dp
+    call c_f_pointer(s%ky, dp, [n]); ky(1:n) = dp
+    call c_f_pointer(s%kz, dp, [n]); kz(1:n) = dp
+  endif
+  call c_f_pointer(s%contrx, dp, [n+1]); contrx(1:n+1) = dp
+  call c_f_pointer(s%contry, dp, [n+1]); contry(1:n+1) = dp
+  call c_f_pointer(s%contrz, dp, [n+1]); contrz(1:n+1) = dp
+  Qunsplit = s%unsplit /= 0
+  nsites1st = s%nsites1st
+endif
+if (Qnotrans) then
+  Qnotrx = iand(s%notrans_axes, 1) /= 0
+  Qnotry = iand(s%notrans_axes, 2) /= 0
+  Qnotrz = iand(s%notrans_axes, 4) /= 0
+  notrx = s%notrx; notry = s%notry; notrz = s%notrz
+endif
 end subroutine
 
 ! SPLINE (spline.f90:19-98) on caller arrays
@@ -595,11 +654,62 @@ if (Qpar) then
     call dynamics1(nsites+nfix+1, ntot)
   elseif (Qprofile) then
     call dynamics2(nsites+nfix+1, ntot)
+  elseif (Qproxdiff) then
+    call dynamics3(nsites+nfix+1, ntot)
   else
     call dynamics0(nsites+nfix+1, ntot)
   endif
 endif
 if (Qnucl) call dynamics0nuc(1, nsites)
+end subroutine
+
+subroutine ref_get_dids(o) bind(C, name='ref_get_dids')
+use ref_harness
+use grandmod
+implicit none
+real(c_double), intent(out) :: o(5, *)
+o(:, 1:ntot) = dids(:, 1:ntot)
+end subroutine
+
+! the NOTRANS lines of the driver's step loop, simul1.f90:382-386
+subroutine ref_notrans() bind(C, name='ref_notrans')
+use ref_harness
+use grandmod
+use nucleotmod
+implicit none
+if (Qnucl .and. Qnotrans) then
+  if (Qnotrx) x(1:nsites)=x(1:nsites)-sum(x(1:nsites))*insites+notrx
+  if (Qnotry) y(1:nsites)=y(1:nsites)-sum(y(1:nsites))*insites+notry
+  if (Qnotrz) z(1:nsites)=z(1:nsites)-sum(z(1:nsites))*insites+notrz
+endif
+end subroutine
+
+! METROPOLIS (metropolis.f90:19-58) with injected uniforms; natom follows ntot
+subroutine ref_metropolis(nmcm, mcmax_in, ikbt_in, u, nu, used, ener_io) &
+    bind(C, name='ref_metropolis')
+use ref_harness
+use grandmod
+implicit none
+integer(c_int32_t), value :: nmcm
+real(c_double), value :: mcmax_in, ikbt_in
+integer(c_int64_t), value :: nu
+real(c_double), intent(in) :: u(*)
+integer(c_int64_t), intent(out) :: used
+real(c_double), intent(inout) :: ener_io
+integer :: n
+if (allocated(ustream)) deallocate(ustream)
+allocate(ustream(max(1_8, nu)))
+ustream(1:nu) = u(1:nu)
+ulen = int(nu)
+upos = 0
+mcmax = mcmax_in
+ikbt = ikbt_in
+natom = ntot - nfix
+ener = ener_io
+n = nmcm
+call metropolis(n)
+ener_io = ener
+used = upos
 end subroutine
 
 ! ------------------------------------------------------------- INTERACT

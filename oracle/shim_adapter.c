@@ -79,6 +79,12 @@ int bromoc_set_ibuffer(bromoc_ctx *ctx, const int32_t *ib) {
     return orc_set_ibuffer(O(ctx), ib);
 }
 int bromoc_get_ibuffer(bromoc_ctx *ctx, int32_t *ib) { return orc_get_ibuffer(O(ctx), ib); }
+int orc_metropolis(orc_ctx *c, int32_t nmcm, double mcmax, double ikbt, const double *u,
+                   int64_t nu, int64_t *used, int32_t *done, double *ener);
+int bromoc_metropolis(bromoc_ctx *ctx, int32_t nmcm, double mcmax, double ikbt,
+                      const double *u, int64_t nu, int64_t *used, int32_t *done, double *ener) {
+    return orc_metropolis(O(ctx), nmcm, mcmax, ikbt, u, nu, used, done, ener);
+}
 int bromoc_grand(bromoc_ctx *ctx, int32_t ng, const double *u, int64_t nu, int64_t *used,
                  int32_t *nat, int32_t *prob, int64_t icyc, double *ener) {
     return orc_grand(O(ctx), ng, u, nu, used, nat, prob, icyc, ener);

@@ -47,3 +47,16 @@ def with_fixed(s, nfix, nonfree=()):
     for k in nonfree:
         t.stfree[k] = 0
     return t
+
+
+def near_wall(s, k, seed=0, rpore=12.0):
+    """Copy of s with its last k ions moved to within half an Angstrom of the pore wall
+    (the pore systems' slab spans |z| <= 10 A)."""
+    s = s.copy()
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0, 2 * np.pi, k)
+    r = rpore + rng.uniform(-0.5, 0.5, k)
+    s.x[s.ntot - k:s.ntot] = r * np.cos(th)
+    s.y[s.ntot - k:s.ntot] = r * np.sin(th)
+    s.z[s.ntot - k:s.ntot] = rng.uniform(-9.0, 9.0, k)
+    return s

@@ -62,7 +62,7 @@ def _plain():
 
 
 def test_create_rejects_unsupported_options():
-    for flag in (abi.QMEMB, abi.QMMIJ, abi.QBSPLINE, abi.QSPHERE, abi.QPROXDIFF, abi.QNOTRANS):
+    for flag in (abi.QMEMB, abi.QMMIJ, abi.QNMCDEN, abi.QSPHERE, abi.QECYL, abi.QDEBYHYB):
         s = _plain()
         s.flags |= flag
         rc, _ = _create(s)
@@ -98,6 +98,13 @@ def test_create_rejects_inconsistent_options():
     s = _plain()
     s.flags |= abi.QRFPAR                         # no grids given
     assert _create(s)[0] == abi.E_ARG
+    s = _plain()
+    s.flags |= abi.QBSPLINE                       # no repulsion map
+    assert _create(s)[0] == abi.E_ARG
+    for flag in (abi.QPROXDIFF, abi.QNOTRANS, abi.QAPFOR, abi.QCONTRANS):
+        s = _plain()                              # ions only: these need DNA sites
+        s.flags |= flag
+        assert _create(s)[0] == abi.E_ARG, flag
 
 
 def test_create_without_device_reports_nodevice():

@@ -132,3 +132,41 @@ def test_diffusion_and_rfpar_through_front_end(which):
             a = f.interact(pos[j], abs(typ[j]), j, True)
             b = r.interact(pos[j], abs(typ[j]), j, True)
             assert abs(a - b) <= 1e-14 * max(abs(b), 1.0)
+
+
+@pytest.mark.parametrize("which", ["bspline", "apfor_contrans", "proxdiff"])
+def test_dna_options_through_front_end(which):
+    """B-spline repulsion map (Qtrln off), APFOR + CONTRANS (apfmod, nucleotmod) and
+    PROXDIFF (DYNAMICS3) handed over from the modules: energies, forces and BD steps."""
+    base = with_fixed(small_pore(seed=41), 4, nonfree=(3,))
+    if which == "bspline":
+        s = configs.with_bspline(base, thold27=3.0)
+    elif which == "apfor_contrans":
+        s = configs.with_contrans(configs.with_apfor(base, seed=5, n=4),
+                                  [(0, (2.0, 1.0, 3.0)), (6, (1.0, 0.0, 2.0))], unsplit=True)
+        s.x[:s.nsites] += 0.3
+    else:
+        s = configs.with_proxdiff(base, beta=2.5, diff0=0.05)
+    seed, c0 = 91, 4
+    f, r = _pair(s)
+    f.L.bromoc_gpu_set_seed(seed, c0)
+    for k in range(3):
+        ef = f.energy()
+        f.dynamics(abi.NOISE_PHILOX)
+        er = r.energy()
+        r.dynamics(abi.NOISE_PHILOX, 0, seed, c0 + k)
+        assert max(energy_errs(ef, er, floor=1e-12).values()) <= 1e-14
+    pf, pr = f.positions()[0], r.positions()[0]
+    assert float(np.max(np.abs(pf - pr))) <= 1e-12
+    assert np.array_equal(f.warn(), r.warn())
+
+
+def test_metropolis_through_front_end():
+    s = with_fixed(small_pore(seed=42, n_bp=5, n_ions=90), 6)
+    f, r = _pair(s)
+    u = np.random.default_rng(12).random(30000)
+    ikbt = 1.0 / (0.0019872 * 300.0)
+    _, _, ea = f.metropolis(250, 2.0, ikbt, u, 0.75)
+    _, _, eb = r.metropolis(250, 2.0, ikbt, u, 0.75)
+    assert abs(ea - eb) <= 1e-12 * max(abs(eb), 1.0)
+    assert np.array_equal(f.positions()[0], r.positions()[0])

@@ -446,3 +446,80 @@ def test_rfpar_reff_force_is_not_a_gradient():
     s = _rf_system(const_reff=None)
     worst, _ = _rf_cd(s, range(12))
     assert worst > 1e-4, worst
+
+
+# --------------------------------------------- B-spline map, CONTRANS, METROPOLIS
+def test_bspline_reproduces_linear_map():
+    """The 27-point quadratic B-spline (vdwgd1spln.f90) reproduces a map linear in the node
+    index: E_i = svdw * (x/h + 2y/h - z/h) and F_i = -svdw * (1, 2, -1)/h for every particle
+    inside the map, which fixes the stencil's node indexing, M3, DM3 and the force sign."""
+    s = plain(6, 30).copy()
+    n, h = 33, 2.0
+    k = np.arange(n) - 16
+    X, Y, Z = np.meshgrid(k, k, k, indexing="ij")
+    s.phiv = GridData((X + 2 * Y - Z).astype(np.int8), h)
+    s.flags = (s.flags | abi.QPHIV | abi.QBSPLINE) & ~abi.QNONBOND
+    s.svdw, s.vzmin, s.vzmax, s.thold27 = 3.0, -32.0, 32.0, 1e9
+    o = Oracle(s)
+    e = o.energy()["evdwgd"]
+    pos = o.positions()[0]
+    expect = s.svdw * np.sum(pos[:, 0] / h + 2 * pos[:, 1] / h - pos[:, 2] / h)
+    assert abs(e - expect) <= 1e-12 * np.sum(np.abs(pos)) * s.svdw
+    f = o.forces()
+    g = -s.svdw * np.array([1.0, 2.0, -1.0]) / h
+    assert np.max(np.abs(f - g[None, :])) <= 1e-12
+
+
+def test_contrans_force_is_minus_gradient():
+    """CONTRANS energy econ and forces (energy.mono.f90:520-580) with every other term off:
+    F = -grad econ by central differences (econ is quadratic, so the quotient is exact up
+    to rounding), for site restraints, the centroid and the two-strand (unsplit) form."""
+    for unsplit in (False, True):
+        s = dna_only(seed=4)
+        s.flags &= ~(abi.QNONBOND | abi.QNOBOND)
+        s = configs.with_contrans(s, [(0, (1.5, 2.0, 0.5)), (3, (4.0, 0.0, 1.0)),
+                                      (s.nsites, (0.0, 3.0, 0.0))], unsplit=unsplit)
+        rng = np.random.default_rng(7)
+        s.x[:s.nsites] += rng.normal(0, 0.5, s.nsites)
+        s.y[:s.nsites] += rng.normal(0, 0.5, s.nsites)
+        s.z[:s.nsites] += 0.7
+        o = Oracle(s)
+        e0 = o.energy()["econ"]
+        assert e0 > 0.0
+        f = o.forces()
+        pos, typ = o.positions()
+        h = 1e-4
+        for j in range(s.nsites):
+            for a in range(3):
+                ev = []
+                for sg in (1.0, -1.0):
+                    p = pos.copy()
+                    p[j, a] += sg * h
+                    o.set_positions(p, typ)
+                    ev.append(o.energy()["econ"])
+                cd = -(ev[0] - ev[1]) / (2 * h)
+                assert abs(cd - f[j, a]) <= 1e-7 * max(1.0, abs(f[j, a])), (unsplit, j, a)
+        o.set_positions(pos, typ)
+
+
+def test_metropolis_energy_bookkeeping():
+    """METROPOLIS adds enew - eold of every accepted move to ener (metropolis.f90:41-52).
+    For ions with pair and field terms INTERACT(j) holds every term involving j, so the
+    running ener changes by exactly E(final) - E(initial) of ENERGY; and with ikbt = 0 every
+    move is accepted."""
+    s, _ = _linear_field_system(40)
+    o = Oracle(s)
+    e_in = o.energy()["ener"]
+    u = np.random.default_rng(3).random(40000)
+    ikbt = 1.0 / (0.0019872 * 300.0)
+    used, done, ener = o.metropolis(400, 1.5, ikbt, u, e_in)
+    assert done == 400
+    e_out = o.energy()["ener"]
+    assert abs((ener - e_in) - (e_out - e_in)) <= 1e-10 * max(abs(e_in), 1.0)
+    p0 = o.positions()[0]
+    used2, done2, _ = o.metropolis(50, 1.5, 0.0, u, 0.0)
+    moved = np.any(o.positions()[0] != p0, axis=1).sum()
+    assert done2 == 50 and used2 >= 4 * 50 and moved >= 1
+    # a uniform stream that runs out stops before an unfinished trial
+    u3, d3, _ = o.metropolis(10, 1.5, ikbt, u[:9], 0.0)
+    assert 1 <= d3 <= 2 and 4 * d3 <= u3 <= 9

@@ -11,7 +11,7 @@ from bromoc_amd import abi, configs
 from oracle.oracle import (Oracle, RefFortran, oracle_table_builder, ref_available,
                            ref_table_builder)
 
-from helpers import energy_errs, force_err, with_fixed
+from helpers import energy_errs, force_err, near_wall, with_fixed
 
 pytestmark = pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built here")
 
@@ -210,3 +210,111 @@ def test_rfpar_interact():
     for xyz, jt, j in cases:
         a, b = o.interact(xyz, jt, j, True), r.interact(xyz, jt, j, True)
         assert abs(a - b) <= 1e-14 * max(abs(b), 1.0), (j, a, b)
+
+
+# ---------------------- VDWGD1SPLN / APFOR / CONTRANS / NOTRANS / PROXDIFF / METROPOLIS
+def _energy_match(s):
+    o, r = Oracle(s), RefFortran(s)
+    eo, er = o.energy(), r.energy()
+    assert max(energy_errs(eo, er, floor=1e-12).values()) <= 1e-14
+    assert force_err(o.forces(), r.forces()) <= 1e-14
+    assert np.array_equal(o.warn(), r.warn())
+    return eo, er
+
+
+@pytest.mark.parametrize("thold27", [27.0, 3.0])
+def test_bspline_energy_steps_interact(thold27):
+    """27-point B-spline repulsion (vdwgd1spln.f90, vdwgd0spln.f90); a low threshold
+    exercises the wall warning and INTERACT's 1e10 wall energy."""
+    s = near_wall(with_fixed(small_pore(seed=31), 5, nonfree=(2,)), 12)
+    s = configs.with_bspline(s, thold27)
+    eo, er = _energy_match(s)
+    assert er["evdwgd"] != 0.0
+    _steps_match(s, nsteps=3)
+    o, r = Oracle(s), RefFortran(s)
+    pos, typ = o.positions()
+    cases = [(pos[j], abs(typ[j]) if j >= s.nsites else 0, j)
+             for j in (0, 3, s.nsites - 1, s.nsites + 5, s.ntot - 1)]
+    cases += [((11.5, 0.0, 2.0), s.nold + 1, s.ntot), ((13.0, 1.0, -3.0), s.nold + 2, s.ntot),
+              ((19.9, -19.9, 29.0), s.nold + 1, s.ntot)]
+    for xyz, jt, j in cases:
+        a, b = o.interact(xyz, jt, j, True), r.interact(xyz, jt, j, True)
+        assert abs(a - b) <= 1e-14 * max(abs(b), 1.0), (j, a, b)
+    assert np.array_equal(o.warn(), r.warn())
+
+
+def test_apfor_contrans_energy():
+    s = with_fixed(small_pore(seed=32), 3, nonfree=(1, 4))
+    s = configs.with_apfor(s, seed=2, n=5)
+    s = configs.with_contrans(s, [(0, (2.0, 0.0, 3.5)), (4, (1.5, 2.5, 0.0)),
+                                  (2, (0.0, 0.0, 4.0))])
+    rng = np.random.default_rng(1)           # move the sites off the restraint centres
+    s.x[:s.nsites] += rng.normal(0, 0.3, s.nsites)
+    s.z[:s.nsites] += rng.normal(0, 0.3, s.nsites)
+    eo, er = _energy_match(s)
+    assert er["econ"] > 0.0
+    _steps_match(s, nsteps=3)
+
+
+def test_contrans_unsplit():
+    s = configs.with_contrans(small_pore(seed=33), [(0, (1.0, 2.0, 3.0)), (7, (0.5, 0.0, 0.0))],
+                              unsplit=True)
+    s.y[:s.nsites] += 0.4
+    eo, er = _energy_match(s)
+    assert er["econ"] > 0.0
+    _steps_match(s, nsteps=3)
+
+
+@pytest.mark.parametrize("axes", [7, 4, 3])
+def test_notrans_steps(axes):
+    s = configs.with_notrans(small_pore(seed=34), axes)
+    po = _steps_match(s, nsteps=4)
+    c = np.array([s.x[:s.nsites].mean(), s.y[:s.nsites].mean(), s.z[:s.nsites].mean()])
+    pc = po[:s.nsites].mean(axis=0)
+    for a in range(3):
+        if axes & (1 << a):
+            assert abs(pc[a] - c[a]) <= 1e-12
+        else:
+            assert abs(pc[a] - c[a]) > 1e-6
+
+
+def test_proxdiff_records():
+    """ENERGY's nearest-site records (energy.mono.f90:583,603-620): the oracle's equal the
+    reference's, and the configuration has ions with a record and ions whose first site is
+    outside its table range while other sites are inside (the reference then leaves the
+    record at zero)."""
+    s = configs.with_proxdiff(with_fixed(small_pore(seed=35, n_ions=160), 4))
+    o, r = Oracle(s), RefFortran(s)
+    o.energy()
+    r.energy()
+    do, dr = o.dids(), r.dids()
+    assert np.array_equal(do, dr)
+    pos = o.positions()[0]
+    ions = np.arange(s.nsites, s.ntot)
+    d0 = np.linalg.norm(pos[ions] - pos[0], axis=1)
+    dmin = np.min(np.linalg.norm(pos[ions, None, :] - pos[None, :s.nsites, :], axis=2), axis=1)
+    far0 = (d0 > 20.0) & (dmin < 15.0)
+    assert far0.sum() > 5 and np.all(dr[ions[far0]] == 0.0)
+    assert np.sum(dr[ions, 0] > 0.0) > 20
+
+
+@pytest.mark.parametrize("cut", [None, 6.0])
+def test_proxdiff_steps(cut):
+    base = with_fixed(small_pore(seed=35, n_ions=160), 4)
+    s = configs.with_proxdiff(base, beta=2.5, diff0=0.05, diffcutoff=cut)
+    p3 = _steps_match(s, nsteps=4)
+    p0 = _steps_match(base, nsteps=4)
+    assert float(np.max(np.abs(p3 - p0))) > 1e-3     # DYNAMICS3 moved the ions differently
+
+
+def test_metropolis():
+    s = with_fixed(small_pore(seed=36, n_bp=5, n_ions=90), 6)
+    o, r = Oracle(s), RefFortran(s)
+    u = np.random.default_rng(9).random(20000)
+    ikbt = 1.0 / (0.0019872 * 300.0)
+    ua, da, ea = o.metropolis(300, 2.0, ikbt, u, 1.5)
+    ub, db, eb = r.metropolis(300, 2.0, ikbt, u, 1.5)
+    assert da == 300 and ua == ub
+    assert abs(ea - eb) <= 1e-12 * max(abs(eb), 1.0)
+    assert np.array_equal(o.positions()[0], r.positions()[0])
+    assert np.array_equal(o.warn(), r.warn())
