@@ -138,6 +138,58 @@ def batch_legs(steps=300, gcmc_every=10, ngcmc=20, small_n=200, small_steps=400)
     return full, small
 
 
+def large_leg(cycles=4, gcmc_every=10, ngcmc=20, cpu=True):
+    """The large translocation workload (56,000 particles: 50,000 ions and a 1000-bp
+    CG-DNA of 6000 sites through a 15 A pore; 321x321x641 repulsion, static-field and
+    reaction-field grids; PROFILE diffusion; GCMC bath buffers): `cycles` cycles of GRAND
+    (`ngcmc` attempts) + `gcmc_every` BD steps on the device, and, with cpu, one step of the
+    C oracle on this host (the reference's fixed arrays hold 23000 particles, fewer than this
+    system has)."""
+    import numpy as np
+    from bromoc_amd import abi, configs
+    from bromoc_amd.engine import Engine
+    t0 = time.perf_counter()
+    s = configs.translocation()
+    gen_s = time.perf_counter() - t0
+    g = s.meta["gcmc"]
+    ib, nat = configs.assign_buffers(s, g)
+    e = Engine(s)
+    e.set_gcmc(g, ib)
+    prob = np.zeros((g.nbuffer, s.capacity + 1), np.int32)
+    rng = np.random.default_rng(5)
+    e.step(4, abi.NOISE_PHILOX, seed=11, counter0=0, energies=False)
+    nmov0 = e.ntot - s.nfix
+    ener, ctr = 0.0, 4
+    t0 = time.perf_counter()
+    for c in range(cycles):
+        _, ener = e.grand(ngcmc, rng.random(6 * ngcmc + 8), nat, prob, c + 1, ener)
+        e.step(gcmc_every, abi.NOISE_PHILOX, seed=11, counter0=ctr, energies=False)
+        ctr += gcmc_every
+    el = time.perf_counter() - t0
+    nmov1 = e.ntot - s.nfix
+    _, kms = e.time_steps(5, abi.NOISE_PHILOX, seed=11, counter0=ctr)
+    nsteps = cycles * gcmc_every
+    out = {"value": 0.5 * (nmov0 + nmov1) * nsteps / el, "unit": "particle-steps/s",
+           "particles": [int(nmov0), int(nmov1)], "ions": s.ntot - s.nsites,
+           "dna_sites": s.nsites, "box_A": [200, 200, 600], "grid": list(s.meta["grid"]),
+           "grid_spacing_A": s.meta["dcel"], "rfpar": True, "diffusion": "profile",
+           "steps": nsteps, "gcmc_every": gcmc_every, "ngcmc": ngcmc,
+           "ms_per_step": 1e3 * el / nsteps, "step_kernel_ms": kms,
+           "dna_list_terms": int(sum(s.count(k) for k in s.LIST_NAMES)), "setup_s": gen_s}
+    e.close()
+    if cpu:
+        from oracle.oracle import Oracle
+        o = Oracle(s)
+        t0 = time.perf_counter()
+        o.step(1, abi.NOISE_PHILOX, seed=11, counter0=0)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": (s.ntot - s.nfix) / dt, "unit": "particle-steps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"1 BD step of the same system, C oracle, 1 thread, "
+                                         f"{dt:.1f} s"}
+    return out
+
+
 def copy_probe(mib):
     """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
     import torch
@@ -160,6 +212,8 @@ def main():
     ap.add_argument("--no-parity", action="store_true", help="skip the start-state parity check")
     ap.add_argument("--no-host", action="store_true", help="skip the host round-trip rate")
     ap.add_argument("--no-batch", action="store_true", help="skip the replica-batch legs")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the large translocation leg (56,000 particles)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -226,6 +280,7 @@ def main():
     host = None if args.no_host else host_roundtrip(eng, nmov)
     eng.close()
     batched, small_n = (None, None) if args.no_batch else batch_legs()
+    large = None if args.no_large else large_leg(cpu=not args.no_cpu)
     cpu = None if args.no_cpu else cpu_baseline(s)
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
@@ -239,7 +294,8 @@ def main():
                                    "frac": gflops / 1e3 / 78.6, "pair_evals_per_step": pairs},
                      "targets_per_block": tpb},
         "cpu_baseline": cpu, "host_roundtrip": host, "batched": batched,
-        "batched_vs_plain_small_n": small_n, "parity_vs_oracle": parity,
+        "batched_vs_plain_small_n": small_n, "large_translocation": large,
+        "parity_vs_oracle": parity,
         "setup_s": build_s,
     }
     print(json.dumps(out))

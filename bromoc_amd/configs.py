@@ -182,10 +182,12 @@ def _angle(p1, p2, p3):
     return math.acos((v1 @ v2) / math.sqrt((v1 @ v1) * (v2 @ v2)))
 
 
-def dna_duplex(seq, z0=0.0, solv=True):
+def dna_duplex(seq, z0=0.0, solv=True, layout=None):
     """Native B-form duplex of `seq` (strand 1, 5'->3') plus its complement: site arrays
     and the bonded / nonbonded lists built by the rules of bonds.f90, angles.f90,
-    dihedral.f90 and go_qq.f90:45-116 on the native geometry."""
+    dihedral.f90 and go_qq.f90:45-116 on the native geometry.  layout(pos, step) may move
+    the sites (e.g. fold a long duplex) before the native values are taken; step is each
+    site's base-pair step along the helix."""
     nb = len(seq)
     rise, twist = 3.38, 36.0
     names, code, strand, pos, nuc = [], [], [], [], []
@@ -214,6 +216,10 @@ def dna_duplex(seq, z0=0.0, solv=True):
         k += 1
     pos = np.array(pos)
     pos[:, 2] += z0 - 0.5 * (nb - 1) * rise
+    if layout is not None:
+        steps = np.array([j for j in range(nb) for _ in range(3)] +
+                         [nb - 1 - j for j in range(nb) for _ in range(3)])
+        pos = layout(pos, steps)
     n = len(names)
     bonds = []
     for st in (1, 2):
@@ -223,7 +229,6 @@ def dna_duplex(seq, z0=0.0, solv=True):
             bonds += [(p, s_), (s_, b)]
             if t + 3 < len(idx):
                 bonds.append((s_, idx[t + 3]))
-    bset = {tuple(sorted(b)) for b in bonds}
     nbr = {i: set() for i in range(n)}
     for a, b in bonds:
         nbr[a].add(b)
@@ -234,31 +239,61 @@ def dna_duplex(seq, z0=0.0, solv=True):
         for ii in range(len(ns_)):
             for jj in range(ii + 1, len(ns_)):
                 angles.append((ns_[ii], c_, ns_[jj]))
-    aset = {tuple(sorted((a, d))) for a, _, d in angles}
     dihes = []
     for b_, c_ in bonds:
         for a in sorted(nbr[b_] - {c_}):
             for d in sorted(nbr[c_] - {b_}):
                 if a != d:
                     dihes.append((a, b_, c_, d))
-    comp_pair = {("Gb", "Cb"), ("Cb", "Gb"), ("Ab", "Tb"), ("Tb", "Ab")}
+    # nonbonded lists over every site pair i > j (go_qq.f90:45-116), in (i, j) order
     nm = ["P " if x == "P" else "S " if x == "S" else x + "b" for x in names]
-    stack, bp, ex, qq, slv = [], [], [], [], []
+    code_a = np.array([{"P ": 0, "S ": 1, "Ab": 2, "Tb": 3, "Cb": 4, "Gb": 5}[x] for x in nm])
+    strand_a = np.array(strand)
+    isbase = code_a >= 2
+    comp_of = np.array([-1, -1, 3, 2, 5, 4])       # A<->T, C<->G
+    bonded = [set() for _ in range(n)]
+    for a, b in bonds:
+        bonded[max(a, b)].add(min(a, b))
+    angend = [set() for _ in range(n)]
+    for a, _, d in angles:
+        angend[max(a, d)].add(min(a, d))
+    parts = {k: [] for k in ("stack", "bp", "ex", "qq", "slv")}
     for i in range(1, n):
-        for j in range(i):
-            if (j, i) not in bset:
-                dij = float(np.sqrt(np.sum((pos[j] - pos[i]) ** 2)))
-                if strand[i] == strand[j] and (dij - 9.0) < 0.0 and abs(dij - 9.0) >= 1e-6:
-                    stack.append((i, j, dij))
-                elif (nm[i], nm[j]) in comp_pair:
-                    bp.append((i, j, 2.8694 if nm[i] in ("Gb", "Cb") else 2.9002))
-                else:
-                    both_base = nm[i] not in ("P ", "S ") and nm[j] not in ("P ", "S ")
-                    ex.append((i, j, 1.0 if both_base else 6.86))
-                if (j, i) not in aset and nm[i] == "P " and nm[j] == "P ":
-                    qq.append((i, j))
-            if solv and strand[i] != strand[j] and nm[i] == "S " and nm[j] == "S ":
-                slv.append((i, j))
+        j = np.arange(i)
+        keep = np.ones(i, bool)
+        if bonded[i]:
+            keep[list(bonded[i])] = False
+        dij = np.sqrt(np.sum((pos[:i] - pos[i]) ** 2, axis=1))
+        same = strand_a[:i] == strand_a[i]
+        st_ = keep & same & ((dij - 9.0) < 0.0) & (np.abs(dij - 9.0) >= 1e-6)
+        bp_ = keep & ~st_ & (comp_of[code_a[i]] == code_a[:i])
+        ex_ = keep & ~st_ & ~bp_
+        if st_.any():
+            parts["stack"].append((np.full(st_.sum(), i), j[st_], dij[st_]))
+        if bp_.any():
+            sg = 2.8694 if nm[i] in ("Gb", "Cb") else 2.9002
+            parts["bp"].append((np.full(bp_.sum(), i), j[bp_], np.full(bp_.sum(), sg)))
+        if ex_.any():
+            sg = np.where(isbase[i] & isbase[:i][ex_], 1.0, 6.86)
+            parts["ex"].append((np.full(ex_.sum(), i), j[ex_], sg))
+        qk = keep & (code_a[i] == 0) & (code_a[:i] == 0)
+        if angend[i]:
+            qk[list(angend[i])] = False
+        if qk.any():
+            parts["qq"].append((np.full(qk.sum(), i), j[qk], None))
+        if solv and code_a[i] == 1:
+            sk = (~same) & (code_a[:i] == 1)
+            if sk.any():
+                parts["slv"].append((np.full(sk.sum(), i), j[sk], None))
+
+    def cat(name):
+        ps = parts[name]
+        if not ps:
+            return np.zeros(0, np.int32), np.zeros(0)
+        a = np.concatenate([q[0] for q in ps]).astype(np.int32)
+        b = np.concatenate([q[1] for q in ps]).astype(np.int32)
+        v = np.concatenate([q[2] for q in ps]) if ps[0][2] is not None else None
+        return np.concatenate([a, b]), v
 
     def cm(lst, k):
         if not lst:
@@ -271,11 +306,11 @@ def dna_duplex(seq, z0=0.0, solv=True):
         "angle": (cm(angles, 3), np.array([_angle(pos[a], pos[b], pos[c]) for a, b, c in angles])),
         "dihe": (cm(dihes, 4), np.array([_dihedral(pos[a], pos[b], pos[c], pos[d])
                                          for a, b, c, d in dihes])),
-        "stack": (cm(stack, 2), np.array([t[2] for t in stack])),
-        "bp": (cm(bp, 2), np.array([t[2] for t in bp])),
-        "ex": (cm(ex, 2), np.array([t[2] for t in ex])),
-        "qq": (cm(qq, 2), None),
-        "slv": (cm(slv, 2), None),
+        "stack": cat("stack"),
+        "bp": cat("bp"),
+        "ex": cat("ex"),
+        "qq": (cat("qq")[0], None),
+        "slv": (cat("slv")[0], None),
     }
     return dict(pos=pos, code=np.array(code, np.int32), strand=np.array(strand), nuc=nuc,
                 lists=lists, nnuc=k)
@@ -311,11 +346,13 @@ def pore_field(shape, dcel, volt, half_slab, rng, noise_volt=0.01, smooth=2.0):
 def pore_dna(seed=1, n_bp=50, n_ions=2000, box=(100.0, 100.0, 200.0), dcel=0.5, rpore=12.0,
              half_slab=20.0, volt=0.5, svdw=1000.0, temp=300.0, cdie=80.0, dt=0.02,
              table_builder=None, dna=True, capacity=None, zcont=(-20.0, 0.0, 20.0),
-             rmax=RMAX):
-    """Ions in a cylindrical pore through a slab, with a CG-DNA duplex on the axis."""
+             rmax=RMAX, grid_shape=None, dna_layout=None, workload=None):
+    """Ions in a cylindrical pore through a slab, with a CG-DNA duplex on the axis (or laid
+    out by dna_layout, see dna_duplex).  The grids cover the box unless grid_shape is
+    given (centred on the box)."""
     rng = np.random.default_rng(seed)
     lx, ly, lz = box
-    shape = tuple(int(round(L / dcel)) + 1 for L in box)
+    shape = grid_shape or tuple(int(round(L / dcel)) + 1 for L in box)
     phiv = pore_repulsion(shape, dcel, rpore, half_slab)
     phix = pore_field(shape, dcel, volt, half_slab, rng)
     if dna:
@@ -323,7 +360,7 @@ def pore_dna(seed=1, n_bp=50, n_ions=2000, box=(100.0, 100.0, 200.0), dcel=0.5, 
         for b in "ACGT":                      # all four base types present
             if b not in seq:
                 seq = b + seq[1:]
-        d = dna_duplex(seq, z0=0.0)
+        d = dna_duplex(seq, z0=0.0, layout=dna_layout)
         spos = d["pos"]
         ns = len(spos)
         nold = d["nnuc"]
@@ -424,10 +461,60 @@ def pore_dna(seed=1, n_bp=50, n_ions=2000, box=(100.0, 100.0, 200.0), dcel=0.5, 
         fact0n1=diffnuc * kbt * dt, fact0n2=math.sqrt(2.0 * dt * diffnuc),
         box=(-hx, hx, -hy, hy, -hz, hz), zcont=np.array(zcont, float), ndna=ndna, nion=nion,
         dt=dt, kbtdt=kbtdt, diffusion=dvec.copy(),
-        meta=dict(workload="pore_dna" if dna else "pore_ions", seed=seed, temp=temp, dt=dt,
+        meta=dict(workload=workload or ("pore_dna" if dna else "pore_ions"), seed=seed,
+                  temp=temp, dt=dt,
                   kbt=kbt, cdie=cdie, n_bp=n_bp if dna else 0, grid=shape, dcel=dcel),
         **kw,
     )
+    return s
+
+
+def serpentine(seg_bp=80, thread_base=-100.0, cis_base=-295.0, columns=None, rise=3.38):
+    """dna_duplex layout: a long duplex cut into segments of seg_bp base-pair steps, each
+    moved rigidly: the first threads the pore on the z axis from z = thread_base, the others
+    stand side by side below the slab (cis side) from z = cis_base, at the given (x, y)."""
+    cols = columns or [(40.0, 0.0), (-40.0, 0.0), (0.0, 40.0), (0.0, -40.0), (40.0, 40.0),
+                       (-40.0, 40.0), (40.0, -40.0), (-40.0, -40.0), (80.0, 0.0),
+                       (-80.0, 0.0), (0.0, 80.0), (0.0, -80.0), (80.0, 80.0),
+                       (-80.0, -80.0)]
+
+    def layout(pos, step):
+        out = pos.copy()
+        zref = float(np.min(pos[:, 2] - step * rise))    # the helix start
+        seg = step // seg_bp
+        if seg.max() > len(cols):
+            raise ValueError("too many segments for the columns")
+        for k in range(seg.max() + 1):
+            m = seg == k
+            base = thread_base if k == 0 else cis_base
+            cx, cy = (0.0, 0.0) if k == 0 else cols[k - 1]
+            out[m, 0] = pos[m, 0] + cx
+            out[m, 1] = pos[m, 1] + cy
+            out[m, 2] = pos[m, 2] + (base - zref - k * seg_bp * rise)
+        return out
+    return layout
+
+
+def translocation(seed=4, n_bp=1000, n_ions=50000, box=(200.0, 200.0, 600.0), dcel=0.625,
+                  grid_shape=(321, 321, 641), rpore=15.0, half_slab=20.0, volt=0.5,
+                  table_builder=None, conc=1.0, rfpar=True, profile=True, gcmc=True,
+                  capacity=None):
+    """The large translocation workload: a 1000-bp CG-DNA (6000 sites) threaded through a
+    pore of radius 15 A in a 200 x 200 x 600 A box (the rest of the duplex folded on the cis
+    side), 50,000 K+/Cl- ions, repulsion / static-field / reaction-field grids of
+    321 x 321 x 641 at 0.625 A, D(z) tanh-reduced to 0.5x inside the pore (PROFILE), and
+    10 A GCMC buffers at both box ends."""
+    s = pore_dna(seed=seed, n_bp=n_bp, n_ions=n_ions, box=box, dcel=dcel, rpore=rpore,
+                 half_slab=half_slab, volt=volt, table_builder=table_builder,
+                 capacity=capacity or (n_ions + 6 * n_bp + 4096), grid_shape=grid_shape,
+                 dna_layout=serpentine(), workload="translocation")
+    if rfpar:
+        s = with_rfpar(s, seed=seed, shape=grid_shape, dcel=dcel, wall_radius=None)
+    if profile:
+        zk, fk = tanh_profile(box[2] * -0.5, box[2] * 0.5, 121, half_slab)
+        s = with_profile_diffusion(s, zk, fk)
+    if gcmc:
+        s.meta["gcmc"] = gcmc_buffers(s, conc=conc, buffer_len=10.0)
     return s
 
 
