@@ -34,7 +34,8 @@ TRAFFIC_SCALE = (2.0, 1.0)
 def step_bytes(s, injected):
     """Algorithmic HBM bytes per particle-step of the fused step kernel: read x,y,z (24)
     and the type word (4), write x,y,z (24), 8 trilinear corners of each grid present
-    (phix float32: 32, phiv int8: 8), injected Gaussians (24).  Partner positions, pair
+    (phix float32: 32, phiv int8: 8, the reaction-field pair gsrfen + greff: 64), injected
+    Gaussians (24).  Partner positions, pair
     tables, DNA lists and type tables are re-read from L2 and are not HBM traffic."""
     from bromoc_amd import abi
     b = 24 + 4 + 24
@@ -42,6 +43,8 @@ def step_bytes(s, injected):
         b += 32
     if s.flags & abi.QPHIV:
         b += 8
+    if s.flags & abi.QRFPAR:    # 8 corners of gsrfen and greff (float32 each), per ion
+        b += 64
     if injected:
         b += 24
     return b
@@ -207,7 +210,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="pore_dna", choices=["pore_dna", "plain_ions"])
+    ap.add_argument("--workload", default="pore_dna",
+                    choices=["pore_dna", "plain_ions", "translocation"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true", help="skip the start-state parity check")
     ap.add_argument("--no-host", action="store_true", help="skip the host round-trip rate")
@@ -236,6 +240,12 @@ def main():
               "dna_bp": s.meta["n_bp"], "box_A": [100, 100, 200], "grid": list(s.meta["grid"]),
               "grid_spacing_A": s.meta["dcel"], "pair_tables": s.npair,
               "table_knots": int(len(s.ep) // 3), "dt_ps": s.meta["dt"], "noise": "philox"}
+    elif args.workload == "translocation":
+        s = configs.translocation()
+        wl = {"workload": "translocation", "ions": s.ntot - s.nsites, "dna_sites": s.nsites,
+              "box_A": [200, 200, 600], "grid": list(s.meta["grid"]),
+              "grid_spacing_A": s.meta["dcel"], "rfpar": True, "diffusion": "profile",
+              "pair_tables": s.npair, "dt_ps": s.meta["dt"], "noise": "philox"}
     else:
         s = configs.plain_ions(seed=0)
         wl = {"workload": "plain_ions", "ions": s.ntot, "box_A": 60, "pair_tables": s.npair,
