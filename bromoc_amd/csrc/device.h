@@ -17,7 +17,9 @@ constexpr int F_DNA = 1;     // CG-DNA sites and lists present
 constexpr int F_PHIX = 2;    // static field grid
 constexpr int F_PHIV = 4;    // repulsion grid
 constexpr int F_UPDATE = 8;  // apply the BD update (else ENERGY only)
-constexpr int F_PROX = 16;   // track each ion's nearest DNA site (PROXDIFF, DYNAMICS3)
+constexpr int F_TAB = 16;    // every pair type tabulated: the analytic pair path is compiled
+                             // out (it would set the register budget of the whole kernel)
+constexpr int F_PROX = 32;   // track each ion's nearest DNA site (PROXDIFF, DYNAMICS3)
 
 // host-side flag values (include/bromoc_gpu.h)
 constexpr uint32_t QPAR = 1u << 0, QNUCL = 1u << 1, QNONBOND = 1u << 2, QNOBOND = 1u << 3;

@@ -34,7 +34,7 @@ __device__ inline void prox_record(const StepParams &P, int i, const double4 &pa
     const double e0x = pa.x - p0.x, e0y = pa.y - p0.y, e0z = pa.z - p0.z;
     const double d20 = e0x * e0x + e0y * e0y + e0z * e0z;
     const DevPair &r0 = row[__double2loint(p0.w)];
-    const bool first_in = (P.all_efpot || (r0.flags & PF_EFPOT)) && !(d20 < r0.dm2lo) &&
+    const bool first_in = (r0.flags & PF_EFPOT) && !(d20 < r0.dm2lo) &&
                           !(d20 > r0.dm2hi);
     if (!first_in && !(d < 0.0)) return;
     const double4 pj = P.pos[jj];
@@ -135,7 +135,10 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
                     // the higher-index ion of an ion-ion pair
                     const bool ce = !skip && (tsite[a] || (j >= ns && j < ti[a]));
                     double y, de;
-                    if (P.all_efpot || (s_row[a][t2j].flags & PF_EFPOT)) {
+                    if constexpr ((FEAT & F_TAB) != 0) {
+                        de = pair_near_far(s_row[a][t2j], s_tail[a][t2j], P.ep, P.ires, d2, y);
+                        et[a] += ce ? y : 0.0;
+                    } else if (s_row[a][t2j].flags & PF_EFPOT) {
                         de = pair_near_far(s_row[a][t2j], s_tail[a][t2j], P.ep, P.ires, d2, y);
                         et[a] += ce ? y : 0.0;
                     } else {
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
                     }
                     if constexpr ((FEAT & F_PROX) != 0) {
                         const DevPair &pr = s_row[a][t2j];
-                        if (!skip && !tsite[a] && j < ns && (P.all_efpot || (pr.flags & PF_EFPOT)) &&
+                        if (!skip && !tsite[a] && j < ns && ((FEAT & F_TAB) || (pr.flags & PF_EFPOT)) &&
                             !(d2 < pr.dm2lo) && !(d2 > pr.dm2hi)) {
                             const double dd = sqrt(d2) - pr.dmi;
                             if (dd < pxd[a] || (dd == pxd[a] && j < pxj[a])) { pxd[a] = dd; pxj[a] = j; }
@@ -552,7 +555,10 @@ __global__ __launch_bounds__(BLOCK) void k_wave(StepParams P) {
                 const bool ce = !skip && (is_site || (j >= ns && j < i));
                 const int tj = skip ? 0 : t2j;
                 double y, de;
-                if (P.all_efpot || (s_row[wv][tj].flags & PF_EFPOT)) {
+                if constexpr ((FEAT & F_TAB) != 0) {
+                    de = pair_near_far(s_row[wv][tj], s_tail[wv][tj], P.ep, P.ires, d2, y);
+                    et += ce ? y : 0.0;
+                } else if (s_row[wv][tj].flags & PF_EFPOT) {
                     de = pair_near_far(s_row[wv][tj], s_tail[wv][tj], P.ep, P.ires, d2, y);
                     et += ce ? y : 0.0;
                 } else {
@@ -689,13 +695,15 @@ hipError_t launch_philox_fill(const int64_t *ctl, uint64_t stream, int nrows, in
     return hipGetLastError();
 }
 
+#define CASES16(B)                                                                         \
+    CASE(B + 0) CASE(B + 1) CASE(B + 2) CASE(B + 3) CASE(B + 4) CASE(B + 5) CASE(B + 6)     \
+    CASE(B + 7) CASE(B + 8) CASE(B + 9) CASE(B + 10) CASE(B + 11) CASE(B + 12)              \
+    CASE(B + 13) CASE(B + 14) CASE(B + 15)
+
 static void dispatch_wave(const StepParams &p, int feat, hipStream_t s) {
     dim3 grid((p.tend - p.tbeg + WPB - 1) / WPB), block(BLOCK);
 #define CASE(F) case F: hipLaunchKernelGGL((k_wave<F>), grid, block, 0, s, p); break;
-    switch (feat) {
-        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
-        CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
-    }
+    switch (feat) { CASES16(0) CASES16(16) }
 #undef CASE
 }
 
@@ -703,10 +711,7 @@ template <int T>
 static void dispatch_step(const StepParams &p, int feat, hipStream_t s) {
     dim3 grid((p.tend - p.tbeg + T - 1) / T), block(BLOCK);
 #define CASE(F) case F: hipLaunchKernelGGL((k_step<F, T>), grid, block, 0, s, p); break;
-    switch (feat) {
-        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
-        CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
-    }
+    switch (feat) { CASES16(0) CASES16(16) }
 #undef CASE
 }
 
@@ -715,7 +720,7 @@ static void dispatch_prox(const StepParams &p, int feat, hipStream_t s) {
     dim3 grid(p.tend - p.tbeg), block(BLOCK);
 #define CASE(F) case F: hipLaunchKernelGGL((k_step<F | F_PROX, 1>), grid, block, 0, s, p); break;
     switch (feat) {
-        CASE(9) CASE(11) CASE(13) CASE(15)
+        CASE(9) CASE(11) CASE(13) CASE(15) CASE(25) CASE(27) CASE(29) CASE(31)
     }
 #undef CASE
 }
@@ -734,6 +739,7 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s) {
     if (p.flags & QPHIX) feat |= F_PHIX;
     if (p.flags & QPHIV) feat |= F_PHIV;
     if (p.mode == 1) feat |= F_UPDATE;
+    if (p.all_efpot) feat |= F_TAB;
     if (p.dmode == 3 && (feat & F_UPDATE) && (feat & F_DNA)) {
         dispatch_prox(p, feat, s);
         return hipGetLastError();
