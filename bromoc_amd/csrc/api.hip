@@ -576,8 +576,9 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
         ACHK(dupload(c, &d_gs, rfon ? s->gsrfen : (const float *)nullptr, ncel3 * ng));
         ACHK(dupload(c, &d_gr, rfon ? s->greff : (const float *)nullptr, ncel3 * ng));
     }
-    double4 *d_rfa = nullptr, *d_rfd = nullptr, *d_frf = nullptr;
+    double4 *d_rfa = nullptr, *d_rfd = nullptr, *d_frf = nullptr, *d_rfq = nullptr;
     ACHK(dalloc(c, &d_rfa, rfon ? cap : 0));
+    ACHK(dalloc(c, &d_rfq, rfon ? cap : 0));
     ACHK(dalloc(c, &d_rfd, rfon ? 2 * (size_t)cap : 0));
     ACHK(dalloc(c, &d_frf, rfon ? cap : 0));
     // DYNAMICS3 needs the diffusion constants even without DYNAMICS1/2
@@ -658,7 +659,7 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.gsrfen = d_gs; p.greff = d_gr; p.sqrfac = s->sqrfac;
     p.rfon = rfon ? 1 : 0;
     p.rfsingle = (s->flags & BROMOC_QRFPSIN) ? 1 : 0;
-    p.rfa = d_rfa; p.rfd = d_rfd; p.frf = d_frf;
+    p.rfa = d_rfa; p.rfd = d_rfd; p.frf = d_frf; p.rfq = d_rfq;
     p.all_efpot = 1;
     c->r2near = 0.0;
     for (int k = 0; k < nt1 * nt1; k++) {

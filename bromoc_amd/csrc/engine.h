@@ -117,7 +117,8 @@ struct StepParams {
     const float *gsrfen, *greff;
     double sqrfac;
     int32_t rfon, rfsingle;
-    double4 *rfa;            // [cap] (srfe, reff, 1 if inside the grid box else 0, 0)
+    double4 *rfa;            // [cap] (srfe, reff, 1 if inside the grid box, 1 if moving)
+    double4 *rfq;            // [cap] (x, y, z, charge) of each ion, for the pair kernel
     double4 *rfd;            // [2*cap] (d srfe / d r, 0), (d reff / d r, 0)
     double4 *frf;            // [cap] reaction-field force on each moving ion (k_rfpair)
     // APFOR: summed applied force of each site (zero where none) [nsites]

@@ -9,6 +9,8 @@
 // DYNAMICS0NUC) into the other position buffer.  Every force is a gather over partners in
 // a fixed order: results are deterministic and need no atomics; energies are per-particle
 // partials reduced in a fixed tree.
+#include <type_traits>
+
 #include "device.h"
 
 namespace bromoc {
@@ -110,9 +112,11 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
 #pragma unroll
     for (int a = 0; a < T; a++) { pxd[a] = INFINITY; pxj[a] = 0x7fffffff; }
 
-    // ---- pair loop (energy.mono.f90:117-194, 584-687)
+    // ---- pair loop (energy.mono.f90:117-194, 584-687), compiled twice: with the energy
+    // sums, and without them for the steps that write no energies
     const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
-    if (nonbond) {
+    auto pair_loop = [&](auto en) {
+        constexpr bool EN = decltype(en)::value;
         for (int k0 = tid; k0 < ntot; k0 += 2 * BLOCK) {
             // partners in spatial order, two per iteration (two load chains in flight)
             const int k1 = k0 + BLOCK;
@@ -137,16 +141,18 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
                     double y, de;
                     if constexpr ((FEAT & F_TAB) != 0) {
                         de = pair_near_far(s_row[a][t2j], s_tail[a][t2j], P.ep, P.ires, d2, y);
-                        et[a] += ce ? y : 0.0;
+                        if constexpr (EN) et[a] += ce ? y : 0.0;
                     } else if (s_row[a][t2j].flags & PF_EFPOT) {
                         de = pair_near_far(s_row[a][t2j], s_tail[a][t2j], P.ep, P.ires, d2, y);
-                        et[a] += ce ? y : 0.0;
+                        if constexpr (EN) et[a] += ce ? y : 0.0;
                     } else {
                         double q0 = 0.0, q1 = 0.0, q2 = 0.0;
                         de = pair_analytic(s_row[a][t2j], P, d2, q0, q1, q2);
-                        eel[a] += ce ? q0 : 0.0;
-                        evd[a] += ce ? q1 : 0.0;
-                        esr[a] += ce ? q2 : 0.0;
+                        if constexpr (EN) {
+                            eel[a] += ce ? q0 : 0.0;
+                            evd[a] += ce ? q1 : 0.0;
+                            esr[a] += ce ? q2 : 0.0;
+                        }
                     }
                     if constexpr ((FEAT & F_PROX) != 0) {
                         const DevPair &pr = s_row[a][t2j];
@@ -163,6 +169,10 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
                 }
             }
         }
+    };
+    if (nonbond) {
+        if (P.write_energy) pair_loop(std::true_type{});
+        else pair_loop(std::false_type{});
     }
     __shared__ double s_pxd[(FEAT & F_PROX) ? WPB : 1][T];
     __shared__ int s_pxj[(FEAT & F_PROX) ? WPB : 1][T];
@@ -769,86 +779,134 @@ __global__ void k_rfpar(StepParams P, int grad) {
     const double4 pa = P.pos[i];
     double srfe, reff, ds[3], dr[3];
     const bool in = rf_params(P, pa.x, pa.y, pa.z, abs(P.typei[i]), grad != 0, srfe, reff, ds, dr);
-    P.rfa[i] = make_double4(srfe, reff, in ? 1.0 : 0.0, 0.0);
+    // (srfe, reff, inside the grid box, moving); the pair kernel's partner record
+    P.rfa[i] = make_double4(srfe, reff, in ? 1.0 : 0.0, i >= P.nsites + P.nfix ? 1.0 : 0.0);
+    P.rfq[i] = make_double4(pa.x, pa.y, pa.z, P.cg[abs(P.typei[i])]);
     if (grad) {
         P.rfd[2 * (size_t)i] = make_double4(ds[0], ds[1], ds[2], 0.0);
         P.rfd[2 * (size_t)i + 1] = make_double4(dr[0], dr[1], dr[2], 0.0);
     }
 }
 
-// k_rfpair: reaction-field energy and force of each ion, one wavefront per target.
-// The reference visits each ion pair once, from its higher index (rfpar.f90:157-223): a
-// pair counts when the lower ion's srfe is nonzero and the higher ion is moving and inside
-// the grid box.  Gathered here per target t over every other ion p, in the reference's
-// operand order (outer = higher index): aux0 = celec*q_hi*q_lo, energy charged to the
-// higher index, and the force on t from the variation of srfe_t, reff_t and |r_t - r_p|.
-// A moving ion's force goes to frf (added by the step kernel); a fixed ion, which the
-// step kernel does not visit, gets its (unused) reaction-field force in fx directly.
+// k_rfpair: reaction-field energy and force of each ion.  The reference visits each ion
+// pair once, from its higher index (rfpar.f90:157-223): a pair counts when the lower ion's
+// srfe is nonzero and the higher ion is moving and inside the grid box.  Gathered here per
+// target t over every other ion p, in the reference's operand order (outer = higher
+// index): aux0 = celec*q_hi*q_lo, energy charged to the higher index, and the force on t
+// from the variation of srfe_t, reff_t and |r_t - r_p|.  A moving ion's force goes to frf
+// (added by the step kernel); a fixed ion, which the step kernel does not visit, gets its
+// (unused) reaction-field force in fx directly.
+// Layout: each wavefront takes RF_TPW targets, the block's RF_TPW * 4 targets share tiles
+// of 256 partner records (position + charge, srfe + reff + inside + moving) staged through
+// LDS, double-buffered; each lane evaluates 4 partners of a tile against its targets.
+constexpr int RF_TPW = 2;
+
 __global__ __launch_bounds__(BLOCK) void k_rfpair(StepParams P) {
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x >> 6;
-    const int t = P.nsites + blockIdx.x * WPB + wv;
-    if (t >= P.ntot) return;
-    const int nmov0 = P.nsites + P.nfix;
-    const bool tmov = t >= nmov0;
-    const double4 rt = P.rfa[t];
-    if (rt.z == 0.0) {   // outside the grid box: no reaction-field term at all
-        if (lane == 0) {
-            P.frf[t] = make_double4(0.0, 0.0, 0.0, 0.0);
-            if (P.write_energy) P.epart[(size_t)E_RFPAR * P.ecap + t] = 0.0;
-            if (!tmov && P.write_forces) { P.fx[t] = 0.0; P.fy[t] = 0.0; P.fz[t] = 0.0; }
+    __shared__ double4 s_q[2][BLOCK], s_a[2][BLOCK];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
+    const int ns = P.nsites, ntot = P.ntot, nmov0 = P.nsites + P.nfix;
+    int t[RF_TPW];
+    bool tval[RF_TPW], tmov[RF_TPW], tin[RF_TPW];
+    double4 pt[RF_TPW], rt[RF_TPW];
+    double A[RF_TPW], B[RF_TPW], cx[RF_TPW], cy[RF_TPW], cz[RF_TPW], e[RF_TPW];
+#pragma unroll
+    for (int a = 0; a < RF_TPW; a++) {
+        t[a] = ns + (blockIdx.x * WPB + wv) * RF_TPW + a;
+        tval[a] = t[a] < ntot;
+        const int tt = tval[a] ? t[a] : ns;
+        pt[a] = P.rfq[tt];
+        rt[a] = P.rfa[tt];
+        tmov[a] = tt >= nmov0;
+        tin[a] = tval[a] && rt[a].z != 0.0;
+        A[a] = B[a] = cx[a] = cy[a] = cz[a] = e[a] = 0.0;
+    }
+    const int nion = ntot - ns;
+    const int ntiles = (nion + BLOCK - 1) / BLOCK;
+    double4 nq = make_double4(0.0, 0.0, 0.0, 0.0), na = nq;
+    if (tid < nion) { nq = P.rfq[ns + tid]; na = P.rfa[ns + tid]; }
+    s_q[0][tid] = nq;
+    s_a[0][tid] = na;
+    __syncthreads();
+    for (int k = 0; k < ntiles; k++) {
+        const int cur = k & 1;
+        const int jn = (k + 1) * BLOCK + tid;
+        if (k + 1 < ntiles) {
+            nq = make_double4(0.0, 0.0, 0.0, 0.0);
+            na = nq;
+            if (jn < nion) { nq = P.rfq[ns + jn]; na = P.rfa[ns + jn]; }
         }
-        return;
+#pragma unroll 1
+        for (int u = 0; u < WPB; u++) {
+            const int idx = u * WAVE + lane;
+            const int p = ns + k * BLOCK + idx;
+            const double4 pp = s_q[cur][idx], rp = s_a[cur][idx];
+#pragma unroll
+            for (int a = 0; a < RF_TPW; a++) {
+                const bool hi_t = p < t[a];
+                // rp.z: partner inside the grid box; rp.w: partner moving
+                const bool on = tin[a] && p < ntot && p != t[a] &&
+                                (hi_t ? (rp.x != 0.0 && tmov[a])
+                                      : (rt[a].x != 0.0 && rp.z != 0.0 && rp.w != 0.0));
+                const double q_hi = hi_t ? pt[a].w : pp.w, q_lo = hi_t ? pp.w : pt[a].w;
+                // masks as factors: every term below is finite (a masked pair's square
+                // distance is lifted by 1), so m * term is exact 0 or the term
+                const double m = on ? 1.0 : 0.0, mh = (on && hi_t) ? 1.0 : 0.0;
+                const double aux0 = P.celec * q_hi * q_lo;
+                const double dx = pt[a].x - pp.x, dy = pt[a].y - pp.y, dz = pt[a].z - pp.z;
+                const double dist2 = dx * dx + dy * dy + dz * dz;
+                const double srfeij = rt[a].x * rp.x, reffij = rt[a].y * rp.y;
+                const double rfdn = rsqrt_nr(reffij * reffij + dist2 + (1.0 - m));
+                const double rfcf = reffij * rfdn;
+                const double aux1 = aux0 * rfcf;
+                const double aux2 = aux0 * srfeij * (rfdn * (rfdn * rfdn));
+                const double aux3 = -aux2 * (dist2 + reffij * 0.5);
+                const double de = m * (aux2 * reffij);
+                e[a] = fma(mh, aux1 * srfeij, e[a]);
+                A[a] = fma(m, -aux1 * rp.x, A[a]);    // times d srfe_t
+                B[a] = fma(m, aux3 * rp.y, B[a]);     // times d reff_t
+                cx[a] = fma(de, dx, cx[a]);
+                cy[a] = fma(de, dy, cy[a]);
+                cz[a] = fma(de, dz, cz[a]);
+            }
+        }
+        if (k + 1 < ntiles) {
+            s_q[cur ^ 1][tid] = nq;
+            s_a[cur ^ 1][tid] = na;
+        }
+        __syncthreads();
     }
-    const double4 pt = P.pos[t];
-    const double cgt = P.cg[abs(P.typei[t])];
-    const double srt = rt.x, rft = rt.y;
-    double A = 0.0, B = 0.0, cx = 0.0, cy = 0.0, cz = 0.0, e = 0.0;
-    for (int p = P.nsites + lane; p < P.ntot; p += WAVE) {
-        if (p == t) continue;
-        const double4 rp = P.rfa[p];
-        // lower ion's srfe nonzero, higher ion moving and inside the box (t is inside)
-        const bool hi_t = p < t;
-        if (hi_t ? (rp.x == 0.0 || !tmov) : (srt == 0.0 || rp.z == 0.0 || p < nmov0)) continue;
-        const double4 pp = P.pos[p];
-        const double cgp = P.cg[abs(P.typei[p])];
-        const double s_lo = hi_t ? rp.x : srt, s_hi = hi_t ? srt : rp.x;
-        const double r_lo = hi_t ? rp.y : rft, r_hi = hi_t ? rft : rp.y;
-        const double x_lo = hi_t ? pp.x : pt.x, x_hi = hi_t ? pt.x : pp.x;
-        const double y_lo = hi_t ? pp.y : pt.y, y_hi = hi_t ? pt.y : pp.y;
-        const double z_lo = hi_t ? pp.z : pt.z, z_hi = hi_t ? pt.z : pp.z;
-        const double aux0 = P.celec * (hi_t ? cgt : cgp) * (hi_t ? cgp : cgt);
-        const double dx = x_lo - x_hi, dy = y_lo - y_hi, dz = z_lo - z_hi;
-        const double dist2 = dx * dx + dy * dy + dz * dz;
-        const double srfeij = s_lo * s_hi, reffij = r_lo * r_hi;
-        const double rfdn = 1.0 / sqrt(reffij * reffij + dist2);
-        const double rfcf = reffij * rfdn;
-        const double aux1 = aux0 * rfcf;
-        if (hi_t) e += aux1 * srfeij;
-        const double aux2 = aux0 * srfeij * (rfdn * (rfdn * rfdn));
-        const double aux3 = -aux2 * (dist2 + r_lo * r_hi * 0.5);
-        A += -aux1 * rp.x;             // times d srfe_t
-        B += aux3 * rp.y;              // times d reff_t
-        const double de = aux2 * reffij;
-        cx += de * (pt.x - pp.x);
-        cy += de * (pt.y - pp.y);
-        cz += de * (pt.z - pp.z);
+#pragma unroll
+    for (int a = 0; a < RF_TPW; a++) {
+        A[a] = wave_sum(A[a]); B[a] = wave_sum(B[a]);
+        cx[a] = wave_sum(cx[a]); cy[a] = wave_sum(cy[a]); cz[a] = wave_sum(cz[a]);
+        if (P.write_energy) e[a] = wave_sum(e[a]);
     }
-    A = wave_sum(A); B = wave_sum(B);
-    cx = wave_sum(cx); cy = wave_sum(cy); cz = wave_sum(cz);
-    if (P.write_energy) e = wave_sum(e);
     if (lane != 0) return;
-    const double4 dst = P.rfd[2 * (size_t)t], drt = P.rfd[2 * (size_t)t + 1];
-    // self term of a moving ion (rfpar.f90:146-156): aux = celec*q*q*srfe, energy
-    // aux*srfe/2, force -aux*dsrfe
-    const double aux = tmov ? P.celec * cgt * cgt * srt : 0.0;
-    const double a_s = A - aux;
-    const double4 f = make_double4(a_s * dst.x + B * drt.x + cx, a_s * dst.y + B * drt.y + cy,
-                                   a_s * dst.z + B * drt.z + cz, 0.0);
-    if (tmov) {
-        P.frf[t] = f;
-        if (P.write_energy) P.epart[(size_t)E_RFPAR * P.ecap + t] = 0.5 * aux * srt + e;
-    } else if (P.write_forces) {
-        P.fx[t] = f.x; P.fy[t] = f.y; P.fz[t] = f.z;
+#pragma unroll
+    for (int a = 0; a < RF_TPW; a++) {
+        if (!tval[a]) continue;
+        const int ta = t[a];
+        if (!tin[a]) {   // outside the grid box: no reaction-field term at all
+            P.frf[ta] = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (P.write_energy) P.epart[(size_t)E_RFPAR * P.ecap + ta] = 0.0;
+            if (!tmov[a] && P.write_forces) { P.fx[ta] = 0.0; P.fy[ta] = 0.0; P.fz[ta] = 0.0; }
+            continue;
+        }
+        const double4 dst = P.rfd[2 * (size_t)ta], drt = P.rfd[2 * (size_t)ta + 1];
+        const double srt = rt[a].x, cgt = pt[a].w;
+        // self term of a moving ion (rfpar.f90:146-156): aux = celec*q*q*srfe, energy
+        // aux*srfe/2, force -aux*dsrfe
+        const double aux = tmov[a] ? P.celec * cgt * cgt * srt : 0.0;
+        const double a_s = A[a] - aux;
+        const double4 f = make_double4(a_s * dst.x + B[a] * drt.x + cx[a],
+                                       a_s * dst.y + B[a] * drt.y + cy[a],
+                                       a_s * dst.z + B[a] * drt.z + cz[a], 0.0);
+        if (tmov[a]) {
+            P.frf[ta] = f;
+            if (P.write_energy) P.epart[(size_t)E_RFPAR * P.ecap + ta] = 0.5 * aux * srt + e[a];
+        } else if (P.write_forces) {
+            P.fx[ta] = f.x; P.fy[ta] = f.y; P.fz[ta] = f.z;
+        }
     }
 }
 
@@ -857,7 +915,8 @@ hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s) {
     if (!p.rfon || nion <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_rfpar, dim3((nion + 255) / 256), dim3(256), 0, s, p, pairs ? 1 : 0);
     if (pairs)
-        hipLaunchKernelGGL(k_rfpair, dim3((nion + WPB - 1) / WPB), dim3(BLOCK), 0, s, p);
+        hipLaunchKernelGGL(k_rfpair, dim3((nion + WPB * RF_TPW - 1) / (WPB * RF_TPW)),
+                           dim3(BLOCK), 0, s, p);
     return hipGetLastError();
 }
 
