@@ -120,6 +120,15 @@ class ReplicaStruct(C.Structure):
                 ("ibuffer", _ip), ("nat", _ip)]
 
 
+class MapHeader(C.Structure):
+    _fields_ = [("nclx", C.c_int32), ("ncly", C.c_int32), ("nclz", C.c_int32),
+                ("reserved", C.c_int32), ("dcel", C.c_double), ("xbcen", C.c_double),
+                ("ybcen", C.c_double), ("zbcen", C.c_double), ("rec2", C.c_double * 6)]
+
+
+MAP_PHIX, MAP_PHIV = 0, 1
+
+
 class BromocError(RuntimeError):
     def __init__(self, where, code, detail=""):
         self.code = code
@@ -170,6 +179,12 @@ def _declare(lib):
         "bromoc_set_gcmc": ([vp, P(GcmcStruct)], C.c_int),
         "bromoc_grand": ([vp, C.c_int32, _dp, C.c_int64, P(C.c_int64), _ip, _ip, C.c_int64,
                           _dp], C.c_int),
+        "bromoc_read_map": ([C.c_char_p, C.c_int32, C.c_int32, _dp, P(MapHeader), C.c_void_p,
+                             C.c_int64], C.c_int),
+        "bromoc_read_rfpar": ([C.c_int32, P(C.c_char_p), C.c_int32, _dp, P(MapHeader), _dp,
+                               P(C.c_float), P(C.c_float), C.c_int64], C.c_int),
+        "bromoc_repwalls": ([C.c_int32, C.c_int32, C.c_int32, P(C.c_int8), C.c_int32, C.c_int],
+                            C.c_int),
         "bromoc_metropolis": ([vp, C.c_int32, C.c_double, C.c_double, _dp, C.c_int64,
                                P(C.c_int64), _ip, _dp], C.c_int),
         "bromoc_build_table": ([C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp, P(Pair)],

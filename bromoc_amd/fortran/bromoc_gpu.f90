@@ -1,11 +1,13 @@
 ! bromoc_gpu.f90 -- iso_c_binding front-end of libbromoc_gpu.so for the cg-gcmc-bd driver.
 !
 ! Link this file's objects instead of energy.mono.o, dynamics0.mono.o, dynamics1.mono.o,
-! dynamics2.mono.o, dynamics3.mono.o, dynamics0nuc.o, interact.o, grand.o and metropolis.o:
+! dynamics2.mono.o, dynamics3.mono.o, dynamics0nuc.o, interact.o, grand.o, metropolis.o and
+! repwalls.o:
 ! the externals at the end carry the reference routines' names and argument lists
 ! (energy.mono.f90:19, dynamics0.mono.f90:19, dynamics1.mono.f90:19, dynamics2.mono.f90:19,
 ! dynamics3.mono.f90:19, dynamics0nuc.f90:19, interact.f90:19, grand.f90:19,
-! metropolis.f90:19), so simul1.f90 and shell_simul.f90 call the GPU path unchanged.  The first call hands the module state (grandmod, efpmod,
+! metropolis.f90:19, repwalls.f90:19), so simul1.f90 and shell_simul.f90 call the GPU path
+! unchanged.  The first call hands the module state (grandmod, efpmod,
 ! extramod, nucleotmod, gsbpmod) to bromoc_create; positions travel to the device before
 ! every ENERGY/INTERACT and back after every DYNAMICS/GRAND.  A faster loop that keeps
 ! positions on the device for nbd steps is bromoc_gpu_bd(nbd) (see the caller notes at
@@ -177,6 +179,12 @@ interface
     integer(c_int64_t), intent(out) :: used
     integer(c_int32_t), intent(out) :: done
     real(c_double), intent(inout) :: ener
+  end function
+  integer(c_int) function bromoc_repwalls(nx, ny, nz, map, walls, device) bind(C, name='bromoc_repwalls')
+    import :: c_int, c_int32_t, c_int8_t
+    integer(c_int32_t), value :: nx, ny, nz, walls
+    integer(c_int8_t), intent(inout) :: map(*)
+    integer(c_int), value :: device
   end function
   integer(c_int) function bromoc_destroy(ctx) bind(C, name='bromoc_destroy')
     import :: c_int, c_ptr
@@ -717,4 +725,14 @@ do while (left > 0)
 enddo
 ener = e
 call bromoc_gpu_download()
+end subroutine
+
+! REPWALLS layers the repulsion map (gsbpmod phiv) on the device, one stencil pass a layer.
+SUBROUTINE REPWALLS(walls)
+use bromoc_gpu
+use gsbpmod
+implicit none
+integer*1 walls
+call bgpu_check(bromoc_repwalls(int(nclx2, c_int32_t), int(ncly2, c_int32_t), &
+                int(nclz2, c_int32_t), phiv, int(walls, c_int32_t), 0_c_int), 'bromoc_repwalls')
 end subroutine

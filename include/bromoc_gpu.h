@@ -21,6 +21,9 @@
  * bromoc_spline                SUBROUTINE SPLINE            spline.f90:19-98
  * bromoc_metropolis            SUBROUTINE METROPOLIS        metropolis.f90:19-58 + MOVE
  *                              (move.f90:19-50)
+ * bromoc_read_map              SUBROUTINE READPHI           readphi.f90:19-349
+ * bromoc_read_rfpar            SUBROUTINE READRFPAR         rfpar.f90:373-582
+ * bromoc_repwalls              SUBROUTINE REPWALLS          repwalls.f90:19-67
  *
  * bromoc_step / bromoc_dynamics move the ions with DYNAMICS0, DYNAMICS1
  * (dynamics1.mono.f90:19-90, flag BROMOC_QDIFFUSE), DYNAMICS2 (dynamics2.mono.f90:19-89,
@@ -303,6 +306,45 @@ int bromoc_grand(bromoc_ctx *ctx, int32_t ngcmc, const double *uniforms, int64_t
 int bromoc_metropolis(bromoc_ctx *ctx, int32_t nmcm, double mcmax, double ikbt,
                       const double *uniforms, int64_t nuniforms, int64_t *used, int32_t *done,
                       double *ener);
+
+/* ---------------------------------------------------------------- map files
+ * Grid files as the reference reads them: Fortran unformatted sequential records with
+ * 4-byte length markers (a negative leading marker continues the record in the next
+ * subrecord).  Record 1: nclx, ncly, nclz (int32), dcel, xbcen, ybcen, zbcen (real*8);
+ * record 2: six real*8 (PHIX/PHIV: epsw, epsp, conc, tmemb, zmemb, epsm; RFPAR: the Born
+ * radius and five unused); then the float32 values, x-major with z fastest. */
+typedef struct bromoc_map_header {
+    int32_t nclx, ncly, nclz, reserved;
+    double dcel, xbcen, ybcen, zbcen;
+    double rec2[6];
+} bromoc_map_header;
+
+#define BROMOC_MAP_PHIX 0   /* static field: float32 values as stored */
+#define BROMOC_MAP_PHIV 1   /* repulsion map: int8, 0 where the file value is nonzero, else 1 */
+
+/* READPHI (readphi.f90:19-349, one map per file, no Qnmcden): reads the map at path.  With
+ * adjust the map is cropped to the system box[6] = (lx2m, lx2p, ly2m, ly2p, lz2m, lz2p)
+ * plus one node beyond its upper faces, as readphi.f90:98-156 does, and hdr describes the
+ * cropped map.  data (float* for PHIX, int8_t* for PHIV) receives nclx*ncly*nclz values
+ * of hdr; with data == NULL only hdr is filled in.  capacity: elements data holds. */
+int bromoc_read_map(const char *path, int32_t kind, int32_t adjust, const double *box,
+                    bromoc_map_header *hdr, void *data, int64_t capacity);
+
+/* READRFPAR (rfpar.f90:373-582): nfiles reaction-field parameter files, one per ion type
+ * (or one with QRFPSIN), each holding record 1, (radion, 5 unused), gsrfen and greff on the
+ * same lattice (files that differ in geometry are rejected, rfpar.f90:455-463).  radion
+ * [nfiles]; gsrfen, greff [nfiles * ncel3] of the (optionally cropped) lattice in hdr;
+ * with gsrfen == NULL only hdr and radion are filled in. */
+int bromoc_read_rfpar(int32_t nfiles, const char *const *paths, int32_t adjust,
+                      const double *box, bromoc_map_header *hdr, double *radion,
+                      float *gsrfen, float *greff, int64_t capacity);
+
+/* REPWALLS (repwalls.f90:19-67) on device `device`: successive wall layers on a binarised
+ * repulsion map in place, phiv[nclx*ncly*nclz] (host memory): layer n+1 marks the nodes of
+ * layer n whose 27-node neighbourhood is all layer n, for n = 1..walls, stopping at the
+ * first layer with no node. */
+int bromoc_repwalls(int32_t nclx, int32_t ncly, int32_t nclz, int8_t *phiv, int32_t walls,
+                    int device);
 
 /* EFPOT table from (r,U) samples at uniform spacing: writes ep[3*n] and pair fields
  * dmi, dm2lo, dm2hi, sc1..sc3 (host-side, splinepot.f90:19-96). */

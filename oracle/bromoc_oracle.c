@@ -2096,3 +2096,30 @@ EXPORT int orc_get_dids(orc_ctx *c, double *out) {
     memcpy(out, c->dids, sizeof(double) * 5 * (size_t)c->s.ntot);
     return BROMOC_OK;
 }
+
+/* REPWALLS, repwalls.f90:19-67 (one map, no Qnmcden): successive layers, in place */
+EXPORT int orc_repwalls(int32_t nclx2, int32_t ncly2, int32_t nclz2, int8_t *phiv, int32_t walls) {
+    size_t ncel3 = (size_t)nclx2 * ncly2 * nclz2;
+    int8_t *phivt = (int8_t *)malloc(ncel3 ? ncel3 : 1);
+    for (int n = 1; n <= walls; n++) {
+        memcpy(phivt, phiv, ncel3);
+        int ck = 0;
+        for (int a = 1 + n; a <= nclx2 - n; a++)
+            for (int b = 1 + n; b <= ncly2 - n; b++)
+                for (int c = 1 + n; c <= nclz2 - n; c++) {
+                    size_t pos = (size_t)(a - 1) * ncly2 * nclz2 + (size_t)(b - 1) * nclz2 + (c - 1);
+                    if (phiv[pos] != n) continue;
+                    ck = ck + 1;
+                    int phisum = 0;
+                    for (int i = a - 1; i <= a + 1; i++)
+                        for (int j = b - 1; j <= b + 1; j++)
+                            for (int k = c - 1; k <= c + 1; k++)
+                                phisum += phiv[(size_t)(i - 1) * ncly2 * nclz2 + (size_t)(j - 1) * nclz2 + (k - 1)];
+                    if (phisum == 27 * n) phivt[pos] = (int8_t)(n + 1);
+                }
+        memcpy(phiv, phivt, ncel3);
+        if (ck == 0) break;
+    }
+    free(phivt);
+    return BROMOC_OK;
+}

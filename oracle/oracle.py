@@ -72,6 +72,7 @@ def oracle_lib():
                                  P(abi.Pair)], C.c_int),
             ("orc_philox4x64", [P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)], None),
             ("orc_get_dids", [vp, _dp], C.c_int),
+            ("orc_repwalls", [C.c_int32, C.c_int32, C.c_int32, P(C.c_int8), C.c_int32], C.c_int),
             ("orc_metropolis", [vp, C.c_int32, C.c_double, C.c_double, _dp, C.c_int64,
                                 P(C.c_int64), _ip, _dp], C.c_int),
             ("orc_philox_gauss3", [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64, _dp], None),
@@ -255,6 +256,13 @@ def ref_lib(path=REF_SO):
             ("ref_spline", [C.c_int32, _dp, _dp, _dp, _dp, _dp], None),
             ("ref_notrans", [], None),
             ("ref_get_dids", [_dp], None),
+            ("ref_write_map", [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                               C.c_double, C.c_double, C.c_double, _dp, P(C.c_float)], None),
+            ("ref_readphi", [C.c_char_p, C.c_int32, C.c_int32, _dp, _ip, _dp, P(C.c_float)],
+             None),
+            ("ref_repwalls", [C.c_int32, C.c_int32, C.c_int32, P(C.c_int8), C.c_int32], None),
+            ("ref_readrfpar", [C.c_int32, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, _dp, _ip,
+                               _dp, _dp, P(C.c_float), P(C.c_float)], None),
             ("ref_metropolis", [C.c_int32, C.c_double, C.c_double, _dp, C.c_int64,
                                 P(C.c_int64), _dp], None),
         ]:
@@ -420,3 +428,60 @@ class RefFortran(_Base):
                                  C.byref(nn))
         self.ntot = nn.value
         return used.value, e.value
+
+
+# ------------------------------------------------------------------ map files
+def oracle_repwalls(phiv, walls=126):
+    """REPWALLS restated in C (oracle/bromoc_oracle.c)."""
+    out = np.ascontiguousarray(phiv, np.int8).copy()
+    nx, ny, nz = out.shape
+    oracle_lib().orc_repwalls(nx, ny, nz, out.ctypes.data_as(C.POINTER(C.c_int8)), walls)
+    return out
+
+
+def ref_repwalls(phiv, walls=126):
+    """REPWALLS of the reference build."""
+    out = np.ascontiguousarray(phiv, np.int8).copy()
+    nx, ny, nz = out.shape
+    ref_lib().ref_repwalls(nx, ny, nz, out.ctypes.data_as(C.POINTER(C.c_int8)), walls)
+    return out
+
+
+def ref_write_map(path, values, dcel, centre=(0.0, 0.0, 0.0), rec2=(80.0, 2.0, 0.0, 0.0, 0.0, 2.0)):
+    """A grid file written by Fortran's own unformatted write (the reference build's
+    compiler), in the record layout READPHI reads."""
+    v = np.ascontiguousarray(values, np.float32)
+    nx, ny, nz = v.shape
+    r2 = np.ascontiguousarray(rec2, float)
+    ref_lib().ref_write_map(str(path).encode(), nx, ny, nz, dcel, *centre, _p(r2),
+                            v.ctypes.data_as(C.POINTER(C.c_float)))
+
+
+def ref_readphi(path, kind, adjust=False, box=(0, 0, 0, 0, 0, 0), maxn=None):
+    """READPHI of the reference build: (dims, (dcel, xbcen, ybcen, zbcen), values)."""
+    b = np.ascontiguousarray(box, float)
+    dims = np.zeros(3, np.int32)
+    geom = np.zeros(4)
+    out = np.zeros(maxn or (1 << 22), np.float32)
+    ref_lib().ref_readphi(str(path).encode(), kind, 1 if adjust else 0, _p(b), _p(dims, C.c_int32),
+                          _p(geom), out.ctypes.data_as(C.POINTER(C.c_float)))
+    n = int(np.prod(dims))
+    return tuple(int(d) for d in dims), tuple(geom), out[:n].reshape(tuple(dims))
+
+
+def ref_readrfpar(paths, single=False, adjust=False, box=(0, 0, 0, 0, 0, 0), maxn=1 << 22):
+    """READRFPAR of the reference build on one or two files."""
+    b = np.ascontiguousarray(box, float)
+    dims = np.zeros(3, np.int32)
+    geom = np.zeros(4)
+    rad = np.zeros(2)
+    gs = np.zeros(maxn, np.float32)
+    gr = np.zeros(maxn, np.float32)
+    p2 = paths[1] if len(paths) > 1 else paths[0]
+    fp = C.POINTER(C.c_float)
+    ref_lib().ref_readrfpar(len(paths), str(paths[0]).encode(), str(p2).encode(),
+                            1 if single else 0, 1 if adjust else 0, _p(b), _p(dims, C.c_int32),
+                            _p(geom), _p(rad), gs.ctypes.data_as(fp), gr.ctypes.data_as(fp))
+    n = int(np.prod(dims)) * len(paths)
+    shape = (len(paths),) + tuple(int(d) for d in dims)
+    return tuple(int(d) for d in dims), tuple(geom), gs[:n].reshape(shape), gr[:n].reshape(shape)

@@ -836,3 +836,142 @@ pout%sc1 = sc(1,1); pout%sc2 = sc(2,1); pout%sc3 = sc(3,1)
 pout%nknots = n; pout%efpot = 1; pout%chr = chr; pout%fct = fctv
 deallocate(xs, ys)
 end subroutine
+
+! ------------------------------------------------------------- map files
+! C string -> Fortran character
+subroutine c2f_path(cpath, fpath)
+use iso_c_binding
+implicit none
+character(kind=c_char), intent(in) :: cpath(*)
+character(len=*), intent(out) :: fpath
+integer :: i
+fpath = ' '
+do i = 1, len(fpath)
+  if (cpath(i) == c_null_char) exit
+  fpath(i:i) = cpath(i)
+enddo
+end subroutine
+
+! a grid file written by Fortran's own unformatted sequential write, in the record layout
+! readphi.f90:43-44 reads
+subroutine ref_write_map(cpath, nx, ny, nz, dcel, xc, yc, zc, rec2, vals) bind(C, name='ref_write_map')
+use iso_c_binding
+implicit none
+character(kind=c_char), intent(in) :: cpath(*)
+integer(c_int32_t), value :: nx, ny, nz
+real(c_double), value :: dcel, xc, yc, zc
+real(c_double), intent(in) :: rec2(6)
+real(c_float), intent(in) :: vals(*)
+character(len=1024) :: path
+integer(4) :: i4x, i4y, i4z
+real(8) :: d8, x8, y8, z8
+integer :: u, i, n
+call c2f_path(cpath, path)
+i4x = nx; i4y = ny; i4z = nz; d8 = dcel; x8 = xc; y8 = yc; z8 = zc
+n = nx*ny*nz
+open(newunit=u, file=trim(path), form='unformatted', access='sequential', status='replace')
+write(u) i4x, i4y, i4z, d8, x8, y8, z8
+write(u) (real(rec2(i), 8), i = 1, 6)
+write(u) (vals(i), i = 1, n)
+close(u)
+end subroutine
+
+! READPHI (readphi.f90) on a file: kind 0 PHIX, 1 PHIV; the system box from box(6)
+subroutine ref_readphi(cpath, kind, adjust, box, dims, geom, out) bind(C, name='ref_readphi')
+use ref_harness
+use grandmod
+use gsbpmod
+implicit none
+character(kind=c_char), intent(in) :: cpath(*)
+integer(c_int32_t), value :: kind, adjust
+real(c_double), intent(in) :: box(6)
+integer(c_int32_t), intent(out) :: dims(3)
+real(c_double), intent(out) :: geom(4)
+real(c_float), intent(out) :: out(*)
+character(len=1024) :: path
+integer :: u, nul, n
+logical(1) :: adj
+call c2f_path(cpath, path)
+lx2m = box(1); lx2p = box(2); ly2m = box(3); ly2p = box(4); lz2m = box(5); lz2p = box(6)
+Qnmcden = .false.
+adj = adjust /= 0
+open(newunit=u, file=trim(path), form='unformatted', access='sequential', status='old')
+open(newunit=nul, file='/dev/null', status='old')
+if (kind == 0) then
+  call readphi(u, nul, 'PHIX', adj)
+  dims = (/ nclx1, ncly1, nclz1 /)
+  geom = (/ dcel1, xbcen1, ybcen1, zbcen1 /)
+  n = nclx1*ncly1*nclz1
+  out(1:n) = phix(1:n)
+else
+  call readphi(u, nul, 'PHIV', adj)
+  dims = (/ nclx2, ncly2, nclz2 /)
+  geom = (/ dcel2, xbcen2, ybcen2, zbcen2 /)
+  n = nclx2*ncly2*nclz2
+  out(1:n) = phiv(1:n)
+endif
+close(u)
+close(nul)
+end subroutine
+
+! REPWALLS (repwalls.f90) on the PHIV map the last ref_readphi left (or the given one)
+subroutine ref_repwalls(nx, ny, nz, map, walls) bind(C, name='ref_repwalls')
+use ref_harness
+use gsbpmod
+implicit none
+integer(c_int32_t), value :: nx, ny, nz, walls
+integer(c_int8_t), intent(inout) :: map(*)
+integer(1) :: w
+integer :: n
+nclx2 = nx; ncly2 = ny; nclz2 = nz
+n = nx*ny*nz
+Qnmcden = .false.
+if (allocated(phiv)) deallocate(phiv)
+allocate(phiv(n))
+phiv(1:n) = map(1:n)
+w = int(walls, 1)
+call repwalls(w)
+map(1:n) = phiv(1:n)
+end subroutine
+
+! READRFPAR (rfpar.f90:373-582) on one or two files; nion is set to the file count
+subroutine ref_readrfpar(nfiles, cpath1, cpath2, single, adjust, box, dims, geom, radion_out, &
+                         gs, gr) bind(C, name='ref_readrfpar')
+use ref_harness
+use grandmod
+use gsbpmod
+use nucleotmod
+implicit none
+integer(c_int32_t), value :: nfiles, single, adjust
+character(kind=c_char), intent(in) :: cpath1(*), cpath2(*)
+real(c_double), intent(in) :: box(6)
+integer(c_int32_t), intent(out) :: dims(3)
+real(c_double), intent(out) :: geom(4), radion_out(*)
+real(c_float), intent(out) :: gs(*), gr(*)
+character(len=1024) :: path1, path2
+integer :: u(2), nul, n, k
+logical(1) :: adj
+call c2f_path(cpath1, path1)
+call c2f_path(cpath2, path2)
+lx2m = box(1); lx2p = box(2); ly2m = box(3); ly2p = box(4); lz2m = box(5); lz2p = box(6)
+Qrfpsin = single /= 0
+nion = nfiles
+ndna = 0
+atnam2(1:2) = (/ 'K   ', 'CL  ' /)
+open(newunit=u(1), file=trim(path1), form='unformatted', access='sequential', status='old')
+if (nfiles > 1) open(newunit=u(2), file=trim(path2), form='unformatted', access='sequential', status='old')
+open(newunit=nul, file='/dev/null', status='old')
+adj = adjust /= 0
+if (allocated(gsrfen)) deallocate(gsrfen, greff)
+call readrfpar(u(1:nfiles), int(nfiles), nul, adj)
+dims = (/ nclx3, ncly3, nclz3 /)
+geom = (/ dcel3, xbcen3, ybcen3, zbcen3 /)
+n = nclx3*ncly3*nclz3*nfiles
+gs(1:n) = gsrfen(1:n)
+gr(1:n) = greff(1:n)
+do k = 1, nfiles
+  close(u(k))
+enddo
+close(nul)
+radion_out(1) = 0.0
+end subroutine

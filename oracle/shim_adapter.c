@@ -89,3 +89,9 @@ int bromoc_grand(bromoc_ctx *ctx, int32_t ng, const double *u, int64_t nu, int64
                  int32_t *nat, int32_t *prob, int64_t icyc, double *ener) {
     return orc_grand(O(ctx), ng, u, nu, used, nat, prob, icyc, ener);
 }
+
+int orc_repwalls(int32_t nx, int32_t ny, int32_t nz, int8_t *phiv, int32_t walls);
+int bromoc_repwalls(int32_t nx, int32_t ny, int32_t nz, int8_t *phiv, int32_t walls, int device) {
+    (void)device;
+    return orc_repwalls(nx, ny, nz, phiv, walls);
+}

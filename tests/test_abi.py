@@ -31,7 +31,8 @@ def test_exports_every_declared_symbol():
 
 def test_struct_layout_matches_header(tmp_path):
     structs = {"bromoc_pair": abi.Pair, "bromoc_grid": abi.Grid, "bromoc_system": abi.SystemStruct,
-               "bromoc_energies": abi.Energies, "bromoc_gcmc": abi.GcmcStruct}
+               "bromoc_energies": abi.Energies, "bromoc_gcmc": abi.GcmcStruct,
+               "bromoc_replica": abi.ReplicaStruct, "bromoc_map_header": abi.MapHeader}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HDR}"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -170,3 +170,18 @@ def test_metropolis_through_front_end():
     _, _, eb = r.metropolis(250, 2.0, ikbt, u, 0.75)
     assert abs(ea - eb) <= 1e-12 * max(abs(eb), 1.0)
     assert np.array_equal(f.positions()[0], r.positions()[0])
+
+
+def test_repwalls_through_front_end():
+    """REPWALLS replaced by the front-end's external (bromoc_repwalls on the module map)."""
+    from oracle.oracle import ref_lib
+    from test_maps import _body_map
+    import ctypes as C
+    phiv = _body_map()
+    outs = []
+    for path in (SHIM_SO, None):
+        L = ref_lib(path) if path else ref_lib()
+        m = phiv.copy()
+        L.ref_repwalls(*m.shape, m.ctypes.data_as(C.POINTER(C.c_int8)), 126)
+        outs.append(m)
+    assert np.array_equal(outs[0], outs[1]) and outs[0].max() > 1
