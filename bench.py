@@ -193,6 +193,42 @@ def large_leg(cycles=4, gcmc_every=10, ngcmc=20, cpu=True):
     return out
 
 
+def adjoint_leg(nsteps=400, checkpoint_every=20, reps=3):
+    """The differentiation workload: 200 K+ + 200 Cl- + a fixed wall site in a 60 A box,
+    4 x 1851 knot parameters, 400 BD steps of fixed injected noise checkpointed every 20,
+    smoothed-RDF loss against a second seed's RDFs.  Times the forward pass alone and the
+    full gradient (forward with checkpoints + reverse pass with recomputation + chain rule
+    through the table build); a central-difference gradient would take 2 forward passes
+    per knot (14,808 passes)."""
+    import numpy as np
+    from bromoc_amd import configs
+    from bromoc_amd.adjoint import RdfSpec, rdf_loss
+    from bromoc_amd.engine import Engine
+    spec = RdfSpec(groups=[(1, 1), (1, 2), (2, 2), (1, 3)])
+    s = configs.adjoint_system(seed=10, n_per_type=200, box=60.0)
+    noise = np.random.default_rng(11).standard_normal((nsteps, 3 * s.ntot))
+    t = configs.with_knots(s, {is_: u for is_, (u, _, _) in configs.adjoint_system(
+        seed=10, n_per_type=200, box=60.0, curve_seed=12).meta["params"].items()})
+    target = rdf_loss(t, noise, nsteps, spec, grad=False)[1]
+    e = Engine(s)
+    rdf_loss(s, noise, nsteps, spec, target=target, grad=True, engine=e)     # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rdf_loss(s, noise, nsteps, spec, target=target, grad=False, engine=e)
+    fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, _, grads = rdf_loss(s, noise, nsteps, spec, target=target, grad=True,
+                               checkpoint_every=checkpoint_every, engine=e)
+    full = (time.perf_counter() - t0) / reps
+    nknots = sum(len(g) for g in grads.values())
+    e.close()
+    return {"particles": s.ntot, "steps": nsteps, "checkpoint_every": checkpoint_every,
+            "knot_parameters": nknots, "forward_s": fwd, "gradient_s": full,
+            "gradient_over_forward": full / fwd,
+            "central_difference_equivalent_s": 2 * nknots * fwd}
+
+
 def copy_probe(mib):
     """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
     import torch
@@ -218,6 +254,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the replica-batch legs")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the large translocation leg (56,000 particles)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -291,6 +328,7 @@ def main():
     eng.close()
     batched, small_n = (None, None) if args.no_batch else batch_legs()
     large = None if args.no_large else large_leg(cpu=not args.no_cpu)
+    adjoint = None if args.no_adjoint else adjoint_leg()
     cpu = None if args.no_cpu else cpu_baseline(s)
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
@@ -305,6 +343,7 @@ def main():
                      "targets_per_block": tpb},
         "cpu_baseline": cpu, "host_roundtrip": host, "batched": batched,
         "batched_vs_plain_small_n": small_n, "large_translocation": large,
+        "adjoint_rdf_gradient": adjoint,
         "parity_vs_oracle": parity,
         "setup_s": build_s,
     }

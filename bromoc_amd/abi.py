@@ -90,6 +90,10 @@ class SystemStruct(C.Structure):
         ("notrx", C.c_double), ("notry", C.c_double), ("notrz", C.c_double),
         ("ibeta", C.c_double), ("diff0", C.c_double), ("diffcutoff", C.c_double),
         ("thold27", C.c_double),
+        ("qpore", C.c_int32), ("reserved4", C.c_int32),
+        ("voltage", C.c_double), ("afact", C.c_double), ("ceps", C.c_double),
+        ("zmemb1", C.c_double), ("zmemb2", C.c_double),
+        ("ampl1", _dp), ("p1", _dp), ("ampl2", _dp), ("p2", _dp), ("rcylinder", _dp),
     ]
 
 

@@ -698,6 +698,37 @@ def with_notrans(s: System, axes=7):
     return s
 
 
+def with_membrane(s: System, thick=20.0, zmemb=0.0, epsm=2.0, volts=0.1, debye=9.6,
+                  amplmemb=None, pmemb=None, pore=None):
+    """MEMBRANE command (shell_simul.f90:1563-1640): a slab of thickness `thick` centred at
+    zmemb with dielectric epsm, the Nernst potential of `volts` across it (Debye length
+    `debye`, shared with the DNA's Debye screening), per atnam2 type wall heights amplmemb
+    and switching lengths pmemb; with pore = (radius, amplpore, ppore) a cylindrical pore
+    through it (Qpore)."""
+    s = s.copy()
+    s.flags |= abi.QMEMB
+    nt1 = s.nttyp + 1
+    half = 0.5 * thick
+    volt = volts * COULOMB / KCALMOL
+    s.kappa, s.ikappa = debye, 1.0 / debye
+    cdie = s.meta.get("cdie", 80.0)
+    m = dict(zmemb1=-half + zmemb, zmemb2=half + zmemb, voltage=volt,
+             afact=epsm * volt / (2.0 * epsm + cdie * s.ikappa * thick), ceps=cdie / epsm,
+             qpore=pore is not None)
+    a1 = np.zeros(nt1)
+    p1 = np.ones(nt1)
+    a2 = np.zeros(nt1)
+    p2 = np.ones(nt1)
+    rc = np.ones(nt1)
+    a1[1:] = 10.0 if amplmemb is None else amplmemb
+    p1[1:] = 2.0 if pmemb is None else pmemb
+    if pore is not None:
+        rc[1:], a2[1:], p2[1:] = pore
+    m.update(ampl1=a1, p1=p1, ampl2=a2, p2=p2, rcylinder=rc)
+    s.membrane = m
+    return s
+
+
 def with_rfpar(s: System, seed=0, shape=None, dcel=None, single=False, reffac=1.5,
                wall_radius=None, zero_frac=0.0, const_reff=None):
     """Synthetic reaction-field parameter grids (RFPAR command, rfpar.f90:373-582): per

@@ -28,7 +28,8 @@ constexpr uint32_t SUPPORTED = BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BR
                                BROMOC_QDEBY | BROMOC_QCOUNTION | BROMOC_QDNAFREE |
                                BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR |
                                BROMOC_QRFPSIN | BROMOC_QBSPLINE | BROMOC_QPROXDIFF |
-                               BROMOC_QNOTRANS | BROMOC_QAPFOR | BROMOC_QCONTRANS;
+                               BROMOC_QNOTRANS | BROMOC_QAPFOR | BROMOC_QCONTRANS |
+                               BROMOC_QMEMB;
 
 // constamod.f90:21-32 (binary64)
 double celec_value() {
@@ -304,6 +305,9 @@ int validate(const bromoc_system *s, std::string &msg) {
             return bad("CONTRANS strand split");
     }
     if ((s->flags & BROMOC_QNOTRANS) && (s->notrans_axes & ~7)) return bad("NOTRANS axes");
+    if ((s->flags & BROMOC_QMEMB) &&
+        (!s->ampl1 || !s->p1 || !s->ampl2 || !s->p2 || !s->rcylinder || !(s->zmemb2 >= s->zmemb1)))
+        return bad("MEMBRANE parameters");
     return BROMOC_OK;
 }
 
@@ -606,6 +610,15 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
         ACHK(dupload(c, &d_c[a], con ? csrc[a] : (const double *)nullptr, con ? ctn + 1 : 0));
     }
     ACHK(dalloc(c, &d_cen, 9));
+    // MEMBRANE per atnam2 type
+    const bool memb = (s->flags & BROMOC_QMEMB) != 0;
+    double *d_m[5] = {};
+    {
+        const double *src[5] = {s->ampl1, s->p1, s->ampl2, s->p2, s->rcylinder};
+        const size_t len[5] = {(size_t)nt1, 2 * (size_t)nt1, (size_t)nt1, 2 * (size_t)nt1, (size_t)nt1};
+        for (int k = 0; k < 5; k++)
+            ACHK(dupload(c, &d_m[k], memb ? src[k] : (const double *)nullptr, memb ? len[k] : 0));
+    }
 #undef ACHK
 
     StepParams &p = c->base;
@@ -651,6 +664,10 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.cen = d_cen;
     p.notrans_axes = (s->flags & BROMOC_QNOTRANS) ? s->notrans_axes : 0;
     p.notrx = s->notrx; p.notry = s->notry; p.notrz = s->notrz;
+    p.qpore = s->qpore ? 1 : 0;
+    p.voltage = s->voltage; p.afact = s->afact; p.ceps = s->ceps;
+    p.zmemb1 = s->zmemb1; p.zmemb2 = s->zmemb2;
+    p.ampl1 = d_m[0]; p.p1 = d_m[1]; p.ampl2 = d_m[2]; p.p2 = d_m[3]; p.rcylinder = d_m[4];
     p.nspline = nsp;
     p.dt = s->dt; p.kbtdt = s->kbtdt; p.plength2 = s->plength2; p.pcenter = s->pcenter;
     p.diffusion = d_diff; p.ampl3 = d_ampl3; p.p3 = d_p3;
@@ -780,6 +797,7 @@ static hipError_t enqueue_step(bromoc_ctx *c, const StepParams &p, hipEvent_t t0
     const int ns = p.nsites, nt = p.ntargets;
     hipError_t e;
     if ((e = launch_rfpar(p, true, c->stream)) != hipSuccess) return e;
+    if ((e = launch_fixed(p, c->stream)) != hipSuccess) return e;
     static const bool fork = [] {
         const char *v = std::getenv("BROMOC_FORK");
         return v && v[0] == '1';

@@ -26,7 +26,7 @@ constexpr uint32_t QPAR = 1u << 0, QNUCL = 1u << 1, QNONBOND = 1u << 2, QNOBOND 
 constexpr uint32_t QPHIX = 1u << 4, QPHIV = 1u << 5, QSVDW = 1u << 6, QSOLV = 1u << 7;
 constexpr uint32_t QDEBY = 1u << 8, QCOUNTION = 1u << 9, QDNAFREE = 1u << 10;
 constexpr uint32_t QBSPLINE = 1u << 18, QPROXDIFF = 1u << 21, QNOTRANS = 1u << 22;
-constexpr uint32_t QAPFOR = 1u << 23, QCONTRANS = 1u << 24;
+constexpr uint32_t QAPFOR = 1u << 23, QCONTRANS = 1u << 24, QMEMB = 1u << 16;
 
 constexpr double PI = 3.14159265358979323846264338327950288419716939937510;
 constexpr double RSMALL = 1E-3;
@@ -271,6 +271,143 @@ __device__ inline bool vdw_grid(const StepParams &P, double x, double y, double 
     // map values are integers: the sum is exact in any order
     const double sum = wave_sum(ph);
     return ok && sum >= ((P.flags & QBSPLINE) ? P.thold27 : P.thold8);
+}
+
+// SWITCH1 (switch1.f90:19-47): the membrane walls' switch along z and its derivative.
+__device__ __forceinline__ void mswitch1(double z, double p1, double ip1, double zm1, double zm2,
+                                         double &sw, double &dsw) {
+    if (z >= zm2 + p1 || z <= zm1 - p1) {
+        sw = 0.0; dsw = 0.0;
+    } else if (z > zm2) {
+        const double d = (z - zm2) * ip1;
+        sw = 1.0 + (2.0 * d - 3.0) * (d * d);
+        dsw = -6.0 * (d - 1.0) * ip1 * d;
+    } else if (z < zm1) {
+        const double d = (zm1 - z) * ip1;
+        sw = 1.0 + (2.0 * d - 3.0) * (d * d);
+        dsw = 6.0 * (d - 1.0) * ip1 * d;
+    } else {
+        sw = 1.0; dsw = 0.0;
+    }
+}
+
+// SWITCH2 (switch2.f90:19-38): the pore wall's switch in r^2; r is set in the wall only.
+__device__ __forceinline__ void mswitch2(double r2, double radius, double p2, double ip2,
+                                         double &sw, double &dsw, double &r) {
+    if (r2 <= radius * radius) {
+        sw = 0.0; dsw = 0.0;
+    } else if (r2 < (radius + p2) * (radius + p2)) {
+        r = sqrt(r2);
+        const double d = (radius + p2 - r) * ip2;
+        sw = 1.0 + (2.0 * d - 3.0) * (d * d);
+        dsw = 6.0 * (d - 1.0) * d * ip2;
+    } else {
+        sw = 1.0; dsw = 0.0;
+    }
+}
+
+// MEMBRANE's term of one particle (membrane.f90:36-104): the Nernst potential of a charge
+// and the membrane / pore walls of atnam2 type t2.  Returns the energy, adds the force to
+// F, sets warn when the particle sits inside the membrane (counted by the caller).
+__device__ inline double membrane_energy(const StepParams &P, double x, double y, double z,
+                                         double charge, int t2, double F[3], bool &warn) {
+    double em = 0.0;
+    warn = false;
+    if (P.voltage != 0.0 && charge != 0.0) {
+        if (z < P.zmemb1) {
+            const double ex = exp(P.ikappa * (z - P.zmemb1));
+            em = em + charge * P.afact * ex;
+            F[2] = F[2] - charge * P.afact * P.ikappa * ex;
+        } else if (z >= P.zmemb1 && z <= P.zmemb2) {
+            em = em + charge * P.afact * (P.ceps * P.ikappa * (z - P.zmemb1) + 1.0);
+            F[2] = F[2] - charge * P.afact * P.ceps * P.ikappa;
+        } else if (z > P.zmemb2) {
+            const double ex = exp(-P.ikappa * (z - P.zmemb2));
+            em = em + charge * (P.voltage - P.afact * ex);
+            F[2] = F[2] - charge * P.afact * P.ikappa * ex;
+        }
+    }
+    const double a1 = P.ampl1[t2];
+    if (a1 > 0.0) {
+        double sw1, dsw1;
+        mswitch1(z, P.p1[2 * t2], P.p1[2 * t2 + 1], P.zmemb1, P.zmemb2, sw1, dsw1);
+        if (sw1 != 0.0) {
+            if (P.qpore) {
+                double sw2, dsw2, r = 0.0;
+                const double a2 = P.ampl2[t2];
+                mswitch2(x * x + y * y, P.rcylinder[t2], P.p2[2 * t2], P.p2[2 * t2 + 1], sw2, dsw2, r);
+                if (sw2 != 0.0) {
+                    if (sw1 == 1.0) {
+                        if (sw2 == 1.0) {
+                            warn = true;
+                            em = em + a2;
+                        } else {
+                            const double ir = 1.0 / r;
+                            F[0] = F[0] + a2 * dsw2 * x * ir;
+                            F[1] = F[1] + a2 * dsw2 * y * ir;
+                            em = em + a2 * sw2;
+                        }
+                    } else if (sw2 == 1.0) {
+                        F[2] = F[2] + a1 * dsw1;
+                        em = em + a1 * sw1;
+                    } else {
+                        const double ir = 1.0 / r;
+                        F[0] = F[0] + 0.5 * a2 * dsw2 * x * ir;
+                        F[1] = F[1] + 0.5 * a2 * dsw2 * y * ir;
+                        F[2] = F[2] + 0.5 * a1 * dsw1;
+                        em = em + 0.5 * (a2 * sw2 + a1 * sw1);
+                    }
+                }
+            } else {
+                if (sw1 == 1.0) warn = true;
+                em = em + a1 * sw1;
+            }
+        }
+    }
+    return em;
+}
+
+// INTERACT's membrane term (interact.f90:58-119): as ENERGY's, but a particle inside the
+// membrane costs 1e10 (and a warning when alert is set).
+__device__ inline double membrane_interact(const StepParams &P, double x, double y, double z,
+                                           double charge, int t2, bool &warn) {
+    double em = 0.0;
+    warn = false;
+    if (P.voltage != 0.0 && charge != 0.0) {
+        if (z < P.zmemb1) em = em + charge * P.afact * exp(P.ikappa * (z - P.zmemb1));
+        else if (z >= P.zmemb1 && z <= P.zmemb2)
+            em = em + charge * P.afact * (P.ceps * P.ikappa * (z - P.zmemb1) + 1.0);
+        else if (z > P.zmemb2)
+            em = em + charge * (P.voltage - P.afact * exp(-P.ikappa * (z - P.zmemb2)));
+    }
+    const double a1 = P.ampl1[t2];
+    if (a1 > 0.0) {
+        double sw1, dsw1;
+        mswitch1(z, P.p1[2 * t2], P.p1[2 * t2 + 1], P.zmemb1, P.zmemb2, sw1, dsw1);
+        if (sw1 != 0.0) {
+            if (P.qpore) {
+                double sw2, dsw2, r = 0.0;
+                const double a2 = P.ampl2[t2];
+                mswitch2(x * x + y * y, P.rcylinder[t2], P.p2[2 * t2], P.p2[2 * t2 + 1], sw2, dsw2, r);
+                if (sw2 != 0.0) {
+                    if (sw1 == 1.0) {
+                        if (sw2 == 1.0) { warn = true; em = em + 1.0e10; }
+                        else em = em + a2 * sw2;
+                    } else if (sw2 == 1.0) {
+                        em = em + a1 * sw1;
+                    } else {
+                        em = em + 0.5 * (a2 * sw2 + a1 * sw1);
+                    }
+                }
+            } else if (sw1 == 1.0) {
+                warn = true;
+                em = em + 1.0e10;
+            } else {
+                em = em + a1 * sw1;
+            }
+        }
+    }
+    return em;
 }
 
 // Diffusion profile of DYNAMICS2 (dynamics2.mono.f90:52-74): the end values outside the

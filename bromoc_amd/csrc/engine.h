@@ -133,6 +133,10 @@ struct StepParams {
     double *cen;             // [9]
     // NOTRANS centre
     double notrx, notry, notrz;
+    // MEMBRANE (per atnam2 type arrays: ampl1, ampl2, rcylinder [nttyp+1], p1, p2 [2*(nttyp+1)])
+    int32_t qpore, pad1;
+    double voltage, afact, ceps, zmemb1, zmemb2;
+    const double *ampl1, *p1, *ampl2, *p2, *rcylinder;
 };
 
 // spatial partner order (sort.hip)
@@ -158,6 +162,7 @@ hipError_t launch_dna(const StepParams &p, hipStream_t s);    // k_dna (CG-DNA l
 hipError_t launch_pairs(const StepParams &p, hipStream_t s);  // k_step (pairs, grids, update)
 hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s);  // RFPARION
 hipError_t launch_notrans(const StepParams &p, hipStream_t s);  // NOTRANS after a step
+hipError_t launch_fixed(const StepParams &p, hipStream_t s);    // terms of the fixed ions
 hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
                                 hipStream_t s);
 hipError_t launch_advance(int64_t *ctr, int64_t by, hipStream_t s);

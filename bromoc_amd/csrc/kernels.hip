@@ -302,6 +302,13 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
     const double4 pa = P.pos[i];
     const int t2a = __double2loint(pa.w);
     if (s_warn[a]) atomicAdd(&P.warn[t2a], 1);
+    if (P.flags & QMEMB) {   // MEMBRANE, membrane.f90:36-104
+        const double chg = is_site ? (P.namsite[i] == 1 ? P.cgnuc : 0.0) : P.cg[abs(P.typei[i])];
+        bool mw;
+        const double em = membrane_energy(P, pa.x, pa.y, pa.z, chg, t2a, F, mw);
+        if (mw) atomicAdd(&P.warn[t2a], 1);
+        if (P.write_energy) P.epart[(size_t)E_MEMB * P.ecap + i] = em;
+    }
     if (P.write_energy) {
         double pe[4];
 #pragma unroll
@@ -455,6 +462,26 @@ __global__ __launch_bounds__(BLOCK) void k_notrans(StepParams P) {
         P.poso[i] = q;
         P.sposo[P.rank[i]] = q;
     }
+}
+
+// k_fixed: the terms of the fixed ions that no step kernel visits (they are never moved):
+// their MEMBRANE energy and wall warnings (membrane.f90 loops over every particle).
+__global__ void k_fixed(StepParams P) {
+    const int i = P.nsites + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nsites + P.nfix) return;
+    const double4 pa = P.pos[i];
+    const int t2 = __double2loint(pa.w);
+    double F[3] = {0.0, 0.0, 0.0};
+    bool mw;
+    const double em = membrane_energy(P, pa.x, pa.y, pa.z, P.cg[abs(P.typei[i])], t2, F, mw);
+    if (mw) atomicAdd(&P.warn[t2], 1);
+    if (P.write_energy) P.epart[(size_t)E_MEMB * P.ecap + i] = em;
+}
+
+hipError_t launch_fixed(const StepParams &p, hipStream_t s) {
+    if ((p.flags & QMEMB) && p.nfix > 0)
+        hipLaunchKernelGGL(k_fixed, dim3((p.nfix + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
 }
 
 hipError_t launch_notrans(const StepParams &p, hipStream_t s) {
@@ -652,6 +679,13 @@ __global__ __launch_bounds__(BLOCK) void k_wave(StepParams P) {
         F[0] += fr.x; F[1] += fr.y; F[2] += fr.z;
     }
     if (warned) atomicAdd(&P.warn[t2i], 1);
+    if (P.flags & QMEMB) {   // MEMBRANE, membrane.f90:36-104
+        const double chg = is_site ? (P.namsite[i] == 1 ? P.cgnuc : 0.0) : P.cg[abs(P.typei[i])];
+        bool mw;
+        const double em = membrane_energy(P, pa.x, pa.y, pa.z, chg, t2i, F, mw);
+        if (mw) atomicAdd(&P.warn[t2i], 1);
+        if (P.write_energy) P.epart[(size_t)E_MEMB * P.ecap + i] = em;
+    }
     if (P.write_energy) {
         double *ep_ = P.epart + i;
         const size_t cs = P.ecap;
@@ -766,6 +800,7 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s) {
 hipError_t launch_step(const StepParams &p, hipStream_t s) {
     hipError_t e = launch_rfpar(p, true, s);
     if (e != hipSuccess) return e;
+    if ((e = launch_fixed(p, s)) != hipSuccess) return e;
     e = launch_dna(p, s);
     if (e != hipSuccess) return e;
     return launch_pairs(p, s);
@@ -1030,6 +1065,14 @@ __global__ __launch_bounds__(BLOCK) void k_interact(InteractParams Q) {
     const int jt2 = beion ? P.nwtype[Q.jtype] : __double2loint(P.pos[j].w);
     const int prow = jt2 * nt1;
     double e = 0.0;
+    // MEMBRANE part of INTERACT (interact.f90:58-119)
+    if ((P.flags & QMEMB) && tid == 0) {
+        const bool site_p = (P.flags & QNUCL) && j < ns && P.namsite[j] == 1;
+        const double chg = beion ? P.cg[Q.jtype] : (site_p ? P.cgnuc : 0.0);
+        bool mw;
+        e += membrane_interact(P, xj, yj, zj, chg, jt2, mw);
+        if (mw && Q.alert) atomicAdd(&P.warn[jt2], 1);
+    }
     // staticf0.f90:19-82 and vdwgd0trln.f90:19-117 on lanes 0..7 of wave 0
     if (tid < WAVE) {
         const bool besite = (P.flags & QNUCL) && j < ns && P.namsite[j] == 1;

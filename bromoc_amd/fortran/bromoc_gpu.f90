@@ -59,6 +59,9 @@ type, bind(C) :: bgpu_system
   type(c_ptr) :: sn, af, csn, kx, ky, kz, contrx, contry, contrz
   integer(c_int32_t) :: unsplit, nsites1st, notrans_axes, reserved3
   real(c_double) :: notrx, notry, notrz, ibeta, diff0, diffcutoff, thold27
+  integer(c_int32_t) :: qpore, reserved4
+  real(c_double) :: voltage, afact, ceps, zmemb1, zmemb2
+  type(c_ptr) :: ampl1, p1, ampl2, p2, rcylinder
 end type
 
 type, bind(C) :: bgpu_energies
@@ -253,6 +256,7 @@ subroutine bromoc_gpu_init(seed)
   real(c_float), allocatable, target :: gx(:), rfs(:), rfr(:)
   real(c_double), allocatable, target :: dif(:), am3(:), pp3(:), spx(:), spy(:), spb(:), spc(:), spd(:)
   integer(c_int32_t), allocatable, target :: asn(:), ccsn(:)
+  real(c_double), allocatable, target :: ma1(:), mp1(:), ma2(:), mp2(:), mrc(:)
   real(c_double), allocatable, target :: aaf(:), ckx(:), cky(:), ckz(:), ccx(:), ccy(:), ccz(:)
   integer :: nc
   integer :: nsp, ngrid
@@ -459,6 +463,20 @@ subroutine bromoc_gpu_init(seed)
   ! PROXDIFF (grandmod beta, diff0, diffcutoff) and the B-spline wall threshold
   s%ibeta = ibeta; s%diff0 = diff0; s%diffcutoff = diffcutoff
   s%thold27 = thold27
+  ! MEMBRANE (grandmod, shell_simul.f90:1563-1640), per atnam2 type
+  allocate(ma1(nt1), mp1(2*nt1), ma2(nt1), mp2(2*nt1), mrc(nt1))
+  ma1 = 0.0; mp1 = 1.0; ma2 = 0.0; mp2 = 1.0; mrc = 1.0
+  if (Qmemb) then
+    ma1(2:nt1) = ampl1(1:nttyp); ma2(2:nt1) = ampl2(1:nttyp); mrc(2:nt1) = rcylinder(1:nttyp)
+    do i = 1, nttyp
+      mp1(2*i+1) = p1(1,i); mp1(2*i+2) = p1(2,i)
+      mp2(2*i+1) = p2(1,i); mp2(2*i+2) = p2(2,i)
+    enddo
+  endif
+  s%qpore = merge(1, 0, Qmemb .and. Qpore)
+  s%voltage = voltage; s%afact = afact; s%ceps = ceps; s%zmemb1 = zmemb1; s%zmemb2 = zmemb2
+  s%ampl1 = c_loc(ma1); s%p1 = c_loc(mp1); s%ampl2 = c_loc(ma2); s%p2 = c_loc(mp2)
+  s%rcylinder = c_loc(mrc)
   call bgpu_check(bromoc_create(s, 0, gctx), 'bromoc_create')
 end subroutine
 

@@ -123,6 +123,9 @@ class System:
     diff0: float = 0.0
     diffcutoff: float = 0.0
     thold27: float = 1.0                 # B-spline wall threshold
+    # MEMBRANE: dict(voltage, afact, ceps, zmemb1, zmemb2, qpore, and per atnam2 type
+    # [nttyp+1] ampl1, p1, ampl2, p2, rcylinder)
+    membrane: dict = None
     ndna: int = 0
     nion: int = 0
     meta: dict = field(default_factory=dict)
@@ -282,6 +285,18 @@ class System:
             s.notrx, s.notry, s.notrz = (float(v) for v in self.notrans[1])
         s.ibeta, s.diff0, s.diffcutoff = self.ibeta, self.diff0, self.diffcutoff
         s.thold27 = self.thold27
+        m = self.membrane
+        if m is not None:
+            s.qpore = 1 if m.get("qpore") else 0
+            s.voltage, s.afact, s.ceps = m["voltage"], m["afact"], m["ceps"]
+            s.zmemb1, s.zmemb2 = m["zmemb1"], m["zmemb2"]
+            s.ampl1 = arr(m["ampl1"], np.float64, C.c_double)
+            s.ampl2 = arr(m["ampl2"], np.float64, C.c_double)
+            s.rcylinder = arr(m["rcylinder"], np.float64, C.c_double)
+            s.p1 = arr(np.stack([m["p1"], 1.0 / np.asarray(m["p1"], float)], 1).reshape(-1),
+                       np.float64, C.c_double)
+            s.p2 = arr(np.stack([m["p2"], 1.0 / np.asarray(m["p2"], float)], 1).reshape(-1),
+                       np.float64, C.c_double)
         return s, keep
 
 

@@ -31,7 +31,8 @@
  * simul1.f90:369-380 selects; bromoc_step also applies NOTRANS (simul1.f90:382-386,
  * BROMOC_QNOTRANS) after each step.  ENERGY and INTERACT include RFPARION / RFPARIONJ
  * (rfpar.f90:20-370) under BROMOC_QRFPAR, the B-spline repulsion map (vdwgd1spln.f90:19-
- * 170, vdwgd0spln.f90) under BROMOC_QBSPLINE; ENERGY includes APFOR and CONTRANS
+ * 170, vdwgd0spln.f90) under BROMOC_QBSPLINE, the membrane (membrane.f90:19-107,
+ * interact.f90:58-119) under BROMOC_QMEMB; ENERGY includes APFOR and CONTRANS
  * (energy.mono.f90:509-580) under BROMOC_QAPFOR / BROMOC_QCONTRANS.
  *
  * Conventions
@@ -92,8 +93,8 @@ extern "C" {
                                         * bromoc_step (simul1.f90:382-386) */
 #define BROMOC_QAPFOR      (1u << 23)  /* APFOR: constant forces on sites (Qapfor) */
 #define BROMOC_QCONTRANS   (1u << 24)  /* CONTRANS: harmonic site/centroid restraints */
-/* options of the reference that this path rejects with BROMOC_E_UNSUPPORTED */
 #define BROMOC_QMEMB       (1u << 16)  /* analytic membrane (membrane.f90) */
+/* options of the reference that this path rejects with BROMOC_E_UNSUPPORTED */
 #define BROMOC_QMMIJ       (1u << 17)  /* MMIJ reaction field (rect_rf1/sphe_rf1) */
 #define BROMOC_QNMCDEN     (1u << 19)  /* per-species phiv maps (Qnmcden) */
 #define BROMOC_QDEBYHYB    (1u << 20)  /* hybrid Debye (Qdebyhyb) */
@@ -205,6 +206,13 @@ typedef struct bromoc_system {
     /* PROXDIFF (grandmod): 1/beta, diff0, diffcutoff of DYNAMICS3 */
     double ibeta, diff0, diffcutoff;
     double thold27;              /* B-spline wall threshold (vdwgd1spln.f90:151) */
+    /* MEMBRANE (grandmod, membrane.f90:19-107): Nernst transmembrane potential and a
+     * membrane slab zmemb1..zmemb2 with switched walls, optionally pierced by a cylindrical
+     * pore (Qpore); the Debye factor is ikappa above.  Per atnam2 type: ampl1, ampl2,
+     * rcylinder [nttyp+1]; p1, p2 [2*(nttyp+1)] as (p, 1/p) pairs */
+    int32_t qpore, reserved4;
+    double voltage, afact, ceps, zmemb1, zmemb2;
+    const double *ampl1, *p1, *ampl2, *p2, *rcylinder;
 } bromoc_system;
 
 /* Energy terms as in grandmod/gsbpmod/nucleotmod (energy.mono.f90:46-68,689). */

@@ -74,6 +74,8 @@ typedef struct orc_ctx {
     /* APFOR / CONTRANS */
     int32_t *sn, *csn;
     double *af, *kx, *ky, *kz, *contrx, *contry, *contrz;
+    /* MEMBRANE per atnam2 type */
+    double *ampl1, *p1, *ampl2, *p2, *rcylinder;
 } orc_ctx;
 
 static void *dupmem(const void *p, size_t n) {
@@ -353,7 +355,8 @@ EXPORT void orc_destroy(orc_ctx *c) {
                   c->kb, c->lzmin, c->lzmax, c->bufferbias, c->diffusion, c->ampl3, c->p3,
                   c->xs, c->ys, c->bs, c->cs, c->ds, c->gsrfen, c->greff, c->srfe, c->reff,
                   c->srfed, c->reffd, c->dids, c->sn, c->csn, c->af, c->kx, c->ky, c->kz,
-                  c->contrx, c->contry, c->contrz};
+                  c->contrx, c->contry, c->contrz, c->ampl1, c->p1, c->ampl2, c->p2,
+                  c->rcylinder};
     for (size_t k = 0; k < sizeof(ps) / sizeof(ps[0]); k++) free(ps[k]);
     for (int k = 0; k < 8; k++) free(c->lists[k]);
     for (int k = 0; k < 6; k++) free(c->lvals[k]);
@@ -442,6 +445,14 @@ EXPORT orc_ctx *orc_create(const bromoc_system *s) {
     c->dids = (double *)calloc(5 * (size_t)cap + 5, sizeof(double));
     if ((s->flags & BROMOC_QPROXDIFF) && !c->diffusion)
         c->diffusion = (double *)dupmem(s->diffusion, sizeof(double) * (s->ntype + 1));
+    if (s->flags & BROMOC_QMEMB) {
+        size_t n1 = (size_t)s->nttyp + 1;
+        c->ampl1 = (double *)dupmem(s->ampl1, sizeof(double) * n1);
+        c->ampl2 = (double *)dupmem(s->ampl2, sizeof(double) * n1);
+        c->rcylinder = (double *)dupmem(s->rcylinder, sizeof(double) * n1);
+        c->p1 = (double *)dupmem(s->p1, sizeof(double) * 2 * n1);
+        c->p2 = (double *)dupmem(s->p2, sizeof(double) * 2 * n1);
+    }
     if (s->flags & BROMOC_QAPFOR) {
         c->sn = (int32_t *)dupmem(s->sn, sizeof(int32_t) * s->afn);
         c->af = (double *)dupmem(s->af, sizeof(double) * 3 * s->afn);
@@ -506,7 +517,7 @@ static uint32_t supported_mask(void) {
            BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV | BROMOC_QDEBY | BROMOC_QCOUNTION |
            BROMOC_QDNAFREE | BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR | BROMOC_QRFPSIN |
            BROMOC_QBSPLINE | BROMOC_QPROXDIFF | BROMOC_QNOTRANS | BROMOC_QAPFOR |
-           BROMOC_QCONTRANS;
+           BROMOC_QCONTRANS | BROMOC_QMEMB;
 }
 
 EXPORT int orc_check_flags(uint32_t flags) {
@@ -732,6 +743,161 @@ static void vdwgd1spln(orc_ctx *c, int qforces, double *evdwgd) {
             c->fz[i] = c->fz[i] + vdwfz;
         }
     }
+}
+
+/* SWITCH1, switch1.f90:19-47 */
+static void switch1(double *sw, double *dsw, double z, double p1, double ip1, double zmemb1,
+                    double zmemb2) {
+    if (z >= (zmemb2 + p1) || z <= (zmemb1 - p1)) {
+        *sw = 0.0; *dsw = 0.0;
+    } else if (z > zmemb2) {
+        double delz = z - zmemb2, delzp1 = delz * ip1;
+        *sw = 1.0 + (2.0 * delzp1 - 3.0) * (delzp1 * delzp1);
+        *dsw = -6.0 * (delzp1 - 1.0) * ip1 * delzp1;
+    } else if (z < zmemb1) {
+        double delz = zmemb1 - z, delzp1 = delz * ip1;
+        *sw = 1.0 + (2.0 * delzp1 - 3.0) * (delzp1 * delzp1);
+        *dsw = 6.0 * (delzp1 - 1.0) * ip1 * delzp1;
+    } else {
+        *sw = 1.0; *dsw = 0.0;
+    }
+}
+
+/* SWITCH2, switch2.f90:19-38 (r is set in the wall branch only) */
+static void switch2(double *sw, double *dsw, double r2, double radius, double p2, double ip2,
+                    double *r) {
+    if (r2 <= radius * radius) {
+        *sw = 0.0; *dsw = 0.0;
+    } else if (r2 < (radius + p2) * (radius + p2)) {
+        *r = sqrt(r2);
+        double delr = radius + p2 - *r, delrp2 = delr * ip2;
+        *sw = 1.0 + (2.0 * delrp2 - 3.0) * (delrp2 * delrp2);
+        *dsw = 6.0 * (delrp2 - 1.0) * delrp2 * ip2;
+    } else {
+        *sw = 1.0; *dsw = 0.0;
+    }
+}
+
+/* MEMBRANE, membrane.f90:19-107 */
+static void membrane(orc_ctx *c, int qforces, double *ememb) {
+    const bromoc_system *s = &c->s;
+    *ememb = 0.0;
+    for (int iat = 0; iat < s->ntot; iat++) {
+        int lgiat = (iat < s->nsites || iat >= s->nsites + s->nfix) && qforces;
+        int itype;
+        double charge;
+        if (iat < s->nsites) {
+            itype = c->typtyp[iat];
+            charge = 0.0;
+            if (c->namsite[iat] == BROMOC_SITE_P) charge = s->cgnuc;
+        } else {
+            int itype2 = abs(c->typei[iat]);
+            charge = c->cg[itype2];
+            itype = c->nwtype[itype2];
+        }
+        double x = c->x[iat], y = c->y[iat], z = c->z[iat];
+        if (s->voltage != 0.0 && charge != 0.0) {
+            if (z < s->zmemb1) {
+                *ememb = *ememb + charge * s->afact * exp(s->ikappa * (z - s->zmemb1));
+                if (lgiat) c->fz[iat] = c->fz[iat] - charge * s->afact * s->ikappa * exp(s->ikappa * (z - s->zmemb1));
+            } else if (z >= s->zmemb1 && z <= s->zmemb2) {
+                *ememb = *ememb + charge * s->afact * (s->ceps * s->ikappa * (z - s->zmemb1) + 1.0);
+                if (lgiat) c->fz[iat] = c->fz[iat] - charge * s->afact * s->ceps * s->ikappa;
+            } else if (z > s->zmemb2) {
+                *ememb = *ememb + charge * (s->voltage - s->afact * exp(-s->ikappa * (z - s->zmemb2)));
+                if (lgiat) c->fz[iat] = c->fz[iat] - charge * s->afact * s->ikappa * exp(-s->ikappa * (z - s->zmemb2));
+            }
+        }
+        if (c->ampl1[itype] > 0.0) {
+            double sw1, dsw1;
+            switch1(&sw1, &dsw1, z, c->p1[2 * itype], c->p1[2 * itype + 1], s->zmemb1, s->zmemb2);
+            if (sw1 != 0.0) {
+                if (s->qpore) {
+                    double r2 = x * x + y * y, sw2, dsw2, r = 0.0;
+                    switch2(&sw2, &dsw2, r2, c->rcylinder[itype], c->p2[2 * itype], c->p2[2 * itype + 1], &r);
+                    if (sw2 != 0.0) {
+                        if (sw1 == 1.0) {
+                            if (sw2 == 1.0) {
+                                c->warn[itype] = c->warn[itype] + 1;
+                                *ememb = *ememb + c->ampl2[itype];
+                            } else {
+                                if (lgiat) {
+                                    double ir = 1.0 / r;
+                                    c->fx[iat] = c->fx[iat] + c->ampl2[itype] * dsw2 * x * ir;
+                                    c->fy[iat] = c->fy[iat] + c->ampl2[itype] * dsw2 * y * ir;
+                                }
+                                *ememb = *ememb + c->ampl2[itype] * sw2;
+                            }
+                        } else {
+                            if (sw2 == 1.0) {
+                                if (lgiat) c->fz[iat] = c->fz[iat] + c->ampl1[itype] * dsw1;
+                                *ememb = *ememb + c->ampl1[itype] * sw1;
+                            } else {
+                                if (lgiat) {
+                                    double ir = 1.0 / r;
+                                    c->fx[iat] = c->fx[iat] + 0.5 * c->ampl2[itype] * dsw2 * x * ir;
+                                    c->fy[iat] = c->fy[iat] + 0.5 * c->ampl2[itype] * dsw2 * y * ir;
+                                    c->fz[iat] = c->fz[iat] + 0.5 * c->ampl1[itype] * dsw1;
+                                }
+                                *ememb = *ememb + 0.5 * (c->ampl2[itype] * sw2 + c->ampl1[itype] * sw1);
+                            }
+                        }
+                    }
+                } else {
+                    if (sw1 == 1.0) c->warn[itype] = c->warn[itype] + 1;
+                    *ememb = *ememb + c->ampl1[itype] * sw1;
+                }
+            }
+        }
+    }
+}
+
+/* the membrane part of INTERACT, interact.f90:58-119 */
+static double membrane_interact(orc_ctx *c, double xj, double yj, double zj, int j, int jtype,
+                                int jtype2, int beion, int alert) {
+    const bromoc_system *s = &c->s;
+    double ememb = 0.0, charge = 0.0;
+    if (s->voltage != 0.0) {
+        if (beion) charge = c->cg[jtype];
+        else if (s->flags & BROMOC_QNUCL) charge = c->namsite[j] == BROMOC_SITE_P ? s->cgnuc : 0.0;
+        if (charge != 0.0) {
+            if (zj < s->zmemb1) ememb = ememb + charge * s->afact * exp(s->ikappa * (zj - s->zmemb1));
+            else if (zj >= s->zmemb1 && zj <= s->zmemb2)
+                ememb = ememb + charge * s->afact * (s->ceps * s->ikappa * (zj - s->zmemb1) + 1.0);
+            else if (zj > s->zmemb2)
+                ememb = ememb + charge * (s->voltage - s->afact * exp(-s->ikappa * (zj - s->zmemb2)));
+        }
+    }
+    if (c->ampl1[jtype2] > 0.0) {
+        double sw1, dsw1;
+        switch1(&sw1, &dsw1, zj, c->p1[2 * jtype2], c->p1[2 * jtype2 + 1], s->zmemb1, s->zmemb2);
+        if (sw1 != 0.0) {
+            if (s->qpore) {
+                double r2 = xj * xj + yj * yj, sw2, dsw2, r = 0.0;
+                switch2(&sw2, &dsw2, r2, c->rcylinder[jtype2], c->p2[2 * jtype2], c->p2[2 * jtype2 + 1], &r);
+                if (sw2 != 0.0) {
+                    if (sw1 == 1.0) {
+                        if (sw2 == 1.0) {
+                            if (alert) c->warn[jtype2] = c->warn[jtype2] + 1;
+                            ememb = ememb + 1.0e10;
+                        } else {
+                            ememb = ememb + c->ampl2[jtype2] * sw2;
+                        }
+                    } else if (sw2 == 1.0) {
+                        ememb = ememb + c->ampl1[jtype2] * sw1;
+                    } else {
+                        ememb = ememb + 0.5 * (c->ampl2[jtype2] * sw2 + c->ampl1[jtype2] * sw1);
+                    }
+                }
+            } else if (sw1 == 1.0) {
+                if (alert) c->warn[jtype2] = c->warn[jtype2] + 1;
+                ememb = ememb + 1.0e10;
+            } else {
+                ememb = ememb + c->ampl1[jtype2] * sw1;
+            }
+        }
+    }
+    return ememb;
 }
 
 /* ------------------------------------------------------------ ENERGY */
@@ -984,6 +1150,7 @@ EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
     if (qforces) {
         for (int i = 0; i < nt; i++) c->fx[i] = c->fy[i] = c->fz[i] = 0.0;
     }
+    if (fl & BROMOC_QMEMB) membrane(c, qforces, &E.ememb);   /* energy.mono.f90:81-83 */
     E.ener = E.ememb;
     if (fl & BROMOC_QPHIX) {
         staticf1(c, qforces, &E.egsbpa);
@@ -1763,6 +1930,7 @@ EXPORT int orc_interact(orc_ctx *c, double xj, double yj, double zj, int32_t jty
     int ns = s->nsites, nt = s->ntot;
     int beion = (fl & BROMOC_QPAR) && j >= ns;
     int jtype2 = beion ? c->nwtype[jtype] : c->typtyp[j];
+    if (fl & BROMOC_QMEMB) dener = membrane_interact(c, xj, yj, zj, j, jtype, jtype2, beion, alert);
     /* interact.f90:144-154; the reference also calls it for a site j, reading srfe
      * outside its array (rfpar.f90:356); sites get no reaction-field term here */
     if ((fl & BROMOC_QRFPAR) && beion) dener = dener + rfparionj(c, xj, yj, zj, j, jtype);

@@ -57,6 +57,9 @@ type, bind(C) :: c_system
   type(c_ptr) :: sn, af, csn, kx, ky, kz, contrx, contry, contrz
   integer(c_int32_t) :: unsplit, nsites1st, notrans_axes, reserved3
   real(c_double) :: notrx, notry, notrz, ibeta, diff0, diffcutoff, thold27
+  integer(c_int32_t) :: qpore, reserved4
+  real(c_double) :: voltage, afact, ceps, zmemb1, zmemb2
+  type(c_ptr) :: ampl1, p1, ampl2, p2, rcylinder
 end type
 
 type, bind(C) :: c_gcmc
@@ -72,7 +75,7 @@ integer(c_int32_t), parameter :: F_PAR=1, F_NUCL=2, F_NONBOND=4, F_NOBOND=8, F_P
      F_SVDW=64, F_SOLV=128, F_DEBY=256, F_COUNTION=512, F_DNAFREE=1024, &
      F_RFPAR=33554432, F_DIFFUSE=268435456, F_PROFILE=536870912, F_RFPSIN=1073741824, &
      F_BSPLINE=262144, F_PROXDIFF=2097152, F_NOTRANS=4194304, F_APFOR=8388608, &
-     F_CONTRANS=16777216
+     F_CONTRANS=16777216, F_MEMB=65536
 integer :: site_names_code(0:5)
 
 contains
@@ -157,7 +160,7 @@ Qcountion = flag(s%flags, F_COUNTION)
 Qdnafree = flag(s%flags, F_DNAFREE)
 Qenergy = .true.
 Qforces = .false.
-Qmemb = .false.
+Qmemb = flag(s%flags, F_MEMB)
 Qmmij = .false.
 Qrfpar = .false.
 Qtrln = .not. flag(s%flags, F_BSPLINE)
@@ -290,6 +293,7 @@ if (Qproxdiff) then
 endif
 dids = 0.0
 call set_dna_options(s)
+if (Qmemb) call set_membrane(s)
 if (Qdiffuse) then
   plength2 = s%plength2; pcenter = s%pcenter
   call c_f_pointer(s%ampl3, dp, [ntype+1]); ampl3(1:ntype) = dp(2:ntype+1)
@@ -345,6 +349,24 @@ call c_f_pointer(s%ys, dp, [n]); ys(1:n) = dp
 call c_f_pointer(s%bs, dp, [n]); sp_b(1:n) = dp
 call c_f_pointer(s%cs, dp, [n]); sp_c(1:n) = dp
 call c_f_pointer(s%ds, dp, [n]); sp_d(1:n) = dp
+end subroutine
+
+! MEMBRANE state as shell_simul.f90:1563-1640 leaves it (per atnam2 type)
+subroutine set_membrane(s)
+use ref_harness
+use grandmod
+implicit none
+type(c_system), intent(in) :: s
+real(c_double), pointer :: dp(:)
+integer :: n
+n = s%nttyp
+Qpore = s%qpore /= 0
+voltage = s%voltage; afact = s%afact; ceps = s%ceps; zmemb1 = s%zmemb1; zmemb2 = s%zmemb2
+call c_f_pointer(s%ampl1, dp, [n+1]); ampl1(1:n) = dp(2:n+1)
+call c_f_pointer(s%ampl2, dp, [n+1]); ampl2(1:n) = dp(2:n+1)
+call c_f_pointer(s%rcylinder, dp, [n+1]); rcylinder(1:n) = dp(2:n+1)
+call c_f_pointer(s%p1, dp, [2*(n+1)]); p1(1:2, 1:n) = reshape(dp(3:2*n+2), [2, n])
+call c_f_pointer(s%p2, dp, [2*(n+1)]); p2(1:2, 1:n) = reshape(dp(3:2*n+2), [2, n])
 end subroutine
 
 ! APFOR (apfmod), CONTRANS and NOTRANS (nucleotmod) state as shell_simul.f90:650-770 leaves it

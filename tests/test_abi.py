@@ -63,7 +63,7 @@ def _plain():
 
 
 def test_create_rejects_unsupported_options():
-    for flag in (abi.QMEMB, abi.QMMIJ, abi.QNMCDEN, abi.QSPHERE, abi.QECYL, abi.QDEBYHYB):
+    for flag in (abi.QMMIJ, abi.QNMCDEN, abi.QSPHERE, abi.QECYL, abi.QDEBYHYB):
         s = _plain()
         s.flags |= flag
         rc, _ = _create(s)
@@ -101,6 +101,9 @@ def test_create_rejects_inconsistent_options():
     assert _create(s)[0] == abi.E_ARG
     s = _plain()
     s.flags |= abi.QBSPLINE                       # no repulsion map
+    assert _create(s)[0] == abi.E_ARG
+    s = _plain()
+    s.flags |= abi.QMEMB                          # no membrane parameters given
     assert _create(s)[0] == abi.E_ARG
     for flag in (abi.QPROXDIFF, abi.QNOTRANS, abi.QAPFOR, abi.QCONTRANS):
         s = _plain()                              # ions only: these need DNA sites

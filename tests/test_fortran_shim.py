@@ -134,7 +134,7 @@ def test_diffusion_and_rfpar_through_front_end(which):
             assert abs(a - b) <= 1e-14 * max(abs(b), 1.0)
 
 
-@pytest.mark.parametrize("which", ["bspline", "apfor_contrans", "proxdiff"])
+@pytest.mark.parametrize("which", ["bspline", "apfor_contrans", "proxdiff", "membrane"])
 def test_dna_options_through_front_end(which):
     """B-spline repulsion map (Qtrln off), APFOR + CONTRANS (apfmod, nucleotmod) and
     PROXDIFF (DYNAMICS3) handed over from the modules: energies, forces and BD steps."""
@@ -145,8 +145,10 @@ def test_dna_options_through_front_end(which):
         s = configs.with_contrans(configs.with_apfor(base, seed=5, n=4),
                                   [(0, (2.0, 1.0, 3.0)), (6, (1.0, 0.0, 2.0))], unsplit=True)
         s.x[:s.nsites] += 0.3
-    else:
+    elif which == "proxdiff":
         s = configs.with_proxdiff(base, beta=2.5, diff0=0.05)
+    else:
+        s = configs.with_membrane(base, thick=16.0, volts=0.15, pore=(11.0, 5.0, 2.5))
     seed, c0 = 91, 4
     f, r = _pair(s)
     f.L.bromoc_gpu_set_seed(seed, c0)

@@ -1,7 +1,7 @@
 """HIP path vs the C oracle for the B-spline repulsion map (vdwgd1spln.f90, vdwgd0spln.f90),
 APFOR and CONTRANS (energy.mono.f90:509-580), NOTRANS (simul1.f90:382-386), PROXDIFF /
-DYNAMICS3 (energy.mono.f90:603-620, dynamics3.mono.f90) and METROPOLIS (metropolis.f90,
-move.f90).
+DYNAMICS3 (energy.mono.f90:603-620, dynamics3.mono.f90), METROPOLIS (metropolis.f90,
+move.f90) and MEMBRANE (membrane.f90, interact.f90:58-119).
 
 Tolerances as the other parity tests: energies and norm-wise forces 1e-10 relative,
 positions after BD steps 1e-9 A, integer outputs (warnings, flux, MC decisions) exact.
@@ -193,3 +193,41 @@ def test_all_options_together():
     s = configs.with_proxdiff(s, beta=2.0, diff0=0.08)
     _steps_parity(s, nsteps=4)
     _steps_parity(s, nsteps=36, graphs=True, tol_pos=1e-8)
+
+
+def _membrane(seed, pore, nfix=4):
+    s = with_fixed(pore_small(seed=seed, n_ions=180), nfix)
+    s.flags &= ~(abi.QPHIV | abi.QPHIX)
+    s.phiv = s.phix = None
+    s = configs.with_membrane(s, thick=16.0, zmemb=1.0, volts=0.15, debye=9.0,
+                              pore=(11.0, 5.0, 2.5) if pore else None)
+    rng = np.random.default_rng(seed)
+    idx = np.arange(s.ntot - 12, s.ntot)
+    s.z[idx] = rng.uniform(-10.0, 12.0, 12)
+    r = rng.uniform(9.0, 15.0, 12)
+    th = rng.uniform(0, 2 * np.pi, 12)
+    s.x[idx], s.y[idx] = r * np.cos(th), r * np.sin(th)
+    return s
+
+
+@pytest.mark.parametrize("pore", [False, True])
+def test_membrane_energy_forces_steps(pore):
+    s = _membrane(73, pore)
+    g, o, eg, eo = _energy_parity(s)
+    assert eo["ememb"] != 0.0
+    _steps_parity(s, nsteps=5)
+    _steps_parity(s, nsteps=40, graphs=True, tol_pos=1e-8)
+
+
+def test_membrane_interact():
+    s = _membrane(74, True)
+    g, o = Engine(s), Oracle(s)
+    pos, typ = o.positions()
+    cases = [(pos[j], abs(typ[j]) if j >= s.nsites else 0, j)
+             for j in (0, 1, s.nsites + 2, s.ntot - 1, s.ntot - 4, s.ntot - 7)]
+    cases += [((0.5, 0.5, 3.0), s.nold + 1, s.ntot), ((12.0, 1.0, -8.5), s.nold + 2, s.ntot),
+              ((14.0, 0.0, 9.5), s.nold + 1, s.ntot), ((3.0, 2.0, 25.0), s.nold + 2, s.ntot)]
+    for xyz, jt, j in cases:
+        a, b = g.interact(xyz, jt, j, True), o.interact(xyz, jt, j, True)
+        assert abs(a - b) <= TOL * max(abs(b), 1.0), (j, a, b)
+    assert np.array_equal(g.warn(), o.warn())

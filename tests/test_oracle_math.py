@@ -523,3 +523,68 @@ def test_metropolis_energy_bookkeeping():
     # a uniform stream that runs out stops before an unfinished trial
     u3, d3, _ = o.metropolis(10, 1.5, ikbt, u[:9], 0.0)
     assert 1 <= d3 <= 2 and 4 * d3 <= u3 <= 9
+
+
+def _membrane_ions(pore, amplmemb=None):
+    s = plain(9, 60).copy()
+    s = configs.with_membrane(s, thick=16.0, zmemb=1.0, volts=0.2, debye=9.0, pore=pore,
+                              amplmemb=amplmemb)
+    s.flags &= ~abi.QNONBOND
+    rng = np.random.default_rng(4)
+    n = s.ntot
+    s.z[:] = rng.uniform(-12.0, 14.0, n)
+    r = rng.uniform(8.0, 16.0, n)
+    th = rng.uniform(0, 2 * np.pi, n)
+    s.x[:], s.y[:] = r * np.cos(th), r * np.sin(th)
+    return s
+
+
+def _membrane_cd(s, pore):
+    o = Oracle(s)
+    o.energy()
+    f = o.forces()
+    pos, typ = o.positions()
+    m = s.membrane
+    h = 1e-6
+    worst, checked = 0.0, 0
+    for j in range(s.ntot):
+        zj, rj = pos[j, 2], np.hypot(pos[j, 0], pos[j, 1])
+        edges = [m["zmemb1"], m["zmemb2"], m["zmemb1"] - 2.0, m["zmemb2"] + 2.0]
+        if min(abs(zj - e) for e in edges) < 1e-3:
+            continue
+        if pore and min(abs(rj - 11.0), abs(rj - 13.5)) < 1e-3:
+            continue
+        for k in range(3):
+            ev = []
+            for sg in (1.0, -1.0):
+                p = pos.copy()
+                p[j, k] += sg * h
+                o.set_positions(p, typ)
+                ev.append(o.energy(forces=False)["ememb"])
+            cd = -(ev[0] - ev[1]) / (2 * h)
+            worst = max(worst, abs(cd - f[j, k]) / max(1.0, abs(f[j, k])))
+            checked += 1
+    return worst, checked, f
+
+
+def test_membrane_force_is_minus_gradient():
+    """MEMBRANE's forces (membrane.f90:36-104) are -grad ememb away from the region edges,
+    by central differences over ions in the membrane walls, the pore wall and the bulk:
+    with the pore, and without it for the Nernst potential alone (its region 2 is linear,
+    its tails exponential)."""
+    worst, checked, _ = _membrane_cd(_membrane_ions((11.0, 5.0, 2.5)), True)
+    assert checked > 100 and worst <= 1e-6, worst
+    worst, checked, _ = _membrane_cd(_membrane_ions(None, amplmemb=0.0), False)
+    assert checked > 100 and worst <= 1e-6, worst
+
+
+def test_membrane_wall_without_pore_has_no_force():
+    """Without a pore the reference adds the wall energy ampl1*sw1 but no force for it
+    (membrane.f90:95-101): the forces equal those of the Nernst potential alone while the
+    energy grows by the wall term."""
+    s = _membrane_ions(None)
+    s0 = _membrane_ions(None, amplmemb=0.0)
+    o, o0 = Oracle(s), Oracle(s0)
+    e, e0 = o.energy()["ememb"], o0.energy()["ememb"]
+    assert np.array_equal(o.forces(), o0.forces())
+    assert e > e0 + 1.0

@@ -318,3 +318,40 @@ def test_metropolis():
     assert abs(ea - eb) <= 1e-12 * max(abs(eb), 1.0)
     assert np.array_equal(o.positions()[0], r.positions()[0])
     assert np.array_equal(o.warn(), r.warn())
+
+
+# ---------------------------------------------------------------- MEMBRANE
+def _membrane(seed, pore, nfix=4):
+    """Plain ions (and a DNA duplex) across a membrane slab; some ions inside the membrane
+    (the reference warns and, in INTERACT, charges 1e10 there) and in the walls."""
+    s = with_fixed(small_pore(seed=seed, n_ions=140), nfix)
+    s.flags &= ~(abi.QPHIV | abi.QPHIX)
+    s.phiv = s.phix = None
+    s = configs.with_membrane(s, thick=16.0, zmemb=1.0, volts=0.15, debye=9.0,
+                              pore=(11.0, 5.0, 2.5) if pore else None)
+    rng = np.random.default_rng(seed)
+    k = 10
+    idx = np.arange(s.ntot - k, s.ntot)
+    s.z[idx] = rng.uniform(-10.0, 12.0, k)           # in and beside the membrane
+    r = rng.uniform(9.0, 15.0, k)
+    th = rng.uniform(0, 2 * np.pi, k)
+    s.x[idx], s.y[idx] = r * np.cos(th), r * np.sin(th)
+    return s
+
+
+@pytest.mark.parametrize("pore", [False, True])
+def test_membrane_energy_steps_interact(pore):
+    s = _membrane(81, pore)
+    eo, er = _energy_match(s)
+    assert er["ememb"] != 0.0
+    _steps_match(s, nsteps=3)
+    o, r = Oracle(s), RefFortran(s)
+    pos, typ = o.positions()
+    cases = [(pos[j], abs(typ[j]) if j >= s.nsites else 0, j)
+             for j in (0, 1, s.nsites + 2, s.ntot - 1, s.ntot - 4, s.ntot - 7)]
+    cases += [((0.5, 0.5, 3.0), s.nold + 1, s.ntot), ((12.0, 1.0, -8.5), s.nold + 2, s.ntot),
+              ((14.0, 0.0, 9.5), s.nold + 1, s.ntot), ((3.0, 2.0, 25.0), s.nold + 2, s.ntot)]
+    for xyz, jt, j in cases:
+        a, b = o.interact(xyz, jt, j, True), r.interact(xyz, jt, j, True)
+        assert abs(a - b) <= 1e-14 * max(abs(b), 1.0), (j, a, b)
+    assert np.array_equal(o.warn(), r.warn())
