@@ -133,6 +133,19 @@ class MapHeader(C.Structure):
 MAP_PHIX, MAP_PHIV = 0, 1
 
 
+class Analysis(C.Structure):
+    """bromoc_analysis: the per-cycle statistics of simul1.f90 (rho, rDNA, g(r), prob2/prob3,
+    virial pressure)."""
+    _fields_ = [("what", C.c_int32), ("nzmax", C.c_int32), ("zmini", C.c_double),
+                ("idelz", C.c_double), ("igr", C.c_int32), ("npoints", C.c_int32),
+                ("dr", C.c_double), ("datom", C.c_int32), ("reserved", C.c_int32),
+                ("czmin", C.c_double), ("czmax", C.c_double), ("tvol", C.c_double),
+                ("temp", C.c_double), ("ikbt", C.c_double)]
+
+
+AN_RHO, AN_RDNA, AN_GR, AN_PROB, AN_PRES = 1, 2, 4, 8, 16
+
+
 class BromocError(RuntimeError):
     def __init__(self, where, code, detail=""):
         self.code = code
@@ -189,6 +202,10 @@ def _declare(lib):
                                P(C.c_float), P(C.c_float), C.c_int64], C.c_int),
         "bromoc_repwalls": ([C.c_int32, C.c_int32, C.c_int32, P(C.c_int8), C.c_int32, C.c_int],
                             C.c_int),
+        "bromoc_set_analysis": ([vp, P(Analysis)], C.c_int),
+        "bromoc_analyze": ([vp], C.c_int),
+        "bromoc_get_analysis": ([vp, P(C.c_int64), P(C.c_int64), P(C.c_int64), _ip,
+                                 P(C.c_int64), P(C.c_int64), _dp, P(C.c_int64)], C.c_int),
         "bromoc_metropolis": ([vp, C.c_int32, C.c_double, C.c_double, _dp, C.c_int64,
                                P(C.c_int64), _ip, _dp], C.c_int),
         "bromoc_build_table": ([C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp, P(Pair)],

@@ -100,11 +100,20 @@ struct bromoc_ctx {
     std::vector<double> avnum, mu, kb, lzmin, lzmax;
     std::vector<int8_t> bufferbias;
     StepParams base{};
+    // per-cycle analyses: integer histograms (freed on re-set), the per-particle virial
+    // sums of the running bromoc_step call, their fold, and the steps folded so far
+    bromoc_analysis an{};
+    unsigned long long *an_rho = nullptr, *an_rdna = nullptr, *an_gr = nullptr;
+    unsigned long long *an_prob2 = nullptr, *an_prob3 = nullptr;
+    int32_t *an_irmax = nullptr, *an_ncnt = nullptr;
+    double *vacc = nullptr, *vsum = nullptr;
+    int64_t pres_steps = 0, pres_ntot_steps = 0;
     // captured K-step graphs, one per starting buffer parity
     struct Graph {
         hipGraphExec_t exec = nullptr;
         int ntot = -1, tpb = -1, noise_mode = -1;
         const double *noise = nullptr;
+        const double *vacc = nullptr;
         int64_t stride = -1;
     } graph[2];
 };
@@ -395,6 +404,9 @@ int bromoc_destroy(bromoc_ctx *c) {
     for (auto &g : c->graph)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (void *p : c->allocs) (void)hipFree(p);
+    for (void *p : {(void *)c->an_rho, (void *)c->an_rdna, (void *)c->an_gr, (void *)c->an_prob2,
+                    (void *)c->an_prob3})
+        if (p) (void)hipFree(p);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
@@ -904,6 +916,8 @@ static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t 
     p.noise_mode = noise_mode;
     p.noise = philox ? c->pbuf : c->noise;
     p.noise_stride = philox ? 3 * (int64_t)c->cap : c->noise_stride;
+    // the virial pressure is a statistic of the step loop (simul1.f90:365-368)
+    p.vacc = (notrans && (c->an.what & BROMOC_AN_PRES)) ? c->vacc : nullptr;
     static const bool graphs_on = [] {
         const char *e = std::getenv("BROMOC_GRAPH");
         return !(e && e[0] == '0');
@@ -914,7 +928,7 @@ static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t 
         if (sst) return sst;
         bromoc_ctx::Graph &g = c->graph[c->cur];
         if (!g.exec || g.ntot != c->ntot || g.tpb != c->tpb || g.noise_mode != noise_mode ||
-            g.noise != p.noise || g.stride != p.noise_stride) {
+            g.noise != p.noise || g.stride != p.noise_stride || g.vacc != p.vacc) {
             if (g.exec) {
                 (void)hipGraphExecDestroy(g.exec);
                 g.exec = nullptr;
@@ -947,7 +961,7 @@ static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t 
             (void)hipGraphDestroy(graph);
             if (ie != hipSuccess) return fail(c, BROMOC_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
             g.ntot = c->ntot; g.tpb = c->tpb; g.noise_mode = noise_mode;
-            g.noise = p.noise; g.stride = p.noise_stride;
+            g.noise = p.noise; g.stride = p.noise_stride; g.vacc = p.vacc;
         }
         HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
         done += GRAPH_K;   // K is even: the buffer parity is unchanged
@@ -976,6 +990,11 @@ static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t 
         }
         done += n;
         if (done < nsteps) HIPCHK(c, launch_advance(c->ctl, n, c->stream));
+    }
+    if (p.vacc) {   // fold this call's virial sums before GRAND can renumber particles
+        HIPCHK(c, launch_vir_fold(c->vacc, c->ntot, c->vsum, c->stream));
+        c->pres_steps += nsteps;
+        c->pres_ntot_steps += (int64_t)nsteps * c->ntot;
     }
     return BROMOC_OK;
 }
@@ -1051,6 +1070,116 @@ int bromoc_get_warn(bromoc_ctx *c, int32_t *warn) {
     (void)hipSetDevice(c->device);
     HIPCHK(c, hipMemcpyAsync(warn, c->warn, (c->sys.nttyp + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     return sync_err(c);
+}
+
+// ---- per-cycle analyses (simul1.f90:365-368, 477-538)
+
+int bromoc_set_analysis(bromoc_ctx *c, const bromoc_analysis *an) {
+    if (!c || !an) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    const int w = an->what;
+    if (w & ~(BROMOC_AN_RHO | BROMOC_AN_RDNA | BROMOC_AN_GR | BROMOC_AN_PROB | BROMOC_AN_PRES))
+        return fail(c, BROMOC_E_ARG, "analysis: unknown bits in what");
+    if ((w & (BROMOC_AN_RHO | BROMOC_AN_RDNA)) &&
+        (an->nzmax < 1 || !(an->idelz > 0.0) || !std::isfinite(an->idelz) || !std::isfinite(an->zmini)))
+        return fail(c, BROMOC_E_ARG, "analysis: nzmax, idelz or zmini");
+    if ((w & BROMOC_AN_GR) && (an->npoints < 1 || !(an->dr > 0.0) || an->igr < 0 || an->igr >= c->ntot))
+        return fail(c, BROMOC_E_ARG, "analysis: npoints, dr or igr");
+    if ((w & BROMOC_AN_PROB) && (an->datom < 0 || an->czmin > an->czmax))
+        return fail(c, BROMOC_E_ARG, "analysis: datom or czmin > czmax");
+    if ((w & BROMOC_AN_PRES) && (!(an->tvol > 0.0) || !std::isfinite(an->ikbt)))
+        return fail(c, BROMOC_E_ARG, "analysis: tvol or ikbt");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned long long **bufs[5] = {&c->an_rho, &c->an_rdna, &c->an_gr, &c->an_prob2, &c->an_prob3};
+    for (auto b : bufs)
+        if (*b) {
+            (void)hipFree(*b);
+            *b = nullptr;
+        }
+    const size_t nt = (size_t)c->sys.ntype, n2 = (size_t)c->sys.nttyp;
+    size_t sz[5] = {0, 0, 0, 0, 0};
+    if (w & BROMOC_AN_RHO) sz[0] = nt * an->nzmax;
+    if (w & BROMOC_AN_RDNA) sz[1] = n2 * an->nzmax;
+    if (w & BROMOC_AN_GR) sz[2] = nt * an->npoints;
+    if (w & BROMOC_AN_PROB) sz[3] = sz[4] = nt * ((size_t)an->datom + 1);
+    for (int k = 0; k < 5; k++) {
+        if (!sz[k]) continue;
+        HIPCHK(c, hipMalloc((void **)bufs[k], sz[k] * sizeof(unsigned long long)));
+        HIPCHK(c, hipMemsetAsync(*bufs[k], 0, sz[k] * sizeof(unsigned long long), c->stream));
+    }
+    if (!c->an_irmax) HIPCHK(c, dalloc(c, &c->an_irmax, 1));
+    if (!c->an_ncnt) HIPCHK(c, dalloc(c, &c->an_ncnt, 2 * (nt + 1)));
+    if (!c->vacc) HIPCHK(c, dalloc(c, &c->vacc, c->cap));
+    if (!c->vsum) HIPCHK(c, dalloc(c, &c->vsum, 1));
+    HIPCHK(c, hipMemsetAsync(c->an_irmax, 0, sizeof(int32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->an_ncnt, 0, 2 * (nt + 1) * sizeof(int32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->vacc, 0, (size_t)c->cap * sizeof(double), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->vsum, 0, sizeof(double), c->stream));
+    c->an = *an;
+    c->pres_steps = c->pres_ntot_steps = 0;
+    return sync_err(c);
+}
+
+int bromoc_analyze(bromoc_ctx *c) {
+    if (!c) return BROMOC_E_ARG;
+    const int w = c->an.what & (BROMOC_AN_RHO | BROMOC_AN_RDNA | BROMOC_AN_GR | BROMOC_AN_PROB);
+    if (!w) return BROMOC_OK;
+    (void)hipSetDevice(c->device);
+    if ((w & BROMOC_AN_GR) && c->an.igr >= c->ntot)
+        return fail(c, BROMOC_E_STATE, "analysis: igr beyond the current particles");
+    AnParams a{};
+    a.pos = c->pos[c->cur];
+    a.typei = c->typei;
+    a.ntot = c->ntot; a.nsites = c->ns; a.nfix = c->nf; a.what = w;
+    a.nzmax = c->an.nzmax; a.igr = c->an.igr; a.npoints = c->an.npoints; a.datom = c->an.datom;
+    a.ntype = c->sys.ntype; a.nold = c->sys.nold; a.nttyp = c->sys.nttyp;
+    // prob2 counts a channel only with a membrane or a czmin..czmax range (simul1.f90:514-516)
+    if (c->sys.flags & BROMOC_QMEMB) a.chan = 1;
+    else a.chan = (c->an.czmin == 0.0 && c->an.czmax == 0.0) ? 0 : 2;
+    a.zmini = c->an.zmini; a.idelz = c->an.idelz; a.dr = c->an.dr;
+    a.czmin = c->an.czmin; a.czmax = c->an.czmax;
+    a.zmemb1 = c->base.zmemb1; a.zmemb2 = c->base.zmemb2;
+    a.rcylinder = c->base.rcylinder; a.nwtype = c->base.nwtype;
+    a.rho = c->an_rho; a.rdna = c->an_rdna; a.gr = c->an_gr;
+    a.prob2 = c->an_prob2; a.prob3 = c->an_prob3;
+    a.irmax = c->an_irmax; a.ncnt = c->an_ncnt;
+    HIPCHK(c, launch_analyze(a, c->stream));
+    return BROMOC_OK;
+}
+
+int bromoc_get_analysis(bromoc_ctx *c, int64_t *rho, int64_t *rdna, int64_t *gr,
+                        int32_t *irmax, int64_t *prob2, int64_t *prob3, double *pres,
+                        int64_t *pres_steps) {
+    if (!c) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    const size_t nt = (size_t)c->sys.ntype, n2 = (size_t)c->sys.nttyp;
+    struct { int64_t *dst; unsigned long long *src; size_t n; } cp[5] = {
+        {rho, c->an_rho, nt * c->an.nzmax},
+        {rdna, c->an_rdna, n2 * c->an.nzmax},
+        {gr, c->an_gr, nt * c->an.npoints},
+        {prob2, c->an_prob2, nt * ((size_t)c->an.datom + 1)},
+        {prob3, c->an_prob3, nt * ((size_t)c->an.datom + 1)}};
+    for (auto &q : cp) {
+        if (!q.dst) continue;
+        if (!q.src) return fail(c, BROMOC_E_STATE, "analysis: that statistic is not gathered");
+        HIPCHK(c, hipMemcpyAsync(q.dst, q.src, q.n * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    }
+    int32_t irm = 0;
+    double vs = 0.0;
+    if (irmax && c->an_irmax) HIPCHK(c, hipMemcpyAsync(&irm, c->an_irmax, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    if (pres && c->vsum) HIPCHK(c, hipMemcpyAsync(&vs, c->vsum, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    int st = sync_err(c);
+    if (st) return st;
+    if (irmax) *irmax = irm;
+    if (pres) {
+        // sum over the steps of (ntot - vir/3 * ikbt) * itvol * kba3bar * temp
+        // (simul1.f90:367, constamod.f90:36), the ntot terms and the virials summed first
+        const double i3 = 1.0 / 3.0, kba3bar = 1.380648813E-23 * 1e25;
+        const double itvol = 1.0 / c->an.tvol;
+        *pres = ((double)c->pres_ntot_steps - vs * i3 * c->an.ikbt) * itvol * kba3bar * c->an.temp;
+    }
+    if (pres_steps) *pres_steps = c->pres_steps;
+    return BROMOC_OK;
 }
 
 static int interact_async(bromoc_ctx *c, double xj, double yj, double zj, int32_t jtype, int32_t j,

@@ -137,6 +137,9 @@ struct StepParams {
     int32_t qpore, pad1;
     double voltage, afact, ceps, zmemb1, zmemb2;
     const double *ampl1, *p1, *ampl2, *p2, *rcylinder;
+    // virial pressure: F . x of every target added per step (the positions before the
+    // update), or null when the pressure is not gathered [cap]
+    double *vacc;
 };
 
 // spatial partner order (sort.hip)
@@ -178,6 +181,24 @@ struct InteractParams {
     double *out;                // [1]
 };
 hipError_t launch_interact(const InteractParams &p, hipStream_t s);
+
+// ---- per-cycle analyses (analysis.hip)
+struct AnParams {
+    const double4 *pos;        // current positions (atnam2 type in the low word of w)
+    const int32_t *typei;
+    int32_t ntot, nsites, nfix, what;
+    int32_t nzmax, igr, npoints, datom;
+    int32_t ntype, nold, nttyp, chan;  // chan: 0 no prob2, 1 membrane pore, 2 czmin..czmax
+    double zmini, idelz, dr, czmin, czmax, zmemb1, zmemb2;
+    const double *rcylinder;   // [nttyp+1] (membrane)
+    const int32_t *nwtype;     // [ntype+1]
+    unsigned long long *rho, *rdna, *gr, *prob2, *prob3;
+    int32_t *irmax;            // [1]
+    int32_t *ncnt;             // [2][ntype+1] channel and system counts of the cycle
+};
+hipError_t launch_analyze(const AnParams &a, hipStream_t s);   // k_hist, then k_prob
+// vsum[0] += sum of vacc[0..n) in a fixed order; vacc zeroed
+hipError_t launch_vir_fold(double *vacc, int n, double *vsum, hipStream_t s);
 
 // ---- trajectory adjoint (adjoint.hip)
 struct AdjParams {

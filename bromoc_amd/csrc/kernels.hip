@@ -337,6 +337,9 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
         P.fz[i] = F[2];
     }
     if constexpr ((FEAT & F_UPDATE) != 0) {
+        // virial of the step, F . x at the positions the forces were taken at
+        // (simul1.f90:365-367); fixed ions carry no force and add nothing
+        if (P.vacc) P.vacc[i] += F[0] * pa.x + F[1] * pa.y + F[2] * pa.z;
         const bool fr = !is_site || ((FEAT & F_DNA) && P.stfree[i] != 0);
         if (!fr) {
             P.poso[i] = pa;
@@ -705,6 +708,9 @@ __global__ __launch_bounds__(BLOCK) void k_wave(StepParams P) {
         P.fz[i] = F[2];
     }
     if constexpr ((FEAT & F_UPDATE) != 0) {
+        // virial of the step, F . x at the positions the forces were taken at
+        // (simul1.f90:365-367); fixed ions carry no force and add nothing
+        if (P.vacc) P.vacc[i] += F[0] * pa.x + F[1] * pa.y + F[2] * pa.z;
         if (!free_me) {
             P.poso[i] = pa;
             P.sposo[P.rank[i]] = pa;

@@ -171,6 +171,66 @@ class Engine:
         return used.value, done.value, e.value
 
 
+    # ------------------------------------------------------- per-cycle statistics
+    def set_analysis(self, **kw):
+        """The statistics of simul1.f90's cycle loop (analysis_struct builds the settings);
+        zeroes the accumulators."""
+        self._an = analysis_struct(self.system, **kw)
+        self._chk("bromoc_set_analysis", self.lib.bromoc_set_analysis(self.ctx, C.byref(self._an)))
+
+    def analyze(self):
+        """One cycle's rho, rDNA, g(r) and prob2/prob3 (simul1.f90:477-538)."""
+        self._chk("bromoc_analyze", self.lib.bromoc_analyze(self.ctx))
+
+    def get_analysis(self):
+        """dict of the accumulators: rho [ntype, nzmax], rdna [nttyp, nzmax], gr [ntype,
+        npoints], irmax, prob2 / prob3 [ntype, datom+1] (row t-1: type t), pres (the
+        reference's sum over the steps) and pres_steps; statistics not selected are None."""
+        a, s = self._an, self.system
+        out = _analysis_arrays(a, s)
+        irm, pres, nps = C.c_int32(), C.c_double(), C.c_int64()
+        i64 = lambda k: ptr(out[k], C.c_int64) if out[k] is not None else None
+        self._chk("bromoc_get_analysis",
+                  self.lib.bromoc_get_analysis(self.ctx, i64("rho"), i64("rdna"), i64("gr"),
+                                               C.byref(irm), i64("prob2"), i64("prob3"),
+                                               C.byref(pres), C.byref(nps)))
+        out.update(irmax=irm.value, pres=pres.value, pres_steps=nps.value)
+        return out
+
+
+def analysis_struct(system, what, nzmax=None, zmini=None, idelz=2.0, igr=0, npoints=10000,
+                    dr=0.1, datom=23000, czmin=0.0, czmax=0.0, tvol=None, temp=None,
+                    ikbt=None):
+    """bromoc_analysis with the reference's defaults (simul1.f90:179-230): z bins from lz2m
+    at idelz = 2 per Angstrom, nzmax = int(lz * idelz) capped at 10000, dr = 0.1, the box
+    volume, the temperature in system.meta (300 K when absent)."""
+    lo, hi = system.box[4], system.box[5]
+    if zmini is None:
+        zmini = lo
+    if nzmax is None:
+        nzmax = min(int((hi - lo) * idelz), 10000)
+    if tvol is None:
+        b = system.box
+        tvol = (b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4])
+    if temp is None:
+        temp = system.meta.get("temp", 300.0)
+    if ikbt is None:
+        from .system import kbt_of
+        ikbt = 1.0 / kbt_of(temp)
+    return abi.Analysis(what=what, nzmax=nzmax, zmini=zmini, idelz=idelz, igr=igr,
+                        npoints=npoints, dr=dr, datom=datom, reserved=0, czmin=czmin,
+                        czmax=czmax, tvol=tvol, temp=temp, ikbt=ikbt)
+
+
+def _analysis_arrays(a, s):
+    nt, n2 = s.ntype, s.nttyp
+    pick = lambda bit, shape: np.zeros(shape, np.int64) if a.what & bit else None
+    return dict(rho=pick(abi.AN_RHO, (nt, a.nzmax)), rdna=pick(abi.AN_RDNA, (n2, a.nzmax)),
+                gr=pick(abi.AN_GR, (nt, a.npoints)),
+                prob2=pick(abi.AN_PROB, (nt, a.datom + 1)),
+                prob3=pick(abi.AN_PROB, (nt, a.datom + 1)))
+
+
 def philox4x64(ctr, key):
     """Raw Philox4x64-10 blocks computed on the device."""
     lib = abi.load_library()

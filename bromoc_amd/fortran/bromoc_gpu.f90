@@ -28,6 +28,18 @@ type, bind(C) :: bgpu_grid
   type(c_ptr) :: data
 end type
 
+! bromoc_analysis: the cycle statistics of simul1.f90 (rho, rDNA, g(r), prob2/prob3 and the
+! virial pressure) gathered on the device; what = sum of 1 (rho), 2 (rDNA), 4 (g(r)),
+! 8 (prob), 16 (pressure)
+type, bind(C) :: bgpu_analysis
+  integer(c_int32_t) :: what, nzmax
+  real(c_double) :: zmini, idelz
+  integer(c_int32_t) :: igr, npoints
+  real(c_double) :: dr
+  integer(c_int32_t) :: datom, reserved
+  real(c_double) :: czmin, czmax, tvol, temp, ikbt
+end type
+
 type, bind(C) :: bgpu_system
   integer(c_int32_t) :: abi_version, ntot, nsites, nfix, capacity, ntype, nold, nttyp, npair
   integer(c_int32_t) :: flags
@@ -188,6 +200,26 @@ interface
     integer(c_int32_t), value :: nx, ny, nz, walls
     integer(c_int8_t), intent(inout) :: map(*)
     integer(c_int), value :: device
+  end function
+  integer(c_int) function bromoc_set_analysis(ctx, an) bind(C, name='bromoc_set_analysis')
+    import :: c_int, c_ptr, bgpu_analysis
+    type(c_ptr), value :: ctx
+    type(bgpu_analysis), intent(in) :: an
+  end function
+  integer(c_int) function bromoc_analyze(ctx) bind(C, name='bromoc_analyze')
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+  ! rho(ntype*nzmax), rdna(nttyp*nzmax), gr(ntype*npoints), prob2/prob3(ntype*(datom+1)):
+  ! row-major with the type as the row, i.e. a Fortran array (nzmax, ntype) etc.; pass
+  ! c_null_ptr for a statistic not gathered
+  integer(c_int) function bromoc_get_analysis(ctx, rho, rdna, gr, irmax, prob2, prob3, pres, &
+      nsteps) bind(C, name='bromoc_get_analysis')
+    import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+    type(c_ptr), value :: ctx, rho, rdna, gr, prob2, prob3
+    integer(c_int32_t), intent(out) :: irmax
+    real(c_double), intent(out) :: pres
+    integer(c_int64_t), intent(out) :: nsteps
   end function
   integer(c_int) function bromoc_destroy(ctx) bind(C, name='bromoc_destroy')
     import :: c_int, c_ptr

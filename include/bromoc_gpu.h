@@ -24,6 +24,9 @@
  * bromoc_read_map              SUBROUTINE READPHI           readphi.f90:19-349
  * bromoc_read_rfpar            SUBROUTINE READRFPAR         rfpar.f90:373-582
  * bromoc_repwalls              SUBROUTINE REPWALLS          repwalls.f90:19-67
+ * bromoc_set_analysis,         the statistics of the cycle loop: rho(z), rDNA(z), g(r),
+ * bromoc_analyze,              prob2/prob3 (simul1.f90:477-538) and the virial pressure
+ * bromoc_get_analysis          (simul1.f90:365-368)
  *
  * bromoc_step / bromoc_dynamics move the ions with DYNAMICS0, DYNAMICS1
  * (dynamics1.mono.f90:19-90, flag BROMOC_QDIFFUSE), DYNAMICS2 (dynamics2.mono.f90:19-89,
@@ -353,6 +356,59 @@ int bromoc_read_rfpar(int32_t nfiles, const char *const *paths, int32_t adjust,
  * first layer with no node. */
 int bromoc_repwalls(int32_t nclx, int32_t ncly, int32_t nclz, int8_t *phiv, int32_t walls,
                     int device);
+
+/* ------------------------------------------------------------- per-cycle analyses
+ * The statistics simul1.f90 gathers in its cycle loop, accumulated on the device so the
+ * host reads them once at the end instead of copying positions every cycle:
+ *   BROMOC_AN_RHO   rho(type, z) of the moving ions          simul1.f90:477-485
+ *   BROMOC_AN_RDNA  rDNA(site type, z) of the DNA sites      simul1.f90:487-495
+ *   BROMOC_AN_GR    g(r) of the moving ions around particle igr, and irmax
+ *                                                            simul1.f90:497-509
+ *   BROMOC_AN_PROB  prob2 (ions in the channel) and prob3 (ions in the system) number
+ *                   distributions per type                   simul1.f90:511-538
+ *   BROMOC_AN_PRES  virial pressure of every bromoc_step step simul1.f90:365-368
+ * bromoc_analyze adds one cycle's histograms from the current positions (the reference
+ * does this after the cycle's BD steps and COUNT); the pressure is added by every step of
+ * bromoc_step while BROMOC_AN_PRES is set (bromoc_dynamics, one DYNAMICS call, adds none).
+ * Bins follow the reference: iz = int((z - zmini) * idelz) + 1 kept when iz <= nzmax,
+ * ir = int(dist / dr) + 1 kept when ir <= npoints, a count n kept when n <= datom.  The
+ * reference writes out of its arrays for iz < 1 (a site or ion more than 1/idelz below
+ * zmini) or a count above datom; those samples are left out here. */
+#define BROMOC_AN_RHO   1
+#define BROMOC_AN_RDNA  2
+#define BROMOC_AN_GR    4
+#define BROMOC_AN_PROB  8
+#define BROMOC_AN_PRES  16
+
+typedef struct bromoc_analysis {
+    int32_t what;        /* BROMOC_AN_* */
+    int32_t nzmax;       /* z bins (simul1.f90:183-188: int(lz*idelz), at most 10000) */
+    double  zmini;       /* lower z of bin 1 (lz2m) */
+    double  idelz;       /* bins per Angstrom (2.0) */
+    int32_t igr;         /* 0-based index of the g(r) centre particle */
+    int32_t npoints;     /* g(r) shells (10000) */
+    double  dr;          /* shell width (0.1) */
+    int32_t datom;       /* largest count of prob2/prob3 (grandmod datom, 23000) */
+    int32_t reserved;
+    double  czmin, czmax;/* channel of prob2 without a membrane; both 0: no prob2 */
+    double  tvol;        /* system volume for the pressure (Angstrom^3) */
+    double  temp;        /* K */
+    double  ikbt;        /* 1/kBT in (kcal/mol)^-1, shell_simul.f90:865-866 */
+} bromoc_analysis;
+
+/* Set the analyses and zero their accumulators (also the pressure). */
+int bromoc_set_analysis(bromoc_ctx *ctx, const bromoc_analysis *an);
+/* One cycle's histograms (rho, rDNA, g(r), prob2/prob3 as selected) from the current
+ * positions; enqueued on the context's stream, no host synchronisation. */
+int bromoc_analyze(bromoc_ctx *ctx);
+/* The accumulators (any pointer may be NULL): rho[ntype*nzmax] (row t-1: atnam type t),
+ * rdna[nttyp*nzmax] (row t-1: atnam2 type t), gr[ntype*npoints], irmax,
+ * prob2[ntype*(datom+1)], prob3[ntype*(datom+1)] (row t-1, column n), the reference's
+ * pres sum over the steps and the number of steps it holds (the reference prints
+ * pres / steps, simul1.f90:656). */
+int bromoc_get_analysis(bromoc_ctx *ctx, int64_t *rho, int64_t *rdna, int64_t *gr,
+                        int32_t *irmax, int64_t *prob2, int64_t *prob3, double *pres,
+                        int64_t *pres_steps);
 
 /* EFPOT table from (r,U) samples at uniform spacing: writes ep[3*n] and pair fields
  * dmi, dm2lo, dm2hi, sc1..sc3 (host-side, splinepot.f90:19-96). */

@@ -76,6 +76,11 @@ typedef struct orc_ctx {
     double *af, *kx, *ky, *kz, *contrx, *contry, *contrz;
     /* MEMBRANE per atnam2 type */
     double *ampl1, *p1, *ampl2, *p2, *rcylinder;
+    /* per-cycle statistics of simul1.f90 (reals as the reference keeps them; prob integer) */
+    bromoc_analysis an;
+    double *rho, *rdna, *gr, pres;
+    int32_t irmax;
+    int64_t *prob2, *prob3, pres_steps;
 } orc_ctx;
 
 static void *dupmem(const void *p, size_t n) {
@@ -356,7 +361,7 @@ EXPORT void orc_destroy(orc_ctx *c) {
                   c->xs, c->ys, c->bs, c->cs, c->ds, c->gsrfen, c->greff, c->srfe, c->reff,
                   c->srfed, c->reffd, c->dids, c->sn, c->csn, c->af, c->kx, c->ky, c->kz,
                   c->contrx, c->contry, c->contrz, c->ampl1, c->p1, c->ampl2, c->p2,
-                  c->rcylinder};
+                  c->rcylinder, c->rho, c->rdna, c->gr, c->prob2, c->prob3};
     for (size_t k = 0; k < sizeof(ps) / sizeof(ps[0]); k++) free(ps[k]);
     for (int k = 0; k < 8; k++) free(c->lists[k]);
     for (int k = 0; k < 6; k++) free(c->lvals[k]);
@@ -1740,6 +1745,21 @@ static void notrans(orc_ctx *c) {
 
 /* simul1.f90:363-390 with Qforces on: ENERGY then DYNAMICS0/1/2/3 then DYNAMICS0NUC and
  * NOTRANS */
+/* the virial pressure of one step, simul1.f90:365-368 (i3 = 1/3, itvol = 1/tvol,
+ * kba3bar = kboltz*1e25, constamod.f90:24,36) */
+static void pressure(orc_ctx *c) {
+    const int n = c->s.ntot;
+    const double i3 = 1.0 / 3.0, kba3bar = 1.380648813E-23 * 1e25;
+    const double itvol = 1.0 / c->an.tvol;
+    double dx = 0.0, dy = 0.0, dz = 0.0;   /* dot_product(fx(1:ntot), x(1:ntot)) etc. */
+    for (int i = 0; i < n; i++) dx += c->fx[i] * c->x[i];
+    for (int i = 0; i < n; i++) dy += c->fy[i] * c->y[i];
+    for (int i = 0; i < n; i++) dz += c->fz[i] * c->z[i];
+    const double vir = dx + dy + dz;
+    c->pres = c->pres + (n - vir * i3 * c->an.ikbt) * itvol * kba3bar * c->an.temp;
+    c->pres_steps++;
+}
+
 EXPORT int orc_step(orc_ctx *c, int32_t nsteps, int32_t mode, int32_t first_row, uint64_t seed,
                     uint64_t counter0, bromoc_energies *energies) {
     double *g = (double *)malloc(sizeof(double) * 3 * (c->s.ntot + 1));
@@ -1747,6 +1767,7 @@ EXPORT int orc_step(orc_ctx *c, int32_t nsteps, int32_t mode, int32_t first_row,
     for (int s = 0; s < nsteps && st == BROMOC_OK; s++) {
         st = orc_energy(c, 1, energies);
         if (st) break;
+        if (c->an.what & BROMOC_AN_PRES) pressure(c);
         st = fill_noise(c, mode, first_row + s, seed, counter0 + (uint64_t)s, g);
         if (st) break;
         dynamics_pass(c, g);
@@ -2289,5 +2310,122 @@ EXPORT int orc_repwalls(int32_t nclx2, int32_t ncly2, int32_t nclz2, int8_t *phi
         if (ck == 0) break;
     }
     free(phivt);
+    return BROMOC_OK;
+}
+
+/* ------------------------------------------------------ per-cycle statistics */
+EXPORT int orc_set_analysis(orc_ctx *c, const bromoc_analysis *an) {
+    free(c->rho); free(c->rdna); free(c->gr); free(c->prob2); free(c->prob3);
+    c->an = *an;
+    const size_t nt = (size_t)c->s.ntype, n2 = (size_t)c->s.nttyp;
+    c->rho = (double *)calloc(nt * (an->nzmax > 0 ? an->nzmax : 1), sizeof(double));
+    c->rdna = (double *)calloc(n2 * (an->nzmax > 0 ? an->nzmax : 1), sizeof(double));
+    c->gr = (double *)calloc(nt * (an->npoints > 0 ? an->npoints : 1), sizeof(double));
+    c->prob2 = (int64_t *)calloc(nt * ((size_t)(an->datom > 0 ? an->datom : 0) + 1), sizeof(int64_t));
+    c->prob3 = (int64_t *)calloc(nt * ((size_t)(an->datom > 0 ? an->datom : 0) + 1), sizeof(int64_t));
+    c->irmax = 0;
+    c->pres = 0.0;
+    c->pres_steps = 0;
+    return BROMOC_OK;
+}
+
+/* int() of a real, with the samples whose int() overflows (the reference's undefined case)
+ * reported as out of range */
+static int int_of(double v, int *out) {
+    if (!(v > -2147483648.0 && v < 2147483647.0)) return 0;
+    *out = (int)v;
+    return 1;
+}
+
+/* one cycle of simul1.f90:477-538 (Qrho, Qrdna, Qgr, Qprob), in the reference's loop order.
+ * Samples with iz < 1 would be written below the reference's arrays: left out. */
+EXPORT int orc_analyze(orc_ctx *c) {
+    const bromoc_system *s = &c->s;
+    const bromoc_analysis *a = &c->an;
+    const int ion0 = s->nsites + s->nfix, nz = a->nzmax;
+    int iz, ir;
+    if (a->what & BROMOC_AN_RHO) {
+        for (int iat = ion0; iat < s->ntot; iat++) {
+            int itype = abs(c->typei[iat]);
+            if (!int_of((c->z[iat] - a->zmini) * a->idelz, &iz)) continue;
+            iz = iz + 1;
+            if (iz <= nz && iz >= 1) c->rho[(size_t)(itype - 1) * nz + iz - 1] += 1.0;
+        }
+    }
+    if (a->what & BROMOC_AN_RDNA) {
+        for (int iat = 0; iat < s->nsites; iat++) {
+            int itype = c->typtyp[iat];
+            if (!int_of((c->z[iat] - a->zmini) * a->idelz, &iz)) continue;
+            iz = iz + 1;
+            if (iz <= nz && iz >= 1) c->rdna[(size_t)(itype - 1) * nz + iz - 1] += 1.0;
+        }
+    }
+    if (a->what & BROMOC_AN_GR) {
+        const int igr = a->igr;
+        for (int iat = ion0; iat < s->ntot; iat++) {
+            int itype = abs(c->typei[iat]);
+            double dist = (c->x[iat] - c->x[igr]) * (c->x[iat] - c->x[igr]) +
+                          (c->y[iat] - c->y[igr]) * (c->y[iat] - c->y[igr]) +
+                          (c->z[iat] - c->z[igr]) * (c->z[iat] - c->z[igr]);
+            dist = sqrt(dist);
+            if (!int_of(dist / a->dr, &ir)) ir = 2147483646;
+            ir = ir + 1;
+            if (ir > c->irmax) c->irmax = ir;
+            if (c->irmax > a->npoints) c->irmax = a->npoints;
+            if (ir <= a->npoints) c->gr[(size_t)(itype - 1) * a->npoints + ir - 1] += 1.0;
+        }
+    }
+    if (a->what & BROMOC_AN_PROB) {
+        const int nt = s->ntype - s->nold;
+        int *ncount = (int *)calloc((size_t)nt + 1, sizeof(int));
+        int qmemb = (s->flags & BROMOC_QMEMB) != 0;
+        int ok = !qmemb && a->czmin == 0.0 && a->czmax == 0.0;
+        if (!ok) {
+            for (int iat = ion0; iat < s->ntot; iat++) {
+                int itype = abs(c->typei[iat]) - s->nold;
+                if (qmemb) {
+                    double r = sqrt(c->x[iat] * c->x[iat] + c->y[iat] * c->y[iat]);
+                    ok = c->z[iat] >= s->zmemb1 && c->z[iat] <= s->zmemb2;
+                    if (ok) {
+                        double rc = c->rcylinder[c->nwtype[itype + s->nold]];
+                        ok = rc > 0.0 && r <= rc;
+                    }
+                } else {
+                    ok = c->z[iat] >= a->czmin && c->z[iat] <= a->czmax;
+                }
+                if (ok) ncount[itype - 1]++;
+            }
+            for (int itype = s->nold + 1; itype <= s->ntype; itype++) {
+                int n = ncount[itype - s->nold - 1];
+                if (n <= a->datom) c->prob2[(size_t)(itype - 1) * (a->datom + 1) + n] += 1;
+            }
+        }
+        memset(ncount, 0, sizeof(int) * ((size_t)nt + 1));
+        for (int iat = ion0; iat < s->ntot; iat++) {
+            int itype = abs(c->typei[iat]) - s->nold;
+            ncount[itype - 1]++;
+        }
+        for (int itype = s->nold + 1; itype <= s->ntype; itype++) {
+            int n = ncount[itype - s->nold - 1];
+            if (n <= a->datom) c->prob3[(size_t)(itype - 1) * (a->datom + 1) + n] += 1;
+        }
+        free(ncount);
+    }
+    return BROMOC_OK;
+}
+
+/* the statistics as accumulated (rho, rdna, gr as the reference's reals) */
+EXPORT int orc_get_analysis(orc_ctx *c, double *rho, double *rdna, double *gr, int32_t *irmax,
+                            int64_t *prob2, int64_t *prob3, double *pres, int64_t *pres_steps) {
+    const size_t nt = (size_t)c->s.ntype, n2 = (size_t)c->s.nttyp;
+    const bromoc_analysis *a = &c->an;
+    if (rho) memcpy(rho, c->rho, sizeof(double) * nt * a->nzmax);
+    if (rdna) memcpy(rdna, c->rdna, sizeof(double) * n2 * a->nzmax);
+    if (gr) memcpy(gr, c->gr, sizeof(double) * nt * a->npoints);
+    if (prob2) memcpy(prob2, c->prob2, sizeof(int64_t) * nt * ((size_t)a->datom + 1));
+    if (prob3) memcpy(prob3, c->prob3, sizeof(int64_t) * nt * ((size_t)a->datom + 1));
+    if (irmax) *irmax = c->irmax;
+    if (pres) *pres = c->pres;
+    if (pres_steps) *pres_steps = c->pres_steps;
     return BROMOC_OK;
 }

@@ -76,6 +76,10 @@ def oracle_lib():
             ("orc_metropolis", [vp, C.c_int32, C.c_double, C.c_double, _dp, C.c_int64,
                                 P(C.c_int64), _ip, _dp], C.c_int),
             ("orc_philox_gauss3", [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64, _dp], None),
+            ("orc_set_analysis", [vp, P(abi.Analysis)], C.c_int),
+            ("orc_analyze", [vp], C.c_int),
+            ("orc_get_analysis", [vp, _dp, _dp, _dp, _ip, P(C.c_int64), P(C.c_int64), _dp,
+                                  P(C.c_int64)], C.c_int),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -224,6 +228,31 @@ class Oracle(_Base):
         if rc:
             raise abi.BromocError("orc_metropolis", rc)
         return used.value, done.value, e.value
+
+
+    def set_analysis(self, **kw):
+        from bromoc_amd.engine import analysis_struct
+        self._an = analysis_struct(self.system, **kw)
+        assert self.L.orc_set_analysis(self.ctx, C.byref(self._an)) == 0
+
+    def analyze(self):
+        assert self.L.orc_analyze(self.ctx) == 0
+
+    def get_analysis(self):
+        """As Engine.get_analysis; rho, rdna and gr come back as the reference's reals
+        (float64 arrays of whole numbers)."""
+        from bromoc_amd.engine import _analysis_arrays
+        out = _analysis_arrays(self._an, self.system)
+        for k in ("rho", "rdna", "gr"):
+            if out[k] is not None:
+                out[k] = out[k].astype(float)
+        irm, pres, nps = C.c_int32(), C.c_double(), C.c_int64()
+        f64 = lambda k: _p(out[k]) if out[k] is not None else None
+        i64 = lambda k: _p(out[k], C.c_int64) if out[k] is not None else None
+        self.L.orc_get_analysis(self.ctx, f64("rho"), f64("rdna"), f64("gr"), C.byref(irm),
+                                i64("prob2"), i64("prob3"), C.byref(pres), C.byref(nps))
+        out.update(irmax=irm.value, pres=pres.value, pres_steps=nps.value)
+        return out
 
 
 _rlibs = {}
