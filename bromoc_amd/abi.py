@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 OK, E_ARG, E_UNSUPPORTED, E_CAPACITY, E_HIP, E_NODEVICE, E_STATE = range(7)
 STATUS_NAMES = {0: "OK", 1: "E_ARG", 2: "E_UNSUPPORTED", 3: "E_CAPACITY", 4: "E_HIP",
@@ -94,6 +94,11 @@ class SystemStruct(C.Structure):
         ("voltage", C.c_double), ("afact", C.c_double), ("ceps", C.c_double),
         ("zmemb1", C.c_double), ("zmemb2", C.c_double),
         ("ampl1", _dp), ("p1", _dp), ("ampl2", _dp), ("p2", _dp), ("rcylinder", _dp),
+        ("mmij_shape", C.c_int32), ("xnpol", C.c_int32), ("ynpol", C.c_int32),
+        ("znpol", C.c_int32), ("ntpol", C.c_int32), ("reserved5", C.c_int32),
+        ("lstpx", _ip), ("lstpy", _ip), ("lstpz", _ip), ("bnorm", _dp), ("mmij", _dp),
+        ("xscal", C.c_double), ("yscal", C.c_double), ("zscal", C.c_double),
+        ("rfscal", C.c_double),
     ]
 
 

@@ -729,6 +729,34 @@ def with_membrane(s: System, thick=20.0, zmemb=0.0, epsm=2.0, volts=0.1, debye=9
     return s
 
 
+def with_mmij(s: System, npol=(4, 4, 5), region=None, rfscal=1.0, seed=0, strength=1e-4):
+    """MMIJ reaction field in a rectangular region (shell_simul.f90:2780-2831, READMMIJ,
+    RECT_NORM): a region of the given widths centred at 0 (default: 80 % of the box), the
+    full tensor basis of xnpol*ynpol*znpol Legendre products (degrees in the basis order a
+    GSBP matrix file lists them), normalisation sqrt((2l+1)(2m+1)(2n+1) xscal yscal zscal / 8)
+    (rect_norm.f90:28-33) and a seeded symmetric negative-definite reaction-field matrix
+    (a dielectric's response lowers the energy) standing in for the file's MMIJ."""
+    s = s.copy()
+    s.flags |= abi.QMMIJ
+    if region is None:
+        b = s.box
+        region = tuple(0.8 * (b[2 * k + 1] - b[2 * k]) for k in range(3))
+    scal = tuple(2.0 / w for w in region)
+    nx, ny, nz = npol
+    lst = np.array([(a, b_, c) for a in range(nx) for b_ in range(ny) for c in range(nz)],
+                   np.int32)
+    ilxyz = scal[0] * scal[1] * scal[2] * 0.125
+    bnorm = np.sqrt((2.0 * lst[:, 0] + 1.0) * (2.0 * lst[:, 1] + 1.0) * (2.0 * lst[:, 2] + 1.0)
+                    * ilxyz)
+    n = len(lst)
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n)) / np.sqrt(n)
+    m = -(a @ a.T + 0.1 * np.eye(n)) * strength / ilxyz
+    m = 0.5 * (m + m.T)
+    s.mmij = dict(shape=1, npol=npol, lstp=lst, bnorm=bnorm, mmij=m, scal=scal, rfscal=rfscal)
+    return s
+
+
 def with_rfpar(s: System, seed=0, shape=None, dcel=None, single=False, reffac=1.5,
                wall_radius=None, zero_frac=0.0, const_reff=None):
     """Synthetic reaction-field parameter grids (RFPAR command, rfpar.f90:373-582): per

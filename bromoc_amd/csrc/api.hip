@@ -29,7 +29,7 @@ constexpr uint32_t SUPPORTED = BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BR
                                BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR |
                                BROMOC_QRFPSIN | BROMOC_QBSPLINE | BROMOC_QPROXDIFF |
                                BROMOC_QNOTRANS | BROMOC_QAPFOR | BROMOC_QCONTRANS |
-                               BROMOC_QMEMB;
+                               BROMOC_QMEMB | BROMOC_QMMIJ;
 
 // constamod.f90:21-32 (binary64)
 double celec_value() {
@@ -149,12 +149,13 @@ hipError_t dupload(bromoc_ctx *c, T **p, const T *src, size_t n) {
     return hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice);
 }
 
-void energies_from(const double *r, bool rfpar, bromoc_energies *out) {
+void energies_from(const double *r, bool rfpar, bool mmij, bromoc_energies *out) {
     bromoc_energies E;
     double *f = &E.eelec;
     for (int k = 0; k < NE; k++) f[k] = r[k];
-    // energy.mono.f90:83-115,689 (ememb stays zero on this path; egsbpb is erfpar)
+    // energy.mono.f90:83-115,689 (egsbpb is the MMIJ reaction field or erfpar)
     E.ener = E.ememb;
+    if (mmij) E.ener = E.ener + E.egsbpb;
     E.ener = E.ener + E.egsbpa;
     E.ener = E.ener + E.evdwgd;
     if (rfpar) {
@@ -317,6 +318,25 @@ int validate(const bromoc_system *s, std::string &msg) {
     if ((s->flags & BROMOC_QMEMB) &&
         (!s->ampl1 || !s->p1 || !s->ampl2 || !s->p2 || !s->rcylinder || !(s->zmemb2 >= s->zmemb1)))
         return bad("MEMBRANE parameters");
+    if (s->flags & BROMOC_QMMIJ) {
+        if (s->mmij_shape == 2) {
+            msg = "the SPHERE reaction field (sphe_rf1.f90) is not supported";
+            return BROMOC_E_UNSUPPORTED;
+        }
+        if (s->mmij_shape != 1) return bad("MMIJ shape");
+        if (s->flags & BROMOC_QRFPAR) return bad("MMIJ and RFPAR together (shell_simul.f90:2869)");
+        // rect_rf1.f90:48-57 fills three polynomials per axis whatever npol is
+        if (s->xnpol < 3 || s->ynpol < 3 || s->znpol < 3 || s->xnpol > 64 || s->ynpol > 64 ||
+            s->znpol > 64 || s->ntpol < 1 || s->ntpol > 32768)
+            return bad("MMIJ polynomial counts");
+        if (!s->lstpx || !s->lstpy || !s->lstpz || !s->bnorm || !s->mmij) return bad("MMIJ arrays missing");
+        if (!(s->xscal > 0.0) || !(s->yscal > 0.0) || !(s->zscal > 0.0) || !std::isfinite(s->rfscal))
+            return bad("MMIJ region or rfscal");
+        for (int k = 0; k < s->ntpol; k++)
+            if (s->lstpx[k] < 0 || s->lstpx[k] >= s->xnpol || s->lstpy[k] < 0 ||
+                s->lstpy[k] >= s->ynpol || s->lstpz[k] < 0 || s->lstpz[k] >= s->znpol)
+                return bad("MMIJ degree out of range");
+    }
     return BROMOC_OK;
 }
 
@@ -378,7 +398,7 @@ int reduce_energies(bromoc_ctx *c, bromoc_energies *out) {
     HIPCHK(c, hipMemcpyAsync(r, c->ered, sizeof(r), hipMemcpyDeviceToHost, c->stream));
     int st = sync_err(c);
     if (st) return st;
-    energies_from(r, (c->sys.flags & BROMOC_QRFPAR) != 0, out);
+    energies_from(r, (c->sys.flags & BROMOC_QRFPAR) != 0, (c->sys.flags & BROMOC_QMMIJ) != 0, out);
     return BROMOC_OK;
 }
 
@@ -596,7 +616,26 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     ACHK(dalloc(c, &d_rfa, rfon ? cap : 0));
     ACHK(dalloc(c, &d_rfq, rfon ? cap : 0));
     ACHK(dalloc(c, &d_rfd, rfon ? 2 * (size_t)cap : 0));
-    ACHK(dalloc(c, &d_frf, rfon ? cap : 0));
+    const bool mmon = (s->flags & BROMOC_QMMIJ) != 0;
+    ACHK(dalloc(c, &d_frf, (rfon || mmon) ? cap : 0));
+    // MMIJ reaction field: degrees packed per basis function, norms, matrix, scratch
+    int32_t *d_lstp = nullptr;
+    double *d_bnorm = nullptr, *d_mmij = nullptr, *d_mleg = nullptr, *d_mchg = nullptr;
+    double *d_mcoef = nullptr, *d_mq = nullptr, *d_mer = nullptr;
+    const int ntpol = mmon ? s->ntpol : 0;
+    const int npl = mmon ? s->xnpol + s->ynpol + s->znpol : 0;
+    {
+        std::vector<int32_t> lp(std::max(ntpol, 1), 0);
+        for (int k = 0; k < ntpol; k++) lp[k] = s->lstpx[k] | (s->lstpy[k] << 8) | (s->lstpz[k] << 16);
+        ACHK(dupload(c, &d_lstp, mmon ? lp.data() : (const int32_t *)nullptr, ntpol));
+        ACHK(dupload(c, &d_bnorm, mmon ? s->bnorm : (const double *)nullptr, ntpol));
+        ACHK(dupload(c, &d_mmij, mmon ? s->mmij : (const double *)nullptr, (size_t)ntpol * ntpol));
+        ACHK(dalloc(c, &d_mleg, mmon ? ((size_t)cap + 1) * npl : 0));
+        ACHK(dalloc(c, &d_mchg, mmon ? (size_t)cap + 1 : 0));
+        ACHK(dalloc(c, &d_mcoef, ntpol));
+        ACHK(dalloc(c, &d_mq, ntpol));
+        ACHK(dalloc(c, &d_mer, ntpol));
+    }
     // DYNAMICS3 needs the diffusion constants even without DYNAMICS1/2
     if (!dyn12 && (s->flags & BROMOC_QPROXDIFF)) ACHK(dupload(c, &d_diff, s->diffusion, s->ntype + 1));
     // APFOR: the applied force of each site summed over its entries in entry order
@@ -689,6 +728,12 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.rfon = rfon ? 1 : 0;
     p.rfsingle = (s->flags & BROMOC_QRFPSIN) ? 1 : 0;
     p.rfa = d_rfa; p.rfd = d_rfd; p.frf = d_frf; p.rfq = d_rfq;
+    p.mmon = mmon ? 1 : 0;
+    p.ntpol = ntpol; p.npl = npl;
+    p.xnpol = mmon ? s->xnpol : 0; p.ynpol = mmon ? s->ynpol : 0; p.znpol = mmon ? s->znpol : 0;
+    p.lstp = d_lstp; p.bnorm = d_bnorm; p.mmij = d_mmij;
+    p.xscal = s->xscal; p.yscal = s->yscal; p.zscal = s->zscal; p.rfscal = s->rfscal;
+    p.mleg = d_mleg; p.mchg = d_mchg; p.mcoef = d_mcoef; p.mq = d_mq; p.mer = d_mer;
     p.all_efpot = 1;
     c->r2near = 0.0;
     for (int k = 0; k < nt1 * nt1; k++) {
@@ -809,6 +854,7 @@ static hipError_t enqueue_step(bromoc_ctx *c, const StepParams &p, hipEvent_t t0
     const int ns = p.nsites, nt = p.ntargets;
     hipError_t e;
     if ((e = launch_rfpar(p, true, c->stream)) != hipSuccess) return e;
+    if ((e = launch_mmij(p, c->stream)) != hipSuccess) return e;
     if ((e = launch_fixed(p, c->stream)) != hipSuccess) return e;
     static const bool fork = [] {
         const char *v = std::getenv("BROMOC_FORK");
@@ -1203,6 +1249,7 @@ static int interact_async(bromoc_ctx *c, double xj, double yj, double zj, int32_
     q.out = c->iout + slot;
     HIPCHK(c, launch_rfpar(q.sp, false, c->stream));
     HIPCHK(c, launch_interact(q, c->stream));
+    HIPCHK(c, launch_mmij_interact(q.sp, xj, yj, zj, jtype, j, q.out, c->stream));
     return BROMOC_OK;
 }
 

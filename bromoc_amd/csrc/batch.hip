@@ -750,9 +750,12 @@ int bromoc_batch_create(const bromoc_system *s, int32_t nrep, const bromoc_repli
                         int device, bromoc_batch **out) {
     if (!s || !reps || !out || nrep < 1 || nrep > (1 << 16)) return BROMOC_E_ARG;
     *out = nullptr;
-    const uint32_t unsupported = BROMOC_QNUCL | BROMOC_QRFPAR | BROMOC_QRFPSIN | BROMOC_QDNAFREE;
+    // the replica kernels implement pairs, the trilinear grids and DYNAMICS0/1/2 only
+    const uint32_t unsupported = BROMOC_QNUCL | BROMOC_QRFPAR | BROMOC_QRFPSIN | BROMOC_QDNAFREE |
+                                 BROMOC_QMEMB | BROMOC_QBSPLINE | BROMOC_QMMIJ;
     if (s->flags & unsupported) {
-        fprintf(stderr, "bromoc_batch_create: replicas are ion-only (no CG-DNA, no RFPAR)\n");
+        fprintf(stderr, "bromoc_batch_create: replicas are ion-only (no CG-DNA, RFPAR, MMIJ, "
+                        "membrane or B-spline map)\n");
         return BROMOC_E_UNSUPPORTED;
     }
     if (s->nsites != 0 || !(s->flags & BROMOC_QPAR)) return BROMOC_E_ARG;

@@ -140,6 +140,15 @@ struct StepParams {
     // virial pressure: F . x of every target added per step (the positions before the
     // update), or null when the pressure is not gathered [cap]
     double *vacc;
+    // MMIJ reaction field, RECTBOX (rect_rf1.f90): its force on each moving ion goes to frf
+    int32_t mmon, ntpol, xnpol, ynpol, znpol, npl;   // npl = xnpol + ynpol + znpol
+    const int32_t *lstp;     // [ntpol] degrees, x | y << 8 | z << 16
+    const double *bnorm;     // [ntpol]
+    const double *mmij;      // [ntpol * ntpol]
+    double xscal, yscal, zscal, rfscal;
+    double *mleg;            // [cap+1][npl] P_0.. of x, y, z scaled (zeros outside the region)
+    double *mchg;            // [cap+1] charge of each moving ion inside the region, else 0
+    double *mcoef, *mq, *mer;   // [ntpol] coefficients, M q rfscal, per-row energy terms
 };
 
 // spatial partner order (sort.hip)
@@ -166,6 +175,11 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s);  // k_step (pairs, 
 hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s);  // RFPARION
 hipError_t launch_notrans(const StepParams &p, hipStream_t s);  // NOTRANS after a step
 hipError_t launch_fixed(const StepParams &p, hipStream_t s);    // terms of the fixed ions
+// MMIJ reaction field (mmij.hip): RECT_RF1 for ENERGY (force into frf, energy into slot 0 of
+// E_GSBPB), RECT_RF0 for INTERACT (added to *out)
+hipError_t launch_mmij(const StepParams &p, hipStream_t s);
+hipError_t launch_mmij_interact(const StepParams &p, double xj, double yj, double zj, int jtype,
+                                int j, double *out, hipStream_t s);
 hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
                                 hipStream_t s);
 hipError_t launch_advance(int64_t *ctr, int64_t by, hipStream_t s);

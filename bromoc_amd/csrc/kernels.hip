@@ -295,7 +295,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
             F[0] += fd.x; F[1] += fd.y; F[2] += fd.z;
         }
     }
-    if (P.rfon && !is_site) {   // reaction-field forces from k_rfpair
+    if ((P.rfon || P.mmon) && !is_site) {   // reaction-field forces (k_rfpair or k_mm_force)
         const double4 fr = P.frf[i];
         F[0] += fr.x; F[1] += fr.y; F[2] += fr.z;
     }
@@ -677,7 +677,7 @@ __global__ __launch_bounds__(BLOCK) void k_wave(StepParams P) {
             F[0] += fd.x; F[1] += fd.y; F[2] += fd.z;
         }
     }
-    if (P.rfon && !is_site) {
+    if ((P.rfon || P.mmon) && !is_site) {
         const double4 fr = P.frf[i];
         F[0] += fr.x; F[1] += fr.y; F[2] += fr.z;
     }
@@ -806,6 +806,7 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s) {
 hipError_t launch_step(const StepParams &p, hipStream_t s) {
     hipError_t e = launch_rfpar(p, true, s);
     if (e != hipSuccess) return e;
+    if ((e = launch_mmij(p, s)) != hipSuccess) return e;
     if ((e = launch_fixed(p, s)) != hipSuccess) return e;
     e = launch_dna(p, s);
     if (e != hipSuccess) return e;

@@ -74,6 +74,9 @@ type, bind(C) :: bgpu_system
   integer(c_int32_t) :: qpore, reserved4
   real(c_double) :: voltage, afact, ceps, zmemb1, zmemb2
   type(c_ptr) :: ampl1, p1, ampl2, p2, rcylinder
+  integer(c_int32_t) :: mmij_shape, xnpol, ynpol, znpol, ntpol, reserved5
+  type(c_ptr) :: lstpx, lstpy, lstpz, bnorm, mmij
+  real(c_double) :: xscal, yscal, zscal, rfscal
 end type
 
 type, bind(C) :: bgpu_energies
@@ -289,6 +292,8 @@ subroutine bromoc_gpu_init(seed)
   real(c_double), allocatable, target :: dif(:), am3(:), pp3(:), spx(:), spy(:), spb(:), spc(:), spd(:)
   integer(c_int32_t), allocatable, target :: asn(:), ccsn(:)
   real(c_double), allocatable, target :: ma1(:), mp1(:), ma2(:), mp2(:), mrc(:)
+  integer(c_int32_t), allocatable, target :: mlx(:), mly(:), mlz(:)
+  real(c_double), allocatable, target :: mbn(:), mmm(:)
   real(c_double), allocatable, target :: aaf(:), ckx(:), cky(:), ckz(:), ccx(:), ccy(:), ccz(:)
   integer :: nc
   integer :: nsp, ngrid
@@ -403,7 +408,7 @@ subroutine bromoc_gpu_init(seed)
   if (Qphiv .and. Qsvdw) scs(2:nt1) = scal(1:nttyp)
   if (cntpts > 0) zc(1:cntpts) = zcont(1:cntpts)
 
-  s%abi_version = 3
+  s%abi_version = 4
   s%ntot = ntot; s%nsites = nsites; s%nfix = nfix; s%capacity = cap
   s%ntype = ntype; s%nold = nold; s%nttyp = nttyp; s%npair = ntc; s%flags = int(fl, c_int32_t)
   s%x = c_loc(xs); s%y = c_loc(ys); s%z = c_loc(zs); s%typei = c_loc(ityp)
@@ -509,6 +514,25 @@ subroutine bromoc_gpu_init(seed)
   s%voltage = voltage; s%afact = afact; s%ceps = ceps; s%zmemb1 = zmemb1; s%zmemb2 = zmemb2
   s%ampl1 = c_loc(ma1); s%p1 = c_loc(mp1); s%ampl2 = c_loc(ma2); s%p2 = c_loc(mp2)
   s%rcylinder = c_loc(mrc)
+  ! MMIJ reaction field (gsbpmod, readmmij.f90 + rect_norm.f90 as shell_simul.f90:2780-2831
+  ! leaves it); shape 2 (SPHERE) is refused by bromoc_create
+  allocate(mlx(max(1,ntpol)), mly(max(1,ntpol)), mlz(max(1,ntpol)), mbn(max(1,ntpol)))
+  allocate(mmm(max(1,ntpol*ntpol)))
+  mlx = 0; mly = 0; mlz = 0; mbn = 0.0; mmm = 0.0
+  s%mmij_shape = 0; s%ntpol = 0
+  if (Qmmij) then
+    s%mmij_shape = merge(1, 2, shapes == 'RECTBOX ')
+    s%ntpol = ntpol
+    if (shapes == 'RECTBOX ') then
+      s%xnpol = xnpol; s%ynpol = ynpol; s%znpol = znpol
+      mlx(1:ntpol) = lstpx(1:ntpol); mly(1:ntpol) = lstpy(1:ntpol); mlz(1:ntpol) = lstpz(1:ntpol)
+    endif
+    mbn(1:ntpol) = bnorm(1:ntpol)
+    mmm(1:ntpol*ntpol) = mmij(1:ntpol*ntpol)
+    s%xscal = xscal; s%yscal = yscal; s%zscal = zscal; s%rfscal = rfscal
+  endif
+  s%lstpx = c_loc(mlx); s%lstpy = c_loc(mly); s%lstpz = c_loc(mlz)
+  s%bnorm = c_loc(mbn); s%mmij = c_loc(mmm)
   call bgpu_check(bromoc_create(s, 0, gctx), 'bromoc_create')
 end subroutine
 

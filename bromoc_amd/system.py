@@ -126,6 +126,9 @@ class System:
     # MEMBRANE: dict(voltage, afact, ceps, zmemb1, zmemb2, qpore, and per atnam2 type
     # [nttyp+1] ampl1, p1, ampl2, p2, rcylinder)
     membrane: dict = None
+    # MMIJ reaction field, RECTBOX: dict(npol (xnpol, ynpol, znpol), lstp [ntpol, 3] degrees,
+    # bnorm [ntpol], mmij [ntpol, ntpol] symmetric, scal (xscal, yscal, zscal), rfscal)
+    mmij: dict = None
     ndna: int = 0
     nion: int = 0
     meta: dict = field(default_factory=dict)
@@ -297,6 +300,17 @@ class System:
                        np.float64, C.c_double)
             s.p2 = arr(np.stack([m["p2"], 1.0 / np.asarray(m["p2"], float)], 1).reshape(-1),
                        np.float64, C.c_double)
+        g = self.mmij
+        if g is not None:
+            s.mmij_shape = int(g.get("shape", 1))
+            s.xnpol, s.ynpol, s.znpol = (int(v) for v in g["npol"])
+            lst = np.asarray(g["lstp"], np.int32).reshape(-1, 3)
+            s.ntpol = len(lst)
+            s.lstpx, s.lstpy, s.lstpz = (arr(lst[:, a], np.int32, C.c_int32) for a in range(3))
+            s.bnorm = arr(g["bnorm"], np.float64, C.c_double)
+            s.mmij = arr(np.asarray(g["mmij"], float).reshape(-1), np.float64, C.c_double)
+            s.xscal, s.yscal, s.zscal = (float(v) for v in g["scal"])
+            s.rfscal = float(g.get("rfscal", 1.0))
         return s, keep
 
 

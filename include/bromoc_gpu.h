@@ -35,7 +35,8 @@
  * BROMOC_QNOTRANS) after each step.  ENERGY and INTERACT include RFPARION / RFPARIONJ
  * (rfpar.f90:20-370) under BROMOC_QRFPAR, the B-spline repulsion map (vdwgd1spln.f90:19-
  * 170, vdwgd0spln.f90) under BROMOC_QBSPLINE, the membrane (membrane.f90:19-107,
- * interact.f90:58-119) under BROMOC_QMEMB; ENERGY includes APFOR and CONTRANS
+ * interact.f90:58-119) under BROMOC_QMEMB, the MMIJ reaction field (rect_rf1.f90,
+ * rect_rf0.f90) under BROMOC_QMMIJ; ENERGY includes APFOR and CONTRANS
  * (energy.mono.f90:509-580) under BROMOC_QAPFOR / BROMOC_QCONTRANS.
  *
  * Conventions
@@ -61,7 +62,7 @@
 extern "C" {
 #endif
 
-#define BROMOC_ABI_VERSION 3
+#define BROMOC_ABI_VERSION 4
 
 /* status codes */
 #define BROMOC_OK              0
@@ -97,8 +98,9 @@ extern "C" {
 #define BROMOC_QAPFOR      (1u << 23)  /* APFOR: constant forces on sites (Qapfor) */
 #define BROMOC_QCONTRANS   (1u << 24)  /* CONTRANS: harmonic site/centroid restraints */
 #define BROMOC_QMEMB       (1u << 16)  /* analytic membrane (membrane.f90) */
+#define BROMOC_QMMIJ       (1u << 17)  /* MMIJ reaction field, RECTBOX (rect_rf1/rect_rf0);
+                                        * the SPHERE shape (sphe_rf1) is rejected */
 /* options of the reference that this path rejects with BROMOC_E_UNSUPPORTED */
-#define BROMOC_QMMIJ       (1u << 17)  /* MMIJ reaction field (rect_rf1/sphe_rf1) */
 #define BROMOC_QNMCDEN     (1u << 19)  /* per-species phiv maps (Qnmcden) */
 #define BROMOC_QDEBYHYB    (1u << 20)  /* hybrid Debye (Qdebyhyb) */
 #define BROMOC_QSPHERE     (1u << 26)  /* spherical system (fixcoor.f90:24-31) */
@@ -216,6 +218,19 @@ typedef struct bromoc_system {
     int32_t qpore, reserved4;
     double voltage, afact, ceps, zmemb1, zmemb2;
     const double *ampl1, *p1, *ampl2, *p2, *rcylinder;
+    /* MMIJ reaction field (QMMIJ, gsbpmod; readmmij.f90, rect_norm.f90, rect_rf1.f90,
+     * rect_rf0.f90): shape 1 = RECTBOX, the only one supported (SPHERE, sphe_rf1.f90, is
+     * rejected).  ntpol = xnpol*ynpol*znpol basis functions, each a product of Legendre
+     * polynomials of degrees lstpx/lstpy/lstpz (0-based degrees, < xnpol etc.; every npol
+     * >= 3) in the scaled coordinates xscal*x, ..., normalised by bnorm (rect_norm.f90);
+     * the region is |x| <= 1/xscal, |y| <= 1/yscal, |z| <= 1/zscal; mmij [ntpol*ntpol] is
+     * the symmetric reaction-field matrix, row-major, both triangles filled (as readmmij.f90
+     * leaves it); rfscal scales the energy. */
+    int32_t mmij_shape, xnpol, ynpol, znpol, ntpol, reserved5;
+    const int32_t *lstpx, *lstpy, *lstpz;   /* [ntpol] */
+    const double *bnorm;                     /* [ntpol] */
+    const double *mmij;                      /* [ntpol*ntpol] */
+    double xscal, yscal, zscal, rfscal;
 } bromoc_system;
 
 /* Energy terms as in grandmod/gsbpmod/nucleotmod (energy.mono.f90:46-68,689). */

@@ -14,12 +14,16 @@
  *   getyd/gety, squarespline/splinepot, ENERGY, STATICF1, VDWGD1TRLN, VDWGD1SPLN,
  *   APFOR/CONTRANS, the PROXDIFF records, DYNAMICS0/0NUC/1/2/3, NOTRANS, FIXCOOR, COUNTIONS,
  *   SPLINE/SEVAL, RFPARION/RFPARIONJ, INTERACT, STATICF0, VDWGD0TRLN, VDWGD0SPLN, GRAND,
- *   METROPOLIS: against the reference Fortran compiled by oracle/Makefile (oracle/_ref),
- *   plus closed forms and invariants (Newton's third law, F = -grad E by central
- *   differences, ENERGY = 1/2 sum INTERACT, exact trilinear interpolation of linear fields,
- *   the B-spline's reproduction of a linear map, closed-form BD step, acceptance rule,
- *   METROPOLIS energy bookkeeping).
+ *   METROPOLIS, MEMBRANE, RECT_RF1/RECT_RF0: against the reference Fortran compiled by
+ *   oracle/Makefile (oracle/_ref), plus closed forms and invariants (Newton's third law,
+ *   F = -grad E by central differences, ENERGY = 1/2 sum INTERACT, exact trilinear
+ *   interpolation of linear fields, the B-spline's reproduction of a linear map,
+ *   closed-form BD step, acceptance rule, METROPOLIS energy bookkeeping, the reaction-field
+ *   energy as a quadratic form of numpy's Legendre coefficients).
  *   philox4x64: bit-exact against numpy.random.Philox (a published library routine).
+ *   The per-cycle statistics (rho, rDNA, g(r), prob2/prob3, virial pressure; inline in
+ *   simul1.f90, no routine to compile): against hand-placed bins, numpy's truncated bins,
+ *   the ideal-gas law and the virial recomputed from the forces.
  */
 #include <math.h>
 #include <stdint.h>
@@ -76,6 +80,9 @@ typedef struct orc_ctx {
     double *af, *kx, *ky, *kz, *contrx, *contry, *contrz;
     /* MEMBRANE per atnam2 type */
     double *ampl1, *p1, *ampl2, *p2, *rcylinder;
+    /* MMIJ reaction field (RECTBOX) */
+    int32_t *lstpx, *lstpy, *lstpz;
+    double *bnorm, *mmij, *coef, *mq, *lpx, *lpy, *lpz;
     /* per-cycle statistics of simul1.f90 (reals as the reference keeps them; prob integer) */
     bromoc_analysis an;
     double *rho, *rdna, *gr, pres;
@@ -361,7 +368,9 @@ EXPORT void orc_destroy(orc_ctx *c) {
                   c->xs, c->ys, c->bs, c->cs, c->ds, c->gsrfen, c->greff, c->srfe, c->reff,
                   c->srfed, c->reffd, c->dids, c->sn, c->csn, c->af, c->kx, c->ky, c->kz,
                   c->contrx, c->contry, c->contrz, c->ampl1, c->p1, c->ampl2, c->p2,
-                  c->rcylinder, c->rho, c->rdna, c->gr, c->prob2, c->prob3};
+                  c->rcylinder, c->rho, c->rdna, c->gr, c->prob2, c->prob3, c->lstpx,
+                  c->lstpy, c->lstpz, c->bnorm, c->mmij, c->coef, c->mq, c->lpx, c->lpy,
+                  c->lpz};
     for (size_t k = 0; k < sizeof(ps) / sizeof(ps[0]); k++) free(ps[k]);
     for (int k = 0; k < 8; k++) free(c->lists[k]);
     for (int k = 0; k < 6; k++) free(c->lvals[k]);
@@ -370,6 +379,10 @@ EXPORT void orc_destroy(orc_ctx *c) {
 
 EXPORT orc_ctx *orc_create(const bromoc_system *s) {
     if (!s || s->ntot < 0 || s->capacity < s->ntot) return NULL;
+    if ((s->flags & BROMOC_QMMIJ) &&
+        (s->mmij_shape != 1 || s->xnpol < 3 || s->ynpol < 3 || s->znpol < 3 || s->xnpol > 64 ||
+         s->ynpol > 64 || s->znpol > 64 || s->ntpol < 1))
+        return NULL;
     orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
     c->s = *s;
     int cap = s->capacity;
@@ -458,6 +471,19 @@ EXPORT orc_ctx *orc_create(const bromoc_system *s) {
         c->p1 = (double *)dupmem(s->p1, sizeof(double) * 2 * n1);
         c->p2 = (double *)dupmem(s->p2, sizeof(double) * 2 * n1);
     }
+    if (s->flags & BROMOC_QMMIJ) {
+        size_t n = (size_t)s->ntpol;
+        c->lstpx = (int32_t *)dupmem(s->lstpx, sizeof(int32_t) * n);
+        c->lstpy = (int32_t *)dupmem(s->lstpy, sizeof(int32_t) * n);
+        c->lstpz = (int32_t *)dupmem(s->lstpz, sizeof(int32_t) * n);
+        c->bnorm = (double *)dupmem(s->bnorm, sizeof(double) * n);
+        c->mmij = (double *)dupmem(s->mmij, sizeof(double) * n * n);
+        c->coef = (double *)calloc(n, sizeof(double));
+        c->mq = (double *)calloc(n, sizeof(double));
+        c->lpx = (double *)calloc(((size_t)cap + 1) * s->xnpol, sizeof(double));
+        c->lpy = (double *)calloc(((size_t)cap + 1) * s->ynpol, sizeof(double));
+        c->lpz = (double *)calloc(((size_t)cap + 1) * s->znpol, sizeof(double));
+    }
     if (s->flags & BROMOC_QAPFOR) {
         c->sn = (int32_t *)dupmem(s->sn, sizeof(int32_t) * s->afn);
         c->af = (double *)dupmem(s->af, sizeof(double) * 3 * s->afn);
@@ -522,7 +548,7 @@ static uint32_t supported_mask(void) {
            BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV | BROMOC_QDEBY | BROMOC_QCOUNTION |
            BROMOC_QDNAFREE | BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR | BROMOC_QRFPSIN |
            BROMOC_QBSPLINE | BROMOC_QPROXDIFF | BROMOC_QNOTRANS | BROMOC_QAPFOR |
-           BROMOC_QCONTRANS | BROMOC_QMEMB;
+           BROMOC_QCONTRANS | BROMOC_QMEMB | BROMOC_QMMIJ;
 }
 
 EXPORT int orc_check_flags(uint32_t flags) {
@@ -1145,6 +1171,135 @@ static double rfparionj(orc_ctx *c, double xj, double yj, double zj, int j, int 
     return E;
 }
 
+/* ------------------------------------------------- MMIJ reaction field, RECTBOX */
+/* Legendre polynomials P_0..P_{n-1} of the scaled coordinate xs into lp[0..n-1], the
+ * recurrence of rect_rf1.f90:51-63 (n >= 3) */
+static void legendre(double xs, int n, double *lp) {
+    lp[0] = 1.0;
+    lp[1] = xs;
+    lp[2] = 0.5 * (3.0 * xs * xs - 1.0);
+    for (int k = 3; k <= n - 1; k++)   /* Fortran n = k: lpol(n+1) from lpol(n), lpol(n-1) */
+        lp[k] = ((2.0 * k - 1) * xs * lp[k - 1] - (k - 1.0) * lp[k - 2]) / k;
+}
+
+static int mmij_inside(const orc_ctx *c, double xg, double yg, double zg) {
+    const bromoc_system *s = &c->s;
+    double xmax = 1.0 / s->xscal, ymax = 1.0 / s->yscal, zmax = 1.0 / s->zscal;
+    double xmin = -1.0 / s->xscal, ymin = -1.0 / s->yscal, zmin = -1.0 / s->zscal;
+    return xg >= xmin && xg <= xmax && yg >= ymin && yg <= ymax && zg >= zmin && zg <= zmax;
+}
+
+/* RECT_RF1, rect_rf1.f90:19-163: the coefficients q_m over the moving ions inside the
+ * region, MQ = M q rfscal, egsbpb = celec sum 0.5 q_m MQ_m and the forces */
+static void rect_rf1(orc_ctx *c, int qforces, double *egsbpb) {
+    const bromoc_system *s = &c->s;
+    const int nt = s->ntpol, nx = s->xnpol, ny = s->ynpol, nz = s->znpol;
+    const int ion0 = s->nsites + s->nfix;
+    for (int ii = 0; ii < nt; ii++) c->coef[ii] = 0.0;
+    for (int i = ion0; i < s->ntot; i++) {
+        int itype = abs(c->typei[i]);
+        double charge = c->cg[itype];
+        double xg = c->x[i], yg = c->y[i], zg = c->z[i];
+        if (!mmij_inside(c, xg, yg, zg)) continue;
+        double *lx = c->lpx + (size_t)i * nx, *ly = c->lpy + (size_t)i * ny, *lz = c->lpz + (size_t)i * nz;
+        legendre(s->xscal * xg, nx, lx);
+        legendre(s->yscal * yg, ny, ly);
+        legendre(s->zscal * zg, nz, lz);
+        for (int ii = 0; ii < nt; ii++) {
+            double norm = c->bnorm[ii];
+            c->coef[ii] = c->coef[ii] + charge * norm * lx[c->lstpx[ii]] * ly[c->lstpy[ii]] * lz[c->lstpz[ii]];
+        }
+    }
+    for (int ii = 0; ii < nt; ii++) {
+        c->mq[ii] = 0.0;
+        for (int jj = 0; jj < nt; jj++)
+            c->mq[ii] = c->mq[ii] + c->mmij[(size_t)ii * nt + jj] * c->coef[jj] * s->rfscal;
+    }
+    double e = 0.0;
+    for (int ii = 0; ii < nt; ii++) e = e + 0.5 * c->coef[ii] * c->mq[ii];
+    *egsbpb = e * c->celec;
+    if (!qforces) return;
+    double dlx[64], dly[64], dlz[64];
+    for (int mm = ion0; mm < s->ntot; mm++) {
+        int itype = abs(c->typei[mm]);
+        double charge = c->cg[itype];
+        double xg = c->x[mm], yg = c->y[mm], zg = c->z[mm];
+        if (!mmij_inside(c, xg, yg, zg)) continue;
+        double rx = 0.0, ry = 0.0, rz = 0.0;
+        double ccc = charge * c->celec;
+        double xxs = s->xscal * xg, yys = s->yscal * yg, zs = s->zscal * zg;
+        const double *lx = c->lpx + (size_t)mm * nx, *ly = c->lpy + (size_t)mm * ny, *lz = c->lpz + (size_t)mm * nz;
+        dlx[0] = 0.0; dly[0] = 0.0; dlz[0] = 0.0;
+        dlx[1] = 1.0; dly[1] = 1.0; dlz[1] = 1.0;
+        dlx[2] = 3.0 * xxs; dly[2] = 3.0 * yys; dlz[2] = 3.0 * zs;
+        for (int k = 3; k <= nx - 1; k++) dlx[k] = xxs * dlx[k - 1] + k * lx[k - 1];
+        for (int k = 3; k <= ny - 1; k++) dly[k] = yys * dly[k - 1] + k * ly[k - 1];
+        for (int k = 3; k <= nz - 1; k++) dlz[k] = zs * dlz[k - 1] + k * lz[k - 1];
+        for (int k = 0; k < nx; k++) dlx[k] = dlx[k] * s->xscal;
+        for (int k = 0; k < ny; k++) dly[k] = dly[k] * s->yscal;
+        for (int k = 0; k < nz; k++) dlz[k] = dlz[k] * s->zscal;
+        for (int ii = 0; ii < nt; ii++) {
+            int xp = c->lstpx[ii], yp = c->lstpy[ii], zp = c->lstpz[ii];
+            double norm = c->bnorm[ii];
+            double xl = lx[xp], yl = ly[yp], zl = lz[zp];
+            double dx = norm * dlx[xp] * yl * zl;
+            double dy = norm * xl * dly[yp] * zl;
+            double dz = norm * xl * yl * dlz[zp];
+            rx = rx - dx * c->mq[ii];
+            ry = ry - dy * c->mq[ii];
+            rz = rz - dz * c->mq[ii];
+        }
+        c->fx[mm] = c->fx[mm] + rx * ccc;
+        c->fy[mm] = c->fy[mm] + ry * ccc;
+        c->fz[mm] = c->fz[mm] + rz * ccc;
+    }
+}
+
+/* RECT_RF0, rect_rf0.f90:19-115: g(ntot) - g(ntot without j), particle j at (xj,yj,zj) of
+ * type jtype (j = ntot: an insertion).  The reference reads j's polynomials whether or not
+ * j is a moving ion inside the region (left unset otherwise); here such a j has no term. */
+static double rect_rf0(orc_ctx *c, double xj, double yj, double zj, int j, int jtype) {
+    const bromoc_system *s = &c->s;
+    const int nt = s->ntpol, nx = s->xnpol, ny = s->ynpol, nz = s->znpol;
+    const int ion0 = s->nsites + s->nfix;
+    const int ntot = j == s->ntot ? s->ntot + 1 : s->ntot;
+    int jin = 0;
+    for (int ii = 0; ii < nt; ii++) c->coef[ii] = 0.0;
+    for (int i = ion0; i < ntot; i++) {
+        int itype;
+        double xg, yg, zg;
+        if (i == j) { itype = jtype; xg = xj; yg = yj; zg = zj; }
+        else { itype = abs(c->typei[i]); xg = c->x[i]; yg = c->y[i]; zg = c->z[i]; }
+        double charge = c->cg[itype];
+        if (!mmij_inside(c, xg, yg, zg)) continue;
+        if (i == j) jin = 1;
+        double *lx = c->lpx + (size_t)i * nx, *ly = c->lpy + (size_t)i * ny, *lz = c->lpz + (size_t)i * nz;
+        legendre(s->xscal * xg, nx, lx);
+        legendre(s->yscal * yg, ny, ly);
+        legendre(s->zscal * zg, nz, lz);
+        for (int ii = 0; ii < nt; ii++) {
+            double norm = c->bnorm[ii];
+            c->coef[ii] = c->coef[ii] + charge * norm * lx[c->lstpx[ii]] * ly[c->lstpy[ii]] * lz[c->lstpz[ii]];
+        }
+    }
+    if (!jin) return 0.0;
+    double charge = c->cg[jtype];
+    const double *lx = c->lpx + (size_t)j * nx, *ly = c->lpy + (size_t)j * ny, *lz = c->lpz + (size_t)j * nz;
+    double *coef2 = c->mq;
+    for (int ii = 0; ii < nt; ii++) {
+        double norm = c->bnorm[ii];
+        coef2[ii] = c->coef[ii] - charge * norm * lx[c->lstpx[ii]] * ly[c->lstpy[ii]] * lz[c->lstpz[ii]];
+    }
+    double e = 0.0;
+    for (int ii = 0; ii < nt; ii++) {
+        double ci = c->coef[ii], cj = coef2[ii];
+        e = e + 0.5 * (ci * ci - cj * cj) * c->mmij[(size_t)ii * nt + ii];
+        for (int jj = ii + 1; jj < nt; jj++)
+            e = e + (ci * c->coef[jj] - cj * coef2[jj]) * c->mmij[(size_t)ii * nt + jj];
+    }
+    return e * c->celec * s->rfscal;
+}
+
 EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
     const bromoc_system *s = &c->s;
     if (orc_check_flags(s->flags)) return BROMOC_E_UNSUPPORTED;
@@ -1157,6 +1312,10 @@ EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
     }
     if (fl & BROMOC_QMEMB) membrane(c, qforces, &E.ememb);   /* energy.mono.f90:81-83 */
     E.ener = E.ememb;
+    if (fl & BROMOC_QMMIJ) {   /* energy.mono.f90:85-92 */
+        rect_rf1(c, qforces, &E.egsbpb);
+        E.ener = E.ener + E.egsbpb;
+    }
     if (fl & BROMOC_QPHIX) {
         staticf1(c, qforces, &E.egsbpa);
         E.ener = E.ener + E.egsbpa;
@@ -1952,6 +2111,7 @@ EXPORT int orc_interact(orc_ctx *c, double xj, double yj, double zj, int32_t jty
     int beion = (fl & BROMOC_QPAR) && j >= ns;
     int jtype2 = beion ? c->nwtype[jtype] : c->typtyp[j];
     if (fl & BROMOC_QMEMB) dener = membrane_interact(c, xj, yj, zj, j, jtype, jtype2, beion, alert);
+    if (fl & BROMOC_QMMIJ) dener = dener + rect_rf0(c, xj, yj, zj, j, jtype);   /* interact.f90:121-141 */
     /* interact.f90:144-154; the reference also calls it for a site j, reading srfe
      * outside its array (rfpar.f90:356); sites get no reaction-field term here */
     if ((fl & BROMOC_QRFPAR) && beion) dener = dener + rfparionj(c, xj, yj, zj, j, jtype);

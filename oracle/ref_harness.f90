@@ -60,6 +60,9 @@ type, bind(C) :: c_system
   integer(c_int32_t) :: qpore, reserved4
   real(c_double) :: voltage, afact, ceps, zmemb1, zmemb2
   type(c_ptr) :: ampl1, p1, ampl2, p2, rcylinder
+  integer(c_int32_t) :: mmij_shape, xnpol, ynpol, znpol, ntpol, reserved5
+  type(c_ptr) :: lstpx, lstpy, lstpz, bnorm, mmij
+  real(c_double) :: xscal, yscal, zscal, rfscal
 end type
 
 type, bind(C) :: c_gcmc
@@ -75,7 +78,7 @@ integer(c_int32_t), parameter :: F_PAR=1, F_NUCL=2, F_NONBOND=4, F_NOBOND=8, F_P
      F_SVDW=64, F_SOLV=128, F_DEBY=256, F_COUNTION=512, F_DNAFREE=1024, &
      F_RFPAR=33554432, F_DIFFUSE=268435456, F_PROFILE=536870912, F_RFPSIN=1073741824, &
      F_BSPLINE=262144, F_PROXDIFF=2097152, F_NOTRANS=4194304, F_APFOR=8388608, &
-     F_CONTRANS=16777216, F_MEMB=65536
+     F_CONTRANS=16777216, F_MEMB=65536, F_MMIJ=131072
 integer :: site_names_code(0:5)
 
 contains
@@ -161,7 +164,7 @@ Qdnafree = flag(s%flags, F_DNAFREE)
 Qenergy = .true.
 Qforces = .false.
 Qmemb = flag(s%flags, F_MEMB)
-Qmmij = .false.
+Qmmij = flag(s%flags, F_MMIJ)
 Qrfpar = .false.
 Qtrln = .not. flag(s%flags, F_BSPLINE)
 Qnmcden = .false.
@@ -294,6 +297,7 @@ endif
 dids = 0.0
 call set_dna_options(s)
 if (Qmemb) call set_membrane(s)
+if (Qmmij) call set_mmij(s)
 if (Qdiffuse) then
   plength2 = s%plength2; pcenter = s%pcenter
   call c_f_pointer(s%ampl3, dp, [ntype+1]); ampl3(1:ntype) = dp(2:ntype+1)
@@ -349,6 +353,32 @@ call c_f_pointer(s%ys, dp, [n]); ys(1:n) = dp
 call c_f_pointer(s%bs, dp, [n]); sp_b(1:n) = dp
 call c_f_pointer(s%cs, dp, [n]); sp_c(1:n) = dp
 call c_f_pointer(s%ds, dp, [n]); sp_d(1:n) = dp
+end subroutine
+
+! MMIJ state (gsbpmod) as shell_simul.f90:2780-2831, READMMIJ and RECT_NORM leave it for
+! a RECTBOX region: the region centred at 0, degrees, norms, the symmetric matrix
+subroutine set_mmij(s)
+use ref_harness
+use gsbpmod
+implicit none
+type(c_system), intent(in) :: s
+integer(c_int32_t), pointer :: ip(:)
+real(c_double), pointer :: dp(:)
+integer :: n
+n = s%ntpol
+shapes = 'RECTBOX '
+xnpol = s%xnpol; ynpol = s%ynpol; znpol = s%znpol; ntpol = n
+if (allocated(lstpx)) deallocate(lstpx, lstpy, lstpz, bnorm, coef, mmij)
+allocate(lstpx(n), lstpy(n), lstpz(n), bnorm(n), coef(n), mmij(n*n))
+call c_f_pointer(s%lstpx, ip, [n]); lstpx = ip
+call c_f_pointer(s%lstpy, ip, [n]); lstpy = ip
+call c_f_pointer(s%lstpz, ip, [n]); lstpz = ip
+call c_f_pointer(s%bnorm, dp, [n]); bnorm = dp
+call c_f_pointer(s%mmij, dp, [n*n]); mmij = dp
+xscal = s%xscal; yscal = s%yscal; zscal = s%zscal; rfscal = s%rfscal
+xmin = -1.0/xscal; xmax = 1.0/xscal
+ymin = -1.0/yscal; ymax = 1.0/yscal
+zmin = -1.0/zscal; zmax = 1.0/zscal
 end subroutine
 
 ! MEMBRANE state as shell_simul.f90:1563-1640 leaves it (per atnam2 type)

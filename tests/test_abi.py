@@ -63,11 +63,14 @@ def _plain():
 
 
 def test_create_rejects_unsupported_options():
-    for flag in (abi.QMMIJ, abi.QNMCDEN, abi.QSPHERE, abi.QECYL, abi.QDEBYHYB):
+    for flag in (abi.QNMCDEN, abi.QSPHERE, abi.QECYL, abi.QDEBYHYB):
         s = _plain()
         s.flags |= flag
         rc, _ = _create(s)
         assert rc == abi.E_UNSUPPORTED
+    s = configs.with_mmij(_plain(), npol=(3, 3, 3))
+    s.mmij = dict(s.mmij, shape=2)                # the SPHERE reaction field (sphe_rf1.f90)
+    assert _create(s)[0] == abi.E_UNSUPPORTED
 
 
 def test_create_rejects_bad_indices():
@@ -105,6 +108,16 @@ def test_create_rejects_inconsistent_options():
     s = _plain()
     s.flags |= abi.QMEMB                          # no membrane parameters given
     assert _create(s)[0] == abi.E_ARG
+    s = _plain()
+    s.flags |= abi.QMMIJ                          # no reaction-field matrix given
+    assert _create(s)[0] == abi.E_ARG
+    s = configs.with_mmij(_plain(), npol=(2, 3, 3))   # rect_rf1.f90 needs 3 polynomials
+    assert _create(s)[0] == abi.E_ARG
+    s = configs.with_mmij(_plain(), npol=(3, 3, 3))
+    s.mmij = dict(s.mmij, lstp=np.where(s.mmij["lstp"] == 2, 3, s.mmij["lstp"]))
+    assert _create(s)[0] == abi.E_ARG             # a degree beyond xnpol - 1
+    s = configs.with_rfpar(configs.with_mmij(_plain(), npol=(3, 3, 3)))
+    assert _create(s)[0] == abi.E_ARG             # shell_simul.f90:2869
     for flag in (abi.QPROXDIFF, abi.QNOTRANS, abi.QAPFOR, abi.QCONTRANS):
         s = _plain()                              # ions only: these need DNA sites
         s.flags |= flag
@@ -142,11 +155,12 @@ def test_batch_create_validation():
     sh, reps = configs.replica_batch(seed=1, nrep=2, n_range=(10, 20), box=(30.0, 30.0, 60.0),
                                      dcel=1.5, rpore=8.0, half_slab=8.0,
                                      table_builder=oracle_table_builder)
-    bad = sh.copy()
-    bad.flags |= abi.QNUCL
-    with pytest.raises(abi.BromocError) as ei:
-        Batch(bad, reps)
-    assert ei.value.code == abi.E_UNSUPPORTED
+    for flag in (abi.QNUCL, abi.QBSPLINE, abi.QMEMB, abi.QMMIJ):
+        bad = sh.copy()
+        bad.flags |= flag
+        with pytest.raises(abi.BromocError) as ei:
+            Batch(bad, reps)
+        assert ei.value.code == abi.E_UNSUPPORTED
     r0 = reps[0]
     r0.typei = r0.typei.copy()
     r0.typei[0] = 9                     # not an ion type

@@ -134,10 +134,11 @@ def test_diffusion_and_rfpar_through_front_end(which):
             assert abs(a - b) <= 1e-14 * max(abs(b), 1.0)
 
 
-@pytest.mark.parametrize("which", ["bspline", "apfor_contrans", "proxdiff", "membrane"])
+@pytest.mark.parametrize("which", ["bspline", "apfor_contrans", "proxdiff", "membrane", "mmij"])
 def test_dna_options_through_front_end(which):
-    """B-spline repulsion map (Qtrln off), APFOR + CONTRANS (apfmod, nucleotmod) and
-    PROXDIFF (DYNAMICS3) handed over from the modules: energies, forces and BD steps."""
+    """B-spline repulsion map (Qtrln off), APFOR + CONTRANS (apfmod, nucleotmod), PROXDIFF
+    (DYNAMICS3), MEMBRANE and the MMIJ reaction field (gsbpmod) handed over from the modules:
+    energies, forces and BD steps."""
     base = with_fixed(small_pore(seed=41), 4, nonfree=(3,))
     if which == "bspline":
         s = configs.with_bspline(base, thold27=3.0)
@@ -147,8 +148,10 @@ def test_dna_options_through_front_end(which):
         s.x[:s.nsites] += 0.3
     elif which == "proxdiff":
         s = configs.with_proxdiff(base, beta=2.5, diff0=0.05)
-    else:
+    elif which == "membrane":
         s = configs.with_membrane(base, thick=16.0, volts=0.15, pore=(11.0, 5.0, 2.5))
+    else:
+        s = configs.with_mmij(base, npol=(3, 4, 4), rfscal=0.8, region=(30.0, 30.0, 40.0))
     seed, c0 = 91, 4
     f, r = _pair(s)
     f.L.bromoc_gpu_set_seed(seed, c0)
