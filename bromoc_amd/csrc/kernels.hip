@@ -20,13 +20,13 @@ namespace bromoc {
 // only at site 1 (j.eq.1) and later sites replace it only when strictly smaller, so when
 // site 1 is outside its table range the record stays zero unless some site gives a
 // negative dist - dmi.
-template <int T>
+template <int T, int NW>
 __device__ inline void prox_record(const StepParams &P, int i, const double4 &pa, int a,
                                    const double (*s_pxd)[T], const int (*s_pxj)[T],
                                    const DevPair *row, double dids[5]) {
     double d = INFINITY;
     int jj = 0x7fffffff;
-    for (int q = 0; q < WPB; q++) {
+    for (int q = 0; q < NW; q++) {
         const double od = s_pxd[q][a];
         const int oj = s_pxj[q][a];
         if (od < d || (od == d && oj < jj)) { d = od; jj = oj; }
@@ -48,17 +48,18 @@ __device__ inline void prox_record(const StepParams &P, int i, const double4 &pa
     dids[4] = ez;
 }
 
-// k_step: T target particles per block of 256 threads (targets tbeg.. of the launch).
-// Every thread walks the partner set with stride 256, two partners per iteration, and
+// k_step: T target particles per block of NW wavefronts (targets tbeg.. of the launch).
+// Every thread walks the partner set with stride NW*64, two partners per iteration, and
 // evaluates each partner against all T targets.  Per-target sums: butterfly within each
-// wavefront, then a fixed-order sum over the 4 wavefronts in LDS.  Wavefront w < T also
+// wavefront, then a fixed-order sum over the NW wavefronts in LDS.  Wavefront w < T also
 // handles target w's 8 grid corners; thread a < T finishes target a (force, energies,
 // BD update).  CG-DNA list forces of site targets come from k_dna (fdna).
-template <int FEAT, int T>
-__global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
+template <int FEAT, int T, int NW>
+__global__ __launch_bounds__(NW * WAVE) void k_step(StepParams P) {
+    constexpr int NB = NW * WAVE;   // threads of the block
     __shared__ DevPair s_row[T][MAX_LDS_TYPES];   // pair parameters by partner type
     __shared__ double4 s_tail[T][MAX_LDS_TYPES];  // (sc3, tail fct, dm2hi, 0) by partner type
-    __shared__ double s_red[WPB][T][7];
+    __shared__ double s_red[NW][T][7];
     __shared__ double s_dg[T][5];
     __shared__ double s_g[T][3];
     __shared__ int s_warn[T];
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
         tfree[a] = !tsite[a] || ((FEAT & F_DNA) && P.stfree[i] != 0);
         // pair rows of this block's targets
         const int row = __double2loint(pa.w) * nt1;
-        for (int k = tid; k < nt1; k += BLOCK) {
+        for (int k = tid; k < nt1; k += NB) {
             const DevPair d = P.rows[row + k];
             s_row[a][k] = d;
             s_tail[a][k] = make_double4(d.sc3, d.fct, d.dm2hi, 0.0);
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
     // Gaussians of the BD update, fetched by the last wavefront (lanes 0..T-1) before its
     // share of the pair loop
     if constexpr ((FEAT & F_UPDATE) != 0) {
-        const int a = tid - (BLOCK - WAVE);
+        const int a = tid - (NB - WAVE);
         if (a >= 0 && a < T && tval[a]) {
             const int i = ti[a];
             const double *nr = noise_row(P, i);
@@ -117,9 +118,9 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
     const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
     auto pair_loop = [&](auto en) {
         constexpr bool EN = decltype(en)::value;
-        for (int k0 = tid; k0 < ntot; k0 += 2 * BLOCK) {
+        for (int k0 = tid; k0 < ntot; k0 += 2 * NB) {
             // partners in spatial order, two per iteration (two load chains in flight)
-            const int k1 = k0 + BLOCK;
+            const int k1 = k0 + NB;
             const bool has1 = k1 < ntot;
             const double4 p0 = P.spos[k0];
             const double4 p1 = P.spos[has1 ? k1 : k0];
@@ -174,8 +175,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
         if (P.write_energy) pair_loop(std::true_type{});
         else pair_loop(std::false_type{});
     }
-    __shared__ double s_pxd[(FEAT & F_PROX) ? WPB : 1][T];
-    __shared__ int s_pxj[(FEAT & F_PROX) ? WPB : 1][T];
+    __shared__ double s_pxd[(FEAT & F_PROX) ? NW : 1][T];
+    __shared__ int s_pxj[(FEAT & F_PROX) ? NW : 1][T];
     if constexpr ((FEAT & F_PROX) != 0) {
 #pragma unroll
         for (int a = 0; a < T; a++) {
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
     for (int k = 0; k < 3; k++) {
         double acc = 0.0;
 #pragma unroll
-        for (int q = 0; q < WPB; q++) acc += s_red[q][a][k];
+        for (int q = 0; q < NW; q++) acc += s_red[q][a][k];
         F[k] = acc + s_dg[a][k];
     }
     if constexpr ((FEAT & F_DNA) != 0) {
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
         for (int k = 0; k < 4; k++) {
             double acc = 0.0;
 #pragma unroll
-            for (int q = 0; q < WPB; q++) acc += s_red[q][a][3 + k];
+            for (int q = 0; q < NW; q++) acc += s_red[q][a][3 + k];
             pe[k] = acc;
         }
         // this kernel owns the pair and grid categories; k_dna writes the DNA ones
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams P) {
         const double g[3] = {s_g[a][0], s_g[a][1], s_g[a][2]};
         double dids[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if constexpr ((FEAT & F_PROX) != 0) {
-            if (!is_site) prox_record(P, i, pa, a, s_pxd, s_pxj, s_row[a], dids);
+            if (!is_site) prox_record<T, NW>(P, i, pa, a, s_pxd, s_pxj, s_row[a], dids);
         }
         const double4 pn = bd_update(P, i, is_site, pa, F, g, dids);
         P.poso[i] = pn;
@@ -498,8 +499,10 @@ hipError_t launch_notrans(const StepParams &p, hipStream_t s) {
 // kind, so wavefronts rarely diverge); sums go butterfly-then-LDS in a fixed order.  The
 // force on the site goes to fdna[i] (zero for a fixed site); the energies of the terms the
 // site heads go to epart.
-__global__ __launch_bounds__(BLOCK) void k_dna(StepParams P) {
-    __shared__ double s_r[WPB][3 + 8];
+template <int NW>
+__global__ __launch_bounds__(NW * WAVE) void k_dna(StepParams P) {
+    constexpr int NB = NW * WAVE;   // threads of the block
+    __shared__ double s_r[NW][3 + 8];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
     const int i = blockIdx.x;
     const bool free_me = P.stfree[i] != 0;
@@ -508,7 +511,7 @@ __global__ __launch_bounds__(BLOCK) void k_dna(StepParams P) {
 #pragma unroll
     for (int k = 0; k < ND; k++) e[k] = 0.0;
     const int k0 = P.inc_ptr[i], k1 = P.inc_ptr[i + 1];
-    for (int k = k0 + tid; k < k1; k += BLOCK) {
+    for (int k = k0 + tid; k < k1; k += NB) {
         const uint32_t wd = P.inc[k];
         const int kind = wd >> 29;
         const bool on = kind <= T_DIHE ? (P.flags & QNOBOND) != 0
@@ -530,7 +533,7 @@ __global__ __launch_bounds__(BLOCK) void k_dna(StepParams P) {
     __syncthreads();
     if (tid < 3 + 8) {
         double acc = 0.0;
-        for (int q = 0; q < WPB; q++) acc += s_r[q][tid];
+        for (int q = 0; q < NW; q++) acc += s_r[q][tid];
         if (tid < 3) {
             if ((P.flags & QAPFOR) && free_me) acc += reinterpret_cast<const double *>(&P.apf[i])[tid];
             if (P.flags & QCONTRANS) acc = contrans_force(P, i, tid, acc);
@@ -757,10 +760,10 @@ static void dispatch_wave(const StepParams &p, int feat, hipStream_t s) {
 #undef CASE
 }
 
-template <int T>
+template <int T, int NW>
 static void dispatch_step(const StepParams &p, int feat, hipStream_t s) {
-    dim3 grid((p.tend - p.tbeg + T - 1) / T), block(BLOCK);
-#define CASE(F) case F: hipLaunchKernelGGL((k_step<F, T>), grid, block, 0, s, p); break;
+    dim3 grid((p.tend - p.tbeg + T - 1) / T), block(NW * WAVE);
+#define CASE(F) case F: hipLaunchKernelGGL((k_step<F, T, NW>), grid, block, 0, s, p); break;
     switch (feat) { CASES16(0) CASES16(16) }
 #undef CASE
 }
@@ -768,7 +771,7 @@ static void dispatch_step(const StepParams &p, int feat, hipStream_t s) {
 // BD steps of DYNAMICS3 (DNA and ions present, update on): one target per block
 static void dispatch_prox(const StepParams &p, int feat, hipStream_t s) {
     dim3 grid(p.tend - p.tbeg), block(BLOCK);
-#define CASE(F) case F: hipLaunchKernelGGL((k_step<F | F_PROX, 1>), grid, block, 0, s, p); break;
+#define CASE(F) case F: hipLaunchKernelGGL((k_step<F | F_PROX, 1, WPB>), grid, block, 0, s, p); break;
     switch (feat) {
         CASE(9) CASE(11) CASE(13) CASE(15) CASE(25) CASE(27) CASE(29) CASE(31)
     }
@@ -778,7 +781,7 @@ static void dispatch_prox(const StepParams &p, int feat, hipStream_t s) {
 hipError_t launch_dna(const StepParams &p, hipStream_t s) {
     if (!(p.flags & QNUCL) || p.nsites <= 0) return hipSuccess;
     if (p.flags & QCONTRANS) hipLaunchKernelGGL(k_centroid, dim3(1), dim3(BLOCK), 0, s, p);
-    hipLaunchKernelGGL(k_dna, dim3(p.nsites), dim3(BLOCK), 0, s, p);
+    hipLaunchKernelGGL(k_dna<WPB>, dim3(p.nsites), dim3(BLOCK), 0, s, p);
     return hipGetLastError();
 }
 
@@ -795,9 +798,11 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s) {
         return hipGetLastError();
     }
     switch (p.tpb) {
-        case 1: dispatch_step<1>(p, feat, s); break;
-        case 2: dispatch_step<2>(p, feat, s); break;
-        case 4: dispatch_step<4>(p, feat, s); break;
+        // one target: two wavefronts per block, so a small system's blocks all fit in one
+        // round of waves on the 256 CUs (measured faster than four at 2300 targets)
+        case 1: dispatch_step<1, 2>(p, feat, s); break;
+        case 2: dispatch_step<2, WPB>(p, feat, s); break;
+        case 4: dispatch_step<4, WPB>(p, feat, s); break;
         default: dispatch_wave(p, feat, s); break;   // one wavefront per target
     }
     return hipGetLastError();
