@@ -24,10 +24,11 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s + % HBM roofline, batched vs plain, FP64 rel err<=1e-10"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the step kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# (KiB per dispatch, one counter per run) on this workload, scaled by what the same counters
-# read for a 256 MiB device copy (bench.py --copy-probe 256): FETCH_SIZE x 2.0, WRITE_SIZE
-# x 1.0.  The counters cannot be read from inside this process.
-TRAFFIC_KIB = {"pore_dna": (3069.0, 144.5)}
+# (KiB per dispatch, one counter per run) on this workload and launch shape (targets per
+# block the autotuner picks), scaled by what the same counters read for a 256 MiB device
+# copy (bench.py --copy-probe 256): FETCH_SIZE x 2.0, WRITE_SIZE x 1.0.  The counters cannot
+# be read from inside this process; a shape without a measurement reports null.
+TRAFFIC_KIB = {("pore_dna", 1): (3073.5, 145.3), ("pore_dna", 2): (3198.4, 144.5)}
 TRAFFIC_SCALE = (2.0, 1.0)
 
 
@@ -315,8 +316,8 @@ def main():
                                   counter0=args.warmup + args.steps)
     bpp = step_bytes(s, injected=False)
     traffic = None
-    if args.workload in TRAFFIC_KIB:
-        fk, wk = TRAFFIC_KIB[args.workload]
+    if TRAFFIC_KIB.get((args.workload, tpb)):
+        fk, wk = TRAFFIC_KIB[(args.workload, tpb)]
         traffic = 1024.0 * (fk * TRAFFIC_SCALE[0] + wk * TRAFFIC_SCALE[1])
     achieved = bpp * nmov / (kernel_ms * 1e-3) / 1e9
     # FP64 work of the kernel: ~34 flops per table pair (r^2 8, sqrt 1, index 2, U 4,

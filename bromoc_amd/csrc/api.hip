@@ -896,28 +896,38 @@ static int autotune(bromoc_ctx *c) {
     int sst = ensure_sorted(c);
     if (sst) return sst;
     const int cand[4] = {1, 2, 4, 8};   // k_step with 1, 2, 4 targets per block; k_wave
-    double best = 1e300;
-    int pick = 4;
+    double best[4] = {1e300, 1e300, 1e300, 1e300};
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
-    for (int t : cand) {
-        StepParams p = params_for(c);
-        p.tpb = t;
-        p.mode = 0;
-        p.write_energy = 0;
-        p.write_forces = 0;
-        p.warn = c->warn_scratch;
-        for (int r = 0; r < 2; r++) HIPCHK(c, launch_step(p, c->stream));
-        HIPCHK(c, hipEventRecord(e0, c->stream));
-        for (int r = 0; r < 5; r++) HIPCHK(c, launch_step(p, c->stream));
-        HIPCHK(c, hipEventRecord(e1, c->stream));
-        int st = sync_err(c);
-        if (st) return st;
-        float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; pick = t; }
+    // three interleaved rounds of five launches per shape; each shape keeps its best round
+    // (shapes within a few per cent of each other are told apart reliably this way)
+    for (int round = 0; round < 3; round++) {
+        for (int q = 0; q < 4; q++) {
+            StepParams p = params_for(c);
+            p.tpb = cand[q];
+            p.mode = 0;
+            p.write_energy = 0;
+            p.write_forces = 0;
+            p.warn = c->warn_scratch;
+            // one whole ENERGY (the DNA, reaction-field and fixed-ion terms the pair kernel
+            // reads), then the pair kernel alone is timed
+            HIPCHK(c, launch_step(p, c->stream));
+            HIPCHK(c, launch_pairs(p, c->stream));
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+            for (int r = 0; r < 5; r++) HIPCHK(c, launch_pairs(p, c->stream));
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            int st = sync_err(c);
+            if (st) return st;
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+            best[q] = std::min(best[q], (double)ms);
+        }
     }
+    int pick = 4;
+    double bt = 1e300;
+    for (int q = 0; q < 4; q++)
+        if (best[q] < bt) { bt = best[q]; pick = cand[q]; }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->tpb = pick;

@@ -118,16 +118,23 @@ __global__ __launch_bounds__(NW * WAVE) void k_step(StepParams P) {
     const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
     auto pair_loop = [&](auto en) {
         constexpr bool EN = decltype(en)::value;
-        for (int k0 = tid; k0 < ntot; k0 += 2 * NB) {
-            // partners in spatial order, two per iteration (two load chains in flight)
-            const int k1 = k0 + NB;
-            const bool has1 = k1 < ntot;
-            const double4 p0 = P.spos[k0];
-            const double4 p1 = P.spos[has1 ? k1 : k0];
+        // partners per iteration: three for one-target blocks (the small-system shape, where
+        // the loop is latency-bound), two otherwise (measured best for each shape)
+        constexpr int PU = T == 1 ? 3 : 2;
+        for (int k0 = tid; k0 < ntot; k0 += PU * NB) {
+            // partners in spatial order, PU per iteration (PU load chains in flight)
+            double4 pp[PU];
+            bool pok[PU];
 #pragma unroll
-            for (int u = 0; u < 2; u++) {
-                const double4 pj = u == 0 ? p0 : p1;
-                const bool jok = u == 0 || has1;
+            for (int u = 0; u < PU; u++) {
+                const int k = k0 + u * NB;
+                pok[u] = k < ntot;
+                pp[u] = P.spos[pok[u] ? k : k0];
+            }
+#pragma unroll
+            for (int u = 0; u < PU; u++) {
+                const double4 pj = pp[u];
+                const bool jok = pok[u];
                 const int t2j = __double2loint(pj.w);
                 const int j = __double2hiint(pj.w);
 #pragma unroll
