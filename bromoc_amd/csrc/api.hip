@@ -76,6 +76,7 @@ struct bromoc_ctx {
     double *cg = nullptr, *fact1a = nullptr, *fact2a = nullptr, *scal = nullptr, *zcont = nullptr;
     int32_t *inc_ptr = nullptr;
     uint32_t *inc = nullptr;
+    double2 *incr = nullptr;
     int32_t *lst[8] = {};
     double *lval[6] = {};
     float *phix = nullptr;
@@ -562,17 +563,35 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     }
     for (int i = 0; i < ns; i++) deg[i + 1] += deg[i];
     std::vector<uint32_t> inc(std::max<int64_t>(deg[ns], 1));
+    std::vector<double2> incr(inc.size(), make_double2(0.0, 0.0));
     {
         std::vector<int32_t> fill(deg.begin(), deg.end());
         for (int k = 0; k < 8 && dna; k++)
             for (int r = 0; r < ar[k]; r++)
                 for (int t = 0; t < cnt[k]; t++) {
                     int site = lp[k][(int64_t)r * cnt[k] + t];
-                    inc[fill[site]++] = inc_pack(k, r, t);
+                    const int at = fill[site]++;
+                    inc[at] = inc_pack(k, r, t);
+                    // resolved record of a two-site term: the other site, the list value,
+                    // and for a base pair whether site a is a C or G base
+                    uint64_t bits = inc[at];
+                    double par = 0.0;
+                    if (ar[k] == 2) {
+                        bits |= (uint64_t)(uint32_t)lp[k][(int64_t)(1 - r) * cnt[k] + t] << 32;
+                        const int sl = pair_lval_slot(k);
+                        if (sl >= 0) par = vp[sl][t];
+                        if (k == T_BP) {
+                            const int na = s->namsite[lp[k][t]];
+                            if (na == 4 || na == 5) bits |= (uint64_t)1 << 63;
+                        }
+                    }
+                    std::memcpy(&incr[at].x, &bits, sizeof(bits));
+                    incr[at].y = par;
                 }
     }
     ACHK(dupload(c, &c->inc_ptr, deg.data(), deg.size()));
     ACHK(dupload(c, &c->inc, inc.data(), inc.size()));
+    ACHK(dupload(c, &c->incr, incr.data(), incr.size()));
     if (s->flags & BROMOC_QPHIX)
         ACHK(dupload(c, &c->phix, (const float *)s->phix.data,
                      (size_t)s->phix.nclx * s->phix.ncly * s->phix.nclz));
@@ -684,7 +703,7 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.ires = s->ires; p.celec = celec_value();
     p.cg = c->cg; p.fact1a = c->fact1a; p.fact2a = c->fact2a; p.nwtype = c->nwtype;
     p.rth = s->rth; p.srpx = s->srpx; p.srpk = s->srpk; p.srpy = s->srpy;
-    p.inc_ptr = c->inc_ptr; p.inc = c->inc;
+    p.inc_ptr = c->inc_ptr; p.inc = c->inc; p.incr = c->incr;
     for (int k = 0; k < 8; k++) { p.lst[k] = c->lst[k]; p.nlist[k] = dna ? cnt[k] : 0; }
     for (int k = 0; k < 6; k++) p.lval[k] = c->lval[k];
     p.epsnuc = s->epsnuc; p.scalepairing = s->scalepairing; p.fctn = s->fctn;

@@ -633,18 +633,17 @@ __device__ constexpr int ND_CAT[ND] = {E_BOND, E_ANG, E_DIHE, E_STACK, E_BP, E_E
                                        E_GSBPA, E_VDWGD};
 enum { D_BOND = 0, D_ANG, D_DIHE, D_STACK, D_BP, D_EX, D_QQ, D_SOLV, D_GSBPA, D_VDWGD };
 
-// One DNA term incident on site `me` in role `role`: force on `me` (added to f) and the
-// term's energy (added to e when role == 0).  energy.mono.f90:205-507.
-__device__ inline void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, double *e) {
-    const int kind = w >> 29, role = (w >> 27) & 3, t = w & 0x7FFFFFF;
+// A two-site DNA term (bond, stacking, base pair, excluded volume, P-P, solvent) incident
+// on site `me` in role `role` (0: site a, 1: site b), given v1 = r_b - r_a, the term's
+// list value (equilibrium distance or sigma) and, for a base pair, whether site a is a C
+// or G base: force on `me` (added to f) and the term's energy (added to e when role == 0).
+// energy.mono.f90:205-232, 335-507.
+__device__ inline void dna_pair(const StepParams &P, int kind, int role, V3 v1, double par,
+                                bool wide_bp, bool free_me, V3 &f, double *e) {
     const double eps = P.epsnuc;
-    const int n = P.nlist[kind];
-    const int32_t *L = P.lst[kind];
     if (kind == T_BOND) {
-        int a = L[t], b = L[n + t];
-        V3 v1 = sub3(ld3(P, b), ld3(P, a));
         double dist = sqrt(dot3(v1, v1));
-        double vard = dist - P.lval[0][t];
+        double vard = dist - par;
         double vard2 = vard * vard;
         if (role == 0) e[D_BOND] += eps * vard2 * (1.0 + 100.0 * vard2);
         if (free_me) {
@@ -653,7 +652,77 @@ __device__ inline void dna_term(const StepParams &P, uint32_t w, bool free_me, V
             double s = role == 0 ? 1.0 : -1.0;
             f.x += s * f1.x; f.y += s * f1.y; f.z += s * f1.z;
         }
-    } else if (kind == T_ANGLE) {
+        return;
+    }
+    double de = 0.0;
+    bool force = free_me;
+    if (kind == T_STACK) {
+        double dist2 = dot3(v1, v1);
+        double idist2 = 1.0 / dist2;
+        double sg = par;
+        double cte1 = pow3(sg * sg * idist2);
+        if (role == 0) e[D_STACK] += 4.0 * eps * cte1 * (cte1 - 1.0);
+        de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
+    } else if (kind == T_BP) {
+        double epsln = wide_bp ? 2.532 * eps * P.scalepairing : 2.0 * eps * P.scalepairing;
+        double dist2 = dot3(v1, v1);
+        double idist2 = 1.0 / dist2;
+        double sg = par;
+        double cte1 = sg * sg * idist2;
+        if (role == 0) e[D_BP] += epsln * pow5(cte1) * (20.0 * cte1 - 24.0);
+        de = 240.0 * epsln * pow5(cte1) * (cte1 - 1.0) * idist2;
+    } else if (kind == T_EX) {
+        double dist2 = dot3(v1, v1);
+        double idist2 = 1.0 / dist2;
+        double sgex2 = par * par;
+        if (dist2 < sgex2) {
+            double cte1 = pow3(sgex2 * idist2);
+            if (role == 0) e[D_EX] += 4.0 * eps * cte1 * (cte1 - 1.0) + eps;
+            de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
+        } else {
+            force = false;
+        }
+    } else if (kind == T_QQ) {
+        double dist = sqrt(dot3(v1, v1));
+        double idist = 1.0 / dist;
+        if (P.flags & QDEBY) {
+            double idistkd = exp(-dist * P.ikappa);
+            if (role == 0) e[D_QQ] += P.fctn * idist * idistkd;
+            de = P.fctn * (dist + P.kappa) * pow3(idist) * P.ikappa * idistkd;
+        } else {
+            if (role == 0) e[D_QQ] += P.fctn * idist;
+            de = P.fctn * pow3(idist);
+        }
+    } else {  // T_SLV
+        double dist = sqrt(dot3(v1, v1));
+        double cte1 = exp((13.38 - dist) * 0.1875);
+        if (role == 0) e[D_SOLV] += P.epsolv * ((1.0 - cte1) * (1.0 - cte1)) - P.epsolv;
+        de = 0.375 * P.epsolv * cte1 * (cte1 - 1.0) / dist;
+    }
+    if (force) {
+        double s = role == 0 ? -de : de;
+        f.x += s * v1.x; f.y += s * v1.y; f.z += s * v1.z;
+    }
+}
+
+// One DNA term incident on site `me` in role `role`: force on `me` (added to f) and the
+// term's energy (added to e when role == 0), the sites read from the term lists.
+// energy.mono.f90:205-507.
+__device__ inline void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, double *e) {
+    const int kind = w >> 29, role = (w >> 27) & 3, t = w & 0x7FFFFFF;
+    const double eps = P.epsnuc;
+    const int n = P.nlist[kind];
+    const int32_t *L = P.lst[kind];
+    if (kind != T_ANGLE && kind != T_DIHE) {
+        int a = L[t], b = L[n + t];
+        V3 v1 = sub3(ld3(P, b), ld3(P, a));
+        const int sl = pair_lval_slot(kind);
+        const double par = sl >= 0 ? P.lval[sl][t] : 0.0;
+        const int na = kind == T_BP ? P.namsite[a] : 0;
+        dna_pair(P, kind, role, v1, par, na == 5 || na == 4, free_me, f, e);
+        return;
+    }
+    if (kind == T_ANGLE) {
         int a = L[t], b = L[n + t], d = L[2 * n + t];
         V3 rb = ld3(P, b);
         V3 v1 = sub3(ld3(P, a), rb), v2 = sub3(ld3(P, d), rb);
@@ -696,60 +765,6 @@ __device__ inline void dna_term(const StepParams &P, uint32_t w, bool free_me, V
             else g = {-f4.x - v12 * f1.x + v23 * f4.x, -f4.y - v12 * f1.y + v23 * f4.y,
                       -f4.z - v12 * f1.z + v23 * f4.z};
             f.x += g.x; f.y += g.y; f.z += g.z;
-        }
-    } else {
-        int a = L[t], b = L[n + t];
-        V3 v1 = sub3(ld3(P, b), ld3(P, a));
-        double de = 0.0;
-        bool force = free_me;
-        if (kind == T_STACK) {
-            double dist2 = dot3(v1, v1);
-            double idist2 = 1.0 / dist2;
-            double sg = P.lval[3][t];
-            double cte1 = pow3(sg * sg * idist2);
-            if (role == 0) e[D_STACK] += 4.0 * eps * cte1 * (cte1 - 1.0);
-            de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
-        } else if (kind == T_BP) {
-            int na = P.namsite[a];
-            double epsln = (na == 5 || na == 4) ? 2.532 * eps * P.scalepairing
-                                                : 2.0 * eps * P.scalepairing;
-            double dist2 = dot3(v1, v1);
-            double idist2 = 1.0 / dist2;
-            double sg = P.lval[4][t];
-            double cte1 = sg * sg * idist2;
-            if (role == 0) e[D_BP] += epsln * pow5(cte1) * (20.0 * cte1 - 24.0);
-            de = 240.0 * epsln * pow5(cte1) * (cte1 - 1.0) * idist2;
-        } else if (kind == T_EX) {
-            double dist2 = dot3(v1, v1);
-            double idist2 = 1.0 / dist2;
-            double sgex2 = P.lval[5][t] * P.lval[5][t];
-            if (dist2 < sgex2) {
-                double cte1 = pow3(sgex2 * idist2);
-                if (role == 0) e[D_EX] += 4.0 * eps * cte1 * (cte1 - 1.0) + eps;
-                de = 24.0 * eps * cte1 * (2.0 * cte1 - 1.0) * idist2;
-            } else {
-                force = false;
-            }
-        } else if (kind == T_QQ) {
-            double dist = sqrt(dot3(v1, v1));
-            double idist = 1.0 / dist;
-            if (P.flags & QDEBY) {
-                double idistkd = exp(-dist * P.ikappa);
-                if (role == 0) e[D_QQ] += P.fctn * idist * idistkd;
-                de = P.fctn * (dist + P.kappa) * pow3(idist) * P.ikappa * idistkd;
-            } else {
-                if (role == 0) e[D_QQ] += P.fctn * idist;
-                de = P.fctn * pow3(idist);
-            }
-        } else {  // T_SLV
-            double dist = sqrt(dot3(v1, v1));
-            double cte1 = exp((13.38 - dist) * 0.1875);
-            if (role == 0) e[D_SOLV] += P.epsolv * ((1.0 - cte1) * (1.0 - cte1)) - P.epsolv;
-            de = 0.375 * P.epsolv * cte1 * (cte1 - 1.0) / dist;
-        }
-        if (force) {
-            double s = role == 0 ? -de : de;
-            f.x += s * v1.x; f.y += s * v1.y; f.z += s * v1.z;
         }
     }
 }

@@ -34,6 +34,10 @@ struct GridGeom {
 
 // DNA term kinds in the per-site incidence list
 enum TermKind { T_BOND = 0, T_ANGLE, T_DIHE, T_STACK, T_BP, T_EX, T_QQ, T_SLV };
+// the list value a two-site term carries (lval slot of its kind; qq and solvent none)
+__host__ __device__ constexpr int pair_lval_slot(int kind) {
+    return kind == T_BOND ? 0 : kind == T_STACK ? 3 : kind == T_BP ? 4 : kind == T_EX ? 5 : -1;
+}
 // incidence word: kind (3 bits) | role (2 bits) | term index (27 bits)
 __host__ __device__ inline uint32_t inc_pack(int kind, int role, int term) {
     return (uint32_t(kind) << 29) | (uint32_t(role) << 27) | uint32_t(term);
@@ -80,6 +84,9 @@ struct StepParams {
     // CG-DNA
     const int32_t *inc_ptr;  // [nsites+1]
     const uint32_t *inc;
+    // the same incidences resolved: bits of x = word | other site << 32 | (base pair of a
+    // C/G site a) << 63 (other site 0 for angles and dihedrals), y = the term's list value
+    const double2 *incr;
     const int32_t *lst[8];   // component-major site lists
     const double *lval[6];
     int32_t nlist[8];

@@ -518,13 +518,25 @@ __global__ __launch_bounds__(NW * WAVE) void k_dna(StepParams P) {
 #pragma unroll
     for (int k = 0; k < ND; k++) e[k] = 0.0;
     const int k0 = P.inc_ptr[i], k1 = P.inc_ptr[i + 1];
+    const V3 rme = ld3(P, i);
     for (int k = k0 + tid; k < k1; k += NB) {
-        const uint32_t wd = P.inc[k];
+        // resolved incidence: one load gives the term, the other site and the list value
+        const double2 rec = P.incr[k];
+        const uint64_t bits = (uint64_t)__double_as_longlong(rec.x);
+        const uint32_t wd = (uint32_t)bits;
         const int kind = wd >> 29;
         const bool on = kind <= T_DIHE ? (P.flags & QNOBOND) != 0
                                        : ((P.flags & QNONBOND) != 0 &&
                                           (kind != T_SLV || (P.flags & QSOLV) != 0));
-        if (on) dna_term(P, wd, free_me, f, e);
+        if (!on) continue;
+        if (kind == T_ANGLE || kind == T_DIHE) {
+            dna_term(P, wd, free_me, f, e);
+            continue;
+        }
+        const int role = (wd >> 27) & 3;
+        const V3 ro = ld3(P, (int)((bits >> 32) & 0x7FFFFFFFu));
+        const V3 v1 = role == 0 ? sub3(ro, rme) : sub3(rme, ro);   // r_b - r_a
+        dna_pair(P, kind, role, v1, rec.y, (bits >> 63) != 0, free_me, f, e);
     }
     f.x = wave_sum(f.x);
     f.y = wave_sum(f.y);
