@@ -207,6 +207,10 @@ def _declare(lib):
                                P(C.c_float), P(C.c_float), C.c_int64], C.c_int),
         "bromoc_repwalls": ([C.c_int32, C.c_int32, C.c_int32, P(C.c_int8), C.c_int32, C.c_int],
                             C.c_int),
+        "bromoc_discretize": ([P(Pair), C.c_int32, C.c_int32, C.c_double, C.c_double,
+                               C.c_double, C.c_double, C.c_double, _dp, P(Pair)], C.c_int),
+        "bromoc_update_tables": ([vp, C.c_int32, _ip, _ip, _dp, _dp], C.c_int),
+        "bromoc_table_vjp_device": ([vp, C.c_int32, _dp, _dp, _dp, _dp], C.c_int),
         "bromoc_set_analysis": ([vp, P(Analysis)], C.c_int),
         "bromoc_analyze": ([vp], C.c_int),
         "bromoc_get_analysis": ([vp, P(C.c_int64), P(C.c_int64), P(C.c_int64), _ip,

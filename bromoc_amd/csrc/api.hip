@@ -16,6 +16,7 @@
 
 #include "../../include/bromoc_gpu.h"
 #include "engine.h"
+#include "tables.h"
 
 using namespace bromoc;
 
@@ -94,6 +95,8 @@ struct bromoc_ctx {
     // host mirrors
     std::vector<int32_t> h_typei, h_nwtype, h_typtyp;
     std::vector<double> h_cg;
+    std::vector<bromoc_pair> h_pairs;   // pair types 0..npair as created (tables updated)
+    std::vector<int32_t> h_pairidx;
     // GCMC
     bool have_gcmc = false;
     bromoc_gcmc g{};
@@ -519,6 +522,8 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     ACHK(dalloc(c, &c->epart, (size_t)NE * cap));
     ACHK(dalloc(c, &c->ered, NE));
     ACHK(dupload(c, &c->pairidx, s->pairidx, (size_t)nt1 * nt1));
+    c->h_pairs.assign(s->pairs, s->pairs + s->npair + 1);
+    c->h_pairidx.assign(s->pairidx, s->pairidx + (size_t)nt1 * nt1);
     std::vector<DevPair> dp(s->npair + 1);
     for (int t = 0; t <= s->npair; t++) {
         const bromoc_pair &p = s->pairs[t];
@@ -1524,141 +1529,43 @@ int bromoc_metropolis(bromoc_ctx *c, int32_t nmcm, double mcmax, double ikbt,
     return BROMOC_OK;
 }
 
-// -------------------------------------------------------------- EFPOT table (host)
-// SPLINEPOT (splinepot.f90:19-96) with SQUARESPLINE (polynom.f90:20-48): the (r,U)
-// samples become the piecewise-quadratic table ep(1:3,k) plus the wall (sc1,sc2) and tail
-// (sc3) coefficients.  Integer powers are formed by binary powering, as the reference's
-// compiler expands x**n, so the coefficients match the reference bit for bit.
-static double ipow(double x, unsigned n) {
-    double r = 0.0, sq = x;
-    bool have = false;
-    for (; n; n >>= 1) {
-        if (n & 1u) { r = have ? r * sq : sq; have = true; }
-        if (n > 1) sq = sq * sq;
-    }
-    return have ? r : 1.0;
-}
-
-struct Quad {
-    double a, b, c;  // U = a + b r + c r^2 (ep1, ep2, ep3)
-};
-
-static void square_spline(const std::vector<double> &x, std::vector<double> y, std::vector<Quad> &q,
-                          double d1, double d2, double dd2) {
-    const int n = (int)x.size();
-    q.assign(n, Quad{0, 0, 0});
-    if (n == 1) { q[0] = {y[0], 0.0, 0.0}; return; }
-    if (n == 2) {
-        double b = (y[1] - y[0]) / (x[1] - x[0]);
-        q[0] = q[1] = {y[0] - b * x[0], b, 0.0};
-        return;
-    }
-    const double y0 = y[0], yn = y[n - 1];
-    // sqfit (polynom.f90:135-169): local quadratics through consecutive triples, averaged
-    auto local = [&](int i, Quad &o) {
-        double tt = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
-        o.c = (tt - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) / (x[i + 1] - x[i - 1]);
-        o.b = tt - o.c * (x[i + 1] + x[i]);
-        o.a = y[i] - x[i] * (o.b + o.c * x[i]);
-    };
-    {
-        double tt = 1.0 / (x[1] - x[0]);
-        q[0].c = ((y[1] - y[0]) * tt - d1) * tt;
-        q[0].b = d1 - 2.0 * q[0].c * x[0];
-        q[0].a = y[0] - x[0] * (q[0].b + q[0].c * x[0]);
-    }
-    local(1, q[1]);
-    for (int i = 2; i <= n - 2; i++) {
-        local(i, q[i]);
-        q[i - 1].c = 0.5 * (q[i - 1].c + q[i].c);
-        q[i - 1].b = 0.5 * (q[i - 1].b + q[i].b);
-        q[i - 1].a = 0.5 * (q[i - 1].a + q[i].a);
-    }
-    {
-        const int m = n - 1;
-        double tt = 1.0 / (x[m] - x[m - 1]);
-        q[m].c = (d2 - (y[m] - y[m - 1]) * tt) * tt;
-        q[m].b = d2 - 2.0 * q[m].c * x[m];
-        q[m].a = y[m] - x[m] * (q[m].b + q[m].c * x[m]);
-        q[m - 1].c = 0.5 * (q[m - 1].c + q[m].c);
-        q[m - 1].b = 0.5 * (q[m - 1].b + q[m].b);
-        q[m - 1].a = 0.5 * (q[m - 1].a + q[m].a);
-    }
-    // fixder (polynom.f90:121-132): knot derivatives averaged across segments
-    std::vector<double> yp(n);
-    yp[0] = q[0].b + 2.0 * q[0].c * x[0];
-    for (int i = 1; i <= n - 2; i++)
-        yp[i] = 0.5 * (q[i].b + 2.0 * q[i].c * x[i] + q[i - 1].b + 2.0 * q[i - 1].c * x[i]);
-    yp[n - 1] = q[n - 1].b + 2.0 * q[n - 1].c * x[n - 1];
-    // linipol (polynom.f90:67-88): derivative as a piecewise-linear function
-    std::vector<double> s0(n), s1(n);
-    for (int i = 0; i <= n - 2; i++) {
-        double tt = 1.0 / (x[i + 1] - x[i]);
-        s1[i] = (yp[i + 1] - yp[i]) * tt;
-        s0[i] = (yp[i] * x[i + 1] - yp[i + 1] * x[i]) * tt;
-    }
-    s1[n - 1] = dd2;
-    s0[n - 1] = yp[n - 1] - s1[n - 1] * x[n - 1];
-    // intlin (polynom.f90:51-64): integrate it, continuous at the knots
-    q[0].c = s1[0] * 0.5;
-    q[0].b = s0[0];
-    q[0].a = y0 - x[0] * (q[0].b + q[0].c * x[0]);
-    for (int i = 1; i < n; i++) {
-        q[i].c = s1[i] * 0.5;
-        q[i].b = s0[i];
-        q[i].a = q[i - 1].a + x[i] * (q[i - 1].b - q[i].b + x[i] * (q[i - 1].c - q[i].c));
-    }
-    // fixdy (polynom.f90:91-118): restore U(x_n) with a correction preserving the ends
-    {
-        const int m = n - 1;
-        double ynn = q[m].a + x[m] * (q[m].b + x[m] * q[m].c);
-        double b = 2.0 * (yn - ynn) / (x[m] - x[0]);
-        double cc = -b * x[0] * 0.5;
-        double h = (x[1] - x[0]) / (double)(n - 1);
-        for (int i = 0; i < m; i++) {
-            q[i].a = q[i].a + cc;
-            q[i].b = q[i].b + b;
-            q[i].c = q[i].c + (-b * 0.5 / (x[i] + h * (double)i));
-        }
-        q[m].a = q[m].a + cc;
-        q[m].b = q[m].b + b;
-        q[m].c = q[m].c + (-b * 0.5 / x[m]);
-    }
-}
-
+// EFPOT tables (splinepot.f90:19-96, discretize.f90:19-120): the shared construction in
+// tables.h, on the host here and on the device in tables.hip.
 int bromoc_build_table(int32_t n, const double *r, const double *u, int32_t chr, double fct,
                        double *ep, bromoc_pair *pair) {
     if (n < 2 || !r || !u || !ep || !pair) return BROMOC_E_ARG;
     if (r[0] == 0.0 || r[n - 1] == 0.0) return BROMOC_E_ARG;
     for (int k = 1; k < n; k++)
         if (!(r[k] > r[k - 1])) return BROMOC_E_ARG;
-    const double r1 = r[0], r2 = r[1], rn = r[n - 1];
-    pair->dmi = r1;
-    pair->dm2lo = r1 * r1;
-    pair->dm2hi = rn * rn;
-    const double kf = chr ? fct : 0.0;
-    // tail U = kf/r + sc3/r^6 matched at r_n (splinepot.f90:66-69)
-    pair->sc3 = (rn * u[n - 1] - kf) * ipow(rn, 5);
-    const double d2 = -kf / ipow(rn, 2) - 6.0 * pair->sc3 / ipow(rn, 7);
-    const double dd2 = 2.0 * kf / ipow(rn, 3) + 42.0 * pair->sc3 / ipow(rn, 8);
-    // wall U = sc1/r^12 + sc2 through the first two samples (splinepot.f90:79-82)
-    const double p1 = ipow(r1, 12), p2 = ipow(r2, 12);
-    const double inv = 1.0 / (p1 - p2);
-    pair->sc1 = (u[1] - u[0]) * p2 * p1 * inv;
-    pair->sc2 = (u[0] * p1 - u[1] * p2) * inv;
-    const double d1 = -12.0 * pair->sc1 / ipow(r1, 13);
-    std::vector<double> x(r, r + n), y(u, u + n);
-    std::vector<Quad> q;
-    square_spline(x, y, q, d1, d2, dd2);
-    for (int k = 0; k < n; k++) {
-        ep[3 * k] = q[k].a;
-        ep[3 * k + 1] = q[k].b;
-        ep[3 * k + 2] = q[k].c;
-    }
+    std::vector<double> scratch(3 * (size_t)n);
+    TabEnds o{};
+    table_splinepot(n, r, u, chr ? fct : 0.0, ep, o, scratch.data());
+    pair->dmi = o.dmi; pair->dm2lo = o.dm2lo; pair->dm2hi = o.dm2hi;
+    pair->sc1 = o.sc1; pair->sc2 = o.sc2; pair->sc3 = o.sc3;
     pair->nknots = n;
     pair->efpot = 1;
     pair->chr = chr;
     pair->fct = fct;
+    return BROMOC_OK;
+}
+
+static TabPair tab_pair_of(const bromoc_pair &p) {
+    TabPair t{};
+    t.chr = p.chr; t.lj = p.lj; t.srpmf = p.srpmf;
+    t.fct = p.fct; t.epp4 = p.epp4; t.sgp2 = p.sgp2;
+    t.c0 = p.c0; t.c1 = p.c1; t.c2 = p.c2; t.c3 = p.c3; t.c4 = p.c4;
+    return t;
+}
+
+int bromoc_discretize(const bromoc_pair *params, int32_t nnp, int32_t mnp, double res, double rth,
+                      double srpx, double srpk, double srpy, double *ep, bromoc_pair *out) {
+    if (!params || !ep || !out || nnp < 1 || mnp - nnp + 1 < 3 || !(res > 0.0)) return BROMOC_E_ARG;
+    const int n = mnp - nnp + 1;
+    std::vector<double> scratch(5 * (size_t)n);
+    TabEnds o{};
+    table_discretize(tab_pair_of(*params), nnp, mnp, res, rth, srpx, srpk, srpy, ep, o,
+                     scratch.data());
+    out->sc1 = o.sc1; out->sc2 = o.sc2; out->sc3 = o.sc3;
     return BROMOC_OK;
 }
 
@@ -1717,6 +1624,111 @@ int bromoc_spline(int32_t n, const double *x, const double *y, double *b, double
         D[2] = 0.0;
     }
     return BROMOC_OK;
+}
+
+int bromoc_update_tables(bromoc_ctx *c, int32_t ntab, const int32_t *is, const int32_t *n,
+                         const double *r, const double *u) {
+    if (!c || ntab < 1 || !is || !n || !r || !u) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    const int np = c->sys.npair;
+    std::vector<int32_t> tab_of(np + 1, -1), nxi(ntab), nk(ntab);
+    std::vector<int64_t> off(ntab + 1, 0);
+    std::vector<double> kf(ntab);
+    for (int t = 0; t < ntab; t++) {
+        if (is[t] < 1 || is[t] > np) return fail(c, BROMOC_E_ARG, "update_tables: pair type");
+        const bromoc_pair &p = c->h_pairs[is[t]];
+        if (!p.efpot || n[t] != p.nknots || n[t] < 2)
+            return fail(c, BROMOC_E_ARG, "update_tables: not a table of that many knots");
+        if (tab_of[is[t]] >= 0) return fail(c, BROMOC_E_ARG, "update_tables: pair type twice");
+        tab_of[is[t]] = t;
+        nxi[t] = p.nxi;
+        nk[t] = n[t];
+        kf[t] = p.chr ? p.fct : 0.0;
+        off[t + 1] = off[t] + n[t];
+    }
+    const int64_t tot = off[ntab];
+    for (int t = 0; t < ntab; t++) {
+        const double *rt = r + off[t];
+        if (rt[0] == 0.0) return fail(c, BROMOC_E_ARG, "update_tables: r");
+        for (int k = 1; k < n[t]; k++)
+            if (!(rt[k] > rt[k - 1])) return fail(c, BROMOC_E_ARG, "update_tables: r not increasing");
+    }
+    // staging: r, u, per-table metadata, scratch and the results
+    TabJob jb{};
+    jb.ntab = ntab;
+    std::vector<void *> tmp;
+    auto up = [&](void **dst, const void *src, size_t bytes) -> hipError_t {
+        hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 8));
+        if (e != hipSuccess) return e;
+        tmp.push_back(*dst);
+        return src ? hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, c->stream) : hipSuccess;
+    };
+    hipError_t e = hipSuccess;
+    if (!e) e = up((void **)&jb.r, r, sizeof(double) * tot);
+    if (!e) e = up((void **)&jb.u, u, sizeof(double) * tot);
+    if (!e) e = up((void **)&jb.off, off.data(), sizeof(int64_t) * (ntab + 1));
+    if (!e) e = up((void **)&jb.nxi, nxi.data(), sizeof(int32_t) * ntab);
+    if (!e) e = up((void **)&jb.kf, kf.data(), sizeof(double) * ntab);
+    if (!e) e = up((void **)&jb.tab_of, tab_of.data(), sizeof(int32_t) * (np + 1));
+    if (!e) e = up((void **)&jb.scratch, nullptr, sizeof(double) * 6 * tot);
+    if (!e) e = up((void **)&jb.ends, nullptr, sizeof(TabEnds) * ntab);
+    if (!e) {
+        jb.ep = c->ep;
+        jb.pairs = c->pairs;
+        jb.rows = c->rows;
+        jb.pairidx = c->pairidx;
+        jb.nrows = (c->sys.nttyp + 1) * (c->sys.nttyp + 1);
+        jb.npair = np;
+        e = launch_table_update(jb, c->stream);
+    }
+    int st = e ? fail(c, BROMOC_E_HIP, std::string("update_tables: ") + hipGetErrorString(e)) : sync_err(c);
+    for (void *q : tmp) (void)hipFree(q);
+    if (st) return st;
+    // the host mirror and the squared range past which every table pair is in its tail
+    c->r2near = 0.0;
+    for (int t = 0; t < ntab; t++) {
+        const double rn = r[off[t] + n[t] - 1], r1 = r[off[t]];
+        bromoc_pair &p = c->h_pairs[is[t]];
+        p.dmi = r1; p.dm2lo = r1 * r1; p.dm2hi = rn * rn;
+    }
+    bool all = true;
+    for (int32_t k : c->h_pairidx) {
+        if (k <= 0) continue;
+        if (!c->h_pairs[k].efpot) all = false;
+        else c->r2near = std::max(c->r2near, c->h_pairs[k].dm2hi);
+    }
+    if (!all) c->r2near = HUGE_VAL;
+    c->base.r2near = c->r2near;
+    return BROMOC_OK;
+}
+
+int bromoc_table_vjp_device(bromoc_ctx *c, int32_t n, const double *r, const double *grad_ep,
+                            const double *grad_sc, double *grad_u) {
+    if (!c || n < 2 || !r || !grad_ep || !grad_sc || !grad_u) return BROMOC_E_ARG;
+    if (r[0] == 0.0) return BROMOC_E_ARG;
+    for (int k = 1; k < n; k++)
+        if (!(r[k] > r[k - 1])) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    // knots per launch: each thread keeps 7n doubles of its own (unit samples, table,
+    // squarespline scratch); at most 256 MiB of them at a time
+    const int64_t per = 7 * (int64_t)n;
+    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n, (256ll << 20) / (8 * per)));
+    double *d_r = nullptr, *d_gep = nullptr, *d_gsc = nullptr, *d_out = nullptr, *d_scr = nullptr;
+    hipError_t e = hipMalloc((void **)&d_r, sizeof(double) * n);
+    if (!e) e = hipMalloc((void **)&d_gep, sizeof(double) * 3 * n);
+    if (!e) e = hipMalloc((void **)&d_gsc, sizeof(double) * 3);
+    if (!e) e = hipMalloc((void **)&d_out, sizeof(double) * n);
+    if (!e) e = hipMalloc((void **)&d_scr, sizeof(double) * per * chunk);
+    if (!e) e = hipMemcpyAsync(d_r, r, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipMemcpyAsync(d_gep, grad_ep, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipMemcpyAsync(d_gsc, grad_sc, sizeof(double) * 3, hipMemcpyHostToDevice, c->stream);
+    for (int k0 = 0; !e && k0 < n; k0 += chunk)
+        e = launch_table_vjp(n, d_r, d_gep, d_gsc, d_out, d_scr, k0, std::min(chunk, n - k0), c->stream);
+    if (!e) e = hipMemcpyAsync(grad_u, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+    int st = e ? fail(c, BROMOC_E_HIP, std::string("table_vjp_device: ") + hipGetErrorString(e)) : sync_err(c);
+    for (double *q : {d_r, d_gep, d_gsc, d_out, d_scr})
+        if (q) (void)hipFree(q);
+    return st;
 }
 
 int bromoc_table_vjp(int32_t n, const double *r, const double *grad_ep, const double *grad_sc,

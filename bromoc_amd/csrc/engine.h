@@ -203,6 +203,27 @@ struct InteractParams {
 };
 hipError_t launch_interact(const InteractParams &p, hipStream_t s);
 
+// ---- EFPOT tables on the device (tables.hip)
+struct TabEnds;
+struct TabJob {
+    int32_t ntab, npair, nrows, pad;
+    const double *r, *u;        // samples of every table, concatenated
+    const int64_t *off;         // [ntab+1] first sample of each table
+    const int32_t *nxi;         // [ntab] first knot of each table in ep
+    const double *kf;           // [ntab] fct of a charged pair, else 0
+    const int32_t *tab_of;      // [npair+1] table of each pair type, -1: not updated
+    double *scratch;            // [6 * total samples]
+    TabEnds *ends;              // [ntab]
+    double4 *ep;                // the context's knots
+    DevPair *pairs, *rows;      // the context's pair parameters
+    const int32_t *pairidx;     // [nrows]
+};
+// k_tab_build (one thread per table: SPLINEPOT into ep) then k_tab_rows (the pair rows)
+hipError_t launch_table_update(const TabJob &j, hipStream_t s);
+// grad_u[k0 + t] = (d table / d u_k) . (grad_ep, grad_sc), one thread per knot
+hipError_t launch_table_vjp(int n, const double *r, const double *gep, const double *gsc,
+                            double *out, double *scratch, int k0, int count, hipStream_t s);
+
 // ---- per-cycle analyses (analysis.hip)
 struct AnParams {
     const double4 *pos;        // current positions (atnam2 type in the low word of w)

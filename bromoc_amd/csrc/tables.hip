@@ -1,0 +1,80 @@
+// tables.hip -- EFPOT tables rebuilt on the device from new samples (the knot values a
+// potential fit changes), into the context's own knot and pair arrays, and the
+// vector-Jacobian product of the table construction for the adjoint.  The construction
+// is the host's (tables.h): a table built here is bit-identical to bromoc_build_table's.
+// Each table is a sequential recurrence over its knots (squarespline's integration and
+// end correction), so a table is one thread; the Jacobian's columns are independent, one
+// thread per knot.
+#include "engine.h"
+#include "tables.h"
+
+namespace bromoc {
+
+namespace {
+
+// one thread per table: SPLINEPOT into a staging copy, then the knots as (ep1, ep2, ep3, 0)
+__global__ void k_tab_build(TabJob j) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= j.ntab) return;
+    const int64_t o = j.off[t];
+    const int n = (int)(j.off[t + 1] - o);
+    double *stage = j.scratch + 6 * o, *scr = stage + 3 * (int64_t)n;
+    TabEnds e{};
+    table_splinepot(n, j.r + o, j.u + o, j.kf[t], stage, e, scr);
+    for (int k = 0; k < n; k++)
+        j.ep[j.nxi[t] + k] = make_double4(stage[3 * k], stage[3 * k + 1], stage[3 * k + 2], 0.0);
+    j.ends[t] = e;
+}
+
+__device__ inline void set_ends(DevPair &d, const TabEnds &e) {
+    d.dmi = e.dmi; d.dm2lo = e.dm2lo; d.dm2hi = e.dm2hi;
+    d.sc1 = e.sc1; d.sc2 = e.sc2; d.sc3 = e.sc3;
+}
+
+// the wall, range and tail of each rebuilt table into its pair type and every type-pair row
+__global__ void k_tab_rows(TabJob j) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < j.nrows) {
+        const int t = j.tab_of[j.pairidx[k]];
+        if (t >= 0) set_ends(j.rows[k], j.ends[t]);
+    }
+    if (k <= j.npair) {
+        const int t = j.tab_of[k];
+        if (t >= 0) set_ends(j.pairs[k], j.ends[t]);
+    }
+}
+
+// knot k = k0 + thread: the table of the unit samples e_k dotted with the gradient, in
+// the order of the host's bromoc_table_vjp
+__global__ void k_tab_vjp(int n, const double *r, const double *gep, const double *gsc,
+                          double *out, double *scratch, int k0, int count) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int k = k0 + t;
+    double *u = scratch + 7 * (int64_t)n * t, *ep = u + n, *scr = ep + 3 * (int64_t)n;
+    for (int q = 0; q < n; q++) u[q] = q == k ? 1.0 : 0.0;
+    TabEnds e{};
+    table_splinepot(n, r, u, 0.0, ep, e, scr);
+    double acc = e.sc1 * gsc[0] + e.sc2 * gsc[1] + e.sc3 * gsc[2];
+    for (int m = 0; m < 3 * n; m++) acc += ep[m] * gep[m];
+    out[k] = acc;
+}
+
+}  // namespace
+
+hipError_t launch_table_update(const TabJob &j, hipStream_t s) {
+    hipLaunchKernelGGL(k_tab_build, dim3((j.ntab + 63) / 64), dim3(64), 0, s, j);
+    const int m = j.nrows > j.npair + 1 ? j.nrows : j.npair + 1;
+    hipLaunchKernelGGL(k_tab_rows, dim3((m + 255) / 256), dim3(256), 0, s, j);
+    return hipGetLastError();
+}
+
+hipError_t launch_table_vjp(int n, const double *r, const double *gep, const double *gsc,
+                            double *out, double *scratch, int k0, int count, hipStream_t s) {
+    hipLaunchKernelGGL(k_tab_vjp, dim3((count + 63) / 64), dim3(64), 0, s, n, r, gep, gsc, out,
+                       scratch, k0, count);
+    return hipGetLastError();
+}
+
+}  // namespace bromoc

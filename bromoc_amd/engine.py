@@ -171,6 +171,29 @@ class Engine:
         return used.value, done.value, e.value
 
 
+    # ------------------------------------------------------------- tables
+    def update_tables(self, tables):
+        """Rebuild tables on the device (bromoc_update_tables): {pair type: (r, u)}."""
+        keys = sorted(tables)
+        is_ = np.array(keys, np.int32)
+        n = np.array([len(tables[k][0]) for k in keys], np.int32)
+        r = np.ascontiguousarray(np.concatenate([tables[k][0] for k in keys]), float)
+        u = np.ascontiguousarray(np.concatenate([tables[k][1] for k in keys]), float)
+        self._chk("bromoc_update_tables",
+                  self.lib.bromoc_update_tables(self.ctx, len(keys), ptr(is_, C.c_int32),
+                                                ptr(n, C.c_int32), ptr(r), ptr(u)))
+
+    def table_vjp(self, r, grad_ep, grad_sc):
+        """bromoc_table_vjp_device on this context's GPU."""
+        r = np.ascontiguousarray(r, float)
+        out = np.zeros(len(r))
+        self._chk("bromoc_table_vjp_device",
+                  self.lib.bromoc_table_vjp_device(self.ctx, len(r), ptr(r),
+                                                   ptr(np.ascontiguousarray(grad_ep, float)),
+                                                   ptr(np.ascontiguousarray(grad_sc, float)),
+                                                   ptr(out)))
+        return out
+
     # ------------------------------------------------------- per-cycle statistics
     def set_analysis(self, **kw):
         """The statistics of simul1.f90's cycle loop (analysis_struct builds the settings);
@@ -196,6 +219,31 @@ class Engine:
                                                C.byref(pres), C.byref(nps)))
         out.update(irmax=irm.value, pres=pres.value, pres_steps=nps.value)
         return out
+
+
+def discretize(pair, nnp, mnp, res, rth=0.0, srpx=0.0, srpk=0.0, srpy=0.0):
+    """DISCRETIZE (discretize.f90:19-120) by the library: the table of an abi.Pair's analytic
+    parameters sampled at k*res, k = nnp..mnp.  Returns (ep [3n], (sc1, sc2, sc3))."""
+    lib = abi.load_library()
+    ep = np.zeros(3 * (mnp - nnp + 1))
+    out = abi.Pair()
+    rc = lib.bromoc_discretize(C.byref(pair), nnp, mnp, res, rth, srpx, srpk, srpy, ptr(ep),
+                               C.byref(out))
+    if rc:
+        raise abi.BromocError("bromoc_discretize", rc)
+    return ep, (out.sc1, out.sc2, out.sc3)
+
+
+def table_vjp(r, grad_ep, grad_sc):
+    """bromoc_table_vjp on the host: d(table)/d(samples) applied to a table gradient."""
+    lib = abi.load_library()
+    r = np.ascontiguousarray(r, float)
+    out = np.zeros(len(r))
+    rc = lib.bromoc_table_vjp(len(r), ptr(r), ptr(np.ascontiguousarray(grad_ep, float)),
+                              ptr(np.ascontiguousarray(grad_sc, float)), ptr(out))
+    if rc:
+        raise abi.BromocError("bromoc_table_vjp", rc)
+    return out
 
 
 def analysis_struct(system, what, nzmax=None, zmini=None, idelz=2.0, igr=0, npoints=10000,

@@ -18,6 +18,8 @@
  * bromoc_grand                 SUBROUTINE GRAND             grand.f90:19-121
  * bromoc_build_table           EFPOT table construction     splinepot.f90:19-96 +
  *                              squarespline polynom.f90:20-48
+ * bromoc_discretize            SUBROUTINE DISCRETIZE        discretize.f90:19-120
+ * bromoc_update_tables         SPLINEPOT on the device, into a live context
  * bromoc_spline                SUBROUTINE SPLINE            spline.f90:19-98
  * bromoc_metropolis            SUBROUTINE METROPOLIS        metropolis.f90:19-58 + MOVE
  *                              (move.f90:19-50)
@@ -430,6 +432,18 @@ int bromoc_get_analysis(bromoc_ctx *ctx, int64_t *rho, int64_t *rdna, int64_t *g
 int bromoc_build_table(int32_t n, const double *r, const double *u, int32_t chr, double fct,
                        double *ep, bromoc_pair *pair);
 
+/* EFPOT table built from a pair type's analytic parameters (discretize.f90:19-120, the
+ * EFPOT ... build command, shell_simul.f90:1840-1853,1869-1871): params' chr, fct, lj,
+ * epp4, sgp2, srpmf and c0..c4 sampled at r = k*res for k = nnp..mnp (Coulomb fct/r of a
+ * charged pair, LJ, SRPMF where r^2 <= rth with the srpx/srpk/srpy smoothing), through
+ * squarespline; the wall takes the table's value and the sampled slope at the first knot.
+ * Writes ep[3*(mnp-nnp+1)] and out->sc1..sc3; dmi and dm2 are the EFPOT command's
+ * (nnp*res and its square, (mnp*res)^2; the reference recomputes nnp = int(dmi*ires)
+ * before calling DISCRETIZE, shell_simul.f90:1869, and callers do the same).  Host-side. */
+int bromoc_discretize(const bromoc_pair *params, int32_t nnp, int32_t mnp, double res,
+                      double rth, double srpx, double srpk, double srpy, double *ep,
+                      bromoc_pair *out);
+
 /* Cubic-spline coefficients of the diffusion profile (spline.f90:19-98, the
  * Forsythe-Malcolm-Moler SPLINE): b, c, d of n >= 2 knots x (strictly monotone), y.
  * Host-side, as shell_simul.f90:1968 calls it for the PROFILE command. */
@@ -473,6 +487,19 @@ int bromoc_adjoint_rdf(bromoc_ctx *ctx, int32_t nsteps, int32_t checkpoint_every
  * offset), so grad_u[k] = sum of grad_ep . d ep / du_k + grad_sc . d(sc1, sc2, sc3) / du_k. */
 int bromoc_table_vjp(int32_t n, const double *r, const double *grad_ep, const double *grad_sc,
                      double *grad_u);
+/* The same product on the context's device, one thread per knot; bit-identical to
+ * bromoc_table_vjp. */
+int bromoc_table_vjp_device(bromoc_ctx *ctx, int32_t n, const double *r, const double *grad_ep,
+                            const double *grad_sc, double *grad_u);
+/* Rebuild ntab tables of the context's pair types on its device from new samples
+ * (SPLINEPOT, splinepot.f90:19-96, the construction bromoc_build_table runs on the host,
+ * bit-identical), in place: table t is pair type is[t] with n[t] samples (its knot count
+ * at bromoc_create) at r, u (all tables' samples concatenated); its Coulomb tail keeps
+ * the pair's chr and fct.  The potential-fitting loop's update of the knot values, with
+ * no new context and no host copy of the knots.  No counterpart in the reference, which
+ * builds its tables once per input deck. */
+int bromoc_update_tables(bromoc_ctx *ctx, int32_t ntab, const int32_t *is, const int32_t *n,
+                         const double *r, const double *u);
 
 /* ------------------------------------------------------------------ replica batches
  * Many independent ion systems (no CG-DNA, no reaction-field grids) that share one

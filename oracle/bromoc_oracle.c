@@ -96,12 +96,17 @@ static void *dupmem(const void *p, size_t n) {
     return q;
 }
 
-static double pow3(double a) { double a2 = a * a; return a2 * a; }
-static double pow6(double a) { double a2 = a * a; double a4 = a2 * a2; return a4 * a2; }
-static double pow5(double a) { double a2 = a * a; double a4 = a2 * a2; return a4 * a; }
-/* x**n for a constant integer n, evaluated by binary powering (square-and-multiply from
- * the low bit), the order in which the Fortran compiler expands integer powers. */
+/* x**n for a constant integer n as the reference's compiler (amdflang, the one that builds
+ * oracle/_ref) expands it: exponents up to 8 multiplied out left to right (x*x*...*x),
+ * larger ones by binary powering (square-and-multiply from the low bit).  Measured on
+ * 2000 random x for every n in 1..14. */
 static double powi(double x, unsigned n) {
+    if (n == 0) return 1.0;
+    if (n <= 8) {
+        double r = x;
+        for (unsigned k = 1; k < n; k++) r = r * x;
+        return r;
+    }
     double r = 0.0, sq = x;
     int have = 0;
     while (n) {
@@ -111,6 +116,9 @@ static double powi(double x, unsigned n) {
     }
     return have ? r : 1.0;
 }
+static double pow3(double a) { return powi(a, 3); }
+static double pow6(double a) { return powi(a, 6); }
+static double pow5(double a) { return powi(a, 5); }
 static double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); } /* SIGN(a,b) */
 
 /* nindex.f90:19-35 through the host-built pair table */
@@ -299,6 +307,62 @@ EXPORT int orc_build_table(int32_t n, const double *xx, const double *yy, int32_
     pair->efpot = 1;
     pair->chr = chr;
     pair->fct = fct;
+    return BROMOC_OK;
+}
+
+/* DISCRETIZE, discretize.f90:19-120: the table of a pair from its analytic parameters
+ * (fct/r when charged, LJ, SRPMF inside rth) sampled at x = k*res, k = nnp..mnp; sets ep
+ * and sc(1:3) (dmi, dm2 are the EFPOT command's, shell_simul.f90:1850-1852) */
+EXPORT int orc_discretize(const bromoc_pair *p, int32_t nnp, int32_t mnp, double res, double rth,
+                          double srpx, double srpk, double srpy, double *ep, bromoc_pair *out) {
+    if (!p || !ep || !out || mnp - nnp + 1 < 3 || nnp < 1 || !(res > 0.0)) return BROMOC_E_ARG;
+    int nn = 1 - nnp + mnp;
+    double *xx = (double *)malloc(sizeof(double) * nn), *yy = (double *)malloc(sizeof(double) * nn);
+    double kf = p->chr ? p->fct : 0.0;
+    double d1 = 0.0;
+    for (int k = nnp; k <= mnp; k++) {
+        double x1 = k * res;
+        double x2 = 1.0 / x1;
+        double scal = kf * x2;
+        double scald = -kf * powi(x2, 2);
+        if (p->lj) {
+            double dist6 = powi(p->sgp2, 3) * powi(x2, 6);
+            double dist12 = powi(dist6, 2);
+            scal = scal + p->epp4 * (dist12 - dist6);
+            scald = scald + p->epp4 * (dist6 - 2.0 * dist12) * 6.0 * x2;
+        }
+        if (p->srpmf) {
+            if (powi(x1, 2) <= rth) {
+                double cc1 = exp((p->c1 - x1) * p->c2);
+                double cc2 = cos(p->c3 * PI * (p->c1 - x1));
+                double cc3 = powi(p->c1 * x2, 6);
+                double cc0 = p->c0 * cc1 * cc2 + p->c4 * cc3;
+                double fdf = 0.0, fdv = 0.0;
+                if (x1 >= srpx) {
+                    fdf = exp(-srpk * (x1 - srpx)) - srpy;
+                    fdv = cc0;
+                    cc0 = fdv * fdf;
+                }
+                scal = scal + cc0;
+                double cc4 = (-p->c2 * cc2 + p->c3 * PI * sin(p->c3 * PI * (p->c1 - x1))) * p->c0 * cc1 -
+                             6.0 * p->c4 * cc3 * x2;
+                if (x1 >= srpx) cc4 = cc4 * fdf - fdv * srpk * (fdf + srpy);
+                scald = scald + cc4;
+            }
+        }
+        xx[k - nnp] = x1;
+        yy[k - nnp] = scal;
+        if (k == nnp) d1 = scald;
+    }
+    out->sc3 = (yy[nn - 1] * xx[nn - 1] - kf) * powi(xx[nn - 1], 5);
+    double d2 = -kf / powi(xx[nn - 1], 2) - 6.0 * out->sc3 / powi(xx[nn - 1], 7);
+    double dd2 = 2.0 * kf / powi(xx[nn - 1], 3) + 42.0 * out->sc3 / powi(xx[nn - 1], 8);
+    squarespline(nn, xx, yy, ep, d1, d2, dd2);
+    out->sc1 = -d1 * powi(xx[0], 13) / 12.0;
+    double y0 = ep[0] + xx[0] * (ep[1] + xx[0] * ep[2]);
+    out->sc2 = y0 - out->sc1 / powi(xx[0], 12);
+    free(xx);
+    free(yy);
     return BROMOC_OK;
 }
 

@@ -76,6 +76,8 @@ def oracle_lib():
             ("orc_metropolis", [vp, C.c_int32, C.c_double, C.c_double, _dp, C.c_int64,
                                 P(C.c_int64), _ip, _dp], C.c_int),
             ("orc_philox_gauss3", [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64, _dp], None),
+            ("orc_discretize", [P(abi.Pair), C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                C.c_double, C.c_double, C.c_double, _dp, P(abi.Pair)], C.c_int),
             ("orc_set_analysis", [vp, P(abi.Analysis)], C.c_int),
             ("orc_analyze", [vp], C.c_int),
             ("orc_get_analysis", [vp, _dp, _dp, _dp, _ip, P(C.c_int64), P(C.c_int64), _dp,
@@ -280,6 +282,8 @@ def ref_lib(path=REF_SO):
             ("ref_get_ibuffer", [_ip], None),
             ("ref_grand", [C.c_int32, _dp, C.c_int64, P(C.c_int64), _ip, _ip, C.c_int32,
                            C.c_int64, _dp], None),
+            ("ref_discretize", [P(abi.Pair), C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                C.c_double, C.c_double, C.c_double, _dp, P(abi.Pair)], None),
             ("ref_build_table", [C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp,
                                  P(abi.Pair)], None),
             ("ref_spline", [C.c_int32, _dp, _dp, _dp, _dp, _dp], None),
@@ -340,6 +344,26 @@ def oracle_spline(x, y):
 def ref_spline(x, y):
     """(b, c, d) from the reference's own SPLINE."""
     return _spline_with(ref_lib().ref_spline, x, y)
+
+
+def oracle_discretize(pair, nnp, mnp, res, rth=0.0, srpx=0.0, srpk=0.0, srpy=0.0):
+    """DISCRETIZE restated: (ep [3n], sc1, sc2, sc3) for an abi.Pair's parameters."""
+    ep = np.zeros(3 * (mnp - nnp + 1))
+    out = abi.Pair()
+    rc = oracle_lib().orc_discretize(C.byref(pair), nnp, mnp, res, rth, srpx, srpk, srpy,
+                                     _p(ep), C.byref(out))
+    if rc:
+        raise RuntimeError(f"orc_discretize {rc}")
+    return ep, (out.sc1, out.sc2, out.sc3)
+
+
+def ref_discretize(pair, nnp, mnp, res, rth=0.0, srpx=0.0, srpk=0.0, srpy=0.0):
+    """The reference's DISCRETIZE (oracle/_ref)."""
+    ep = np.zeros(3 * (mnp - nnp + 1))
+    out = abi.Pair()
+    ref_lib().ref_discretize(C.byref(pair), nnp, mnp, res, rth, srpx, srpk, srpy, _p(ep),
+                             C.byref(out))
+    return ep, (out.sc1, out.sc2, out.sc3)
 
 
 def ref_table_builder(n, r, u, chr_, fct):

@@ -889,6 +889,43 @@ pout%nknots = n; pout%efpot = 1; pout%chr = chr; pout%fct = fctv
 deallocate(xs, ys)
 end subroutine
 
+! DISCRETIZE (discretize.f90:19-120) of one pair type from its analytic parameters into a
+! scratch slot: samples at k*res for k = nnp..mnp
+subroutine ref_discretize(pin, nnp, mnp, resv, rthv, srpxv, srpkv, srpyv, ep_out, pout) &
+    bind(C, name='ref_discretize')
+use ref_harness
+use efpmod
+use extramod
+use grandmod
+implicit none
+type(c_pair), intent(in) :: pin
+integer(c_int32_t), value :: nnp, mnp
+real(c_double), value :: resv, rthv, srpxv, srpkv, srpyv
+real(c_double), intent(out) :: ep_out(3, *)
+type(c_pair), intent(inout) :: pout
+integer :: n, nxf1
+n = mnp - nnp + 1
+if (allocated(ep)) deallocate(ep)
+if (allocated(nxi)) deallocate(nxi, dmi, dm2, sc)
+if (allocated(fct)) deallocate(fct, Qchr)
+if (allocated(epp4)) deallocate(epp4, sgp2)
+if (allocated(c0)) deallocate(c0, c1, c2, c3, c4)
+if (allocated(Qlj)) deallocate(Qlj, Qsrpmfi, Qefpot)
+allocate(ep(3, n), nxi(1), dmi(1), dm2(2,1), sc(3,1), fct(1), Qchr(1))
+allocate(epp4(1), sgp2(1), c0(1), c1(1), c2(1), c3(1), c4(1), Qlj(1), Qsrpmfi(1), Qefpot(1))
+nxi(1) = 1
+nxf1 = n
+res = resv; ires = 1.0/resv
+rth = rthv; srpx = srpxv; srpk = srpkv; srpy = srpyv
+fct(1) = pin%fct; Qchr(1) = pin%chr /= 0
+epp4(1) = pin%epp4; sgp2(1) = pin%sgp2
+c0(1) = pin%c0; c1(1) = pin%c1; c2(1) = pin%c2; c3(1) = pin%c3; c4(1) = pin%c4
+Qlj(1) = pin%lj /= 0; Qsrpmfi(1) = pin%srpmf /= 0; Qefpot(1) = .true.
+call discretize(1, nnp, mnp, nxf1)
+ep_out(1:3, 1:n) = ep(1:3, 1:n)
+pout%sc1 = sc(1,1); pout%sc2 = sc(2,1); pout%sc3 = sc(3,1)
+end subroutine
+
 ! ------------------------------------------------------------- map files
 ! C string -> Fortran character
 subroutine c2f_path(cpath, fpath)
