@@ -37,7 +37,8 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// integer powers in the order the Fortran compiler expands them (binary powering)
+// small integer powers by repeated squaring (the step kernels agree with the reference to
+// rounding, not bitwise; the bit-exact table construction has its own, tables.h)
 __device__ __forceinline__ double pow3(double a) { double a2 = a * a; return a * a2; }
 __device__ __forceinline__ double pow5(double a) { double a2 = a * a; double a4 = a2 * a2; return a * a4; }
 __device__ __forceinline__ double pow6(double a) { double a2 = a * a; double a4 = a2 * a2; return a2 * a4; }
