@@ -173,3 +173,14 @@ def test_batch_create_validation():
     with pytest.raises(abi.BromocError) as ei:
         Batch(sh, reps)
     assert ei.value.code == abi.E_ARG
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """One GPU per box: bench.py --gpus N > 1 exits with a message instead of running."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
