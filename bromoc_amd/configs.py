@@ -729,6 +729,19 @@ def with_membrane(s: System, thick=20.0, zmemb=0.0, epsm=2.0, volts=0.1, debye=9
     return s
 
 
+def with_debyhyb(s: System, debye=9.6, outside=()):
+    """The DNA command's DEBYHYB option (shell_simul.f90:820-846): free DNA (DNAFREE) whose
+    P-P Coulomb pairs are Debye-screened (length `debye`) when both sites lie outside the
+    box (energy.mono.f90:439-441); `outside` lists sites moved out past the box's +x face."""
+    s = s.copy()
+    s.flags = (s.flags | abi.QDEBYHYB | abi.QDNAFREE) & ~abi.QDEBY
+    s.kappa, s.ikappa = debye, 1.0 / debye
+    s.x = s.x.copy()
+    for k, i in enumerate(outside):
+        s.x[i] = s.box[1] + 1.0 + 0.7 * k
+    return s
+
+
 def with_mmij(s: System, npol=(4, 4, 5), region=None, rfscal=1.0, seed=0, strength=1e-4):
     """MMIJ reaction field in a rectangular region (shell_simul.f90:2780-2831, READMMIJ,
     RECT_NORM): a region of the given widths centred at 0 (default: 80 % of the box), the

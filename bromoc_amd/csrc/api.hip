@@ -30,7 +30,7 @@ constexpr uint32_t SUPPORTED = BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BR
                                BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR |
                                BROMOC_QRFPSIN | BROMOC_QBSPLINE | BROMOC_QPROXDIFF |
                                BROMOC_QNOTRANS | BROMOC_QAPFOR | BROMOC_QCONTRANS |
-                               BROMOC_QMEMB | BROMOC_QMMIJ;
+                               BROMOC_QMEMB | BROMOC_QMMIJ | BROMOC_QDEBYHYB;
 
 // constamod.f90:21-32 (binary64)
 double celec_value() {
@@ -322,6 +322,11 @@ int validate(const bromoc_system *s, std::string &msg) {
     if ((s->flags & BROMOC_QMEMB) &&
         (!s->ampl1 || !s->p1 || !s->ampl2 || !s->p2 || !s->rcylinder || !(s->zmemb2 >= s->zmemb1)))
         return bad("MEMBRANE parameters");
+    // hybrid Debye (shell_simul.f90:820-846): not with plain Debye, only for free DNA
+    if ((s->flags & BROMOC_QDEBYHYB) &&
+        ((s->flags & BROMOC_QDEBY) || !(s->flags & BROMOC_QDNAFREE) || !(s->flags & BROMOC_QNUCL) ||
+         !(s->ikappa > 0.0)))
+        return bad("DEBYHYB needs DNAFREE, no DEBY and a Debye length");
     if (s->flags & BROMOC_QMMIJ) {
         if (s->mmij_shape == 2) {
             msg = "the SPHERE reaction field (sphe_rf1.f90) is not supported";

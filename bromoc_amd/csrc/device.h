@@ -27,6 +27,7 @@ constexpr uint32_t QPHIX = 1u << 4, QPHIV = 1u << 5, QSVDW = 1u << 6, QSOLV = 1u
 constexpr uint32_t QDEBY = 1u << 8, QCOUNTION = 1u << 9, QDNAFREE = 1u << 10;
 constexpr uint32_t QBSPLINE = 1u << 18, QPROXDIFF = 1u << 21, QNOTRANS = 1u << 22;
 constexpr uint32_t QAPFOR = 1u << 23, QCONTRANS = 1u << 24, QMEMB = 1u << 16;
+constexpr uint32_t QDEBYHYB = 1u << 20;
 
 constexpr double PI = 3.14159265358979323846264338327950288419716939937510;
 constexpr double RSMALL = 1E-3;
@@ -634,13 +635,25 @@ __device__ constexpr int ND_CAT[ND] = {E_BOND, E_ANG, E_DIHE, E_STACK, E_BP, E_E
                                        E_GSBPA, E_VDWGD};
 enum { D_BOND = 0, D_ANG, D_DIHE, D_STACK, D_BP, D_EX, D_QQ, D_SOLV, D_GSBPA, D_VDWGD };
 
+// OUTBOX (grandmod.f90:74-97) of the box system: outside lx2m..lx2p, ly2m..ly2p, lz2m..lz2p
+__device__ __forceinline__ bool outbox(const StepParams &P, const V3 &r) {
+    return r.x < P.lx2m || r.x > P.lx2p || r.y < P.ly2m || r.y > P.ly2p || r.z < P.lz2m ||
+           r.z > P.lz2p;
+}
+
+// Debye screening of a P-P pair (energy.mono.f90:439-441): always with QDEBY; with the
+// hybrid option only when both sites are outside the box
+__device__ __forceinline__ bool qq_debye(const StepParams &P, const V3 &ra, const V3 &rb) {
+    return (P.flags & QDEBY) || ((P.flags & QDEBYHYB) && outbox(P, ra) && outbox(P, rb));
+}
+
 // A two-site DNA term (bond, stacking, base pair, excluded volume, P-P, solvent) incident
 // on site `me` in role `role` (0: site a, 1: site b), given v1 = r_b - r_a, the term's
 // list value (equilibrium distance or sigma) and, for a base pair, whether site a is a C
 // or G base: force on `me` (added to f) and the term's energy (added to e when role == 0).
 // energy.mono.f90:205-232, 335-507.
 __device__ inline void dna_pair(const StepParams &P, int kind, int role, V3 v1, double par,
-                                bool wide_bp, bool free_me, V3 &f, double *e) {
+                                bool wide_bp, bool deby, bool free_me, V3 &f, double *e) {
     const double eps = P.epsnuc;
     if (kind == T_BOND) {
         double dist = sqrt(dot3(v1, v1));
@@ -686,7 +699,7 @@ __device__ inline void dna_pair(const StepParams &P, int kind, int role, V3 v1, 
     } else if (kind == T_QQ) {
         double dist = sqrt(dot3(v1, v1));
         double idist = 1.0 / dist;
-        if (P.flags & QDEBY) {
+        if (deby) {
             double idistkd = exp(-dist * P.ikappa);
             if (role == 0) e[D_QQ] += P.fctn * idist * idistkd;
             de = P.fctn * (dist + P.kappa) * pow3(idist) * P.ikappa * idistkd;
@@ -716,11 +729,13 @@ __device__ inline void dna_term(const StepParams &P, uint32_t w, bool free_me, V
     const int32_t *L = P.lst[kind];
     if (kind != T_ANGLE && kind != T_DIHE) {
         int a = L[t], b = L[n + t];
-        V3 v1 = sub3(ld3(P, b), ld3(P, a));
+        const V3 ra = ld3(P, a), rb = ld3(P, b);
+        V3 v1 = sub3(rb, ra);
         const int sl = pair_lval_slot(kind);
         const double par = sl >= 0 ? P.lval[sl][t] : 0.0;
         const int na = kind == T_BP ? P.namsite[a] : 0;
-        dna_pair(P, kind, role, v1, par, na == 5 || na == 4, free_me, f, e);
+        const bool deby = kind == T_QQ && qq_debye(P, ra, rb);
+        dna_pair(P, kind, role, v1, par, na == 5 || na == 4, deby, free_me, f, e);
         return;
     }
     if (kind == T_ANGLE) {

@@ -536,7 +536,8 @@ __global__ __launch_bounds__(NW * WAVE) void k_dna(StepParams P) {
         const int role = (wd >> 27) & 3;
         const V3 ro = ld3(P, (int)((bits >> 32) & 0x7FFFFFFFu));
         const V3 v1 = role == 0 ? sub3(ro, rme) : sub3(rme, ro);   // r_b - r_a
-        dna_pair(P, kind, role, v1, rec.y, (bits >> 63) != 0, free_me, f, e);
+        const bool deby = kind == T_QQ && qq_debye(P, rme, ro);
+        dna_pair(P, kind, role, v1, rec.y, (bits >> 63) != 0, deby, free_me, f, e);
     }
     f.x = wave_sum(f.x);
     f.y = wave_sum(f.y);
@@ -1049,7 +1050,8 @@ __device__ double dna_term_interact(const StepParams &P, uint32_t w) {
         return 28.0 * eps * (1.0 - cos(vardihe));
     }
     int a = L[t], b = L[n + t];
-    V3 v = sub3(ld3(P, b), ld3(P, a));
+    const V3 ra = ld3(P, a), rb = ld3(P, b);
+    V3 v = sub3(rb, ra);
     double dist2 = dot3(v, v);
     if (kind == T_STACK) {
         double sg = P.lval[3][t];
@@ -1071,7 +1073,7 @@ __device__ double dna_term_interact(const StepParams &P, uint32_t w) {
     } else if (kind == T_QQ) {
         double dist = sqrt(dist2);
         double idist = 1.0 / dist;
-        if (!(P.flags & QDEBY)) return P.fctn * idist;
+        if (!qq_debye(P, ra, rb)) return P.fctn * idist;   // interact.f90:323-331
         return P.fctn * idist * exp(-dist * P.ikappa);
     }
     double dist = sqrt(dist2);

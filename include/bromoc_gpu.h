@@ -102,9 +102,10 @@ extern "C" {
 #define BROMOC_QMEMB       (1u << 16)  /* analytic membrane (membrane.f90) */
 #define BROMOC_QMMIJ       (1u << 17)  /* MMIJ reaction field, RECTBOX (rect_rf1/rect_rf0);
                                         * the SPHERE shape (sphe_rf1) is rejected */
+#define BROMOC_QDEBYHYB    (1u << 20)  /* hybrid Debye (Qdebyhyb): P-P pairs screened when
+                                        * both sites are outside the box; needs QDNAFREE */
 /* options of the reference that this path rejects with BROMOC_E_UNSUPPORTED */
 #define BROMOC_QNMCDEN     (1u << 19)  /* per-species phiv maps (Qnmcden) */
-#define BROMOC_QDEBYHYB    (1u << 20)  /* hybrid Debye (Qdebyhyb) */
 #define BROMOC_QSPHERE     (1u << 26)  /* spherical system (fixcoor.f90:24-31) */
 #define BROMOC_QECYL       (1u << 27)  /* elliptic-cylinder system (fixcoor.f90:33-46) */
 

@@ -612,7 +612,13 @@ static uint32_t supported_mask(void) {
            BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV | BROMOC_QDEBY | BROMOC_QCOUNTION |
            BROMOC_QDNAFREE | BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR | BROMOC_QRFPSIN |
            BROMOC_QBSPLINE | BROMOC_QPROXDIFF | BROMOC_QNOTRANS | BROMOC_QAPFOR |
-           BROMOC_QCONTRANS | BROMOC_QMEMB | BROMOC_QMMIJ;
+           BROMOC_QCONTRANS | BROMOC_QMEMB | BROMOC_QMMIJ | BROMOC_QDEBYHYB;
+}
+
+/* OUTBOX, grandmod.f90:74-97 (box system) */
+static int outbox(const bromoc_system *s, double xi, double yi, double zi) {
+    return xi < s->lx2m || xi > s->lx2p || yi < s->ly2m || yi > s->ly2p || zi < s->lz2m ||
+           zi > s->lz2p;
 }
 
 EXPORT int orc_check_flags(uint32_t flags) {
@@ -1564,7 +1570,12 @@ EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
                 double dist = sqrt(dot3(v1, v1));
                 double idist = 1.0 / dist;
                 double de;
-                if (fl & BROMOC_QDEBY) {
+                /* energy.mono.f90:439-441: hybrid Debye screens pairs of two sites outside */
+                int deby = (fl & BROMOC_QDEBY) != 0;
+                if ((fl & BROMOC_QDEBYHYB) && outbox(s, c->x[a], c->y[a], c->z[a]) &&
+                    outbox(s, c->x[b], c->y[b], c->z[b]))
+                    deby = 1;
+                if (deby) {
                     double idistkd = exp(-dist * s->ikappa);
                     E.eqq = E.eqq + s->fctn * idist * idistkd;
                     de = s->fctn * (dist + s->kappa) * pow3(idist) * s->ikappa * idistkd;
@@ -2284,7 +2295,11 @@ EXPORT int orc_interact(orc_ctx *c, double xj, double yj, double zj, int32_t jty
                 double dx = c->x[b] - c->x[a], dy = c->y[b] - c->y[a], dz = c->z[b] - c->z[a];
                 double dist = sqrt(dx * dx + dy * dy + dz * dz);
                 double idist = 1.0 / dist;
-                if (!(fl & BROMOC_QDEBY)) dener = dener + s->fctn * idist;
+                int deby = (fl & BROMOC_QDEBY) != 0;   /* interact.f90:323-331 */
+                if ((fl & BROMOC_QDEBYHYB) && outbox(s, c->x[a], c->y[a], c->z[a]) &&
+                    outbox(s, c->x[b], c->y[b], c->z[b]))
+                    deby = 1;
+                if (!deby) dener = dener + s->fctn * idist;
                 else dener = dener + s->fctn * idist * exp(-dist * s->ikappa);
             }
             if (fl & BROMOC_QSOLV) {

@@ -78,7 +78,7 @@ integer(c_int32_t), parameter :: F_PAR=1, F_NUCL=2, F_NONBOND=4, F_NOBOND=8, F_P
      F_SVDW=64, F_SOLV=128, F_DEBY=256, F_COUNTION=512, F_DNAFREE=1024, &
      F_RFPAR=33554432, F_DIFFUSE=268435456, F_PROFILE=536870912, F_RFPSIN=1073741824, &
      F_BSPLINE=262144, F_PROXDIFF=2097152, F_NOTRANS=4194304, F_APFOR=8388608, &
-     F_CONTRANS=16777216, F_MEMB=65536, F_MMIJ=131072
+     F_CONTRANS=16777216, F_MEMB=65536, F_MMIJ=131072, F_DEBYHYB=1048576
 integer :: site_names_code(0:5)
 
 contains
@@ -169,7 +169,7 @@ Qrfpar = .false.
 Qtrln = .not. flag(s%flags, F_BSPLINE)
 Qnmcden = .false.
 Qproxdiff = flag(s%flags, F_PROXDIFF)
-Qdebyhyb = .false.
+Qdebyhyb = flag(s%flags, F_DEBYHYB)
 Qapfor = flag(s%flags, F_APFOR)
 Qcontrans = flag(s%flags, F_CONTRANS)
 Qwarn = .false.

@@ -63,7 +63,7 @@ def _plain():
 
 
 def test_create_rejects_unsupported_options():
-    for flag in (abi.QNMCDEN, abi.QSPHERE, abi.QECYL, abi.QDEBYHYB):
+    for flag in (abi.QNMCDEN, abi.QSPHERE, abi.QECYL):
         s = _plain()
         s.flags |= flag
         rc, _ = _create(s)
@@ -107,6 +107,9 @@ def test_create_rejects_inconsistent_options():
     assert _create(s)[0] == abi.E_ARG
     s = _plain()
     s.flags |= abi.QMEMB                          # no membrane parameters given
+    assert _create(s)[0] == abi.E_ARG
+    s = _plain()
+    s.flags |= abi.QDEBYHYB                       # needs DNA, DNAFREE and a Debye length
     assert _create(s)[0] == abi.E_ARG
     s = _plain()
     s.flags |= abi.QMMIJ                          # no reaction-field matrix given

@@ -231,3 +231,18 @@ def test_membrane_interact():
         a, b = g.interact(xyz, jt, j, True), o.interact(xyz, jt, j, True)
         assert abs(a - b) <= TOL * max(abs(b), 1.0), (j, a, b)
     assert np.array_equal(g.warn(), o.warn())
+
+
+def test_debyhyb_energy_forces_steps():
+    """Hybrid Debye (energy.mono.f90:439-478): P-P pairs of two sites outside the box
+    screened, the rest plain Coulomb; ENERGY, INTERACT and BD steps."""
+    s0 = pore_small(seed=75, n_ions=120)
+    p_sites = [i for i in range(s0.nsites) if s0.namsite[i] == abi.SITE_P]
+    s = configs.with_debyhyb(s0, debye=9.6, outside=p_sites[:6])
+    g, o, eg, eo = _energy_parity(s)
+    pos, typ = o.positions()
+    for j in p_sites[:8]:
+        a, b = g.interact(pos[j], 0, j, True), o.interact(pos[j], 0, j, True)
+        assert abs(a - b) <= TOL * max(abs(b), 1.0), (j, a, b)
+    _steps_parity(s, nsteps=5)
+    _steps_parity(s, nsteps=40, graphs=True, tol_pos=1e-8)
