@@ -742,6 +742,38 @@ def with_debyhyb(s: System, debye=9.6, outside=()):
     return s
 
 
+def with_nmcden(s: System, seed=0, density=0.15, mode="scatter"):
+    """The PHIV command's NMCDEN option (shell_simul.f90:2720-2750, readphi.f90:176-240):
+    one repulsion map per DNA site name (S, P, Ab, Tb, Cb, Gb, with DNA) and per ion type
+    nold+1..ntype, so every species reads a map of its own.  mode "scatter": the system's map
+    with a seeded scatter of extra wall nodes (density of the grid) -- every particle sees
+    its own map, but the rough maps make BD trajectories chaotic; mode "pore": the wall grown
+    by 0, 3 or 6 nodes across x and y (a per-species pore radius) -- smooth trajectories.
+    SVDW is off (shell_simul.f90:2875)."""
+    s = s.copy()
+    s.flags = (s.flags | abi.QNMCDEN) & ~abi.QSVDW
+    nmaps = (6 if s.flags & abi.QNUCL else 0) + (s.ntype - s.nold if s.flags & abi.QPAR else 0)
+    base = np.asarray(s.phiv.data, np.int8)
+    rng = np.random.default_rng(seed)
+    if mode == "scatter":
+        maps = [base | (rng.random(base.shape) < density).astype(np.int8) for _ in range(nmaps)]
+    else:
+        maps = []
+        for k in range(nmaps):
+            m = base.copy()
+            for _ in range(3 * ((k + int(seed)) % 3)):
+                g = m.copy()                  # one node of wall more along +-x and +-y
+                g[1:] |= m[:-1]
+                g[:-1] |= m[1:]
+                g[:, 1:] |= m[:, :-1]
+                g[:, :-1] |= m[:, 1:]
+                m = g
+            maps.append(m)
+    maps = np.stack(maps)
+    s.phiv = GridData(maps, s.phiv.dcel, s.phiv.xbcen, s.phiv.ybcen, s.phiv.zbcen)
+    return s
+
+
 def with_mmij(s: System, npol=(4, 4, 5), region=None, rfscal=1.0, seed=0, strength=1e-4):
     """MMIJ reaction field in a rectangular region (shell_simul.f90:2780-2831, READMMIJ,
     RECT_NORM): a region of the given widths centred at 0 (default: 80 % of the box), the

@@ -30,7 +30,14 @@ constexpr uint32_t SUPPORTED = BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BR
                                BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR |
                                BROMOC_QRFPSIN | BROMOC_QBSPLINE | BROMOC_QPROXDIFF |
                                BROMOC_QNOTRANS | BROMOC_QAPFOR | BROMOC_QCONTRANS |
-                               BROMOC_QMEMB | BROMOC_QMMIJ | BROMOC_QDEBYHYB;
+                               BROMOC_QMEMB | BROMOC_QMMIJ | BROMOC_QDEBYHYB | BROMOC_QNMCDEN;
+
+// Repulsion maps the phiv grid holds: one, or with NMCDEN (readphi.f90:176-240) six site
+// maps (S, P, Ab, Tb, Cb, Gb) with DNA followed by one per ion type nold+1..ntype.
+static int64_t phiv_maps(const bromoc_system *s) {
+    if (!(s->flags & BROMOC_QNMCDEN)) return 1;
+    return ((s->flags & BROMOC_QNUCL) ? 6 : 0) + ((s->flags & BROMOC_QPAR) ? s->ntype - s->nold : 0);
+}
 
 // constamod.f90:21-32 (binary64)
 double celec_value() {
@@ -268,6 +275,16 @@ int validate(const bromoc_system *s, std::string &msg) {
         if (s->vzmax > s->phiv.zbcen + tz || s->vzmin < s->phiv.zbcen - tz)
             return bad("vzmin/vzmax outside the repulsion map");
         if ((s->flags & BROMOC_QSVDW) && !s->scal) return bad("scal missing");
+    }
+    if (s->flags & BROMOC_QNMCDEN) {
+        if (!(s->flags & BROMOC_QPHIV)) return bad("NMCDEN needs QPHIV");
+        if (s->flags & BROMOC_QSVDW) return bad("SVDW not compatible with NMCDEN (shell_simul.f90:2875)");
+        // shell_simul.f90:2730-2737; a moving ion without PARTICLE would read the map left
+        // by the particle before it (vdwgd1trln.f90:61-80)
+        if (!(s->flags & (BROMOC_QNUCL | BROMOC_QPAR))) return bad("NMCDEN needs NUCLEOTIDES or PARTICLE");
+        if (!(s->flags & BROMOC_QPAR) && s->ntot > s->nsites + s->nfix)
+            return bad("NMCDEN maps of moving ions need PARTICLE");
+        if (phiv_maps(s) < 1) return bad("NMCDEN map count");
     }
     if ((s->flags & BROMOC_QCOUNTION) && (s->cntpts < 0 || (s->cntpts > 0 && !s->zcont)))
         return bad("flux planes");
@@ -607,7 +624,7 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
                      (size_t)s->phix.nclx * s->phix.ncly * s->phix.nclz));
     if (s->flags & BROMOC_QPHIV)
         ACHK(dupload(c, &c->phiv, (const int8_t *)s->phiv.data,
-                     (size_t)s->phiv.nclx * s->phiv.ncly * s->phiv.nclz));
+                     (size_t)phiv_maps(s) * s->phiv.nclx * s->phiv.ncly * s->phiv.nclz));
     std::vector<double> scal(nt1, 1.0);
     if ((s->flags & BROMOC_QSVDW) && s->scal) scal.assign(s->scal, s->scal + nt1);
     ACHK(dupload(c, &c->scal, scal.data(), nt1));

@@ -752,10 +752,10 @@ int bromoc_batch_create(const bromoc_system *s, int32_t nrep, const bromoc_repli
     *out = nullptr;
     // the replica kernels implement pairs, the trilinear grids and DYNAMICS0/1/2 only
     const uint32_t unsupported = BROMOC_QNUCL | BROMOC_QRFPAR | BROMOC_QRFPSIN | BROMOC_QDNAFREE |
-                                 BROMOC_QMEMB | BROMOC_QBSPLINE | BROMOC_QMMIJ;
+                                 BROMOC_QMEMB | BROMOC_QBSPLINE | BROMOC_QMMIJ | BROMOC_QNMCDEN;
     if (s->flags & unsupported) {
         fprintf(stderr, "bromoc_batch_create: replicas are ion-only (no CG-DNA, RFPAR, MMIJ, "
-                        "membrane or B-spline map)\n");
+                        "membrane, B-spline or per-species map)\n");
         return BROMOC_E_UNSUPPORTED;
     }
     if (s->nsites != 0 || !(s->flags & BROMOC_QPAR)) return BROMOC_E_ARG;

@@ -27,7 +27,7 @@ constexpr uint32_t QPHIX = 1u << 4, QPHIV = 1u << 5, QSVDW = 1u << 6, QSOLV = 1u
 constexpr uint32_t QDEBY = 1u << 8, QCOUNTION = 1u << 9, QDNAFREE = 1u << 10;
 constexpr uint32_t QBSPLINE = 1u << 18, QPROXDIFF = 1u << 21, QNOTRANS = 1u << 22;
 constexpr uint32_t QAPFOR = 1u << 23, QCONTRANS = 1u << 24, QMEMB = 1u << 16;
-constexpr uint32_t QDEBYHYB = 1u << 20;
+constexpr uint32_t QDEBYHYB = 1u << 20, QNMCDEN = 1u << 19;
 
 constexpr double PI = 3.14159265358979323846264338327950288419716939937510;
 constexpr double RSMALL = 1E-3;
@@ -223,16 +223,38 @@ __device__ __forceinline__ void spline_cell(const GridGeom &g, double x, double 
     if (iz == g.nz - 1) iz = g.nz - 2;
 }
 
+// First node of particle i's repulsion map (vdwgd1trln.f90:61-80, vdwgd0trln.f90:56-75):
+// one map for all without NMCDEN.  With it, a DNA site reads its site name's map (S, P, Ab,
+// Tb, Cb, Gb: maps 0..5), a moving ion of atnam type itype the map of its type, numbered
+// from the first ion type (nold + 1) after the six site maps when there is DNA; a fixed
+// ion, or an ion without PARTICLE, reads map 0.
+__device__ __forceinline__ size_t vdw_map(const StepParams &P, int i, int itype) {
+    if (!(P.flags & QNMCDEN)) return 0;
+    const size_t ncel3 = (size_t)P.gv.nx * P.gv.ny * P.gv.nz;
+    if ((P.flags & QNUCL) && i < P.nsites) return (size_t)P.namsite[i] * ncel3;
+    if (!(P.flags & QPAR) || i < P.nsites + P.nfix) return 0;
+    return (size_t)(((P.flags & QNUCL) ? 6 : 0) + itype - P.nold - 1) * ncel3;
+}
+
+// The same for a stored particle i (its type from typei).
+__device__ __forceinline__ size_t vdw_map_of(const StepParams &P, int i) {
+    if (!(P.flags & QNMCDEN)) return 0;
+    return vdw_map(P, i, i < P.nsites ? 0 : abs(P.typei[i]));
+}
+
 // Repulsion map of one ENERGY target at (x, y, z), atnam2 type t2, by a whole wavefront:
 // the trilinear stencil of vdwgd1trln.f90:50-161 on lanes 0..7, or with QBSPLINE the
 // 27-point B-spline of vdwgd1spln.f90:50-170 on lanes 0..26 (a node counts only where
 // none of its three factors vanishes with its derivative, vdwgd1spln.f90:106-124).  Adds
 // the lane's force and energy shares; returns, on every lane, whether the map values
-// summed over the stencil reach the wall threshold (warn).
+// summed over the stencil reach the wall threshold (warn).  map: the first node of the
+// target's map (vdw_map).
 __device__ inline bool vdw_grid(const StepParams &P, double x, double y, double z, int t2,
-                                double &fx, double &fy, double &fz, double &e) {
+                                double &fx, double &fy, double &fz, double &e,
+                                size_t map = 0) {
     const int lane = threadIdx.x & (WAVE - 1);
     const GridGeom &g = P.gv;
+    const int8_t *phiv = P.phiv + map;
     double esvdw = P.svdw;
     if (P.flags & QSVDW) esvdw = P.svdw * P.scal[t2];
     double ph = 0.0;
@@ -251,7 +273,7 @@ __device__ inline bool vdw_grid(const StepParams &P, double x, double y, double 
             bspline_m3(1.5 - (yi - l * g.dcel) * g.idcel, bi, dbi);
             bspline_m3(1.5 - (zi - m * g.dcel) * g.idcel, ci, dci);
             if ((ai != 0.0 || dai != 0.0) && (bi != 0.0 || dbi != 0.0) && (ci != 0.0 || dci != 0.0)) {
-                ph = (double)P.phiv[((size_t)k * g.ny + l) * g.nz + m];
+                ph = (double)phiv[((size_t)k * g.ny + l) * g.nz + m];
                 const double phivsv = ph * esvdw;
                 const double prefac = phivsv * g.idcel;
                 fx += dai * bi * ci * prefac;
@@ -267,7 +289,7 @@ __device__ inline bool vdw_grid(const StepParams &P, double x, double y, double 
             double xi, yi, zi;
             int ix, iy, iz;
             grid_cell(g, x, y, z, xi, yi, zi, ix, iy, iz);
-            ph = tri_corner(g, P.phiv, xi, yi, zi, ix, iy, iz, lane, esvdw, fx, fy, fz, e);
+            ph = tri_corner(g, phiv, xi, yi, zi, ix, iy, iz, lane, esvdw, fx, fy, fz, e);
         }
     }
     // map values are integers: the sum is exact in any order
@@ -927,10 +949,11 @@ __device__ __forceinline__ double gety(const DevPair &p, const double4 *__restri
 // (xj, yj, zj) with atnam2 type jt2: called by a whole wavefront, lanes 0..7 take the 8
 // corners; the energy comes back on every lane.  charged: the particle feels the static
 // field with charge chi (an ion, or a P site).  wall: the repulsion corners reach thold8,
-// in which case the repulsion energy is 1e10.
+// in which case the repulsion energy is 1e10.  vmap: the first node of the particle's
+// repulsion map (vdw_map).
 __device__ inline double interact_grids(const StepParams &P, const float *phix, bool charged,
                                         double chi, double xj, double yj, double zj, int jt2,
-                                        bool &wall) {
+                                        bool &wall, size_t vmap = 0) {
     const int lane = threadIdx.x & (WAVE - 1);
     double es = 0.0, ev = 0.0, ph = 0.0;
     bool okv = false;
@@ -969,7 +992,7 @@ __device__ inline double interact_grids(const StepParams &P, const float *phix, 
             bspline_m3(1.5 - (zi - m * g.dcel) * g.idcel, ci, dd);
             const double fi = ai * bi * ci;
             if (ai != 0.0 && bi != 0.0 && fi != 0.0) {
-                ph = (double)P.phiv[((size_t)k * g.ny + l) * g.nz + m];
+                ph = (double)(P.phiv + vmap)[((size_t)k * g.ny + l) * g.nz + m];
                 ev = fi * esvdw * ph;
             }
         }
@@ -988,7 +1011,7 @@ __device__ inline double interact_grids(const StepParams &P, const float *phix, 
             double bi = 1.0 - fabs(gy_ - n2 * g.dcel) * g.idcel;
             double ci = 1.0 - fabs(gz_ - n3 * g.dcel) * g.idcel;
             double fi = ai * bi * ci;
-            ph = (double)P.phiv[((size_t)n1 * g.ny + n2) * g.nz + n3];
+            ph = (double)(P.phiv + vmap)[((size_t)n1 * g.ny + n2) * g.nz + n3];
             ev = fi * esvdw * ph;
         }
     }

@@ -264,7 +264,7 @@ __global__ __launch_bounds__(NW * WAVE) void k_step(StepParams P) {
         }
     }
     if constexpr ((FEAT & F_PHIV) != 0) {
-        if (mine && val_me) warned = vdw_grid(P, xi, yi, zi, t2i, gfx, gfy, gfz, egv);
+        if (mine && val_me) warned = vdw_grid(P, xi, yi, zi, t2i, gfx, gfy, gfz, egv, vdw_map_of(P, i_me));
     }
     if (mine) {
         gfx = wave_sum(gfx);
@@ -680,7 +680,8 @@ __global__ __launch_bounds__(BLOCK) void k_wave(StepParams P) {
             }
         }
     }
-    if constexpr ((FEAT & F_PHIV) != 0) warned = vdw_grid(P, pa.x, pa.y, pa.z, t2i, fx, fy, fz, egv);
+    if constexpr ((FEAT & F_PHIV) != 0)
+        warned = vdw_grid(P, pa.x, pa.y, pa.z, t2i, fx, fy, fz, egv, vdw_map_of(P, i));
     fx = wave_sum(fx);
     fy = wave_sum(fy);
     fz = wave_sum(fz);
@@ -1111,7 +1112,8 @@ __global__ __launch_bounds__(BLOCK) void k_interact(InteractParams Q) {
         const bool besite = (P.flags & QNUCL) && j < ns && P.namsite[j] == 1;
         const double chi = beion ? P.cg[Q.jtype] : P.cgnuc;
         bool wall = false;
-        const double eg = interact_grids(P, P.phix, beion || besite, chi, xj, yj, zj, jt2, wall);
+        const double eg = interact_grids(P, P.phix, beion || besite, chi, xj, yj, zj, jt2, wall,
+                                         vdw_map(P, j, Q.jtype));
         if (tid == 0) {
             e += eg;
             if (wall && Q.alert) atomicAdd(&P.warn[jt2], 1);

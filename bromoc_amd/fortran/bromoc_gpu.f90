@@ -296,7 +296,7 @@ subroutine bromoc_gpu_init(seed)
   real(c_double), allocatable, target :: mbn(:), mmm(:)
   real(c_double), allocatable, target :: aaf(:), ckx(:), cky(:), ckz(:), ccx(:), ccy(:), ccz(:)
   integer :: nc
-  integer :: nsp, ngrid
+  integer :: nsp, ngrid, nmaps
   integer(c_int8_t), allocatable, target :: gv(:)
   integer :: i, a, b, nk, nt1, ntc, nindex, cap
   integer(c_uint32_t) :: fl
@@ -433,6 +433,17 @@ subroutine bromoc_gpu_init(seed)
     s%phix = bgpu_grid(nclx1, ncly1, nclz1, 0, dcel1, xbcen1, ybcen1, zbcen1, c_loc(gx))
   endif
   if (Qphiv) then
+    ! with NMCDEN the library reads the six site maps (with DNA) and one map per ion type,
+    ! contiguous; readphi.f90:188-192 allocates phiv over the last ion map only when
+    ! PARTICLE is on, so a shorter phiv is refused rather than read past its end
+    nmaps = 1
+    if (Qnmcden) then
+      nmaps = 0
+      if (Qnucl) nmaps = 6
+      if (Qpar) nmaps = nmaps + ntype - nold
+    endif
+    if (size(phiv, kind=8) /= int(nmaps, 8) * nclx2 * ncly2 * nclz2) &
+      call bgpu_check(-1_c_int, 'phiv size (NMCDEN maps)')
     allocate(gv(size(phiv)))
     gv = phiv
     s%phiv = bgpu_grid(nclx2, ncly2, nclz2, 0, dcel2, xbcen2, ybcen2, zbcen2, c_loc(gv))

@@ -34,15 +34,15 @@ def kbt_of(temp):
 
 @dataclass
 class GridData:
-    data: np.ndarray          # (nclx, ncly, nclz), float32 (phix) or int8 (phiv)
-    dcel: float
+    data: np.ndarray          # (nclx, ncly, nclz), float32 (phix) or int8 (phiv);
+    dcel: float               # per-species phiv maps (NMCDEN): (nmaps, nclx, ncly, nclz)
     xbcen: float = 0.0
     ybcen: float = 0.0
     zbcen: float = 0.0
 
     @property
     def shape(self):
-        return self.data.shape
+        return self.data.shape[-3:]
 
     def tran(self):
         n = self.data.shape
@@ -231,7 +231,7 @@ class System:
                 continue
             d = np.ascontiguousarray(g.data, dtype=dtype)
             keep.append(d)
-            cg.nclx, cg.ncly, cg.nclz = d.shape
+            cg.nclx, cg.ncly, cg.nclz = d.shape[-3:]
             cg.dcel, cg.xbcen, cg.ybcen, cg.zbcen = g.dcel, g.xbcen, g.ybcen, g.zbcen
             cg.data = d.ctypes.data
         s.svdw, s.thold8, s.vzmin, s.vzmax = self.svdw, self.thold8, self.vzmin, self.vzmax

@@ -104,8 +104,12 @@ extern "C" {
                                         * the SPHERE shape (sphe_rf1) is rejected */
 #define BROMOC_QDEBYHYB    (1u << 20)  /* hybrid Debye (Qdebyhyb): P-P pairs screened when
                                         * both sites are outside the box; needs QDNAFREE */
+#define BROMOC_QNMCDEN     (1u << 19)  /* per-species phiv maps (Qnmcden): phiv.data holds
+                                        * nclx*ncly*nclz bytes per map, the six site maps
+                                        * S, P, Ab, Tb, Cb, Gb (with QNUCL) then one per ion
+                                        * type nold+1..ntype (with QPAR); needs QPHIV, not
+                                        * with QSVDW */
 /* options of the reference that this path rejects with BROMOC_E_UNSUPPORTED */
-#define BROMOC_QNMCDEN     (1u << 19)  /* per-species phiv maps (Qnmcden) */
 #define BROMOC_QSPHERE     (1u << 26)  /* spherical system (fixcoor.f90:24-31) */
 #define BROMOC_QECYL       (1u << 27)  /* elliptic-cylinder system (fixcoor.f90:33-46) */
 
@@ -134,7 +138,7 @@ typedef struct bromoc_grid {
     int32_t     nclx, ncly, nclz, reserved;
     double      dcel;
     double      xbcen, ybcen, zbcen;
-    const void *data;    /* phix: float[ncl^3]; phiv: int8_t[ncl^3] */
+    const void *data;    /* phix: float[ncl^3]; phiv: int8_t[ncl^3] per map */
 } bromoc_grid;
 
 /* Complete ENERGY/DYNAMICS state.  Pointers are read during bromoc_create only. */

@@ -14,8 +14,8 @@
  *   getyd/gety, squarespline/splinepot, ENERGY, STATICF1, VDWGD1TRLN, VDWGD1SPLN,
  *   APFOR/CONTRANS, the PROXDIFF records, DYNAMICS0/0NUC/1/2/3, NOTRANS, FIXCOOR, COUNTIONS,
  *   SPLINE/SEVAL, RFPARION/RFPARIONJ, INTERACT, STATICF0, VDWGD0TRLN, VDWGD0SPLN, GRAND,
- *   METROPOLIS, MEMBRANE, RECT_RF1/RECT_RF0: against the reference Fortran compiled by
- *   oracle/Makefile (oracle/_ref), plus closed forms and invariants (Newton's third law,
+ *   METROPOLIS, MEMBRANE, RECT_RF1/RECT_RF0, the Qnmcden map choice: against the reference
+ *   Fortran compiled by oracle/Makefile (oracle/_ref), plus closed forms and invariants (Newton's third law,
  *   F = -grad E by central differences, ENERGY = 1/2 sum INTERACT, exact trilinear
  *   interpolation of linear fields, the B-spline's reproduction of a linear map,
  *   closed-form BD step, acceptance rule, METROPOLIS energy bookkeeping, the reaction-field
@@ -491,6 +491,9 @@ EXPORT orc_ctx *orc_create(const bromoc_system *s) {
     }
     if (s->flags & BROMOC_QPHIV) {
         size_t n = (size_t)s->phiv.nclx * s->phiv.ncly * s->phiv.nclz;
+        if (s->flags & BROMOC_QNMCDEN)   /* readphi.f90:176-240 */
+            n *= (size_t)(((s->flags & BROMOC_QNUCL) ? 6 : 0) +
+                          ((s->flags & BROMOC_QPAR) ? s->ntype - s->nold : 0));
         c->phiv = (int8_t *)dupmem(s->phiv.data, n);
     }
     c->warn = (int32_t *)calloc(s->nttyp + 2, sizeof(int32_t));
@@ -612,7 +615,7 @@ static uint32_t supported_mask(void) {
            BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV | BROMOC_QDEBY | BROMOC_QCOUNTION |
            BROMOC_QDNAFREE | BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR | BROMOC_QRFPSIN |
            BROMOC_QBSPLINE | BROMOC_QPROXDIFF | BROMOC_QNOTRANS | BROMOC_QAPFOR |
-           BROMOC_QCONTRANS | BROMOC_QMEMB | BROMOC_QMMIJ | BROMOC_QDEBYHYB;
+           BROMOC_QCONTRANS | BROMOC_QMEMB | BROMOC_QMMIJ | BROMOC_QDEBYHYB | BROMOC_QNMCDEN;
 }
 
 /* OUTBOX, grandmod.f90:74-97 (box system) */
@@ -692,7 +695,31 @@ static void staticf1(orc_ctx *c, int qforces, double *egsbpa) {
     }
 }
 
-/* VDWGD1TRLN, vdwgd1trln.f90:19-161 (single map, Qnmcden off) */
+/* Qnmcden map choice of particle i (vdwgd1trln.f90:61-80, vdwgd0trln.f90:56-75,
+ * vdwgd1spln.f90:61-80, vdwgd0spln.f90:58-77): sets *ifir to the first node of the map and
+ * returns 1, or returns 0 where none of the reference's branches applies (the reference
+ * then keeps ifir as it was).  Maps: S, P, Ab, Tb, Cb, Gb with DNA, then ion types from
+ * nold+1 (the reference's inuc+1 or 2*inuc+1 for one or two strands: nold = istrs*inuc). */
+static int nmcden_map(const orc_ctx *c, int i, int itype, size_t ncel3, size_t *ifir) {
+    const bromoc_system *s = &c->s;
+    int qnucl = (s->flags & BROMOC_QNUCL) != 0, qpar = (s->flags & BROMOC_QPAR) != 0;
+    if (qnucl && i < s->nsites) {
+        *ifir = (size_t)c->namsite[i] * ncel3;   /* BROMOC_SITE_S..GB = 0..5 */
+        return 1;
+    }
+    if (qnucl && qpar && i >= s->nsites + s->nfix) {
+        int numb = itype - (s->nold + 1);
+        *ifir = (size_t)(6 + numb) * ncel3;
+        return 1;
+    }
+    if (!qnucl && qpar && i >= s->nfix) {
+        *ifir = (size_t)(itype - 1) * ncel3;
+        return 1;
+    }
+    return 0;
+}
+
+/* VDWGD1TRLN, vdwgd1trln.f90:19-161 */
 static void vdwgd1trln(orc_ctx *c, int qforces, double *evdwgd) {
     const bromoc_system *s = &c->s;
     const bromoc_grid *g = &s->phiv;
@@ -703,6 +730,7 @@ static void vdwgd1trln(orc_ctx *c, int qforces, double *evdwgd) {
            tranz = 0.5 * (g->nclz - 1) * dcel;
     double esvdw = s->svdw;
     int qsvdw = (s->flags & BROMOC_QSVDW) != 0;
+    size_t ncel3 = (size_t)g->nclx * ncyz, ifir = 0;
     *evdwgd = 0.0;
     for (int i = 0; i < s->ntot; i++) {
         if (!(i < s->nsites || i >= s->nsites + s->nfix)) continue;
@@ -714,7 +742,8 @@ static void vdwgd1trln(orc_ctx *c, int qforces, double *evdwgd) {
         double yi = c->y[i] + trany - g->ybcen;
         double zi = c->z[i] + tranz - g->zbcen;
         int itype2 = (i < s->nsites) ? c->typtyp[i] : c->nwtype[abs(c->typei[i])];
-        if (qsvdw) esvdw = s->svdw * c->scal[itype2];
+        if (s->flags & BROMOC_QNMCDEN) nmcden_map(c, i, abs(c->typei[i]), ncel3, &ifir);
+        else if (qsvdw) esvdw = s->svdw * c->scal[itype2];
         double vdwfx = 0.0, vdwfy = 0.0, vdwfz = 0.0;
         int ix = (int)(xi * idcel), iy = (int)(yi * idcel), iz = (int)(zi * idcel);
         if (ix == g->nclx - 1) ix = g->nclx - 2;
@@ -735,7 +764,7 @@ static void vdwgd1trln(orc_ctx *c, int qforces, double *evdwgd) {
                     ci = 1.0 - fabs(ci) * idcel;
                     double fi = ai * bi * ci;
                     size_t in3 = (size_t)n1 * ncyz + (size_t)n2 * g->nclz + n3;
-                    double phis = (double)c->phiv[in3];
+                    double phis = (double)c->phiv[in3 + ifir];
                     phisum = phisum + phis;
                     if (qforces) {
                         double prefac = phis * esvdw * idcel;
@@ -771,7 +800,7 @@ static double dm3(double x) {
 /* NINT: nearest integer, halves away from zero */
 static int nint(double a) { return (int)round(a); }
 
-/* VDWGD1SPLN, vdwgd1spln.f90:19-170 (without the Qnmcden per-species maps) */
+/* VDWGD1SPLN, vdwgd1spln.f90:19-170 */
 static void vdwgd1spln(orc_ctx *c, int qforces, double *evdwgd) {
     const bromoc_system *s = &c->s;
     const bromoc_grid *g = &s->phiv;
@@ -781,6 +810,7 @@ static void vdwgd1spln(orc_ctx *c, int qforces, double *evdwgd) {
     double tranx2 = 0.5 * (nclx2 - 1) * dcel2, trany2 = 0.5 * (ncly2 - 1) * dcel2,
            tranz2 = 0.5 * (nclz2 - 1) * dcel2;
     double esvdw = s->svdw;
+    size_t ncel3 = (size_t)nclx2 * ncyz, ifir = 0;
     *evdwgd = 0.0;
     for (int i = 0; i < s->ntot; i++) {
         if (!(i < s->nsites || i >= s->nsites + s->nfix)) continue;
@@ -792,7 +822,8 @@ static void vdwgd1spln(orc_ctx *c, int qforces, double *evdwgd) {
         double yi = c->y[i] + trany2 - g->ybcen;
         double zi = c->z[i] + tranz2 - g->zbcen;
         int itype2 = (i < s->nsites) ? c->typtyp[i] : c->nwtype[abs(c->typei[i])];
-        if (s->flags & BROMOC_QSVDW) esvdw = s->svdw * c->scal[itype2];
+        if (s->flags & BROMOC_QNMCDEN) nmcden_map(c, i, abs(c->typei[i]), ncel3, &ifir);
+        else if (s->flags & BROMOC_QSVDW) esvdw = s->svdw * c->scal[itype2];
         double vdwfx = 0.0, vdwfy = 0.0, vdwfz = 0.0;
         int ix = nint(xi * idcel2), iy = nint(yi * idcel2), iz = nint(zi * idcel2);
         if (ix == 0) ix = 1;
@@ -824,7 +855,7 @@ static void vdwgd1spln(orc_ctx *c, int qforces, double *evdwgd) {
                     ci = m3(ci);
                     if (!(ci != 0.0 || dci != 0.0)) continue;
                     double fi = ai * bi * ci;
-                    double phivi = (double)c->phiv[ipz];
+                    double phivi = (double)c->phiv[ipz + ifir];
                     double phivsv = phivi * esvdw;
                     phisum = phisum + phivi;
                     if (qforces) {
@@ -2056,7 +2087,8 @@ static double staticf0(const orc_ctx *c, double xj, double yj, double zj, int j,
 }
 
 /* VDWGD0TRLN, vdwgd0trln.f90:19-117 */
-static double vdwgd0trln(orc_ctx *c, double xj, double yj, double zj, int jtype2, int alert) {
+static double vdwgd0trln(orc_ctx *c, double xj, double yj, double zj, int j, int jtype, int jtype2,
+                         int alert) {
     const bromoc_system *s = &c->s;
     const bromoc_grid *g = &s->phiv;
     double evdwgd = 0.0;
@@ -2069,7 +2101,9 @@ static double vdwgd0trln(orc_ctx *c, double xj, double yj, double zj, int jtype2
     int ncyz = g->ncly * g->nclz;
     double xi = xj + tranx - g->xbcen, yi = yj + trany - g->ybcen, zi = zj + tranz - g->zbcen;
     double esvdw = s->svdw;
-    if (s->flags & BROMOC_QSVDW) esvdw = esvdw * c->scal[jtype2];
+    size_t ifir = 0;
+    if (s->flags & BROMOC_QNMCDEN) nmcden_map(c, j, jtype, (size_t)g->nclx * ncyz, &ifir);
+    else if (s->flags & BROMOC_QSVDW) esvdw = esvdw * c->scal[jtype2];
     int ix = (int)(xi * idcel), iy = (int)(yi * idcel), iz = (int)(zi * idcel);
     if (ix == g->nclx - 1) ix = g->nclx - 2;
     if (iy == g->ncly - 1) iy = g->ncly - 2;
@@ -2086,7 +2120,7 @@ static double vdwgd0trln(orc_ctx *c, double xj, double yj, double zj, int jtype2
                 ci = 1.0 - fabs(ci) * idcel;
                 double fi = ai * bi * ci;
                 size_t in3 = (size_t)n1 * ncyz + (size_t)n2 * g->nclz + n3;
-                double phis = (double)c->phiv[in3];
+                double phis = (double)c->phiv[in3 + ifir];
                 phisum = phisum + phis;
                 evdwgd = evdwgd + fi * esvdw * phis;
             }
@@ -2099,8 +2133,9 @@ static double vdwgd0trln(orc_ctx *c, double xj, double yj, double zj, int jtype2
     return evdwgd;
 }
 
-/* VDWGD0SPLN, vdwgd0spln.f90:19-130 (without Qnmcden) */
-static double vdwgd0spln(orc_ctx *c, double xj, double yj, double zj, int jtype2, int alert) {
+/* VDWGD0SPLN, vdwgd0spln.f90:19-130 */
+static double vdwgd0spln(orc_ctx *c, double xj, double yj, double zj, int j, int jtype, int jtype2,
+                         int alert) {
     const bromoc_system *s = &c->s;
     const bromoc_grid *g = &s->phiv;
     int nclx2 = g->nclx, ncly2 = g->ncly, nclz2 = g->nclz;
@@ -2114,7 +2149,9 @@ static double vdwgd0spln(orc_ctx *c, double xj, double yj, double zj, int jtype2
     if (!ok) return evdwgd;
     double xi = xj + tranx2 - g->xbcen, yi = yj + trany2 - g->ybcen, zi = zj + tranz2 - g->zbcen;
     double esvdw = s->svdw;
-    if (s->flags & BROMOC_QSVDW) esvdw = esvdw * c->scal[jtype2];
+    size_t ifir = 0;
+    if (s->flags & BROMOC_QNMCDEN) nmcden_map(c, j, jtype, (size_t)nclx2 * ncyz, &ifir);
+    else if (s->flags & BROMOC_QSVDW) esvdw = esvdw * c->scal[jtype2];
     int ix = nint(xi * idcel2), iy = nint(yi * idcel2), iz = nint(zi * idcel2);
     if (ix == 0) ix = 1;
     if (iy == 0) iy = 1;
@@ -2139,7 +2176,7 @@ static double vdwgd0spln(orc_ctx *c, double xj, double yj, double zj, int jtype2
                 double ci = 1.5 - (zi - zc) * idcel2;
                 double fi = ai * bi * m3(ci);
                 if (fi != 0.0) {
-                    double phis = (double)c->phiv[ipz];
+                    double phis = (double)c->phiv[ipz + ifir];
                     phisum = phisum + phis;
                     evdwgd = evdwgd + fi * esvdw * phis;
                 }
@@ -2192,8 +2229,8 @@ EXPORT int orc_interact(orc_ctx *c, double xj, double yj, double zj, int32_t jty
     if ((fl & BROMOC_QRFPAR) && beion) dener = dener + rfparionj(c, xj, yj, zj, j, jtype);
     if (fl & BROMOC_QPHIX) dener = dener + staticf0(c, xj, yj, zj, j, jtype);
     if (fl & BROMOC_QPHIV) {
-        if (fl & BROMOC_QBSPLINE) dener = dener + vdwgd0spln(c, xj, yj, zj, jtype2, alert);
-        else dener = dener + vdwgd0trln(c, xj, yj, zj, jtype2, alert);
+        if (fl & BROMOC_QBSPLINE) dener = dener + vdwgd0spln(c, xj, yj, zj, j, jtype, jtype2, alert);
+        else dener = dener + vdwgd0trln(c, xj, yj, zj, j, jtype, jtype2, alert);
     }
     if (beion && (fl & BROMOC_QNONBOND)) {
         for (int i = ns; i < nt; i++) {

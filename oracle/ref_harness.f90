@@ -78,7 +78,8 @@ integer(c_int32_t), parameter :: F_PAR=1, F_NUCL=2, F_NONBOND=4, F_NOBOND=8, F_P
      F_SVDW=64, F_SOLV=128, F_DEBY=256, F_COUNTION=512, F_DNAFREE=1024, &
      F_RFPAR=33554432, F_DIFFUSE=268435456, F_PROFILE=536870912, F_RFPSIN=1073741824, &
      F_BSPLINE=262144, F_PROXDIFF=2097152, F_NOTRANS=4194304, F_APFOR=8388608, &
-     F_CONTRANS=16777216, F_MEMB=65536, F_MMIJ=131072, F_DEBYHYB=1048576
+     F_CONTRANS=16777216, F_MEMB=65536, F_MMIJ=131072, F_DEBYHYB=1048576, &
+     F_NMCDEN=524288
 integer :: site_names_code(0:5)
 
 contains
@@ -139,7 +140,7 @@ integer(c_int32_t), pointer :: ip(:)
 integer(c_int8_t), pointer :: bp(:)
 real(c_float), pointer :: fp(:)
 type(c_pair), pointer :: pp(:)
-integer :: i, n, ns, ntc, ncel3, nt
+integer :: i, n, ns, ntc, ncel3, nt, nmaps
 
 character(len=2), parameter :: names(0:5) = (/'S ','P ','Ab','Tb','Cb','Gb'/)
 
@@ -167,7 +168,11 @@ Qmemb = flag(s%flags, F_MEMB)
 Qmmij = flag(s%flags, F_MMIJ)
 Qrfpar = .false.
 Qtrln = .not. flag(s%flags, F_BSPLINE)
-Qnmcden = .false.
+Qnmcden = flag(s%flags, F_NMCDEN)
+! the map numbering of vdwgd1trln.f90:74-76 counts ion types from inuc+1 (one strand):
+! nold nucleotide types (shell_simul.f90:530)
+istrs = 1
+inuc = s%nold
 Qproxdiff = flag(s%flags, F_PROXDIFF)
 Qdebyhyb = flag(s%flags, F_DEBYHYB)
 Qapfor = flag(s%flags, F_APFOR)
@@ -268,9 +273,15 @@ if (Qphiv) then
   tranx2 = 0.5*(nclx2-1)*dcel2; trany2 = 0.5*(ncly2-1)*dcel2; tranz2 = 0.5*(nclz2-1)*dcel2
   xbcen2 = s%phiv%xbcen; ybcen2 = s%phiv%ybcen; zbcen2 = s%phiv%zbcen
   ncel3 = nclx2*ncly2*nclz2
+  nmaps = 1
+  if (Qnmcden) then   ! readphi.f90:176-240: six site maps, then one per ion type
+    nmaps = 0
+    if (Qnucl) nmaps = 6
+    if (Qpar) nmaps = nmaps + ntype - nold
+  endif
   if (allocated(phiv)) deallocate(phiv)
-  allocate(phiv(ncel3))
-  call c_f_pointer(s%phiv%data, bp, [ncel3]); phiv = bp
+  allocate(phiv(ncel3*nmaps))
+  call c_f_pointer(s%phiv%data, bp, [ncel3*nmaps]); phiv = bp
   svdw = s%svdw; thold8 = s%thold8; vzmin = s%vzmin; vzmax = s%vzmax; thold27 = s%thold27
   if (allocated(scal)) deallocate(scal)
   allocate(scal(nttyp))
@@ -608,6 +619,9 @@ implicit none
 integer(c_int32_t), value :: qf
 real(c_double), intent(out) :: e(23)
 Qforces = qf /= 0
+! terms ENERGY sets only under their option (erfpar under Qrfpar, energy.mono.f90:110-114)
+! start from zero, not from a previous system's value
+erfpar = 0.0; egsbpb = 0.0; egsbpa = 0.0; evdwgd = 0.0; ememb = 0.0; econ = 0.0
 call energy
 e = 0.0
 e(1) = ener

@@ -12,6 +12,8 @@ import pytest
 from bromoc_amd import abi, configs
 from oracle.oracle import oracle_table_builder
 
+from helpers import pore_small
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HDR = os.path.join(ROOT, "include", "bromoc_gpu.h")
 
@@ -63,7 +65,7 @@ def _plain():
 
 
 def test_create_rejects_unsupported_options():
-    for flag in (abi.QNMCDEN, abi.QSPHERE, abi.QECYL):
+    for flag in (abi.QSPHERE, abi.QECYL):
         s = _plain()
         s.flags |= flag
         rc, _ = _create(s)
@@ -90,6 +92,14 @@ def test_create_rejects_bad_indices():
 
 
 def test_create_rejects_inconsistent_options():
+    s = _plain()
+    s.flags |= abi.QNMCDEN                        # per-species maps without a PHIV map
+    s.flags &= ~abi.QPHIV
+    assert _create(s)[0] == abi.E_ARG
+    s = configs.with_nmcden(pore_small(seed=3, n_ions=4))
+    assert _create(s)[0] in (abi.OK, abi.E_NODEVICE)  # valid: only the device is missing
+    s.flags |= abi.QSVDW                          # shell_simul.f90:2875
+    assert _create(s)[0] == abi.E_ARG
     s = _plain()
     s.flags |= abi.QDIFFUSE | abi.QPROFILE        # simul1.f90:338-340
     assert _create(s)[0] == abi.E_ARG
@@ -158,7 +168,7 @@ def test_batch_create_validation():
     sh, reps = configs.replica_batch(seed=1, nrep=2, n_range=(10, 20), box=(30.0, 30.0, 60.0),
                                      dcel=1.5, rpore=8.0, half_slab=8.0,
                                      table_builder=oracle_table_builder)
-    for flag in (abi.QNUCL, abi.QBSPLINE, abi.QMEMB, abi.QMMIJ):
+    for flag in (abi.QNUCL, abi.QBSPLINE, abi.QMEMB, abi.QMMIJ, abi.QNMCDEN):
         bad = sh.copy()
         bad.flags |= flag
         with pytest.raises(abi.BromocError) as ei:

@@ -134,11 +134,12 @@ def test_diffusion_and_rfpar_through_front_end(which):
             assert abs(a - b) <= 1e-14 * max(abs(b), 1.0)
 
 
-@pytest.mark.parametrize("which", ["bspline", "apfor_contrans", "proxdiff", "membrane", "mmij"])
+@pytest.mark.parametrize("which", ["bspline", "apfor_contrans", "proxdiff", "membrane", "mmij",
+                                   "nmcden"])
 def test_dna_options_through_front_end(which):
     """B-spline repulsion map (Qtrln off), APFOR + CONTRANS (apfmod, nucleotmod), PROXDIFF
-    (DYNAMICS3), MEMBRANE and the MMIJ reaction field (gsbpmod) handed over from the modules:
-    energies, forces and BD steps."""
+    (DYNAMICS3), MEMBRANE, the MMIJ reaction field and the per-species repulsion maps
+    (gsbpmod) handed over from the modules: energies, forces and BD steps."""
     base = with_fixed(small_pore(seed=41), 4, nonfree=(3,))
     if which == "bspline":
         s = configs.with_bspline(base, thold27=3.0)
@@ -150,8 +151,10 @@ def test_dna_options_through_front_end(which):
         s = configs.with_proxdiff(base, beta=2.5, diff0=0.05)
     elif which == "membrane":
         s = configs.with_membrane(base, thick=16.0, volts=0.15, pore=(11.0, 5.0, 2.5))
-    else:
+    elif which == "mmij":
         s = configs.with_mmij(base, npol=(3, 4, 4), rfscal=0.8, region=(30.0, 30.0, 40.0))
+    else:
+        s = configs.with_nmcden(base, seed=7, density=0.2)
     seed, c0 = 91, 4
     f, r = _pair(s)
     f.L.bromoc_gpu_set_seed(seed, c0)

@@ -1,0 +1,73 @@
+"""Per-species repulsion maps (NMCDEN: vdwgd1trln.f90:61-80, vdwgd1spln.f90:61-80,
+vdwgd0trln.f90:56-75, vdwgd0spln.f90:58-77) on the HIP path against the C oracle, through
+every launch shape of the step kernels (BROMOC_TPB 1, 2, 4 targets per block and the
+one-target-per-wavefront kernel, 8).
+
+Bar as the other parity tests: energies and norm-wise forces 1e-10 relative, positions
+after BD steps 1e-9 A (1e-8 after 40 graph-replayed steps), INTERACT 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+from bromoc_amd import abi, configs
+from bromoc_amd.engine import Engine
+from oracle.oracle import Oracle
+
+from helpers import energy_errs, force_err, pore_small, with_fixed
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _sys(seed, dna=True, bspline=False, mode="scatter"):
+    s = with_fixed(pore_small(seed=seed, n_ions=150, dna=dna), 3)
+    if bspline:
+        s.flags |= abi.QBSPLINE
+    return configs.with_nmcden(s, seed=seed, density=0.2, mode=mode)
+
+
+@pytest.mark.parametrize("tpb", [1, 2, 4, 8])
+@pytest.mark.parametrize("dna,bspline", [(True, False), (True, True), (False, False)])
+def test_energy_forces(dna, bspline, tpb, monkeypatch):
+    monkeypatch.setenv("BROMOC_TPB", str(tpb))
+    s = _sys(61, dna, bspline)
+    g, o = Engine(s), Oracle(s)
+    eg, eo = g.energy(), o.energy()
+    assert eo["evdwgd"] != 0.0
+    bad = {k: v for k, v in energy_errs(eg, eo).items() if v > TOL}
+    assert not bad, bad
+    assert force_err(g.forces(), o.forces()) <= TOL
+
+
+@pytest.mark.parametrize("dna", [True, False])
+def test_steps_and_graphs(dna):
+    """Per-species pore radii (walls grown by 0, 3 or 6 nodes): the scattered maps of the
+    other tests make BD trajectories chaotic (a 1e-11 A change in one ion grows past 0.01 A within
+    40 steps in the oracle alone), so trajectories are compared on smooth maps."""
+    s = _sys(62, dna, mode="pore")
+    assert Oracle(s).energy()["evdwgd"] != 0.0
+    noise = np.random.default_rng(3).standard_normal((40, 3 * s.ntot))
+    for nsteps, tol in ((5, 1e-9), (40, 1e-8)):
+        g, o = Engine(s), Oracle(s)
+        g.upload_noise(noise)
+        o.upload_noise(noise)
+        g.step(nsteps, abi.NOISE_INJECTED)
+        o.step(nsteps, abi.NOISE_INJECTED)
+        err = float(np.max(np.abs(g.positions()[0] - o.positions()[0])))
+        print(f"dna={dna} {nsteps} steps: position err {err:.3e}")
+        assert err <= tol
+        assert np.array_equal(g.warn(), o.warn())
+
+
+@pytest.mark.parametrize("dna,bspline", [(True, False), (True, True), (False, False)])
+def test_interact(dna, bspline):
+    s = _sys(63, dna, bspline)
+    g, o = Engine(s), Oracle(s)
+    pos, typ = o.positions()
+    cases = [(pos[j] + 0.1, abs(int(typ[j])), j)
+             for j in range(s.nsites, s.ntot, 11)]          # fixed (map 0) and moving ions
+    cases += [(pos[j], 0, j) for j in range(0, s.nsites, 3)]  # every site name
+    cases += [((1.5, -2.0, 14.0), jt, s.ntot) for jt in range(s.nold + 1, s.ntype + 1)]
+    for xyz, jt, j in cases:
+        a, b = g.interact(xyz, jt, j, True), o.interact(xyz, jt, j, True)
+        assert abs(a - b) <= TOL * max(abs(b), 1.0), (j, jt, a, b)
