@@ -71,3 +71,35 @@ def test_interact(dna, bspline):
     for xyz, jt, j in cases:
         a, b = g.interact(xyz, jt, j, True), o.interact(xyz, jt, j, True)
         assert abs(a - b) <= TOL * max(abs(b), 1.0), (j, jt, a, b)
+
+
+def test_grand_reads_species_maps():
+    """GRAND (grand.f90:40-118): insertions and deletions weigh each trial with INTERACT,
+    which reads the trial ion's type map; then BD steps on the new particle set."""
+    s = pore_small(seed=64, n_bp=0, n_ions=120, dna=False)
+    s = configs.with_nmcden(s, seed=64, mode="pore")
+    s.capacity = s.ntot + 200
+    gc = configs.gcmc_buffers(s, conc=1.0, buffer_len=8.0, mu=(-0.5, -0.4))
+    ib, nat = configs.assign_buffers(s, gc)
+    g, o = Engine(s), Oracle(s)
+    g.set_gcmc(gc, ib)
+    o.set_gcmc(gc, ib)
+    u = np.random.default_rng(4).random(20000)
+    natg, nato = nat.copy(), nat.copy()
+    probg = np.zeros((gc.nbuffer, s.capacity + 1), np.int32)
+    probo = probg.copy()
+    ug, eg = g.grand(200, u, natg, probg, 1, 0.0)
+    uo, eo = o.grand(200, u, nato, probo, 1, 0.0)
+    assert ug == uo
+    assert np.array_equal(natg, nato) and np.array_equal(probg, probo)
+    assert g.ntot == o.ntot
+    assert abs(eg - eo) <= TOL * max(abs(eo), 1.0)
+    pg, tg = g.positions()
+    po, to = o.positions()
+    assert np.array_equal(pg, po) and np.array_equal(tg, to)
+    noise = np.random.default_rng(5).standard_normal((5, 3 * g.ntot))
+    g.upload_noise(noise)
+    o.upload_noise(noise)
+    g.step(5, abi.NOISE_INJECTED)
+    o.step(5, abi.NOISE_INJECTED)
+    assert float(np.max(np.abs(g.positions()[0] - o.positions()[0]))) <= 1e-9
