@@ -969,6 +969,84 @@ def adjoint_system(seed=0, n_per_type=200, box=60.0, temp=300.0, cdie=80.0, dt=0
     return s
 
 
+def type_charges(s: System):
+    """Charge of each atnam2 type as shell_simul.f90:602-607 assigns it: the P sites carry
+    cgnuc (type 2, when the DNA has more than one site type), ion type i its cg(i)."""
+    cg2 = np.zeros(s.nttyp + 1)
+    if s.ndna > 1:
+        cg2[2] = s.cgnuc
+    for i in range(s.nold + 1, s.ntype + 1):
+        cg2[s.nwtype[i]] = s.cg[i]
+    return cg2
+
+
+def analytic_pair_types(s: System, kind="all"):
+    """Pair types (nindex) of the ion-ion pairs ("ion"), the site-ion pairs ("site") or
+    both ("all")."""
+    nt1, ndna = s.nttyp + 1, s.ndna
+    out = []
+    for a in range(1, s.nttyp + 1):
+        for b in range(a, s.nttyp + 1):
+            if b <= ndna:
+                continue
+            is_ = int(s.pairidx[a * nt1 + b])
+            if (a > ndna and kind in ("all", "ion")) or (a <= ndna and kind in ("all", "site")):
+                out.append(is_)
+    return sorted(set(out))
+
+
+def with_analytic(s: System, types=None, seed=0, lj=True, srpmf=True, coulomb=True,
+                  rth=8.0, cdie=None):
+    """Copy of s whose pair types `types` (default: every ion-ion and site-ion type) take
+    ENERGY's and INTERACT's analytic branch (energy.mono.f90:135-178,623-668,
+    interact.f90:182-204,365-404) instead of an EFPOT table:
+
+    - Coulomb fct = celec q_a q_b / cdie where both atnam2 types are charged
+      (shell_simul.f90:606-617);
+    - 12-6 Lennard-Jones, epp4 = 4 eps and sgp2 = sigma^2 (shell_simul.f90:1183-1184),
+      sigma in [3.0, 4.5] A and eps in [0.1, 0.5] kcal/mol from the seed;
+    - SRPMF, the damped-cosine short-range term c0 exp((c1-r)/c2) cos(c3 pi (c1-r)) +
+      c4 (c1/r)^6 (shell_simul.f90:1702-1711, c2 kept as its inverse, :1758) up to
+      r = rth, smoothed from srpx = rth - 0.5 with srpk = 6.90775527898 / 0.5 and
+      srpy = 0.001 (shell_simul.f90:1682-1686).
+
+    Pair types not listed keep their tables, so a partial list gives a mixed system."""
+    t = s.copy()
+    rng = np.random.default_rng(seed)
+    types = analytic_pair_types(t) if types is None else list(types)
+    cdie = cdie if cdie is not None else t.meta.get("cdie", 80.0)
+    cg2 = type_charges(t)
+    nt1 = t.nttyp + 1
+    pt = {}
+    for a in range(1, nt1):
+        for b in range(a, nt1):
+            pt.setdefault(int(t.pairidx[a * nt1 + b]), (a, b))
+    t.pairs = [None if p is None else dict(p) for p in t.pairs]
+    for is_ in types:
+        a, b = pt[is_]
+        d = t.pairs[is_]
+        qq = cg2[a] * cg2[b]
+        sig, eps = rng.uniform(3.0, 4.5), rng.uniform(0.1, 0.5)
+        d.update(efpot=0, nknots=0,
+                 chr=1 if (coulomb and qq != 0.0) else 0,
+                 fct=CELEC * qq / cdie if (coulomb and qq != 0.0) else 0.0,
+                 lj=1 if lj else 0, epp4=4.0 * eps if lj else 0.0,
+                 sgp2=sig * sig if lj else 0.0,
+                 srpmf=1 if srpmf else 0,
+                 c0=rng.uniform(-0.6, 0.6) if srpmf else 0.0,
+                 c1=rng.uniform(2.8, 4.0) if srpmf else 0.0,
+                 c2=1.0 / rng.uniform(0.4, 0.8) if srpmf else 0.0,
+                 c3=rng.uniform(0.3, 0.6) if srpmf else 0.0,
+                 c4=rng.uniform(0.0, 0.05) if srpmf else 0.0)
+    if srpmf:
+        t.rth = rth * rth
+        t.srpx = rth - 0.5
+        t.srpk = 6.90775527898 / 0.5
+        t.srpy = 0.001
+    t.meta = dict(t.meta, analytic=sorted(types))
+    return t
+
+
 def with_knots(s: System, new_u, table_builder=None):
     """Copy of s whose parameter pair types `is` (keys of new_u) take the knot values
     new_u[is]; their tables are rebuilt in place in ep."""

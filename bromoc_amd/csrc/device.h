@@ -807,8 +807,8 @@ __device__ inline void dna_term(const StepParams &P, uint32_t w, bool free_me, V
     }
 }
 
-// 1/sqrt(x2) to within an ulp: hardware estimate plus two Newton steps.  Every branch
-// of the table lookup derives r, 1/r and 1/r^2 from it.
+// 1/sqrt(x2) to within an ulp: hardware estimate plus two Newton steps.  The wall and
+// tail forms of the table lookup and the force factor derive 1/r and 1/r^2 from it.
 __device__ __forceinline__ double rsqrt_nr(double x2) {
     double r = __builtin_amdgcn_rsq(x2);
     double e = fma(-x2 * r, r, 1.0);
@@ -818,31 +818,6 @@ __device__ __forceinline__ double rsqrt_nr(double x2) {
     return r;
 }
 
-// Table pair (getyd, polynom.f90:197-230) from x2 and rinv = 1/sqrt(x2): wall below dmi,
-// analytic tail above the last knot, piecewise quadratic in between.
-__device__ __forceinline__ void efpot_eval(const DevPair &p, const double4 *__restrict__ ep,
-                                           double ires, double x2, double rinv, double &y,
-                                           double &yp) {
-    const double ix2 = rinv * rinv;
-    if (x2 < p.dm2lo) {
-        const double c = p.sc1 * pow6(ix2);
-        y = c + p.sc2;
-        yp = 12.0 * c * ix2;
-    } else if (x2 > p.dm2hi) {
-        const double c = p.sc3 * pow3(ix2);
-        const double f = p.fct * rinv;     // fct is zero for uncharged pairs
-        y = c + f;
-        yp = (6.0 * c + f) * ix2;
-    } else {
-        const double x = x2 * rinv;
-        int i = (int)((x - p.dmi) * ires) + p.nxi;
-        i = min(max(i, p.nxi), p.nlast);
-        const double4 e = ep[i];
-        y = e.x + x * (e.y + x * e.z);
-        yp = -e.y * rinv - 2.0 * e.z;
-    }
-}
-
 // Gaussians of particle i for this launch's step: injected rows are addressed by the
 // step counter, Philox rows were generated for the current chunk of steps by
 // k_philox_fill (row = step_off).
@@ -850,29 +825,6 @@ __device__ __forceinline__ const double *noise_row(const StepParams &P, int i) {
     const int64_t row = P.noise_mode == 0 ? P.step_ctr[1] + P.step_ctr[0] + P.step_off
                                           : P.step_off;
     return P.noise + row * P.noise_stride + 3 * (int64_t)i;
-}
-
-// One partner j of a target: the table pair of getyd (polynom.f90:197-230) evaluated
-// without branches.  Wall, tail and table forms are all formed from rinv = 1/sqrt(d2);
-// the knot index is clamped into the pair's table, so the gather is always in bounds and
-// is simply not selected outside [dmi, r_n].  Returns the force factor; adds the energy.
-__device__ __forceinline__ double pair_table(const DevPair &pp, const double4 *__restrict__ ep,
-                                            double ires, double d2, double &y) {
-    const double rinv = rsqrt_nr(d2);
-    const double ix2 = rinv * rinv;
-    const double x = d2 * rinv;
-    int k = (int)((x - pp.dmi) * ires) + pp.nxi;
-    k = min(max(k, pp.nxi), pp.nlast);
-    const double4 e = ep[k];
-    const double c3 = pp.sc3 * pow3(ix2);
-    const double f = pp.fct * rinv;              // zero for uncharged pairs
-    const double c12 = pp.sc1 * pow6(ix2);
-    const bool wall = d2 < pp.dm2lo;
-    const bool tail = d2 > pp.dm2hi;
-    const double yt = e.x + x * (e.y + x * e.z);
-    const double ypt = -e.y * rinv - 2.0 * e.z;
-    y = wall ? c12 + pp.sc2 : (tail ? c3 + f : yt);
-    return wall ? 12.0 * c12 * ix2 : (tail ? (6.0 * c3 + f) * ix2 : ypt);
 }
 
 // Table pair (getyd, polynom.f90:197-230) in tail-first form: every lane forms the
@@ -895,7 +847,10 @@ __device__ __forceinline__ double pair_near_far(const DevPair &pp, const double4
             y = cw + pp.sc2;
             de = 12.0 * cw * ix2;
         } else {
-            const double x = d2 * rinv;
+            // the knot index from the correctly rounded root, as getyd forms it
+            // (polynom.f90:225-226): 1/sqrt is within an ulp only, and one ulp at a knot
+            // selects the neighbouring segment, where the table jumps by ~1e-9
+            const double x = sqrt(d2);
             int k = (int)((x - pp.dmi) * ires) + pp.nxi;
             k = min(max(k, pp.nxi), pp.nlast);
             const double4 e = ep[k];

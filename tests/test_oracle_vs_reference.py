@@ -355,3 +355,68 @@ def test_membrane_energy_steps_interact(pore):
         a, b = o.interact(xyz, jt, j, True), r.interact(xyz, jt, j, True)
         assert abs(a - b) <= 1e-14 * max(abs(b), 1.0), (j, a, b)
     assert np.array_equal(o.warn(), r.warn())
+
+
+def test_scalepairing_vs_reference_and_interact_quirk():
+    """Base pairing scaled by scalepairing (energy.mono.f90:372-376) against the reference;
+    the energy is linear in it (ebp at 1.7 is 1.7 times ebp at 1.0), while INTERACT's base
+    pair term leaves it out (interact.f90:288-298): a site's INTERACT is the same at both."""
+    base = small_pore(seed=9, n_bp=6, n_ions=60)
+    sys_ = {}
+    for sp in (1.0, 1.7):
+        s = base.copy()
+        s.scalepairing = sp
+        sys_[sp] = s
+        o, r = Oracle(s), RefFortran(s)
+        eo, er = o.energy(), r.energy()
+        assert max(energy_errs(eo, er, floor=1e-12).values()) <= 1e-14
+        assert force_err(o.forces(), r.forces()) <= 1e-14
+        pos, _ = o.positions()
+        for j in range(0, s.nsites, 5):
+            a, b = o.interact(pos[j], 0, j, False), r.interact(pos[j], 0, j, False)
+            assert abs(a - b) <= 1e-14 * max(abs(b), 1.0)
+    e1, e17 = Oracle(sys_[1.0]).energy(), Oracle(sys_[1.7]).energy()
+    assert e1["ebp"] != 0.0
+    assert e17["ebp"] == pytest.approx(1.7 * e1["ebp"], rel=1e-13)
+    o1, o17 = Oracle(sys_[1.0]), Oracle(sys_[1.7])
+    pos, _ = o1.positions()
+    for j in range(0, base.nsites, 3):
+        assert o1.interact(pos[j], 0, j, False) == o17.interact(pos[j], 0, j, False)
+
+
+def knot_cases(s, every=23, rmin=2.5):
+    """Two-ion separations exactly on table knots r_k (as EFPOT files list them) and one ulp
+    either side, with the energy jump the table has there (fixdy, polynom.f90:91-118):
+    a lookup that picks the neighbouring segment is off by that jump."""
+    nt1 = s.nttyp + 1
+    p = s.pairs[int(s.pairidx[1 * nt1 + 2])]
+    r = configs.knot_grid()
+    out = []
+    for k in range(1, len(r) - 1, every):
+        if r[k] < rmin:
+            continue
+        for x in (r[k], np.nextafter(r[k], 0.0), np.nextafter(r[k], 1e9)):
+            out.append(float(x))
+    return p, out
+
+
+def test_knot_boundaries_vs_reference():
+    """Pairs placed on and one ulp around knot boundaries select the reference's segment:
+    ENERGY's table energy and force equal the reference's bit for bit (getyd's index from
+    the correctly rounded sqrt, polynom.f90:225-226)."""
+    s = small_plain(3, 1)
+    p, xs = knot_cases(s)
+    o, r = Oracle(s), RefFortran(s)
+    ep = s.ep.reshape(-1, 3)
+    jumps = 0
+    for x in xs:
+        pos = np.array([[0.0, 0.0, 0.0], [x, 0.0, 0.0]])
+        o.set_positions(pos)
+        r.set_positions(pos)
+        eo, er = o.energy(), r.energy()
+        assert eo["eefpot"] == er["eefpot"], x
+        assert np.array_equal(o.forces(), r.forces()), x
+        k = int((x - p["dmi"]) * (1.0 / s.res))
+        a, b = ep[p["nxi"] + k], ep[p["nxi"] + k - 1]
+        jumps += abs((a[0] + x * (a[1] + x * a[2])) - (b[0] + x * (b[1] + x * b[2]))) > 1e-12 * abs(eo["eefpot"])
+    assert jumps > len(xs) // 2     # the neighbouring segment would be seen
