@@ -1,14 +1,19 @@
 """Throughput of the BROMOC BD step path on one MI355X.
 
-Workload (default): ions in a cylindrical pore through a slab plus a 50-bp CG-DNA duplex
--- 2000 K+/Cl- ions and 300 DNA sites in a 100x100x200 A box, 201x201x401 repulsion and
-static-field grids at 0.5 A, 15 EFPOT tables (r = 1.5..20 A at 0.01 A), FP64, Philox noise.
-One step = ENERGY + DYNAMICS0 + DYNAMICS0NUC over every moving particle (one kernel launch).
+Workload (default): the large translocation system -- a 1000-bp CG-DNA (6000 sites)
+threaded through a 15 A pore in a 200 x 200 x 600 A box with 50,000 K+/Cl- ions (56,000
+particles), 321 x 321 x 641 repulsion, static-field and reaction-field grids at 0.625 A,
+D(z) from a PROFILE spline, GCMC bath buffers at both box ends; FP64, Philox noise.  A step
+is ENERGY + DYNAMICS2 + DYNAMICS0NUC over every moving particle; every 10 steps GRAND
+makes 20 insertion / deletion attempts, and each 10-step call returns its last step's
+energies, as the drop-in's BD call does.
 
-Prints one JSON line: particle-steps/s over the timed steps, the roofline of the step
-kernel (bytes from HIP events on the kernel's own stream), and a CPU baseline: the
-reference Fortran routines (oracle/_ref, one thread) -- or the C oracle where that build
-is absent -- timed on this host over a bounded sample of the same workload.
+Prints one JSON line: particle-steps/s over the timed steps, the roofline of the dominant
+kernel (its duration from HIP events on its own stream), and the CPU baseline: the
+reference Fortran routines (oracle/_ref) at one thread and with the reference's OpenMP
+ENERGY on all cores, timed on this host on the largest sample of the workload the
+reference's fixed arrays hold, scaled to the full system by its pair count.  The other
+workloads (the 2300-particle pore system, replica batches, the adjoint) are extra keys.
 """
 from __future__ import annotations
 
@@ -23,20 +28,29 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s + % HBM roofline, batched vs plain, FP64 rel err<=1e-10"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the step kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# (KiB per dispatch, one counter per run) on this workload and launch shape (targets per
-# block the autotuner picks), scaled by what the same counters read for a 256 MiB device
-# copy (bench.py --copy-probe 256): FETCH_SIZE x 2.0, WRITE_SIZE x 1.0.  The counters cannot
-# be read from inside this process; a shape without a measurement reports null.
-TRAFFIC_KIB = {("pore_dna", 1): (3073.5, 145.3), ("pore_dna", 2): (3198.4, 144.5)}
+FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch, one counter per run) on each workload and launch shape,
+# scaled by what the same counters read for a 256 MiB device copy (bench.py --copy-probe
+# 256): FETCH_SIZE x 2.0, WRITE_SIZE x 1.0.  The counters cannot be read from inside this
+# process; a shape without a measurement reports null.
+TRAFFIC_KIB = {}
 TRAFFIC_SCALE = (2.0, 1.0)
+# FP64 operations per pair evaluation: a table pair ~34 (r^2 8, sqrt 1, index 2, U 4, F/r 3,
+# energy 1, force update on both particles ~15); the reaction-field cross term of an ion
+# pair ~30 (reff and srfe products 2, 1/sqrt 1, rfcf, aux0-aux3 9, energy 2, the srfe and
+# reff variation of both ions 4, the radial force on both 12)
+FLOP_PAIR, FLOP_RF = 34.0, 30.0
+TPB_NAMES = {1: "k_step, 1 target per block", 2: "k_step, 2 targets per block",
+             4: "k_step, 4 targets per block", 8: "k_wave, 1 target per wavefront",
+             17: "tile mode, tiles of 64", 18: "tile mode, tiles of 128",
+             20: "tile mode, tiles of 256"}
 
 
 def step_bytes(s, injected):
-    """Algorithmic HBM bytes per particle-step of the fused step kernel: read x,y,z (24)
-    and the type word (4), write x,y,z (24), 8 trilinear corners of each grid present
-    (phix float32: 32, phiv int8: 8, the reaction-field pair gsrfen + greff: 64), injected
-    Gaussians (24).  Partner positions, pair
+    """Algorithmic HBM bytes per particle-step: read x,y,z (24) and the type word (4), write
+    x,y,z (24), 8 trilinear corners of each grid present (phix float32: 32, phiv int8: 8, the
+    reaction-field pair gsrfen + greff: 64), injected Gaussians (24).  Partner positions, pair
     tables, DNA lists and type tables are re-read from L2 and are not HBM traffic."""
     from bromoc_amd import abi
     b = 24 + 4 + 24
@@ -51,31 +65,97 @@ def step_bytes(s, injected):
     return b
 
 
-def cpu_baseline(s, target_s=12.0):
-    """Time the reference routines (or the C oracle) on a bounded number of steps."""
+def pair_counts(s, ntot=None):
+    """Pair evaluations of one ENERGY as the reference makes them (each pair once): ion-ion
+    pairs with a moving ion (energy.mono.f90:120-123), site-ion pairs (:586-588), the
+    reaction-field cross term of the ion pairs (rfpar.f90:177-223, at most one per ion pair
+    with a moving ion), and the CG-DNA list terms."""
+    from bromoc_amd import abi
+    n = s.ntot if ntot is None else ntot
+    ni, nf = n - s.nsites, s.nfix
+    ii = ni * (ni - 1) // 2 - nf * (nf - 1) // 2
+    si = s.nsites * ni
+    rf = ii if s.flags & abi.QRFPAR else 0
+    lists = int(sum(s.count(k) for k in s.LIST_NAMES)) if s.nsites else 0
+    return {"ion_ion": ii, "site_ion": si, "rf_cross": rf, "dna_list_terms": lists}
+
+
+def fp64_rate(pc, ms):
+    flops = FLOP_PAIR * (pc["ion_ion"] + pc["site_ion"]) + FLOP_RF * pc["rf_cross"]
+    t = flops / (ms * 1e-3) / 1e12
+    return {"achieved_tflops": t, "peak_tflops": FP64_PEAK_TFLOPS, "frac": t / FP64_PEAK_TFLOPS,
+            "pairs_per_step": pc, "flops_per_step": flops}
+
+
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def time_reference(s, lib, target_s, min_steps=1):
+    """Seconds per BD step of the reference routines (or the C oracle) on system s."""
     import numpy as np
     from bromoc_amd import abi
-    try:
-        from oracle.oracle import RefFortran, ref_available, Oracle
-    except ImportError:
-        return None
-    kind = "reference" if ref_available() else "port"
-    eng = RefFortran(s) if kind == "reference" else Oracle(s)
-    nmov = s.nsites + (s.ntot - s.nsites - s.nfix)
+    from oracle.oracle import Oracle, RefFortran
+    eng = Oracle(s) if lib == "port" else (RefFortran(s) if lib is None else RefFortran(s, lib=lib))
     rng = np.random.default_rng(99)
-    # one probe step sizes the sample to about target_s seconds
     eng.upload_noise(rng.standard_normal((1, 3 * s.ntot)))
     t0 = time.perf_counter()
     eng.step(1, abi.NOISE_INJECTED, 0)
     probe = time.perf_counter() - t0
-    nsteps = max(1, min(2000, int(target_s / max(probe, 1e-6))))
+    nsteps = max(min_steps, min(2000, int(target_s / max(probe, 1e-6))))
     eng.upload_noise(rng.standard_normal((nsteps, 3 * s.ntot)))
     t0 = time.perf_counter()
     eng.step(nsteps, abi.NOISE_INJECTED, 0)
     dt = time.perf_counter() - t0
-    return {"value": nmov * nsteps / dt, "unit": "particle-steps/s", "cores": 1, "kind": kind,
-            "sample": f"{nsteps} BD steps (ENERGY+DYNAMICS0+DYNAMICS0NUC) of the same "
-                      f"{s.ntot}-particle system, 1 thread, {dt:.1f} s"}
+    del eng
+    return dt / nsteps, nsteps, dt
+
+
+def cpu_baseline(s, sample=None, target_s=10.0):
+    """The reference on this host's cores: one thread (the .mono routines) and all cores
+    (the OpenMP ENERGY, energy.para.f90), on `sample` -- the same workload shrunk to what
+    the reference's fixed arrays hold (datom = 23000 particles, dtype = 1000 types,
+    grandmod.f90:24) -- scaled to s by the pair count of one step; on s itself when no
+    sample is given.  Falls back to the C oracle where the reference build is absent."""
+    from oracle.oracle import REF_OMP_SO, ref_available
+    base = sample or s
+    kind = "reference" if ref_available() else "port"
+    pc_full, pc_base = pair_counts(s), pair_counts(base)
+    work = lambda pc: FLOP_PAIR * (pc["ion_ion"] + pc["site_ion"]) + FLOP_RF * pc["rf_cross"]
+    scale = work(pc_full) / work(pc_base)
+    nmov = s.ntot - s.nfix
+    info = cpu_info()
+    out = {}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info["affinity"] or info["nproc"]
+    for key, lib, cores in (("one_thread", None if kind == "reference" else "port", 1),
+                            ("all_cores", REF_OMP_SO if kind == "reference" else None, threads)):
+        if key == "all_cores" and kind != "reference":
+            continue
+        per_step, nsteps, dt = time_reference(base, lib, target_s)
+        full_step = per_step * scale
+        out[key] = {"value": nmov / full_step, "unit": "particle-steps/s", "cores": cores,
+                    "kind": kind, "s_per_step_sample": per_step,
+                    "sample": (f"{nsteps} BD steps of a {base.ntot}-particle sample "
+                               f"({base.nsites} sites) of the workload, {dt:.1f} s on {cores} "
+                               f"thread(s), scaled x{scale:.2f} by the pair work of one step"
+                               if sample else
+                               f"{nsteps} BD steps of the same {base.ntot}-particle system, "
+                               f"{dt:.1f} s on {cores} thread(s)")}
+    res = dict(out["one_thread"])
+    res.update(info)
+    if "all_cores" in out:
+        res["all_cores"] = out["all_cores"]
+    return res
 
 
 def host_roundtrip(eng, nmov, nsteps=200):
@@ -93,6 +173,127 @@ def host_roundtrip(eng, nmov, nsteps=200):
     dt = time.perf_counter() - t0
     return {"value": nmov * nsteps / dt, "unit": "particle-steps/s", "steps": nsteps,
             "pattern": "set_positions + energy + get_forces + dynamics + get_positions per step"}
+
+
+def gcmc_cycles(e, s, nsteps, g, nat, prob, rng, counter0, every=10, ngcmc=20, cycle0=0,
+                ener=0.0):
+    """nsteps BD steps in cycles of GRAND (ngcmc attempts) + `every` steps, each step call
+    returning its last step's energies.  Returns (ener, counter, moving-particle steps)."""
+    from bromoc_amd import abi
+    done, ctr, cyc, psteps = 0, counter0, cycle0, 0
+    while done < nsteps:
+        _, ener = e.grand(ngcmc, rng.random(6 * ngcmc + 8), nat, prob, cyc + 1, ener)
+        n = min(every, nsteps - done)
+        e.step(n, abi.NOISE_PHILOX, seed=11, counter0=ctr, energies=True)
+        psteps += n * (e.ntot - s.nfix)
+        ctr += n
+        done += n
+        cyc += 1
+    return ener, ctr, psteps
+
+
+def translocation_leg(steps, warmup, cpu=True, parity=True):
+    """The headline workload (see the module docstring)."""
+    import numpy as np
+    from bromoc_amd import abi, configs
+    from bromoc_amd.engine import Engine
+    t0 = time.perf_counter()
+    s = configs.translocation()
+    build_s = time.perf_counter() - t0
+    g = s.meta["gcmc"]
+    ib, nat = configs.assign_buffers(s, g)
+    e = Engine(s)
+    par = None
+    if parity:
+        from oracle.oracle import Oracle
+        o = Oracle(s)
+        eg, eo = e.energy(), o.energy()
+        par = {"force_normwise_relerr": float(np.max(np.abs(e.forces() - o.forces())) /
+                                              np.max(np.abs(o.forces()))),
+               "energy_relerr": abs(eg["ener"] - eo["ener"]) / abs(eo["ener"])}
+        del o
+    e.set_gcmc(g, ib)
+    prob = np.zeros((g.nbuffer, s.capacity + 1), np.int32)
+    rng = np.random.default_rng(5)
+    tpb = e.tuning()
+    ener, ctr, _ = gcmc_cycles(e, s, max(warmup, 1), g, nat, prob, rng, 0)
+    ncyc0 = (max(warmup, 1) + 9) // 10
+    t0 = time.perf_counter()
+    ener, ctr, psteps = gcmc_cycles(e, s, steps, g, nat, prob, rng, ctr, cycle0=ncyc0, ener=ener)
+    elapsed = time.perf_counter() - t0
+    nk = 10
+    total_ms, kernel_ms = e.time_steps(nk, abi.NOISE_PHILOX, seed=11, counter0=ctr)
+    # the step without GRAND, for the cost of the GCMC cycles
+    t0 = time.perf_counter()
+    e.step(steps, abi.NOISE_PHILOX, seed=11, counter0=ctr + nk, energies=True)
+    plain_ms = 1e3 * (time.perf_counter() - t0) / steps
+    n_now = e.ntot
+    e.close()
+    pc = pair_counts(s, n_now)
+    bpp = step_bytes(s, injected=False)
+    achieved = bpp * (n_now - s.nfix) / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    if TRAFFIC_KIB.get(("translocation", tpb)):
+        fk, wk = TRAFFIC_KIB[("translocation", tpb)]
+        traffic = 1024.0 * (fk * TRAFFIC_SCALE[0] + wk * TRAFFIC_SCALE[1])
+    wl = {"workload": "translocation", "ions": s.ntot - s.nsites, "dna_sites": s.nsites,
+          "dna_bp": s.meta["n_bp"], "box_A": [200, 200, 600], "grid": list(s.meta["grid"]),
+          "grid_spacing_A": s.meta["dcel"], "rfpar": True, "diffusion": "profile",
+          "pair_tables": s.npair, "dt_ps": s.meta["dt"], "noise": "philox",
+          "gcmc": "20 GRAND attempts every 10 steps", "energies": "last step of each call"}
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "k_tiles" if tpb > 16 else "k_step", "kernel_ms": kernel_ms,
+            "launch_shape": TPB_NAMES.get(tpb, str(tpb)), "targets_per_block": tpb,
+            "bytes_per_particle_step": bpp,
+            "fp64_valu": fp64_rate(pc, kernel_ms)}
+    out = {"value": psteps / elapsed, "elapsed_s": elapsed, "ms_per_step": 1e3 * elapsed / steps,
+           "config": wl, "roofline": roof, "step_ms_without_gcmc": plain_ms,
+           "step_kernels_ms": total_ms / nk, "particles": n_now, "setup_s": build_s,
+           "parity_vs_oracle": par}
+    if cpu:
+        sample = configs.translocation(n_bp=300, n_ions=20000, gcmc=False, capacity=1)
+        sample.capacity = sample.ntot
+        out["cpu_baseline"] = cpu_baseline(s, sample=sample)
+    return out
+
+
+def pore_dna_leg(steps=2000, warmup=200, cpu=True):
+    """The 2300-particle pore workload: 2000 K+/Cl- ions and a 50-bp CG-DNA (300 sites) in a
+    100 x 100 x 200 A box, 201 x 201 x 401 repulsion and static-field grids at 0.5 A, 15 EFPOT
+    tables, Philox noise, no GCMC."""
+    import numpy as np
+    from bromoc_amd import abi, configs
+    from bromoc_amd.engine import Engine
+    s = configs.pore_dna(seed=1)
+    nmov = s.nsites + (s.ntot - s.nsites - s.nfix)
+    e = Engine(s)
+    e.step(warmup, abi.NOISE_PHILOX, seed=2024, counter0=0, energies=False)
+    tpb = e.tuning()
+    t0 = time.perf_counter()
+    e.step(steps, abi.NOISE_PHILOX, seed=2024, counter0=warmup, energies=True)
+    el = time.perf_counter() - t0
+    _, kms = e.time_steps(200, abi.NOISE_PHILOX, seed=2024, counter0=warmup + steps)
+    host = host_roundtrip(e, nmov)
+    e.close()
+    bpp = step_bytes(s, injected=False)
+    traffic = None
+    if TRAFFIC_KIB.get(("pore_dna", tpb)):
+        fk, wk = TRAFFIC_KIB[("pore_dna", tpb)]
+        traffic = 1024.0 * (fk * TRAFFIC_SCALE[0] + wk * TRAFFIC_SCALE[1])
+    ach = bpp * nmov / (kms * 1e-3) / 1e9
+    out = {"value": nmov * steps / el, "unit": "particle-steps/s", "ms_per_step": 1e3 * el / steps,
+           "steps": steps, "ions": s.ntot - s.nsites, "dna_sites": s.nsites,
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                        "kernel": "k_tiles" if tpb > 16 else "k_step", "kernel_ms": kms,
+                        "launch_shape": TPB_NAMES.get(tpb, str(tpb)),
+                        "targets_per_block": tpb, "bytes_per_particle_step": bpp,
+                        "fp64_valu": fp64_rate(pair_counts(s), kms)},
+           "host_roundtrip": host}
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(s, target_s=8.0)
+    return out
 
 
 def batch_legs(steps=300, gcmc_every=10, ngcmc=20, small_n=200, small_steps=400):
@@ -140,58 +341,6 @@ def batch_legs(steps=300, gcmc_every=10, ngcmc=20, small_n=200, small_steps=400)
     small = {"n": small_n, "replicas": len(reps2), "plain": plain, "batched": batched,
              "ratio": batched / plain, "unit": "particle-steps/s", "steps": small_steps}
     return full, small
-
-
-def large_leg(cycles=4, gcmc_every=10, ngcmc=20, cpu=True):
-    """The large translocation workload (56,000 particles: 50,000 ions and a 1000-bp
-    CG-DNA of 6000 sites through a 15 A pore; 321x321x641 repulsion, static-field and
-    reaction-field grids; PROFILE diffusion; GCMC bath buffers): `cycles` cycles of GRAND
-    (`ngcmc` attempts) + `gcmc_every` BD steps on the device, and, with cpu, one step of the
-    C oracle on this host (the reference's fixed arrays hold 23000 particles, fewer than this
-    system has)."""
-    import numpy as np
-    from bromoc_amd import abi, configs
-    from bromoc_amd.engine import Engine
-    t0 = time.perf_counter()
-    s = configs.translocation()
-    gen_s = time.perf_counter() - t0
-    g = s.meta["gcmc"]
-    ib, nat = configs.assign_buffers(s, g)
-    e = Engine(s)
-    e.set_gcmc(g, ib)
-    prob = np.zeros((g.nbuffer, s.capacity + 1), np.int32)
-    rng = np.random.default_rng(5)
-    e.step(4, abi.NOISE_PHILOX, seed=11, counter0=0, energies=False)
-    nmov0 = e.ntot - s.nfix
-    ener, ctr = 0.0, 4
-    t0 = time.perf_counter()
-    for c in range(cycles):
-        _, ener = e.grand(ngcmc, rng.random(6 * ngcmc + 8), nat, prob, c + 1, ener)
-        e.step(gcmc_every, abi.NOISE_PHILOX, seed=11, counter0=ctr, energies=False)
-        ctr += gcmc_every
-    el = time.perf_counter() - t0
-    nmov1 = e.ntot - s.nfix
-    _, kms = e.time_steps(5, abi.NOISE_PHILOX, seed=11, counter0=ctr)
-    nsteps = cycles * gcmc_every
-    out = {"value": 0.5 * (nmov0 + nmov1) * nsteps / el, "unit": "particle-steps/s",
-           "particles": [int(nmov0), int(nmov1)], "ions": s.ntot - s.nsites,
-           "dna_sites": s.nsites, "box_A": [200, 200, 600], "grid": list(s.meta["grid"]),
-           "grid_spacing_A": s.meta["dcel"], "rfpar": True, "diffusion": "profile",
-           "steps": nsteps, "gcmc_every": gcmc_every, "ngcmc": ngcmc,
-           "ms_per_step": 1e3 * el / nsteps, "step_kernel_ms": kms,
-           "dna_list_terms": int(sum(s.count(k) for k in s.LIST_NAMES)), "setup_s": gen_s}
-    e.close()
-    if cpu:
-        from oracle.oracle import Oracle
-        o = Oracle(s)
-        t0 = time.perf_counter()
-        o.step(1, abi.NOISE_PHILOX, seed=11, counter0=0)
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": (s.ntot - s.nfix) / dt, "unit": "particle-steps/s",
-                               "cores": 1, "kind": "port",
-                               "sample": f"1 BD step of the same system, C oracle, 1 thread, "
-                                         f"{dt:.1f} s"}
-    return out
 
 
 def adjoint_leg(nsteps=400, checkpoint_every=20, reps=3):
@@ -245,16 +394,14 @@ def copy_probe(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="pore_dna",
-                    choices=["pore_dna", "plain_ions", "translocation"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="translocation",
+                    choices=["translocation", "pore_dna"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-parity", action="store_true", help="skip the start-state parity check")
-    ap.add_argument("--no-host", action="store_true", help="skip the host round-trip rate")
     ap.add_argument("--no-batch", action="store_true", help="skip the replica-batch legs")
-    ap.add_argument("--no-large", action="store_true",
-                    help="skip the large translocation leg (56,000 particles)")
+    ap.add_argument("--no-pore", action="store_true", help="skip the 2300-particle pore leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
@@ -267,87 +414,33 @@ def main():
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    from bromoc_amd import abi, configs
-    from bromoc_amd.engine import Engine
-
-    t0 = time.perf_counter()
-    if args.workload == "pore_dna":
-        s = configs.pore_dna(seed=1)
-        wl = {"workload": "pore_dna", "ions": s.ntot - s.nsites, "dna_sites": s.nsites,
-              "dna_bp": s.meta["n_bp"], "box_A": [100, 100, 200], "grid": list(s.meta["grid"]),
-              "grid_spacing_A": s.meta["dcel"], "pair_tables": s.npair,
-              "table_knots": int(len(s.ep) // 3), "dt_ps": s.meta["dt"], "noise": "philox"}
-    elif args.workload == "translocation":
-        s = configs.translocation()
-        wl = {"workload": "translocation", "ions": s.ntot - s.nsites, "dna_sites": s.nsites,
-              "box_A": [200, 200, 600], "grid": list(s.meta["grid"]),
-              "grid_spacing_A": s.meta["dcel"], "rfpar": True, "diffusion": "profile",
-              "pair_tables": s.npair, "dt_ps": s.meta["dt"], "noise": "philox"}
+    if args.workload == "translocation":
+        head = translocation_leg(args.steps, args.warmup, cpu=not args.no_cpu,
+                                 parity=not args.no_parity)
+        pore = None if args.no_pore else pore_dna_leg(cpu=not args.no_cpu)
     else:
-        s = configs.plain_ions(seed=0)
-        wl = {"workload": "plain_ions", "ions": s.ntot, "box_A": 60, "pair_tables": s.npair,
-              "dt_ps": s.meta["dt"], "noise": "philox"}
-    build_s = time.perf_counter() - t0
-    nmov = s.nsites + (s.ntot - s.nsites - s.nfix)
-    eng = Engine(s)
-
-    parity = None
-    if not args.no_parity:
-        from oracle.oracle import Oracle
-        o = Oracle(s)
-        eg, eo = eng.energy(), o.energy()
-        fg, fo = eng.forces(), o.forces()
-        parity = {"force_normwise_relerr": float(np.max(np.abs(fg - fo)) / np.max(np.abs(fo))),
-                  "energy_relerr": abs(eg["ener"] - eo["ener"]) / abs(eo["ener"])}
-        del o
-
-    eng.step(args.warmup, abi.NOISE_PHILOX, seed=2024, counter0=0, energies=False)
-    tpb = eng.tuning()
-    # timed region: the steps as production runs them (captured graphs), host clock around
-    # a synchronous call
-    t0 = time.perf_counter()
-    eng.step(args.steps, abi.NOISE_PHILOX, seed=2024, counter0=args.warmup, energies=False)
-    elapsed = time.perf_counter() - t0
-    value = nmov * args.steps / elapsed
-    # per-launch duration of the step kernel, HIP events on its own stream
-    nk = min(args.steps, 500)
-    _, kernel_ms = eng.time_steps(nk, abi.NOISE_PHILOX, seed=2024,
-                                  counter0=args.warmup + args.steps)
-    bpp = step_bytes(s, injected=False)
-    traffic = None
-    if TRAFFIC_KIB.get((args.workload, tpb)):
-        fk, wk = TRAFFIC_KIB[(args.workload, tpb)]
-        traffic = 1024.0 * (fk * TRAFFIC_SCALE[0] + wk * TRAFFIC_SCALE[1])
-    achieved = bpp * nmov / (kernel_ms * 1e-3) / 1e9
-    # FP64 work of the kernel: ~34 flops per table pair (r^2 8, sqrt 1, index 2, U 4,
-    # F/r 3, energy 1, force update ~15), every ordered pair of a target with its partners
-    nions = s.ntot - s.nsites
-    pairs = (s.ntot - s.nsites - s.nfix) * (s.ntot - 1) + s.nsites * nions
-    gflops = 34.0 * pairs / (kernel_ms * 1e-3) / 1e9
-    host = None if args.no_host else host_roundtrip(eng, nmov)
-    eng.close()
+        pore = pore_dna_leg(steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu)
+        head = dict(pore)
+        head["config"] = {"workload": "pore_dna", "ions": pore["ions"],
+                          "dna_sites": pore["dna_sites"], "noise": "philox"}
+        head["elapsed_s"] = head["ms_per_step"] * args.steps * 1e-3
+        pore = None
     batched, small_n = (None, None) if args.no_batch else batch_legs()
-    large = None if args.no_large else large_leg(cpu=not args.no_cpu)
     adjoint = None if args.no_adjoint else adjoint_leg()
-    cpu = None if args.no_cpu else cpu_baseline(s)
+    cpu = head.pop("cpu_baseline", None)
     out = {
-        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+        "metric": METRIC, "value": head["value"], "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded positions, grids and EFPOT tables)", "config": wl,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_step", "kernel_ms": kernel_ms, "bytes_per_particle_step": bpp,
-                     "fp64_valu": {"achieved_tflops": gflops / 1e3, "peak_tflops": 78.6,
-                                   "frac": gflops / 1e3 / 78.6, "pair_evals_per_step": pairs},
-                     "targets_per_block": tpb},
-        "cpu_baseline": cpu, "host_roundtrip": host, "batched": batched,
-        "batched_vs_plain_small_n": small_n, "large_translocation": large,
-        "adjoint_rdf_gradient": adjoint,
-        "parity_vs_oracle": parity,
-        "setup_s": build_s,
+        "data": "synthetic (seeded positions, grids and EFPOT tables)", "config": head["config"],
+        "roofline": head["roofline"], "cpu_baseline": cpu,
     }
+    for k in ("step_ms_without_gcmc", "step_kernels_ms", "particles", "setup_s",
+              "parity_vs_oracle", "host_roundtrip"):
+        if k in head:
+            out[k] = head[k]
+    out.update({"pore_dna": pore, "batched": batched, "batched_vs_plain_small_n": small_n,
+                "adjoint_rdf_gradient": adjoint})
     print(json.dumps(out))
 
 

@@ -221,6 +221,7 @@ def _declare(lib):
                                C.c_int),
         "bromoc_set_ibuffer": ([vp, _ip], C.c_int),
         "bromoc_get_tuning": ([vp, _ip], C.c_int),
+        "bromoc_spatial_order": ([vp, P(C.c_uint32), _ip, _ip, _ip, _ip], C.c_int),
         "bromoc_time_steps": ([vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_uint64,
                                _dp, _dp], C.c_int),
         "bromoc_get_ibuffer": ([vp, _ip], C.c_int),

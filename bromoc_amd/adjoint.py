@@ -23,19 +23,23 @@ from .system import System, ptr
 class RdfStruct(C.Structure):
     _fields_ = [("ngroup", C.c_int32), ("nbins", C.c_int32), ("groups", C.POINTER(C.c_int32)),
                 ("r0", C.c_double), ("dr", C.c_double), ("sigma", C.c_double),
-                ("cut", C.c_double), ("target", C.POINTER(C.c_double))]
+                ("cut", C.c_double), ("target", C.POINTER(C.c_double)),
+                ("wflux", C.c_double), ("jtarget", C.c_double)]
 
 
 @dataclass
 class RdfSpec:
     """Smoothed RDFs of atnam ion-type pairs: bins of dr from r0 (nbins of them), Gaussian
-    smoothing sigma, terms beyond cut dropped."""
+    smoothing sigma, terms beyond cut dropped; with wflux > 0 the loss also holds
+    wflux (J - jtarget)^2 for the charge current J through the box (e A / ps)."""
     groups: list = field(default_factory=lambda: [(1, 1), (1, 2), (2, 2)])
     r0: float = 2.0
     dr: float = 0.1
     nbins: int = 130
     sigma: float = 0.15
     cut: float = 0.75
+    wflux: float = 0.0
+    jtarget: float = 0.0
 
     def centres(self):
         return self.r0 + (np.arange(self.nbins) + 0.5) * self.dr
@@ -50,6 +54,19 @@ def _declare(lib):
     lib.bromoc_table_vjp.argtypes = [C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
                                      P(C.c_double)]
     lib.bromoc_table_vjp.restype = C.c_int
+    lib.bromoc_tangent_rdf.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(RdfStruct),
+                                       P(C.c_double), P(C.c_double), P(C.c_double),
+                                       P(C.c_double)]
+    lib.bromoc_tangent_rdf.restype = C.c_int
+
+
+def _rdf_struct(spec: RdfSpec, target):
+    groups = np.ascontiguousarray(np.array(spec.groups, np.int32).reshape(-1))
+    tgt = None if target is None else np.ascontiguousarray(np.asarray(target, float).reshape(-1))
+    rs = RdfStruct(len(spec.groups), spec.nbins, ptr(groups, C.c_int32), spec.r0, spec.dr,
+                   spec.sigma, spec.cut, ptr(tgt) if tgt is not None else None, spec.wflux,
+                   spec.jtarget)
+    return rs, (groups, tgt)
 
 
 def rdf_loss(system: System, noise, nsteps, spec: RdfSpec, target=None, grad=True,
@@ -58,12 +75,11 @@ def rdf_loss(system: System, noise, nsteps, spec: RdfSpec, target=None, grad=Tru
     type in system.meta['params']."""
     eng = engine or Engine(system)
     _declare(eng.lib)
+    eng.set_positions(np.stack([system.x, system.y, system.z], 1)[:system.ntot],
+                      system.typei[:system.ntot])
     eng.upload_noise(noise)
     ng, nb = len(spec.groups), spec.nbins
-    groups = np.ascontiguousarray(np.array(spec.groups, np.int32).reshape(-1))
-    tgt = None if target is None else np.ascontiguousarray(np.asarray(target, float).reshape(-1))
-    rs = RdfStruct(ng, nb, ptr(groups, C.c_int32), spec.r0, spec.dr, spec.sigma, spec.cut,
-                   ptr(tgt) if tgt is not None else None)
+    rs, keep = _rdf_struct(spec, target)
     loss = C.c_double()
     g = np.zeros(ng * nb)
     nknots = len(system.ep) // 3
@@ -75,6 +91,45 @@ def rdf_loss(system: System, noise, nsteps, spec: RdfSpec, target=None, grad=Tru
     eng._chk("bromoc_adjoint_rdf", rc)
     grads = knot_gradients(system, gep, gsc, eng.lib) if grad else None
     return loss.value, g.reshape(ng, nb), grads
+
+
+def tangent(system: System, noise, nsteps, spec: RdfSpec, v_ep, v_sc, target=None,
+            checkpoint_every=20, engine: Engine | None = None):
+    """(loss, <dL/d(ep, sc), v>): the loss's derivative along the table-coefficient
+    direction (v_ep [3*nknots], v_sc [3*(npair+1)]) by the tangent-linear pass."""
+    eng = engine or Engine(system)
+    _declare(eng.lib)
+    eng.set_positions(np.stack([system.x, system.y, system.z], 1)[:system.ntot],
+                      system.typei[:system.ntot])
+    eng.upload_noise(noise)
+    rs, keep = _rdf_struct(spec, target)
+    ve = np.ascontiguousarray(v_ep, float)
+    vs = np.ascontiguousarray(v_sc, float)
+    loss, dl = C.c_double(), C.c_double()
+    rc = eng.lib.bromoc_tangent_rdf(eng.ctx, nsteps, checkpoint_every, C.byref(rs), ptr(ve),
+                                    ptr(vs), C.byref(loss), C.byref(dl))
+    eng._chk("bromoc_tangent_rdf", rc)
+    return loss.value, dl.value
+
+
+def table_gradients(system: System, noise, nsteps, spec: RdfSpec, target=None,
+                    checkpoint_every=20, engine: Engine | None = None):
+    """(loss, gep [3*nknots], gsc [3*(npair+1)]): the reverse pass's gradient with respect to
+    the table coefficients themselves, before the chain rule to the knot values."""
+    eng = engine or Engine(system)
+    _declare(eng.lib)
+    eng.set_positions(np.stack([system.x, system.y, system.z], 1)[:system.ntot],
+                      system.typei[:system.ntot])
+    eng.upload_noise(noise)
+    rs, keep = _rdf_struct(spec, target)
+    loss = C.c_double()
+    nknots = len(system.ep) // 3
+    gep = np.zeros(3 * max(nknots, 1))
+    gsc = np.zeros(3 * (system.npair + 1))
+    rc = eng.lib.bromoc_adjoint_rdf(eng.ctx, nsteps, checkpoint_every, C.byref(rs),
+                                    C.byref(loss), None, ptr(gep), ptr(gsc))
+    eng._chk("bromoc_adjoint_rdf", rc)
+    return loss.value, gep, gsc
 
 
 def knot_gradients(system: System, gep, gsc, lib=None):

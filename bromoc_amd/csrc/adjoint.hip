@@ -15,6 +15,15 @@
 //   dL/dth    += sum_pairs (dphi/dth) (mu_i - mu_j).d
 // with phi = -ep2/r - 2 ep3 inside the table (knot k), 12 sc1/r^14 below it and
 // 6 sc3/r^8 + fct/r^3 above it.
+//
+// The tangent-linear map of the same step (k_tlm) carries a parameter direction v forward:
+//   dx' = D_box dx + f1 D_bd D_box (H(x) dx + (dF/dth) v)
+// and the derivative of every RDF histogram bin; <dL/dth, v> from the reverse pass must
+// equal the tangent's dL for every v (the two are transposes of one linear map).
+//
+// Every sum a kernel gathers for one particle runs in a fixed order; sums that many
+// threads add into one place (histogram bins, knot and pair-type gradients) are integer
+// fixed-point sums (fix_add), so the results are bit-reproducible run to run.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -58,13 +67,13 @@ __device__ __forceinline__ double rdf_k(double u, double sigma) {
 // k_rdf: raw smoothed histograms of the pairs each moving ion heads (a moving partner of
 // higher index, or any fixed partner), added to the ion's own row.
 __global__ __launch_bounds__(256) void k_rdf(AdjParams A) {
-    extern __shared__ double s_h[];
+    extern __shared__ unsigned long long s_h[];   // [nh][2] fixed-point bins
     const StepParams &P = A.sp;
     const int nf0 = P.nsites + P.nfix;
     const int i = nf0 + blockIdx.x;
     const int tid = threadIdx.x;
     const int nh = A.ngroup * A.nbins, nt1 = P.nttyp + 1;
-    for (int k = tid; k < nh; k += blockDim.x) s_h[k] = 0.0;
+    for (int k = tid; k < 2 * nh; k += blockDim.x) s_h[k] = 0ull;
     __syncthreads();
     const double4 xi = A.x[i];
     const int ti = __double2loint(xi.w);
@@ -79,12 +88,12 @@ __global__ __launch_bounds__(256) void k_rdf(AdjParams A) {
         const int b1 = min(A.nbins - 1, (int)floor((r + A.cut - A.r0) / A.dr - 0.5));
         for (int b = b0; b <= b1; b++) {
             const double u = r - (A.r0 + (b + 0.5) * A.dr);
-            if (fabs(u) <= A.cut) atomicAdd(&s_h[g * A.nbins + b], rdf_k(u, A.sigma));
+            if (fabs(u) <= A.cut) fix_add<2>(&s_h[2 * (g * A.nbins + b)], rdf_k(u, A.sigma));
         }
     }
     __syncthreads();
     double *row = A.hist + (size_t)i * nh;
-    for (int k = tid; k < nh; k += blockDim.x) row[k] += s_h[k];
+    for (int k = tid; k < nh; k += blockDim.x) row[k] += fix_value(&s_h[2 * k], 2);
 }
 
 // k_adj_force: F(x_s) of each moving ion, pairs in index order
@@ -152,8 +161,11 @@ __global__ __launch_bounds__(256) void k_adj_pairs(AdjParams A) {
     const int i = nf0 + blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nt1 = P.nttyp + 1;
+    __shared__ unsigned long long s_sc[MAX_LDS_TYPES][2][ADJ_W];   // wall / tail by partner type
     const double4 xi = A.x[i], mi = A.mu[i];
     const int ti = __double2loint(xi.w);
+    for (int k = tid; k < MAX_LDS_TYPES * 2 * ADJ_W; k += blockDim.x) (&s_sc[0][0][0])[k] = 0ull;
+    __syncthreads();
     double hx = 0.0, hy = 0.0, hz = 0.0, ox = 0.0, oy = 0.0, oz = 0.0;
     const double is2 = 1.0 / (A.sigma * A.sigma);
     for (int j = tid; j < P.ntot; j += blockDim.x) {
@@ -178,13 +190,14 @@ __global__ __launch_bounds__(256) void k_adj_pairs(AdjParams A) {
         // parameter gradient, once per pair
         if (!jmov || j < i) {
             if (kind == 1) {
-                atomicAdd(&A.gep[2 * (size_t)k], -s / r);
-                atomicAdd(&A.gep[2 * (size_t)k + 1], -2.0 * s);
+                fix_add<ADJ_W>(&A.gep[2 * ADJ_W * (size_t)k], -s / r);
+                fix_add<ADJ_W>(&A.gep[2 * ADJ_W * (size_t)k + ADJ_W], -2.0 * s);
             } else {
-                const int is = P.pairidx[ti * nt1 + tj];
+                // wall and tail terms of the block's pairs gather by partner type in LDS
+                // first: one global sum per pair type and block instead of one per pair
                 const double ix2 = 1.0 / d2;
-                if (kind == 0) atomicAdd(&A.gsc[2 * is], 12.0 * pow6(ix2) * ix2 * s);
-                else atomicAdd(&A.gsc[2 * is + 1], 6.0 * pow3(ix2) * ix2 * s);
+                if (kind == 0) fix_add<ADJ_W>(&s_sc[tj][0][0], 12.0 * pow6(ix2) * ix2 * s);
+                else fix_add<ADJ_W>(&s_sc[tj][1][0], 6.0 * pow3(ix2) * ix2 * s);
             }
         }
         // RDF-loss term: d/dx_i of the group's smoothed histogram, weighted
@@ -211,12 +224,115 @@ __global__ __launch_bounds__(256) void k_adj_pairs(AdjParams A) {
         if (lane == 0) s_r[c][wv] = v[c];
     }
     __syncthreads();
+    if (tid < nt1 * 2) {
+        // this block's wall (q = 0) and tail (q = 1) sums of partner type tj, into the pair
+        // type's fixed-point gradient (integer sums: the order of blocks does not matter)
+        const int tj = tid >> 1, q = tid & 1;
+        unsigned long long v[ADJ_W];
+        bool any = false;
+        for (int w = 0; w < ADJ_W; w++) { v[w] = s_sc[tj][q][w]; any |= v[w] != 0ull; }
+        if (any) {
+            fix_carry<ADJ_W>(v);
+            const int is = P.pairidx[ti * nt1 + tj];
+            for (int w = 0; w < ADJ_W; w++)
+                atomicAdd(&A.gsc[ADJ_W * (2 * (size_t)is + q) + w], v[w]);
+        }
+    }
     if (tid == 0) {
         double t[6];
         for (int c = 0; c < 6; c++) t[c] = s_r[c][0] + s_r[c][1] + s_r[c][2] + s_r[c][3];
         const double4 l1 = A.lam1[i], db = A.dbox[i];
+        // flux loss at the last state: d(charge displacement)/dz_i = q_i
+        const double zf = A.zflux != 0.0 ? A.zflux * P.cg[abs(P.typei[i])] : 0.0;
         A.lam0[i] = make_double4(db.x * l1.x + t[0] + t[3], db.y * l1.y + t[1] + t[4],
-                                 db.z * l1.z + t[2] + t[5], 0.0);
+                                 db.z * l1.z + t[2] + t[5] + zf, 0.0);
+    }
+}
+
+// k_tlm: the tangent-linear step of moving ion i.  Gathers dF_i = (H dx)_i + (dF/dth v)_i
+// over every partner, and the derivative of the smoothed histograms along dx for the
+// pairs ion i heads (as k_rdf counts them) when obs is set; writes dx' = D_box dx +
+// mu (.) dF, with mu = f1 D_bd D_box from k_adj_mu (run with lambda' = 1).
+__global__ __launch_bounds__(256) void k_tlm(AdjParams A) {
+    extern __shared__ unsigned long long s_dh[];   // [nh][ADJ_W]
+    __shared__ double s_r[3][4];
+    const StepParams &P = A.sp;
+    const int nf0 = P.nsites + P.nfix;
+    const int i = nf0 + blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nt1 = P.nttyp + 1, nh = A.ngroup * A.nbins;
+    if (A.obs)
+        for (int k = tid; k < ADJ_W * nh; k += blockDim.x) s_dh[k] = 0ull;
+    __syncthreads();
+    const double4 xi = A.x[i], di = A.dx[i];
+    const int ti = __double2loint(xi.w);
+    const double is2 = 1.0 / (A.sigma * A.sigma);
+    double hx = 0.0, hy = 0.0, hz = 0.0;
+    for (int j = tid; j < P.ntot; j += blockDim.x) {
+        if (j == i) continue;
+        const double4 xj = A.x[j];
+        const int tj = __double2loint(xj.w);
+        const bool jmov = j >= nf0;
+        const double4 dj = jmov ? A.dx[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const double r = sqrt(d2);
+        const DevPair &p = P.rows[ti * nt1 + tj];
+        double phi, dphi;
+        int kind, k;
+        pair_phi(p, P.ep, P.ires, d2, r, phi, dphi, kind, k);
+        // the parameters' share: d phi = (d phi / d th) . v
+        double vphi;
+        if (kind == 1) {
+            vphi = -A.vep[2 * (size_t)k] / r - 2.0 * A.vep[2 * (size_t)k + 1];
+        } else {
+            const int is = P.pairidx[ti * nt1 + tj];
+            const double ix2 = 1.0 / d2;
+            vphi = kind == 0 ? 12.0 * pow6(ix2) * ix2 * A.vsc[2 * is]
+                             : 6.0 * pow3(ix2) * ix2 * A.vsc[2 * is + 1];
+        }
+        const double ux = di.x - dj.x, uy = di.y - dj.y, uz = di.z - dj.z;
+        const double s = ux * dx + uy * dy + uz * dz;
+        const double q = dphi / r * s;
+        hx += phi * ux + q * dx + vphi * dx;
+        hy += phi * uy + q * dy + vphi * dy;
+        hz += phi * uz + q * dz + vphi * dz;
+        if (A.obs && !(jmov && j < i)) {
+            const int g = A.group[ti * nt1 + tj];
+            if (g >= 0) {
+                const int b0 = max(0, (int)ceil((r - A.cut - A.r0) / A.dr - 0.5));
+                const int b1 = min(A.nbins - 1, (int)floor((r + A.cut - A.r0) / A.dr - 0.5));
+                const double sr = s / r;
+                for (int b = b0; b <= b1; b++) {
+                    const double u = r - (A.r0 + (b + 0.5) * A.dr);
+                    if (fabs(u) <= A.cut)
+                        fix_add<ADJ_W>(&s_dh[ADJ_W * (g * A.nbins + b)],
+                                       -u * is2 * rdf_k(u, A.sigma) * sr);
+                }
+            }
+        }
+    }
+    hx = wave_sum(hx); hy = wave_sum(hy); hz = wave_sum(hz);
+    if (lane == 0) { s_r[0][wv] = hx; s_r[1][wv] = hy; s_r[2][wv] = hz; }
+    __syncthreads();
+    if (A.obs) {
+        for (int k = tid; k < nh; k += blockDim.x) {
+            unsigned long long v[ADJ_W];
+            bool any = false;
+            for (int w = 0; w < ADJ_W; w++) { v[w] = s_dh[ADJ_W * k + w]; any |= v[w] != 0ull; }
+            if (any) {
+                fix_carry<ADJ_W>(v);
+                for (int w = 0; w < ADJ_W; w++) atomicAdd(&A.dh[ADJ_W * (size_t)k + w], v[w]);
+            }
+        }
+    }
+    if (tid == 0) {
+        const double fx = s_r[0][0] + s_r[0][1] + s_r[0][2] + s_r[0][3];
+        const double fy = s_r[1][0] + s_r[1][1] + s_r[1][2] + s_r[1][3];
+        const double fz = s_r[2][0] + s_r[2][1] + s_r[2][2] + s_r[2][3];
+        const double4 m = A.mu[i], db = A.dbox[i];
+        A.dxo[i] = make_double4(db.x * di.x + m.x * fx, db.y * di.y + m.y * fy,
+                                db.z * di.z + m.z * fz, 0.0);
     }
 }
 
@@ -225,13 +341,18 @@ hipError_t launch_adj(int which, const AdjParams &a, hipStream_t s) {
     if (nmov <= 0) return hipSuccess;
     switch (which) {
         case 0: {
-            const size_t lds = sizeof(double) * (size_t)a.ngroup * a.nbins;
+            const size_t lds = 2 * sizeof(unsigned long long) * (size_t)a.ngroup * a.nbins;
             hipLaunchKernelGGL(k_rdf, dim3(nmov), dim3(256), lds, s, a);
             break;
         }
         case 1: hipLaunchKernelGGL(k_adj_force, dim3(nmov), dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL(k_adj_mu, dim3((nmov + 255) / 256), dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_adj_pairs, dim3(nmov), dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_adj_pairs, dim3(nmov), dim3(256), 0, s, a); break;
+        default: {
+            const size_t lds = a.obs ? ADJ_W * sizeof(unsigned long long) * (size_t)a.ngroup * a.nbins : 0;
+            hipLaunchKernelGGL(k_tlm, dim3(nmov), dim3(256), lds, s, a);
+            break;
+        }
     }
     return hipGetLastError();
 }

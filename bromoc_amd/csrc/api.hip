@@ -72,6 +72,9 @@ struct bromoc_ctx {
     double r2near = 0.0;
     int cur = 0;
     int tpb = 0;                             // targets per block chosen by the autotuner
+                                             // (TPB_TILE + targets per lane: tile mode)
+    double *tpart = nullptr, *tener = nullptr;   // tile-mode partial sums and pair energies
+    size_t tpart_n = 0, tener_n = 0;
     int32_t *warn_scratch = nullptr;
     int32_t *typei = nullptr, *t2 = nullptr, *namsite = nullptr, *nwtype = nullptr;
     int8_t *stfree = nullptr;
@@ -110,6 +113,19 @@ struct bromoc_ctx {
     std::vector<int32_t> ibfftyp, ntotat, ibuffer;
     std::vector<double> avnum, mu, kb, lzmin, lzmax;
     std::vector<int8_t> bufferbias;
+    // device GRAND: buffer parameters, nat, prob, the buffer of each slot, state, the
+    // per-workgroup INTERACT partials and the caller's uniforms
+    struct GcDev {
+        int nb = -1;
+        double *avnum = nullptr, *mu = nullptr, *kb = nullptr, *lzmin = nullptr, *lzmax = nullptr;
+        int32_t *ibfftyp = nullptr, *ntotat = nullptr, *nat = nullptr, *prob = nullptr;
+        int32_t *ibuffer = nullptr;
+        int8_t *bias = nullptr;
+        GcState *st = nullptr;
+        double *part = nullptr, *u = nullptr;
+        int64_t ucap = 0;
+        std::vector<void *> mem;
+    } gd;
     StepParams base{};
     // per-cycle analyses: integer histograms (freed on re-set), the per-particle virial
     // sums of the running bromoc_step call, their fold, and the steps folded so far
@@ -399,7 +415,48 @@ StepParams params_for(bromoc_ctx *c) {
     p.spos = c->spos[c->cur];
     p.sposo = c->spos[c->cur ^ 1];
     p.tpb = c->tpb > 0 ? c->tpb : 4;
+    if (tile_mode(p.tpb)) {
+        const int tt = 64 * (p.tpb - TPB_TILE);
+        p.ttpl = p.tpb - TPB_TILE;
+        p.ntile = (p.ntot + tt - 1) / tt;
+        p.titems = p.ntile * (p.ntile + 1) / 2;
+        p.tnc = p.rfon ? 5 : 3;
+        p.tpart = c->tpart;
+        p.tener = c->tener;
+    }
     return p;
+}
+
+// Tile-mode buffers for the context's capacity at ttpl targets per lane (kept; grown when
+// a larger shape needs more).  Fails with BROMOC_E_CAPACITY when they would pass
+// TILE_MAX_BYTES.
+constexpr size_t TILE_MAX_BYTES = size_t(3) << 30;
+int ensure_tiles(bromoc_ctx *c, int ttpl) {
+    const size_t np = tile_part_doubles(c->cap, ttpl, c->base.rfon);
+    const size_t ne = tile_ener_doubles(c->cap, ttpl);
+    if (8 * (np + ne) > TILE_MAX_BYTES)
+        return fail(c, BROMOC_E_CAPACITY, "tile buffers beyond their limit");
+    if (np > c->tpart_n) {
+        if (c->tpart) (void)hipFree(c->tpart);
+        c->tpart = nullptr;
+        c->tpart_n = 0;
+        HIPCHK(c, hipMalloc(&c->tpart, 8 * np));
+        c->tpart_n = np;
+    }
+    if (ne > c->tener_n) {
+        if (c->tener) (void)hipFree(c->tener);
+        c->tener = nullptr;
+        c->tener_n = 0;
+        HIPCHK(c, hipMalloc(&c->tener, 8 * ne));
+        c->tener_n = ne;
+    }
+    return BROMOC_OK;
+}
+
+// tile mode is not offered where the step kernel does more than the tile kernels: the
+// nearest-site records of DYNAMICS3 with DNA
+bool tiles_allowed(const bromoc_ctx *c) {
+    return !(c->base.dmode == 3 && c->ns > 0);
 }
 
 // Re-sort the partner array when positions were replaced or after GRAPH_K steps.
@@ -419,7 +476,10 @@ int sync_err(bromoc_ctx *c) {
 }
 
 int reduce_energies(bromoc_ctx *c, bromoc_energies *out) {
-    HIPCHK(c, launch_reduce_energy(c->epart, c->cap, c->ntot, c->ered, c->stream));
+    const StepParams p = params_for(c);
+    const bool tiles = tile_mode(p.tpb);
+    HIPCHK(c, launch_reduce_energy(c->epart, c->cap, c->ntot, c->ered, c->stream,
+                                   tiles ? c->tener : nullptr, tiles ? p.titems : 0));
     double r[NE];
     HIPCHK(c, hipMemcpyAsync(r, c->ered, sizeof(r), hipMemcpyDeviceToHost, c->stream));
     int st = sync_err(c);
@@ -450,7 +510,9 @@ int bromoc_destroy(bromoc_ctx *c) {
     for (auto &g : c->graph)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (void *p : c->allocs) (void)hipFree(p);
-    for (void *p : {(void *)c->an_rho, (void *)c->an_rdna, (void *)c->an_gr, (void *)c->an_prob2,
+    for (void *p : c->gd.mem) (void)hipFree(p);
+    if (c->gd.u) (void)hipFree(c->gd.u);
+    for (void *p : {(void *)c->tpart, (void *)c->tener, (void *)c->an_rho, (void *)c->an_rdna, (void *)c->an_gr, (void *)c->an_prob2,
                     (void *)c->an_prob3})
         if (p) (void)hipFree(p);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -514,8 +576,10 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     ACHK(dalloc(c, &c->sb.vals, cap));
     ACHK(dalloc(c, &c->sb.order, cap));
     ACHK(dalloc(c, &c->sb.rank, cap));
-    c->sb.temp_bytes = sort_temp_bytes(cap);
-    ACHK(dalloc(c, (char **)&c->sb.temp, c->sb.temp_bytes));
+    ACHK(dalloc(c, &c->sb.hist, sort_hist_ints(cap)));
+    ACHK(dalloc(c, &c->sb.cflag, cap));
+    ACHK(dalloc(c, &c->sb.cell_start, (size_t)cap + 1));
+    ACHK(dalloc(c, &c->sb.ncell, 1));
     {
         // Morton cells of 8 A over the box (10 bits per axis)
         SortGeom &g = c->sb.geom;
@@ -899,13 +963,22 @@ static hipError_t enqueue_step(bromoc_ctx *c, const StepParams &p, hipEvent_t t0
                                hipEvent_t t1 = nullptr) {
     const int ns = p.nsites, nt = p.ntargets;
     hipError_t e;
-    if ((e = launch_rfpar(p, true, c->stream)) != hipSuccess) return e;
+    if ((e = launch_rfpar(p, !tile_mode(p.tpb), c->stream, true)) != hipSuccess) return e;
     if ((e = launch_mmij(p, c->stream)) != hipSuccess) return e;
     if ((e = launch_fixed(p, c->stream)) != hipSuccess) return e;
     static const bool fork = [] {
         const char *v = std::getenv("BROMOC_FORK");
         return v && v[0] == '1';
     }();
+    if (tile_mode(p.tpb)) {
+        // timing brackets the pair sweep (k_tiles), the dominant kernel of the step
+        if ((e = launch_dna(p, c->stream)) != hipSuccess) return e;
+        if (t0 && (e = hipEventRecord(t0, c->stream)) != hipSuccess) return e;
+        if ((e = launch_tiles(p, c->stream, 1)) != hipSuccess) return e;
+        if (t1 && (e = hipEventRecord(t1, c->stream)) != hipSuccess) return e;
+        if ((e = launch_tiles(p, c->stream, 2)) != hipSuccess) return e;
+        return launch_notrans(p, c->stream);
+    }
     if (!fork || ns == 0 || nt == ns) {
         if ((e = launch_dna(p, c->stream)) != hipSuccess) return e;
         if (t0 && (e = hipEventRecord(t0, c->stream)) != hipSuccess) return e;
@@ -930,38 +1003,60 @@ static hipError_t enqueue_step(bromoc_ctx *c, const StepParams &p, hipEvent_t t0
     return launch_notrans(p, c->stream);
 }
 
-// Autotuner: targets per block of k_step (1, 2 or 4) for this system, chosen by timing
-// ENERGY-only launches (warnings go to a scratch counter) on the ctx's stream.  The
+// Autotuner: the pair kernel's launch shape for this system -- k_step with 1, 2 or 4
+// targets per block, k_wave (8), or tile mode with 1, 2 or 4 targets per lane (17, 18,
+// 20) -- chosen by timing ENERGY-only launches (warnings go to a scratch counter) on the
+// ctx's stream: the reaction-field kernels the shape needs plus its pair kernels.  The
 // environment variable BROMOC_TPB forces a value.
 static int autotune(bromoc_ctx *c) {
     if (c->tpb > 0) return BROMOC_OK;
     if (const char *env = std::getenv("BROMOC_TPB")) {
         int v = std::atoi(env);
         if (v == 1 || v == 2 || v == 4 || v == 8) { c->tpb = v; return BROMOC_OK; }
+        if ((v == TPB_TILE + 1 || v == TPB_TILE + 2 || v == TPB_TILE + 4) && tiles_allowed(c)) {
+            int st = ensure_tiles(c, v - TPB_TILE);
+            if (st) return st;
+            c->tpb = v;
+            return BROMOC_OK;
+        }
     }
     int sst = ensure_sorted(c);
     if (sst) return sst;
-    const int cand[4] = {1, 2, 4, 8};   // k_step with 1, 2, 4 targets per block; k_wave
-    double best[4] = {1e300, 1e300, 1e300, 1e300};
+    std::vector<int> cand = {1, 2, 4, 8};
+    if (tiles_allowed(c)) {
+        for (int t : {1, 2, 4}) {
+            const size_t bytes = 8 * (tile_part_doubles(c->cap, t, c->base.rfon) +
+                                      tile_ener_doubles(c->cap, t));
+            // a tile must hold enough slots for the shape to be worth timing, and the
+            // partial sums must stay within their limit
+            if (bytes <= TILE_MAX_BYTES && c->ntot >= 64 * t * 4 && ensure_tiles(c, t) == BROMOC_OK)
+                cand.push_back(TPB_TILE + t);
+        }
+    }
+    const int nc = (int)cand.size();
+    std::vector<double> best(nc, 1e300);
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     // three interleaved rounds of five launches per shape; each shape keeps its best round
     // (shapes within a few per cent of each other are told apart reliably this way)
     for (int round = 0; round < 3; round++) {
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < nc; q++) {
+            c->tpb = cand[q];
             StepParams p = params_for(c);
-            p.tpb = cand[q];
+            c->tpb = 0;
             p.mode = 0;
             p.write_energy = 0;
             p.write_forces = 0;
             p.warn = c->warn_scratch;
             // one whole ENERGY (the DNA, reaction-field and fixed-ion terms the pair kernel
-            // reads), then the pair kernel alone is timed
+            // reads), then the shape's reaction-field and pair kernels are timed
             HIPCHK(c, launch_step(p, c->stream));
-            HIPCHK(c, launch_pairs(p, c->stream));
             HIPCHK(c, hipEventRecord(e0, c->stream));
-            for (int r = 0; r < 5; r++) HIPCHK(c, launch_pairs(p, c->stream));
+            for (int r = 0; r < 5; r++) {
+                HIPCHK(c, launch_rfpar(p, !tile_mode(p.tpb), c->stream, true));
+                HIPCHK(c, launch_pairs(p, c->stream));
+            }
             HIPCHK(c, hipEventRecord(e1, c->stream));
             int st = sync_err(c);
             if (st) return st;
@@ -972,7 +1067,7 @@ static int autotune(bromoc_ctx *c) {
     }
     int pick = 4;
     double bt = 1e300;
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < nc; q++)
         if (best[q] < bt) { bt = best[q]; pick = cand[q]; }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1157,6 +1252,30 @@ int bromoc_dynamics(bromoc_ctx *c, int32_t noise_mode, int32_t row, uint64_t see
     return sync_err(c);
 }
 
+int bromoc_spatial_order(bromoc_ctx *c, uint32_t *keys, int32_t *order, int32_t *rank,
+                         int32_t *cell_start, int32_t *ncell) {
+    if (!c) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    int st = ensure_sorted(c, true);
+    if (st) return st;
+    const int n = c->ntot;
+    int32_t nc = 0;
+    HIPCHK(c, hipMemcpyAsync(&nc, c->sb.ncell, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    if (keys) HIPCHK(c, hipMemcpyAsync(keys, c->sb.keys_out, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    if (order) HIPCHK(c, hipMemcpyAsync(order, c->sb.order, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    if (rank) HIPCHK(c, hipMemcpyAsync(rank, c->sb.rank, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    st = sync_err(c);
+    if (st) return st;
+    if (cell_start) {
+        HIPCHK(c, hipMemcpyAsync(cell_start, c->sb.cell_start, sizeof(int32_t) * ((size_t)nc + 1),
+                                 hipMemcpyDeviceToHost, c->stream));
+        st = sync_err(c);
+        if (st) return st;
+    }
+    if (ncell) *ncell = nc;
+    return BROMOC_OK;
+}
+
 int bromoc_get_flux(bromoc_ctx *c, int32_t *nforward, int32_t *nbackward) {
     if (!c || !nforward || !nbackward) return BROMOC_E_ARG;
     (void)hipSetDevice(c->device);
@@ -1338,6 +1457,34 @@ int bromoc_set_gcmc(bromoc_ctx *c, const bromoc_gcmc *g) {
     c->lzmax.assign(g->lzmax, g->lzmax + nb + 1);
     c->bufferbias.assign(g->bufferbias, g->bufferbias + nb + 1);
     c->ibuffer.assign(c->cap + 1, 0);
+    // device copies for the device GRAND (arrays of a previous call are freed here)
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->gd.mem) (void)hipFree(p);
+    c->gd.mem.clear();
+    auto dev = [&](auto **ptr, size_t n, const void *src) -> hipError_t {
+        *ptr = nullptr;
+        hipError_t e = hipMalloc((void **)ptr, sizeof(**ptr) * std::max<size_t>(n, 1));
+        if (e != hipSuccess) return e;
+        c->gd.mem.push_back((void *)*ptr);
+        if (src) return hipMemcpy(*ptr, src, sizeof(**ptr) * n, hipMemcpyHostToDevice);
+        return hipMemset(*ptr, 0, sizeof(**ptr) * std::max<size_t>(n, 1));
+    };
+    const size_t n1 = nb + 1;
+    HIPCHK(c, dev(&c->gd.avnum, n1, nb > 0 ? g->avnum : nullptr));
+    HIPCHK(c, dev(&c->gd.mu, n1, nb > 0 ? g->mu : nullptr));
+    HIPCHK(c, dev(&c->gd.kb, n1, nb > 0 ? g->kb : nullptr));
+    HIPCHK(c, dev(&c->gd.lzmin, n1, nb > 0 ? g->lzmin : nullptr));
+    HIPCHK(c, dev(&c->gd.lzmax, n1, nb > 0 ? g->lzmax : nullptr));
+    HIPCHK(c, dev(&c->gd.ibfftyp, n1, nb > 0 ? g->ibfftyp : nullptr));
+    HIPCHK(c, dev(&c->gd.ntotat, n1, nb > 0 ? g->ntotat : nullptr));
+    HIPCHK(c, dev(&c->gd.bias, n1, nb > 0 ? g->bufferbias : nullptr));
+    HIPCHK(c, dev(&c->gd.nat, n1, nullptr));
+    HIPCHK(c, dev(&c->gd.prob, (size_t)std::max(nb, 1) * (c->cap + 1), nullptr));
+    HIPCHK(c, dev(&c->gd.ibuffer, (size_t)c->cap + 1, nullptr));
+    HIPCHK(c, dev(&c->gd.st, 1, nullptr));
+    HIPCHK(c, dev(&c->gd.part, GC_BLOCKS, nullptr));
+    c->gd.nb = nb;
     c->have_gcmc = true;
     return BROMOC_OK;
 }
@@ -1356,12 +1503,82 @@ int bromoc_get_ibuffer(bromoc_ctx *c, int32_t *ib) {
     return BROMOC_OK;
 }
 
-// GRAND, grand.f90:40-118 (INSERT insert.f90:29-41 rectangular; FIND find.f90:26-32)
+// GRAND on the device (grand.hip): the caller's uniforms, nat and prob go up, ngcmc
+// attempts run back to back on the stream, and the state comes back in one copy.
+static int grand_device(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu,
+                        int64_t *used, int32_t *nat, int32_t *prob, int64_t icycle,
+                        double *ener) {
+    const bromoc_gcmc &g = c->g;
+    const int nb = g.nbuffer;
+    const int pstride = c->cap + 1;
+    auto &D = c->gd;
+    if (nu > D.ucap) {
+        if (D.u) (void)hipFree(D.u);
+        D.u = nullptr;
+        D.ucap = 0;
+        HIPCHK(c, hipMalloc(&D.u, sizeof(double) * nu));
+        D.ucap = nu;
+    }
+    const int n0 = c->ntot;
+    if (nu > 0) HIPCHK(c, hipMemcpyAsync(D.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.nat, nat, sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
+    if (nb > 0)
+        HIPCHK(c, hipMemcpyAsync(D.prob, prob, sizeof(int32_t) * (size_t)nb * pstride, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.ibuffer, c->ibuffer.data(), sizeof(int32_t) * n0, hipMemcpyHostToDevice, c->stream));
+    GcState h{};
+    h.k = 0; h.ntot = n0; h.err = GCE_OK; h.ener = *ener;
+    HIPCHK(c, hipMemcpyAsync(D.st, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+    GcParams G{};
+    G.sp = params_for(c);
+    G.pos0 = c->pos[0]; G.pos1 = c->pos[1];
+    G.typei = c->typei; G.t2 = c->t2; G.ibuffer = D.ibuffer;
+    G.u = D.u; G.nu = nu;
+    G.nbuffer = nb; G.pstride = pstride;
+    G.nat = D.nat; G.prob = D.prob;
+    G.avnum = D.avnum; G.mu = D.mu; G.kb = D.kb; G.lzmin = D.lzmin; G.lzmax = D.lzmax;
+    G.ibfftyp = D.ibfftyp; G.ntotat = D.ntotat; G.bias = D.bias;
+    G.lx = g.lx; G.ly = g.ly; G.cx = g.cx; G.cy = g.cy; G.cz = g.cz; G.ikbt = g.ikbt;
+    G.icycle = icycle;
+    G.cap = c->cap;
+    G.st = D.st; G.part = D.part;
+    for (int ig = 0; ig < ngcmc; ig++) HIPCHK(c, launch_grand_attempt(G, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&h, D.st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    int st = sync_err(c);
+    if (st) return st;
+    const int n1 = h.ntot;
+    HIPCHK(c, hipMemcpyAsync(nat, D.nat, sizeof(int32_t) * (nb + 1), hipMemcpyDeviceToHost, c->stream));
+    if (nb > 0)
+        HIPCHK(c, hipMemcpyAsync(prob, D.prob, sizeof(int32_t) * (size_t)nb * pstride, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->ibuffer.data(), D.ibuffer, sizeof(int32_t) * n1, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_typei.data(), c->typei, sizeof(int32_t) * n1, hipMemcpyDeviceToHost, c->stream));
+    st = sync_err(c);
+    if (st) return st;
+    if (h.nacc > 0) c->sort_dirty = true;
+    c->ntot = n1;
+    *ener = h.ener;
+    if (used) *used = h.k;
+    switch (h.err) {
+        case GCE_UNIFORMS: return fail(c, BROMOC_E_ARG, "uniform stream exhausted");
+        case GCE_CAPACITY: return fail(c, BROMOC_E_CAPACITY, "ntot exceeds capacity");
+        case GCE_STATE: return fail(c, BROMOC_E_STATE, "buffer bookkeeping inconsistent");
+        default: return BROMOC_OK;
+    }
+}
+
+// GRAND, grand.f90:40-118 (INSERT insert.f90:29-41 rectangular; FIND find.f90:26-32).  On
+// the device (grand_device) except for MMIJ systems, whose INTERACT term (rect_rf0.f90)
+// runs as its own kernel sequence: their attempts go through the host loop below.
 int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int64_t *used,
                  int32_t *nat, int32_t *prob, int64_t icycle, double *ener) {
     if (!c || !u || !nat || !prob || !ener || ngcmc < 0 || icycle <= 0) return BROMOC_E_ARG;
     if (!c->have_gcmc) return fail(c, BROMOC_E_STATE, "bromoc_set_gcmc first");
     (void)hipSetDevice(c->device);
+    static const bool host_grand = [] {
+        const char *v = std::getenv("BROMOC_HOST_GRAND");
+        return v && v[0] == '1';
+    }();
+    if (!c->base.mmon && !host_grand)
+        return grand_device(c, ngcmc, u, nu, used, nat, prob, icycle, ener);
     const bromoc_gcmc &g = c->g;
     const int nbuffer = g.nbuffer;
     const double nbufferr = (double)nbuffer;
@@ -1771,75 +1988,90 @@ int bromoc_table_vjp(int32_t n, const double *r, const double *grad_ep, const do
 }
 
 // ---- trajectory adjoint of an RDF loss (kernels in adjoint.hip)
-int bromoc_adjoint_rdf(bromoc_ctx *c, int32_t nsteps, int32_t K, const bromoc_rdf *rdf,
-                       double *loss, double *rdf_out, double *grad_ep, double *grad_sc) {
-    if (!c || nsteps < 1 || K < 1 || !rdf || !loss || rdf->ngroup < 1 || rdf->nbins < 1 ||
-        rdf->ngroup * (int64_t)rdf->nbins > 4096 || !rdf->groups || !(rdf->dr > 0.0) ||
-        !(rdf->sigma > 0.0) || !(rdf->cut >= 0.0))
-        return BROMOC_E_ARG;
-    (void)hipSetDevice(c->device);
-    const bromoc_system &S = c->sys;
-    const uint32_t allowed = BROMOC_QPAR | BROMOC_QNONBOND | BROMOC_QNOBOND | BROMOC_QCOUNTION;
-    if ((S.flags & ~allowed) || c->ns != 0 || !c->base.all_efpot)
-        return fail(c, BROMOC_E_UNSUPPORTED,
-                    "adjoint: moving ions with tabulated pairs only (no grids, CG-DNA, RFPAR, "
-                    "diffusion profiles)");
-    const int nt = c->ntot, nt1 = S.nttyp + 1, nf0 = c->nf;
-    if (!c->noise || c->noise_rows < nsteps || c->noise_stride < 3 * (int64_t)nt)
-        return fail(c, BROMOC_E_STATE, "adjoint: injected noise rows 0..nsteps-1 required");
-    const int ng = rdf->ngroup, nb = rdf->nbins, nh = ng * nb;
-    // RDF groups by atnam2 type, and each group's pair count
-    std::vector<int32_t> group((size_t)nt1 * nt1, -1);
-    std::vector<int64_t> nmov(S.ntype + 1, 0), nfix(S.ntype + 1, 0);
-    for (int i = 0; i < nt; i++) (i < nf0 ? nfix : nmov)[std::abs(c->h_typei[i])]++;
-    std::vector<double> npairs(ng, 0.0);
-    for (int g = 0; g < ng; g++) {
-        const int a = rdf->groups[2 * g], b = rdf->groups[2 * g + 1];
-        if (a <= S.nold || a > S.ntype || b <= S.nold || b > S.ntype)
-            return fail(c, BROMOC_E_ARG, "adjoint: RDF group type is not an ion type");
-        const int ta = c->h_nwtype[a], tb = c->h_nwtype[b];
-        if (group[ta * nt1 + tb] >= 0) return fail(c, BROMOC_E_ARG, "adjoint: RDF groups overlap");
-        group[ta * nt1 + tb] = group[tb * nt1 + ta] = g;
-        double pc = a == b ? 0.5 * (double)nmov[a] * (double)(nmov[a] - 1)
-                           : (double)nmov[a] * (double)nmov[b];
-        pc += (double)nmov[a] * (double)nfix[b] + (a != b ? (double)nmov[b] * (double)nfix[a] : 0.0);
-        npairs[g] = pc;
-    }
-    int st = autotune(c);
-    if (st) return st;
-    const int nseg = (nsteps + K - 1) / K;
-    // scratch of this call
+namespace {
+
+// What the adjoint and the tangent pass share: the checks, the RDF groups and each group's
+// pair count, scratch buffers, the forward steps and the loss.
+struct AdjRun {
+    bromoc_ctx *c;
+    int nsteps, K, nt, nt1, nf0, ng, nb, nh, nk, nseg;
+    const bromoc_rdf *rdf;
+    std::vector<int32_t> group;
+    std::vector<double> npairs;
     std::vector<void *> tmp;
-    auto tfree = [&]() { for (void *q : tmp) (void)hipFree(q); tmp.clear(); };
-    auto talloc = [&](void **q, size_t bytes) -> hipError_t {
+    int32_t *d_group = nullptr;
+    double4 *ckpt = nullptr;
+    double *hist = nullptr;
+    AdjParams A{};
+
+    ~AdjRun() { for (void *q : tmp) (void)hipFree(q); }
+
+    hipError_t alloc(void **q, size_t bytes) {
         hipError_t e = hipMalloc(q, std::max<size_t>(bytes, 8));
-        if (e == hipSuccess) { tmp.push_back(*q); e = hipMemset(*q, 0, std::max<size_t>(bytes, 8)); }
+        if (e == hipSuccess) {
+            tmp.push_back(*q);
+            e = hipMemsetAsync(*q, 0, std::max<size_t>(bytes, 8), c->stream);
+        }
         return e;
-    };
-#define TCHK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) { tfree(); return fail(c, BROMOC_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); } } while (0)
-    const size_t v4 = sizeof(double4) * (size_t)nt;
-    double4 *ckpt, *traj, *xS, *F, *mu, *dbox, *lam[2];
-    double *hist, *d_w, *gep, *gsc;
-    int32_t *d_group;
-    TCHK(talloc((void **)&ckpt, v4 * nseg));
-    TCHK(talloc((void **)&traj, v4 * K));
-    TCHK(talloc((void **)&xS, v4));
-    TCHK(talloc((void **)&F, v4));
-    TCHK(talloc((void **)&mu, v4));
-    TCHK(talloc((void **)&dbox, v4));
-    TCHK(talloc((void **)&lam[0], v4));
-    TCHK(talloc((void **)&lam[1], v4));
-    TCHK(talloc((void **)&hist, sizeof(double) * (size_t)nt * nh));
-    TCHK(talloc((void **)&d_w, sizeof(double) * nh));
-    const int nk = S.nknots + 2;
-    TCHK(talloc((void **)&gep, sizeof(double) * 2 * (size_t)nk));
-    TCHK(talloc((void **)&gsc, sizeof(double) * 2 * (size_t)(S.npair + 1)));
-    TCHK(talloc((void **)&d_group, sizeof(int32_t) * group.size()));
-    TCHK(hipMemcpy(d_group, group.data(), sizeof(int32_t) * group.size(), hipMemcpyHostToDevice));
-    // injected noise row s of step s: ctl = {0, 0, ...} and step_off = s
-    c->h_ctl[0] = 0; c->h_ctl[1] = 0; c->h_ctl[2] = 0; c->h_ctl[3] = 0;
-    TCHK(hipMemcpyAsync(c->ctl, c->h_ctl, sizeof(c->h_ctl), hipMemcpyHostToDevice, c->stream));
-    auto step_params = [&](int s) {
+    }
+
+    int setup(bromoc_ctx *c_, int32_t nsteps_, int32_t K_, const bromoc_rdf *rdf_) {
+        c = c_; nsteps = nsteps_; K = K_; rdf = rdf_;
+        if (nsteps < 1 || K < 1 || !rdf || rdf->ngroup < 1 || rdf->nbins < 1 ||
+            rdf->ngroup * (int64_t)rdf->nbins > 4096 || !rdf->groups || !(rdf->dr > 0.0) ||
+            !(rdf->sigma > 0.0) || !(rdf->cut >= 0.0) || !(rdf->wflux >= 0.0))
+            return BROMOC_E_ARG;
+        const bromoc_system &S = c->sys;
+        const uint32_t allowed = BROMOC_QPAR | BROMOC_QNONBOND | BROMOC_QNOBOND | BROMOC_QCOUNTION;
+        if ((S.flags & ~allowed) || c->ns != 0 || !c->base.all_efpot)
+            return fail(c, BROMOC_E_UNSUPPORTED,
+                        "adjoint: moving ions with tabulated pairs only (no grids, CG-DNA, RFPAR, "
+                        "diffusion profiles)");
+        if (rdf->wflux > 0.0 && !(c->base.dt > 0.0))
+            return fail(c, BROMOC_E_ARG, "adjoint: the flux term needs the time step dt");
+        nt = c->ntot; nt1 = S.nttyp + 1; nf0 = c->nf;
+        if (nt1 > MAX_LDS_TYPES) return fail(c, BROMOC_E_UNSUPPORTED, "adjoint: too many types");
+        if (!c->noise || c->noise_rows < nsteps || c->noise_stride < 3 * (int64_t)nt)
+            return fail(c, BROMOC_E_STATE, "adjoint: injected noise rows 0..nsteps-1 required");
+        ng = rdf->ngroup; nb = rdf->nbins; nh = ng * nb;
+        group.assign((size_t)nt1 * nt1, -1);
+        std::vector<int64_t> nmov(S.ntype + 1, 0), nfix(S.ntype + 1, 0);
+        for (int i = 0; i < nt; i++) (i < nf0 ? nfix : nmov)[std::abs(c->h_typei[i])]++;
+        npairs.assign(ng, 0.0);
+        for (int g = 0; g < ng; g++) {
+            const int a = rdf->groups[2 * g], b = rdf->groups[2 * g + 1];
+            if (a <= S.nold || a > S.ntype || b <= S.nold || b > S.ntype)
+                return fail(c, BROMOC_E_ARG, "adjoint: RDF group type is not an ion type");
+            const int ta = c->h_nwtype[a], tb = c->h_nwtype[b];
+            if (group[ta * nt1 + tb] >= 0) return fail(c, BROMOC_E_ARG, "adjoint: RDF groups overlap");
+            group[ta * nt1 + tb] = group[tb * nt1 + ta] = g;
+            double pc = a == b ? 0.5 * (double)nmov[a] * (double)(nmov[a] - 1)
+                               : (double)nmov[a] * (double)nmov[b];
+            pc += (double)nmov[a] * (double)nfix[b] + (a != b ? (double)nmov[b] * (double)nfix[a] : 0.0);
+            npairs[g] = pc;
+        }
+        int st = autotune(c);
+        if (st) return st;
+        nseg = (nsteps + K - 1) / K;
+        nk = S.nknots + 2;
+        const size_t v4 = sizeof(double4) * (size_t)nt;
+        HIPCHK(c, alloc((void **)&ckpt, v4 * nseg));
+        HIPCHK(c, alloc((void **)&hist, sizeof(double) * (size_t)nt * nh));
+        HIPCHK(c, alloc((void **)&d_group, sizeof(int32_t) * group.size()));
+        HIPCHK(c, hipMemcpyAsync(d_group, group.data(), sizeof(int32_t) * group.size(),
+                                 hipMemcpyHostToDevice, c->stream));
+        // injected noise row s of step s: ctl = {0, 0, ...} and step_off = s
+        c->h_ctl[0] = 0; c->h_ctl[1] = 0; c->h_ctl[2] = 0; c->h_ctl[3] = 0;
+        HIPCHK(c, hipMemcpyAsync(c->ctl, c->h_ctl, sizeof(c->h_ctl), hipMemcpyHostToDevice, c->stream));
+        A.sp = params_for(c);
+        A.group = d_group;
+        A.ngroup = ng; A.nbins = nb;
+        A.r0 = rdf->r0; A.dr = rdf->dr; A.sigma = rdf->sigma; A.cut = rdf->cut;
+        A.hist = hist;
+        return BROMOC_OK;
+    }
+
+    StepParams step_params(int s) {
         StepParams p = params_for(c);
         p.mode = 1;
         p.write_energy = 0;
@@ -1850,64 +2082,129 @@ int bromoc_adjoint_rdf(bromoc_ctx *c, int32_t nsteps, int32_t K, const bromoc_rd
         p.step_off = s;
         p.cntpts = 0;          // the trajectory is replayed: no flux counting
         return p;
-    };
-    AdjParams A{};
-    A.sp = params_for(c);
-    A.group = d_group;
-    A.ngroup = ng; A.nbins = nb;
-    A.r0 = rdf->r0; A.dr = rdf->dr; A.sigma = rdf->sigma; A.cut = rdf->cut;
-    A.w = d_w; A.hist = hist; A.gep = gep; A.gsc = gsc;
-    A.f = F; A.mu = mu; A.dbox = dbox;
-    // ---- forward: checkpoints every K steps, smoothed RDF of every state x_1..x_nsteps
-    for (int s = 0; s < nsteps; s++) {
+    }
+
+    // step s of the forward pass, from the current positions (the partner order is rebuilt
+    // at every checkpoint, so a segment recomputed from its checkpoint is bitwise the same)
+    int forward_step(int s) {
         if (s % K == 0) {
-            TCHK(hipMemcpyAsync(ckpt + (size_t)(s / K) * nt, c->pos[c->cur], v4, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(ckpt + (size_t)(s / K) * nt, c->pos[c->cur],
+                                     sizeof(double4) * (size_t)nt, hipMemcpyDeviceToDevice, c->stream));
             int sst = ensure_sorted(c, true);
-            if (sst) { tfree(); return sst; }
+            if (sst) return sst;
         }
-        TCHK(launch_step(step_params(s), c->stream));
+        HIPCHK(c, launch_step(step_params(s), c->stream));
         c->cur ^= 1;
         A.x = c->pos[c->cur];
-        TCHK(launch_adj(0, A, c->stream));
+        HIPCHK(c, launch_adj(0, A, c->stream));
+        return BROMOC_OK;
     }
-    TCHK(hipMemcpyAsync(xS, c->pos[c->cur], v4, hipMemcpyDeviceToDevice, c->stream));
-    std::vector<double> H((size_t)nt * nh);
-    TCHK(hipMemcpyAsync(H.data(), hist, sizeof(double) * H.size(), hipMemcpyDeviceToHost, c->stream));
-    st = sync_err(c);
-    if (st) { tfree(); return st; }
-    const double PI_ = 3.14159265358979323846264338327950288419716939937510;
-    const double V = (S.lx2p - S.lx2m) * (S.ly2p - S.ly2m) * (S.lz2p - S.lz2m);
-    std::vector<double> g(nh, 0.0), w(nh, 0.0);
-    double L = 0.0;
-    for (int k = 0; k < nh; k++) {
-        double h = 0.0;
-        for (int i = nf0; i < nt; i++) h += H[(size_t)i * nh + k];
-        const int gi = k / nb, b = k % nb;
-        const double rb = rdf->r0 + (b + 0.5) * rdf->dr;
-        const double cn = npairs[gi] > 0.0
-            ? V / ((double)nsteps * npairs[gi] * 4.0 * PI_ * rb * rb * rdf->sigma * std::sqrt(2.0 * PI_))
-            : 0.0;
-        g[k] = cn * h;
-        const double res = g[k] - (rdf->target ? rdf->target[k] : 0.0);
-        L += res * res;
-        w[k] = 2.0 * res * cn;
+
+    // the loss of the forward pass (positions at their last state), its per-bin weights
+    // dL/dH, and the flux term's coefficient dL/dJ / (nsteps dt)
+    int loss(double *L, double *rdf_out, std::vector<double> &w, double &zcoef) {
+        std::vector<double> H((size_t)nt * nh);
+        std::vector<double4> x0(nt), xT(nt);
+        HIPCHK(c, hipMemcpyAsync(H.data(), hist, sizeof(double) * H.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(x0.data(), ckpt, sizeof(double4) * nt, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(xT.data(), c->pos[c->cur], sizeof(double4) * nt, hipMemcpyDeviceToHost, c->stream));
+        int st = sync_err(c);
+        if (st) return st;
+        const bromoc_system &S = c->sys;
+        const double PI_ = 3.14159265358979323846264338327950288419716939937510;
+        const double V = (S.lx2p - S.lx2m) * (S.ly2p - S.ly2m) * (S.lz2p - S.lz2m);
+        std::vector<double> g(nh, 0.0);
+        w.assign(nh, 0.0);
+        double l = 0.0;
+        for (int k = 0; k < nh; k++) {
+            double h = 0.0;
+            for (int i = nf0; i < nt; i++) h += H[(size_t)i * nh + k];
+            const int gi = k / nb, b = k % nb;
+            const double rb = rdf->r0 + (b + 0.5) * rdf->dr;
+            const double cn = npairs[gi] > 0.0
+                ? V / ((double)nsteps * npairs[gi] * 4.0 * PI_ * rb * rb * rdf->sigma * std::sqrt(2.0 * PI_))
+                : 0.0;
+            g[k] = cn * h;
+            const double res = g[k] - (rdf->target ? rdf->target[k] : 0.0);
+            l += res * res;
+            w[k] = 2.0 * res * cn;
+        }
+        zcoef = 0.0;
+        if (rdf->wflux > 0.0) {
+            double q = 0.0;
+            for (int i = nf0; i < nt; i++) q += c->h_cg[std::abs(c->h_typei[i])] * (xT[i].z - x0[i].z);
+            const double tdt = (double)nsteps * c->base.dt;
+            const double J = q / tdt;
+            l += rdf->wflux * (J - rdf->jtarget) * (J - rdf->jtarget);
+            zcoef = 2.0 * rdf->wflux * (J - rdf->jtarget) / tdt;
+        }
+        *L = l;
+        if (rdf_out) std::memcpy(rdf_out, g.data(), sizeof(double) * nh);
+        return BROMOC_OK;
     }
-    *loss = L;
-    if (rdf_out) std::memcpy(rdf_out, g.data(), sizeof(double) * nh);
+};
+
+}  // namespace
+
+int bromoc_adjoint_rdf(bromoc_ctx *c, int32_t nsteps, int32_t K, const bromoc_rdf *rdf,
+                       double *loss, double *rdf_out, double *grad_ep, double *grad_sc) {
+    if (!c || !loss) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    const bromoc_system &S = c->sys;
+    AdjRun R;
+    int st = R.setup(c, nsteps, K, rdf);
+    if (st) return st;
+    const int nt = R.nt;
+    const size_t v4 = sizeof(double4) * (size_t)nt;
+    double4 *traj, *xS, *F, *mu, *dbox, *lam[2];
+    double *d_w;
+    unsigned long long *gep, *gsc;
+    HIPCHK(c, R.alloc((void **)&traj, v4 * K));
+    HIPCHK(c, R.alloc((void **)&xS, v4));
+    HIPCHK(c, R.alloc((void **)&F, v4));
+    HIPCHK(c, R.alloc((void **)&mu, v4));
+    HIPCHK(c, R.alloc((void **)&dbox, v4));
+    HIPCHK(c, R.alloc((void **)&lam[0], v4));
+    HIPCHK(c, R.alloc((void **)&lam[1], v4));
+    HIPCHK(c, R.alloc((void **)&d_w, sizeof(double) * R.nh));
+    HIPCHK(c, R.alloc((void **)&gep, sizeof(unsigned long long) * 2 * ADJ_W * (size_t)R.nk));
+    HIPCHK(c, R.alloc((void **)&gsc, sizeof(unsigned long long) * 2 * ADJ_W * (size_t)(S.npair + 1)));
+    AdjParams &A = R.A;
+    A.w = d_w; A.gep = gep; A.gsc = gsc;
+    A.f = F; A.mu = mu; A.dbox = dbox;
+    // ---- forward: checkpoints every K steps, smoothed RDF of every state x_1..x_nsteps
+    for (int s = 0; s < nsteps; s++)
+        if ((st = R.forward_step(s))) return st;
+    HIPCHK(c, hipMemcpyAsync(xS, c->pos[c->cur], v4, hipMemcpyDeviceToDevice, c->stream));
+    std::vector<double> w;
+    double zcoef;
+    if ((st = R.loss(loss, rdf_out, w, zcoef))) return st;
     if (grad_ep || grad_sc) {
-        TCHK(hipMemcpyAsync(d_w, w.data(), sizeof(double) * nh, hipMemcpyHostToDevice, c->stream));
-        // lambda_nsteps = dl/dx at the last state (mu = 0: no step after it)
-        A.x = xS; A.lam1 = lam[0]; A.lam0 = lam[1]; A.obs = 1;
-        TCHK(launch_adj(3, A, c->stream));
+        // the reverse pass is linear in its sources (the bin weights and the flux
+        // coefficient): they go in scaled by a power of two that brings the largest to
+        // about 1, and the gradients come out scaled back, both exactly, so the fixed-point
+        // sums keep their relative precision whatever the loss's size
+        double big = std::fabs(zcoef);
+        for (double v : w) big = std::max(big, std::fabs(v));
+        const int sc = big > 0.0 ? -std::ilogb(big) : 0;
+        for (double &v : w) v = std::ldexp(v, sc);
+        HIPCHK(c, hipMemcpyAsync(d_w, w.data(), sizeof(double) * R.nh, hipMemcpyHostToDevice, c->stream));
+        st = sync_err(c);       // w is the source of that copy
+        if (st) return st;
+        // lambda_nsteps = dl/dx at the last state (mu = 0: no step after it), with the flux
+        // term's share
+        A.x = xS; A.lam1 = lam[0]; A.lam0 = lam[1]; A.obs = 1; A.zflux = std::ldexp(zcoef, sc);
+        HIPCHK(c, launch_adj(3, A, c->stream));
+        A.zflux = 0.0;
         int lc = 1;
-        for (int seg = nseg - 1; seg >= 0; seg--) {
+        for (int seg = R.nseg - 1; seg >= 0; seg--) {
             const int s0 = seg * K, s1 = std::min(s0 + K, nsteps);
-            TCHK(hipMemcpyAsync(c->pos[c->cur], ckpt + (size_t)seg * nt, v4, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->pos[c->cur], R.ckpt + (size_t)seg * nt, v4, hipMemcpyDeviceToDevice, c->stream));
             int sst = ensure_sorted(c, true);
-            if (sst) { tfree(); return sst; }
+            if (sst) return sst;
             for (int s = s0; s < s1; s++) {
-                TCHK(hipMemcpyAsync(traj + (size_t)(s - s0) * nt, c->pos[c->cur], v4, hipMemcpyDeviceToDevice, c->stream));
-                TCHK(launch_step(step_params(s), c->stream));
+                HIPCHK(c, hipMemcpyAsync(traj + (size_t)(s - s0) * nt, c->pos[c->cur], v4, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, launch_step(R.step_params(s), c->stream));
                 c->cur ^= 1;
             }
             for (int s = s1 - 1; s >= s0; s--) {
@@ -1916,37 +2213,133 @@ int bromoc_adjoint_rdf(bromoc_ctx *c, int32_t nsteps, int32_t K, const bromoc_rd
                 A.lam1 = lam[lc];
                 A.lam0 = lam[lc ^ 1];
                 A.obs = s >= 1 ? 1 : 0;
-                TCHK(launch_adj(1, A, c->stream));
-                TCHK(launch_adj(2, A, c->stream));
-                TCHK(launch_adj(3, A, c->stream));
+                HIPCHK(c, launch_adj(1, A, c->stream));
+                HIPCHK(c, launch_adj(2, A, c->stream));
+                HIPCHK(c, launch_adj(3, A, c->stream));
                 lc ^= 1;
             }
         }
-        std::vector<double> hg(2 * (size_t)nk), hs(2 * (size_t)(S.npair + 1));
-        TCHK(hipMemcpyAsync(hg.data(), gep, sizeof(double) * hg.size(), hipMemcpyDeviceToHost, c->stream));
-        TCHK(hipMemcpyAsync(hs.data(), gsc, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, c->stream));
+        std::vector<unsigned long long> hg(2 * ADJ_W * (size_t)R.nk), hs(2 * ADJ_W * (size_t)(S.npair + 1));
+        HIPCHK(c, hipMemcpyAsync(hg.data(), gep, sizeof(unsigned long long) * hg.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(hs.data(), gsc, sizeof(unsigned long long) * hs.size(), hipMemcpyDeviceToHost, c->stream));
         // the last state back in place
-        TCHK(hipMemcpyAsync(c->pos[c->cur], xS, v4, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->pos[c->cur], xS, v4, hipMemcpyDeviceToDevice, c->stream));
         st = sync_err(c);
-        if (st) { tfree(); return st; }
+        if (st) return st;
+        auto val = [&](const std::vector<unsigned long long> &h, size_t e) {
+            return std::ldexp(fix_value(&h[ADJ_W * e], ADJ_W), -sc);
+        };
         if (grad_ep)
             for (int k = 0; k < S.nknots; k++) {
                 grad_ep[3 * k] = 0.0;
-                grad_ep[3 * k + 1] = hg[2 * k];
-                grad_ep[3 * k + 2] = hg[2 * k + 1];
+                grad_ep[3 * k + 1] = val(hg, 2 * (size_t)k);
+                grad_ep[3 * k + 2] = val(hg, 2 * (size_t)k + 1);
             }
         if (grad_sc)
             for (int t = 0; t <= S.npair; t++) {
-                grad_sc[3 * t] = hs[2 * t];
+                grad_sc[3 * t] = val(hs, 2 * (size_t)t);
                 grad_sc[3 * t + 1] = 0.0;
-                grad_sc[3 * t + 2] = hs[2 * t + 1];
+                grad_sc[3 * t + 2] = val(hs, 2 * (size_t)t + 1);
             }
     }
     c->sort_dirty = true;
+    return sync_err(c);
+}
+
+int bromoc_tangent_rdf(bromoc_ctx *c, int32_t nsteps, int32_t K, const bromoc_rdf *rdf,
+                       const double *v_ep, const double *v_sc, double *loss, double *dloss) {
+    if (!c || !loss || !dloss || !v_ep || !v_sc) return BROMOC_E_ARG;
+    (void)hipSetDevice(c->device);
+    const bromoc_system &S = c->sys;
+    AdjRun R;
+    int st = R.setup(c, nsteps, K, rdf);
+    if (st) return st;
+    const int nt = R.nt;
+    const size_t v4 = sizeof(double4) * (size_t)nt;
+    double4 *F, *mu, *dbox, *ones, *dxs[2], *junk;
+    double *vep, *vsc;
+    unsigned long long *dh;
+    int vscale = 0;
+    HIPCHK(c, R.alloc((void **)&F, v4));
+    HIPCHK(c, R.alloc((void **)&mu, v4));
+    HIPCHK(c, R.alloc((void **)&dbox, v4));
+    HIPCHK(c, R.alloc((void **)&ones, v4));
+    HIPCHK(c, R.alloc((void **)&dxs[0], v4));
+    HIPCHK(c, R.alloc((void **)&dxs[1], v4));
+    HIPCHK(c, R.alloc((void **)&junk, v4));
+    HIPCHK(c, R.alloc((void **)&vep, sizeof(double) * 2 * (size_t)R.nk));
+    HIPCHK(c, R.alloc((void **)&vsc, sizeof(double) * 2 * (size_t)(S.npair + 1)));
+    HIPCHK(c, R.alloc((void **)&dh, sizeof(unsigned long long) * ADJ_W * (size_t)R.nh));
+    {
+        std::vector<double4> h1(nt, make_double4(1.0, 1.0, 1.0, 0.0));
+        // the tangent is linear in v: v goes in scaled by the power of two that brings its
+        // largest entry to about 1 (exact), dloss comes out scaled back
+        double big = 0.0;
+        for (int k = 0; k < S.nknots; k++) big = std::max({big, std::fabs(v_ep[3 * k + 1]), std::fabs(v_ep[3 * k + 2])});
+        for (int t = 0; t <= S.npair; t++) big = std::max({big, std::fabs(v_sc[3 * t]), std::fabs(v_sc[3 * t + 2])});
+        vscale = big > 0.0 ? -std::ilogb(big) : 0;
+        std::vector<double> he(2 * (size_t)R.nk, 0.0), hs(2 * (size_t)(S.npair + 1), 0.0);
+        for (int k = 0; k < S.nknots; k++) {
+            he[2 * k] = std::ldexp(v_ep[3 * k + 1], vscale);
+            he[2 * k + 1] = std::ldexp(v_ep[3 * k + 2], vscale);
+        }
+        for (int t = 0; t <= S.npair; t++) {
+            hs[2 * t] = std::ldexp(v_sc[3 * t], vscale);
+            hs[2 * t + 1] = std::ldexp(v_sc[3 * t + 2], vscale);
+        }
+        HIPCHK(c, hipMemcpyAsync(ones, h1.data(), v4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(vep, he.data(), sizeof(double) * he.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(vsc, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice, c->stream));
+        st = sync_err(c);     // the host staging vectors go out of scope here
+        if (st) return st;
+    }
+    AdjParams &A = R.A;
+    A.f = F; A.mu = mu; A.dbox = dbox;
+    A.vep = vep; A.vsc = vsc; A.dh = dh;
+    A.lam1 = ones;            // k_adj_mu then gives mu = f1 D_bd D_box
+    int dc = 0;               // dxs[dc] holds dx of the current state (zero at x_0)
+    for (int s = 0; s < nsteps; s++) {
+        // the tangent of step s at x_s, before the step moves the positions
+        if (s % K == 0) {      // the checkpoint's sort, as the forward pass makes it
+            int sst = ensure_sorted(c, true);
+            if (sst) return sst;
+        }
+        A.x = c->pos[c->cur];
+        A.xi = c->noise + (size_t)s * c->noise_stride;
+        A.dx = dxs[dc];
+        A.dxo = dxs[dc ^ 1];
+        A.obs = s >= 1 ? 1 : 0;
+        HIPCHK(c, launch_adj(1, A, c->stream));
+        HIPCHK(c, launch_adj(2, A, c->stream));
+        HIPCHK(c, launch_adj(4, A, c->stream));
+        dc ^= 1;
+        if ((st = R.forward_step(s))) return st;
+    }
+    // the histogram derivative at the last state
+    A.x = c->pos[c->cur];
+    A.dx = dxs[dc];
+    A.dxo = junk;
+    A.obs = 1;
+    A.xi = c->noise + (size_t)(nsteps - 1) * c->noise_stride;
+    HIPCHK(c, launch_adj(1, A, c->stream));
+    HIPCHK(c, launch_adj(2, A, c->stream));
+    HIPCHK(c, launch_adj(4, A, c->stream));
+    std::vector<double> w;
+    double zcoef;
+    if ((st = R.loss(loss, nullptr, w, zcoef))) return st;
+    std::vector<unsigned long long> hdh(ADJ_W * (size_t)R.nh);
+    std::vector<double4> hdx(nt);
+    HIPCHK(c, hipMemcpyAsync(hdh.data(), dh, sizeof(unsigned long long) * hdh.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hdx.data(), dxs[dc], v4, hipMemcpyDeviceToHost, c->stream));
     st = sync_err(c);
-    tfree();
-#undef TCHK
-    return st;
+    if (st) return st;
+    double d = 0.0;
+    for (int k = 0; k < R.nh; k++) d += w[k] * fix_value(&hdh[ADJ_W * (size_t)k], ADJ_W);
+    if (zcoef != 0.0)
+        for (int i = R.nf0; i < nt; i++) d += zcoef * c->h_cg[std::abs(c->h_typei[i])] * hdx[i].z;
+    *dloss = std::ldexp(d, -vscale);
+    c->sort_dirty = true;
+    return BROMOC_OK;
 }
 
 // raw Philox4x64-10 blocks on the device (the generator behind BROMOC_NOISE_PHILOX)

@@ -38,6 +38,36 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Order-independent sums of doubles: a term v is cut at the 2^-32 grid into an integer part
+// floor(v 2^32) and W - 1 further 32-bit digits below it (units 2^-64, 2^-96), each added
+// with an integer atomic; integer addition is associative, so the total is the same
+// whatever the order of the terms -- bit-reproducible -- and exact in units of 2^-32W for
+// |total| < 2^31 and fewer than 2^32 terms (fix_value, engine.h, reads a sum back).
+template <int W>
+__device__ __forceinline__ void fix_add(unsigned long long *acc, double v) {
+    double a = v * 4294967296.0;
+    const double f = floor(a);
+    atomicAdd(acc, (unsigned long long)(long long)f);
+    a = a - f;
+#pragma unroll
+    for (int w = 1; w < W; w++) {
+        a = a * 4294967296.0;
+        const double d = floor(a);
+        atomicAdd(acc + w, (unsigned long long)d);
+        a = a - d;
+    }
+}
+// A block's partial sum before it goes to a global one: the carries of the lower words
+// moved up, so each lower word adds less than 2^32
+template <int W>
+__device__ __forceinline__ void fix_carry(unsigned long long *acc) {
+#pragma unroll
+    for (int w = W - 1; w > 0; w--) {
+        acc[w - 1] += acc[w] >> 32;
+        acc[w] &= 0xFFFFFFFFull;
+    }
+}
+
 // small integer powers by repeated squaring (the step kernels agree with the reference to
 // rounding, not bitwise; the bit-exact table construction has its own, tables.h)
 __device__ __forceinline__ double pow3(double a) { double a2 = a * a; return a * a2; }

@@ -17,6 +17,18 @@ enum ECat {
     E_GSBPA, E_GSBPB, E_VDWGD, E_RFPAR, E_CON, E_MEMB
 };
 
+// The value of an order-independent fixed-point sum of W words (fix_add, device.h): the
+// first in units of 2^-32 (two's complement), the others in units of 2^-64, 2^-96.
+__host__ __device__ inline double fix_value(const unsigned long long *acc, int W = 2) {
+    double v = (double)(long long)acc[0] * (1.0 / 4294967296.0);
+    double unit = 1.0 / 4294967296.0;
+    for (int w = 1; w < W; w++) {
+        unit *= 1.0 / 4294967296.0;
+        v += (double)acc[w] * unit;
+    }
+    return v;
+}
+
 // pair-type flags
 constexpr int PF_EFPOT = 1, PF_CHR = 2, PF_LJ = 4, PF_SRPMF = 8;
 
@@ -156,7 +168,22 @@ struct StepParams {
     double *mleg;            // [cap+1][npl] P_0.. of x, y, z scaled (zeros outside the region)
     double *mchg;            // [cap+1] charge of each moving ion inside the region, else 0
     double *mcoef, *mq, *mer;   // [ntpol] coefficients, M q rfscal, per-row energy terms
+    // tile mode (tiles.hip): tiles of 64 * ttpl consecutive slots of the spatial order;
+    // tpart[((K * ntile + I) * tnc + c) * TT + t]: component c (fx, fy, fz, and with the
+    // fused reaction field A, B) of slot t of tile I from the tile pair of I and K, each
+    // entry written once per step; tener[cat][item]: pair energies of each tile pair
+    double *tpart, *tener;
+    int32_t ntile, ttpl, tnc, titems;
 };
+
+// tpb values of tile mode: TPB_TILE + targets per lane (1, 2 or 4)
+constexpr int TPB_TILE = 16;
+__host__ __device__ inline bool tile_mode(int tpb) { return tpb > TPB_TILE; }
+
+// pair-energy categories of the tile kernel (tener rows) and the ECat each one feeds
+constexpr int NTE = 9;
+enum { TE_EFPOT = 0, TE_ELEC, TE_VDW, TE_SRPMF, TE_EFPOTMX, TE_QQMX, TE_VDWMX, TE_SRPMFMX,
+       TE_RFPAR };
 
 // spatial partner order (sort.hip)
 struct SortGeom {
@@ -166,12 +193,14 @@ struct SortGeom {
 };
 struct SortBufs {
     SortGeom geom;
-    uint32_t *keys, *keys_out;
-    int32_t *vals, *order, *rank;
-    void *temp;
-    size_t temp_bytes;
+    uint32_t *keys, *keys_out;     // [cap]: keys, then the sorted keys
+    int32_t *vals, *order, *rank;  // [cap]: slot of each particle; particle in each slot
+    int32_t *hist;                 // [sort_hist_ints(cap)] digit histograms of a pass
+    int32_t *cflag;                // [cap] cell numbers (exclusive scan of the key changes)
+    int32_t *cell_start;           // [cap+1] first slot of each occupied cell, then n
+    int32_t *ncell;                // [1] occupied cells
 };
-size_t sort_temp_bytes(int n);
+size_t sort_hist_ints(int n);
 hipError_t sort_partners(const SortBufs &b, const double4 *pos, int n, double4 *spos0,
                          double4 *spos1, hipStream_t s);
 
@@ -179,7 +208,9 @@ hipError_t sort_partners(const SortBufs &b, const double4 *pos, int n, double4 *
 hipError_t launch_step(const StepParams &p, hipStream_t s);   // launch_dna + launch_pairs
 hipError_t launch_dna(const StepParams &p, hipStream_t s);    // k_dna (CG-DNA list terms)
 hipError_t launch_pairs(const StepParams &p, hipStream_t s);  // k_step (pairs, grids, update)
-hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s);  // RFPARION
+// RFPARION: k_rfpar (srfe, reff, and their gradients when pairs or grad) then, with pairs,
+// the cross term k_rfpair
+hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s, bool grad = false);
 hipError_t launch_notrans(const StepParams &p, hipStream_t s);  // NOTRANS after a step
 hipError_t launch_fixed(const StepParams &p, hipStream_t s);    // terms of the fixed ions
 // MMIJ reaction field (mmij.hip): RECT_RF1 for ENERGY (force into frf, energy into slot 0 of
@@ -187,8 +218,17 @@ hipError_t launch_fixed(const StepParams &p, hipStream_t s);    // terms of the 
 hipError_t launch_mmij(const StepParams &p, hipStream_t s);
 hipError_t launch_mmij_interact(const StepParams &p, double xj, double yj, double zj, int jtype,
                                 int j, double *out, hipStream_t s);
+// out[cat] = sum over particles of epart[cat], plus for the pair categories the sum over
+// the tile pairs of tener (tile mode), both in a fixed order
 hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
-                                hipStream_t s);
+                                hipStream_t s, const double *tener = nullptr, int titems = 0);
+// tile mode (tiles.hip): k_tiles over every tile pair (which & 1), then k_tfin over every
+// slot (which & 2)
+hipError_t launch_tiles(const StepParams &p, hipStream_t s, int which = 3);
+// device bytes of the tile buffers for `cap` particles at ttpl targets per lane (fused
+// reaction field: rf != 0)
+size_t tile_part_doubles(int cap, int ttpl, int rf);
+size_t tile_ener_doubles(int cap, int ttpl);
 hipError_t launch_advance(int64_t *ctr, int64_t by, hipStream_t s);
 hipError_t launch_philox_fill(const int64_t *ctl, uint64_t stream, int nrows, int ntot,
                               int64_t stride, double *out, hipStream_t s);
@@ -202,6 +242,41 @@ struct InteractParams {
     double *out;                // [1]
 };
 hipError_t launch_interact(const InteractParams &p, hipStream_t s);
+
+// ---- GRAND of one system on the device (grand.hip)
+constexpr int GC_BLOCKS = 256;   // workgroups of k_gc_interact (partials per attempt)
+
+// error codes of GcState::err
+enum { GCE_OK = 0, GCE_UNIFORMS = 1, GCE_CAPACITY = 2, GCE_STATE = 3 };
+
+struct GcState {
+    int64_t k;               // uniforms drawn
+    int32_t ntot, err;
+    int32_t mode, ib, ip, itype, j;   // trial: mode 0 none, 1 insertion, 2 deletion
+    int32_t nacc;            // accepted insertions and deletions
+    double x, y, z, ener;
+};
+
+struct GcParams {
+    StepParams sp;           // system tables and grids; sp.pos is the current buffer
+    double4 *pos0, *pos1;    // both position buffers (a change is written to both)
+    int32_t *typei, *t2, *ibuffer;
+    const double *u;
+    int64_t nu;
+    int32_t nbuffer, pstride;            // prob rows: [nbuffer][pstride]
+    int32_t *nat, *prob;
+    const double *avnum, *mu, *kb, *lzmin, *lzmax;   // [nbuffer+1]
+    const int32_t *ibfftyp, *ntotat;
+    const int8_t *bias;
+    double lx, ly, cx, cy, cz, ikbt;
+    int64_t icycle;
+    int32_t cap, pad;
+    GcState *st;
+    double *part;            // [GC_BLOCKS]
+};
+
+// one attempt: k_gc_pre, (k_gc_rfpar), k_gc_interact, k_gc_post
+hipError_t launch_grand_attempt(const GcParams &g, hipStream_t s);
 
 // ---- EFPOT tables on the device (tables.hip)
 struct TabEnds;
@@ -257,12 +332,24 @@ struct AdjParams {
     double r0, dr, sigma, cut;
     const double *w;        // [ngroup*nbins] residual weights dL/dH
     double *hist;           // [ntot][ngroup*nbins] per-target raw histograms
-    double *gep;            // [nknots][2]: dL/d ep2, dL/d ep3 per knot
-    double *gsc;            // [npair+1][2]: dL/d sc1, dL/d sc3 per pair type
+    // gradients as order-independent fixed-point sums of ADJ_W words (fix_add):
+    // [nknots][2 (ep2, ep3)][ADJ_W] and [npair+1][2 (sc1, sc3)][ADJ_W]
+    unsigned long long *gep, *gsc;
     int32_t obs, pad;
+    double zflux;           // terminal call: lambda_z of ion i gains zflux * q_i (flux loss)
+    // tangent-linear pass (k_tlm): dx_s in, dx_{s+1} out, the direction of the parameters
+    // ([nknots][2] ep2, ep3; [npair+1][2] sc1, sc3) and the derivative of every histogram
+    // bin as fixed-point sums [ngroup*nbins][ADJ_W]
+    const double4 *dx;
+    double4 *dxo;
+    const double *vep, *vsc;
+    unsigned long long *dh;
 };
 
-// which: 0 k_rdf, 1 k_adj_force, 2 k_adj_mu, 3 k_adj_pairs
+// words of the adjoint's and the tangent's fixed-point sums (resolution 2^-96 of a sum
+// whose sources the host scales to about 1)
+constexpr int ADJ_W = 3;
+// which: 0 k_rdf, 1 k_adj_force, 2 k_adj_mu, 3 k_adj_pairs, 4 k_tlm
 hipError_t launch_adj(int which, const AdjParams &a, hipStream_t s);
 
 // ---- replica batches (batch.hip)

@@ -807,6 +807,7 @@ hipError_t launch_dna(const StepParams &p, hipStream_t s) {
 }
 
 hipError_t launch_pairs(const StepParams &p, hipStream_t s) {
+    if (tile_mode(p.tpb)) return launch_tiles(p, s);
     if (p.tend <= p.tbeg) return hipSuccess;
     int feat = 0;
     if (p.flags & QNUCL) feat |= F_DNA;
@@ -830,7 +831,7 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s) {
 }
 
 hipError_t launch_step(const StepParams &p, hipStream_t s) {
-    hipError_t e = launch_rfpar(p, true, s);
+    hipError_t e = launch_rfpar(p, !tile_mode(p.tpb), s, true);
     if (e != hipSuccess) return e;
     if ((e = launch_mmij(p, s)) != hipSuccess) return e;
     if ((e = launch_fixed(p, s)) != hipSuccess) return e;
@@ -978,10 +979,11 @@ __global__ __launch_bounds__(BLOCK) void k_rfpair(StepParams P) {
     }
 }
 
-hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s) {
+hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s, bool grad) {
     const int nion = p.ntot - p.nsites;
     if (!p.rfon || nion <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rfpar, dim3((nion + 255) / 256), dim3(256), 0, s, p, pairs ? 1 : 0);
+    hipLaunchKernelGGL(k_rfpar, dim3((nion + 255) / 256), dim3(256), 0, s, p,
+                       (pairs || grad) ? 1 : 0);
     if (pairs)
         hipLaunchKernelGGL(k_rfpair, dim3((nion + WPB * RF_TPW - 1) / (WPB * RF_TPW)),
                            dim3(BLOCK), 0, s, p);
@@ -989,13 +991,34 @@ hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s) {
 }
 
 // ---- energy reduction: one block per category, fixed-order tree
+__device__ __forceinline__ int tile_cat_of(int cat) {
+    switch (cat) {
+        case E_EFPOT: return TE_EFPOT;
+        case E_ELEC: return TE_ELEC;
+        case E_VDW: return TE_VDW;
+        case E_SRPMF: return TE_SRPMF;
+        case E_EFPOTMX: return TE_EFPOTMX;
+        case E_QQMX: return TE_QQMX;
+        case E_VDWMX: return TE_VDWMX;
+        case E_SRPMFMX: return TE_SRPMFMX;
+        case E_RFPAR: return TE_RFPAR;
+        default: return -1;
+    }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_reduce_energy(const double *epart, int ecap, int n,
-                                                         double *out) {
+                                                         double *out, const double *tener,
+                                                         int titems) {
     __shared__ double s[BLOCK];
     const int cat = blockIdx.x, tid = threadIdx.x;
     const double *src = epart + (size_t)cat * ecap;
     double acc = 0.0;
     for (int k = tid; k < n; k += BLOCK) acc += src[k];
+    const int tc = tener ? tile_cat_of(cat) : -1;
+    if (tc >= 0) {
+        const double *t = tener + (size_t)tc * titems;
+        for (int k = tid; k < titems; k += BLOCK) acc += t[k];
+    }
     s[tid] = acc;
     __syncthreads();
     for (int h = BLOCK / 2; h > 0; h >>= 1) {
@@ -1006,8 +1029,9 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_energy(const double *epart, in
 }
 
 hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
-                                hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_energy, dim3(NE), dim3(BLOCK), 0, s, epart, ecap, n, out);
+                                hipStream_t s, const double *tener, int titems) {
+    hipLaunchKernelGGL(k_reduce_energy, dim3(NE), dim3(BLOCK), 0, s, epart, ecap, n, out, tener,
+                       titems);
     return hipGetLastError();
 }
 

@@ -1,17 +1,26 @@
-// sort.hip -- spatial order of the partner array (cell list as an integer key sort).
+// sort.hip -- spatial order of the partner array: a cell list as an integer key sort plus a
+// prefix scan.
 //
 // Every particle gets the Morton code of its cell on a regular lattice over the box
-// (10 bits per axis); a stable radix sort of (key, particle index) gives the partner order,
-// and a gather writes the partner array spos (x, y, z, packed type and original index) in
-// that order plus rank[i], the slot of particle i.  The step kernel keeps spos current
-// between sorts by writing each updated particle to its slot, so a sort is only needed
-// every few steps; the order only decides which partners share a wavefront (and thus
-// whether the wavefront can take the analytic-tail path), never which pairs are summed.
-#include <hipcub/hipcub.hpp>
+// (10 bits per axis).  A stable least-significant-digit radix sort of (key, particle index)
+// -- four passes of 8 bits, each a per-block digit histogram, one exclusive scan of the
+// histograms in (digit, block) order and a stable scatter -- gives the partner order; a
+// second scan over the sorted keys numbers the occupied cells and records the first slot of
+// each (cell_start).  A gather writes the partner array spos (x, y, z, packed type and
+// original index) in that order plus rank[i], the slot of particle i.  The step kernels
+// keep spos current between sorts by writing each updated particle to its slot, so a sort
+// is only needed every few steps; the order only decides which partners share a wavefront
+// or a tile, never which pairs are summed.
+#include <utility>
 
 #include "engine.h"
 
 namespace bromoc {
+
+constexpr int SORT_BLOCK = 256;        // threads of the histogram / scatter blocks
+constexpr int SORT_RADIX = 256;        // 8-bit digits
+constexpr int SORT_PASSES = 4;         // 32 bits of key (the Morton code uses 30)
+constexpr int SCAN_BLOCK = 1024;
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
     v &= 0x3FFu;
@@ -40,6 +49,115 @@ __global__ void k_keys(const double4 *__restrict__ pos, int n, SortGeom g, uint3
     vals[i] = i;
 }
 
+// digit counts of block b's chunk [b*chunk, min((b+1)*chunk, n)): hist[d * nb + b]
+__global__ __launch_bounds__(SORT_BLOCK) void k_digit_hist(const uint32_t *__restrict__ keys,
+                                                            int n, int chunk, int shift,
+                                                            int32_t *hist) {
+    __shared__ int32_t s_c[SORT_RADIX];
+    const int b = blockIdx.x, nb = gridDim.x, tid = threadIdx.x;
+    for (int d = tid; d < SORT_RADIX; d += SORT_BLOCK) s_c[d] = 0;
+    __syncthreads();
+    const int lo = b * chunk, hi = min(lo + chunk, n);
+    for (int k = lo + tid; k < hi; k += SORT_BLOCK)
+        atomicAdd(&s_c[(keys[k] >> shift) & (SORT_RADIX - 1)], 1);   // integer: exact
+    __syncthreads();
+    for (int d = tid; d < SORT_RADIX; d += SORT_BLOCK) hist[d * nb + b] = s_c[d];
+}
+
+// exclusive prefix sum of a[0..m) in place, one block: each thread scans a contiguous
+// segment, the segment totals are scanned in LDS, then added back; also total[0] = sum
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_excl(int32_t *a, int m, int32_t *total) {
+    __shared__ int32_t s[SCAN_BLOCK];
+    const int tid = threadIdx.x;
+    const int seg = (m + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    const int lo = min(tid * seg, m), hi = min(lo + seg, m);
+    int32_t sum = 0;
+    for (int k = lo; k < hi; k++) sum += a[k];
+    s[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < SCAN_BLOCK; off <<= 1) {   // Hillis-Steele inclusive scan
+        const int32_t v = tid >= off ? s[tid - off] : 0;
+        __syncthreads();
+        s[tid] += v;
+        __syncthreads();
+    }
+    int32_t run = s[tid] - sum;                        // exclusive start of the segment
+    for (int k = lo; k < hi; k++) {
+        const int32_t v = a[k];
+        a[k] = run;
+        run += v;
+    }
+    if (tid == SCAN_BLOCK - 1 && total) total[0] = s[tid];
+}
+
+// stable scatter of block b's chunk: element k of digit d goes to offset[d * nb + b] plus
+// the number of earlier elements of the chunk with digit d.  The chunk is taken in tiles of
+// SORT_BLOCK in order; within a tile, lanes of equal digit are ranked by lane (eight
+// ballots over the digit's bits) and wavefronts in order.
+__global__ __launch_bounds__(SORT_BLOCK) void k_digit_scatter(
+    const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin, uint32_t *kout,
+    int32_t *vout, int n, int chunk, int shift, const int32_t *__restrict__ offset) {
+    constexpr int NWV = SORT_BLOCK / 64;
+    __shared__ int32_t s_run[SORT_RADIX];
+    __shared__ int32_t s_wc[NWV][SORT_RADIX];
+    const int b = blockIdx.x, nb = gridDim.x, tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    for (int d = tid; d < SORT_RADIX; d += SORT_BLOCK) {
+        s_run[d] = offset[d * nb + b];
+        for (int w = 0; w < NWV; w++) s_wc[w][d] = 0;
+    }
+    __syncthreads();
+    const int lo = b * chunk, hi = min(lo + chunk, n);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int base = lo; base < hi; base += SORT_BLOCK) {
+        const int k = base + tid;
+        const bool ok = k < hi;
+        const uint32_t key = ok ? kin[k] : 0u;
+        const int32_t val = ok ? vin[k] : 0;
+        const int d = (key >> shift) & (SORT_RADIX - 1);
+        // lanes of this wavefront holding the same digit
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const uint64_t m = __ballot((d >> bit) & 1);
+            peers &= ((d >> bit) & 1) ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt);
+        if (ok && rank == 0) s_wc[wv][d] = __popcll(peers);
+        __syncthreads();
+        if (ok) {
+            int before = s_run[d] + rank;
+            for (int w = 0; w < wv; w++) before += s_wc[w][d];
+            kout[before] = key;
+            vout[before] = val;
+        }
+        __syncthreads();
+        for (int dd = tid; dd < SORT_RADIX; dd += SORT_BLOCK) {
+            int t = 0;
+            for (int w = 0; w < NWV; w++) { t += s_wc[w][dd]; s_wc[w][dd] = 0; }
+            s_run[dd] += t;
+        }
+        __syncthreads();
+    }
+}
+
+// cell numbering: flag[s] = 1 where the sorted key changes (and at slot 0)
+__global__ void k_cell_flags(const uint32_t *__restrict__ keys, int n, int32_t *flag) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    flag[s] = (s == 0 || keys[s] != keys[s - 1]) ? 1 : 0;
+}
+
+// after the exclusive scan of the flags: cell_start[cell of s] = s at every flagged slot,
+// and cell_start[ncell] = n
+__global__ void k_cell_starts(const uint32_t *__restrict__ keys, const int32_t *__restrict__ excl,
+                              int n, const int32_t *ncell, int32_t *cell_start) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) cell_start[ncell[0]] = n;
+    if (s >= n) return;
+    if (s == 0 || keys[s] != keys[s - 1]) cell_start[excl[s]] = s;
+}
+
 __global__ void k_gather(const double4 *__restrict__ pos, const int32_t *__restrict__ order, int n,
                          double4 *spos0, double4 *spos1, int32_t *rank) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -51,25 +169,43 @@ __global__ void k_gather(const double4 *__restrict__ pos, const int32_t *__restr
     rank[j] = k;
 }
 
-size_t sort_temp_bytes(int n) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr,
-                                             (uint32_t *)nullptr, (const int32_t *)nullptr,
-                                             (int32_t *)nullptr, n, 0, 30);
-    return bytes;
+// blocks of the radix passes for n keys: enough chunks to spread the work, few enough
+// that the (digit, block) histogram stays small for the one-block scan
+static int sort_blocks(int n) {
+    const int nb = (n + 4 * SORT_BLOCK - 1) / (4 * SORT_BLOCK);
+    return nb < 1 ? 1 : (nb > 128 ? 128 : nb);
 }
+
+size_t sort_hist_ints(int n) { return (size_t)SORT_RADIX * sort_blocks(n) + 2; }
 
 hipError_t sort_partners(const SortBufs &b, const double4 *pos, int n, double4 *spos0,
                          double4 *spos1, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int blk = 256, grid = (n + blk - 1) / blk;
     hipLaunchKernelGGL(k_keys, dim3(grid), dim3(blk), 0, s, pos, n, b.geom, b.keys, b.vals);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    size_t bytes = b.temp_bytes;
-    e = hipcub::DeviceRadixSort::SortPairs(b.temp, bytes, b.keys, b.keys_out, b.vals, b.order, n,
-                                           0, 30, s);
-    if (e != hipSuccess) return e;
+    const int nb = sort_blocks(n);
+    const int chunk = (n + nb - 1) / nb;
+    uint32_t *kin = b.keys, *kout = b.keys_out;
+    int32_t *vin = b.vals, *vout = b.order;
+    for (int pass = 0; pass < SORT_PASSES; pass++) {
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(k_digit_hist, dim3(nb), dim3(SORT_BLOCK), 0, s, kin, n, chunk, shift,
+                           b.hist);
+        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(SCAN_BLOCK), 0, s, b.hist,
+                           SORT_RADIX * nb, (int32_t *)nullptr);
+        hipLaunchKernelGGL(k_digit_scatter, dim3(nb), dim3(SORT_BLOCK), 0, s, kin, vin, kout, vout,
+                           n, chunk, shift, b.hist);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    // an even number of passes leaves the sorted keys and order in keys / vals; order
+    // and keys_out take them for the gather and the cells
+    hipLaunchKernelGGL(k_cell_flags, dim3(grid), dim3(blk), 0, s, kin, n, b.cflag);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(SCAN_BLOCK), 0, s, b.cflag, n, b.ncell);
+    hipLaunchKernelGGL(k_cell_starts, dim3(grid), dim3(blk), 0, s, kin, b.cflag, n, b.ncell,
+                       b.cell_start);
+    (void)hipMemcpyAsync(b.order, vin, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(b.keys_out, kin, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
     hipLaunchKernelGGL(k_gather, dim3(grid), dim3(blk), 0, s, pos, b.order, n, spos0, spos1,
                        b.rank);
     return hipGetLastError();

@@ -106,6 +106,20 @@ class Engine:
                                              C.byref(tot), C.byref(ker)))
         return tot.value, ker.value
 
+    def spatial_order(self):
+        """(keys, order, rank, cell_start): the spatial order the step kernels use."""
+        n = self.ntot
+        keys = np.zeros(n, np.uint32)
+        order = np.zeros(n, np.int32)
+        rank = np.zeros(n, np.int32)
+        cs = np.zeros(n + 1, np.int32)
+        nc = C.c_int32()
+        self._chk("bromoc_spatial_order",
+                  self.lib.bromoc_spatial_order(self.ctx, ptr(keys, C.c_uint32), ptr(order, C.c_int32),
+                                                ptr(rank, C.c_int32), ptr(cs, C.c_int32),
+                                                C.byref(nc)))
+        return keys, order, rank, cs[:nc.value + 1]
+
     def tuning(self):
         """Targets per block the autotuner picked for this system."""
         t = C.c_int32()

@@ -643,12 +643,28 @@ call bgpu_store_energies(e)
 sites_moved = .false.
 end subroutine
 
+! The device pass moves every moving ion (and every free site): the ranges the driver
+! hands DYNAMICS0..3 and DYNAMICS0NUC (simul1.f90:369-381); any other range is refused
+! rather than ignored.
+subroutine bgpu_check_range(ninit, nfinal, lo, hi, where)
+  integer, intent(in) :: ninit, nfinal, lo, hi
+  character(*), intent(in) :: where
+  if (ninit /= lo .or. nfinal /= hi) then
+    write(*,'(a,a,a,i0,a,i0,a,i0,a,i0)') 'bromoc_gpu: ', where, ' moves ', lo, '..', hi, &
+         ' only, not ', ninit, '..', nfinal
+    stop 1
+  endif
+end subroutine
+
 ! DYNAMICS0 moves the ions and, in the same device pass, the free DNA sites; the
 ! DYNAMICS0NUC that follows it in simul1.f90:380 then has nothing left to do.
 SUBROUTINE DYNAMICS0(ninit, nfinal)
 use bromoc_gpu
+use grandmod
+use nucleotmod
 implicit none
 integer ninit, nfinal
+call bgpu_check_range(ninit, nfinal, nsites + nfix + 1, ntot, 'DYNAMICS')
 call bromoc_gpu_init()
 call bgpu_check(bromoc_dynamics(gctx, NOISE_PHILOX, 0_c_int32_t, bd_seed, bd_counter), 'bromoc_dynamics')
 bd_counter = bd_counter + 1
@@ -681,8 +697,10 @@ end subroutine
 
 SUBROUTINE DYNAMICS0NUC(ninit, nfinal)
 use bromoc_gpu
+use nucleotmod
 implicit none
 integer ninit, nfinal
+call bgpu_check_range(ninit, nfinal, 1, nsites, 'DYNAMICS0NUC')
 if (sites_moved) then
   sites_moved = .false.
   return

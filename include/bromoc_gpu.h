@@ -292,6 +292,15 @@ int bromoc_upload_noise(bromoc_ctx *ctx, int32_t nsteps, const double *noise);
 int bromoc_step(bromoc_ctx *ctx, int32_t nsteps, int32_t noise_mode, int32_t first_row,
                 uint64_t seed, uint64_t counter0, bromoc_energies *energies);
 
+/* The spatial order the step kernels use, rebuilt from the current positions: keys[ntot]
+ * the sorted cell keys (Morton codes of the 8 A cells of the box, 10 bits per axis),
+ * order[ntot] the particle in each slot (a stable sort: equal keys keep index order),
+ * rank[ntot] the slot of each particle, cell_start[ncell+1] the first slot of each occupied
+ * cell (then ntot) and *ncell.  Any output may be NULL.  No counterpart in the reference,
+ * whose pair loops run over all pairs in index order. */
+int bromoc_spatial_order(bromoc_ctx *ctx, uint32_t *keys, int32_t *order, int32_t *rank,
+                         int32_t *cell_start, int32_t *ncell);
+
 /* Measurement: nsteps steps as bromoc_step (launched one by one, no graph), with a HIP
  * event pair around every launch of the step kernel on the library's stream; *total_ms is
  * first-to-last, *kernel_ms the mean step-kernel duration.  No counterpart in the reference (it times runs with date_and_time,
@@ -469,13 +478,17 @@ int bromoc_philox4x64(int32_t n, const uint64_t *ctr, const uint64_t *key, uint6
  *   g[g][b] = V / (nsteps * P_g * 4 pi r_b^2 sigma sqrt(2 pi)) * sum over the steps' states
  *             x_1..x_nsteps and the group's pairs of exp(-(r - r_b)^2 / (2 sigma^2)),
  * terms with |r - r_b| > cut left out, r_b = r0 + (b + 1/2) dr, V the box volume, P_g the
- * number of the group's pairs (moving-moving, or moving-fixed); the loss is
- * L = sum_{g,b} (g[g][b] - target[g][b])^2. */
+ * number of the group's pairs (moving-moving, or moving-fixed).  The flux is the charge
+ * current through the box, J = sum over the moving ions of q_i (z_i(x_nsteps) - z_i(x_0)) /
+ * (nsteps dt) in e A / ps (the smooth counterpart of the reference's plane-crossing counts,
+ * countions.f90:19-32).  The loss is
+ * L = sum_{g,b} (g[g][b] - target[g][b])^2 + wflux (J - jtarget)^2. */
 typedef struct bromoc_rdf {
     int32_t ngroup, nbins;
     const int32_t *groups;       /* [2*ngroup] atnam types (a, b) of each group */
     double r0, dr, sigma, cut;
     const double *target;        /* [ngroup*nbins], NULL: zero */
+    double wflux, jtarget;       /* weight of the flux term (0: none) and its target */
 } bromoc_rdf;
 
 /* Forward pass (storing the positions every checkpoint_every steps) and, when grad_ep is
@@ -486,6 +499,17 @@ typedef struct bromoc_rdf {
 int bromoc_adjoint_rdf(bromoc_ctx *ctx, int32_t nsteps, int32_t checkpoint_every,
                        const bromoc_rdf *rdf, double *loss, double *rdf_out, double *grad_ep,
                        double *grad_sc);
+
+/* Tangent-linear pass of the same trajectory: dloss = <dL/d(ep, sc), v>, the derivative of
+ * the loss along the direction v_ep[3*nknots] (per knot: ep1, ep2, ep3) and
+ * v_sc[3*(npair+1)] (per pair type: sc1, sc2, sc3), carried forward step by step with the
+ * trajectory (the ep1 and sc2 entries do not move the forces and take no part).  With the
+ * same checkpoint_every the trajectory is bitwise the one bromoc_adjoint_rdf runs, so
+ * <grad, v> of the reverse pass equals dloss to rounding for every v (the dot-product
+ * test of the adjoint).  Also returns the loss.  Positions end at the last state. */
+int bromoc_tangent_rdf(bromoc_ctx *ctx, int32_t nsteps, int32_t checkpoint_every,
+                       const bromoc_rdf *rdf, const double *v_ep, const double *v_sc,
+                       double *loss, double *dloss);
 
 /* Chain rule through bromoc_build_table for one pair type's table of n samples at r: the
  * table (ep[3n], sc1..sc3) is affine in the sample values u (the charge term is the

@@ -23,6 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libbromoc_ref.so")
 SHIM_SO = os.path.join(HERE, "_ref", "libbromoc_shimcheck.so")
+REF_OMP_SO = os.path.join(HERE, "_ref", "libbromoc_ref_omp.so")   # OpenMP ENERGY
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

@@ -4,7 +4,8 @@ The trajectory comes from the C oracle or the reference routines stepping with i
 noise; the loss follows the definition in include/bromoc_gpu.h (bromoc_rdf): for group
 (a, b) and bin b, g = V / (S P 4 pi r_b^2 sigma sqrt(2 pi)) * sum over states x_1..x_S and
 the group's pairs (moving-moving once, moving-fixed) of exp(-(r - r_b)^2 / (2 sigma^2)),
-|r - r_b| <= cut; L = sum (g - target)^2.
+|r - r_b| <= cut; L = sum (g - target)^2, plus wflux (J - jtarget)^2 for the charge current
+J = sum over moving ions of q_i (z_i(x_S) - z_i(x_0)) / (S dt) when the spec has wflux > 0.
 """
 from __future__ import annotations
 
@@ -23,8 +24,17 @@ def trajectory(engine, noise, nsteps):
     return out
 
 
-def rdf_loss(states, system, spec, target=None):
-    """(loss, g [ngroup, nbins]) of the states of a trajectory of `system`."""
+def charge_current(x0, xS, system, nsteps):
+    """J = sum over the moving ions of q_i (z_i(x_S) - z_i(x_0)) / (S dt)."""
+    t = np.abs(np.asarray(system.typei[:system.ntot]))
+    q = np.asarray(system.cg)[t]
+    m = np.arange(system.ntot) >= system.nfix
+    return float(np.sum((q * (xS[:, 2] - x0[:, 2]))[m])) / (nsteps * system.dt)
+
+
+def rdf_loss(states, system, spec, target=None, x0=None):
+    """(loss, g [ngroup, nbins]) of the states x_1..x_S of a trajectory of `system` from x0
+    (needed for the flux term)."""
     t = np.abs(np.asarray(system.typei[:system.ntot]))
     nf = system.nfix
     n = system.ntot
@@ -54,4 +64,8 @@ def rdf_loss(states, system, spec, target=None):
             c = V / (S * P * 4.0 * np.pi * rb ** 2 * spec.sigma * np.sqrt(2.0 * np.pi))
             g_out[g] = c * H[g]
     tgt = np.zeros_like(g_out) if target is None else np.asarray(target).reshape(g_out.shape)
-    return float(np.sum((g_out - tgt) ** 2)), g_out
+    loss = float(np.sum((g_out - tgt) ** 2))
+    if getattr(spec, "wflux", 0.0) > 0.0:
+        J = charge_current(x0, states[-1], system, S)
+        loss += spec.wflux * (J - spec.jtarget) ** 2
+    return loss, g_out

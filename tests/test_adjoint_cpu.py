@@ -50,3 +50,60 @@ def test_trajectory_loss_reference_equals_oracle():
     lr, gr = rdf_loss(xr, s, spec)
     assert abs(lo - lr) <= 1e-12 * lr and np.max(np.abs(go - gr)) <= 1e-12 * np.max(gr)
     assert lo > 0.0
+
+
+def g_cube(r, L):
+    """Pair-distance density of two points uniform in a cube of side L over that of an
+    unbounded ideal gas, for r <= L: the cube's distance density r^2 (4 pi - 6 pi r + 8 r^2
+    - r^3) in units of L (Philip, "The probability distribution of the distance between two
+    random points in a box", 2007) divided by 4 pi r^2."""
+    x = r / L
+    return 1.0 - 1.5 * x + (2.0 / np.pi) * x ** 2 - x ** 3 / (4.0 * np.pi)
+
+
+def test_rdf_normalisation_ideal_gas():
+    """The loss's RDF normalisation V / (S P 4 pi r_b^2 sigma sqrt(2 pi)) against the closed
+    form of an ideal gas in a box: uniform random states of ions (no forces) give
+    g(r) = g_cube(r / L), the cube's edge effect, for every group (like, unlike and with a
+    fixed ion).  A normalisation off by a factor (a missing 2 for like pairs, sqrt(2 pi),
+    the fixed-partner count) misses by far more than the 3 % statistical bar."""
+    s = configs.adjoint_system(seed=5, n_per_type=150, box=40.0, table_builder=oracle_table_builder)
+    L = 40.0
+    spec = RdfSpec(groups=[(1, 1), (1, 2), (2, 2), (1, 3)], r0=2.0, dr=0.25, nbins=24, sigma=0.15,
+                   cut=0.75)
+    rng = np.random.default_rng(6)
+    states = []
+    fixed = np.stack([s.x, s.y, s.z], 1)[: s.nfix]
+    for _ in range(300):
+        x = rng.uniform(-0.5 * L, 0.5 * L, (s.ntot, 3))
+        x[: s.nfix] = fixed
+        states.append(x)
+    _, g = rdf_loss(states, s, spec)
+    rb = spec.r0 + (np.arange(spec.nbins) + 0.5) * spec.dr
+    for gi in range(3):      # ion-ion groups: pair counts large enough for a 3 % bar
+        want = g_cube(rb, L)
+        rel = np.abs(g[gi] - want) / want
+        assert np.mean(rel) <= 0.03, (gi, np.mean(rel))
+    # the group with the fixed site: the fixed particle sits at the centre, where the
+    # density around it is that of an unbounded gas for r < L / 2
+    assert np.mean(np.abs(g[3] - 1.0)) <= 0.12
+
+
+def test_flux_term():
+    """J = sum q_i dz_i / (S dt) over the moving ions: one ion of charge q moved by dz gives
+    J = q dz / (S dt); the loss adds wflux (J - jtarget)^2."""
+    from oracle.rdf import charge_current
+    s = small(seed=3, n=5)
+    x0 = np.stack([s.x, s.y, s.z], 1)
+    x1 = x0.copy()
+    k = s.nfix + 2
+    x1[k, 2] += 0.75
+    q = s.cg[abs(int(s.typei[k]))]
+    assert charge_current(x0, x1, s, 10) == pytest.approx(q * 0.75 / (10 * s.dt), rel=1e-15)
+    x1[0, 2] += 5.0          # a fixed particle does not count
+    assert charge_current(x0, x1, s, 10) == pytest.approx(q * 0.75 / (10 * s.dt), rel=1e-15)
+    spec = RdfSpec(groups=[(1, 2)], wflux=2.0, jtarget=0.1)
+    l1, _ = rdf_loss([x1], s, spec, x0=x0)
+    l0, _ = rdf_loss([x1], s, RdfSpec(groups=[(1, 2)]), x0=x0)
+    J = charge_current(x0, x1, s, 1)
+    assert l1 - l0 == pytest.approx(2.0 * (J - 0.1) ** 2, rel=1e-12)

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from bromoc_amd import configs
-from bromoc_amd.adjoint import RdfLoss, RdfSpec, rdf_loss
+from bromoc_amd.adjoint import RdfLoss, RdfSpec, rdf_loss, table_gradients, tangent
 from oracle.oracle import Oracle, oracle_table_builder
 from oracle.rdf import rdf_loss as cpu_rdf_loss, trajectory
 
@@ -144,9 +144,8 @@ def test_config_differentiation_64_knots():
 
 
 def test_autograd_function():
-    """RdfLoss.backward hands back the adjoint's gradient.  The knot gradients are summed
-    with device atomics in whatever order the pairs arrive, so two runs agree to rounding
-    of the largest terms (1e-12 of the largest gradient), not entry by entry."""
+    """RdfLoss.backward hands back the adjoint's gradient, bit for bit (the gradient sums
+    are integer fixed-point sums, independent of the order the pairs arrive in)."""
     import torch
     s = _system(seed=14, n=30, box=30.0)
     nsteps = 20
@@ -157,4 +156,83 @@ def test_autograd_function():
     loss.backward()
     _, _, grads = rdf_loss(s, noise, nsteps, SPEC)
     want = np.stack([grads[k] for k in keys])
-    assert _normwise(knots.grad.numpy(), want) <= 1e-12, _normwise(knots.grad.numpy(), want)
+    assert np.array_equal(knots.grad.numpy(), want)
+
+
+def test_gradient_bit_reproducible():
+    """Two reverse passes over the same trajectory give identical gradients."""
+    s = _system(seed=16, n=80, box=40.0)
+    nsteps = 40
+    noise = np.random.default_rng(17).standard_normal((nsteps, 3 * s.ntot))
+    target = _target(s, noise, nsteps, curve_seed=79)
+    a = table_gradients(s, noise, nsteps, SPEC, target=target)
+    b = table_gradients(s, noise, nsteps, SPEC, target=target)
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert np.count_nonzero(a[1]) > 100
+
+
+FLUX = RdfSpec(groups=[(1, 1), (1, 2), (2, 2), (1, 3)], wflux=3.0, jtarget=0.05)
+
+
+@pytest.mark.parametrize("spec", [SPEC, FLUX], ids=["rdf", "rdf+flux"])
+@pytest.mark.parametrize("nsteps,ckpt", [(60, 20), (45, 7)])
+def test_dot_product_adjoint_vs_tangent(spec, nsteps, ckpt):
+    """<dL/dth, v> from the reverse pass equals the tangent-linear pass's dL along v, for
+    random directions over every table coefficient (ep2, ep3 of every knot; sc1, sc3 of
+    every pair type): the two passes are transposes of one linear map, so they agree to
+    rounding -- free of the knot-jump noise central differences carry."""
+    s = _system(seed=18, n=60, box=36.0)
+    noise = np.random.default_rng(19).standard_normal((nsteps, 3 * s.ntot))
+    target = _target(s, noise, nsteps, curve_seed=80)
+    loss, gep, gsc = table_gradients(s, noise, nsteps, spec, target=target, checkpoint_every=ckpt)
+    rng = np.random.default_rng(20)
+    for trial in range(3):
+        ve = rng.standard_normal(gep.shape)
+        vs = rng.standard_normal(gsc.shape) * np.abs(s.ep).max() * 1e-3
+        if trial == 2:               # the wall and tail coefficients alone
+            ve[:] = 0.0
+            vs = rng.standard_normal(gsc.shape)
+        ve[0::3] = 0.0               # ep1 and sc2 do not move the forces
+        vs[1::3] = 0.0
+        lt, dl = tangent(s, noise, nsteps, spec, ve, vs, target=target, checkpoint_every=ckpt)
+        dot = float(np.dot(gep, ve) + np.dot(gsc, vs))
+        print(f"{spec.wflux} {nsteps}/{ckpt} trial {trial}: adjoint {dot:.15e} tangent {dl:.15e}")
+        assert lt == loss
+        assert abs(dot - dl) <= 1e-10 * max(abs(dl), 1e-300), (dot, dl)
+
+
+def test_dot_product_config_workload():
+    """The dot-product test on the 400-step, 401-particle differentiation workload."""
+    s = _system(seed=10, n=200, box=60.0)
+    nsteps = 400
+    noise = np.random.default_rng(11).standard_normal((nsteps, 3 * s.ntot))
+    target = _target(s, noise, nsteps, curve_seed=12)
+    loss, gep, gsc = table_gradients(s, noise, nsteps, FLUX, target=target)
+    ve = np.random.default_rng(21).standard_normal(gep.shape)
+    ve[0::3] = 0.0
+    vs = np.zeros_like(gsc)
+    lt, dl = tangent(s, noise, nsteps, FLUX, ve, vs, target=target)
+    dot = float(np.dot(gep, ve))
+    print(f"400 steps: adjoint {dot:.15e} tangent {dl:.15e}")
+    assert abs(dot - dl) <= 1e-10 * abs(dl)
+
+
+def test_flux_loss_vs_cpu_and_central_differences():
+    """The flux term: the GPU forward loss equals the CPU restatement's on the oracle's
+    trajectory, and its knot gradient agrees with central differences of the GPU loss."""
+    s = _system(seed=22, n=40, box=30.0)
+    nsteps = 30
+    noise = np.random.default_rng(23).standard_normal((nsteps, 3 * s.ntot))
+    spec = RdfSpec(groups=[(1, 1)], wflux=50.0, jtarget=0.0)
+    lg, _, grads = rdf_loss(s, noise, nsteps, spec)
+    x0 = np.stack([s.x, s.y, s.z], 1)
+    lc, _ = cpu_rdf_loss(trajectory(Oracle(s), noise, nsteps), s, spec, None, x0=x0)
+    assert abs(lg - lc) <= 1e-10 * lc
+    fwd = lambda t: rdf_loss(t, noise, nsteps, spec, grad=False)[0]
+    rng = np.random.default_rng(24)
+    adj, cds = [], []
+    for is_, k in _pick(grads, rng, 8, r=s.meta["rgrid"]):
+        cds.append(_cd(s, noise, nsteps, None, is_, k, fwd))
+        adj.append(grads[is_][k])
+    assert _normwise(adj, cds) <= 1e-6, _normwise(adj, cds)

@@ -58,6 +58,33 @@ def test_philox_gaussians_are_standard_normal():
     assert abs(np.mean(x ** 4) - 3.0) < 0.1
 
 
+def test_box_muller_from_raw_philox_words():
+    """Each Gaussian triple is the Box-Muller transform of the raw Philox block of its
+    (step, particle, stream) counter, checked term by term from numpy's Philox words: with
+    u1 = (w0 >> 11 + 1) 2^-53, u2 = (w1 >> 11) 2^-53, u3 = (w2 >> 11 + 1) 2^-53, u4 = (w3 >> 11)
+    2^-53, g0^2 + g1^2 = -2 ln u1, atan2(g1, g0) recovers 2 pi u2 (mod 2 pi), and
+    |g2| = sqrt(-2 ln u3) |cos 2 pi u4|.  A transform with its words or uniforms swapped, a
+    dropped +1 or the wrong shift fails these identities."""
+    L = oracle_lib()
+    for seed, step, stream in ((7, 3, 0), (2**40 + 5, 123456, 17)):
+        for i in (0, 1, 31, 999, 2**33 + 7):
+            key = np.array([seed, 0], np.uint64)
+            bg = np.random.Philox(counter=np.array([step - 1 if step else 0, i, stream, 0],
+                                                   np.uint64), key=key)
+            if step == 0:
+                continue
+            w = bg.random_raw(4)
+            u = [((int(w[0]) >> 11) + 1) * 2.0 ** -53, (int(w[1]) >> 11) * 2.0 ** -53,
+                 ((int(w[2]) >> 11) + 1) * 2.0 ** -53, (int(w[3]) >> 11) * 2.0 ** -53]
+            g = np.zeros(3)
+            L.orc_philox_gauss3(seed, step, stream, i, _p(g))
+            assert g[0] ** 2 + g[1] ** 2 == pytest.approx(-2.0 * np.log(u[0]), rel=1e-14)
+            ang = np.arctan2(g[1], g[0]) % (2.0 * np.pi)
+            assert ang == pytest.approx((2.0 * np.pi * u[1]) % (2.0 * np.pi), rel=1e-12, abs=1e-12)
+            assert abs(g[2]) == pytest.approx(np.sqrt(-2.0 * np.log(u[2])) *
+                                              abs(np.cos(2.0 * np.pi * u[3])), rel=1e-13, abs=1e-15)
+
+
 # --------------------------------------------------------------- invariants
 @pytest.mark.parametrize("seed,n", [(1, 40), (2, 120)])
 def test_newton_third_law(seed, n):
