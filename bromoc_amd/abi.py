@@ -211,6 +211,7 @@ def _declare(lib):
                                C.c_double, C.c_double, C.c_double, _dp, P(Pair)], C.c_int),
         "bromoc_update_tables": ([vp, C.c_int32, _ip, _ip, _dp, _dp], C.c_int),
         "bromoc_table_vjp_device": ([vp, C.c_int32, _dp, _dp, _dp, _dp], C.c_int),
+        "bromoc_table_vjp_device_noalloc": ([vp, C.c_int32, _dp, _dp, _dp, _dp], C.c_int),
         "bromoc_set_analysis": ([vp, P(Analysis)], C.c_int),
         "bromoc_analyze": ([vp], C.c_int),
         "bromoc_get_analysis": ([vp, P(C.c_int64), P(C.c_int64), P(C.c_int64), _ip,

@@ -551,6 +551,8 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     c->nf = s->nfix;
     auto cleanup = [&](int code) { c->err = c->err.empty() ? "create failed" : c->err; bromoc_destroy(c); return code; };
     if (hipSetDevice(device) != hipSuccess) return cleanup(BROMOC_E_HIP);
+    // the device address of the allocation-free entry point's workspace on this device
+    if (vjp_scratch_init(device, nullptr) != hipSuccess) return cleanup(BROMOC_E_HIP);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(BROMOC_E_HIP);
     if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) return cleanup(BROMOC_E_HIP);
     if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) return cleanup(BROMOC_E_HIP);
@@ -1968,6 +1970,39 @@ int bromoc_table_vjp_device(bromoc_ctx *c, int32_t n, const double *r, const dou
     for (double *q : {d_r, d_gep, d_gsc, d_out, d_scr})
         if (q) (void)hipFree(q);
     return st;
+}
+
+// The same product with no allocation: inputs, outputs and the per-knot workspace live in
+// the device's fixed `scratch` array (tables.hip), whose address was taken at bromoc_create.
+// Layout: r [n], grad_ep [3n], grad_sc [3], grad_u [n], then 7n doubles per knot of a launch.
+int bromoc_table_vjp_device_noalloc(bromoc_ctx *c, int32_t n, const double *r,
+                                    const double *grad_ep, const double *grad_sc,
+                                    double *grad_u) {
+    if (!c || n < 2 || !r || !grad_ep || !grad_sc || !grad_u) return BROMOC_E_ARG;
+    if (r[0] == 0.0) return BROMOC_E_ARG;
+    for (int k = 1; k < n; k++)
+        if (!(r[k] > r[k - 1])) return BROMOC_E_ARG;
+    const size_t cap = vjp_scratch_doubles();
+    const size_t fixed = 5 * (size_t)n + 3, per = 7 * (size_t)n;
+    if (fixed + per > cap)
+        return fail(c, BROMOC_E_CAPACITY,
+                    "table_vjp_device_noalloc: n = " + std::to_string(n) + " needs " +
+                        std::to_string(fixed + per) + " doubles of workspace at least; " +
+                        std::to_string(cap) + " exist (n <= " + std::to_string((cap - 3) / 12) + ")");
+    (void)hipSetDevice(c->device);
+    double *base = nullptr;
+    HIPCHK(c, vjp_scratch_init(c->device, &base));
+    double *d_r = base, *d_gep = d_r + n, *d_gsc = d_gep + 3 * (size_t)n, *d_out = d_gsc + 3;
+    double *d_scr = d_out + n;
+    const int chunk = (int)std::min<size_t>((size_t)n, (cap - fixed) / per);
+    HIPCHK(c, hipMemcpyAsync(d_r, r, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_gep, grad_ep, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_gsc, grad_sc, sizeof(double) * 3, hipMemcpyHostToDevice, c->stream));
+    for (int k0 = 0; k0 < n; k0 += chunk)
+        HIPCHK(c, launch_table_vjp(n, d_r, d_gep, d_gsc, d_out, d_scr, k0, std::min(chunk, n - k0),
+                                   c->stream));
+    HIPCHK(c, hipMemcpyAsync(grad_u, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    return sync_err(c);
 }
 
 int bromoc_table_vjp(int32_t n, const double *r, const double *grad_ep, const double *grad_sc,

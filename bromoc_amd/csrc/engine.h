@@ -295,6 +295,10 @@ struct TabJob {
 };
 // k_tab_build (one thread per table: SPLINEPOT into ep) then k_tab_rows (the pair rows)
 hipError_t launch_table_update(const TabJob &j, hipStream_t s);
+// the fixed workspace of the allocation-free table VJP (tables.hip): its device address on
+// `device` (current), taken once per device; and its size in doubles
+hipError_t vjp_scratch_init(int device, double **out);
+size_t vjp_scratch_doubles();
 // grad_u[k0 + t] = (d table / d u_k) . (grad_ep, grad_sc), one thread per knot
 hipError_t launch_table_vjp(int n, const double *r, const double *gep, const double *gsc,
                             double *out, double *scratch, int k0, int count, hipStream_t s);

@@ -5,10 +5,41 @@
 // Each table is a sequential recurrence over its knots (squarespline's integration and
 // end correction), so a table is one thread; the Jacobian's columns are independent, one
 // thread per knot.
+#include <mutex>
+#include <vector>
+
 #include "engine.h"
 #include "tables.h"
 
 namespace bromoc {
+
+// Workspace of bromoc_table_vjp_device_noalloc, a file-scope device array: each GPU the
+// library runs on holds its own copy until the process ends.  `scratch` names a host-side
+// stand-in; the device address of each GPU's copy comes from hipGetSymbolAddress, taken once
+// per device (vjp_scratch_init) and used for every copy and kernel argument.
+constexpr size_t VJP_SCRATCH_DOUBLES = size_t(2) << 20;   // 16 MiB
+static __device__ __align__(256) double scratch[VJP_SCRATCH_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static std::vector<double *> g_scratch_ptr;   // by device; null until taken
+
+// Takes the address of this device's copy of scratch (the current device must be
+// `device`); kept for later calls.
+hipError_t vjp_scratch_init(int device, double **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) return hipErrorInvalidDevice;
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        double *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = ptr;
+    }
+    if (out) *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+size_t vjp_scratch_doubles() { return VJP_SCRATCH_DOUBLES; }
 
 namespace {
 

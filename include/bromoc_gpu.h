@@ -520,6 +520,14 @@ int bromoc_table_vjp(int32_t n, const double *r, const double *grad_ep, const do
  * bromoc_table_vjp. */
 int bromoc_table_vjp_device(bromoc_ctx *ctx, int32_t n, const double *r, const double *grad_ep,
                             const double *grad_sc, double *grad_u);
+/* The same product, results bit-identical, with no allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (a fitting loop under HIP graph
+ * capture, a serving loop).  Inputs, outputs and the per-knot workspace are staged in a
+ * fixed 16 MiB device array per GPU, so n <= 174762 (BROMOC_E_CAPACITY beyond).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU race. */
+int bromoc_table_vjp_device_noalloc(bromoc_ctx *ctx, int32_t n, const double *r,
+                                    const double *grad_ep, const double *grad_sc,
+                                    double *grad_u);
 /* Rebuild ntab tables of the context's pair types on its device from new samples
  * (SPLINEPOT, splinepot.f90:19-96, the construction bromoc_build_table runs on the host,
  * bit-identical), in place: table t is pair type is[t] with n[t] samples (its knot count
