@@ -65,17 +65,38 @@ def step_bytes(s, injected):
     return b
 
 
-def pair_counts(s, ntot=None):
+def rf_pairs(s, pos):
+    """Ion pairs whose reaction-field cross term counts (rfpar.f90:177-223: the lower ion
+    has srfe != 0 and the higher one moves inside the grid box), taking srfe != 0 as inside
+    the grid box (true of grids without zero nodes): C(m, 2) + f m for m moving and f fixed
+    ions inside the box at positions pos."""
+    import numpy as np
+    shp = s.rf.data.shape[-3:]
+    cen = np.array([s.rf.xbcen, s.rf.ybcen, s.rf.zbcen])
+    tran = 0.5 * (np.array(shp) - 1) * s.rf.dcel
+    inside = np.all(np.abs(pos - cen) <= tran, axis=1)
+    idx = np.arange(len(pos))
+    m = int(np.sum(inside & (idx >= s.nsites + s.nfix)))
+    f = int(np.sum(inside & (idx >= s.nsites) & (idx < s.nsites + s.nfix)))
+    return m * (m - 1) // 2 + f * m
+
+
+def pair_counts(s, ntot=None, pos=None):
     """Pair evaluations of one ENERGY as the reference makes them (each pair once): ion-ion
     pairs with a moving ion (energy.mono.f90:120-123), site-ion pairs (:586-588), the
-    reaction-field cross term of the ion pairs (rfpar.f90:177-223, at most one per ion pair
-    with a moving ion), and the CG-DNA list terms."""
+    reaction-field cross terms that count (rf_pairs, at the positions pos, by default the
+    system's own), and the CG-DNA list terms."""
+    import numpy as np
     from bromoc_amd import abi
     n = s.ntot if ntot is None else ntot
     ni, nf = n - s.nsites, s.nfix
     ii = ni * (ni - 1) // 2 - nf * (nf - 1) // 2
     si = s.nsites * ni
-    rf = ii if s.flags & abi.QRFPAR else 0
+    rf = 0
+    if s.flags & abi.QRFPAR:
+        if pos is None:
+            pos = np.stack([s.x, s.y, s.z], 1)[:n]
+        rf = rf_pairs(s, pos)
     lists = int(sum(s.count(k) for k in s.LIST_NAMES)) if s.nsites else 0
     return {"ion_ion": ii, "site_ion": si, "rf_cross": rf, "dna_list_terms": lists}
 
@@ -85,6 +106,29 @@ def fp64_rate(pc, ms):
     t = flops / (ms * 1e-3) / 1e12
     return {"achieved_tflops": t, "peak_tflops": FP64_PEAK_TFLOPS, "frac": t / FP64_PEAK_TFLOPS,
             "pairs_per_step": pc, "flops_per_step": flops}
+
+
+def roofline(workload, tpb, pc, kernel_ms, bpp, nmov):
+    """Roofline of the dominant kernel (all pair work of a step is in it): FP64-bound, so
+    achieved = the algorithmic FP64 operations of one launch (each pair once, at FLOP_PAIR /
+    FLOP_RF) over its event-timed duration, against MI355X's FP64 peak; the HBM figures
+    beside it (bpp algorithmic bytes per moving particle) and, when counted, traffic = the
+    HBM bytes of one launch from FETCH_SIZE / WRITE_SIZE."""
+    f = fp64_rate(pc, kernel_ms)
+    traffic = None
+    if TRAFFIC_KIB.get((workload, tpb)):
+        fk, wk = TRAFFIC_KIB[(workload, tpb)]
+        traffic = 1024.0 * (fk * TRAFFIC_SCALE[0] + wk * TRAFFIC_SCALE[1])
+    hbm = bpp * nmov / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "mfma", "achieved": f["achieved_tflops"], "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": f["frac"], "traffic": traffic,
+            "pipe": "FP64 VALU (pair math is not GEMM-shaped; MI355X's FP64 vector and "
+                    "matrix peaks are both 78.6 TFLOP/s)",
+            "kernel": "k_tiles" if tpb > 16 else "k_step", "kernel_ms": kernel_ms,
+            "launch_shape": TPB_NAMES.get(tpb, str(tpb)), "targets_per_block": tpb,
+            "flops_per_launch": f["flops_per_step"], "pairs_per_launch": pc,
+            "hbm": {"achieved_gbs": hbm, "peak_gbs": HBM_PEAK_GBS, "frac": hbm / HBM_PEAK_GBS,
+                    "bytes_per_particle_step": bpp, "algorithmic_bytes_per_launch": bpp * nmov}}
 
 
 def cpu_info():
@@ -228,25 +272,16 @@ def translocation_leg(steps, warmup, cpu=True, parity=True):
     e.step(steps, abi.NOISE_PHILOX, seed=11, counter0=ctr + nk, energies=True)
     plain_ms = 1e3 * (time.perf_counter() - t0) / steps
     n_now = e.ntot
+    pos_now = e.positions()[0]
     e.close()
-    pc = pair_counts(s, n_now)
-    bpp = step_bytes(s, injected=False)
-    achieved = bpp * (n_now - s.nfix) / (kernel_ms * 1e-3) / 1e9
-    traffic = None
-    if TRAFFIC_KIB.get(("translocation", tpb)):
-        fk, wk = TRAFFIC_KIB[("translocation", tpb)]
-        traffic = 1024.0 * (fk * TRAFFIC_SCALE[0] + wk * TRAFFIC_SCALE[1])
+    pc = pair_counts(s, n_now, pos_now)
+    roof = roofline("translocation", tpb, pc, kernel_ms, step_bytes(s, injected=False),
+                    n_now - s.nfix)
     wl = {"workload": "translocation", "ions": s.ntot - s.nsites, "dna_sites": s.nsites,
           "dna_bp": s.meta["n_bp"], "box_A": [200, 200, 600], "grid": list(s.meta["grid"]),
           "grid_spacing_A": s.meta["dcel"], "rfpar": True, "diffusion": "profile",
           "pair_tables": s.npair, "dt_ps": s.meta["dt"], "noise": "philox",
           "gcmc": "20 GRAND attempts every 10 steps", "energies": "last step of each call"}
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "k_tiles" if tpb > 16 else "k_step", "kernel_ms": kernel_ms,
-            "launch_shape": TPB_NAMES.get(tpb, str(tpb)), "targets_per_block": tpb,
-            "bytes_per_particle_step": bpp,
-            "fp64_valu": fp64_rate(pc, kernel_ms)}
     out = {"value": psteps / elapsed, "elapsed_s": elapsed, "ms_per_step": 1e3 * elapsed / steps,
            "config": wl, "roofline": roof, "step_ms_without_gcmc": plain_ms,
            "step_kernels_ms": total_ms / nk, "particles": n_now, "setup_s": build_s,
@@ -276,20 +311,10 @@ def pore_dna_leg(steps=2000, warmup=200, cpu=True):
     _, kms = e.time_steps(200, abi.NOISE_PHILOX, seed=2024, counter0=warmup + steps)
     host = host_roundtrip(e, nmov)
     e.close()
-    bpp = step_bytes(s, injected=False)
-    traffic = None
-    if TRAFFIC_KIB.get(("pore_dna", tpb)):
-        fk, wk = TRAFFIC_KIB[("pore_dna", tpb)]
-        traffic = 1024.0 * (fk * TRAFFIC_SCALE[0] + wk * TRAFFIC_SCALE[1])
-    ach = bpp * nmov / (kms * 1e-3) / 1e9
     out = {"value": nmov * steps / el, "unit": "particle-steps/s", "ms_per_step": 1e3 * el / steps,
            "steps": steps, "ions": s.ntot - s.nsites, "dna_sites": s.nsites,
-           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                        "kernel": "k_tiles" if tpb > 16 else "k_step", "kernel_ms": kms,
-                        "launch_shape": TPB_NAMES.get(tpb, str(tpb)),
-                        "targets_per_block": tpb, "bytes_per_particle_step": bpp,
-                        "fp64_valu": fp64_rate(pair_counts(s), kms)},
+           "roofline": roofline("pore_dna", tpb, pair_counts(s), kms,
+                                step_bytes(s, injected=False), nmov),
            "host_roundtrip": host}
     if cpu:
         out["cpu_baseline"] = cpu_baseline(s, target_s=8.0)
@@ -397,7 +422,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="translocation",
-                    choices=["translocation", "pore_dna"])
+                    choices=["translocation", "pore_dna", "adjoint"],
+                    help="the headline workload; adjoint prints the adjoint leg alone")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-parity", action="store_true", help="skip the start-state parity check")
     ap.add_argument("--no-batch", action="store_true", help="skip the replica-batch legs")
@@ -408,6 +434,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload == "adjoint":
+        print(json.dumps({"adjoint_rdf_gradient": adjoint_leg()}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "

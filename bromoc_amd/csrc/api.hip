@@ -588,6 +588,8 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
         const double lo[3] = {s->lx2m, s->ly2m, s->lz2m}, hi[3] = {s->lx2p, s->ly2p, s->lz2p};
         const double cell = 8.0;
         g.inv = 1.0 / cell;
+        g.nsites = ns;
+        g.nfix = s->nfix;
         for (int k = 0; k < 3; k++) {
             g.lo[k] = lo[k];
             const double ext = hi[k] - lo[k];

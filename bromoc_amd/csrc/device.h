@@ -848,6 +848,15 @@ __device__ __forceinline__ double rsqrt_nr(double x2) {
     return r;
 }
 
+// 1/sqrt(x2) from the hardware estimate and one third-order step (the series of
+// (1 - e)^(-1/2) to e^2): the step leaves about (5/16) e^3 of the estimate's relative error
+// e, so the result is within a few ulp; for force factors, not for the table index.
+__device__ __forceinline__ double rsqrt_c3(double x2) {
+    const double r = __builtin_amdgcn_rsq(x2);
+    const double e = fma(-x2 * r, r, 1.0);
+    return fma(r * e, fma(0.375, e, 0.5), r);
+}
+
 // Gaussians of particle i for this launch's step: injected rows are addressed by the
 // step counter, Philox rows were generated for the current chunk of steps by
 // k_philox_fill (row = step_off).
@@ -861,11 +870,12 @@ __device__ __forceinline__ const double *noise_row(const StepParams &P, int i) {
 // analytic tail sc3/r^6 + fct/r from rinv = 1/sqrt(d2); only lanes inside the table range
 // (d2 <= dm2hi) enter the branch that replaces it with the wall or the tabulated quadratic,
 // so far partners cost ~25 operations and near ones pay for the knot gather alone.
-// tl = (sc3, tail fct, dm2hi, unused) of the pair type.
+// tl = (sc3, tail fct, dm2hi, unused) of the pair type.  C3: 1/r from rsqrt_c3.
+template <bool C3 = false>
 __device__ __forceinline__ double pair_near_far(const DevPair &pp, const double4 &tl,
                                                const double4 *__restrict__ ep, double ires,
                                                double d2, double &y) {
-    const double rinv = rsqrt_nr(d2);
+    const double rinv = C3 ? rsqrt_c3(d2) : rsqrt_nr(d2);
     const double ix2 = rinv * rinv;
     const double c = tl.x * pow3(ix2);
     const double f = tl.y * rinv;

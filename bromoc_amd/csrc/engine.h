@@ -189,7 +189,8 @@ enum { TE_EFPOT = 0, TE_ELEC, TE_VDW, TE_SRPMF, TE_EFPOTMX, TE_QQMX, TE_VDWMX, T
 struct SortGeom {
     double lo[3];
     double inv;        // 1 / cell size
-    int32_t n[3], pad;
+    int32_t n[3];
+    int32_t nsites, nfix, pad;   // particle classes: sites, fixed ions, then moving ions
 };
 struct SortBufs {
     SortGeom geom;

@@ -1,8 +1,10 @@
 // sort.hip -- spatial order of the partner array: a cell list as an integer key sort plus a
 // prefix scan.
 //
-// Every particle gets the Morton code of its cell on a regular lattice over the box
-// (10 bits per axis).  A stable least-significant-digit radix sort of (key, particle index)
+// Every particle gets a 32-bit key: its class in the top two bits (0 CG-DNA site, 1 fixed
+// ion, 2 moving ion) over the Morton code of its cell on a regular lattice over the box
+// (10 bits per axis), so each class is one run of the order and its tiles hold one class
+// only, but for the tile a run boundary falls in.  A stable least-significant-digit radix sort of (key, particle index)
 // -- four passes of 8 bits, each a per-block digit histogram, one exclusive scan of the
 // histograms in (digit, block) order and a stable scatter -- gives the partner order; a
 // second scan over the sorted keys numbers the occupied cells and records the first slot of
@@ -19,7 +21,7 @@ namespace bromoc {
 
 constexpr int SORT_BLOCK = 256;        // threads of the histogram / scatter blocks
 constexpr int SORT_RADIX = 256;        // 8-bit digits
-constexpr int SORT_PASSES = 4;         // 32 bits of key (the Morton code uses 30)
+constexpr int SORT_PASSES = 4;         // 32 bits of key (class 2, Morton code 30)
 constexpr int SCAN_BLOCK = 1024;
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
@@ -45,7 +47,8 @@ __global__ void k_keys(const double4 *__restrict__ pos, int n, SortGeom g, uint3
     const uint32_t cx = cell_coord(p.x, g.lo[0], g.inv, g.n[0]);
     const uint32_t cy = cell_coord(p.y, g.lo[1], g.inv, g.n[1]);
     const uint32_t cz = cell_coord(p.z, g.lo[2], g.inv, g.n[2]);
-    keys[i] = spread10(cx) | (spread10(cy) << 1) | (spread10(cz) << 2);
+    const uint32_t cls = i < g.nsites ? 0u : i < g.nsites + g.nfix ? 1u : 2u;
+    keys[i] = (cls << 30) | spread10(cx) | (spread10(cy) << 1) | (spread10(cz) << 2);
     vals[i] = i;
 }
 

@@ -16,16 +16,30 @@
 //    end the wavefront writes tile I's sums to tpart[J][I] and tile J's to tpart[I][J]
 //    (a diagonal tile pair writes their sum once): every entry of tpart is written exactly
 //    once per step, and the pair energies go to tener[cat][item];
+//  * the sweep a wavefront runs depends on what its two tiles hold (the sort puts CG-DNA
+//    sites, fixed ions and moving ions in separate runs, so most tiles hold one class):
+//    two tiles of moving ions off the diagonal need no per-pair masks, and their
+//    reaction-field term, when every ion of both tiles is either inside the grid with a
+//    nonzero srfe or outside it with none, reduces to per-ion weights folded into the
+//    charges (or vanishes, when one tile has no ion inside); a tile of sites against a tile
+//    of moving ions is all site-ion pairs with no reaction field; tiles of sites and fixed
+//    ions only have no pair to evaluate.  Everything else takes the masked sweep;
 //  * k_tfin: eight lanes per slot sum the slot's ntile partial rows in a fixed order,
 //    gather the grid corners and finish the particle as k_step does (CG-DNA list forces,
 //    reaction-field gradients and self term, MEMBRANE, energies, BD update).
+#include <type_traits>
+
 #include "device.h"
 
 namespace bromoc {
 
 constexpr int F_RFT = 64;   // reaction-field cross term fused into the pair sweep
-// slot flags (bits above the atnam2 type, which takes the low 16)
-constexpr int SL_VALID = 1, SL_SITE = 2, SL_FIXED = 4, SL_RECV = 8, SL_HI = 16;
+// slot flags (bits above the atnam2 type, which takes the low 16): a particle (VALID), a
+// site, a fixed ion, receiving pair forces (RECV), moving and inside the reaction-field grid
+// (HI), a moving ion (ION); for the reaction field, a moving ion with HI and srfe != 0
+// (RFON) and one with exactly one of the two (RFODD)
+constexpr int SL_VALID = 1, SL_SITE = 2, SL_FIXED = 4, SL_RECV = 8, SL_HI = 16, SL_ION = 32,
+              SL_RFON = 64, SL_RFODD = 128;
 
 size_t tile_part_doubles(int cap, int ttpl, int rf) {
     const size_t tt = 64 * (size_t)ttpl, n = (cap + tt - 1) / tt;
@@ -54,8 +68,9 @@ __device__ __forceinline__ SlotInfo slot_info(const StepParams &P, int s) {
     if constexpr ((FEAT & F_DNA) != 0) {
         if (site) recv = P.stfree[idx] != 0;
     }
+    const bool ion = ok && !site && !fixed;
     int fl = (ok ? SL_VALID : 0) | (site ? SL_SITE : 0) | (fixed ? SL_FIXED : 0) |
-             (recv ? SL_RECV : 0);
+             (recv ? SL_RECV : 0) | (ion ? SL_ION : 0);
     o.x = pa.x; o.y = pa.y; o.z = pa.z;
     o.sr = 0.0; o.rr = 0.0; o.q = 0.0;
     if constexpr ((FEAT & F_RFT) != 0) {
@@ -64,12 +79,31 @@ __device__ __forceinline__ SlotInfo slot_info(const StepParams &P, int s) {
             const double4 ra = P.rfa[idx];
             o.sr = ra.x; o.rr = ra.y;
             o.q = P.rfq[idx].w;
-            if (ra.z != 0.0 && ra.w != 0.0) fl |= SL_HI;
+            const bool hi = ra.z != 0.0 && ra.w != 0.0;
+            if (hi) fl |= SL_HI;
+            if (ion && hi && ra.x != 0.0) fl |= SL_RFON;
+            if (ion && hi != (ra.x != 0.0)) fl |= SL_RFODD;
         }
     }
     o.tf = t2 | (fl << 16);
     o.idx = idx;
     return o;
+}
+
+// the low 8 bits of v AND-ed (OR-ed) over the wavefront
+__device__ __forceinline__ int wave_and8(int v) {
+    int r = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+        if (__ballot(((v >> b) & 1) == 0) == 0ull) r |= 1 << b;
+    return r;
+}
+__device__ __forceinline__ int wave_or8(int v) {
+    int r = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+        if (__ballot((v >> b) & 1) != 0ull) r |= 1 << b;
+    return r;
 }
 
 // item b of the n (n + 1) / 2 tile pairs -> (I, J), I <= J
@@ -226,8 +260,128 @@ __global__ __launch_bounds__(WAVE) void k_tiles(StepParams P) {
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
         }
     };
-    if (P.write_energy) sweep(std::true_type{});
-    else sweep(std::false_type{});
+    // Mask-free sweep over tile pairs of one class each (see the file comment).  RFW: the
+    // reaction-field term with the ion weights in the charges (a pair counts when both
+    // ions are RFON; with no RFODD ion that is the reference's rule, rfpar.f90:177-223, in
+    // either index order); MX: sites against moving ions, all pairs mixed, forces kept
+    // only for receiving particles.  The radial reaction-field force joins the pair force.
+    auto sweep_fast = [&](auto en_on, auto rf_on, auto mx_on) {
+        constexpr bool EN = decltype(en_on)::value;
+        constexpr bool RFW = decltype(rf_on)::value;
+        constexpr bool MX = decltype(mx_on)::value;
+        double tcq[TPL];
+#pragma unroll
+        for (int u = 0; u < TPL; u++)
+            tcq[u] = ((ttf[u] >> 16) & SL_RFON) ? P.celec * tq[u] : 0.0;
+        for (int k = 0; k < TT; k++) {
+            const int p = ((lane + k / TPL) & (WAVE - 1)) + WAVE * (k % TPL);
+            const double px = s_x[p], py = s_y[p], pz = s_z[p];
+            const int ptf = s_tf[p];
+            const int t2p = ptf & 0xFFFF, pfl = ptf >> 16;
+            double psr = 0.0, prr = 0.0, pq = 0.0;
+            if constexpr (RFW) {
+                psr = s_sr[p];
+                prr = s_rr[p];
+                pq = (pfl & SL_RFON) ? s_q[p] : 0.0;
+            }
+            double gx = 0.0, gy = 0.0, gz = 0.0, ga = 0.0, gb = 0.0;
+#pragma unroll
+            for (int u = 0; u < TPL; u++) {
+                const int t2t = ttf[u] & 0xFFFF;
+                const double dx = tx[u] - px, dy = ty[u] - py, dz = tz[u] - pz;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const int row = t2t * nt1 + t2p;
+                const double4 tl = s_tail[row];
+                double y, de;
+                if constexpr ((FEAT & F_TAB) != 0) {
+                    de = pair_near_far<true>(P.rows[row], tl, P.ep, P.ires, d2, y);
+                    if constexpr (EN) en[MX ? TE_EFPOTMX : TE_EFPOT] += y;
+                } else if ((int)tl.w & PF_EFPOT) {
+                    de = pair_near_far<true>(P.rows[row], tl, P.ep, P.ires, d2, y);
+                    if constexpr (EN) en[MX ? TE_EFPOTMX : TE_EFPOT] += y;
+                } else {
+                    double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+                    de = pair_analytic(P.rows[row], P, d2, q0, q1, q2);
+                    if constexpr (EN) {
+                        en[MX ? TE_QQMX : TE_ELEC] += q0;
+                        en[MX ? TE_VDWMX : TE_VDW] += q1;
+                        en[MX ? TE_SRPMFMX : TE_SRPMF] += q2;
+                    }
+                }
+                if constexpr (RFW) {
+                    const double srfeij = tsr[u] * psr, reffij = trr[u] * prr;
+                    const double rfdn = rsqrt_c3(fma(reffij, reffij, d2));
+                    const double aux0 = tcq[u] * pq;
+                    const double aux1 = aux0 * (reffij * rfdn);
+                    const double aux2 = aux0 * srfeij * (rfdn * (rfdn * rfdn));
+                    const double aux3 = -aux2 * (d2 + reffij * 0.5);
+                    if constexpr (EN) en[TE_RFPAR] = fma(aux1, srfeij, en[TE_RFPAR]);
+                    aa_[u] = fma(-aux1, psr, aa_[u]);
+                    ab_[u] = fma(aux3, prr, ab_[u]);
+                    ga = fma(-aux1, tsr[u], ga);
+                    gb = fma(aux3, trr[u], gb);
+                    de = fma(aux2, reffij, de);
+                }
+                ax_[u] = fma(de, dx, ax_[u]);
+                ay_[u] = fma(de, dy, ay_[u]);
+                az_[u] = fma(de, dz, az_[u]);
+                gx = fma(de, dx, gx);
+                gy = fma(de, dy, gy);
+                gz = fma(de, dz, gz);
+            }
+            if constexpr (MX) {
+                const double pw = (pfl & SL_RECV) ? 1.0 : 0.0;
+                s_fx[p] -= pw * gx;
+                s_fy[p] -= pw * gy;
+                s_fz[p] -= pw * gz;
+            } else {
+                s_fx[p] -= gx;
+                s_fy[p] -= gy;
+                s_fz[p] -= gz;
+            }
+            if constexpr (RFW) {
+                s_fa[p] += ga;
+                s_fb[p] += gb;
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        }
+        if constexpr (MX) {
+#pragma unroll
+            for (int u = 0; u < TPL; u++)
+                if (!((ttf[u] >> 16) & SL_RECV)) ax_[u] = ay_[u] = az_[u] = 0.0;
+        }
+    };
+
+    // the classes of the two tiles, uniform over the wavefront
+    int andI = ~0, orI = 0, andJ = ~0, orJ = 0;
+#pragma unroll
+    for (int u = 0; u < TPL; u++) {
+        const int fi = ttf[u] >> 16, fj = s_tf[lane + WAVE * u] >> 16;
+        andI &= fi; orI |= fi; andJ &= fj; orJ |= fj;
+    }
+    andI = wave_and8(andI); orI = wave_or8(orI);
+    andJ = wave_and8(andJ); orJ = wave_or8(orJ);
+    const bool ionI = andI & SL_ION, ionJ = andJ & SL_ION;
+    const bool siteI = (andI & (SL_VALID | SL_SITE)) == (SL_VALID | SL_SITE);
+    const bool siteJ = (andJ & (SL_VALID | SL_SITE)) == (SL_VALID | SL_SITE);
+    const bool wr = P.write_energy != 0;
+    if (!(orI & SL_ION) && !(orJ & SL_ION)) {
+        // sites, fixed ions and empty slots only: no pair of these is evaluated
+    } else if (!diag && ionI && ionJ && !(RF && ((orI | orJ) & SL_RFODD))) {
+        if (RF && (orI & SL_RFON) && (orJ & SL_RFON)) {
+            if (wr) sweep_fast(std::true_type{}, std::bool_constant<RF>{}, std::false_type{});
+            else sweep_fast(std::false_type{}, std::bool_constant<RF>{}, std::false_type{});
+        } else {
+            if (wr) sweep_fast(std::true_type{}, std::false_type{}, std::false_type{});
+            else sweep_fast(std::false_type{}, std::false_type{}, std::false_type{});
+        }
+    } else if (!diag && ((ionI && siteJ) || (siteI && ionJ))) {
+        if (wr) sweep_fast(std::true_type{}, std::false_type{}, std::true_type{});
+        else sweep_fast(std::false_type{}, std::false_type{}, std::true_type{});
+    } else {
+        if (wr) sweep(std::true_type{});
+        else sweep(std::false_type{});
+    }
     __syncthreads();
 
     const int NC = RF ? 5 : 3;

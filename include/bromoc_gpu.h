@@ -293,7 +293,8 @@ int bromoc_step(bromoc_ctx *ctx, int32_t nsteps, int32_t noise_mode, int32_t fir
                 uint64_t seed, uint64_t counter0, bromoc_energies *energies);
 
 /* The spatial order the step kernels use, rebuilt from the current positions: keys[ntot]
- * the sorted cell keys (Morton codes of the 8 A cells of the box, 10 bits per axis),
+ * the sorted keys (the particle's class in bits 30-31 -- 0 CG-DNA site, 1 fixed ion,
+ * 2 moving ion -- over the Morton code of its 8 A cell of the box, 10 bits per axis),
  * order[ntot] the particle in each slot (a stable sort: equal keys keep index order),
  * rank[ntot] the slot of each particle, cell_start[ncell+1] the first slot of each occupied
  * cell (then ntot) and *ncell.  Any output may be NULL.  No counterpart in the reference,

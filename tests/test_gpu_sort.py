@@ -2,8 +2,8 @@
 cells' Morton keys and a prefix scan numbering the occupied cells.
 
 Checked on the host: the order is a permutation, rank is its inverse, the keys are the
-Morton codes of the particles' 8 A cells (10 bits per axis, interleaved x, y, z) and
-non-decreasing, equal keys keep index order (the order equals numpy's stable argsort of the
+particles' classes (sites, fixed ions, moving ions) over the Morton codes of their 8 A cells
+(10 bits per axis, interleaved x, y, z) and non-decreasing, equal keys keep index order (the order equals numpy's stable argsort of the
 same keys), and cell_start marks exactly the slots where the key changes.  Sizes from one
 particle to the large translocation system, with particles on and outside the box edges
 (clamped to the edge cells) and many particles sharing cells.
@@ -14,12 +14,12 @@ import pytest
 from bromoc_amd import configs
 from bromoc_amd.engine import Engine
 
-from helpers import plain, pore_small
+from helpers import plain, pore_small, with_fixed
 
 pytestmark = pytest.mark.gpu
 
 
-def morton(pos, box, cell=8.0):
+def morton(pos, box, nsites=0, nfix=0, cell=8.0):
     lo = np.array([box[0], box[2], box[4]])
     hi = np.array([box[1], box[3], box[5]])
     n = np.clip(np.ceil((hi - lo) / cell).astype(int), 1, 1024)
@@ -32,7 +32,10 @@ def morton(pos, box, cell=8.0):
         for b in range(10):
             out |= ((v >> b) & 1) << (3 * b)
         return out
-    return (spread(k[:, 0]) | (spread(k[:, 1]) << 1) | (spread(k[:, 2]) << 2)).astype(np.uint32)
+    i = np.arange(len(pos))
+    cls = np.where(i < nsites, 0, np.where(i < nsites + nfix, 1, 2)).astype(np.int64)
+    return ((cls << 30) | spread(k[:, 0]) | (spread(k[:, 1]) << 1) |
+            (spread(k[:, 2]) << 2)).astype(np.uint32)
 
 
 def check(e, s, pos):
@@ -40,7 +43,7 @@ def check(e, s, pos):
     n = len(pos)
     assert np.array_equal(np.sort(order), np.arange(n))
     assert np.array_equal(rank[order], np.arange(n))
-    want = morton(pos, s.box)
+    want = morton(pos, s.box, s.nsites, s.nfix)
     assert np.array_equal(keys, want[order])
     assert np.all(np.diff(keys.astype(np.int64)) >= 0)
     assert np.array_equal(order, np.argsort(want, kind="stable"))
@@ -64,7 +67,8 @@ def test_random_positions(n):
 
 
 def test_pore_dna_and_translocation():
-    for s in (pore_small(seed=3), configs.pore_dna(seed=1), configs.translocation()):
+    for s in (pore_small(seed=3), with_fixed(pore_small(seed=4), 12), configs.pore_dna(seed=1),
+              configs.translocation()):
         e = Engine(s)
         pos = np.stack([s.x, s.y, s.z], 1)[: s.ntot]
         ncell = check(e, s, pos)
