@@ -42,7 +42,7 @@ class Pair(C.Structure):
 
 class Grid(C.Structure):
     _fields_ = [("nclx", C.c_int32), ("ncly", C.c_int32), ("nclz", C.c_int32),
-                ("reserved", C.c_int32), ("dcel", C.c_double), ("xbcen", C.c_double),
+                ("nmaps", C.c_int32), ("dcel", C.c_double), ("xbcen", C.c_double),
                 ("ybcen", C.c_double), ("zbcen", C.c_double), ("data", C.c_void_p)]
 
 

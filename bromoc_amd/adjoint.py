@@ -132,8 +132,10 @@ def table_gradients(system: System, noise, nsteps, spec: RdfSpec, target=None,
     return loss.value, gep, gsc
 
 
-def knot_gradients(system: System, gep, gsc, lib=None):
-    """{pair type: dL/du} through the table build of each parameter pair type."""
+def knot_gradients(system: System, gep, gsc, lib=None, ctx=None):
+    """{pair type: dL/du} through the table build of each parameter pair type: on the
+    device of context ctx when one is given (one thread per knot), else on the host;
+    both give the same bits."""
     lib = lib or abi.load_library()
     _declare(lib)
     r = np.ascontiguousarray(system.meta["rgrid"], float)
@@ -144,9 +146,13 @@ def knot_gradients(system: System, gep, gsc, lib=None):
         ge = np.ascontiguousarray(gep[3 * off:3 * (off + n)])
         gs = np.ascontiguousarray(gsc[3 * is_:3 * is_ + 3])
         gu = np.zeros(n)
-        rc = lib.bromoc_table_vjp(n, ptr(r), ptr(ge), ptr(gs), ptr(gu))
+        if ctx is not None:
+            rc = lib.bromoc_table_vjp_device(ctx, n, ptr(r), ptr(ge), ptr(gs), ptr(gu))
+        else:
+            rc = lib.bromoc_table_vjp(n, ptr(r), ptr(ge), ptr(gs), ptr(gu))
         if rc:
-            raise abi.BromocError("bromoc_table_vjp", rc)
+            raise abi.BromocError("bromoc_table_vjp_device" if ctx is not None
+                                  else "bromoc_table_vjp", rc)
         out[is_] = gu
     return out
 

@@ -196,7 +196,7 @@ void energies_from(const double *r, bool rfpar, bool mmij, bromoc_energies *out)
 }
 
 int validate(const bromoc_system *s, std::string &msg) {
-    auto bad = [&](const char *m) { msg = m; return BROMOC_E_ARG; };
+    auto bad = [&](const std::string &m) { msg = m; return BROMOC_E_ARG; };
     if (s->abi_version != BROMOC_ABI_VERSION) return bad("abi_version mismatch");
     if (s->flags & ~SUPPORTED) {
         msg = "unsupported reference option in flags";
@@ -301,6 +301,14 @@ int validate(const bromoc_system *s, std::string &msg) {
         if (!(s->flags & BROMOC_QPAR) && s->ntot > s->nsites + s->nfix)
             return bad("NMCDEN maps of moving ions need PARTICLE");
         if (phiv_maps(s) < 1) return bad("NMCDEN map count");
+        // the ion maps follow the site maps from type nold+1; without CG-DNA there are no
+        // nucleotide types for them to follow
+        if (!(s->flags & BROMOC_QNUCL) && s->nold != 0) return bad("NMCDEN with nold != 0 needs NUCLEOTIDES");
+        if (s->phiv.nmaps != phiv_maps(s))
+            return bad("phiv.nmaps = " + std::to_string(s->phiv.nmaps) + " but NMCDEN reads " +
+                       std::to_string(phiv_maps(s)) + " maps");
+    } else if ((s->flags & BROMOC_QPHIV) && (s->phiv.nmaps < 0 || s->phiv.nmaps > 1)) {
+        return bad("phiv.nmaps > 1 without NMCDEN");
     }
     if ((s->flags & BROMOC_QCOUNTION) && (s->cntpts < 0 || (s->cntpts > 0 && !s->zcont)))
         return bad("flux planes");
@@ -1952,21 +1960,16 @@ int bromoc_table_vjp_device(bromoc_ctx *c, int32_t n, const double *r, const dou
     for (int k = 1; k < n; k++)
         if (!(r[k] > r[k - 1])) return BROMOC_E_ARG;
     (void)hipSetDevice(c->device);
-    // knots per launch: each thread keeps 7n doubles of its own (unit samples, table,
-    // squarespline scratch); at most 256 MiB of them at a time
-    const int64_t per = 7 * (int64_t)n;
-    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n, (256ll << 20) / (8 * per)));
     double *d_r = nullptr, *d_gep = nullptr, *d_gsc = nullptr, *d_out = nullptr, *d_scr = nullptr;
     hipError_t e = hipMalloc((void **)&d_r, sizeof(double) * n);
     if (!e) e = hipMalloc((void **)&d_gep, sizeof(double) * 3 * n);
     if (!e) e = hipMalloc((void **)&d_gsc, sizeof(double) * 3);
     if (!e) e = hipMalloc((void **)&d_out, sizeof(double) * n);
-    if (!e) e = hipMalloc((void **)&d_scr, sizeof(double) * per * chunk);
+    if (!e) e = hipMalloc((void **)&d_scr, sizeof(double) * 6 * n);
     if (!e) e = hipMemcpyAsync(d_r, r, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
     if (!e) e = hipMemcpyAsync(d_gep, grad_ep, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream);
     if (!e) e = hipMemcpyAsync(d_gsc, grad_sc, sizeof(double) * 3, hipMemcpyHostToDevice, c->stream);
-    for (int k0 = 0; !e && k0 < n; k0 += chunk)
-        e = launch_table_vjp(n, d_r, d_gep, d_gsc, d_out, d_scr, k0, std::min(chunk, n - k0), c->stream);
+    if (!e) e = launch_table_vjp(n, d_r, d_gep, d_gsc, d_out, d_scr, c->stream);
     if (!e) e = hipMemcpyAsync(grad_u, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
     int st = e ? fail(c, BROMOC_E_HIP, std::string("table_vjp_device: ") + hipGetErrorString(e)) : sync_err(c);
     for (double *q : {d_r, d_gep, d_gsc, d_out, d_scr})
@@ -1974,9 +1977,9 @@ int bromoc_table_vjp_device(bromoc_ctx *c, int32_t n, const double *r, const dou
     return st;
 }
 
-// The same product with no allocation: inputs, outputs and the per-knot workspace live in
+// The same product with no allocation: inputs, outputs and the sweep's workspace live in
 // the device's fixed `scratch` array (tables.hip), whose address was taken at bromoc_create.
-// Layout: r [n], grad_ep [3n], grad_sc [3], grad_u [n], then 7n doubles per knot of a launch.
+// Layout: r [n], grad_ep [3n], grad_sc [3], grad_u [n], workspace [6n]: 11n + 3 doubles.
 int bromoc_table_vjp_device_noalloc(bromoc_ctx *c, int32_t n, const double *r,
                                     const double *grad_ep, const double *grad_sc,
                                     double *grad_u) {
@@ -1985,24 +1988,21 @@ int bromoc_table_vjp_device_noalloc(bromoc_ctx *c, int32_t n, const double *r,
     for (int k = 1; k < n; k++)
         if (!(r[k] > r[k - 1])) return BROMOC_E_ARG;
     const size_t cap = vjp_scratch_doubles();
-    const size_t fixed = 5 * (size_t)n + 3, per = 7 * (size_t)n;
-    if (fixed + per > cap)
+    const size_t need = 11 * (size_t)n + 3;
+    if (need > cap)
         return fail(c, BROMOC_E_CAPACITY,
                     "table_vjp_device_noalloc: n = " + std::to_string(n) + " needs " +
-                        std::to_string(fixed + per) + " doubles of workspace at least; " +
-                        std::to_string(cap) + " exist (n <= " + std::to_string((cap - 3) / 12) + ")");
+                        std::to_string(need) + " doubles of workspace; " + std::to_string(cap) +
+                        " exist (n <= " + std::to_string((cap - 3) / 11) + ")");
     (void)hipSetDevice(c->device);
     double *base = nullptr;
     HIPCHK(c, vjp_scratch_init(c->device, &base));
     double *d_r = base, *d_gep = d_r + n, *d_gsc = d_gep + 3 * (size_t)n, *d_out = d_gsc + 3;
     double *d_scr = d_out + n;
-    const int chunk = (int)std::min<size_t>((size_t)n, (cap - fixed) / per);
     HIPCHK(c, hipMemcpyAsync(d_r, r, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(d_gep, grad_ep, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(d_gsc, grad_sc, sizeof(double) * 3, hipMemcpyHostToDevice, c->stream));
-    for (int k0 = 0; k0 < n; k0 += chunk)
-        HIPCHK(c, launch_table_vjp(n, d_r, d_gep, d_gsc, d_out, d_scr, k0, std::min(chunk, n - k0),
-                                   c->stream));
+    HIPCHK(c, launch_table_vjp(n, d_r, d_gep, d_gsc, d_out, d_scr, c->stream));
     HIPCHK(c, hipMemcpyAsync(grad_u, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     return sync_err(c);
 }
@@ -2010,17 +2010,11 @@ int bromoc_table_vjp_device_noalloc(bromoc_ctx *c, int32_t n, const double *r,
 int bromoc_table_vjp(int32_t n, const double *r, const double *grad_ep, const double *grad_sc,
                      double *grad_u) {
     if (n < 2 || !r || !grad_ep || !grad_sc || !grad_u) return BROMOC_E_ARG;
-    std::vector<double> u(n, 0.0), ep(3 * (size_t)n);
-    bromoc_pair p{};
-    for (int k = 0; k < n; k++) {
-        u[k] = 1.0;
-        const int st = bromoc_build_table(n, r, u.data(), 0, 0.0, ep.data(), &p);
-        u[k] = 0.0;
-        if (st) return st;
-        double acc = p.sc1 * grad_sc[0] + p.sc2 * grad_sc[1] + p.sc3 * grad_sc[2];
-        for (int m = 0; m < 3 * n; m++) acc += ep[m] * grad_ep[m];
-        grad_u[k] = acc;
-    }
+    if (r[0] == 0.0 || r[n - 1] == 0.0) return BROMOC_E_ARG;
+    for (int k = 1; k < n; k++)
+        if (!(r[k] > r[k - 1])) return BROMOC_E_ARG;
+    std::vector<double> scratch(6 * (size_t)n);
+    table_splinepot_vjp(n, r, grad_ep, grad_sc, grad_u, scratch.data());
     return BROMOC_OK;
 }
 

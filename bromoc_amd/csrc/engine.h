@@ -178,6 +178,7 @@ struct StepParams {
 
 // tpb values of tile mode: TPB_TILE + targets per lane (1, 2 or 4)
 constexpr int TPB_TILE = 16;
+constexpr int TILE_WAVES = 2;      // wavefronts of a k_tiles block (tile pairs sharing tile J)
 __host__ __device__ inline bool tile_mode(int tpb) { return tpb > TPB_TILE; }
 
 // pair-energy categories of the tile kernel (tener rows) and the ECat each one feeds
@@ -300,9 +301,10 @@ hipError_t launch_table_update(const TabJob &j, hipStream_t s);
 // `device` (current), taken once per device; and its size in doubles
 hipError_t vjp_scratch_init(int device, double **out);
 size_t vjp_scratch_doubles();
-// grad_u[k0 + t] = (d table / d u_k) . (grad_ep, grad_sc), one thread per knot
+// grad_u = (d table / d u)^T (grad_ep, grad_sc) by the reverse sweep (table_splinepot_vjp),
+// one thread; scratch: 6n doubles
 hipError_t launch_table_vjp(int n, const double *r, const double *gep, const double *gsc,
-                            double *out, double *scratch, int k0, int count, hipStream_t s);
+                            double *out, double *scratch, hipStream_t s);
 
 // ---- per-cycle analyses (analysis.hip)
 struct AnParams {

@@ -146,6 +146,144 @@ TAB_HD void table_splinepot(int n, const double *r, const double *u, double kf, 
     square_spline(n, r, u, ep, d1, d2, dd2, scratch, scratch + n, scratch + 2 * n);
 }
 
+// The transpose of table_splinepot's dependence on its samples: gu[k] = d/du_k of
+// ep . gq + (sc1, sc2, sc3) . gsc for a table of n >= 2 samples at r.  The table is affine
+// in u (the charge term kf only shifts it), so this is one reverse sweep through the
+// stages of squarespline -- fixdy, intlin, linipol, fixder, the averaging of the local
+// quadratics and the quadratics themselves -- and the end coefficients, O(n) in place of
+// n table builds.  Scratch: 6n doubles.
+TAB_HD void table_splinepot_vjp(int n, const double *x, const double *gq, const double *gsc,
+                                double *gu, double *scratch) {
+#pragma clang fp contract(off)
+    for (int k = 0; k < n; k++) gu[k] = 0.0;
+    double gd1 = 0.0, gd2 = 0.0, gdd2 = 0.0;
+    if (n == 2) {
+        // q[0] = q[3] = y0 - b x0, q[1] = q[4] = b, b = (y1 - y0) / (x1 - x0)
+        const double gb = gq[1] + gq[4] - (gq[0] + gq[3]) * x[0];
+        gu[0] += gq[0] + gq[3];
+        gu[1] += gb / (x[1] - x[0]);
+        gu[0] -= gb / (x[1] - x[0]);
+    } else {
+        const int m = n - 1;
+        double *gA = scratch, *gB = scratch + n, *gC = scratch + 2 * n;
+        double *gyp = scratch + 3 * n, *g0 = scratch + 4 * n, *g1 = scratch + 5 * n;
+        for (int k = 0; k < n; k++) {
+            gA[k] = gq[3 * k]; gB[k] = gq[3 * k + 1]; gC[k] = gq[3 * k + 2];
+        }
+        // fixdy: QA += cc, QB += b, QC -= b / 2 / (x_i + h i) (x_m at the last knot), with
+        // b = 2 (y_m - ynn) / (x_m - x_0), cc = -b x_0 / 2, ynn = U(x_m) after intlin
+        const double h = (x[1] - x[0]) / (double)(n - 1);
+        double gcc = 0.0, gb = 0.0;
+        for (int i = 0; i < m; i++) {
+            gcc += gA[i];
+            gb += gB[i];
+            gb += gC[i] * (-0.5 / (x[i] + h * (double)i));
+        }
+        gcc += gA[m];
+        gb += gB[m];
+        gb += gC[m] * (-0.5 / x[m]);
+        gb += gcc * (-x[0] * 0.5);
+        const double gy = gb * 2.0 / (x[m] - x[0]);
+        gu[m] += gy;
+        gA[m] -= gy;
+        gB[m] -= gy * x[m];
+        gC[m] -= gy * x[m] * x[m];
+        // intlin: QA(i) = QA(i-1) + x_i (QB(i-1) - QB(i) + x_i (QC(i-1) - QC(i))),
+        // QA(0) = y_0 - x_0 (QB(0) + QC(0) x_0), QB = s0, QC = s1 / 2
+        for (int i = m; i >= 1; i--) {
+            const double g = gA[i];
+            gA[i - 1] += g;
+            gB[i - 1] += g * x[i];
+            gB[i] -= g * x[i];
+            gC[i - 1] += g * x[i] * x[i];
+            gC[i] -= g * x[i] * x[i];
+        }
+        gu[0] += gA[0];
+        gB[0] -= gA[0] * x[0];
+        gC[0] -= gA[0] * x[0] * x[0];
+        // linipol: s1_i = (yp_{i+1} - yp_i) / dx_i, s0_i = (yp_i x_{i+1} - yp_{i+1} x_i) / dx_i;
+        // s1_m = dd2, s0_m = yp_m - s1_m x_m
+        for (int k = 0; k < n; k++) {
+            g0[k] = gB[k];
+            g1[k] = 0.5 * gC[k];
+            gyp[k] = 0.0;
+        }
+        gyp[m] += g0[m];
+        g1[m] -= g0[m] * x[m];
+        gdd2 += g1[m];
+        for (int i = 0; i < m; i++) {
+            const double tt = 1.0 / (x[i + 1] - x[i]);
+            gyp[i + 1] += g1[i] * tt;
+            gyp[i] -= g1[i] * tt;
+            gyp[i] += g0[i] * x[i + 1] * tt;
+            gyp[i + 1] -= g0[i] * x[i] * tt;
+        }
+        // fixder: yp from the averaged quadratics' QB + 2 QC x (only QB, QC reach here)
+        for (int k = 0; k < n; k++) gB[k] = gC[k] = 0.0;
+        gB[0] += gyp[0];
+        gC[0] += 2.0 * x[0] * gyp[0];
+        for (int i = 1; i < m; i++) {
+            const double g = 0.5 * gyp[i];
+            gB[i] += g;
+            gC[i] += 2.0 * x[i] * g;
+            gB[i - 1] += g;
+            gC[i - 1] += 2.0 * x[i] * g;
+        }
+        gB[m] += gyp[m];
+        gC[m] += 2.0 * x[m] * gyp[m];
+        // sqfit's averaging: Q(i) = (L(i) + L(i+1)) / 2 for 0 < i < m, Q(0) = L(0), Q(m) = L(m)
+        for (int k = 0; k < n; k++) g0[k] = g1[k] = 0.0;
+        g0[0] += gB[0];
+        g1[0] += gC[0];
+        for (int i = 1; i < m; i++) {
+            g0[i] += 0.5 * gB[i];
+            g0[i + 1] += 0.5 * gB[i];
+            g1[i] += 0.5 * gC[i];
+            g1[i + 1] += 0.5 * gC[i];
+        }
+        g0[m] += gB[m];
+        g1[m] += gC[m];
+        // the local quadratics (QB in g0, QC in g1): the first, with the start slope d1
+        {
+            const double tt = 1.0 / (x[1] - x[0]);
+            const double gc = g1[0] - 2.0 * x[0] * g0[0];
+            gd1 += g0[0];
+            gu[1] += gc * tt * tt;
+            gu[0] -= gc * tt * tt;
+            gd1 -= gc * tt;
+        }
+        for (int i = 1; i < m; i++) {
+            const double gc = g1[i] - (x[i + 1] + x[i]) * g0[i];
+            const double den = x[i + 1] - x[i - 1];
+            const double gt = g0[i] + gc / den, gp = -gc / den;
+            gu[i + 1] += gt / (x[i + 1] - x[i]);
+            gu[i] -= gt / (x[i + 1] - x[i]);
+            gu[i] += gp / (x[i] - x[i - 1]);
+            gu[i - 1] -= gp / (x[i] - x[i - 1]);
+        }
+        {   // the last, with the end slope d2
+            const double tt = 1.0 / (x[m] - x[m - 1]);
+            const double gc = g1[m] - 2.0 * x[m] * g0[m];
+            gd2 += g0[m] + gc * tt;
+            gu[m] -= gc * tt * tt;
+            gu[m - 1] += gc * tt * tt;
+        }
+    }
+    // the ends: sc3 = (r_n u_n - kf) r_n^5, d2 = -kf / r_n^2 - 6 sc3 / r_n^7,
+    // dd2 = 2 kf / r_n^3 + 42 sc3 / r_n^8, sc1 = (u_2 - u_1) p2 p1 / (p1 - p2),
+    // sc2 = (u_1 p1 - u_2 p2) / (p1 - p2), d1 = -12 sc1 / r_1^13
+    const double r1 = x[0], r2 = x[1], rn = x[n - 1];
+    const double g3 = gsc[2] + gd2 * (-6.0 / tab_ipow(rn, 7)) + gdd2 * (42.0 / tab_ipow(rn, 8));
+    gu[n - 1] += g3 * rn * tab_ipow(rn, 5);
+    const double p1 = tab_ipow(r1, 12), p2 = tab_ipow(r2, 12);
+    const double inv = 1.0 / (p1 - p2);
+    const double gs1 = gsc[0] + gd1 * (-12.0 / tab_ipow(r1, 13));
+    gu[1] += gs1 * p2 * p1 * inv;
+    gu[0] -= gs1 * p2 * p1 * inv;
+    gu[0] += gsc[1] * p1 * inv;
+    gu[1] -= gsc[1] * p2 * inv;
+}
+
 // The analytic pair parameters DISCRETIZE samples (extramod/grandmod per pair type).
 struct TabPair {
     int chr, lj, srpmf, pad;

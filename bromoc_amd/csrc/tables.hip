@@ -3,8 +3,7 @@
 // vector-Jacobian product of the table construction for the adjoint.  The construction
 // is the host's (tables.h): a table built here is bit-identical to bromoc_build_table's.
 // Each table is a sequential recurrence over its knots (squarespline's integration and
-// end correction), so a table is one thread; the Jacobian's columns are independent, one
-// thread per knot.
+// end correction), so a table is one thread, and so is the reverse sweep of its transpose.
 #include <mutex>
 #include <vector>
 
@@ -77,19 +76,11 @@ __global__ void k_tab_rows(TabJob j) {
 
 // knot k = k0 + thread: the table of the unit samples e_k dotted with the gradient, in
 // the order of the host's bromoc_table_vjp
+// the transposed table build (table_splinepot_vjp), one thread: the sweep is sequential
 __global__ void k_tab_vjp(int n, const double *r, const double *gep, const double *gsc,
-                          double *out, double *scratch, int k0, int count) {
-#pragma clang fp contract(off)
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int k = k0 + t;
-    double *u = scratch + 7 * (int64_t)n * t, *ep = u + n, *scr = ep + 3 * (int64_t)n;
-    for (int q = 0; q < n; q++) u[q] = q == k ? 1.0 : 0.0;
-    TabEnds e{};
-    table_splinepot(n, r, u, 0.0, ep, e, scr);
-    double acc = e.sc1 * gsc[0] + e.sc2 * gsc[1] + e.sc3 * gsc[2];
-    for (int m = 0; m < 3 * n; m++) acc += ep[m] * gep[m];
-    out[k] = acc;
+                          double *out, double *scratch) {
+    if (blockIdx.x * blockDim.x + threadIdx.x != 0) return;
+    table_splinepot_vjp(n, r, gep, gsc, out, scratch);
 }
 
 }  // namespace
@@ -102,9 +93,8 @@ hipError_t launch_table_update(const TabJob &j, hipStream_t s) {
 }
 
 hipError_t launch_table_vjp(int n, const double *r, const double *gep, const double *gsc,
-                            double *out, double *scratch, int k0, int count, hipStream_t s) {
-    hipLaunchKernelGGL(k_tab_vjp, dim3((count + 63) / 64), dim3(64), 0, s, n, r, gep, gsc, out,
-                       scratch, k0, count);
+                            double *out, double *scratch, hipStream_t s) {
+    hipLaunchKernelGGL(k_tab_vjp, dim3(1), dim3(64), 0, s, n, r, gep, gsc, out, scratch);
     return hipGetLastError();
 }
 

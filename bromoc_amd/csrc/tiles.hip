@@ -7,7 +7,8 @@
 // mode keeps the reference's once-per-pair work and stays deterministic:
 //
 //  * the particles in spatial order (spos) are cut into tiles of TT = 64 * TPL slots;
-//  * k_tiles: one wavefront per tile pair (I, J), I <= J.  Lane l holds the TPL targets
+//  * k_tiles: one wavefront per tile pair (I, J), I <= J, two per workgroup on the same
+//    column J, which share tile J's partner data in LDS.  Lane l holds the TPL targets
 //    l, l + 64, .. of tile I in registers; tile J's partners sit in LDS.  At step k every
 //    lane takes a different partner (a rotation, so no two lanes touch one partner at
 //    once), evaluates it against its TPL targets, adds the targets' shares in registers
@@ -106,49 +107,69 @@ __device__ __forceinline__ int wave_or8(int v) {
     return r;
 }
 
-// item b of the n (n + 1) / 2 tile pairs -> (I, J), I <= J
-__device__ __forceinline__ void item_tiles(int b, int &I, int &J) {
-    int j = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
-    while (j * (j + 1) / 2 > b) j--;
-    while ((j + 1) * (j + 2) / 2 <= b) j++;
-    J = j;
-    I = b - j * (j + 1) / 2;
+// Blocks of k_tiles: column J of the tile pairs (I, J), I <= J, in pairs of rows -- block
+// c of column J holds the items (2c, J) and (2c + 1, J), one per wavefront, so the two
+// wavefronts share tile J's partner data in LDS.  Column j has j / 2 + 1 blocks; the
+// columns before J have tile_blocks(J) = m (m + 1) for J = 2m and (m + 1)^2 for J = 2m + 1.
+__host__ __device__ __forceinline__ int tile_blocks(int J) {
+    const int m = J >> 1;
+    return (J & 1) ? (m + 1) * (m + 1) : m * (m + 1);
 }
+__device__ __forceinline__ void block_tiles(int b, int &J, int &c) {
+    int j = (int)(2.0 * sqrt((double)b));
+    while (j > 0 && tile_blocks(j) > b) j--;
+    while (tile_blocks(j + 1) <= b) j++;
+    J = j;
+    c = b - tile_blocks(j);
+}
+// LDS row stride of the type-pair tables: even, so that the rows (t, p) and (p, t), which a
+// wavefront reads at once, fall in different banks
+__host__ __device__ __forceinline__ int tail_stride(int nt1) { return nt1 + (nt1 & 1); }
 
 template <int FEAT, int TPL>
-__global__ __launch_bounds__(WAVE) void k_tiles(StepParams P) {
+__global__ __launch_bounds__(TILE_WAVES * WAVE) void k_tiles(StepParams P) {
     constexpr int TT = WAVE * TPL;
     constexpr bool RF = (FEAT & F_RFT) != 0;
     constexpr int NR = RF ? TT : 1;
+    // tile J's partner data, shared by the block's wavefronts
     __shared__ double s_x[TT], s_y[TT], s_z[TT];
     __shared__ int s_tf[TT], s_ix[RF ? TT : 1];
     __shared__ double s_sr[NR], s_rr[NR], s_q[NR];
-    __shared__ double s_fx[TT], s_fy[TT], s_fz[TT];
-    __shared__ double s_fa[NR], s_fb[NR];
-    extern __shared__ double4 s_tail[];   // (sc3, tail fct, dm2hi, flags) by type pair
+    // each wavefront's partner sums
+    __shared__ double s_afx[TILE_WAVES][TT], s_afy[TILE_WAVES][TT], s_afz[TILE_WAVES][TT];
+    __shared__ double s_afa[TILE_WAVES][NR], s_afb[TILE_WAVES][NR];
+    // by type pair (row stride tail_stride): (sc3, tail fct, dm2hi, flags); the near branch
+    // reads the rest of the pair row from global memory
+    extern __shared__ double4 s_tail[];
 
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
     const int b = blockIdx.x;
-    if (b >= P.titems) return;
-    int I, J;
-    item_tiles(b, I, J);
+    const int n = P.ntile, nt1 = P.nttyp + 1, S = tail_stride(nt1);
+    if (b >= tile_blocks(n)) return;
+    int J, cp;
+    block_tiles(b, J, cp);
+    const int I = TILE_WAVES * cp + wv;
+    const bool active = I <= J;
     const bool diag = I == J;
-    const int n = P.ntile, nt1 = P.nttyp + 1;
-    for (int k = lane; k < nt1 * nt1; k += WAVE) {
+    const int item = J * (J + 1) / 2 + I;   // the tile pair's index: (I, J) by columns
+    for (int k = tid; k < nt1 * nt1; k += TILE_WAVES * WAVE) {
         const DevPair &d = P.rows[k];
-        s_tail[k] = make_double4(d.sc3, d.fct, d.dm2hi, (double)d.flags);
+        const int l = (k / nt1) * S + k % nt1;
+        s_tail[l] = make_double4(d.sc3, d.fct, d.dm2hi, (double)d.flags);
     }
-    // partners: tile J into LDS, accumulators cleared
+    double *const s_fx = s_afx[wv], *const s_fy = s_afy[wv], *const s_fz = s_afz[wv];
+    double *const s_fa = s_afa[wv], *const s_fb = s_afb[wv];
+    // partners: tile J into LDS; this wavefront's sums cleared
+    for (int p = tid; p < TT; p += TILE_WAVES * WAVE) {
+        const SlotInfo o = slot_info<FEAT>(P, J * TT + p);
+        s_x[p] = o.x; s_y[p] = o.y; s_z[p] = o.z; s_tf[p] = o.tf;
+        if constexpr (RF) { s_ix[p] = o.idx; s_sr[p] = o.sr; s_rr[p] = o.rr; s_q[p] = o.q; }
+    }
 #pragma unroll
     for (int u = 0; u < TPL; u++) {
         const int p = lane + WAVE * u;
-        const SlotInfo o = slot_info<FEAT>(P, J * TT + p);
-        s_x[p] = o.x; s_y[p] = o.y; s_z[p] = o.z; s_tf[p] = o.tf;
         s_fx[p] = 0.0; s_fy[p] = 0.0; s_fz[p] = 0.0;
-        if constexpr (RF) {
-            s_ix[p] = o.idx; s_sr[p] = o.sr; s_rr[p] = o.rr; s_q[p] = o.q;
-            s_fa[p] = 0.0; s_fb[p] = 0.0;
-        }
+        if constexpr (RF) { s_fa[p] = 0.0; s_fb[p] = 0.0; }
     }
     // targets: tile I in registers
     double tx[TPL], ty[TPL], tz[TPL], tsr[TPL], trr[TPL], tq[TPL];
@@ -156,12 +177,14 @@ __global__ __launch_bounds__(WAVE) void k_tiles(StepParams P) {
     double ax_[TPL], ay_[TPL], az_[TPL], aa_[TPL], ab_[TPL];
 #pragma unroll
     for (int u = 0; u < TPL; u++) {
-        const SlotInfo o = slot_info<FEAT>(P, I * TT + lane + WAVE * u);
+        const SlotInfo o = slot_info<FEAT>(P, (active ? I : J) * TT + lane + WAVE * u);
         tx[u] = o.x; ty[u] = o.y; tz[u] = o.z; ttf[u] = o.tf; tix[u] = o.idx;
         tsr[u] = o.sr; trr[u] = o.rr; tq[u] = o.q;
         ax_[u] = ay_[u] = az_[u] = aa_[u] = ab_[u] = 0.0;
     }
     __syncthreads();
+    // the block's last barrier: a wavefront past the column's diagonal has no tile pair
+    if (!active) return;
 
     double en[NTE];
 #pragma unroll
@@ -190,17 +213,17 @@ __global__ __launch_bounds__(WAVE) void k_tiles(StepParams P) {
                 const double dx = tx[u] - px, dy = ty[u] - py, dz = tz[u] - pz;
                 double d2 = dx * dx + dy * dy + dz * dz;
                 d2 = skip ? 1e30 : d2;
-                const int row = t2t * nt1 + t2p;
-                const double4 tl = s_tail[row];
+                const int row = t2t * nt1 + t2p, lrow = t2t * S + t2p;
+                const double4 tl = s_tail[lrow];
                 const bool mx = ((tfl | pfl) & SL_SITE) != 0;
                 double y, de;
                 if constexpr ((FEAT & F_TAB) != 0) {
-                    de = pair_near_far(P.rows[row], tl, P.ep, P.ires, d2, y);
+                    de = pair_near_far<false>(P.rows[row], tl, P.ep, P.ires, d2, y);
                     if constexpr (EN) {
                         if (!skip) { if (mx) en[TE_EFPOTMX] += y; else en[TE_EFPOT] += y; }
                     }
                 } else if ((int)tl.w & PF_EFPOT) {
-                    de = pair_near_far(P.rows[row], tl, P.ep, P.ires, d2, y);
+                    de = pair_near_far<false>(P.rows[row], tl, P.ep, P.ires, d2, y);
                     if constexpr (EN) {
                         if (!skip) { if (mx) en[TE_EFPOTMX] += y; else en[TE_EFPOT] += y; }
                     }
@@ -290,8 +313,8 @@ __global__ __launch_bounds__(WAVE) void k_tiles(StepParams P) {
                 const int t2t = ttf[u] & 0xFFFF;
                 const double dx = tx[u] - px, dy = ty[u] - py, dz = tz[u] - pz;
                 const double d2 = dx * dx + dy * dy + dz * dz;
-                const int row = t2t * nt1 + t2p;
-                const double4 tl = s_tail[row];
+                const int row = t2t * nt1 + t2p, lrow = t2t * S + t2p;
+                const double4 tl = s_tail[lrow];
                 double y, de;
                 if constexpr ((FEAT & F_TAB) != 0) {
                     de = pair_near_far<true>(P.rows[row], tl, P.ep, P.ires, d2, y);
@@ -382,7 +405,10 @@ __global__ __launch_bounds__(WAVE) void k_tiles(StepParams P) {
         if (wr) sweep(std::true_type{});
         else sweep(std::false_type{});
     }
-    __syncthreads();
+    // the wavefront reads its own sums back: its LDS operations complete in order, and the
+    // fence keeps the compiler from moving the reads above the sweep's last updates
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
 
     const int NC = RF ? 5 : 3;
     double *const part = P.tpart;
@@ -408,7 +434,7 @@ __global__ __launch_bounds__(WAVE) void k_tiles(StepParams P) {
 #pragma unroll
         for (int c = 0; c < NTE; c++) {
             const double v = wave_sum(en[c]);
-            if (lane == 0) P.tener[(size_t)c * P.titems + b] = v;
+            if (lane == 0) P.tener[(size_t)c * P.titems + item] = v;
         }
     }
 }
@@ -617,11 +643,12 @@ __global__ __launch_bounds__(BLOCK) void k_tfin(StepParams P) {
 template <int FEAT>
 static void dispatch_tiles(const StepParams &p, hipStream_t s, int which) {
     const int nt1 = p.nttyp + 1;
-    const size_t lds = sizeof(double4) * nt1 * nt1;
+    const size_t lds = sizeof(double4) * nt1 * tail_stride(nt1);
+    const int nblk = tile_blocks(p.ntile);
     if (which & 1) switch (p.ttpl) {
-        case 1: hipLaunchKernelGGL((k_tiles<FEAT & (F_DNA | F_TAB | F_RFT), 1>), dim3(p.titems), dim3(WAVE), lds, s, p); break;
-        case 2: hipLaunchKernelGGL((k_tiles<FEAT & (F_DNA | F_TAB | F_RFT), 2>), dim3(p.titems), dim3(WAVE), lds, s, p); break;
-        default: hipLaunchKernelGGL((k_tiles<FEAT & (F_DNA | F_TAB | F_RFT), 4>), dim3(p.titems), dim3(WAVE), lds, s, p); break;
+        case 1: hipLaunchKernelGGL((k_tiles<FEAT & (F_DNA | F_TAB | F_RFT), 1>), dim3(nblk), dim3(TILE_WAVES * WAVE), lds, s, p); break;
+        case 2: hipLaunchKernelGGL((k_tiles<FEAT & (F_DNA | F_TAB | F_RFT), 2>), dim3(nblk), dim3(TILE_WAVES * WAVE), lds, s, p); break;
+        default: hipLaunchKernelGGL((k_tiles<FEAT & (F_DNA | F_TAB | F_RFT), 4>), dim3(nblk), dim3(TILE_WAVES * WAVE), lds, s, p); break;
     }
     if (which & 2)
         hipLaunchKernelGGL((k_tfin<FEAT & ~F_TAB>), dim3((p.ntot + BLOCK / 8 - 1) / (BLOCK / 8)),

@@ -23,7 +23,7 @@ type, bind(C) :: bgpu_pair
 end type
 
 type, bind(C) :: bgpu_grid
-  integer(c_int32_t) :: nclx, ncly, nclz, reserved
+  integer(c_int32_t) :: nclx, ncly, nclz, nmaps
   real(c_double) :: dcel, xbcen, ybcen, zbcen
   type(c_ptr) :: data
 end type
@@ -446,7 +446,7 @@ subroutine bromoc_gpu_init(seed)
       call bgpu_check(-1_c_int, 'phiv size (NMCDEN maps)')
     allocate(gv(size(phiv)))
     gv = phiv
-    s%phiv = bgpu_grid(nclx2, ncly2, nclz2, 0, dcel2, xbcen2, ybcen2, zbcen2, c_loc(gv))
+    s%phiv = bgpu_grid(nclx2, ncly2, nclz2, nmaps, dcel2, xbcen2, ybcen2, zbcen2, c_loc(gv))
   endif
   s%svdw = svdw; s%thold8 = thold8; s%vzmin = vzmin; s%vzmax = vzmax; s%scal = c_loc(scs)
   s%bdmax = bdmax; s%fact0n1 = fact0n1; s%fact0n2 = fact0n2

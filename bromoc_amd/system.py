@@ -165,6 +165,12 @@ class System:
         import copy
         return copy.deepcopy(self)
 
+    def nmcden_maps(self):
+        """Repulsion maps an NMCDEN phiv holds (readphi.f90:176-240): the six site maps
+        with CG-DNA, then one per ion type nold+1..ntype with ions."""
+        return ((6 if self.flags & abi.QNUCL else 0) +
+                (self.ntype - self.nold if self.flags & abi.QPAR else 0))
+
     # ------------------------------------------------------------ C struct
     def to_struct(self):
         """Returns (SystemStruct, keepalive) -- keep both alive while C reads them."""
@@ -230,8 +236,15 @@ class System:
             if g is None:
                 continue
             d = np.ascontiguousarray(g.data, dtype=dtype)
+            # the library checks the map count against the flags; a 4-D phiv must hold
+            # exactly the maps NMCDEN reads
+            assert d.ndim in ((3, 4) if gname == "phiv" else (3,)), (gname, d.shape)
+            if d.ndim == 4:
+                assert self.flags & abi.QNMCDEN and d.shape[0] == self.nmcden_maps(), (
+                    f"phiv of {d.shape[0]} maps; NMCDEN reads {self.nmcden_maps()}")
             keep.append(d)
             cg.nclx, cg.ncly, cg.nclz = d.shape[-3:]
+            cg.nmaps = (d.shape[0] if d.ndim == 4 else 1) if gname == "phiv" else 0
             cg.dcel, cg.xbcen, cg.ybcen, cg.zbcen = g.dcel, g.xbcen, g.ybcen, g.zbcen
             cg.data = d.ctypes.data
         s.svdw, s.thold8, s.vzmin, s.vzmax = self.svdw, self.thold8, self.vzmin, self.vzmax

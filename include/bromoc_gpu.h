@@ -135,7 +135,10 @@ typedef struct bromoc_pair {
 
 /* A regular grid as read by readphi.f90:43-49,160-170. */
 typedef struct bromoc_grid {
-    int32_t     nclx, ncly, nclz, reserved;
+    int32_t     nclx, ncly, nclz;
+    int32_t     nmaps;   /* phiv: maps in data -- with NMCDEN exactly 6 (with CG-DNA) plus one
+                          * per ion type nold+1..ntype (with PARTICLE), else 0 or 1 (one map);
+                          * phix and the reaction-field grids: 0 */
     double      dcel;
     double      xbcen, ybcen, zbcen;
     const void *data;    /* phix: float[ncl^3]; phiv: int8_t[ncl^3] per map */
@@ -514,17 +517,19 @@ int bromoc_tangent_rdf(bromoc_ctx *ctx, int32_t nsteps, int32_t checkpoint_every
 
 /* Chain rule through bromoc_build_table for one pair type's table of n samples at r: the
  * table (ep[3n], sc1..sc3) is affine in the sample values u (the charge term is the
- * offset), so grad_u[k] = sum of grad_ep . d ep / du_k + grad_sc . d(sc1, sc2, sc3) / du_k. */
+ * offset), so grad_u[k] = sum of grad_ep . d ep / du_k + grad_sc . d(sc1, sc2, sc3) / du_k,
+ * computed by one reverse sweep through the construction's stages (O(n)). */
 int bromoc_table_vjp(int32_t n, const double *r, const double *grad_ep, const double *grad_sc,
                      double *grad_u);
-/* The same product on the context's device, one thread per knot; bit-identical to
- * bromoc_table_vjp. */
+/* The same product on the context's device (the same reverse sweep, one thread);
+ * bit-identical to bromoc_table_vjp. */
 int bromoc_table_vjp_device(bromoc_ctx *ctx, int32_t n, const double *r, const double *grad_ep,
                             const double *grad_sc, double *grad_u);
 /* The same product, results bit-identical, with no allocation: no counterpart in the
  * reference; for callers that cannot allocate device memory (a fitting loop under HIP graph
- * capture, a serving loop).  Inputs, outputs and the per-knot workspace are staged in a
- * fixed 16 MiB device array per GPU, so n <= 174762 (BROMOC_E_CAPACITY beyond).  Every
+ * capture, a serving loop).  Inputs, outputs and the sweep's workspace (11n + 3 doubles)
+ * are staged in a fixed 16 MiB device array per GPU, so n <= 190649 (BROMOC_E_CAPACITY
+ * beyond).  Every
  * call on one GPU shares that array: calls that overlap on one GPU race. */
 int bromoc_table_vjp_device_noalloc(bromoc_ctx *ctx, int32_t n, const double *r,
                                     const double *grad_ep, const double *grad_sc,

@@ -491,9 +491,12 @@ EXPORT orc_ctx *orc_create(const bromoc_system *s) {
     }
     if (s->flags & BROMOC_QPHIV) {
         size_t n = (size_t)s->phiv.nclx * s->phiv.ncly * s->phiv.nclz;
-        if (s->flags & BROMOC_QNMCDEN)   /* readphi.f90:176-240 */
-            n *= (size_t)(((s->flags & BROMOC_QNUCL) ? 6 : 0) +
-                          ((s->flags & BROMOC_QPAR) ? s->ntype - s->nold : 0));
+        if (s->flags & BROMOC_QNMCDEN) {  /* readphi.f90:176-240 */
+            const int nm = ((s->flags & BROMOC_QNUCL) ? 6 : 0) +
+                           ((s->flags & BROMOC_QPAR) ? s->ntype - s->nold : 0);
+            if (s->phiv.nmaps != nm) { orc_destroy(c); return NULL; }   /* caller's buffer */
+            n *= (size_t)nm;
+        }
         c->phiv = (int8_t *)dupmem(s->phiv.data, n);
     }
     c->warn = (int32_t *)calloc(s->nttyp + 2, sizeof(int32_t));

@@ -100,6 +100,18 @@ def test_create_rejects_inconsistent_options():
     assert _create(s)[0] in (abi.OK, abi.E_NODEVICE)  # valid: only the device is missing
     s.flags |= abi.QSVDW                          # shell_simul.f90:2875
     assert _create(s)[0] == abi.E_ARG
+    # the map count crosses the ABI: a phiv of another map count than NMCDEN reads, or of
+    # several maps without NMCDEN, is refused before anything is read
+    lib = abi.load_library()
+    for nm, flip in ((-1, False), (1, False), (0, True)):
+        s = configs.with_nmcden(pore_small(seed=3, n_ions=4))
+        st, keep = s.to_struct()
+        assert st.phiv.nmaps == 6 + s.ntype - s.nold
+        st.phiv.nmaps += nm
+        if flip:
+            st.flags &= ~abi.QNMCDEN
+        ctx = C.c_void_p()
+        assert lib.bromoc_create(C.byref(st), 0, C.byref(ctx)) == abi.E_ARG, (nm, flip)
     s = _plain()
     s.flags |= abi.QDIFFUSE | abi.QPROFILE        # simul1.f90:338-340
     assert _create(s)[0] == abi.E_ARG

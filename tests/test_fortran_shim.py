@@ -5,6 +5,8 @@ routines: shows that the front-end hands the driver's module state (flags, pair 
 knot ranges, DNA lists, grids, GCMC buffers) to the C ABI and writes the results back
 where the driver reads them.  CPU only; skipped where oracle/_ref is not built.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -193,3 +195,33 @@ def test_repwalls_through_front_end():
         L.ref_repwalls(*m.shape, m.ctypes.data_as(C.POINTER(C.c_int8)), 126)
         outs.append(m)
     assert np.array_equal(outs[0], outs[1]) and outs[0].max() > 1
+
+
+FRONT_END = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                         "bromoc_amd", "fortran", "bromoc_gpu.f90")
+
+
+def test_front_end_entry_points_are_exported():
+    """Every C entry point the Fortran front-end binds (an interface block's
+    bind(C, name=...)) is exported by the HIP library."""
+    import re
+    src = open(FRONT_END).read()
+    iface = src[src.index("interface"):src.index("end interface")]
+    names = set(re.findall(r"bind\(C,\s*name='(\w+)'\)", iface))
+    assert {"bromoc_create", "bromoc_step", "bromoc_grand", "bromoc_interact"} <= names
+    lib = abi.load_library()
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference sources not present here")
+def test_front_end_links_against_the_library():
+    """The front-end and the reference objects it is linked with (a driver's build) link
+    against libbromoc_gpu.so with no undefined symbol; the linked library loads."""
+    import ctypes
+    import subprocess
+    oracle_dir = os.path.join(os.path.dirname(FRONT_END), "..", "..", "oracle")
+    r = subprocess.run(["make", "-s", "-C", oracle_dir, "linkcheck"], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ctypes.CDLL(os.path.join(oracle_dir, "_ref", "libbromoc_linkcheck.so"))

@@ -44,7 +44,7 @@ def test_noalloc_matches_device_vjp_twice():
 def test_noalloc_rejects_beyond_its_workspace():
     lib = abi.load_library()
     e = Engine(configs.plain_ions(seed=1, n_per_type=4))
-    n = 174763
+    n = 190650
     r = 1.0 + 0.001 * np.arange(n)
     rc, _ = _call(lib, e, "bromoc_table_vjp_device_noalloc", r, np.zeros(3 * n), np.zeros(3))
     assert rc == abi.E_CAPACITY
