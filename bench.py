@@ -34,7 +34,7 @@ FP64_PEAK_TFLOPS = 78.6
 # scaled by what the same counters read for a 256 MiB device copy (bench.py --copy-probe
 # 256): FETCH_SIZE x 2.0, WRITE_SIZE x 1.0.  The counters cannot be read from inside this
 # process; a shape without a measurement reports null.
-TRAFFIC_KIB = {}
+TRAFFIC_KIB = {("translocation", 18): (448291.2, 960598.1)}
 TRAFFIC_SCALE = (2.0, 1.0)
 # FP64 operations per pair evaluation: a table pair ~34 (r^2 8, sqrt 1, index 2, U 4, F/r 3,
 # energy 1, force update on both particles ~15); the reaction-field cross term of an ion

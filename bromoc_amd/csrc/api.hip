@@ -85,7 +85,7 @@ struct bromoc_ctx {
     DevPair *pairs = nullptr, *rows = nullptr;
     double4 *ep = nullptr;
     double *cg = nullptr, *fact1a = nullptr, *fact2a = nullptr, *scal = nullptr, *zcont = nullptr;
-    int32_t *inc_ptr = nullptr;
+    int32_t *inc_ptr = nullptr, *inc_split = nullptr;
     uint32_t *inc = nullptr;
     double2 *incr = nullptr;
     int32_t *lst[8] = {};
@@ -118,8 +118,12 @@ struct bromoc_ctx {
     struct GcDev {
         int nb = -1;
         double *avnum = nullptr, *mu = nullptr, *kb = nullptr, *lzmin = nullptr, *lzmax = nullptr;
-        int32_t *ibfftyp = nullptr, *ntotat = nullptr, *nat = nullptr, *prob = nullptr;
+        int32_t *ibfftyp = nullptr, *ntotat = nullptr, *nat = nullptr;
+        int32_t *natlog = nullptr;        // [lcap] buffer counts after each attempt
+        int64_t lcap = 0;
+        std::vector<int32_t> h_natlog;
         int32_t *ibuffer = nullptr;
+        bool ib_current = false;          // ibuffer equals the host's copy
         int8_t *bias = nullptr;
         GcState *st = nullptr;
         double *part = nullptr, *u = nullptr;
@@ -520,6 +524,7 @@ int bromoc_destroy(bromoc_ctx *c) {
     for (void *p : c->allocs) (void)hipFree(p);
     for (void *p : c->gd.mem) (void)hipFree(p);
     if (c->gd.u) (void)hipFree(c->gd.u);
+    if (c->gd.natlog) (void)hipFree(c->gd.natlog);
     for (void *p : {(void *)c->tpart, (void *)c->tener, (void *)c->an_rho, (void *)c->an_rdna, (void *)c->an_gr, (void *)c->an_prob2,
                     (void *)c->an_prob3})
         if (p) (void)hipFree(p);
@@ -618,7 +623,7 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     ACHK(dalloc(c, &c->fz, cap));
     ACHK(dalloc(c, &c->fdna, std::max(ns, 1)));
     ACHK(dalloc(c, &c->epart, (size_t)NE * cap));
-    ACHK(dalloc(c, &c->ered, NE));
+    ACHK(dalloc(c, &c->ered, (size_t)NE * (1 + RED_BLOCKS)));
     ACHK(dupload(c, &c->pairidx, s->pairidx, (size_t)nt1 * nt1));
     c->h_pairs.assign(s->pairs, s->pairs + s->npair + 1);
     c->h_pairidx.assign(s->pairidx, s->pairidx + (size_t)nt1 * nt1);
@@ -667,10 +672,17 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     for (int i = 0; i < ns; i++) deg[i + 1] += deg[i];
     std::vector<uint32_t> inc(std::max<int64_t>(deg[ns], 1));
     std::vector<double2> incr(inc.size(), make_double2(0.0, 0.0));
+    // each site's angle and dihedral records first, then its two-site records (bond,
+    // stacking, base pair, excluded volume, P-P, solvent) from split[i] on
+    std::vector<int32_t> split(std::max(ns, 1), 0);
     {
         std::vector<int32_t> fill(deg.begin(), deg.end());
-        for (int k = 0; k < 8 && dna; k++)
-            for (int r = 0; r < ar[k]; r++)
+        for (int i = 0; i < ns; i++) split[i] = deg[i];
+        for (int k = T_ANGLE; k <= T_DIHE && dna; k++)
+            for (int64_t q = 0; q < (int64_t)cnt[k] * ar[k]; q++) split[lp[k][q]]++;
+        static const int order[8] = {T_ANGLE, T_DIHE, T_BOND, T_STACK, T_BP, T_EX, T_QQ, T_SLV};
+        for (int ko = 0; ko < 8 && dna; ko++)
+            for (int r = 0, k = order[ko]; r < ar[k]; r++)
                 for (int t = 0; t < cnt[k]; t++) {
                     int site = lp[k][(int64_t)r * cnt[k] + t];
                     const int at = fill[site]++;
@@ -693,6 +705,7 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
                 }
     }
     ACHK(dupload(c, &c->inc_ptr, deg.data(), deg.size()));
+    ACHK(dupload(c, &c->inc_split, split.data(), split.size()));
     ACHK(dupload(c, &c->inc, inc.data(), inc.size()));
     ACHK(dupload(c, &c->incr, incr.data(), incr.size()));
     if (s->flags & BROMOC_QPHIX)
@@ -806,7 +819,7 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.ires = s->ires; p.celec = celec_value();
     p.cg = c->cg; p.fact1a = c->fact1a; p.fact2a = c->fact2a; p.nwtype = c->nwtype;
     p.rth = s->rth; p.srpx = s->srpx; p.srpk = s->srpk; p.srpy = s->srpy;
-    p.inc_ptr = c->inc_ptr; p.inc = c->inc; p.incr = c->incr;
+    p.inc_ptr = c->inc_ptr; p.inc_split = c->inc_split; p.inc = c->inc; p.incr = c->incr;
     for (int k = 0; k < 8; k++) { p.lst[k] = c->lst[k]; p.nlist[k] = dna ? cnt[k] : 0; }
     for (int k = 0; k < 6; k++) p.lval[k] = c->lval[k];
     p.epsnuc = s->epsnuc; p.scalepairing = s->scalepairing; p.fctn = s->fctn;
@@ -876,6 +889,7 @@ int bromoc_set_positions(bromoc_ctx *c, int32_t ntot, const double *x, const dou
     if (!c || !x || !y || !z) return BROMOC_E_ARG;
     if (ntot < c->ns + c->nf || ntot > c->cap) return fail(c, BROMOC_E_ARG, "ntot out of range");
     (void)hipSetDevice(c->device);
+    c->gd.ib_current = false;   // the particle count may change: upload ibuffer again
     std::vector<int32_t> t2(ntot);
     if (typei) {
         for (int i = c->ns; i < ntot; i++) {
@@ -1469,6 +1483,7 @@ int bromoc_set_gcmc(bromoc_ctx *c, const bromoc_gcmc *g) {
     c->lzmax.assign(g->lzmax, g->lzmax + nb + 1);
     c->bufferbias.assign(g->bufferbias, g->bufferbias + nb + 1);
     c->ibuffer.assign(c->cap + 1, 0);
+    c->gd.ib_current = false;
     // device copies for the device GRAND (arrays of a previous call are freed here)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
@@ -1492,7 +1507,6 @@ int bromoc_set_gcmc(bromoc_ctx *c, const bromoc_gcmc *g) {
     HIPCHK(c, dev(&c->gd.ntotat, n1, nb > 0 ? g->ntotat : nullptr));
     HIPCHK(c, dev(&c->gd.bias, n1, nb > 0 ? g->bufferbias : nullptr));
     HIPCHK(c, dev(&c->gd.nat, n1, nullptr));
-    HIPCHK(c, dev(&c->gd.prob, (size_t)std::max(nb, 1) * (c->cap + 1), nullptr));
     HIPCHK(c, dev(&c->gd.ibuffer, (size_t)c->cap + 1, nullptr));
     HIPCHK(c, dev(&c->gd.st, 1, nullptr));
     HIPCHK(c, dev(&c->gd.part, GC_BLOCKS, nullptr));
@@ -1506,6 +1520,7 @@ int bromoc_set_ibuffer(bromoc_ctx *c, const int32_t *ib) {
     for (int i = 0; i < c->ntot; i++)
         if (ib[i] < 0 || ib[i] > c->g.nbuffer) return fail(c, BROMOC_E_ARG, "ibuffer out of range");
     std::copy(ib, ib + c->ntot, c->ibuffer.begin());
+    c->gd.ib_current = false;
     return BROMOC_OK;
 }
 
@@ -1531,12 +1546,20 @@ static int grand_device(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t n
         HIPCHK(c, hipMalloc(&D.u, sizeof(double) * nu));
         D.ucap = nu;
     }
+    const int64_t nlog = (int64_t)std::max(ngcmc, 1) * std::max(nb, 1);
+    if (nlog > D.lcap) {
+        if (D.natlog) (void)hipFree(D.natlog);
+        D.natlog = nullptr;
+        D.lcap = 0;
+        HIPCHK(c, hipMalloc(&D.natlog, sizeof(int32_t) * nlog));
+        D.lcap = nlog;
+    }
     const int n0 = c->ntot;
     if (nu > 0) HIPCHK(c, hipMemcpyAsync(D.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.nat, nat, sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
-    if (nb > 0)
-        HIPCHK(c, hipMemcpyAsync(D.prob, prob, sizeof(int32_t) * (size_t)nb * pstride, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(D.ibuffer, c->ibuffer.data(), sizeof(int32_t) * n0, hipMemcpyHostToDevice, c->stream));
+    // the device's ibuffer is the host's unless the host changed it since the last call
+    if (!D.ib_current)
+        HIPCHK(c, hipMemcpyAsync(D.ibuffer, c->ibuffer.data(), sizeof(int32_t) * n0, hipMemcpyHostToDevice, c->stream));
     GcState h{};
     h.k = 0; h.ntot = n0; h.err = GCE_OK; h.ener = *ener;
     HIPCHK(c, hipMemcpyAsync(D.st, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
@@ -1545,26 +1568,33 @@ static int grand_device(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t n
     G.pos0 = c->pos[0]; G.pos1 = c->pos[1];
     G.typei = c->typei; G.t2 = c->t2; G.ibuffer = D.ibuffer;
     G.u = D.u; G.nu = nu;
-    G.nbuffer = nb; G.pstride = pstride;
-    G.nat = D.nat; G.prob = D.prob;
+    G.nbuffer = nb;
+    G.nat = D.nat; G.natlog = D.natlog;
     G.avnum = D.avnum; G.mu = D.mu; G.kb = D.kb; G.lzmin = D.lzmin; G.lzmax = D.lzmax;
     G.ibfftyp = D.ibfftyp; G.ntotat = D.ntotat; G.bias = D.bias;
     G.lx = g.lx; G.ly = g.ly; G.cx = g.cx; G.cy = g.cy; G.cz = g.cz; G.ikbt = g.ikbt;
     G.icycle = icycle;
     G.cap = c->cap;
     G.st = D.st; G.part = D.part;
-    for (int ig = 0; ig < ngcmc; ig++) HIPCHK(c, launch_grand_attempt(G, c->stream));
+    HIPCHK(c, launch_grand(G, ngcmc, c->stream));
     HIPCHK(c, hipMemcpyAsync(&h, D.st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     int st = sync_err(c);
     if (st) return st;
     const int n1 = h.ntot;
     HIPCHK(c, hipMemcpyAsync(nat, D.nat, sizeof(int32_t) * (nb + 1), hipMemcpyDeviceToHost, c->stream));
-    if (nb > 0)
-        HIPCHK(c, hipMemcpyAsync(prob, D.prob, sizeof(int32_t) * (size_t)nb * pstride, hipMemcpyDeviceToHost, c->stream));
+    D.h_natlog.resize((size_t)nlog);
+    if (nb > 0 && h.natt > 0)
+        HIPCHK(c, hipMemcpyAsync(D.h_natlog.data(), D.natlog, sizeof(int32_t) * (size_t)h.natt * nb,
+                                 hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->ibuffer.data(), D.ibuffer, sizeof(int32_t) * n1, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->h_typei.data(), c->typei, sizeof(int32_t) * n1, hipMemcpyDeviceToHost, c->stream));
     st = sync_err(c);
     if (st) return st;
+    D.ib_current = true;
+    // prob(b, count) += 1 for every logged count, attempt by attempt
+    for (int a = 0; a < h.natt; a++)
+        for (int b = 1; b <= nb; b++)
+            prob[(size_t)(b - 1) * pstride + D.h_natlog[(size_t)a * nb + (b - 1)]] += 1;
     if (h.nacc > 0) c->sort_dirty = true;
     c->ntot = n1;
     *ener = h.ener;
@@ -1591,6 +1621,7 @@ int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int6
     }();
     if (!c->base.mmon && !host_grand)
         return grand_device(c, ngcmc, u, nu, used, nat, prob, icycle, ener);
+    c->gd.ib_current = false;   // the host loop below edits ibuffer
     const bromoc_gcmc &g = c->g;
     const int nbuffer = g.nbuffer;
     const double nbufferr = (double)nbuffer;

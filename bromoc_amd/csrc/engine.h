@@ -11,6 +11,7 @@ namespace bromoc {
 
 // Number of per-particle energy slots: the bromoc_energies fields after `ener`.
 constexpr int NE = 22;
+constexpr int RED_BLOCKS = 32;   // blocks per energy term in the reduction
 enum ECat {
     E_ELEC = 0, E_VDW, E_SRPMF, E_SRPMFMX, E_EFPOT, E_EFPOTMX, E_QQMX, E_VDWMX,
     E_BOND, E_ANG, E_DIHE, E_STACK, E_BP, E_EX, E_QQ, E_SOLV,
@@ -95,6 +96,7 @@ struct StepParams {
     double rth, srpx, srpk, srpy;
     // CG-DNA
     const int32_t *inc_ptr;  // [nsites+1]
+    const int32_t *inc_split;  // [nsites] first two-site record of each site (angles, dihedrals before)
     const uint32_t *inc;
     // the same incidences resolved: bits of x = word | other site << 32 | (base pair of a
     // C/G site a) << 63 (other site 0 for angles and dihedrals), y = the term's list value
@@ -221,7 +223,8 @@ hipError_t launch_mmij(const StepParams &p, hipStream_t s);
 hipError_t launch_mmij_interact(const StepParams &p, double xj, double yj, double zj, int jtype,
                                 int j, double *out, hipStream_t s);
 // out[cat] = sum over particles of epart[cat], plus for the pair categories the sum over
-// the tile pairs of tener (tile mode), both in a fixed order
+// the tile pairs of tener (tile mode), both in a fixed order; out holds NE * (1 +
+// RED_BLOCKS) doubles (the range sums follow the NE results)
 hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
                                 hipStream_t s, const double *tener = nullptr, int titems = 0);
 // tile mode (tiles.hip): k_tiles over every tile pair (which & 1), then k_tfin over every
@@ -256,6 +259,7 @@ struct GcState {
     int32_t ntot, err;
     int32_t mode, ib, ip, itype, j;   // trial: mode 0 none, 1 insertion, 2 deletion
     int32_t nacc;            // accepted insertions and deletions
+    int32_t natt, pad;       // attempts finished (rows of GcParams::natlog written)
     double x, y, z, ener;
 };
 
@@ -265,8 +269,10 @@ struct GcParams {
     int32_t *typei, *t2, *ibuffer;
     const double *u;
     int64_t nu;
-    int32_t nbuffer, pstride;            // prob rows: [nbuffer][pstride]
-    int32_t *nat, *prob;
+    int32_t nbuffer, pad0;
+    int32_t *nat;
+    int32_t *natlog;         // [attempts][nbuffer]: each buffer's count after each attempt
+                             // (the host adds them to the caller's prob histogram)
     const double *avnum, *mu, *kb, *lzmin, *lzmax;   // [nbuffer+1]
     const int32_t *ibfftyp, *ntotat;
     const int8_t *bias;
@@ -278,7 +284,8 @@ struct GcParams {
 };
 
 // one attempt: k_gc_pre, (k_gc_rfpar), k_gc_interact, k_gc_post
-hipError_t launch_grand_attempt(const GcParams &g, hipStream_t s);
+// a GRAND call of ngcmc attempts (grand.hip)
+hipError_t launch_grand(const GcParams &g, int ngcmc, hipStream_t s);
 
 // ---- EFPOT tables on the device (tables.hip)
 struct TabEnds;

@@ -2,19 +2,21 @@
 // insert.f90:29-41, and FIND, find.f90:26-32).
 //
 // The attempts of a GRAND call are sequential -- each one's acceptance changes the
-// particle set the next one sees -- but each attempt's INTERACT is an O(N) sum.  Per attempt
-// three (four with reaction-field grids) launches run back to back on the stream, with no
-// host round trip:
-//   k_gc_pre       one workgroup: draws the trial from the uniform stream in the
-//                  reference's order, and for a deletion finds the ip-th ion of the buffer
-//                  (ballot prefix over the slots);
-//   k_gc_rfpar     (reaction field only) srfe, reff of every ion at the current positions;
+// particle set the next one sees -- but each attempt's INTERACT is an O(N) sum.  A call
+// launches, back to back on the stream with no host round trip:
+//   k_gc_rfpar     (reaction field only, once per call) srfe, reff of every ion at the
+//                  current positions; an accepted attempt updates the one slot it changes;
+//   k_gc_pre       one workgroup: the first trial, drawn from the uniform stream in the
+//                  reference's order; for a deletion FIND, the ip-th ion of the buffer
+//                  (per-thread segment counts, a block scan, one segment searched);
+// then per attempt
 //   k_gc_interact  G workgroups: INTERACT of the trial ion (interact.f90:19-419 for an ion:
 //                  MEMBRANE, grids, ion and site pairs, RFPARIONJ), one partial sum per
 //                  workgroup;
 //   k_gc_post      one workgroup: the partials summed in a fixed order, the acceptance
 //                  test (grand.f90:50-59, 88-97), insertion at the end or deletion by
-//                  swap-with-last (grand.f90:60-76, 98-110), and the prob histogram.
+//                  swap-with-last (grand.f90:60-76, 98-110), the prob histogram, and then
+//                  the next attempt's trial (k_gc_pre's work) in the same launch.
 // The state (uniform cursor, particle count, energy, error) lives in device memory; the
 // host reads it back once per call.  A failed attempt (uniform stream exhausted, capacity,
 // inconsistent buffers) sets the error and every later kernel of the call does nothing.
@@ -22,7 +24,10 @@
 
 namespace bromoc {
 
-__global__ __launch_bounds__(BLOCK) void k_gc_pre(GcParams G) {
+// The next trial (grand.f90:42-50, 77-84) and, for a deletion, FIND (find.f90:26-32): the
+// ip-th slot, in slot order, of the moving ions whose buffer is ib.  Called by a whole
+// workgroup.
+__device__ inline void gc_pre(const GcParams &G) {
     // the trial position and acceptance arithmetic as written, unfused: a fused multiply-add
     // rounds differently from the reference's separate operations
 #pragma clang fp contract(off)
@@ -32,7 +37,6 @@ __global__ __launch_bounds__(BLOCK) void k_gc_pre(GcParams G) {
     if (S.err) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
     if (tid == 0) {
-        // the trial, in the reference's draw order (grand.f90:42-50, 77-84)
         int64_t k = S.k;
         int mode = 0, ib = 0, ip = 0, itype = 0;
         bool ok = true;
@@ -75,24 +79,28 @@ __global__ __launch_bounds__(BLOCK) void k_gc_pre(GcParams G) {
     }
     __syncthreads();
     if (S.mode != 2) return;
-    // FIND: the ip-th slot (in slot order) of the moving ions whose buffer is ib
+    // thread t counts the buffer's ions in its contiguous segment of the moving slots; a
+    // block scan of the counts names the segment holding the ip-th, which its thread scans
     const int ib = S.ib, ip = S.ip, ntot = S.ntot;
-    const int first = G.sp.nsites + G.sp.nfix;
-    int seen = 0;
-    for (int c0 = first; c0 < ntot && seen < ip; c0 += BLOCK) {
-        const int a = c0 + tid;
-        const bool hit = a < ntot && G.ibuffer[a] == ib;
-        const uint64_t m = __ballot(hit);
-        if (lane == 0) s_cnt[wv] = __popcll(m);
-        __syncthreads();
-        int before = seen;
-        for (int q = 0; q < wv; q++) before += s_cnt[q];
-        const int rank = before + __popcll(m & ((1ull << lane) - 1ull)) + 1;
-        if (hit && rank == ip) s_found = a;
-        int tot = 0;
-        for (int q = 0; q < WPB; q++) tot += s_cnt[q];
-        __syncthreads();
-        seen += tot;
+    const int first = G.sp.nsites + G.sp.nfix, cnt = ntot > first ? ntot - first : 0;
+    const int seg = (cnt + BLOCK - 1) / BLOCK;
+    const int lo = first + min(tid * seg, cnt), hi = first + min((tid + 1) * seg, cnt);
+    int c = 0;
+    for (int a = lo; a < hi; a++) c += G.ibuffer[a] == ib ? 1 : 0;
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const int v = __shfl_up(incl, off, WAVE);
+        if (lane >= off) incl += v;
+    }
+    if (lane == WAVE - 1) s_cnt[wv] = incl;
+    __syncthreads();
+    int before = incl - c;
+    for (int q = 0; q < wv; q++) before += s_cnt[q];
+    if (c > 0 && ip > before && ip <= before + c) {
+        int need = ip - before;
+        for (int a = lo; a < hi; a++)
+            if (G.ibuffer[a] == ib && --need == 0) { s_found = a; break; }
     }
     __syncthreads();
     if (tid == 0) {
@@ -108,11 +116,13 @@ __global__ __launch_bounds__(BLOCK) void k_gc_pre(GcParams G) {
     }
 }
 
+__global__ __launch_bounds__(BLOCK) void k_gc_pre(GcParams G) { gc_pre(G); }
+
 // srfe, reff (inside, moving) of every ion at the current positions, as k_rfpar without
 // gradients, over the device particle count
 __global__ void k_gc_rfpar(GcParams G) {
     const GcState &S = *G.st;
-    if (S.err || S.mode == 0) return;
+    if (S.err) return;
     const StepParams &P = G.sp;
     const int i = P.nsites + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S.ntot) return;
@@ -209,7 +219,8 @@ __global__ __launch_bounds__(BLOCK) void k_gc_interact(GcParams G) {
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_gc_post(GcParams G) {
+// The attempt's acceptance (grand.f90:50-116).  Called by a whole workgroup.
+__device__ inline void gc_post(const GcParams &G) {
 #pragma clang fp contract(off)
     __shared__ double s[BLOCK];
     GcState &S = *G.st;
@@ -262,6 +273,11 @@ __global__ __launch_bounds__(BLOCK) void k_gc_post(GcParams G) {
                 G.typei[n] = ti;
                 G.t2[n] = t2;
                 G.ibuffer[n] = ib;
+                if (G.sp.rfon) {   // the new ion's srfe, reff, as k_gc_rfpar computes them
+                    double sr, rr, ds[3], dr[3];
+                    const bool in = rf_params(G.sp, S.x, S.y, S.z, itype, false, sr, rr, ds, dr);
+                    G.sp.rfa[n] = make_double4(sr, rr, in ? 1.0 : 0.0, 1.0);
+                }
                 S.ntot = n + 1;
                 S.nacc++;
             }
@@ -285,28 +301,42 @@ __global__ __launch_bounds__(BLOCK) void k_gc_post(GcParams G) {
                 G.typei[iat] = G.typei[last];
                 G.t2[iat] = G.t2[last];
                 G.ibuffer[iat] = G.ibuffer[last];
+                if (G.sp.rfon) G.sp.rfa[iat] = G.sp.rfa[last];
                 G.nat[ib] = G.nat[ib] - 1;
                 S.ntot = last;
                 S.nacc++;
             }
         }
     }
-    // prob histogram (grand.f90:114-116), after every attempt
+    // prob histogram (grand.f90:114-116), after every attempt: each buffer's count is
+    // logged, and the host adds one to prob(b, count) for each entry
+    const int a = S.natt;
     for (int b = 1; b <= G.nbuffer; b++) {
         const int nb = G.nat[b];
         if (nb < 0 || nb > G.cap) { S.err = GCE_STATE; return; }
-        G.prob[(size_t)(b - 1) * G.pstride + nb] += 1;
+        G.natlog[(size_t)a * G.nbuffer + (b - 1)] = nb;
     }
+    S.natt = a + 1;
 }
 
-hipError_t launch_grand_attempt(const GcParams &g, hipStream_t s) {
-    hipLaunchKernelGGL(k_gc_pre, dim3(1), dim3(BLOCK), 0, s, g);
+// one attempt's acceptance, then (next) the following attempt's trial
+__global__ __launch_bounds__(BLOCK) void k_gc_post(GcParams G, int next) {
+    gc_post(G);
+    __syncthreads();
+    if (next) gc_pre(G);
+}
+
+hipError_t launch_grand(const GcParams &g, int ngcmc, hipStream_t s) {
+    if (ngcmc <= 0) return hipSuccess;
     if (g.sp.rfon) {
         const int nion = g.cap - g.sp.nsites;
         if (nion > 0) hipLaunchKernelGGL(k_gc_rfpar, dim3((nion + 255) / 256), dim3(256), 0, s, g);
     }
-    hipLaunchKernelGGL(k_gc_interact, dim3(GC_BLOCKS), dim3(BLOCK), 0, s, g);
-    hipLaunchKernelGGL(k_gc_post, dim3(1), dim3(BLOCK), 0, s, g);
+    hipLaunchKernelGGL(k_gc_pre, dim3(1), dim3(BLOCK), 0, s, g);
+    for (int a = 0; a < ngcmc; a++) {
+        hipLaunchKernelGGL(k_gc_interact, dim3(GC_BLOCKS), dim3(BLOCK), 0, s, g);
+        hipLaunchKernelGGL(k_gc_post, dim3(1), dim3(BLOCK), 0, s, g, a + 1 < ngcmc ? 1 : 0);
+    }
     return hipGetLastError();
 }
 

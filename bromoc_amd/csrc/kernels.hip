@@ -517,27 +517,43 @@ __global__ __launch_bounds__(NW * WAVE) void k_dna(StepParams P) {
     double e[ND];
 #pragma unroll
     for (int k = 0; k < ND; k++) e[k] = 0.0;
-    const int k0 = P.inc_ptr[i], k1 = P.inc_ptr[i + 1];
+    // the site's angle and dihedral records come first, its two-site records from ks on
+    const int k0 = P.inc_ptr[i], ks = P.inc_split[i], k1 = P.inc_ptr[i + 1];
     const V3 rme = ld3(P, i);
-    for (int k = k0 + tid; k < k1; k += NB) {
-        // resolved incidence: one load gives the term, the other site and the list value
-        const double2 rec = P.incr[k];
-        const uint64_t bits = (uint64_t)__double_as_longlong(rec.x);
-        const uint32_t wd = (uint32_t)bits;
-        const int kind = wd >> 29;
-        const bool on = kind <= T_DIHE ? (P.flags & QNOBOND) != 0
-                                       : ((P.flags & QNONBOND) != 0 &&
-                                          (kind != T_SLV || (P.flags & QSOLV) != 0));
-        if (!on) continue;
-        if (kind == T_ANGLE || kind == T_DIHE) {
-            dna_term(P, wd, free_me, f, e);
-            continue;
+    if (P.flags & QNOBOND) {
+        for (int k = k0 + tid; k < ks; k += NB)
+            dna_term(P, (uint32_t)__double_as_longlong(P.incr[k].x), free_me, f, e);
+    }
+    // two-site terms, DU records per thread at a time: the records, then the other sites'
+    // positions, then the terms, so DU loads of each kind are in flight together
+    constexpr int DU = 4;
+    for (int kb = ks + tid; kb < k1; kb += DU * NB) {
+        double2 rec[DU];
+        V3 ro[DU];
+#pragma unroll
+        for (int u = 0; u < DU; u++) {
+            // resolved incidence: one load gives the term, the other site and the list value
+            const int k = kb + u * NB;
+            rec[u] = P.incr[k < k1 ? k : ks];
         }
-        const int role = (wd >> 27) & 3;
-        const V3 ro = ld3(P, (int)((bits >> 32) & 0x7FFFFFFFu));
-        const V3 v1 = role == 0 ? sub3(ro, rme) : sub3(rme, ro);   // r_b - r_a
-        const bool deby = kind == T_QQ && qq_debye(P, rme, ro);
-        dna_pair(P, kind, role, v1, rec.y, (bits >> 63) != 0, deby, free_me, f, e);
+#pragma unroll
+        for (int u = 0; u < DU; u++)
+            ro[u] = ld3(P, (int)(((uint64_t)__double_as_longlong(rec[u].x) >> 32) & 0x7FFFFFFFu));
+#pragma unroll
+        for (int u = 0; u < DU; u++) {
+            if (kb + u * NB >= k1) break;
+            const uint64_t bits = (uint64_t)__double_as_longlong(rec[u].x);
+            const uint32_t wd = (uint32_t)bits;
+            const int kind = wd >> 29;
+            const bool on = kind == T_BOND ? (P.flags & QNOBOND) != 0
+                                           : ((P.flags & QNONBOND) != 0 &&
+                                              (kind != T_SLV || (P.flags & QSOLV) != 0));
+            if (!on) continue;
+            const int role = (wd >> 27) & 3;
+            const V3 v1 = role == 0 ? sub3(ro[u], rme) : sub3(rme, ro[u]);   // r_b - r_a
+            const bool deby = kind == T_QQ && qq_debye(P, rme, ro[u]);
+            dna_pair(P, kind, role, v1, rec[u].y, (bits >> 63) != 0, deby, free_me, f, e);
+        }
     }
     f.x = wave_sum(f.x);
     f.y = wave_sum(f.y);
@@ -1006,32 +1022,55 @@ __device__ __forceinline__ int tile_cat_of(int cat) {
     }
 }
 
+// Energy terms: category cat sums its per-particle partials followed by its tile pairs'
+// energies (tile mode), as one sequence cut into RED_BLOCKS contiguous ranges; block b
+// sums range b (four interleaved accumulators per thread, so loads overlap, then a tree),
+// and k_reduce_finish adds the RED_BLOCKS range sums in order.  Fixed shapes throughout:
+// the same inputs give the same bits.
 __global__ __launch_bounds__(BLOCK) void k_reduce_energy(const double *epart, int ecap, int n,
-                                                         double *out, const double *tener,
+                                                         double *part, const double *tener,
                                                          int titems) {
     __shared__ double s[BLOCK];
-    const int cat = blockIdx.x, tid = threadIdx.x;
+    const int cat = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
     const double *src = epart + (size_t)cat * ecap;
-    double acc = 0.0;
-    for (int k = tid; k < n; k += BLOCK) acc += src[k];
     const int tc = tener ? tile_cat_of(cat) : -1;
-    if (tc >= 0) {
-        const double *t = tener + (size_t)tc * titems;
-        for (int k = tid; k < titems; k += BLOCK) acc += t[k];
+    const double *tsrc = tc >= 0 ? tener + (size_t)tc * titems : nullptr;
+    const int m = n + (tc >= 0 ? titems : 0);
+    const int chunk = (m + RED_BLOCKS - 1) / RED_BLOCKS;
+    const int lo = b * chunk, hi = min(lo + chunk, m);
+    auto val = [&](int k) { return k < n ? src[k] : tsrc[k - n]; };
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = lo + tid;
+    for (; k + 3 * BLOCK < hi; k += 4 * BLOCK) {
+        a0 += val(k);
+        a1 += val(k + BLOCK);
+        a2 += val(k + 2 * BLOCK);
+        a3 += val(k + 3 * BLOCK);
     }
-    s[tid] = acc;
+    for (; k < hi; k += BLOCK) a0 += val(k);
+    s[tid] = (a0 + a1) + (a2 + a3);
     __syncthreads();
     for (int h = BLOCK / 2; h > 0; h >>= 1) {
         if (tid < h) s[tid] += s[tid + h];
         __syncthreads();
     }
-    if (tid == 0) out[cat] = s[0];
+    if (tid == 0) part[(size_t)cat * RED_BLOCKS + b] = s[0];
+}
+
+__global__ void k_reduce_finish(const double *part, double *out) {
+    const int cat = threadIdx.x;
+    if (cat >= NE) return;
+    double acc = 0.0;
+    for (int b = 0; b < RED_BLOCKS; b++) acc += part[(size_t)cat * RED_BLOCKS + b];
+    out[cat] = acc;
 }
 
 hipError_t launch_reduce_energy(const double *epart, int ecap, int n, double *out,
                                 hipStream_t s, const double *tener, int titems) {
-    hipLaunchKernelGGL(k_reduce_energy, dim3(NE), dim3(BLOCK), 0, s, epart, ecap, n, out, tener,
-                       titems);
+    double *part = out + NE;   // the caller's buffer holds NE * (1 + RED_BLOCKS) doubles
+    hipLaunchKernelGGL(k_reduce_energy, dim3(NE, RED_BLOCKS), dim3(BLOCK), 0, s, epart, ecap, n,
+                       part, tener, titems);
+    hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(64), 0, s, (const double *)part, out);
     return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@ constexpr int SORT_BLOCK = 256;        // threads of the histogram / scatter blo
 constexpr int SORT_RADIX = 256;        // 8-bit digits
 constexpr int SORT_PASSES = 4;         // 32 bits of key (class 2, Morton code 30)
 constexpr int SCAN_BLOCK = 1024;
+constexpr int SWAVE = 64;              // lanes of a wavefront
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
     v &= 0x3FFu;
@@ -67,30 +68,71 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_digit_hist(const uint32_t *__res
     for (int d = tid; d < SORT_RADIX; d += SORT_BLOCK) hist[d * nb + b] = s_c[d];
 }
 
-// exclusive prefix sum of a[0..m) in place, one block: each thread scans a contiguous
-// segment, the segment totals are scanned in LDS, then added back; also total[0] = sum
+// exclusive prefix sum of a[0..m) in place, one block, in rounds of SCAN_BLOCK * SCAN_PER
+// elements: thread t loads its SCAN_PER consecutive elements of the round at once (16-byte
+// loads, all in flight together), the thread totals are scanned by wavefront shuffles and
+// the wavefront totals in LDS, and each thread writes its elements' prefixes; a running
+// carry joins the rounds.  Also total[0] = the sum.  Integer sums: exact in any order.
+constexpr int SCAN_PER = 64;
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_excl(int32_t *a, int m, int32_t *total) {
-    __shared__ int32_t s[SCAN_BLOCK];
-    const int tid = threadIdx.x;
-    const int seg = (m + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    const int lo = min(tid * seg, m), hi = min(lo + seg, m);
-    int32_t sum = 0;
-    for (int k = lo; k < hi; k++) sum += a[k];
-    s[tid] = sum;
+    constexpr int NWS = SCAN_BLOCK / SWAVE, NV = SCAN_PER / 4;
+    __shared__ int32_t s_w[NWS];
+    __shared__ int32_t s_carry;
+    const int tid = threadIdx.x, lane = tid & (SWAVE - 1), wv = tid / SWAVE;
+    if (tid == 0) s_carry = 0;
     __syncthreads();
-    for (int off = 1; off < SCAN_BLOCK; off <<= 1) {   // Hillis-Steele inclusive scan
-        const int32_t v = tid >= off ? s[tid - off] : 0;
+    for (int base = 0; base < m; base += SCAN_BLOCK * SCAN_PER) {
+        const int lo = base + tid * SCAN_PER;
+        int4 v[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            const int k = lo + 4 * q;
+            if (k + 3 < m) {
+                v[q] = *reinterpret_cast<const int4 *>(a + k);
+            } else {
+                v[q].x = k < m ? a[k] : 0;
+                v[q].y = k + 1 < m ? a[k + 1] : 0;
+                v[q].z = k + 2 < m ? a[k + 2] : 0;
+                v[q].w = 0;
+            }
+        }
+        int32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < NV; q++) sum += v[q].x + v[q].y + v[q].z + v[q].w;
+        int32_t incl = sum;
+#pragma unroll
+        for (int off = 1; off < SWAVE; off <<= 1) {
+            const int32_t u = __shfl_up(incl, off, SWAVE);
+            if (lane >= off) incl += u;
+        }
+        if (lane == SWAVE - 1) s_w[wv] = incl;
         __syncthreads();
-        s[tid] += v;
+        int32_t run = s_carry + incl - sum, tot = 0;
+        for (int q = 0; q < NWS; q++) {
+            if (q < wv) run += s_w[q];
+            tot += s_w[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            const int k = lo + 4 * q;
+            int4 o;
+            o.x = run; run += v[q].x;
+            o.y = run; run += v[q].y;
+            o.z = run; run += v[q].z;
+            o.w = run; run += v[q].w;
+            if (k + 3 < m) {
+                *reinterpret_cast<int4 *>(a + k) = o;
+            } else {
+                if (k < m) a[k] = o.x;
+                if (k + 1 < m) a[k + 1] = o.y;
+                if (k + 2 < m) a[k + 2] = o.z;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) s_carry += tot;
         __syncthreads();
     }
-    int32_t run = s[tid] - sum;                        // exclusive start of the segment
-    for (int k = lo; k < hi; k++) {
-        const int32_t v = a[k];
-        a[k] = run;
-        run += v;
-    }
-    if (tid == SCAN_BLOCK - 1 && total) total[0] = s[tid];
+    if (tid == 0 && total) total[0] = s_carry;
 }
 
 // stable scatter of block b's chunk: element k of digit d goes to offset[d * nb + b] plus
