@@ -435,6 +435,8 @@ StepParams params_for(bromoc_ctx *c) {
         p.tnc = p.rfon ? 5 : 3;
         p.tpart = c->tpart;
         p.tener = c->tener;
+        // one word per tile pair after the energies: its reaction-field sums were written
+        p.tflag = reinterpret_cast<int32_t *>(c->tener + (size_t)NTE * p.titems);
     }
     return p;
 }

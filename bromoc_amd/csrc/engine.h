@@ -175,6 +175,7 @@ struct StepParams {
     // fused reaction field A, B) of slot t of tile I from the tile pair of I and K, each
     // entry written once per step; tener[cat][item]: pair energies of each tile pair
     double *tpart, *tener;
+    int32_t *tflag;          // [titems] 1 where a tile pair wrote its reaction-field rows
     int32_t ntile, ttpl, tnc, titems;
 };
 

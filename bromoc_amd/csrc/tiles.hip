@@ -48,7 +48,7 @@ size_t tile_part_doubles(int cap, int ttpl, int rf) {
 }
 size_t tile_ener_doubles(int cap, int ttpl) {
     const size_t tt = 64 * (size_t)ttpl, n = (cap + tt - 1) / tt;
-    return (size_t)NTE * (n * (n + 1) / 2);
+    return (size_t)(NTE + 1) * (n * (n + 1) / 2);   // energies, then the tflag words
 }
 
 // The particle in slot s of the spatial order: position, and a word of atnam2 type and
@@ -388,10 +388,12 @@ __global__ __launch_bounds__(TILE_WAVES * WAVE) void k_tiles(StepParams P) {
     const bool siteI = (andI & (SL_VALID | SL_SITE)) == (SL_VALID | SL_SITE);
     const bool siteJ = (andJ & (SL_VALID | SL_SITE)) == (SL_VALID | SL_SITE);
     const bool wr = P.write_energy != 0;
+    bool rfsums = false;   // this tile pair's reaction-field sums (A, B) may be nonzero
     if (!(orI & SL_ION) && !(orJ & SL_ION)) {
         // sites, fixed ions and empty slots only: no pair of these is evaluated
     } else if (!diag && ionI && ionJ && !(RF && ((orI | orJ) & SL_RFODD))) {
         if (RF && (orI & SL_RFON) && (orJ & SL_RFON)) {
+            rfsums = true;
             if (wr) sweep_fast(std::true_type{}, std::bool_constant<RF>{}, std::false_type{});
             else sweep_fast(std::false_type{}, std::bool_constant<RF>{}, std::false_type{});
         } else {
@@ -402,6 +404,7 @@ __global__ __launch_bounds__(TILE_WAVES * WAVE) void k_tiles(StepParams P) {
         if (wr) sweep_fast(std::true_type{}, std::false_type{}, std::true_type{});
         else sweep_fast(std::false_type{}, std::false_type{}, std::true_type{});
     } else {
+        rfsums = RF;
         if (wr) sweep(std::true_type{});
         else sweep(std::false_type{});
     }
@@ -410,8 +413,13 @@ __global__ __launch_bounds__(TILE_WAVES * WAVE) void k_tiles(StepParams P) {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_wave_barrier();
 
-    const int NC = RF ? 5 : 3;
+    // a tile pair without reaction-field sums writes the three force components only and
+    // says so in tflag[item], which k_tfin reads before it reads the A and B rows
+    const int NC = RF ? 5 : 3, nc = rfsums ? NC : 3;
     double *const part = P.tpart;
+    if constexpr (RF) {
+        if (lane == 0) P.tflag[item] = rfsums ? 1 : 0;
+    }
 #pragma unroll
     for (int u = 0; u < TPL; u++) {
         const int t = lane + WAVE * u;
@@ -422,12 +430,14 @@ __global__ __launch_bounds__(TILE_WAVES * WAVE) void k_tiles(StepParams P) {
         }
         double *dst = part + ((size_t)(J * n + I) * NC) * TT + t;
 #pragma unroll
-        for (int c = 0; c < NC; c++) dst[(size_t)c * TT] = v[c];
+        for (int c = 0; c < NC; c++)
+            if (c < nc) dst[(size_t)c * TT] = v[c];
         if (!diag) {
             const double w[5] = {s_fx[t], s_fy[t], s_fz[t], RF ? s_fa[t] : 0.0, RF ? s_fb[t] : 0.0};
             double *dp = part + ((size_t)(I * n + J) * NC) * TT + t;
 #pragma unroll
-            for (int c = 0; c < NC; c++) dp[(size_t)c * TT] = w[c];
+            for (int c = 0; c < NC; c++)
+                if (c < nc) dp[(size_t)c * TT] = w[c];
         }
     }
     if (P.write_energy) {
@@ -469,8 +479,15 @@ __global__ __launch_bounds__(BLOCK) void k_tfin(StepParams P) {
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int K = sub; K < n; K += 8) {
         const double *src = P.tpart + ((size_t)(K * n + I) * NC) * TT + t;
+        // the A and B rows only where the tile pair (min, max) wrote them
+        int nc = 3;
+        if constexpr (RF) {
+            const int lo = min(I, K), hi = max(I, K);
+            nc = P.tflag[hi * (hi + 1) / 2 + lo] ? 5 : 3;
+        }
 #pragma unroll
-        for (int c = 0; c < NC; c++) acc[c] += src[(size_t)c * TT];
+        for (int c = 0; c < NC; c++)
+            if (c < nc) acc[c] += src[(size_t)c * TT];
     }
 #pragma unroll
     for (int c = 0; c < NC; c++) acc[c] = group8_sum(acc[c]);
