@@ -1034,8 +1034,8 @@ static hipError_t enqueue_step(bromoc_ctx *c, const StepParams &p, hipEvent_t t0
 // Autotuner: the pair kernel's launch shape for this system -- k_step with 1, 2 or 4
 // targets per block, k_wave (8), or tile mode with 1, 2 or 4 targets per lane (17, 18,
 // 20) -- chosen by timing ENERGY-only launches (warnings go to a scratch counter) on the
-// ctx's stream: the reaction-field kernels the shape needs plus its pair kernels.  The
-// environment variable BROMOC_TPB forces a value.
+// ctx's stream: the reaction-field kernels the shape needs, the CG-DNA kernel and its pair
+// kernels.  The environment variable BROMOC_TPB forces a value.
 static int autotune(bromoc_ctx *c) {
     if (c->tpb > 0) return BROMOC_OK;
     if (const char *env = std::getenv("BROMOC_TPB")) {
@@ -1078,11 +1078,13 @@ static int autotune(bromoc_ctx *c) {
             p.write_forces = 0;
             p.warn = c->warn_scratch;
             // one whole ENERGY (the DNA, reaction-field and fixed-ion terms the pair kernel
-            // reads), then the shape's reaction-field and pair kernels are timed
+            // reads), then the shape's reaction-field, CG-DNA and pair kernels are timed (the
+            // one-target shape computes the CG-DNA terms in its pair kernel)
             HIPCHK(c, launch_step(p, c->stream));
             HIPCHK(c, hipEventRecord(e0, c->stream));
             for (int r = 0; r < 5; r++) {
                 HIPCHK(c, launch_rfpar(p, !tile_mode(p.tpb), c->stream, true));
+                HIPCHK(c, launch_dna(p, c->stream));
                 HIPCHK(c, launch_pairs(p, c->stream));
             }
             HIPCHK(c, hipEventRecord(e1, c->stream));

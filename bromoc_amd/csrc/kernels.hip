@@ -1,7 +1,8 @@
 // kernels.hip -- gfx950 kernels of the BROMOC force / Brownian-dynamics step of one system.
 //
-// One BD step is k_rfpar/k_rfpair (reaction-field parameters, when present), k_dna (CG-DNA
-// list terms of each site, energy.mono.f90:196-508) and k_step (or k_wave): every target
+// One BD step is k_rfpar/k_rfpair (reaction-field parameters, when present), the CG-DNA
+// list terms of each site (energy.mono.f90:196-508; in k_step's own one-target blocks, else
+// k_dna) and k_step (or k_wave): every target
 // particle -- each free DNA site and each moving ion -- gathers its pair forces over all
 // other particles (the reference's ENERGY has no cutoff: energy.mono.f90:117-194,584-687),
 // its 8 trilinear corners of the static-field and repulsion grids (staticf1.f90:45-118,
@@ -48,14 +49,129 @@ __device__ inline void prox_record(const StepParams &P, int i, const double4 &pa
     dids[4] = ez;
 }
 
+// CONTRANS force on site i along axis ax (energy.mono.f90:520-580), added to f entry by
+// entry: a restrained site is pulled to its centre, every site by a restrained centroid
+// (of all sites, or of its strand with unsplit); no stfree gate, as in the reference.
+__device__ inline double contrans_force(const StepParams &P, int i, int ax, double f) {
+    const double *kk = ax == 0 ? P.kx : (ax == 1 ? P.ky : P.kz);
+    const double *cc = ax == 0 ? P.contrx : (ax == 1 ? P.contry : P.contrz);
+    const double xi = ax == 0 ? P.pos[i].x : (ax == 1 ? P.pos[i].y : P.pos[i].z);
+    for (int q = 0; q < P.ctn; q++) {
+        const double k = kk[q];
+        if (k == 0.0) continue;
+        const int j = P.csn[q];
+        if (j == 0) {
+            if (P.unsplit) {
+                if (i < P.nsites1st) {
+                    const double con = P.cen[3 + ax] * P.insites * 2.0 - cc[q];
+                    f = f - k * con * P.insites * 2.0;
+                } else {
+                    const double con = P.cen[6 + ax] * P.insites * 2.0 - cc[P.ctn];
+                    f = f - k * con * P.insites * 2.0;
+                }
+            } else {
+                const double con = P.cen[ax] * P.insites - cc[q];
+                f = f - k * con * P.insites;
+            }
+        } else if (j - 1 == i) {
+            f = f - k * (xi - cc[q]);
+        }
+    }
+    return f;
+}
+
+// CG-DNA list terms of site i (energy.mono.f90:196-508): this thread's share -- records
+// tid, tid + NB, ... of the site's incidence list -- of the force on the site and of the
+// energies of the terms it heads (e[8], slots D_BOND..D_SOLV).  The site's angle and
+// dihedral records come first; its two-site records, from inc_split on, are taken DU at a
+// time: the records, then the other sites' positions, then the terms, so DU loads of each
+// kind are in flight together.
+// records per thread in flight when k_step computes a site's terms
+constexpr int DNA_DU = 2;
+
+template <int NB, int DU>
+__device__ inline void dna_site_terms(const StepParams &P, int i, int tid, V3 &f, double e[8]) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) e[k] = 0.0;
+    const bool free_me = P.stfree[i] != 0;
+    const int k0 = P.inc_ptr[i], ks = P.inc_split[i], k1 = P.inc_ptr[i + 1];
+    const V3 rme = ld3(P, i);
+    if (P.flags & QNOBOND) {
+        for (int k = k0 + tid; k < ks; k += NB)
+            dna_term(P, (uint32_t)__double_as_longlong(P.incr[k].x), free_me, f, e);
+    }
+    for (int kb = ks + tid; kb < k1; kb += DU * NB) {
+        double2 rec[DU];
+        V3 ro[DU];
+#pragma unroll
+        for (int u = 0; u < DU; u++) {
+            // resolved incidence: one load gives the term, the other site and the list value
+            const int k = kb + u * NB;
+            rec[u] = P.incr[k < k1 ? k : ks];
+        }
+#pragma unroll
+        for (int u = 0; u < DU; u++)
+            ro[u] = ld3(P, (int)(((uint64_t)__double_as_longlong(rec[u].x) >> 32) & 0x7FFFFFFFu));
+#pragma unroll
+        for (int u = 0; u < DU; u++) {
+            if (kb + u * NB >= k1) break;
+            const uint64_t bits = (uint64_t)__double_as_longlong(rec[u].x);
+            const uint32_t wd = (uint32_t)bits;
+            const int kind = wd >> 29;
+            const bool on = kind == T_BOND ? (P.flags & QNOBOND) != 0
+                                           : ((P.flags & QNONBOND) != 0 &&
+                                              (kind != T_SLV || (P.flags & QSOLV) != 0));
+            if (!on) continue;
+            const int role = (wd >> 27) & 3;
+            const V3 v1 = role == 0 ? sub3(ro[u], rme) : sub3(rme, ro[u]);   // r_b - r_a
+            const bool deby = kind == T_QQ && qq_debye(P, rme, ro[u]);
+            dna_pair(P, kind, role, v1, rec[u].y, (bits >> 63) != 0, deby, free_me, f, e);
+        }
+    }
+}
+
+// The wavefront sums of dna_site_terms' outputs into s_r[wv][0..10] (force, then energies).
+template <int NW>
+__device__ inline void dna_block_sums(const StepParams &P, int lane, int wv, V3 f,
+                                      double e[8], double (*s_r)[3 + 8]) {
+    f.x = wave_sum(f.x);
+    f.y = wave_sum(f.y);
+    f.z = wave_sum(f.z);
+    if (P.write_energy) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) e[k] = wave_sum(e[k]);
+    }
+    if (lane == 0) {
+        s_r[wv][0] = f.x; s_r[wv][1] = f.y; s_r[wv][2] = f.z;
+        for (int k = 0; k < 8; k++) s_r[wv][3 + k] = e[k];
+    }
+}
+
+// Entry c of site i's CG-DNA result from the block's wavefront sums, in wavefront order: a
+// force component (c < 3, with APFOR and CONTRANS added) or the energy of slot c - 3.
+template <int NW>
+__device__ inline double dna_site_finish(const StepParams &P, int i, int c,
+                                         const double (*s_r)[3 + 8]) {
+    double acc = 0.0;
+    for (int q = 0; q < NW; q++) acc += s_r[q][c];
+    if (c < 3) {
+        if ((P.flags & QAPFOR) && P.stfree[i] != 0) acc += reinterpret_cast<const double *>(&P.apf[i])[c];
+        if (P.flags & QCONTRANS) acc = contrans_force(P, i, c, acc);
+    }
+    return acc;
+}
+
 // k_step: T target particles per block of NW wavefronts (targets tbeg.. of the launch).
 // Every thread walks the partner set with stride NW*64, two partners per iteration, and
 // evaluates each partner against all T targets.  Per-target sums: butterfly within each
 // wavefront, then a fixed-order sum over the NW wavefronts in LDS.  Wavefront w < T also
 // handles target w's 8 grid corners; thread a < T finishes target a (force, energies,
-// BD update).  CG-DNA list forces of site targets come from k_dna (fdna).
+// BD update).  A one-target block computes its site target's CG-DNA list terms itself
+// (dna_site_terms, before the pair loop); blocks of several targets read them from k_dna
+// (fdna).
 template <int FEAT, int T, int NW>
-__global__ __launch_bounds__(NW * WAVE) void k_step(StepParams P) {
+__global__ __launch_bounds__(NW * WAVE) __attribute__((amdgpu_waves_per_eu(T == 1 ? 5 : 1)))
+void k_step(StepParams P) {
     constexpr int NB = NW * WAVE;   // threads of the block
     __shared__ DevPair s_row[T][MAX_LDS_TYPES];   // pair parameters by partner type
     __shared__ double4 s_tail[T][MAX_LDS_TYPES];  // (sc3, tail fct, dm2hi, 0) by partner type
@@ -91,6 +207,18 @@ __global__ __launch_bounds__(NW * WAVE) void k_step(StepParams P) {
         }
     }
     __syncthreads();
+
+    // CG-DNA list terms of a site target (the block-uniform branch of a one-target block)
+    constexpr bool DNA_HERE = (FEAT & F_DNA) != 0 && T == 1;
+    __shared__ double s_dn[DNA_HERE ? NW : 1][3 + 8];
+    if constexpr (DNA_HERE) {
+        if (tval[0] && tsite[0]) {
+            V3 f = {0.0, 0.0, 0.0};
+            double e[8];
+            dna_site_terms<NB, DNA_DU>(P, ti[0], tid, f, e);
+            dna_block_sums<NW>(P, lane, wv, f, e, s_dn);
+        }
+    }
 
     // Gaussians of the BD update, fetched by the last wavefront (lanes 0..T-1) before its
     // share of the pair loop
@@ -297,7 +425,12 @@ __global__ __launch_bounds__(NW * WAVE) void k_step(StepParams P) {
         for (int q = 0; q < NW; q++) acc += s_red[q][a][k];
         F[k] = acc + s_dg[a][k];
     }
-    if constexpr ((FEAT & F_DNA) != 0) {
+    if constexpr (DNA_HERE) {
+        if (is_site) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) F[k] += dna_site_finish<NW>(P, i, k, s_dn);
+        }
+    } else if constexpr ((FEAT & F_DNA) != 0) {
         if (is_site) {
             const double4 fd = P.fdna[i];   // CG-DNA list terms from k_dna
             F[0] += fd.x; F[1] += fd.y; F[2] += fd.z;
@@ -326,9 +459,15 @@ __global__ __launch_bounds__(NW * WAVE) void k_step(StepParams P) {
             for (int q = 0; q < NW; q++) acc += s_red[q][a][3 + k];
             pe[k] = acc;
         }
-        // this kernel owns the pair and grid categories; k_dna writes the DNA ones
+        // this kernel owns the pair and grid categories, and the DNA ones when it computed
+        // them; otherwise k_dna writes those
         double *ep_ = P.epart + i;
         const size_t cs = P.ecap;
+        if constexpr (DNA_HERE) {
+            if (is_site) {
+                for (int k = 0; k < 8; k++) ep_[ND_CAT[k] * cs] = dna_site_finish<NW>(P, i, 3 + k, s_dn);
+            }
+        }
         ep_[E_GSBPA * cs] = s_dg[a][3];
         ep_[E_VDWGD * cs] = s_dg[a][4];
         if (is_site) {
@@ -363,37 +502,6 @@ __global__ __launch_bounds__(NW * WAVE) void k_step(StepParams P) {
         P.poso[i] = pn;
         P.sposo[P.rank[i]] = pn;
     }
-}
-
-// CONTRANS force on site i along axis ax (energy.mono.f90:520-580), added to f entry by
-// entry: a restrained site is pulled to its centre, every site by a restrained centroid
-// (of all sites, or of its strand with unsplit); no stfree gate, as in the reference.
-__device__ inline double contrans_force(const StepParams &P, int i, int ax, double f) {
-    const double *kk = ax == 0 ? P.kx : (ax == 1 ? P.ky : P.kz);
-    const double *cc = ax == 0 ? P.contrx : (ax == 1 ? P.contry : P.contrz);
-    const double xi = ax == 0 ? P.pos[i].x : (ax == 1 ? P.pos[i].y : P.pos[i].z);
-    for (int q = 0; q < P.ctn; q++) {
-        const double k = kk[q];
-        if (k == 0.0) continue;
-        const int j = P.csn[q];
-        if (j == 0) {
-            if (P.unsplit) {
-                if (i < P.nsites1st) {
-                    const double con = P.cen[3 + ax] * P.insites * 2.0 - cc[q];
-                    f = f - k * con * P.insites * 2.0;
-                } else {
-                    const double con = P.cen[6 + ax] * P.insites * 2.0 - cc[P.ctn];
-                    f = f - k * con * P.insites * 2.0;
-                }
-            } else {
-                const double con = P.cen[ax] * P.insites - cc[q];
-                f = f - k * con * P.insites;
-            }
-        } else if (j - 1 == i) {
-            f = f - k * (xi - cc[q]);
-        }
-    }
-    return f;
 }
 
 // k_centroid: site position sums of the current positions -- all sites, the first strand
@@ -501,82 +609,25 @@ hipError_t launch_notrans(const StepParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
-// k_dna: CG-DNA bonded and list terms (energy.mono.f90:196-508), one block per site.
-// The block's threads take the site's term incidences in order (they are grouped by term
-// kind, so wavefronts rarely diverge); sums go butterfly-then-LDS in a fixed order.  The
-// force on the site goes to fdna[i] (zero for a fixed site); the energies of the terms the
-// site heads go to epart.
+// k_dna: CG-DNA bonded and list terms (energy.mono.f90:196-508), one block per site, for
+// the pair kernels that do not compute them in their site targets' blocks.  Sums go
+// butterfly-then-LDS in a fixed order.  The force on the site goes to fdna[i] (zero for a
+// fixed site); the energies of the terms the site heads go to epart.
 template <int NW>
 __global__ __launch_bounds__(NW * WAVE) void k_dna(StepParams P) {
     constexpr int NB = NW * WAVE;   // threads of the block
     __shared__ double s_r[NW][3 + 8];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
     const int i = blockIdx.x;
-    const bool free_me = P.stfree[i] != 0;
     V3 f = {0.0, 0.0, 0.0};
-    double e[ND];
-#pragma unroll
-    for (int k = 0; k < ND; k++) e[k] = 0.0;
-    // the site's angle and dihedral records come first, its two-site records from ks on
-    const int k0 = P.inc_ptr[i], ks = P.inc_split[i], k1 = P.inc_ptr[i + 1];
-    const V3 rme = ld3(P, i);
-    if (P.flags & QNOBOND) {
-        for (int k = k0 + tid; k < ks; k += NB)
-            dna_term(P, (uint32_t)__double_as_longlong(P.incr[k].x), free_me, f, e);
-    }
-    // two-site terms, DU records per thread at a time: the records, then the other sites'
-    // positions, then the terms, so DU loads of each kind are in flight together
-    constexpr int DU = 4;
-    for (int kb = ks + tid; kb < k1; kb += DU * NB) {
-        double2 rec[DU];
-        V3 ro[DU];
-#pragma unroll
-        for (int u = 0; u < DU; u++) {
-            // resolved incidence: one load gives the term, the other site and the list value
-            const int k = kb + u * NB;
-            rec[u] = P.incr[k < k1 ? k : ks];
-        }
-#pragma unroll
-        for (int u = 0; u < DU; u++)
-            ro[u] = ld3(P, (int)(((uint64_t)__double_as_longlong(rec[u].x) >> 32) & 0x7FFFFFFFu));
-#pragma unroll
-        for (int u = 0; u < DU; u++) {
-            if (kb + u * NB >= k1) break;
-            const uint64_t bits = (uint64_t)__double_as_longlong(rec[u].x);
-            const uint32_t wd = (uint32_t)bits;
-            const int kind = wd >> 29;
-            const bool on = kind == T_BOND ? (P.flags & QNOBOND) != 0
-                                           : ((P.flags & QNONBOND) != 0 &&
-                                              (kind != T_SLV || (P.flags & QSOLV) != 0));
-            if (!on) continue;
-            const int role = (wd >> 27) & 3;
-            const V3 v1 = role == 0 ? sub3(ro[u], rme) : sub3(rme, ro[u]);   // r_b - r_a
-            const bool deby = kind == T_QQ && qq_debye(P, rme, ro[u]);
-            dna_pair(P, kind, role, v1, rec[u].y, (bits >> 63) != 0, deby, free_me, f, e);
-        }
-    }
-    f.x = wave_sum(f.x);
-    f.y = wave_sum(f.y);
-    f.z = wave_sum(f.z);
-    if (P.write_energy) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) e[k] = wave_sum(e[k]);
-    }
-    if (lane == 0) {
-        s_r[wv][0] = f.x; s_r[wv][1] = f.y; s_r[wv][2] = f.z;
-        for (int k = 0; k < 8; k++) s_r[wv][3 + k] = e[k];
-    }
+    double e[8];
+    dna_site_terms<NB, 4>(P, i, tid, f, e);
+    dna_block_sums<NW>(P, lane, wv, f, e, s_r);
     __syncthreads();
     if (tid < 3 + 8) {
-        double acc = 0.0;
-        for (int q = 0; q < NW; q++) acc += s_r[q][tid];
-        if (tid < 3) {
-            if ((P.flags & QAPFOR) && free_me) acc += reinterpret_cast<const double *>(&P.apf[i])[tid];
-            if (P.flags & QCONTRANS) acc = contrans_force(P, i, tid, acc);
-            reinterpret_cast<double *>(&P.fdna[i])[tid] = acc;
-        } else if (P.write_energy) {
-            P.epart[(size_t)ND_CAT[tid - 3] * P.ecap + i] = acc;
-        }
+        const double acc = dna_site_finish<NW>(P, i, tid, s_r);
+        if (tid < 3) reinterpret_cast<double *>(&P.fdna[i])[tid] = acc;
+        else if (P.write_energy) P.epart[(size_t)ND_CAT[tid - 3] * P.ecap + i] = acc;
     }
 }
 
@@ -815,10 +866,18 @@ static void dispatch_prox(const StepParams &p, int feat, hipStream_t s) {
 #undef CASE
 }
 
+// Whether the pair kernel launch_pairs picks computes the CG-DNA list terms of its site
+// targets itself: the one-target k_step (tpb 1, and every DYNAMICS3 step).
+static bool dna_in_pairs(const StepParams &p) {
+    if (tile_mode(p.tpb)) return false;
+    if (p.dmode == 3 && p.mode == 1) return true;
+    return p.tpb == 1;
+}
+
 hipError_t launch_dna(const StepParams &p, hipStream_t s) {
     if (!(p.flags & QNUCL) || p.nsites <= 0) return hipSuccess;
     if (p.flags & QCONTRANS) hipLaunchKernelGGL(k_centroid, dim3(1), dim3(BLOCK), 0, s, p);
-    hipLaunchKernelGGL(k_dna<WPB>, dim3(p.nsites), dim3(BLOCK), 0, s, p);
+    if (!dna_in_pairs(p)) hipLaunchKernelGGL(k_dna<WPB>, dim3(p.nsites), dim3(BLOCK), 0, s, p);
     return hipGetLastError();
 }
 
