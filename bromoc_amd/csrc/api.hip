@@ -85,9 +85,12 @@ struct bromoc_ctx {
     DevPair *pairs = nullptr, *rows = nullptr;
     double4 *ep = nullptr;
     double *cg = nullptr, *fact1a = nullptr, *fact2a = nullptr, *scal = nullptr, *zcont = nullptr;
-    int32_t *inc_ptr = nullptr, *inc_split = nullptr;
-    uint32_t *inc = nullptr;
+    int32_t *inc_ptr = nullptr, *inc_split = nullptr, *incr_ptr = nullptr, *incb_ptr = nullptr;
+    uint32_t *inc = nullptr, *incb = nullptr;
     double2 *incr = nullptr;
+    double4 *fmul = nullptr;
+    double *bpar = nullptr;
+    double2 *dihcs = nullptr;
     int32_t *lst[8] = {};
     double *lval[6] = {};
     float *phix = nullptr;
@@ -672,44 +675,110 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
         for (int64_t q = 0; q < (int64_t)n * ar[k]; q++) deg[lp[k][q] + 1]++;
     }
     for (int i = 0; i < ns; i++) deg[i + 1] += deg[i];
+    // each site's terms (INTERACT's list) in the order angle, dihedral, bond, stacking, base
+    // pair, excluded volume, P-P, solvent
+    static const int order[8] = {T_ANGLE, T_DIHE, T_BOND, T_STACK, T_BP, T_EX, T_QQ, T_SLV};
     std::vector<uint32_t> inc(std::max<int64_t>(deg[ns], 1));
-    std::vector<double2> incr(inc.size(), make_double2(0.0, 0.0));
-    // each site's angle and dihedral records first, then its two-site records (bond,
-    // stacking, base pair, excluded volume, P-P, solvent) from split[i] on
-    std::vector<int32_t> split(std::max(ns, 1), 0);
     {
         std::vector<int32_t> fill(deg.begin(), deg.end());
-        for (int i = 0; i < ns; i++) split[i] = deg[i];
+        for (int ko = 0; ko < 8 && dna; ko++)
+            for (int r = 0, k = order[ko]; r < ar[k]; r++)
+                for (int t = 0; t < cnt[k]; t++) inc[fill[lp[k][(int64_t)r * cnt[k] + t]]++] = inc_pack(k, r, t);
+    }
+    // Resolved records for the force kernels.  The two-site terms that come in bulk
+    // (excluded volume, P-P, solvent, base pairs) take one word each when the sites fit in
+    // 24 bits and the list's values in a table of 16: the other site, the term, the site's
+    // role, whether site a of a base pair is a C or G base, and the value's table slot.
+    // The others take 16 bytes: the word (kind, role, list index), the other site, that
+    // flag and the list value; a site's angle and dihedral records come first.
+    static const int bulk_kind[4] = {T_EX, T_QQ, T_SLV, T_BP};
+    bool bulk[8] = {};
+    std::vector<double> bpar(4 * 16, 0.0);
+    for (int q = 0; q < 4 && dna && ns <= (1 << 24); q++) {
+        const int k = bulk_kind[q], sl = pair_lval_slot(k);
+        bool ok = true;
+        std::vector<double> vals;
+        for (int t = 0; t < cnt[k] && ok && sl >= 0; t++) {
+            const double v = vp[sl][t];
+            bool seen = false;
+            for (size_t m = 0; m < vals.size() && !seen; m++)
+                seen = std::memcmp(&vals[m], &v, sizeof(v)) == 0;
+            if (!seen) {
+                ok = vals.size() < 16;
+                vals.push_back(v);
+            }
+        }
+        if (!ok) continue;
+        bulk[k] = true;
+        for (size_t m = 0; m < vals.size(); m++) bpar[16 * q + m] = vals[m];
+    }
+    std::vector<int32_t> rptr(ns + 1, 0), bptr(ns + 1, 0), split(std::max(ns, 1), 0);
+    for (int k = 0; k < 8 && dna; k++)
+        for (int64_t q = 0; q < (int64_t)cnt[k] * ar[k]; q++) (bulk[k] ? bptr : rptr)[lp[k][q] + 1]++;
+    for (int i = 0; i < ns; i++) {
+        rptr[i + 1] += rptr[i];
+        bptr[i + 1] += bptr[i];
+    }
+    std::vector<double2> incr(std::max(rptr[ns], 1), make_double2(0.0, 0.0));
+    std::vector<uint32_t> incb(std::max(bptr[ns], 1), 0u);
+    {
+        std::vector<int32_t> rfill(rptr.begin(), rptr.end()), bfill(bptr.begin(), bptr.end());
+        for (int i = 0; i < ns; i++) split[i] = rptr[i];
         for (int k = T_ANGLE; k <= T_DIHE && dna; k++)
             for (int64_t q = 0; q < (int64_t)cnt[k] * ar[k]; q++) split[lp[k][q]]++;
-        static const int order[8] = {T_ANGLE, T_DIHE, T_BOND, T_STACK, T_BP, T_EX, T_QQ, T_SLV};
         for (int ko = 0; ko < 8 && dna; ko++)
             for (int r = 0, k = order[ko]; r < ar[k]; r++)
                 for (int t = 0; t < cnt[k]; t++) {
-                    int site = lp[k][(int64_t)r * cnt[k] + t];
-                    const int at = fill[site]++;
-                    inc[at] = inc_pack(k, r, t);
-                    // resolved record of a two-site term: the other site, the list value,
-                    // and for a base pair whether site a is a C or G base
-                    uint64_t bits = inc[at];
-                    double par = 0.0;
+                    const int site = lp[k][(int64_t)r * cnt[k] + t];
+                    const int sl = pair_lval_slot(k);
+                    const double par = sl >= 0 ? vp[sl][t] : 0.0;
+                    const int other = ar[k] == 2 ? lp[k][(int64_t)(1 - r) * cnt[k] + t] : 0;
+                    if (ar[k] > 2) {   // angle, dihedral: the other sites in list order
+                        uint32_t os[3] = {0u, 0u, 0u};
+                        for (int rr = 0, q = 0; rr < ar[k]; rr++)
+                            if (rr != r) os[q++] = (uint32_t)lp[k][(int64_t)rr * cnt[k] + t];
+                        const int at = rfill[site]++;
+                        const uint64_t bx = inc_pack(k, r, t) | (uint64_t)os[0] << 32;
+                        const uint64_t by = (uint64_t)os[1] | (uint64_t)os[2] << 32;
+                        std::memcpy(&incr[at].x, &bx, sizeof(bx));
+                        std::memcpy(&incr[at].y, &by, sizeof(by));
+                        continue;
+                    }
+                    const int na = k == T_BP ? s->namsite[lp[k][t]] : 0;
+                    const bool wide = na == 4 || na == 5;
+                    if (bulk[k]) {
+                        const int q = k == T_EX ? 0 : k == T_QQ ? 1 : k == T_SLV ? 2 : 3;
+                        int slot = 0;
+                        while (sl >= 0 && std::memcmp(&bpar[16 * q + slot], &par, sizeof(par)) != 0) slot++;
+                        incb[bfill[site]++] = (uint32_t)other | (uint32_t)q << 24 | (uint32_t)r << 26 |
+                                              (uint32_t)(wide ? 1 : 0) << 27 | (uint32_t)slot << 28;
+                        continue;
+                    }
+                    const int at = rfill[site]++;
+                    uint64_t bits = inc_pack(k, r, t);
                     if (ar[k] == 2) {
-                        bits |= (uint64_t)(uint32_t)lp[k][(int64_t)(1 - r) * cnt[k] + t] << 32;
-                        const int sl = pair_lval_slot(k);
-                        if (sl >= 0) par = vp[sl][t];
-                        if (k == T_BP) {
-                            const int na = s->namsite[lp[k][t]];
-                            if (na == 4 || na == 5) bits |= (uint64_t)1 << 63;
-                        }
+                        bits |= (uint64_t)(uint32_t)other << 32;
+                        if (wide) bits |= (uint64_t)1 << 63;
                     }
                     std::memcpy(&incr[at].x, &bits, sizeof(bits));
                     incr[at].y = par;
                 }
     }
     ACHK(dupload(c, &c->inc_ptr, deg.data(), deg.size()));
-    ACHK(dupload(c, &c->inc_split, split.data(), split.size()));
     ACHK(dupload(c, &c->inc, inc.data(), inc.size()));
+    ACHK(dupload(c, &c->incr_ptr, rptr.data(), rptr.size()));
+    ACHK(dupload(c, &c->inc_split, split.data(), split.size()));
     ACHK(dupload(c, &c->incr, incr.data(), incr.size()));
+    ACHK(dalloc(c, &c->fmul, incr.size()));
+    ACHK(dupload(c, &c->incb_ptr, bptr.data(), bptr.size()));
+    ACHK(dupload(c, &c->incb, incb.data(), incb.size()));
+    ACHK(dupload(c, &c->bpar, bpar.data(), bpar.size()));
+    {
+        std::vector<double2> cs(std::max(dna ? s->ndihe : 0, 1), make_double2(1.0, 0.0));
+        for (int t = 0; dna && t < s->ndihe; t++)
+            cs[t] = make_double2(std::cos(s->valdihe[t]), std::sin(s->valdihe[t]));
+        ACHK(dupload(c, &c->dihcs, cs.data(), cs.size()));
+    }
     if (s->flags & BROMOC_QPHIX)
         ACHK(dupload(c, &c->phix, (const float *)s->phix.data,
                      (size_t)s->phix.nclx * s->phix.ncly * s->phix.nclz));
@@ -822,6 +891,8 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.cg = c->cg; p.fact1a = c->fact1a; p.fact2a = c->fact2a; p.nwtype = c->nwtype;
     p.rth = s->rth; p.srpx = s->srpx; p.srpk = s->srpk; p.srpy = s->srpy;
     p.inc_ptr = c->inc_ptr; p.inc_split = c->inc_split; p.inc = c->inc; p.incr = c->incr;
+    p.incr_ptr = c->incr_ptr; p.incb_ptr = c->incb_ptr; p.incb = c->incb; p.bpar = c->bpar;
+    p.fmul = c->fmul; p.dihcs = c->dihcs;
     for (int k = 0; k < 8; k++) { p.lst[k] = c->lst[k]; p.nlist[k] = dna ? cnt[k] : 0; }
     for (int k = 0; k < 6; k++) p.lval[k] = c->lval[k];
     p.epsnuc = s->epsnuc; p.scalepairing = s->scalepairing; p.fctn = s->fctn;
@@ -2139,6 +2210,10 @@ struct AdjRun {
 
     StepParams step_params(int s) {
         StepParams p = params_for(c);
+        // one fixed launch shape (two targets per workgroup), not the autotuner's timed
+        // pick: every pass, and every context of the same system, replays the trajectory
+        // bitwise
+        p.tpb = 2;
         p.mode = 1;
         p.write_energy = 0;
         p.write_forces = 0;

@@ -771,58 +771,63 @@ __device__ inline void dna_pair(const StepParams &P, int kind, int role, V3 v1, 
     }
 }
 
-// One DNA term incident on site `me` in role `role`: force on `me` (added to f) and the
-// term's energy (added to e when role == 0), the sites read from the term lists.
-// energy.mono.f90:205-507.
-__device__ inline void dna_term(const StepParams &P, uint32_t w, bool free_me, V3 &f, double *e) {
+// An angle or dihedral term incident on site `me` (energy.mono.f90:240-334), from its
+// resolved record: x = word (kind | role | list index) | first other site << 32, y = the
+// second and third other sites (32 bits each); the term's sites in list order are the
+// others with `me` at position `role`.  Force on `me` (added to f) and the term's energy
+// (added to e when role == 0 and EN).  The angle's sin(theta) is sqrt((1 - c)(1 + c)) of
+// c = cos(theta); the dihedral's sin(phi - phi0) for the force comes from cos and sin of phi
+// (the two arguments of the reference's atan2 over their norm) and the stored cos and sin
+// of phi0 by the angle-difference identity, so a pass without energies calls acos alone.
+// The dihedral energy is formed as the reference forms it, 1 - cos(atan2(..) - phi0): near
+// phi0 that difference is rounding, which any other form would not reproduce.
+template <bool EN>
+__device__ inline void dna_multi(const StepParams &P, double2 rec, int me, bool free_me, V3 &f,
+                                 double *e) {
+    const uint64_t bx = (uint64_t)__double_as_longlong(rec.x);
+    const uint64_t by = (uint64_t)__double_as_longlong(rec.y);
+    const uint32_t w = (uint32_t)bx;
     const int kind = w >> 29, role = (w >> 27) & 3, t = w & 0x7FFFFFF;
+    const int o[3] = {(int)(bx >> 32), (int)(uint32_t)by, (int)(by >> 32)};
+    int sl[4];
+    for (int r = 0, q = 0; r < 4; r++) sl[r] = r == role ? me : o[q++];
     const double eps = P.epsnuc;
-    const int n = P.nlist[kind];
-    const int32_t *L = P.lst[kind];
-    if (kind != T_ANGLE && kind != T_DIHE) {
-        int a = L[t], b = L[n + t];
-        const V3 ra = ld3(P, a), rb = ld3(P, b);
-        V3 v1 = sub3(rb, ra);
-        const int sl = pair_lval_slot(kind);
-        const double par = sl >= 0 ? P.lval[sl][t] : 0.0;
-        const int na = kind == T_BP ? P.namsite[a] : 0;
-        const bool deby = kind == T_QQ && qq_debye(P, ra, rb);
-        dna_pair(P, kind, role, v1, par, na == 5 || na == 4, deby, free_me, f, e);
-        return;
-    }
     if (kind == T_ANGLE) {
-        int a = L[t], b = L[n + t], d = L[2 * n + t];
-        V3 rb = ld3(P, b);
-        V3 v1 = sub3(ld3(P, a), rb), v2 = sub3(ld3(P, d), rb);
-        double m1 = dot3(v1, v1), m2 = dot3(v2, v2), m3 = dot3(v1, v2);
-        double modval = 1.0 / sqrt(m1 * m2);
-        double ang = acos(m3 * modval);
-        double varang = ang - P.lval[1][t];
-        if (role == 0) e[D_ANG] += 700.0 * eps * (varang * varang);
+        const V3 rb = ld3(P, sl[1]);
+        const V3 v1 = sub3(ld3(P, sl[0]), rb), v2 = sub3(ld3(P, sl[2]), rb);
+        const double ang0 = P.lval[1][t];
+        const double m1 = dot3(v1, v1), m2 = dot3(v2, v2), m3 = dot3(v1, v2);
+        const double modval = 1.0 / sqrt(m1 * m2);
+        const double c = m3 * modval;
+        const double varang = acos(c) - ang0;
+        if (EN && role == 0) e[D_ANG] += 700.0 * eps * (varang * varang);
         if (free_me) {
-            double de = 1400.0 * eps * varang * modval / sin(ang);
-            V3 f1 = scl3(de, v1), f2 = scl3(de, v2);
+            const double de = 1400.0 * eps * varang * modval / sqrt((1.0 - c) * (1.0 + c));
+            const V3 f1 = scl3(de, v1), f2 = scl3(de, v2);
             if (role == 0) { f.x += f2.x; f.y += f2.y; f.z += f2.z; }
             else if (role == 1) { f.x -= f1.x + f2.x; f.y -= f1.y + f2.y; f.z -= f1.z + f2.z; }
             else { f.x += f1.x; f.y += f1.y; f.z += f1.z; }
         }
-    } else if (kind == T_DIHE) {
-        int i1 = L[t], i2 = L[n + t], i3 = L[2 * n + t], i4 = L[3 * n + t];
-        V3 r2 = ld3(P, i2), r3 = ld3(P, i3);
-        V3 v1 = sub3(ld3(P, i1), r2), v2 = sub3(r3, r2), v3 = sub3(r3, ld3(P, i4));
-        V3 n1 = cross3(v1, v2), n2 = cross3(v2, v3);
-        double v22 = dot3(v2, v2);
-        double v2v = sqrt(v22);
-        double av = dot3(v1, n2), bv = dot3(n1, n2);
-        double dihe = atan2(v2v * av, bv);
-        double vardihe = dihe - P.lval[2][t];
-        if (role == 0) e[D_DIHE] += 28.0 * eps * (1.0 - cos(vardihe));
+    } else {
+        const V3 r2 = ld3(P, sl[1]), r3 = ld3(P, sl[2]);
+        const V3 v1 = sub3(ld3(P, sl[0]), r2), v2 = sub3(r3, r2), v3 = sub3(r3, ld3(P, sl[3]));
+        const double2 cs0 = P.dihcs[t];   // cos and sin of the list's phi0
+        const V3 n1 = cross3(v1, v2), n2 = cross3(v2, v3);
+        const double v22 = dot3(v2, v2);
+        const double v2v = sqrt(v22);
+        const double yv = v2v * dot3(v1, n2), xv = dot3(n1, n2);
+        const double hv = sqrt(xv * xv + yv * yv);
+        const double cphi = hv > 0.0 ? xv / hv : 1.0, sphi = hv > 0.0 ? yv / hv : 0.0;
+        const double svar = sphi * cs0.x - cphi * cs0.y;   // sin(phi - phi0)
+        if constexpr (EN) {
+            if (role == 0) e[D_DIHE] += 28.0 * eps * (1.0 - cos(atan2(yv, xv) - P.lval[2][t]));
+        }
         if (free_me) {
-            double n1v = dot3(n1, n1), n2v = dot3(n2, n2);
+            const double n1v = dot3(n1, n1), n2v = dot3(n2, n2);
             double v12 = dot3(v1, v2), v23 = dot3(v2, v3);
-            double de = 28.0 * eps * sin(vardihe) * v2v;
-            V3 f1 = scl3(-de / n1v, n1), f4 = scl3(de / n2v, n2);
-            double iv22 = 1.0 / v22;
+            const double de = 28.0 * eps * svar * v2v;
+            const V3 f1 = scl3(-de / n1v, n1), f4 = scl3(de / n2v, n2);
+            const double iv22 = 1.0 / v22;
             v12 = v12 * iv22;
             v23 = v23 * iv22;
             V3 g;

@@ -95,12 +95,23 @@ struct StepParams {
     const int32_t *nwtype;
     double rth, srpx, srpk, srpy;
     // CG-DNA
-    const int32_t *inc_ptr;  // [nsites+1]
-    const int32_t *inc_split;  // [nsites] first two-site record of each site (angles, dihedrals before)
+    const int32_t *inc_ptr;  // [nsites+1] each site's term incidences in inc (INTERACT)
     const uint32_t *inc;
-    // the same incidences resolved: bits of x = word | other site << 32 | (base pair of a
-    // C/G site a) << 63 (other site 0 for angles and dihedrals), y = the term's list value
+    // resolved incidences for the force kernels, the site's angles and dihedrals first
+    // (bits of x = word | first other site << 32, bits of y = second | third other site
+    // << 32, the other sites in list order), then the two-site terms (bits of x = word |
+    // other site << 32 | (base pair of a C/G site a) << 63, y = the term's list value)
+    const int32_t *incr_ptr;   // [nsites+1]
+    const int32_t *inc_split;  // [nsites] first two-site record of each site in incr
     const double2 *incr;
+    double4 *fmul;             // [records of incr] angle / dihedral results of k_dna_multi
+    // one-word records of the bulk two-site terms: other site (bits 0-23), term (24-25:
+    // excluded volume, P-P, solvent, base pair), role (26), C/G site a of a base pair (27),
+    // slot of the list value in bpar[16 * term + slot] (28-31)
+    const int32_t *incb_ptr;   // [nsites+1]
+    const uint32_t *incb;
+    const double *bpar;        // [4 * 16]
+    const double2 *dihcs;      // [ndihe] cos and sin of each dihedral's phi0
     const int32_t *lst[8];   // component-major site lists
     const double *lval[6];
     int32_t nlist[8];

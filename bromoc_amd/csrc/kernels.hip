@@ -81,24 +81,53 @@ __device__ inline double contrans_force(const StepParams &P, int i, int ax, doub
 }
 
 // CG-DNA list terms of site i (energy.mono.f90:196-508): this thread's share -- records
-// tid, tid + NB, ... of the site's incidence list -- of the force on the site and of the
-// energies of the terms it heads (e[8], slots D_BOND..D_SOLV).  The site's angle and
-// dihedral records come first; its two-site records, from inc_split on, are taken DU at a
-// time: the records, then the other sites' positions, then the terms, so DU loads of each
-// kind are in flight together.
+// tid, tid + NB, ... of the site's incidence lists -- of the force on the site and of the
+// energies of the terms it heads (e[8], slots D_BOND..D_SOLV; formed as the reference forms
+// them with EN, left incomplete without).  The site's angle and
+// dihedral records come first; its other records, from inc_split on and then the one-word
+// bulk records, are taken DU at a time: the records, then the other sites' positions, then
+// the terms, so DU loads of each kind are in flight together.
 // records per thread in flight when k_step computes a site's terms
 constexpr int DNA_DU = 2;
 
-template <int NB, int DU>
+// The angle and dihedral terms (energy.mono.f90:240-334) of the sites' incidences, one
+// wavefront per site, lane by lane over the site's records: the force on the site and the
+// term's energy (for the record of its first site) into fmul at the record's slot, for
+// k_dna, whose own registers then stay few enough for a full complement of wavefronts.
+__global__ __launch_bounds__(WAVE) void k_dna_multi(StepParams P) {
+    const int i = blockIdx.x, lane = threadIdx.x;
+    const bool free_me = P.stfree[i] != 0;
+    const int k0 = P.incr_ptr[i], ks = P.inc_split[i];
+    for (int k = k0 + lane; k < ks; k += WAVE) {
+        V3 f = {0.0, 0.0, 0.0};
+        double e[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) e[q] = 0.0;
+        if (P.write_energy) dna_multi<true>(P, P.incr[k], i, free_me, f, e);
+        else dna_multi<false>(P, P.incr[k], i, free_me, f, e);
+        P.fmul[k] = make_double4(f.x, f.y, f.z, e[D_ANG] + e[D_DIHE]);
+    }
+}
+
+template <int NB, int DU, bool MULTI, bool EN>
 __device__ inline void dna_site_terms(const StepParams &P, int i, int tid, V3 &f, double e[8]) {
 #pragma unroll
     for (int k = 0; k < 8; k++) e[k] = 0.0;
     const bool free_me = P.stfree[i] != 0;
-    const int k0 = P.inc_ptr[i], ks = P.inc_split[i], k1 = P.inc_ptr[i + 1];
+    const int k0 = P.incr_ptr[i], ks = P.inc_split[i], k1 = P.incr_ptr[i + 1];
+    const int b0 = P.incb_ptr[i], b1 = P.incb_ptr[i + 1];
     const V3 rme = ld3(P, i);
     if (P.flags & QNOBOND) {
-        for (int k = k0 + tid; k < ks; k += NB)
-            dna_term(P, (uint32_t)__double_as_longlong(P.incr[k].x), free_me, f, e);
+        for (int k = k0 + tid; k < ks; k += NB) {
+            if constexpr (MULTI) {
+                dna_multi<EN>(P, P.incr[k], i, free_me, f, e);
+            } else {   // evaluated by k_dna_multi
+                const uint32_t w = (uint32_t)__double_as_longlong(P.incr[k].x);
+                const double4 m = P.fmul[k];
+                f.x += m.x; f.y += m.y; f.z += m.z;
+                e[(w >> 29) == T_ANGLE ? D_ANG : D_DIHE] += m.w;
+            }
+        }
     }
     for (int kb = ks + tid; kb < k1; kb += DU * NB) {
         double2 rec[DU];
@@ -128,22 +157,51 @@ __device__ inline void dna_site_terms(const StepParams &P, int i, int tid, V3 &f
             dna_pair(P, kind, role, v1, rec[u].y, (bits >> 63) != 0, deby, free_me, f, e);
         }
     }
+    // the bulk two-site terms, one word per record
+    if (!(P.flags & QNONBOND)) return;
+    for (int kb = b0 + tid; kb < b1; kb += DU * NB) {
+        uint32_t rec[DU];
+        V3 ro[DU];
+#pragma unroll
+        for (int u = 0; u < DU; u++) {
+            const int k = kb + u * NB;
+            rec[u] = P.incb[k < b1 ? k : b0];
+        }
+#pragma unroll
+        for (int u = 0; u < DU; u++) ro[u] = ld3(P, (int)(rec[u] & 0xFFFFFFu));
+#pragma unroll
+        for (int u = 0; u < DU; u++) {
+            if (kb + u * NB >= b1) break;
+            const uint32_t w = rec[u];
+            const int q = (w >> 24) & 3;
+            const int kind = q == 0 ? T_EX : q == 1 ? T_QQ : q == 2 ? T_SLV : T_BP;
+            if (kind == T_SLV && !(P.flags & QSOLV)) continue;
+            const int role = (w >> 26) & 1;
+            const V3 v1 = role == 0 ? sub3(ro[u], rme) : sub3(rme, ro[u]);   // r_b - r_a
+            const bool deby = kind == T_QQ && qq_debye(P, rme, ro[u]);
+            dna_pair(P, kind, role, v1, P.bpar[16 * q + (w >> 28)], ((w >> 27) & 1) != 0, deby,
+                     free_me, f, e);
+        }
+    }
 }
 
-// The wavefront sums of dna_site_terms' outputs into s_r[wv][0..10] (force, then energies).
-template <int NW>
+// The wavefront sums of dna_site_terms' outputs into s_r[wv][0..10] (force, then, with EN,
+// energies).
+template <int NW, bool EN>
 __device__ inline void dna_block_sums(const StepParams &P, int lane, int wv, V3 f,
                                       double e[8], double (*s_r)[3 + 8]) {
     f.x = wave_sum(f.x);
     f.y = wave_sum(f.y);
     f.z = wave_sum(f.z);
-    if (P.write_energy) {
+    if (EN && P.write_energy) {
 #pragma unroll
         for (int k = 0; k < 8; k++) e[k] = wave_sum(e[k]);
     }
     if (lane == 0) {
         s_r[wv][0] = f.x; s_r[wv][1] = f.y; s_r[wv][2] = f.z;
-        for (int k = 0; k < 8; k++) s_r[wv][3 + k] = e[k];
+        if constexpr (EN) {
+            for (int k = 0; k < 8; k++) s_r[wv][3 + k] = e[k];
+        }
     }
 }
 
@@ -166,9 +224,9 @@ __device__ inline double dna_site_finish(const StepParams &P, int i, int c,
 // evaluates each partner against all T targets.  Per-target sums: butterfly within each
 // wavefront, then a fixed-order sum over the NW wavefronts in LDS.  Wavefront w < T also
 // handles target w's 8 grid corners; thread a < T finishes target a (force, energies,
-// BD update).  A one-target block computes its site target's CG-DNA list terms itself
-// (dna_site_terms, before the pair loop); blocks of several targets read them from k_dna
-// (fdna).
+// BD update).  In a launch that writes no energies (the step loop) a one-target block
+// computes its site target's CG-DNA list terms itself (dna_site_terms, before the pair
+// loop); otherwise the site targets read them from k_dna (fdna).
 template <int FEAT, int T, int NW>
 __global__ __launch_bounds__(NW * WAVE) __attribute__((amdgpu_waves_per_eu(T == 1 ? 5 : 1)))
 void k_step(StepParams P) {
@@ -208,15 +266,17 @@ void k_step(StepParams P) {
     }
     __syncthreads();
 
-    // CG-DNA list terms of a site target (the block-uniform branch of a one-target block)
+    // CG-DNA list terms of a site target (the block-uniform branch of a one-target block),
+    // in the launches that write no energies; the others read k_dna's (fdna)
     constexpr bool DNA_HERE = (FEAT & F_DNA) != 0 && T == 1;
+    const bool dna_here = DNA_HERE && !P.write_energy;
     __shared__ double s_dn[DNA_HERE ? NW : 1][3 + 8];
     if constexpr (DNA_HERE) {
-        if (tval[0] && tsite[0]) {
+        if (dna_here && tval[0] && tsite[0]) {
             V3 f = {0.0, 0.0, 0.0};
             double e[8];
-            dna_site_terms<NB, DNA_DU>(P, ti[0], tid, f, e);
-            dna_block_sums<NW>(P, lane, wv, f, e, s_dn);
+            dna_site_terms<NB, DNA_DU, true, false>(P, ti[0], tid, f, e);
+            dna_block_sums<NW, false>(P, lane, wv, f, e, s_dn);
         }
     }
 
@@ -249,15 +309,19 @@ void k_step(StepParams P) {
         // partners per iteration: three for one-target blocks (the small-system shape, where
         // the loop is latency-bound), two otherwise (measured best for each shape)
         constexpr int PU = T == 1 ? 3 : 2;
+        // the next iteration's partners are loaded while this one's are evaluated
+        double4 nx[PU];
+#pragma unroll
+        for (int u = 0; u < PU; u++) nx[u] = P.spos[min(tid + u * NB, ntot - 1)];
         for (int k0 = tid; k0 < ntot; k0 += PU * NB) {
-            // partners in spatial order, PU per iteration (PU load chains in flight)
+            // partners in spatial order, PU per iteration
             double4 pp[PU];
             bool pok[PU];
 #pragma unroll
             for (int u = 0; u < PU; u++) {
-                const int k = k0 + u * NB;
-                pok[u] = k < ntot;
-                pp[u] = P.spos[pok[u] ? k : k0];
+                pp[u] = nx[u];
+                pok[u] = k0 + u * NB < ntot;
+                nx[u] = P.spos[min(k0 + (PU + u) * NB, ntot - 1)];
             }
 #pragma unroll
             for (int u = 0; u < PU; u++) {
@@ -425,13 +489,11 @@ void k_step(StepParams P) {
         for (int q = 0; q < NW; q++) acc += s_red[q][a][k];
         F[k] = acc + s_dg[a][k];
     }
-    if constexpr (DNA_HERE) {
-        if (is_site) {
+    if constexpr ((FEAT & F_DNA) != 0) {
+        if (is_site && dna_here) {
 #pragma unroll
             for (int k = 0; k < 3; k++) F[k] += dna_site_finish<NW>(P, i, k, s_dn);
-        }
-    } else if constexpr ((FEAT & F_DNA) != 0) {
-        if (is_site) {
+        } else if (is_site) {
             const double4 fd = P.fdna[i];   // CG-DNA list terms from k_dna
             F[0] += fd.x; F[1] += fd.y; F[2] += fd.z;
         }
@@ -459,15 +521,9 @@ void k_step(StepParams P) {
             for (int q = 0; q < NW; q++) acc += s_red[q][a][3 + k];
             pe[k] = acc;
         }
-        // this kernel owns the pair and grid categories, and the DNA ones when it computed
-        // them; otherwise k_dna writes those
+        // this kernel owns the pair and grid categories; k_dna writes the DNA ones
         double *ep_ = P.epart + i;
         const size_t cs = P.ecap;
-        if constexpr (DNA_HERE) {
-            if (is_site) {
-                for (int k = 0; k < 8; k++) ep_[ND_CAT[k] * cs] = dna_site_finish<NW>(P, i, 3 + k, s_dn);
-            }
-        }
         ep_[E_GSBPA * cs] = s_dg[a][3];
         ep_[E_VDWGD * cs] = s_dg[a][4];
         if (is_site) {
@@ -610,8 +666,8 @@ hipError_t launch_notrans(const StepParams &p, hipStream_t s) {
 }
 
 // k_dna: CG-DNA bonded and list terms (energy.mono.f90:196-508), one block per site, for
-// the pair kernels that do not compute them in their site targets' blocks.  Sums go
-// butterfly-then-LDS in a fixed order.  The force on the site goes to fdna[i] (zero for a
+// the pair kernels that do not compute them in their site targets' blocks; the angles and
+// dihedrals come from k_dna_multi (fmul).  Sums go butterfly-then-LDS in a fixed order.  The force on the site goes to fdna[i] (zero for a
 // fixed site); the energies of the terms the site heads go to epart.
 template <int NW>
 __global__ __launch_bounds__(NW * WAVE) void k_dna(StepParams P) {
@@ -621,8 +677,8 @@ __global__ __launch_bounds__(NW * WAVE) void k_dna(StepParams P) {
     const int i = blockIdx.x;
     V3 f = {0.0, 0.0, 0.0};
     double e[8];
-    dna_site_terms<NB, 4>(P, i, tid, f, e);
-    dna_block_sums<NW>(P, lane, wv, f, e, s_r);
+    dna_site_terms<NB, 4, false, true>(P, i, tid, f, e);
+    dna_block_sums<NW, true>(P, lane, wv, f, e, s_r);
     __syncthreads();
     if (tid < 3 + 8) {
         const double acc = dna_site_finish<NW>(P, i, tid, s_r);
@@ -867,9 +923,10 @@ static void dispatch_prox(const StepParams &p, int feat, hipStream_t s) {
 }
 
 // Whether the pair kernel launch_pairs picks computes the CG-DNA list terms of its site
-// targets itself: the one-target k_step (tpb 1, and every DYNAMICS3 step).
+// targets itself: the one-target k_step (tpb 1, and every DYNAMICS3 step) in a launch that
+// writes no energies.
 static bool dna_in_pairs(const StepParams &p) {
-    if (tile_mode(p.tpb)) return false;
+    if (tile_mode(p.tpb) || p.write_energy) return false;
     if (p.dmode == 3 && p.mode == 1) return true;
     return p.tpb == 1;
 }
@@ -877,7 +934,10 @@ static bool dna_in_pairs(const StepParams &p) {
 hipError_t launch_dna(const StepParams &p, hipStream_t s) {
     if (!(p.flags & QNUCL) || p.nsites <= 0) return hipSuccess;
     if (p.flags & QCONTRANS) hipLaunchKernelGGL(k_centroid, dim3(1), dim3(BLOCK), 0, s, p);
-    if (!dna_in_pairs(p)) hipLaunchKernelGGL(k_dna<WPB>, dim3(p.nsites), dim3(BLOCK), 0, s, p);
+    if (!dna_in_pairs(p)) {
+        if (p.flags & QNOBOND) hipLaunchKernelGGL(k_dna_multi, dim3(p.nsites), dim3(WAVE), 0, s, p);
+        hipLaunchKernelGGL(k_dna<WPB>, dim3(p.nsites), dim3(BLOCK), 0, s, p);
+    }
     return hipGetLastError();
 }
 

