@@ -239,7 +239,8 @@ __global__ __launch_bounds__(SMALL_SORT_THREADS) void k_sort_small(
     __shared__ uint32_t s_k[2][SMALL_SORT_MAX];
     __shared__ int32_t s_v[2][SMALL_SORT_MAX];
     __shared__ int32_t s_run[SORT_RADIX];
-    __shared__ int32_t s_wc[NWV][SORT_RADIX];
+    __shared__ int32_t s_wc[NWV][SORT_RADIX];   // a tile's digit counts per wavefront
+    __shared__ int32_t s_wo[NWV][SORT_RADIX];   // and the slots their runs start at
     __shared__ int32_t s_tot[NWV];
     const int tid = threadIdx.x, lane = tid & (SWAVE - 1), wv = tid / SWAVE;
     for (int i = tid; i < n; i += SMALL_SORT_THREADS) {
@@ -294,17 +295,22 @@ __global__ __launch_bounds__(SMALL_SORT_THREADS) void k_sort_small(
             const int r = __popcll(peers & lt);
             if (ok && r == 0) s_wc[wv][d] = __popcll(peers);
             __syncthreads();
-            if (ok) {
-                int before = s_run[d] + r;
-                for (int w = 0; w < wv; w++) before += s_wc[w][d];
-                s_k[cur ^ 1][before] = key;
-                s_v[cur ^ 1][before] = val;
+            // per digit, in wavefront order: the slot where each wavefront's run of it
+            // starts; the counts are cleared for the next tile
+            for (int dd = tid; dd < SORT_RADIX; dd += SMALL_SORT_THREADS) {
+                int run = s_run[dd];
+                for (int w = 0; w < NWV; w++) {
+                    s_wo[w][dd] = run;
+                    run += s_wc[w][dd];
+                    s_wc[w][dd] = 0;
+                }
+                s_run[dd] = run;
             }
             __syncthreads();
-            for (int dd = tid; dd < SORT_RADIX; dd += SMALL_SORT_THREADS) {
-                int t = 0;
-                for (int w = 0; w < NWV; w++) { t += s_wc[w][dd]; s_wc[w][dd] = 0; }
-                s_run[dd] += t;
+            if (ok) {
+                const int at = s_wo[wv][d] + r;
+                s_k[cur ^ 1][at] = key;
+                s_v[cur ^ 1][at] = val;
             }
             __syncthreads();
         }
