@@ -30,11 +30,12 @@ METRIC = "particle-steps/s + % HBM roofline, batched vs plain, FP64 rel err<=1e-
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (KiB per dispatch, one counter per run) on each workload and launch shape,
-# scaled by what the same counters read for a 256 MiB device copy (bench.py --copy-probe
-# 256): FETCH_SIZE x 2.0, WRITE_SIZE x 1.0.  The counters cannot be read from inside this
-# process; a shape without a measurement reports null.
-TRAFFIC_KIB = {("translocation", 18): (448291.2, 960598.1)}
+# WRITE_SIZE (KiB per dispatch) on each workload and launch shape (the pore_dna step kernel
+# with BROMOC_GRAPH=0, so that its launches are counted outside the graphs), scaled by what
+# the same counters read for a 256 MiB device copy (bench.py --copy-probe 256): FETCH_SIZE
+# x 2.0, WRITE_SIZE x 1.0.  The counters cannot be read from inside this process; a shape
+# without a measurement reports null.
+TRAFFIC_KIB = {("translocation", 18): (378440.4, 732727.8), ("pore_dna", 1): (3541.9, 215.0)}
 TRAFFIC_SCALE = (2.0, 1.0)
 # FP64 operations per pair evaluation: a table pair ~34 (r^2 8, sqrt 1, index 2, U 4, F/r 3,
 # energy 1, force update on both particles ~15); the reaction-field cross term of an ion
