@@ -1308,9 +1308,11 @@ int bromoc_step(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t first
     return sync_err(c);
 }
 
-// nsteps BD steps with a HIP event pair around every launch of the step kernel k_step, on
-// the stream the kernels run on: *total_ms is first-to-last, *kernel_ms the mean k_step
-// duration (k_dna, launched before each k_step for CG-DNA systems, is outside the pair).
+// nsteps BD steps with a HIP event pair around every launch of the pair kernel (k_step,
+// k_wave or, in tile mode, k_tiles), on the stream the kernels run on: *total_ms is
+// first-to-last, *kernel_ms the mean pair-kernel duration.  A separate k_dna launch (tile
+// mode, several targets per block) falls outside the pair; the one-target k_step's own
+// CG-DNA terms fall inside it.
 int bromoc_time_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t first_row,
                       uint64_t seed, uint64_t counter0, double *total_ms, double *kernel_ms) {
     if (!c || nsteps <= 0 || !total_ms || !kernel_ms) return BROMOC_E_ARG;

@@ -1,15 +1,15 @@
 // kernels.hip -- gfx950 kernels of the BROMOC force / Brownian-dynamics step of one system.
 //
 // One BD step is k_rfpar/k_rfpair (reaction-field parameters, when present), the CG-DNA
-// list terms of each site (energy.mono.f90:196-508; in k_step's own one-target blocks, else
-// k_dna) and k_step (or k_wave): every target
-// particle -- each free DNA site and each moving ion -- gathers its pair forces over all
-// other particles (the reference's ENERGY has no cutoff: energy.mono.f90:117-194,584-687),
-// its 8 trilinear corners of the static-field and repulsion grids (staticf1.f90:45-118,
-// vdwgd1trln.f90:50-161), then applies the Ermak-McCammon update (DYNAMICS0/1/2,
-// DYNAMICS0NUC) into the other position buffer.  Every force is a gather over partners in
-// a fixed order: results are deterministic and need no atomics; energies are per-particle
-// partials reduced in a fixed tree.
+// list terms of each site (energy.mono.f90:196-508; in the one-target k_step's own site
+// blocks when the launch writes no energies, else k_dna_multi + k_dna) and k_step (or
+// k_wave): every target particle -- each free DNA site and each moving ion -- gathers its
+// pair forces over all other particles (the reference's ENERGY has no cutoff:
+// energy.mono.f90:117-194,584-687), its 8 trilinear corners of the static-field and
+// repulsion grids (staticf1.f90:45-118, vdwgd1trln.f90:50-161), then applies the
+// Ermak-McCammon update (DYNAMICS0/1/2, DYNAMICS0NUC) into the other position buffer.
+// Every force is a gather over partners in a fixed order: results are deterministic and
+// need no atomics; energies are per-particle partials reduced in a fixed tree.
 #include <type_traits>
 
 #include "device.h"

@@ -477,17 +477,43 @@ __global__ __launch_bounds__(BLOCK) void k_tfin(StepParams P) {
     const int TT = WAVE * P.ttpl, n = P.ntile, NC = RF ? 5 : 3;
     const int I = ss / TT, t = ss - I * TT;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int K = sub; K < n; K += 8) {
-        const double *src = P.tpart + ((size_t)(K * n + I) * NC) * TT + t;
-        // the A and B rows only where the tile pair (min, max) wrote them
-        int nc = 3;
-        if constexpr (RF) {
-            const int lo = min(I, K), hi = max(I, K);
-            nc = P.tflag[hi * (hi + 1) / 2 + lo] ? 5 : 3;
+    // rows K = sub, sub + 8, .. added in that order, KU of them per iteration so that their
+    // loads (the flags, then the rows) are in flight together
+    constexpr int KU = 4;
+    for (int K0 = sub; K0 < n; K0 += 8 * KU) {
+        bool ok[KU], ab[KU];
+        const double *src[KU];
+#pragma unroll
+        for (int u = 0; u < KU; u++) {
+            const int K = K0 + 8 * u;
+            ok[u] = K < n;
+            const int Kc = ok[u] ? K : I;
+            src[u] = P.tpart + ((size_t)(Kc * n + I) * NC) * TT + t;
+            ab[u] = false;
+            // the A and B rows only where the tile pair (min, max) wrote them
+            if constexpr (RF) {
+                const int lo = min(I, Kc), hi = max(I, Kc);
+                const int32_t fl = P.tflag[hi * (hi + 1) / 2 + lo];   // a valid item even when !ok
+                ab[u] = ok[u] & (fl != 0);
+            }
+        }
+        double r[KU][5];
+#pragma unroll
+        for (int u = 0; u < KU; u++) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) r[u][c] = src[u][(size_t)c * TT];
+            r[u][3] = r[u][4] = 0.0;
+            if constexpr (RF) {
+                if (ab[u]) { r[u][3] = src[u][3 * (size_t)TT]; r[u][4] = src[u][4 * (size_t)TT]; }
+            }
         }
 #pragma unroll
-        for (int c = 0; c < NC; c++)
-            if (c < nc) acc[c] += src[(size_t)c * TT];
+        for (int u = 0; u < KU; u++) {
+            if (!ok[u]) continue;
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                if (c < 3 || ab[u]) acc[c] += r[u][c];
+        }
     }
 #pragma unroll
     for (int c = 0; c < NC; c++) acc[c] = group8_sum(acc[c]);
