@@ -8,7 +8,8 @@
 //                  current positions; an accepted attempt updates the one slot it changes;
 //   k_gc_pre       one workgroup: the first trial, drawn from the uniform stream in the
 //                  reference's order; for a deletion FIND, the ip-th ion of the buffer
-//                  (per-thread segment counts, a block scan, one segment searched);
+//                  (ballot counts of 64 consecutive slots per wavefront, by groups; the
+//                  wavefront that holds it searches one group);
 // then per attempt
 //   k_gc_interact  G workgroups: INTERACT of the trial ion (interact.f90:19-419 for an ion:
 //                  MEMBRANE, grids, ion and site pairs, RFPARIONJ), one partial sum per
@@ -24,6 +25,9 @@
 
 namespace bromoc {
 
+constexpr int FIND_GROUP = 512;   // slots per group of FIND's counts (at least)
+constexpr int FIND_MAXG = 256;    // groups per wavefront
+
 // The next trial (grand.f90:42-50, 77-84) and, for a deletion, FIND (find.f90:26-32): the
 // ip-th slot, in slot order, of the moving ions whose buffer is ib.  Called by a whole
 // workgroup.
@@ -32,6 +36,7 @@ __device__ inline void gc_pre(const GcParams &G) {
     // rounds differently from the reference's separate operations
 #pragma clang fp contract(off)
     __shared__ int s_cnt[WPB];
+    __shared__ int s_grp[WPB][FIND_MAXG];
     __shared__ int s_found;
     GcState &S = *G.st;
     if (S.err) return;
@@ -79,28 +84,52 @@ __device__ inline void gc_pre(const GcParams &G) {
     }
     __syncthreads();
     if (S.mode != 2) return;
-    // thread t counts the buffer's ions in its contiguous segment of the moving slots; a
-    // block scan of the counts names the segment holding the ip-th, which its thread scans
+    // Wavefront w takes the contiguous w-th part of the moving slots and counts the buffer's
+    // ions in it by ballots over 64 consecutive slots, eight loads in flight, keeping the
+    // count of every group of FIND_GROUP slots; the wavefront whose part holds the ip-th
+    // walks its group counts to the group, then the group's chunks to the slot.
     const int ib = S.ib, ip = S.ip, ntot = S.ntot;
     const int first = G.sp.nsites + G.sp.nfix, cnt = ntot > first ? ntot - first : 0;
-    const int seg = (cnt + BLOCK - 1) / BLOCK;
-    const int lo = first + min(tid * seg, cnt), hi = first + min((tid + 1) * seg, cnt);
+    int gsz = FIND_GROUP;   // slots per group: a multiple of 512, at most FIND_MAXG groups
+    while ((cnt + WPB * gsz - 1) / (WPB * gsz) > FIND_MAXG) gsz *= 2;
+    const int ngw = (cnt + WPB * gsz - 1) / (WPB * gsz);   // groups per wavefront
+    const int lo = first + min(wv * ngw * gsz, cnt), hi = first + min((wv + 1) * ngw * gsz, cnt);
     int c = 0;
-    for (int a = lo; a < hi; a++) c += G.ibuffer[a] == ib ? 1 : 0;
-    int incl = c;
+    for (int g = 0; g < ngw; g++) {
+        int cg = 0;
+        const int g0 = lo + g * gsz, g1 = min(g0 + gsz, hi);
+        for (int base = g0; base < g1; base += 8 * WAVE) {
+            bool m[8];
 #pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) {
-        const int v = __shfl_up(incl, off, WAVE);
-        if (lane >= off) incl += v;
+            for (int u = 0; u < 8; u++) {
+                const int a = base + u * WAVE + lane;
+                m[u] = a < g1 && G.ibuffer[a] == ib;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) cg += __popcll(__ballot(m[u]));
+        }
+        if (lane == 0) s_grp[wv][g] = cg;
+        c += cg;
     }
-    if (lane == WAVE - 1) s_cnt[wv] = incl;
+    if (lane == 0) s_cnt[wv] = c;
     __syncthreads();
-    int before = incl - c;
+    int before = 0;
     for (int q = 0; q < wv; q++) before += s_cnt[q];
-    if (c > 0 && ip > before && ip <= before + c) {
-        int need = ip - before;
-        for (int a = lo; a < hi; a++)
-            if (G.ibuffer[a] == ib && --need == 0) { s_found = a; break; }
+    if (ip > before && ip <= before + c) {   // the same for the whole wavefront
+        int need = ip - before, g = 0;
+        while (need > s_grp[wv][g]) need -= s_grp[wv][g++];
+        const int g0 = lo + g * gsz, g1 = min(g0 + gsz, hi);
+        for (int base = g0; base < g1; base += WAVE) {
+            const int a = base + lane;
+            const uint64_t m = __ballot(a < g1 && G.ibuffer[a] == ib);
+            const int k = __popcll(m);
+            if (need <= k) {   // the need-th of this chunk's matches, in lane order
+                if (a < g1 && G.ibuffer[a] == ib && __popcll(m & ((1ull << lane) - 1ull)) == need - 1)
+                    s_found = a;
+                break;
+            }
+            need -= k;
+        }
     }
     __syncthreads();
     if (tid == 0) {
