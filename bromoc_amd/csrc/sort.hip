@@ -264,6 +264,11 @@ __global__ __launch_bounds__(SMALL_SORT_THREADS) void k_sort_small(
             if (ok && __popcll(peers & lt) == 0) atomicAdd(&s_run[d], __popcll(peers));   // integer: exact
         }
         __syncthreads();
+        // a pass whose digit is the same for every key leaves the order as it is (every
+        // thread reads the count before any thread changes s_run again)
+        const bool same = s_run[(s_k[cur][0] >> shift) & (SORT_RADIX - 1)] == n;
+        __syncthreads();
+        if (same) continue;
         if (wv == 0) {   // exclusive scan of the 256 digit counts, four per lane
             int c[SORT_RADIX / SWAVE], sum = 0;
 #pragma unroll
@@ -316,10 +321,10 @@ __global__ __launch_bounds__(SMALL_SORT_THREADS) void k_sort_small(
         }
         cur ^= 1;
     }
-    // an even number of passes leaves the sorted keys and particles in s_k[0], s_v[0]
+    // the sorted keys and particles are in s_k[cur], s_v[cur]
     for (int k = tid; k < n; k += SMALL_SORT_THREADS) {
-        const int j = s_v[0][k];
-        keys_out[k] = s_k[0][k];
+        const int j = s_v[cur][k];
+        keys_out[k] = s_k[cur][k];
         order[k] = j;
         rank[j] = k;
         const double4 p = pos[j];
@@ -331,7 +336,7 @@ __global__ __launch_bounds__(SMALL_SORT_THREADS) void k_sort_small(
 #pragma unroll
     for (int q = 0; q < SMALL_SORT_PER; q++) {
         const int sl = SMALL_SORT_PER * tid + q;
-        f[q] = (sl < n && (sl == 0 || s_k[0][sl] != s_k[0][sl - 1])) ? 1 : 0;
+        f[q] = (sl < n && (sl == 0 || s_k[cur][sl] != s_k[cur][sl - 1])) ? 1 : 0;
         sum += f[q];
     }
     int incl = sum;
