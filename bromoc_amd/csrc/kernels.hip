@@ -306,9 +306,9 @@ void k_step(StepParams P) {
     const bool nonbond = (P.flags & QNONBOND) && (P.flags & QPAR);
     auto pair_loop = [&](auto en) {
         constexpr bool EN = decltype(en)::value;
-        // partners per iteration: three for one-target blocks (the small-system shape, where
-        // the loop is latency-bound), two otherwise (measured best for each shape)
-        constexpr int PU = T == 1 ? 3 : 2;
+        // partners per iteration: two (with the next iteration's partners loaded during
+        // this one's, measured best for every shape; three or four only added registers)
+        constexpr int PU = 2;
         // the next iteration's partners are loaded while this one's are evaluated
         double4 nx[PU];
 #pragma unroll
