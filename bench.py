@@ -36,6 +36,9 @@ FP64_PEAK_TFLOPS = 78.6
 # x 2.0, WRITE_SIZE x 1.0.  The counters cannot be read from inside this process; a shape
 # without a measurement reports null.
 TRAFFIC_KIB = {("translocation", 18): (378440.4, 732727.8), ("pore_dna", 1): (3541.9, 215.0)}
+# L2 hit rate of the same launches, TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum) from the
+# WRITE_SIZE run's two L2 counters
+L2_HIT = {("translocation", 18): 0.777, ("pore_dna", 1): 0.930}
 TRAFFIC_SCALE = (2.0, 1.0)
 # FP64 operations per pair evaluation: a table pair ~34 (r^2 8, sqrt 1, index 2, U 4, F/r 3,
 # energy 1, force update on both particles ~15); the reaction-field cross term of an ion
@@ -114,13 +117,15 @@ def roofline(workload, tpb, pc, kernel_ms, bpp, nmov):
     achieved = the algorithmic FP64 operations of one launch (each pair once, at FLOP_PAIR /
     FLOP_RF) over its event-timed duration, against MI355X's FP64 peak; the HBM figures
     beside it (bpp algorithmic bytes per moving particle) and, when counted, traffic = the
-    HBM bytes of one launch from FETCH_SIZE / WRITE_SIZE."""
+    HBM bytes of one launch from FETCH_SIZE / WRITE_SIZE, its rate over the same duration,
+    and the launch's L2 hit rate."""
     f = fp64_rate(pc, kernel_ms)
     traffic = None
     if TRAFFIC_KIB.get((workload, tpb)):
         fk, wk = TRAFFIC_KIB[(workload, tpb)]
         traffic = 1024.0 * (fk * TRAFFIC_SCALE[0] + wk * TRAFFIC_SCALE[1])
     hbm = bpp * nmov / (kernel_ms * 1e-3) / 1e9
+    traffic_gbs = traffic / (kernel_ms * 1e-3) / 1e9 if traffic is not None else None
     return {"bound": "mfma", "achieved": f["achieved_tflops"], "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": f["frac"], "traffic": traffic,
             "pipe": "FP64 VALU (pair math is not GEMM-shaped; MI355X's FP64 vector and "
@@ -129,7 +134,8 @@ def roofline(workload, tpb, pc, kernel_ms, bpp, nmov):
             "launch_shape": TPB_NAMES.get(tpb, str(tpb)), "targets_per_block": tpb,
             "flops_per_launch": f["flops_per_step"], "pairs_per_launch": pc,
             "hbm": {"achieved_gbs": hbm, "peak_gbs": HBM_PEAK_GBS, "frac": hbm / HBM_PEAK_GBS,
-                    "bytes_per_particle_step": bpp, "algorithmic_bytes_per_launch": bpp * nmov}}
+                    "bytes_per_particle_step": bpp, "algorithmic_bytes_per_launch": bpp * nmov,
+                    "traffic_gbs": traffic_gbs, "l2_hit_rate": L2_HIT.get((workload, tpb))}}
 
 
 def cpu_info():

@@ -479,7 +479,7 @@ __global__ __launch_bounds__(BLOCK) void k_tfin(StepParams P) {
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     // rows K = sub, sub + 8, .. added in that order, KU of them per iteration so that their
     // loads (the flags, then the rows) are in flight together
-    constexpr int KU = 4;
+    constexpr int KU = 8;
     for (int K0 = sub; K0 < n; K0 += 8 * KU) {
         bool ok[KU], ab[KU];
         const double *src[KU];
