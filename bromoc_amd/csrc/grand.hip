@@ -25,8 +25,10 @@
 
 namespace bromoc {
 
+constexpr int GC_THREADS = 1024;              // threads of k_gc_pre and k_gc_post
+constexpr int GC_WAVES = GC_THREADS / WAVE;
 constexpr int FIND_GROUP = 512;   // slots per group of FIND's counts (at least)
-constexpr int FIND_MAXG = 256;    // groups per wavefront
+constexpr int FIND_MAXG = 128;    // groups per wavefront
 
 // The next trial (grand.f90:42-50, 77-84) and, for a deletion, FIND (find.f90:26-32): the
 // ip-th slot, in slot order, of the moving ions whose buffer is ib.  Called by a whole
@@ -35,8 +37,8 @@ __device__ inline void gc_pre(const GcParams &G) {
     // the trial position and acceptance arithmetic as written, unfused: a fused multiply-add
     // rounds differently from the reference's separate operations
 #pragma clang fp contract(off)
-    __shared__ int s_cnt[WPB];
-    __shared__ int s_grp[WPB][FIND_MAXG];
+    __shared__ int s_cnt[GC_WAVES];
+    __shared__ int s_grp[GC_WAVES][FIND_MAXG];
     __shared__ int s_found;
     GcState &S = *G.st;
     if (S.err) return;
@@ -91,8 +93,8 @@ __device__ inline void gc_pre(const GcParams &G) {
     const int ib = S.ib, ip = S.ip, ntot = S.ntot;
     const int first = G.sp.nsites + G.sp.nfix, cnt = ntot > first ? ntot - first : 0;
     int gsz = FIND_GROUP;   // slots per group: a multiple of 512, at most FIND_MAXG groups
-    while ((cnt + WPB * gsz - 1) / (WPB * gsz) > FIND_MAXG) gsz *= 2;
-    const int ngw = (cnt + WPB * gsz - 1) / (WPB * gsz);   // groups per wavefront
+    while ((cnt + GC_WAVES * gsz - 1) / (GC_WAVES * gsz) > FIND_MAXG) gsz *= 2;
+    const int ngw = (cnt + GC_WAVES * gsz - 1) / (GC_WAVES * gsz);   // groups per wavefront
     const int lo = first + min(wv * ngw * gsz, cnt), hi = first + min((wv + 1) * ngw * gsz, cnt);
     int c = 0;
     for (int g = 0; g < ngw; g++) {
@@ -145,7 +147,7 @@ __device__ inline void gc_pre(const GcParams &G) {
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_gc_pre(GcParams G) { gc_pre(G); }
+__global__ __launch_bounds__(GC_THREADS) void k_gc_pre(GcParams G) { gc_pre(G); }
 
 // srfe, reff (inside, moving) of every ion at the current positions, as k_rfpar without
 // gradients, over the device particle count
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void k_gc_interact(GcParams G) {
 // The attempt's acceptance (grand.f90:50-116).  Called by a whole workgroup.
 __device__ inline void gc_post(const GcParams &G) {
 #pragma clang fp contract(off)
-    __shared__ double s[BLOCK];
+    __shared__ double s[GC_THREADS];
     GcState &S = *G.st;
     const int tid = threadIdx.x;
     if (S.err) return;
@@ -259,10 +261,10 @@ __device__ inline void gc_post(const GcParams &G) {
     double dener = 0.0;
     if (mode != 0) {
         double a = 0.0;
-        for (int b = tid; b < GC_BLOCKS; b += BLOCK) a += G.part[b];
+        for (int b = tid; b < GC_BLOCKS; b += GC_THREADS) a += G.part[b];
         s[tid] = a;
         __syncthreads();
-        for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        for (int h = GC_THREADS / 2; h > 0; h >>= 1) {
             if (tid < h) s[tid] += s[tid + h];
             __syncthreads();
         }
@@ -349,7 +351,7 @@ __device__ inline void gc_post(const GcParams &G) {
 }
 
 // one attempt's acceptance, then (next) the following attempt's trial
-__global__ __launch_bounds__(BLOCK) void k_gc_post(GcParams G, int next) {
+__global__ __launch_bounds__(GC_THREADS) void k_gc_post(GcParams G, int next) {
     gc_post(G);
     __syncthreads();
     if (next) gc_pre(G);
@@ -361,10 +363,10 @@ hipError_t launch_grand(const GcParams &g, int ngcmc, hipStream_t s) {
         const int nion = g.cap - g.sp.nsites;
         if (nion > 0) hipLaunchKernelGGL(k_gc_rfpar, dim3((nion + 255) / 256), dim3(256), 0, s, g);
     }
-    hipLaunchKernelGGL(k_gc_pre, dim3(1), dim3(BLOCK), 0, s, g);
+    hipLaunchKernelGGL(k_gc_pre, dim3(1), dim3(GC_THREADS), 0, s, g);
     for (int a = 0; a < ngcmc; a++) {
         hipLaunchKernelGGL(k_gc_interact, dim3(GC_BLOCKS), dim3(BLOCK), 0, s, g);
-        hipLaunchKernelGGL(k_gc_post, dim3(1), dim3(BLOCK), 0, s, g, a + 1 < ngcmc ? 1 : 0);
+        hipLaunchKernelGGL(k_gc_post, dim3(1), dim3(GC_THREADS), 0, s, g, a + 1 < ngcmc ? 1 : 0);
     }
     return hipGetLastError();
 }
