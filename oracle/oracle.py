@@ -6,7 +6,8 @@ RefFortran -- oracle/_ref/libbromoc_ref.so, the reference Fortran routines thems
               harness drives oracle/_ref/libbromoc_shimcheck.so, where the reference's
               ENERGY/DYNAMICS0/DYNAMICS0NUC/INTERACT/GRAND are replaced by the Fortran
               front-end bromoc_amd/fortran/bromoc_gpu.f90 over the C ABI, served by the
-              C restatement (shim_adapter.c)
+              C restatement (shim_adapter.c); with lib=FRONT_GPU_SO the front-end is served
+              by the HIP library itself (a GPU is needed)
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
 """
@@ -24,6 +25,7 @@ ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libbromoc_ref.so")
 SHIM_SO = os.path.join(HERE, "_ref", "libbromoc_shimcheck.so")
 REF_OMP_SO = os.path.join(HERE, "_ref", "libbromoc_ref_omp.so")   # OpenMP ENERGY
+FRONT_GPU_SO = os.path.join(HERE, "_ref", "libbromoc_linkcheck.so")   # front-end over HIP
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -303,7 +305,7 @@ def ref_lib(path=REF_SO):
             f = getattr(L, name)
             f.argtypes = args
             f.restype = res
-        if path == SHIM_SO:
+        if path in (SHIM_SO, FRONT_GPU_SO):
             L.bromoc_gpu_reset.argtypes = []
             L.bromoc_gpu_reset.restype = None
             L.bromoc_gpu_set_seed.argtypes = [C.c_int64, C.c_int64]
@@ -379,7 +381,7 @@ class RefFortran(_Base):
 
     def __init__(self, system, lib=REF_SO):
         self.L = ref_lib(lib)
-        if lib == SHIM_SO:
+        if lib in (SHIM_SO, FRONT_GPU_SO):
             self.L.bromoc_gpu_reset()
         self.system = system
         self.L.ref_set_nindex(system.ndna, system.nion)
