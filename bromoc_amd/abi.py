@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 OK, E_ARG, E_UNSUPPORTED, E_CAPACITY, E_HIP, E_NODEVICE, E_STATE = range(7)
 STATUS_NAMES = {0: "OK", 1: "E_ARG", 2: "E_UNSUPPORTED", 3: "E_CAPACITY", 4: "E_HIP",
@@ -99,6 +99,8 @@ class SystemStruct(C.Structure):
         ("lstpx", _ip), ("lstpy", _ip), ("lstpz", _ip), ("bnorm", _dp), ("mmij", _dp),
         ("xscal", C.c_double), ("yscal", C.c_double), ("zscal", C.c_double),
         ("rfscal", C.c_double),
+        ("cx", C.c_double), ("cy", C.c_double), ("cz", C.c_double), ("rsphe", C.c_double),
+        ("rsphe2", C.c_double), ("iecx", C.c_double), ("iecy", C.c_double),
     ]
 
 
@@ -119,7 +121,8 @@ class GcmcStruct(C.Structure):
                 ("ibfftyp", _ip), ("avnum", _dp), ("mu", _dp), ("kb", _dp),
                 ("lzmin", _dp), ("lzmax", _dp), ("bufferbias", _bp), ("ntotat", _ip),
                 ("lx", C.c_double), ("ly", C.c_double), ("cx", C.c_double),
-                ("cy", C.c_double), ("cz", C.c_double), ("ikbt", C.c_double)]
+                ("cy", C.c_double), ("cz", C.c_double), ("ikbt", C.c_double),
+                ("rmin", _dp), ("rmax", _dp)]
 
 
 class ReplicaStruct(C.Structure):

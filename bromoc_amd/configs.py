@@ -549,16 +549,96 @@ def assign_buffers(s: System, g: Gcmc):
     """ibuffer of each particle: the buffer whose slab holds it and whose type matches
     (0 if none), as the reference's initial count does (count.f90:39-92)."""
     ib = np.zeros(s.capacity, np.int32)
+    sphere = bool(s.flags & abi.QSPHERE)
+    cx, cy, cz = s.center
     for i in range(s.nsites + s.nfix, s.ntot):
         t = abs(int(s.typei[i]))
         for b in range(1, g.nbuffer + 1):
             if g.ibfftyp[b] == t and g.lzmin[b] <= s.z[i] <= g.lzmax[b]:
+                if sphere:      # the shell rmin..rmax too (count.f90:52-55)
+                    r2 = (s.x[i] - cx) ** 2 + (s.y[i] - cy) ** 2 + (s.z[i] - cz) ** 2
+                    if not (g.rmin[b] ** 2 <= r2 <= g.rmax[b] ** 2):
+                        continue
                 ib[i] = b
                 break
     nat = np.zeros(g.nbuffer + 1, np.int32)
     for b in range(1, g.nbuffer + 1):
         nat[b] = int(np.sum(ib[:s.ntot] == b))
     return ib, nat
+
+
+# ------------------------------------------------------------------ curved systems
+def with_sphere(s: System, rsphe, center=(0.0, 0.0, 0.0), seed=0, shell=0.9):
+    """SYSTEM ... SPHERE RADI rsphe CX CY CZ (shell_simul.f90:870-912): a spherical system
+    whose bounding box is the cube of side 2 rsphe around the centre.  Moving ions outside
+    the sphere are moved radially inside it, to between shell*rsphe and rsphe, so that BD
+    steps reach the surface."""
+    s = s.copy()
+    s.flags = (s.flags | abi.QSPHERE) & ~abi.QECYL
+    cx, cy, cz = (float(v) for v in center)
+    s.center, s.rsphe = (cx, cy, cz), float(rsphe)
+    s.box = (cx - 0.5 * 2.0 * rsphe, cx + 0.5 * 2.0 * rsphe, cy - 0.5 * 2.0 * rsphe,
+             cy + 0.5 * 2.0 * rsphe, cz - 0.5 * 2.0 * rsphe, cz + 0.5 * 2.0 * rsphe)
+    s.x, s.y, s.z = s.x.copy(), s.y.copy(), s.z.copy()
+    rng = np.random.default_rng(seed)
+    for i in range(s.nsites + s.nfix, s.ntot):
+        d = np.array([s.x[i] - cx, s.y[i] - cy, s.z[i] - cz])
+        r = float(np.sqrt(d @ d))
+        if r > shell * rsphe:
+            f = rsphe * rng.uniform(shell, 0.995) / max(r, 1e-9)
+            s.x[i], s.y[i], s.z[i] = cx + f * d[0], cy + f * d[1], cz + f * d[2]
+    return s
+
+
+def with_ecyl(s: System, seed=0, shell=0.9):
+    """SYSTEM ... ECYL LX LY LZ CX CY CZ (shell_simul.f90:883-895): an elliptic cylinder
+    inscribed in the system box (semi-axes lx/2, ly/2 around the box centre; iecx = 2/lx,
+    iecy = 2/ly), z bounded by the box.  Moving ions outside the ellipse are moved inside
+    it, to between shell and 1 of the way out, so that BD steps reach the wall."""
+    s = s.copy()
+    s.flags = (s.flags | abi.QECYL) & ~abi.QSPHERE
+    lx2m, lx2p, ly2m, ly2p, lz2m, lz2p = s.box
+    lx, ly = lx2p - lx2m, ly2p - ly2m
+    cx, cy, cz = 0.5 * (lx2m + lx2p), 0.5 * (ly2m + ly2p), 0.5 * (lz2m + lz2p)
+    s.center = (cx, cy, cz)
+    s.iecx, s.iecy = 2.0 / lx, 2.0 / ly
+    s.x, s.y = s.x.copy(), s.y.copy()
+    rng = np.random.default_rng(seed)
+    for i in range(s.nsites + s.nfix, s.ntot):
+        ex, ey = (s.x[i] - cx) * s.iecx, (s.y[i] - cy) * s.iecy
+        r = float(np.hypot(ex, ey))
+        if r > shell:
+            f = rng.uniform(shell, 0.995) / max(r, 1e-9)
+            s.x[i], s.y[i] = cx + f * (s.x[i] - cx), cy + f * (s.y[i] - cy)
+    return s
+
+
+def sphere_buffers(s: System, conc, shell_len=6.0, mu=(0.0, 0.0)):
+    """GCMC buffers of a spherical system: per ion type, the lower and the upper half of
+    the outer shell rsphe - shell_len .. rsphe (BUFFER ... RMIN RMAX LZMIN LZMAX,
+    shell_simul.f90:975-1000); avnum = conc * the half shell's volume."""
+    cx, cy, cz = s.center
+    R, r0 = s.rsphe, s.rsphe - shell_len
+    ib_types, lzmin, lzmax, avnum, mus, rmin, rmax = [0], [0.0], [0.0], [0.0], [0.0], [0.0], [0.0]
+    for half in (0, 1):
+        for k in range(s.nion):
+            t = s.nold + 1 + k
+            lo, hi = (cz - R, cz) if half == 0 else (cz, cz + R)
+            dens = conc * (AVOGADRO / LITER) * ANGSTROM ** 3
+            ib_types.append(t)
+            lzmin.append(lo)
+            lzmax.append(hi)
+            rmin.append(r0)
+            rmax.append(R)
+            avnum.append(dens * 2.0 / 3.0 * PI * (R ** 3 - r0 ** 3))
+            mus.append(mu[k])
+    nb = len(ib_types)
+    return Gcmc(ibfftyp=np.array(ib_types, np.int32), avnum=np.array(avnum), mu=np.array(mus),
+                kb=np.zeros(nb), lzmin=np.array(lzmin), lzmax=np.array(lzmax),
+                bufferbias=np.zeros(nb, np.int8),
+                ntotat=np.array([0] + [int(round(a)) for a in avnum[1:]], np.int32),
+                lx=2.0 * R, ly=2.0 * R, cx=cx, cy=cy, cz=cz, ikbt=1.0 / s.meta["kbt"],
+                rmin=np.array(rmin), rmax=np.array(rmax))
 
 
 # ------------------------------------------------------ diffusion and reaction field

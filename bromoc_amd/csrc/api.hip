@@ -30,7 +30,23 @@ constexpr uint32_t SUPPORTED = BROMOC_QPAR | BROMOC_QNUCL | BROMOC_QNONBOND | BR
                                BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR |
                                BROMOC_QRFPSIN | BROMOC_QBSPLINE | BROMOC_QPROXDIFF |
                                BROMOC_QNOTRANS | BROMOC_QAPFOR | BROMOC_QCONTRANS |
-                               BROMOC_QMEMB | BROMOC_QMMIJ | BROMOC_QDEBYHYB | BROMOC_QNMCDEN;
+                               BROMOC_QMEMB | BROMOC_QMMIJ | BROMOC_QDEBYHYB | BROMOC_QNMCDEN |
+                               BROMOC_QSPHERE | BROMOC_QECYL;
+
+// The system's shape for OUTBOX, FIXCOOR, MOVE and INSERT (grandmod.f90:57,74-97)
+static Curved curved_of(const bromoc_system *s) {
+    Curved v{};
+    v.kind = (s->flags & BROMOC_QSPHERE) ? 1 : (s->flags & BROMOC_QECYL) ? 2 : 0;
+    v.cx = s->cx; v.cy = s->cy; v.cz = s->cz;
+    v.rsphe = s->rsphe; v.rsphe2 = s->rsphe2; v.iecx = s->iecx; v.iecy = s->iecy;
+    v.lx2m = s->lx2m; v.lx2p = s->lx2p; v.ly2m = s->ly2m; v.ly2p = s->ly2p;
+    v.lz2m = s->lz2m; v.lz2p = s->lz2p;
+    v.sx = v.kind == 1 ? 1.0 : v.kind == 2 ? s->iecx : 0.0;
+    v.sy = v.kind == 1 ? 1.0 : v.kind == 2 ? s->iecy : 0.0;
+    v.sz = v.kind == 1 ? 1.0 : 0.0;
+    v.thr = v.kind == 1 ? s->rsphe2 : v.kind == 2 ? 1.0 : HUGE_VAL;
+    return v;
+}
 
 // Repulsion maps the phiv grid holds: one, or with NMCDEN (readphi.f90:176-240) six site
 // maps (S, P, Ab, Tb, Cb, Gb) with DNA followed by one per ion type nold+1..ntype.
@@ -114,13 +130,14 @@ struct bromoc_ctx {
     bool have_gcmc = false;
     bromoc_gcmc g{};
     std::vector<int32_t> ibfftyp, ntotat, ibuffer;
-    std::vector<double> avnum, mu, kb, lzmin, lzmax;
+    std::vector<double> avnum, mu, kb, lzmin, lzmax, rmin, rmax;
     std::vector<int8_t> bufferbias;
     // device GRAND: buffer parameters, nat, prob, the buffer of each slot, state, the
     // per-workgroup INTERACT partials and the caller's uniforms
     struct GcDev {
         int nb = -1;
         double *avnum = nullptr, *mu = nullptr, *kb = nullptr, *lzmin = nullptr, *lzmax = nullptr;
+        double *rmin = nullptr, *rmax = nullptr;
         int32_t *ibfftyp = nullptr, *ntotat = nullptr, *nat = nullptr;
         int32_t *natlog = nullptr;        // [lcap] buffer counts after each attempt
         int64_t lcap = 0;
@@ -212,6 +229,12 @@ int validate(const bromoc_system *s, std::string &msg) {
     if (s->ntot < 0 || s->nsites < 0 || s->nfix < 0 || s->nsites + s->nfix > s->ntot)
         return bad("particle counts");
     if (s->capacity < s->ntot || s->capacity > (1 << 26)) return bad("capacity");
+    if ((s->flags & BROMOC_QSPHERE) && (s->flags & BROMOC_QECYL))
+        return bad("QSPHERE and QECYL together");
+    if ((s->flags & BROMOC_QSPHERE) && !(s->rsphe > 0.0 && s->rsphe2 > 0.0))
+        return bad("QSPHERE needs rsphe > 0 and rsphe2");
+    if ((s->flags & BROMOC_QECYL) && !(s->iecx > 0.0 && s->iecy > 0.0))
+        return bad("QECYL needs iecx, iecy > 0");
     if (!s->x || !s->y || !s->z || !s->typei) return bad("positions/typei missing");
     if (s->ntype < 1 || s->nttyp < 1 || s->npair < 0) return bad("type counts");
     if (s->nttyp + 1 > MAX_LDS_TYPES) return bad("too many atnam2 types for the LDS pair table");
@@ -905,6 +928,15 @@ int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     p.bdmax = s->bdmax; p.fact0n1 = s->fact0n1; p.fact0n2 = s->fact0n2;
     p.lx2m = s->lx2m; p.lx2p = s->lx2p; p.ly2m = s->ly2m; p.ly2p = s->ly2p;
     p.lz2m = s->lz2m; p.lz2p = s->lz2p;
+    p.cv = curved_of(s);
+    if (p.cv.kind != 0) {   // the step kernels clamp the box axes only; k_curved does the rest
+        p.lx2m = p.ly2m = -HUGE_VAL;
+        p.lx2p = p.ly2p = HUGE_VAL;
+        if (p.cv.kind == 1) {
+            p.lz2m = -HUGE_VAL;
+            p.lz2p = HUGE_VAL;
+        }
+    }
     p.cntpts = (s->flags & BROMOC_QCOUNTION) ? s->cntpts : 0;
     p.zcont = c->zcont; p.nfwd = c->nfwd; p.nbwd = c->nbwd;
     p.step_ctr = c->ctl;
@@ -1547,6 +1579,9 @@ int bromoc_set_gcmc(bromoc_ctx *c, const bromoc_gcmc *g) {
     if (nb > 0 && (!g->ibfftyp || !g->avnum || !g->mu || !g->kb || !g->lzmin || !g->lzmax ||
                    !g->bufferbias || !g->ntotat))
         return fail(c, BROMOC_E_ARG, "gcmc arrays missing");
+    const bool sphere = (c->sys.flags & BROMOC_QSPHERE) != 0;
+    if (nb > 0 && sphere && (!g->rmin || !g->rmax))
+        return fail(c, BROMOC_E_ARG, "gcmc rmin/rmax missing for a spherical system");
     for (int b = 1; b <= nb; b++)
         if (g->ibfftyp[b] <= c->sys.nold || g->ibfftyp[b] > c->sys.ntype)
             return fail(c, BROMOC_E_ARG, "buffer ion type out of range");
@@ -1558,6 +1593,14 @@ int bromoc_set_gcmc(bromoc_ctx *c, const bromoc_gcmc *g) {
     c->kb.assign(g->kb, g->kb + nb + 1);
     c->lzmin.assign(g->lzmin, g->lzmin + nb + 1);
     c->lzmax.assign(g->lzmax, g->lzmax + nb + 1);
+    if (sphere && nb > 0) {
+        c->rmin.assign(g->rmin, g->rmin + nb + 1);
+        c->rmax.assign(g->rmax, g->rmax + nb + 1);
+    } else {
+        c->rmin.assign(nb + 1, 0.0);
+        c->rmax.assign(nb + 1, 0.0);
+    }
+    c->g.rmin = c->g.rmax = nullptr;   // the caller's arrays are not kept
     c->bufferbias.assign(g->bufferbias, g->bufferbias + nb + 1);
     c->ibuffer.assign(c->cap + 1, 0);
     c->gd.ib_current = false;
@@ -1580,6 +1623,8 @@ int bromoc_set_gcmc(bromoc_ctx *c, const bromoc_gcmc *g) {
     HIPCHK(c, dev(&c->gd.kb, n1, nb > 0 ? g->kb : nullptr));
     HIPCHK(c, dev(&c->gd.lzmin, n1, nb > 0 ? g->lzmin : nullptr));
     HIPCHK(c, dev(&c->gd.lzmax, n1, nb > 0 ? g->lzmax : nullptr));
+    HIPCHK(c, dev(&c->gd.rmin, n1, c->rmin.data()));
+    HIPCHK(c, dev(&c->gd.rmax, n1, c->rmax.data()));
     HIPCHK(c, dev(&c->gd.ibfftyp, n1, nb > 0 ? g->ibfftyp : nullptr));
     HIPCHK(c, dev(&c->gd.ntotat, n1, nb > 0 ? g->ntotat : nullptr));
     HIPCHK(c, dev(&c->gd.bias, n1, nb > 0 ? g->bufferbias : nullptr));
@@ -1648,6 +1693,7 @@ static int grand_device(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t n
     G.nbuffer = nb;
     G.nat = D.nat; G.natlog = D.natlog;
     G.avnum = D.avnum; G.mu = D.mu; G.kb = D.kb; G.lzmin = D.lzmin; G.lzmax = D.lzmax;
+    G.rmin = D.rmin; G.rmax = D.rmax;
     G.ibfftyp = D.ibfftyp; G.ntotat = D.ntotat; G.bias = D.bias;
     G.lx = g.lx; G.ly = g.ly; G.cx = g.cx; G.cy = g.cy; G.cz = g.cz; G.ikbt = g.ikbt;
     G.icycle = icycle;
@@ -1703,6 +1749,7 @@ int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int6
     const int nbuffer = g.nbuffer;
     const double nbufferr = (double)nbuffer;
     const int pstride = c->cap + 1;
+    const Curved cv = curved_of(&c->sys);
     int64_t k = 0;
     auto rndm = [&](double &v) {
         if (k >= nu) return false;
@@ -1720,11 +1767,14 @@ int bromoc_grand(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t nu, int6
             const int ib = (int)(nbufferr * r1) + 1;
             if (ib <= nbuffer && c->avnum[ib] > 0.0) {
                 const int itype = c->ibfftyp[ib];
-                double a, b, d;
-                RNDM(a); RNDM(b); RNDM(d);
-                const double xn = g.lx * (a - 0.5) + g.cx;
-                const double yn = g.ly * (b - 0.5) + g.cy;
-                const double zn = c->lzmin[ib] + (c->lzmax[ib] - c->lzmin[ib]) * d;
+                double xn, yn, zn;
+                do {   // INSERT (insert.f90:29-41)
+                    double a, b, d;
+                    RNDM(a); RNDM(b); RNDM(d);
+                    xn = g.lx * (a - 0.5) + g.cx;
+                    yn = g.ly * (b - 0.5) + g.cy;
+                    zn = c->lzmin[ib] + (c->lzmax[ib] - c->lzmin[ib]) * d;
+                } while (!cv.insert_ok(xn, yn, zn, g.cx, g.cy, g.cz, c->rmin[ib], c->rmax[ib]));
                 int st = bromoc_interact(c, xn, yn, zn, itype, c->ntot, 0, &dener);
                 if (st) return st;
                 double rate;
@@ -1837,7 +1887,7 @@ int bromoc_metropolis(bromoc_ctx *c, int32_t nmcm, double mcmax, double ikbt,
                              hipMemcpyDeviceToHost, c->stream));
     int st = sync_err(c);
     if (st) return st;
-    const bromoc_system &s = c->sys;
+    const Curved cv = curved_of(&c->sys);
     const double natom2 = (double)nmove;   // natom - nsites (move.f90:33)
     int64_t k = 0;
     int32_t ndone = 0;
@@ -1855,8 +1905,7 @@ int bromoc_metropolis(bromoc_ctx *c, int32_t nmcm, double mcmax, double ikbt,
             yn = pa.y + mcmax * (u[kk + 2] - 0.5);
             zn = pa.z + mcmax * (u[kk + 3] - 0.5);
             kk += 4;
-            out = xn < s.lx2m || xn > s.lx2p || yn < s.ly2m || yn > s.ly2p || zn < s.lz2m ||
-                  zn > s.lz2p;
+            out = cv.outside(xn, yn, zn);
         }
         if (out || kk + 1 > nu) break;   // the trial could not finish
         const int jt = std::abs(c->h_typei[iat]);

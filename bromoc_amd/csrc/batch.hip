@@ -752,10 +752,11 @@ int bromoc_batch_create(const bromoc_system *s, int32_t nrep, const bromoc_repli
     *out = nullptr;
     // the replica kernels implement pairs, the trilinear grids and DYNAMICS0/1/2 only
     const uint32_t unsupported = BROMOC_QNUCL | BROMOC_QRFPAR | BROMOC_QRFPSIN | BROMOC_QDNAFREE |
-                                 BROMOC_QMEMB | BROMOC_QBSPLINE | BROMOC_QMMIJ | BROMOC_QNMCDEN;
+                                 BROMOC_QMEMB | BROMOC_QBSPLINE | BROMOC_QMMIJ | BROMOC_QNMCDEN |
+                                 BROMOC_QSPHERE | BROMOC_QECYL;
     if (s->flags & unsupported) {
-        fprintf(stderr, "bromoc_batch_create: replicas are ion-only (no CG-DNA, RFPAR, MMIJ, "
-                        "membrane, B-spline or per-species map)\n");
+        fprintf(stderr, "bromoc_batch_create: replicas are ion-only boxes (no CG-DNA, RFPAR, "
+                        "MMIJ, membrane, B-spline, per-species map or curved system)\n");
         return BROMOC_E_UNSUPPORTED;
     }
     if (s->nsites != 0 || !(s->flags & BROMOC_QPAR)) return BROMOC_E_ARG;

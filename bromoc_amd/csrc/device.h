@@ -687,10 +687,19 @@ __device__ constexpr int ND_CAT[ND] = {E_BOND, E_ANG, E_DIHE, E_STACK, E_BP, E_E
                                        E_GSBPA, E_VDWGD};
 enum { D_BOND = 0, D_ANG, D_DIHE, D_STACK, D_BP, D_EX, D_QQ, D_SOLV, D_GSBPA, D_VDWGD };
 
-// OUTBOX (grandmod.f90:74-97) of the box system: outside lx2m..lx2p, ly2m..ly2p, lz2m..lz2p
+// OUTBOX (grandmod.f90:74-97): outside the box, the sphere or the elliptic cylinder
+// One form for the three shapes, without branches: q = ((x-cx) sx)^2 + ((y-cy) sy)^2 +
+// ((z-cz) sz)^2 against thr -- the sphere's r2 > rsphe2 (sx = sy = sz = 1), the ellipse's
+// > 1 (sx = iecx, sy = iecy, sz = 0), never in a box (thr = +inf) -- or outside the clamp
+// box, which is open along the curved axes (all three for the sphere, x and y for the
+// cylinder).  The products by 1 and 0 are exact, so each shape's arithmetic is the
+// reference's.
 __device__ __forceinline__ bool outbox(const StepParams &P, const V3 &r) {
-    return r.x < P.lx2m || r.x > P.lx2p || r.y < P.ly2m || r.y > P.ly2p || r.z < P.lz2m ||
-           r.z > P.lz2p;
+#pragma clang fp contract(off)
+    const double a = (r.x - P.cv.cx) * P.cv.sx, b = (r.y - P.cv.cy) * P.cv.sy;
+    const double c = (r.z - P.cv.cz) * P.cv.sz;
+    return a * a + b * b + c * c > P.cv.thr || r.x < P.lx2m || r.x > P.lx2p ||
+           r.y < P.ly2m || r.y > P.ly2p || r.z < P.lz2m || r.z > P.lz2p;
 }
 
 // Debye screening of a P-P pair (energy.mono.f90:439-441): always with QDEBY; with the

@@ -59,6 +59,72 @@ __host__ __device__ inline uint32_t inc_pack(int kind, int role, int term) {
 constexpr int MAX_LDS_PAIRS = 64;     // pair types held in LDS
 constexpr int MAX_LDS_TYPES = 16;     // atnam2 types (pair index table in LDS)
 
+// Curved systems (grandmod cx, cy, cz, rsphe, rsphe2, iecx, iecy): kind 0 box, 1 sphere,
+// 2 elliptic cylinder, with the bounding box lx2m..lz2p.  The same code serves the host
+// (MOVE, the host GRAND loop) and the device, in the reference's order of operations.
+struct Curved {
+    int32_t kind, pad;
+    double cx, cy, cz, rsphe, rsphe2, iecx, iecy;
+    double lx2m, lx2p, ly2m, ly2p, lz2m, lz2p;
+    double sx, sy, sz, thr;   // the device OUTBOX's quadratic form (device.h outbox)
+    // OUTBOX (grandmod.f90:74-97); MOVE's test is the same (move.f90:39-46)
+    __host__ __device__ __forceinline__ bool outside(double x, double y, double z) const {
+#pragma clang fp contract(off)
+        if (kind == 1)
+            return (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz) > rsphe2;
+        if (kind == 2) {
+            const double ex = (x - cx) * iecx, ey = (y - cy) * iecy;
+            return ex * ex + ey * ey > 1.0 || z < lz2m || z > lz2p;
+        }
+        return x < lx2m || x > lx2p || y < ly2m || y > ly2p || z < lz2m || z > lz2p;
+    }
+    // FIXCOOR's curved branches (fixcoor.f90:24-46): outside the sphere a particle is
+    // reflected through the surface along its radius; outside the ellipse it is scaled back
+    // onto it by 0.999999999999999/sqrt(r2) and its z clamped.  Returns whether it moved.
+    __host__ __device__ __forceinline__ bool fix(double &x, double &y, double &z) const {
+#pragma clang fp contract(off)
+        bool moved = false;
+        if (kind == 1) {
+            const double r2 = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz);
+            if (r2 > rsphe2) {
+                const double lzrad = 2.0 * rsphe / sqrt(r2) - 1.0;
+                x = lzrad * (x - cx) + cx;
+                y = lzrad * (y - cy) + cy;
+                z = lzrad * (z - cz) + cz;
+                moved = true;
+            }
+        } else if (kind == 2) {
+            const double ex = (x - cx) * iecx, ey = (y - cy) * iecy;
+            const double r2 = ex * ex + ey * ey;
+            if (r2 > 1.0) {
+                const double r = 0.999999999999999 / sqrt(r2);
+                x = r * (x - cx) + cx;
+                y = r * (y - cy) + cy;
+                moved = true;
+            }
+            if (z < lz2m) { z = lz2m; moved = true; }
+            else if (z > lz2p) { z = lz2p; moved = true; }
+        }
+        return moved;
+    }
+    // INSERT's acceptance (insert.f90:33-40) of a point drawn in buffer slab ib: strictly
+    // inside the shell rmin..rmax around the GCMC centre (gx, gy, gz) of the sphere, inside
+    // the ellipse, anywhere in the box
+    __host__ __device__ __forceinline__ bool insert_ok(double x, double y, double z, double gx, double gy,
+                                       double gz, double rmin, double rmax) const {
+#pragma clang fp contract(off)
+        if (kind == 1) {
+            const double r2 = (x - gx) * (x - gx) + (y - gy) * (y - gy) + (z - gz) * (z - gz);
+            return r2 > rmin * rmin && r2 < rmax * rmax;
+        }
+        if (kind == 2) {
+            const double ex = (x - gx) * iecx, ey = (y - gy) * iecy;
+            return ex * ex + ey * ey < 1.0;
+        }
+        return true;
+    }
+};
+
 struct StepParams {
     // sizes
     int32_t ntot, nsites, nfix, ntargets;
@@ -125,7 +191,8 @@ struct StepParams {
     int32_t *warn;
     // Brownian dynamics
     double bdmax, fact0n1, fact0n2;
-    double lx2m, lx2p, ly2m, ly2p, lz2m, lz2p;
+    double lx2m, lx2p, ly2m, ly2p, lz2m, lz2p;   // FIXCOOR's clamp (open along curved axes)
+    Curved cv;                                    // OUTBOX; the curved FIXCOOR after a step
     int32_t cntpts, noise_mode;
     const double *zcont;
     int32_t *nfwd, *nbwd;
@@ -227,7 +294,7 @@ hipError_t launch_pairs(const StepParams &p, hipStream_t s);  // k_step (pairs, 
 // RFPARION: k_rfpar (srfe, reff, and their gradients when pairs or grad) then, with pairs,
 // the cross term k_rfpair
 hipError_t launch_rfpar(const StepParams &p, bool pairs, hipStream_t s, bool grad = false);
-hipError_t launch_notrans(const StepParams &p, hipStream_t s);  // NOTRANS after a step
+hipError_t launch_notrans(const StepParams &p, hipStream_t s);  // curved FIXCOOR, NOTRANS after a step
 hipError_t launch_fixed(const StepParams &p, hipStream_t s);    // terms of the fixed ions
 // MMIJ reaction field (mmij.hip): RECT_RF1 for ENERGY (force into frf, energy into slot 0 of
 // E_GSBPB), RECT_RF0 for INTERACT (added to *out)
@@ -286,6 +353,7 @@ struct GcParams {
     int32_t *natlog;         // [attempts][nbuffer]: each buffer's count after each attempt
                              // (the host adds them to the caller's prob histogram)
     const double *avnum, *mu, *kb, *lzmin, *lzmax;   // [nbuffer+1]
+    const double *rmin, *rmax;                        // [nbuffer+1] (sphere)
     const int32_t *ibfftyp, *ntotat;
     const int8_t *bias;
     double lx, ly, cx, cy, cz, ikbt;

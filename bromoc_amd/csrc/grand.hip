@@ -58,13 +58,24 @@ __device__ inline void gc_pre(const GcParams &G) {
             rndm(r1);
             ib = (int)(nbufferr * r1) + 1;
             if (ok && ib <= G.nbuffer && G.avnum[ib] > 0.0) {
-                double a, b, d;
-                rndm(a); rndm(b); rndm(d);
                 mode = 1;
                 itype = G.ibfftyp[ib];
-                S.x = G.lx * (a - 0.5) + G.cx;
-                S.y = G.ly * (b - 0.5) + G.cy;
-                S.z = G.lzmin[ib] + (G.lzmax[ib] - G.lzmin[ib]) * d;
+                // INSERT (insert.f90:29-41): three draws, again until the point is in the
+                // buffer's region (at most as many tries as the uniforms allow)
+                const Curved cv = G.sp.cv;
+                const double rlo = G.rmin[ib], rhi = G.rmax[ib];
+                double x = 0.0, y = 0.0, z = 0.0;
+                for (;;) {
+                    if (k + 3 > G.nu) { ok = false; k = G.nu; break; }
+                    x = G.lx * (G.u[k] - 0.5) + G.cx;
+                    y = G.ly * (G.u[k + 1] - 0.5) + G.cy;
+                    z = G.lzmin[ib] + (G.lzmax[ib] - G.lzmin[ib]) * G.u[k + 2];
+                    k += 3;
+                    if (cv.insert_ok(x, y, z, G.cx, G.cy, G.cz, rlo, rhi)) break;
+                }
+                S.x = x;
+                S.y = y;
+                S.z = z;
             }
         } else if (ok) {
             double r2;

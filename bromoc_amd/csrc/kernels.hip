@@ -659,7 +659,24 @@ hipError_t launch_fixed(const StepParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
+// FIXCOOR of a curved system (fixcoor.f90:24-46) on the positions a step wrote: every moving
+// ion, and every free site unless the sites are free of the system (dynamics0.mono.f90:45,
+// dynamics0nuc.f90:42).  The step kernels clamp only the box axes (z of the cylinder), so
+// this completes FIXCOOR; it reads and writes only the new positions, as the step does.
+__global__ void k_curved(StepParams P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.ntot) return;
+    if (i < P.nsites ? (P.stfree[i] == 0 || (P.flags & QDNAFREE) != 0) : i < P.nsites + P.nfix)
+        return;
+    double4 q = P.poso[i];
+    if (!P.cv.fix(q.x, q.y, q.z)) return;
+    P.poso[i] = q;
+    P.sposo[P.rank[i]] = q;
+}
+
 hipError_t launch_notrans(const StepParams &p, hipStream_t s) {
+    if (p.cv.kind != 0 && p.mode == 1 && p.ntot > 0)
+        hipLaunchKernelGGL(k_curved, dim3((p.ntot + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, p);
     if ((p.flags & QNOTRANS) && p.nsites > 0)
         hipLaunchKernelGGL(k_notrans, dim3(1), dim3(BLOCK), 0, s, p);
     return hipGetLastError();

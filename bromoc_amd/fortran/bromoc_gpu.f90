@@ -77,6 +77,7 @@ type, bind(C) :: bgpu_system
   integer(c_int32_t) :: mmij_shape, xnpol, ynpol, znpol, ntpol, reserved5
   type(c_ptr) :: lstpx, lstpy, lstpz, bnorm, mmij
   real(c_double) :: xscal, yscal, zscal, rfscal
+  real(c_double) :: cx, cy, cz, rsphe, rsphe2, iecx, iecy
 end type
 
 type, bind(C) :: bgpu_energies
@@ -89,8 +90,11 @@ type, bind(C) :: bgpu_gcmc
   integer(c_int32_t) :: nbuffer, reserved
   type(c_ptr) :: ibfftyp, avnum, mu, kb, lzmin, lzmax, bufferbias, ntotat
   real(c_double) :: lx, ly, cx, cy, cz, ikbt
+  type(c_ptr) :: rmin, rmax
 end type
 
+! BROMOC_ABI_VERSION of include/bromoc_gpu.h: the layout of bgpu_system and bgpu_gcmc above
+integer(c_int32_t), parameter :: BGPU_ABI_VERSION = 5
 integer(c_int32_t), parameter :: F_PAR=1, F_NUCL=2, F_NONBOND=4, F_NOBOND=8, F_PHIX=16, &
      F_PHIV=32, F_SVDW=64, F_SOLV=128, F_DEBY=256, F_COUNTION=512, F_DNAFREE=1024, &
      F_MEMB=65536, F_MMIJ=131072, F_BSPLINE=262144, F_NMCDEN=524288, F_DEBYHYB=1048576, &
@@ -408,7 +412,7 @@ subroutine bromoc_gpu_init(seed)
   if (Qphiv .and. Qsvdw) scs(2:nt1) = scal(1:nttyp)
   if (cntpts > 0) zc(1:cntpts) = zcont(1:cntpts)
 
-  s%abi_version = 4
+  s%abi_version = BGPU_ABI_VERSION
   s%ntot = ntot; s%nsites = nsites; s%nfix = nfix; s%capacity = cap
   s%ntype = ntype; s%nold = nold; s%nttyp = nttyp; s%npair = ntc; s%flags = int(fl, c_int32_t)
   s%x = c_loc(xs); s%y = c_loc(ys); s%z = c_loc(zs); s%typei = c_loc(ityp)
@@ -544,6 +548,8 @@ subroutine bromoc_gpu_init(seed)
   endif
   s%lstpx = c_loc(mlx); s%lstpy = c_loc(mly); s%lstpz = c_loc(mlz)
   s%bnorm = c_loc(mbn); s%mmij = c_loc(mmm)
+  s%cx = cx; s%cy = cy; s%cz = cz
+  s%rsphe = rsphe; s%rsphe2 = rsphe2; s%iecx = iecx; s%iecy = iecy
   call bgpu_check(bromoc_create(s, 0, gctx), 'bromoc_create')
 end subroutine
 
@@ -725,8 +731,10 @@ call bgpu_check(bromoc_interact(gctx, xj, yj, zj, int(jtype, c_int32_t), int(j -
 dener = d
 end subroutine
 
-! GRAND draws every uniform from the reference's rndm() up front (at most 6 per attempt)
-! and hands them to the device loop, which consumes them in grand.f90's order.
+! GRAND draws every uniform from the reference's rndm() up front (at most 6 per attempt in
+! a box; in a sphere or elliptic cylinder INSERT redraws its 3 until the point is inside, so
+! 768 more per attempt are drawn) and hands them to the device loop, which consumes them in
+! grand.f90's order.
 SUBROUTINE GRAND(ngcmc, prob, icycle)
 use bromoc_gpu
 use grandmod
@@ -738,7 +746,7 @@ integer(c_int32_t), allocatable :: natv(:), pr(:), ib(:)
 integer(c_int64_t) :: used
 type(bgpu_gcmc) :: g
 integer(c_int32_t), allocatable, target :: bt(:), tot(:)
-real(c_double), allocatable, target :: av(:), muv(:), kbv(:), zl(:), zh(:)
+real(c_double), allocatable, target :: av(:), muv(:), kbv(:), zl(:), zh(:), rl(:), rh(:)
 integer(c_int8_t), allocatable, target :: bb(:)
 real(c_double) :: e
 integer :: k, b, n
@@ -746,22 +754,28 @@ call bromoc_gpu_init()
 call bromoc_gpu_upload()
 if (gcmc_ready == 0) then
   allocate(bt(nbuffer+1), tot(nbuffer+1), av(nbuffer+1), muv(nbuffer+1), kbv(nbuffer+1), &
-           zl(nbuffer+1), zh(nbuffer+1), bb(nbuffer+1))
+           zl(nbuffer+1), zh(nbuffer+1), bb(nbuffer+1), rl(nbuffer+1), rh(nbuffer+1))
   bt(1) = 0; tot(1) = 0; av(1) = 0; muv(1) = 0; kbv(1) = 0; zl(1) = 0; zh(1) = 0; bb(1) = 0
+  rl(1) = 0; rh(1) = 0
+  rl(2:) = Rmin(1:nbuffer); rh(2:) = Rmax(1:nbuffer)
   bt(2:) = ibfftyp(1:nbuffer); tot(2:) = ntotat(1:nbuffer); av(2:) = avnum(1:nbuffer)
   muv(2:) = mu(1:nbuffer); kbv(2:) = kb(1:nbuffer); zl(2:) = LZmin(1:nbuffer); zh(2:) = LZmax(1:nbuffer)
   do b = 1, nbuffer
     bb(b+1) = merge(1_c_int8_t, 0_c_int8_t, Qbufferbias(b))
   enddo
   g = bgpu_gcmc(nbuffer, 0, c_loc(bt), c_loc(av), c_loc(muv), c_loc(kbv), c_loc(zl), c_loc(zh), &
-                c_loc(bb), c_loc(tot), lx, ly, cx, cy, cz, ikbt)
+                c_loc(bb), c_loc(tot), lx, ly, cx, cy, cz, ikbt, c_loc(rl), c_loc(rh))
   call bgpu_check(bromoc_set_gcmc(gctx, g), 'bromoc_set_gcmc')
   gcmc_ready = 1
 endif
 allocate(ib(ntot))
 ib = ibuffer(1:ntot)
 call bgpu_check(bromoc_set_ibuffer(gctx, ib), 'bromoc_set_ibuffer')
-allocate(u(6*ngcmc + 1))
+if (Qsphere .or. Qecyl) then
+  allocate(u(774*ngcmc + 1))
+else
+  allocate(u(6*ngcmc + 1))
+endif
 do k = 1, size(u)
   u(k) = rndm()
 enddo

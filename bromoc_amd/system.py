@@ -97,6 +97,12 @@ class System:
     fact0n1: float = 0.0
     fact0n2: float = 0.0
     box: tuple = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0)   # lx2m, lx2p, ly2m, ly2p, lz2m, lz2p
+    # curved systems (QSPHERE / QECYL): centre (cx, cy, cz), sphere radius rsphe (rsphe2 =
+    # rsphe**2 as the reference holds it), inverse semi-axes iecx, iecy of the cylinder
+    center: tuple = (0.0, 0.0, 0.0)
+    rsphe: float = 0.0
+    iecx: float = 0.0
+    iecy: float = 0.0
     zcont: np.ndarray = None
     # position-dependent diffusion (DYNAMICS1 / DYNAMICS2)
     dt: float = 0.0
@@ -324,6 +330,9 @@ class System:
             s.mmij = arr(np.asarray(g["mmij"], float).reshape(-1), np.float64, C.c_double)
             s.xscal, s.yscal, s.zscal = (float(v) for v in g["scal"])
             s.rfscal = float(g.get("rfscal", 1.0))
+        s.cx, s.cy, s.cz = (float(v) for v in self.center)
+        s.rsphe, s.rsphe2 = float(self.rsphe), float(self.rsphe) ** 2
+        s.iecx, s.iecy = float(self.iecx), float(self.iecy)
         return s, keep
 
 
@@ -358,6 +367,8 @@ class Gcmc:
     cy: float
     cz: float
     ikbt: float
+    rmin: np.ndarray = None          # [nbuffer+1] spherical shells (QSPHERE)
+    rmax: np.ndarray = None
 
     @property
     def nbuffer(self):
@@ -383,6 +394,9 @@ class Gcmc:
         g.ntotat = arr(self.ntotat, np.int32, C.c_int32)
         g.lx, g.ly, g.cx, g.cy, g.cz, g.ikbt = (self.lx, self.ly, self.cx, self.cy, self.cz,
                                                 self.ikbt)
+        if self.rmin is not None:
+            g.rmin = arr(self.rmin, np.float64, C.c_double)
+            g.rmax = arr(self.rmax, np.float64, C.c_double)
         return g, keep
 
 

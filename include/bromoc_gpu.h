@@ -64,7 +64,7 @@
 extern "C" {
 #endif
 
-#define BROMOC_ABI_VERSION 4
+#define BROMOC_ABI_VERSION 5
 
 /* status codes */
 #define BROMOC_OK              0
@@ -109,9 +109,14 @@ extern "C" {
                                         * S, P, Ab, Tb, Cb, Gb (with QNUCL) then one per ion
                                         * type nold+1..ntype (with QPAR); needs QPHIV, not
                                         * with QSVDW */
-/* options of the reference that this path rejects with BROMOC_E_UNSUPPORTED */
-#define BROMOC_QSPHERE     (1u << 26)  /* spherical system (fixcoor.f90:24-31) */
-#define BROMOC_QECYL       (1u << 27)  /* elliptic-cylinder system (fixcoor.f90:33-46) */
+#define BROMOC_QSPHERE     (1u << 26)  /* spherical system of radius rsphe around (cx, cy, cz):
+                                        * FIXCOOR reflects inward (fixcoor.f90:24-31), OUTBOX,
+                                        * MOVE, INSERT (rmin..rmax shells) and COUNT test the
+                                        * sphere (grandmod.f90:79-84, move.f90:39-41,
+                                        * insert.f90:35-37, count.f90:52-64) */
+#define BROMOC_QECYL       (1u << 27)  /* elliptic-cylinder system: semi-axes 1/iecx, 1/iecy
+                                        * around (cx, cy), z in lz2m..lz2p (fixcoor.f90:33-46,
+                                        * grandmod.f90:85-90, move.f90:42-43, insert.f90:38-39) */
 
 /* site names (nucleotmod namsite) */
 #define BROMOC_SITE_S   0
@@ -241,6 +246,10 @@ typedef struct bromoc_system {
     const double *bnorm;                     /* [ntpol] */
     const double *mmij;                      /* [ntpol*ntpol] */
     double xscal, yscal, zscal, rfscal;
+    /* curved systems (grandmod cx, cy, cz, rsphe, rsphe2, iecx, iecy): the centre, with
+     * QSPHERE the radius and its square as the reference holds both, with QECYL the
+     * inverse semi-axes.  lx2m..lz2p stay the bounding box (shell_simul.f90:875-895). */
+    double cx, cy, cz, rsphe, rsphe2, iecx, iecy;
 } bromoc_system;
 
 /* Energy terms as in grandmod/gsbpmod/nucleotmod (energy.mono.f90:46-68,689). */
@@ -260,6 +269,8 @@ typedef struct bromoc_gcmc {
     const int8_t  *bufferbias;   /* [nbuffer+1] */
     const int32_t *ntotat;       /* [nbuffer+1] */
     double lx, ly, cx, cy, cz, ikbt;
+    const double  *rmin, *rmax;  /* [nbuffer+1] spherical shells of the buffers (QSPHERE;
+                                  * may be null otherwise) */
 } bromoc_gcmc;
 
 typedef struct bromoc_ctx bromoc_ctx;
@@ -342,7 +353,7 @@ int bromoc_get_ibuffer(bromoc_ctx *ctx, int32_t *ibuffer);
 int bromoc_grand(bromoc_ctx *ctx, int32_t ngcmc, const double *uniforms, int64_t nuniforms,
                  int64_t *used, int32_t *nat, int32_t *prob, int64_t icycle, double *ener);
 
-/* METROPOLIS(nmcm) (metropolis.f90:31-55 with MOVE, move.f90:32-47, rectangular box):
+/* METROPOLIS(nmcm) (metropolis.f90:31-55 with MOVE, move.f90:32-47; box, sphere or cylinder):
  * nmcm trials, each moving one random moving ion by up to mcmax/2 per axis (mcmax as
  * grandmod holds it: twice the MCMAX keyword), accepted when the INTERACT energy falls or
  * with probability exp(-dE ikbt).  uniforms[] supplies every rndm() in call order.  A trial

@@ -64,7 +64,7 @@ typedef struct orc_ctx {
     /* GCMC */
     bromoc_gcmc g;
     int32_t *ibfftyp, *ntotat, *ibuffer;
-    double *avnum, *mu, *kb, *lzmin, *lzmax;
+    double *avnum, *mu, *kb, *lzmin, *lzmax, *rmin, *rmax;
     int8_t *bufferbias;
     /* DYNAMICS1/2 */
     double *diffusion, *ampl3, *p3, *xs, *ys, *bs, *cs, *ds;
@@ -428,8 +428,8 @@ EXPORT void orc_destroy(orc_ctx *c) {
                   c->nwtype, c->pairidx, c->stfree, c->cg, c->fact1a, c->fact2a, c->ep,
                   c->scal, c->zcont, c->pairs, c->phix, c->phiv, c->warn, c->nforward,
                   c->nbackward, c->noise, c->ibfftyp, c->ntotat, c->ibuffer, c->avnum, c->mu,
-                  c->kb, c->lzmin, c->lzmax, c->bufferbias, c->diffusion, c->ampl3, c->p3,
-                  c->xs, c->ys, c->bs, c->cs, c->ds, c->gsrfen, c->greff, c->srfe, c->reff,
+                  c->kb, c->lzmin, c->lzmax, c->rmin, c->rmax, c->bufferbias, c->diffusion,
+                  c->ampl3, c->p3, c->xs, c->ys, c->bs, c->cs, c->ds, c->gsrfen, c->greff, c->srfe, c->reff,
                   c->srfed, c->reffd, c->dids, c->sn, c->csn, c->af, c->kx, c->ky, c->kz,
                   c->contrx, c->contry, c->contrz, c->ampl1, c->p1, c->ampl2, c->p2,
                   c->rcylinder, c->rho, c->rdna, c->gr, c->prob2, c->prob3, c->lstpx,
@@ -446,6 +446,10 @@ EXPORT orc_ctx *orc_create(const bromoc_system *s) {
     if ((s->flags & BROMOC_QMMIJ) &&
         (s->mmij_shape != 1 || s->xnpol < 3 || s->ynpol < 3 || s->znpol < 3 || s->xnpol > 64 ||
          s->ynpol > 64 || s->znpol > 64 || s->ntpol < 1))
+        return NULL;
+    if (((s->flags & BROMOC_QSPHERE) && (s->flags & BROMOC_QECYL)) ||
+        ((s->flags & BROMOC_QSPHERE) && !(s->rsphe > 0.0)) ||
+        ((s->flags & BROMOC_QECYL) && !(s->iecx > 0.0 && s->iecy > 0.0)))
         return NULL;
     orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
     c->s = *s;
@@ -618,11 +622,19 @@ static uint32_t supported_mask(void) {
            BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QSOLV | BROMOC_QDEBY | BROMOC_QCOUNTION |
            BROMOC_QDNAFREE | BROMOC_QDIFFUSE | BROMOC_QPROFILE | BROMOC_QRFPAR | BROMOC_QRFPSIN |
            BROMOC_QBSPLINE | BROMOC_QPROXDIFF | BROMOC_QNOTRANS | BROMOC_QAPFOR |
-           BROMOC_QCONTRANS | BROMOC_QMEMB | BROMOC_QMMIJ | BROMOC_QDEBYHYB | BROMOC_QNMCDEN;
+           BROMOC_QCONTRANS | BROMOC_QMEMB | BROMOC_QMMIJ | BROMOC_QDEBYHYB | BROMOC_QNMCDEN |
+           BROMOC_QSPHERE | BROMOC_QECYL;
 }
 
-/* OUTBOX, grandmod.f90:74-97 (box system) */
+/* OUTBOX, grandmod.f90:74-97: outside the sphere, the elliptic cylinder or the box */
 static int outbox(const bromoc_system *s, double xi, double yi, double zi) {
+    if (s->flags & BROMOC_QSPHERE)
+        return (xi - s->cx) * (xi - s->cx) + (yi - s->cy) * (yi - s->cy) +
+               (zi - s->cz) * (zi - s->cz) > s->rsphe2;
+    if (s->flags & BROMOC_QECYL) {
+        const double ex = (xi - s->cx) * s->iecx, ey = (yi - s->cy) * s->iecy;
+        return ex * ex + ey * ey > 1.0 || zi < s->lz2m || zi > s->lz2p;
+    }
     return xi < s->lx2m || xi > s->lx2p || yi < s->ly2m || yi > s->ly2p || zi < s->lz2m ||
            zi > s->lz2p;
 }
@@ -1745,8 +1757,34 @@ EXPORT int orc_energy(orc_ctx *c, int32_t qforces, bromoc_energies *out) {
 }
 
 /* ---------------------------------------------------------- dynamics */
-/* FIXCOOR rectangular branch, fixcoor.f90:48-64 */
+/* FIXCOOR, fixcoor.f90:19-67: a particle outside the sphere is reflected through its
+ * surface along the radius; outside the elliptic cylinder it is put back on the ellipse
+ * (scale 0.999999999999999/sqrt(r2)) and its z clamped; in the box each coordinate is
+ * clamped. */
 static void fixcoor(const bromoc_system *s, double *xi, double *yi, double *zi) {
+    if (s->flags & BROMOC_QSPHERE) {
+        const double r2 = (*xi - s->cx) * (*xi - s->cx) + (*yi - s->cy) * (*yi - s->cy) +
+                          (*zi - s->cz) * (*zi - s->cz);
+        if (r2 > s->rsphe2) {
+            const double lzrad = 2.0 * s->rsphe / sqrt(r2) - 1.0;
+            *xi = lzrad * (*xi - s->cx) + s->cx;
+            *yi = lzrad * (*yi - s->cy) + s->cy;
+            *zi = lzrad * (*zi - s->cz) + s->cz;
+        }
+        return;
+    }
+    if (s->flags & BROMOC_QECYL) {
+        const double ex = (*xi - s->cx) * s->iecx, ey = (*yi - s->cy) * s->iecy;
+        const double r2 = ex * ex + ey * ey;
+        if (r2 > 1.0) {
+            const double r = 0.999999999999999 / sqrt(r2);
+            *xi = r * (*xi - s->cx) + s->cx;
+            *yi = r * (*yi - s->cy) + s->cy;
+        }
+        if (*zi < s->lz2m) *zi = s->lz2m;
+        else if (*zi > s->lz2p) *zi = s->lz2p;
+        return;
+    }
     if (*xi < s->lx2m) *xi = s->lx2m;
     else if (*xi > s->lx2p) *xi = s->lx2p;
     if (*yi < s->ly2m) *yi = s->ly2m;
@@ -2391,9 +2429,13 @@ EXPORT int orc_interact(orc_ctx *c, double xj, double yj, double zj, int32_t jty
 /* ----------------------------------------------------------- GRAND */
 EXPORT int orc_set_gcmc(orc_ctx *c, const bromoc_gcmc *g, const int32_t *ibuffer) {
     int nb = g->nbuffer;
+    if ((c->s.flags & BROMOC_QSPHERE) && (!g->rmin || !g->rmax)) return BROMOC_E_ARG;
     c->g = *g;
     free(c->ibfftyp); free(c->ntotat); free(c->avnum); free(c->mu); free(c->kb);
     free(c->lzmin); free(c->lzmax); free(c->bufferbias); free(c->ibuffer);
+    free(c->rmin); free(c->rmax);
+    c->rmin = g->rmin ? (double *)dupmem(g->rmin, sizeof(double) * (nb + 1)) : NULL;
+    c->rmax = g->rmax ? (double *)dupmem(g->rmax, sizeof(double) * (nb + 1)) : NULL;
     c->ibfftyp = (int32_t *)dupmem(g->ibfftyp, sizeof(int32_t) * (nb + 1));
     c->ntotat = (int32_t *)dupmem(g->ntotat, sizeof(int32_t) * (nb + 1));
     c->avnum = (double *)dupmem(g->avnum, sizeof(double) * (nb + 1));
@@ -2424,7 +2466,24 @@ static double rndm(ustream *st) {
     return st->u[st->k++];
 }
 
-/* GRAND, grand.f90:40-118 with INSERT (insert.f90:29-41, rectangular) and FIND
+/* INSERT's acceptance, insert.f90:33-40: inside the spherical shell rmin..rmax of buffer
+ * ib (strictly), inside the ellipse, or anywhere in the box slab. */
+static int insert_ok(const orc_ctx *c, int ib, double x, double y, double z) {
+    const bromoc_system *s = &c->s;
+    const bromoc_gcmc *g = &c->g;
+    if (s->flags & BROMOC_QSPHERE) {
+        const double r2 = (x - g->cx) * (x - g->cx) + (y - g->cy) * (y - g->cy) +
+                          (z - g->cz) * (z - g->cz);
+        return r2 > c->rmin[ib] * c->rmin[ib] && r2 < c->rmax[ib] * c->rmax[ib];
+    }
+    if (s->flags & BROMOC_QECYL) {
+        const double ex = (x - g->cx) * s->iecx, ey = (y - g->cy) * s->iecy;
+        return ex * ex + ey * ey < 1.0;
+    }
+    return 1;
+}
+
+/* GRAND, grand.f90:40-118 with INSERT (insert.f90:29-41) and FIND
  * (find.f90:26-32); every rndm() is taken from `uniforms` in call order. */
 EXPORT int orc_grand(orc_ctx *c, int32_t ngcmc, const double *uniforms, int64_t nuniforms,
                      int64_t *used, int32_t *nat, int32_t *prob, int64_t icycle, double *ener) {
@@ -2439,9 +2498,13 @@ EXPORT int orc_grand(orc_ctx *c, int32_t ngcmc, const double *uniforms, int64_t 
             int ib = (int)(nbufferr * rndm(&st)) + 1;
             if (ib <= nbuffer && c->avnum[ib] > 0.0) {
                 int itype = c->ibfftyp[ib];
-                double xnew = g->lx * (rndm(&st) - 0.5) + g->cx;
-                double ynew = g->ly * (rndm(&st) - 0.5) + g->cy;
-                double znew = c->lzmin[ib] + (c->lzmax[ib] - c->lzmin[ib]) * rndm(&st);
+                double xnew, ynew, znew;
+                for (;;) {   /* INSERT: draw until the point is in the buffer's region */
+                    xnew = g->lx * (rndm(&st) - 0.5) + g->cx;
+                    ynew = g->ly * (rndm(&st) - 0.5) + g->cy;
+                    znew = c->lzmin[ib] + (c->lzmax[ib] - c->lzmin[ib]) * rndm(&st);
+                    if (st.err || insert_ok(c, ib, xnew, ynew, znew)) break;
+                }
                 double dener;
                 int rc = orc_interact(c, xnew, ynew, znew, itype, s->ntot, 0, &dener);
                 if (rc) return rc;
@@ -2531,8 +2594,8 @@ EXPORT int orc_metropolis(orc_ctx *c, int32_t nmcm, double mcmax, double ikbt,
             ynew = c->y[iat] + mcmax * (u[kk + 2] - 0.5);
             znew = c->z[iat] + mcmax * (u[kk + 3] - 0.5);
             kk += 4;
-            out = xnew < s->lx2m || xnew > s->lx2p || ynew < s->ly2m || ynew > s->ly2p ||
-                  znew < s->lz2m || znew > s->lz2p;
+            /* MOVE's test (move.f90:39-46) is OUTBOX's */
+            out = outbox(s, xnew, ynew, znew);
         }
         if (out || kk + 1 > nu) break;
         double eold, enew;

@@ -63,12 +63,14 @@ type, bind(C) :: c_system
   integer(c_int32_t) :: mmij_shape, xnpol, ynpol, znpol, ntpol, reserved5
   type(c_ptr) :: lstpx, lstpy, lstpz, bnorm, mmij
   real(c_double) :: xscal, yscal, zscal, rfscal
+  real(c_double) :: cx, cy, cz, rsphe, rsphe2, iecx, iecy
 end type
 
 type, bind(C) :: c_gcmc
   integer(c_int32_t) :: nbuffer, reserved
   type(c_ptr) :: ibfftyp, avnum, mu, kb, lzmin, lzmax, bufferbias, ntotat
   real(c_double) :: lx, ly, cx, cy, cz, ikbt
+  type(c_ptr) :: rmin, rmax
 end type
 
 ! injected streams
@@ -76,7 +78,8 @@ real(c_double), allocatable :: gstream(:), ustream(:)
 integer :: gpos = 0, upos = 0, ulen = 0
 integer(c_int32_t), parameter :: F_PAR=1, F_NUCL=2, F_NONBOND=4, F_NOBOND=8, F_PHIX=16, F_PHIV=32, &
      F_SVDW=64, F_SOLV=128, F_DEBY=256, F_COUNTION=512, F_DNAFREE=1024, &
-     F_RFPAR=33554432, F_DIFFUSE=268435456, F_PROFILE=536870912, F_RFPSIN=1073741824, &
+     F_RFPAR=33554432, F_SPHERE=67108864, F_ECYL=134217728, F_DIFFUSE=268435456, &
+     F_PROFILE=536870912, F_RFPSIN=1073741824, &
      F_BSPLINE=262144, F_PROXDIFF=2097152, F_NOTRANS=4194304, F_APFOR=8388608, &
      F_CONTRANS=16777216, F_MEMB=65536, F_MMIJ=131072, F_DEBYHYB=1048576, &
      F_NMCDEN=524288
@@ -178,8 +181,10 @@ Qdebyhyb = flag(s%flags, F_DEBYHYB)
 Qapfor = flag(s%flags, F_APFOR)
 Qcontrans = flag(s%flags, F_CONTRANS)
 Qwarn = .false.
-Qsphere = .false.
-Qecyl = .false.
+Qsphere = flag(s%flags, F_SPHERE)
+Qecyl = flag(s%flags, F_ECYL)
+cx = s%cx; cy = s%cy; cz = s%cz
+rsphe = s%rsphe; rsphe2 = s%rsphe2; iecx = s%iecx; iecy = s%iecy
 Qnotrans = flag(s%flags, F_NOTRANS)
 Qdiffuse = flag(s%flags, F_DIFFUSE)
 Qprofile = flag(s%flags, F_PROFILE)
@@ -819,6 +824,10 @@ do i = 1, nb
   Qbufferbias(i) = bp(i+1) /= 0
 enddo
 lx = g%lx; ly = g%ly; cx = g%cx; cy = g%cy; cz = g%cz; ikbt = g%ikbt
+if (c_associated(g%rmin)) then
+  call c_f_pointer(g%rmin, dp, [nb+1]); Rmin(1:nb) = dp(2:nb+1)
+  call c_f_pointer(g%rmax, dp, [nb+1]); Rmax(1:nb) = dp(2:nb+1)
+endif
 ibuffer(1:ntot) = ib_in(1:ntot)
 ninsert = 0; nremove = 0
 end subroutine

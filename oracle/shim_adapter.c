@@ -34,6 +34,7 @@ int orc_grand(orc_ctx *c, int32_t ng, const double *u, int64_t nu, int64_t *used
 int bromoc_create(const bromoc_system *s, int device, bromoc_ctx **out) {
     (void)device;
     if (!s || !out) return BROMOC_E_ARG;
+    if (s->abi_version != BROMOC_ABI_VERSION) return BROMOC_E_ARG;   /* as the library checks */
     int rc = orc_check_flags((uint32_t)s->flags);
     if (rc) return rc;
     orc_ctx *c = orc_create(s);

@@ -65,11 +65,15 @@ def _plain():
 
 
 def test_create_rejects_unsupported_options():
+    # a curved system needs its shape (rsphe, or iecx and iecy), and only one of the two
     for flag in (abi.QSPHERE, abi.QECYL):
         s = _plain()
         s.flags |= flag
         rc, _ = _create(s)
-        assert rc == abi.E_UNSUPPORTED
+        assert rc == abi.E_ARG
+    s = configs.with_sphere(_plain(), 20.0)
+    s.flags |= abi.QECYL
+    assert _create(s)[0] == abi.E_ARG
     s = configs.with_mmij(_plain(), npol=(3, 3, 3))
     s.mmij = dict(s.mmij, shape=2)                # the SPHERE reaction field (sphe_rf1.f90)
     assert _create(s)[0] == abi.E_UNSUPPORTED

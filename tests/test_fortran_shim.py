@@ -225,3 +225,46 @@ def test_front_end_links_against_the_library():
                        text=True)
     assert r.returncode == 0, r.stderr[-2000:]
     ctypes.CDLL(os.path.join(oracle_dir, "_ref", "libbromoc_linkcheck.so"))
+
+
+@pytest.mark.parametrize("which", ["sphere", "ecyl"])
+def test_curved_systems_through_front_end(which):
+    """Spherical and elliptic-cylinder systems (Qsphere, Qecyl with cx, cy, cz, rsphe,
+    rsphe2, iecx, iecy and the buffers' Rmin/Rmax): BD steps whose noise carries ions past
+    the wall (FIXCOOR), then GRAND (INSERT's redraws)."""
+    base = configs.pore_dna(seed=10, n_bp=0, n_ions=80, box=(40.0, 40.0, 60.0), dcel=0.5,
+                            rpore=12.0, half_slab=10.0, dna=False,
+                            table_builder=oracle_table_builder)
+    if which == "sphere":
+        s = configs.with_sphere(base, 20.0, seed=2).copy()
+    else:
+        s = configs.with_ecyl(base, seed=2).copy()
+    s.capacity = s.ntot + 150
+    seed, c0 = 55, 9
+    f, r = _pair(s)
+    f.L.bromoc_gpu_set_seed(seed, c0)
+    for k in range(3):
+        f.energy()
+        f.dynamics(abi.NOISE_PHILOX)
+        r.energy()
+        r.dynamics(abi.NOISE_PHILOX, 0, seed, c0 + k)
+    assert float(np.max(np.abs(f.positions()[0] - r.positions()[0]))) <= 1e-12
+    s2 = s.copy()
+    s2.x, s2.y, s2.z = (np.ascontiguousarray(v) for v in r.positions()[0].T)
+    if which == "sphere":
+        gc = configs.sphere_buffers(s2, conc=1.0, shell_len=6.0, mu=(-0.5, -0.4))
+    else:
+        gc = configs.gcmc_buffers(s2, conc=1.0, buffer_len=8.0, mu=(-0.5, -0.4))
+    ib, nat = configs.assign_buffers(s2, gc)
+    f.set_gcmc(gc, ib)
+    r.set_gcmc(gc, ib)
+    u = np.random.default_rng(4).random(20000)
+    pa = np.zeros((gc.nbuffer, s.capacity + 1), np.int32)
+    pb = pa.copy()
+    na, nb = nat.copy(), nat.copy()
+    _, ea = f.grand(60, u, na, pa, 1, 0.0)
+    _, eb = r.grand(60, u, nb, pb, 1, 0.0)
+    assert np.array_equal(na, nb) and np.array_equal(pa, pb)
+    assert abs(ea - eb) <= 1e-12 * max(abs(eb), 1.0)
+    assert f.ntot == r.ntot
+    assert float(np.max(np.abs(f.positions()[0] - r.positions()[0]))) <= 1e-12
