@@ -567,6 +567,21 @@ def assign_buffers(s: System, g: Gcmc):
     return ib, nat
 
 
+def with_static_field(s: System, seed=0, dcel=1.0, volt=0.5, noise_volt=0.05):
+    """A static field phix (STATICF1) over the whole box: a linear drop of `volt` volts
+    along z plus smoothed noise (pore_field), on a grid of spacing dcel centred on the box."""
+    s = s.copy()
+    lx2m, lx2p, ly2m, ly2p, lz2m, lz2p = s.box
+    shape = tuple(int(math.ceil((hi - lo) / dcel)) + 3
+                  for lo, hi in ((lx2m, lx2p), (ly2m, ly2p), (lz2m, lz2p)))
+    half = 0.5 * (lz2p - lz2m)
+    field = pore_field(shape, dcel, volt, half, np.random.default_rng(seed), noise_volt=noise_volt)
+    s.phix = GridData(field, dcel, xbcen=0.5 * (lx2m + lx2p), ybcen=0.5 * (ly2m + ly2p),
+                      zbcen=0.5 * (lz2m + lz2p))
+    s.flags |= abi.QPHIX
+    return s
+
+
 # ------------------------------------------------------------------ curved systems
 def with_sphere(s: System, rsphe, center=(0.0, 0.0, 0.0), seed=0, shell=0.9):
     """SYSTEM ... SPHERE RADI rsphe CX CY CZ (shell_simul.f90:870-912): a spherical system

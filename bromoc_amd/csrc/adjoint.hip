@@ -2,7 +2,8 @@
 // the tabulated pair potentials, for a loss on the trajectory's smoothed radial
 // distribution functions.  No counterpart in the reference (its potentials are fitted by
 // the separate inverse-Monte-Carlo program); the forward model is the reference's BD step
-// (ENERGY pair forces through getyd, polynom.f90:197-230; DYNAMICS0, dynamics0.mono.f90:
+// (ENERGY pair forces through getyd, polynom.f90:197-230, and the static field of STATICF1,
+// staticf1.f90, whose Jacobian J enters as J^T mu and J dx; DYNAMICS0, dynamics0.mono.f90:
 // 31-46 with its bdmax clamp; FIXCOOR's box clamp, fixcoor.f90:49-64) with injected noise.
 //
 // Step map x' = fixcoor(x + clamp(F(x) f1) + f2 xi).  With D_box, D_bd the 0/1 masks of
@@ -121,10 +122,13 @@ __global__ __launch_bounds__(256) void k_adj_force(AdjParams A) {
     fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
     if (lane == 0) { s_r[0][wv] = fx; s_r[1][wv] = fy; s_r[2][wv] = fz; }
     __syncthreads();
-    if (tid == 0)
-        A.f[i] = make_double4(s_r[0][0] + s_r[0][1] + s_r[0][2] + s_r[0][3],
-                              s_r[1][0] + s_r[1][1] + s_r[1][2] + s_r[1][3],
-                              s_r[2][0] + s_r[2][1] + s_r[2][2] + s_r[2][3], 0.0);
+    if (tid == 0) {
+        double G[3] = {0.0, 0.0, 0.0}, J[6];
+        if (P.flags & QPHIX) phix_force_jac(P, P.cg[abs(P.typei[i])], xi.x, xi.y, xi.z, G, J);
+        A.f[i] = make_double4(s_r[0][0] + s_r[0][1] + s_r[0][2] + s_r[0][3] + G[0],
+                              s_r[1][0] + s_r[1][1] + s_r[1][2] + s_r[1][3] + G[1],
+                              s_r[2][0] + s_r[2][1] + s_r[2][2] + s_r[2][3] + G[2], 0.0);
+    }
 }
 
 // k_adj_mu: the clamp masks of DYNAMICS0 + FIXCOOR at x_s and mu = f1 D_bd D_box lambda'
@@ -244,8 +248,17 @@ __global__ __launch_bounds__(256) void k_adj_pairs(AdjParams A) {
         const double4 l1 = A.lam1[i], db = A.dbox[i];
         // flux loss at the last state: d(charge displacement)/dz_i = q_i
         const double zf = A.zflux != 0.0 ? A.zflux * P.cg[abs(P.typei[i])] : 0.0;
-        A.lam0[i] = make_double4(db.x * l1.x + t[0] + t[3], db.y * l1.y + t[1] + t[4],
-                                 db.z * l1.z + t[2] + t[5] + zf, 0.0);
+        // the static field's share: J^T mu (J = dF/dx of STATICF1 at x_i)
+        double g[3] = {0.0, 0.0, 0.0};
+        if (P.flags & QPHIX) {
+            double G[3], J[6];
+            phix_force_jac(P, P.cg[abs(P.typei[i])], xi.x, xi.y, xi.z, G, J);
+            g[0] = J[2] * mi.y + J[4] * mi.z;
+            g[1] = J[0] * mi.x + J[5] * mi.z;
+            g[2] = J[1] * mi.x + J[3] * mi.y;
+        }
+        A.lam0[i] = make_double4(db.x * l1.x + t[0] + t[3] + g[0], db.y * l1.y + t[1] + t[4] + g[1],
+                                 db.z * l1.z + t[2] + t[5] + g[2] + zf, 0.0);
     }
 }
 
@@ -327,9 +340,16 @@ __global__ __launch_bounds__(256) void k_tlm(AdjParams A) {
         }
     }
     if (tid == 0) {
-        const double fx = s_r[0][0] + s_r[0][1] + s_r[0][2] + s_r[0][3];
-        const double fy = s_r[1][0] + s_r[1][1] + s_r[1][2] + s_r[1][3];
-        const double fz = s_r[2][0] + s_r[2][1] + s_r[2][2] + s_r[2][3];
+        double fx = s_r[0][0] + s_r[0][1] + s_r[0][2] + s_r[0][3];
+        double fy = s_r[1][0] + s_r[1][1] + s_r[1][2] + s_r[1][3];
+        double fz = s_r[2][0] + s_r[2][1] + s_r[2][2] + s_r[2][3];
+        if (P.flags & QPHIX) {   // the static field's share: J dx (J = dF/dx of STATICF1)
+            double G[3], J[6];
+            phix_force_jac(P, P.cg[abs(P.typei[i])], xi.x, xi.y, xi.z, G, J);
+            fx += J[0] * di.y + J[1] * di.z;
+            fy += J[2] * di.x + J[3] * di.z;
+            fz += J[4] * di.x + J[5] * di.y;
+        }
         const double4 m = A.mu[i], db = A.dbox[i];
         A.dxo[i] = make_double4(db.x * di.x + m.x * fx, db.y * di.y + m.y * fy,
                                 db.z * di.z + m.z * fz, 0.0);

@@ -226,6 +226,56 @@ __device__ __forceinline__ void grid_cell(const GridGeom &g, double x, double y,
     if (iz == g.nz - 1) iz = g.nz - 2;
 }
 
+// STATICF1 (staticf1.f90) on an ion of charge chi at (x, y, z): the force F and its Jacobian
+// J = dF/dx from the 8 corners in order (the step kernels' per-corner terms).  Within a
+// cell the trilinear interpolant is linear in each coordinate, so J's diagonal is zero; an
+// off-diagonal entry is the mixed second derivative of the force component it belongs to,
+// under that component's near-node guard, so J is not symmetric where the guards differ.
+// J[0..5] = dFx/dy, dFx/dz, dFy/dx, dFy/dz, dFz/dx, dFz/dy.  Nothing outside the grid box.
+__device__ inline void phix_force_jac(const StepParams &P, double chi, double x, double y,
+                                      double z, double F[3], double J[6]) {
+    for (int k = 0; k < 3; k++) F[k] = 0.0;
+    for (int k = 0; k < 6; k++) J[k] = 0.0;
+    const GridGeom &g = P.gx;
+    if (!(x <= g.xbcen + g.tranx && x >= g.xbcen - g.tranx && y <= g.ybcen + g.trany &&
+          y >= g.ybcen - g.trany && z <= g.zbcen + g.tranz && z >= g.zbcen - g.tranz))
+        return;
+    double gx, gy, gz;
+    int ix, iy, iz;
+    grid_cell(g, x, y, z, gx, gy, gz, ix, iy, iz);
+    const double one = 1.0;
+    for (int c = 0; c < 8; c++) {
+        const int n1 = ix + (c >> 2), n2 = iy + ((c >> 1) & 1), n3 = iz + (c & 1);
+        double ai = gx - n1 * g.dcel;
+        const double as = fsign(one, ai);
+        ai = one - fabs(ai) * g.idcel;
+        double bi = gy - n2 * g.dcel;
+        const double bs = fsign(one, bi);
+        bi = one - fabs(bi) * g.idcel;
+        double ci = gz - n3 * g.dcel;
+        const double cs = fsign(one, ci);
+        ci = one - fabs(ci) * g.idcel;
+        const double ph = (double)P.phix[((size_t)n1 * g.ny + n2) * g.nz + n3];
+        const double prefac = ph * P.celec * chi * g.idcel;
+        const double q = prefac * g.idcel;   // d(1 - |u|/dcel)/du = -sign(u)/dcel
+        if (ai < (1.0 - RSMALL) && ai > RSMALL) {
+            F[0] += as * bi * ci * prefac;
+            J[0] -= as * bs * ci * q;
+            J[1] -= as * bi * cs * q;
+        }
+        if (bi < (1.0 - RSMALL) && bi > RSMALL) {
+            F[1] += bs * ai * ci * prefac;
+            J[2] -= bs * as * ci * q;
+            J[3] -= bs * ai * cs * q;
+        }
+        if (ci < (1.0 - RSMALL) && ci > RSMALL) {
+            F[2] += cs * ai * bi * prefac;
+            J[4] -= cs * as * bi * q;
+            J[5] -= cs * ai * bs * q;
+        }
+    }
+}
+
 // Quadratic B-spline M3 and its derivative DM3 (m3.f90:19-42, dm3.f90:19-38).
 __device__ __forceinline__ void bspline_m3(double x, double &m, double &dm) {
     const double m2m = (x >= 0.0 && x <= 2.0) ? 1.0 - fabs(x - 1.0) : 0.0;

@@ -61,8 +61,11 @@ def _normwise(adj, cd):
     return float(np.max(np.abs(adj - cd)) / np.max(np.abs(cd)))
 
 
-def test_forward_loss_matches_cpu():
+@pytest.mark.parametrize("field", [False, True], ids=["pairs", "static_field"])
+def test_forward_loss_matches_cpu(field):
     s = _system(seed=1, n=40)
+    if field:
+        s = configs.with_static_field(s, seed=3)
     nsteps = 30
     noise = np.random.default_rng(2).standard_normal((nsteps, 3 * s.ntot))
     target = np.random.default_rng(3).random((len(SPEC.groups), SPEC.nbins))
@@ -72,9 +75,11 @@ def test_forward_loss_matches_cpu():
     assert np.max(np.abs(gg - gc)) <= 1e-10 * np.max(gc)
 
 
-@pytest.mark.parametrize("nsteps,ckpt", [(60, 20), (45, 7)])
-def test_gradient_vs_gpu_central_differences(nsteps, ckpt):
+@pytest.mark.parametrize("nsteps,ckpt,field", [(60, 20, False), (45, 7, False), (45, 7, True)])
+def test_gradient_vs_gpu_central_differences(nsteps, ckpt, field):
     s = _system(seed=4, n=50)
+    if field:   # the static field's Jacobian enters the reverse pass
+        s = configs.with_static_field(s, seed=5)
     noise = np.random.default_rng(5).standard_normal((nsteps, 3 * s.ntot))
     target = _target(s, noise, nsteps, curve_seed=77)
     loss, _, grads = rdf_loss(s, noise, nsteps, SPEC, target=target, checkpoint_every=ckpt)
@@ -180,14 +185,17 @@ def test_gradient_bit_reproducible():
 FLUX = RdfSpec(groups=[(1, 1), (1, 2), (2, 2), (1, 3)], wflux=3.0, jtarget=0.05)
 
 
+@pytest.mark.parametrize("field", [False, True], ids=["pairs", "static_field"])
 @pytest.mark.parametrize("spec", [SPEC, FLUX], ids=["rdf", "rdf+flux"])
 @pytest.mark.parametrize("nsteps,ckpt", [(60, 20), (45, 7)])
-def test_dot_product_adjoint_vs_tangent(spec, nsteps, ckpt):
+def test_dot_product_adjoint_vs_tangent(spec, nsteps, ckpt, field):
     """<dL/dth, v> from the reverse pass equals the tangent-linear pass's dL along v, for
     random directions over every table coefficient (ep2, ep3 of every knot; sc1, sc3 of
     every pair type): the two passes are transposes of one linear map, so they agree to
     rounding -- free of the knot-jump noise central differences carry."""
     s = _system(seed=18, n=60, box=36.0)
+    if field:
+        s = configs.with_static_field(s, seed=19)
     noise = np.random.default_rng(19).standard_normal((nsteps, 3 * s.ntot))
     target = _target(s, noise, nsteps, curve_seed=80)
     loss, gep, gsc = table_gradients(s, noise, nsteps, spec, target=target, checkpoint_every=ckpt)
