@@ -582,6 +582,27 @@ def with_static_field(s: System, seed=0, dcel=1.0, volt=0.5, noise_volt=0.05):
     return s
 
 
+def with_soft_repulsion(s: System, seed=0, dcel=1.0, svdw=0.5, vmax=3, scal=True):
+    """A repulsion map phiv (VDWGD1TRLN) over the whole box with small smooth integer
+    values 0..vmax (from smoothed noise), scaled by svdw, with per-type scal factors (QSVDW)
+    when scal is set; vzmin..vzmax is the box's z range."""
+    from scipy.ndimage import gaussian_filter
+    s = s.copy()
+    lx2m, lx2p, ly2m, ly2p, lz2m, lz2p = s.box
+    shape = tuple(int(math.ceil((hi - lo) / dcel)) + 3
+                  for lo, hi in ((lx2m, lx2p), (ly2m, ly2p), (lz2m, lz2p)))
+    v = gaussian_filter(np.random.default_rng(seed).standard_normal(shape), 2.0)
+    v = (v - v.min()) / max(float(v.max() - v.min()), 1e-30)
+    s.phiv = GridData(np.rint(v * vmax).astype(np.int8), dcel, xbcen=0.5 * (lx2m + lx2p),
+                      ybcen=0.5 * (ly2m + ly2p), zbcen=0.5 * (lz2m + lz2p))
+    s.svdw, s.thold8, s.vzmin, s.vzmax = svdw, 8.0, lz2m, lz2p
+    s.flags |= abi.QPHIV
+    if scal:
+        s.scal = np.linspace(0.5, 1.5, s.nttyp + 1)
+        s.flags |= abi.QSVDW
+    return s
+
+
 # ------------------------------------------------------------------ curved systems
 def with_sphere(s: System, rsphe, center=(0.0, 0.0, 0.0), seed=0, shell=0.9):
     """SYSTEM ... SPHERE RADI rsphe CX CY CZ (shell_simul.f90:870-912): a spherical system

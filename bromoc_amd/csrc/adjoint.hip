@@ -2,8 +2,8 @@
 // the tabulated pair potentials, for a loss on the trajectory's smoothed radial
 // distribution functions.  No counterpart in the reference (its potentials are fitted by
 // the separate inverse-Monte-Carlo program); the forward model is the reference's BD step
-// (ENERGY pair forces through getyd, polynom.f90:197-230, and the static field of STATICF1,
-// staticf1.f90, whose Jacobian J enters as J^T mu and J dx; DYNAMICS0, dynamics0.mono.f90:
+// (ENERGY pair forces through getyd, polynom.f90:197-230, and the trilinear grids of STATICF1
+// and VDWGD1TRLN, whose Jacobian J enters as J^T mu and J dx; DYNAMICS0, dynamics0.mono.f90:
 // 31-46 with its bdmax clamp; FIXCOOR's box clamp, fixcoor.f90:49-64) with injected noise.
 //
 // Step map x' = fixcoor(x + clamp(F(x) f1) + f2 xi).  With D_box, D_bd the 0/1 masks of
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_adj_force(AdjParams A) {
     __syncthreads();
     if (tid == 0) {
         double G[3] = {0.0, 0.0, 0.0}, J[6];
-        if (P.flags & QPHIX) phix_force_jac(P, P.cg[abs(P.typei[i])], xi.x, xi.y, xi.z, G, J);
+        if (P.flags & (QPHIX | QPHIV)) grid_force_jac(P, i, xi.x, xi.y, xi.z, ti, G, J);
         A.f[i] = make_double4(s_r[0][0] + s_r[0][1] + s_r[0][2] + s_r[0][3] + G[0],
                               s_r[1][0] + s_r[1][1] + s_r[1][2] + s_r[1][3] + G[1],
                               s_r[2][0] + s_r[2][1] + s_r[2][2] + s_r[2][3] + G[2], 0.0);
@@ -248,11 +248,11 @@ __global__ __launch_bounds__(256) void k_adj_pairs(AdjParams A) {
         const double4 l1 = A.lam1[i], db = A.dbox[i];
         // flux loss at the last state: d(charge displacement)/dz_i = q_i
         const double zf = A.zflux != 0.0 ? A.zflux * P.cg[abs(P.typei[i])] : 0.0;
-        // the static field's share: J^T mu (J = dF/dx of STATICF1 at x_i)
+        // the grids' share: J^T mu (J = dF/dx of STATICF1 and VDWGD1TRLN at x_i)
         double g[3] = {0.0, 0.0, 0.0};
-        if (P.flags & QPHIX) {
+        if (P.flags & (QPHIX | QPHIV)) {
             double G[3], J[6];
-            phix_force_jac(P, P.cg[abs(P.typei[i])], xi.x, xi.y, xi.z, G, J);
+            grid_force_jac(P, i, xi.x, xi.y, xi.z, ti, G, J);
             g[0] = J[2] * mi.y + J[4] * mi.z;
             g[1] = J[0] * mi.x + J[5] * mi.z;
             g[2] = J[1] * mi.x + J[3] * mi.y;
@@ -343,9 +343,9 @@ __global__ __launch_bounds__(256) void k_tlm(AdjParams A) {
         double fx = s_r[0][0] + s_r[0][1] + s_r[0][2] + s_r[0][3];
         double fy = s_r[1][0] + s_r[1][1] + s_r[1][2] + s_r[1][3];
         double fz = s_r[2][0] + s_r[2][1] + s_r[2][2] + s_r[2][3];
-        if (P.flags & QPHIX) {   // the static field's share: J dx (J = dF/dx of STATICF1)
+        if (P.flags & (QPHIX | QPHIV)) {   // the grids' share: J dx (J = dF/dx at x_i)
             double G[3], J[6];
-            phix_force_jac(P, P.cg[abs(P.typei[i])], xi.x, xi.y, xi.z, G, J);
+            grid_force_jac(P, i, xi.x, xi.y, xi.z, ti, G, J);
             fx += J[0] * di.y + J[1] * di.z;
             fy += J[2] * di.x + J[3] * di.z;
             fz += J[4] * di.x + J[5] * di.y;

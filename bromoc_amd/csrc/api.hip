@@ -2211,11 +2211,11 @@ struct AdjRun {
             return BROMOC_E_ARG;
         const bromoc_system &S = c->sys;
         const uint32_t allowed = BROMOC_QPAR | BROMOC_QNONBOND | BROMOC_QNOBOND | BROMOC_QCOUNTION |
-                                 BROMOC_QPHIX;
+                                 BROMOC_QPHIX | BROMOC_QPHIV | BROMOC_QSVDW | BROMOC_QNMCDEN;
         if ((S.flags & ~allowed) || c->ns != 0 || !c->base.all_efpot)
             return fail(c, BROMOC_E_UNSUPPORTED,
-                        "adjoint: moving ions with tabulated pairs and the static field only (no "
-                        "repulsion grid, CG-DNA, RFPAR, diffusion profiles)");
+                        "adjoint: moving ions with tabulated pairs and the trilinear grids only "
+                        "(no B-spline map, CG-DNA, RFPAR, diffusion profiles)");
         if (rdf->wflux > 0.0 && !(c->base.dt > 0.0))
             return fail(c, BROMOC_E_ARG, "adjoint: the flux term needs the time step dt");
         nt = c->ntot; nt1 = S.nttyp + 1; nf0 = c->nf;

@@ -226,20 +226,15 @@ __device__ __forceinline__ void grid_cell(const GridGeom &g, double x, double y,
     if (iz == g.nz - 1) iz = g.nz - 2;
 }
 
-// STATICF1 (staticf1.f90) on an ion of charge chi at (x, y, z): the force F and its Jacobian
-// J = dF/dx from the 8 corners in order (the step kernels' per-corner terms).  Within a
-// cell the trilinear interpolant is linear in each coordinate, so J's diagonal is zero; an
-// off-diagonal entry is the mixed second derivative of the force component it belongs to,
-// under that component's near-node guard, so J is not symmetric where the guards differ.
-// J[0..5] = dFx/dy, dFx/dz, dFy/dx, dFy/dz, dFz/dx, dFz/dy.  Nothing outside the grid box.
-__device__ inline void phix_force_jac(const StepParams &P, double chi, double x, double y,
-                                      double z, double F[3], double J[6]) {
-    for (int k = 0; k < 3; k++) F[k] = 0.0;
-    for (int k = 0; k < 6; k++) J[k] = 0.0;
-    const GridGeom &g = P.gx;
-    if (!(x <= g.xbcen + g.tranx && x >= g.xbcen - g.tranx && y <= g.ybcen + g.trany &&
-          y >= g.ybcen - g.trany && z <= g.zbcen + g.tranz && z >= g.zbcen - g.tranz))
-        return;
+// The trilinear grid force of one ion and its Jacobian J = dF/dx from the 8 corners in
+// order (the step kernels' per-corner terms, tri_corner), added to F and J.  Within a cell
+// the interpolant is linear in each coordinate, so J's diagonal is zero; an off-diagonal
+// entry is the mixed second derivative of the force component it belongs to, under that
+// component's near-node guard, so J is not symmetric where the guards differ.
+// J[0..5] = dFx/dy, dFx/dz, dFy/dx, dFy/dz, dFz/dx, dFz/dy.
+template <typename T>
+__device__ inline void tri_force_jac(const GridGeom &g, const T *__restrict__ data, double scale,
+                                     double x, double y, double z, double F[3], double J[6]) {
     double gx, gy, gz;
     int ix, iy, iz;
     grid_cell(g, x, y, z, gx, gy, gz, ix, iy, iz);
@@ -255,8 +250,8 @@ __device__ inline void phix_force_jac(const StepParams &P, double chi, double x,
         double ci = gz - n3 * g.dcel;
         const double cs = fsign(one, ci);
         ci = one - fabs(ci) * g.idcel;
-        const double ph = (double)P.phix[((size_t)n1 * g.ny + n2) * g.nz + n3];
-        const double prefac = ph * P.celec * chi * g.idcel;
+        const double ph = (double)data[((size_t)n1 * g.ny + n2) * g.nz + n3];
+        const double prefac = ph * scale * g.idcel;
         const double q = prefac * g.idcel;   // d(1 - |u|/dcel)/du = -sign(u)/dcel
         if (ai < (1.0 - RSMALL) && ai > RSMALL) {
             F[0] += as * bi * ci * prefac;
@@ -320,6 +315,28 @@ __device__ __forceinline__ size_t vdw_map(const StepParams &P, int i, int itype)
 __device__ __forceinline__ size_t vdw_map_of(const StepParams &P, int i) {
     if (!(P.flags & QNMCDEN)) return 0;
     return vdw_map(P, i, i < P.nsites ? 0 : abs(P.typei[i]));
+}
+
+// The grid forces of moving ion i at (x, y, z), atnam2 type t2, and their Jacobian:
+// STATICF1 (staticf1.f90, charge cg) and the trilinear VDWGD1TRLN (vdwgd1trln.f90, svdw,
+// scal with QSVDW, the ion's own map with NMCDEN), inside the same bounds as the step.
+__device__ inline void grid_force_jac(const StepParams &P, int i, double x, double y, double z,
+                                      int t2, double F[3], double J[6]) {
+    for (int k = 0; k < 3; k++) F[k] = 0.0;
+    for (int k = 0; k < 6; k++) J[k] = 0.0;
+    if (P.flags & QPHIX) {
+        const GridGeom &g = P.gx;
+        if (x <= g.xbcen + g.tranx && x >= g.xbcen - g.tranx && y <= g.ybcen + g.trany &&
+            y >= g.ybcen - g.trany && z <= g.zbcen + g.tranz && z >= g.zbcen - g.tranz)
+            tri_force_jac(g, P.phix, P.celec * P.cg[abs(P.typei[i])], x, y, z, F, J);
+    }
+    if (P.flags & QPHIV) {
+        const GridGeom &g = P.gv;
+        const double esvdw = (P.flags & QSVDW) ? P.svdw * P.scal[t2] : P.svdw;
+        if (x <= g.xbcen + g.tranx && x >= g.xbcen - g.tranx && y <= g.ybcen + g.trany &&
+            y >= g.ybcen - g.trany && z <= P.vzmax && z >= P.vzmin)
+            tri_force_jac(g, P.phiv + vdw_map_of(P, i), esvdw, x, y, z, F, J);
+    }
 }
 
 // Repulsion map of one ENERGY target at (x, y, z), atnam2 type t2, by a whole wavefront:

@@ -488,8 +488,9 @@ int bromoc_philox4x64(int32_t n, const uint64_t *ctr, const uint64_t *key, uint6
  * with respect to the pair tables, by a reverse pass through the steps (no counterpart in
  * the reference, whose potentials are fitted by the separate inverse-Monte-Carlo
  * program).  The trajectory: nsteps steps of ENERGY + DYNAMICS0 (moving ions, every pair
- * tabulated, the static field phix allowed; no repulsion grid, no CG-DNA) from the current
- * positions with the injected noise rows
+ * tabulated; the static field phix and the trilinear repulsion map phiv, with QSVDW and
+ * NMCDEN, allowed; no B-spline map, no CG-DNA) from the current positions with the injected
+ * noise rows
  * 0..nsteps-1.  The RDF of group g (a pair of atnam ion types) in bin b is
  *   g[g][b] = V / (nsteps * P_g * 4 pi r_b^2 sigma sqrt(2 pi)) * sum over the steps' states
  *             x_1..x_nsteps and the group's pairs of exp(-(r - r_b)^2 / (2 sigma^2)),

@@ -27,6 +27,16 @@ def _system(seed=0, n=60, box=36.0, curve_seed=None):
     return configs.adjoint_system(seed=seed, n_per_type=n, box=box, curve_seed=curve_seed)
 
 
+def _grids(s, field, seed):
+    """s with the static field (field True), or with the static field and a soft repulsion
+    map with per-type scaling (field "grids")."""
+    if field:
+        s = configs.with_static_field(s, seed=seed)
+    if field == "grids":
+        s = configs.with_soft_repulsion(s, seed=seed + 1)
+    return s
+
+
 def _target(s, noise, nsteps, curve_seed):
     """RDFs of the same start and noise under a second seed's potentials."""
     t = configs.with_knots(s, {is_: u for is_, (u, _, _) in configs.adjoint_system(
@@ -61,11 +71,9 @@ def _normwise(adj, cd):
     return float(np.max(np.abs(adj - cd)) / np.max(np.abs(cd)))
 
 
-@pytest.mark.parametrize("field", [False, True], ids=["pairs", "static_field"])
+@pytest.mark.parametrize("field", [False, True, "grids"], ids=["pairs", "static_field", "grids"])
 def test_forward_loss_matches_cpu(field):
-    s = _system(seed=1, n=40)
-    if field:
-        s = configs.with_static_field(s, seed=3)
+    s = _grids(_system(seed=1, n=40), field, 3)
     nsteps = 30
     noise = np.random.default_rng(2).standard_normal((nsteps, 3 * s.ntot))
     target = np.random.default_rng(3).random((len(SPEC.groups), SPEC.nbins))
@@ -75,11 +83,10 @@ def test_forward_loss_matches_cpu(field):
     assert np.max(np.abs(gg - gc)) <= 1e-10 * np.max(gc)
 
 
-@pytest.mark.parametrize("nsteps,ckpt,field", [(60, 20, False), (45, 7, False), (45, 7, True)])
+@pytest.mark.parametrize("nsteps,ckpt,field", [(60, 20, False), (45, 7, False), (45, 7, True),
+                                               (45, 7, "grids")])
 def test_gradient_vs_gpu_central_differences(nsteps, ckpt, field):
-    s = _system(seed=4, n=50)
-    if field:   # the static field's Jacobian enters the reverse pass
-        s = configs.with_static_field(s, seed=5)
+    s = _grids(_system(seed=4, n=50), field, 5)   # the grids' Jacobian enters the reverse pass
     noise = np.random.default_rng(5).standard_normal((nsteps, 3 * s.ntot))
     target = _target(s, noise, nsteps, curve_seed=77)
     loss, _, grads = rdf_loss(s, noise, nsteps, SPEC, target=target, checkpoint_every=ckpt)
@@ -185,7 +192,7 @@ def test_gradient_bit_reproducible():
 FLUX = RdfSpec(groups=[(1, 1), (1, 2), (2, 2), (1, 3)], wflux=3.0, jtarget=0.05)
 
 
-@pytest.mark.parametrize("field", [False, True], ids=["pairs", "static_field"])
+@pytest.mark.parametrize("field", [False, True, "grids"], ids=["pairs", "static_field", "grids"])
 @pytest.mark.parametrize("spec", [SPEC, FLUX], ids=["rdf", "rdf+flux"])
 @pytest.mark.parametrize("nsteps,ckpt", [(60, 20), (45, 7)])
 def test_dot_product_adjoint_vs_tangent(spec, nsteps, ckpt, field):
@@ -193,9 +200,7 @@ def test_dot_product_adjoint_vs_tangent(spec, nsteps, ckpt, field):
     random directions over every table coefficient (ep2, ep3 of every knot; sc1, sc3 of
     every pair type): the two passes are transposes of one linear map, so they agree to
     rounding -- free of the knot-jump noise central differences carry."""
-    s = _system(seed=18, n=60, box=36.0)
-    if field:
-        s = configs.with_static_field(s, seed=19)
+    s = _grids(_system(seed=18, n=60, box=36.0), field, 19)
     noise = np.random.default_rng(19).standard_normal((nsteps, 3 * s.ntot))
     target = _target(s, noise, nsteps, curve_seed=80)
     loss, gep, gsc = table_gradients(s, noise, nsteps, spec, target=target, checkpoint_every=ckpt)
