@@ -103,6 +103,22 @@ def test_steps_vs_reference(which, base):
 
 @needs_ref
 @pytest.mark.parametrize("which", ["sphere", "ecyl"])
+def test_notrans_after_fixcoor_vs_reference(which):
+    """NOTRANS resets the DNA centroid after DYNAMICS0NUC's FIXCOOR (simul1.f90:369-386):
+    the curved wall acts on the sites first."""
+    s = configs.with_notrans(shaped(which, pore(seed=12)), axes=7)
+    noise = 3.0 * np.random.default_rng(13).standard_normal((5, 3 * s.ntot))
+    o, r = Oracle(s), RefFortran(s)
+    o.upload_noise(noise)
+    r.upload_noise(noise)
+    for k in range(5):
+        o.step(1, abi.NOISE_INJECTED, k)
+        r.step(1, abi.NOISE_INJECTED, k)
+    assert float(np.max(np.abs(o.positions()[0] - r.positions()[0]))) <= 1e-12
+
+
+@needs_ref
+@pytest.mark.parametrize("which", ["sphere", "ecyl"])
 def test_debyhyb_outbox_vs_reference(which):
     base = shaped(which, pore(seed=9))
     s = configs.with_debyhyb(base, outside=())

@@ -44,10 +44,12 @@ def _inside(s, pos):
 
 @pytest.mark.parametrize("tpb", [1, 2, 4, 8, 17])
 @pytest.mark.parametrize("which", ["sphere", "ecyl"])
-@pytest.mark.parametrize("base", ["plain", "pore"])
+@pytest.mark.parametrize("base", ["plain", "pore", "pore_notrans"])
 def test_steps(base, which, tpb, monkeypatch):
     monkeypatch.setenv("BROMOC_TPB", str(tpb))
     s = _shaped(which, _plain() if base == "plain" else with_fixed(_pore(), 5, nonfree=(2,)))
+    if base == "pore_notrans":   # the centroid reset runs after the curved FIXCOOR
+        s = configs.with_notrans(s, axes=7)
     nsteps = 6
     noise = 3.0 * np.random.default_rng(3).standard_normal((nsteps, 3 * s.ntot))
     g, o = Engine(s), Oracle(s)
