@@ -118,6 +118,8 @@ struct bromoc_ctx {
     int64_t noise_stride = 0;
     int64_t *ctl = nullptr;     // [step, row0, counter0, seed]
     int64_t h_ctl[4] = {0, 0, 0, 0};   // host source of the ctl upload (outlives the copy)
+    void *pin = nullptr;               // pinned staging of a call's small copies
+    size_t pin_cap = 0;
     double4 h_move{};                  // staging of one particle moved by GRAND
     double *iout = nullptr;
     std::vector<void *> allocs;
@@ -509,6 +511,21 @@ int ensure_sorted(bromoc_ctx *c, bool force = false) {
     return BROMOC_OK;
 }
 
+// Pinned host staging for a call's small copies (the step loop's control words, GRAND's
+// uniforms, counts and state): from pageable memory each such copy is a slow staged
+// transfer that leaves the GPU idle between calls.  Every call synchronises before it
+// returns, so the next call may reuse or regrow the buffer.
+static unsigned char *pinned(bromoc_ctx *c, size_t bytes) {
+    if (bytes > c->pin_cap) {
+        if (c->pin) (void)hipHostFree(c->pin);
+        c->pin = nullptr;
+        c->pin_cap = 0;
+        if (hipHostMalloc(&c->pin, bytes) != hipSuccess) return nullptr;
+        c->pin_cap = bytes;
+    }
+    return (unsigned char *)c->pin;
+}
+
 int sync_err(bromoc_ctx *c) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(c, BROMOC_E_HIP, std::string("kernel: ") + hipGetErrorString(e));
@@ -551,6 +568,7 @@ int bromoc_destroy(bromoc_ctx *c) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (void *p : c->allocs) (void)hipFree(p);
     for (void *p : c->gd.mem) (void)hipFree(p);
+    if (c->pin) (void)hipHostFree(c->pin);
     if (c->gd.u) (void)hipFree(c->gd.u);
     if (c->gd.natlog) (void)hipFree(c->gd.natlog);
     for (void *p : {(void *)c->tpart, (void *)c->tener, (void *)c->an_rho, (void *)c->an_rdna, (void *)c->an_gr, (void *)c->an_prob2,
@@ -1238,7 +1256,10 @@ static int run_steps(bromoc_ctx *c, int32_t nsteps, int32_t noise_mode, int32_t 
     int st0 = autotune(c);
     if (st0) return st0;
     c->h_ctl[0] = 0; c->h_ctl[1] = first_row; c->h_ctl[2] = (int64_t)counter0; c->h_ctl[3] = (int64_t)seed;
-    HIPCHK(c, hipMemcpyAsync(c->ctl, c->h_ctl, sizeof(c->h_ctl), hipMemcpyHostToDevice, c->stream));
+    int64_t *hc = (int64_t *)pinned(c, 4096);
+    if (!hc) return fail(c, BROMOC_E_HIP, "pinned staging");
+    std::memcpy(hc, c->h_ctl, sizeof(c->h_ctl));
+    HIPCHK(c, hipMemcpyAsync(c->ctl, hc, sizeof(c->h_ctl), hipMemcpyHostToDevice, c->stream));
     const bool philox = noise_mode == BROMOC_NOISE_PHILOX;
     StepParams p = params_for(c);
     p.mode = 1;
@@ -1677,14 +1698,26 @@ static int grand_device(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t n
         D.lcap = nlog;
     }
     const int n0 = c->ntot;
-    if (nu > 0) HIPCHK(c, hipMemcpyAsync(D.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(D.nat, nat, sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
+    // pinned staging: uniforms, nat and the state up; the state, nat and the log back
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_nat = al(sizeof(double) * nu), o_st = o_nat + al(sizeof(int32_t) * (nb + 1));
+    const size_t o_st2 = o_st + al(sizeof(GcState)), o_nat2 = o_st2 + al(sizeof(GcState));
+    const size_t o_log = o_nat2 + al(sizeof(int32_t) * (nb + 1));
+    unsigned char *pin = pinned(c, o_log + sizeof(int32_t) * (size_t)nlog);
+    if (!pin) return fail(c, BROMOC_E_HIP, "pinned staging");
+    if (nu > 0) {
+        std::memcpy(pin, u, sizeof(double) * nu);
+        HIPCHK(c, hipMemcpyAsync(D.u, pin, sizeof(double) * nu, hipMemcpyHostToDevice, c->stream));
+    }
+    std::memcpy(pin + o_nat, nat, sizeof(int32_t) * (nb + 1));
+    HIPCHK(c, hipMemcpyAsync(D.nat, pin + o_nat, sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
     // the device's ibuffer is the host's unless the host changed it since the last call
     if (!D.ib_current)
         HIPCHK(c, hipMemcpyAsync(D.ibuffer, c->ibuffer.data(), sizeof(int32_t) * n0, hipMemcpyHostToDevice, c->stream));
     GcState h{};
     h.k = 0; h.ntot = n0; h.err = GCE_OK; h.ener = *ener;
-    HIPCHK(c, hipMemcpyAsync(D.st, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+    std::memcpy(pin + o_st, &h, sizeof(h));
+    HIPCHK(c, hipMemcpyAsync(D.st, pin + o_st, sizeof(h), hipMemcpyHostToDevice, c->stream));
     GcParams G{};
     G.sp = params_for(c);
     G.pos0 = c->pos[0]; G.pos1 = c->pos[1];
@@ -1700,19 +1733,23 @@ static int grand_device(bromoc_ctx *c, int32_t ngcmc, const double *u, int64_t n
     G.cap = c->cap;
     G.st = D.st; G.part = D.part;
     HIPCHK(c, launch_grand(G, ngcmc, c->stream));
-    HIPCHK(c, hipMemcpyAsync(&h, D.st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pin + o_st2, D.st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     int st = sync_err(c);
     if (st) return st;
+    std::memcpy(&h, pin + o_st2, sizeof(h));
     const int n1 = h.ntot;
-    HIPCHK(c, hipMemcpyAsync(nat, D.nat, sizeof(int32_t) * (nb + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pin + o_nat2, D.nat, sizeof(int32_t) * (nb + 1), hipMemcpyDeviceToHost, c->stream));
     D.h_natlog.resize((size_t)nlog);
     if (nb > 0 && h.natt > 0)
-        HIPCHK(c, hipMemcpyAsync(D.h_natlog.data(), D.natlog, sizeof(int32_t) * (size_t)h.natt * nb,
+        HIPCHK(c, hipMemcpyAsync(pin + o_log, D.natlog, sizeof(int32_t) * (size_t)h.natt * nb,
                                  hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->ibuffer.data(), D.ibuffer, sizeof(int32_t) * n1, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->h_typei.data(), c->typei, sizeof(int32_t) * n1, hipMemcpyDeviceToHost, c->stream));
     st = sync_err(c);
     if (st) return st;
+    std::memcpy(nat, pin + o_nat2, sizeof(int32_t) * (nb + 1));
+    if (nb > 0 && h.natt > 0)
+        std::memcpy(D.h_natlog.data(), pin + o_log, sizeof(int32_t) * (size_t)h.natt * nb);
     D.ib_current = true;
     // prob(b, count) += 1 for every logged count, attempt by attempt
     for (int a = 0; a < h.natt; a++)
